@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""Benchmark of the strand-generation hot path on one MI355X.
+
+Workload (one step = one full generation pass, guide preparation + strand
+kernel, over one synthetic description resident in HBM):
+  plane-524288tri: 512x512-quad displaced plane (524,288 triangles) with
+  log-normal(0, 0.5) area weights, 4,096 guides x 16 CVs, stencil K=8,
+  2,000,000 strands x 16 CVs, taper 0.01 -> 0.001, two stacked noise octaves
+  (0.1 @ 4, 0.05 @ 8); outputs 32,000,000 float4 points + 2,000,000 counts.
+
+Prints one JSON line: points/s, the strand kernel's HBM roofline fraction
+(algorithmic bytes / kernel time, timed with HIP events on the launch
+stream), and the CPU oracle's points/s on a bounded sample of the same
+workload on this host's cores.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "output points/sec + % HBM roofline; surrogate vs plain expr at <=1e-3 rel err"
+HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "strand_kernel_plane2M_pmc.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(blob, w, target_s, threads):
+    """The oracle's multithreaded CPU path (128-strand tiles, atomic counter)
+    on the first S strands of the same workload, S sized to ~target_s."""
+    import numpy as np
+    import oracle
+    oracle.lib()
+    calib = np.arange(4096, dtype=np.uint32)
+    t = time.perf_counter()
+    oracle.generate(blob, w.params, octaves=w.octaves, strands=calib, workers=threads)
+    dt = time.perf_counter() - t
+    s = int(min(w.params.strand_count, max(4096, 4096 * target_s / max(dt, 1e-6))))
+    sample = np.arange(s, dtype=np.uint32)
+    t = time.perf_counter()
+    oracle.generate(blob, w.params, octaves=w.octaves, strands=sample, workers=threads)
+    dt = time.perf_counter() - t
+    pts = s * w.params.cvs_per_strand
+    return {"value": pts / dt, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"strands 0..{s - 1} of the same description ({pts} points), {dt:.2f} s, "
+                      f"float restatement of the reference CPU path (oracle/), {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the strand path shards by strand range; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import workloads
+
+    w = workloads.config2()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    t = time.perf_counter()
+    blob = nx.build_asset(w.build)
+    log(f"built {w.name}: {blob.nbytes} B in {time.perf_counter() - t:.2f} s")
+    asset = nx.DeviceAsset(blob)
+    out = nx.PackedOutputs(N, Cv, roots=False, root_uvs=False, point_counts=True)
+    ws = torch.empty(asset.workspace_bytes(w.params), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(stream)
+        nx.prepare_packed(asset, w.params, ws, stream=stream)
+        if ev:
+            ev[1].record(stream)
+        nx.generate_packed_prepared(asset, w.params, out, ws, octaves=w.octaves, stream=stream)
+        if ev:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(events[i])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    prep_ms = [e[0].elapsed_time(e[1]) for e in events]
+    kern_ms = [e[1].elapsed_time(e[2]) for e in events]
+    kern_ms_sorted = sorted(kern_ms)
+    kernel_ms = sum(kern_ms) / len(kern_ms)
+    points = N * Cv
+    alg_bytes = points * 16 + N * 4 + blob.nbytes
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PMC_SUMMARY):
+        with open(PMC_SUMMARY) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "strand_kernel<noise, 2 octaves>", "kernel_ms_mean": round(kernel_ms, 4),
+                "kernel_ms_median": round(kern_ms_sorted[len(kern_ms) // 2], 4),
+                "kernel_ms_p90": round(kern_ms_sorted[min(len(kern_ms) - 1, int(0.9 * len(kern_ms)))], 4),
+                "prep_ms_mean": round(sum(prep_ms) / len(prep_ms), 4),
+                "algorithmic_bytes_per_launch": alg_bytes}
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(blob, w, args.cpu_seconds, threads)
+    line = {"metric": METRIC, "value": points / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": w.name, "triangles": asset.header.triangle_count,
+                       "guides": asset.header.guide_count, "strands": N, "cvs_per_strand": Cv,
+                       "stencil_k": 8, "octaves": [[0.1, 4.0], [0.05, 8.0]], "parallelism": "single-gpu"},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,335 @@
+/*
+ * nanoxgen_amd.h -- C ABI of the MI355X (gfx950) strand generator.
+ *
+ * Every type and entry point here replaces one piece of the reference's C++
+ * interface for the strand-generation path; the reference file:line is given
+ * beside each.  All functions are plain C: pointers, sizes and POD structs,
+ * no exceptions, no hidden allocation, no hidden synchronisation.  Device
+ * pointers are owned by the caller; work is enqueued on the caller's stream.
+ *
+ * Status codes 0..20 are the reference's DeviceGenerationError values
+ * (include/nanoxgen/device_contract.h:58-80) verbatim; 21.. are extensions.
+ */
+#ifndef NANOXGEN_AMD_H
+#define NANOXGEN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- .nxg blob format (include/nanoxgen/types.h:17-22, 77-101) ---------- */
+#define NXG_MAGIC 0x4e4547584f4e414eull /* "NANOXGEN", little endian */
+#define NXG_VERSION_MAJOR 0u
+#define NXG_VERSION_MINOR 2u
+#define NXG_GUIDE_STENCIL_SIZE 8u
+#define NXG_NOISE_GRADIENT_COUNT 256u
+#define NXG_INVALID_INDEX 0xffffffffu
+#define NXG_BLOB_ALIGNMENT 64u
+#define NXG_ASSET_HAS_NORMALS 1u   /* AssetFlags::HasNormals   types.h:71 */
+#define NXG_ASSET_HAS_TEXCOORDS 2u /* AssetFlags::HasTexcoords types.h:72 */
+/* Extension: at most this many stacked noise octaves per strand. */
+#define NXG_MAX_NOISE_OCTAVES 4u
+
+typedef struct nxg_vec2 { float x, y; } nxg_vec2;         /* types.h:24-27 */
+typedef struct nxg_vec3 { float x, y, z; } nxg_vec3;      /* types.h:29-33 */
+typedef struct nxg_uint3 { uint32_t x, y, z; } nxg_uint3; /* types.h:35-39 */
+
+/* Walker alias bucket (types.h:53-56). */
+typedef struct nxg_alias_entry {
+    float probability;
+    uint32_t alias;
+} nxg_alias_entry;
+
+/* 56-byte guide record (types.h:58-68). */
+typedef struct nxg_guide_record {
+    uint32_t first_cv;
+    uint16_t cv_count;
+    uint16_t flags;
+    uint32_t triangle_index;
+    float support_radius;
+    nxg_vec2 barycentric; /* weights of triangle vertices y and z */
+    nxg_vec2 root_uv;
+    nxg_vec3 root_position;
+    nxg_vec3 root_normal;
+} nxg_guide_record;
+
+/* 128-byte blob header (types.h:77-101).  The blob base must be 64-byte
+ * aligned in memory; this ABI struct itself carries no over-alignment so it
+ * can be embedded by value in descriptors. */
+typedef struct nxg_asset_header {
+    uint64_t magic;
+    uint16_t version_major;
+    uint16_t version_minor;
+    uint32_t flags;
+    uint64_t byte_size;
+    uint64_t content_hash;
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+    uint32_t guide_count;
+    uint32_t guide_cv_count;
+    uint32_t guide_stencil_size;
+    uint32_t noise_gradient_count;
+    uint64_t positions_offset;
+    uint64_t normals_offset;
+    uint64_t texcoords_offset;
+    uint64_t triangles_offset;
+    uint64_t alias_table_offset;
+    uint64_t guides_offset;
+    uint64_t guide_cvs_offset;
+    uint64_t triangle_guides_offset;
+    uint64_t noise_gradients_offset;
+} nxg_asset_header;
+
+/* GenerationParams (types.h:108-124), field for field. */
+typedef struct nxg_generation_params {
+    uint32_t strand_count;
+    uint32_t cvs_per_strand;
+    uint32_t seed;
+    uint32_t reserved;
+    float guide_support_scale;
+    float guide_weight_power;
+    float normal_rejection_cos;
+    float length_scale;
+    float root_width;
+    float tip_width;
+    float noise_amplitude;
+    float noise_frequency;
+    float noise_mask;
+    float noise_correlation;
+    float noise_preserve_length;
+} nxg_generation_params;
+
+/* Extension (no reference counterpart): stacked noise octaves.  count 0 or 1
+ * is the reference's single octave (noise_amplitude, noise_frequency).  For
+ * count k > 1, amplitude[i]/frequency[i] (i < k) replace the single pair and
+ * the per-octave displacements are summed in the one transported frame. */
+typedef struct nxg_noise_octaves {
+    uint32_t count;
+    float amplitude[NXG_MAX_NOISE_OCTAVES];
+    float frequency[NXG_MAX_NOISE_OCTAVES];
+} nxg_noise_octaves;
+
+/* RootSample (types.h:126-135): 48 bytes. */
+typedef struct nxg_root_sample {
+    nxg_vec3 position;
+    nxg_vec3 normal;
+    nxg_vec2 uv;
+    uint32_t triangle_index;
+    nxg_vec2 barycentric;
+    uint32_t surface_face_id;
+} nxg_root_sample;
+
+/* PackedCurvePoint (types.h:139-147): float4(position, radius). */
+typedef struct nxg_packed_point { float x, y, z, radius; } nxg_packed_point;
+
+/* ---- checked device descriptors (device_contract.h:14-56) -------------- */
+
+/* DeviceAssetDescriptor (device_contract.h:14-18): device pointer to the
+ * uploaded blob, the validated host mirror of its header, and the device
+ * allocation size in bytes. */
+typedef struct nxg_device_asset {
+    const void *data;
+    nxg_asset_header header;
+    uint64_t byte_capacity;
+} nxg_device_asset;
+
+/* DeviceDeformedGeometryDescriptor (device_contract.h:20-25, types.h:158-162).
+ * A null pointer selects the asset's rest array; its count must then be 0. */
+typedef struct nxg_deformed_geometry {
+    const nxg_vec3 *positions;
+    const nxg_vec3 *normals;
+    const nxg_vec3 *guide_cvs;
+    uint64_t position_count;
+    uint64_t normal_count;
+    uint64_t guide_cv_count;
+} nxg_deformed_geometry;
+
+/* DevicePackedCurveOutputDescriptor (device_contract.h:34-40, types.h:164-170).
+ * Capacities are in elements.  points is required; the others are optional
+ * and a null pointer must come with capacity 0. */
+typedef struct nxg_packed_output {
+    nxg_packed_point *points;
+    nxg_root_sample *roots;
+    nxg_vec2 *root_uvs;
+    float radius_scale;
+    uint32_t *point_counts;
+    uint64_t point_capacity;
+    uint64_t root_capacity;
+    uint64_t root_uv_capacity;
+    uint64_t point_count_capacity;
+} nxg_packed_output;
+
+/* Extension: per-strand guide-blend record (GuideBlendState.indices/weights,
+ * generate.h:207-212), strand-major, NXG_GUIDE_STENCIL_SIZE entries per
+ * strand.  Slots the blend rejected hold NXG_INVALID_INDEX and weight 0. */
+typedef struct nxg_stencil_output {
+    uint32_t *indices;
+    float *weights;
+    uint64_t capacity; /* entries, >= strand_count * 8 */
+} nxg_stencil_output;
+
+/* DeviceLaunchConfig (device_contract.h:54-56).  block_size is validated to
+ * [1, 1024] as in the reference; the gfx950 kernels use their own tile of
+ * four 64-lane wavefronts per workgroup whatever its value. */
+typedef struct nxg_launch_config {
+    uint32_t block_size;
+} nxg_launch_config;
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    NXG_OK = 0,                              /* DeviceGenerationError::None */
+    NXG_ERROR_NULL_ASSET = 1,
+    NXG_ERROR_MISALIGNED_ASSET = 2,
+    NXG_ERROR_INVALID_ASSET_METADATA = 3,
+    NXG_ERROR_ASSET_CAPACITY_TOO_SMALL = 4,
+    NXG_ERROR_INVALID_GENERATION_PARAMETERS = 5,
+    NXG_ERROR_INVALID_BLOCK_SIZE = 6,
+    NXG_ERROR_DEFORMED_POSITION_COUNT_MISMATCH = 7,
+    NXG_ERROR_DEFORMED_NORMAL_COUNT_MISMATCH = 8,
+    NXG_ERROR_DEFORMED_GUIDE_CV_COUNT_MISMATCH = 9,
+    NXG_ERROR_MISSING_OUTPUT = 10,
+    NXG_ERROR_POINT_CAPACITY_TOO_SMALL = 11,
+    NXG_ERROR_WIDTH_CAPACITY_TOO_SMALL = 12,
+    NXG_ERROR_ROOT_CAPACITY_TOO_SMALL = 13,
+    NXG_ERROR_ROOT_UV_CAPACITY_TOO_SMALL = 14,
+    NXG_ERROR_POINT_COUNT_CAPACITY_TOO_SMALL = 15,
+    NXG_ERROR_INVALID_RADIUS_SCALE = 16,
+    NXG_ERROR_MISSING_MOTION_SAMPLES = 17,
+    NXG_ERROR_INVALID_MOTION_TIMES = 18,
+    NXG_ERROR_MOTION_POINT_CAPACITY_TOO_SMALL = 19,
+    NXG_ERROR_OUTPUT_SIZE_OVERFLOW = 20,
+    /* extensions */
+    NXG_ERROR_HIP = 21,                /* a HIP runtime call failed */
+    NXG_ERROR_WORKSPACE_TOO_SMALL = 22,
+    NXG_ERROR_INVALID_OCTAVES = 23,
+    NXG_ERROR_NULL_ARGUMENT = 24,
+    NXG_ERROR_STENCIL_CAPACITY_TOO_SMALL = 25,
+    NXG_ERROR_INVALID_ROOTS = 26,
+    NXG_ERROR_INVALID_BUILD_INPUT = 27,
+    NXG_ERROR_BUFFER_TOO_SMALL = 28,
+    NXG_ERROR_INVALID_ASSET = 29
+};
+
+/* device_generation_error_message (device_contract.h:82-116); extension codes
+ * get messages of their own.  Never returns NULL. */
+const char *nxg_status_message(uint32_t status);
+
+/* ---- host asset builder (asset.h:304-334, 409-418; asset.cpp:229-378) --- */
+
+/* Flattened AssetBuildInput/GuideInput (asset.h:304-334).  Optional arrays
+ * may be NULL.  Guide i owns guide_cv_counts[i] consecutive CVs of
+ * guide_cvs; its first CV is its root. */
+typedef struct nxg_asset_build_input {
+    const nxg_vec3 *positions;
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+    const nxg_vec3 *normals;          /* optional: vertex_count */
+    const nxg_vec2 *texcoords;        /* optional: vertex_count */
+    const nxg_uint3 *triangles;
+    uint32_t guide_count;
+    uint32_t stencil_k;               /* extension: 0 means 8 (reference) */
+    const uint32_t *guide_cv_counts;  /* guide_count entries, each 2..65535 */
+    const nxg_vec3 *guide_cvs;        /* sum(guide_cv_counts) entries */
+    const nxg_vec3 *guide_root_normals;  /* optional: default (0,1,0) */
+    const nxg_vec2 *guide_root_uvs;      /* optional */
+    const uint32_t *guide_triangles;     /* optional: default invalid */
+    const nxg_vec2 *guide_barycentrics;  /* optional */
+    const float *guide_support_radii;    /* optional: 0 selects automatic */
+    const float *triangle_weights;       /* extension, optional: alias
+                                            weights are area * weight */
+} nxg_asset_build_input;
+
+/* Size in bytes of the blob nxg_asset_build would write (cheap). */
+uint32_t nxg_asset_build_size(const nxg_asset_build_input *input, uint64_t *byte_size);
+
+/* build_asset (asset.cpp:229-293) into caller memory `out` (64-byte
+ * aligned, capacity bytes).  On a refused input, writes the reference's
+ * std::invalid_argument message into err (if non-NULL). */
+uint32_t nxg_asset_build(const nxg_asset_build_input *input, void *out, uint64_t capacity,
+                         uint64_t *byte_size, char *err, uint64_t err_capacity);
+
+/* validate_asset (asset.cpp:295-338): writes "" or the reference's message
+ * into msg; returns NXG_OK or NXG_ERROR_INVALID_ASSET. */
+uint32_t nxg_asset_validate(const void *blob, uint64_t byte_size, char *msg, uint64_t msg_capacity);
+
+/* make_device_asset_descriptor (asset.cpp:364-378): validates the host blob
+ * and fills the descriptor for its device copy. */
+uint32_t nxg_make_device_asset(const void *host_blob, uint64_t host_byte_size,
+                               const void *device_data, uint64_t device_byte_capacity,
+                               nxg_device_asset *out);
+
+/* ---- device generation (device_contract.h:221-254; generate.h:612-676) -- */
+
+/* validate_device_packed_generation_request (device_contract.h:221-254).
+ * deformed and config may be NULL (reference defaults). */
+uint32_t nxg_validate_packed_generation(const nxg_device_asset *asset,
+                                        const nxg_deformed_geometry *deformed,
+                                        const nxg_generation_params *params,
+                                        const nxg_packed_output *output,
+                                        const nxg_launch_config *config);
+
+/* Bytes of device workspace nxg_generate_packed needs for this request. */
+uint64_t nxg_packed_generation_workspace_size(const nxg_asset_header *header,
+                                              const nxg_generation_params *params);
+
+/* generate_packed_cpu's device equivalent: runs generate_packed_strand
+ * (generate.h:612-642) for every strand on `stream` (a hipStream_t; NULL is
+ * the null stream).  Validates first, launches nothing on error, allocates
+ * nothing and does not synchronise.  octaves, stencil, deformed and config
+ * may be NULL. */
+uint32_t nxg_generate_packed(const nxg_device_asset *asset,
+                             const nxg_deformed_geometry *deformed,
+                             const nxg_generation_params *params,
+                             const nxg_noise_octaves *octaves,
+                             const nxg_packed_output *output,
+                             const nxg_stencil_output *stencil,
+                             const nxg_launch_config *config,
+                             void *workspace, uint64_t workspace_bytes,
+                             void *stream);
+
+/* The two stream-ordered halves of nxg_generate_packed.  prepare fills the
+ * workspace with the per-guide state and the guide offset table, which
+ * depend only on the asset, the deformed geometry and cvs_per_strand (the
+ * reference recomputes them inside interpolate_offset, generate.h:246-267);
+ * generate_packed_prepared runs the strand kernel from a workspace that a
+ * prepare call with the same asset, deformed geometry and cvs_per_strand
+ * filled earlier on the same stream (or one ordered before it). */
+uint32_t nxg_prepare_packed_generation(const nxg_device_asset *asset,
+                                       const nxg_deformed_geometry *deformed,
+                                       const nxg_generation_params *params,
+                                       void *workspace, uint64_t workspace_bytes,
+                                       void *stream);
+uint32_t nxg_generate_packed_prepared(const nxg_device_asset *asset,
+                                      const nxg_deformed_geometry *deformed,
+                                      const nxg_generation_params *params,
+                                      const nxg_noise_octaves *octaves,
+                                      const nxg_packed_output *output,
+                                      const nxg_stencil_output *stencil,
+                                      const nxg_launch_config *config,
+                                      void *workspace, uint64_t workspace_bytes,
+                                      void *stream);
+
+/* generate_packed_roots_cpu's device equivalent (asset.cpp:451-501,
+ * generate.h:644-676): strands start from caller-supplied device roots
+ * (strand_count records).  Root validity is the caller's contract on device
+ * (the host-side check of asset.cpp:468-483 cannot read device memory). */
+uint32_t nxg_generate_packed_from_roots(const nxg_device_asset *asset,
+                                        const nxg_generation_params *params,
+                                        const nxg_noise_octaves *octaves,
+                                        const nxg_root_sample *roots,
+                                        const nxg_packed_output *output,
+                                        const nxg_launch_config *config,
+                                        void *workspace, uint64_t workspace_bytes,
+                                        void *stream);
+
+/* Library build identity: gfx target and compile flags that affect parity. */
+const char *nxg_build_info(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* NANOXGEN_AMD_H */

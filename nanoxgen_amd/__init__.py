@@ -1,0 +1,228 @@
+"""nanoxgen_amd: MI355X (gfx950) strand generation behind a C ABI.
+
+The compute path is libnanoxgen_amd.so (hand-written HIP kernels plus the
+C++ host contract, include/nanoxgen_amd.h).  This module is the reference-
+side ctypes binding of that ABI, mirroring the reference's C++ interface
+(build_asset / validate_asset / make_device_asset_descriptor /
+validate_device_packed_generation_request / generate_packed):
+  build_asset(build_input)            -> host .nxg blob (numpy uint8)
+  validate_asset(blob)                -> "" or the reference's message
+  DeviceAsset(blob, device)           -> uploaded blob + checked descriptor
+  generate_packed(asset, params, ...) -> device outputs, on the given stream
+PyTorch provides device memory and streams only.  There is no CPU fallback:
+loading fails loudly if the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import GenerationParams, NoiseOctaves  # noqa: F401  (re-export)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnanoxgen_amd.so")
+_LIB = None
+
+# every symbol of include/nanoxgen_amd.h: (name, restype, argtypes)
+_P = C.c_void_p
+SYMBOLS = [
+    ("nxg_status_message", C.c_char_p, [C.c_uint32]),
+    ("nxg_asset_build_size", C.c_uint32, [_P, _P]),
+    ("nxg_asset_build", C.c_uint32, [_P, _P, C.c_uint64, _P, C.c_char_p, C.c_uint64]),
+    ("nxg_asset_validate", C.c_uint32, [_P, C.c_uint64, C.c_char_p, C.c_uint64]),
+    ("nxg_make_device_asset", C.c_uint32, [_P, C.c_uint64, _P, C.c_uint64, _P]),
+    ("nxg_validate_packed_generation", C.c_uint32, [_P, _P, _P, _P, _P]),
+    ("nxg_packed_generation_workspace_size", C.c_uint64, [_P, _P]),
+    ("nxg_generate_packed", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_generate_packed_from_roots", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_prepare_packed_generation", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_generate_packed_prepared", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_build_info", C.c_char_p, []),
+]
+
+
+class NxgError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        msg = lib().nxg_status_message(status).decode()
+        super().__init__(f"{where}: status {status} ({abi.STATUS[status] if status < len(abi.STATUS) else '?'}): {msg}")
+
+
+def lib():
+    """The loaded libnanoxgen_amd.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "or `make -C nanoxgen_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SYMBOLS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status, where):
+    if status != 0:
+        raise NxgError(status, where)
+
+
+def build_asset(build_input):
+    """build_asset (asset.cpp:229-293) -> 64-byte aligned numpy uint8 blob."""
+    L = lib()
+    size = C.c_uint64()
+    st = L.nxg_asset_build_size(C.byref(build_input.struct), C.byref(size))
+    err = C.create_string_buffer(512)
+    if st == abi.INVALID_BUILD_INPUT:
+        L.nxg_asset_build(C.byref(build_input.struct), C.c_void_p(64), 0, None, err, 512)
+        raise ValueError(err.value.decode())
+    check(st, "nxg_asset_build_size")
+    blob = abi.aligned_bytes(size.value)
+    st = L.nxg_asset_build(C.byref(build_input.struct), blob.ctypes.data, blob.nbytes, None, err, 512)
+    if st != 0:
+        raise ValueError(err.value.decode() or L.nxg_status_message(st).decode())
+    return blob
+
+
+def validate_asset(blob):
+    """validate_asset (asset.cpp:295-338): "" when valid, else the message."""
+    msg = C.create_string_buffer(512)
+    lib().nxg_asset_validate(blob.ctypes.data, blob.nbytes, msg, 512)
+    return msg.value.decode()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+class DeviceAsset:
+    """A blob uploaded to device memory with its checked descriptor
+    (make_device_asset_descriptor, asset.cpp:364-378)."""
+
+    def __init__(self, blob, device="cuda"):
+        torch = _torch()
+        msg = validate_asset(blob)
+        if msg:
+            raise ValueError(f"invalid asset: {msg}")
+        self.blob = blob
+        self.header = abi.header_of(blob)
+        # over-allocate so the base can be moved to a 64-byte boundary
+        raw = torch.empty(blob.nbytes + 64, dtype=torch.uint8, device=device)
+        off = (-raw.data_ptr()) % 64
+        self.storage = raw
+        self.tensor = raw[off:off + blob.nbytes]
+        self.tensor.copy_(torch.from_numpy(blob))
+        self.desc = abi.DeviceAsset()
+        check(lib().nxg_make_device_asset(blob.ctypes.data, blob.nbytes, self.tensor.data_ptr(),
+                                          self.tensor.numel(), C.byref(self.desc)),
+              "nxg_make_device_asset")
+
+    def workspace_bytes(self, params):
+        return lib().nxg_packed_generation_workspace_size(C.byref(self.desc.header), C.byref(params))
+
+
+class PackedOutputs:
+    """Caller-owned device outputs of one request (torch tensors)."""
+
+    def __init__(self, strands, cvs, device="cuda", roots=True, root_uvs=True, point_counts=True,
+                 stencil=False, radius_scale=1.0):
+        torch = _torch()
+        self.points = torch.empty((strands * cvs, 4), dtype=torch.float32, device=device)
+        self.roots = torch.empty((strands, 12), dtype=torch.int32, device=device) if roots else None
+        self.root_uvs = torch.empty((strands, 2), dtype=torch.float32, device=device) if root_uvs else None
+        self.point_counts = torch.empty(strands, dtype=torch.int32, device=device) if point_counts else None
+        self.stencil_indices = torch.empty((strands, 8), dtype=torch.int32, device=device) if stencil else None
+        self.stencil_weights = torch.empty((strands, 8), dtype=torch.float32, device=device) if stencil else None
+        self.radius_scale = radius_scale
+        self.strands = strands
+        self.cvs = cvs
+
+    def descriptor(self):
+        t = lambda x: 0 if x is None else x.data_ptr()
+        n = lambda x, k: 0 if x is None else x.numel() // k
+        return abi.PackedOutput(points=t(self.points), roots=t(self.roots) or None,
+                                root_uvs=t(self.root_uvs) or None, radius_scale=self.radius_scale,
+                                point_counts=t(self.point_counts) or None,
+                                point_capacity=n(self.points, 4), root_capacity=n(self.roots, 12),
+                                root_uv_capacity=n(self.root_uvs, 2),
+                                point_count_capacity=n(self.point_counts, 1))
+
+    def stencil_descriptor(self):
+        if self.stencil_indices is None:
+            return None
+        return abi.StencilOutput(indices=self.stencil_indices.data_ptr(),
+                                 weights=self.stencil_weights.data_ptr(),
+                                 capacity=self.stencil_indices.numel())
+
+    def to_host(self):
+        """Copy to numpy (synchronises)."""
+        out = dict(points=self.points.cpu().numpy())
+        if self.roots is not None:
+            out["roots"] = self.roots.cpu().numpy().view(abi.ROOT_DTYPE).reshape(-1)
+        if self.root_uvs is not None:
+            out["root_uvs"] = self.root_uvs.cpu().numpy()
+        if self.point_counts is not None:
+            out["point_counts"] = self.point_counts.cpu().numpy().view(np.uint32)
+        if self.stencil_indices is not None:
+            out["stencil_indices"] = self.stencil_indices.cpu().numpy().view(np.uint32)
+            out["stencil_weights"] = self.stencil_weights.cpu().numpy()
+        return out
+
+
+def _ref(x):
+    return None if x is None else C.byref(x)
+
+
+def generate_packed(asset, params, outputs, *, octaves=None, deformed=None, workspace=None,
+                    stream=None, roots=None):
+    """Enqueue generate_packed (generate.h:612-676) for every strand on
+    `stream` (torch stream; default the current one).  `roots` (optional
+    device tensor of RootSample records) selects generate-from-roots."""
+    torch = _torch()
+    L = lib()
+    if workspace is None:
+        workspace = torch.empty(max(1, asset.workspace_bytes(params)), dtype=torch.uint8,
+                                device=asset.tensor.device)
+    out = outputs.descriptor()
+    st = outputs.stencil_descriptor()
+    if roots is not None:
+        status = L.nxg_generate_packed_from_roots(
+            C.byref(asset.desc), C.byref(params), _ref(octaves), C.c_void_p(roots.data_ptr()),
+            C.byref(out), None, C.c_void_p(workspace.data_ptr()), workspace.numel(),
+            _stream_handle(stream))
+    else:
+        status = L.nxg_generate_packed(
+            C.byref(asset.desc), _ref(deformed), C.byref(params), _ref(octaves), C.byref(out),
+            _ref(st), None, C.c_void_p(workspace.data_ptr()), workspace.numel(),
+            _stream_handle(stream))
+    check(status, "nxg_generate_packed")
+    return outputs
+
+
+def prepare_packed(asset, params, workspace, *, deformed=None, stream=None):
+    """First half of generate_packed: guide state + offset table into workspace."""
+    check(lib().nxg_prepare_packed_generation(C.byref(asset.desc), _ref(deformed), C.byref(params),
+                                              C.c_void_p(workspace.data_ptr()), workspace.numel(),
+                                              _stream_handle(stream)), "nxg_prepare_packed_generation")
+
+
+def generate_packed_prepared(asset, params, outputs, workspace, *, octaves=None, deformed=None, stream=None):
+    """Second half of generate_packed: the strand kernel from a prepared workspace."""
+    out = outputs.descriptor()
+    st = outputs.stencil_descriptor()
+    check(lib().nxg_generate_packed_prepared(C.byref(asset.desc), _ref(deformed), C.byref(params), _ref(octaves),
+                                             C.byref(out), _ref(st), None, C.c_void_p(workspace.data_ptr()),
+                                             workspace.numel(), _stream_handle(stream)),
+          "nxg_generate_packed_prepared")
+    return outputs

@@ -1,0 +1,260 @@
+"""ctypes mirror of include/nanoxgen_amd.h.
+
+These are the POD layouts of the C ABI (the .nxg blob types of the
+reference's include/nanoxgen/types.h and the checked descriptors of
+include/nanoxgen/device_contract.h).  Pure layout definitions: importing this
+module loads no library.
+"""
+import ctypes as C
+
+import numpy as np
+
+MAGIC = 0x4E4547584F4E414E
+GUIDE_STENCIL_SIZE = 8
+NOISE_GRADIENT_COUNT = 256
+INVALID_INDEX = 0xFFFFFFFF
+MAX_NOISE_OCTAVES = 4
+HAS_NORMALS = 1
+HAS_TEXCOORDS = 2
+
+
+class Vec2(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class AssetHeader(C.Structure):
+    """types.h:77-101 (128 bytes)."""
+    _fields_ = [
+        ("magic", C.c_uint64),
+        ("version_major", C.c_uint16),
+        ("version_minor", C.c_uint16),
+        ("flags", C.c_uint32),
+        ("byte_size", C.c_uint64),
+        ("content_hash", C.c_uint64),
+        ("vertex_count", C.c_uint32),
+        ("triangle_count", C.c_uint32),
+        ("guide_count", C.c_uint32),
+        ("guide_cv_count", C.c_uint32),
+        ("guide_stencil_size", C.c_uint32),
+        ("noise_gradient_count", C.c_uint32),
+        ("positions_offset", C.c_uint64),
+        ("normals_offset", C.c_uint64),
+        ("texcoords_offset", C.c_uint64),
+        ("triangles_offset", C.c_uint64),
+        ("alias_table_offset", C.c_uint64),
+        ("guides_offset", C.c_uint64),
+        ("guide_cvs_offset", C.c_uint64),
+        ("triangle_guides_offset", C.c_uint64),
+        ("noise_gradients_offset", C.c_uint64),
+    ]
+
+
+class GenerationParams(C.Structure):
+    """types.h:108-124, defaults as in the reference."""
+    _fields_ = [
+        ("strand_count", C.c_uint32),
+        ("cvs_per_strand", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("guide_support_scale", C.c_float),
+        ("guide_weight_power", C.c_float),
+        ("normal_rejection_cos", C.c_float),
+        ("length_scale", C.c_float),
+        ("root_width", C.c_float),
+        ("tip_width", C.c_float),
+        ("noise_amplitude", C.c_float),
+        ("noise_frequency", C.c_float),
+        ("noise_mask", C.c_float),
+        ("noise_correlation", C.c_float),
+        ("noise_preserve_length", C.c_float),
+    ]
+
+    def __init__(self, **kw):
+        defaults = dict(strand_count=1024, cvs_per_strand=8, seed=1, reserved=0,
+                        guide_support_scale=1.0, guide_weight_power=2.0,
+                        normal_rejection_cos=-0.25, length_scale=1.0, root_width=0.01,
+                        tip_width=0.001, noise_amplitude=0.0, noise_frequency=2.0,
+                        noise_mask=1.0, noise_correlation=0.0, noise_preserve_length=0.0)
+        unknown = set(kw) - set(defaults)
+        if unknown:
+            raise TypeError(f"unknown GenerationParams fields: {sorted(unknown)}")
+        defaults.update(kw)
+        super().__init__(**defaults)
+
+    def copy(self, **kw):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d.update(kw)
+        return GenerationParams(**d)
+
+
+class NoiseOctaves(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32),
+        ("amplitude", C.c_float * MAX_NOISE_OCTAVES),
+        ("frequency", C.c_float * MAX_NOISE_OCTAVES),
+    ]
+
+    @classmethod
+    def of(cls, pairs):
+        o = cls()
+        o.count = len(pairs)
+        if len(pairs) > MAX_NOISE_OCTAVES:
+            raise ValueError("too many octaves")
+        for i, (a, f) in enumerate(pairs):
+            o.amplitude[i] = a
+            o.frequency[i] = f
+        return o
+
+
+class RootSample(C.Structure):
+    """types.h:126-135 (48 bytes)."""
+    _fields_ = [
+        ("position", Vec3),
+        ("normal", Vec3),
+        ("uv", Vec2),
+        ("triangle_index", C.c_uint32),
+        ("barycentric", Vec2),
+        ("surface_face_id", C.c_uint32),
+    ]
+
+
+class DeviceAsset(C.Structure):
+    """device_contract.h:14-18."""
+    _fields_ = [("data", C.c_void_p), ("header", AssetHeader), ("byte_capacity", C.c_uint64)]
+
+
+class DeformedGeometry(C.Structure):
+    _fields_ = [
+        ("positions", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("guide_cvs", C.c_void_p),
+        ("position_count", C.c_uint64),
+        ("normal_count", C.c_uint64),
+        ("guide_cv_count", C.c_uint64),
+    ]
+
+
+class PackedOutput(C.Structure):
+    """device_contract.h:34-40."""
+    _fields_ = [
+        ("points", C.c_void_p),
+        ("roots", C.c_void_p),
+        ("root_uvs", C.c_void_p),
+        ("radius_scale", C.c_float),
+        ("point_counts", C.c_void_p),
+        ("point_capacity", C.c_uint64),
+        ("root_capacity", C.c_uint64),
+        ("root_uv_capacity", C.c_uint64),
+        ("point_count_capacity", C.c_uint64),
+    ]
+
+
+class StencilOutput(C.Structure):
+    _fields_ = [("indices", C.c_void_p), ("weights", C.c_void_p), ("capacity", C.c_uint64)]
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("block_size", C.c_uint32)]
+
+
+class AssetBuildInput(C.Structure):
+    _fields_ = [
+        ("positions", C.c_void_p),
+        ("vertex_count", C.c_uint32),
+        ("triangle_count", C.c_uint32),
+        ("normals", C.c_void_p),
+        ("texcoords", C.c_void_p),
+        ("triangles", C.c_void_p),
+        ("guide_count", C.c_uint32),
+        ("stencil_k", C.c_uint32),
+        ("guide_cv_counts", C.c_void_p),
+        ("guide_cvs", C.c_void_p),
+        ("guide_root_normals", C.c_void_p),
+        ("guide_root_uvs", C.c_void_p),
+        ("guide_triangles", C.c_void_p),
+        ("guide_barycentrics", C.c_void_p),
+        ("guide_support_radii", C.c_void_p),
+        ("triangle_weights", C.c_void_p),
+    ]
+
+
+# numpy record views of the device/host arrays
+ROOT_DTYPE = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("uv", "<f4", 2),
+                       ("triangle_index", "<u4"), ("barycentric", "<f4", 2),
+                       ("surface_face_id", "<u4")])
+GUIDE_DTYPE = np.dtype([("first_cv", "<u4"), ("cv_count", "<u2"), ("flags", "<u2"),
+                        ("triangle_index", "<u4"), ("support_radius", "<f4"),
+                        ("barycentric", "<f4", 2), ("root_uv", "<f4", 2),
+                        ("root_position", "<f4", 3), ("root_normal", "<f4", 3)])
+
+assert C.sizeof(AssetHeader) == 128
+assert C.sizeof(GenerationParams) == 60
+assert C.sizeof(RootSample) == 48 and ROOT_DTYPE.itemsize == 48
+assert GUIDE_DTYPE.itemsize == 56
+
+# status codes (device_contract.h:58-80 then extensions)
+STATUS = [
+    "OK", "NULL_ASSET", "MISALIGNED_ASSET", "INVALID_ASSET_METADATA", "ASSET_CAPACITY_TOO_SMALL",
+    "INVALID_GENERATION_PARAMETERS", "INVALID_BLOCK_SIZE", "DEFORMED_POSITION_COUNT_MISMATCH",
+    "DEFORMED_NORMAL_COUNT_MISMATCH", "DEFORMED_GUIDE_CV_COUNT_MISMATCH", "MISSING_OUTPUT",
+    "POINT_CAPACITY_TOO_SMALL", "WIDTH_CAPACITY_TOO_SMALL", "ROOT_CAPACITY_TOO_SMALL",
+    "ROOT_UV_CAPACITY_TOO_SMALL", "POINT_COUNT_CAPACITY_TOO_SMALL", "INVALID_RADIUS_SCALE",
+    "MISSING_MOTION_SAMPLES", "INVALID_MOTION_TIMES", "MOTION_POINT_CAPACITY_TOO_SMALL",
+    "OUTPUT_SIZE_OVERFLOW", "HIP", "WORKSPACE_TOO_SMALL", "INVALID_OCTAVES", "NULL_ARGUMENT",
+    "STENCIL_CAPACITY_TOO_SMALL", "INVALID_ROOTS", "INVALID_BUILD_INPUT", "BUFFER_TOO_SMALL",
+    "INVALID_ASSET",
+]
+globals().update({name: i for i, name in enumerate(STATUS)})
+
+
+def ptr(a):
+    """Address of a numpy array's data (or None for None)."""
+    if a is None:
+        return None
+    return a.ctypes.data
+
+
+class BuildInputArrays:
+    """Owns contiguous numpy arrays for an AssetBuildInput and exposes the struct."""
+
+    def __init__(self, positions, triangles, guide_cvs, guide_cv_counts, *, normals=None,
+                 texcoords=None, guide_root_normals=None, guide_root_uvs=None,
+                 guide_triangles=None, guide_barycentrics=None, guide_support_radii=None,
+                 stencil_k=0, triangle_weights=None):
+        f = lambda a, n: None if a is None else np.ascontiguousarray(a, dtype=np.float32).reshape(-1, n)
+        self.positions = f(positions, 3)
+        self.triangles = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
+        self.guide_cvs = f(guide_cvs, 3)
+        self.guide_cv_counts = np.ascontiguousarray(guide_cv_counts, dtype=np.uint32)
+        self.normals = f(normals, 3)
+        self.texcoords = f(texcoords, 2)
+        self.guide_root_normals = f(guide_root_normals, 3)
+        self.guide_root_uvs = f(guide_root_uvs, 2)
+        self.guide_triangles = None if guide_triangles is None else np.ascontiguousarray(guide_triangles, dtype=np.uint32)
+        self.guide_barycentrics = f(guide_barycentrics, 2)
+        self.guide_support_radii = None if guide_support_radii is None else np.ascontiguousarray(guide_support_radii, dtype=np.float32)
+        self.triangle_weights = None if triangle_weights is None else np.ascontiguousarray(triangle_weights, dtype=np.float32)
+        self.struct = AssetBuildInput(
+            positions=ptr(self.positions), vertex_count=len(self.positions),
+            triangle_count=len(self.triangles), normals=ptr(self.normals),
+            texcoords=ptr(self.texcoords), triangles=ptr(self.triangles),
+            guide_count=len(self.guide_cv_counts), stencil_k=stencil_k,
+            guide_cv_counts=ptr(self.guide_cv_counts), guide_cvs=ptr(self.guide_cvs),
+            guide_root_normals=ptr(self.guide_root_normals), guide_root_uvs=ptr(self.guide_root_uvs),
+            guide_triangles=ptr(self.guide_triangles), guide_barycentrics=ptr(self.guide_barycentrics),
+            guide_support_radii=ptr(self.guide_support_radii), triangle_weights=ptr(self.triangle_weights))
+
+
+def header_of(blob: np.ndarray) -> AssetHeader:
+    return AssetHeader.from_buffer_copy(blob[:128].tobytes())
+
+
+def aligned_bytes(size, alignment=64):
+    """A zeroed uint8 numpy array of `size` bytes whose data is `alignment`-aligned."""
+    raw = np.zeros(size + alignment, dtype=np.uint8)
+    off = (-raw.ctypes.data) % alignment
+    return raw[off:off + size]

@@ -1,0 +1,169 @@
+// Checked device contract of the C ABI: status messages and request
+// validation, restating include/nanoxgen/device_contract.h:58-254 of the
+// reference in C-ABI terms (status codes 0..20 are its DeviceGenerationError
+// values; 21.. are extensions).
+#include <cmath>
+#include <cstdint>
+#include <limits>
+
+#include "internal.h"
+
+namespace nxg {
+
+namespace {
+constexpr uint64_t kHeaderBytes = sizeof(nxg_asset_header);
+
+uint64_t align256(uint64_t v) { return (v + 255u) & ~uint64_t{255u}; }
+}  // namespace
+
+bool valid_generation_parameters(const nxg_generation_params &p) {
+    using std::isfinite;
+    return p.strand_count != 0u && p.cvs_per_strand >= 2u && isfinite(p.guide_support_scale) &&
+           p.guide_support_scale >= 0.0f && isfinite(p.guide_weight_power) &&
+           p.guide_weight_power >= 0.0f && isfinite(p.normal_rejection_cos) &&
+           p.normal_rejection_cos >= -1.0f && p.normal_rejection_cos <= 1.0f &&
+           isfinite(p.length_scale) && p.length_scale >= 0.0f && isfinite(p.root_width) &&
+           p.root_width >= 0.0f && isfinite(p.tip_width) && p.tip_width >= 0.0f &&
+           isfinite(p.noise_amplitude) && p.noise_amplitude >= 0.0f &&
+           isfinite(p.noise_frequency) && p.noise_frequency >= 0.0f && isfinite(p.noise_mask) &&
+           p.noise_mask >= 0.0f && p.noise_mask <= 1.0f && isfinite(p.noise_correlation) &&
+           p.noise_correlation >= 0.0f && p.noise_correlation <= 1.0f &&
+           isfinite(p.noise_preserve_length) && p.noise_preserve_length >= 0.0f &&
+           p.noise_preserve_length <= 1.0f;
+}
+
+bool valid_octaves(const nxg_noise_octaves *o) {
+    if (!o) return true;
+    if (o->count > NXG_MAX_NOISE_OCTAVES) return false;
+    for (uint32_t k = 0; k < o->count; ++k) {
+        if (!std::isfinite(o->amplitude[k]) || o->amplitude[k] < 0.0f ||
+            !std::isfinite(o->frequency[k]) || o->frequency[k] < 0.0f)
+            return false;
+    }
+    return true;
+}
+
+uint32_t validate_common(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                         const nxg_generation_params *params, const nxg_launch_config *config) {
+    if (!asset || !params) return NXG_ERROR_NULL_ARGUMENT;
+    if (!asset->data) return NXG_ERROR_NULL_ASSET;
+    if (reinterpret_cast<uintptr_t>(asset->data) % NXG_BLOB_ALIGNMENT != 0u)
+        return NXG_ERROR_MISALIGNED_ASSET;
+    const nxg_asset_header &h = asset->header;
+    if (h.magic != NXG_MAGIC || h.version_major != NXG_VERSION_MAJOR ||
+        h.version_minor != NXG_VERSION_MINOR || h.byte_size < kHeaderBytes ||
+        h.vertex_count == 0u || h.triangle_count == 0u || h.guide_count == 0u ||
+        h.guide_cv_count == 0u || h.guide_stencil_size != NXG_GUIDE_STENCIL_SIZE ||
+        h.noise_gradient_count != NXG_NOISE_GRADIENT_COUNT ||
+        h.noise_gradients_offset < kHeaderBytes || h.noise_gradients_offset > h.byte_size ||
+        uint64_t{h.noise_gradient_count} * sizeof(nxg_vec3) > h.byte_size - h.noise_gradients_offset)
+        return NXG_ERROR_INVALID_ASSET_METADATA;
+    if (asset->byte_capacity < h.byte_size) return NXG_ERROR_ASSET_CAPACITY_TOO_SMALL;
+    if (!valid_generation_parameters(*params)) return NXG_ERROR_INVALID_GENERATION_PARAMETERS;
+    const uint32_t block = config ? config->block_size : 128u;
+    if (block == 0u || block > 1024u) return NXG_ERROR_INVALID_BLOCK_SIZE;
+    if (deformed) {
+        const nxg_deformed_geometry &d = *deformed;
+        if ((d.positions == nullptr) != (d.position_count == 0u) ||
+            (d.positions && d.position_count != h.vertex_count))
+            return NXG_ERROR_DEFORMED_POSITION_COUNT_MISMATCH;
+        if ((d.normals == nullptr) != (d.normal_count == 0u) ||
+            (d.normals && d.normal_count != h.vertex_count))
+            return NXG_ERROR_DEFORMED_NORMAL_COUNT_MISMATCH;
+        if ((d.guide_cvs == nullptr) != (d.guide_cv_count == 0u) ||
+            (d.guide_cvs && d.guide_cv_count != h.guide_cv_count))
+            return NXG_ERROR_DEFORMED_GUIDE_CV_COUNT_MISMATCH;
+    }
+    return NXG_OK;
+}
+
+uint32_t validate_packed_output(const nxg_generation_params &p, const nxg_packed_output *out) {
+    if (!out || !out->points) return NXG_ERROR_MISSING_OUTPUT;
+    if (!std::isfinite(out->radius_scale) || out->radius_scale < 0.0f)
+        return NXG_ERROR_INVALID_RADIUS_SCALE;
+    const uint64_t points = uint64_t{p.strand_count} * p.cvs_per_strand;
+    if (out->point_capacity < points) return NXG_ERROR_POINT_CAPACITY_TOO_SMALL;
+    auto optional_ok = [](const void *ptr, uint64_t cap, uint64_t need) {
+        return ptr ? cap >= need : cap == 0u;
+    };
+    if (!optional_ok(out->roots, out->root_capacity, p.strand_count))
+        return NXG_ERROR_ROOT_CAPACITY_TOO_SMALL;
+    if (!optional_ok(out->root_uvs, out->root_uv_capacity, p.strand_count))
+        return NXG_ERROR_ROOT_UV_CAPACITY_TOO_SMALL;
+    if (!optional_ok(out->point_counts, out->point_count_capacity, p.strand_count))
+        return NXG_ERROR_POINT_COUNT_CAPACITY_TOO_SMALL;
+    return NXG_OK;
+}
+
+WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand) {
+    WorkspaceLayout l{};
+    l.guide_state_offset = 0;
+    l.guide_offsets_offset = align256(uint64_t{guide_count} * 32u);
+    l.total = align256(l.guide_offsets_offset + uint64_t{guide_count} * cvs_per_strand * 16u);
+    return l;
+}
+
+}  // namespace nxg
+
+extern "C" {
+
+const char *nxg_status_message(uint32_t status) {
+    switch (status) {
+        case NXG_OK: return "";
+        case NXG_ERROR_NULL_ASSET: return "device asset pointer is null";
+        case NXG_ERROR_MISALIGNED_ASSET: return "device asset pointer is not 64-byte aligned";
+        case NXG_ERROR_INVALID_ASSET_METADATA: return "device asset metadata is invalid";
+        case NXG_ERROR_ASSET_CAPACITY_TOO_SMALL: return "device asset allocation is too small";
+        case NXG_ERROR_INVALID_GENERATION_PARAMETERS: return "generation parameters are invalid";
+        case NXG_ERROR_INVALID_BLOCK_SIZE: return "GPU block size must be in [1, 1024]";
+        case NXG_ERROR_DEFORMED_POSITION_COUNT_MISMATCH:
+            return "deformed position count does not match the asset";
+        case NXG_ERROR_DEFORMED_NORMAL_COUNT_MISMATCH:
+            return "deformed normal count does not match the asset";
+        case NXG_ERROR_DEFORMED_GUIDE_CV_COUNT_MISMATCH:
+            return "deformed guide-CV count does not match the asset";
+        case NXG_ERROR_MISSING_OUTPUT: return "no device output was supplied";
+        case NXG_ERROR_POINT_CAPACITY_TOO_SMALL: return "device point output is too small";
+        case NXG_ERROR_WIDTH_CAPACITY_TOO_SMALL: return "device width output is too small";
+        case NXG_ERROR_ROOT_CAPACITY_TOO_SMALL: return "device root output is too small";
+        case NXG_ERROR_ROOT_UV_CAPACITY_TOO_SMALL: return "device root-UV output is too small";
+        case NXG_ERROR_POINT_COUNT_CAPACITY_TOO_SMALL:
+            return "device point-count output is too small";
+        case NXG_ERROR_INVALID_RADIUS_SCALE:
+            return "device curve radius scale must be finite and non-negative";
+        case NXG_ERROR_MISSING_MOTION_SAMPLES: return "no motion samples were supplied";
+        case NXG_ERROR_INVALID_MOTION_TIMES:
+            return "motion sample times must be finite and strictly increasing";
+        case NXG_ERROR_MOTION_POINT_CAPACITY_TOO_SMALL: return "device motion-point output is too small";
+        case NXG_ERROR_OUTPUT_SIZE_OVERFLOW: return "device output size overflows uint64";
+        case NXG_ERROR_HIP: return "a HIP runtime call failed";
+        case NXG_ERROR_WORKSPACE_TOO_SMALL: return "device workspace is too small";
+        case NXG_ERROR_INVALID_OCTAVES:
+            return "noise octaves must number at most 4 with finite non-negative amplitude and frequency";
+        case NXG_ERROR_NULL_ARGUMENT: return "a required argument pointer is null";
+        case NXG_ERROR_STENCIL_CAPACITY_TOO_SMALL: return "device stencil output is too small";
+        case NXG_ERROR_INVALID_ROOTS: return "explicit roots are missing";
+        case NXG_ERROR_INVALID_BUILD_INPUT: return "asset build input is invalid";
+        case NXG_ERROR_BUFFER_TOO_SMALL: return "output buffer is too small";
+        case NXG_ERROR_INVALID_ASSET: return "asset is invalid";
+    }
+    return "unknown device generation error";
+}
+
+uint32_t nxg_validate_packed_generation(const nxg_device_asset *asset,
+                                        const nxg_deformed_geometry *deformed,
+                                        const nxg_generation_params *params,
+                                        const nxg_packed_output *output,
+                                        const nxg_launch_config *config) {
+    const uint32_t common = nxg::validate_common(asset, deformed, params, config);
+    if (common != NXG_OK) return common;
+    return nxg::validate_packed_output(*params, output);
+}
+
+uint64_t nxg_packed_generation_workspace_size(const nxg_asset_header *header,
+                                              const nxg_generation_params *params) {
+    if (!header || !params) return 0;
+    return nxg::workspace_layout(header->guide_count, params->cvs_per_strand).total;
+}
+
+}  // extern "C"

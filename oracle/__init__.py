@@ -1,0 +1,235 @@
+"""ORACLE (test infrastructure only).
+
+ctypes access to the C restatement of the reference's strand-generation path
+(liboracle.so, built from oracle/*.c by oracle/Makefile) and, where it was
+built in this container, to the reference's own CPU core (_ref/libnxg_ref.so).
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this package, and only as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from nanoxgen_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+
+def build(force=False):
+    """Compile liboracle.so (and _ref/ when the reference sources are present)."""
+    if force or not os.path.exists(os.path.join(HERE, "liboracle.so")):
+        subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    elif force is None:
+        pass
+    if os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = C.CDLL(path)
+        L.orc_hash32.restype = C.c_uint32
+        L.orc_hash32.argtypes = [C.c_uint32]
+        L.orc_random01.restype = C.c_float
+        L.orc_random01.argtypes = [C.c_uint32] * 3
+        L.orc_noise_hash.restype = C.c_uint32
+        L.orc_noise_hash.argtypes = [C.c_int32] * 3
+        L.orc_fnv1a.restype = C.c_uint64
+        L.orc_fnv1a.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_gradient_noise_f32.restype = C.c_float
+        L.orc_gradient_noise_f32.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
+        L.orc_gradient_noise_f64.restype = C.c_double
+        L.orc_gradient_noise_f64.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.orc_generate.restype = C.c_int
+        L.orc_generate.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_build_asset.restype = C.c_int
+        L.orc_build_asset.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
+                                      C.c_char_p, C.c_uint64]
+        L.orc_validate_asset.restype = C.c_int
+        L.orc_validate_asset.argtypes = [C.c_void_p, C.c_uint64, C.c_char_p, C.c_uint64]
+        L.orc_free.argtypes = [C.c_void_p]
+        L.orc_strand_state_f32.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        L.orc_strand_state_f64.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def ref():
+    """The reference's own CPU core, or None where it was not built here."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libnxg_ref.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        L.ref_build_asset.restype = C.c_int
+        L.ref_build_asset.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
+                                      C.c_char_p, C.c_uint64]
+        L.ref_free.argtypes = [C.c_void_p]
+        L.ref_validate_asset.restype = C.c_int
+        L.ref_validate_asset.argtypes = [C.c_void_p, C.c_uint64, C.c_char_p, C.c_uint64]
+        L.ref_generate_packed.restype = C.c_int
+        L.ref_generate_packed.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_float, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint64]
+        L.ref_gradient_noise.restype = C.c_float
+        L.ref_gradient_noise.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
+        L.ref_blend.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ref_strand_state.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ref_hash32.restype = C.c_uint32
+        L.ref_hash32.argtypes = [C.c_uint32]
+        L.ref_random01.restype = C.c_float
+        L.ref_random01.argtypes = [C.c_uint32] * 3
+        _REF = L
+    return _REF
+
+
+def _take_blob(L, free, fn, build_input):
+    out = C.c_void_p()
+    size = C.c_uint64()
+    err = C.create_string_buffer(512)
+    rc = fn(C.byref(build_input.struct), C.byref(out), C.byref(size), err, 512)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    blob = abi.aligned_bytes(size.value)
+    C.memmove(blob.ctypes.data, out.value, size.value)
+    free(out)
+    return blob
+
+
+def build_asset(build_input):
+    """Oracle build_asset (asset.cpp:229-293) -> 64-aligned uint8 blob."""
+    L = lib()
+    return _take_blob(L, L.orc_free, L.orc_build_asset, build_input)
+
+
+def ref_build_asset(build_input):
+    R = ref()
+    return _take_blob(R, R.ref_free, R.ref_build_asset, build_input)
+
+
+def validate_asset(blob):
+    msg = C.create_string_buffer(512)
+    lib().orc_validate_asset(blob.ctypes.data, blob.nbytes, msg, 512)
+    return msg.value.decode()
+
+
+def ref_validate_asset(blob):
+    msg = C.create_string_buffer(512)
+    ref().ref_validate_asset(blob.ctypes.data, blob.nbytes, msg, 512)
+    return msg.value.decode()
+
+
+class _Job(C.Structure):
+    _fields_ = [("blob", C.c_void_p), ("deformed", C.c_void_p), ("params", C.c_void_p),
+                ("octaves", C.c_void_p), ("explicit_roots", C.c_void_p), ("radius_scale", C.c_float),
+                ("points", C.c_void_p), ("roots", C.c_void_p), ("root_uvs", C.c_void_p),
+                ("point_counts", C.c_void_p), ("stencil_indices", C.c_void_p),
+                ("stencil_weights", C.c_void_p), ("strand_list", C.c_void_p),
+                ("list_count", C.c_uint32)]
+
+
+def _deformed_struct(blob, deformed):
+    if deformed is None:
+        return None, []
+    h = abi.header_of(blob)
+    keep = []
+    d = abi.DeformedGeometry()
+    for name, count in (("positions", h.vertex_count), ("normals", h.vertex_count),
+                        ("guide_cvs", h.guide_cv_count)):
+        a = deformed.get(name)
+        if a is not None:
+            a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1, 3)
+            keep.append(a)
+            setattr(d, name, a.ctypes.data)
+            setattr(d, {"positions": "position_count", "normals": "normal_count",
+                        "guide_cvs": "guide_cv_count"}[name], len(a))
+    keep.append(d)
+    return d, keep
+
+
+def generate(blob, params, *, octaves=None, deformed=None, roots=None, radius_scale=1.0,
+             precision=32, workers=0, block=128, stencil=False, strands=None):
+    """Run the oracle's packed generation.  precision 32: the float
+    restatement (points float32 [N*C,4], roots ROOT_DTYPE); 64: the FP64 twin
+    (points float64 [N*C,4], roots float64 [N,12]).  `strands` (optional)
+    selects a subset of strand indices; outputs are then in that order."""
+    L = lib()
+    slist = None if strands is None else np.ascontiguousarray(strands, dtype=np.uint32)
+    n = params.strand_count if slist is None else len(slist)
+    cvs = params.cvs_per_strand
+    dt = np.float32 if precision == 32 else np.float64
+    points = np.zeros((n * cvs, 4), dtype=dt)
+    root_arr = np.zeros(n, dtype=abi.ROOT_DTYPE) if precision == 32 else np.zeros((n, 12))
+    uvs = np.zeros((n, 2), dtype=np.float32)
+    counts = np.zeros(n, dtype=np.uint32)
+    sidx = np.zeros((n, 8), dtype=np.uint32) if stencil else None
+    sw = np.zeros((n, 8), dtype=np.float32) if stencil else None
+    d, keep = _deformed_struct(blob, deformed)
+    eroots = None if roots is None else np.ascontiguousarray(roots)
+    job = _Job(blob=blob.ctypes.data, deformed=C.addressof(d) if d is not None else None,
+               params=C.addressof(params), octaves=C.addressof(octaves) if octaves is not None else None,
+               explicit_roots=abi.ptr(eroots), radius_scale=radius_scale, points=points.ctypes.data,
+               roots=root_arr.ctypes.data, root_uvs=uvs.ctypes.data if precision == 32 else None,
+               point_counts=counts.ctypes.data, stencil_indices=abi.ptr(sidx),
+               stencil_weights=abi.ptr(sw), strand_list=abi.ptr(slist),
+               list_count=0 if slist is None else len(slist))
+    L.orc_generate(C.byref(job), precision, workers, block)
+    del keep
+    out = dict(points=points, roots=root_arr, point_counts=counts)
+    if precision == 32:
+        out["root_uvs"] = uvs
+    if stencil:
+        out["stencil_indices"] = sidx
+        out["stencil_weights"] = sw
+    return out
+
+
+def ref_generate(blob, params, *, deformed=None, roots=None, radius_scale=1.0, workers=0, block=128):
+    """The reference's generate_packed_cpu / _deformed_cpu / _roots_cpu."""
+    R = ref()
+    n, cvs = params.strand_count, params.cvs_per_strand
+    points = np.zeros((n * cvs, 4), dtype=np.float32)
+    root_arr = np.zeros(n, dtype=abi.ROOT_DTYPE)
+    uvs = np.zeros((n, 2), dtype=np.float32)
+    counts = np.zeros(n, dtype=np.uint32)
+    d, keep = _deformed_struct(blob, deformed)
+    eroots = None if roots is None else np.ascontiguousarray(roots)
+    err = C.create_string_buffer(512)
+    rc = R.ref_generate_packed(blob.ctypes.data, blob.nbytes, C.addressof(params),
+                               C.addressof(d) if d is not None else None, abi.ptr(eroots),
+                               radius_scale, workers, block, points.ctypes.data, root_arr.ctypes.data,
+                               uvs.ctypes.data, counts.ctypes.data, err, 512)
+    del keep
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return dict(points=points, roots=root_arr, root_uvs=uvs, point_counts=counts)
+
+
+def gradients_of(blob):
+    h = abi.header_of(blob)
+    return blob[h.noise_gradients_offset:h.noise_gradients_offset + 256 * 12]
+
+
+def gradient_noise(blob, x, y, z, precision=32):
+    g = gradients_of(blob)
+    if precision == 32:
+        return lib().orc_gradient_noise_f32(g.ctypes.data, x, y, z)
+    return lib().orc_gradient_noise_f64(g.ctypes.data, x, y, z)
+
+
+def strand_state(blob, params, strand, octaves=None, precision=32):
+    job = _Job(blob=blob.ctypes.data, params=C.addressof(params),
+               octaves=C.addressof(octaves) if octaves is not None else None)
+    out = np.zeros(8, dtype=np.float32 if precision == 32 else np.float64)
+    fn = lib().orc_strand_state_f32 if precision == 32 else lib().orc_strand_state_f64
+    fn(C.byref(job), strand, out.ctypes.data)
+    return out

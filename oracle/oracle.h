@@ -1,0 +1,75 @@
+/*
+ * ORACLE (test infrastructure only): C restatement of the reference's
+ * strand-generation path, used by tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg as the checker.  The shipped library
+ * (nanoxgen_amd/) never links or calls it.
+ *
+ * Layout and types come from include/nanoxgen_amd.h (the .nxg format of
+ * include/nanoxgen/types.h:77-101).
+ */
+#ifndef NXG_ORACLE_H
+#define NXG_ORACLE_H
+
+#include <stdint.h>
+#include "../include/nanoxgen_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One generation request (host pointers).  The f32 driver writes float
+ * points (4 per CV), nxg_root_sample roots and root_uvs; the f64 twin
+ * writes double points (4 per CV) and 12 doubles per root. */
+typedef struct orc_job {
+    const void *blob;
+    const nxg_deformed_geometry *deformed;
+    const nxg_generation_params *params;
+    const nxg_noise_octaves *octaves;
+    const nxg_root_sample *explicit_roots;
+    float radius_scale;
+    void *points;
+    void *roots;
+    nxg_vec2 *root_uvs;
+    uint32_t *point_counts;
+    uint32_t *stencil_indices;
+    float *stencil_weights;
+    /* optional: generate only strands strand_list[0..list_count) into
+     * output slots 0..list_count (explicit_roots then index by slot) */
+    const uint32_t *strand_list;
+    uint32_t list_count;
+} orc_job;
+
+/* generate.h:72-84 */
+uint32_t orc_hash32(uint32_t x);
+float orc_random01(uint32_t seed, uint32_t strand, uint32_t dimension);
+/* generate.h:353-364 */
+uint32_t orc_noise_hash(int32_t x, int32_t y, int32_t z);
+/* asset.cpp:34-41 */
+uint64_t orc_fnv1a(const void *bytes, uint64_t size);
+
+float orc_gradient_noise_f32(const nxg_vec3 *gradients, float x, float y, float z);
+double orc_gradient_noise_f64(const nxg_vec3 *gradients, double x, double y, double z);
+void orc_generate_strand_f32(const orc_job *job, uint32_t strand, uint64_t slot);
+void orc_generate_strand_f64(const orc_job *job, uint32_t strand, uint64_t slot);
+void orc_strand_state_f32(const orc_job *job, uint32_t strand, float *out8);
+void orc_strand_state_f64(const orc_job *job, uint32_t strand, double *out8);
+
+/* asset.cpp:160-202: persistent 128-strand tiles claimed from an atomic
+ * counter by `workers` threads (0 = min(hardware, ceil(tiles/4))).
+ * precision: 32 runs the restatement, 64 the FP64 twin. */
+int orc_generate(const orc_job *job, uint32_t precision, uint32_t workers,
+                 uint32_t strands_per_block);
+
+/* asset.cpp:229-293 (build_asset) into a malloc'd 64-aligned blob the
+ * caller frees with orc_free.  Returns 0, or 1 with the reference's
+ * exception message in err. */
+int orc_build_asset(const nxg_asset_build_input *in, void **blob, uint64_t *size,
+                    char *err, uint64_t err_capacity);
+void orc_free(void *p);
+/* asset.cpp:295-338: 0 and "" when valid, else 1 and the message. */
+int orc_validate_asset(const void *blob, uint64_t size, char *msg, uint64_t msg_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,391 @@
+#define _POSIX_C_SOURCE 200809L
+/*
+ * ORACLE (test infrastructure only): integer hashes, the .nxg asset builder
+ * and validator, and the multithreaded CPU driver, restated in C from the
+ * reference.  Used by tests/, smoke() and bench.py's CPU-baseline leg; never
+ * linked into the shipped library.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "oracle.h"
+
+/* generate.h:72-79 */
+uint32_t orc_hash32(uint32_t x) {
+    x ^= x >> 16u;
+    x *= 0x7feb352du;
+    x ^= x >> 15u;
+    x *= 0x846ca68bu;
+    x ^= x >> 16u;
+    return x;
+}
+
+/* generate.h:81-84 */
+float orc_random01(uint32_t seed, uint32_t strand, uint32_t dimension) {
+    const uint32_t h = orc_hash32(seed ^ orc_hash32(strand + 0x9e3779b9u * (dimension + 1u)));
+    return (float)(h >> 8u) * (1.0f / 16777216.0f);
+}
+
+/* generate.h:353-364 */
+uint32_t orc_noise_hash(int32_t x, int32_t y, int32_t z) {
+    uint32_t seed = 0u;
+    seed = seed * 1664525u + (uint32_t)x + 1013904223u;
+    seed = seed * 1664525u + (uint32_t)y + 1013904223u;
+    seed = seed * 1664525u + (uint32_t)z + 1013904223u;
+    seed ^= seed >> 11u;
+    seed ^= (seed << 7u) & 0x9d2c5680u;
+    seed ^= (seed << 15u) & 0xefc60000u;
+    seed ^= seed >> 18u;
+    return (((seed & 0x00ff0000u) >> 4u) + (seed & 0xffu)) & 0xffu;
+}
+
+/* asset.cpp:34-41 */
+uint64_t orc_fnv1a(const void *bytes, uint64_t size) {
+    const uint8_t *b = (const uint8_t *)bytes;
+    uint64_t hash = 1469598103934665603ull;
+    for (uint64_t i = 0; i < size; ++i) {
+        hash ^= b[i];
+        hash *= 1099511628211ull;
+    }
+    return hash;
+}
+
+void orc_free(void *p) { free(p); }
+
+/* ------------------------------------------------------------------------ */
+/* asset builder: asset.cpp:43-97, 99-150, 229-293                           */
+
+typedef struct { float x, y, z; } f3;
+static f3 F3(nxg_vec3 v) { f3 r = {v.x, v.y, v.z}; return r; }
+static f3 fadd(f3 a, f3 b) { f3 r = {a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
+static f3 fsub(f3 a, f3 b) { f3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static f3 fdiv(f3 a, float s) { f3 r = {a.x / s, a.y / s, a.z / s}; return r; }
+static float fdot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static f3 fcross(f3 a, f3 b) {
+    f3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return r;
+}
+static f3 fnormalize(f3 v) {
+    const float l2 = fdot(v, v);
+    if (l2 > 1.0e-20f) return fdiv(v, sqrtf(l2));
+    f3 up = {0.0f, 1.0f, 0.0f};
+    return up;
+}
+static nxg_vec3 V(f3 a) { nxg_vec3 r = {a.x, a.y, a.z}; return r; }
+
+static int fail(char *err, uint64_t cap, const char *msg) {
+    if (err && cap) { snprintf(err, (size_t)cap, "%s", msg); }
+    return 1;
+}
+
+typedef struct { float key; uint32_t index; } cand;
+static int cand_cmp(const void *pa, const void *pb) {
+    const cand *a = (const cand *)pa, *b = (const cand *)pb;
+    if (a->key < b->key) return -1;
+    if (b->key < a->key) return 1;
+    return a->index < b->index ? -1 : (a->index > b->index ? 1 : 0);
+}
+
+static uint64_t align16(uint64_t v) { return (v + 15u) & ~(uint64_t)15u; }
+
+int orc_build_asset(const nxg_asset_build_input *in, void **out_blob, uint64_t *out_size,
+                    char *err, uint64_t err_cap) {
+    *out_blob = NULL;
+    *out_size = 0;
+    if (!in->positions || in->vertex_count == 0u) return fail(err, err_cap, "asset needs vertices");
+    if (!in->triangles || in->triangle_count == 0u) return fail(err, err_cap, "asset needs triangles");
+    if (in->guide_count == 0u) return fail(err, err_cap, "asset needs at least one guide");
+    for (uint32_t i = 0; i < in->triangle_count; ++i) {
+        const nxg_uint3 t = in->triangles[i];
+        if (t.x >= in->vertex_count || t.y >= in->vertex_count || t.z >= in->vertex_count)
+            return fail(err, err_cap, "triangle index out of bounds");
+    }
+    uint64_t total_cvs = 0;
+    for (uint32_t g = 0; g < in->guide_count; ++g) {
+        if (in->guide_cv_counts[g] < 2u || in->guide_cv_counts[g] > 65535u)
+            return fail(err, err_cap, "each guide needs 2..65535 CVs");
+        total_cvs += in->guide_cv_counts[g];
+    }
+    if (in->stencil_k > 8u) return fail(err, err_cap, "stencil_k must be in 0..8");
+    if (in->triangle_weights) {
+        for (uint32_t i = 0; i < in->triangle_count; ++i) {
+            const float w = in->triangle_weights[i];
+            if (!isfinite(w) || w < 0.0f)
+                return fail(err, err_cap, "triangle weights must be finite and non-negative");
+        }
+    }
+    const uint32_t V_ = in->vertex_count, T = in->triangle_count, G = in->guide_count;
+
+    /* make_normals (asset.cpp:75-86) */
+    nxg_vec3 *normals = (nxg_vec3 *)calloc(V_, sizeof(nxg_vec3));
+    if (in->normals) {
+        memcpy(normals, in->normals, (size_t)V_ * sizeof(nxg_vec3));
+    } else {
+        for (uint32_t i = 0; i < T; ++i) {
+            const nxg_uint3 t = in->triangles[i];
+            const f3 p0 = F3(in->positions[t.x]);
+            const f3 n = fcross(fsub(F3(in->positions[t.y]), p0), fsub(F3(in->positions[t.z]), p0));
+            normals[t.x] = V(fadd(F3(normals[t.x]), n));
+            normals[t.y] = V(fadd(F3(normals[t.y]), n));
+            normals[t.z] = V(fadd(F3(normals[t.z]), n));
+        }
+        for (uint32_t i = 0; i < V_; ++i) normals[i] = V(fnormalize(F3(normals[i])));
+    }
+
+    /* make_alias_table (asset.cpp:43-73); weights multiply the area */
+    double *scaled = (double *)malloc((size_t)T * sizeof(double));
+    double area_sum = 0.0;
+    for (uint32_t i = 0; i < T; ++i) {
+        const nxg_uint3 t = in->triangles[i];
+        const f3 a = F3(in->positions[t.x]);
+        const f3 c = fcross(fsub(F3(in->positions[t.y]), a), fsub(F3(in->positions[t.z]), a));
+        const float area = 0.5f * sqrtf(fdot(c, c));
+        scaled[i] = (double)area;
+        if (in->triangle_weights) scaled[i] = scaled[i] * (double)in->triangle_weights[i];
+        area_sum += scaled[i];
+    }
+    if (!(area_sum > 0.0)) {
+        free(normals); free(scaled);
+        return fail(err, err_cap, "mesh has zero total area");
+    }
+    for (uint32_t i = 0; i < T; ++i) scaled[i] = scaled[i] * (double)T / area_sum;
+    nxg_alias_entry *alias = (nxg_alias_entry *)calloc(T, sizeof(nxg_alias_entry));
+    uint32_t *small = (uint32_t *)malloc((size_t)T * 4u), *large = (uint32_t *)malloc((size_t)T * 4u);
+    uint32_t ns = 0, nl = 0;
+    for (uint32_t i = 0; i < T; ++i) {
+        if (scaled[i] < 1.0) small[ns++] = i; else large[nl++] = i;
+    }
+    while (ns > 0 && nl > 0) {
+        const uint32_t s = small[--ns];
+        const uint32_t l = large[--nl];
+        alias[s].probability = (float)scaled[s];
+        alias[s].alias = l;
+        scaled[l] = scaled[l] + scaled[s] - 1.0;
+        if (scaled[l] < 1.0) small[ns++] = l; else large[nl++] = l;
+    }
+    for (uint32_t i = 0; i < nl; ++i) { alias[large[i]].probability = 1.0f; alias[large[i]].alias = large[i]; }
+    for (uint32_t i = 0; i < ns; ++i) { alias[small[i]].probability = 1.0f; alias[small[i]].alias = small[i]; }
+    free(small); free(large); free(scaled);
+
+    /* guide records (asset.cpp:234-250, automatic radius 88-97) */
+    nxg_guide_record *guides = (nxg_guide_record *)calloc(G, sizeof(nxg_guide_record));
+    uint64_t *first = (uint64_t *)malloc((size_t)G * sizeof(uint64_t));
+    {
+        uint64_t acc = 0;
+        for (uint32_t g = 0; g < G; ++g) { first[g] = acc; acc += in->guide_cv_counts[g]; }
+    }
+    for (uint32_t g = 0; g < G; ++g) {
+        nxg_guide_record *r = &guides[g];
+        r->first_cv = (uint32_t)first[g];
+        r->cv_count = (uint16_t)in->guide_cv_counts[g];
+        r->flags = 0;
+        r->triangle_index = in->guide_triangles ? in->guide_triangles[g] : NXG_INVALID_INDEX;
+        float radius = in->guide_support_radii ? in->guide_support_radii[g] : 0.0f;
+        if (!(radius > 0.0f)) {
+            const f3 root = F3(in->guide_cvs[first[g]]);
+            float nearest = 3.402823466e+38f;
+            for (uint32_t j = 0; j < G; ++j) {
+                if (j == g) continue;
+                const f3 d = fsub(root, F3(in->guide_cvs[first[j]]));
+                const float dist = sqrtf(fdot(d, d));
+                nearest = dist < nearest ? dist : nearest;
+            }
+            /* asset.cpp:96: FLT_MAX is finite, so a lone guide gets
+             * FLT_MAX * 1.75f = +inf. */
+            radius = isfinite(nearest) ? fmaxf(nearest * 1.75f, 1.0e-4f) : 1.0f;
+        }
+        r->support_radius = radius;
+        if (in->guide_barycentrics) r->barycentric = in->guide_barycentrics[g];
+        if (in->guide_root_uvs) r->root_uv = in->guide_root_uvs[g];
+        r->root_position = in->guide_cvs[first[g]];
+        nxg_vec3 up = {0.0f, 1.0f, 0.0f};
+        r->root_normal = V(fnormalize(F3(in->guide_root_normals ? in->guide_root_normals[g] : up)));
+    }
+
+    /* triangle guide stencil (asset.cpp:252-269): the k smallest
+     * (d^2 + penalty, index) pairs in ascending order */
+    const uint32_t k_req = in->stencil_k == 0u ? 8u : in->stencil_k;
+    uint32_t *stencil = (uint32_t *)malloc((size_t)T * 8u * 4u);
+    cand *cands = (cand *)malloc((size_t)G * sizeof(cand));
+    for (uint32_t ti = 0; ti < T; ++ti) {
+        const nxg_uint3 t = in->triangles[ti];
+        const f3 p0 = F3(in->positions[t.x]), p1 = F3(in->positions[t.y]), p2 = F3(in->positions[t.z]);
+        const f3 centroid = fdiv(fadd(fadd(p0, p1), p2), 3.0f);
+        const f3 fn = fnormalize(fcross(fsub(p1, p0), fsub(p2, p0)));
+        for (uint32_t g = 0; g < G; ++g) {
+            const float penalty = fdot(fn, F3(guides[g].root_normal)) < -0.25f ? 1.0e20f : 0.0f;
+            const f3 d = fsub(centroid, F3(guides[g].root_position));
+            cands[g].key = fdot(d, d) + penalty;
+            cands[g].index = g;
+        }
+        qsort(cands, G, sizeof(cand), cand_cmp);
+        const uint32_t count = k_req < G ? k_req : G;
+        for (uint32_t j = 0; j < 8u; ++j)
+            stencil[(uint64_t)ti * 8u + j] = j < count ? cands[j].index : NXG_INVALID_INDEX;
+    }
+    free(cands);
+
+    /* blob assembly (asset.cpp:19-32, 271-291) */
+    nxg_asset_header h;
+    memset(&h, 0, sizeof(h));
+    h.magic = NXG_MAGIC;
+    h.version_major = NXG_VERSION_MAJOR;
+    h.version_minor = NXG_VERSION_MINOR;
+    h.flags = NXG_ASSET_HAS_NORMALS | (in->texcoords ? NXG_ASSET_HAS_TEXCOORDS : 0u);
+    h.vertex_count = V_;
+    h.triangle_count = T;
+    h.guide_count = G;
+    h.guide_cv_count = (uint32_t)total_cvs;
+    h.guide_stencil_size = 8u;
+    h.noise_gradient_count = NXG_NOISE_GRADIENT_COUNT;
+    uint64_t size = sizeof(nxg_asset_header);
+#define PLACE(field, bytes) do { size = align16(size); h.field = size; size += (bytes); } while (0)
+    PLACE(positions_offset, (uint64_t)V_ * 12u);
+    PLACE(normals_offset, (uint64_t)V_ * 12u);
+    if (in->texcoords) PLACE(texcoords_offset, (uint64_t)V_ * 8u);
+    PLACE(triangles_offset, (uint64_t)T * 12u);
+    PLACE(alias_table_offset, (uint64_t)T * 8u);
+    PLACE(guides_offset, (uint64_t)G * sizeof(nxg_guide_record));
+    PLACE(guide_cvs_offset, total_cvs * 12u);
+    PLACE(triangle_guides_offset, (uint64_t)T * 32u);
+    PLACE(noise_gradients_offset, (uint64_t)NXG_NOISE_GRADIENT_COUNT * 12u);
+#undef PLACE
+    h.byte_size = size;
+    uint8_t *blob = NULL;
+    if (posix_memalign((void **)&blob, 64, (size_t)size) != 0) {
+        free(normals); free(alias); free(guides); free(first); free(stencil);
+        return fail(err, err_cap, "out of memory");
+    }
+    memset(blob, 0, (size_t)size);
+    memcpy(blob + h.positions_offset, in->positions, (size_t)V_ * 12u);
+    memcpy(blob + h.normals_offset, normals, (size_t)V_ * 12u);
+    if (in->texcoords) memcpy(blob + h.texcoords_offset, in->texcoords, (size_t)V_ * 8u);
+    memcpy(blob + h.triangles_offset, in->triangles, (size_t)T * 12u);
+    memcpy(blob + h.alias_table_offset, alias, (size_t)T * 8u);
+    memcpy(blob + h.guides_offset, guides, (size_t)G * sizeof(nxg_guide_record));
+    memcpy(blob + h.guide_cvs_offset, in->guide_cvs, (size_t)total_cvs * 12u);
+    memcpy(blob + h.triangle_guides_offset, stencil, (size_t)T * 32u);
+    {
+        static const float grads[256][3] = {
+#define NXG_GRADIENT(x, y, z) {x, y, z},
+#include "../nanoxgen_amd/csrc/seexpr_gradients.inc"
+#undef NXG_GRADIENT
+        };
+        memcpy(blob + h.noise_gradients_offset, grads, sizeof(grads));
+    }
+    h.content_hash = orc_fnv1a(blob + sizeof(h), size - sizeof(h));
+    memcpy(blob, &h, sizeof(h));
+    free(normals); free(alias); free(guides); free(first); free(stencil);
+    *out_blob = blob;
+    *out_size = size;
+    return 0;
+}
+
+/* asset.cpp:152-158 */
+static int section_fits(const nxg_asset_header *h, uint64_t offset, uint64_t count, uint64_t elem) {
+    if (count == 0u) return offset == 0u;
+    if (offset < sizeof(nxg_asset_header) || offset > h->byte_size) return 0;
+    return count * elem <= h->byte_size - offset;
+}
+
+/* asset.cpp:295-338 */
+int orc_validate_asset(const void *bytes, uint64_t size, char *msg, uint64_t cap) {
+    if (msg && cap) msg[0] = 0;
+    if (size < sizeof(nxg_asset_header)) return fail(msg, cap, "blob is smaller than AssetHeader");
+    if (((uintptr_t)bytes) % 64u != 0u) return fail(msg, cap, "blob base is not 64-byte aligned");
+    nxg_asset_header h;
+    memcpy(&h, bytes, sizeof(h));
+    if (h.magic != NXG_MAGIC) return fail(msg, cap, "bad NanoXGen magic");
+    if (h.version_major != NXG_VERSION_MAJOR || h.version_minor != NXG_VERSION_MINOR)
+        return fail(msg, cap, "unsupported NanoXGen asset version");
+    if (h.byte_size != size) return fail(msg, cap, "header byte_size does not match file size");
+    if (h.triangle_count == 0u || h.vertex_count == 0u || h.guide_count == 0u)
+        return fail(msg, cap, "required section is empty");
+    if (h.guide_stencil_size != 8u) return fail(msg, cap, "unsupported guide stencil size");
+    if (h.noise_gradient_count != NXG_NOISE_GRADIENT_COUNT)
+        return fail(msg, cap, "unsupported noise gradient table");
+    if (!section_fits(&h, h.positions_offset, h.vertex_count, 12u) ||
+        !section_fits(&h, h.normals_offset, h.vertex_count, 12u) ||
+        !section_fits(&h, h.texcoords_offset, (h.flags & NXG_ASSET_HAS_TEXCOORDS) ? h.vertex_count : 0u, 8u) ||
+        !section_fits(&h, h.triangles_offset, h.triangle_count, 12u) ||
+        !section_fits(&h, h.alias_table_offset, h.triangle_count, 8u) ||
+        !section_fits(&h, h.guides_offset, h.guide_count, sizeof(nxg_guide_record)) ||
+        !section_fits(&h, h.guide_cvs_offset, h.guide_cv_count, 12u) ||
+        !section_fits(&h, h.triangle_guides_offset, (uint64_t)h.triangle_count * h.guide_stencil_size, 4u) ||
+        !section_fits(&h, h.noise_gradients_offset, h.noise_gradient_count, 12u))
+        return fail(msg, cap, "one or more sections are out of bounds");
+    if (orc_fnv1a((const uint8_t *)bytes + sizeof(h), size - sizeof(h)) != h.content_hash)
+        return fail(msg, cap, "content hash mismatch");
+    const uint8_t *b = (const uint8_t *)bytes;
+    const nxg_uint3 *tris = (const nxg_uint3 *)(b + h.triangles_offset);
+    for (uint32_t i = 0; i < h.triangle_count; ++i) {
+        if (tris[i].x >= h.vertex_count || tris[i].y >= h.vertex_count || tris[i].z >= h.vertex_count)
+            return fail(msg, cap, "triangle index out of bounds");
+    }
+    const nxg_guide_record *g = (const nxg_guide_record *)(b + h.guides_offset);
+    for (uint32_t i = 0; i < h.guide_count; ++i) {
+        if (g[i].cv_count < 2u || g[i].first_cv > h.guide_cv_count ||
+            g[i].cv_count > h.guide_cv_count - g[i].first_cv)
+            return fail(msg, cap, "guide CV range out of bounds");
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* asset.cpp:160-202: persistent tiles over an atomic counter                */
+
+typedef struct {
+    const orc_job *job;
+    uint32_t precision, block, blocks, strands;
+    atomic_uint next;
+} pool_t;
+
+static void *worker(void *arg) {
+    pool_t *p = (pool_t *)arg;
+    for (;;) {
+        const uint32_t block = atomic_fetch_add_explicit(&p->next, 1u, memory_order_relaxed);
+        if (block >= p->blocks) return NULL;
+        const uint64_t first = (uint64_t)block * p->block;
+        uint64_t end = first + p->block;
+        if (end > p->strands) end = p->strands;
+        for (uint64_t s = first; s < end; ++s) {
+            const uint32_t strand = p->job->strand_list ? p->job->strand_list[s] : (uint32_t)s;
+            if (p->precision == 64u) orc_generate_strand_f64(p->job, strand, s);
+            else orc_generate_strand_f32(p->job, strand, s);
+        }
+    }
+}
+
+int orc_generate(const orc_job *job, uint32_t precision, uint32_t workers, uint32_t block) {
+    if (block == 0u) block = 128u;
+    const uint32_t n = job->strand_list ? job->list_count : job->params->strand_count;
+    pool_t p;
+    p.job = job;
+    p.precision = precision;
+    p.block = block;
+    p.strands = n;
+    p.blocks = (uint32_t)(((uint64_t)n + block - 1u) / block);
+    atomic_init(&p.next, 0u);
+    uint32_t w = workers;
+    if (w == 0u) {
+        long hw = sysconf(_SC_NPROCESSORS_ONLN);
+        w = hw > 0 ? (uint32_t)hw : 1u;
+        const uint32_t useful = (p.blocks + 3u) / 4u;
+        if (useful < w) w = useful > 0u ? useful : 1u;
+    }
+    if (w > p.blocks) w = p.blocks;
+    if (w <= 1u) {
+        worker(&p);
+        return 0;
+    }
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * w);
+    for (uint32_t i = 0; i < w; ++i) pthread_create(&th[i], NULL, worker, &p);
+    for (uint32_t i = 0; i < w; ++i) pthread_join(th[i], NULL);
+    free(th);
+    return 0;
+}

@@ -1,0 +1,171 @@
+// ORACLE (test infrastructure only): C entry points over the reference's own
+// CPU core, compiled from the reference sources where they lie (see Makefile
+// target `ref`).  Used only in this container to pin the C restatement and to
+// write golden fixtures; nothing here ships or runs on the GPU.
+#include "nanoxgen/asset.h"
+#include "nanoxgen/generate.h"
+#include "nanoxgen/xgen_expression.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <vector>
+
+#include "../include/nanoxgen_amd.h"
+
+using namespace nanoxgen;
+
+namespace {
+
+Asset asset_from(const void *blob, uint64_t size) {
+    AlignedByteVector bytes(size);
+    std::memcpy(bytes.data(), blob, size);
+    return Asset{std::move(bytes)};
+}
+
+int fail(char *err, uint64_t cap, const char *msg) {
+    if (err && cap) std::snprintf(err, cap, "%s", msg);
+    return 1;
+}
+
+GenerationParams params_from(const nxg_generation_params *p) {
+    static_assert(sizeof(GenerationParams) == sizeof(nxg_generation_params));
+    GenerationParams out{};
+    std::memcpy(&out, p, sizeof(out));
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_build_asset(const nxg_asset_build_input *in, void **blob, uint64_t *size, char *err,
+                    uint64_t cap) {
+    *blob = nullptr;
+    *size = 0;
+    try {
+        AssetBuildInput input{};
+        input.positions.assign(reinterpret_cast<const Vec3 *>(in->positions),
+                               reinterpret_cast<const Vec3 *>(in->positions) + in->vertex_count);
+        if (in->normals)
+            input.normals.assign(reinterpret_cast<const Vec3 *>(in->normals),
+                                 reinterpret_cast<const Vec3 *>(in->normals) + in->vertex_count);
+        if (in->texcoords)
+            input.texcoords.assign(reinterpret_cast<const Vec2 *>(in->texcoords),
+                                   reinterpret_cast<const Vec2 *>(in->texcoords) + in->vertex_count);
+        input.triangles.assign(reinterpret_cast<const UInt3 *>(in->triangles),
+                               reinterpret_cast<const UInt3 *>(in->triangles) + in->triangle_count);
+        uint64_t cv = 0;
+        for (uint32_t g = 0; g < in->guide_count; ++g) {
+            GuideInput guide{};
+            const uint32_t n = in->guide_cv_counts[g];
+            guide.cvs.assign(reinterpret_cast<const Vec3 *>(in->guide_cvs) + cv,
+                             reinterpret_cast<const Vec3 *>(in->guide_cvs) + cv + n);
+            cv += n;
+            if (in->guide_root_normals) {
+                guide.root_normal = {in->guide_root_normals[g].x, in->guide_root_normals[g].y,
+                                     in->guide_root_normals[g].z};
+            }
+            if (in->guide_root_uvs) guide.root_uv = {in->guide_root_uvs[g].x, in->guide_root_uvs[g].y};
+            if (in->guide_triangles) guide.triangle_index = in->guide_triangles[g];
+            if (in->guide_barycentrics)
+                guide.barycentric = {in->guide_barycentrics[g].x, in->guide_barycentrics[g].y};
+            if (in->guide_support_radii) guide.support_radius = in->guide_support_radii[g];
+            input.guides.push_back(std::move(guide));
+        }
+        const Asset asset = build_asset(input);
+        void *out = nullptr;
+        if (posix_memalign(&out, 64, asset.bytes().size()) != 0) return fail(err, cap, "oom");
+        std::memcpy(out, asset.bytes().data(), asset.bytes().size());
+        *blob = out;
+        *size = asset.bytes().size();
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(err, cap, e.what());
+    }
+}
+
+void ref_free(void *p) { std::free(p); }
+
+int ref_validate_asset(const void *blob, uint64_t size, char *msg, uint64_t cap) {
+    const std::string m = validate_asset({static_cast<const std::byte *>(blob), size});
+    if (msg && cap) std::snprintf(msg, cap, "%s", m.c_str());
+    return m.empty() ? 0 : 1;
+}
+
+// generate_packed_cpu / generate_packed_deformed_cpu / generate_packed_roots_cpu
+int ref_generate_packed(const void *blob, uint64_t size, const nxg_generation_params *p,
+                        const nxg_deformed_geometry *deformed, const nxg_root_sample *explicit_roots,
+                        float radius_scale, uint32_t workers, uint32_t block,
+                        nxg_packed_point *points, nxg_root_sample *roots, nxg_vec2 *uvs,
+                        uint32_t *counts, char *err, uint64_t cap) {
+    try {
+        const Asset asset = asset_from(blob, size);
+        const GenerationParams params = params_from(p);
+        const CpuGenerationOptions options{workers, block};
+        PackedGeneratedCurves curves;
+        if (explicit_roots) {
+            std::vector<RootSample> rs(p->strand_count);
+            std::memcpy(rs.data(), explicit_roots, rs.size() * sizeof(RootSample));
+            curves = generate_packed_roots_cpu(asset, params, rs, radius_scale, options);
+        } else if (deformed) {
+            const uint32_t v = asset.view().header().vertex_count;
+            const uint32_t gcv = asset.view().header().guide_cv_count;
+            DeformedGeometryView view{};
+            if (deformed->positions)
+                view.positions = {reinterpret_cast<const Vec3 *>(deformed->positions), v};
+            if (deformed->normals) view.normals = {reinterpret_cast<const Vec3 *>(deformed->normals), v};
+            if (deformed->guide_cvs)
+                view.guide_cvs = {reinterpret_cast<const Vec3 *>(deformed->guide_cvs), gcv};
+            curves = generate_packed_deformed_cpu(asset, params, view, radius_scale, options);
+        } else {
+            curves = generate_packed_cpu(asset, params, radius_scale, options);
+        }
+        std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
+        if (roots) std::memcpy(roots, curves.roots.data(), curves.roots.size() * sizeof(RootSample));
+        if (uvs) std::memcpy(uvs, curves.root_uvs.data(), curves.root_uvs.size() * sizeof(Vec2));
+        if (counts) std::memcpy(counts, curves.point_counts.data(), curves.point_counts.size() * 4u);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(err, cap, e.what());
+    }
+}
+
+float ref_gradient_noise(const void *blob, float x, float y, float z) {
+    return gradient_noise(DeviceAssetView{blob}, {x, y, z});
+}
+
+// make_guide_blend_state (generate.h:214-244) for one strand's sampled root.
+void ref_blend(const void *blob, const nxg_generation_params *p, uint32_t strand, uint32_t *indices,
+               float *weights, uint32_t *nearest) {
+    const GenerationParams params = params_from(p);
+    const DeviceAssetView view{blob};
+    const RootSample root = sample_root(view, {}, params, strand);
+    const GuideBlendState b = make_guide_blend_state(view, {}, params, root);
+    for (int j = 0; j < 8; ++j) {
+        indices[j] = b.indices[j];
+        weights[j] = b.weights[j];
+    }
+    *nearest = b.nearest;
+}
+
+// make_strand_generation_state (generate.h:534-566)
+void ref_strand_state(const void *blob, const nxg_generation_params *p, uint32_t strand, float *out8) {
+    const GenerationParams params = params_from(p);
+    const DeviceAssetView view{blob};
+    const StrandGenerationState s = make_strand_generation_state(view, {}, params, strand);
+    out8[0] = s.noise_domain.x;
+    out8[1] = s.noise_domain.y;
+    out8[2] = s.noise_domain.z;
+    out8[3] = s.effective_noise_frequency;
+    out8[4] = s.original_length;
+    out8[5] = s.preserve_scale;
+    out8[6] = s.surface_u.x;
+    out8[7] = s.surface_u.y;
+}
+
+uint32_t ref_hash32(uint32_t x) { return hash32(x); }
+float ref_random01(uint32_t seed, uint32_t strand, uint32_t dim) { return random01(seed, strand, dim); }
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""The C ABI on the CPU: symbol exports, the host asset builder/validator
+against the oracle, and the checked device contract (validation only; no
+kernel is launched).  CPU only."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, aligned_copy
+import nanoxgen_amd as nx
+from nanoxgen_amd import abi, workloads
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "nanoxgen_amd.h")).read()
+    return sorted(set(re.findall(r"^(?:const )?\w+\s*\**\s*(nxg_\w+)\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = nx.lib()
+    declared = header_symbols()
+    assert len(declared) >= 10
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(n for n, _, _ in nx.SYMBOLS) == declared
+
+
+def test_build_info_names_gfx950():
+    assert b"gfx950" in nx.lib().nxg_build_info()
+
+
+@pytest.mark.parametrize("make", [
+    lambda: workloads.icosphere_description(),
+    lambda: workloads.icosphere_description(stencil_k=0),
+    lambda: workloads.icosphere_description(stencil_k=1, guides=5),
+    lambda: workloads.plane_description(quads=32, guides_per_side=8),
+    lambda: workloads.plane_description(quads=8, guides_per_side=2, stencil_k=3),
+])
+def test_builder_matches_oracle_bytes(orc, make):
+    b = make()
+    assert np.array_equal(nx.build_asset(b), orc.build_asset(b))
+
+
+def test_builder_without_normals_or_uvs(orc):
+    b = workloads.icosphere_description()
+    b2 = abi.BuildInputArrays(b.positions, b.triangles, b.guide_cvs, b.guide_cv_counts, stencil_k=6)
+    blob = nx.build_asset(b2)
+    assert np.array_equal(blob, orc.build_asset(b2))
+    h = abi.header_of(blob)
+    assert h.texcoords_offset == 0 and h.flags == abi.HAS_NORMALS
+
+
+@pytest.mark.parametrize("mutate,message", [
+    (lambda a: a.update(triangles=np.array([[0, 1, 9]])), "triangle index out of bounds"),
+    (lambda a: a.update(guide_cv_counts=np.array([1]), guide_cvs=np.zeros((1, 3))), "each guide needs 2..65535 CVs"),
+    (lambda a: a.update(stencil_k=9), "stencil_k must be in 0..8"),
+    (lambda a: a.update(positions=np.zeros((3, 3))), "mesh has zero total area"),
+])
+def test_builder_rejections_match_oracle(orc, mutate, message):
+    args = dict(positions=np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1]]), triangles=np.array([[0, 1, 2]]),
+                guide_cvs=np.array([[0, 0, 0], [0, 1, 0]]), guide_cv_counts=np.array([2]))
+    mutate(args)
+    b = abi.BuildInputArrays(**args)
+    with pytest.raises(ValueError, match=re.escape(message)):
+        nx.build_asset(b)
+    with pytest.raises(ValueError, match=re.escape(message)):
+        orc.build_asset(b)
+
+
+def test_validator_messages_match_oracle(orc):
+    blob = nx.build_asset(workloads.icosphere_description())
+    assert nx.validate_asset(blob) == "" == orc.validate_asset(blob)
+    cases = []
+    c = aligned_copy(blob); c[-1] ^= 1; cases.append(c)                         # hash
+    c = aligned_copy(blob); c[0] ^= 1; cases.append(c)                          # magic
+    c = aligned_copy(blob); c[8] = 7; cases.append(c)                           # version
+    cases.append(aligned_copy(blob[:100]))                                      # short
+    cases.append(aligned_copy(blob[:-4]))                                       # size
+    raw = np.zeros(blob.nbytes + 64, dtype=np.uint8)
+    off = (-raw.ctypes.data) % 64 + 1
+    mis = raw[off:off + blob.nbytes]; mis[:] = blob; cases.append(mis)         # misaligned
+    expected = ["content hash mismatch", "bad NanoXGen magic", "unsupported NanoXGen asset version",
+                "blob is smaller than AssetHeader", "header byte_size does not match file size",
+                "blob base is not 64-byte aligned"]
+    for case, msg in zip(cases, expected):
+        assert nx.validate_asset(case) == msg == orc.validate_asset(case)
+
+
+# ---- checked device contract (tests/test_main.cpp:483-588) ---------------
+
+class Request:
+    """Host stand-ins for device pointers: validation never dereferences them."""
+
+    def __init__(self):
+        self.blob = nx.build_asset(workloads.icosphere_description())
+        self.desc = abi.DeviceAsset()
+        assert nx.lib().nxg_make_device_asset(self.blob.ctypes.data, self.blob.nbytes, self.blob.ctypes.data,
+                                              self.blob.nbytes, C.byref(self.desc)) == 0
+        self.params = abi.GenerationParams(strand_count=7, cvs_per_strand=5)
+        n = 7 * 5
+        self.points = np.zeros((n, 4), np.float32)
+        self.roots = np.zeros(7, abi.ROOT_DTYPE)
+        self.uvs = np.zeros((7, 2), np.float32)
+        self.counts = np.zeros(7, np.uint32)
+        self.out = abi.PackedOutput(points=self.points.ctypes.data, roots=self.roots.ctypes.data,
+                                    root_uvs=self.uvs.ctypes.data, radius_scale=1.0,
+                                    point_counts=self.counts.ctypes.data, point_capacity=n,
+                                    root_capacity=7, root_uv_capacity=7, point_count_capacity=7)
+
+    def validate(self, deformed=None, config=None, desc=None, params=None, out=None):
+        return nx.lib().nxg_validate_packed_generation(
+            C.byref(desc or self.desc), C.byref(deformed) if deformed else None,
+            C.byref(params or self.params), C.byref(out or self.out), C.byref(config) if config else None)
+
+
+def test_contract_valid_request():
+    assert Request().validate() == abi.OK
+
+
+def test_contract_error_codes():
+    r = Request()
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.point_capacity = 34
+    assert r.validate(out=out) == abi.POINT_CAPACITY_TOO_SMALL
+    d = abi.DeformedGeometry(positions=r.blob.ctypes.data, position_count=r.desc.header.vertex_count - 1)
+    assert r.validate(deformed=d) == abi.DEFORMED_POSITION_COUNT_MISMATCH
+    assert r.validate(config=abi.LaunchConfig(block_size=0)) == abi.INVALID_BLOCK_SIZE
+    assert r.validate(config=abi.LaunchConfig(block_size=1025)) == abi.INVALID_BLOCK_SIZE
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.radius_scale = float("nan")
+    assert r.validate(out=out) == abi.INVALID_RADIUS_SCALE
+    short = abi.DeviceAsset.from_buffer_copy(r.desc)
+    short.byte_capacity = r.desc.header.byte_size - 1
+    assert r.validate(desc=short) == abi.ASSET_CAPACITY_TOO_SMALL
+    mis = abi.DeviceAsset.from_buffer_copy(r.desc)
+    mis.data = r.desc.data + 1
+    assert r.validate(desc=mis) == abi.MISALIGNED_ASSET
+    bad = abi.DeviceAsset.from_buffer_copy(r.desc)
+    bad.header.noise_gradients_offset = bad.header.byte_size - 4
+    assert r.validate(desc=bad) == abi.INVALID_ASSET_METADATA
+    null = abi.DeviceAsset.from_buffer_copy(r.desc)
+    null.data = None
+    assert r.validate(desc=null) == abi.NULL_ASSET
+    assert r.validate(params=r.params.copy(cvs_per_strand=1)) == abi.INVALID_GENERATION_PARAMETERS
+    assert r.validate(params=r.params.copy(noise_mask=1.5)) == abi.INVALID_GENERATION_PARAMETERS
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.roots = None
+    assert r.validate(out=out) == abi.ROOT_CAPACITY_TOO_SMALL  # null pointer with capacity 7
+    out.root_capacity = 0
+    assert r.validate(out=out) == abi.OK
+    out.points = None
+    assert r.validate(out=out) == abi.MISSING_OUTPUT
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.point_count_capacity = 6
+    assert r.validate(out=out) == abi.POINT_COUNT_CAPACITY_TOO_SMALL
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.root_uv_capacity = 6
+    assert r.validate(out=out) == abi.ROOT_UV_CAPACITY_TOO_SMALL
+
+
+def test_generate_rejects_before_any_device_work():
+    """Every refusal happens on the host, before a HIP call."""
+    r = Request()
+    L = nx.lib()
+    ws = np.zeros(16, np.uint8)
+    st = L.nxg_generate_packed(C.byref(r.desc), None, C.byref(r.params), None, C.byref(r.out), None, None,
+                               ws.ctypes.data, ws.nbytes, None)
+    assert st == abi.WORKSPACE_TOO_SMALL
+    oct5 = abi.NoiseOctaves(count=5)
+    st = L.nxg_generate_packed(C.byref(r.desc), None, C.byref(r.params), C.byref(oct5), C.byref(r.out), None,
+                               None, ws.ctypes.data, ws.nbytes, None)
+    assert st == abi.INVALID_OCTAVES
+    sten = abi.StencilOutput(indices=ws.ctypes.data, weights=ws.ctypes.data, capacity=8)
+    st = L.nxg_generate_packed(C.byref(r.desc), None, C.byref(r.params), None, C.byref(r.out), C.byref(sten),
+                               None, ws.ctypes.data, ws.nbytes, None)
+    assert st == abi.STENCIL_CAPACITY_TOO_SMALL
+    st = L.nxg_generate_packed_from_roots(C.byref(r.desc), C.byref(r.params), None, None, C.byref(r.out), None,
+                                          ws.ctypes.data, ws.nbytes, None)
+    assert st == abi.INVALID_ROOTS
+    assert L.nxg_packed_generation_workspace_size(C.byref(r.desc.header), C.byref(r.params)) > 16
+
+
+def test_status_messages():
+    L = nx.lib()
+    assert L.nxg_status_message(0) == b""
+    assert L.nxg_status_message(6) == b"GPU block size must be in [1, 1024]"
+    assert L.nxg_status_message(20) == b"device output size overflows uint64"
+    assert L.nxg_status_message(999) == b"unknown device generation error"
+    for s in range(1, len(abi.STATUS)):
+        assert L.nxg_status_message(s)

@@ -1,0 +1,270 @@
+"""Pins of the CPU oracle (oracle/): reference golden vectors, reference test
+values (tests/test_main.cpp of the reference), and mathematical invariants.
+CPU only."""
+import numpy as np
+import pytest
+
+from conftest import aligned_copy, params_from
+from nanoxgen_amd import abi, workloads
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint8)
+
+
+# ---- reference golden fixtures (tests/golden/make_golden.py) -------------
+
+@pytest.mark.parametrize("case", ["noise", "plain", "preserve", "cv13"])
+def test_oracle_matches_reference_generation(orc, golden, case):
+    blob = aligned_copy(golden["ico8_blob"])
+    p = params_from(golden[f"ico8_{case}_params"])
+    out = orc.generate(blob, p, workers=2, block=32)
+    assert np.array_equal(bits(out["points"]), bits(golden[f"ico8_{case}_points"]))
+    assert np.array_equal(bits(out["roots"]), bits(golden[f"ico8_{case}_roots"]))
+    assert np.array_equal(bits(out["root_uvs"]), bits(golden[f"ico8_{case}_root_uvs"]))
+    assert np.array_equal(out["point_counts"], golden[f"ico8_{case}_point_counts"])
+
+
+@pytest.mark.parametrize("case", ["plane", "plane_deformed", "plane_patchonly", "plane_explicit"])
+def test_oracle_matches_reference_plane(orc, golden, case):
+    blob = aligned_copy(golden["plane_blob"])
+    p = params_from(golden["plane_params"])
+    kw = {}
+    h = abi.header_of(blob)
+    gcv = blob[h.guide_cvs_offset:h.guide_cvs_offset + 12 * h.guide_cv_count].view(np.float32).reshape(-1, 3)
+    nrm = blob[h.normals_offset:h.normals_offset + 12 * h.vertex_count].view(np.float32).reshape(-1, 3)
+    shift = np.array([2.0, 3.0, -4.0], dtype=np.float32)
+    if case == "plane_deformed":
+        kw["deformed"] = dict(positions=golden["plane_deformed_positions"], normals=nrm, guide_cvs=gcv + shift)
+    elif case == "plane_patchonly":
+        kw["deformed"] = dict(positions=golden["plane_deformed_positions"])
+    elif case == "plane_explicit":
+        kw["roots"] = golden["plane_explicit_roots"]
+    out = orc.generate(blob, p, **kw)
+    assert np.array_equal(bits(out["points"]), bits(golden[f"{case}_points"]))
+    assert np.array_equal(bits(out["roots"]), bits(golden[f"{case}_roots"]))
+
+
+def test_oracle_builder_matches_reference_blob(orc, golden):
+    blob = orc.build_asset(workloads.icosphere_description(stencil_k=0))
+    assert np.array_equal(blob, golden["ico8_blob"])
+    plane = workloads.plane_description(quads=16, guides_per_side=4, stencil_k=0)
+    plane.struct.triangle_weights = None
+    assert np.array_equal(orc.build_asset(plane), golden["plane_blob"])
+
+
+def test_oracle_primitives_match_reference(orc, golden):
+    L = orc.lib()
+    assert [L.orc_hash32(int(x)) for x in golden["hash_inputs"]] == golden["hash_values"].tolist()
+    got = np.array([L.orc_random01(*map(int, r)) for r in golden["random_inputs"]], dtype=np.float32)
+    assert np.array_equal(got, golden["random_values"])
+    blob = aligned_copy(golden["ico8_blob"])
+    nv = np.array([orc.gradient_noise(blob, *map(float, q)) for q in golden["noise_points"]], dtype=np.float32)
+    assert np.array_equal(nv, golden["noise_values"])
+
+
+def test_oracle_blend_and_state_match_reference(orc, golden):
+    blob = aligned_copy(golden["ico8_blob"])
+    p = params_from(golden["ico8_preserve_params"]).copy(strand_count=64)
+    out = orc.generate(blob, p, stencil=True)
+    assert np.array_equal(out["stencil_indices"], golden["blend_indices"])
+    assert np.array_equal(bits(out["stencil_weights"]), bits(golden["blend_weights"]))
+    st = np.stack([orc.strand_state(blob, p, s) for s in range(64)])
+    assert np.array_equal(bits(st), bits(golden["strand_state"]))
+
+
+def test_reference_live_when_built(orc):
+    """Where the reference core was compiled here, compare live on a fresh seed."""
+    if orc.ref() is None:
+        pytest.skip("reference core not built in this environment")
+    w = workloads.config1(strands=512)
+    b = workloads.icosphere_description(stencil_k=0)
+    blob = orc.build_asset(b)
+    assert np.array_equal(blob, orc.ref_build_asset(b))
+    for p in (w.params, w.params.copy(noise_preserve_length=0.7, noise_correlation=0.5, seed=9)):
+        o = orc.generate(blob, p)
+        r = orc.ref_generate(blob, p)
+        assert np.array_equal(bits(o["points"]), bits(r["points"]))
+        assert np.array_equal(bits(o["roots"]), bits(r["roots"]))
+
+
+# ---- the reference's own test values (tests/test_main.cpp) ---------------
+
+def one_triangle_fixture():
+    """tests/test_main.cpp:39-53."""
+    return abi.BuildInputArrays(
+        positions=[[0, 0, 0], [1, 0, 0], [0, 0, 1]], triangles=[[0, 1, 2]],
+        guide_cvs=[[0.25, 0.0, 0.25], [0.25, 0.5, 0.25], [0.5, 1.0, 0.25]], guide_cv_counts=[3],
+        normals=[[0, 1, 0]] * 3, texcoords=[[0, 0], [1, 0], [0, 1]], guide_root_normals=[[0, 1, 0]],
+        guide_triangles=[0], guide_barycentrics=[[0.25, 0.25]], guide_support_radii=[10.0])
+
+
+def test_noise_golden_samples(orc):
+    """tests/test_main.cpp:233-244: six gradient_noise samples within 2e-6."""
+    blob = orc.build_asset(one_triangle_fixture())
+    samples = [((0.0, 0.0, 0.0), 0.5), ((0.0457031233125, 0.0, 0.0), 0.513789508165),
+               ((0.0, 0.0457031233125, 0.0), 0.518000278518), ((0.0, 0.0, 0.0457031233125), 0.496315375404),
+               ((1.234, -2.5, 3.75), 0.672824373032), ((-3.2, 4.1, -0.7), 0.515891027958)]
+    for (x, y, z), expected in samples:
+        assert abs(orc.gradient_noise(blob, x, y, z) - expected) < 2e-6
+        assert abs(orc.gradient_noise(blob, x, y, z, precision=64) - expected) < 2e-6
+
+
+def test_correlation_domain_and_frequency_floor(orc):
+    """tests/test_main.cpp:246-262."""
+    blob = orc.build_asset(one_triangle_fixture())
+    p = abi.GenerationParams(strand_count=32, cvs_per_strand=33, seed=173, noise_amplitude=0.24,
+                             noise_frequency=0.0, noise_correlation=0.75)
+    out = orc.generate(blob, p.copy(strand_count=1))
+    r = out["roots"][0]
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1]], dtype=np.float32)
+    b1, b2 = r["barycentric"]
+    rest = tri[0] * (np.float32(1) - b1 - b2) + tri[1] * b1 + tri[2] * b2
+    expected = (rest + np.array([0.419276, 0.184247, 0.805721], dtype=np.float32)) * np.float32(6.25)
+    st = orc.strand_state(blob, p, 0)
+    assert np.sum((st[:3] - expected) ** 2) < 1e-11
+    assert abs(st[3] - 0.5 / st[4]) < 1e-6
+
+
+def arc_lengths(points, cvs):
+    p = points[:, :3].astype(np.float64).reshape(-1, cvs, 3)
+    return np.sqrt(((p[:, 1:] - p[:, :-1]) ** 2).sum(-1)).sum(-1)
+
+
+def test_preserve_length_invariants(orc):
+    """tests/test_main.cpp:264-291."""
+    blob = orc.build_asset(one_triangle_fixture())
+    p = abi.GenerationParams(strand_count=32, cvs_per_strand=33, seed=173, noise_amplitude=0.24,
+                             noise_frequency=2.25, noise_correlation=0.75)
+    base = orc.generate(blob, p.copy(noise_amplitude=0.0), workers=1, block=16)["points"]
+    noisy = orc.generate(blob, p, workers=1, block=16)["points"]
+    full = orc.generate(blob, p.copy(noise_preserve_length=1.0), workers=1, block=16)["points"]
+    part = orc.generate(blob, p.copy(noise_preserve_length=0.4), workers=1, block=16)["points"]
+    roots = np.arange(32) * 33
+    assert np.allclose(noisy[roots, :3], base[roots, :3], atol=1e-6)
+    assert np.allclose(full[roots, :3], base[roots, :3], atol=1e-6)
+    lb, ln, lf, lp = (arc_lengths(x, 33) for x in (base, noisy, full, part))
+    exercised = np.abs(ln - lb) > 2e-4
+    assert exercised.any()
+    assert np.all(np.abs(lf - lb)[exercised] < 2e-5)
+    assert np.all(np.abs(lp - (0.4 * lb + 0.6 * ln))[exercised] < 2e-5)
+
+
+def test_guide_interpolation_exact(orc):
+    """tests/test_main.cpp:87-100."""
+    blob = orc.build_asset(one_triangle_fixture())
+    p = abi.GenerationParams(strand_count=1, cvs_per_strand=3, seed=7)
+    out = orc.generate(blob, p)
+    root = out["roots"][0]["position"]
+    mid, tip = out["points"][1], out["points"][2]
+    assert abs(mid[1] - 0.5) < 1e-6 and abs(tip[1] - 1.0) < 1e-6
+    assert abs((tip[0] - root[0]) - 0.25) < 1e-6
+
+
+def test_roots_inside_triangle_and_first_cv(orc):
+    """tests/test_main.cpp:66-85 plus bitwise determinism."""
+    blob = orc.build_asset(one_triangle_fixture())
+    p = abi.GenerationParams(strand_count=128, cvs_per_strand=7, seed=42)
+    a = orc.generate(blob, p)
+    b = orc.generate(blob, p)
+    assert np.array_equal(bits(a["points"]), bits(b["points"]))
+    pos = a["roots"]["position"]
+    assert np.all(pos[:, 0] >= 0) and np.all(pos[:, 2] >= 0) and np.all(pos[:, 0] + pos[:, 2] <= 1.00001)
+    first = a["points"][::7, :3]
+    assert np.all(((first - pos) ** 2).sum(1) < 1e-10)
+
+
+def test_serial_equals_parallel(orc):
+    """tests/test_main.cpp:151-170 (17-strand tiles, 1 vs 4 workers)."""
+    blob = orc.build_asset(one_triangle_fixture())
+    p = abi.GenerationParams(strand_count=257, cvs_per_strand=11, seed=91, noise_amplitude=0.03)
+    s = orc.generate(blob, p, workers=1, block=17)
+    q = orc.generate(blob, p, workers=4, block=17)
+    for k in ("points", "roots", "root_uvs"):
+        assert np.array_equal(bits(s[k]), bits(q[k]))
+
+
+def test_area_weighted_sampling(orc):
+    """tests/test_main.cpp:309-332: areas 1.0 and 0.5 -> frequency in (0.64, 0.69)."""
+    b = abi.BuildInputArrays(
+        positions=[[0, 0, 0], [2, 0, 0], [0, 0, 1], [10, 0, 0], [11, 0, 0], [10, 0, 1]],
+        triangles=[[0, 1, 2], [3, 4, 5]], guide_cvs=[[0, 0, 0], [0, 1, 0]], guide_cv_counts=[2],
+        normals=[[0, 1, 0]] * 6, guide_root_normals=[[0, 1, 0]], guide_support_radii=[100.0])
+    blob = orc.build_asset(b)
+    out = orc.generate(blob, abi.GenerationParams(strand_count=30000, cvs_per_strand=2, seed=123))
+    f = np.mean(out["roots"]["triangle_index"] == 0)
+    assert 0.64 < f < 0.69
+
+
+def test_explicit_roots_reproduce(orc):
+    """tests/test_main.cpp:463-470: generating from the sampled roots is bitwise identical."""
+    blob = orc.build_asset(one_triangle_fixture())
+    p = abi.GenerationParams(strand_count=19, cvs_per_strand=11, seed=117, root_width=0.03, tip_width=0.002,
+                             noise_amplitude=0.04, noise_frequency=3.25)
+    a = orc.generate(blob, p, radius_scale=1.5)
+    b = orc.generate(blob, p, roots=a["roots"], radius_scale=1.5)
+    assert np.array_equal(bits(a["points"]), bits(b["points"]))
+
+
+def test_corruption_detected(orc):
+    """tests/test_main.cpp:55-64."""
+    blob = orc.build_asset(one_triangle_fixture())
+    assert orc.validate_asset(blob) == ""
+    bad = aligned_copy(blob)
+    bad[-1] ^= 1
+    assert orc.validate_asset(bad) == "content hash mismatch"
+
+
+# ---- invariants the mathematics fixes -------------------------------------
+
+def test_alias_table_reconstructs_weights(orc):
+    """The Walker table's implied distribution equals area * weight / sum."""
+    b = workloads.plane_description(quads=24, guides_per_side=4)
+    blob = orc.build_asset(b)
+    h = abi.header_of(blob)
+    T = h.triangle_count
+    alias = blob[h.alias_table_offset:h.alias_table_offset + 8 * T].view(np.dtype([("p", "<f4"), ("a", "<u4")]))
+    implied = np.zeros(T)
+    implied += alias["p"].astype(np.float64) / T
+    np.add.at(implied, alias["a"], (1.0 - alias["p"].astype(np.float64)) / T)
+    p = b.positions.astype(np.float64)
+    t = b.triangles
+    area = 0.5 * np.linalg.norm(np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]]), axis=1)
+    target = area * b.triangle_weights
+    target /= target.sum()
+    assert np.max(np.abs(implied - target)) < 1e-7
+
+
+def test_stencil_is_k_smallest_by_brute_force(orc):
+    """Each triangle's stencil lists the k smallest (d^2 + penalty, index)
+    pairs in ascending order (computed here with float32 numpy)."""
+    for k in (4, 8):
+        b = workloads.icosphere_description(stencil_k=k)
+        blob = orc.build_asset(b)
+        h = abi.header_of(blob)
+        st = blob[h.triangle_guides_offset:h.triangle_guides_offset + 32 * h.triangle_count].view(np.uint32).reshape(-1, 8)
+        g = blob[h.guides_offset:h.guides_offset + 56 * h.guide_count].view(abi.GUIDE_DTYPE)
+        P = b.positions
+        t = b.triangles
+        c = ((P[t[:, 0]] + P[t[:, 1]]) + P[t[:, 2]]) / np.float32(3.0)
+        fn = np.cross(P[t[:, 1]] - P[t[:, 0]], P[t[:, 2]] - P[t[:, 0]])
+        fn = fn / np.sqrt((fn * fn).sum(1, keepdims=True))
+        d = c[:, None, :] - g["root_position"][None, :, :]
+        key = (d * d).sum(-1, dtype=np.float32)
+        key = key + np.where((fn[:, None, :] * g["root_normal"][None]).sum(-1) < -0.25, np.float32(1e20), np.float32(0))
+        order = np.lexsort((np.broadcast_to(np.arange(h.guide_count), key.shape), key), axis=-1)[:, :k]
+        assert np.array_equal(st[:, :k], order.astype(np.uint32))
+        assert np.all(st[:, k:] == abi.INVALID_INDEX)
+
+
+def test_fp64_twin_agrees(orc):
+    """The FP64 twin of the same formulas stays within float rounding of the
+    restatement (and samples the same roots)."""
+    w = workloads.config1(strands=2000)
+    blob = orc.build_asset(w.build)
+    f32 = orc.generate(blob, w.params)
+    f64 = orc.generate(blob, w.params, precision=64)
+    assert np.array_equal(f32["roots"]["triangle_index"], f64["roots"][:, 8].astype(np.uint32))
+    err = np.abs(f32["points"].astype(np.float64) - f64["points"]).max()
+    assert err < 2e-5
