@@ -210,7 +210,9 @@ enum {
     NXG_ERROR_INVALID_ROOTS = 26,
     NXG_ERROR_INVALID_BUILD_INPUT = 27,
     NXG_ERROR_BUFFER_TOO_SMALL = 28,
-    NXG_ERROR_INVALID_ASSET = 29
+    NXG_ERROR_INVALID_ASSET = 29,
+    NXG_ERROR_INVALID_PROGRAM = 30,
+    NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL = 31
 };
 
 /* device_generation_error_message (device_contract.h:82-116); extension codes
@@ -321,6 +323,137 @@ uint32_t nxg_generate_packed_from_roots(const nxg_device_asset *asset,
                                         const nxg_noise_octaves *octaves,
                                         const nxg_root_sample *roots,
                                         const nxg_packed_output *output,
+                                        const nxg_launch_config *config,
+                                        void *workspace, uint64_t workspace_bytes,
+                                        void *stream);
+
+/* ---- float expression runtime (xgen_expression.h:12-100) --------------- */
+
+/* XgenScalarOp (xgen_expression.h:12-40) values 0..26 verbatim; 27.. are
+ * extensions with no reference counterpart:
+ *   POW(a, b)        powf(a, b)
+ *   MIX(a, b, t)     fit(t, 0, 1, a, b), operation for operation
+ *   NOISE3(x, y, z)  gradient_noise (generate.h:370-402), value in [0, 1]
+ *   TEXEL(f, u, v)   point sample of texel map `auxiliary` (nxg_texel_map):
+ *                    face clamp(floor(f), 0, faces-1), column
+ *                    clamp(floor(u*R), 0, R-1), row clamp(floor(v*R), 0, R-1) */
+enum {
+    NXG_OP_CONSTANT = 0, NXG_OP_INPUT, NXG_OP_NEGATE, NXG_OP_LOGICAL_NOT, NXG_OP_ADD,
+    NXG_OP_SUBTRACT, NXG_OP_MULTIPLY, NXG_OP_DIVIDE, NXG_OP_LESS, NXG_OP_LESS_EQUAL,
+    NXG_OP_GREATER, NXG_OP_GREATER_EQUAL, NXG_OP_EQUAL, NXG_OP_NOT_EQUAL, NXG_OP_LOGICAL_AND,
+    NXG_OP_LOGICAL_OR, NXG_OP_SELECT, NXG_OP_HASH, NXG_OP_RANDOM, NXG_OP_MINIMUM, NXG_OP_MAXIMUM,
+    NXG_OP_CLAMP, NXG_OP_FIT, NXG_OP_GAMMA, NXG_OP_CONTRAST, NXG_OP_SMOOTHSTEP, NXG_OP_RAMP,
+    NXG_OP_POW = 27, NXG_OP_MIX = 28, NXG_OP_NOISE3 = 29, NXG_OP_TEXEL = 30,
+    NXG_OP_COUNT = 31
+};
+
+/* XgenFloatScalarInstruction (xgen_expression.h:83-89): 16 bytes. */
+typedef struct nxg_expr_instruction {
+    uint8_t op;
+    uint8_t reserved;
+    uint16_t operand_count;
+    uint32_t operand_offset;
+    uint32_t auxiliary;
+    float immediate;
+} nxg_expr_instruction;
+
+/* XgenFloatRampPoint / XgenRamp (xgen_expression.h:48-57). */
+typedef struct nxg_expr_ramp_point { float position; float value; uint32_t interpolation; } nxg_expr_ramp_point;
+typedef struct nxg_expr_ramp { uint32_t point_offset; uint32_t point_count; } nxg_expr_ramp;
+
+/* Per-root variables a program input can bind to (the generator's analogue
+ * of the Classic runtime's input binding, xgen_classic_runtime.cpp:504-553). */
+enum {
+    NXG_ROOT_VAR_POSITION_X = 0, NXG_ROOT_VAR_POSITION_Y, NXG_ROOT_VAR_POSITION_Z, /* rest pose */
+    NXG_ROOT_VAR_NORMAL_X, NXG_ROOT_VAR_NORMAL_Y, NXG_ROOT_VAR_NORMAL_Z,
+    NXG_ROOT_VAR_U, NXG_ROOT_VAR_V,      /* root texcoord */
+    NXG_ROOT_VAR_FACE,                   /* triangle index */
+    NXG_ROOT_VAR_BARY1, NXG_ROOT_VAR_BARY2,
+    NXG_ROOT_VAR_ID,                     /* strand index */
+    NXG_ROOT_VAR_COUNT
+};
+
+/* XgenFloatExpressionProgram (xgen_expression.h:91-100), flattened, host
+ * memory.  Program input i reads root variable input_bindings[i]. */
+typedef struct nxg_expr_program {
+    const nxg_expr_instruction *instructions;
+    const uint32_t *operands;
+    const nxg_expr_ramp_point *ramp_points;
+    const nxg_expr_ramp *ramps;
+    const uint32_t *input_bindings;
+    uint32_t instruction_count;
+    uint32_t operand_count;
+    uint32_t ramp_point_count;
+    uint32_t ramp_count;
+    uint32_t input_count;
+    uint32_t result;
+    uint32_t expression_seed;
+    uint32_t random_call_count;
+} nxg_expr_program;
+
+/* A face-major R x R float texel map (device memory for the GPU entry
+ * points, host memory for the CPU ones). */
+typedef struct nxg_texel_map {
+    const float *texels;
+    uint32_t face_count;
+    uint32_t resolution;
+} nxg_texel_map;
+
+/* Per-root attribute slots (4 floats per strand, strand-major).  Program k
+ * of a set writes slot k; slots without a program hold 1.0.  Generation
+ * multiplies length_scale by slot 0, the width profile by slot 1 and every
+ * octave's noise amplitude by slot 2; slot 3 is carried for consumers. */
+enum { NXG_ATTR_LENGTH = 0, NXG_ATTR_WIDTH = 1, NXG_ATTR_NOISE = 2, NXG_ATTR_CLUMP = 3, NXG_ATTR_COUNT = 4 };
+#define NXG_EXPR_MAX_VALUES 256u
+#define NXG_EXPR_MAX_PROGRAMS 4u
+#define NXG_EXPR_MAX_TEXEL_MAPS 4u
+
+/* Checks one program the way the reference interpreter does while it runs
+ * (operand ranges, SSA order, arity, input/ramp indices, result index), plus
+ * the runtime limit NXG_EXPR_MAX_VALUES.  NXG_OK or NXG_ERROR_INVALID_PROGRAM. */
+uint32_t nxg_expr_validate(const nxg_expr_program *program);
+
+/* Host descriptor of a packed program set: nxg_expr_pack fills everything
+ * but `image`, which the caller points at its device copy of the packed
+ * bytes (16-byte aligned). */
+typedef struct nxg_expr_set {
+    const void *image;
+    uint64_t byte_size;
+    uint32_t program_count;
+    uint32_t max_values;
+    uint32_t program_offset[NXG_EXPR_MAX_PROGRAMS];
+    uint32_t texel_maps_required; /* 1 + the largest texel-map index used */
+} nxg_expr_set;
+
+/* Packs up to NXG_EXPR_MAX_PROGRAMS validated programs into one position-
+ * independent byte image for device upload.  With out == NULL only
+ * set->byte_size is written (size query). */
+uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t program_count, void *out,
+                       uint64_t capacity, nxg_expr_set *set);
+
+/* Evaluates the set's programs for every strand's root (sample_root,
+ * generate.h:101-135) on the GPU: attributes[4*s + k] = program k at root s.
+ * Context per root: u, v = root texcoord, face_seed = runtime hash of
+ * (0, 0, face), t = 0, no random prefix.  A non-finite value (an error in
+ * the reference interpreter) writes NaN for that root and sets bit 0 of
+ * *error_word (optional, device memory, never cleared by the library). */
+uint32_t nxg_evaluate_root_attributes(const nxg_device_asset *asset,
+                                      const nxg_generation_params *params,
+                                      const nxg_expr_set *set,
+                                      const nxg_texel_map *maps, uint32_t map_count,
+                                      float *attributes, uint64_t attribute_capacity,
+                                      uint32_t *error_word, void *stream);
+
+/* nxg_generate_packed with per-root attribute multipliers (device memory,
+ * 4 floats per strand; see NXG_ATTR_*).  All-ones attributes reproduce
+ * nxg_generate_packed bit for bit. */
+uint32_t nxg_generate_packed_attributed(const nxg_device_asset *asset,
+                                        const nxg_deformed_geometry *deformed,
+                                        const nxg_generation_params *params,
+                                        const nxg_noise_octaves *octaves,
+                                        const float *attributes, uint64_t attribute_capacity,
+                                        const nxg_packed_output *output,
+                                        const nxg_stencil_output *stencil,
                                         const nxg_launch_config *config,
                                         void *workspace, uint64_t workspace_bytes,
                                         void *stream);

@@ -38,6 +38,10 @@ SYMBOLS = [
     ("nxg_generate_packed_from_roots", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_prepare_packed_generation", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_generate_packed_prepared", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_expr_validate", C.c_uint32, [_P]),
+    ("nxg_expr_pack", C.c_uint32, [_P, C.c_uint32, _P, C.c_uint64, _P]),
+    ("nxg_evaluate_root_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, _P, _P]),
+    ("nxg_generate_packed_attributed", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_build_info", C.c_char_p, []),
 ]
 
@@ -185,10 +189,12 @@ def _ref(x):
 
 
 def generate_packed(asset, params, outputs, *, octaves=None, deformed=None, workspace=None,
-                    stream=None, roots=None):
+                    stream=None, roots=None, attributes=None):
     """Enqueue generate_packed (generate.h:612-676) for every strand on
     `stream` (torch stream; default the current one).  `roots` (optional
-    device tensor of RootSample records) selects generate-from-roots."""
+    device tensor of RootSample records) selects generate-from-roots;
+    `attributes` (optional float32 [N, 4] device tensor) the per-root
+    attribute multipliers."""
     torch = _torch()
     L = lib()
     if workspace is None:
@@ -196,7 +202,12 @@ def generate_packed(asset, params, outputs, *, octaves=None, deformed=None, work
                                 device=asset.tensor.device)
     out = outputs.descriptor()
     st = outputs.stencil_descriptor()
-    if roots is not None:
+    if attributes is not None:
+        status = L.nxg_generate_packed_attributed(
+            C.byref(asset.desc), _ref(deformed), C.byref(params), _ref(octaves),
+            C.c_void_p(attributes.data_ptr()), attributes.numel(), C.byref(out), _ref(st), None,
+            C.c_void_p(workspace.data_ptr()), workspace.numel(), _stream_handle(stream))
+    elif roots is not None:
         status = L.nxg_generate_packed_from_roots(
             C.byref(asset.desc), C.byref(params), _ref(octaves), C.c_void_p(roots.data_ptr()),
             C.byref(out), None, C.c_void_p(workspace.data_ptr()), workspace.numel(),

@@ -182,6 +182,48 @@ class AssetBuildInput(C.Structure):
     ]
 
 
+class ExprInstruction(C.Structure):
+    _fields_ = [("op", C.c_uint8), ("reserved", C.c_uint8), ("operand_count", C.c_uint16),
+                ("operand_offset", C.c_uint32), ("auxiliary", C.c_uint32), ("immediate", C.c_float)]
+
+
+class ExprRampPoint(C.Structure):
+    _fields_ = [("position", C.c_float), ("value", C.c_float), ("interpolation", C.c_uint32)]
+
+
+class ExprRamp(C.Structure):
+    _fields_ = [("point_offset", C.c_uint32), ("point_count", C.c_uint32)]
+
+
+class ExprProgram(C.Structure):
+    _fields_ = [("instructions", C.c_void_p), ("operands", C.c_void_p), ("ramp_points", C.c_void_p),
+                ("ramps", C.c_void_p), ("input_bindings", C.c_void_p), ("instruction_count", C.c_uint32),
+                ("operand_count", C.c_uint32), ("ramp_point_count", C.c_uint32), ("ramp_count", C.c_uint32),
+                ("input_count", C.c_uint32), ("result", C.c_uint32), ("expression_seed", C.c_uint32),
+                ("random_call_count", C.c_uint32)]
+
+
+class ExprSet(C.Structure):
+    _fields_ = [("image", C.c_void_p), ("byte_size", C.c_uint64), ("program_count", C.c_uint32),
+                ("max_values", C.c_uint32), ("program_offset", C.c_uint32 * 4),
+                ("texel_maps_required", C.c_uint32)]
+
+
+class TexelMap(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("face_count", C.c_uint32), ("resolution", C.c_uint32)]
+
+
+EXPR_MAX_VALUES = 256
+EXPR_MAX_PROGRAMS = 4
+OPS = ["constant", "input", "negate", "logical_not", "add", "subtract", "multiply", "divide", "less",
+       "less_equal", "greater", "greater_equal", "equal", "not_equal", "logical_and", "logical_or",
+       "select", "hash", "random", "minimum", "maximum", "clamp", "fit", "gamma", "contrast",
+       "smoothstep", "ramp", "pow", "mix", "noise3", "texel"]
+OP = {name: i for i, name in enumerate(OPS)}
+ROOT_VARS = ["position_x", "position_y", "position_z", "normal_x", "normal_y", "normal_z", "u", "v",
+             "face", "bary1", "bary2", "id"]
+ROOT_VAR = {name: i for i, name in enumerate(ROOT_VARS)}
+
 # numpy record views of the device/host arrays
 ROOT_DTYPE = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("uv", "<f4", 2),
                        ("triangle_index", "<u4"), ("barycentric", "<f4", 2),
@@ -195,6 +237,7 @@ assert C.sizeof(AssetHeader) == 128
 assert C.sizeof(GenerationParams) == 60
 assert C.sizeof(RootSample) == 48 and ROOT_DTYPE.itemsize == 48
 assert GUIDE_DTYPE.itemsize == 56
+assert C.sizeof(ExprInstruction) == 16
 
 # status codes (device_contract.h:58-80 then extensions)
 STATUS = [
@@ -206,7 +249,7 @@ STATUS = [
     "MISSING_MOTION_SAMPLES", "INVALID_MOTION_TIMES", "MOTION_POINT_CAPACITY_TOO_SMALL",
     "OUTPUT_SIZE_OVERFLOW", "HIP", "WORKSPACE_TOO_SMALL", "INVALID_OCTAVES", "NULL_ARGUMENT",
     "STENCIL_CAPACITY_TOO_SMALL", "INVALID_ROOTS", "INVALID_BUILD_INPUT", "BUFFER_TOO_SMALL",
-    "INVALID_ASSET",
+    "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL",
 ]
 globals().update({name: i for i, name in enumerate(STATUS)})
 

@@ -22,8 +22,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 
 #include "../../include/nanoxgen_amd.h"
+#include "expr_format.h"
 #include "internal.h"
 
 namespace nxg {
@@ -57,6 +59,25 @@ __device__ __forceinline__ V3 normalize(V3 v) {  // generate.h:67-70
     return l2 > 1.0e-20f ? v / sqrtf(l2) : v3(0.0f, 1.0f, 0.0f);
 }
 
+// Positions-only arithmetic.  Everything below that is marked "fast" feeds
+// only point positions, which are checked against the reference to a
+// tolerance; it uses fused multiply-add and the hardware's 1-ulp reciprocal,
+// square root and reciprocal square root.  Root samples, stencil membership,
+// blend weights and radii never go through these helpers.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fma3(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ V3 fma_v(V3 a, float s, V3 c) {  // a * s + c
+    return {fmaf(a.x, s, c.x), fmaf(a.y, s, c.y), fmaf(a.z, s, c.z)};
+}
+__device__ __forceinline__ V3 cross_fast(V3 a, V3 b) {
+    return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
+}
+__device__ __forceinline__ V3 normalize_fast(V3 v) {
+    const float l2 = fma3(v, v);
+    return l2 > 1.0e-20f ? v * fast_rsqrt(l2) : v3(0.0f, 1.0f, 0.0f);
+}
+
 __device__ __forceinline__ uint32_t hash32(uint32_t x) {  // generate.h:72-79
     x ^= x >> 16u;
     x *= 0x7feb352du;
@@ -88,12 +109,14 @@ __device__ __forceinline__ uint32_t temper_index(uint32_t seed) {
     return (((seed & 0x00ff0000u) >> 4u) + (seed & 0xffu)) & 0xffu;
 }
 
-__device__ __forceinline__ float s_curve(float t) {  // generate.h:366-368
-    return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f);
+// noise_s_curve (generate.h:366-368), fast form
+__device__ __forceinline__ float s_curve(float t) {
+    return (t * t * t) * fmaf(t, fmaf(t, 6.0f, -15.0f), 10.0f);
 }
 
-__device__ __forceinline__ float lerp_ref(float a, float b, float alpha) {
-    return (1.0f - alpha) * a + alpha * b;  // generate.h:397-398
+// (1 - alpha) * a + alpha * b (generate.h:397-398), fast form
+__device__ __forceinline__ float lerp_fast(float a, float b, float alpha) {
+    return fmaf(alpha, b - a, a);
 }
 
 // One lattice coordinate of the fixed (non-moving) axes of a noise sample.
@@ -137,8 +160,8 @@ __device__ __forceinline__ NoiseDomain make_domain(V3 d) {
     return n;
 }
 
-// gradient_noise (generate.h:370-402) at the domain point displaced by
-// `distance` along axis A, with the gradient table in LDS.
+// 2 * (gradient_noise - 0.5) (generate.h:370-402) at the domain point
+// displaced by `distance` along axis A, with the gradient table in LDS.
 template <int A>
 __device__ __forceinline__ float gradient_noise_axis(const NoiseDomain &n, float distance,
                                                      const float4 *grad) {
@@ -163,24 +186,30 @@ __device__ __forceinline__ float gradient_noise_axis(const NoiseDomain &n, float
         const int f2 = A == 2 ? oy : oz;
         const uint32_t seed = h0 + (om ? mult : 0u) + n.base[A][f1 | (f2 << 1)];
         const float4 g = grad[temper_index(seed)];
-        v[c] = g.x * w[0][ox] + g.y * w[1][oy] + g.z * w[2][oz];
+        v[c] = fmaf(g.z, w[2][oz], fmaf(g.y, w[1][oy], g.x * w[0][ox]));
     }
-    // x, then y, then z reduction, exactly as the reference's loop nest
-    v[0] = lerp_ref(v[0], v[1], alpha[0]);
-    v[2] = lerp_ref(v[2], v[3], alpha[0]);
-    v[4] = lerp_ref(v[4], v[5], alpha[0]);
-    v[6] = lerp_ref(v[6], v[7], alpha[0]);
-    v[0] = lerp_ref(v[0], v[2], alpha[1]);
-    v[4] = lerp_ref(v[4], v[6], alpha[1]);
-    v[0] = lerp_ref(v[0], v[4], alpha[2]);
-    return 0.5f * v[0] + 0.5f;
+    // x, then y, then z reduction, in the reference's loop-nest order
+    v[0] = lerp_fast(v[0], v[1], alpha[0]);
+    v[2] = lerp_fast(v[2], v[3], alpha[0]);
+    v[4] = lerp_fast(v[4], v[5], alpha[0]);
+    v[6] = lerp_fast(v[6], v[7], alpha[0]);
+    v[0] = lerp_fast(v[0], v[2], alpha[1]);
+    v[4] = lerp_fast(v[4], v[6], alpha[1]);
+    // gradient_noise returns 0.5 * v + 0.5 and the caller subtracts 0.5:
+    // the signed value 0.5 * v is returned directly
+    return lerp_fast(v[0], v[4], alpha[2]);
 }
 
-__device__ __forceinline__ V3 rotate_by(V3 v, V3 a, float angle) {  // generate.h:404-418
-    const float c = cosf(angle), s = sinf(angle), omc = 1.0f - c;
-    return {dot(v, v3(a.x * a.x * omc + c, a.x * a.y * omc - a.z * s, a.x * a.z * omc + a.y * s)),
-            dot(v, v3(a.x * a.y * omc + a.z * s, a.y * a.y * omc + c, a.y * a.z * omc - a.x * s)),
-            dot(v, v3(a.x * a.z * omc - a.y * s, a.y * a.z * omc + a.x * s, a.z * a.z * omc + c))};
+// rotate_by(value, axis, acos(c)) (generate.h:404-418, 477-480) with
+// cos(acos c) = c and sin(acos c) = sqrt((1 - c)(1 + c)); the axis is the
+// unnormalised cross(tangent, next_tangent) exactly as in the reference.
+__device__ __forceinline__ V3 rotate_by_cos(V3 v, V3 a, float c) {
+    const float omc = 1.0f - c;
+    const float s = fast_sqrt(omc * (1.0f + c));
+    const V3 ao = a * omc;
+    return {fma3(v, v3(fmaf(a.x, ao.x, c), fmaf(a.x, ao.y, -(a.z * s)), fmaf(a.x, ao.z, a.y * s))),
+            fma3(v, v3(fmaf(a.x, ao.y, a.z * s), fmaf(a.y, ao.y, c), fmaf(a.y, ao.z, -(a.x * s)))),
+            fma3(v, v3(fmaf(a.x, ao.z, -(a.y * s)), fmaf(a.y, ao.z, a.x * s), fmaf(a.z, ao.z, c)))};
 }
 
 struct Args {
@@ -196,10 +225,11 @@ struct Args {
     const float4 *guide_state;    // 2 float4 per guide
     const float4 *guide_offsets;  // guide-major, cvs float4 per guide
     const nxg_root_sample *explicit_roots;
+    const float4 *attributes;  // optional per-root multipliers (NXG_ATTR_*)
     uint32_t strand_count, cvs, seed, triangle_count, guide_count, has_normals;
     float support_scale, weight_power, rejection_cos, length_scale;
     float root_width, tip_width, radius_scale;
-    float noise_mask, decorrelation_scale, preserve;
+    float noise_mask, decorrelation_scale, preserve, inv_span;
     float amp[NXG_MAX_NOISE_OCTAVES];
     float freq[NXG_MAX_NOISE_OCTAVES];
     nxg_packed_point *points;
@@ -214,6 +244,8 @@ struct Blend {
     uint32_t idx[8];
     float w[8];
     float sum;
+    float inv_sum;  // 1 / sum (fast path of the weighted mean)
+    float length_scale;  // length_scale * length attribute
     uint32_t nearest;
 };
 
@@ -295,20 +327,23 @@ __device__ __forceinline__ Blend make_blend(const Args &a, const Root &r) {
         b.w[j] = w;
         b.sum += w;
     }
+    b.inv_sum = 1.0f / b.sum;
     return b;
 }
 
-// evaluate_base_strand_cv (generate.h:246-267, 300-310) from the offset table
+// evaluate_base_strand_cv (generate.h:246-267, 300-310) from the offset
+// table, fast form (positions only): the weighted mean multiplies by 1/sum.
+// At cv 0 every offset is exactly zero, so the first CV is the root.
 __device__ __forceinline__ V3 base_cv(const Args &a, const Root &r, const Blend &b, uint32_t cv) {
     V3 off = v3(0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         if (b.idx[j] == NXG_INVALID_INDEX) continue;
         const float4 o = a.guide_offsets[uint64_t{b.idx[j]} * a.cvs + cv];
-        off = off + v3(o.x, o.y, o.z) * b.w[j];
+        off = fma_v(v3(o.x, o.y, o.z), b.w[j], off);
     }
     if (b.sum > 1.0e-12f) {
-        off = off / b.sum;
+        off = off * b.inv_sum;
     } else if (b.nearest != NXG_INVALID_INDEX) {
         const float4 o = a.guide_offsets[uint64_t{b.nearest} * a.cvs + cv];
         off = v3(o.x, o.y, o.z);
@@ -316,7 +351,7 @@ __device__ __forceinline__ V3 base_cv(const Args &a, const Root &r, const Blend 
         const float t = static_cast<float>(cv) / static_cast<float>(a.cvs - 1u);
         off = r.normal * t;
     }
-    return r.position + off * a.length_scale;
+    return fma_v(off, b.length_scale, r.position);
 }
 
 // root_surface_u (generate.h:312-341)
@@ -360,42 +395,33 @@ struct NoiseState {
 template <int OCT>
 __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend &b,
                                       const NoiseState<OCT> &ns, Cursor &c, const float4 *grad) {
-    if (c.cv > 0u) c.arc += sqrtf(len2(c.cur - c.prev));
+    if (c.cv > 0u) c.arc += fast_sqrt(fma3(c.cur - c.prev, c.cur - c.prev));
     V3 nt = c.tangent;
     if (c.cv + 1u < a.cvs) {
         const V3 seg = c.next - c.cur;
-        if (len2(seg) > 0.0f) nt = normalize(seg);
+        if (fma3(seg, seg) > 0.0f) nt = normalize_fast(seg);
     }
-    const V3 axis = cross(c.tangent, nt);
-    if (len2(axis) > 0.0f) {
-        const float angle = acosf(clamp_ref(dot(c.tangent, nt), -1.0f, 1.0f));
-        c.tn = normalize(rotate_by(c.tn, axis, angle));
+    const V3 axis = cross_fast(c.tangent, nt);
+    if (fma3(axis, axis) > 0.0f) {
+        const float cosine = clamp_ref(fma3(c.tangent, nt), -1.0f, 1.0f);
+        c.tn = normalize_fast(rotate_by_cos(c.tn, axis, cosine));
     }
     const V3 normal = c.tn;
-    const V3 binormal = cross(normal, nt);
-    const V3 tangent = cross(binormal, normal);
+    const V3 binormal = cross_fast(normal, nt);
+    const V3 tangent = cross_fast(binormal, normal);
     V3 point = c.cur;
     if (c.cv > 0u) {
-        const float t = static_cast<float>(c.cv) / static_cast<float>(a.cvs - 1u);
+        const float half_t = 0.5f * static_cast<float>(c.cv) * a.inv_span;
         float lx = 0.0f, ly = 0.0f, lz = 0.0f;
 #pragma unroll
         for (int k = 0; k < OCT; ++k) {
             const float distance = c.arc * ns.freq[k];
-            const float magnitude = ns.mag[k] * t;
-            const float nx = (gradient_noise_axis<0>(ns.dom, distance, grad) - 0.5f) * magnitude;
-            const float ny = (gradient_noise_axis<1>(ns.dom, distance, grad) - 0.5f) * magnitude;
-            const float nz = (gradient_noise_axis<2>(ns.dom, distance, grad) - 0.5f) * magnitude;
-            if (k == 0) {
-                lx = nx;
-                ly = ny;
-                lz = nz;
-            } else {
-                lx += nx;
-                ly += ny;
-                lz += nz;
-            }
+            const float m = ns.mag[k] * half_t;  // noise_mask * amplitude * t / 2
+            lx = fmaf(gradient_noise_axis<0>(ns.dom, distance, grad), m, lx);
+            ly = fmaf(gradient_noise_axis<1>(ns.dom, distance, grad), m, ly);
+            lz = fmaf(gradient_noise_axis<2>(ns.dom, distance, grad), m, lz);
         }
-        point = point + normal * lx + binormal * ly + tangent * lz;
+        point = fma_v(tangent, lz, fma_v(binormal, ly, fma_v(normal, lx, point)));
     }
     c.tangent = nt;
     c.prev = c.cur;
@@ -531,7 +557,10 @@ __global__ void __launch_bounds__(256) strand_kernel(Args a) {
     } else {
         r = sample_root(a, strand);
     }
-    const Blend b = make_blend(a, r);
+    Blend b = make_blend(a, r);
+    // per-root attribute multipliers; 1.0 (no attributes) is exact
+    const float4 attr = a.attributes ? a.attributes[strand] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    b.length_scale = a.length_scale * attr.x;
     if (live) {
         if (a.roots) {
             nxg_root_sample o;
@@ -569,7 +598,7 @@ __global__ void __launch_bounds__(256) strand_kernel(Args a) {
 #pragma unroll
         for (int k = 0; k < OCT; ++k) {
             ns.freq[k] = base_length > 0.0f ? fmax_ref(0.5f / base_length, a.freq[k]) : a.freq[k];
-            ns.mag[k] = a.noise_mask * a.amp[k];
+            ns.mag[k] = a.noise_mask * a.amp[k] * attr.z;
         }
         // rest_root_position (generate.h:343-351), noise domain (generate.h:548-552)
         const nxg_uint3 t = a.triangles[r.tri];
@@ -610,7 +639,7 @@ __global__ void __launch_bounds__(256) strand_kernel(Args a) {
                 if (preserve_scale != 1.0f) p = r.position + (p - r.position) * preserve_scale;
                 // evaluate_strand_width (generate.h:577-581) and the packed radius
                 const float t = static_cast<float>(cv) / static_cast<float>(a.cvs - 1u);
-                const float width = a.root_width * (1.0f - t) + a.tip_width * t;
+                const float width = (a.root_width * (1.0f - t) + a.tip_width * t) * attr.y;
                 stage[(k * 4u + 0u) * kStageStride + lane] = p.x;
                 stage[(k * 4u + 1u) * kStageStride + lane] = p.y;
                 stage[(k * 4u + 2u) * kStageStride + lane] = p.z;
@@ -628,6 +657,276 @@ __global__ void __launch_bounds__(256) strand_kernel(Args a) {
             if ((diff < 0.0f ? -diff : diff) >= 0.0001f) preserve_scale = target / noisy;
         }
     }
+}
+
+// ---- float expression runtime ------------------------------------------
+
+// gradient_noise (generate.h:370-402) operation for operation, without
+// contraction: the expression runtime's noise3 is bit-identical to it.
+__device__ float gradient_noise_exact(float sx, float sy, float sz, const float4 *grad) {
+    const float fx = floorf(sx), fy = floorf(sy), fz = floorf(sz);
+    const int32_t ix = static_cast<int32_t>(fx), iy = static_cast<int32_t>(fy), iz = static_cast<int32_t>(fz);
+    const float wx = sx - fx, wy = sy - fy, wz = sz - fz;
+    float v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        const uint32_t seed = static_cast<uint32_t>(ix + ox) * kLcgA2 + static_cast<uint32_t>(iy + oy) * kLcgA +
+                              static_cast<uint32_t>(iz + oz) + kLcgK;
+        const float4 g = grad[temper_index(seed)];
+        v[c] = g.x * (wx - static_cast<float>(ox)) + g.y * (wy - static_cast<float>(oy)) +
+               g.z * (wz - static_cast<float>(oz));
+    }
+    const float ax = wx * wx * wx * (wx * (wx * 6.0f - 15.0f) + 10.0f);
+    const float ay = wy * wy * wy * (wy * (wy * 6.0f - 15.0f) + 10.0f);
+    const float az = wz * wz * wz * (wz * (wz * 6.0f - 15.0f) + 10.0f);
+    v[0] = (1.0f - ax) * v[0] + ax * v[1];
+    v[2] = (1.0f - ax) * v[2] + ax * v[3];
+    v[4] = (1.0f - ax) * v[4] + ax * v[5];
+    v[6] = (1.0f - ax) * v[6] + ax * v[7];
+    v[0] = (1.0f - ay) * v[0] + ay * v[2];
+    v[4] = (1.0f - ay) * v[4] + ay * v[6];
+    v[0] = (1.0f - az) * v[0] + az * v[4];
+    return 0.5f * v[0] + 0.5f;
+}
+
+__constant__ uint8_t kPermutation[256] = {
+#include "seexpr_permutation.inc"
+};
+
+// xgen_seexpr_hash_finish (xgen_expression.cpp:637-651)
+__device__ uint32_t seexpr_hash_finish(uint32_t state) {
+    state ^= state >> 11u;
+    state ^= (state << 7u) & 0x9d2c5680u;
+    state ^= (state << 15u) & 0xefc60000u;
+    state ^= state >> 18u;
+    const uint32_t b3 = kPermutation[state & 0xffu];
+    const uint32_t b2 = kPermutation[((state >> 8u) + b3) & 0xffu];
+    const uint32_t b1 = kPermutation[((state >> 16u) + b2) & 0xffu];
+    const uint32_t b0 = kPermutation[((state >> 24u) + b1) & 0xffu];
+    return b0 | (b1 << 8u) | (b2 << 16u) | (b3 << 24u);
+}
+
+// xgen_runtime_hash_component (xgen_expression.cpp:663-709): the top 32
+// fraction bits of frexp(double(x) * 8.539734222673566) xor the exponent,
+// from a 24 x 53-bit integer product (no FP64).
+__device__ uint32_t runtime_hash_component(float argument) {
+    constexpr uint64_t cm = 0x001114580b45d474ull;
+    const uint32_t bits = __float_as_uint(argument);
+    const uint32_t absolute = bits & 0x7fffffffu;
+    if (absolute == 0u) return 0u;
+    const uint32_t fexp = absolute >> 23u;
+    const uint32_t mant = fexp == 0u ? (absolute & 0x7fffffu) : ((absolute & 0x7fffffu) | 0x800000u);
+    const uint64_t p0 = uint64_t{mant} * static_cast<uint32_t>(cm);
+    const uint64_t p1 = uint64_t{mant} * static_cast<uint32_t>(cm >> 32u);
+    const uint64_t low = p0 + (p1 << 32u);
+    const uint64_t high = (p1 >> 32u) + (low < p0 ? 1u : 0u);
+    const uint32_t length = high != 0u ? 128u - static_cast<uint32_t>(__clzll(high))
+                                       : 64u - static_cast<uint32_t>(__clzll(low));
+    const uint32_t shift = length - 32u;
+    const uint32_t top = static_cast<uint32_t>((low >> shift) | (high << (64u - shift)));
+    const uint64_t remainder = low & ((uint64_t{1} << shift) - 1u);
+    const uint64_t ceil_div = ((low >> 32u) | (high << 32u)) + ((low & 0xffffffffu) != 0u ? 1u : 0u);
+    const uint32_t fraction = top - (remainder < ceil_div ? 1u : 0u);
+    const int32_t value_exponent = fexp == 0u ? -149 : static_cast<int32_t>(fexp) - 150;
+    const uint32_t exponent = static_cast<uint32_t>(static_cast<int32_t>(length) + value_exponent - 49);
+    return ((bits >> 31u) != 0u ? 0u - fraction : fraction) ^ exponent;
+}
+
+__device__ __forceinline__ uint32_t hash_step(uint32_t state, float x) {
+    return state * 1664525u + runtime_hash_component(x) + 1013904223u;
+}
+__device__ __forceinline__ float hash_to_unit(uint32_t state) {
+    return static_cast<float>(seexpr_hash_finish(state)) * 0x1p-32f;
+}
+
+__device__ __forceinline__ float clamp_std(float v, float lo, float hi) {  // std::clamp
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+
+// evaluate_ramp_points<float> (xgen_expression.cpp:559-615)
+__device__ float ramp_eval(const nxg_expr_ramp_point *pts, uint32_t n, float x) {
+    if (x <= pts[0].position) return clamp_std(pts[0].value, 0.0f, 1.0f);
+    if (x >= pts[n - 1u].position) return clamp_std(pts[n - 1u].value, 0.0f, 1.0f);
+    uint32_t upper = 1u;
+    while (upper < n && x >= pts[upper].position) ++upper;
+    const uint32_t lower = upper - 1u;
+    const float p0 = pts[lower].position, p1 = pts[upper].position;
+    const float k0 = pts[lower].value, k1 = pts[upper].value;
+    const float u = (x - p0) / (p1 - p0);
+    float value = k0;
+    const uint32_t mode = pts[lower].interpolation;
+    if (mode == 1u) {
+        value = k0 + u * (k1 - k0);
+    } else if (mode >= 2u) {
+        value = k0 * (u - 1.0f) * (u - 1.0f) * (2.0f * u + 1.0f) + k1 * u * u * (3.0f - 2.0f * u);
+        if (mode == 3u) {
+            const float scale = 1.0f / (p1 - p0);
+            const float d0 = lower == 0u ? (k1 - k0) / (scale * (2.0f * p1 - 2.0f * p0))
+                                         : (k1 - pts[lower - 1u].value) / (scale * (p1 - pts[lower - 1u].position));
+            const float d1 = upper + 1u == n ? (k1 - k0) / (scale * (2.0f * p1 - 2.0f * p0))
+                                             : (pts[upper + 1u].value - k0) / (scale * (pts[upper + 1u].position - p0));
+            value = k0 * (u - 1.0f) * (u - 1.0f) * (2.0f * u + 1.0f) + k1 * u * u * (3.0f - 2.0f * u) +
+                    d0 * u * (u - 1.0f) * (u - 1.0f) + d1 * u * u * (u - 1.0f);
+        }
+    }
+    return clamp_std(value, 0.0f, 1.0f);
+}
+
+struct AttrArgs {
+    const nxg_vec3 *positions;
+    const nxg_vec3 *normals;
+    const nxg_vec2 *texcoords;
+    const nxg_uint3 *triangles;
+    const nxg_alias_entry *alias;
+    const nxg_vec3 *gradients;
+    const uint8_t *programs;  // packed set image
+    nxg_texel_map maps[NXG_EXPR_MAX_TEXEL_MAPS];
+    uint32_t map_count, strand_count, seed, triangle_count, has_normals, program_count;
+    uint32_t program_offset[NXG_EXPR_MAX_PROGRAMS];
+    float4 *attributes;
+    uint32_t *error_word;
+};
+
+// One lane per root.  The program is the same for every lane, so opcode
+// dispatch is wave-uniform (scalar branches, instructions fetched with scalar
+// loads); SSA values live in LDS at values[index * blockDim + lane].
+__global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
+    extern __shared__ float lds[];
+    float4 *grad = reinterpret_cast<float4 *>(lds);
+    float *values = lds + 4u * NXG_NOISE_GRADIENT_COUNT;
+    for (uint32_t i = threadIdx.x; i < NXG_NOISE_GRADIENT_COUNT; i += blockDim.x) {
+        const nxg_vec3 g = a.gradients[i];
+        grad[i] = make_float4(g.x, g.y, g.z, 0.0f);
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x, B = blockDim.x;
+    const uint64_t strand64 = uint64_t{blockIdx.x} * B + lane;
+    const bool live = strand64 < a.strand_count;
+    const uint32_t strand = live ? static_cast<uint32_t>(strand64) : a.strand_count - 1u;
+
+    Args ra{};
+    ra.positions = a.positions;
+    ra.normals = a.normals;
+    ra.texcoords = a.texcoords;
+    ra.triangles = a.triangles;
+    ra.alias = a.alias;
+    ra.seed = a.seed;
+    ra.triangle_count = a.triangle_count;
+    ra.has_normals = a.has_normals;
+    const Root r = sample_root(ra, strand);
+    const float vars[NXG_ROOT_VAR_COUNT] = {r.position.x, r.position.y, r.position.z, r.normal.x,
+                                            r.normal.y, r.normal.z, r.u, r.v, static_cast<float>(r.tri),
+                                            r.b1, r.b2, static_cast<float>(strand)};
+    const float face_seed = hash_to_unit(hash_step(hash_step(hash_step(0u, 0.0f), 0.0f), static_cast<float>(r.tri)));
+
+    float result[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    bool bad = false;
+    for (uint32_t k = 0; k < a.program_count; ++k) {
+        const PackedProgram *prog = reinterpret_cast<const PackedProgram *>(a.programs + a.program_offset[k]);
+        const PackedInstruction *ins = reinterpret_cast<const PackedInstruction *>(a.programs + prog->instructions_offset);
+        const uint32_t *operands = reinterpret_cast<const uint32_t *>(a.programs + prog->operands_offset);
+        const nxg_expr_ramp *ramps = reinterpret_cast<const nxg_expr_ramp *>(a.programs + prog->ramps_offset);
+        const nxg_expr_ramp_point *points =
+            reinterpret_cast<const nxg_expr_ramp_point *>(a.programs + prog->ramp_points_offset);
+        const uint32_t n = prog->instruction_count;
+        for (uint32_t i = 0; i < n; ++i) {
+            const PackedInstruction in = ins[i];
+            const uint32_t *ops = operands + in.operand_offset;
+#define OPV(j) values[ops[j] * B + lane]
+            float value = 0.0f;
+            switch (in.op) {
+                case NXG_OP_CONSTANT: value = in.immediate; break;
+                case NXG_OP_INPUT: {
+                    value = vars[0];
+#pragma unroll
+                    for (uint32_t q = 1; q < NXG_ROOT_VAR_COUNT; ++q)
+                        if (in.auxiliary == q) value = vars[q];
+                    break;
+                }
+                case NXG_OP_NEGATE: value = -OPV(0); break;
+                case NXG_OP_LOGICAL_NOT: value = OPV(0) == 0.0f; break;
+                case NXG_OP_ADD: value = OPV(0) + OPV(1); break;
+                case NXG_OP_SUBTRACT: value = OPV(0) - OPV(1); break;
+                case NXG_OP_MULTIPLY: value = OPV(0) * OPV(1); break;
+                case NXG_OP_DIVIDE: value = OPV(0) / OPV(1); break;
+                case NXG_OP_LESS: value = OPV(0) < OPV(1); break;
+                case NXG_OP_LESS_EQUAL: value = OPV(0) <= OPV(1); break;
+                case NXG_OP_GREATER: value = OPV(0) > OPV(1); break;
+                case NXG_OP_GREATER_EQUAL: value = OPV(0) >= OPV(1); break;
+                case NXG_OP_EQUAL: value = OPV(0) == OPV(1); break;
+                case NXG_OP_NOT_EQUAL: value = OPV(0) != OPV(1); break;
+                case NXG_OP_LOGICAL_AND: value = OPV(0) != 0.0f && OPV(1) != 0.0f; break;
+                case NXG_OP_LOGICAL_OR: value = OPV(0) != 0.0f || OPV(1) != 0.0f; break;
+                case NXG_OP_SELECT: value = OPV(0) != 0.0f ? OPV(1) : OPV(2); break;
+                case NXG_OP_HASH: {
+                    uint32_t state = 0u;
+                    for (uint32_t j = 0; j < in.operand_count; ++j) state = hash_step(state, OPV(j));
+                    value = hash_to_unit(state);
+                    break;
+                }
+                case NXG_OP_RANDOM: {
+                    const bool explicit_seed = in.operand_count == 1u || in.operand_count == 3u;
+                    uint32_t state = hash_step(hash_step(hash_step(hash_step(0u, r.u), r.v), face_seed), in.immediate);
+                    if (explicit_seed) state = hash_step(state, OPV(in.operand_count - 1u));
+                    value = hash_to_unit(state);
+                    if (in.operand_count >= 2u) value = (OPV(1) - OPV(0)) * value + OPV(0);
+                    break;
+                }
+                case NXG_OP_MINIMUM: { const float x = OPV(0), y = OPV(1); value = y < x ? y : x; break; }
+                case NXG_OP_MAXIMUM: { const float x = OPV(0), y = OPV(1); value = x < y ? y : x; break; }
+                case NXG_OP_CLAMP: value = clamp_std(OPV(0), OPV(1), OPV(2)); break;
+                case NXG_OP_FIT: value = OPV(3) + (OPV(0) - OPV(1)) * (OPV(4) - OPV(3)) / (OPV(2) - OPV(1)); break;
+                case NXG_OP_GAMMA: value = powf(OPV(0), 1.0f / OPV(1)); break;
+                case NXG_OP_CONTRAST: {
+                    const float x = OPV(0);
+                    const float exponent = -log2f(x < 0.5f ? 2.0f * x : 2.0f - 2.0f * x);
+                    const float shaped = 0.5f * powf(1.0f - OPV(1), exponent);
+                    value = x < 0.5f ? shaped : 1.0f - shaped;
+                    break;
+                }
+                case NXG_OP_SMOOTHSTEP: {
+                    const float t = clamp_std((OPV(0) - OPV(1)) / (OPV(2) - OPV(1)), 0.0f, 1.0f);
+                    value = t * t * (3.0f - 2.0f * t);
+                    break;
+                }
+                case NXG_OP_RAMP: {
+                    const nxg_expr_ramp rp = ramps[in.auxiliary];
+                    value = ramp_eval(points + rp.point_offset, rp.point_count, 0.0f);
+                    break;
+                }
+                case NXG_OP_POW: value = powf(OPV(0), OPV(1)); break;
+                case NXG_OP_MIX: {
+                    const float x = OPV(0), y = OPV(1), t = OPV(2);
+                    value = x + (t - 0.0f) * (y - x) / (1.0f - 0.0f);
+                    break;
+                }
+                case NXG_OP_NOISE3: value = gradient_noise_exact(OPV(0), OPV(1), OPV(2), grad); break;
+                case NXG_OP_TEXEL: {
+                    const nxg_texel_map m = a.maps[in.auxiliary < a.map_count ? in.auxiliary : 0u];
+                    const float R = static_cast<float>(m.resolution);
+                    const float ff = clamp_std(floorf(OPV(0)), 0.0f, static_cast<float>(m.face_count - 1u));
+                    const float fu = clamp_std(floorf(OPV(1) * R), 0.0f, R - 1.0f);
+                    const float fv = clamp_std(floorf(OPV(2) * R), 0.0f, R - 1.0f);
+                    value = m.texels[(static_cast<uint64_t>(ff) * m.resolution + static_cast<uint32_t>(fv)) *
+                                         m.resolution + static_cast<uint32_t>(fu)];
+                    break;
+                }
+                default: value = __builtin_nanf(""); break;
+            }
+#undef OPV
+            if (!isfinite(value)) bad = true;
+            values[i * B + lane] = value;
+        }
+        result[k] = values[prog->result * B + lane];
+    }
+    if (!live) return;
+    if (bad) {
+        const float nan = __builtin_nanf("");
+        result[0] = result[1] = result[2] = result[3] = nan;
+        if (a.error_word) atomicOr(a.error_word, 1u);
+    }
+    a.attributes[strand] = make_float4(result[0], result[1], result[2], result[3]);
 }
 
 bool noise_enabled(const nxg_generation_params &p, const nxg_noise_octaves *o) {
@@ -648,7 +947,8 @@ enum Stage : uint32_t { kPrepare = 1u, kStrands = 2u };
 
 uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
                 const nxg_generation_params *params, const nxg_noise_octaves *octaves,
-                const nxg_root_sample *explicit_roots, const nxg_packed_output *output,
+                const nxg_root_sample *explicit_roots, const float *attributes,
+                uint64_t attribute_capacity, const nxg_packed_output *output,
                 const nxg_stencil_output *stencil, const nxg_launch_config *config, void *workspace,
                 uint64_t workspace_bytes, void *stream_ptr, uint32_t stages) {
     if (uint32_t s = validate_common(asset, deformed, params, config); s != NXG_OK) return s;
@@ -656,6 +956,8 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
         if (uint32_t s = validate_packed_output(*params, output); s != NXG_OK) return s;
     }
     if (!valid_octaves(octaves)) return NXG_ERROR_INVALID_OCTAVES;
+    if (attributes && attribute_capacity < uint64_t{params->strand_count} * NXG_ATTR_COUNT)
+        return NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL;
     if (stencil && (stencil->indices || stencil->weights)) {
         if (!stencil->indices || !stencil->weights ||
             stencil->capacity < uint64_t{params->strand_count} * 8u)
@@ -698,6 +1000,7 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     a.guide_state = state;
     a.guide_offsets = offsets;
     a.explicit_roots = explicit_roots;
+    a.attributes = reinterpret_cast<const float4 *>(attributes);
     a.strand_count = params->strand_count;
     a.cvs = params->cvs_per_strand;
     a.seed = params->seed;
@@ -715,6 +1018,7 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     const float decorrelation = 1.0f - params->noise_correlation;
     a.decorrelation_scale = 100.0f * decorrelation * decorrelation;
     a.preserve = params->noise_preserve_length;
+    a.inv_span = 1.0f / static_cast<float>(params->cvs_per_strand - 1u);
     uint32_t oct = 1u;
     if (octaves && octaves->count > 1u) {
         oct = octaves->count;
@@ -747,17 +1051,83 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
+uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
+                             const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count,
+                             float *attributes, uint64_t attribute_capacity, uint32_t *error_word,
+                             void *stream_ptr) {
+    if (uint32_t s = validate_common(asset, nullptr, params, nullptr); s != NXG_OK) return s;
+    if (!set || !set->image || reinterpret_cast<uintptr_t>(set->image) % 16u != 0u ||
+        set->byte_size < sizeof(PackedSet) || set->program_count == 0u ||
+        set->program_count > NXG_EXPR_MAX_PROGRAMS || set->max_values == 0u ||
+        set->max_values > NXG_EXPR_MAX_VALUES)
+        return NXG_ERROR_INVALID_PROGRAM;
+    for (uint32_t k = 0; k < set->program_count; ++k)
+        if (set->program_offset[k] < sizeof(PackedSet) || set->program_offset[k] >= set->byte_size)
+            return NXG_ERROR_INVALID_PROGRAM;
+    if (!attributes || attribute_capacity < uint64_t{params->strand_count} * NXG_ATTR_COUNT)
+        return NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL;
+    if (map_count > NXG_EXPR_MAX_TEXEL_MAPS || (map_count && !maps) || map_count < set->texel_maps_required)
+        return NXG_ERROR_INVALID_PROGRAM;
+    for (uint32_t m = 0; m < map_count; ++m)
+        if (!maps[m].texels || maps[m].face_count == 0u || maps[m].resolution == 0u) return NXG_ERROR_INVALID_PROGRAM;
+    const nxg_asset_header &h = asset->header;
+    AttrArgs a{};
+    a.positions = section<nxg_vec3>(*asset, h.positions_offset);
+    a.normals = section<nxg_vec3>(*asset, h.normals_offset);
+    a.texcoords = (h.flags & NXG_ASSET_HAS_TEXCOORDS) ? section<nxg_vec2>(*asset, h.texcoords_offset) : nullptr;
+    a.triangles = section<nxg_uint3>(*asset, h.triangles_offset);
+    a.alias = section<nxg_alias_entry>(*asset, h.alias_table_offset);
+    a.gradients = section<nxg_vec3>(*asset, h.noise_gradients_offset);
+    a.programs = static_cast<const uint8_t *>(set->image);
+    for (uint32_t m = 0; m < map_count; ++m) a.maps[m] = maps[m];
+    a.map_count = map_count;
+    a.strand_count = params->strand_count;
+    a.seed = params->seed;
+    a.triangle_count = h.triangle_count;
+    a.has_normals = (h.flags & NXG_ASSET_HAS_NORMALS) != 0u;
+    a.program_count = set->program_count;
+    for (uint32_t k = 0; k < set->program_count; ++k) a.program_offset[k] = set->program_offset[k];
+    a.attributes = reinterpret_cast<float4 *>(attributes);
+    a.error_word = error_word;
+    // values[max_values][block] floats plus the gradient table in 64 KB of LDS
+    uint32_t block = 256u;
+    while (block > 64u && uint64_t{set->max_values} * block * 4u + 4096u > 65536u) block -= 64u;
+    const uint32_t lds = set->max_values * block * 4u + 4u * 4u * NXG_NOISE_GRADIENT_COUNT;
+    hipLaunchKernelGGL(attribute_kernel, dim3((params->strand_count + block - 1u) / block), dim3(block), lds,
+                       static_cast<hipStream_t>(stream_ptr), a);
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
 }  // namespace
 }  // namespace nxg
 
 extern "C" {
+
+uint32_t nxg_evaluate_root_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
+                                      const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count,
+                                      float *attributes, uint64_t attribute_capacity, uint32_t *error_word,
+                                      void *stream) {
+    return nxg::evaluate_attributes(asset, params, set, maps, map_count, attributes, attribute_capacity,
+                                    error_word, stream);
+}
+
+uint32_t nxg_generate_packed_attributed(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                                        const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                                        const float *attributes, uint64_t attribute_capacity,
+                                        const nxg_packed_output *output, const nxg_stencil_output *stencil,
+                                        const nxg_launch_config *config, void *workspace,
+                                        uint64_t workspace_bytes, void *stream) {
+    if (!attributes) return NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL;
+    return nxg::launch(asset, deformed, params, octaves, nullptr, attributes, attribute_capacity, output,
+                       stencil, config, workspace, workspace_bytes, stream, nxg::kPrepare | nxg::kStrands);
+}
 
 uint32_t nxg_generate_packed(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
                              const nxg_generation_params *params, const nxg_noise_octaves *octaves,
                              const nxg_packed_output *output, const nxg_stencil_output *stencil,
                              const nxg_launch_config *config, void *workspace,
                              uint64_t workspace_bytes, void *stream) {
-    return nxg::launch(asset, deformed, params, octaves, nullptr, output, stencil, config, workspace,
+    return nxg::launch(asset, deformed, params, octaves, nullptr, nullptr, 0, output, stencil, config, workspace,
                        workspace_bytes, stream, nxg::kPrepare | nxg::kStrands);
 }
 
@@ -765,7 +1135,7 @@ uint32_t nxg_prepare_packed_generation(const nxg_device_asset *asset,
                                        const nxg_deformed_geometry *deformed,
                                        const nxg_generation_params *params, void *workspace,
                                        uint64_t workspace_bytes, void *stream) {
-    return nxg::launch(asset, deformed, params, nullptr, nullptr, nullptr, nullptr, nullptr, workspace,
+    return nxg::launch(asset, deformed, params, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, workspace,
                        workspace_bytes, stream, nxg::kPrepare);
 }
 
@@ -777,7 +1147,7 @@ uint32_t nxg_generate_packed_prepared(const nxg_device_asset *asset,
                                       const nxg_stencil_output *stencil,
                                       const nxg_launch_config *config, void *workspace,
                                       uint64_t workspace_bytes, void *stream) {
-    return nxg::launch(asset, deformed, params, octaves, nullptr, output, stencil, config, workspace,
+    return nxg::launch(asset, deformed, params, octaves, nullptr, nullptr, 0, output, stencil, config, workspace,
                        workspace_bytes, stream, nxg::kStrands);
 }
 
@@ -789,7 +1159,7 @@ uint32_t nxg_generate_packed_from_roots(const nxg_device_asset *asset,
                                         const nxg_launch_config *config, void *workspace,
                                         uint64_t workspace_bytes, void *stream) {
     if (!roots) return NXG_ERROR_INVALID_ROOTS;
-    return nxg::launch(asset, nullptr, params, octaves, roots, output, nullptr, config, workspace,
+    return nxg::launch(asset, nullptr, params, octaves, roots, nullptr, 0, output, nullptr, config, workspace,
                        workspace_bytes, stream, nxg::kPrepare | nxg::kStrands);
 }
 

@@ -196,3 +196,17 @@ def config2(seed=2, strands=2_000_000, quads=512, guides_per_side=64):
     octaves = NoiseOctaves.of([(0.1, 4.0), (0.05, 8.0)])
     name = f"plane-{2 * quads * quads}tri-{guides_per_side ** 2}guides-{strands}-strands-16cv-2oct"
     return Workload(name, build, params, octaves)
+
+
+def config3(seed=1, strands=1_000_000, program_seed=3, map_seed=4):
+    """Plain method with expressions: the icosphere description, 1,000,000
+    roots x 24 CVs, per-root length/width/noise-magnitude programs of 40 ops
+    (4 noise3, 2 texel reads of a 16x16 per-face U(0,1) map)."""
+    from . import expr
+    build = icosphere_description(seed=seed)
+    params = GenerationParams(strand_count=strands, cvs_per_strand=24, seed=seed, root_width=0.02,
+                              tip_width=0.002, noise_amplitude=0.05, noise_frequency=5.0)
+    programs = expr.attribute_programs(seed=program_seed)
+    maps = [expr.texel_map(len(build.triangles), 16, seed=map_seed)]
+    return Workload(f"icosphere-1280tri-{strands}-strands-24cv-3x40op-expr", build, params, None,
+                    info=dict(programs=programs, maps=maps))

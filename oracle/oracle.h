@@ -37,6 +37,8 @@ typedef struct orc_job {
      * output slots 0..list_count (explicit_roots then index by slot) */
     const uint32_t *strand_list;
     uint32_t list_count;
+    /* optional per-root attribute multipliers, 4 floats per output slot */
+    const float *attributes;
 } orc_job;
 
 /* generate.h:72-84 */
@@ -59,6 +61,8 @@ void orc_strand_state_f64(const orc_job *job, uint32_t strand, double *out8);
  * precision: 32 runs the restatement, 64 the FP64 twin. */
 int orc_generate(const orc_job *job, uint32_t precision, uint32_t workers,
                  uint32_t strands_per_block);
+/* The same persistent-tile pool for any per-item function. */
+void orc_run_tiles(uint32_t count, uint32_t workers, uint32_t block, void (*fn)(void *, uint32_t), void *ctx);
 
 /* asset.cpp:229-293 (build_asset) into a malloc'd 64-aligned blob the
  * caller frees with orc_free.  Returns 0, or 1 with the reference's
@@ -68,6 +72,22 @@ int orc_build_asset(const nxg_asset_build_input *in, void **blob, uint64_t *size
 void orc_free(void *p);
 /* asset.cpp:295-338: 0 and "" when valid, else 1 and the message. */
 int orc_validate_asset(const void *blob, uint64_t size, char *msg, uint64_t msg_capacity);
+
+void orc_sample_root_f32(const void *blob, uint32_t seed, uint32_t strand, nxg_root_sample *out);
+
+/* float expression runtime (xgen_expression.cpp:637-722, 984-1148) */
+uint32_t orc_seexpr_hash_finish(uint32_t state);
+uint32_t orc_runtime_hash_component(float argument);
+float orc_runtime_hash(const float *args, uint32_t n);
+int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u, float v, float face_seed,
+                  float t, const nxg_vec3 *gradients, const nxg_texel_map *maps, uint32_t map_count,
+                  float *scratch, float *out);
+/* Attributes of strands (all, or job->strand_list) into attributes[4*slot+k]:
+ * the root variables and context of nxg_evaluate_root_attributes.
+ * Returns the number of roots whose evaluation failed (their slots get NaN). */
+uint32_t orc_root_attributes(const orc_job *job, const nxg_expr_program *programs, uint32_t program_count,
+                             const nxg_texel_map *maps, uint32_t map_count, float *attributes,
+                             uint32_t workers);
 
 #ifdef __cplusplus
 }

@@ -340,8 +340,9 @@ int orc_validate_asset(const void *bytes, uint64_t size, char *msg, uint64_t cap
 /* asset.cpp:160-202: persistent tiles over an atomic counter                */
 
 typedef struct {
-    const orc_job *job;
-    uint32_t precision, block, blocks, strands;
+    void (*fn)(void *, uint32_t);
+    void *ctx;
+    uint32_t block, blocks, count;
     atomic_uint next;
 } pool_t;
 
@@ -352,24 +353,19 @@ static void *worker(void *arg) {
         if (block >= p->blocks) return NULL;
         const uint64_t first = (uint64_t)block * p->block;
         uint64_t end = first + p->block;
-        if (end > p->strands) end = p->strands;
-        for (uint64_t s = first; s < end; ++s) {
-            const uint32_t strand = p->job->strand_list ? p->job->strand_list[s] : (uint32_t)s;
-            if (p->precision == 64u) orc_generate_strand_f64(p->job, strand, s);
-            else orc_generate_strand_f32(p->job, strand, s);
-        }
+        if (end > p->count) end = p->count;
+        for (uint64_t i = first; i < end; ++i) p->fn(p->ctx, (uint32_t)i);
     }
 }
 
-int orc_generate(const orc_job *job, uint32_t precision, uint32_t workers, uint32_t block) {
+void orc_run_tiles(uint32_t count, uint32_t workers, uint32_t block, void (*fn)(void *, uint32_t), void *ctx) {
     if (block == 0u) block = 128u;
-    const uint32_t n = job->strand_list ? job->list_count : job->params->strand_count;
     pool_t p;
-    p.job = job;
-    p.precision = precision;
+    p.fn = fn;
+    p.ctx = ctx;
     p.block = block;
-    p.strands = n;
-    p.blocks = (uint32_t)(((uint64_t)n + block - 1u) / block);
+    p.count = count;
+    p.blocks = (uint32_t)(((uint64_t)count + block - 1u) / block);
     atomic_init(&p.next, 0u);
     uint32_t w = workers;
     if (w == 0u) {
@@ -381,11 +377,26 @@ int orc_generate(const orc_job *job, uint32_t precision, uint32_t workers, uint3
     if (w > p.blocks) w = p.blocks;
     if (w <= 1u) {
         worker(&p);
-        return 0;
+        return;
     }
     pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * w);
     for (uint32_t i = 0; i < w; ++i) pthread_create(&th[i], NULL, worker, &p);
     for (uint32_t i = 0; i < w; ++i) pthread_join(th[i], NULL);
     free(th);
+}
+
+typedef struct { const orc_job *job; uint32_t precision; } gen_ctx;
+
+static void gen_one(void *c, uint32_t slot) {
+    const gen_ctx *g = (const gen_ctx *)c;
+    const uint32_t strand = g->job->strand_list ? g->job->strand_list[slot] : slot;
+    if (g->precision == 64u) orc_generate_strand_f64(g->job, strand, slot);
+    else orc_generate_strand_f32(g->job, strand, slot);
+}
+
+int orc_generate(const orc_job *job, uint32_t precision, uint32_t workers, uint32_t block) {
+    gen_ctx c = {job, precision};
+    const uint32_t n = job->strand_list ? job->list_count : job->params->strand_count;
+    orc_run_tiles(n, workers, block, gen_one, &c);
     return 0;
 }

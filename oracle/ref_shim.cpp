@@ -10,6 +10,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <span>
+#include <string>
 #include <vector>
 
 #include "../include/nanoxgen_amd.h"
@@ -166,6 +168,43 @@ void ref_strand_state(const void *blob, const nxg_generation_params *p, uint32_t
 }
 
 uint32_t ref_hash32(uint32_t x) { return hash32(x); }
+
+// evaluate_xgen_scalar_expression_float (xgen_expression.cpp:984-1148) on a
+// flattened program (reference op set only); inputs[i] is program input i.
+// Returns 0 and *out, or 1 where the reference throws.
+int ref_expr_eval(const nxg_expr_instruction *ins, uint32_t n_ins, const uint32_t *operands, uint32_t n_ops,
+                  const nxg_expr_ramp_point *points, uint32_t n_points, const nxg_expr_ramp *ramps,
+                  uint32_t n_ramps, const float *inputs, uint32_t n_inputs, uint32_t result, float u, float v,
+                  float face_seed, float t, float *out) {
+    try {
+        XgenFloatExpressionProgram p{};
+        for (uint32_t i = 0; i < n_ins; ++i) {
+            p.instructions.push_back({static_cast<XgenScalarOp>(ins[i].op), ins[i].operand_offset,
+                                      ins[i].operand_count, ins[i].auxiliary, ins[i].immediate});
+        }
+        p.operands.assign(operands, operands + n_ops);
+        for (uint32_t i = 0; i < n_inputs; ++i) p.inputs.push_back("in" + std::to_string(i));
+        for (uint32_t i = 0; i < n_points; ++i)
+            p.ramp_points.push_back({points[i].position, points[i].value,
+                                     static_cast<std::uint8_t>(points[i].interpolation)});
+        for (uint32_t i = 0; i < n_ramps; ++i)
+            p.ramps.push_back({ramps[i].point_offset, static_cast<std::uint16_t>(ramps[i].point_count)});
+        p.result = result;
+        XgenExpressionFloatContext ctx{};
+        ctx.inputs = std::span<const float>(inputs, n_inputs);
+        ctx.u = u;
+        ctx.v = v;
+        ctx.face_seed = face_seed;
+        ctx.t = t;
+        *out = evaluate_xgen_scalar_expression_float(p, ctx);
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+uint32_t ref_runtime_hash_component(float x) { return xgen_runtime_hash_component(x); }
+uint32_t ref_seexpr_hash_finish(uint32_t s) { return xgen_seexpr_hash_finish(s); }
 float ref_random01(uint32_t seed, uint32_t strand, uint32_t dim) { return random01(seed, strand, dim); }
 
 }  // extern "C"

@@ -126,5 +126,109 @@ def main():
     print("wrote", path, os.path.getsize(path), "bytes,", len(fixtures), "arrays")
 
 
+REF_OPS = list(range(27))  # the reference's XgenScalarOp values
+
+
+def random_reference_program(rng, max_ops=30):
+    """A random SSA program over the reference op set (xgen_expression.h:12-40):
+    instructions (op, operand_offset, operand_count, aux, imm), operands, ramp
+    points (pos, value, interp), ramps (offset, count), input count, result."""
+    n_inputs = int(rng.integers(0, 5))
+    ins, ops, pts, ramps = [], [], [], []
+
+    def emit(op, args=(), aux=0, imm=0.0):
+        ins.append((op, len(ops), len(args), aux, np.float32(imm)))
+        ops.extend(args)
+        return len(ins) - 1
+
+    for i in range(n_inputs):
+        emit(1, aux=i)
+    for _ in range(int(rng.integers(1, 4))):
+        emit(0, imm=rng.choice([rng.uniform(-3, 3), rng.uniform(0, 1), 0.0, 0.5, 1.0, 2.0]))
+    for _ in range(int(rng.integers(3, max_ops))):
+        n = len(ins)
+        pick = lambda: int(rng.integers(0, n))
+        op = int(rng.choice([0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23,
+                             24, 25, 26, 4, 6, 22, 25]))
+        arity = {0: 0, 2: 1, 3: 1, 16: 3, 21: 3, 22: 5, 25: 3, 26: 0}.get(op, 2)
+        if op == 0:
+            emit(0, imm=rng.uniform(-2, 2))
+        elif op == 17:
+            emit(op, [pick() for _ in range(int(rng.integers(1, 4)))])
+        elif op == 18:
+            emit(op, [pick() for _ in range(int(rng.integers(0, 4)))], aux=0, imm=float(rng.integers(0, 50)))
+        elif op == 26:
+            k = int(rng.integers(1, 5))
+            pos = np.sort(rng.uniform(-0.2, 1.2, size=k)).astype(np.float32)
+            ramps.append((len(pts), k))
+            for q in range(k):
+                pts.append((pos[q], np.float32(rng.uniform(-0.2, 1.2)), int(rng.integers(0, 4))))
+            emit(op, aux=len(ramps) - 1)
+        else:
+            emit(op, [pick() for _ in range(arity)])
+    return dict(ins=ins, ops=ops, pts=pts, ramps=ramps, n_inputs=n_inputs, result=len(ins) - 1)
+
+
+def expression_fixtures(R, count=400):
+    rng = np.random.default_rng(21)
+    I, O, P, RA = [], [], [], []
+    meta = []  # per program: ins_off, ins_n, ops_off, ops_n, pts_off, pts_n, ramps_off, ramps_n, n_inputs, result
+    inputs, ctx, expected, failed = [], [], [], []
+    for _ in range(count):
+        prog = random_reference_program(rng)
+        ins = np.array(prog["ins"], dtype=[("op", "u1"), ("offset", "<u4"), ("count", "<u2"), ("aux", "<u4"), ("imm", "<f4")])
+        flat = np.zeros(len(ins), dtype=np.dtype([("op", "u1"), ("reserved", "u1"), ("count", "<u2"), ("offset", "<u4"),
+                                                   ("aux", "<u4"), ("imm", "<f4")]))
+        for k in ("op", "count", "offset", "aux", "imm"):
+            flat[k] = ins[k]
+        opsa = np.array(prog["ops"] or [0], dtype=np.uint32)
+        ptsa = np.zeros(max(1, len(prog["pts"])), dtype=np.dtype([("p", "<f4"), ("v", "<f4"), ("i", "<u4")]))
+        for q, pt in enumerate(prog["pts"]):
+            ptsa[q] = pt
+        ra = np.array(prog["ramps"] or [(0, 0)], dtype=np.uint32).reshape(-1, 2)
+        meta.append((sum(len(x) for x in I), len(flat), sum(len(x) for x in O), len(prog["ops"]),
+                     sum(len(x) for x in P), len(prog["pts"]), sum(len(x) for x in RA), len(prog["ramps"]),
+                     prog["n_inputs"], prog["result"]))
+        I.append(flat); O.append(opsa[:len(prog["ops"])]); P.append(ptsa[:len(prog["pts"])]); RA.append(ra[:len(prog["ramps"])])
+        vals = rng.uniform(-2, 2, size=4).astype(np.float32)
+        c = np.array([rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0, 1)], dtype=np.float32)
+        out = C.c_float()
+        rc = R.ref_expr_eval(flat.ctypes.data, len(flat), opsa.ctypes.data, len(prog["ops"]), ptsa.ctypes.data,
+                             len(prog["pts"]), ra.ctypes.data, len(prog["ramps"]), vals.ctypes.data, prog["n_inputs"],
+                             prog["result"], float(c[0]), float(c[1]), float(c[2]), float(c[3]), C.byref(out))
+        inputs.append(vals); ctx.append(c); expected.append(out.value if rc == 0 else np.nan); failed.append(rc)
+    cat = lambda xs, dt: np.concatenate([x for x in xs if len(x)]) if any(len(x) for x in xs) else np.zeros(0, dt)
+    return dict(expr_instructions=cat(I, None), expr_operands=cat(O, None), expr_ramp_points=cat(P, None),
+                expr_ramps=np.concatenate([x for x in RA if len(x)]) if any(len(x) for x in RA) else np.zeros((0, 2), np.uint32),
+                expr_meta=np.array(meta, dtype=np.uint32), expr_inputs=np.array(inputs), expr_context=np.array(ctx),
+                expr_expected=np.array(expected, dtype=np.float32), expr_failed=np.array(failed, dtype=np.uint32))
+
+
+def main_expressions():
+    oracle.build()
+    R = oracle.ref()
+    R.ref_expr_eval.restype = C.c_int
+    R.ref_expr_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                C.c_float, C.c_void_p]
+    R.ref_runtime_hash_component.restype = C.c_uint32
+    R.ref_runtime_hash_component.argtypes = [C.c_float]
+    R.ref_seexpr_hash_finish.restype = C.c_uint32
+    R.ref_seexpr_hash_finish.argtypes = [C.c_uint32]
+    fx = expression_fixtures(R)
+    rng = np.random.default_rng(22)
+    hx = np.concatenate([rng.uniform(-1e6, 1e6, 200), rng.uniform(-1, 1, 200), [0.0, -0.0, 1e-40, -3e-39, 1.0, 0.5]]).astype(np.float32)
+    fx["hash_component_inputs"] = hx
+    fx["hash_component_values"] = np.array([R.ref_runtime_hash_component(float(x)) for x in hx], dtype=np.uint32)
+    st = rng.integers(0, 2**32, size=300, dtype=np.uint64).astype(np.uint32)
+    fx["hash_finish_inputs"] = st
+    fx["hash_finish_values"] = np.array([R.ref_seexpr_hash_finish(int(x)) for x in st], dtype=np.uint32)
+    path = os.path.join(HERE, "reference_expressions.npz")
+    np.savez_compressed(path, **fx)
+    print("wrote", path, os.path.getsize(path), "bytes;", int(fx["expr_failed"].sum()), "of", len(fx["expr_failed"]),
+          "programs raise in the reference")
+
+
 if __name__ == "__main__":
     main()
+    main_expressions()
