@@ -67,15 +67,129 @@ def cpu_baseline(blob, w, target_s, threads):
                       f"(oracle/), {threads} threads"}
 
 
+def bench_surrogate(args):
+    """Learned-surrogate workload: train on 2^20 exact-runtime pairs from
+    seed-disjoint roots (outside the timed region), then time the exact
+    pipeline (exact attributes -> generation) and the surrogate pipeline
+    (fused features + MLP -> sampled exact check with per-tile fallback ->
+    generation) on the same 1,000,000 roots."""
+    import numpy as np
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import expr, surrogate, workloads
+
+    w = workloads.config4()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    blob = nx.build_asset(w.build)
+    asset = nx.DeviceAsset(blob)
+    es, maps = expr.ExprSet(w.info["programs"]), expr.TexelMaps(w.info["maps"])
+    train_p = w.params.copy(seed=w.info["train_seed"], strand_count=1 << 20)
+    t0 = time.perf_counter()
+    feats = surrogate.root_features(asset, train_p, maps)
+    targets = torch.empty((1 << 20, 4), device="cuda")
+    expr.evaluate_root_attributes(asset, train_p, es, maps, targets)
+    net = surrogate.Surrogate(seed=1)
+    net.fit_norm(feats, targets)
+    hist = net.train(feats, targets, epochs=args.train_epochs, batch=4096, lr=3e-3, log_every=256)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"trained {len(hist)} log points, loss {hist[0][1]:.4f} -> {hist[-1][1]:.5f} in {train_s:.1f} s")
+
+    out = nx.PackedOutputs(N, Cv, roots=False, root_uvs=False, point_counts=True)
+    ws = torch.empty(asset.workspace_bytes(w.params), dtype=torch.uint8, device="cuda")
+    attrs = torch.empty((N, 4), dtype=torch.float32, device="cuda")
+    flags = torch.zeros((N + 63) // 64, dtype=torch.int32, device="cuda")
+    stats = torch.zeros(2, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def exact_step():
+        expr.evaluate_root_attributes(asset, w.params, es, maps, attrs, stream=stream)
+        nx.generate_packed(asset, w.params, out, octaves=w.octaves, workspace=ws, stream=stream, attributes=attrs)
+
+    def surrogate_step():
+        surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream)
+        surrogate.verify_attributes(asset, w.params, es, maps, attrs, args.threshold, args.sample_stride, flags,
+                                    stats, stream=stream)
+        nx.generate_packed(asset, w.params, out, octaves=w.octaves, workspace=ws, stream=stream, attributes=attrs)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / args.steps
+
+    def attr_ms(fn_attr):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ms = []
+        for _ in range(5):
+            ev[0].record(stream)
+            fn_attr()
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            ms.append(ev[0].elapsed_time(ev[1]))
+        return sorted(ms)[2]
+
+    exact_ms = timed(exact_step)
+    exact_attrs = attrs.clone()
+    exact_points = out.points.clone()
+    sur_ms = timed(surrogate_step)
+    torch.cuda.synchronize()
+    rel = ((attrs - exact_attrs).abs() / exact_attrs.abs().clamp_min(1e-6)).max(1).values
+    tiles_redone, samples = stats.tolist()
+    ex_attr_ms = attr_ms(lambda: expr.evaluate_root_attributes(asset, w.params, es, maps, attrs, stream=stream))
+    su_attr_ms = attr_ms(lambda: (surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream),
+                                  surrogate.verify_attributes(asset, w.params, es, maps, attrs, args.threshold,
+                                                              args.sample_stride, flags, stats, stream=stream)))
+    mlp_ms = attr_ms(lambda: surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream))
+    counts_exact = bool((out.point_counts == Cv).all())
+    points = N * Cv
+    line = {"metric": METRIC, "value": points / (sur_ms * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sur_ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (surrogate: bf16 mfma)",
+            "data": "synthetic",
+            "config": {"workload": w.name, "strands": N, "cvs_per_strand": Cv, "attribute_programs": 4,
+                       "threshold": args.threshold, "sample_stride": args.sample_stride,
+                       "parallelism": "single-gpu"},
+            "surrogate": {"exact_pipeline_points_per_s": points / (exact_ms * 1e-3),
+                          "exact_pipeline_ms": round(exact_ms, 4),
+                          "speedup_vs_exact": round(exact_ms / sur_ms, 4),
+                          "attributes_ms_exact": round(ex_attr_ms, 4),
+                          "attributes_ms_surrogate_plus_check": round(su_attr_ms, 4),
+                          "mlp_inference_ms": round(mlp_ms, 4),
+                          "tiles_reevaluated": tiles_redone, "tiles": int(flags.numel()),
+                          "samples_checked": samples,
+                          "max_rel_attribute_error": float(rel.max().item()),
+                          "roots_over_threshold": int((rel > args.threshold).sum().item()),
+                          "point_counts_exact": counts_exact,
+                          "train_pairs": 1 << 20, "train_seconds": round(train_s, 2),
+                          "train_loss_first_last": [hist[0][1], hist[-1][1]]},
+            "roofline": {"bound": "mfma", "kernel": "surrogate_kernel (root features + MLP)",
+                         "achieved": round(26368.0 * N / (mlp_ms * 1e-3) / 1e12, 3), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(26368.0 * N / (mlp_ms * 1e-3) / 1e12 / 2500.0, 6),
+                         "traffic": None, "flop_per_root": 26368},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="plane", choices=["plane", "expr"],
+    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
-                         "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs")
+                         "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
+                         "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
+                         "the trained surrogate + sampled exact check against the exact runtime")
+    ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
+    ap.add_argument("--sample-stride", type=int, default=16, help="surrogate: exact check of every n-th root")
+    ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="instead of the benchmark, run a plain device copy of this many MiB "
@@ -91,6 +205,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
         return 0
+    if args.workload == "surrogate" and args.gpus == 1:
+        return bench_surrogate(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(strands are independent: a multi-GPU run shards strand ranges, one process per GPU)", file=sys.stderr)

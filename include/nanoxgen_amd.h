@@ -212,7 +212,8 @@ enum {
     NXG_ERROR_BUFFER_TOO_SMALL = 28,
     NXG_ERROR_INVALID_ASSET = 29,
     NXG_ERROR_INVALID_PROGRAM = 30,
-    NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL = 31
+    NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL = 31,
+    NXG_ERROR_INVALID_MLP_REQUEST = 32
 };
 
 /* device_generation_error_message (device_contract.h:82-116); extension codes
@@ -457,6 +458,86 @@ uint32_t nxg_generate_packed_attributed(const nxg_device_asset *asset,
                                         const nxg_launch_config *config,
                                         void *workspace, uint64_t workspace_bytes,
                                         void *stream);
+
+/* ---- attribute surrogate (extension, no reference counterpart) --------- */
+
+/* A 10 -> 64 -> 64 -> 64 -> 64 -> 4 MLP with SiLU hidden activations, BF16
+ * weights and activations, FP32 accumulation (gfx950 MFMA).  Inputs are the
+ * root features written by nxg_root_features; outputs are the NXG_ATTR_*
+ * attributes.  All state lives in one caller-owned device buffer laid out by
+ * nxg_mlp_get_layout: FP32 master parameters (input-major padded weights
+ * 16x64, 64x64 x3, 64x16, each followed by its bias), Adam moments, the BF16
+ * parameter copy, normalisation (input mean[10], input 1/std[10], output
+ * scale[4], output offset[4]), the last loss and per-block gradient
+ * partials.  The caller uploads parameters and normalisation, then calls
+ * nxg_mlp_sync. */
+#define NXG_MLP_INPUTS 10u
+#define NXG_MLP_HIDDEN 64u
+#define NXG_MLP_HIDDEN_LAYERS 4u
+#define NXG_MLP_OUTPUTS 4u
+#define NXG_MLP_MAX_BATCH 8192u
+
+typedef struct nxg_mlp_layout {
+    uint64_t param_count;
+    uint64_t params_offset;
+    uint64_t adam_m_offset;
+    uint64_t adam_v_offset;
+    uint64_t params_bf16_offset;
+    uint64_t norm_offset;
+    uint64_t loss_offset;
+    uint64_t partials_offset; /* [batch/64][param_count + 1] FP32 */
+    uint64_t max_batch;
+    uint64_t total_bytes;
+} nxg_mlp_layout;
+
+void nxg_mlp_get_layout(nxg_mlp_layout *layout);
+
+/* Refresh the BF16 parameter copy from the FP32 master parameters. */
+uint32_t nxg_mlp_sync(void *state, void *stream);
+
+/* Forward + backward on samples [batch_first, batch_first + batch_size)
+ * (modulo sample_count) of features [n][10] / targets [n][4] (attributes),
+ * mean-squared loss on normalised targets; leaves each 64-sample tile's
+ * gradient (and loss in the last slot) in the partials region. */
+uint32_t nxg_mlp_gradients(void *state, const float *features, const float *targets,
+                           uint64_t sample_count, uint64_t batch_first, uint32_t batch_size,
+                           void *stream);
+
+/* nxg_mlp_gradients, then the partials reduced in tile order and one Adam
+ * step (beta1 0.9, beta2 0.999, eps 1e-8; step counts from 1). */
+uint32_t nxg_mlp_train_step(void *state, const float *features, const float *targets,
+                            uint64_t sample_count, uint64_t batch_first, uint32_t batch_size,
+                            float learning_rate, uint32_t step, void *stream);
+
+/* outputs[n][4] = attributes predicted for features[n][10]. */
+uint32_t nxg_mlp_forward(const void *state, const float *features, float *outputs, uint64_t count,
+                         void *stream);
+
+/* Surrogate features of every strand's root (sample_root, generate.h:101-135):
+ * position xyz, normal xyz, texcoord uv, texel(face, b1, b2) and
+ * texel(face, b2, b1) of `map`; features[n][10]. */
+uint32_t nxg_root_features(const nxg_device_asset *asset, const nxg_generation_params *params,
+                           const nxg_texel_map *map, float *features, uint64_t capacity,
+                           void *stream);
+
+/* Fused root features + MLP: attributes[4*s + k] predicted for strand s. */
+uint32_t nxg_surrogate_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
+                                  const void *state, const nxg_texel_map *map, float *attributes,
+                                  uint64_t attribute_capacity, void *stream);
+
+/* The exact runtime as judge: evaluates the set's programs exactly for every
+ * sample_stride-th root (strands s with s % stride == stride / 2); a 64-root
+ * tile any of whose samples differs from `attributes` by more than
+ * relative_threshold * max(|exact|, 1e-6) in any attribute is re-evaluated
+ * exactly for all its roots.  tile_flags: ceil(N/64) words of device
+ * scratch; stats (device, optional): [0] tiles re-evaluated, [1] samples
+ * checked.  Both are cleared by this call. */
+uint32_t nxg_verify_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
+                               const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count,
+                               float *attributes, uint64_t attribute_capacity,
+                               float relative_threshold, uint32_t sample_stride,
+                               uint32_t *tile_flags, uint64_t tile_flag_capacity, uint32_t *stats,
+                               void *stream);
 
 /* Library build identity: gfx target and compile flags that affect parity. */
 const char *nxg_build_info(void);

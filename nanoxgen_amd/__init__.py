@@ -42,6 +42,15 @@ SYMBOLS = [
     ("nxg_expr_pack", C.c_uint32, [_P, C.c_uint32, _P, C.c_uint64, _P]),
     ("nxg_evaluate_root_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, _P, _P]),
     ("nxg_generate_packed_attributed", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_mlp_get_layout", None, [_P]),
+    ("nxg_mlp_sync", C.c_uint32, [_P, _P]),
+    ("nxg_mlp_gradients", C.c_uint32, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P]),
+    ("nxg_mlp_train_step", C.c_uint32, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, C.c_float, C.c_uint32, _P]),
+    ("nxg_mlp_forward", C.c_uint32, [_P, _P, _P, C.c_uint64, _P]),
+    ("nxg_root_features", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_surrogate_attributes", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_verify_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, C.c_float, C.c_uint32,
+                                           _P, C.c_uint64, _P, _P]),
     ("nxg_build_info", C.c_char_p, []),
 ]
 

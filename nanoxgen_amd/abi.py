@@ -249,7 +249,7 @@ STATUS = [
     "MISSING_MOTION_SAMPLES", "INVALID_MOTION_TIMES", "MOTION_POINT_CAPACITY_TOO_SMALL",
     "OUTPUT_SIZE_OVERFLOW", "HIP", "WORKSPACE_TOO_SMALL", "INVALID_OCTAVES", "NULL_ARGUMENT",
     "STENCIL_CAPACITY_TOO_SMALL", "INVALID_ROOTS", "INVALID_BUILD_INPUT", "BUFFER_TOO_SMALL",
-    "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL",
+    "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL", "INVALID_MLP_REQUEST",
 ]
 globals().update({name: i for i, name in enumerate(STATUS)})
 

@@ -786,7 +786,15 @@ struct AttrArgs {
     uint32_t program_offset[NXG_EXPR_MAX_PROGRAMS];
     float4 *attributes;
     uint32_t *error_word;
+    // verification modes (nxg_verify_attributes)
+    uint32_t mode;          // kAttrAll, kAttrCheck or kAttrFallback
+    uint32_t stride;        // kAttrCheck: roots s with s % stride == stride / 2
+    float threshold;        // kAttrCheck: relative error bound
+    uint32_t *tile_flags;   // one word per 64-root tile
+    uint32_t *stats;        // [0] tiles re-evaluated, [1] samples checked
 };
+
+enum AttrMode : uint32_t { kAttrAll = 0u, kAttrCheck = 1u, kAttrFallback = 2u };
 
 // One lane per root.  The program is the same for every lane, so opcode
 // dispatch is wave-uniform (scalar branches, instructions fetched with scalar
@@ -801,9 +809,13 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
     }
     __syncthreads();
     const uint32_t lane = threadIdx.x, B = blockDim.x;
-    const uint64_t strand64 = uint64_t{blockIdx.x} * B + lane;
+    uint64_t strand64 = uint64_t{blockIdx.x} * B + lane;
+    if (a.mode == kAttrCheck) strand64 = strand64 * a.stride + a.stride / 2u;
     const bool live = strand64 < a.strand_count;
     const uint32_t strand = live ? static_cast<uint32_t>(strand64) : a.strand_count - 1u;
+    // fallback: a wavefront covers exactly one 64-root tile (B is a multiple
+    // of 64), so this exit is uniform per wavefront
+    if (a.mode == kAttrFallback && a.tile_flags[strand / 64u] == 0u) return;
 
     Args ra{};
     ra.positions = a.positions;
@@ -926,7 +938,25 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
         result[0] = result[1] = result[2] = result[3] = nan;
         if (a.error_word) atomicOr(a.error_word, 1u);
     }
+    if (a.mode == kAttrCheck) {
+        const float4 g = a.attributes[strand];
+        const float guess[4] = {g.x, g.y, g.z, g.w};
+        bool fail = bad;
+        for (uint32_t k = 0; k < a.program_count; ++k) {
+            const float e = result[k], d = guess[k] - e;
+            const float mag = (e < 0.0f ? -e : e) > 1.0e-6f ? (e < 0.0f ? -e : e) : 1.0e-6f;
+            if (!((d < 0.0f ? -d : d) <= a.threshold * mag)) fail = true;
+        }
+        if (a.stats) atomicAdd(&a.stats[1], 1u);
+        if (fail && atomicOr(&a.tile_flags[strand / 64u], 1u) == 0u && a.stats) atomicAdd(&a.stats[0], 1u);
+    }
     a.attributes[strand] = make_float4(result[0], result[1], result[2], result[3]);
+}
+
+__global__ void clear_words_kernel(uint32_t *words, uint64_t count, uint32_t *stats) {
+    const uint64_t i = uint64_t{blockIdx.x} * blockDim.x + threadIdx.x;
+    if (i < count) words[i] = 0u;
+    if (i < 2u && stats) stats[i] = 0u;
 }
 
 bool noise_enabled(const nxg_generation_params &p, const nxg_noise_octaves *o) {
@@ -1054,7 +1084,9 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
 uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
                              const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count,
                              float *attributes, uint64_t attribute_capacity, uint32_t *error_word,
-                             void *stream_ptr) {
+                             void *stream_ptr, uint32_t mode = kAttrAll, uint32_t stride = 1u,
+                             float threshold = 0.0f, uint32_t *tile_flags = nullptr,
+                             uint32_t *stats = nullptr) {
     if (uint32_t s = validate_common(asset, nullptr, params, nullptr); s != NXG_OK) return s;
     if (!set || !set->image || reinterpret_cast<uintptr_t>(set->image) % 16u != 0u ||
         set->byte_size < sizeof(PackedSet) || set->program_count == 0u ||
@@ -1089,11 +1121,18 @@ uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation
     for (uint32_t k = 0; k < set->program_count; ++k) a.program_offset[k] = set->program_offset[k];
     a.attributes = reinterpret_cast<float4 *>(attributes);
     a.error_word = error_word;
+    a.mode = mode;
+    a.stride = stride;
+    a.threshold = threshold;
+    a.tile_flags = tile_flags;
+    a.stats = stats;
     // values[max_values][block] floats plus the gradient table in 64 KB of LDS
     uint32_t block = 256u;
     while (block > 64u && uint64_t{set->max_values} * block * 4u + 4096u > 65536u) block -= 64u;
     const uint32_t lds = set->max_values * block * 4u + 4u * 4u * NXG_NOISE_GRADIENT_COUNT;
-    hipLaunchKernelGGL(attribute_kernel, dim3((params->strand_count + block - 1u) / block), dim3(block), lds,
+    const uint64_t lanes = mode == kAttrCheck ? (uint64_t{params->strand_count} + stride - 1u) / stride
+                                              : params->strand_count;
+    hipLaunchKernelGGL(attribute_kernel, dim3(static_cast<uint32_t>((lanes + block - 1u) / block)), dim3(block), lds,
                        static_cast<hipStream_t>(stream_ptr), a);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
@@ -1109,6 +1148,27 @@ uint32_t nxg_evaluate_root_attributes(const nxg_device_asset *asset, const nxg_g
                                       void *stream) {
     return nxg::evaluate_attributes(asset, params, set, maps, map_count, attributes, attribute_capacity,
                                     error_word, stream);
+}
+
+uint32_t nxg_verify_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
+                               const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count,
+                               float *attributes, uint64_t attribute_capacity, float relative_threshold,
+                               uint32_t sample_stride, uint32_t *tile_flags, uint64_t tile_flag_capacity,
+                               uint32_t *stats, void *stream) {
+    if (!params || !tile_flags) return NXG_ERROR_NULL_ARGUMENT;
+    const uint64_t tiles = (uint64_t{params->strand_count} + 63u) / 64u;
+    if (tile_flag_capacity < tiles) return NXG_ERROR_BUFFER_TOO_SMALL;
+    if (sample_stride == 0u || !(relative_threshold >= 0.0f)) return NXG_ERROR_INVALID_MLP_REQUEST;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(nxg::clear_words_kernel, dim3(static_cast<uint32_t>((tiles + 255u) / 256u)), dim3(256), 0, s,
+                       tile_flags, tiles, stats);
+    if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
+    uint32_t st = nxg::evaluate_attributes(asset, params, set, maps, map_count, attributes, attribute_capacity,
+                                           nullptr, stream, nxg::kAttrCheck, sample_stride, relative_threshold,
+                                           tile_flags, stats);
+    if (st != NXG_OK) return st;
+    return nxg::evaluate_attributes(asset, params, set, maps, map_count, attributes, attribute_capacity, nullptr,
+                                    stream, nxg::kAttrFallback, 1u, 0.0f, tile_flags, stats);
 }
 
 uint32_t nxg_generate_packed_attributed(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,

@@ -37,6 +37,7 @@ __host__ __device__ constexpr int w_off(int l) {
 __host__ __device__ constexpr int b_off(int l) { return w_off(l) + rows_of(l) * cols_of(l); }
 constexpr int kParams = w_off(kLayers);
 static_assert(kParams == 14608, "parameter count");
+static_assert(NXG_MLP_MAX_BATCH % 64u == 0u, "batch tiles");
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -46,7 +47,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     u += 0x7fffu + ((u >> 16u) & 1u);
     return static_cast<uint16_t>(u >> 16u);
 }
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t{b} << 16u); }
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 // One 16 x 16 output tile of C = A(64 x K) * B(K x N): a(m, k) and b(k, n)
@@ -73,26 +73,34 @@ struct MlpState {
     uint16_t *params_bf16;  // BF16 copy used by the kernels
     float *norm;            // in_mean[10], in_inv_std[10], out_scale[4], out_offset[4]
     float *partials;        // [max_blocks][kParams + 1] per-block gradients and loss
-    float *loss;            // mean loss of the last step
+    float *loss;            // mean squared error of the last step (normalised targets)
 };
 
-// Forward pass of one 64-sample tile held in LDS.  h0: [64][kInPad] BF16
-// (normalised inputs).  Keeps z (FP32) and h (BF16) of every hidden layer
-// when `keep` is set; returns normalised outputs in yout[64][4].
+// Forward pass of one 64-sample tile held in LDS; h0: [64][kInPad] BF16
+// normalised inputs, y: [64][kOutPad] normalised outputs.  The training form
+// keeps every hidden layer's z (FP32, for SiLU') and h (BF16); the
+// inference form ping-pongs two activation buffers.
 struct TileBuffers {
     uint16_t h0[kTile * kInPad];
     uint16_t h[NXG_MLP_HIDDEN_LAYERS][kTile * kHid];
     float z[NXG_MLP_HIDDEN_LAYERS][kTile * kHid];
     float y[kTile * kOutPad];
 };
+struct InferBuffers {
+    uint16_t h0[kTile * kInPad];
+    uint16_t h[2][kTile * kHid];
+    float y[kTile * kOutPad];
+};
 
-__device__ void tile_forward(const uint16_t *W, const float *P, TileBuffers &t) {
+template <bool Train, class Buf>
+__device__ void tile_forward(const uint16_t *W, const float *P, Buf &t) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int l = 0; l < kLayers; ++l) {
         const int K = rows_of(l), N = cols_of(l);
         const uint16_t *Wl = W + w_off(l);
         const float *bl = P + b_off(l);
-        const uint16_t *hin = l == 0 ? t.h0 : t.h[l - 1];
+        const uint16_t *hin = l == 0 ? t.h0 : t.h[Train ? l - 1 : (l - 1) & 1];
+        uint16_t *hout = l < kLayers - 1 ? t.h[Train ? l : l & 1] : nullptr;
         const int ldin = l == 0 ? kInPad : kHid;
         const int tiles = (kTile / 16) * (N / 16);
         for (int tt = wave; tt < tiles; tt += 4) {
@@ -105,8 +113,8 @@ __device__ void tile_forward(const uint16_t *W, const float *P, TileBuffers &t) 
                 const int m = tm * 16 + 4 * (lane >> 4) + j, n = tn * 16 + (lane & 15);
                 const float z = acc[j] + bl[n];
                 if (l < kLayers - 1) {
-                    t.z[l][m * kHid + n] = z;
-                    t.h[l][m * kHid + n] = f2bf(z * sigmoid(z));
+                    if constexpr (Train) t.z[l][m * kHid + n] = z;
+                    hout[m * kHid + n] = f2bf(z * sigmoid(z));
                 } else {
                     t.y[m * kOutPad + n] = z;
                 }
@@ -154,7 +162,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
         t.f.h0[i] = f2bf(v);
     }
     __syncthreads();
-    tile_forward(s.params_bf16, s.params, t.f);
+    tile_forward<true>(s.params_bf16, s.params, t.f);
     float *part = s.partials + uint64_t{blockIdx.x} * (kParams + 1);
     // output gradient dy = 2 (y - t) / (batch * 4) on valid rows; loss partial
     float loss = 0.0f;
@@ -236,13 +244,13 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
 
 // Reduce the per-block partials in block order, then Adam on the FP32
 // master parameters and refresh of the BF16 copy.
-__global__ void __launch_bounds__(256) mlp_adam_kernel(MlpState s, uint32_t blocks, uint32_t step, float lr,
-                                                       float beta1, float beta2, float eps) {
+__global__ void __launch_bounds__(256) mlp_adam_kernel(MlpState s, uint32_t blocks, uint32_t batch, uint32_t step,
+                                                       float lr, float beta1, float beta2, float eps) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         float loss = 0.0f;
         for (uint32_t b = 0; b < blocks; ++b) loss += s.partials[uint64_t{b} * (kParams + 1) + kParams];
-        s.loss[0] = loss;
+        s.loss[0] = loss / (static_cast<float>(batch) * kOut);
     }
     if (i >= static_cast<uint32_t>(kParams)) return;
     float g = 0.0f;
@@ -267,10 +275,10 @@ __global__ void mlp_sync_kernel(MlpState s) {
 __global__ void __launch_bounds__(256) mlp_forward_kernel(MlpState s, const float *features, float *outputs,
                                                           uint64_t count) {
     extern __shared__ uint8_t smem[];
-    TileBuffers &t = *reinterpret_cast<TileBuffers *>(smem);
+    InferBuffers &t = *reinterpret_cast<InferBuffers *>(smem);
     const uint64_t first = uint64_t{blockIdx.x} * kTile;
     load_inputs(features, first, count, s.norm, t.h0);
-    tile_forward(s.params_bf16, s.params, t);
+    tile_forward<false>(s.params_bf16, s.params, t);
     for (int i = threadIdx.x; i < kTile * kOut; i += blockDim.x) {
         const int m = i / kOut, n = i % kOut;
         if (first + m < count)
@@ -385,7 +393,7 @@ __global__ void __launch_bounds__(256) features_kernel(FeatureArgs a, float *out
 // Fused: features of 64 roots, normalisation, MLP, attributes.
 __global__ void __launch_bounds__(256) surrogate_kernel(FeatureArgs a, MlpState s, float4 *attributes) {
     extern __shared__ uint8_t smem[];
-    TileBuffers &t = *reinterpret_cast<TileBuffers *>(smem);
+    InferBuffers &t = *reinterpret_cast<InferBuffers *>(smem);
     const uint64_t first = uint64_t{blockIdx.x} * kTile;
     if (threadIdx.x < kTile) {
         const uint64_t sidx = first + threadIdx.x;
@@ -395,7 +403,7 @@ __global__ void __launch_bounds__(256) surrogate_kernel(FeatureArgs a, MlpState 
             t.h0[threadIdx.x * kInPad + k] = k < kIn ? f2bf((f[k] - s.norm[k]) * s.norm[kIn + k]) : 0;
     }
     __syncthreads();
-    tile_forward(s.params_bf16, s.params, t);
+    tile_forward<false>(s.params_bf16, s.params, t);
     if (threadIdx.x < kTile && first + threadIdx.x < a.strand_count) {
         const int m = threadIdx.x;
         float o[kOut];
@@ -405,6 +413,15 @@ __global__ void __launch_bounds__(256) surrogate_kernel(FeatureArgs a, MlpState 
 }
 
 }  // namespace
+
+// The training tile needs more than the default 64 KB of dynamic LDS
+// (gfx950 has 160 KB per workgroup).
+bool allow_train_lds() {
+    static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&mlp_train_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(sizeof(TrainBuffers))) == hipSuccess;
+    return ok;
+}
 
 MlpState mlp_state_of(void *base) {
     nxg_mlp_layout L;
@@ -477,11 +494,12 @@ uint32_t nxg_mlp_train_step(void *state, const float *features, const float *tar
     hipStream_t s = static_cast<hipStream_t>(stream);
     const nxg::MlpState st = nxg::mlp_state_of(state);
     const uint32_t blocks = (batch_size + 63u) / 64u;
+    if (!nxg::allow_train_lds()) return NXG_ERROR_HIP;
     hipLaunchKernelGGL(nxg::mlp_train_kernel, dim3(blocks), dim3(256), sizeof(nxg::TrainBuffers), s, st, features,
                        targets, sample_count, batch_first % sample_count, batch_size);
     if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
-    hipLaunchKernelGGL(nxg::mlp_adam_kernel, dim3((nxg::kParams + 255) / 256), dim3(256), 0, s, st, blocks, step,
-                       learning_rate, 0.9f, 0.999f, 1.0e-8f);
+    hipLaunchKernelGGL(nxg::mlp_adam_kernel, dim3((nxg::kParams + 255) / 256), dim3(256), 0, s, st, blocks,
+                       batch_size, step, learning_rate, 0.9f, 0.999f, 1.0e-8f);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
@@ -490,6 +508,7 @@ uint32_t nxg_mlp_gradients(void *state, const float *features, const float *targ
     if (!state || !features || !targets) return NXG_ERROR_NULL_ARGUMENT;
     if (sample_count == 0u || batch_size == 0u || batch_size > NXG_MLP_MAX_BATCH) return NXG_ERROR_INVALID_MLP_REQUEST;
     const uint32_t blocks = (batch_size + 63u) / 64u;
+    if (!nxg::allow_train_lds()) return NXG_ERROR_HIP;
     hipLaunchKernelGGL(nxg::mlp_train_kernel, dim3(blocks), dim3(256), sizeof(nxg::TrainBuffers),
                        static_cast<hipStream_t>(stream), nxg::mlp_state_of(state), features, targets, sample_count,
                        batch_first % sample_count, batch_size);
@@ -500,7 +519,7 @@ uint32_t nxg_mlp_forward(const void *state, const float *features, float *output
     if (!state || !features || !outputs) return NXG_ERROR_NULL_ARGUMENT;
     if (count == 0u) return NXG_OK;
     hipLaunchKernelGGL(nxg::mlp_forward_kernel, dim3(static_cast<uint32_t>((count + 63u) / 64u)), dim3(256),
-                       sizeof(nxg::TileBuffers), static_cast<hipStream_t>(stream),
+                       sizeof(nxg::InferBuffers), static_cast<hipStream_t>(stream),
                        nxg::mlp_state_of(const_cast<void *>(state)), features, outputs, count);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
@@ -525,7 +544,7 @@ uint32_t nxg_surrogate_attributes(const nxg_device_asset *asset, const nxg_gener
     if (!attributes || attribute_capacity < uint64_t{params->strand_count} * NXG_ATTR_COUNT)
         return NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL;
     hipLaunchKernelGGL(nxg::surrogate_kernel, dim3((params->strand_count + 63u) / 64u), dim3(256),
-                       sizeof(nxg::TileBuffers), static_cast<hipStream_t>(stream),
+                       sizeof(nxg::InferBuffers), static_cast<hipStream_t>(stream),
                        nxg::feature_args(*asset, *params, *map), nxg::mlp_state_of(const_cast<void *>(state)),
                        reinterpret_cast<float4 *>(attributes));
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;

@@ -67,6 +67,8 @@ def lib():
         L.orc_expr_eval.restype = C.c_int
         L.orc_expr_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float,
                                     C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+        L.orc_sample_root_f32.restype = None
+        L.orc_sample_root_f32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.orc_root_attributes.restype = C.c_uint32
         L.orc_root_attributes.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                           C.c_void_p, C.c_uint32]

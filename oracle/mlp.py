@@ -105,11 +105,14 @@ def loss_and_grads(flat, x, targets_norm, norm, bf16_round=False):
     dz = dy
     for i in range(LAYERS, -1, -1):
         h, Wr, z = cache[i]
-        dW = h.T @ dz
+        # the device feeds BF16 copies of dz to both products and sums the
+        # FP32 dz for the bias
+        dzr = bf16(dz) if bf16_round else dz
+        dW = h.T @ dzr
         db = dz.sum(0)
         grads.append((dW, db))
         if i > 0:
-            dh = dz @ Wr.T
+            dh = dzr @ Wr.T
             dz = dh * dsilu(cache[i - 1][2])
     grads.reverse()
     g = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in grads])
