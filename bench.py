@@ -176,17 +176,125 @@ def bench_surrogate(args):
     return 0
 
 
+def bench_batch(args):
+    """Batched uneven form: nine descriptions (2,456,139 strands, 4..64 CVs,
+    meshes of 2,048..199,712 triangles, per-description expression programs)
+    in one captured graph: attribute programs, one guide preparation over
+    all descriptions and the persistent strand kernel.  The tuning comes
+    from the library's autotuner, run before the timed region."""
+    import numpy as np
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import batch, expr, workloads
+
+    t = time.perf_counter()
+    ws = workloads.config5()
+    blobs = [nx.build_asset(w.build) for w in ws]
+    log(f"built {len(ws)} descriptions ({sum(b.nbytes for b in blobs)} B of assets) in "
+        f"{time.perf_counter() - t:.1f} s")
+    assets = [nx.DeviceAsset(b) for b in blobs]
+    stream = torch.cuda.current_stream()
+    full = batch.Batch()
+    kern = batch.Batch()  # the same generation from the evaluated attributes (no programs)
+    outs = []
+    for w, a in zip(ws, assets):
+        out = nx.PackedOutputs(w.params.strand_count, w.params.cvs_per_strand, roots=False, root_uvs=False)
+        outs.append(out)
+        k = full.add(a, w.params, out, octaves=w.octaves, exprs=expr.ExprSet(w.info["programs"]),
+                     maps=expr.TexelMaps(w.info["maps"]))
+        kern.add(a, w.params, out, octaves=w.octaves, attributes=full.entries[k]["attributes"],
+                 workspace=full.entries[k]["workspace"])
+    best, table = full.autotune(repeats=3)
+    torch.cuda.synchronize()
+    for tun, ms in sorted(table, key=lambda x: (x[1] is None, x[1] or 0.0)):
+        log(f"  tuning {tun.as_tuple()}: {ms if ms is None else round(ms, 3)} ms")
+    log(f"autotuner picked {best}")
+    graph = full.graph(best)
+    for _ in range(args.warmup):
+        graph.launch(stream)
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        events[i][0].record(stream)
+        graph.launch(stream)
+        events[i][1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    step_ms = sorted(e[0].elapsed_time(e[1]) for e in events)
+    # strand kernel (+ the guide preparation launch before it) from the
+    # evaluated attributes, timed on the launch stream
+    kev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(max(5, args.steps // 2))]
+    for e in kev:
+        e[0].record(stream)
+        kern.generate(best, stream)
+        e[1].record(stream)
+    torch.cuda.synchronize()
+    kern_ms = sorted(e[0].elapsed_time(e[1]) for e in kev)
+    kernel_ms = sum(kern_ms) / len(kern_ms)
+    points = full.points
+    strands = full.strands
+    alg_bytes = sum(w.params.strand_count * (w.params.cvs_per_strand * 16 + 4 + 16) for w in ws) + \
+        sum(b.nbytes for b in blobs)
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "batch_kernel<%d, %d> (+ batch_prep_kernel)" % (best.waves_per_block, best.chunk_cvs),
+                "kernel_ms_mean": round(kernel_ms, 4), "kernel_ms_median": round(kern_ms[len(kern_ms) // 2], 4),
+                "step_ms_median": round(step_ms[len(step_ms) // 2], 4),
+                "algorithmic_bytes_per_launch": alg_bytes}
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        oracle.lib()
+        threads = min(16, os.cpu_count() or 1)
+        frac = 0.002
+        cpu_pts, cpu_s, tries = 0, 0.0, 0
+        while tries < 4:
+            cpu_pts, cpu_s = 0, 0.0
+            for w, b in zip(ws, blobs):
+                s = np.arange(max(64, int(w.params.strand_count * frac)), dtype=np.uint32)
+                t = time.perf_counter()
+                attrs, _ = oracle.root_attributes(b, w.params, w.info["programs"], w.info["maps"], strands=s,
+                                                  workers=threads)
+                oracle.generate(b, w.params, octaves=w.octaves, strands=s, workers=threads, attributes=attrs)
+                cpu_s += time.perf_counter() - t
+                cpu_pts += len(s) * w.params.cvs_per_strand
+            if cpu_s >= 0.5 * args.cpu_seconds:
+                break
+            frac = min(1.0, frac * args.cpu_seconds / max(cpu_s, 1e-3))
+            tries += 1
+        cpu = {"value": cpu_pts / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
+               "sample": f"the first {frac:.4f} of every description's strands ({cpu_pts} points), attribute "
+                         f"programs + generation, {cpu_s:.2f} s; float restatement of the reference CPU path "
+                         f"(oracle/), {threads} threads"}
+    line = {"metric": METRIC, "value": points / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"batch-{len(ws)}-descriptions-{strands}-strands-{points}-points",
+                       "descriptions": [w.name for w in ws], "strands": strands, "points": points,
+                       "tuning": dict(zip(["waves_per_block", "chunk_cvs", "tiles_per_claim", "blocks_per_cu",
+                                           "order"], best.as_tuple())),
+                       "launch": "one graph: attribute programs, batched guide preparation, persistent strand kernel",
+                       "parallelism": "single-gpu"},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate"],
+    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
-                         "the trained surrogate + sampled exact check against the exact runtime")
+                         "the trained surrogate + sampled exact check against the exact runtime; "
+                         "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph")
     ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
     ap.add_argument("--sample-stride", type=int, default=16, help="surrogate: exact check of every n-th root")
     ap.add_argument("--train-epochs", type=int, default=40)
@@ -207,6 +315,8 @@ def main():
         return 0
     if args.workload == "surrogate" and args.gpus == 1:
         return bench_surrogate(args)
+    if args.workload == "batch" and args.gpus == 1:
+        return bench_batch(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(strands are independent: a multi-GPU run shards strand ranges, one process per GPU)", file=sys.stderr)

@@ -213,7 +213,8 @@ enum {
     NXG_ERROR_INVALID_ASSET = 29,
     NXG_ERROR_INVALID_PROGRAM = 30,
     NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL = 31,
-    NXG_ERROR_INVALID_MLP_REQUEST = 32
+    NXG_ERROR_INVALID_MLP_REQUEST = 32,
+    NXG_ERROR_INVALID_BATCH = 33
 };
 
 /* device_generation_error_message (device_contract.h:82-116); extension codes
@@ -538,6 +539,78 @@ uint32_t nxg_verify_attributes(const nxg_device_asset *asset, const nxg_generati
                                float relative_threshold, uint32_t sample_stride,
                                uint32_t *tile_flags, uint64_t tile_flag_capacity, uint32_t *stats,
                                void *stream);
+
+/* ---- batched descriptions ------------------------------------------------
+ * Many descriptions generated by one persistent launch.  The reference runs
+ * a collection as one chain of dispatches per description
+ * (src/luisa/xgen_classic_collection.cpp:108-221, ClassicCollectionPipeline::
+ * encode); here every wavefront claims 64-strand tiles of the whole batch
+ * from one device counter, so small and large descriptions share the GPU.
+ * Each item's results are identical to nxg_generate_packed (or
+ * nxg_generate_packed_attributed) called for that item alone. */
+#define NXG_BATCH_MAX_ITEMS 12u
+
+typedef struct nxg_batch_item {
+    const nxg_device_asset *asset;
+    const nxg_deformed_geometry *deformed;  /* optional */
+    const nxg_generation_params *params;
+    const nxg_noise_octaves *octaves;       /* optional */
+    float *attributes;                      /* optional float4 per strand (see exprs) */
+    uint64_t attribute_capacity;            /* floats */
+    const nxg_expr_set *exprs;              /* optional: attributes are first evaluated by these programs */
+    const nxg_texel_map *maps;              /* texel maps of exprs */
+    uint32_t map_count;
+    uint32_t reserved;
+    const nxg_packed_output *output;
+    const nxg_stencil_output *stencil;      /* optional */
+    void *workspace;                        /* nxg_packed_generation_workspace_size bytes */
+    uint64_t workspace_bytes;
+} nxg_batch_item;
+
+#define NXG_BATCH_ORDER_GIVEN 0u     /* descriptions in item order */
+#define NXG_BATCH_ORDER_HEAVY_FIRST 1u /* costliest strands first: CVs x noise passes x octaves */
+
+typedef struct nxg_batch_tuning {
+    uint32_t waves_per_block;  /* 1, 2 or 4 */
+    uint32_t chunk_cvs;        /* CVs staged in LDS per store burst: 4, 8 or 16 (16 needs waves <= 2) */
+    uint32_t tiles_per_claim;  /* 64-strand tiles a wavefront takes per counter claim: 1..16 */
+    uint32_t blocks_per_cu;    /* persistent grid = blocks_per_cu x CUs; 0 = resident maximum */
+    uint32_t order;            /* NXG_BATCH_ORDER_* */
+} nxg_batch_tuning;
+
+/* Defaults used when a NULL tuning is passed. */
+void nxg_batch_default_tuning(nxg_batch_tuning *tuning);
+
+/* Validates every item as nxg_generate_packed would, plus the tuning. */
+uint32_t nxg_batch_validate(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning);
+
+/* On `stream`: the attribute programs of items that carry exprs
+ * (nxg_evaluate_root_attributes, one launch each), one guide preparation
+ * over all items (which also zeroes *counter, a caller-owned device word),
+ * then the persistent strand kernel.  Allocates nothing, never synchronises, records no memset: the
+ * call may be captured into a graph.  Calls sharing one counter must not
+ * overlap. */
+uint32_t nxg_generate_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning,
+                            uint32_t *counter, void *stream);
+
+/* nxg_generate_batch captured once into an instantiated graph (host handle
+ * in *graph); replay with nxg_batch_graph_launch on any stream; the items'
+ * device buffers must outlive the graph. */
+uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning,
+                                uint32_t *counter, void **graph);
+uint32_t nxg_batch_graph_launch(void *graph, void *stream);
+uint32_t nxg_batch_graph_destroy(void *graph);
+
+/* Built-in candidate tunings (returns how many; writes at most capacity). */
+uint32_t nxg_batch_tuning_candidates(nxg_batch_tuning *candidates, uint32_t capacity);
+
+/* Autotuner: runs each candidate `repeats` times after one warm-up on
+ * `stream`, synchronising, and writes the median milliseconds of each
+ * (times_ms[candidate_count]) and the fastest tuning to *best.  Not
+ * capturable; the items' outputs hold the last run's results. */
+uint32_t nxg_batch_autotune(const nxg_batch_item *items, uint32_t count, uint32_t *counter, void *stream,
+                            const nxg_batch_tuning *candidates, uint32_t candidate_count, uint32_t repeats,
+                            nxg_batch_tuning *best, float *times_ms);
 
 /* Library build identity: gfx target and compile flags that affect parity. */
 const char *nxg_build_info(void);

@@ -51,6 +51,14 @@ SYMBOLS = [
     ("nxg_surrogate_attributes", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_verify_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, C.c_float, C.c_uint32,
                                            _P, C.c_uint64, _P, _P]),
+    ("nxg_batch_default_tuning", None, [_P]),
+    ("nxg_batch_validate", C.c_uint32, [_P, C.c_uint32, _P]),
+    ("nxg_generate_batch", C.c_uint32, [_P, C.c_uint32, _P, _P, _P]),
+    ("nxg_batch_graph_create", C.c_uint32, [_P, C.c_uint32, _P, _P, _P]),
+    ("nxg_batch_graph_launch", C.c_uint32, [_P, _P]),
+    ("nxg_batch_graph_destroy", C.c_uint32, [_P]),
+    ("nxg_batch_tuning_candidates", C.c_uint32, [_P, C.c_uint32]),
+    ("nxg_batch_autotune", C.c_uint32, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P]),
     ("nxg_build_info", C.c_char_p, []),
 ]
 

@@ -240,6 +240,29 @@ assert GUIDE_DTYPE.itemsize == 56
 assert C.sizeof(ExprInstruction) == 16
 
 # status codes (device_contract.h:58-80 then extensions)
+BATCH_MAX_ITEMS = 12
+BATCH_ORDER_GIVEN, BATCH_ORDER_HEAVY_FIRST = 0, 1
+
+
+class BatchItem(C.Structure):
+    """nxg_batch_item: one description's generation request, by pointer."""
+    _fields_ = [("asset", C.c_void_p), ("deformed", C.c_void_p), ("params", C.c_void_p), ("octaves", C.c_void_p),
+                ("attributes", C.c_void_p), ("attribute_capacity", C.c_uint64), ("exprs", C.c_void_p),
+                ("maps", C.c_void_p), ("map_count", C.c_uint32), ("reserved", C.c_uint32), ("output", C.c_void_p),
+                ("stencil", C.c_void_p), ("workspace", C.c_void_p), ("workspace_bytes", C.c_uint64)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("waves_per_block", C.c_uint32), ("chunk_cvs", C.c_uint32), ("tiles_per_claim", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("order", C.c_uint32)]
+
+    def as_tuple(self):
+        return tuple(getattr(self, n) for n, _ in self._fields_)
+
+    def __repr__(self):
+        return "BatchTuning(" + ", ".join(f"{n}={getattr(self, n)}" for n, _ in self._fields_) + ")"
+
+
 STATUS = [
     "OK", "NULL_ASSET", "MISALIGNED_ASSET", "INVALID_ASSET_METADATA", "ASSET_CAPACITY_TOO_SMALL",
     "INVALID_GENERATION_PARAMETERS", "INVALID_BLOCK_SIZE", "DEFORMED_POSITION_COUNT_MISMATCH",
@@ -249,7 +272,7 @@ STATUS = [
     "MISSING_MOTION_SAMPLES", "INVALID_MOTION_TIMES", "MOTION_POINT_CAPACITY_TOO_SMALL",
     "OUTPUT_SIZE_OVERFLOW", "HIP", "WORKSPACE_TOO_SMALL", "INVALID_OCTAVES", "NULL_ARGUMENT",
     "STENCIL_CAPACITY_TOO_SMALL", "INVALID_ROOTS", "INVALID_BUILD_INPUT", "BUFFER_TOO_SMALL",
-    "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL", "INVALID_MLP_REQUEST",
+    "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL", "INVALID_MLP_REQUEST", "INVALID_BATCH",
 ]
 globals().update({name: i for i, name in enumerate(STATUS)})
 

@@ -146,6 +146,10 @@ const char *nxg_status_message(uint32_t status) {
         case NXG_ERROR_INVALID_BUILD_INPUT: return "asset build input is invalid";
         case NXG_ERROR_BUFFER_TOO_SMALL: return "output buffer is too small";
         case NXG_ERROR_INVALID_ASSET: return "asset is invalid";
+        case NXG_ERROR_INVALID_PROGRAM: return "expression program or texel map is invalid";
+        case NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL: return "device attribute buffer is too small";
+        case NXG_ERROR_INVALID_MLP_REQUEST: return "surrogate request is invalid";
+        case NXG_ERROR_INVALID_BATCH: return "batch item count or tuning is invalid";
     }
     return "unknown device generation error";
 }

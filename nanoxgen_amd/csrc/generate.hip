@@ -21,6 +21,7 @@
 // library and feed positions only.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 
@@ -227,6 +228,7 @@ struct Args {
     const nxg_root_sample *explicit_roots;
     const float4 *attributes;  // optional per-root multipliers (NXG_ATTR_*)
     uint32_t strand_count, cvs, seed, triangle_count, guide_count, has_normals;
+    uint32_t noise, octave_count;  // runtime form of the template switches (batched path)
     float support_scale, weight_power, rejection_cos, length_scale;
     float root_width, tip_width, radius_scale;
     float noise_mask, decorrelation_scale, preserve, inv_span;
@@ -392,7 +394,7 @@ struct NoiseState {
 
 // evaluate_and_advance_noise_cursor (generate.h:459-515); OCT octaves share
 // the transported frame and the arc length and their displacements add.
-template <int OCT>
+template <int OCT, bool RT = false>
 __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend &b,
                                       const NoiseState<OCT> &ns, Cursor &c, const float4 *grad) {
     if (c.cv > 0u) c.arc += fast_sqrt(fma3(c.cur - c.prev, c.cur - c.prev));
@@ -415,6 +417,7 @@ __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend 
         float lx = 0.0f, ly = 0.0f, lz = 0.0f;
 #pragma unroll
         for (int k = 0; k < OCT; ++k) {
+            if (RT && k >= static_cast<int>(a.octave_count)) break;
             const float distance = c.arc * ns.freq[k];
             const float m = ns.mag[k] * half_t;  // noise_mask * amplitude * t / 2
             lx = fmaf(gradient_noise_axis<0>(ns.dom, distance, grad), m, lx);
@@ -446,14 +449,23 @@ __device__ __forceinline__ Cursor make_cursor(const Args &a, const Root &r, cons
     return c;
 }
 
-// guide state and offset table: one lane per (guide, cv)
-__global__ void __launch_bounds__(256) guide_prep_kernel(
-    const nxg_guide_record *guides, const nxg_vec3 *guide_cvs, const nxg_uint3 *triangles,
-    const nxg_vec3 *d_positions, const nxg_vec3 *d_normals, const nxg_vec3 *d_guide_cvs,
-    const nxg_vec3 *frame_positions, uint32_t guide_count, uint32_t triangle_count,
-    uint32_t cvs, float4 *state, float4 *offsets) {
-    const uint64_t i = uint64_t{blockIdx.x} * blockDim.x + threadIdx.x;
-    if (i >= uint64_t{guide_count} * cvs) return;
+struct Prep {
+    const nxg_guide_record *guides;
+    const nxg_vec3 *guide_cvs;
+    const nxg_uint3 *triangles;
+    const nxg_vec3 *d_positions, *d_normals, *d_guide_cvs, *frame_positions;
+    uint32_t guide_count, triangle_count, cvs;
+    float4 *state, *offsets;
+};
+
+// guide state and offset table: one lane per (guide, cv), i < guide_count * cvs
+__device__ __forceinline__ void guide_prep_lane(const Prep &p, uint64_t i) {
+    const nxg_guide_record *guides = p.guides;
+    const nxg_vec3 *guide_cvs = p.guide_cvs, *d_positions = p.d_positions, *d_normals = p.d_normals;
+    const nxg_vec3 *d_guide_cvs = p.d_guide_cvs, *frame_positions = p.frame_positions;
+    const nxg_uint3 *triangles = p.triangles;
+    const uint32_t triangle_count = p.triangle_count, cvs = p.cvs;
+    float4 *state = p.state, *offsets = p.offsets;
     const uint32_t gi = static_cast<uint32_t>(i / cvs);
     const uint32_t cv = static_cast<uint32_t>(i % cvs);
     const nxg_guide_record g = guides[gi];
@@ -503,6 +515,11 @@ __global__ void __launch_bounds__(256) guide_prep_kernel(
     }
 }
 
+__global__ void __launch_bounds__(256) guide_prep_kernel(Prep p) {
+    const uint64_t i = uint64_t{blockIdx.x} * blockDim.x + threadIdx.x;
+    if (i < uint64_t{p.guide_count} * p.cvs) guide_prep_lane(p, i);
+}
+
 // Writes the wave's staged chunk [cv0, cv0 + n) for its 64 strands as
 // contiguous runs of n float4 per strand.
 __device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, uint64_t wave_first,
@@ -522,23 +539,26 @@ __device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, u
     }
 }
 
-template <bool NOISE, int OCT>
-__global__ void __launch_bounds__(256) strand_kernel(Args a) {
-    __shared__ float4 grad[NXG_NOISE_GRADIENT_COUNT];
-    __shared__ float stage_all[256 / kWave][kChunk * 4 * kStageStride];
-    for (uint32_t i = threadIdx.x; i < NXG_NOISE_GRADIENT_COUNT; i += blockDim.x) {
-        const nxg_vec3 g = a.gradients[i];
-        grad[i] = make_float4(g.x, g.y, g.z, 0.0f);
-    }
-    __syncthreads();
-    const uint32_t lane = threadIdx.x % kWave;
-    const uint32_t wave = threadIdx.x / kWave;
-    float *stage = stage_all[wave];
-    const uint64_t wave_first = uint64_t{blockIdx.x} * blockDim.x + wave * kWave;
+// Orders this wavefront's LDS traffic: every lane's earlier LDS writes are
+// visible to every lane's later reads (the staging area is per wavefront,
+// so no workgroup barrier is needed and wavefronts may work on different
+// descriptions).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront generates strands [wave_first, wave_first + 64) of the
+// description `a`: stage is this wavefront's CHUNK-CV staging area, grad the
+// description's gradient table in LDS.  NOISE/OCT fix the noise path at
+// compile time; with RT the octave count comes from a.octave_count (<= OCT).
+template <bool NOISE, int OCT, bool RT, int CHUNK>
+__device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, uint32_t lane, float *stage, const float4 *grad) {
     const uint64_t strand64 = wave_first + lane;
     const bool live = strand64 < a.strand_count;
     // lanes past the end compute the last strand and store nothing, so every
-    // lane runs the same CV loop and reaches every barrier
+    // lane runs the same CV loop
     const uint32_t strand = live ? static_cast<uint32_t>(strand64) : a.strand_count - 1u;
 
     Root r;
@@ -620,14 +640,14 @@ __global__ void __launch_bounds__(256) strand_kernel(Args a) {
         if constexpr (NOISE) cur = make_cursor<OCT>(a, r, b, surface_u);
         V3 last = v3(0.0f, 0.0f, 0.0f);
         float noisy = 0.0f;
-        for (uint32_t cv0 = 0; cv0 < a.cvs; cv0 += kChunk) {
-            const uint32_t n = a.cvs - cv0 < kChunk ? a.cvs - cv0 : kChunk;
+        for (uint32_t cv0 = 0; cv0 < a.cvs; cv0 += CHUNK) {
+            const uint32_t n = a.cvs - cv0 < CHUNK ? a.cvs - cv0 : CHUNK;
 #pragma unroll 1
             for (uint32_t k = 0; k < n; ++k) {
                 const uint32_t cv = cv0 + k;
                 V3 p;
                 if constexpr (NOISE) {
-                    p = advance<OCT>(a, r, b, ns, cur, grad);
+                    p = advance<OCT, RT>(a, r, b, ns, cur, grad);
                 } else {
                     p = base_cv(a, r, b, cv);
                 }
@@ -646,9 +666,9 @@ __global__ void __launch_bounds__(256) strand_kernel(Args a) {
                 stage[(k * 4u + 3u) * kStageStride + lane] = 0.5f * width * a.radius_scale;
             }
             if (emit) {
-                __syncthreads();
+                wave_sync();
                 flush_chunk(a, stage, wave_first, cv0, n, lane);
-                __syncthreads();
+                wave_sync();
             }
         }
         if (!emit && noisy > 0.0f) {  // generate.h:553-563
@@ -658,6 +678,101 @@ __global__ void __launch_bounds__(256) strand_kernel(Args a) {
         }
     }
 }
+
+
+template <bool NOISE, int OCT>
+__global__ void __launch_bounds__(256) strand_kernel(Args a) {
+    __shared__ float4 grad[NXG_NOISE_GRADIENT_COUNT];
+    __shared__ float stage_all[256 / kWave][kChunk * 4 * kStageStride];
+    for (uint32_t i = threadIdx.x; i < NXG_NOISE_GRADIENT_COUNT; i += blockDim.x) {
+        const nxg_vec3 g = a.gradients[i];
+        grad[i] = make_float4(g.x, g.y, g.z, 0.0f);
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint64_t wave_first = uint64_t{blockIdx.x} * blockDim.x + wave * kWave;
+    strand_tile<NOISE, OCT, false, kChunk>(a, wave_first, lane, stage_all[wave], grad);
+}
+
+// ---- batched descriptions ------------------------------------------------
+
+// Guide preparation of every item in one launch: item k owns the 256-lane
+// blocks [block_end[k-1], block_end[k]).  Block 0 also zeroes the tile
+// counter of the strand launch that follows on the same stream.
+struct BatchPrepArgs {
+    Prep items[NXG_BATCH_MAX_ITEMS];
+    uint32_t block_end[NXG_BATCH_MAX_ITEMS];
+    uint32_t count;
+    uint32_t *counter;
+};
+
+__global__ void __launch_bounds__(256) batch_prep_kernel(BatchPrepArgs b) {
+    if (blockIdx.x == 0u && threadIdx.x == 0u) *b.counter = 0u;
+    uint32_t item = 0u;
+    while (item + 1u < b.count && blockIdx.x >= b.block_end[item]) ++item;
+    const uint32_t first = item ? b.block_end[item - 1u] : 0u;
+    const Prep &p = b.items[item];
+    const uint64_t i = uint64_t{blockIdx.x - first} * 256u + threadIdx.x;
+    if (i < uint64_t{p.guide_count} * p.cvs) guide_prep_lane(p, i);
+}
+
+struct BatchArgs {
+    Args items[NXG_BATCH_MAX_ITEMS];
+    uint32_t tile_end[NXG_BATCH_MAX_ITEMS];  // inclusive scan of 64-strand tiles, processing order
+    uint32_t order[NXG_BATCH_MAX_ITEMS];     // item processed at each position
+    uint32_t count;
+    uint32_t total_tiles;
+    uint32_t tiles_per_claim;
+    uint32_t *counter;                       // zeroed by batch_prep_kernel
+};
+
+// Persistent: every wavefront claims runs of tiles_per_claim 64-strand
+// tiles from one atomic counter until the work list is exhausted (every
+// wavefront leaves once its claim starts past the last tile); each tile's
+// description comes from a binary search of the tile scan held in LDS.
+template <int WAVES, int CHUNK>
+__global__ void __launch_bounds__(WAVES * 64) batch_kernel(BatchArgs b) {
+    __shared__ float4 grad_all[WAVES][NXG_NOISE_GRADIENT_COUNT];
+    __shared__ float stage_all[WAVES][CHUNK * 4 * kStageStride];
+    __shared__ uint32_t scan[NXG_BATCH_MAX_ITEMS];
+    if (threadIdx.x < b.count) scan[threadIdx.x] = b.tile_end[threadIdx.x];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    float4 *grad = grad_all[wave];
+    uint32_t loaded = 0xffffffffu;
+    for (;;) {
+        uint32_t claim = 0u;
+        if (lane == 0u) claim = atomicAdd(b.counter, 1u);
+        const uint32_t first_tile = __builtin_amdgcn_readfirstlane(__shfl(claim, 0, kWave)) * b.tiles_per_claim;
+        if (first_tile >= b.total_tiles) break;
+        const uint32_t last_tile = min(first_tile + b.tiles_per_claim, b.total_tiles);
+        for (uint32_t tile = first_tile; tile < last_tile; ++tile) {
+            uint32_t lo = 0u, hi = b.count - 1u;  // first position whose scan exceeds tile
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2u;
+                if (scan[mid] > tile) hi = mid; else lo = mid + 1u;
+            }
+            lo = __builtin_amdgcn_readfirstlane(lo);
+            const uint32_t item = b.order[lo];
+            const uint32_t item_first = lo ? scan[lo - 1u] : 0u;
+            const Args &a = b.items[item];
+            if (item != loaded) {
+                for (uint32_t i = lane; i < NXG_NOISE_GRADIENT_COUNT; i += kWave) {
+                    const nxg_vec3 g = a.gradients[i];
+                    grad[i] = make_float4(g.x, g.y, g.z, 0.0f);
+                }
+                wave_sync();
+                loaded = item;
+            }
+            const uint64_t wave_first = uint64_t{tile - item_first} * kWave;
+            if (a.noise) strand_tile<true, NXG_MAX_NOISE_OCTAVES, true, CHUNK>(a, wave_first, lane, stage_all[wave], grad);
+            else strand_tile<false, 1, false, CHUNK>(a, wave_first, lane, stage_all[wave], grad);
+            wave_sync();
+        }
+    }
+}
+
 
 // ---- float expression runtime ------------------------------------------
 
@@ -975,12 +1090,20 @@ const T *section(const nxg_device_asset &asset, uint64_t offset) {
 
 enum Stage : uint32_t { kPrepare = 1u, kStrands = 2u };
 
-uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
-                const nxg_generation_params *params, const nxg_noise_octaves *octaves,
-                const nxg_root_sample *explicit_roots, const float *attributes,
-                uint64_t attribute_capacity, const nxg_packed_output *output,
-                const nxg_stencil_output *stencil, const nxg_launch_config *config, void *workspace,
-                uint64_t workspace_bytes, void *stream_ptr, uint32_t stages) {
+// Everything one description's launch needs, validated.
+struct Plan {
+    Args a;
+    Prep p;
+    uint64_t prep_lanes;
+    bool noise;
+};
+
+uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                   const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                   const nxg_root_sample *explicit_roots, const float *attributes, uint64_t attribute_capacity,
+                   const nxg_packed_output *output, const nxg_stencil_output *stencil,
+                   const nxg_launch_config *config, void *workspace, uint64_t workspace_bytes, uint32_t stages,
+                   Plan &plan) {
     if (uint32_t s = validate_common(asset, deformed, params, config); s != NXG_OK) return s;
     if (stages & kStrands) {
         if (uint32_t s = validate_packed_output(*params, output); s != NXG_OK) return s;
@@ -996,7 +1119,6 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     const nxg_asset_header &h = asset->header;
     const WorkspaceLayout L = workspace_layout(h.guide_count, params->cvs_per_strand);
     if (!workspace || workspace_bytes < L.total) return NXG_ERROR_WORKSPACE_TOO_SMALL;
-    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
 
     const nxg_vec3 *d_pos = deformed ? deformed->positions : nullptr;
     const nxg_vec3 *d_nrm = deformed ? deformed->normals : nullptr;
@@ -1006,19 +1128,14 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     float4 *state = reinterpret_cast<float4 *>(static_cast<uint8_t *>(workspace) + L.guide_state_offset);
     float4 *offsets = reinterpret_cast<float4 *>(static_cast<uint8_t *>(workspace) + L.guide_offsets_offset);
 
-    if (stages & kPrepare) {
-        const uint64_t prep_lanes = uint64_t{h.guide_count} * params->cvs_per_strand;
-        hipLaunchKernelGGL(guide_prep_kernel, dim3(static_cast<uint32_t>((prep_lanes + 255u) / 256u)),
-                           dim3(256), 0, stream, section<nxg_guide_record>(*asset, h.guides_offset),
-                           section<nxg_vec3>(*asset, h.guide_cvs_offset),
-                           section<nxg_uint3>(*asset, h.triangles_offset), d_pos, d_nrm, d_gcv,
-                           frame_pos, h.guide_count, h.triangle_count, params->cvs_per_strand, state,
-                           offsets);
-        if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
-    }
-    if (!(stages & kStrands)) return NXG_OK;
+    plan.prep_lanes = uint64_t{h.guide_count} * params->cvs_per_strand;
+    plan.p = Prep{section<nxg_guide_record>(*asset, h.guides_offset), section<nxg_vec3>(*asset, h.guide_cvs_offset),
+                  section<nxg_uint3>(*asset, h.triangles_offset), d_pos, d_nrm, d_gcv, frame_pos,
+                  h.guide_count, h.triangle_count, params->cvs_per_strand, state, offsets};
+    plan.noise = noise_enabled(*params, octaves);
 
-    Args a{};
+    Args &a = plan.a;
+    a = Args{};
     a.positions = frame_pos;
     a.normals = d_nrm ? d_nrm : section<nxg_vec3>(*asset, h.normals_offset);
     a.rest_positions = rest_pos;
@@ -1043,16 +1160,16 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     a.length_scale = params->length_scale;
     a.root_width = params->root_width;
     a.tip_width = params->tip_width;
-    a.radius_scale = output->radius_scale;
+    a.radius_scale = output ? output->radius_scale : 1.0f;
     a.noise_mask = params->noise_mask;
     const float decorrelation = 1.0f - params->noise_correlation;
     a.decorrelation_scale = 100.0f * decorrelation * decorrelation;
     a.preserve = params->noise_preserve_length;
     a.inv_span = 1.0f / static_cast<float>(params->cvs_per_strand - 1u);
-    uint32_t oct = 1u;
+    a.octave_count = 1u;
     if (octaves && octaves->count > 1u) {
-        oct = octaves->count;
-        for (uint32_t k = 0; k < oct; ++k) {
+        a.octave_count = octaves->count;
+        for (uint32_t k = 0; k < a.octave_count; ++k) {
             a.amp[k] = octaves->amplitude[k];
             a.freq[k] = octaves->frequency[k];
         }
@@ -1060,18 +1177,42 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
         a.amp[0] = params->noise_amplitude;
         a.freq[0] = params->noise_frequency;
     }
-    a.points = output->points;
-    a.roots = output->roots;
-    a.root_uvs = output->root_uvs;
-    a.point_counts = output->point_counts;
+    a.noise = plan.noise ? 1u : 0u;
+    if (output) {
+        a.points = output->points;
+        a.roots = output->roots;
+        a.root_uvs = output->root_uvs;
+        a.point_counts = output->point_counts;
+    }
     a.stencil_indices = stencil ? stencil->indices : nullptr;
     a.stencil_weights = stencil ? stencil->weights : nullptr;
+    return NXG_OK;
+}
 
+uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                const nxg_root_sample *explicit_roots, const float *attributes,
+                uint64_t attribute_capacity, const nxg_packed_output *output,
+                const nxg_stencil_output *stencil, const nxg_launch_config *config, void *workspace,
+                uint64_t workspace_bytes, void *stream_ptr, uint32_t stages) {
+    Plan plan;
+    if (uint32_t s = make_plan(asset, deformed, params, octaves, explicit_roots, attributes, attribute_capacity,
+                               output, stencil, config, workspace, workspace_bytes, stages, plan);
+        s != NXG_OK)
+        return s;
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    if (stages & kPrepare) {
+        hipLaunchKernelGGL(guide_prep_kernel, dim3(static_cast<uint32_t>((plan.prep_lanes + 255u) / 256u)),
+                           dim3(256), 0, stream, plan.p);
+        if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
+    }
+    if (!(stages & kStrands)) return NXG_OK;
+    const Args &a = plan.a;
     const dim3 grid((params->strand_count + 255u) / 256u), block(256);
-    if (!noise_enabled(*params, octaves)) {
+    if (!plan.noise) {
         hipLaunchKernelGGL((strand_kernel<false, 1>), grid, block, 0, stream, a);
     } else {
-        switch (oct) {
+        switch (a.octave_count) {
             case 1: hipLaunchKernelGGL((strand_kernel<true, 1>), grid, block, 0, stream, a); break;
             case 2: hipLaunchKernelGGL((strand_kernel<true, 2>), grid, block, 0, stream, a); break;
             case 3: hipLaunchKernelGGL((strand_kernel<true, 3>), grid, block, 0, stream, a); break;
@@ -1081,12 +1222,16 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
-uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
-                             const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count,
-                             float *attributes, uint64_t attribute_capacity, uint32_t *error_word,
-                             void *stream_ptr, uint32_t mode = kAttrAll, uint32_t stride = 1u,
-                             float threshold = 0.0f, uint32_t *tile_flags = nullptr,
-                             uint32_t *stats = nullptr) {
+struct AttrLaunch {
+    AttrArgs a;
+    uint32_t block, lds;
+    uint64_t lanes;
+};
+
+uint32_t make_attr_launch(const nxg_device_asset *asset, const nxg_generation_params *params,
+                          const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count, float *attributes,
+                          uint64_t attribute_capacity, uint32_t *error_word, uint32_t mode, uint32_t stride,
+                          float threshold, uint32_t *tile_flags, uint32_t *stats, AttrLaunch &out) {
     if (uint32_t s = validate_common(asset, nullptr, params, nullptr); s != NXG_OK) return s;
     if (!set || !set->image || reinterpret_cast<uintptr_t>(set->image) % 16u != 0u ||
         set->byte_size < sizeof(PackedSet) || set->program_count == 0u ||
@@ -1103,7 +1248,8 @@ uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation
     for (uint32_t m = 0; m < map_count; ++m)
         if (!maps[m].texels || maps[m].face_count == 0u || maps[m].resolution == 0u) return NXG_ERROR_INVALID_PROGRAM;
     const nxg_asset_header &h = asset->header;
-    AttrArgs a{};
+    AttrArgs &a = out.a;
+    a = AttrArgs{};
     a.positions = section<nxg_vec3>(*asset, h.positions_offset);
     a.normals = section<nxg_vec3>(*asset, h.normals_offset);
     a.texcoords = (h.flags & NXG_ASSET_HAS_TEXCOORDS) ? section<nxg_vec2>(*asset, h.texcoords_offset) : nullptr;
@@ -1129,11 +1275,159 @@ uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation
     // values[max_values][block] floats plus the gradient table in 64 KB of LDS
     uint32_t block = 256u;
     while (block > 64u && uint64_t{set->max_values} * block * 4u + 4096u > 65536u) block -= 64u;
-    const uint32_t lds = set->max_values * block * 4u + 4u * 4u * NXG_NOISE_GRADIENT_COUNT;
-    const uint64_t lanes = mode == kAttrCheck ? (uint64_t{params->strand_count} + stride - 1u) / stride
-                                              : params->strand_count;
-    hipLaunchKernelGGL(attribute_kernel, dim3(static_cast<uint32_t>((lanes + block - 1u) / block)), dim3(block), lds,
-                       static_cast<hipStream_t>(stream_ptr), a);
+    out.block = block;
+    out.lds = set->max_values * block * 4u + 4u * 4u * NXG_NOISE_GRADIENT_COUNT;
+    out.lanes = mode == kAttrCheck ? (uint64_t{params->strand_count} + stride - 1u) / stride : params->strand_count;
+    return NXG_OK;
+}
+
+uint32_t launch_attributes(const AttrLaunch &l, hipStream_t stream) {
+    if (l.lanes == 0u) return NXG_OK;
+    hipLaunchKernelGGL(attribute_kernel, dim3(static_cast<uint32_t>((l.lanes + l.block - 1u) / l.block)),
+                       dim3(l.block), l.lds, stream, l.a);
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
+uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
+                             const nxg_expr_set *set, const nxg_texel_map *maps, uint32_t map_count,
+                             float *attributes, uint64_t attribute_capacity, uint32_t *error_word,
+                             void *stream_ptr, uint32_t mode = kAttrAll, uint32_t stride = 1u,
+                             float threshold = 0.0f, uint32_t *tile_flags = nullptr,
+                             uint32_t *stats = nullptr) {
+    AttrLaunch l;
+    if (uint32_t s = make_attr_launch(asset, params, set, maps, map_count, attributes, attribute_capacity, error_word,
+                                      mode, stride, threshold, tile_flags, stats, l);
+        s != NXG_OK)
+        return s;
+    return launch_attributes(l, static_cast<hipStream_t>(stream_ptr));
+}
+
+// ---- batch host side --------------------------------------------------
+
+constexpr uint32_t kBatchWaveChunk[][2] = {{1, 4}, {1, 8}, {1, 16}, {2, 4}, {2, 8}, {2, 16}, {4, 4}, {4, 8}};
+
+const void *batch_kernel_for(uint32_t waves, uint32_t chunk) {
+    if (waves == 1u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<1, 4>);
+    if (waves == 1u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<1, 8>);
+    if (waves == 1u && chunk == 16u) return reinterpret_cast<const void *>(&batch_kernel<1, 16>);
+    if (waves == 2u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<2, 4>);
+    if (waves == 2u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<2, 8>);
+    if (waves == 2u && chunk == 16u) return reinterpret_cast<const void *>(&batch_kernel<2, 16>);
+    if (waves == 4u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<4, 4>);
+    if (waves == 4u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<4, 8>);
+    return nullptr;
+}
+
+nxg_batch_tuning default_tuning() { return nxg_batch_tuning{4u, 8u, 1u, 0u, NXG_BATCH_ORDER_HEAVY_FIRST}; }
+
+bool valid_tuning(const nxg_batch_tuning &t) {
+    return batch_kernel_for(t.waves_per_block, t.chunk_cvs) && t.tiles_per_claim >= 1u && t.tiles_per_claim <= 16u &&
+           t.blocks_per_cu <= 64u && t.order <= NXG_BATCH_ORDER_HEAVY_FIRST;
+}
+
+// Relative cost of one strand: CV count times noise passes and octaves.
+double strand_cost(const Args &a) {
+    const double passes = (a.noise && a.preserve > 0.001f) ? 2.0 : 1.0;
+    return static_cast<double>(a.cvs) * passes * (a.noise ? 1.0 + 3.0 * a.octave_count : 1.0);
+}
+
+struct BatchLaunch {
+    AttrLaunch attrs[NXG_BATCH_MAX_ITEMS];
+    bool has_attrs[NXG_BATCH_MAX_ITEMS];
+    BatchPrepArgs prep;
+    BatchArgs strands;
+    uint32_t prep_blocks;
+    nxg_batch_tuning tuning;
+};
+
+uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning, uint32_t *counter,
+                    BatchLaunch &out) {
+    if (!items || !counter) return NXG_ERROR_NULL_ARGUMENT;
+    if (count == 0u || count > NXG_BATCH_MAX_ITEMS) return NXG_ERROR_INVALID_BATCH;
+    out.tuning = tuning ? *tuning : default_tuning();
+    if (!valid_tuning(out.tuning)) return NXG_ERROR_INVALID_BATCH;
+    std::memset(&out.prep, 0, sizeof(out.prep));
+    std::memset(&out.strands, 0, sizeof(out.strands));
+    uint64_t blocks = 0u;
+    double cost[NXG_BATCH_MAX_ITEMS];
+    for (uint32_t k = 0; k < count; ++k) {
+        const nxg_batch_item &it = items[k];
+        if (!it.params || !it.output) return NXG_ERROR_NULL_ARGUMENT;
+        out.has_attrs[k] = it.exprs != nullptr;
+        if (it.exprs) {
+            if (uint32_t s = make_attr_launch(it.asset, it.params, it.exprs, it.maps, it.map_count, it.attributes,
+                                              it.attribute_capacity, nullptr, kAttrAll, 1u, 0.0f, nullptr, nullptr,
+                                              out.attrs[k]);
+                s != NXG_OK)
+                return s;
+        }
+        Plan plan;
+        if (uint32_t s = make_plan(it.asset, it.deformed, it.params, it.octaves, nullptr, it.attributes,
+                                   it.attribute_capacity, it.output, it.stencil, nullptr, it.workspace,
+                                   it.workspace_bytes, kPrepare | kStrands, plan);
+            s != NXG_OK)
+            return s;
+        out.prep.items[k] = plan.p;
+        blocks += (plan.prep_lanes + 255u) / 256u;
+        out.prep.block_end[k] = static_cast<uint32_t>(blocks);
+        out.strands.items[k] = plan.a;
+        cost[k] = strand_cost(plan.a);
+        out.strands.order[k] = k;
+    }
+    if (out.tuning.order == NXG_BATCH_ORDER_HEAVY_FIRST) {
+        // stable insertion sort, heaviest strands first
+        for (uint32_t i = 1; i < count; ++i)
+            for (uint32_t j = i; j > 0 && cost[out.strands.order[j]] > cost[out.strands.order[j - 1u]]; --j) {
+                const uint32_t t = out.strands.order[j];
+                out.strands.order[j] = out.strands.order[j - 1u];
+                out.strands.order[j - 1u] = t;
+            }
+    }
+    uint64_t tiles = 0u;
+    for (uint32_t k = 0; k < count; ++k) {
+        tiles += (uint64_t{out.strands.items[out.strands.order[k]].strand_count} + 63u) / 64u;
+        out.strands.tile_end[k] = static_cast<uint32_t>(tiles);
+    }
+    if (blocks > 0x7fffffffu || tiles > 0x7fffffffu) return NXG_ERROR_OUTPUT_SIZE_OVERFLOW;
+    out.prep.count = count;
+    out.prep.counter = counter;
+    out.prep_blocks = static_cast<uint32_t>(blocks);
+    out.strands.count = count;
+    out.strands.total_tiles = static_cast<uint32_t>(tiles);
+    out.strands.tiles_per_claim = out.tuning.tiles_per_claim;
+    out.strands.counter = counter;
+    return NXG_OK;
+}
+
+uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
+    for (uint32_t k = 0; k < b.strands.count; ++k)
+        if (b.has_attrs[k])
+            if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
+    if (b.prep_blocks) {
+        hipLaunchKernelGGL(batch_prep_kernel, dim3(b.prep_blocks), dim3(256), 0, stream, b.prep);
+    } else {
+        // no guides anywhere: the counter still has to start at zero
+        hipLaunchKernelGGL(batch_prep_kernel, dim3(1), dim3(1), 0, stream, b.prep);
+    }
+    if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
+    if (b.strands.total_tiles == 0u) return NXG_OK;
+    const nxg_batch_tuning &t = b.tuning;
+    const void *kernel = batch_kernel_for(t.waves_per_block, t.chunk_cvs);
+    const uint32_t threads = t.waves_per_block * kWave;
+    int device = 0, cus = 0, resident = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, static_cast<int>(threads), 0) != hipSuccess)
+        return NXG_ERROR_HIP;
+    if (resident < 1) resident = 1;
+    const uint32_t per_cu = t.blocks_per_cu ? min(t.blocks_per_cu, static_cast<uint32_t>(resident))
+                                            : static_cast<uint32_t>(resident);
+    const uint64_t claims = (uint64_t{b.strands.total_tiles} + t.tiles_per_claim - 1u) / t.tiles_per_claim;
+    uint64_t grid = uint64_t{per_cu} * static_cast<uint32_t>(cus);
+    grid = std::min<uint64_t>(grid, (claims + t.waves_per_block - 1u) / t.waves_per_block);
+    void *args[] = {&b.strands};
+    if (hipLaunchKernel(kernel, dim3(static_cast<uint32_t>(grid)), dim3(threads), args, 0, stream) != hipSuccess)
+        return NXG_ERROR_HIP;
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
@@ -1221,6 +1515,127 @@ uint32_t nxg_generate_packed_from_roots(const nxg_device_asset *asset,
     if (!roots) return NXG_ERROR_INVALID_ROOTS;
     return nxg::launch(asset, nullptr, params, octaves, roots, nullptr, 0, output, nullptr, config, workspace,
                        workspace_bytes, stream, nxg::kPrepare | nxg::kStrands);
+}
+
+void nxg_batch_default_tuning(nxg_batch_tuning *tuning) {
+    if (tuning) *tuning = nxg::default_tuning();
+}
+
+uint32_t nxg_batch_validate(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning) {
+    static_assert(sizeof(nxg::BatchArgs) <= 4096u, "batch kernel arguments exceed 4 KiB");
+    uint32_t counter_stub = 0u;  // validation only, never dereferenced
+    nxg::BatchLaunch b;
+    return nxg::make_batch(items, count, tuning, &counter_stub, b);
+}
+
+uint32_t nxg_generate_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning,
+                            uint32_t *counter, void *stream) {
+    nxg::BatchLaunch b;
+    if (uint32_t s = nxg::make_batch(items, count, tuning, counter, b); s != NXG_OK) return s;
+    return nxg::launch_batch(b, static_cast<hipStream_t>(stream));
+}
+
+struct nxg_batch_graph_handle {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning,
+                                uint32_t *counter, void **graph) {
+    if (!graph) return NXG_ERROR_NULL_ARGUMENT;
+    *graph = nullptr;
+    nxg::BatchLaunch b;
+    if (uint32_t s = nxg::make_batch(items, count, tuning, counter, b); s != NXG_OK) return s;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NXG_ERROR_HIP;
+    hipGraph_t g = nullptr;
+    uint32_t status = NXG_OK;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        status = NXG_ERROR_HIP;
+    } else {
+        status = nxg::launch_batch(b, s);
+        if (hipStreamEndCapture(s, &g) != hipSuccess) status = NXG_ERROR_HIP;
+    }
+    (void)hipStreamDestroy(s);
+    hipGraphExec_t exec = nullptr;
+    if (status == NXG_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) status = NXG_ERROR_HIP;
+    if (status != NXG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return status;
+    }
+    *graph = new nxg_batch_graph_handle{g, exec};
+    return NXG_OK;
+}
+
+uint32_t nxg_batch_graph_launch(void *graph, void *stream) {
+    if (!graph) return NXG_ERROR_NULL_ARGUMENT;
+    auto *h = static_cast<nxg_batch_graph_handle *>(graph);
+    return hipGraphLaunch(h->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
+uint32_t nxg_batch_graph_destroy(void *graph) {
+    if (!graph) return NXG_OK;
+    auto *h = static_cast<nxg_batch_graph_handle *>(graph);
+    uint32_t status = NXG_OK;
+    if (hipGraphExecDestroy(h->exec) != hipSuccess) status = NXG_ERROR_HIP;
+    if (hipGraphDestroy(h->graph) != hipSuccess) status = NXG_ERROR_HIP;
+    delete h;
+    return status;
+}
+
+uint32_t nxg_batch_tuning_candidates(nxg_batch_tuning *candidates, uint32_t capacity) {
+    uint32_t n = 0u;
+    for (const auto &wc : nxg::kBatchWaveChunk)
+        for (uint32_t claim : {1u, 4u})
+            for (uint32_t order : {NXG_BATCH_ORDER_GIVEN, NXG_BATCH_ORDER_HEAVY_FIRST}) {
+                if (candidates && n < capacity) candidates[n] = nxg_batch_tuning{wc[0], wc[1], claim, 0u, order};
+                ++n;
+            }
+    return n;
+}
+
+uint32_t nxg_batch_autotune(const nxg_batch_item *items, uint32_t count, uint32_t *counter, void *stream,
+                            const nxg_batch_tuning *candidates, uint32_t candidate_count, uint32_t repeats,
+                            nxg_batch_tuning *best, float *times_ms) {
+    if (!candidates || !best || !times_ms || candidate_count == 0u || repeats == 0u || repeats > 1000u)
+        return NXG_ERROR_INVALID_BATCH;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t start = nullptr, stop = nullptr;
+    if (hipEventCreate(&start) != hipSuccess) return NXG_ERROR_HIP;
+    if (hipEventCreate(&stop) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        return NXG_ERROR_HIP;
+    }
+    uint32_t status = NXG_OK;
+    float best_ms = 0.0f;
+    bool any = false;
+    float *samples = new float[repeats];
+    for (uint32_t c = 0; c < candidate_count && status == NXG_OK; ++c) {
+        nxg::BatchLaunch b;
+        times_ms[c] = -1.0f;  // invalid candidate
+        if (nxg::make_batch(items, count, &candidates[c], counter, b) != NXG_OK) continue;
+        status = nxg::launch_batch(b, s);  // warm-up
+        for (uint32_t r = 0; r < repeats && status == NXG_OK; ++r) {
+            if (hipEventRecord(start, s) != hipSuccess) status = NXG_ERROR_HIP;
+            if (status == NXG_OK) status = nxg::launch_batch(b, s);
+            if (status == NXG_OK && (hipEventRecord(stop, s) != hipSuccess || hipEventSynchronize(stop) != hipSuccess ||
+                                     hipEventElapsedTime(&samples[r], start, stop) != hipSuccess))
+                status = NXG_ERROR_HIP;
+        }
+        if (status != NXG_OK) break;
+        std::sort(samples, samples + repeats);
+        times_ms[c] = samples[repeats / 2u];
+        if (!any || times_ms[c] < best_ms) {
+            best_ms = times_ms[c];
+            *best = candidates[c];
+            any = true;
+        }
+    }
+    delete[] samples;
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    if (status == NXG_OK && !any) status = NXG_ERROR_INVALID_BATCH;
+    return status;
 }
 
 const char *nxg_build_info(void) {

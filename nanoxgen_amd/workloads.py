@@ -14,7 +14,10 @@ Workloads (sizes as stated for the benchmark configurations):
                           with log-normal(0, 0.5) per-triangle area weights;
                           4,096 guides of 16 CVs, length U(0.5, 1.5), curl
                           from a seeded sine sweep; K=8.
-Both take a `scale` so tests can run the same construction at reduced size.
+  config5                 nine descriptions of uneven size for the batched
+                          launch (see its docstring).
+Every builder takes its sizes as arguments, so tests can run the same
+construction at reduced size.
 """
 from dataclasses import dataclass, field
 
@@ -226,3 +229,56 @@ def config4(seed=2, strands=1_000_000, quads=512, guides_per_side=64, program_se
     maps = [expr.texel_map(len(build.triangles), 16, seed=map_seed)]
     return Workload(f"plane-{2 * quads * quads}tri-{strands}-strands-24cv-2oct-4x40op-surrogate", build, params,
                     octaves, info=dict(programs=programs, maps=maps, train_seed=seed + 1000))
+
+
+# Batched uneven form: (strands, CVs per strand, mesh, stencil K, octaves,
+# preserve-length, program ops).  CVs are paired with strand counts in
+# ascending order (description i gets the i-th CV count), which gives
+# 104,492,896 output points; meshes span 2,048 to 199,712 triangles and
+# guides 256 to 4,096.
+CONFIG5_DESCRIPTIONS = [
+    (5_000, 4, ("plane", 32, 16), 4, 1, 0.0, 12),
+    (20_000, 8, ("sphere", 4, 256), 8, 1, 0.0, 24),
+    (60_000, 12, ("plane", 64, 24), 4, 2, 0.0, 16),
+    (120_000, 16, ("sphere", 5, 1024), 8, 1, 0.4, 32),
+    (200_000, 20, ("plane", 128, 32), 4, 2, 0.0, 20),
+    (300_000, 24, ("sphere", 6, 2048), 8, 1, 0.0, 40),
+    (400_000, 32, ("plane", 200, 48), 4, 2, 0.0, 24),
+    (550_000, 48, ("plane", 256, 56), 8, 1, 0.0, 40),
+    (801_139, 64, ("plane", 316, 64), 8, 2, 0.0, 48),
+]
+
+
+def config5(seed=50, strand_divisor=1, mesh_divisor=1, descriptions=None):
+    """Batched uneven form: nine descriptions, each with its own seeded mesh,
+    guides, stencil K, octaves and per-root length/width/noise programs
+    (plus a 4x4 per-face texel map), generated in one batched launch.
+    strand_divisor / mesh_divisor shrink strand counts and mesh resolution
+    (plane quads and guides per side, sphere subdivision) for tests."""
+    from . import expr
+    out = []
+    table = CONFIG5_DESCRIPTIONS if descriptions is None else [CONFIG5_DESCRIPTIONS[i] for i in descriptions]
+    for i, (strands, cvs, mesh, k, octs, preserve, ops) in enumerate(table):
+        dseed = seed + 7 * i
+        if mesh[0] == "plane":
+            quads = max(8, mesh[1] // mesh_divisor)
+            side = max(4, min(mesh[2] // mesh_divisor, quads))
+            build = plane_description(seed=dseed, quads=quads, guides_per_side=side, guide_cvs=16, stencil_k=k)
+            base_len, root_w, tip_w = 1.0, 0.01, 0.001
+        else:
+            sub = max(2, mesh[1] - (mesh_divisor.bit_length() - 1))
+            guides = max(32, mesh[2] // mesh_divisor)
+            build = icosphere_description(seed=dseed, subdivisions=sub, guides=guides, guide_cvs=8, stencil_k=k)
+            base_len, root_w, tip_w = 0.3, 0.02, 0.002
+        params = GenerationParams(strand_count=max(1, strands // strand_divisor), cvs_per_strand=cvs, seed=dseed,
+                                  root_width=root_w, tip_width=tip_w, noise_amplitude=0.08 * base_len,
+                                  noise_frequency=4.0, noise_preserve_length=preserve)
+        octaves = NoiseOctaves.of([(0.08 * base_len, 4.0), (0.04 * base_len, 8.0)]) if octs == 2 else None
+        noise_calls = 4 if ops >= 24 else 2
+        programs = [expr.random_attribute_program(dseed * 1000 + j, *expr.ATTRIBUTE_RANGES[n], n_ops=ops,
+                                                  noise_calls=noise_calls)
+                    for j, n in enumerate(["length", "width", "noise"])]
+        maps = [expr.texel_map(len(build.triangles), 4, seed=dseed + 1)]
+        name = f"d{i}-{mesh[0]}-{len(build.triangles)}tri-{params.strand_count}x{cvs}cv-K{k}-{octs}oct-{ops}op"
+        out.append(Workload(name, build, params, octaves, info=dict(programs=programs, maps=maps, ops=ops)))
+    return out

@@ -10,7 +10,7 @@ import pytest
 
 from conftest import ROOT, aligned_copy
 import nanoxgen_amd as nx
-from nanoxgen_amd import abi, workloads
+from nanoxgen_amd import abi, batch, workloads
 
 
 def header_symbols():
@@ -189,4 +189,68 @@ def test_status_messages():
     assert L.nxg_status_message(20) == b"device output size overflows uint64"
     assert L.nxg_status_message(999) == b"unknown device generation error"
     for s in range(1, len(abi.STATUS)):
-        assert L.nxg_status_message(s)
+        assert L.nxg_status_message(s) and L.nxg_status_message(s) != b"unknown device generation error"
+
+
+# ---- batched descriptions (host-side validation, no device work) ---------
+
+def _batch_items(r, count, ws_bytes=None):
+    ws = np.zeros(1 << 16, np.uint8)
+    r.keep = [ws]
+    items = (abi.BatchItem * count)()
+    for it in items:
+        it.asset = C.addressof(r.desc)
+        it.params = C.addressof(r.params)
+        it.output = C.addressof(r.out)
+        it.workspace = ws.ctypes.data
+        it.workspace_bytes = ws.nbytes if ws_bytes is None else ws_bytes
+    return items
+
+
+def test_batch_validation():
+    r = Request()
+    L = nx.lib()
+    items = _batch_items(r, 3)
+    assert L.nxg_batch_validate(items, 3, None) == abi.OK
+    assert L.nxg_batch_validate(items, 0, None) == abi.INVALID_BATCH
+    many = _batch_items(r, abi.BATCH_MAX_ITEMS + 1)
+    assert L.nxg_batch_validate(many, abi.BATCH_MAX_ITEMS + 1, None) == abi.INVALID_BATCH
+    assert L.nxg_batch_validate(many, abi.BATCH_MAX_ITEMS, None) == abi.OK
+    for bad in [(3, 8, 1, 0, 0), (4, 16, 1, 0, 0), (1, 8, 0, 0, 0), (1, 8, 17, 0, 0), (1, 8, 1, 65, 0),
+                (1, 8, 1, 0, 2)]:
+        assert L.nxg_batch_validate(items, 3, C.byref(abi.BatchTuning(*bad))) == abi.INVALID_BATCH, bad
+    for t in batch.tuning_candidates():
+        assert L.nxg_batch_validate(items, 3, C.byref(t)) == abi.OK
+    # one bad item refuses the whole batch with that item's status
+    small = _batch_items(r, 3)
+    small[1].workspace_bytes = 16
+    assert L.nxg_batch_validate(small, 3, None) == abi.WORKSPACE_TOO_SMALL
+    noparams = _batch_items(r, 2)
+    noparams[1].params = None
+    assert L.nxg_batch_validate(noparams, 2, None) == abi.NULL_ARGUMENT
+    bad_oct = abi.NoiseOctaves(count=5)
+    o = _batch_items(r, 2)
+    o[0].octaves = C.addressof(bad_oct)
+    assert L.nxg_batch_validate(o, 2, None) == abi.INVALID_OCTAVES
+    # attributes to be evaluated need a valid expression set and capacity
+    ex = _batch_items(r, 1)
+    es = abi.ExprSet()
+    ex[0].exprs = C.addressof(es)
+    attrs = np.zeros((7, 4), np.float32)
+    ex[0].attributes = attrs.ctypes.data
+    ex[0].attribute_capacity = attrs.size
+    assert L.nxg_batch_validate(ex, 1, None) == abi.INVALID_PROGRAM
+    # generation and graph capture refuse before any device work
+    counter = np.zeros(1, np.uint32)
+    assert L.nxg_generate_batch(small, 3, None, counter.ctypes.data, None) == abi.WORKSPACE_TOO_SMALL
+    assert L.nxg_generate_batch(items, 3, None, None, None) == abi.NULL_ARGUMENT
+    h = C.c_void_p()
+    assert L.nxg_batch_graph_create(small, 3, None, counter.ctypes.data, C.byref(h)) == abi.WORKSPACE_TOO_SMALL
+    assert not h.value
+    assert L.nxg_batch_graph_destroy(None) == abi.OK
+
+
+def test_batch_default_tuning_is_a_candidate():
+    d = batch.default_tuning()
+    assert d.as_tuple() in [c.as_tuple() for c in batch.tuning_candidates()]
+    assert len(batch.tuning_candidates()) == nx.lib().nxg_batch_tuning_candidates(None, 0) == 32
