@@ -120,25 +120,14 @@ __device__ __forceinline__ float lerp_fast(float a, float b, float alpha) {
     return fmaf(alpha, b - a, a);
 }
 
-// One lattice coordinate of the fixed (non-moving) axes of a noise sample.
-struct Axis {
-    int32_t i;
-    float w, alpha;
-};
-__device__ __forceinline__ Axis make_axis(float v) {
-    const float f = floorf(v);
-    Axis a;
-    a.i = static_cast<int32_t>(f);
-    a.w = v - f;
-    a.alpha = s_curve(a.w);
-    return a;
-}
-
-// Per-strand constants of the three noise samples (x, y or z moving).
+// Per-strand constants of the three noise samples.  A strand's noise point
+// for output axis A is the domain point moved along lattice axis A by
+// distance = arc * frequency (generate.h:486-495), so the two other lattice
+// coordinates never change along the strand.
 struct NoiseDomain {
-    float d[3];
-    Axis fixed[3];
-    uint32_t base[3][4];  // per moving axis: LCG constant of the 4 fixed corners
+    float d[3];           // domain point
+    float w[3], alpha[3]; // fractional lattice offset and its s-curve per axis
+    uint32_t lcg[3];      // lattice coordinate of each axis times its LCG weight
 };
 
 __device__ __forceinline__ NoiseDomain make_domain(V3 d) {
@@ -146,59 +135,82 @@ __device__ __forceinline__ NoiseDomain make_domain(V3 d) {
     n.d[0] = d.x;
     n.d[1] = d.y;
     n.d[2] = d.z;
-    n.fixed[0] = make_axis(d.x);
-    n.fixed[1] = make_axis(d.y);
-    n.fixed[2] = make_axis(d.z);
-    const uint32_t X0 = static_cast<uint32_t>(n.fixed[0].i), Y0 = static_cast<uint32_t>(n.fixed[1].i),
-                   Z0 = static_cast<uint32_t>(n.fixed[2].i);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const uint32_t o1 = c & 1, o2 = (c >> 1) & 1;
-        n.base[0][c] = (Y0 + o1) * kLcgA + (Z0 + o2) + kLcgK;           // x moves: (oy, oz)
-        n.base[1][c] = (X0 + o1) * kLcgA2 + (Z0 + o2) + kLcgK;          // y moves: (ox, oz)
-        n.base[2][c] = (X0 + o1) * kLcgA2 + (Y0 + o2) * kLcgA + kLcgK;  // z moves: (ox, oy)
+    for (int ax = 0; ax < 3; ++ax) {
+        const float f = floorf(n.d[ax]);
+        n.w[ax] = n.d[ax] - f;
+        n.alpha[ax] = s_curve(n.w[ax]);
+        const uint32_t mult = ax == 0 ? kLcgA2 : (ax == 1 ? kLcgA : 1u);
+        n.lcg[ax] = static_cast<uint32_t>(static_cast<int32_t>(f)) * mult;
     }
     return n;
 }
 
-// 2 * (gradient_noise - 0.5) (generate.h:370-402) at the domain point
-// displaced by `distance` along axis A, with the gradient table in LDS.
+// The lattice cube of gradient_noise (generate.h:370-402) has two faces
+// across the moving axis A, at lattice coordinates j and j + 1.  Each
+// corner's contribution g . (w - corner) is linear in the moving offset, and
+// the trilinear reduction is linear in the corner values, so a face reduces
+// over the two fixed axes to  value(w_A) = w_A * p + q.  A strand crosses a
+// new lattice plane only every few CVs, so faces are cached and the noise at
+// a CV costs one face (four corners) at most instead of eight corners.
+struct FaceCache {
+    int32_t cell;          // lattice coordinate of face 0 (face 1 is cell + 1)
+    float p0, q0, p1, q1;
+};
+
 template <int A>
-__device__ __forceinline__ float gradient_noise_axis(const NoiseDomain &n, float distance,
+__device__ __forceinline__ void noise_face(const NoiseDomain &n, int32_t j, const float4 *grad, float &p,
+                                           float &q) {
+    constexpr int F1 = A == 0 ? 1 : 0;  // fixed axes in x, y, z order
+    constexpr int F2 = A == 2 ? 1 : 2;
+    constexpr uint32_t kMult[3] = {kLcgA2, kLcgA, 1u};
+    // the LCG seed of corner (x, y, z) is x*a^2 + y*a + z + K (generate.h:356-358)
+    const uint32_t h = static_cast<uint32_t>(j) * kMult[A] + n.lcg[F1] + n.lcg[F2] + kLcgK;
+    float gp[4], gq[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const float o1 = static_cast<float>(f & 1), o2 = static_cast<float>(f >> 1);
+        const uint32_t corner = (f & 1 ? kMult[F1] : 0u) + (f & 2 ? kMult[F2] : 0u);
+        const float4 g = grad[temper_index(h + corner)];
+        const float ga = A == 0 ? g.x : (A == 1 ? g.y : g.z);
+        const float g1 = F1 == 0 ? g.x : g.y;
+        const float g2 = F2 == 1 ? g.y : g.z;
+        gp[f] = ga;
+        gq[f] = fmaf(g2, n.w[F2] - o2, g1 * (n.w[F1] - o1));
+    }
+    p = lerp_fast(lerp_fast(gp[0], gp[1], n.alpha[F1]), lerp_fast(gp[2], gp[3], n.alpha[F1]), n.alpha[F2]);
+    q = lerp_fast(lerp_fast(gq[0], gq[1], n.alpha[F1]), lerp_fast(gq[2], gq[3], n.alpha[F1]), n.alpha[F2]);
+}
+
+__device__ __forceinline__ FaceCache empty_face_cache() {
+    FaceCache c;
+    c.cell = -2147483647 - 1;
+    c.p0 = c.q0 = c.p1 = c.q1 = 0.0f;
+    return c;
+}
+
+// 2 * (gradient_noise - 0.5) (generate.h:370-402) at the domain point moved
+// by `distance` along axis A, with the gradient table in LDS.
+template <int A>
+__device__ __forceinline__ float gradient_noise_axis(const NoiseDomain &n, float distance, FaceCache &fc,
                                                      const float4 *grad) {
-    const Axis m = make_axis(n.d[A] + distance);
-    const uint32_t mult = A == 0 ? kLcgA2 : (A == 1 ? kLcgA : 1u);
-    const uint32_t h0 = static_cast<uint32_t>(m.i) * mult;
-    // weights[axis] - offset and alpha per axis, in x, y, z order
-    float w[3][2], alpha[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        const Axis a = ax == A ? m : n.fixed[ax];
-        w[ax][0] = a.w - 0.0f;
-        w[ax][1] = a.w - 1.0f;
-        alpha[ax] = a.alpha;
+    const float x = n.d[A] + distance;
+    const float fl = floorf(x);
+    const int32_t i = static_cast<int32_t>(fl);
+    const float w = x - fl;
+    if (i != fc.cell) {
+        if (i != fc.cell + 1) {
+            noise_face<A>(n, i, grad, fc.p0, fc.q0);
+        } else {
+            fc.p0 = fc.p1;
+            fc.q0 = fc.q1;
+        }
+        noise_face<A>(n, i + 1, grad, fc.p1, fc.q1);
+        fc.cell = i;
     }
-    float v[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
-        const int om = A == 0 ? ox : (A == 1 ? oy : oz);
-        const int f1 = A == 0 ? oy : ox;
-        const int f2 = A == 2 ? oy : oz;
-        const uint32_t seed = h0 + (om ? mult : 0u) + n.base[A][f1 | (f2 << 1)];
-        const float4 g = grad[temper_index(seed)];
-        v[c] = fmaf(g.z, w[2][oz], fmaf(g.y, w[1][oy], g.x * w[0][ox]));
-    }
-    // x, then y, then z reduction, in the reference's loop-nest order
-    v[0] = lerp_fast(v[0], v[1], alpha[0]);
-    v[2] = lerp_fast(v[2], v[3], alpha[0]);
-    v[4] = lerp_fast(v[4], v[5], alpha[0]);
-    v[6] = lerp_fast(v[6], v[7], alpha[0]);
-    v[0] = lerp_fast(v[0], v[2], alpha[1]);
-    v[4] = lerp_fast(v[4], v[6], alpha[1]);
-    // gradient_noise returns 0.5 * v + 0.5 and the caller subtracts 0.5:
-    // the signed value 0.5 * v is returned directly
-    return lerp_fast(v[0], v[4], alpha[2]);
+    // gradient_noise returns 0.5 * v + 0.5 and the caller subtracts 0.5;
+    // the caller's scale carries the 0.5
+    return lerp_fast(fmaf(w, fc.p0, fc.q0), fmaf(w - 1.0f, fc.p1, fc.q1), s_curve(w));
 }
 
 // rotate_by(value, axis, acos(c)) (generate.h:404-418, 477-480) with
@@ -390,13 +402,14 @@ struct NoiseState {
     NoiseDomain dom;
     float freq[OCT];
     float mag[OCT];  // noise_mask * amplitude
+    FaceCache face[OCT][3];
 };
 
 // evaluate_and_advance_noise_cursor (generate.h:459-515); OCT octaves share
 // the transported frame and the arc length and their displacements add.
 template <int OCT, bool RT = false>
 __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend &b,
-                                      const NoiseState<OCT> &ns, Cursor &c, const float4 *grad) {
+                                      NoiseState<OCT> &ns, Cursor &c, const float4 *grad) {
     if (c.cv > 0u) c.arc += fast_sqrt(fma3(c.cur - c.prev, c.cur - c.prev));
     V3 nt = c.tangent;
     if (c.cv + 1u < a.cvs) {
@@ -420,9 +433,9 @@ __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend 
             if (RT && k >= static_cast<int>(a.octave_count)) break;
             const float distance = c.arc * ns.freq[k];
             const float m = ns.mag[k] * half_t;  // noise_mask * amplitude * t / 2
-            lx = fmaf(gradient_noise_axis<0>(ns.dom, distance, grad), m, lx);
-            ly = fmaf(gradient_noise_axis<1>(ns.dom, distance, grad), m, ly);
-            lz = fmaf(gradient_noise_axis<2>(ns.dom, distance, grad), m, lz);
+            lx = fmaf(gradient_noise_axis<0>(ns.dom, distance, ns.face[k][0], grad), m, lx);
+            ly = fmaf(gradient_noise_axis<1>(ns.dom, distance, ns.face[k][1], grad), m, ly);
+            lz = fmaf(gradient_noise_axis<2>(ns.dom, distance, ns.face[k][2], grad), m, lz);
         }
         point = fma_v(tangent, lz, fma_v(binormal, ly, fma_v(normal, lx, point)));
     }
@@ -637,7 +650,13 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
     for (uint32_t pass = 0; pass < passes; ++pass) {
         const bool emit = pass + 1u == passes;
         Cursor cur;
-        if constexpr (NOISE) cur = make_cursor<OCT>(a, r, b, surface_u);
+        if constexpr (NOISE) {
+            cur = make_cursor<OCT>(a, r, b, surface_u);
+#pragma unroll
+            for (int k = 0; k < OCT; ++k)
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) ns.face[k][ax] = empty_face_cache();
+        }
         V3 last = v3(0.0f, 0.0f, 0.0f);
         float noisy = 0.0f;
         for (uint32_t cv0 = 0; cv0 < a.cvs; cv0 += CHUNK) {
@@ -680,8 +699,9 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
 }
 
 
+// Up to two octaves fit 168 VGPRs, so that three wavefronts share each SIMD.
 template <bool NOISE, int OCT>
-__global__ void __launch_bounds__(256) strand_kernel(Args a) {
+__global__ void __launch_bounds__(256, (OCT <= 2 ? 3 : 2)) strand_kernel(Args a) {
     __shared__ float4 grad[NXG_NOISE_GRADIENT_COUNT];
     __shared__ float stage_all[256 / kWave][kChunk * 4 * kStageStride];
     for (uint32_t i = threadIdx.x; i < NXG_NOISE_GRADIENT_COUNT; i += blockDim.x) {
@@ -731,8 +751,8 @@ struct BatchArgs {
 // tiles from one atomic counter until the work list is exhausted (every
 // wavefront leaves once its claim starts past the last tile); each tile's
 // description comes from a binary search of the tile scan held in LDS.
-template <int WAVES, int CHUNK>
-__global__ void __launch_bounds__(WAVES * 64) batch_kernel(BatchArgs b) {
+template <int WAVES, int CHUNK, int OCTMAX>
+__global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 ? 3 : 2)) batch_kernel(BatchArgs b) {
     __shared__ float4 grad_all[WAVES][NXG_NOISE_GRADIENT_COUNT];
     __shared__ float stage_all[WAVES][CHUNK * 4 * kStageStride];
     __shared__ uint32_t scan[NXG_BATCH_MAX_ITEMS];
@@ -766,7 +786,7 @@ __global__ void __launch_bounds__(WAVES * 64) batch_kernel(BatchArgs b) {
                 loaded = item;
             }
             const uint64_t wave_first = uint64_t{tile - item_first} * kWave;
-            if (a.noise) strand_tile<true, NXG_MAX_NOISE_OCTAVES, true, CHUNK>(a, wave_first, lane, stage_all[wave], grad);
+            if (a.noise) strand_tile<true, OCTMAX, true, CHUNK>(a, wave_first, lane, stage_all[wave], grad);
             else strand_tile<false, 1, false, CHUNK>(a, wave_first, lane, stage_all[wave], grad);
             wave_sync();
         }
@@ -1306,16 +1326,23 @@ uint32_t evaluate_attributes(const nxg_device_asset *asset, const nxg_generation
 
 constexpr uint32_t kBatchWaveChunk[][2] = {{1, 4}, {1, 8}, {1, 16}, {2, 4}, {2, 8}, {2, 16}, {4, 4}, {4, 8}};
 
-const void *batch_kernel_for(uint32_t waves, uint32_t chunk) {
-    if (waves == 1u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<1, 4>);
-    if (waves == 1u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<1, 8>);
-    if (waves == 1u && chunk == 16u) return reinterpret_cast<const void *>(&batch_kernel<1, 16>);
-    if (waves == 2u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<2, 4>);
-    if (waves == 2u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<2, 8>);
-    if (waves == 2u && chunk == 16u) return reinterpret_cast<const void *>(&batch_kernel<2, 16>);
-    if (waves == 4u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<4, 4>);
-    if (waves == 4u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<4, 8>);
+template <int OCTMAX>
+const void *batch_kernel_of(uint32_t waves, uint32_t chunk) {
+    if (waves == 1u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<1, 4, OCTMAX>);
+    if (waves == 1u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<1, 8, OCTMAX>);
+    if (waves == 1u && chunk == 16u) return reinterpret_cast<const void *>(&batch_kernel<1, 16, OCTMAX>);
+    if (waves == 2u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<2, 4, OCTMAX>);
+    if (waves == 2u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<2, 8, OCTMAX>);
+    if (waves == 2u && chunk == 16u) return reinterpret_cast<const void *>(&batch_kernel<2, 16, OCTMAX>);
+    if (waves == 4u && chunk == 4u) return reinterpret_cast<const void *>(&batch_kernel<4, 4, OCTMAX>);
+    if (waves == 4u && chunk == 8u) return reinterpret_cast<const void *>(&batch_kernel<4, 8, OCTMAX>);
     return nullptr;
+}
+
+// The batch kernel carries noise state for two octaves when no item uses
+// more (fewer registers, three wavefronts per SIMD), else for four.
+const void *batch_kernel_for(uint32_t waves, uint32_t chunk, uint32_t max_octaves = 2u) {
+    return max_octaves <= 2u ? batch_kernel_of<2>(waves, chunk) : batch_kernel_of<4>(waves, chunk);
 }
 
 nxg_batch_tuning default_tuning() { return nxg_batch_tuning{4u, 8u, 1u, 0u, NXG_BATCH_ORDER_HEAVY_FIRST}; }
@@ -1337,6 +1364,7 @@ struct BatchLaunch {
     BatchPrepArgs prep;
     BatchArgs strands;
     uint32_t prep_blocks;
+    uint32_t max_octaves;
     nxg_batch_tuning tuning;
 };
 
@@ -1348,6 +1376,7 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
     if (!valid_tuning(out.tuning)) return NXG_ERROR_INVALID_BATCH;
     std::memset(&out.prep, 0, sizeof(out.prep));
     std::memset(&out.strands, 0, sizeof(out.strands));
+    out.max_octaves = 1u;
     uint64_t blocks = 0u;
     double cost[NXG_BATCH_MAX_ITEMS];
     for (uint32_t k = 0; k < count; ++k) {
@@ -1371,6 +1400,7 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
         blocks += (plan.prep_lanes + 255u) / 256u;
         out.prep.block_end[k] = static_cast<uint32_t>(blocks);
         out.strands.items[k] = plan.a;
+        if (plan.noise && plan.a.octave_count > out.max_octaves) out.max_octaves = plan.a.octave_count;
         cost[k] = strand_cost(plan.a);
         out.strands.order[k] = k;
     }
@@ -1412,7 +1442,7 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
     if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     if (b.strands.total_tiles == 0u) return NXG_OK;
     const nxg_batch_tuning &t = b.tuning;
-    const void *kernel = batch_kernel_for(t.waves_per_block, t.chunk_cvs);
+    const void *kernel = batch_kernel_for(t.waves_per_block, t.chunk_cvs, b.max_octaves);
     const uint32_t threads = t.waves_per_block * kWave;
     int device = 0, cus = 0, resident = 0;
     if (hipGetDevice(&device) != hipSuccess ||
