@@ -535,11 +535,21 @@ __global__ void __launch_bounds__(256) guide_prep_kernel(Prep p) {
 
 // Writes the wave's staged chunk [cv0, cv0 + n) for its 64 strands as
 // contiguous runs of n float4 per strand.
+// Stores a staged chunk of n CVs of the wavefront's 64 strands: lane L of
+// pass `it` writes flat element it*64 + L of the strand-major chunk, so each
+// store instruction covers consecutive float4s.  N is n when it is known at
+// compile time (full chunks), else 0.
+template <uint32_t N>
 __device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, uint64_t wave_first,
                                             uint32_t cv0, uint32_t n, uint32_t lane) {
-    for (uint32_t it = 0; it < n; ++it) {
+    // f / n for f < 64 * n <= 1024 as floor((f + 0.5) / n) in float: the
+    // half keeps exact quotients away from rounding below an integer
+    const float inv_n = 1.0f / static_cast<float>(N ? N : n);
+#pragma unroll 1
+    for (uint32_t it = 0; it < (N ? N : n); ++it) {
         const uint32_t f = it * kWave + lane;
-        const uint32_t s = f / n, k = f % n;
+        const uint32_t s = static_cast<uint32_t>((static_cast<float>(f) + 0.5f) * inv_n);
+        const uint32_t k = f - s * (N ? N : n);
         const uint64_t strand = wave_first + s;
         if (strand < a.strand_count) {
             float4 p;
@@ -661,6 +671,13 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
         float noisy = 0.0f;
         for (uint32_t cv0 = 0; cv0 < a.cvs; cv0 += CHUNK) {
             const uint32_t n = a.cvs - cv0 < CHUNK ? a.cvs - cv0 : CHUNK;
+            // evaluate_strand_width's taper (generate.h:577-581) of the
+            // chunk's CVs, one per lane, read back per CV with readlane
+            float taper = 0.0f;
+            if (emit) {
+                const float t = static_cast<float>(cv0 + lane) / static_cast<float>(a.cvs - 1u);
+                taper = a.root_width * (1.0f - t) + a.tip_width * t;
+            }
 #pragma unroll 1
             for (uint32_t k = 0; k < n; ++k) {
                 const uint32_t cv = cv0 + k;
@@ -677,8 +694,8 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
                 }
                 if (preserve_scale != 1.0f) p = r.position + (p - r.position) * preserve_scale;
                 // evaluate_strand_width (generate.h:577-581) and the packed radius
-                const float t = static_cast<float>(cv) / static_cast<float>(a.cvs - 1u);
-                const float width = (a.root_width * (1.0f - t) + a.tip_width * t) * attr.y;
+                const float width =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(taper), static_cast<int>(k))) * attr.y;
                 stage[(k * 4u + 0u) * kStageStride + lane] = p.x;
                 stage[(k * 4u + 1u) * kStageStride + lane] = p.y;
                 stage[(k * 4u + 2u) * kStageStride + lane] = p.z;
@@ -686,7 +703,7 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
             }
             if (emit) {
                 wave_sync();
-                flush_chunk(a, stage, wave_first, cv0, n, lane);
+                flush_chunk<0>(a, stage, wave_first, cv0, n, lane);
                 wave_sync();
             }
         }
