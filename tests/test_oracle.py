@@ -268,3 +268,51 @@ def test_fp64_twin_agrees(orc):
     assert np.array_equal(f32["roots"]["triangle_index"], f64["roots"][:, 8].astype(np.uint32))
     err = np.abs(f32["points"].astype(np.float64) - f64["points"]).max()
     assert err < 2e-5
+
+
+# ---- extensions without a reference counterpart, pinned to the reference's
+# ---- single-octave, unattributed generation (itself pinned above)
+
+def test_octave_stacking_is_the_sum_of_single_octaves(orc):
+    """The transported frame and arc length come from the base polyline, so a
+    stacked displacement is the sum of the single-octave displacements (each
+    the reference's own formula, frequency floor included) and stacking with a
+    zero second amplitude is the single octave bit for bit."""
+    w = workloads.config2(strands=1500, quads=32, guides_per_side=8)
+    blob = orc.build_asset(w.build)
+    p = w.params.copy(noise_preserve_length=0.0)
+    (a1, f1), (a2, f2) = (0.1, 4.0), (0.05, 8.0)
+    base = orc.generate(blob, p.copy(noise_amplitude=0.0))["points"].astype(np.float64)
+    one = orc.generate(blob, p.copy(noise_amplitude=a1, noise_frequency=f1))["points"].astype(np.float64)
+    two = orc.generate(blob, p.copy(noise_amplitude=a2, noise_frequency=f2))["points"].astype(np.float64)
+    both = orc.generate(blob, p, octaves=abi.NoiseOctaves.of([(a1, f1), (a2, f2)]))["points"].astype(np.float64)
+    err = np.abs((both - base) - ((one - base) + (two - base)))[:, :3].max()
+    assert err < 2e-6, err
+    assert np.array_equal(both[:, 3], one[:, 3])
+    solo = orc.generate(blob, p, octaves=abi.NoiseOctaves.of([(a1, f1), (0.0, f2)]))["points"]
+    assert np.array_equal(solo.view(np.uint32), orc.generate(
+        blob, p.copy(noise_amplitude=a1, noise_frequency=f1))["points"].view(np.uint32))
+
+
+def test_attributes_act_as_per_root_parameters(orc):
+    """Attribute slot 0 scales length_scale, slot 1 the width, slot 2 the
+    noise amplitude: a strand generated with attributes (s0, s1, s2, c)
+    matches the reference's generation of that strand with length_scale*s0,
+    root/tip width*s1 and amplitude*s2 (slot 3 is carried, not consumed)."""
+    w = workloads.config1(strands=64)
+    blob = orc.build_asset(w.build)
+    rng = np.random.default_rng(5)
+    attrs = np.stack([rng.uniform(0.5, 1.5, 64), rng.uniform(0.5, 2.0, 64), rng.uniform(0.0, 2.0, 64),
+                      rng.uniform(0, 1, 64)], 1).astype(np.float32)
+    got = orc.generate(blob, w.params, attributes=attrs)["points"]
+    C = w.params.cvs_per_strand
+    worst = 0.0
+    for s in range(0, 64, 7):
+        q = w.params.copy(length_scale=w.params.length_scale * attrs[s, 0],
+                          root_width=w.params.root_width * attrs[s, 1], tip_width=w.params.tip_width * attrs[s, 1],
+                          noise_amplitude=w.params.noise_amplitude * attrs[s, 2])
+        want = orc.generate(blob, q, strands=[s])["points"]
+        g = got[s * C:(s + 1) * C]
+        worst = max(worst, float(np.abs(g[:, :3].astype(np.float64) - want[:, :3]).max()))
+        assert np.allclose(g[:, 3], want[:, 3], rtol=1e-6, atol=0)
+    assert worst < 1e-6, worst
