@@ -5,6 +5,8 @@
 // input and ramp indices, result index), so the device interpreter can run
 // without them; only non-finite values remain a run-time condition.
 #include <cstring>
+#include <functional>
+#include <queue>
 #include <vector>
 
 #include "expr_format.h"
@@ -72,9 +74,53 @@ namespace {
 uint64_t align16(uint64_t v) { return (v + 15u) & ~uint64_t{15u}; }
 
 uint64_t program_bytes(const nxg_expr_program &p) {
-    return align16(sizeof(PackedProgram)) + align16(uint64_t{p.instruction_count} * sizeof(PackedInstruction)) +
+    return align16(sizeof(PackedProgram)) + align16(uint64_t{p.instruction_count + 1u} * sizeof(PackedInstruction)) +
            align16(uint64_t{p.operand_count} * 4u) + align16(uint64_t{p.ramp_count} * sizeof(nxg_expr_ramp)) +
            align16(uint64_t{p.ramp_point_count} * sizeof(nxg_expr_ramp_point));
+}
+
+
+// Liveness-based slot allocation over the SSA order of a validated program:
+// a value's slot returns to the pool after its last reader (operands are
+// read before the result is written, so a result may take the slot of one of
+// its own operands); the result value is never released; inputs use their
+// root variable's reserved slot.  Returns the slot count, vars included.
+uint32_t allocate_slots(const nxg_expr_program &p, std::vector<uint16_t> &slot_of) {
+    const uint32_t n = p.instruction_count;
+    std::vector<int64_t> last_use(n, -1);
+    for (uint32_t i = 0; i < n; ++i) {
+        const nxg_expr_instruction &in = p.instructions[i];
+        for (uint32_t k = 0; k < in.operand_count; ++k) last_use[p.operands[in.operand_offset + k]] = i;
+    }
+    last_use[p.result] = n;
+    slot_of.assign(n, 0u);
+    std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> free_slots;
+    std::vector<bool> released(n, false);
+    uint32_t next = kVarSlots;
+    for (uint32_t i = 0; i < n; ++i) {
+        const nxg_expr_instruction &in = p.instructions[i];
+        for (uint32_t k = 0; k < in.operand_count; ++k) {
+            const uint32_t v = p.operands[in.operand_offset + k];
+            if (last_use[v] == i && !released[v] && p.instructions[v].op != NXG_OP_INPUT) {
+                released[v] = true;
+                free_slots.push(slot_of[v]);
+            }
+        }
+        if (in.op == NXG_OP_INPUT) {
+            slot_of[i] = static_cast<uint16_t>(p.input_bindings[in.auxiliary]);
+            continue;
+        }
+        uint32_t slot;
+        if (free_slots.empty()) {
+            slot = next++;
+        } else {
+            slot = free_slots.top();
+            free_slots.pop();
+        }
+        slot_of[i] = static_cast<uint16_t>(slot);
+        if (last_use[i] < 0) free_slots.push(slot);  // never read: free after the write
+    }
+    return next;
 }
 
 }  // namespace
@@ -93,6 +139,8 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
     if (!programs || count == 0u || count > NXG_EXPR_MAX_PROGRAMS) return NXG_ERROR_INVALID_PROGRAM;
     uint64_t total = align16(sizeof(PackedSet));
     uint32_t max_values = 0, maps_required = 0;
+    std::vector<std::vector<uint16_t>> slots(count);
+    std::vector<uint32_t> slot_counts(count);
     for (uint32_t k = 0; k < count; ++k) {
         if (!validate_program(programs[k])) return NXG_ERROR_INVALID_PROGRAM;
         for (uint32_t i = 0; i < programs[k].instruction_count; ++i) {
@@ -100,7 +148,8 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
             if (in.op == NXG_OP_TEXEL && in.auxiliary + 1u > maps_required) maps_required = in.auxiliary + 1u;
         }
         total += program_bytes(programs[k]);
-        if (programs[k].instruction_count > max_values) max_values = programs[k].instruction_count;
+        slot_counts[k] = allocate_slots(programs[k], slots[k]);
+        if (slot_counts[k] > max_values) max_values = slot_counts[k];
     }
     if (!info) return NXG_ERROR_NULL_ARGUMENT;
     info->byte_size = total;
@@ -116,15 +165,17 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
     uint64_t at = align16(sizeof(PackedSet));
     for (uint32_t k = 0; k < count; ++k) {
         const nxg_expr_program &p = programs[k];
+        const std::vector<uint16_t> &slot_of = slots[k];
         set.program_offset[k] = static_cast<uint32_t>(at);
         PackedProgram pp{};
         pp.instruction_count = p.instruction_count;
-        pp.result = p.result;
+        pp.result = slot_of[p.result];
         pp.input_count = p.input_count;
+        pp.slot_count = slot_counts[k];
         for (uint32_t i = 0; i < p.input_count; ++i) pp.input_bindings[i] = p.input_bindings[i];
         uint64_t cursor = at + align16(sizeof(PackedProgram));
         pp.instructions_offset = static_cast<uint32_t>(cursor);
-        cursor += align16(uint64_t{p.instruction_count} * sizeof(PackedInstruction));
+        cursor += align16(uint64_t{p.instruction_count + 1u} * sizeof(PackedInstruction));
         pp.operands_offset = static_cast<uint32_t>(cursor);
         cursor += align16(uint64_t{p.operand_count} * 4u);
         pp.ramps_offset = static_cast<uint32_t>(cursor);
@@ -133,15 +184,22 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
         cursor += align16(uint64_t{p.ramp_point_count} * sizeof(nxg_expr_ramp_point));
         std::memcpy(base + at, &pp, sizeof(pp));
         PackedInstruction *ins = reinterpret_cast<PackedInstruction *>(base + pp.instructions_offset);
+        uint32_t *operand_slots = reinterpret_cast<uint32_t *>(base + pp.operands_offset);
         for (uint32_t i = 0; i < p.instruction_count; ++i) {
             const nxg_expr_instruction &in = p.instructions[i];
-            ins[i].op = in.op;
-            ins[i].operand_count = in.operand_count;
-            ins[i].operand_offset = in.operand_offset;
-            ins[i].auxiliary = in.op == NXG_OP_INPUT ? p.input_bindings[in.auxiliary] : in.auxiliary;
-            ins[i].immediate = in.immediate;
+            PackedInstruction &o = ins[i];
+            o.op = in.op;
+            o.operand_count = in.operand_count;
+            o.dest = slot_of[i];
+            o.operand_offset = in.operand_offset;
+            o.auxiliary = in.op == NXG_OP_INPUT ? p.input_bindings[in.auxiliary] : in.auxiliary;
+            o.immediate = in.immediate;
+            for (uint32_t j = 0; j < in.operand_count; ++j) {
+                const uint16_t s = slot_of[p.operands[in.operand_offset + j]];
+                if (in.operand_count <= 4u) o.slot[j] = s;
+                operand_slots[in.operand_offset + j] = s;
+            }
         }
-        if (p.operand_count) std::memcpy(base + pp.operands_offset, p.operands, uint64_t{p.operand_count} * 4u);
         if (p.ramp_count)
             std::memcpy(base + pp.ramps_offset, p.ramps, uint64_t{p.ramp_count} * sizeof(nxg_expr_ramp));
         if (p.ramp_point_count)

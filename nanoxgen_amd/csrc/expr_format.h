@@ -13,37 +13,47 @@ constexpr uint32_t kPackedMagic = 0x5045584eu;  // "NXEP"
 constexpr uint32_t kMaxInputs = 32u;
 constexpr uint32_t kMaxOperands = 16u;
 
+// LDS value slots: the root variables (NXG_ROOT_VAR_*) occupy slots
+// 0..NXG_ROOT_VAR_COUNT-1 for the whole set; every other SSA value gets a
+// slot from a liveness-based allocation made by the packer, so a slot is
+// reused once the value it held has no later reader.
+constexpr uint32_t kVarSlots = NXG_ROOT_VAR_COUNT;
+
 struct PackedSet {
     uint32_t magic;
     uint32_t program_count;
-    uint32_t max_values;  // largest instruction count of the set
+    uint32_t max_values;  // value slots per root the largest program needs (vars included)
     uint32_t byte_size;
     uint32_t program_offset[NXG_EXPR_MAX_PROGRAMS];
 };
 
 struct PackedProgram {
     uint32_t instruction_count;
-    uint32_t result;
+    uint32_t result;  // slot holding the result value
     uint32_t input_count;
-    uint32_t instructions_offset;
-    uint32_t operands_offset;
+    uint32_t instructions_offset;  // instruction_count + 1 entries (a zeroed one ends the array)
+    uint32_t operands_offset;      // slots of the operands of instructions with more than 4
     uint32_t ramps_offset;
     uint32_t ramp_points_offset;
-    uint32_t reserved;
+    uint32_t slot_count;
     uint32_t input_bindings[kMaxInputs];
 };
 
-// One instruction; for NXG_OP_INPUT the auxiliary already holds the bound
-// root variable (NXG_ROOT_VAR_*).
+// One instruction, 32 bytes (one scalar load).  For NXG_OP_INPUT the
+// auxiliary holds the bound root variable, which is also its slot.
 struct PackedInstruction {
     uint8_t op;
     uint8_t reserved;
     uint16_t operand_count;
-    uint32_t operand_offset;  // index into the program's operand array
+    uint16_t dest;            // slot the value is written to
+    uint16_t reserved2;
+    uint32_t operand_offset;  // operand_count > 4: index into the program's operand slots
     uint32_t auxiliary;
     float immediate;
+    uint16_t slot[4];         // operand_count <= 4: operand slots
+    uint32_t reserved3;
 };
-static_assert(sizeof(PackedInstruction) == 16, "packed instruction is 16 bytes");
+static_assert(sizeof(PackedInstruction) == 32, "packed instruction is 32 bytes");
 
 bool validate_program(const nxg_expr_program &p);
 

@@ -984,30 +984,32 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
                                             r.b1, r.b2, static_cast<float>(strand)};
     const float face_seed = hash_to_unit(hash_step(hash_step(hash_step(0u, 0.0f), 0.0f), static_cast<float>(r.tri)));
 
+    // the root variables live in their reserved slots for the whole set
+#pragma unroll
+    for (uint32_t q = 0; q < NXG_ROOT_VAR_COUNT; ++q) values[q * B + lane] = vars[q];
+
     float result[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     bool bad = false;
     for (uint32_t k = 0; k < a.program_count; ++k) {
         const PackedProgram *prog = reinterpret_cast<const PackedProgram *>(a.programs + a.program_offset[k]);
         const PackedInstruction *ins = reinterpret_cast<const PackedInstruction *>(a.programs + prog->instructions_offset);
-        const uint32_t *operands = reinterpret_cast<const uint32_t *>(a.programs + prog->operands_offset);
+        const uint32_t *operand_slots = reinterpret_cast<const uint32_t *>(a.programs + prog->operands_offset);
         const nxg_expr_ramp *ramps = reinterpret_cast<const nxg_expr_ramp *>(a.programs + prog->ramps_offset);
         const nxg_expr_ramp_point *points =
             reinterpret_cast<const nxg_expr_ramp_point *>(a.programs + prog->ramp_points_offset);
         const uint32_t n = prog->instruction_count;
+        // the next instruction is fetched while the current one runs (the
+        // array ends with a zeroed entry)
+        PackedInstruction next = ins[0];
         for (uint32_t i = 0; i < n; ++i) {
-            const PackedInstruction in = ins[i];
-            const uint32_t *ops = operands + in.operand_offset;
-#define OPV(j) values[ops[j] * B + lane]
+            const PackedInstruction in = next;
+            next = ins[i + 1u];
+#define OPV(j) values[uint32_t{in.slot[j]} * B + lane]
+#define OPA(j) values[operand_slots[in.operand_offset + (j)] * B + lane]
             float value = 0.0f;
             switch (in.op) {
                 case NXG_OP_CONSTANT: value = in.immediate; break;
-                case NXG_OP_INPUT: {
-                    value = vars[0];
-#pragma unroll
-                    for (uint32_t q = 1; q < NXG_ROOT_VAR_COUNT; ++q)
-                        if (in.auxiliary == q) value = vars[q];
-                    break;
-                }
+                case NXG_OP_INPUT: value = values[in.auxiliary * B + lane]; break;
                 case NXG_OP_NEGATE: value = -OPV(0); break;
                 case NXG_OP_LOGICAL_NOT: value = OPV(0) == 0.0f; break;
                 case NXG_OP_ADD: value = OPV(0) + OPV(1); break;
@@ -1025,22 +1027,22 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
                 case NXG_OP_SELECT: value = OPV(0) != 0.0f ? OPV(1) : OPV(2); break;
                 case NXG_OP_HASH: {
                     uint32_t state = 0u;
-                    for (uint32_t j = 0; j < in.operand_count; ++j) state = hash_step(state, OPV(j));
+                    for (uint32_t j = 0; j < in.operand_count; ++j) state = hash_step(state, OPA(j));
                     value = hash_to_unit(state);
                     break;
                 }
                 case NXG_OP_RANDOM: {
                     const bool explicit_seed = in.operand_count == 1u || in.operand_count == 3u;
                     uint32_t state = hash_step(hash_step(hash_step(hash_step(0u, r.u), r.v), face_seed), in.immediate);
-                    if (explicit_seed) state = hash_step(state, OPV(in.operand_count - 1u));
+                    if (explicit_seed) state = hash_step(state, OPA(in.operand_count - 1u));
                     value = hash_to_unit(state);
-                    if (in.operand_count >= 2u) value = (OPV(1) - OPV(0)) * value + OPV(0);
+                    if (in.operand_count >= 2u) value = (OPA(1) - OPA(0)) * value + OPA(0);
                     break;
                 }
                 case NXG_OP_MINIMUM: { const float x = OPV(0), y = OPV(1); value = y < x ? y : x; break; }
                 case NXG_OP_MAXIMUM: { const float x = OPV(0), y = OPV(1); value = x < y ? y : x; break; }
                 case NXG_OP_CLAMP: value = clamp_std(OPV(0), OPV(1), OPV(2)); break;
-                case NXG_OP_FIT: value = OPV(3) + (OPV(0) - OPV(1)) * (OPV(4) - OPV(3)) / (OPV(2) - OPV(1)); break;
+                case NXG_OP_FIT: value = OPA(3) + (OPA(0) - OPA(1)) * (OPA(4) - OPA(3)) / (OPA(2) - OPA(1)); break;
                 case NXG_OP_GAMMA: value = powf(OPV(0), 1.0f / OPV(1)); break;
                 case NXG_OP_CONTRAST: {
                     const float x = OPV(0);
@@ -1079,10 +1081,14 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
                 default: value = __builtin_nanf(""); break;
             }
 #undef OPV
+#undef OPA
             if (!isfinite(value)) bad = true;
-            values[i * B + lane] = value;
+            values[uint32_t{in.dest} * B + lane] = value;
         }
-        result[k] = values[prog->result * B + lane];
+        // constant indices keep result[] in registers
+        const float rv = values[prog->result * B + lane];
+#pragma unroll
+        for (uint32_t q = 0; q < NXG_EXPR_MAX_PROGRAMS; ++q) result[q] = q == k ? rv : result[q];
     }
     if (!live) return;
     if (bad) {
@@ -1094,7 +1100,9 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
         const float4 g = a.attributes[strand];
         const float guess[4] = {g.x, g.y, g.z, g.w};
         bool fail = bad;
-        for (uint32_t k = 0; k < a.program_count; ++k) {
+#pragma unroll
+        for (uint32_t k = 0; k < NXG_EXPR_MAX_PROGRAMS; ++k) {
+            if (k >= a.program_count) break;
             const float e = result[k], d = guess[k] - e;
             const float mag = (e < 0.0f ? -e : e) > 1.0e-6f ? (e < 0.0f ? -e : e) : 1.0e-6f;
             if (!((d < 0.0f ? -d : d) <= a.threshold * mag)) fail = true;
@@ -1273,7 +1281,7 @@ uint32_t make_attr_launch(const nxg_device_asset *asset, const nxg_generation_pa
     if (!set || !set->image || reinterpret_cast<uintptr_t>(set->image) % 16u != 0u ||
         set->byte_size < sizeof(PackedSet) || set->program_count == 0u ||
         set->program_count > NXG_EXPR_MAX_PROGRAMS || set->max_values == 0u ||
-        set->max_values > NXG_EXPR_MAX_VALUES)
+        set->max_values > NXG_EXPR_MAX_VALUES + kVarSlots)
         return NXG_ERROR_INVALID_PROGRAM;
     for (uint32_t k = 0; k < set->program_count; ++k)
         if (set->program_offset[k] < sizeof(PackedSet) || set->program_offset[k] >= set->byte_size)
@@ -1309,7 +1317,8 @@ uint32_t make_attr_launch(const nxg_device_asset *asset, const nxg_generation_pa
     a.threshold = threshold;
     a.tile_flags = tile_flags;
     a.stats = stats;
-    // values[max_values][block] floats plus the gradient table in 64 KB of LDS
+    // values[max_values][block] floats plus the gradient table, within 64 KB
+    // of LDS where a block of at least 64 lanes allows it
     uint32_t block = 256u;
     while (block > 64u && uint64_t{set->max_values} * block * 4u + 4096u > 65536u) block -= 64u;
     out.block = block;
@@ -1320,6 +1329,10 @@ uint32_t make_attr_launch(const nxg_device_asset *asset, const nxg_generation_pa
 
 uint32_t launch_attributes(const AttrLaunch &l, hipStream_t stream) {
     if (l.lanes == 0u) return NXG_OK;
+    if (l.lds > 65536u &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&attribute_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(l.lds)) != hipSuccess)
+        return NXG_ERROR_HIP;
     hipLaunchKernelGGL(attribute_kernel, dim3(static_cast<uint32_t>((l.lanes + l.block - 1u) / l.block)),
                        dim3(l.block), l.lds, stream, l.a);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;

@@ -163,8 +163,10 @@ def test_pack_layout_and_limits():
     assert L.nxg_expr_pack(arr, 3, buf.ctypes.data, buf.nbytes - 1, C.byref(info)) == abi.BUFFER_TOO_SMALL
     assert L.nxg_expr_pack(arr, 3, buf.ctypes.data, buf.nbytes, C.byref(info)) == abi.OK
     assert info.program_count == 3 and info.texel_maps_required == 1
-    assert info.max_values == max(len(q.instructions) for q in progs)
     assert buf[:4].view(np.uint32)[0] == 0x5045584E
+    # liveness-based slots: far fewer than one per SSA value
+    assert ROOT_VARS <= info.max_values < max(len(q.instructions) for q in progs)
+    check_slot_allocation(buf, progs)
     assert L.nxg_expr_pack(arr, 0, None, 0, C.byref(info)) == abi.INVALID_PROGRAM
     five = (abi.ExprProgram * 5)(*([progs[0].struct] * 5))
     assert L.nxg_expr_pack(five, 5, None, 0, C.byref(info)) == abi.INVALID_PROGRAM
@@ -190,3 +192,56 @@ def test_unit_attributes_reproduce_plain_generation(orc):
     a = orc.generate(blob, w.params)
     b = orc.generate(blob, w.params, attributes=np.ones((500, 4), np.float32))
     assert np.array_equal(a["points"].view(np.uint32), b["points"].view(np.uint32))
+
+
+ROOT_VARS = len(abi.ROOT_VARS)
+PACKED_INSTRUCTION = np.dtype([("op", "u1"), ("reserved", "u1"), ("operand_count", "<u2"), ("dest", "<u2"),
+                               ("reserved2", "<u2"), ("operand_offset", "<u4"), ("auxiliary", "<u4"),
+                               ("immediate", "<f4"), ("slot", "<u2", 4), ("reserved3", "<u4")])
+
+
+def check_slot_allocation(image, progs):
+    """Replays the packed instruction stream on value identities: every
+    operand slot must still hold the SSA value the program names when the
+    instruction reads it, and the result slot the result at the end (inputs
+    are their root variable's reserved slot)."""
+    assert PACKED_INSTRUCTION.itemsize == 32
+    header = image[:32].view(np.uint32)
+    slots = int(header[2])
+    for k, q in enumerate(progs):
+        off = int(header[4 + k])
+        pp = image[off:off + 160].view(np.uint32)
+        n, result, ins_off, opnd_off, slot_count = int(pp[0]), int(pp[1]), int(pp[3]), int(pp[4]), int(pp[7])
+        assert n == len(q.instructions) and slot_count <= slots
+        ins = image[ins_off:ins_off + 32 * (n + 1)].view(PACKED_INSTRUCTION)
+        assert ins[n]["op"] == 0 and ins[n]["operand_count"] == 0  # zeroed end entry
+        opnd = image[opnd_off:opnd_off + 4 * len(q.operands)].view(np.uint32)
+        bindings = q.bindings
+        ident = lambda v: ("var", int(bindings[q.instructions[v][3]])) if q.instructions[v][0] == abi.OP["input"] else v
+        held = {s: ("var", s) for s in range(ROOT_VARS)}
+        for i, (op, off_i, cnt, aux, imm) in enumerate(q.instructions):
+            for j in range(cnt):
+                v = q.operands[off_i + j]
+                slot = int(ins[i]["slot"][j]) if cnt <= 4 else int(opnd[ins[i]["operand_offset"] + j])
+                assert slot == int(opnd[off_i + j]) and slot < slot_count
+                assert held.get(slot) == ident(v), (k, i, j)
+            dest = int(ins[i]["dest"])
+            assert dest < slot_count and (op != abi.OP["input"] or dest == int(bindings[aux]))
+            assert dest >= ROOT_VARS or op == abi.OP["input"]
+            held[dest] = ident(i)
+        assert held.get(result) == ident(q.result)
+
+
+def test_slot_allocation_of_reference_programs(fx):
+    """The 400 recorded reference programs (hashes of up to 16 operands, fit,
+    select, ramps) packed four at a time: every operand read finds its value."""
+    progs = fixture_programs(fx)
+    L = nx.lib()
+    for start in range(0, len(progs), 4):
+        chunk = progs[start:start + 4]
+        arr = (abi.ExprProgram * len(chunk))(*[q.struct for q in chunk])
+        info = abi.ExprSet()
+        assert L.nxg_expr_pack(arr, len(chunk), None, 0, C.byref(info)) == abi.OK
+        buf = np.zeros(info.byte_size, np.uint8)
+        assert L.nxg_expr_pack(arr, len(chunk), buf.ctypes.data, buf.nbytes, C.byref(info)) == abi.OK
+        check_slot_allocation(buf, chunk)
