@@ -540,6 +540,45 @@ uint32_t nxg_verify_attributes(const nxg_device_asset *asset, const nxg_generati
                                uint32_t *tile_flags, uint64_t tile_flag_capacity, uint32_t *stats,
                                void *stream);
 
+/* ---- motion samples ------------------------------------------------------ */
+
+/* DeviceMotionSampleDescriptor (device_contract.h:42-45): one deformation of
+ * the asset at one time. */
+typedef struct nxg_motion_sample {
+    nxg_deformed_geometry deformation;
+    float time;
+} nxg_motion_sample;
+
+/* DeviceMotionOutputDescriptor (device_contract.h:47-52): sample-major
+ * absolute positions, strand_count * cvs_per_strand per sample; capacity in
+ * elements. */
+typedef struct nxg_motion_output {
+    nxg_vec3 *points;
+    uint64_t point_capacity;
+} nxg_motion_output;
+
+/* validate_device_motion_generation_request (device_contract.h:256-288):
+ * samples present, times finite and strictly increasing, each sample's
+ * deformation valid for the asset, output large enough. */
+uint32_t nxg_validate_motion_generation(const nxg_device_asset *asset, const nxg_motion_sample *samples,
+                                        uint32_t sample_count, const nxg_generation_params *params,
+                                        const nxg_motion_output *output, const nxg_launch_config *config);
+
+/* Workspace of nxg_generate_motion: sample_count packed-generation workspaces. */
+uint64_t nxg_motion_generation_workspace_size(const nxg_asset_header *header, const nxg_generation_params *params,
+                                              uint32_t sample_count);
+
+/* The positions of generate_packed under each sample's deformation
+ * (generate.h:612-642 with the deformed frame, generate.h:137-187): every
+ * sample draws the same roots (triangle and barycentrics); positions are
+ * those nxg_generate_packed writes for that deformation.  Two kernels per
+ * sample on `stream`; allocates nothing. */
+uint32_t nxg_generate_motion(const nxg_device_asset *asset, const nxg_motion_sample *samples,
+                             uint32_t sample_count, const nxg_generation_params *params,
+                             const nxg_noise_octaves *octaves, const nxg_motion_output *output,
+                             const nxg_launch_config *config, void *workspace, uint64_t workspace_bytes,
+                             void *stream);
+
 /* ---- batched descriptions ------------------------------------------------
  * Many descriptions generated by one persistent launch.  The reference runs
  * a collection as one chain of dispatches per description

@@ -51,6 +51,9 @@ SYMBOLS = [
     ("nxg_surrogate_attributes", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_verify_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, C.c_float, C.c_uint32,
                                            _P, C.c_uint64, _P, _P]),
+    ("nxg_validate_motion_generation", C.c_uint32, [_P, _P, C.c_uint32, _P, _P, _P]),
+    ("nxg_motion_generation_workspace_size", C.c_uint64, [_P, _P, C.c_uint32]),
+    ("nxg_generate_motion", C.c_uint32, [_P, _P, C.c_uint32, _P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_batch_default_tuning", None, [_P]),
     ("nxg_batch_validate", C.c_uint32, [_P, C.c_uint32, _P]),
     ("nxg_generate_batch", C.c_uint32, [_P, C.c_uint32, _P, _P, _P]),
@@ -236,6 +239,22 @@ def generate_packed(asset, params, outputs, *, octaves=None, deformed=None, work
             _stream_handle(stream))
     check(status, "nxg_generate_packed")
     return outputs
+
+
+def generate_motion(asset, params, samples, points, *, octaves=None, workspace=None, stream=None):
+    """Enqueue motion-sample generation: `samples` is a list of
+    (time, DeformedGeometry); `points` a float32 [S * N * C, 3] device tensor
+    receiving sample-major positions."""
+    torch = _torch()
+    arr = (abi.MotionSample * len(samples))(*[abi.MotionSample(deformation=d, time=t) for t, d in samples])
+    out = abi.MotionOutput(points=points.data_ptr(), point_capacity=points.shape[0])
+    if workspace is None:
+        n = lib().nxg_motion_generation_workspace_size(C.byref(asset.desc.header), C.byref(params), len(samples))
+        workspace = torch.empty(max(1, n), dtype=torch.uint8, device=asset.tensor.device)
+    check(lib().nxg_generate_motion(C.byref(asset.desc), arr, len(samples), C.byref(params), _ref(octaves),
+                                    C.byref(out), None, C.c_void_p(workspace.data_ptr()), workspace.numel(),
+                                    _stream_handle(stream)), "nxg_generate_motion")
+    return points
 
 
 def prepare_packed(asset, params, workspace, *, deformed=None, stream=None):

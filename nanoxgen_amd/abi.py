@@ -240,6 +240,16 @@ assert GUIDE_DTYPE.itemsize == 56
 assert C.sizeof(ExprInstruction) == 16
 
 # status codes (device_contract.h:58-80 then extensions)
+class MotionSample(C.Structure):
+    """nxg_motion_sample (DeviceMotionSampleDescriptor)."""
+    _fields_ = [("deformation", DeformedGeometry), ("time", C.c_float)]
+
+
+class MotionOutput(C.Structure):
+    """nxg_motion_output (DeviceMotionOutputDescriptor): sample-major Vec3 points."""
+    _fields_ = [("points", C.c_void_p), ("point_capacity", C.c_uint64)]
+
+
 BATCH_MAX_ITEMS = 12
 BATCH_ORDER_GIVEN, BATCH_ORDER_HEAVY_FIRST = 0, 1
 

@@ -170,4 +170,26 @@ uint64_t nxg_packed_generation_workspace_size(const nxg_asset_header *header,
     return nxg::workspace_layout(header->guide_count, params->cvs_per_strand).total;
 }
 
+uint32_t nxg_validate_motion_generation(const nxg_device_asset *asset, const nxg_motion_sample *samples,
+                                        uint32_t sample_count, const nxg_generation_params *params,
+                                        const nxg_motion_output *output, const nxg_launch_config *config) {
+    if (!samples || sample_count == 0u) return NXG_ERROR_MISSING_MOTION_SAMPLES;
+    float previous = -std::numeric_limits<float>::infinity();
+    for (uint32_t s = 0; s < sample_count; ++s) {
+        if (!std::isfinite(samples[s].time) || samples[s].time <= previous) return NXG_ERROR_INVALID_MOTION_TIMES;
+        previous = samples[s].time;
+        if (uint32_t e = nxg::validate_common(asset, &samples[s].deformation, params, config); e != NXG_OK) return e;
+    }
+    const uint64_t per_sample = uint64_t{params->strand_count} * params->cvs_per_strand;
+    if (per_sample > std::numeric_limits<uint64_t>::max() / sample_count) return NXG_ERROR_OUTPUT_SIZE_OVERFLOW;
+    if (!output || !output->points || output->point_capacity < per_sample * sample_count)
+        return NXG_ERROR_MOTION_POINT_CAPACITY_TOO_SMALL;
+    return NXG_OK;
+}
+
+uint64_t nxg_motion_generation_workspace_size(const nxg_asset_header *header, const nxg_generation_params *params,
+                                              uint32_t sample_count) {
+    return nxg_packed_generation_workspace_size(header, params) * sample_count;
+}
+
 }  // extern "C"

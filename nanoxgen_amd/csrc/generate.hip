@@ -241,6 +241,7 @@ struct Args {
     const float4 *attributes;  // optional per-root multipliers (NXG_ATTR_*)
     uint32_t strand_count, cvs, seed, triangle_count, guide_count, has_normals;
     uint32_t noise, octave_count;  // runtime form of the template switches (batched path)
+    uint32_t vec3_points;          // positions only, 3 floats per point (motion samples)
     float support_scale, weight_power, rejection_cos, length_scale;
     float root_width, tip_width, radius_scale;
     float noise_mask, decorrelation_scale, preserve, inv_span;
@@ -393,8 +394,9 @@ __device__ __forceinline__ V3 root_surface_u(const Args &a, const Root &r) {
 
 struct Cursor {
     uint32_t cv;
-    V3 prev, cur, next, tangent, tn;
+    V3 cur, next, tangent, tn;
     float arc;
+    float step;  // |base(cv) - base(cv - 1)|, added to arc when cv is reached
 };
 
 template <int OCT>
@@ -410,11 +412,13 @@ struct NoiseState {
 template <int OCT, bool RT = false>
 __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend &b,
                                       NoiseState<OCT> &ns, Cursor &c, const float4 *grad) {
-    if (c.cv > 0u) c.arc += fast_sqrt(fma3(c.cur - c.prev, c.cur - c.prev));
+    if (c.cv > 0u) c.arc += c.step;
     V3 nt = c.tangent;
     if (c.cv + 1u < a.cvs) {
         const V3 seg = c.next - c.cur;
-        if (fma3(seg, seg) > 0.0f) nt = normalize_fast(seg);
+        const float seg2 = fma3(seg, seg);
+        c.step = fast_sqrt(seg2);
+        if (seg2 > 0.0f) nt = normalize_fast(seg);
     }
     const V3 axis = cross_fast(c.tangent, nt);
     if (fma3(axis, axis) > 0.0f) {
@@ -440,7 +444,6 @@ __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend 
         point = fma_v(tangent, lz, fma_v(binormal, ly, fma_v(normal, lx, point)));
     }
     c.tangent = nt;
-    c.prev = c.cur;
     if (c.cv + 1u < a.cvs) {
         c.cur = c.next;
         if (c.cv + 2u < a.cvs) c.next = base_cv(a, r, b, c.cv + 2u);
@@ -454,8 +457,8 @@ __device__ __forceinline__ Cursor make_cursor(const Args &a, const Root &r, cons
     Cursor c;
     c.cv = 0u;
     c.cur = base_cv(a, r, b, 0u);
-    c.prev = c.cur;
     c.next = base_cv(a, r, b, 1u);
+    c.step = 0.0f;
     c.tangent = r.normal;
     c.tn = surface_u;
     c.arc = 0.0f;
@@ -559,6 +562,25 @@ __device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, u
             p.w = stage[(k * 4u + 3u) * kStageStride + s];
             reinterpret_cast<float4 *>(a.points)[strand * a.cvs + cv0 + k] = p;
         }
+    }
+}
+
+// The same chunk as positions only (motion-sample output): 3 floats per
+// point, stored as consecutive dwords of the strand-major chunk.
+__device__ __forceinline__ void flush_chunk_vec3(const Args &a, const float *stage, uint64_t wave_first,
+                                                 uint32_t cv0, uint32_t n, uint32_t lane) {
+    float *out = reinterpret_cast<float *>(a.points);
+    const uint32_t per = 3u * n;
+    // f / per for f < 64 * per, per <= 48, as in flush_chunk
+    const float inv_per = 1.0f / static_cast<float>(per);
+#pragma unroll 1
+    for (uint32_t it = 0; it < per; ++it) {
+        const uint32_t f = it * kWave + lane;
+        const uint32_t s = static_cast<uint32_t>((static_cast<float>(f) + 0.5f) * inv_per);
+        const uint32_t rem = f - s * per;
+        const uint32_t k = rem / 3u, c = rem - 3u * k;
+        const uint64_t strand = wave_first + s;
+        if (strand < a.strand_count) out[(strand * a.cvs + cv0 + k) * 3u + c] = stage[(k * 4u + c) * kStageStride + s];
     }
 }
 
@@ -703,7 +725,8 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
             }
             if (emit) {
                 wave_sync();
-                flush_chunk<0>(a, stage, wave_first, cv0, n, lane);
+                if (a.vec3_points) flush_chunk_vec3(a, stage, wave_first, cv0, n, lane);
+                else flush_chunk<0>(a, stage, wave_first, cv0, n, lane);
                 wave_sync();
             }
         }
@@ -1234,18 +1257,7 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
     return NXG_OK;
 }
 
-uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
-                const nxg_generation_params *params, const nxg_noise_octaves *octaves,
-                const nxg_root_sample *explicit_roots, const float *attributes,
-                uint64_t attribute_capacity, const nxg_packed_output *output,
-                const nxg_stencil_output *stencil, const nxg_launch_config *config, void *workspace,
-                uint64_t workspace_bytes, void *stream_ptr, uint32_t stages) {
-    Plan plan;
-    if (uint32_t s = make_plan(asset, deformed, params, octaves, explicit_roots, attributes, attribute_capacity,
-                               output, stencil, config, workspace, workspace_bytes, stages, plan);
-        s != NXG_OK)
-        return s;
-    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+uint32_t launch_plan(const Plan &plan, const nxg_generation_params &params, hipStream_t stream, uint32_t stages) {
     if (stages & kPrepare) {
         hipLaunchKernelGGL(guide_prep_kernel, dim3(static_cast<uint32_t>((plan.prep_lanes + 255u) / 256u)),
                            dim3(256), 0, stream, plan.p);
@@ -1253,7 +1265,7 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
     }
     if (!(stages & kStrands)) return NXG_OK;
     const Args &a = plan.a;
-    const dim3 grid((params->strand_count + 255u) / 256u), block(256);
+    const dim3 grid((params.strand_count + 255u) / 256u), block(256);
     if (!plan.noise) {
         hipLaunchKernelGGL((strand_kernel<false, 1>), grid, block, 0, stream, a);
     } else {
@@ -1265,6 +1277,52 @@ uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *defo
         }
     }
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
+uint32_t launch(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                const nxg_root_sample *explicit_roots, const float *attributes,
+                uint64_t attribute_capacity, const nxg_packed_output *output,
+                const nxg_stencil_output *stencil, const nxg_launch_config *config, void *workspace,
+                uint64_t workspace_bytes, void *stream_ptr, uint32_t stages) {
+    Plan plan;
+    if (uint32_t s = make_plan(asset, deformed, params, octaves, explicit_roots, attributes, attribute_capacity,
+                               output, stencil, config, workspace, workspace_bytes, stages, plan);
+        s != NXG_OK)
+        return s;
+    return launch_plan(plan, *params, static_cast<hipStream_t>(stream_ptr), stages);
+}
+
+// Motion samples: every sample is planned (and validated) before the first
+// launch; each writes its positions at sample * strand_count * cvs.
+uint32_t launch_motion(const nxg_device_asset *asset, const nxg_motion_sample *samples, uint32_t count,
+                       const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                       const nxg_motion_output *output, const nxg_launch_config *config, void *workspace,
+                       uint64_t workspace_bytes, hipStream_t stream) {
+    if (uint32_t s = nxg_validate_motion_generation(asset, samples, count, params, output, config); s != NXG_OK)
+        return s;
+    if (!valid_octaves(octaves)) return NXG_ERROR_INVALID_OCTAVES;
+    const uint64_t one = workspace_layout(asset->header.guide_count, params->cvs_per_strand).total;
+    if (!workspace || workspace_bytes < one * count) return NXG_ERROR_WORKSPACE_TOO_SMALL;
+    const uint64_t per_sample = uint64_t{params->strand_count} * params->cvs_per_strand;
+    for (uint32_t pass = 0; pass < 2u; ++pass) {  // pass 0 validates every sample, pass 1 launches
+        for (uint32_t k = 0; k < count; ++k) {
+            nxg_packed_output out{};
+            out.points = reinterpret_cast<nxg_packed_point *>(output->points + per_sample * k);
+            out.point_capacity = per_sample;
+            out.radius_scale = 1.0f;
+            Plan plan;
+            if (uint32_t s = make_plan(asset, &samples[k].deformation, params, octaves, nullptr, nullptr, 0u, &out,
+                                       nullptr, config, static_cast<uint8_t *>(workspace) + one * k, one,
+                                       kPrepare | kStrands, plan);
+                s != NXG_OK)
+                return s;
+            plan.a.vec3_points = 1u;
+            if (pass == 1u)
+                if (uint32_t s = launch_plan(plan, *params, stream, kPrepare | kStrands); s != NXG_OK) return s;
+        }
+    }
+    return NXG_OK;
 }
 
 struct AttrLaunch {
@@ -1696,6 +1754,15 @@ uint32_t nxg_batch_autotune(const nxg_batch_item *items, uint32_t count, uint32_
     (void)hipEventDestroy(stop);
     if (status == NXG_OK && !any) status = NXG_ERROR_INVALID_BATCH;
     return status;
+}
+
+uint32_t nxg_generate_motion(const nxg_device_asset *asset, const nxg_motion_sample *samples,
+                             uint32_t sample_count, const nxg_generation_params *params,
+                             const nxg_noise_octaves *octaves, const nxg_motion_output *output,
+                             const nxg_launch_config *config, void *workspace, uint64_t workspace_bytes,
+                             void *stream) {
+    return nxg::launch_motion(asset, samples, sample_count, params, octaves, output, config, workspace,
+                              workspace_bytes, static_cast<hipStream_t>(stream));
 }
 
 const char *nxg_build_info(void) {

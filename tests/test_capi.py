@@ -254,3 +254,36 @@ def test_batch_default_tuning_is_a_candidate():
     d = batch.default_tuning()
     assert d.as_tuple() in [c.as_tuple() for c in batch.tuning_candidates()]
     assert len(batch.tuning_candidates()) == nx.lib().nxg_batch_tuning_candidates(None, 0) == 32
+
+
+# ---- motion samples (device_contract.h:256-288; tests/test_main.cpp:504-527)
+
+def test_motion_contract():
+    r = Request()
+    L = nx.lib()
+    n = 7 * 5
+    pts = np.zeros((2 * n, 3), np.float32)
+    out = abi.MotionOutput(points=pts.ctypes.data, point_capacity=2 * n)
+    samples = (abi.MotionSample * 2)(abi.MotionSample(time=0.25), abi.MotionSample(time=0.75))
+    v = lambda s, c, o: L.nxg_validate_motion_generation(C.byref(r.desc), s, c, C.byref(r.params), C.byref(o), None)
+    assert v(samples, 2, out) == abi.OK
+    same = (abi.MotionSample * 2)(abi.MotionSample(time=0.25), abi.MotionSample(time=0.25))
+    assert v(same, 2, out) == abi.INVALID_MOTION_TIMES
+    nan = (abi.MotionSample * 1)(abi.MotionSample(time=float("nan")))
+    assert v(nan, 1, out) == abi.INVALID_MOTION_TIMES
+    assert v(None, 0, out) == abi.MISSING_MOTION_SAMPLES
+    assert v(samples, 0, out) == abi.MISSING_MOTION_SAMPLES
+    short = abi.MotionOutput(points=pts.ctypes.data, point_capacity=2 * n - 1)
+    assert v(samples, 2, short) == abi.MOTION_POINT_CAPACITY_TOO_SMALL
+    bad = (abi.MotionSample * 2)(abi.MotionSample(time=0.25), abi.MotionSample(time=0.75))
+    bad[1].deformation.positions = r.blob.ctypes.data
+    bad[1].deformation.position_count = r.desc.header.vertex_count - 1
+    assert v(bad, 2, out) == abi.DEFORMED_POSITION_COUNT_MISMATCH
+    # generation refuses on the host, before any device work
+    ws = np.zeros(16, np.uint8)
+    assert L.nxg_generate_motion(C.byref(r.desc), samples, 2, C.byref(r.params), None, C.byref(out), None,
+                                 ws.ctypes.data, ws.nbytes, None) == abi.WORKSPACE_TOO_SMALL
+    assert L.nxg_generate_motion(C.byref(r.desc), same, 2, C.byref(r.params), None, C.byref(out), None,
+                                 ws.ctypes.data, ws.nbytes, None) == abi.INVALID_MOTION_TIMES
+    one = L.nxg_packed_generation_workspace_size(C.byref(r.desc.header), C.byref(r.params))
+    assert L.nxg_motion_generation_workspace_size(C.byref(r.desc.header), C.byref(r.params), 3) == 3 * one

@@ -182,3 +182,42 @@ def test_config2_full_size_properties(torch, orc):
                point_counts=out.point_counts.cpu().numpy().view(np.uint32)[sample],
                stencil_indices=out.stencil_indices.cpu().numpy().view(np.uint32)[sample])
     compare(gpu, ref, orc.generate(blob, w.params, octaves=w.octaves, strands=sample, precision=64), "config2 sample")
+
+
+def test_motion_samples(torch, orc, golden):
+    """Motion samples (device_contract.h:42-52, 256-288): sample k's positions
+    equal generate_packed under sample k's deformation, bit for bit, and the
+    oracle's deformed generation within tolerance; the rest pose (no
+    deformation) is a valid sample too."""
+    blob = aligned_copy(golden["plane_blob"])
+    p = params_from(golden["plane_params"])
+    h = abi.header_of(blob)
+    nrm = blob[h.normals_offset:h.normals_offset + 12 * h.vertex_count].view(np.float32).reshape(-1, 3)
+    gcv = blob[h.guide_cvs_offset:h.guide_cvs_offset + 12 * h.guide_cv_count].view(np.float32).reshape(-1, 3)
+    dpos = golden["plane_deformed_positions"]
+    shift = np.array([2.0, 3.0, -4.0], dtype=np.float32)
+    poses = [None, dict(positions=dpos, normals=nrm, guide_cvs=gcv + shift), dict(positions=dpos)]
+    keep, samples = [], []
+    for t, d in zip((0.0, 0.5, 1.0), poses):
+        dd = abi.DeformedGeometry()
+        for name, cnt in (("positions", "position_count"), ("normals", "normal_count"), ("guide_cvs", "guide_cv_count")):
+            if d is not None and d.get(name) is not None:
+                x = torch.from_numpy(np.ascontiguousarray(d[name], dtype=np.float32)).cuda()
+                keep.append(x)
+                setattr(dd, name, x.data_ptr())
+                setattr(dd, cnt, x.shape[0])
+        samples.append((t, dd))
+    asset = nx.DeviceAsset(blob)
+    N, Cv = p.strand_count, p.cvs_per_strand
+    pts = torch.full((3 * N * Cv, 3), float("nan"), device="cuda")
+    nx.generate_motion(asset, p, samples, pts)
+    torch.cuda.synchronize()
+    got = pts.cpu().numpy().reshape(3, N * Cv, 3)
+    for k, d in enumerate(poses):
+        single = run_gpu(blob, p, deformed=d)
+        assert np.array_equal(got[k].view(np.uint32), single["points"][:, :3].view(np.uint32)), f"sample {k}"
+        ref = orc.generate(blob, p, deformed=d)
+        diag = float(np.linalg.norm(ref["points"][:, :3].max(0).astype(np.float64) - ref["points"][:, :3].min(0)))
+        err = np.abs(got[k].astype(np.float64) - ref["points"][:, :3]).max()
+        print(f"motion sample {k}: max |gpu - oracle| = {err:.3e} (tol {POSITION_TOL * diag:.3e})")
+        assert err <= POSITION_TOL * diag
