@@ -255,6 +255,17 @@ uint32_t nxg_asset_build_size(const nxg_asset_build_input *input, uint64_t *byte
 uint32_t nxg_asset_build(const nxg_asset_build_input *input, void *out, uint64_t capacity,
                          uint64_t *byte_size, char *err, uint64_t err_capacity);
 
+/* build_asset (asset.cpp:75-293) with its O(triangles x guides) stencil
+ * search (asset.cpp:252-269) run on the GPU: the host writes the blob into
+ * `out` (as nxg_asset_build), copies it to `device_out` (64-byte aligned,
+ * device_capacity bytes, caller-owned), the stencil kernel fills the
+ * device copy's stencil section, and the section and the final content
+ * hash come back so that both copies are the blob nxg_asset_build writes,
+ * byte for byte.  Synchronises `stream`. */
+uint32_t nxg_asset_build_gpu(const nxg_asset_build_input *input, void *out, uint64_t capacity, uint64_t *byte_size,
+                             char *err, uint64_t err_capacity, void *device_out, uint64_t device_capacity,
+                             void *stream);
+
 /* validate_asset (asset.cpp:295-338): writes "" or the reference's message
  * into msg; returns NXG_OK or NXG_ERROR_INVALID_ASSET. */
 uint32_t nxg_asset_validate(const void *blob, uint64_t byte_size, char *msg, uint64_t msg_capacity);

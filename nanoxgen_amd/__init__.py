@@ -30,6 +30,7 @@ SYMBOLS = [
     ("nxg_status_message", C.c_char_p, [C.c_uint32]),
     ("nxg_asset_build_size", C.c_uint32, [_P, _P]),
     ("nxg_asset_build", C.c_uint32, [_P, _P, C.c_uint64, _P, C.c_char_p, C.c_uint64]),
+    ("nxg_asset_build_gpu", C.c_uint32, [_P, _P, C.c_uint64, _P, C.c_char_p, C.c_uint64, _P, C.c_uint64, _P]),
     ("nxg_asset_validate", C.c_uint32, [_P, C.c_uint64, C.c_char_p, C.c_uint64]),
     ("nxg_make_device_asset", C.c_uint32, [_P, C.c_uint64, _P, C.c_uint64, _P]),
     ("nxg_validate_packed_generation", C.c_uint32, [_P, _P, _P, _P, _P]),
@@ -111,6 +112,26 @@ def build_asset(build_input):
     return blob
 
 
+def build_asset_gpu(build_input, device="cuda", stream=None):
+    """build_asset with the stencil search on the GPU (nxg_asset_build_gpu):
+    returns (host blob, DeviceAsset of the device copy); both are the bytes
+    build_asset writes."""
+    torch = _torch()
+    L = lib()
+    size = C.c_uint64()
+    check(L.nxg_asset_build_size(C.byref(build_input.struct), C.byref(size)), "nxg_asset_build_size")
+    blob = abi.aligned_bytes(size.value)
+    raw = torch.empty(size.value + 64, dtype=torch.uint8, device=device)
+    off = (-raw.data_ptr()) % 64
+    dev = raw[off:off + size.value]
+    err = C.create_string_buffer(512)
+    st = L.nxg_asset_build_gpu(C.byref(build_input.struct), blob.ctypes.data, blob.nbytes, None, err, 512,
+                               C.c_void_p(dev.data_ptr()), dev.numel(), _stream_handle(stream))
+    if st != 0:
+        raise ValueError(err.value.decode() or L.nxg_status_message(st).decode())
+    return blob, DeviceAsset(blob, device=device, uploaded=(raw, dev))
+
+
 def validate_asset(blob):
     """validate_asset (asset.cpp:295-338): "" when valid, else the message."""
     msg = C.create_string_buffer(512)
@@ -134,19 +155,22 @@ class DeviceAsset:
     """A blob uploaded to device memory with its checked descriptor
     (make_device_asset_descriptor, asset.cpp:364-378)."""
 
-    def __init__(self, blob, device="cuda"):
+    def __init__(self, blob, device="cuda", uploaded=None):
         torch = _torch()
         msg = validate_asset(blob)
         if msg:
             raise ValueError(f"invalid asset: {msg}")
         self.blob = blob
         self.header = abi.header_of(blob)
-        # over-allocate so the base can be moved to a 64-byte boundary
-        raw = torch.empty(blob.nbytes + 64, dtype=torch.uint8, device=device)
-        off = (-raw.data_ptr()) % 64
-        self.storage = raw
-        self.tensor = raw[off:off + blob.nbytes]
-        self.tensor.copy_(torch.from_numpy(blob))
+        if uploaded is not None:  # (storage, 64-byte aligned view) already holding the blob
+            self.storage, self.tensor = uploaded
+        else:
+            # over-allocate so the base can be moved to a 64-byte boundary
+            raw = torch.empty(blob.nbytes + 64, dtype=torch.uint8, device=device)
+            off = (-raw.data_ptr()) % 64
+            self.storage = raw
+            self.tensor = raw[off:off + blob.nbytes]
+            self.tensor.copy_(torch.from_numpy(blob))
         self.desc = abi.DeviceAsset()
         check(lib().nxg_make_device_asset(blob.ctypes.data, blob.nbytes, self.tensor.data_ptr(),
                                           self.tensor.numel(), C.byref(self.desc)),

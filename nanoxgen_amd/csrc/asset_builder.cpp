@@ -283,6 +283,15 @@ uint32_t nxg_asset_build_size(const nxg_asset_build_input *input, uint64_t *byte
 
 uint32_t nxg_asset_build(const nxg_asset_build_input *input, void *out, uint64_t capacity,
                          uint64_t *byte_size, char *err, uint64_t err_capacity) {
+    return nxg::build_blob(input, out, capacity, byte_size, err, err_capacity, true);
+}
+
+}  // extern "C"
+
+// The whole build with host_stencil; without it the stencil section stays
+// zero and the content hash is left for the caller to set (finish_blob).
+uint32_t nxg::build_blob(const nxg_asset_build_input *input, void *out, uint64_t capacity, uint64_t *byte_size,
+                         char *err, uint64_t err_capacity, bool host_stencil) {
     set_err(err, err_capacity, "");
     if (!input || !out) return NXG_ERROR_NULL_ARGUMENT;
     const nxg_asset_build_input &in = *input;
@@ -364,13 +373,25 @@ uint32_t nxg_asset_build(const nxg_asset_build_input *input, void *out, uint64_t
     if (in.texcoords) std::memcpy(blob + h.texcoords_offset, in.texcoords, std::size_t{V} * 8u);
     std::memcpy(blob + h.triangles_offset, in.triangles, std::size_t{T} * 12u);
     std::memcpy(blob + h.guide_cvs_offset, in.guide_cvs, std::size_t(L.total_cvs) * 12u);
-    build_stencil(in, guides, in.stencil_k == 0u ? NXG_GUIDE_STENCIL_SIZE : in.stencil_k,
-                  reinterpret_cast<uint32_t *>(blob + h.triangle_guides_offset));
+    if (host_stencil)
+        build_stencil(in, guides, in.stencil_k == 0u ? NXG_GUIDE_STENCIL_SIZE : in.stencil_k,
+                      reinterpret_cast<uint32_t *>(blob + h.triangle_guides_offset));
     std::memcpy(blob + h.noise_gradients_offset, kGradients, sizeof(kGradients));
-    h.content_hash = fnv1a(blob + kHeader, h.byte_size - kHeader);
     std::memcpy(blob, &h, sizeof(h));
+    if (host_stencil) finish_blob(blob);
     return NXG_OK;
 }
+
+// content_hash (asset.cpp:34-41, 288): FNV-1a over every byte after the header
+void nxg::finish_blob(void *out) {
+    uint8_t *blob = static_cast<uint8_t *>(out);
+    nxg_asset_header h;
+    std::memcpy(&h, blob, sizeof(h));
+    h.content_hash = fnv1a(blob + kHeader, h.byte_size - kHeader);
+    std::memcpy(blob, &h, sizeof(h));
+}
+
+extern "C" {
 
 uint32_t nxg_asset_validate(const void *blob, uint64_t byte_size, char *msg, uint64_t msg_capacity) {
     set_err(msg, msg_capacity, "");

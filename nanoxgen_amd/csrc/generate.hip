@@ -792,7 +792,7 @@ struct BatchArgs {
 // wavefront leaves once its claim starts past the last tile); each tile's
 // description comes from a binary search of the tile scan held in LDS.
 template <int WAVES, int CHUNK, int OCTMAX>
-__global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 ? 3 : 2)) batch_kernel(BatchArgs b) {
+__global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 && CHUNK < 16 ? 3 : 2)) batch_kernel(BatchArgs b) {
     __shared__ float4 grad_all[WAVES][NXG_NOISE_GRADIENT_COUNT];
     __shared__ float stage_all[WAVES][CHUNK * 4 * kStageStride];
     __shared__ uint32_t scan[NXG_BATCH_MAX_ITEMS];

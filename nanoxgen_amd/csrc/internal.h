@@ -28,4 +28,11 @@ struct WorkspaceLayout {
 };
 WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand);
 
+// Asset builder (asset_builder.cpp).  build_blob with host_stencil = false
+// leaves the triangle stencil zeroed and the content hash unset; finish_blob
+// sets the hash once the stencil is in place.
+uint32_t build_blob(const nxg_asset_build_input *input, void *out, uint64_t capacity, uint64_t *byte_size,
+                    char *err, uint64_t err_capacity, bool host_stencil);
+void finish_blob(void *blob);
+
 }  // namespace nxg
