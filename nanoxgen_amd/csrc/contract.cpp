@@ -95,11 +95,17 @@ uint32_t validate_packed_output(const nxg_generation_params &p, const nxg_packed
     return NXG_OK;
 }
 
-WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand) {
+WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand, uint32_t strand_count,
+                                 uint32_t triangle_count) {
+    (void)triangle_count;
     WorkspaceLayout l{};
     l.guide_state_offset = 0;
     l.guide_offsets_offset = align256(uint64_t{guide_count} * 32u);
-    l.total = align256(l.guide_offsets_offset + uint64_t{guide_count} * cvs_per_strand * 16u);
+    l.order_offset = align256(l.guide_offsets_offset + uint64_t{guide_count} * cvs_per_strand * 16u);
+    l.counts_offset = align256(l.order_offset + uint64_t{strand_count} * 4u);
+    const uint64_t counts = uint64_t{kOrderBins} * order_tiles(strand_count);
+    l.partials_offset = align256(l.counts_offset + counts * 4u);
+    l.total = align256(l.partials_offset + (counts / kScanBlock + 1u) * 4u);
     return l;
 }
 
@@ -167,7 +173,9 @@ uint32_t nxg_validate_packed_generation(const nxg_device_asset *asset,
 uint64_t nxg_packed_generation_workspace_size(const nxg_asset_header *header,
                                               const nxg_generation_params *params) {
     if (!header || !params) return 0;
-    return nxg::workspace_layout(header->guide_count, params->cvs_per_strand).total;
+    return nxg::workspace_layout(header->guide_count, params->cvs_per_strand, params->strand_count,
+                                 header->triangle_count)
+        .total;
 }
 
 uint32_t nxg_validate_motion_generation(const nxg_device_asset *asset, const nxg_motion_sample *samples,

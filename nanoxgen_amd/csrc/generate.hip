@@ -242,6 +242,7 @@ struct Args {
     uint32_t strand_count, cvs, seed, triangle_count, guide_count, has_normals;
     uint32_t noise, octave_count;  // runtime form of the template switches (batched path)
     uint32_t vec3_points;          // positions only, 3 floats per point (motion samples)
+    const uint32_t *order;         // processing order of the strands (null: index order)
     float support_scale, weight_power, rejection_cos, length_scale;
     float root_width, tip_width, radius_scale;
     float noise_mask, decorrelation_scale, preserve, inv_span;
@@ -272,15 +273,21 @@ struct Root {
 };
 
 // sample_root (generate.h:101-135)
-__device__ __forceinline__ Root sample_root(const Args &a, uint32_t strand) {
-    const float choose = random01(a.seed, strand, 0u) * static_cast<float>(a.triangle_count);
+// The alias-table pick of sample_root (generate.h:107-112)
+__device__ __forceinline__ uint32_t sample_triangle(const nxg_alias_entry *alias, uint32_t seed,
+                                                    uint32_t triangle_count, uint32_t strand) {
+    const float choose = random01(seed, strand, 0u) * static_cast<float>(triangle_count);
     uint32_t bucket = static_cast<uint32_t>(choose);
-    if (bucket >= a.triangle_count) bucket = a.triangle_count - 1u;
-    const nxg_alias_entry e = a.alias[bucket];
-    uint32_t tri = random01(a.seed, strand, 1u) < e.probability ? bucket : e.alias;
+    if (bucket >= triangle_count) bucket = triangle_count - 1u;
+    const nxg_alias_entry e = alias[bucket];
+    const uint32_t tri = random01(seed, strand, 1u) < e.probability ? bucket : e.alias;
     // validate_asset does not range-check alias targets; a corrupt entry
     // falls back to its own bucket instead of reading past the triangles
-    if (tri >= a.triangle_count) tri = bucket;
+    return tri < triangle_count ? tri : bucket;
+}
+
+__device__ __forceinline__ Root sample_root(const Args &a, uint32_t strand) {
+    const uint32_t tri = sample_triangle(a.alias, a.seed, a.triangle_count, strand);
     const float r1 = random01(a.seed, strand, 2u);
     const float r2 = random01(a.seed, strand, 3u);
     const float s = sqrtf(r1);
@@ -543,8 +550,8 @@ __global__ void __launch_bounds__(256) guide_prep_kernel(Prep p) {
 // store instruction covers consecutive float4s.  N is n when it is known at
 // compile time (full chunks), else 0.
 template <uint32_t N>
-__device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, uint64_t wave_first,
-                                            uint32_t cv0, uint32_t n, uint32_t lane) {
+__device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, const uint32_t *ids,
+                                            uint64_t wave_first, uint32_t cv0, uint32_t n, uint32_t lane) {
     // f / n for f < 64 * n <= 1024 as floor((f + 0.5) / n) in float: the
     // half keeps exact quotients away from rounding below an integer
     const float inv_n = 1.0f / static_cast<float>(N ? N : n);
@@ -553,8 +560,8 @@ __device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, u
         const uint32_t f = it * kWave + lane;
         const uint32_t s = static_cast<uint32_t>((static_cast<float>(f) + 0.5f) * inv_n);
         const uint32_t k = f - s * (N ? N : n);
-        const uint64_t strand = wave_first + s;
-        if (strand < a.strand_count) {
+        if (wave_first + s < a.strand_count) {
+            const uint64_t strand = ids[s];
             float4 p;
             p.x = stage[(k * 4u + 0u) * kStageStride + s];
             p.y = stage[(k * 4u + 1u) * kStageStride + s];
@@ -567,8 +574,8 @@ __device__ __forceinline__ void flush_chunk(const Args &a, const float *stage, u
 
 // The same chunk as positions only (motion-sample output): 3 floats per
 // point, stored as consecutive dwords of the strand-major chunk.
-__device__ __forceinline__ void flush_chunk_vec3(const Args &a, const float *stage, uint64_t wave_first,
-                                                 uint32_t cv0, uint32_t n, uint32_t lane) {
+__device__ __forceinline__ void flush_chunk_vec3(const Args &a, const float *stage, const uint32_t *ids,
+                                                 uint64_t wave_first, uint32_t cv0, uint32_t n, uint32_t lane) {
     float *out = reinterpret_cast<float *>(a.points);
     const uint32_t per = 3u * n;
     // f / per for f < 64 * per, per <= 48, as in flush_chunk
@@ -579,8 +586,8 @@ __device__ __forceinline__ void flush_chunk_vec3(const Args &a, const float *sta
         const uint32_t s = static_cast<uint32_t>((static_cast<float>(f) + 0.5f) * inv_per);
         const uint32_t rem = f - s * per;
         const uint32_t k = rem / 3u, c = rem - 3u * k;
-        const uint64_t strand = wave_first + s;
-        if (strand < a.strand_count) out[(strand * a.cvs + cv0 + k) * 3u + c] = stage[(k * 4u + c) * kStageStride + s];
+        if (wave_first + s < a.strand_count)
+            out[(uint64_t{ids[s]} * a.cvs + cv0 + k) * 3u + c] = stage[(k * 4u + c) * kStageStride + s];
     }
 }
 
@@ -599,12 +606,16 @@ __device__ __forceinline__ void wave_sync() {
 // description's gradient table in LDS.  NOISE/OCT fix the noise path at
 // compile time; with RT the octave count comes from a.octave_count (<= OCT).
 template <bool NOISE, int OCT, bool RT, int CHUNK>
-__device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, uint32_t lane, float *stage, const float4 *grad) {
-    const uint64_t strand64 = wave_first + lane;
-    const bool live = strand64 < a.strand_count;
-    // lanes past the end compute the last strand and store nothing, so every
-    // lane runs the same CV loop
-    const uint32_t strand = live ? static_cast<uint32_t>(strand64) : a.strand_count - 1u;
+__device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, uint32_t lane, float *stage,
+                                            uint32_t *ids, const float4 *grad) {
+    // position wave_first + lane of the processing order; lanes past the end
+    // compute the last position's strand and store nothing, so every lane
+    // runs the same CV loop
+    const uint64_t slot64 = wave_first + lane;
+    const bool live = slot64 < a.strand_count;
+    const uint32_t slot = live ? static_cast<uint32_t>(slot64) : a.strand_count - 1u;
+    const uint32_t strand = a.order ? a.order[slot] : slot;
+    ids[lane] = strand;  // read by the chunk flushes, after a wavefront sync
 
     Root r;
     if (a.explicit_roots) {
@@ -638,7 +649,7 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
             a.roots[strand] = o;
         }
         if (a.root_uvs) a.root_uvs[strand] = {r.u, r.v};
-        if (a.point_counts) a.point_counts[strand] = a.cvs;
+        if (a.point_counts && !a.order) a.point_counts[strand] = a.cvs;  // else written in strand order
         if (a.stencil_indices) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -725,8 +736,8 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
             }
             if (emit) {
                 wave_sync();
-                if (a.vec3_points) flush_chunk_vec3(a, stage, wave_first, cv0, n, lane);
-                else flush_chunk<0>(a, stage, wave_first, cv0, n, lane);
+                if (a.vec3_points) flush_chunk_vec3(a, stage, ids, wave_first, cv0, n, lane);
+                else flush_chunk<0>(a, stage, ids, wave_first, cv0, n, lane);
                 wave_sync();
             }
         }
@@ -744,6 +755,7 @@ template <bool NOISE, int OCT>
 __global__ void __launch_bounds__(256, (OCT <= 2 ? 3 : 2)) strand_kernel(Args a) {
     __shared__ float4 grad[NXG_NOISE_GRADIENT_COUNT];
     __shared__ float stage_all[256 / kWave][kChunk * 4 * kStageStride];
+    __shared__ uint32_t ids_all[256 / kWave][kWave];
     for (uint32_t i = threadIdx.x; i < NXG_NOISE_GRADIENT_COUNT; i += blockDim.x) {
         const nxg_vec3 g = a.gradients[i];
         grad[i] = make_float4(g.x, g.y, g.z, 0.0f);
@@ -752,7 +764,7 @@ __global__ void __launch_bounds__(256, (OCT <= 2 ? 3 : 2)) strand_kernel(Args a)
     const uint32_t lane = threadIdx.x % kWave;
     const uint32_t wave = threadIdx.x / kWave;
     const uint64_t wave_first = uint64_t{blockIdx.x} * blockDim.x + wave * kWave;
-    strand_tile<NOISE, OCT, false, kChunk>(a, wave_first, lane, stage_all[wave], grad);
+    strand_tile<NOISE, OCT, false, kChunk>(a, wave_first, lane, stage_all[wave], ids_all[wave], grad);
 }
 
 // ---- batched descriptions ------------------------------------------------
@@ -795,6 +807,7 @@ template <int WAVES, int CHUNK, int OCTMAX>
 __global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 && CHUNK < 16 ? 3 : 2)) batch_kernel(BatchArgs b) {
     __shared__ float4 grad_all[WAVES][NXG_NOISE_GRADIENT_COUNT];
     __shared__ float stage_all[WAVES][CHUNK * 4 * kStageStride];
+    __shared__ uint32_t ids_all[WAVES][kWave];
     __shared__ uint32_t scan[NXG_BATCH_MAX_ITEMS];
     if (threadIdx.x < b.count) scan[threadIdx.x] = b.tile_end[threadIdx.x];
     __syncthreads();
@@ -826,13 +839,126 @@ __global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 && CHUNK < 16 ? 3 : 2
                 loaded = item;
             }
             const uint64_t wave_first = uint64_t{tile - item_first} * kWave;
-            if (a.noise) strand_tile<true, OCTMAX, true, CHUNK>(a, wave_first, lane, stage_all[wave], grad);
-            else strand_tile<false, 1, false, CHUNK>(a, wave_first, lane, stage_all[wave], grad);
+            if (a.noise) strand_tile<true, OCTMAX, true, CHUNK>(a, wave_first, lane, stage_all[wave], ids_all[wave], grad);
+            else strand_tile<false, 1, false, CHUNK>(a, wave_first, lane, stage_all[wave], ids_all[wave], grad);
             wave_sync();
         }
     }
 }
 
+
+// ---- strand order ----------------------------------------------------------
+// Strands are independent, so they may run in any order; grouping them by
+// root triangle makes the 64 strands of a wavefront share stencil guides,
+// so their guide-offset reads hit the same cache lines (and their root and
+// noise-lattice gathers cluster too).  A counting sort into kOrderBins bins
+// of 2^shift consecutive triangles, with no device-wide atomics: each tile
+// of kOrderTile strands histograms its bins in LDS, the per-tile counts are
+// scanned bin-major, and each tile scatters its strands from its LDS copy of
+// the starts.  The order within a bin depends on LDS atomic timing, which
+// changes no output.
+
+struct OrderArgs {
+    const nxg_alias_entry *alias;
+    uint32_t seed, triangle_count, strand_count, cvs;
+    uint32_t shift, tiles;
+    uint32_t *counts, *partials, *order;
+    uint32_t *point_counts;  // optional, written here in strand order
+};
+
+__global__ void __launch_bounds__(1024) order_count_kernel(OrderArgs o) {
+    __shared__ uint32_t hist[kOrderBins];
+    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) hist[b] = 0u;
+    __syncthreads();
+    const uint32_t first = blockIdx.x * kOrderTile;
+    for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
+        const uint32_t s = first + i;
+        if (s >= o.strand_count) break;
+        atomicAdd(&hist[sample_triangle(o.alias, o.seed, o.triangle_count, s) >> o.shift], 1u);
+        if (o.point_counts) o.point_counts[s] = o.cvs;
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) o.counts[b * o.tiles + blockIdx.x] = hist[b];
+}
+
+__global__ void __launch_bounds__(1024) order_scatter_kernel(OrderArgs o) {
+    __shared__ uint32_t next[kOrderBins];
+    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) next[b] = o.counts[b * o.tiles + blockIdx.x];
+    __syncthreads();
+    const uint32_t first = blockIdx.x * kOrderTile;
+    for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
+        const uint32_t s = first + i;
+        if (s >= o.strand_count) break;
+        const uint32_t bin = sample_triangle(o.alias, o.seed, o.triangle_count, s) >> o.shift;
+        o.order[atomicAdd(&next[bin], 1u)] = s;
+    }
+}
+
+// Exclusive scan of 256 values, one per thread of a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *wave_totals, uint32_t &total) {
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    uint32_t inc = v;
+#pragma unroll
+    for (uint32_t d = 1; d < static_cast<uint32_t>(kWave); d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc += y;
+    }
+    if (lane == kWave - 1) wave_totals[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0u;
+    total = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < 256u / kWave; ++w) {
+        const uint32_t t = wave_totals[w];
+        if (w < wave) before += t;
+        total += t;
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+// data[0..n): exclusive scan within blocks of kScanBlock values (4 per
+// thread); partials[block] = the block's total.
+__global__ void __launch_bounds__(256) scan_blocks_kernel(uint32_t *data, uint32_t n, uint32_t *partials) {
+    __shared__ uint32_t wave_totals[256 / kWave];
+    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    uint32_t v[4], sum = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        v[j] = base + j < n ? data[base + j] : 0u;
+        sum += v[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, wave_totals, total);
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        if (base + j < n) data[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0u) partials[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the block totals in place, 256 at a time.
+__global__ void __launch_bounds__(256) scan_partials_kernel(uint32_t *partials, uint32_t count) {
+    __shared__ uint32_t wave_totals[256 / kWave];
+    uint32_t carry = 0u;
+    for (uint32_t first = 0; first < count; first += 256u) {
+        const uint32_t i = first + threadIdx.x;
+        const uint32_t v = i < count ? partials[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, wave_totals, total);
+        if (i < count) partials[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(256) scan_add_kernel(uint32_t *data, uint32_t n, const uint32_t *partials) {
+    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    const uint32_t add = partials[blockIdx.x];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j)
+        if (base + j < n) data[base + j] += add;
+}
 
 // ---- float expression runtime ------------------------------------------
 
@@ -1164,6 +1290,8 @@ struct Plan {
     Prep p;
     uint64_t prep_lanes;
     bool noise;
+    bool ordered;    // strands run grouped by root triangle
+    OrderArgs order;
 };
 
 uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
@@ -1185,7 +1313,8 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
             return NXG_ERROR_STENCIL_CAPACITY_TOO_SMALL;
     }
     const nxg_asset_header &h = asset->header;
-    const WorkspaceLayout L = workspace_layout(h.guide_count, params->cvs_per_strand);
+    const WorkspaceLayout L =
+        workspace_layout(h.guide_count, params->cvs_per_strand, params->strand_count, h.triangle_count);
     if (!workspace || workspace_bytes < L.total) return NXG_ERROR_WORKSPACE_TOO_SMALL;
 
     const nxg_vec3 *d_pos = deformed ? deformed->positions : nullptr;
@@ -1254,7 +1383,29 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
     }
     a.stencil_indices = stencil ? stencil->indices : nullptr;
     a.stencil_weights = stencil ? stencil->weights : nullptr;
+    // explicit roots keep the caller's order; sampled roots run grouped by triangle
+    plan.ordered = explicit_roots == nullptr;
+    uint8_t *ws = static_cast<uint8_t *>(workspace);
+    plan.order = OrderArgs{a.alias, a.seed, a.triangle_count, a.strand_count, a.cvs,
+                           order_shift(a.triangle_count), order_tiles(a.strand_count),
+                           reinterpret_cast<uint32_t *>(ws + L.counts_offset),
+                           reinterpret_cast<uint32_t *>(ws + L.partials_offset),
+                           reinterpret_cast<uint32_t *>(ws + L.order_offset), a.point_counts};
+    a.order = plan.ordered ? plan.order.order : nullptr;
     return NXG_OK;
+}
+
+// The counting sort of the strands by root triangle (five small kernels).
+uint32_t launch_order(const OrderArgs &o, hipStream_t stream) {
+    const uint32_t n = kOrderBins * o.tiles;
+    const uint32_t scan_blocks = (n + kScanBlock - 1u) / kScanBlock;
+    hipLaunchKernelGGL(order_count_kernel, dim3(o.tiles), dim3(1024), 0, stream, o);
+    hipLaunchKernelGGL(scan_blocks_kernel, dim3(scan_blocks), dim3(256), 0, stream, o.counts, n, o.partials);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(256), 0, stream, o.partials, scan_blocks);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(scan_blocks), dim3(256), 0, stream, o.counts, n,
+                       static_cast<const uint32_t *>(o.partials));
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(o.tiles), dim3(1024), 0, stream, o);
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
 uint32_t launch_plan(const Plan &plan, const nxg_generation_params &params, hipStream_t stream, uint32_t stages) {
@@ -1264,6 +1415,8 @@ uint32_t launch_plan(const Plan &plan, const nxg_generation_params &params, hipS
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     }
     if (!(stages & kStrands)) return NXG_OK;
+    if (plan.ordered)
+        if (uint32_t s = launch_order(plan.order, stream); s != NXG_OK) return s;
     const Args &a = plan.a;
     const dim3 grid((params.strand_count + 255u) / 256u), block(256);
     if (!plan.noise) {
@@ -1302,7 +1455,10 @@ uint32_t launch_motion(const nxg_device_asset *asset, const nxg_motion_sample *s
     if (uint32_t s = nxg_validate_motion_generation(asset, samples, count, params, output, config); s != NXG_OK)
         return s;
     if (!valid_octaves(octaves)) return NXG_ERROR_INVALID_OCTAVES;
-    const uint64_t one = workspace_layout(asset->header.guide_count, params->cvs_per_strand).total;
+    const uint64_t one =
+        workspace_layout(asset->header.guide_count, params->cvs_per_strand, params->strand_count,
+                         asset->header.triangle_count)
+            .total;
     if (!workspace || workspace_bytes < one * count) return NXG_ERROR_WORKSPACE_TOO_SMALL;
     const uint64_t per_sample = uint64_t{params->strand_count} * params->cvs_per_strand;
     for (uint32_t pass = 0; pass < 2u; ++pass) {  // pass 0 validates every sample, pass 1 launches
@@ -1449,6 +1605,8 @@ double strand_cost(const Args &a) {
 struct BatchLaunch {
     AttrLaunch attrs[NXG_BATCH_MAX_ITEMS];
     bool has_attrs[NXG_BATCH_MAX_ITEMS];
+    OrderArgs orders[NXG_BATCH_MAX_ITEMS];
+    bool ordered[NXG_BATCH_MAX_ITEMS];
     BatchPrepArgs prep;
     BatchArgs strands;
     uint32_t prep_blocks;
@@ -1488,6 +1646,8 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
         blocks += (plan.prep_lanes + 255u) / 256u;
         out.prep.block_end[k] = static_cast<uint32_t>(blocks);
         out.strands.items[k] = plan.a;
+        out.orders[k] = plan.order;
+        out.ordered[k] = plan.ordered;
         if (plan.noise && plan.a.octave_count > out.max_octaves) out.max_octaves = plan.a.octave_count;
         cost[k] = strand_cost(plan.a);
         out.strands.order[k] = k;
@@ -1518,9 +1678,12 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
 }
 
 uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
-    for (uint32_t k = 0; k < b.strands.count; ++k)
+    for (uint32_t k = 0; k < b.strands.count; ++k) {
         if (b.has_attrs[k])
             if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
+        if (b.ordered[k])
+            if (uint32_t s = launch_order(b.orders[k], stream); s != NXG_OK) return s;
+    }
     if (b.prep_blocks) {
         hipLaunchKernelGGL(batch_prep_kernel, dim3(b.prep_blocks), dim3(256), 0, stream, b.prep);
     } else {

@@ -19,14 +19,34 @@ bool valid_octaves(const nxg_noise_octaves *octaves);
 // Output checks of device_contract.h:232-253.
 uint32_t validate_packed_output(const nxg_generation_params &p, const nxg_packed_output *out);
 
-// Workspace layout: per-guide state (2 float4) then the guide offset table
-// (guide_count * cvs_per_strand float4), each 256-byte aligned.
+// Workspace layout, each region 256-byte aligned:
+//   guide state       guide_count x 2 float4 (root + support radius, normal)
+//   guide offsets     guide_count x cvs_per_strand float4
+//   strand order      strand_count u32: strands grouped by root triangle
+//   bin counts        kOrderBins x order tiles u32: per-tile counting-sort
+//                     histograms, bin-major, scanned in place into starts
+//   scan partials     one u32 per kScanBlock counts
 struct WorkspaceLayout {
     uint64_t guide_state_offset;
     uint64_t guide_offsets_offset;
+    uint64_t order_offset;
+    uint64_t counts_offset;
+    uint64_t partials_offset;
     uint64_t total;
 };
-WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand);
+constexpr uint32_t kScanBlock = 1024u;   // values per scan block
+constexpr uint32_t kOrderBins = 8192u;   // counting-sort bins (LDS histogram per tile)
+constexpr uint32_t kOrderTile = 8192u;   // strands per counting-sort tile
+// Triangles per bin: the smallest power of two that fits the triangles in
+// kOrderBins bins (bins hold consecutive triangle indices).
+inline uint32_t order_shift(uint32_t triangle_count) {
+    uint32_t shift = 0u;
+    while ((uint64_t{triangle_count} + (uint64_t{1} << shift) - 1u) >> shift > kOrderBins) ++shift;
+    return shift;
+}
+inline uint32_t order_tiles(uint32_t strand_count) { return (strand_count + kOrderTile - 1u) / kOrderTile; }
+WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand, uint32_t strand_count,
+                                 uint32_t triangle_count);
 
 // Asset builder (asset_builder.cpp).  build_blob with host_stencil = false
 // leaves the triangle stencil zeroed and the content hash unset; finish_blob
