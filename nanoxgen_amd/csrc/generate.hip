@@ -866,11 +866,11 @@ struct OrderArgs {
     uint32_t *point_counts;  // optional, written here in strand order
 };
 
-__global__ void __launch_bounds__(1024) order_count_kernel(OrderArgs o) {
+__device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t tile) {
     __shared__ uint32_t hist[kOrderBins];
     for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) hist[b] = 0u;
     __syncthreads();
-    const uint32_t first = blockIdx.x * kOrderTile;
+    const uint32_t first = tile * kOrderTile;
     for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
         const uint32_t s = first + i;
         if (s >= o.strand_count) break;
@@ -878,14 +878,14 @@ __global__ void __launch_bounds__(1024) order_count_kernel(OrderArgs o) {
         if (o.point_counts) o.point_counts[s] = o.cvs;
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) o.counts[b * o.tiles + blockIdx.x] = hist[b];
+    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) o.counts[b * o.tiles + tile] = hist[b];
 }
 
-__global__ void __launch_bounds__(1024) order_scatter_kernel(OrderArgs o) {
+__device__ __forceinline__ void order_scatter_tile(const OrderArgs &o, uint32_t tile) {
     __shared__ uint32_t next[kOrderBins];
-    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) next[b] = o.counts[b * o.tiles + blockIdx.x];
+    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) next[b] = o.counts[b * o.tiles + tile];
     __syncthreads();
-    const uint32_t first = blockIdx.x * kOrderTile;
+    const uint32_t first = tile * kOrderTile;
     for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
         const uint32_t s = first + i;
         if (s >= o.strand_count) break;
@@ -894,7 +894,7 @@ __global__ void __launch_bounds__(1024) order_scatter_kernel(OrderArgs o) {
     }
 }
 
-// Exclusive scan of 256 values, one per thread of a 256-thread block.
+// Exclusive scan of one value per thread of a 256-thread block.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *wave_totals, uint32_t &total) {
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     uint32_t inc = v;
@@ -917,11 +917,11 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *w
     return before + inc - v;
 }
 
-// data[0..n): exclusive scan within blocks of kScanBlock values (4 per
+// data[0..n): exclusive scan of block `block` of kScanBlock values (4 per
 // thread); partials[block] = the block's total.
-__global__ void __launch_bounds__(256) scan_blocks_kernel(uint32_t *data, uint32_t n, uint32_t *partials) {
+__device__ __forceinline__ void scan_block(uint32_t *data, uint32_t n, uint32_t *partials, uint32_t block) {
     __shared__ uint32_t wave_totals[256 / kWave];
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    const uint32_t base = block * kScanBlock + threadIdx.x * 4u;
     uint32_t v[4], sum = 0u;
 #pragma unroll
     for (uint32_t j = 0; j < 4u; ++j) {
@@ -935,11 +935,11 @@ __global__ void __launch_bounds__(256) scan_blocks_kernel(uint32_t *data, uint32
         if (base + j < n) data[base + j] = run;
         run += v[j];
     }
-    if (threadIdx.x == 0u) partials[blockIdx.x] = total;
+    if (threadIdx.x == 0u) partials[block] = total;
 }
 
-// One block: exclusive scan of the block totals in place, 256 at a time.
-__global__ void __launch_bounds__(256) scan_partials_kernel(uint32_t *partials, uint32_t count) {
+// Exclusive scan of the block totals in place, 256 at a time (one block).
+__device__ __forceinline__ void scan_partials(uint32_t *partials, uint32_t count) {
     __shared__ uint32_t wave_totals[256 / kWave];
     uint32_t carry = 0u;
     for (uint32_t first = 0; first < count; first += 256u) {
@@ -952,12 +952,70 @@ __global__ void __launch_bounds__(256) scan_partials_kernel(uint32_t *partials, 
     }
 }
 
-__global__ void __launch_bounds__(256) scan_add_kernel(uint32_t *data, uint32_t n, const uint32_t *partials) {
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
-    const uint32_t add = partials[blockIdx.x];
+__device__ __forceinline__ void scan_add(uint32_t *data, uint32_t n, const uint32_t *partials, uint32_t block) {
+    const uint32_t base = block * kScanBlock + threadIdx.x * 4u;
+    const uint32_t add = partials[block];
 #pragma unroll
     for (uint32_t j = 0; j < 4u; ++j)
         if (base + j < n) data[base + j] += add;
+}
+
+__device__ __forceinline__ uint32_t order_count_values(const OrderArgs &o) { return kOrderBins * o.tiles; }
+__device__ __forceinline__ uint32_t order_scan_blocks(const OrderArgs &o) {
+    return (order_count_values(o) + kScanBlock - 1u) / kScanBlock;
+}
+
+__global__ void __launch_bounds__(1024) order_count_kernel(OrderArgs o) { order_count_tile(o, blockIdx.x); }
+__global__ void __launch_bounds__(1024) order_scatter_kernel(OrderArgs o) { order_scatter_tile(o, blockIdx.x); }
+__global__ void __launch_bounds__(256) order_scan_blocks_kernel(OrderArgs o) {
+    scan_block(o.counts, order_count_values(o), o.partials, blockIdx.x);
+}
+__global__ void __launch_bounds__(256) order_scan_partials_kernel(OrderArgs o) {
+    scan_partials(o.partials, order_scan_blocks(o));
+}
+__global__ void __launch_bounds__(256) order_scan_add_kernel(OrderArgs o) {
+    scan_add(o.counts, order_count_values(o), o.partials, blockIdx.x);
+}
+
+// The same five steps over every item of a batch: block ranges per item
+// come from inclusive scans of tiles and of scan blocks.
+struct BatchOrderArgs {
+    OrderArgs items[NXG_BATCH_MAX_ITEMS];
+    uint32_t tile_end[NXG_BATCH_MAX_ITEMS];
+    uint32_t scan_end[NXG_BATCH_MAX_ITEMS];
+    uint32_t count;
+};
+
+__device__ __forceinline__ uint32_t batch_item_of(const uint32_t *end, uint32_t count, uint32_t block,
+                                                  uint32_t &first) {
+    uint32_t item = 0u;
+    while (item + 1u < count && block >= end[item]) ++item;
+    first = item ? end[item - 1u] : 0u;
+    return item;
+}
+
+__global__ void __launch_bounds__(1024) batch_order_count_kernel(BatchOrderArgs b) {
+    uint32_t first;
+    const uint32_t k = batch_item_of(b.tile_end, b.count, blockIdx.x, first);
+    order_count_tile(b.items[k], blockIdx.x - first);
+}
+__global__ void __launch_bounds__(1024) batch_order_scatter_kernel(BatchOrderArgs b) {
+    uint32_t first;
+    const uint32_t k = batch_item_of(b.tile_end, b.count, blockIdx.x, first);
+    order_scatter_tile(b.items[k], blockIdx.x - first);
+}
+__global__ void __launch_bounds__(256) batch_order_scan_blocks_kernel(BatchOrderArgs b) {
+    uint32_t first;
+    const uint32_t k = batch_item_of(b.scan_end, b.count, blockIdx.x, first);
+    scan_block(b.items[k].counts, order_count_values(b.items[k]), b.items[k].partials, blockIdx.x - first);
+}
+__global__ void __launch_bounds__(256) batch_order_scan_partials_kernel(BatchOrderArgs b) {
+    scan_partials(b.items[blockIdx.x].partials, order_scan_blocks(b.items[blockIdx.x]));
+}
+__global__ void __launch_bounds__(256) batch_order_scan_add_kernel(BatchOrderArgs b) {
+    uint32_t first;
+    const uint32_t k = batch_item_of(b.scan_end, b.count, blockIdx.x, first);
+    scan_add(b.items[k].counts, order_count_values(b.items[k]), b.items[k].partials, blockIdx.x - first);
 }
 
 // ---- float expression runtime ------------------------------------------
@@ -1397,13 +1455,11 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
 
 // The counting sort of the strands by root triangle (five small kernels).
 uint32_t launch_order(const OrderArgs &o, hipStream_t stream) {
-    const uint32_t n = kOrderBins * o.tiles;
-    const uint32_t scan_blocks = (n + kScanBlock - 1u) / kScanBlock;
+    const uint32_t scan_blocks = (kOrderBins * o.tiles + kScanBlock - 1u) / kScanBlock;
     hipLaunchKernelGGL(order_count_kernel, dim3(o.tiles), dim3(1024), 0, stream, o);
-    hipLaunchKernelGGL(scan_blocks_kernel, dim3(scan_blocks), dim3(256), 0, stream, o.counts, n, o.partials);
-    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(256), 0, stream, o.partials, scan_blocks);
-    hipLaunchKernelGGL(scan_add_kernel, dim3(scan_blocks), dim3(256), 0, stream, o.counts, n,
-                       static_cast<const uint32_t *>(o.partials));
+    hipLaunchKernelGGL(order_scan_blocks_kernel, dim3(scan_blocks), dim3(256), 0, stream, o);
+    hipLaunchKernelGGL(order_scan_partials_kernel, dim3(1), dim3(256), 0, stream, o);
+    hipLaunchKernelGGL(order_scan_add_kernel, dim3(scan_blocks), dim3(256), 0, stream, o);
     hipLaunchKernelGGL(order_scatter_kernel, dim3(o.tiles), dim3(1024), 0, stream, o);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
@@ -1605,8 +1661,8 @@ double strand_cost(const Args &a) {
 struct BatchLaunch {
     AttrLaunch attrs[NXG_BATCH_MAX_ITEMS];
     bool has_attrs[NXG_BATCH_MAX_ITEMS];
-    OrderArgs orders[NXG_BATCH_MAX_ITEMS];
-    bool ordered[NXG_BATCH_MAX_ITEMS];
+    BatchOrderArgs order;
+    uint32_t order_tiles, order_scan_blocks;
     BatchPrepArgs prep;
     BatchArgs strands;
     uint32_t prep_blocks;
@@ -1646,8 +1702,7 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
         blocks += (plan.prep_lanes + 255u) / 256u;
         out.prep.block_end[k] = static_cast<uint32_t>(blocks);
         out.strands.items[k] = plan.a;
-        out.orders[k] = plan.order;
-        out.ordered[k] = plan.ordered;
+        out.order.items[k] = plan.order;  // batch items always sample their roots
         if (plan.noise && plan.a.octave_count > out.max_octaves) out.max_octaves = plan.a.octave_count;
         cost[k] = strand_cost(plan.a);
         out.strands.order[k] = k;
@@ -1666,6 +1721,15 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
         tiles += (uint64_t{out.strands.items[out.strands.order[k]].strand_count} + 63u) / 64u;
         out.strands.tile_end[k] = static_cast<uint32_t>(tiles);
     }
+    out.order.count = count;
+    out.order_tiles = out.order_scan_blocks = 0u;
+    for (uint32_t k = 0; k < count; ++k) {
+        const OrderArgs &o = out.order.items[k];
+        out.order_tiles += o.tiles;
+        out.order_scan_blocks += (kOrderBins * o.tiles + kScanBlock - 1u) / kScanBlock;
+        out.order.tile_end[k] = out.order_tiles;
+        out.order.scan_end[k] = out.order_scan_blocks;
+    }
     if (blocks > 0x7fffffffu || tiles > 0x7fffffffu) return NXG_ERROR_OUTPUT_SIZE_OVERFLOW;
     out.prep.count = count;
     out.prep.counter = counter;
@@ -1678,11 +1742,16 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
 }
 
 uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
-    for (uint32_t k = 0; k < b.strands.count; ++k) {
+    for (uint32_t k = 0; k < b.strands.count; ++k)
         if (b.has_attrs[k])
             if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
-        if (b.ordered[k])
-            if (uint32_t s = launch_order(b.orders[k], stream); s != NXG_OK) return s;
+    if (b.order_tiles) {
+        hipLaunchKernelGGL(batch_order_count_kernel, dim3(b.order_tiles), dim3(1024), 0, stream, b.order);
+        hipLaunchKernelGGL(batch_order_scan_blocks_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
+        hipLaunchKernelGGL(batch_order_scan_partials_kernel, dim3(b.order.count), dim3(256), 0, stream, b.order);
+        hipLaunchKernelGGL(batch_order_scan_add_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
+        hipLaunchKernelGGL(batch_order_scatter_kernel, dim3(b.order_tiles), dim3(1024), 0, stream, b.order);
+        if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     }
     if (b.prep_blocks) {
         hipLaunchKernelGGL(batch_prep_kernel, dim3(b.prep_blocks), dim3(256), 0, stream, b.prep);
