@@ -1157,7 +1157,12 @@ enum AttrMode : uint32_t { kAttrAll = 0u, kAttrCheck = 1u, kAttrFallback = 2u };
 
 // One lane per root.  The program is the same for every lane, so opcode
 // dispatch is wave-uniform (scalar branches, instructions fetched with scalar
-// loads); SSA values live in LDS at values[index * blockDim + lane].
+// loads).  With REG the value slots are a kRegSlots-entry register array,
+// indexed with the wave-uniform slot numbers (M0-relative register moves);
+// otherwise they live in LDS at values[slot * blockDim + lane].
+constexpr uint32_t kRegSlots = 32u;
+
+template <bool REG>
 __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
     extern __shared__ float lds[];
     float4 *grad = reinterpret_cast<float4 *>(lds);
@@ -1192,8 +1197,17 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
     const float face_seed = hash_to_unit(hash_step(hash_step(hash_step(0u, 0.0f), 0.0f), static_cast<float>(r.tri)));
 
     // the root variables live in their reserved slots for the whole set
+    // value slots: a plain local array (kept in registers: a struct around it
+    // would send it to scratch) or the LDS rows
+    float regs[kRegSlots];
+#define SLOT_GET(x) (REG ? regs[__builtin_amdgcn_readfirstlane(x) & (kRegSlots - 1u)] : values[(x) * B + lane])
+#define SLOT_SET(x, v)                                                        \
+    do {                                                                      \
+        if constexpr (REG) regs[__builtin_amdgcn_readfirstlane(x) & (kRegSlots - 1u)] = (v); \
+        else values[(x) * B + lane] = (v);                                    \
+    } while (0)
 #pragma unroll
-    for (uint32_t q = 0; q < NXG_ROOT_VAR_COUNT; ++q) values[q * B + lane] = vars[q];
+    for (uint32_t q = 0; q < NXG_ROOT_VAR_COUNT; ++q) SLOT_SET(q, vars[q]);
 
     float result[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     bool bad = false;
@@ -1211,12 +1225,12 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
         for (uint32_t i = 0; i < n; ++i) {
             const PackedInstruction in = next;
             next = ins[i + 1u];
-#define OPV(j) values[uint32_t{in.slot[j]} * B + lane]
-#define OPA(j) values[operand_slots[in.operand_offset + (j)] * B + lane]
+#define OPV(j) SLOT_GET(uint32_t{in.slot[j]})
+#define OPA(j) SLOT_GET(operand_slots[in.operand_offset + (j)])
             float value = 0.0f;
             switch (in.op) {
                 case NXG_OP_CONSTANT: value = in.immediate; break;
-                case NXG_OP_INPUT: value = values[in.auxiliary * B + lane]; break;
+                case NXG_OP_INPUT: value = SLOT_GET(in.auxiliary); break;
                 case NXG_OP_NEGATE: value = -OPV(0); break;
                 case NXG_OP_LOGICAL_NOT: value = OPV(0) == 0.0f; break;
                 case NXG_OP_ADD: value = OPV(0) + OPV(1); break;
@@ -1290,13 +1304,15 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
 #undef OPV
 #undef OPA
             if (!isfinite(value)) bad = true;
-            values[uint32_t{in.dest} * B + lane] = value;
+            SLOT_SET(uint32_t{in.dest}, value);
         }
         // constant indices keep result[] in registers
-        const float rv = values[prog->result * B + lane];
+        const float rv = SLOT_GET(prog->result);
 #pragma unroll
         for (uint32_t q = 0; q < NXG_EXPR_MAX_PROGRAMS; ++q) result[q] = q == k ? rv : result[q];
     }
+#undef SLOT_GET
+#undef SLOT_SET
     if (!live) return;
     if (bad) {
         const float nan = __builtin_nanf("");
@@ -1540,6 +1556,7 @@ uint32_t launch_motion(const nxg_device_asset *asset, const nxg_motion_sample *s
 struct AttrLaunch {
     AttrArgs a;
     uint32_t block, lds;
+    bool registers;  // value slots in registers (attribute_kernel<true>)
     uint64_t lanes;
 };
 
@@ -1592,7 +1609,8 @@ uint32_t make_attr_launch(const nxg_device_asset *asset, const nxg_generation_pa
     uint32_t block = 256u;
     while (block > 64u && uint64_t{set->max_values} * block * 4u + 4096u > 65536u) block -= 64u;
     out.block = block;
-    out.lds = set->max_values * block * 4u + 4u * 4u * NXG_NOISE_GRADIENT_COUNT;
+    out.registers = set->max_values <= kRegSlots;
+    out.lds = (out.registers ? 0u : set->max_values * block * 4u) + 4u * 4u * NXG_NOISE_GRADIENT_COUNT;
     out.lanes = mode == kAttrCheck ? (uint64_t{params->strand_count} + stride - 1u) / stride : params->strand_count;
     return NXG_OK;
 }
@@ -1600,11 +1618,12 @@ uint32_t make_attr_launch(const nxg_device_asset *asset, const nxg_generation_pa
 uint32_t launch_attributes(const AttrLaunch &l, hipStream_t stream) {
     if (l.lanes == 0u) return NXG_OK;
     if (l.lds > 65536u &&
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&attribute_kernel),
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&attribute_kernel<false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(l.lds)) != hipSuccess)
         return NXG_ERROR_HIP;
-    hipLaunchKernelGGL(attribute_kernel, dim3(static_cast<uint32_t>((l.lanes + l.block - 1u) / l.block)),
-                       dim3(l.block), l.lds, stream, l.a);
+    const dim3 grid(static_cast<uint32_t>((l.lanes + l.block - 1u) / l.block)), block(l.block);
+    if (l.registers) hipLaunchKernelGGL(attribute_kernel<true>, grid, block, l.lds, stream, l.a);
+    else hipLaunchKernelGGL(attribute_kernel<false>, grid, block, l.lds, stream, l.a);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
