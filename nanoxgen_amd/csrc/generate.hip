@@ -1104,7 +1104,23 @@ __device__ __forceinline__ float clamp_std(float v, float lo, float hi) {  // st
 }
 
 // evaluate_ramp_points<float> (xgen_expression.cpp:559-615)
-__device__ float ramp_eval(const nxg_expr_ramp_point *pts, uint32_t n, float x) {
+// Pointers into the packed program image, in the constant address space.
+#define ImageOf(T) const __attribute__((address_space(4))) T *
+typedef const __attribute__((address_space(4))) uint8_t *Image;
+
+// A whole instruction as eight scalar-loaded words (structs cannot be
+// copied out of another address space directly).
+__device__ __forceinline__ PackedInstruction load_instruction(ImageOf(PackedInstruction) p) {
+    ImageOf(uint32_t) w = (ImageOf(uint32_t))p;
+    uint32_t v[sizeof(PackedInstruction) / 4u];
+#pragma unroll
+    for (uint32_t j = 0; j < sizeof(PackedInstruction) / 4u; ++j) v[j] = w[j];
+    PackedInstruction r;
+    __builtin_memcpy(&r, v, sizeof(r));
+    return r;
+}
+
+__device__ float ramp_eval(ImageOf(nxg_expr_ramp_point) pts, uint32_t n, float x) {
     if (x <= pts[0].position) return clamp_std(pts[0].value, 0.0f, 1.0f);
     if (x >= pts[n - 1u].position) return clamp_std(pts[n - 1u].value, 0.0f, 1.0f);
     uint32_t upper = 1u;
@@ -1212,19 +1228,22 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
     float result[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     bool bad = false;
     for (uint32_t k = 0; k < a.program_count; ++k) {
-        const PackedProgram *prog = reinterpret_cast<const PackedProgram *>(a.programs + a.program_offset[k]);
-        const PackedInstruction *ins = reinterpret_cast<const PackedInstruction *>(a.programs + prog->instructions_offset);
-        const uint32_t *operand_slots = reinterpret_cast<const uint32_t *>(a.programs + prog->operands_offset);
-        const nxg_expr_ramp *ramps = reinterpret_cast<const nxg_expr_ramp *>(a.programs + prog->ramps_offset);
-        const nxg_expr_ramp_point *points =
-            reinterpret_cast<const nxg_expr_ramp_point *>(a.programs + prog->ramp_points_offset);
+        // the image is read-only for the whole launch: reading it through
+        // the constant address space lets every (wave-uniform) fetch be a
+        // scalar load
+        const Image image = (Image)a.programs;
+        const ImageOf(PackedProgram) prog = (ImageOf(PackedProgram))(image + a.program_offset[k]);
+        const ImageOf(PackedInstruction) ins = (ImageOf(PackedInstruction))(image + prog->instructions_offset);
+        const ImageOf(uint32_t) operand_slots = (ImageOf(uint32_t))(image + prog->operands_offset);
+        const ImageOf(nxg_expr_ramp) ramps = (ImageOf(nxg_expr_ramp))(image + prog->ramps_offset);
+        const ImageOf(nxg_expr_ramp_point) points = (ImageOf(nxg_expr_ramp_point))(image + prog->ramp_points_offset);
         const uint32_t n = prog->instruction_count;
         // the next instruction is fetched while the current one runs (the
         // array ends with a zeroed entry)
-        PackedInstruction next = ins[0];
+        PackedInstruction next = load_instruction(ins);
         for (uint32_t i = 0; i < n; ++i) {
             const PackedInstruction in = next;
-            next = ins[i + 1u];
+            next = load_instruction(ins + i + 1u);
 #define OPV(j) SLOT_GET(uint32_t{in.slot[j]})
 #define OPA(j) SLOT_GET(operand_slots[in.operand_offset + (j)])
             float value = 0.0f;
@@ -1278,8 +1297,7 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
                     break;
                 }
                 case NXG_OP_RAMP: {
-                    const nxg_expr_ramp rp = ramps[in.auxiliary];
-                    value = ramp_eval(points + rp.point_offset, rp.point_count, 0.0f);
+                    value = ramp_eval(points + ramps[in.auxiliary].point_offset, ramps[in.auxiliary].point_count, 0.0f);
                     break;
                 }
                 case NXG_OP_POW: value = powf(OPV(0), OPV(1)); break;
