@@ -4,6 +4,7 @@
 // (src/xgen_expression.cpp:1007-1148: operand ranges, SSA order, arity,
 // input and ramp indices, result index), so the device interpreter can run
 // without them; only non-finite values remain a run-time condition.
+#include <cmath>
 #include <cstring>
 #include <functional>
 #include <queue>
@@ -80,35 +81,93 @@ uint64_t program_bytes(const nxg_expr_program &p) {
 }
 
 
-// Liveness-based slot allocation over the SSA order of a validated program:
-// a value's slot returns to the pool after its last reader (operands are
-// read before the result is written, so a result may take the slot of one of
-// its own operands); the result value is never released; inputs use their
-// root variable's reserved slot.  Returns the slot count, vars included.
-uint32_t allocate_slots(const nxg_expr_program &p, std::vector<uint16_t> &slot_of) {
+// How one validated program is packed.
+//   dispatched[i]  instruction i runs on the device (inputs never do: their
+//                  uses read the root variable's slot; finite constants whose
+//                  every reader has at most 4 operands and a free immediate
+//                  are inlined into those readers)
+//   operand[i][j]  the encoded operand slot (a value slot, or kImmediate | k)
+//   imm[i][k]      the immediates of instruction i
+// Slots: a liveness-based allocation over the SSA order: a value's slot
+// returns to the pool after its last reader (operands are read before the
+// result is written, so a result may take the slot of one of its own
+// operands); the result value is never released.
+struct PackPlan {
+    std::vector<bool> dispatched;
+    std::vector<uint16_t> slot_of;
+    std::vector<std::vector<uint16_t>> operand;
+    std::vector<float> imm0, imm1;
+    uint32_t slot_count = kVarSlots;
+    uint32_t var_mask = 0u;
+    uint32_t dispatched_count = 0u;
+};
+
+bool is_finite(float v) { return std::isfinite(v); }
+
+PackPlan plan_program(const nxg_expr_program &p) {
     const uint32_t n = p.instruction_count;
-    std::vector<int64_t> last_use(n, -1);
+    PackPlan plan;
+    plan.dispatched.assign(n, true);
+    plan.slot_of.assign(n, 0u);
+    plan.operand.assign(n, {});
+    plan.imm0.assign(n, 0.0f);
+    plan.imm1.assign(n, 0.0f);
+    auto ins = [&](uint32_t i) -> const nxg_expr_instruction & { return p.instructions[i]; };
+    auto opnd = [&](uint32_t i, uint32_t j) { return p.operands[ins(i).operand_offset + j]; };
+    // immediates each instruction can host: RANDOM keeps `immediate` for its seed
+    std::vector<uint32_t> free_imm(n);
     for (uint32_t i = 0; i < n; ++i) {
-        const nxg_expr_instruction &in = p.instructions[i];
-        for (uint32_t k = 0; k < in.operand_count; ++k) last_use[p.operands[in.operand_offset + k]] = i;
+        const uint32_t op = ins(i).op;
+        free_imm[i] = (op == NXG_OP_CONSTANT || op == NXG_OP_INPUT || ins(i).operand_count > 4u) ? 0u
+                      : op == NXG_OP_RANDOM                                                      ? 1u
+                                                                                                 : 2u;
     }
+    // which constants can be inlined everywhere they are read
+    std::vector<bool> inline_const(n, false);
+    std::vector<std::vector<uint32_t>> readers(n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < ins(i).operand_count; ++j) {
+            const uint32_t v = opnd(i, j);
+            if (readers[v].empty() || readers[v].back() != i) readers[v].push_back(i);
+        }
+    std::vector<uint32_t> used_imm(n, 0u);
+    for (uint32_t c = 0; c < n; ++c) {
+        if (ins(c).op != NXG_OP_CONSTANT || c == p.result || !is_finite(ins(c).immediate)) continue;
+        bool ok = true;
+        for (uint32_t r : readers[c]) ok = ok && used_imm[r] < free_imm[r];
+        if (!ok) continue;
+        inline_const[c] = true;
+        for (uint32_t r : readers[c]) ++used_imm[r];
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (ins(i).op == NXG_OP_INPUT) {
+            plan.dispatched[i] = false;
+            plan.var_mask |= 1u << p.input_bindings[ins(i).auxiliary];
+        } else if (inline_const[i]) {
+            plan.dispatched[i] = false;
+        }
+    }
+    // slots
+    std::vector<int64_t> last_use(n, -1);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < ins(i).operand_count; ++j) last_use[opnd(i, j)] = i;
     last_use[p.result] = n;
-    slot_of.assign(n, 0u);
     std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> free_slots;
     std::vector<bool> released(n, false);
     uint32_t next = kVarSlots;
     for (uint32_t i = 0; i < n; ++i) {
-        const nxg_expr_instruction &in = p.instructions[i];
-        for (uint32_t k = 0; k < in.operand_count; ++k) {
-            const uint32_t v = p.operands[in.operand_offset + k];
-            if (last_use[v] == i && !released[v] && p.instructions[v].op != NXG_OP_INPUT) {
-                released[v] = true;
-                free_slots.push(slot_of[v]);
-            }
-        }
+        const nxg_expr_instruction &in = ins(i);
         if (in.op == NXG_OP_INPUT) {
-            slot_of[i] = static_cast<uint16_t>(p.input_bindings[in.auxiliary]);
+            plan.slot_of[i] = static_cast<uint16_t>(p.input_bindings[in.auxiliary]);
             continue;
+        }
+        if (!plan.dispatched[i]) continue;
+        for (uint32_t j = 0; j < in.operand_count; ++j) {
+            const uint32_t v = opnd(i, j);
+            if (last_use[v] == i && !released[v] && plan.dispatched[v]) {
+                released[v] = true;
+                free_slots.push(plan.slot_of[v]);
+            }
         }
         uint32_t slot;
         if (free_slots.empty()) {
@@ -117,12 +176,36 @@ uint32_t allocate_slots(const nxg_expr_program &p, std::vector<uint16_t> &slot_o
             slot = free_slots.top();
             free_slots.pop();
         }
-        slot_of[i] = static_cast<uint16_t>(slot);
+        plan.slot_of[i] = static_cast<uint16_t>(slot);
         if (last_use[i] < 0) free_slots.push(slot);  // never read: free after the write
     }
-    return next;
+    plan.slot_count = next;
+    // operand encodings and immediates
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!plan.dispatched[i]) continue;
+        ++plan.dispatched_count;
+        plan.imm0[i] = ins(i).immediate;
+        uint32_t k = ins(i).op == NXG_OP_RANDOM || ins(i).op == NXG_OP_CONSTANT ? 1u : 0u;
+        std::vector<std::pair<uint32_t, uint16_t>> seen;  // constant -> encoding within this reader
+        for (uint32_t j = 0; j < ins(i).operand_count; ++j) {
+            const uint32_t v = opnd(i, j);
+            uint16_t enc = plan.slot_of[v];
+            if (inline_const[v]) {
+                enc = 0xffffu;
+                for (const auto &e : seen)
+                    if (e.first == v) enc = e.second;
+                if (enc == 0xffffu) {
+                    enc = static_cast<uint16_t>(kImmediate | k);
+                    (k == 0u ? plan.imm0[i] : plan.imm1[i]) = ins(v).immediate;
+                    seen.push_back({v, enc});
+                    ++k;
+                }
+            }
+            plan.operand[i].push_back(enc);
+        }
+    }
+    return plan;
 }
-
 }  // namespace
 }  // namespace nxg
 
@@ -139,8 +222,8 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
     if (!programs || count == 0u || count > NXG_EXPR_MAX_PROGRAMS) return NXG_ERROR_INVALID_PROGRAM;
     uint64_t total = align16(sizeof(PackedSet));
     uint32_t max_values = 0, maps_required = 0;
-    std::vector<std::vector<uint16_t>> slots(count);
-    std::vector<uint32_t> slot_counts(count);
+    std::vector<PackPlan> plans;
+    plans.reserve(count);
     for (uint32_t k = 0; k < count; ++k) {
         if (!validate_program(programs[k])) return NXG_ERROR_INVALID_PROGRAM;
         for (uint32_t i = 0; i < programs[k].instruction_count; ++i) {
@@ -148,8 +231,8 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
             if (in.op == NXG_OP_TEXEL && in.auxiliary + 1u > maps_required) maps_required = in.auxiliary + 1u;
         }
         total += program_bytes(programs[k]);
-        slot_counts[k] = allocate_slots(programs[k], slots[k]);
-        if (slot_counts[k] > max_values) max_values = slot_counts[k];
+        plans.push_back(plan_program(programs[k]));
+        if (plans.back().slot_count > max_values) max_values = plans.back().slot_count;
     }
     if (!info) return NXG_ERROR_NULL_ARGUMENT;
     info->byte_size = total;
@@ -165,13 +248,14 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
     uint64_t at = align16(sizeof(PackedSet));
     for (uint32_t k = 0; k < count; ++k) {
         const nxg_expr_program &p = programs[k];
-        const std::vector<uint16_t> &slot_of = slots[k];
+        const PackPlan &plan = plans[k];
         set.program_offset[k] = static_cast<uint32_t>(at);
         PackedProgram pp{};
-        pp.instruction_count = p.instruction_count;
-        pp.result = slot_of[p.result];
+        pp.instruction_count = plan.dispatched_count;
+        pp.result = plan.slot_of[p.result];
         pp.input_count = p.input_count;
-        pp.slot_count = slot_counts[k];
+        pp.slot_count = plan.slot_count;
+        pp.var_mask = plan.var_mask;
         for (uint32_t i = 0; i < p.input_count; ++i) pp.input_bindings[i] = p.input_bindings[i];
         uint64_t cursor = at + align16(sizeof(PackedProgram));
         pp.instructions_offset = static_cast<uint32_t>(cursor);
@@ -185,19 +269,22 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
         std::memcpy(base + at, &pp, sizeof(pp));
         PackedInstruction *ins = reinterpret_cast<PackedInstruction *>(base + pp.instructions_offset);
         uint32_t *operand_slots = reinterpret_cast<uint32_t *>(base + pp.operands_offset);
+        uint32_t d = 0;
         for (uint32_t i = 0; i < p.instruction_count; ++i) {
+            if (!plan.dispatched[i]) continue;
             const nxg_expr_instruction &in = p.instructions[i];
-            PackedInstruction &o = ins[i];
+            PackedInstruction &o = ins[d++];
             o.op = in.op;
             o.operand_count = in.operand_count;
-            o.dest = slot_of[i];
+            o.dest = plan.slot_of[i];
             o.operand_offset = in.operand_offset;
-            o.auxiliary = in.op == NXG_OP_INPUT ? p.input_bindings[in.auxiliary] : in.auxiliary;
-            o.immediate = in.immediate;
+            o.auxiliary = in.auxiliary;
+            o.immediate = plan.imm0[i];
+            o.immediate1 = plan.imm1[i];
             for (uint32_t j = 0; j < in.operand_count; ++j) {
-                const uint16_t s = slot_of[p.operands[in.operand_offset + j]];
-                if (in.operand_count <= 4u) o.slot[j] = s;
-                operand_slots[in.operand_offset + j] = s;
+                const uint16_t e = plan.operand[i][j];
+                if (in.operand_count <= 4u) o.slot[j] = e;
+                operand_slots[in.operand_offset + j] = e;
             }
         }
         if (p.ramp_count)

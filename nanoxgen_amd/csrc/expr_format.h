@@ -28,19 +28,26 @@ struct PackedSet {
 };
 
 struct PackedProgram {
-    uint32_t instruction_count;
-    uint32_t result;  // slot holding the result value
+    uint32_t instruction_count;    // dispatched instructions
+    uint32_t result;               // slot holding the result value
     uint32_t input_count;
     uint32_t instructions_offset;  // instruction_count + 1 entries (a zeroed one ends the array)
-    uint32_t operands_offset;      // slots of the operands of instructions with more than 4
+    uint32_t operands_offset;      // operand slots of instructions with more than 4 operands
     uint32_t ramps_offset;
     uint32_t ramp_points_offset;
     uint32_t slot_count;
+    uint32_t var_mask;             // root variables the program reads (checked finite once per root)
+    uint32_t reserved[3];
     uint32_t input_bindings[kMaxInputs];
 };
 
-// One instruction, 32 bytes (one scalar load).  For NXG_OP_INPUT the
-// auxiliary holds the bound root variable, which is also its slot.
+// Operand slot encoding: below kImmediate a value slot; kImmediate | k the
+// instruction's k-th immediate (an inlined finite constant).
+constexpr uint16_t kImmediate = 0x8000u;
+
+// One dispatched instruction, 32 bytes (one scalar load).  Inputs are not
+// dispatched (their uses read the root variable's slot) and finite constants
+// used only by instructions of at most 4 operands are inlined as immediates.
 struct PackedInstruction {
     uint8_t op;
     uint8_t reserved;
@@ -49,9 +56,9 @@ struct PackedInstruction {
     uint16_t reserved2;
     uint32_t operand_offset;  // operand_count > 4: index into the program's operand slots
     uint32_t auxiliary;
-    float immediate;
-    uint16_t slot[4];         // operand_count <= 4: operand slots
-    uint32_t reserved3;
+    float immediate;          // constant value, random call-site seed, or immediate 0
+    uint16_t slot[4];         // operand_count <= 4: operand slots (see kImmediate)
+    float immediate1;         // immediate 1
 };
 static_assert(sizeof(PackedInstruction) == 32, "packed instruction is 32 bytes");
 

@@ -1238,14 +1238,22 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
         const ImageOf(nxg_expr_ramp) ramps = (ImageOf(nxg_expr_ramp))(image + prog->ramps_offset);
         const ImageOf(nxg_expr_ramp_point) points = (ImageOf(nxg_expr_ramp_point))(image + prog->ramp_points_offset);
         const uint32_t n = prog->instruction_count;
+        // inputs are not dispatched: the root variables the program reads are
+        // checked once here (an input is an error when its value is not finite)
+        const uint32_t var_mask = prog->var_mask;
+#pragma unroll
+        for (uint32_t q = 0; q < NXG_ROOT_VAR_COUNT; ++q)
+            if (((var_mask >> q) & 1u) && !isfinite(vars[q])) bad = true;
         // the next instruction is fetched while the current one runs (the
         // array ends with a zeroed entry)
         PackedInstruction next = load_instruction(ins);
         for (uint32_t i = 0; i < n; ++i) {
             const PackedInstruction in = next;
             next = load_instruction(ins + i + 1u);
-#define OPV(j) SLOT_GET(uint32_t{in.slot[j]})
-#define OPA(j) SLOT_GET(operand_slots[in.operand_offset + (j)])
+#define OPERAND(e)                                                                          \
+    (((e) & kImmediate) ? (((e) & 1u) ? in.immediate1 : in.immediate) : SLOT_GET(e))
+#define OPV(j) OPERAND(uint32_t{in.slot[j]})
+#define OPA(j) OPERAND(operand_slots[in.operand_offset + (j)])
             float value = 0.0f;
             switch (in.op) {
                 case NXG_OP_CONSTANT: value = in.immediate; break;
@@ -1321,6 +1329,7 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
             }
 #undef OPV
 #undef OPA
+#undef OPERAND
             if (!isfinite(value)) bad = true;
             SLOT_SET(uint32_t{in.dest}, value);
         }

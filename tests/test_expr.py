@@ -166,6 +166,9 @@ def test_pack_layout_and_limits():
     assert buf[:4].view(np.uint32)[0] == 0x5045584E
     # liveness-based slots: far fewer than one per SSA value
     assert ROOT_VARS <= info.max_values < max(len(q.instructions) for q in progs)
+    # constants and inputs are folded: only the 40 ops are dispatched
+    header = buf[:32].view(np.uint32)
+    assert [int(buf[int(header[4 + k]):][:4].view(np.uint32)[0]) for k in range(3)] == [40, 40, 40]
     check_slot_allocation(buf, progs)
     assert L.nxg_expr_pack(arr, 0, None, 0, C.byref(info)) == abi.INVALID_PROGRAM
     five = (abi.ExprProgram * 5)(*([progs[0].struct] * 5))
@@ -195,40 +198,69 @@ def test_unit_attributes_reproduce_plain_generation(orc):
 
 
 ROOT_VARS = len(abi.ROOT_VARS)
+IMMEDIATE = 0x8000
 PACKED_INSTRUCTION = np.dtype([("op", "u1"), ("reserved", "u1"), ("operand_count", "<u2"), ("dest", "<u2"),
                                ("reserved2", "<u2"), ("operand_offset", "<u4"), ("auxiliary", "<u4"),
-                               ("immediate", "<f4"), ("slot", "<u2", 4), ("reserved3", "<u4")])
+                               ("immediate", "<f4"), ("slot", "<u2", 4), ("immediate1", "<f4")])
 
 
 def check_slot_allocation(image, progs):
-    """Replays the packed instruction stream on value identities: every
-    operand slot must still hold the SSA value the program names when the
-    instruction reads it, and the result slot the result at the end (inputs
-    are their root variable's reserved slot)."""
+    """Replays the packed instruction stream on value identities (an SSA
+    value, a root variable for inputs, the bits of a constant): every
+    dispatched instruction's operands must still hold the values the program
+    names when it reads them (or carry the constant as an immediate), inputs
+    are never dispatched and their variables are in the program's var mask,
+    and the result slot holds the result at the end."""
     assert PACKED_INSTRUCTION.itemsize == 32
     header = image[:32].view(np.uint32)
     slots = int(header[2])
     for k, q in enumerate(progs):
         off = int(header[4 + k])
-        pp = image[off:off + 160].view(np.uint32)
-        n, result, ins_off, opnd_off, slot_count = int(pp[0]), int(pp[1]), int(pp[3]), int(pp[4]), int(pp[7])
-        assert n == len(q.instructions) and slot_count <= slots
+        pp = image[off:off + 176].view(np.uint32)
+        n, result, ins_off, opnd_off, slot_count, var_mask = (int(pp[0]), int(pp[1]), int(pp[3]), int(pp[4]),
+                                                               int(pp[7]), int(pp[8]))
+        assert n <= len(q.instructions) and slot_count <= slots
         ins = image[ins_off:ins_off + 32 * (n + 1)].view(PACKED_INSTRUCTION)
         assert ins[n]["op"] == 0 and ins[n]["operand_count"] == 0  # zeroed end entry
         opnd = image[opnd_off:opnd_off + 4 * len(q.operands)].view(np.uint32)
         bindings = q.bindings
-        ident = lambda v: ("var", int(bindings[q.instructions[v][3]])) if q.instructions[v][0] == abi.OP["input"] else v
+        const_bits = lambda v: np.float32(q.instructions[v][4]).view(np.uint32).item()
+
+        def ident(v):
+            op = q.instructions[v][0]
+            if op == abi.OP["input"]:
+                return ("var", int(bindings[q.instructions[v][3]]))
+            if op == abi.OP["constant"]:
+                return ("c", const_bits(v))
+            return v
+
         held = {s: ("var", s) for s in range(ROOT_VARS)}
+        d = 0
         for i, (op, off_i, cnt, aux, imm) in enumerate(q.instructions):
+            if op == abi.OP["input"]:
+                assert (var_mask >> int(bindings[aux])) & 1
+                continue
+            if op == abi.OP["constant"]:
+                if d < n and ins[d]["op"] == op and ins[d]["immediate"].view(np.uint32) == const_bits(i):
+                    held[int(ins[d]["dest"])] = ident(i)
+                    d += 1
+                continue  # otherwise inlined into its readers
+            assert d < n and ins[d]["op"] == op, (k, i)
+            x = ins[d]
             for j in range(cnt):
                 v = q.operands[off_i + j]
-                slot = int(ins[i]["slot"][j]) if cnt <= 4 else int(opnd[ins[i]["operand_offset"] + j])
-                assert slot == int(opnd[off_i + j]) and slot < slot_count
-                assert held.get(slot) == ident(v), (k, i, j)
-            dest = int(ins[i]["dest"])
-            assert dest < slot_count and (op != abi.OP["input"] or dest == int(bindings[aux]))
-            assert dest >= ROOT_VARS or op == abi.OP["input"]
+                e = int(x["slot"][j]) if cnt <= 4 else int(opnd[x["operand_offset"] + j])
+                assert e == int(opnd[off_i + j])
+                if e & IMMEDIATE:
+                    val = x["immediate1"] if e & 1 else x["immediate"]
+                    assert ("c", np.float32(val).view(np.uint32).item()) == ident(v), (k, i, j)
+                else:
+                    assert e < slot_count and held.get(e) == ident(v), (k, i, j)
+            dest = int(x["dest"])
+            assert ROOT_VARS <= dest < slot_count
             held[dest] = ident(i)
+            d += 1
+        assert d == n
         assert held.get(result) == ident(q.result)
 
 
