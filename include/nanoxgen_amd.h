@@ -308,10 +308,13 @@ uint32_t nxg_generate_packed(const nxg_device_asset *asset,
 /* The two stream-ordered halves of nxg_generate_packed.  prepare fills the
  * workspace with the per-guide state and the guide offset table, which
  * depend only on the asset, the deformed geometry and cvs_per_strand (the
- * reference recomputes them inside interpolate_offset, generate.h:246-267);
+ * reference recomputes them inside interpolate_offset, generate.h:246-267),
+ * and with the processing order of the strands (grouped by root triangle;
+ * it depends on the asset, seed and strand_count and changes no output);
  * generate_packed_prepared runs the strand kernel from a workspace that a
- * prepare call with the same asset, deformed geometry and cvs_per_strand
- * filled earlier on the same stream (or one ordered before it). */
+ * prepare call with the same asset, deformed geometry, cvs_per_strand, seed
+ * and strand_count filled earlier on the same stream (or one ordered before
+ * it). */
 uint32_t nxg_prepare_packed_generation(const nxg_device_asset *asset,
                                        const nxg_deformed_geometry *deformed,
                                        const nxg_generation_params *params,

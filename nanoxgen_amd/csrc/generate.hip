@@ -649,7 +649,9 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
             a.roots[strand] = o;
         }
         if (a.root_uvs) a.root_uvs[strand] = {r.u, r.v};
-        if (a.point_counts && !a.order) a.point_counts[strand] = a.cvs;  // else written in strand order
+        // every count is cvs, so writing them by processing position fills the
+        // same array with coalesced stores whatever the order
+        if (a.point_counts) a.point_counts[slot] = a.cvs;
         if (a.stencil_indices) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -863,7 +865,6 @@ struct OrderArgs {
     uint32_t seed, triangle_count, strand_count, cvs;
     uint32_t shift, tiles;
     uint32_t *counts, *partials, *order;
-    uint32_t *point_counts;  // optional, written here in strand order
 };
 
 __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t tile) {
@@ -875,7 +876,6 @@ __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t ti
         const uint32_t s = first + i;
         if (s >= o.strand_count) break;
         atomicAdd(&hist[sample_triangle(o.alias, o.seed, o.triangle_count, s) >> o.shift], 1u);
-        if (o.point_counts) o.point_counts[s] = o.cvs;
     }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) o.counts[b * o.tiles + tile] = hist[b];
@@ -1491,7 +1491,7 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
                            order_shift(a.triangle_count), order_tiles(a.strand_count),
                            reinterpret_cast<uint32_t *>(ws + L.counts_offset),
                            reinterpret_cast<uint32_t *>(ws + L.partials_offset),
-                           reinterpret_cast<uint32_t *>(ws + L.order_offset), a.point_counts};
+                           reinterpret_cast<uint32_t *>(ws + L.order_offset)};
     a.order = plan.ordered ? plan.order.order : nullptr;
     return NXG_OK;
 }
@@ -1512,10 +1512,11 @@ uint32_t launch_plan(const Plan &plan, const nxg_generation_params &params, hipS
         hipLaunchKernelGGL(guide_prep_kernel, dim3(static_cast<uint32_t>((plan.prep_lanes + 255u) / 256u)),
                            dim3(256), 0, stream, plan.p);
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
+        // the strand order depends on the asset, seed and strand count only
+        if (plan.ordered)
+            if (uint32_t s = launch_order(plan.order, stream); s != NXG_OK) return s;
     }
     if (!(stages & kStrands)) return NXG_OK;
-    if (plan.ordered)
-        if (uint32_t s = launch_order(plan.order, stream); s != NXG_OK) return s;
     const Args &a = plan.a;
     const dim3 grid((params.strand_count + 255u) / 256u), block(256);
     if (!plan.noise) {
