@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "output points/sec + % HBM roofline; surrogate vs plain expr at <=1e-3 rel err"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_GINST = 614.4  # 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 VALU instruction
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "strand_kernel_plane2M_pmc.json")
 
 
@@ -377,10 +378,18 @@ def main():
     points = N * Cv
     alg_bytes = points * 16 + N * 4 + blob.nbytes
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    traffic = None
+    traffic = valu = None
     if os.path.exists(PMC_SUMMARY) and args.workload == "plane":
         with open(PMC_SUMMARY) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            pmc = json.load(f)
+        traffic = pmc.get("hbm_bytes_per_launch")
+        # VALU issue roofline: wave64 VALU instructions per launch (profiled)
+        # over the kernel time, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+        insts = pmc.get("sq", {}).get("SQ_INSTS_VALU")
+        if insts:
+            rate = insts / (kernel_ms * 1e-3)
+            valu = {"achieved": round(rate / 1e9, 2), "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                    "frac": round(rate / 1e9 / VALU_PEAK_GINST, 4), "instructions_per_launch": insts}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": ("strand_kernel<true, %d>" % (w.octaves.count if w.octaves else 1)
@@ -390,7 +399,7 @@ def main():
                 "kernel_ms_p90": round(kern_ms_sorted[min(len(kern_ms) - 1, int(0.9 * len(kern_ms)))], 4),
                 ("attributes_ms_mean" if attrs is not None else "prep_and_order_ms_mean"):
                     round(sum(prep_ms) / len(prep_ms), 4),
-                "algorithmic_bytes_per_launch": alg_bytes}
+                "algorithmic_bytes_per_launch": alg_bytes, "valu": valu}
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
