@@ -214,7 +214,8 @@ enum {
     NXG_ERROR_INVALID_PROGRAM = 30,
     NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL = 31,
     NXG_ERROR_INVALID_MLP_REQUEST = 32,
-    NXG_ERROR_INVALID_BATCH = 33
+    NXG_ERROR_INVALID_BATCH = 33,
+    NXG_ERROR_INVALID_CLASSIC_REQUEST = 34
 };
 
 /* device_generation_error_message (device_contract.h:82-116); extension codes
@@ -664,6 +665,82 @@ uint32_t nxg_batch_tuning_candidates(nxg_batch_tuning *candidates, uint32_t capa
 uint32_t nxg_batch_autotune(const nxg_batch_item *items, uint32_t count, uint32_t *counter, void *stream,
                             const nxg_batch_tuning *candidates, uint32_t candidate_count, uint32_t repeats,
                             nxg_batch_tuning *best, float *times_ms);
+
+/* ---- Classic CutFX on generated curves -------------------------------- */
+
+/* The reference's Classic float runtime (apply_xgen_classic_float_runtime_
+ * plan_cpu, include/nanoxgen/xgen_classic_runtime.h:185-190) run with one
+ * CutFX module whose amount is amount_scale * $cLength + amount_offset
+ * (times an optional per-strand multiplier), no length/width/taper
+ * expressions, on packed curves such as nxg_generate_packed writes.  Per
+ * strand (src/xgen_classic_runtime.cpp:1648-1678, 1806-1821):
+ *   cLength = curve_spline_length (2n+4 chords of the uniform cubic
+ *             B-spline with reflected ends, src/xgen_classic_runtime.cpp:
+ *             587-663); width = 2 * radius[0] / radius_scale (0 when
+ *             radius_scale <= 0);
+ *   culled when cLength < 1e-4 or width < 1e-4;
+ *   cut_from_tip_and_rebuild (src/xgen_classic_runtime.cpp:692-743):
+ *             walk back from the tip in 1/(2n+4) parameter steps until
+ *             max(amount, 0) of chord length is removed, then resample n
+ *             CVs uniformly on [0, cut]; culled when cut < 1e-4;
+ * then the kept strands are compacted in order, with their point counts,
+ * roots and root uvs (src/xgen_classic_runtime.cpp:1893-1918).  The
+ * device counterpart in the reference is make_classic_runtime_cut_kernel
+ * (src/luisa/xgen_classic_runtime.cpp:409-468), which flags dropped strands
+ * instead of compacting them. */
+#define NXG_CLASSIC_MAX_CVS 256u
+#define NXG_CLASSIC_MAX_STRANDS (1u << 30)
+#define NXG_CLASSIC_NONFINITE_AMOUNT 1u /* "Classic CutFX amount produced a non-finite value" */
+#define NXG_CLASSIC_NONFINITE_WIDTH 2u  /* "Classic width expression produced a non-finite value" */
+
+typedef struct nxg_classic_curves {
+    const nxg_packed_point *points;  /* strand_count x cvs_per_strand, strand-major, 16-byte aligned */
+    const uint32_t *point_counts;    /* optional: copied for kept strands (else cvs_per_strand) */
+    const nxg_root_sample *roots;    /* optional */
+    const nxg_vec2 *root_uvs;        /* optional */
+    uint32_t strand_count;           /* 0..NXG_CLASSIC_MAX_STRANDS */
+    uint32_t cvs_per_strand;         /* 2..NXG_CLASSIC_MAX_CVS */
+} nxg_classic_curves;
+
+typedef struct nxg_classic_cut_params {
+    float amount_scale;          /* amount = (amount_scale * cLength + amount_offset) * multiplier[s] */
+    float amount_offset;
+    float radius_scale;          /* the radius_scale the curves were written with */
+    uint32_t multiplier_stride;  /* floats between consecutive strands' multipliers (4: an attribute column) */
+    const float *multiplier;     /* optional, device memory */
+} nxg_classic_cut_params;
+
+/* Outputs must not overlap the inputs.  Capacities are in elements and must
+ * hold every input strand (the kept count is known only on the device). */
+typedef struct nxg_classic_output {
+    nxg_packed_point *points;
+    uint32_t *point_counts;      /* optional */
+    nxg_root_sample *roots;      /* optional; needs curves.roots */
+    nxg_vec2 *root_uvs;          /* optional; needs curves.root_uvs */
+    uint64_t point_capacity;
+    uint64_t point_count_capacity;
+    uint64_t root_capacity;
+    uint64_t root_uv_capacity;
+    uint32_t *kept_count;        /* device word: strands written (0 when status != 0) */
+    uint32_t *status;            /* optional device word: 0 or NXG_CLASSIC_NONFINITE_* bits */
+} nxg_classic_output;
+
+/* Host-side checks of a request; NXG_OK or the first failing code. */
+uint32_t nxg_validate_classic_cut(const nxg_classic_curves *curves, const nxg_classic_cut_params *cut,
+                                  const nxg_classic_output *output);
+
+/* Device workspace for nxg_classic_cut_apply: per-strand cut parameters and
+ * the per-tile kept counts. */
+uint64_t nxg_classic_cut_workspace_size(uint32_t strand_count);
+
+/* On `stream`, without synchronising or allocating: the cut chain above
+ * over every strand, then the stable compaction.  Where the reference would
+ * throw (a non-finite width or amount on any strand) nothing is written,
+ * *kept_count is 0 and *status carries the NXG_CLASSIC_NONFINITE_* bits.
+ * Positions follow the reference's CPU operation order bit for bit. */
+uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, const nxg_classic_cut_params *cut,
+                               const nxg_classic_output *output, void *workspace, uint64_t workspace_bytes,
+                               void *stream);
 
 /* Library build identity: gfx target and compile flags that affect parity. */
 const char *nxg_build_info(void);

@@ -250,6 +250,30 @@ class MotionOutput(C.Structure):
     _fields_ = [("points", C.c_void_p), ("point_capacity", C.c_uint64)]
 
 
+CLASSIC_MAX_CVS = 256
+CLASSIC_NONFINITE_AMOUNT, CLASSIC_NONFINITE_WIDTH = 1, 2
+
+
+class ClassicCurves(C.Structure):
+    """nxg_classic_curves: packed input curves (device pointers)."""
+    _fields_ = [("points", C.c_void_p), ("point_counts", C.c_void_p), ("roots", C.c_void_p),
+                ("root_uvs", C.c_void_p), ("strand_count", C.c_uint32), ("cvs_per_strand", C.c_uint32)]
+
+
+class ClassicCutParams(C.Structure):
+    """nxg_classic_cut_params: amount = (amount_scale * cLength + amount_offset) * multiplier."""
+    _fields_ = [("amount_scale", C.c_float), ("amount_offset", C.c_float), ("radius_scale", C.c_float),
+                ("multiplier_stride", C.c_uint32), ("multiplier", C.c_void_p)]
+
+
+class ClassicOutput(C.Structure):
+    """nxg_classic_output: compacted curves plus the device kept-count and status words."""
+    _fields_ = [("points", C.c_void_p), ("point_counts", C.c_void_p), ("roots", C.c_void_p),
+                ("root_uvs", C.c_void_p), ("point_capacity", C.c_uint64), ("point_count_capacity", C.c_uint64),
+                ("root_capacity", C.c_uint64), ("root_uv_capacity", C.c_uint64), ("kept_count", C.c_void_p),
+                ("status", C.c_void_p)]
+
+
 BATCH_MAX_ITEMS = 12
 BATCH_ORDER_GIVEN, BATCH_ORDER_HEAVY_FIRST = 0, 1
 
@@ -283,6 +307,7 @@ STATUS = [
     "OUTPUT_SIZE_OVERFLOW", "HIP", "WORKSPACE_TOO_SMALL", "INVALID_OCTAVES", "NULL_ARGUMENT",
     "STENCIL_CAPACITY_TOO_SMALL", "INVALID_ROOTS", "INVALID_BUILD_INPUT", "BUFFER_TOO_SMALL",
     "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL", "INVALID_MLP_REQUEST", "INVALID_BATCH",
+    "INVALID_CLASSIC_REQUEST",
 ]
 globals().update({name: i for i, name in enumerate(STATUS)})
 

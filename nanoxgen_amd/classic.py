@@ -1,0 +1,103 @@
+"""Classic CutFX on device curves (host side of nxg_classic_cut_apply).
+
+The reference runs a Classic description's CutFX module inside its float
+runtime (apply_xgen_classic_float_runtime_plan_cpu,
+include/nanoxgen/xgen_classic_runtime.h:185-190): per strand cLength, the
+width/length culling, cut_from_tip_and_rebuild with the module's amount
+expression, then an in-order compaction of the kept strands
+(src/xgen_classic_runtime.cpp:1648-1678, 1806-1821, 1893-1918).  Here the
+amount is amount_scale * cLength + amount_offset, optionally times a
+per-strand multiplier (e.g. one column of the attribute runtime's output),
+and the whole chain runs on the GPU into separate output buffers.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi, check, lib, _stream_handle, _torch
+
+
+class ClassicCutOutputs:
+    """Compacted outputs sized for every input strand, plus the device
+    kept-count and status words."""
+
+    def __init__(self, strands, cvs, device="cuda", point_counts=True, roots=True, root_uvs=True):
+        torch = _torch()
+        self.strands, self.cvs = strands, cvs
+        self.points = torch.empty((max(1, strands * cvs), 4), dtype=torch.float32, device=device)
+        self.point_counts = torch.empty(max(1, strands), dtype=torch.int32, device=device) if point_counts else None
+        self.roots = torch.empty((max(1, strands), 12), dtype=torch.int32, device=device) if roots else None
+        self.root_uvs = torch.empty((max(1, strands), 2), dtype=torch.float32, device=device) if root_uvs else None
+        self.words = torch.zeros(64, dtype=torch.int32, device=device)  # [0] kept count, [32] status
+
+    def descriptor(self):
+        t = lambda x: None if x is None else x.data_ptr()
+        n = lambda x, k: 0 if x is None else self.strands * k
+        return abi.ClassicOutput(points=self.points.data_ptr(), point_counts=t(self.point_counts),
+                                 roots=t(self.roots), root_uvs=t(self.root_uvs),
+                                 point_capacity=self.strands * self.cvs, point_count_capacity=n(self.point_counts, 1),
+                                 root_capacity=n(self.roots, 1), root_uv_capacity=n(self.root_uvs, 1),
+                                 kept_count=self.words.data_ptr(), status=self.words.data_ptr() + 128)
+
+    def kept(self):
+        """Kept strand count (synchronises)."""
+        return int(self.words[0].item())
+
+    def status(self):
+        """0, or the NXG_CLASSIC_NONFINITE_* bits of the call (synchronises)."""
+        return int(self.words[32].item())
+
+    def to_host(self):
+        """The kept strands as numpy arrays (synchronises)."""
+        k = self.kept()
+        out = dict(strand_count=k, status=self.status(), points=self.points[:k * self.cvs].cpu().numpy())
+        if self.point_counts is not None:
+            out["point_counts"] = self.point_counts[:k].cpu().numpy().view(np.uint32)
+        if self.roots is not None:
+            out["roots"] = self.roots[:k].cpu().numpy().view(abi.ROOT_DTYPE).reshape(-1)
+        if self.root_uvs is not None:
+            out["root_uvs"] = self.root_uvs[:k].cpu().numpy()
+        return out
+
+
+def workspace_bytes(strands):
+    return int(lib().nxg_classic_cut_workspace_size(strands))
+
+
+def classic_cut(points, cvs, amount_scale, amount_offset, *, point_counts=None, roots=None, root_uvs=None,
+                multiplier=None, multiplier_stride=1, radius_scale=1.0, outputs=None, workspace=None,
+                stream=None):
+    """Enqueue the CutFX chain on `stream` over device curves: `points`
+    float32 [N * cvs, 4]; optional `point_counts` int32 [N], `roots` int32
+    [N, 12] (RootSample records), `root_uvs` float32 [N, 2] and `multiplier`
+    float32 (strand s reads multiplier.flatten()[s * multiplier_stride]).
+    Returns the ClassicCutOutputs (its kept() and to_host() synchronise)."""
+    torch = _torch()
+    n = points.shape[0] // cvs
+    if outputs is None:
+        outputs = ClassicCutOutputs(n, cvs, device=points.device, point_counts=True, roots=roots is not None,
+                                    root_uvs=root_uvs is not None)
+    if workspace is None:
+        workspace = torch.empty(workspace_bytes(n), dtype=torch.uint8, device=points.device)
+    if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
+        raise ValueError("points must be a contiguous float32 [N * cvs, 4] tensor")
+    if multiplier is not None:
+        reach = (n - 1) * multiplier_stride + 1 if n else 0
+        have = multiplier.untyped_storage().nbytes() // 4 - multiplier.storage_offset()
+        if multiplier.dtype != torch.float32 or multiplier_stride < 1 or have < reach:
+            raise ValueError(f"multiplier must be float32 with {reach} elements from its first at stride "
+                             f"{multiplier_stride}")
+    for name, x, k in (("point_counts", point_counts, 1), ("roots", roots, 12), ("root_uvs", root_uvs, 2)):
+        if x is not None and (x.numel() < n * k or not x.is_contiguous()):
+            raise ValueError(f"{name} must be contiguous with {n * k} elements")
+    t = lambda x: None if x is None else x.data_ptr()
+    curves = abi.ClassicCurves(points=points.data_ptr(), point_counts=t(point_counts), roots=t(roots),
+                               root_uvs=t(root_uvs), strand_count=n, cvs_per_strand=cvs)
+    cut = abi.ClassicCutParams(amount_scale=amount_scale, amount_offset=amount_offset, radius_scale=radius_scale,
+                               multiplier_stride=multiplier_stride if multiplier is not None else 0,
+                               multiplier=t(multiplier))
+    out = outputs.descriptor()
+    check(lib().nxg_classic_cut_apply(C.byref(curves), C.byref(cut), C.byref(out),
+                                      C.c_void_p(workspace.data_ptr()), workspace.numel(), _stream_handle(stream)),
+          "nxg_classic_cut_apply")
+    return outputs

@@ -1,0 +1,373 @@
+// Classic CutFX on generated curves (gfx950): cLength, culling, cut from the
+// tip, uniform rebuild and stable compaction, in the operation order of the
+// reference's CPU float runtime (src/xgen_classic_runtime.cpp:587-663 spline
+// evaluation and length, 692-743 cut_from_tip_and_rebuild, 1648-1678 and
+// 1806-1821 the per-strand chain, 1893-1918 compaction), so positions are
+// bit-identical to it (no FP contraction, correctly rounded div/sqrt).
+//
+// Four launches on the caller's stream, none of them allocating:
+//   cut_search_kernel   one 64-lane wavefront per tile of 64 strands.  The
+//                       tile's points are one contiguous run of 64 x n
+//                       float4s: the wave reads it coalesced into LDS,
+//                       CV-major with row c rotated by c lanes, so the
+//                       staging stores and the lanes' data-dependent CV
+//                       reads are both bank-conflict-free.  Each lane
+//                       measures its spline (2n+4 chords), culls, walks
+//                       back from the tip and stores its cut parameter
+//                       (-1: culled); lane 0 stores the tile's kept count
+//                       and the error bits the tile raised.
+//   cut_scan_tiles      exclusive scan of the tile counts, kScanBlock tiles
+//                       per block, each block's total into the group array.
+//   cut_scan_groups     one block: exclusive scan of the group totals, the
+//                       kept count and the call's status word.
+//   cut_write_kernel    the tile again through LDS; each kept lane resamples
+//                       its n CVs on [0, cut] and writes them, with its
+//                       point count, root and root uv, at its compacted slot
+//                       (group prefix + tile prefix + rank in the wave).
+// Above 64 CVs a tile no longer fits 64 KB of LDS and the lanes read their
+// CVs from memory instead (same arithmetic).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/nanoxgen_amd.h"
+#include "internal.h"
+
+namespace nxg {
+namespace {
+
+constexpr uint32_t kWave = kClassicTile;  // strands per tile = lanes per wavefront
+constexpr uint32_t kStagedCvs = 64u;      // 64 strands x 64 CVs x 16 B = 64 KB of LDS
+constexpr uint32_t kErrShift = 30u;
+constexpr uint32_t kCountMask = (1u << kErrShift) - 1u;
+constexpr uint32_t kScanThreads = 256u;
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float length_squared(F3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+
+struct CutArgs {
+    const float4 *points;
+    const uint32_t *counts;
+    const nxg_root_sample *roots;
+    const nxg_vec2 *uvs;
+    uint32_t strands, cvs;
+    uint32_t div_magic;  // i / cvs == __umulhi(i, div_magic) for i < 2^24
+    float scale, offset, radius_scale;
+    const float *multiplier;
+    uint32_t multiplier_stride;
+    float *cut;
+    uint32_t *tiles, *groups, *status_word;
+    uint32_t tile_count, group_count;
+    float4 *out_points;
+    uint32_t *out_counts;
+    nxg_root_sample *out_roots;
+    nxg_vec2 *out_uvs;
+    uint32_t *kept, *status;
+};
+
+// A lane's CVs: staged in LDS, or read from memory.
+struct StagedCurve {
+    const float4 *tile;
+    uint32_t lane;
+    __device__ __forceinline__ float4 operator[](uint32_t c) const {
+        return tile[c * kWave + ((lane + c) & (kWave - 1u))];
+    }
+};
+struct MemoryCurve {
+    const float4 *strand;
+    __device__ __forceinline__ float4 operator[](uint32_t c) const { return strand[c]; }
+};
+
+// The tile's `count` strands, coalesced, into LDS (CV-major, rotated rows).
+__device__ __forceinline__ void stage_tile(const CutArgs &a, float4 *tile, uint32_t first, uint32_t count) {
+    const float4 *src = a.points + uint64_t{first} * a.cvs;
+    const uint32_t total = count * a.cvs;
+    for (uint32_t i = threadIdx.x; i < total; i += kWave) {
+        const uint32_t l = __umulhi(i, a.div_magic);
+        const uint32_t c = i - l * a.cvs;
+        tile[c * kWave + ((l + c) & (kWave - 1u))] = src[i];
+    }
+    __syncthreads();
+}
+
+// xgen_curve_eval (src/xgen_classic_runtime.cpp:614-642): uniform cubic
+// B-spline through n CVs with reflected end points.
+template <class Curve>
+__device__ __forceinline__ F3 curve_eval(const Curve &p, uint32_t n, float t) {
+    if (t < 1.0e-7f) return xyz(p[0]);
+    if (t > 1.0f - 1.0e-7f) return xyz(p[n - 1u]);
+    const uint32_t spans = n - 1u;
+    const float scaled = t * static_cast<float>(spans);
+    const uint32_t raw = static_cast<uint32_t>(scaled);
+    const float f = scaled - static_cast<float>(raw);
+    const uint32_t span = raw < n - 2u ? raw : n - 2u;  // t < 1 keeps raw <= n - 2; the clamp guards memory only
+    const float f2 = f * f;
+    const float f3 = f2 * f;
+    const float omf = 1.0f - f;
+    const float b0 = omf * omf * omf;
+    const float b1 = 3.0f * f3 - 6.0f * f2 + 4.0f;
+    const float b2 = -3.0f * f3 + 3.0f * f2 + 3.0f * f + 1.0f;
+    const float b3 = f3;
+    const F3 p1 = xyz(p[span]);
+    const F3 p2 = xyz(p[span + 1u]);
+    const F3 q0 = xyz(p[span == 0u ? 0u : span - 1u]);
+    const F3 q3 = xyz(p[span + 2u >= n ? n - 1u : span + 2u]);
+    const F3 p0 = span == 0u ? p1 * 2.0f - p2 : q0;
+    const F3 p3 = span + 2u >= n ? p2 * 2.0f - p1 : q3;
+    return (p0 * b0 + p1 * b1 + p2 * b2 + p3 * b3) * (1.0f / 6.0f);
+}
+
+// curve_spline_length (src/xgen_classic_runtime.cpp:644-663).
+template <class Curve>
+__device__ __forceinline__ float spline_length(const Curve &p, uint32_t n) {
+    const uint32_t intervals = n * 2u + 4u;
+    const float step = 1.0f / static_cast<float>(intervals);
+    F3 previous = xyz(p[0]);
+    float result = 0.0f;
+    for (uint32_t k = 1u; k < intervals; ++k) {
+        const F3 current = curve_eval(p, n, step * static_cast<float>(k));
+        result += sqrtf(length_squared(current - previous));
+        previous = current;
+    }
+    result += sqrtf(length_squared(xyz(p[n - 1u]) - previous));
+    return result;
+}
+
+// The search half of cut_from_tip_and_rebuild (src/xgen_classic_runtime.cpp:
+// 701-728): the rebuild parameter for a finite amount >= 0 (0: nothing
+// left).  The parameter falls by `step` per pass and is clamped at 0, so the
+// walk ends within 2n+5 passes; the guard is never reached.
+template <class Curve>
+__device__ __forceinline__ float cut_parameter(const Curve &p, uint32_t n, float amount) {
+    if (!(amount >= 1.0e-10f)) return 1.0f;
+    const float step = 1.0f / (2.0f * static_cast<float>(n) + 4.0f);
+    float previous_parameter = 1.0f;
+    F3 previous = xyz(p[n - 1u]);
+    float accumulated = 0.0f;
+    for (uint32_t guard = 0u; guard < 4u * n + 16u; ++guard) {
+        const float lowered = previous_parameter - step;
+        const float parameter = lowered < 0.0f ? 0.0f : lowered;
+        const F3 current = curve_eval(p, n, parameter);
+        const float segment = sqrtf(length_squared(previous - current));
+        accumulated += segment;
+        if (accumulated >= amount && segment > 0.0f)
+            return parameter + ((accumulated - amount) / segment) * (previous_parameter - parameter);
+        if (parameter <= 1.0e-10f) return 0.0f;
+        previous_parameter = parameter;
+        previous = current;
+    }
+    return 0.0f;
+}
+
+// One strand of the chain (src/xgen_classic_runtime.cpp:1650-1678, 1806-1821):
+// the cut parameter of a kept strand, -1 for a culled one; *err gets the
+// NXG_CLASSIC_NONFINITE_* bit where the reference throws.
+template <class Curve>
+__device__ __forceinline__ float strand_cut(const CutArgs &a, const Curve &p, uint32_t s, uint32_t *err) {
+    const uint32_t n = a.cvs;
+    const float c_length = spline_length(p, n);
+    const float radius = p[0].w;
+    const float width = a.radius_scale > 0.0f ? 2.0f * radius / a.radius_scale : 0.0f;
+    if (!isfinite(width)) {
+        *err = NXG_CLASSIC_NONFINITE_WIDTH;
+        return -1.0f;
+    }
+    if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
+    float amount = a.scale * c_length + a.offset;
+    if (a.multiplier) amount = amount * a.multiplier[uint64_t{s} * a.multiplier_stride];
+    if (!isfinite(amount)) {
+        *err = NXG_CLASSIC_NONFINITE_AMOUNT;
+        return -1.0f;
+    }
+    const float cut = cut_parameter(p, n, amount < 0.0f ? 0.0f : amount);
+    return cut < 1.0e-4f ? -1.0f : cut;
+}
+
+template <bool STAGED>
+__global__ void __launch_bounds__(kWave) cut_search_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    if constexpr (STAGED) stage_tile(a, tile, first, count);
+    const uint32_t s = first + lane;
+    float cut = -1.0f;
+    uint32_t err = 0u;
+    if (lane < count) {
+        if constexpr (STAGED)
+            cut = strand_cut(a, StagedCurve{tile, lane}, s, &err);
+        else
+            cut = strand_cut(a, MemoryCurve{a.points + uint64_t{s} * a.cvs}, s, &err);
+        a.cut[s] = cut;
+    }
+    const uint64_t kept = __ballot(cut >= 0.0f);
+    const uint32_t bits = (__ballot(err & NXG_CLASSIC_NONFINITE_AMOUNT) ? NXG_CLASSIC_NONFINITE_AMOUNT : 0u) |
+                          (__ballot(err & NXG_CLASSIC_NONFINITE_WIDTH) ? NXG_CLASSIC_NONFINITE_WIDTH : 0u);
+    if (lane == 0u) a.tiles[t] = static_cast<uint32_t>(__popcll(kept)) | (bits << kErrShift);
+}
+
+// Exclusive scan of one value per thread of a 256-thread block, and the
+// block's total and OR of `flags`.
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t flags, uint32_t &total, uint32_t &all_flags) {
+    __shared__ uint32_t sums[kScanThreads / 64u], ors[kScanThreads / 64u];
+    const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
+    uint32_t inc = v, fl = flags;
+#pragma unroll
+    for (uint32_t d = 1u; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+        fl |= __shfl_xor(fl, d, 64);
+    }
+    if (lane == 63u) {
+        sums[wave] = inc;
+        ors[wave] = fl;
+    }
+    __syncthreads();
+    uint32_t before = 0u;
+    total = 0u;
+    all_flags = 0u;
+#pragma unroll
+    for (uint32_t w = 0u; w < kScanThreads / 64u; ++w) {
+        if (w < wave) before += sums[w];
+        total += sums[w];
+        all_flags |= ors[w];
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) cut_scan_tiles_kernel(CutArgs a) {
+    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+    uint32_t v[4], sum = 0u, flags = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        const uint32_t w = base + j < a.tile_count ? a.tiles[base + j] : 0u;
+        v[j] = w & kCountMask;
+        flags |= w >> kErrShift;
+        sum += v[j];
+    }
+    uint32_t total, all_flags;
+    uint32_t run = block_scan(sum, flags, total, all_flags);
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        if (base + j < a.tile_count) a.tiles[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0u) a.groups[blockIdx.x] = total | (all_flags << kErrShift);
+}
+
+__global__ void __launch_bounds__(kScanThreads) cut_scan_groups_kernel(CutArgs a) {
+    uint32_t carry = 0u, flags = 0u;
+    for (uint32_t first = 0u; first < a.group_count; first += kScanThreads) {
+        const uint32_t i = first + threadIdx.x;
+        const uint32_t w = i < a.group_count ? a.groups[i] : 0u;
+        uint32_t total, all_flags;
+        const uint32_t ex = block_scan(w & kCountMask, w >> kErrShift, total, all_flags);
+        if (i < a.group_count) a.groups[i] = carry + ex;
+        carry += total;
+        flags |= all_flags;
+    }
+    if (threadIdx.x == 0u) {
+        *a.status_word = flags;
+        if (a.status) *a.status = flags;
+        *a.kept = flags ? 0u : carry;
+    }
+}
+
+template <bool STAGED>
+__global__ void __launch_bounds__(kWave) cut_write_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    const uint32_t s = first + lane;
+    const float cut = lane < count ? a.cut[s] : -1.0f;
+    const uint64_t kept = __ballot(cut >= 0.0f);
+    if (kept == 0u) return;  // the block is this one wavefront
+    if constexpr (STAGED) stage_tile(a, tile, first, count);
+    if (!(cut >= 0.0f)) return;
+    const uint32_t n = a.cvs;
+    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
+    const uint32_t slot = a.groups[t / kScanBlock] + a.tiles[t] + rank;
+    float4 *dst = a.out_points + uint64_t{slot} * n;
+    auto rebuild = [&](const auto &p) {  // src/xgen_classic_runtime.cpp:729-742
+        const float step = cut / static_cast<float>(n - 1u);
+        dst[0] = p[0];
+        for (uint32_t i = 1u; i < n; ++i) {
+            const F3 q = curve_eval(p, n, step * static_cast<float>(i));
+            dst[i] = make_float4(q.x, q.y, q.z, p[i].w);
+        }
+    };
+    if constexpr (STAGED)
+        rebuild(StagedCurve{tile, lane});
+    else
+        rebuild(MemoryCurve{a.points + uint64_t{s} * n});
+    if (a.out_counts) a.out_counts[slot] = a.counts ? a.counts[s] : n;
+    if (a.out_roots) a.out_roots[slot] = a.roots[s];
+    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
+}
+
+}  // namespace
+}  // namespace nxg
+
+extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, const nxg_classic_cut_params *cut,
+                                          const nxg_classic_output *output, void *workspace,
+                                          uint64_t workspace_bytes, void *stream_ptr) {
+    using namespace nxg;
+    if (uint32_t s = nxg_validate_classic_cut(curves, cut, output); s != NXG_OK) return s;
+    const uint32_t n = curves->strand_count, cvs = curves->cvs_per_strand;
+    const ClassicLayout layout = classic_layout(n);
+    if (!workspace || workspace_bytes < layout.total || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
+        return NXG_ERROR_WORKSPACE_TOO_SMALL;
+    uint8_t *ws = static_cast<uint8_t *>(workspace);
+    CutArgs a{};
+    a.points = reinterpret_cast<const float4 *>(curves->points);
+    a.counts = curves->point_counts;
+    a.roots = curves->roots;
+    a.uvs = curves->root_uvs;
+    a.strands = n;
+    a.cvs = cvs;
+    a.div_magic = static_cast<uint32_t>(0xffffffffu / cvs + 1u);
+    a.scale = cut->amount_scale;
+    a.offset = cut->amount_offset;
+    a.radius_scale = cut->radius_scale;
+    a.multiplier = cut->multiplier;
+    a.multiplier_stride = cut->multiplier_stride;
+    a.cut = reinterpret_cast<float *>(ws + layout.cut_offset);
+    a.tiles = reinterpret_cast<uint32_t *>(ws + layout.tiles_offset);
+    a.groups = reinterpret_cast<uint32_t *>(ws + layout.groups_offset);
+    a.status_word = reinterpret_cast<uint32_t *>(ws + layout.status_offset);
+    a.tile_count = classic_tiles(n);
+    a.group_count = classic_groups(n);
+    a.out_points = reinterpret_cast<float4 *>(output->points);
+    a.out_counts = output->point_counts;
+    a.out_roots = output->roots;
+    a.out_uvs = output->root_uvs;
+    a.kept = output->kept_count;
+    a.status = output->status;
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    const bool staged = cvs <= kStagedCvs;
+    const size_t lds = staged ? size_t{kWave} * cvs * sizeof(float4) : 0u;
+    if (a.tile_count) {
+        if (staged)
+            hipLaunchKernelGGL(cut_search_kernel<true>, dim3(a.tile_count), dim3(kWave), lds, stream, a);
+        else
+            hipLaunchKernelGGL(cut_search_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
+        hipLaunchKernelGGL(cut_scan_tiles_kernel, dim3(a.group_count), dim3(kScanThreads), 0, stream, a);
+    }
+    hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
+    if (a.tile_count) {
+        if (staged)
+            hipLaunchKernelGGL(cut_write_kernel<true>, dim3(a.tile_count), dim3(kWave), lds, stream, a);
+        else
+            hipLaunchKernelGGL(cut_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
+    }
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}

@@ -156,6 +156,8 @@ const char *nxg_status_message(uint32_t status) {
         case NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL: return "device attribute buffer is too small";
         case NXG_ERROR_INVALID_MLP_REQUEST: return "surrogate request is invalid";
         case NXG_ERROR_INVALID_BATCH: return "batch item count or tuning is invalid";
+        case NXG_ERROR_INVALID_CLASSIC_REQUEST:
+            return "Classic cut curves, parameters or outputs are invalid (CV count, alignment, overlap)";
     }
     return "unknown device generation error";
 }
@@ -201,3 +203,46 @@ uint64_t nxg_motion_generation_workspace_size(const nxg_asset_header *header, co
 }
 
 }  // extern "C"
+
+namespace {
+bool overlaps(const void *a, uint64_t a_bytes, const void *b, uint64_t b_bytes) {
+    if (!a || !b || a_bytes == 0u || b_bytes == 0u) return false;
+    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    return x < y + b_bytes && y < x + a_bytes;
+}
+}  // namespace
+
+uint32_t nxg_validate_classic_cut(const nxg_classic_curves *curves, const nxg_classic_cut_params *cut,
+                                  const nxg_classic_output *output) {
+    if (!curves || !cut || !output || !output->kept_count) return NXG_ERROR_NULL_ARGUMENT;
+    const uint64_t n = curves->strand_count, cvs = curves->cvs_per_strand;
+    if (cvs < 2u || cvs > NXG_CLASSIC_MAX_CVS || n > NXG_CLASSIC_MAX_STRANDS)
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (cut->multiplier && cut->multiplier_stride == 0u) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (n == 0u) return NXG_OK;
+    if (!curves->points || !output->points) return NXG_ERROR_NULL_ARGUMENT;
+    if (reinterpret_cast<uintptr_t>(curves->points) % 16u != 0u ||
+        reinterpret_cast<uintptr_t>(output->points) % 16u != 0u)
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (output->point_capacity < n * cvs) return NXG_ERROR_POINT_CAPACITY_TOO_SMALL;
+    if (output->point_counts && output->point_count_capacity < n) return NXG_ERROR_POINT_COUNT_CAPACITY_TOO_SMALL;
+    if (output->roots && !curves->roots) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (output->roots && output->root_capacity < n) return NXG_ERROR_ROOT_CAPACITY_TOO_SMALL;
+    if (output->root_uvs && !curves->root_uvs) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (output->root_uvs && output->root_uv_capacity < n) return NXG_ERROR_ROOT_UV_CAPACITY_TOO_SMALL;
+    // the kernels read every input strand while others are being written
+    const uint64_t point_bytes = n * cvs * sizeof(nxg_packed_point);
+    const void *outs[] = {output->points, output->point_counts, output->roots, output->root_uvs,
+                          output->kept_count, output->status};
+    const uint64_t out_bytes[] = {point_bytes, output->point_counts ? n * 4u : 0u,
+                                  output->roots ? n * sizeof(nxg_root_sample) : 0u,
+                                  output->root_uvs ? n * sizeof(nxg_vec2) : 0u, 4u, 4u};
+    const void *ins[] = {curves->points, curves->point_counts, curves->roots, curves->root_uvs};
+    const uint64_t in_bytes[] = {point_bytes, n * 4u, n * sizeof(nxg_root_sample), n * sizeof(nxg_vec2)};
+    for (int o = 0; o < 6; ++o)
+        for (int i = 0; i < 4; ++i)
+            if (overlaps(outs[o], out_bytes[o], ins[i], in_bytes[i])) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    return NXG_OK;
+}
+
+uint64_t nxg_classic_cut_workspace_size(uint32_t strand_count) { return nxg::classic_layout(strand_count).total; }

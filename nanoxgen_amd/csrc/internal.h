@@ -48,6 +48,36 @@ inline uint32_t order_tiles(uint32_t strand_count) { return (strand_count + kOrd
 WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand, uint32_t strand_count,
                                  uint32_t triangle_count);
 
+// Classic cut workspace, each region 256-byte aligned:
+//   cut parameters   strand_count float: the rebuild parameter of a kept
+//                    strand, -1 for a culled one
+//   tile counts      one u32 per 64-strand tile: kept strands (bits 0..29)
+//                    and the NXG_CLASSIC_NONFINITE_* bits seen (30..31),
+//                    scanned in place into the tile's first output slot
+//                    within its group of kScanBlock tiles
+//   group totals     one u32 per kScanBlock tiles, scanned likewise
+//   status           one u32: the error bits of the whole call
+struct ClassicLayout {
+    uint64_t cut_offset;
+    uint64_t tiles_offset;
+    uint64_t groups_offset;
+    uint64_t status_offset;
+    uint64_t total;
+};
+constexpr uint32_t kClassicTile = 64u;
+inline uint32_t classic_tiles(uint32_t strand_count) { return (strand_count + kClassicTile - 1u) / kClassicTile; }
+inline uint32_t classic_groups(uint32_t strand_count) { return (classic_tiles(strand_count) + kScanBlock - 1u) / kScanBlock; }
+inline ClassicLayout classic_layout(uint32_t strand_count) {
+    auto a = [](uint64_t v) { return (v + 255u) & ~uint64_t{255u}; };
+    ClassicLayout l{};
+    l.cut_offset = 0u;
+    l.tiles_offset = a(uint64_t{strand_count} * 4u);
+    l.groups_offset = l.tiles_offset + a(uint64_t{classic_tiles(strand_count)} * 4u);
+    l.status_offset = l.groups_offset + a(uint64_t{classic_groups(strand_count)} * 4u);
+    l.total = l.status_offset + 256u;
+    return l;
+}
+
 // Asset builder (asset_builder.cpp).  build_blob with host_stencil = false
 // leaves the triangle stencil zeroed and the content hash unset; finish_blob
 // sets the hash once the stencil is in place.

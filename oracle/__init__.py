@@ -72,6 +72,14 @@ def lib():
         L.orc_root_attributes.restype = C.c_uint32
         L.orc_root_attributes.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                           C.c_void_p, C.c_uint32]
+        L.orc_curve_eval.restype = C.c_float
+        L.orc_curve_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]
+        L.orc_curve_spline_length.restype = C.c_float
+        L.orc_curve_spline_length.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_classic_cut.restype = C.c_uint32
+        L.orc_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                      C.c_float, C.c_float, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -102,6 +110,9 @@ def ref():
         L.ref_hash32.argtypes = [C.c_uint32]
         L.ref_random01.restype = C.c_float
         L.ref_random01.argtypes = [C.c_uint32] * 3
+        L.ref_classic_cut.restype = C.c_uint32
+        L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                      C.c_float, C.c_float, C.c_float]
         _REF = L
     return _REF
 
@@ -280,3 +291,51 @@ def expr_eval(program, root_vars, u=0.0, v=0.0, face_seed=0.0, t=0.0, gradients=
     rc = L.orc_expr_eval(C.byref(program.struct), rv.ctypes.data, u, v, face_seed, t, g, mstructs, len(mkeep),
                          scratch.ctypes.data, C.byref(out))
     return None if rc else out.value
+
+
+def curve_spline_length(points):
+    """curve_spline_length of one strand (float32 [C, 4] or [C, 3])."""
+    p = np.zeros((len(points), 4), np.float32)
+    p[:, :3] = np.asarray(points, np.float32)[:, :3]
+    return lib().orc_curve_spline_length(p.ctypes.data, len(p))
+
+
+def classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, roots=None, uvs=None, multiplier=None,
+                radius_scale=1.0):
+    """The CutFX chain (cLength, culling, cut from tip, compaction) over
+    packed float4 points [N * C, 4]: returns dict(points, point_counts,
+    roots, root_uvs, keep, strand_count), or None where the reference
+    would throw (a non-finite amount)."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    counts = np.full(n, cvs, np.uint32) if counts is None else np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros_like(pts)
+    oc = np.zeros(n, np.uint32)
+    orr = np.zeros(n, abi.ROOT_DTYPE) if roots is not None else None
+    ou = np.zeros((n, 2), np.float32) if uvs is not None else None
+    keep = np.zeros(n, np.uint8)
+    m = None if multiplier is None else np.ascontiguousarray(multiplier, np.float32)
+    kept = lib().orc_classic_cut(pts.ctypes.data, counts.ctypes.data, abi.ptr(roots), abi.ptr(uvs), n, cvs,
+                                 amount_scale, amount_offset, abi.ptr(m), radius_scale, out.ctypes.data,
+                                 oc.ctypes.data, abi.ptr(orr), abi.ptr(ou), keep.ctypes.data)
+    if kept == 0xFFFFFFFF:
+        return None
+    return dict(points=out[:kept * cvs], point_counts=oc[:kept], roots=None if orr is None else orr[:kept],
+                root_uvs=None if ou is None else ou[:kept], keep=keep, strand_count=kept)
+
+
+def ref_classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, roots=None, uvs=None,
+                    radius_scale=1.0):
+    """The reference's apply_xgen_classic_float_runtime_plan_cpu with one
+    CutFX module (amount = amount_scale * $cLength + amount_offset)."""
+    pts = np.array(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    counts = np.full(n, cvs, np.uint32) if counts is None else np.array(counts, np.uint32)
+    rr = np.zeros(n, abi.ROOT_DTYPE) if roots is None else np.array(roots)
+    uu = None if uvs is None else np.array(uvs, np.float32)
+    kept = ref().ref_classic_cut(pts.ctypes.data, counts.ctypes.data, rr.ctypes.data, abi.ptr(uu), n, cvs,
+                                 amount_scale, amount_offset, radius_scale)
+    if kept == 0xFFFFFFFF:
+        return None
+    return dict(points=pts[:kept * cvs], point_counts=counts[:kept], roots=rr[:kept],
+                root_uvs=None if uu is None else uu[:kept], strand_count=kept)

@@ -4,6 +4,7 @@
 // write golden fixtures; nothing here ships or runs on the GPU.
 #include "nanoxgen/asset.h"
 #include "nanoxgen/generate.h"
+#include "nanoxgen/xgen_classic_runtime.h"
 #include "nanoxgen/xgen_expression.h"
 
 #include <cstdio>
@@ -200,6 +201,58 @@ int ref_expr_eval(const nxg_expr_instruction *ins, uint32_t n_ins, const uint32_
         return 0;
     } catch (const std::exception &) {
         return 1;
+    }
+}
+
+// apply_xgen_classic_float_runtime_plan_cpu (xgen_classic_runtime.cpp:1580-1918)
+// with one CutFX module whose amount is amount_scale * $cLength +
+// amount_offset (an IR program over the bound input "cLength"), no length,
+// width or taper expressions.  Strands are packed float4 points; roots,
+// uvs and counts are carried.  Returns the kept strand count (outputs
+// compacted in place), or ~0u where the reference throws.
+uint32_t ref_classic_cut(float *points, uint32_t *counts, nxg_root_sample *roots, nxg_vec2 *uvs,
+                         uint32_t strand_count, uint32_t cvs, float amount_scale, float amount_offset,
+                         float radius_scale) {
+    try {
+        PackedGeneratedCurves curves{};
+        curves.strand_count = strand_count;
+        curves.cvs_per_strand = cvs;
+        curves.point_counts.assign(counts, counts + strand_count);
+        curves.points.resize(std::size_t{strand_count} * cvs);
+        std::memcpy(curves.points.data(), points, curves.points.size() * sizeof(PackedCurvePoint));
+        curves.roots.resize(strand_count);
+        std::memcpy(curves.roots.data(), roots, std::size_t{strand_count} * sizeof(RootSample));
+        if (uvs) {
+            curves.root_uvs.resize(strand_count);
+            std::memcpy(curves.root_uvs.data(), uvs, std::size_t{strand_count} * sizeof(Vec2));
+        }
+        XgenFloatExpressionProgram amount{};
+        amount.inputs = {"cLength"};
+        amount.instructions = {{XgenScalarOp::input, 0u, 0u, 0u, 0.0f},
+                               {XgenScalarOp::constant, 0u, 0u, 0u, amount_scale},
+                               {XgenScalarOp::multiply, 0u, 2u, 0u, 0.0f},
+                               {XgenScalarOp::constant, 0u, 0u, 0u, amount_offset},
+                               {XgenScalarOp::add, 2u, 2u, 0u, 0.0f}};
+        amount.operands = {1u, 0u, 2u, 3u};
+        amount.result = 4u;
+        ClassicFloatRuntimePlan plan{};
+        plan.description_name = "cut";
+        plan.fx_cv_count = cvs;
+        ClassicFloatCutModule cut{};
+        cut.amount.object_type = "CutFXModule";
+        cut.amount.attribute = "amount";
+        cut.amount.program = amount;
+        cut.rebuild_type = 1u;
+        plan.cuts.push_back(cut);
+        plan.effects.push_back({ClassicFloatEffectType::Cut, 0u});
+        apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale);
+        std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
+        std::memcpy(counts, curves.point_counts.data(), curves.point_counts.size() * sizeof(uint32_t));
+        std::memcpy(roots, curves.roots.data(), curves.roots.size() * sizeof(RootSample));
+        if (uvs) std::memcpy(uvs, curves.root_uvs.data(), curves.root_uvs.size() * sizeof(Vec2));
+        return curves.strand_count;
+    } catch (const std::exception &) {
+        return ~0u;
     }
 }
 

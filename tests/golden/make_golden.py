@@ -8,10 +8,13 @@ into oracle/_ref/libnxg_ref.so) and records, for small seeded inputs:
   * generate_packed_cpu / _deformed_cpu / _roots_cpu outputs,
   * gradient_noise at seeded sample points, hash32 and random01 values,
   * make_guide_blend_state slots and make_strand_generation_state fields.
+  * the Classic CutFX chain (cLength, width/length culling, cut from the
+    tip, compaction) of apply_xgen_classic_float_runtime_plan_cpu over
+    generated and random strands (reference_classic.npz).
 The fixtures are data (inputs and expected outputs); the reference itself
 never leaves this container.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [generation] [expressions] [classic]
 """
 import os
 import sys
@@ -229,6 +232,72 @@ def main_expressions():
           "programs raise in the reference")
 
 
+CLASSIC_CASES = [  # (strand source, amount_scale, amount_offset, radius_scale)
+    ("ico5", 0.25, 0.0, 1.0),
+    ("ico5", 0.0, 0.3, 1.0),
+    ("ico13", 0.5, 0.05, 1.0),
+    ("walk", 0.4, 0.1, 2.0),
+    ("walk", 1.0, 0.0, 1.0),
+    ("walk", 0.0, 0.0, 1.0),
+]
+
+
+def classic_strands(name):
+    """Packed float4 strands for the CutFX fixtures: the reference generator's
+    own icosphere strands (5 and 13 CVs), and seeded random walks of 7 CVs
+    with zero-width, zero-length and very short strands among them."""
+    if name.startswith("ico"):
+        cvs = int(name[3:])
+        with np.load(os.path.join(HERE, "reference_generation.npz"), allow_pickle=False) as d:
+            blob = d["ico8_blob"].copy()
+        p = workloads.config1().params.copy(strand_count=300, cvs_per_strand=cvs, seed=31 + cvs)
+        out = oracle.ref_generate(blob, p, workers=3, block=16)
+        return out["points"].reshape(-1, 4), cvs, out["roots"], out["root_uvs"]
+    rng = np.random.default_rng(41)
+    n, cvs = 400, 7
+    steps = rng.normal(0.0, 1.0, size=(n, cvs, 3)).astype(np.float32) * rng.uniform(0.0, 0.5, size=(n, 1, 1))
+    steps[:, 0] = rng.uniform(-5, 5, size=(n, 3))
+    pts = np.cumsum(steps, axis=1).astype(np.float32)
+    rad = rng.uniform(0.0, 0.02, size=(n, cvs)).astype(np.float32)
+    rad[::17] = 0.0                                 # culled by width
+    pts[5::23, 1:] = pts[5::23, :1]                 # zero length
+    pts[7::29, 1:] = pts[7::29, :1] + 1.0e-5        # below the 1e-4 length floor
+    packed = np.concatenate([pts, rad[..., None]], axis=2).reshape(-1, 4)
+    roots = np.zeros(n, abi.ROOT_DTYPE)
+    roots["triangle_index"] = np.arange(n, dtype=np.uint32) * 3
+    roots["barycentric"] = rng.uniform(0, 1, size=(n, 2)).astype(np.float32)
+    roots["position"] = pts[:, 0]
+    uvs = rng.uniform(0, 1, size=(n, 2)).astype(np.float32)
+    return packed, cvs, roots, uvs
+
+
+def main_classic():
+    oracle.build()
+    if oracle.ref() is None:
+        raise SystemExit("reference core not built (needs /root/reference)")
+    fx = {}
+    for src in sorted({c[0] for c in CLASSIC_CASES}):
+        pts, cvs, roots, uvs = classic_strands(src)
+        fx[f"{src}_points"], fx[f"{src}_roots"], fx[f"{src}_uvs"] = pts, roots, uvs
+        fx[f"{src}_cvs"] = np.uint32(cvs)
+    for i, (src, scale, offset, rscale) in enumerate(CLASSIC_CASES):
+        cvs = int(fx[f"{src}_cvs"])
+        out = oracle.ref_classic_cut(fx[f"{src}_points"], cvs, scale, offset, roots=fx[f"{src}_roots"],
+                                     uvs=fx[f"{src}_uvs"], radius_scale=rscale)
+        fx[f"case{i}_args"] = np.array([scale, offset, rscale], np.float32)
+        for k in ("points", "point_counts", "roots", "root_uvs"):
+            fx[f"case{i}_out_{k}"] = out[k]
+    path = os.path.join(HERE, "reference_classic.npz")
+    np.savez_compressed(path, **fx)
+    print("wrote", path, os.path.getsize(path), "bytes;",
+          [int(len(fx[f"case{i}_out_point_counts"])) for i in range(len(CLASSIC_CASES))], "strands kept")
+
+
 if __name__ == "__main__":
-    main()
-    main_expressions()
+    which = sys.argv[1:] or ["generation", "expressions", "classic"]
+    if "generation" in which:
+        main()
+    if "expressions" in which:
+        main_expressions()
+    if "classic" in which:
+        main_classic()

@@ -287,3 +287,58 @@ def test_motion_contract():
                                  ws.ctypes.data, ws.nbytes, None) == abi.INVALID_MOTION_TIMES
     one = L.nxg_packed_generation_workspace_size(C.byref(r.desc.header), C.byref(r.params))
     assert L.nxg_motion_generation_workspace_size(C.byref(r.desc.header), C.byref(r.params), 3) == 3 * one
+
+
+def test_classic_cut_contract():
+    L = nx.lib()
+    n, cvs = 100, 6
+    pin = np.zeros((n * cvs, 4), np.float32)
+    pout = np.zeros((n * cvs, 4), np.float32)
+    counts = np.zeros(n, np.uint32)
+    roots_in = np.zeros(n, abi.ROOT_DTYPE)
+    roots_out = np.zeros(n, abi.ROOT_DTYPE)
+    words = np.zeros(64, np.uint32)
+    curves = abi.ClassicCurves(points=pin.ctypes.data, strand_count=n, cvs_per_strand=cvs)
+    cut = abi.ClassicCutParams(amount_scale=0.25, radius_scale=1.0)
+    out = abi.ClassicOutput(points=pout.ctypes.data, point_capacity=n * cvs, kept_count=words.ctypes.data)
+    v = lambda c=curves, k=cut, o=out: L.nxg_validate_classic_cut(C.byref(c), C.byref(k), C.byref(o))
+    assert v() == abi.OK
+    assert L.nxg_validate_classic_cut(None, C.byref(cut), C.byref(out)) == abi.NULL_ARGUMENT
+    for bad_cvs in (0, 1, abi.CLASSIC_MAX_CVS + 1):
+        assert v(abi.ClassicCurves(points=pin.ctypes.data, strand_count=n, cvs_per_strand=bad_cvs)) \
+            == abi.INVALID_CLASSIC_REQUEST
+    # zero strands need no buffers
+    assert v(abi.ClassicCurves(strand_count=0, cvs_per_strand=cvs),
+             o=abi.ClassicOutput(kept_count=words.ctypes.data)) == abi.OK
+    short = abi.ClassicOutput(points=pout.ctypes.data, point_capacity=n * cvs - 1, kept_count=words.ctypes.data)
+    assert v(o=short) == abi.POINT_CAPACITY_TOO_SMALL
+    no_kept = abi.ClassicOutput(points=pout.ctypes.data, point_capacity=n * cvs)
+    assert v(o=no_kept) == abi.NULL_ARGUMENT
+    # outputs may not overlap inputs, and must be 16-byte aligned
+    alias = abi.ClassicOutput(points=pin.ctypes.data + 16 * cvs, point_capacity=n * cvs,
+                              kept_count=words.ctypes.data)
+    assert v(o=alias) == abi.INVALID_CLASSIC_REQUEST
+    odd = abi.ClassicOutput(points=pout.ctypes.data + 4, point_capacity=n * cvs, kept_count=words.ctypes.data)
+    assert v(o=odd) == abi.INVALID_CLASSIC_REQUEST
+    # roots out need roots in; capacities per buffer
+    with_roots = abi.ClassicOutput(points=pout.ctypes.data, point_capacity=n * cvs, roots=roots_out.ctypes.data,
+                                   root_capacity=n, kept_count=words.ctypes.data)
+    assert v(o=with_roots) == abi.INVALID_CLASSIC_REQUEST
+    rin = abi.ClassicCurves(points=pin.ctypes.data, roots=roots_in.ctypes.data, strand_count=n,
+                            cvs_per_strand=cvs)
+    assert v(rin, o=with_roots) == abi.OK
+    with_roots.root_capacity = n - 1
+    assert v(rin, o=with_roots) == abi.ROOT_CAPACITY_TOO_SMALL
+    cnt = abi.ClassicOutput(points=pout.ctypes.data, point_capacity=n * cvs, point_counts=counts.ctypes.data,
+                            point_count_capacity=n - 1, kept_count=words.ctypes.data)
+    assert v(o=cnt) == abi.POINT_COUNT_CAPACITY_TOO_SMALL
+    assert v(k=abi.ClassicCutParams(multiplier=pin.ctypes.data, multiplier_stride=0)) == abi.INVALID_CLASSIC_REQUEST
+    # workspace: 256-byte regions for cut parameters, tile counts, group totals and status
+    assert L.nxg_classic_cut_workspace_size(0) == 256  # the status word alone
+    assert L.nxg_classic_cut_workspace_size(2_000_000) >= 2_000_000 * 4 + 31250 * 4
+    # refused on the host before any device work
+    ws = np.zeros(16, np.uint8)
+    assert L.nxg_classic_cut_apply(C.byref(curves), C.byref(cut), C.byref(out), ws.ctypes.data, ws.nbytes,
+                                   None) == abi.WORKSPACE_TOO_SMALL
+    assert L.nxg_classic_cut_apply(C.byref(curves), C.byref(cut), C.byref(alias), ws.ctypes.data, ws.nbytes,
+                                   None) == abi.INVALID_CLASSIC_REQUEST
