@@ -284,18 +284,136 @@ def bench_batch(args):
     return 0
 
 
+CUT_PMC_SUMMARY = os.path.join(ROOT, "profiles", "cut_search_kernel_plane2M_pmc.json")
+
+
+def bench_classic(args):
+    """Classic CutFX (nxg_classic_cut_apply) over the plane workload's
+    generated curves: 2,000,000 strands x 16 CVs with point counts, roots and
+    root uvs, amount = 0.25 x cLength.  One step = the whole chain (search,
+    two scan launches, rebuild + compaction); the curves are generated once,
+    before the timed region, and stay resident in HBM."""
+    import numpy as np
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import workloads
+    from nanoxgen_amd.classic import ClassicCutOutputs, classic_cut, workspace_bytes
+
+    scale, offset = 0.25, 0.0
+    w = workloads.config2()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    blob = nx.build_asset(w.build)
+    asset = nx.DeviceAsset(blob)
+    gen = nx.PackedOutputs(N, Cv)
+    nx.generate_packed(asset, w.params, gen, octaves=w.octaves)
+    out = ClassicCutOutputs(N, Cv)
+    ws = torch.empty(workspace_bytes(N), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    log(f"generated {N} x {Cv} curves")
+
+    def step():
+        classic_cut(gen.points, Cv, scale, offset, point_counts=gen.point_counts, roots=gen.roots,
+                    root_uvs=gen.root_uvs, outputs=out, workspace=ws, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        events[i][0].record(stream)
+        step()
+        events[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    chain_ms = sorted(e[0].elapsed_time(e[1]) for e in events)
+    chain_mean = sum(chain_ms) / len(chain_ms)
+    kept = out.kept()
+    assert out.status() == 0
+    # algorithmic bytes of one chain: every input strand read (points, count,
+    # root, uv), every kept strand written, 4 B of cut parameter written and
+    # read per strand, the tile counts negligible
+    per_in, per_out = Cv * 16 + 4 + 48 + 8, Cv * 16 + 4 + 48 + 8
+    alg = N * per_in + kept * per_out + N * 8
+    achieved = alg / (chain_mean * 1e-3) / 1e9
+    valu = traffic = None
+    if os.path.exists(CUT_PMC_SUMMARY):
+        with open(CUT_PMC_SUMMARY) as f:
+            pmc = json.load(f)
+        traffic = pmc.get("hbm_bytes_per_chain")
+        insts = pmc.get("valu_instructions_per_chain")
+        if insts:
+            rate = insts / (chain_mean * 1e-3)
+            valu = {"achieved": round(rate / 1e9, 2), "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                    "frac": round(rate / 1e9 / VALU_PEAK_GINST, 4), "instructions_per_chain": insts}
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "cut chain: cut_search_kernel<true> + cut_scan_tiles + cut_scan_groups + "
+                          "cut_write_kernel<true> (one C-ABI call)",
+                "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
+                "algorithmic_bytes_per_launch": alg, "valu": valu}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = classic_cpu_baseline(gen, N, Cv, scale, offset, args.cpu_seconds)
+    line = {"metric": METRIC, "value": kept * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "classic-cut-plane2M", "input": w.name, "strands": N, "cvs_per_strand": Cv,
+                       "kept_strands": kept, "amount": f"{scale} * cLength + {offset}",
+                       "parallelism": "single-gpu"},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def classic_cpu_baseline(gen, N, Cv, scale, offset, target_s):
+    """The reference's own apply_xgen_classic_float_runtime_plan_cpu (built
+    from its sources into oracle/_ref) on the first S generated strands, with
+    the worker count it picks itself (the process's CPU affinity); without
+    that build, the oracle's single-threaded restatement."""
+    import numpy as np
+    import oracle
+    host = gen.to_host()
+    pts, counts, roots, uvs = host["points"], host["point_counts"], host["roots"], host["root_uvs"]
+    use_ref = oracle.ref() is not None
+    cores = len(os.sched_getaffinity(0)) if use_ref else 1
+
+    def run(s):
+        if use_ref:
+            r = oracle.ref_classic_cut(pts[:s * Cv], Cv, scale, offset, counts=counts[:s], roots=roots[:s],
+                                       uvs=uvs[:s])
+            return r["seconds"]
+        t = time.perf_counter()
+        oracle.classic_cut(pts[:s * Cv], Cv, scale, offset)
+        return time.perf_counter() - t
+
+    s = min(N, 65536)
+    dt = run(s)
+    s = int(min(N, max(s, s * target_s / max(dt, 1e-6))))
+    times = [run(s)]
+    while s == N and sum(times) < target_s and len(times) < 5:
+        times.append(run(s))
+    dt = sorted(times)[len(times) // 2]
+    what = ("the reference's apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread pool)"
+            if use_ref else "the oracle's restatement (oracle/oracle_classic.c), one thread")
+    return {"value": s * Cv / dt, "unit": "points/s", "cores": cores, "kind": "reference" if use_ref else "port",
+            "sample": f"strands 0..{s - 1} of the generated curves ({s * Cv} points), one CutFX at "
+                      f"{scale} * cLength, median {dt:.3f} s of {len(times)} run(s); {what}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch"],
+    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch", "classic"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
                          "the trained surrogate + sampled exact check against the exact runtime; "
-                         "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph")
+                         "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph; "
+                         "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands")
     ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
     ap.add_argument("--sample-stride", type=int, default=16, help="surrogate: exact check of every n-th root")
     ap.add_argument("--train-epochs", type=int, default=40)
@@ -318,6 +436,8 @@ def main():
         return bench_surrogate(args)
     if args.workload == "batch" and args.gpus == 1:
         return bench_batch(args)
+    if args.workload == "classic" and args.gpus == 1:
+        return bench_classic(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(strands are independent: a multi-GPU run shards strand ranges, one process per GPU)", file=sys.stderr)

@@ -112,7 +112,7 @@ def ref():
         L.ref_random01.argtypes = [C.c_uint32] * 3
         L.ref_classic_cut.restype = C.c_uint32
         L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
-                                      C.c_float, C.c_float, C.c_float]
+                                      C.c_float, C.c_float, C.c_float, C.c_void_p]
         _REF = L
     return _REF
 
@@ -327,15 +327,17 @@ def classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, roots=
 def ref_classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, roots=None, uvs=None,
                     radius_scale=1.0):
     """The reference's apply_xgen_classic_float_runtime_plan_cpu with one
-    CutFX module (amount = amount_scale * $cLength + amount_offset)."""
+    CutFX module (amount = amount_scale * $cLength + amount_offset); the
+    result's "seconds" is the wall time of the reference call alone."""
     pts = np.array(points, np.float32).reshape(-1, 4)
     n = len(pts) // cvs
     counts = np.full(n, cvs, np.uint32) if counts is None else np.array(counts, np.uint32)
     rr = np.zeros(n, abi.ROOT_DTYPE) if roots is None else np.array(roots)
     uu = None if uvs is None else np.array(uvs, np.float32)
+    seconds = C.c_double()
     kept = ref().ref_classic_cut(pts.ctypes.data, counts.ctypes.data, rr.ctypes.data, abi.ptr(uu), n, cvs,
-                                 amount_scale, amount_offset, radius_scale)
+                                 amount_scale, amount_offset, radius_scale, C.byref(seconds))
     if kept == 0xFFFFFFFF:
         return None
     return dict(points=pts[:kept * cvs], point_counts=counts[:kept], roots=rr[:kept],
-                root_uvs=None if uu is None else uu[:kept], strand_count=kept)
+                root_uvs=None if uu is None else uu[:kept], strand_count=kept, seconds=seconds.value)

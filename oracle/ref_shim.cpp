@@ -7,6 +7,7 @@
 #include "nanoxgen/xgen_classic_runtime.h"
 #include "nanoxgen/xgen_expression.h"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -209,10 +210,11 @@ int ref_expr_eval(const nxg_expr_instruction *ins, uint32_t n_ins, const uint32_
 // amount_offset (an IR program over the bound input "cLength"), no length,
 // width or taper expressions.  Strands are packed float4 points; roots,
 // uvs and counts are carried.  Returns the kept strand count (outputs
-// compacted in place), or ~0u where the reference throws.
+// compacted in place), or ~0u where the reference throws; *seconds (when
+// given) receives the wall time of the reference call alone.
 uint32_t ref_classic_cut(float *points, uint32_t *counts, nxg_root_sample *roots, nxg_vec2 *uvs,
                          uint32_t strand_count, uint32_t cvs, float amount_scale, float amount_offset,
-                         float radius_scale) {
+                         float radius_scale, double *seconds) {
     try {
         PackedGeneratedCurves curves{};
         curves.strand_count = strand_count;
@@ -245,7 +247,9 @@ uint32_t ref_classic_cut(float *points, uint32_t *counts, nxg_root_sample *roots
         cut.rebuild_type = 1u;
         plan.cuts.push_back(cut);
         plan.effects.push_back({ClassicFloatEffectType::Cut, 0u});
+        const auto start = std::chrono::steady_clock::now();
         apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale);
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
         std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
         std::memcpy(counts, curves.point_counts.data(), curves.point_counts.size() * sizeof(uint32_t));
         std::memcpy(roots, curves.roots.data(), curves.roots.size() * sizeof(RootSample));
