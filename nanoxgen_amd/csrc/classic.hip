@@ -5,13 +5,21 @@
 // 1806-1821 the per-strand chain, 1893-1918 compaction), so positions are
 // bit-identical to it (no FP contraction, correctly rounded div/sqrt).
 //
-// Four launches on the caller's stream, none of them allocating:
+// Five launches on the caller's stream, none of them allocating:
+//   cut_table_kernel    the spline samples every strand shares: all strands
+//                       of a call have n CVs, so cLength's 2n+3 parameters
+//                       k/(2n+4) and the cut walk's parameters 1 - j/(2n+4)
+//                       (each step subtracted from the last, as the walk
+//                       does), their spans and B-spline weights are the same
+//                       for every strand; one block writes them to the
+//                       workspace, and the search reads them with scalar
+//                       loads instead of recomputing them in every lane.
 //   cut_search_kernel   one 64-lane wavefront per tile of 64 strands.  The
 //                       tile's points are one contiguous run of 64 x n
 //                       float4s: the wave reads it coalesced into LDS,
 //                       CV-major with row c rotated by c lanes, so the
-//                       staging stores and the lanes' data-dependent CV
-//                       reads are both bank-conflict-free.  Each lane
+//                       staging stores are bank-conflict-free, and so are
+//                       the reads (every lane reads the same row).  Each lane
 //                       measures its spline (2n+4 chords), culls, walks
 //                       back from the tip and stores its cut parameter
 //                       (-1: culled); lane 0 stores the tile's kept count
@@ -21,8 +29,9 @@
 //   cut_scan_groups     one block: exclusive scan of the group totals, the
 //                       kept count and the call's status word.
 //   cut_write_kernel    the tile again through LDS; each kept lane resamples
-//                       its n CVs on [0, cut] and writes them, with its
-//                       point count, root and root uv, at its compacted slot
+//                       its n CVs on [0, cut] (its own parameters, so no
+//                       shared table) and writes them, with its point
+//                       count, root and root uv, at its compacted slot
 //                       (group prefix + tile prefix + rank in the wave).
 // Above 64 CVs a tile no longer fits 64 KB of LDS and the lanes read their
 // CVs from memory instead (same arithmetic).
@@ -51,6 +60,19 @@ __device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b
 __device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ float length_squared(F3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
 
+// One spline sample shared by every strand of a call (all have n CVs): the
+// parameter and, for interior samples, the blended span and its weights, as
+// xgen_curve_eval computes them (src/xgen_classic_runtime.cpp:614-642).
+struct Sample {
+    float b0, b1, b2, b3;
+    float t;
+    float previous_t;  // cut walk: the parameter one step nearer the tip
+    uint32_t span;     // CVs span-1 .. span+2 are blended
+    uint32_t kind;     // kAtRoot, kAtTip, or the kReflect* flags of an interior sample
+};
+static_assert(sizeof(Sample) == 32, "one 32-byte scalar load per sample");
+constexpr uint32_t kAtRoot = 1u, kAtTip = 2u, kReflectLow = 4u, kReflectHigh = 8u;
+
 struct CutArgs {
     const float4 *points;
     const uint32_t *counts;
@@ -64,6 +86,9 @@ struct CutArgs {
     float *cut;
     uint32_t *tiles, *groups, *status_word;
     uint32_t tile_count, group_count;
+    Sample *length_samples;  // 2n+3 entries: k/(2n+4), k = 1..2n+3
+    Sample *walk_samples;    // walk_count entries, the last at parameter <= 1e-10
+    uint32_t walk_count;
     float4 *out_points;
     uint32_t *out_counts;
     nxg_root_sample *out_roots;
@@ -71,67 +96,116 @@ struct CutArgs {
     uint32_t *kept, *status;
 };
 
-// A lane's CVs: staged in LDS, or read from memory.
+// A lane's CVs: staged in LDS (CV-major; ROTATED: row c turned by c lanes),
+// or read from memory.  Rotated rows make the coalesced staging stores
+// conflict-free; plain rows make reads of different rows by different lanes
+// conflict-free.  The search reads one row per step for all lanes, the
+// rebuild reads per-lane rows.
+template <bool ROTATED>
+__device__ __forceinline__ uint32_t staged_index(uint32_t c, uint32_t lane) {
+    return ROTATED ? c * kWave + ((lane + c) & (kWave - 1u)) : c * kWave + lane;
+}
+template <bool ROTATED>
 struct StagedCurve {
     const float4 *tile;
     uint32_t lane;
-    __device__ __forceinline__ float4 operator[](uint32_t c) const {
-        return tile[c * kWave + ((lane + c) & (kWave - 1u))];
-    }
+    __device__ __forceinline__ float4 operator[](uint32_t c) const { return tile[staged_index<ROTATED>(c, lane)]; }
 };
 struct MemoryCurve {
     const float4 *strand;
     __device__ __forceinline__ float4 operator[](uint32_t c) const { return strand[c]; }
 };
 
-// The tile's `count` strands, coalesced, into LDS (CV-major, rotated rows).
+// The tile's `count` strands, read coalesced, into LDS.
+template <bool ROTATED>
 __device__ __forceinline__ void stage_tile(const CutArgs &a, float4 *tile, uint32_t first, uint32_t count) {
     const float4 *src = a.points + uint64_t{first} * a.cvs;
     const uint32_t total = count * a.cvs;
     for (uint32_t i = threadIdx.x; i < total; i += kWave) {
         const uint32_t l = __umulhi(i, a.div_magic);
         const uint32_t c = i - l * a.cvs;
-        tile[c * kWave + ((l + c) & (kWave - 1u))] = src[i];
+        tile[staged_index<ROTATED>(c, l)] = src[i];
     }
     __syncthreads();
 }
 
-// xgen_curve_eval (src/xgen_classic_runtime.cpp:614-642): uniform cubic
-// B-spline through n CVs with reflected end points.
-template <class Curve>
-__device__ __forceinline__ F3 curve_eval(const Curve &p, uint32_t n, float t) {
-    if (t < 1.0e-7f) return xyz(p[0]);
-    if (t > 1.0f - 1.0e-7f) return xyz(p[n - 1u]);
+// xgen_curve_eval's parameter-only half: span and weights at t.
+__device__ __forceinline__ Sample make_sample(uint32_t n, float t, float previous_t) {
+    Sample s{};
+    s.t = t;
+    s.previous_t = previous_t;
+    if (t < 1.0e-7f) {
+        s.kind = kAtRoot;
+        return s;
+    }
+    if (t > 1.0f - 1.0e-7f) {
+        s.kind = kAtTip;
+        return s;
+    }
     const uint32_t spans = n - 1u;
     const float scaled = t * static_cast<float>(spans);
     const uint32_t raw = static_cast<uint32_t>(scaled);
     const float f = scaled - static_cast<float>(raw);
-    const uint32_t span = raw < n - 2u ? raw : n - 2u;  // t < 1 keeps raw <= n - 2; the clamp guards memory only
+    s.span = raw < n - 2u ? raw : n - 2u;  // t < 1 keeps raw <= n - 2; the clamp guards memory only
     const float f2 = f * f;
     const float f3 = f2 * f;
     const float omf = 1.0f - f;
-    const float b0 = omf * omf * omf;
-    const float b1 = 3.0f * f3 - 6.0f * f2 + 4.0f;
-    const float b2 = -3.0f * f3 + 3.0f * f2 + 3.0f * f + 1.0f;
-    const float b3 = f3;
-    const F3 p1 = xyz(p[span]);
-    const F3 p2 = xyz(p[span + 1u]);
-    const F3 q0 = xyz(p[span == 0u ? 0u : span - 1u]);
-    const F3 q3 = xyz(p[span + 2u >= n ? n - 1u : span + 2u]);
-    const F3 p0 = span == 0u ? p1 * 2.0f - p2 : q0;
-    const F3 p3 = span + 2u >= n ? p2 * 2.0f - p1 : q3;
-    return (p0 * b0 + p1 * b1 + p2 * b2 + p3 * b3) * (1.0f / 6.0f);
+    s.b0 = omf * omf * omf;
+    s.b1 = 3.0f * f3 - 6.0f * f2 + 4.0f;
+    s.b2 = -3.0f * f3 + 3.0f * f2 + 3.0f * f + 1.0f;
+    s.b3 = f3;
+    s.kind = (s.span == 0u ? kReflectLow : 0u) | (s.span + 2u >= n ? kReflectHigh : 0u);
+    return s;
 }
 
-// curve_spline_length (src/xgen_classic_runtime.cpp:644-663).
+// xgen_curve_eval's point half.  Sample fields that are uniform across the
+// wave (a shared table entry) make every branch here a scalar one.
 template <class Curve>
-__device__ __forceinline__ float spline_length(const Curve &p, uint32_t n) {
-    const uint32_t intervals = n * 2u + 4u;
-    const float step = 1.0f / static_cast<float>(intervals);
+__device__ __forceinline__ F3 blend(const Curve &p, uint32_t n, const Sample &s) {
+    if (s.kind & kAtRoot) return xyz(p[0]);
+    if (s.kind & kAtTip) return xyz(p[n - 1u]);
+    const F3 p1 = xyz(p[s.span]);
+    const F3 p2 = xyz(p[s.span + 1u]);
+    const F3 p0 = (s.kind & kReflectLow) ? p1 * 2.0f - p2 : xyz(p[s.span - 1u]);
+    const F3 p3 = (s.kind & kReflectHigh) ? p2 * 2.0f - p1 : xyz(p[s.span + 2u]);
+    return (p0 * s.b0 + p1 * s.b1 + p2 * s.b2 + p3 * s.b3) * (1.0f / 6.0f);
+}
+
+// xgen_curve_eval (src/xgen_classic_runtime.cpp:614-642) at a per-lane t.
+template <class Curve>
+__device__ __forceinline__ F3 curve_eval(const Curve &p, uint32_t n, float t) {
+    if (t < 1.0e-7f) return xyz(p[0]);
+    if (t > 1.0f - 1.0e-7f) return xyz(p[n - 1u]);
+    const Sample s = make_sample(n, t, 0.0f);
+    const F3 p1 = xyz(p[s.span]);
+    const F3 p2 = xyz(p[s.span + 1u]);
+    const F3 q0 = xyz(p[s.span == 0u ? 0u : s.span - 1u]);
+    const F3 q3 = xyz(p[s.span + 2u >= n ? n - 1u : s.span + 2u]);
+    const F3 p0 = (s.kind & kReflectLow) ? p1 * 2.0f - p2 : q0;
+    const F3 p3 = (s.kind & kReflectHigh) ? p2 * 2.0f - p1 : q3;
+    return (p0 * s.b0 + p1 * s.b1 + p2 * s.b2 + p3 * s.b3) * (1.0f / 6.0f);
+}
+
+// A shared sample through the scalar cache: the index is uniform, so the
+// entry lands in scalar registers and the blend's branches are scalar.
+typedef const __attribute__((address_space(4))) uint32_t *ConstWords;
+__device__ __forceinline__ Sample load_sample(const Sample *table, uint32_t k) {
+    ConstWords w = (ConstWords)(table + k);
+    uint32_t v[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8u; ++j) v[j] = w[j];
+    Sample s;
+    __builtin_memcpy(&s, v, sizeof(s));
+    return s;
+}
+
+// curve_spline_length (src/xgen_classic_runtime.cpp:644-663) over the shared samples.
+template <class Curve>
+__device__ __forceinline__ float spline_length(const Sample *samples, const Curve &p, uint32_t n) {
     F3 previous = xyz(p[0]);
     float result = 0.0f;
-    for (uint32_t k = 1u; k < intervals; ++k) {
-        const F3 current = curve_eval(p, n, step * static_cast<float>(k));
+    for (uint32_t k = 0u; k < 2u * n + 3u; ++k) {
+        const F3 current = blend(p, n, load_sample(samples, k));
         result += sqrtf(length_squared(current - previous));
         previous = current;
     }
@@ -140,26 +214,23 @@ __device__ __forceinline__ float spline_length(const Curve &p, uint32_t n) {
 }
 
 // The search half of cut_from_tip_and_rebuild (src/xgen_classic_runtime.cpp:
-// 701-728): the rebuild parameter for a finite amount >= 0 (0: nothing
-// left).  The parameter falls by `step` per pass and is clamped at 0, so the
-// walk ends within 2n+5 passes; the guard is never reached.
+// 701-728) over the shared walk samples: the rebuild parameter for a finite
+// amount >= 0 (0: nothing left).  The last walk sample has parameter
+// <= 1e-10, where every lane still walking stops.
 template <class Curve>
-__device__ __forceinline__ float cut_parameter(const Curve &p, uint32_t n, float amount) {
+__device__ __forceinline__ float cut_parameter(const Sample *walk, uint32_t walk_count, const Curve &p, uint32_t n,
+                                               float amount) {
     if (!(amount >= 1.0e-10f)) return 1.0f;
-    const float step = 1.0f / (2.0f * static_cast<float>(n) + 4.0f);
-    float previous_parameter = 1.0f;
     F3 previous = xyz(p[n - 1u]);
     float accumulated = 0.0f;
-    for (uint32_t guard = 0u; guard < 4u * n + 16u; ++guard) {
-        const float lowered = previous_parameter - step;
-        const float parameter = lowered < 0.0f ? 0.0f : lowered;
-        const F3 current = curve_eval(p, n, parameter);
+    for (uint32_t j = 0u; j < walk_count; ++j) {
+        const Sample s = load_sample(walk, j);
+        const F3 current = blend(p, n, s);
         const float segment = sqrtf(length_squared(previous - current));
         accumulated += segment;
         if (accumulated >= amount && segment > 0.0f)
-            return parameter + ((accumulated - amount) / segment) * (previous_parameter - parameter);
-        if (parameter <= 1.0e-10f) return 0.0f;
-        previous_parameter = parameter;
+            return s.t + ((accumulated - amount) / segment) * (s.previous_t - s.t);
+        if (s.t <= 1.0e-10f) return 0.0f;
         previous = current;
     }
     return 0.0f;
@@ -169,9 +240,10 @@ __device__ __forceinline__ float cut_parameter(const Curve &p, uint32_t n, float
 // the cut parameter of a kept strand, -1 for a culled one; *err gets the
 // NXG_CLASSIC_NONFINITE_* bit where the reference throws.
 template <class Curve>
-__device__ __forceinline__ float strand_cut(const CutArgs &a, const Curve &p, uint32_t s, uint32_t *err) {
+__device__ __forceinline__ float strand_cut(const CutArgs &a, const Sample *samples, const Curve &p, uint32_t s,
+                                            uint32_t *err) {
     const uint32_t n = a.cvs;
-    const float c_length = spline_length(p, n);
+    const float c_length = spline_length(samples, p, n);
     const float radius = p[0].w;
     const float width = a.radius_scale > 0.0f ? 2.0f * radius / a.radius_scale : 0.0f;
     if (!isfinite(width)) {
@@ -185,25 +257,49 @@ __device__ __forceinline__ float strand_cut(const CutArgs &a, const Curve &p, ui
         *err = NXG_CLASSIC_NONFINITE_AMOUNT;
         return -1.0f;
     }
-    const float cut = cut_parameter(p, n, amount < 0.0f ? 0.0f : amount);
+    const float cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
     return cut < 1.0e-4f ? -1.0f : cut;
+}
+
+// The shared samples: cLength's k/(2n+4) in parallel, the walk's parameters
+// by the walk's own repeated subtraction (one lane), then their weights.
+__global__ void __launch_bounds__(256) cut_table_kernel(CutArgs a) {
+    __shared__ float walk_t[kClassicTableEntries];
+    const uint32_t n = a.cvs;
+    const uint32_t intervals = n * 2u + 4u;
+    const float step = 1.0f / static_cast<float>(intervals);
+    for (uint32_t k = threadIdx.x + 1u; k < intervals; k += blockDim.x)
+        a.length_samples[k - 1u] = make_sample(n, step * static_cast<float>(k), 0.0f);
+    if (threadIdx.x == 0u) {
+        const float walk_step = 1.0f / (2.0f * static_cast<float>(n) + 4.0f);
+        float previous = 1.0f;
+        for (uint32_t j = 0u; j < a.walk_count; ++j) {
+            const float lowered = previous - walk_step;
+            walk_t[j] = lowered < 0.0f ? 0.0f : lowered;
+            previous = walk_t[j];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < a.walk_count; j += blockDim.x)
+        a.walk_samples[j] = make_sample(n, walk_t[j], j ? walk_t[j - 1u] : 1.0f);
 }
 
 template <bool STAGED>
 __global__ void __launch_bounds__(kWave) cut_search_kernel(CutArgs a) {
     extern __shared__ float4 tile[];
+    const uint32_t n = a.cvs;
     const uint32_t t = blockIdx.x, lane = threadIdx.x;
     const uint32_t first = t * kWave;
     const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
-    if constexpr (STAGED) stage_tile(a, tile, first, count);
+    if constexpr (STAGED) stage_tile<true>(a, tile, first, count);
     const uint32_t s = first + lane;
     float cut = -1.0f;
     uint32_t err = 0u;
     if (lane < count) {
         if constexpr (STAGED)
-            cut = strand_cut(a, StagedCurve{tile, lane}, s, &err);
+            cut = strand_cut(a, a.length_samples, StagedCurve<true>{tile, lane}, s, &err);
         else
-            cut = strand_cut(a, MemoryCurve{a.points + uint64_t{s} * a.cvs}, s, &err);
+            cut = strand_cut(a, a.length_samples, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
         a.cut[s] = cut;
     }
     const uint64_t kept = __ballot(cut >= 0.0f);
@@ -284,6 +380,7 @@ template <bool STAGED>
 __global__ void __launch_bounds__(kWave) cut_write_kernel(CutArgs a) {
     extern __shared__ float4 tile[];
     if (*a.status_word != 0u) return;  // the reference throws: nothing is written
+    const uint32_t n = a.cvs;
     const uint32_t t = blockIdx.x, lane = threadIdx.x;
     const uint32_t first = t * kWave;
     const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
@@ -291,9 +388,8 @@ __global__ void __launch_bounds__(kWave) cut_write_kernel(CutArgs a) {
     const float cut = lane < count ? a.cut[s] : -1.0f;
     const uint64_t kept = __ballot(cut >= 0.0f);
     if (kept == 0u) return;  // the block is this one wavefront
-    if constexpr (STAGED) stage_tile(a, tile, first, count);
+    if constexpr (STAGED) stage_tile<false>(a, tile, first, count);
     if (!(cut >= 0.0f)) return;
-    const uint32_t n = a.cvs;
     const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
     const uint32_t slot = a.groups[t / kScanBlock] + a.tiles[t] + rank;
     float4 *dst = a.out_points + uint64_t{slot} * n;
@@ -306,7 +402,7 @@ __global__ void __launch_bounds__(kWave) cut_write_kernel(CutArgs a) {
         }
     };
     if constexpr (STAGED)
-        rebuild(StagedCurve{tile, lane});
+        rebuild(StagedCurve<false>{tile, lane});
     else
         rebuild(MemoryCurve{a.points + uint64_t{s} * n});
     if (a.out_counts) a.out_counts[slot] = a.counts ? a.counts[s] : n;
@@ -346,6 +442,22 @@ extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, cons
     a.status_word = reinterpret_cast<uint32_t *>(ws + layout.status_offset);
     a.tile_count = classic_tiles(n);
     a.group_count = classic_groups(n);
+    a.length_samples = reinterpret_cast<Sample *>(ws + layout.table_offset);
+    a.walk_samples = a.length_samples + kClassicTableEntries;
+    // The walk's parameter sequence (1 - step, repeatedly, clamped at 0) up
+    // to its first value <= 1e-10, counted here in the same float steps.
+    {
+        const float walk_step = 1.0f / (2.0f * static_cast<float>(cvs) + 4.0f);
+        float previous = 1.0f;
+        uint32_t count = 0u;
+        while (count < kClassicTableEntries) {
+            const float lowered = previous - walk_step;
+            previous = lowered < 0.0f ? 0.0f : lowered;
+            ++count;
+            if (previous <= 1.0e-10f) break;
+        }
+        a.walk_count = count;
+    }
     a.out_points = reinterpret_cast<float4 *>(output->points);
     a.out_counts = output->point_counts;
     a.out_roots = output->roots;
@@ -354,10 +466,11 @@ extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, cons
     a.status = output->status;
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
     const bool staged = cvs <= kStagedCvs;
-    const size_t lds = staged ? size_t{kWave} * cvs * sizeof(float4) : 0u;
+    const size_t tile_lds = staged ? size_t{kWave} * cvs * sizeof(float4) : 0u;
     if (a.tile_count) {
+        hipLaunchKernelGGL(cut_table_kernel, dim3(1), dim3(256), 0, stream, a);
         if (staged)
-            hipLaunchKernelGGL(cut_search_kernel<true>, dim3(a.tile_count), dim3(kWave), lds, stream, a);
+            hipLaunchKernelGGL(cut_search_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
             hipLaunchKernelGGL(cut_search_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
         hipLaunchKernelGGL(cut_scan_tiles_kernel, dim3(a.group_count), dim3(kScanThreads), 0, stream, a);
@@ -365,7 +478,7 @@ extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, cons
     hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
     if (a.tile_count) {
         if (staged)
-            hipLaunchKernelGGL(cut_write_kernel<true>, dim3(a.tile_count), dim3(kWave), lds, stream, a);
+            hipLaunchKernelGGL(cut_write_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
             hipLaunchKernelGGL(cut_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
     }

@@ -57,14 +57,19 @@ WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand, 
 //                    within its group of kScanBlock tiles
 //   group totals     one u32 per kScanBlock tiles, scanned likewise
 //   status           one u32: the error bits of the whole call
+//   sample tables    the spline samples every strand of the call shares
+//                    (cLength's 2n+3 and the cut walk's), 32 B each
 struct ClassicLayout {
     uint64_t cut_offset;
     uint64_t tiles_offset;
     uint64_t groups_offset;
     uint64_t status_offset;
+    uint64_t table_offset;
     uint64_t total;
 };
 constexpr uint32_t kClassicTile = 64u;
+constexpr uint32_t kClassicTableEntries = 2u * NXG_CLASSIC_MAX_CVS + 8u;  // per table
+constexpr uint64_t kClassicTableBytes = 2u * kClassicTableEntries * 32u;
 inline uint32_t classic_tiles(uint32_t strand_count) { return (strand_count + kClassicTile - 1u) / kClassicTile; }
 inline uint32_t classic_groups(uint32_t strand_count) { return (classic_tiles(strand_count) + kScanBlock - 1u) / kScanBlock; }
 inline ClassicLayout classic_layout(uint32_t strand_count) {
@@ -74,7 +79,8 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
     l.tiles_offset = a(uint64_t{strand_count} * 4u);
     l.groups_offset = l.tiles_offset + a(uint64_t{classic_tiles(strand_count)} * 4u);
     l.status_offset = l.groups_offset + a(uint64_t{classic_groups(strand_count)} * 4u);
-    l.total = l.status_offset + 256u;
+    l.table_offset = l.status_offset + 256u;
+    l.total = l.table_offset + a(kClassicTableBytes);
     return l;
 }
 

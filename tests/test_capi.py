@@ -333,8 +333,9 @@ def test_classic_cut_contract():
                             point_count_capacity=n - 1, kept_count=words.ctypes.data)
     assert v(o=cnt) == abi.POINT_COUNT_CAPACITY_TOO_SMALL
     assert v(k=abi.ClassicCutParams(multiplier=pin.ctypes.data, multiplier_stride=0)) == abi.INVALID_CLASSIC_REQUEST
-    # workspace: 256-byte regions for cut parameters, tile counts, group totals and status
-    assert L.nxg_classic_cut_workspace_size(0) == 256  # the status word alone
+    # workspace: cut parameters, tile counts, group totals, status and the shared sample tables
+    table = 2 * (2 * abi.CLASSIC_MAX_CVS + 8) * 32
+    assert L.nxg_classic_cut_workspace_size(0) == 256 + (table + 255) // 256 * 256
     assert L.nxg_classic_cut_workspace_size(2_000_000) >= 2_000_000 * 4 + 31250 * 4
     # refused on the host before any device work
     ws = np.zeros(16, np.uint8)
