@@ -33,6 +33,11 @@
 //                       shared table) and writes them, with its point
 //                       count, root and root uv, at its compacted slot
 //                       (group prefix + tile prefix + rank in the wave).
+//                       Up to 32 CVs (cut_write_held_kernel) the resampled
+//                       strand stays in registers until the whole wave is
+//                       done reading, then goes back through the same LDS in
+//                       output order, so the wave's stores are one
+//                       contiguous run.
 // Above 64 CVs a tile no longer fits 64 KB of LDS and the lanes read their
 // CVs from memory instead (same arithmetic).
 #include <hip/hip_runtime.h>
@@ -410,6 +415,60 @@ __global__ void __launch_bounds__(kWave) cut_write_kernel(CutArgs a) {
     if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
 }
 
+// The rebuild with each strand held in registers (n <= MAXN): once every
+// lane has read its CVs, the tile's LDS is reused to lay the kept strands
+// out in output order (rows of n + 1 float4s, bank-skewed), and the wave
+// stores them as one contiguous run, so the point stores are coalesced.
+template <uint32_t MAXN>
+__global__ void __launch_bounds__(kWave) cut_write_held_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    const uint32_t s = first + lane;
+    const float cut = lane < count ? a.cut[s] : -1.0f;
+    const bool keep = cut >= 0.0f;
+    const uint64_t kept = __ballot(keep);
+    if (kept == 0u) return;  // the block is this one wavefront
+    stage_tile<false>(a, tile, first, count);
+    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
+    const uint32_t slot0 = a.groups[t / kScanBlock] + a.tiles[t];
+    float4 held[MAXN];
+    if (keep) {  // src/xgen_classic_runtime.cpp:729-742
+        const StagedCurve<false> p{tile, lane};
+        const float step = cut / static_cast<float>(n - 1u);
+        held[0] = p[0];
+#pragma unroll
+        for (uint32_t i = 1u; i < MAXN; ++i) {
+            if (i < n) {
+                const F3 q = curve_eval(p, n, step * static_cast<float>(i));
+                held[i] = make_float4(q.x, q.y, q.z, p[i].w);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t stride = n + 1u;
+    if (keep) {
+#pragma unroll
+        for (uint32_t i = 0u; i < MAXN; ++i)
+            if (i < n) tile[rank * stride + i] = held[i];
+    }
+    __syncthreads();
+    const uint32_t total = static_cast<uint32_t>(__popcll(kept)) * n;
+    float4 *dst = a.out_points + uint64_t{slot0} * n;
+    for (uint32_t e = lane; e < total; e += kWave) {
+        const uint32_t r = __umulhi(e, a.div_magic);
+        dst[e] = tile[r * stride + (e - r * n)];
+    }
+    if (!keep) return;
+    const uint32_t slot = slot0 + rank;
+    if (a.out_counts) a.out_counts[slot] = a.counts ? a.counts[s] : n;
+    if (a.out_roots) a.out_roots[slot] = a.roots[s];
+    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
+}
+
 }  // namespace
 }  // namespace nxg
 
@@ -477,7 +536,14 @@ extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, cons
     }
     hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
     if (a.tile_count) {
-        if (staged)
+        const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);
+        if (cvs <= 8u)
+            hipLaunchKernelGGL(cut_write_held_kernel<8>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+        else if (cvs <= 16u)
+            hipLaunchKernelGGL(cut_write_held_kernel<16>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+        else if (cvs <= 32u)
+            hipLaunchKernelGGL(cut_write_held_kernel<32>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+        else if (staged)
             hipLaunchKernelGGL(cut_write_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
             hipLaunchKernelGGL(cut_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
