@@ -352,6 +352,27 @@ def bench_classic(args):
                           "cut_write_kernel<true> (one C-ABI call)",
                 "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
                 "algorithmic_bytes_per_launch": alg, "valu": valu}
+    # the renderer endpoint pass on the cut's output, its strand count read on the device
+    from nanoxgen_amd.classic import add_endpoints
+    ends = torch.empty((N * (Cv + 2), 4), dtype=torch.float32, device="cuda")
+    end_counts = torch.empty(N, dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        add_endpoints(out.points, Cv, strands=N, device_strand_count=out.words, out=ends, point_counts=end_counts,
+                      stream=stream)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    for e in ev:
+        e[0].record(stream)
+        add_endpoints(out.points, Cv, strands=N, device_strand_count=out.words, out=ends, point_counts=end_counts,
+                      stream=stream)
+        e[1].record(stream)
+    torch.cuda.synchronize()
+    end_ms = sorted(e[0].elapsed_time(e[1]) for e in ev)
+    end_mean = sum(end_ms) / len(end_ms)
+    end_bytes = kept * (Cv * 16 + (Cv + 2) * 16 + 4)
+    roofline["endpoints"] = {"kernel": "endpoints_kernel", "ms_mean": round(end_mean, 4),
+                             "algorithmic_bytes": end_bytes,
+                             "achieved_gbs": round(end_bytes / (end_mean * 1e-3) / 1e9, 1),
+                             "frac": round(end_bytes / (end_mean * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = classic_cpu_baseline(gen, N, Cv, scale, offset, args.cpu_seconds)

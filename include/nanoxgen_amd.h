@@ -742,6 +742,19 @@ uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, const nxg_class
                                const nxg_classic_output *output, void *workspace, uint64_t workspace_bytes,
                                void *stream);
 
+/* Renderer endpoints (add_xgen_classic_renderer_endpoints,
+ * src/xgen_classic_runtime.cpp:1921-1956): every strand of cvs_per_strand
+ * points becomes cvs_per_strand + 2, a phantom point 2 p[0] - p[1] before the
+ * root and 2 p[n-1] - p[n-2] after the tip, each with its neighbour's radius;
+ * point_counts (optional) receive cvs_per_strand + 2.  With
+ * device_strand_count non-NULL the strand count is read on the device (for
+ * instance the kept count nxg_classic_cut_apply wrote) and strand_count is
+ * only its upper bound.  Output must not overlap input.  On `stream`. */
+uint32_t nxg_classic_add_endpoints(const nxg_packed_point *points, uint32_t strand_count, uint32_t cvs_per_strand,
+                                   const uint32_t *device_strand_count, nxg_packed_point *out_points,
+                                   uint64_t out_point_capacity, uint32_t *out_point_counts,
+                                   uint64_t out_point_count_capacity, void *stream);
+
 /* Library build identity: gfx target and compile flags that affect parity. */
 const char *nxg_build_info(void);
 

@@ -101,3 +101,22 @@ def classic_cut(points, cvs, amount_scale, amount_offset, *, point_counts=None, 
                                       C.c_void_p(workspace.data_ptr()), workspace.numel(), _stream_handle(stream)),
           "nxg_classic_cut_apply")
     return outputs
+
+
+def add_endpoints(points, cvs, *, strands=None, device_strand_count=None, out=None, point_counts=None, stream=None):
+    """Renderer endpoints on device curves (nxg_classic_add_endpoints):
+    `points` float32 [N * cvs, 4] -> `out` float32 [N * (cvs + 2), 4].
+    `device_strand_count` (an int32 device tensor, e.g. ClassicCutOutputs.words)
+    bounds the strands on the device without synchronising."""
+    torch = _torch()
+    if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
+        raise ValueError("points must be a contiguous float32 [N * cvs, 4] tensor")
+    n = points.shape[0] // cvs if strands is None else strands
+    if out is None:
+        out = torch.empty((max(1, n * (cvs + 2)), 4), dtype=torch.float32, device=points.device)
+    t = lambda x: None if x is None else x.data_ptr()
+    check(lib().nxg_classic_add_endpoints(points.data_ptr(), n, cvs, t(device_strand_count), out.data_ptr(),
+                                          out.shape[0] if n else 0, t(point_counts),
+                                          0 if point_counts is None else point_counts.numel(),
+                                          _stream_handle(stream)), "nxg_classic_add_endpoints")
+    return out

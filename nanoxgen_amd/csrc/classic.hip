@@ -469,6 +469,45 @@ __global__ void __launch_bounds__(kWave) cut_write_held_kernel(CutArgs a) {
     if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
 }
 
+// add_xgen_classic_renderer_endpoints (src/xgen_classic_runtime.cpp:
+// 1921-1956): one 256-thread block per 64-strand tile, each thread one
+// output point of the tile's contiguous 64 x (n + 2) run (input and output
+// both read and written in order).
+struct EndpointArgs {
+    const float4 *points;
+    float4 *out;
+    uint32_t *counts;
+    const uint32_t *device_count;
+    uint32_t strands, cvs, magic;  // e / (cvs + 2) == __umulhi(e, magic) for e < 2^24
+};
+
+__global__ void __launch_bounds__(256) endpoints_kernel(EndpointArgs a) {
+    const uint32_t strands = a.device_count ? min(*a.device_count, a.strands) : a.strands;
+    const uint32_t first = blockIdx.x * kWave;
+    if (first >= strands) return;
+    const uint32_t count = strands - first < kWave ? strands - first : kWave;
+    const uint32_t n = a.cvs, m = n + 2u;
+    const float4 *src = a.points + uint64_t{first} * n;
+    float4 *dst = a.out + uint64_t{first} * m;
+    for (uint32_t e = threadIdx.x; e < count * m; e += blockDim.x) {
+        const uint32_t r = __umulhi(e, a.magic);
+        const uint32_t k = e - r * m;
+        const float4 *p = src + r * n;
+        float4 v;
+        if (k == 0u) {
+            const float4 p0 = p[0], p1 = p[1];
+            v = make_float4(2.0f * p0.x - p1.x, 2.0f * p0.y - p1.y, 2.0f * p0.z - p1.z, p0.w);
+        } else if (k == m - 1u) {
+            const float4 p0 = p[n - 1u], p1 = p[n - 2u];
+            v = make_float4(2.0f * p0.x - p1.x, 2.0f * p0.y - p1.y, 2.0f * p0.z - p1.z, p0.w);
+        } else {
+            v = p[k - 1u];
+        }
+        dst[e] = v;
+    }
+    if (a.counts && threadIdx.x < count) a.counts[first + threadIdx.x] = m;
+}
+
 }  // namespace
 }  // namespace nxg
 
@@ -548,5 +587,35 @@ extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, cons
         else
             hipLaunchKernelGGL(cut_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
     }
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
+extern "C" uint32_t nxg_classic_add_endpoints(const nxg_packed_point *points, uint32_t strand_count,
+                                              uint32_t cvs_per_strand, const uint32_t *device_strand_count,
+                                              nxg_packed_point *out_points, uint64_t out_point_capacity,
+                                              uint32_t *out_point_counts, uint64_t out_point_count_capacity,
+                                              void *stream_ptr) {
+    using namespace nxg;
+    const uint64_t n = strand_count, cvs = cvs_per_strand;
+    if (cvs < 2u || cvs > NXG_CLASSIC_MAX_CVS || n > NXG_CLASSIC_MAX_STRANDS) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (n == 0u) return NXG_OK;
+    if (!points || !out_points) return NXG_ERROR_NULL_ARGUMENT;
+    if (reinterpret_cast<uintptr_t>(points) % 16u != 0u || reinterpret_cast<uintptr_t>(out_points) % 16u != 0u)
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (out_point_capacity < n * (cvs + 2u)) return NXG_ERROR_POINT_CAPACITY_TOO_SMALL;
+    if (out_point_counts && out_point_count_capacity < n) return NXG_ERROR_POINT_COUNT_CAPACITY_TOO_SMALL;
+    const uintptr_t in0 = reinterpret_cast<uintptr_t>(points), in1 = in0 + n * cvs * 16u;
+    const uintptr_t o0 = reinterpret_cast<uintptr_t>(out_points), o1 = o0 + n * (cvs + 2u) * 16u;
+    if (o0 < in1 && in0 < o1) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    EndpointArgs a{};
+    a.points = reinterpret_cast<const float4 *>(points);
+    a.out = reinterpret_cast<float4 *>(out_points);
+    a.counts = out_point_counts;
+    a.device_count = device_strand_count;
+    a.strands = strand_count;
+    a.cvs = cvs_per_strand;
+    a.magic = static_cast<uint32_t>(0xffffffffu / (cvs_per_strand + 2u) + 1u);
+    hipLaunchKernelGGL(endpoints_kernel, dim3(classic_tiles(strand_count)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream_ptr), a);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }

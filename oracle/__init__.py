@@ -76,6 +76,7 @@ def lib():
         L.orc_curve_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]
         L.orc_curve_spline_length.restype = C.c_float
         L.orc_curve_spline_length.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.orc_classic_cut.restype = C.c_uint32
         L.orc_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p,
@@ -110,6 +111,8 @@ def ref():
         L.ref_hash32.argtypes = [C.c_uint32]
         L.ref_random01.restype = C.c_float
         L.ref_random01.argtypes = [C.c_uint32] * 3
+        L.ref_classic_endpoints.restype = C.c_uint32
+        L.ref_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
         L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -341,3 +344,20 @@ def ref_classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, ro
         return None
     return dict(points=pts[:kept * cvs], point_counts=counts[:kept], roots=rr[:kept],
                 root_uvs=None if uu is None else uu[:kept], strand_count=kept, seconds=seconds.value)
+
+
+def classic_endpoints(points, cvs):
+    """Renderer endpoints: [N * cvs, 4] -> [N * (cvs + 2), 4]."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    out = np.zeros((n * (cvs + 2), 4), np.float32)
+    lib().orc_classic_endpoints(pts.ctypes.data, n, cvs, out.ctypes.data)
+    return out
+
+
+def ref_classic_endpoints(points, cvs):
+    """The reference's add_xgen_classic_renderer_endpoints (None where it throws)."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    out = np.zeros((n * (cvs + 2), 4), np.float32)
+    return None if ref().ref_classic_endpoints(pts.ctypes.data, n, cvs, out.ctypes.data) else out

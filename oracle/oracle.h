@@ -92,6 +92,7 @@ uint32_t orc_root_attributes(const orc_job *job, const nxg_expr_program *program
 /* Classic CutFX chain (oracle_classic.c). */
 float orc_curve_eval(const float *points4, uint32_t n, float t, float *out3);
 float orc_curve_spline_length(const float *points4, uint32_t n);
+void orc_classic_endpoints(const float *points, uint32_t strand_count, uint32_t cvs, float *out);
 uint32_t orc_classic_cut(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
                          const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, float amount_scale,
                          float amount_offset, const float *multiplier, float radius_scale, float *out_points,

@@ -10,6 +10,7 @@
  *                          default width, 1e-4 culling), 1806-1821 (CutFX,
  *                          culling below cut parameter 1e-4)
  *   compaction             src/xgen_classic_runtime.cpp:1893-1918 (stable)
+ *   renderer endpoints     src/xgen_classic_runtime.cpp:1921-1956
  * Float operations in the reference's order, built with -ffp-contract=off.
  */
 #include <math.h>
@@ -165,4 +166,23 @@ uint32_t orc_classic_cut(const float *points, const uint32_t *counts, const nxg_
         ++write;
     }
     return write;
+}
+
+/* add_xgen_classic_renderer_endpoints (src/xgen_classic_runtime.cpp:1921-
+ * 1956): each strand of cvs points becomes cvs + 2, a phantom point
+ * 2 p0 - p1 before the root and 2 p[n-1] - p[n-2] after the tip, each with
+ * its neighbour's radius.  out holds strand_count * (cvs + 2) float4s. */
+void orc_classic_endpoints(const float *points, uint32_t strand_count, uint32_t cvs, float *out) {
+    const uint32_t m = cvs + 2u;
+    for (uint32_t s = 0; s < strand_count; ++s) {
+        const float *in = points + (uint64_t)s * cvs * 4u;
+        float *o = out + (uint64_t)s * m * 4u;
+        memcpy(o + 4, in, sizeof(float) * 4u * cvs);
+        for (int c = 0; c < 3; ++c) {
+            o[c] = 2.0f * in[c] - in[4 + c];
+            o[4u * (m - 1u) + c] = 2.0f * in[4u * (cvs - 1u) + c] - in[4u * (cvs - 2u) + c];
+        }
+        o[3] = in[3];
+        o[4u * (m - 1u) + 3] = in[4u * (cvs - 1u) + 3];
+    }
 }

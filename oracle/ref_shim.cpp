@@ -260,6 +260,25 @@ uint32_t ref_classic_cut(float *points, uint32_t *counts, nxg_root_sample *roots
     }
 }
 
+// add_xgen_classic_renderer_endpoints (xgen_classic_runtime.cpp:1921-1956)
+// on strand_count packed strands of cvs points; out holds (cvs + 2) per
+// strand.  Returns 0, or 1 where the reference throws.
+uint32_t ref_classic_endpoints(const float *points, uint32_t strand_count, uint32_t cvs, float *out) {
+    try {
+        PackedGeneratedCurves curves{};
+        curves.strand_count = strand_count;
+        curves.cvs_per_strand = cvs;
+        curves.point_counts.assign(strand_count, cvs);
+        curves.points.resize(std::size_t{strand_count} * cvs);
+        std::memcpy(curves.points.data(), points, curves.points.size() * sizeof(PackedCurvePoint));
+        add_xgen_classic_renderer_endpoints(curves);
+        std::memcpy(out, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
+        return 0u;
+    } catch (const std::exception &) {
+        return 1u;
+    }
+}
+
 uint32_t ref_runtime_hash_component(float x) { return xgen_runtime_hash_component(x); }
 uint32_t ref_seexpr_hash_finish(uint32_t s) { return xgen_seexpr_hash_finish(s); }
 float ref_random01(uint32_t seed, uint32_t strand, uint32_t dim) { return random01(seed, strand, dim); }

@@ -287,6 +287,9 @@ def main_classic():
         fx[f"case{i}_args"] = np.array([scale, offset, rscale], np.float32)
         for k in ("points", "point_counts", "roots", "root_uvs"):
             fx[f"case{i}_out_{k}"] = out[k]
+    # add_xgen_classic_renderer_endpoints on the random walks and on case 3's output
+    fx["walk_endpoints"] = oracle.ref_classic_endpoints(fx["walk_points"], int(fx["walk_cvs"]))
+    fx["case3_endpoints"] = oracle.ref_classic_endpoints(fx["case3_out_points"], int(fx["walk_cvs"]))
     path = os.path.join(HERE, "reference_classic.npz")
     np.savez_compressed(path, **fx)
     print("wrote", path, os.path.getsize(path), "bytes;",

@@ -343,3 +343,22 @@ def test_classic_cut_contract():
                                    None) == abi.WORKSPACE_TOO_SMALL
     assert L.nxg_classic_cut_apply(C.byref(curves), C.byref(cut), C.byref(alias), ws.ctypes.data, ws.nbytes,
                                    None) == abi.INVALID_CLASSIC_REQUEST
+
+
+
+def test_classic_endpoints_contract():
+    L = nx.lib()
+    n, cvs = 10, 4
+    pin = np.zeros((n * cvs, 4), np.float32)
+    pout = np.zeros((n * (cvs + 2), 4), np.float32)
+    f = lambda i=pin.ctypes.data, c=cvs, o=pout.ctypes.data, cap=n * (cvs + 2), cnt=None, ccap=0, strands=n: \
+        L.nxg_classic_add_endpoints(i, strands, c, None, o, cap, cnt, ccap, None)
+    assert f(c=1) == abi.INVALID_CLASSIC_REQUEST
+    assert f(c=abi.CLASSIC_MAX_CVS + 1) == abi.INVALID_CLASSIC_REQUEST
+    assert f(strands=0) == abi.OK                      # nothing to do, nothing launched
+    assert f(i=None) == abi.NULL_ARGUMENT
+    assert f(cap=n * (cvs + 2) - 1) == abi.POINT_CAPACITY_TOO_SMALL
+    counts = np.zeros(n, np.uint32)
+    assert f(cnt=counts.ctypes.data, ccap=n - 1) == abi.POINT_COUNT_CAPACITY_TOO_SMALL
+    assert f(o=pin.ctypes.data + 16, cap=10 ** 6) == abi.INVALID_CLASSIC_REQUEST  # overlaps the input
+    assert f(o=pout.ctypes.data + 8) == abi.INVALID_CLASSIC_REQUEST               # misaligned

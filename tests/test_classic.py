@@ -169,3 +169,30 @@ def test_compaction_is_stable(orc, classic):
     assert np.array_equal(r["roots"]["triangle_index"], classic["walk_roots"]["triangle_index"][kept])
     assert np.array_equal(r["root_uvs"], classic["walk_uvs"][kept])
     assert abi.ROOT_DTYPE.itemsize == 48
+
+
+# ---- renderer endpoints (src/xgen_classic_runtime.cpp:1921-1956) ---------
+
+def test_endpoints_pin(orc):
+    # tests/classic_test.cpp:243-249: CVs at x = 0, 1.5, 3 gain -1.5 and 4.5,
+    # with the neighbouring radii
+    p = straight(3, 3.0)
+    p[:, 3] = [0.125, 0.075, 0.025]
+    e = orc.classic_endpoints(p, 3)
+    assert e.shape == (5, 4)
+    assert abs(e[0, 0] + 1.5) < 1e-6 and abs(e[4, 0] - 4.5) < 1e-6
+    assert e[0, 3] == e[1, 3] and e[4, 3] == e[3, 3]
+    assert np.array_equal(e[1:4], p)
+
+
+def test_endpoints_match_reference(orc, classic):
+    cvs = int(classic["walk_cvs"])
+    got = orc.classic_endpoints(classic["walk_points"], cvs)
+    assert np.array_equal(bits(got), bits(classic["walk_endpoints"]))
+    got = orc.classic_endpoints(classic["case3_out_points"], cvs)
+    assert np.array_equal(bits(got), bits(classic["case3_endpoints"]))
+    if orc.ref() is not None:
+        rng = np.random.default_rng(9)
+        for n in (2, 5, 64):
+            p = rng.normal(size=(33 * n, 4)).astype(np.float32)
+            assert np.array_equal(bits(orc.classic_endpoints(p, n)), bits(orc.ref_classic_endpoints(p, n)))

@@ -7,7 +7,7 @@ import pytest
 
 import nanoxgen_amd as nx
 from nanoxgen_amd import abi, workloads
-from nanoxgen_amd.classic import ClassicCutOutputs, classic_cut
+from nanoxgen_amd.classic import ClassicCutOutputs, add_endpoints, classic_cut
 
 pytestmark = pytest.mark.gpu
 
@@ -170,3 +170,37 @@ def test_generated_plane_chain(torch, orc):
                            uvs=host["root_uvs"])
     assert 0 < want["strand_count"] <= w.params.strand_count
     same(got, want, "generated plane, 0.25 cLength")
+
+
+def test_endpoints_reference_fixture(torch, classic):
+    cvs = int(classic["walk_cvs"])
+    got = add_endpoints(dev(torch, classic["walk_points"]), cvs).cpu().numpy()
+    assert np.array_equal(bits(got), bits(classic["walk_endpoints"]))
+
+
+@pytest.mark.parametrize("cvs", [2, 3, 16, 64, 256])
+def test_endpoints_oracle(torch, orc, cvs):
+    rng = np.random.default_rng(cvs + 100)
+    n = 3000 // cvs + 65
+    pts = rng.normal(size=(n * cvs, 4)).astype(np.float32)
+    counts = torch.zeros(n, dtype=torch.int32, device="cuda")
+    got = add_endpoints(dev(torch, pts), cvs, point_counts=counts).cpu().numpy()
+    assert np.array_equal(bits(got), bits(orc.classic_endpoints(pts, cvs)))
+    assert bool((counts == cvs + 2).all())
+
+
+def test_cut_then_endpoints_on_device(torch, orc, classic):
+    """The kept count stays on the device: endpoints read it there."""
+    cvs = int(classic["walk_cvs"])
+    pts = classic["walk_points"]
+    n = len(pts) // cvs
+    out = classic_cut(dev(torch, pts), cvs, 0.4, 0.1, radius_scale=2.0)
+    sentinel = np.float32(-7.0)
+    ends = torch.full((n * (cvs + 2), 4), float(sentinel), dtype=torch.float32, device="cuda")
+    add_endpoints(out.points, cvs, strands=n, device_strand_count=out.words, out=ends)
+    k = out.kept()
+    want = classic["case3_endpoints"]
+    assert k * (cvs + 2) == len(want)
+    got = ends.cpu().numpy()
+    assert np.array_equal(bits(got[:len(want)]), bits(want))
+    assert np.all(got[len(want):] == sentinel)  # nothing past the kept strands
