@@ -755,6 +755,60 @@ uint32_t nxg_classic_add_endpoints(const nxg_packed_point *points, uint32_t stra
                                    uint64_t out_point_capacity, uint32_t *out_point_counts,
                                    uint64_t out_point_count_capacity, void *stream);
 
+/* ClumpingFX (one ClassicFloatClumpModule of the reference's float runtime,
+ * src/xgen_classic_runtime.cpp:1680-1793) with world-space clump guides
+ * (ClassicClumpRuntimeData, include/nanoxgen/xgen_classic_runtime.h:59-88,
+ * prepared as prepare_xgen_classic_clump_runtime_data does,
+ * src/xgen_classic_runtime.cpp:839-890) and constant module expressions.
+ * Per strand whose guide index is not NXG_INVALID_INDEX and whose mask is
+ * positive: cut_to_length_and_rebuild to the guide's spline length
+ * (src/xgen_classic_runtime.cpp:750-799), blend CVs 1.. toward the guide's
+ * render axis by mask * amount * (1 - 2 clump_scale), then add the guide's
+ * clump-noise axis (build_clump_noise_axis, src/xgen_classic_runtime.cpp:
+ * 1041-1186: two gradient-noise samples along a transported frame, scaled by
+ * mask * noise * noise_scale).  Other strands are copied unchanged.  The
+ * device counterpart in the reference is make_classic_runtime_clump_kernel
+ * (src/luisa/xgen_classic_runtime.cpp:481-764).  The noise frame uses
+ * acosf/cosf/sinf, so noisy positions agree with the reference to a
+ * tolerance; with noise off they are bit-identical. */
+#define NXG_CLUMP_MAX_CVS 64u
+#define NXG_CLASSIC_INVALID_GUIDE 4u  /* "Classic ClumpingFX guide index is invalid" */
+
+typedef struct nxg_clump_guides {
+    const nxg_vec3 *axes;                /* guide_count x cvs_per_strand CVs (guide_axes), device */
+    const nxg_vec3 *normals;             /* guide_count, device */
+    const nxg_vec3 *tangents;            /* guide_count, device */
+    const nxg_vec3 *reference_positions; /* optional noise-domain points (guide_reference_positions),
+                                            else each guide's first CV */
+    uint32_t guide_count;
+    uint32_t reserved;
+} nxg_clump_guides;
+
+typedef struct nxg_clump_params {
+    float mask;               /* mask, clamped to [0, 1]; <= 0 leaves strands alone */
+    float amount;             /* clump, clamped to [0, 1] (after the multiplier) */
+    float clump_scale;        /* clumpScale */
+    float noise;              /* noise */
+    float noise_scale;        /* noiseScale */
+    float noise_frequency;    /* noiseFrequency */
+    float noise_correlation;  /* noiseCorrelation, in percent */
+    uint32_t amount_stride;   /* floats between consecutive strands' amount multipliers */
+    const float *amount_multiplier; /* optional, device: clump = amount * multiplier[s * stride] */
+} nxg_clump_params;
+
+/* Device workspace of nxg_classic_clump_apply. */
+uint64_t nxg_classic_clump_workspace_size(uint32_t guide_count, uint32_t cvs_per_strand);
+
+/* On `stream`: out_points may equal points (in place) or must not overlap
+ * them; strand_guides is device memory (one index per strand); *status
+ * (optional device word) receives 0 or NXG_CLASSIC_INVALID_GUIDE / _NONFINITE_AMOUNT
+ * bits (such strands are copied unchanged). */
+uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand_count, uint32_t cvs_per_strand,
+                                 const uint32_t *strand_guides, const nxg_clump_guides *guides,
+                                 const nxg_clump_params *params, nxg_packed_point *out_points,
+                                 uint64_t out_point_capacity, uint32_t *status, void *workspace,
+                                 uint64_t workspace_bytes, void *stream);
+
 /* Library build identity: gfx target and compile flags that affect parity. */
 const char *nxg_build_info(void);
 

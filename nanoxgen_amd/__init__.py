@@ -67,6 +67,9 @@ SYMBOLS = [
     ("nxg_classic_cut_workspace_size", C.c_uint64, [C.c_uint32]),
     ("nxg_classic_cut_apply", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_classic_add_endpoints", C.c_uint32, [_P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint64, _P, C.c_uint64, _P]),
+    ("nxg_classic_clump_workspace_size", C.c_uint64, [C.c_uint32, C.c_uint32]),
+    ("nxg_classic_clump_apply", C.c_uint32, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, C.c_uint64, _P, _P,
+                                             C.c_uint64, _P]),
     ("nxg_build_info", C.c_char_p, []),
 ]
 

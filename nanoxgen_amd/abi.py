@@ -274,6 +274,24 @@ class ClassicOutput(C.Structure):
                 ("status", C.c_void_p)]
 
 
+CLUMP_MAX_CVS = 64
+CLASSIC_INVALID_GUIDE = 4
+INVALID_INDEX = 0xFFFFFFFF
+
+
+class ClumpGuides(C.Structure):
+    """nxg_clump_guides: world-space clump guides (device pointers)."""
+    _fields_ = [("axes", C.c_void_p), ("normals", C.c_void_p), ("tangents", C.c_void_p),
+                ("reference_positions", C.c_void_p), ("guide_count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ClumpParams(C.Structure):
+    """nxg_clump_params: the ClumpingFX module's constant expressions."""
+    _fields_ = [("mask", C.c_float), ("amount", C.c_float), ("clump_scale", C.c_float), ("noise", C.c_float),
+                ("noise_scale", C.c_float), ("noise_frequency", C.c_float), ("noise_correlation", C.c_float),
+                ("amount_stride", C.c_uint32), ("amount_multiplier", C.c_void_p)]
+
+
 BATCH_MAX_ITEMS = 12
 BATCH_ORDER_GIVEN, BATCH_ORDER_HEAVY_FIRST = 0, 1
 

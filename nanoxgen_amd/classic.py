@@ -120,3 +120,52 @@ def add_endpoints(points, cvs, *, strands=None, device_strand_count=None, out=No
                                           0 if point_counts is None else point_counts.numel(),
                                           _stream_handle(stream)), "nxg_classic_add_endpoints")
     return out
+
+
+def clump_workspace_bytes(guide_count, cvs):
+    return int(lib().nxg_classic_clump_workspace_size(guide_count, cvs))
+
+
+def clump(points, cvs, strand_guides, guides, params, *, amount_multiplier=None, amount_stride=1, out=None,
+          status=None, workspace=None, stream=None):
+    """ClumpingFX with constant module expressions (nxg_classic_clump_apply)
+    on device curves: `points` float32 [N * cvs, 4]; `strand_guides` int32 [N]
+    (-1: no guide); `guides` dict of device float32 tensors axes [G * cvs, 3],
+    normals [G, 3], tangents [G, 3] and optionally reference_positions [G, 3];
+    `params` dict with mask, amount, clump_scale, noise, noise_scale,
+    noise_frequency, noise_correlation.  `out` defaults to a new tensor; pass
+    `points` itself to clump in place; `status` (int32 device tensor) receives
+    the NXG_CLASSIC_* error bits."""
+    torch = _torch()
+    if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
+        raise ValueError("points must be a contiguous float32 [N * cvs, 4] tensor")
+    n = points.shape[0] // cvs
+    axes = guides["axes"]
+    g = axes.shape[0] // cvs
+    for name in ("axes", "normals", "tangents", "reference_positions"):
+        x = guides.get(name)
+        if x is not None and (x.dtype != torch.float32 or not x.is_contiguous() or x.shape[-1] != 3):
+            raise ValueError(f"guides[{name!r}] must be a contiguous float32 [..., 3] tensor")
+    if strand_guides.numel() < n or not strand_guides.is_contiguous():
+        raise ValueError("strand_guides must hold one index per strand")
+    if amount_multiplier is not None:
+        reach = (n - 1) * amount_stride + 1 if n else 0
+        if amount_multiplier.untyped_storage().nbytes() // 4 - amount_multiplier.storage_offset() < reach:
+            raise ValueError("amount_multiplier is too short for its stride")
+    if out is None:
+        out = torch.empty_like(points)
+    if workspace is None:
+        workspace = torch.empty(clump_workspace_bytes(g, cvs), dtype=torch.uint8, device=points.device)
+    t = lambda x: None if x is None else x.data_ptr()
+    gd = abi.ClumpGuides(axes=axes.data_ptr(), normals=guides["normals"].data_ptr(),
+                         tangents=guides["tangents"].data_ptr(),
+                         reference_positions=t(guides.get("reference_positions")), guide_count=g)
+    pr = abi.ClumpParams(amount_stride=amount_stride if amount_multiplier is not None else 0,
+                         amount_multiplier=t(amount_multiplier),
+                         **{k: float(params[k]) for k in ("mask", "amount", "clump_scale", "noise", "noise_scale",
+                                                          "noise_frequency", "noise_correlation")})
+    check(lib().nxg_classic_clump_apply(points.data_ptr(), n, cvs, strand_guides.data_ptr(), C.byref(gd),
+                                        C.byref(pr), out.data_ptr(), out.shape[0], t(status),
+                                        C.c_void_p(workspace.data_ptr()), workspace.numel(),
+                                        _stream_handle(stream)), "nxg_classic_clump_apply")
+    return out

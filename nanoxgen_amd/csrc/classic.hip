@@ -42,6 +42,7 @@
 // CVs from memory instead (same arithmetic).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 #include "../../include/nanoxgen_amd.h"
@@ -289,6 +290,8 @@ __global__ void __launch_bounds__(256) cut_table_kernel(CutArgs a) {
         a.walk_samples[j] = make_sample(n, walk_t[j], j ? walk_t[j - 1u] : 1.0f);
 }
 
+__global__ void clear_word_kernel(uint32_t *word) { *word = 0u; }
+
 template <bool STAGED>
 __global__ void __launch_bounds__(kWave) cut_search_kernel(CutArgs a) {
     extern __shared__ float4 tile[];
@@ -508,6 +511,314 @@ __global__ void __launch_bounds__(256) endpoints_kernel(EndpointArgs a) {
     if (a.counts && threadIdx.x < count) a.counts[first + threadIdx.x] = m;
 }
 
+// ---- ClumpingFX ---------------------------------------------------------
+
+__constant__ float4 kClumpGradients[256] = {
+#define NXG_GRADIENT(x, y, z) {x, y, z, 0.0f},
+#include "seexpr_gradients.inc"
+#undef NXG_GRADIENT
+};
+
+struct ClumpArgs {
+    const float4 *points;
+    float4 *out;
+    const uint32_t *strand_guides;
+    const nxg_vec3 *axes, *normals, *tangents, *reference;
+    uint32_t strands, cvs, guides, div_magic;
+    float mask;        // clamped to [0, 1]
+    float amount, clump_scale;
+    float magnitude;   // mask * noise * max(noise_scale, 0); 0 when the noise axis is off
+    float frequency;   // max(noise_frequency, 0)
+    float correlation; // clamp(noise_correlation * 0.01, 0, 1)
+    const float *amount_multiplier;
+    uint32_t amount_stride;
+    float4 *render;    // [guides x cvs] guide render axes
+    float4 *noise;     // [guides x cvs] clump-noise axes, already scaled
+    float *lengths;    // [guides] guide spline lengths
+    const Sample *length_samples;
+    uint32_t *status;
+};
+
+struct Vec3Curve {
+    const nxg_vec3 *p;
+    __device__ __forceinline__ float4 operator[](uint32_t c) const { return make_float4(p[c].x, p[c].y, p[c].z, 0.0f); }
+};
+
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ F3 normalize3(F3 v) {  // generate.h:67-70
+    const float l2 = length_squared(v);
+    return l2 > 1.0e-20f ? F3{v.x / sqrtf(l2), v.y / sqrtf(l2), v.z / sqrtf(l2)} : F3{0.0f, 1.0f, 0.0f};
+}
+__device__ __forceinline__ float max_std(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float clamp_std(float x, float lo, float hi) { return x < lo ? lo : (hi < x ? hi : x); }
+
+// xgen_classic_noise_float (src/xgen_classic_runtime.cpp:802-837)
+__device__ float classic_noise(float sx, float sy, float sz) {
+    const float fx = floorf(sx), fy = floorf(sy), fz = floorf(sz);
+    const int32_t ix = static_cast<int32_t>(fx), iy = static_cast<int32_t>(fy), iz = static_cast<int32_t>(fz);
+    const float wx = sx - fx, wy = sy - fy, wz = sz - fz;
+    float v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        uint32_t seed = 0u;  // noise_hash (generate.h:353-364)
+        seed = seed * 1664525u + static_cast<uint32_t>(ix + ox) + 1013904223u;
+        seed = seed * 1664525u + static_cast<uint32_t>(iy + oy) + 1013904223u;
+        seed = seed * 1664525u + static_cast<uint32_t>(iz + oz) + 1013904223u;
+        seed ^= seed >> 11u;
+        seed ^= (seed << 7u) & 0x9d2c5680u;
+        seed ^= (seed << 15u) & 0xefc60000u;
+        seed ^= seed >> 18u;
+        const float4 g = kClumpGradients[(((seed & 0x00ff0000u) >> 4u) + (seed & 0xffu)) & 0xffu];
+        v[c] = g.x * (wx - static_cast<float>(ox)) + g.y * (wy - static_cast<float>(oy)) +
+               g.z * (wz - static_cast<float>(oz));
+    }
+    const float ax = wx * wx * wx * (wx * (wx * 6.0f - 15.0f) + 10.0f);
+    const float ay = wy * wy * wy * (wy * (wy * 6.0f - 15.0f) + 10.0f);
+    const float az = wz * wz * wz * (wz * (wz * 6.0f - 15.0f) + 10.0f);
+    v[0] = (1.0f - ax) * v[0] + ax * v[1];
+    v[2] = (1.0f - ax) * v[2] + ax * v[3];
+    v[4] = (1.0f - ax) * v[4] + ax * v[5];
+    v[6] = (1.0f - ax) * v[6] + ax * v[7];
+    v[0] = (1.0f - ay) * v[0] + ay * v[2];
+    v[4] = (1.0f - ay) * v[4] + ay * v[6];
+    v[0] = (1.0f - az) * v[0] + az * v[4];
+    return 0.5f * v[0] + 0.5f;
+}
+
+// rotate_between (src/xgen_classic_runtime.cpp:1109-1122)
+__device__ F3 rotate_between(F3 value, F3 from, F3 to, float fraction) {
+    F3 axis = cross3(from, to);
+    const float als = length_squared(axis);
+    if (!(als >= 1.0e-12f)) return value;
+    axis = axis * (1.0f / sqrtf(als));
+    const float angle = acosf(clamp_std(dot3(from, to), -1.0f, 1.0f)) * fraction;
+    const float c = cosf(angle), s = sinf(angle);
+    return value * c + cross3(axis, value) * s + axis * (dot3(axis, value) * (1.0f - c));
+}
+
+// One thread per guide: prepare_xgen_classic_clump_runtime_data's render
+// axis and spline length (src/xgen_classic_runtime.cpp:839-875), then the
+// guide's clump-noise axis (build_clump_noise_axis, 1041-1186), shared by
+// every strand the guide clumps since the module's expressions are constants.
+__global__ void __launch_bounds__(64) clump_guide_kernel(ClumpArgs a) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.guides) return;
+    const uint32_t n = a.cvs;
+    const Vec3Curve src{a.axes + uint64_t{g} * n};
+    float4 *render = a.render + uint64_t{g} * n;
+    float4 *noise = a.noise + uint64_t{g} * n;
+    render[0] = src[0];
+    for (uint32_t cv = 1u; cv + 1u < n; ++cv) {
+        const F3 q = curve_eval(src, n, static_cast<float>(cv) / static_cast<float>(n - 1u));
+        render[cv] = make_float4(q.x, q.y, q.z, 0.0f);
+    }
+    render[n - 1u] = src[n - 1u];
+    {  // curve_spline_length (src/xgen_classic_runtime.cpp:665-690)
+        const uint32_t intervals = n * 2u + 4u;
+        const float step = 1.0f / static_cast<float>(intervals);
+        F3 previous = xyz(src[0]);
+        float result = 0.0f;
+        for (uint32_t k = 1u; k < intervals; ++k) {
+            const F3 current = curve_eval(src, n, step * static_cast<float>(k));
+            result += sqrtf(length_squared(current - previous));
+            previous = current;
+        }
+        result += sqrtf(length_squared(xyz(src[n - 1u]) - previous));
+        a.lengths[g] = result;
+    }
+    for (uint32_t cv = 0u; cv < n; ++cv) noise[cv] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a.magnitude == 0.0f) return;  // the noise or mask test failed: a zero axis
+    float polyline = 0.0f;
+    for (uint32_t cv = 1u; cv < n; ++cv) polyline += sqrtf(length_squared(xyz(src[cv]) - xyz(src[cv - 1u])));
+    const float effective = polyline > 0.0f ? max_std(0.5f / polyline, a.frequency) : a.frequency;
+    const float decorrelation = 1.0f - a.correlation;
+    const float domain_scale = 100.0f * decorrelation * decorrelation;
+    const nxg_vec3 dp = a.reference ? a.reference[g] : a.axes[uint64_t{g} * n];
+    const F3 domain = (F3{dp.x, dp.y, dp.z} + F3{0.419276f, 0.184247f, 0.805721f}) * domain_scale;
+    const nxg_vec3 rn = a.normals[g], rt = a.tangents[g];
+    F3 normal{rn.x, rn.y, rn.z}, tangent{rt.x, rt.y, rt.z};
+    normal = !(length_squared(normal) > 1.0e-20f) ? F3{0.0f, 1.0f, 0.0f} : normalize3(normal);
+    if (!(length_squared(tangent) > 1.0e-20f)) {  // fallback_surface_u (generate.h:312-316)
+        const F3 axis = normal.z > -0.999f && normal.z < 0.999f ? F3{0.0f, 0.0f, 1.0f} : F3{0.0f, 1.0f, 0.0f};
+        tangent = normalize3(cross3(normal, axis));
+    } else {
+        tangent = normalize3(tangent);
+    }
+    F3 tu = tangent, tv = cross3(normal, tangent), current = normal;
+    const F3 first_segment = xyz(render[1]) - xyz(render[0]);
+    if (length_squared(first_segment) > 1.0e-20f) current = normalize3(first_segment);
+    if (dot3(normal, current) < 0.99999f) {
+        tu = rotate_between(tu, normal, current, 1.0f);
+        tv = rotate_between(tv, normal, current, 1.0f);
+    }
+    float travelled = 0.0f;
+    for (uint32_t cv = 1u; cv < n; ++cv) {
+        travelled += sqrtf(length_squared(xyz(src[cv]) - xyz(src[cv - 1u])));
+        F3 su = tu, sv = tv, next = current;
+        bool advance = false;
+        if (cv + 1u < n) {
+            const F3 segment = xyz(render[cv + 1u]) - xyz(render[cv]);
+            if (length_squared(segment) > 1.0e-20f) {
+                next = normalize3(segment);
+                if (fabsf(dot3(current, next)) < 0.99999f) {
+                    su = rotate_between(tu, current, next, 0.5f);
+                    sv = rotate_between(tv, current, next, 0.5f);
+                    advance = true;
+                } else {
+                    current = next;
+                }
+            }
+        }
+        const float distance = travelled * effective;
+        const float first = classic_noise(domain.x + distance, domain.y, domain.z) - 0.5f;
+        const float second = classic_noise(domain.x, domain.y, domain.z + distance) - 0.5f;
+        const F3 d = (su * first + sv * second) * a.magnitude;
+        noise[cv] = make_float4(d.x, d.y, d.z, 0.0f);
+        if (advance) {
+            tu = rotate_between(tu, current, next, 1.0f);
+            tv = rotate_between(tv, current, next, 1.0f);
+            current = next;
+        }
+    }
+}
+
+// The clump step of one strand (src/xgen_classic_runtime.cpp:1752-1792),
+// CVs read from the staged tile, point i handed to out(i, value); false
+// leaves the strand alone.  MAXN > 0 unrolls the CV loop (i < MAXN), so a
+// caller holding the strand in registers indexes them statically.
+template <uint32_t MAXN, class Out>
+__device__ __forceinline__ bool clump_strand(const ClumpArgs &a, const StagedCurve<false> &p, uint32_t s, Out &&out,
+                                             uint32_t *err) {
+    const uint32_t n = a.cvs;
+    const uint32_t g = a.strand_guides[s];
+    if (g == NXG_INVALID_INDEX || !(a.mask > 0.0f)) return false;
+    if (g >= a.guides) {
+        *err |= NXG_CLASSIC_INVALID_GUIDE;
+        return false;
+    }
+    float raw = a.amount;
+    if (a.amount_multiplier) raw = raw * a.amount_multiplier[uint64_t{s} * a.amount_stride];
+    if (!isfinite(raw)) {
+        *err |= NXG_CLASSIC_NONFINITE_AMOUNT;
+        return false;
+    }
+    const float amount = clamp_std(raw, 0.0f, 1.0f);
+    // cut_to_length_and_rebuild (src/xgen_classic_runtime.cpp:750-799)
+    const float target = a.lengths[g];
+    float cut = 1.0f;
+    if (target <= 1.0e-10f) {
+        cut = -1.0f;  // every CV at the root
+    } else {
+        float previous_t = 0.0f, accumulated = 0.0f;
+        F3 previous = xyz(p[0]);
+        const uint32_t samples = 2u * n + 3u;
+        for (uint32_t k = 0u; k <= samples; ++k) {
+            float t;
+            F3 current;
+            if (k < samples) {
+                const Sample smp = load_sample(a.length_samples, k);
+                t = smp.t;
+                current = blend(p, n, smp);
+            } else {
+                t = 1.0f;
+                current = xyz(p[n - 1u]);
+            }
+            const float segment = sqrtf(length_squared(current - previous));
+            accumulated += segment;
+            if (accumulated >= target && segment > 0.0f) {
+                cut = t - ((accumulated - target) / segment) * (t - previous_t);
+                break;
+            }
+            previous_t = t;
+            previous = current;
+        }
+    }
+    const float factor = a.mask * amount * (1.0f - 2.0f * a.clump_scale);
+    const float4 *render = a.render + uint64_t{g} * n;
+    const float4 *noise = a.noise + uint64_t{g} * n;
+    const float step = cut / static_cast<float>(n - 1u);
+    auto point = [&](uint32_t i) {
+        F3 q = cut < 0.0f ? xyz(p[0]) : curve_eval(p, n, step * static_cast<float>(i));
+        q = q + (xyz(render[i]) - q) * factor;
+        q = q + xyz(noise[i]);
+        return make_float4(q.x, q.y, q.z, p[i].w);
+    };
+    out(0u, p[0]);
+    if constexpr (MAXN > 0u) {
+#pragma unroll
+        for (uint32_t i = 1u; i < MAXN; ++i)
+            if (i < n) out(i, point(i));
+    } else {
+        for (uint32_t i = 1u; i < n; ++i) out(i, point(i));
+    }
+    return true;
+}
+
+__device__ __forceinline__ void clump_report(const ClumpArgs &a, uint32_t err) {
+    const uint32_t bits = (__ballot(err & NXG_CLASSIC_INVALID_GUIDE) ? NXG_CLASSIC_INVALID_GUIDE : 0u) |
+                          (__ballot(err & NXG_CLASSIC_NONFINITE_AMOUNT) ? NXG_CLASSIC_NONFINITE_AMOUNT : 0u);
+    if (bits && threadIdx.x == 0u && a.status) atomicOr(a.status, bits);
+}
+
+// One wavefront per 64-strand tile.  MAXN > 0: each strand's CVs are held
+// in registers, then laid out in LDS in output order and stored as one
+// contiguous run; MAXN == 0: per-lane stores (the source stays in LDS, so
+// the call may run in place).
+template <uint32_t MAXN>
+__global__ void __launch_bounds__(kWave) clump_strand_kernel(ClumpArgs a) {
+    extern __shared__ float4 tile[];
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    {
+        CutArgs c{};
+        c.points = a.points;
+        c.cvs = n;
+        c.div_magic = a.div_magic;
+        stage_tile<false>(c, tile, first, count);
+    }
+    const StagedCurve<false> p{tile, lane};
+    const uint32_t s = first + lane;
+    uint32_t err = 0u;
+    if constexpr (MAXN == 0u) {
+        if (lane < count) {
+            float4 *dst = a.out + uint64_t{s} * n;
+            if (!clump_strand<0>(a, p, s, [&](uint32_t i, float4 v) { dst[i] = v; }, &err))
+                for (uint32_t i = 0u; i < n; ++i) dst[i] = p[i];
+        }
+    } else {
+        float4 held[MAXN];
+        bool done = false;
+        if (lane < count) {
+            done = clump_strand<MAXN>(a, p, s, [&](uint32_t i, float4 v) { held[i] = v; }, &err);
+            if (!done) {
+#pragma unroll
+                for (uint32_t j = 0u; j < MAXN; ++j)
+                    if (j < n) held[j] = p[j];
+            }
+        }
+        __syncthreads();
+        const uint32_t stride = n + 1u;
+        if (lane < count) {
+#pragma unroll
+            for (uint32_t j = 0u; j < MAXN; ++j)
+                if (j < n) tile[lane * stride + j] = held[j];
+        }
+        __syncthreads();
+        float4 *dst = a.out + uint64_t{first} * n;
+        for (uint32_t e = lane; e < count * n; e += kWave) {
+            const uint32_t r = __umulhi(e, a.div_magic);
+            dst[e] = tile[r * stride + (e - r * n)];
+        }
+    }
+    clump_report(a, err);
+}
+
 }  // namespace
 }  // namespace nxg
 
@@ -617,5 +928,112 @@ extern "C" uint32_t nxg_classic_add_endpoints(const nxg_packed_point *points, ui
     a.magic = static_cast<uint32_t>(0xffffffffu / (cvs_per_strand + 2u) + 1u);
     hipLaunchKernelGGL(endpoints_kernel, dim3(classic_tiles(strand_count)), dim3(256), 0,
                        static_cast<hipStream_t>(stream_ptr), a);
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
+namespace nxg {
+namespace {
+struct ClumpLayout {
+    uint64_t render_offset, noise_offset, lengths_offset, table_offset, total;
+};
+ClumpLayout clump_layout(uint32_t guides, uint32_t cvs) {
+    auto al = [](uint64_t v) { return (v + 255u) & ~uint64_t{255u}; };
+    ClumpLayout l{};
+    const uint64_t axis = uint64_t{guides} * cvs * sizeof(float4);
+    l.render_offset = 0u;
+    l.noise_offset = al(axis);
+    l.lengths_offset = l.noise_offset + al(axis);
+    l.table_offset = l.lengths_offset + al(uint64_t{guides} * 4u);
+    l.total = l.table_offset + al(uint64_t{kClassicTableEntries} * sizeof(Sample));
+    return l;
+}
+}  // namespace
+}  // namespace nxg
+
+extern "C" uint64_t nxg_classic_clump_workspace_size(uint32_t guide_count, uint32_t cvs_per_strand) {
+    return nxg::clump_layout(guide_count, cvs_per_strand).total;
+}
+
+extern "C" uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand_count,
+                                            uint32_t cvs_per_strand, const uint32_t *strand_guides,
+                                            const nxg_clump_guides *guides, const nxg_clump_params *params,
+                                            nxg_packed_point *out_points, uint64_t out_point_capacity,
+                                            uint32_t *status, void *workspace, uint64_t workspace_bytes,
+                                            void *stream_ptr) {
+    using namespace nxg;
+    if (!guides || !params) return NXG_ERROR_NULL_ARGUMENT;
+    const uint64_t n = strand_count, cvs = cvs_per_strand;
+    if (cvs < 3u || cvs > NXG_CLUMP_MAX_CVS || n > NXG_CLASSIC_MAX_STRANDS) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    const float k[7] = {params->mask, params->amount, params->clump_scale, params->noise, params->noise_scale,
+                        params->noise_frequency, params->noise_correlation};
+    for (float v : k)
+        if (!std::isfinite(v)) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (params->amount_multiplier && params->amount_stride == 0u) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (n == 0u) return NXG_OK;
+    if (!points || !out_points || !strand_guides || !guides->axes || !guides->normals || !guides->tangents)
+        return NXG_ERROR_NULL_ARGUMENT;
+    if (guides->guide_count == 0u) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (reinterpret_cast<uintptr_t>(points) % 16u != 0u || reinterpret_cast<uintptr_t>(out_points) % 16u != 0u)
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (out_point_capacity < n * cvs) return NXG_ERROR_POINT_CAPACITY_TOO_SMALL;
+    const uintptr_t in0 = reinterpret_cast<uintptr_t>(points), in1 = in0 + n * cvs * 16u;
+    const uintptr_t o0 = reinterpret_cast<uintptr_t>(out_points), o1 = o0 + n * cvs * 16u;
+    if (o0 != in0 && o0 < in1 && in0 < o1) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    const ClumpLayout layout = clump_layout(guides->guide_count, cvs_per_strand);
+    if (!workspace || workspace_bytes < layout.total || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
+        return NXG_ERROR_WORKSPACE_TOO_SMALL;
+    // the module's constants, evaluated as the reference evaluates them per strand
+    const float mask = params->mask < 0.0f ? 0.0f : (1.0f < params->mask ? 1.0f : params->mask);
+    const float noise = params->noise < 0.0f ? 0.0f : params->noise;
+    const float noise_scale = params->noise_scale < 0.0f ? 0.0f : params->noise_scale;
+    const bool noisy = noise > 1.0e-5f && mask > 1.0e-4f;
+    const float magnitude = noisy ? mask * noise * noise_scale : 0.0f;
+    if (!std::isfinite(magnitude)) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    uint8_t *ws = static_cast<uint8_t *>(workspace);
+    ClumpArgs a{};
+    a.points = reinterpret_cast<const float4 *>(points);
+    a.out = reinterpret_cast<float4 *>(out_points);
+    a.strand_guides = strand_guides;
+    a.axes = guides->axes;
+    a.normals = guides->normals;
+    a.tangents = guides->tangents;
+    a.reference = guides->reference_positions;
+    a.strands = strand_count;
+    a.cvs = cvs_per_strand;
+    a.guides = guides->guide_count;
+    a.div_magic = static_cast<uint32_t>(0xffffffffu / cvs_per_strand + 1u);
+    a.mask = mask;
+    a.amount = params->amount;
+    a.clump_scale = params->clump_scale;
+    a.magnitude = magnitude;
+    a.frequency = params->noise_frequency < 0.0f ? 0.0f : params->noise_frequency;
+    const float corr = params->noise_correlation * 0.01f;
+    a.correlation = corr < 0.0f ? 0.0f : (1.0f < corr ? 1.0f : corr);
+    a.amount_multiplier = params->amount_multiplier;
+    a.amount_stride = params->amount_stride;
+    a.render = reinterpret_cast<float4 *>(ws + layout.render_offset);
+    a.noise = reinterpret_cast<float4 *>(ws + layout.noise_offset);
+    a.lengths = reinterpret_cast<float *>(ws + layout.lengths_offset);
+    a.length_samples = reinterpret_cast<const Sample *>(ws + layout.table_offset);
+    a.status = status;
+    CutArgs table{};
+    table.cvs = cvs_per_strand;
+    table.length_samples = reinterpret_cast<Sample *>(ws + layout.table_offset);
+    table.walk_count = 0u;
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    if (status) hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, status);
+    hipLaunchKernelGGL(cut_table_kernel, dim3(1), dim3(256), 0, stream, table);
+    hipLaunchKernelGGL(clump_guide_kernel, dim3((a.guides + 63u) / 64u), dim3(64), 0, stream, a);
+    const uint32_t tiles = classic_tiles(strand_count);
+    const size_t tile_lds = size_t{kWave} * cvs * sizeof(float4);
+    const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);
+    if (cvs <= 8u)
+        hipLaunchKernelGGL(clump_strand_kernel<8>, dim3(tiles), dim3(kWave), held_lds, stream, a);
+    else if (cvs <= 16u)
+        hipLaunchKernelGGL(clump_strand_kernel<16>, dim3(tiles), dim3(kWave), held_lds, stream, a);
+    else if (cvs <= 32u)
+        hipLaunchKernelGGL(clump_strand_kernel<32>, dim3(tiles), dim3(kWave), held_lds, stream, a);
+    else
+        hipLaunchKernelGGL(clump_strand_kernel<0>, dim3(tiles), dim3(kWave), tile_lds, stream, a);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }

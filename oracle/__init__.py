@@ -77,6 +77,11 @@ def lib():
         L.orc_curve_spline_length.restype = C.c_float
         L.orc_curve_spline_length.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_clump_prepare.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+        L.orc_clump_apply.restype = C.c_uint32
+        L.orc_clump_apply.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_uint32]
         L.orc_classic_cut.restype = C.c_uint32
         L.orc_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p,
@@ -111,6 +116,9 @@ def ref():
         L.ref_hash32.argtypes = [C.c_uint32]
         L.ref_random01.restype = C.c_float
         L.ref_random01.argtypes = [C.c_uint32] * 3
+        L.ref_clump_apply.restype = C.c_uint32
+        L.ref_clump_apply.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
         L.ref_classic_endpoints.restype = C.c_uint32
         L.ref_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
@@ -361,3 +369,62 @@ def ref_classic_endpoints(points, cvs):
     n = len(pts) // cvs
     out = np.zeros((n * (cvs + 2), 4), np.float32)
     return None if ref().ref_classic_endpoints(pts.ctypes.data, n, cvs, out.ctypes.data) else out
+
+
+CLUMP_KEYS = ("mask", "amount", "clump_scale", "noise", "noise_scale", "noise_frequency", "noise_correlation")
+
+
+def _f3(a, rows):
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(rows, 3))
+
+
+def clump_prepare(axes, cvs):
+    """prepare_xgen_classic_clump_runtime_data: (render axes [G*cvs, 3], spline lengths [G])."""
+    ax = _f3(axes, -1)
+    g = len(ax) // cvs
+    render = np.zeros_like(ax)
+    lengths = np.zeros(g, np.float32)
+    lib().orc_clump_prepare(ax.ctypes.data, g, cvs, render.ctypes.data, lengths.ctypes.data)
+    return render, lengths
+
+
+def clump_apply(points, cvs, strand_guides, guides, params, gradients, amount_multiplier=None, multiplier_stride=1):
+    """ClumpingFX with constant expressions (params: dict of CLUMP_KEYS) on
+    packed strands; guides: dict(axes, normals, tangents[, reference_positions]);
+    gradients: the 256 x 3 SeExpr table.  Returns the new points, or None
+    where the reference throws."""
+    pts = np.array(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    ax = _f3(guides["axes"], -1)
+    g = len(ax) // cvs
+    render, lengths = clump_prepare(ax, cvs)
+    nrm, tan = _f3(guides["normals"], g), _f3(guides["tangents"], g)
+    ref3 = guides.get("reference_positions")
+    ref3 = None if ref3 is None else _f3(ref3, g)
+    sg = np.ascontiguousarray(strand_guides, np.uint32)
+    k = (C.c_float * 7)(*[params[key] for key in CLUMP_KEYS])
+    grad = np.ascontiguousarray(gradients, np.float32)
+    m = None if amount_multiplier is None else np.ascontiguousarray(amount_multiplier, np.float32)
+    rc = lib().orc_clump_apply(pts.ctypes.data, n, cvs, sg.ctypes.data, ax.ctypes.data, render.ctypes.data,
+                               lengths.ctypes.data, nrm.ctypes.data, tan.ctypes.data, abi.ptr(ref3), g,
+                               grad.ctypes.data, k, abi.ptr(m), multiplier_stride)
+    return None if rc else pts
+
+
+def ref_clump_apply(points, cvs, strand_guides, guides, params):
+    """The reference's float runtime with one constant-expression ClumpingFX module."""
+    pts = np.array(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    ax = _f3(guides["axes"], -1)
+    g = len(ax) // cvs
+    nrm, tan = _f3(guides["normals"], g), _f3(guides["tangents"], g)
+    ref3 = guides.get("reference_positions")
+    ref3 = None if ref3 is None else _f3(ref3, g)
+    sg = np.ascontiguousarray(strand_guides, np.uint32)
+    k = np.array([params[key] for key in CLUMP_KEYS], np.float32)
+    kept = ref().ref_clump_apply(pts.ctypes.data, n, cvs, sg.ctypes.data, ax.ctypes.data, nrm.ctypes.data,
+                                 tan.ctypes.data, abi.ptr(ref3), g, k.ctypes.data)
+    if kept == 0xFFFFFFFF:
+        return None
+    assert kept == n, "strands were culled by the runtime; pick inputs with length and width"
+    return pts

@@ -93,6 +93,17 @@ uint32_t orc_root_attributes(const orc_job *job, const nxg_expr_program *program
 float orc_curve_eval(const float *points4, uint32_t n, float t, float *out3);
 float orc_curve_spline_length(const float *points4, uint32_t n);
 void orc_classic_endpoints(const float *points, uint32_t strand_count, uint32_t cvs, float *out);
+
+/* Classic ClumpingFX with constant module expressions (oracle_clump.c). */
+typedef struct orc_clump_params {
+    float mask, amount, clump_scale, noise, noise_scale, noise_frequency, noise_correlation;
+} orc_clump_params;
+void orc_clump_prepare(const float *axes3, uint32_t guides, uint32_t cvs, float *render3, float *lengths);
+uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, const uint32_t *strand_guides,
+                         const float *axes3, const float *render3, const float *lengths, const float *normals3,
+                         const float *tangents3, const float *reference3, uint32_t guides,
+                         const nxg_vec3 *gradients, const orc_clump_params *p, const float *amount_multiplier,
+                         uint32_t multiplier_stride);
 uint32_t orc_classic_cut(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
                          const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, float amount_scale,
                          float amount_offset, const float *multiplier, float radius_scale, float *out_points,

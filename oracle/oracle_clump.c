@@ -1,0 +1,225 @@
+/*
+ * ORACLE (test infrastructure only): the Classic ClumpingFX module of the
+ * reference's float runtime restated in C, for a world-space guide binding
+ * (no root-relative axes) and module expressions that are constants (the
+ * amount optionally times a per-strand multiplier):
+ *   prepare_xgen_classic_clump_runtime_data  src/xgen_classic_runtime.cpp:839-890
+ *                                            (render axes, guide spline lengths)
+ *   cut_to_length_and_rebuild                src/xgen_classic_runtime.cpp:750-799
+ *   build_clump_noise_axis                   src/xgen_classic_runtime.cpp:1041-1186
+ *   xgen_classic_noise_float                 src/xgen_classic_runtime.cpp:802-837
+ *   per-strand clump step                    src/xgen_classic_runtime.cpp:1680-1793
+ * Float operations in the reference's order, built with -ffp-contract=off;
+ * acosf/cosf/sinf are the C library's, as the reference's std:: calls are.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "oracle.h"
+
+typedef struct { float x, y, z; } v3;
+
+static v3 mk(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 mul(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static v3 divs(v3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+static float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static float len2(v3 v) { return dot(v, v); }
+static v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static v3 normalize(v3 v) { /* generate.h:67-70 */
+    const float l2 = len2(v);
+    return l2 > 1.0e-20f ? divs(v, sqrtf(l2)) : mk(0.0f, 1.0f, 0.0f);
+}
+static v3 fallback_surface_u(v3 n) { /* generate.h:312-316 */
+    const v3 axis = n.z > -0.999f && n.z < 0.999f ? mk(0.0f, 0.0f, 1.0f) : mk(0.0f, 1.0f, 0.0f);
+    return normalize(cross(n, axis));
+}
+static float max_std(float a, float b) { return a < b ? b : a; }             /* std::max */
+static float clamp_std(float x, float lo, float hi) { return x < lo ? lo : (hi < x ? hi : x); } /* std::clamp */
+
+/* xgen_curve_eval over n points (src/xgen_classic_runtime.cpp:587-612) */
+static v3 curve_eval(const v3 *p, uint32_t n, float t) {
+    if (t < 1.0e-7f) return p[0];
+    if (t > 1.0f - 1.0e-7f) return p[n - 1u];
+    const uint32_t spans = n - 1u;
+    const float scaled = t * (float)spans;
+    const uint32_t span = (uint32_t)scaled;
+    const float f = scaled - (float)span;
+    const float f2 = f * f;
+    const float f3 = f2 * f;
+    const float omf = 1.0f - f;
+    const float b0 = omf * omf * omf;
+    const float b1 = 3.0f * f3 - 6.0f * f2 + 4.0f;
+    const float b2 = -3.0f * f3 + 3.0f * f2 + 3.0f * f + 1.0f;
+    const float b3 = f3;
+    const v3 p0 = span == 0u ? sub(mul(p[0], 2.0f), p[1]) : p[span - 1u];
+    const v3 p1 = p[span];
+    const v3 p2 = p[span + 1u];
+    const v3 p3 = span + 2u == n ? sub(mul(p[n - 1u], 2.0f), p[n - 2u]) : p[span + 2u];
+    return mul(add(add(add(mul(p0, b0), mul(p1, b1)), mul(p2, b2)), mul(p3, b3)), 1.0f / 6.0f);
+}
+
+/* curve_spline_length over Vec3 points (src/xgen_classic_runtime.cpp:665-690) */
+static float spline_length(const v3 *p, uint32_t n) {
+    const uint32_t intervals = n * 2u + 4u;
+    const float step = 1.0f / (float)intervals;
+    v3 previous = p[0];
+    float result = 0.0f;
+    for (uint32_t k = 1u; k < intervals; ++k) {
+        const v3 current = curve_eval(p, n, step * (float)k);
+        result += sqrtf(len2(sub(current, previous)));
+        previous = current;
+    }
+    result += sqrtf(len2(sub(p[n - 1u], previous)));
+    return result;
+}
+
+void orc_clump_prepare(const float *axes3, uint32_t guides, uint32_t cvs, float *render3, float *lengths) {
+    for (uint32_t g = 0; g < guides; ++g) {
+        const v3 *src = (const v3 *)(axes3 + (uint64_t)g * cvs * 3u);
+        v3 *dst = (v3 *)(render3 + (uint64_t)g * cvs * 3u);
+        dst[0] = src[0];
+        for (uint32_t cv = 1u; cv + 1u < cvs; ++cv) dst[cv] = curve_eval(src, cvs, (float)cv / (float)(cvs - 1u));
+        dst[cvs - 1u] = src[cvs - 1u];
+        lengths[g] = spline_length(src, cvs);
+    }
+}
+
+/* rotate_between (src/xgen_classic_runtime.cpp:1109-1122) */
+static v3 rotate_between(v3 value, v3 from, v3 to, float fraction) {
+    v3 axis = cross(from, to);
+    const float als = len2(axis);
+    if (!(als >= 1.0e-12f)) return value;
+    axis = mul(axis, 1.0f / sqrtf(als));
+    const float angle = acosf(clamp_std(dot(from, to), -1.0f, 1.0f)) * fraction;
+    const float c = cosf(angle);
+    const float s = sinf(angle);
+    return add(add(mul(value, c), mul(cross(axis, value), s)), mul(axis, dot(axis, value) * (1.0f - c)));
+}
+
+/* build_clump_noise_axis with constant noise expressions: out[0..n) */
+static void noise_axis(const v3 *axis, const v3 *distance_axis, uint32_t n, const orc_clump_params *p, v3 domain_position,
+                       v3 normal, v3 tangent, float mask, const nxg_vec3 *gradients, v3 *out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = mk(0.0f, 0.0f, 0.0f);
+    const float noise = max_std(p->noise, 0.0f);
+    if (!(noise > 1.0e-5f) || !(mask > 1.0e-4f)) return;
+    const float frequency = max_std(p->noise_frequency, 0.0f);
+    const float correlation = clamp_std(p->noise_correlation * 0.01f, 0.0f, 1.0f);
+    float polyline = 0.0f;
+    for (uint32_t cv = 1u; cv < n; ++cv) polyline += sqrtf(len2(sub(distance_axis[cv], distance_axis[cv - 1u])));
+    const float effective = polyline > 0.0f ? max_std(0.5f / polyline, frequency) : frequency;
+    const float decorrelation = 1.0f - correlation;
+    const float domain_scale = 100.0f * decorrelation * decorrelation;
+    const v3 domain = mul(add(domain_position, mk(0.419276f, 0.184247f, 0.805721f)), domain_scale);
+    normal = !(len2(normal) > 1.0e-20f) ? mk(0.0f, 1.0f, 0.0f) : normalize(normal);
+    tangent = !(len2(tangent) > 1.0e-20f) ? fallback_surface_u(normal) : normalize(tangent);
+    v3 tu = tangent, tv = cross(normal, tangent), current = normal;
+    const v3 first_segment = sub(axis[1], axis[0]);
+    if (len2(first_segment) > 1.0e-20f) current = normalize(first_segment);
+    if (dot(normal, current) < 0.99999f) {
+        tu = rotate_between(tu, normal, current, 1.0f);
+        tv = rotate_between(tv, normal, current, 1.0f);
+    }
+    float travelled = 0.0f;
+    for (uint32_t cv = 1u; cv < n; ++cv) {
+        travelled += sqrtf(len2(sub(distance_axis[cv], distance_axis[cv - 1u])));
+        v3 su = tu, sv = tv, next = current;
+        int advance = 0;
+        if (cv + 1u < n) {
+            const v3 segment = sub(axis[cv + 1u], axis[cv]);
+            if (len2(segment) > 1.0e-20f) {
+                next = normalize(segment);
+                const float td = dot(current, next);
+                if (fabsf(td) < 0.99999f) {
+                    su = rotate_between(tu, current, next, 0.5f);
+                    sv = rotate_between(tv, current, next, 0.5f);
+                    advance = 1;
+                } else {
+                    current = next;
+                }
+            }
+        }
+        const float scale = max_std(p->noise_scale, 0.0f);
+        const float magnitude = mask * noise * scale;
+        const float distance = travelled * effective;
+        const float first = orc_gradient_noise_f32(gradients, domain.x + distance, domain.y, domain.z) - 0.5f;
+        const float second = orc_gradient_noise_f32(gradients, domain.x, domain.y, domain.z + distance) - 0.5f;
+        out[cv] = mul(add(mul(su, first), mul(sv, second)), magnitude);
+        if (advance) {
+            tu = rotate_between(tu, current, next, 1.0f);
+            tv = rotate_between(tv, current, next, 1.0f);
+            current = next;
+        }
+    }
+}
+
+/* cut_to_length_and_rebuild: points in place, n <= 256 */
+static void cut_to_length(v3 *points, uint32_t n, float target) {
+    v3 source[256];
+    memcpy(source, points, sizeof(v3) * n);
+    if (target <= 1.0e-10f) {
+        for (uint32_t i = 0; i < n; ++i) points[i] = source[0];
+        return;
+    }
+    const uint32_t intervals = n * 2u + 4u;
+    const float step = 1.0f / (float)intervals;
+    float previous_parameter = 0.0f, accumulated = 0.0f, cut = 1.0f;
+    v3 previous = source[0];
+    for (uint32_t sample = 1u; sample <= intervals; ++sample) {
+        const float parameter = sample == intervals ? 1.0f : step * (float)sample;
+        const v3 current = sample == intervals ? source[n - 1u] : curve_eval(source, n, parameter);
+        const float segment = sqrtf(len2(sub(current, previous)));
+        accumulated += segment;
+        if (accumulated >= target && segment > 0.0f) {
+            cut = parameter - ((accumulated - target) / segment) * (parameter - previous_parameter);
+            break;
+        }
+        previous_parameter = parameter;
+        previous = current;
+    }
+    const float rebuild = cut / (float)(n - 1u);
+    points[0] = source[0];
+    for (uint32_t i = 1u; i < n; ++i) points[i] = curve_eval(source, n, rebuild * (float)i);
+}
+
+uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, const uint32_t *strand_guides,
+                         const float *axes3, const float *render3, const float *lengths, const float *normals3,
+                         const float *tangents3, const float *reference3, uint32_t guides,
+                         const nxg_vec3 *gradients, const orc_clump_params *p, const float *amount_multiplier,
+                         uint32_t multiplier_stride) {
+    v3 pts[256], noise[256];
+    if (cvs < 3u || cvs > 256u) return ~0u;
+    for (uint32_t s = 0; s < strand_count; ++s) {
+        const uint32_t g = strand_guides[s];
+        if (g == 0xffffffffu) continue;
+        if (g >= guides) return ~0u;
+        const float mask = clamp_std(p->mask, 0.0f, 1.0f);
+        if (mask <= 0.0f) continue;
+        float *io = points + (uint64_t)s * cvs * 4u;
+        const v3 *render = (const v3 *)(render3 + (uint64_t)g * cvs * 3u);
+        const v3 *source = (const v3 *)(axes3 + (uint64_t)g * cvs * 3u);
+        const v3 root = source[0];
+        const v3 domain_position = reference3 ? *(const v3 *)(reference3 + 3u * g) : root;
+        noise_axis(render, source, cvs, p, domain_position, *(const v3 *)(normals3 + 3u * g),
+                   *(const v3 *)(tangents3 + 3u * g), mask, gradients, noise);
+        for (uint32_t i = 0; i < cvs; ++i) pts[i] = mk(io[4 * i], io[4 * i + 1], io[4 * i + 2]);
+        cut_to_length(pts, cvs, lengths[g]);
+        float raw_amount = p->amount;
+        if (amount_multiplier) raw_amount = raw_amount * amount_multiplier[(uint64_t)s * multiplier_stride];
+        if (!isfinite(raw_amount)) return ~0u;
+        const float amount = clamp_std(raw_amount, 0.0f, 1.0f);
+        for (uint32_t cv = 1u; cv < cvs; ++cv) {
+            const v3 goal = sub(render[cv], mk(0.0f, 0.0f, 0.0f));
+            pts[cv] = add(pts[cv], mul(sub(goal, pts[cv]), mask * amount * (1.0f - 2.0f * p->clump_scale)));
+        }
+        for (uint32_t cv = 1u; cv < cvs; ++cv) pts[cv] = add(pts[cv], noise[cv]);
+        for (uint32_t i = 0; i < cvs; ++i) {
+            io[4 * i] = pts[i].x;
+            io[4 * i + 1] = pts[i].y;
+            io[4 * i + 2] = pts[i].z;
+        }
+    }
+    return 0u;
+}

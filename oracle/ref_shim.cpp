@@ -279,6 +279,72 @@ uint32_t ref_classic_endpoints(const float *points, uint32_t strand_count, uint3
     }
 }
 
+// apply_xgen_classic_float_runtime_plan_cpu with one ClumpingFX module whose
+// expressions are the constants k[0..6] (mask, clump, clumpScale, noise,
+// noiseScale, noiseFrequency, noiseCorrelation), bound to world-space guides
+// (ClassicClumpRuntimeData, xgen_classic_runtime.h:59-88; uvs, face ids and
+// random prefixes zero; reference positions optional).  Strands must survive
+// the runtime's length/width culling.  Returns the kept strand count, or ~0u
+// where the reference throws.
+uint32_t ref_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, const uint32_t *strand_guides,
+                         const float *axes3, const float *normals3, const float *tangents3,
+                         const float *reference3, uint32_t guides, const float *k) {
+    try {
+        PackedGeneratedCurves curves{};
+        curves.strand_count = strand_count;
+        curves.cvs_per_strand = cvs;
+        curves.point_counts.assign(strand_count, cvs);
+        curves.points.resize(std::size_t{strand_count} * cvs);
+        std::memcpy(curves.points.data(), points, curves.points.size() * sizeof(PackedCurvePoint));
+        curves.roots.resize(strand_count);
+        const auto constant = [](const char *attribute, float value) {
+            ClassicFloatRuntimeExpression e{};
+            e.object_type = "ClumpingFXModule";
+            e.attribute = attribute;
+            e.program.instructions = {{XgenScalarOp::constant, 0u, 0u, 0u, value}};
+            e.program.result = 0u;
+            return e;
+        };
+        ClassicFloatClumpModule clump{};
+        clump.name = "clump";
+        clump.mask = constant("mask", k[0]);
+        clump.clump = constant("clump", k[1]);
+        clump.clump_scale = constant("clumpScale", k[2]);
+        clump.noise = constant("noise", k[3]);
+        clump.noise_scale = constant("noiseScale", k[4]);
+        clump.noise_frequency = constant("noiseFrequency", k[5]);
+        clump.noise_correlation = constant("noiseCorrelation", k[6]);
+        ClassicFloatRuntimePlan plan{};
+        plan.description_name = "clump";
+        plan.fx_cv_count = cvs;
+        plan.clumps.push_back(clump);
+        plan.effects.push_back({ClassicFloatEffectType::Clump, 0u});
+        ClassicClumpRuntimeData data{};
+        data.module_name = "clump";
+        data.cvs_per_guide = cvs;
+        const std::size_t axis_count = std::size_t{guides} * cvs;
+        data.guide_axes.resize(axis_count);
+        std::memcpy(data.guide_axes.data(), axes3, axis_count * sizeof(Vec3));
+        data.guide_normals.resize(guides);
+        std::memcpy(data.guide_normals.data(), normals3, guides * sizeof(Vec3));
+        data.guide_tangents.resize(guides);
+        std::memcpy(data.guide_tangents.data(), tangents3, guides * sizeof(Vec3));
+        if (reference3) {
+            data.guide_reference_positions.resize(guides);
+            std::memcpy(data.guide_reference_positions.data(), reference3, guides * sizeof(Vec3));
+        }
+        data.guide_uvs.assign(guides, Vec2{});
+        data.guide_face_ids.assign(guides, 0u);
+        data.guide_random_prefixes.assign(guides, 0u);
+        data.strand_guide_indices.assign(strand_guides, strand_guides + strand_count);
+        apply_xgen_classic_float_runtime_plan_cpu(curves, plan, 1.0f, {}, {}, {}, {&data, 1u});
+        std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
+        return curves.strand_count;
+    } catch (const std::exception &) {
+        return ~0u;
+    }
+}
+
 uint32_t ref_runtime_hash_component(float x) { return xgen_runtime_hash_component(x); }
 uint32_t ref_seexpr_hash_finish(uint32_t s) { return xgen_seexpr_hash_finish(s); }
 float ref_random01(uint32_t seed, uint32_t strand, uint32_t dim) { return random01(seed, strand, dim); }

@@ -14,7 +14,10 @@ into oracle/_ref/libnxg_ref.so) and records, for small seeded inputs:
 The fixtures are data (inputs and expected outputs); the reference itself
 never leaves this container.
 
-    python tests/golden/make_golden.py [generation] [expressions] [classic]
+  * one constant-expression ClumpingFX module of the same runtime over
+    seeded clump guides and strands (reference_clump.npz).
+
+    python tests/golden/make_golden.py [generation] [expressions] [classic] [clump]
 """
 import os
 import sys
@@ -296,11 +299,77 @@ def main_classic():
           [int(len(fx[f"case{i}_out_point_counts"])) for i in range(len(CLASSIC_CASES))], "strands kept")
 
 
+CLUMP_PARAMS = dict(mask=0.8, amount=0.7, clump_scale=0.1, noise=0.3, noise_scale=1.0, noise_frequency=2.0,
+                    noise_correlation=20.0)
+CLUMP_CASES = [  # (cvs, parameter overrides, with reference positions)
+    (5, {}, False),
+    (5, dict(noise=0.0), False),
+    (16, {}, True),
+    (16, dict(amount=1.5, clump_scale=-0.2, noise_correlation=150.0), False),
+    (3, dict(mask=0.0), False),
+    (24, dict(noise_frequency=0.0, noise=2.0, noise_scale=0.5), False),
+]
+
+
+def clump_inputs(cvs, guides=16, strands=300, seed=61):
+    """Seeded clump guides (curled verticals over a plane) and strands with
+    width and length (the runtime culls neither), every 7th without a guide."""
+    rng = np.random.default_rng(seed + cvs)
+    groots = np.stack([rng.uniform(-1, 1, guides), np.zeros(guides), rng.uniform(-1, 1, guides)], 1)
+    t = np.linspace(0, 1, cvs, dtype=np.float32)
+    length = rng.uniform(0.5, 1.5, guides)
+    phase = rng.uniform(0, 6.28, guides)
+    axes = np.zeros((guides, cvs, 3), np.float32)
+    axes[:, :, 0] = groots[:, None, 0] + 0.1 * np.sin(3 * t[None] + phase[:, None])
+    axes[:, :, 1] = length[:, None] * t[None]
+    axes[:, :, 2] = groots[:, None, 2] + 0.1 * np.cos(2 * t[None] + phase[:, None])
+    normals = np.tile(np.float32([0, 1, 0]), (guides, 1))
+    tangents = np.tile(np.float32([1, 0, 0]), (guides, 1))
+    tangents[::5] = 0.0  # degenerate: the fallback tangent
+    sroots = np.stack([rng.uniform(-1, 1, strands), np.zeros(strands), rng.uniform(-1, 1, strands)], 1)
+    slen = rng.uniform(0.3, 1.8, strands)
+    pts = np.zeros((strands, cvs, 4), np.float32)
+    pts[:, :, 0] = sroots[:, None, 0] + 0.05 * rng.normal(size=(strands, cvs)).cumsum(1)
+    pts[:, :, 1] = slen[:, None] * t[None]
+    pts[:, :, 2] = sroots[:, None, 2] + 0.05 * rng.normal(size=(strands, cvs)).cumsum(1)
+    pts[:, :, 3] = rng.uniform(0.005, 0.02, size=(strands, 1))
+    d2 = ((sroots[:, None, [0, 2]] - groots[None, :, [0, 2]]) ** 2).sum(-1)
+    sg = d2.argmin(1).astype(np.uint32)
+    sg[::7] = 0xFFFFFFFF
+    reference = (axes[:, 0] + np.float32([0.3, 0.0, -0.2])).astype(np.float32)
+    return dict(axes=axes.reshape(-1, 3), normals=normals, tangents=tangents, reference_positions=reference), \
+        pts.reshape(-1, 4), sg
+
+
+def main_clump():
+    oracle.build()
+    if oracle.ref() is None:
+        raise SystemExit("reference core not built (needs /root/reference)")
+    fx = {}
+    for i, (cvs, over, with_ref) in enumerate(CLUMP_CASES):
+        guides, pts, sg = clump_inputs(cvs)
+        if not with_ref:
+            guides.pop("reference_positions")
+        params = dict(CLUMP_PARAMS, **over)
+        out = oracle.ref_clump_apply(pts, cvs, sg, guides, params)
+        for k, v in guides.items():
+            fx[f"case{i}_{k}"] = v
+        fx[f"case{i}_points"] = pts
+        fx[f"case{i}_strand_guides"] = sg
+        fx[f"case{i}_params"] = np.array([params[k] for k in oracle.CLUMP_KEYS], np.float32)
+        fx[f"case{i}_out"] = out
+    path = os.path.join(HERE, "reference_clump.npz")
+    np.savez_compressed(path, **fx)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["generation", "expressions", "classic"]
+    which = sys.argv[1:] or ["generation", "expressions", "classic", "clump"]
     if "generation" in which:
         main()
     if "expressions" in which:
         main_expressions()
     if "classic" in which:
         main_classic()
+    if "clump" in which:
+        main_clump()
