@@ -387,6 +387,128 @@ def bench_classic(args):
     return 0
 
 
+CLUMP_PARAMS = dict(mask=0.8, amount=0.7, clump_scale=0.1, noise=0.3, noise_scale=1.0, noise_frequency=2.0,
+                    noise_correlation=20.0)
+
+
+def bench_clump(args):
+    """ClumpingFX (nxg_classic_clump_apply) over the plane workload's
+    generated curves: 2,000,000 strands x 16 CVs, each clumped to the nearest
+    guide of its stencil (slot 0) among the description's 4,096 guides of 16
+    CVs, constant module expressions with clump noise on.  One step = the
+    guide pass (render axes, lengths, noise axes) + the strand pass."""
+    import numpy as np
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import abi, workloads
+    from nanoxgen_amd.classic import clump, clump_workspace_bytes
+
+    w = workloads.config2()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    blob = nx.build_asset(w.build)
+    asset = nx.DeviceAsset(blob)
+    gen = nx.PackedOutputs(N, Cv, stencil=True)
+    nx.generate_packed(asset, w.params, gen, octaves=w.octaves)
+    h = abi.header_of(blob)
+    recs = blob[h.guides_offset:h.guides_offset + 56 * h.guide_count].view(abi.GUIDE_DTYPE)
+    assert np.all(recs["cv_count"] == Cv) and np.array_equal(recs["first_cv"], np.arange(h.guide_count) * Cv)
+    cvs3 = blob[h.guide_cvs_offset:h.guide_cvs_offset + 12 * h.guide_cv_count].view(np.float32).reshape(-1, 3)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+    guides = dict(axes=d(cvs3), normals=d(recs["root_normal"]), tangents=d(np.zeros((h.guide_count, 3))))
+    strand_guides = gen.stencil_indices[:, 0].contiguous()
+    out = torch.empty_like(gen.points)
+    ws = torch.empty(clump_workspace_bytes(h.guide_count, Cv), dtype=torch.uint8, device="cuda")
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    log(f"generated {N} x {Cv} curves; {h.guide_count} clump guides")
+
+    def step():
+        clump(gen.points, Cv, strand_guides, guides, CLUMP_PARAMS, out=out, status=status, workspace=ws,
+              stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        events[i][0].record(stream)
+        step()
+        events[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    call_ms = sorted(e[0].elapsed_time(e[1]) for e in events)
+    call_mean = sum(call_ms) / len(call_ms)
+    assert int(status.item()) == 0
+    # algorithmic bytes: every strand read and written, its guide index read,
+    # the guides' CVs, normals and tangents read once
+    alg = N * Cv * 32 + N * 4 + h.guide_count * (Cv * 12 + 24)
+    achieved = alg / (call_mean * 1e-3) / 1e9
+    traffic = valu = None
+    pmc_path = os.path.join(ROOT, "profiles", "clump_strand_kernel_plane2M_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            pmc = json.load(f)
+        traffic = pmc.get("hbm_bytes_per_call")
+        insts = pmc.get("valu_instructions_per_call")
+        if insts:
+            rate = insts / (call_mean * 1e-3)
+            valu = {"achieved": round(rate / 1e9, 2), "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                    "frac": round(rate / 1e9 / VALU_PEAK_GINST, 4), "instructions_per_call": insts}
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "clump call: cut_table_kernel + clump_guide_kernel + clump_strand_kernel<16> "
+                          "(one C-ABI call)",
+                "kernel_ms_mean": round(call_mean, 4), "kernel_ms_median": round(call_ms[len(call_ms) // 2], 4),
+                "algorithmic_bytes_per_launch": alg, "valu": valu}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = clump_cpu_baseline(gen, N, Cv, cvs3, recs, args.cpu_seconds)
+    line = {"metric": METRIC, "value": N * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "classic-clump-plane2M", "input": w.name, "strands": N, "cvs_per_strand": Cv,
+                       "clump_guides": int(h.guide_count), "params": CLUMP_PARAMS, "parallelism": "single-gpu"},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def clump_cpu_baseline(gen, N, Cv, cvs3, recs, target_s):
+    """The reference's own float runtime with the same ClumpingFX module
+    (oracle/_ref, its own thread pool) on the first S generated strands; the
+    oracle's single-threaded restatement without that build."""
+    import numpy as np
+    import oracle
+    host = gen.to_host()
+    pts = host["points"]
+    sg = host["stencil_indices"][:, 0].copy()
+    guides = dict(axes=cvs3, normals=np.ascontiguousarray(recs["root_normal"], np.float32),
+                  tangents=np.zeros((len(recs), 3), np.float32))
+    use_ref = oracle.ref() is not None
+    cores = len(os.sched_getaffinity(0)) if use_ref else 1
+
+    def run(s):
+        t = time.perf_counter()
+        if use_ref:
+            oracle.ref_clump_apply(pts[:s * Cv], Cv, sg[:s], guides, CLUMP_PARAMS)
+        else:
+            raise RuntimeError("no CPU baseline without the reference build")
+        return time.perf_counter() - t
+
+    s = min(N, 65536)
+    dt = run(s)
+    s = int(min(N, max(s, s * target_s / max(dt, 1e-6))))
+    times = [run(s)]
+    while s == N and sum(times) < target_s and len(times) < 5:
+        times.append(run(s))
+    dt = sorted(times)[len(times) // 2]
+    return {"value": s * Cv / dt, "unit": "points/s", "cores": cores, "kind": "reference",
+            "sample": f"strands 0..{s - 1} of the generated curves ({s * Cv} points), one ClumpingFX module, "
+                      f"median {dt:.3f} s of {len(times)} run(s) including the shim's copies in and out; "
+                      "the reference's apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread pool)"}
+
+
 def classic_cpu_baseline(gen, N, Cv, scale, offset, target_s):
     """The reference's own apply_xgen_classic_float_runtime_plan_cpu (built
     from its sources into oracle/_ref) on the first S generated strands, with
@@ -428,13 +550,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch", "classic"],
+    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch", "classic", "clump"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
                          "the trained surrogate + sampled exact check against the exact runtime; "
                          "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph; "
-                         "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands")
+                         "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands; "
+                         "clump: ClumpingFX toward the plane's 4,096 guides over the same strands")
     ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
     ap.add_argument("--sample-stride", type=int, default=16, help="surrogate: exact check of every n-th root")
     ap.add_argument("--train-epochs", type=int, default=40)
@@ -459,6 +582,8 @@ def main():
         return bench_batch(args)
     if args.workload == "classic" and args.gpus == 1:
         return bench_classic(args)
+    if args.workload == "clump" and args.gpus == 1:
+        return bench_clump(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(strands are independent: a multi-GPU run shards strand ranges, one process per GPU)", file=sys.stderr)
