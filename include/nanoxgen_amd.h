@@ -809,6 +809,85 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
                                  uint64_t out_point_capacity, uint32_t *status, void *workspace,
                                  uint64_t workspace_bytes, void *stream);
 
+/* ---- .nxc curve cache (include/nanoxgen/curve_cache.h) ---------------- */
+
+/* A pointer-free, directly uploadable blob of finished curves: the 128-byte
+ * CurveCacheHeader (curve_cache.h:29-47), then the sections in this order,
+ * each 16-byte aligned: point counts, packed points, motion times, motion
+ * points (sample-major), texcoords, patch uvs, face uvs, face ids; absent
+ * sections have offset 0.  content_hash is the word hash of every byte after
+ * the header (src/curve_cache.cpp:35-53). */
+#define NXG_CURVE_CACHE_MAGIC 0x454843414347584eull /* "NXGCACHE" */
+#define NXG_CURVE_CACHE_VERSION_MAJOR 1u
+#define NXG_CURVE_CACHE_HAS_TEXCOORDS 1u
+#define NXG_CURVE_CACHE_HAS_PATCH_UVS 2u
+#define NXG_CURVE_CACHE_HAS_FACE_UVS 4u
+#define NXG_CURVE_CACHE_HAS_FACE_IDS 8u
+#define NXG_CURVE_CACHE_HAS_MOTION 16u
+
+typedef struct nxg_curve_cache_header {
+    uint64_t magic;
+    uint16_t version_major;
+    uint16_t version_minor;
+    uint32_t flags;
+    uint64_t byte_size;
+    uint64_t content_hash;
+    uint32_t strand_count;
+    uint32_t point_count;
+    uint32_t motion_sample_count;
+    uint32_t reserved;
+    uint64_t point_counts_offset;
+    uint64_t points_offset;
+    uint64_t motion_times_offset;
+    uint64_t motion_points_offset;
+    uint64_t texcoords_offset;
+    uint64_t patch_uvs_offset;
+    uint64_t face_uvs_offset;
+    uint64_t face_ids_offset;
+    uint8_t padding[16]; /* the reference header is 64-byte aligned: 128 bytes */
+} nxg_curve_cache_header;
+
+/* CurveCacheBuildInput (curve_cache.h:56-67).  Optional arrays may be NULL
+ * (counts then ignored); motion_times has motion_sample_count entries and
+ * motion_points motion_sample_count * point_count, sample-major. */
+typedef struct nxg_curve_cache_input {
+    const uint32_t *point_counts;
+    const nxg_packed_point *points;
+    const nxg_vec2 *texcoords;    /* point_count */
+    const nxg_vec2 *patch_uvs;    /* strand_count */
+    const nxg_vec2 *face_uvs;     /* strand_count */
+    const uint32_t *face_ids;     /* strand_count */
+    const float *motion_times;    /* always host memory */
+    const nxg_vec3 *motion_points;
+    uint32_t strand_count;
+    uint32_t point_count;
+    uint32_t motion_sample_count;
+    uint32_t reserved;
+} nxg_curve_cache_input;
+
+/* The header of the cache the input would make (sizes, flags, offsets; no
+ * hash), from counts alone: NXG_OK or NXG_ERROR_INVALID_BUILD_INPUT. */
+uint32_t nxg_curve_cache_layout(const nxg_curve_cache_input *input, nxg_curve_cache_header *header);
+
+/* build_curve_cache (src/curve_cache.cpp:104-164) from host arrays into a
+ * 64-byte aligned caller buffer; *byte_size always receives the size.  The
+ * reference's message goes to err on NXG_ERROR_INVALID_BUILD_INPUT. */
+uint32_t nxg_curve_cache_build(const nxg_curve_cache_input *input, void *out, uint64_t capacity, uint64_t *byte_size,
+                               char *err, uint64_t err_capacity);
+
+/* validate_curve_cache (src/curve_cache.cpp:166-239): "" or the reference's
+ * message; NXG_OK or NXG_ERROR_INVALID_ASSET. */
+uint32_t nxg_curve_cache_validate(const void *blob, uint64_t byte_size, char *msg, uint64_t msg_capacity);
+
+/* The same cache from device arrays (points, counts, uvs, ids and motion
+ * points on the device; motion_times on the host): one kernel lays every
+ * section out in device_scratch (>= the layout's byte_size, 64-byte aligned),
+ * one download fills `out`, and the host adds the content hash and runs the
+ * validator, as build_curve_cache does.  Synchronises `stream`. */
+uint32_t nxg_curve_cache_build_gpu(const nxg_curve_cache_input *input, void *device_scratch,
+                                   uint64_t scratch_capacity, void *out, uint64_t capacity, uint64_t *byte_size,
+                                   char *err, uint64_t err_capacity, void *stream);
+
 /* Library build identity: gfx target and compile flags that affect parity. */
 const char *nxg_build_info(void);
 

@@ -274,6 +274,27 @@ class ClassicOutput(C.Structure):
                 ("status", C.c_void_p)]
 
 
+class CurveCacheHeader(C.Structure):
+    """nxg_curve_cache_header (CurveCacheHeader, curve_cache.h:29-47): 128 bytes."""
+    _fields_ = [("magic", C.c_uint64), ("version_major", C.c_uint16), ("version_minor", C.c_uint16),
+                ("flags", C.c_uint32), ("byte_size", C.c_uint64), ("content_hash", C.c_uint64),
+                ("strand_count", C.c_uint32), ("point_count", C.c_uint32), ("motion_sample_count", C.c_uint32),
+                ("reserved", C.c_uint32), ("point_counts_offset", C.c_uint64), ("points_offset", C.c_uint64),
+                ("motion_times_offset", C.c_uint64), ("motion_points_offset", C.c_uint64),
+                ("texcoords_offset", C.c_uint64), ("patch_uvs_offset", C.c_uint64),
+                ("face_uvs_offset", C.c_uint64), ("face_ids_offset", C.c_uint64), ("padding", C.c_uint8 * 16)]
+
+
+class CurveCacheInput(C.Structure):
+    """nxg_curve_cache_input (CurveCacheBuildInput, curve_cache.h:56-67)."""
+    _fields_ = [("point_counts", C.c_void_p), ("points", C.c_void_p), ("texcoords", C.c_void_p),
+                ("patch_uvs", C.c_void_p), ("face_uvs", C.c_void_p), ("face_ids", C.c_void_p),
+                ("motion_times", C.c_void_p), ("motion_points", C.c_void_p), ("strand_count", C.c_uint32),
+                ("point_count", C.c_uint32), ("motion_sample_count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+assert C.sizeof(CurveCacheHeader) == 128
+
 CLUMP_MAX_CVS = 64
 CLASSIC_INVALID_GUIDE = 4
 INVALID_INDEX = 0xFFFFFFFF

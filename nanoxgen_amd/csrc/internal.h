@@ -84,6 +84,14 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
     return l;
 }
 
+// Curve cache (curve_cache.cpp): layout from counts, the builder's data
+// checks, the content hash, and the finishing step (hash, header, validation).
+uint32_t curve_cache_layout(const nxg_curve_cache_input &in, nxg_curve_cache_header &h, char *err, uint64_t cap);
+uint32_t curve_cache_check_data(const nxg_curve_cache_header &h, const uint32_t *counts, bool motion_present,
+                                const float *motion_times, char *err, uint64_t cap);
+uint64_t curve_cache_hash(const uint8_t *bytes, uint64_t size);
+uint32_t curve_cache_finish(uint8_t *blob, nxg_curve_cache_header h, char *err, uint64_t cap);
+
 // Asset builder (asset_builder.cpp).  build_blob with host_stencil = false
 // leaves the triangle stencil zeroed and the content hash unset; finish_blob
 // sets the hash once the stencil is in place.

@@ -119,6 +119,11 @@ def ref():
         L.ref_clump_apply.restype = C.c_uint32
         L.ref_clump_apply.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.ref_build_curve_cache.restype = C.c_int
+        L.ref_build_curve_cache.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                            C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_char_p, C.c_uint64]
+        L.ref_validate_curve_cache.argtypes = [C.c_void_p, C.c_uint64, C.c_char_p, C.c_uint64]
         L.ref_classic_endpoints.restype = C.c_uint32
         L.ref_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
@@ -428,3 +433,30 @@ def ref_clump_apply(points, cvs, strand_guides, guides, params):
         return None
     assert kept == n, "strands were culled by the runtime; pick inputs with length and width"
     return pts
+
+
+def ref_build_curve_cache(counts, points, *, texcoords=None, patch_uvs=None, face_uvs=None, face_ids=None,
+                          motion_times=None, motion_points=None):
+    """The reference's build_curve_cache: (bytes as uint8, "") or (None, message)."""
+    R = ref()
+    a = lambda x, dt: None if x is None else np.ascontiguousarray(x, dt)
+    counts, points = a(counts, np.uint32), a(points, np.float32).reshape(-1, 4)
+    tex, pu, fu, fid = a(texcoords, np.float32), a(patch_uvs, np.float32), a(face_uvs, np.float32), a(face_ids, np.uint32)
+    mt, mp = a(motion_times, np.float32), a(motion_points, np.float32)
+    out, size = C.c_void_p(), C.c_uint64()
+    err = C.create_string_buffer(512)
+    rc = R.ref_build_curve_cache(counts.ctypes.data, len(counts), points.ctypes.data, len(points), abi.ptr(tex),
+                                 abi.ptr(pu), abi.ptr(fu), abi.ptr(fid), abi.ptr(mt), 0 if mt is None else len(mt),
+                                 abi.ptr(mp), C.byref(out), C.byref(size), err, 512)
+    if rc:
+        return None, err.value.decode()
+    data = np.frombuffer(C.string_at(out.value, size.value), np.uint8).copy()
+    R.ref_free(out)
+    return data, ""
+
+
+def ref_validate_curve_cache(blob):
+    """The reference's validate_curve_cache on a 64-byte aligned copy."""
+    msg = C.create_string_buffer(512)
+    ref().ref_validate_curve_cache(blob.ctypes.data, blob.nbytes, msg, 512)
+    return msg.value.decode()

@@ -3,6 +3,7 @@
 // target `ref`).  Used only in this container to pin the C restatement and to
 // write golden fixtures; nothing here ships or runs on the GPU.
 #include "nanoxgen/asset.h"
+#include "nanoxgen/curve_cache.h"
 #include "nanoxgen/generate.h"
 #include "nanoxgen/xgen_classic_runtime.h"
 #include "nanoxgen/xgen_expression.h"
@@ -343,6 +344,41 @@ uint32_t ref_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, con
     } catch (const std::exception &) {
         return ~0u;
     }
+}
+
+// build_curve_cache (curve_cache.cpp:104-164) from flat host arrays (any
+// optional one NULL); the bytes go to *out (free with ref_free).  Returns
+// 0, or 1 with the message in err.
+int ref_build_curve_cache(const uint32_t *counts, uint32_t strands, const float *points, uint32_t point_count,
+                          const float *texcoords, const float *patch_uvs, const float *face_uvs,
+                          const uint32_t *face_ids, const float *motion_times, uint32_t samples,
+                          const float *motion_points, void **out, uint64_t *size, char *err, uint64_t cap) {
+    try {
+        CurveCacheBuildInput in{};
+        in.point_counts = {counts, strands};
+        in.points = {reinterpret_cast<const PackedCurvePoint *>(points), point_count};
+        if (texcoords) in.texcoords = {reinterpret_cast<const Vec2 *>(texcoords), point_count};
+        if (patch_uvs) in.patch_uvs = {reinterpret_cast<const Vec2 *>(patch_uvs), strands};
+        if (face_uvs) in.face_uvs = {reinterpret_cast<const Vec2 *>(face_uvs), strands};
+        if (face_ids) in.face_ids = {face_ids, strands};
+        if (motion_times) in.motion_times = {motion_times, samples};
+        if (motion_points) in.motion_points = {reinterpret_cast<const Vec3 *>(motion_points),
+                                               std::size_t{samples} * point_count};
+        const CurveCache cache = build_curve_cache(in);
+        *size = cache.bytes().size();
+        *out = std::malloc(*size);
+        std::memcpy(*out, cache.bytes().data(), *size);
+        return 0;
+    } catch (const std::exception &e) {
+        std::snprintf(err, cap, "%s", e.what());
+        return 1;
+    }
+}
+
+// validate_curve_cache (curve_cache.cpp:166-239) into msg ("" when valid).
+void ref_validate_curve_cache(const void *bytes, uint64_t size, char *msg, uint64_t cap) {
+    const std::string e = validate_curve_cache({static_cast<const std::byte *>(bytes), size});
+    std::snprintf(msg, cap, "%s", e.c_str());
 }
 
 uint32_t ref_runtime_hash_component(float x) { return xgen_runtime_hash_component(x); }
