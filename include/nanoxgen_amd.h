@@ -809,6 +809,72 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
                                  uint64_t out_point_capacity, uint32_t *status, void *workspace,
                                  uint64_t workspace_bytes, void *stream);
 
+/* ---- renderer curve payload (include/nanoxgen/curve_payload.h) -------- */
+
+/* build_curve_batches (src/curve_payload.cpp:150-301) on the GPU: curves
+ * split into batches of at most max_strands_per_batch strands, each point
+ * transformed by the row-major affine transform (skipped when it is the
+ * identity), widths turned into radii (0.5 * width * radius_scale), curves
+ * optionally resampled to resample_cv_count CVs in normalised index space
+ * (linear between CVs), motion samples transformed (and resampled) alike.
+ * The batches are written back to back, so batch b is the strands
+ * [b * max, min((b + 1) * max, N)) and its points start at
+ * batch_points[b] (batch_points[batch_count] = the total).  Motion points
+ * are batch-major and, within a batch, sample-major, as CurveBatch holds them
+ * (curve_payload.h:62-72).  Uniform primvars need no copy: batch b's slice
+ * of a per-strand array starts at strand b * max. */
+#define NXG_PAYLOAD_MAX_MOTION_SAMPLES 16u
+#define NXG_PAYLOAD_INVALID_COUNTS 8u  /* a curve with < 2 CVs, or counts not summing to point_count */
+
+typedef struct nxg_curve_set {   /* CurveSetView (curve_payload.h:27-38), device arrays */
+    const uint32_t *point_counts; /* optional: every curve then has fixed_cvs_per_strand CVs */
+    const nxg_vec3 *points;       /* point_count */
+    const float *widths;          /* optional, per point (full diameters); else constant_width */
+    const nxg_root_sample *roots; /* optional (root uvs when root_uvs is NULL) */
+    const nxg_vec2 *root_uvs;     /* optional */
+    uint64_t point_count;         /* < 2^32 */
+    uint32_t strand_count;
+    uint32_t fixed_cvs_per_strand;
+    float constant_width;
+    float time;                   /* motion samples: the sample time */
+} nxg_curve_set;
+
+typedef struct nxg_curve_payload_options {  /* CurvePayloadOptions (curve_payload.h:78-85) */
+    uint32_t max_strands_per_batch;         /* 64K in the reference's defaults */
+    uint32_t resample_cv_count;             /* 0: keep each curve's CVs; else >= 2 */
+    float radius_scale;
+    float transform[12];                    /* row-major 3x4 */
+} nxg_curve_payload_options;
+
+typedef struct nxg_curve_payload_output {
+    nxg_packed_point *points;      /* all batches' points */
+    uint32_t *point_counts;        /* one per strand */
+    nxg_vec2 *root_uvs;            /* one per strand, when the base has uvs or roots; else may be NULL */
+    nxg_vec3 *motion_points;       /* motion_sample_count x all points */
+    uint64_t *batch_points;        /* batch_count + 1 prefix offsets (device) */
+    uint32_t *status;              /* optional device word: 0 or NXG_PAYLOAD_INVALID_COUNTS */
+    uint64_t point_capacity;
+    uint64_t point_count_capacity;
+    uint64_t root_uv_capacity;
+    uint64_t motion_point_capacity;
+    uint64_t batch_capacity;
+} nxg_curve_payload_output;
+
+/* Batch count and total output points the request makes; host-side checks of
+ * src/curve_payload.cpp:150-196 (NXG_ERROR_INVALID_BUILD_INPUT otherwise;
+ * the message in err).  With variable counts and no resampling the total is
+ * base->point_count. */
+uint32_t nxg_curve_payload_plan(const nxg_curve_set *base, const nxg_curve_set *motion, uint32_t motion_count,
+                                const nxg_curve_payload_options *options, uint32_t *batch_count,
+                                uint64_t *output_points, char *err, uint64_t err_capacity);
+
+uint64_t nxg_curve_payload_workspace_size(uint32_t strand_count, uint32_t motion_count);
+
+/* On `stream`, without synchronising. */
+uint32_t nxg_curve_payload_build(const nxg_curve_set *base, const nxg_curve_set *motion, uint32_t motion_count,
+                                 const nxg_curve_payload_options *options, const nxg_curve_payload_output *output,
+                                 void *workspace, uint64_t workspace_bytes, void *stream);
+
 /* ---- .nxc curve cache (include/nanoxgen/curve_cache.h) ---------------- */
 
 /* A pointer-free, directly uploadable blob of finished curves: the 128-byte

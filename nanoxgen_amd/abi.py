@@ -274,6 +274,31 @@ class ClassicOutput(C.Structure):
                 ("status", C.c_void_p)]
 
 
+PAYLOAD_INVALID_COUNTS = 8
+PAYLOAD_MAX_MOTION_SAMPLES = 16
+
+
+class CurveSet(C.Structure):
+    """nxg_curve_set (CurveSetView, curve_payload.h:27-38): device arrays."""
+    _fields_ = [("point_counts", C.c_void_p), ("points", C.c_void_p), ("widths", C.c_void_p), ("roots", C.c_void_p),
+                ("root_uvs", C.c_void_p), ("point_count", C.c_uint64), ("strand_count", C.c_uint32),
+                ("fixed_cvs_per_strand", C.c_uint32), ("constant_width", C.c_float), ("time", C.c_float)]
+
+
+class CurvePayloadOptions(C.Structure):
+    """nxg_curve_payload_options (CurvePayloadOptions, curve_payload.h:78-85)."""
+    _fields_ = [("max_strands_per_batch", C.c_uint32), ("resample_cv_count", C.c_uint32),
+                ("radius_scale", C.c_float), ("transform", C.c_float * 12)]
+
+
+class CurvePayloadOutput(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("point_counts", C.c_void_p), ("root_uvs", C.c_void_p),
+                ("motion_points", C.c_void_p), ("batch_points", C.c_void_p), ("status", C.c_void_p),
+                ("point_capacity", C.c_uint64), ("point_count_capacity", C.c_uint64),
+                ("root_uv_capacity", C.c_uint64), ("motion_point_capacity", C.c_uint64),
+                ("batch_capacity", C.c_uint64)]
+
+
 class CurveCacheHeader(C.Structure):
     """nxg_curve_cache_header (CurveCacheHeader, curve_cache.h:29-47): 128 bytes."""
     _fields_ = [("magic", C.c_uint64), ("version_major", C.c_uint16), ("version_minor", C.c_uint16),

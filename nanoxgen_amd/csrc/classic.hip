@@ -46,6 +46,7 @@
 #include <cstdint>
 
 #include "../../include/nanoxgen_amd.h"
+#include "block_scan.h"
 #include "internal.h"
 
 namespace nxg {
@@ -55,7 +56,6 @@ constexpr uint32_t kWave = kClassicTile;  // strands per tile = lanes per wavefr
 constexpr uint32_t kStagedCvs = 64u;      // 64 strands x 64 CVs x 16 B = 64 KB of LDS
 constexpr uint32_t kErrShift = 30u;
 constexpr uint32_t kCountMask = (1u << kErrShift) - 1u;
-constexpr uint32_t kScanThreads = 256u;
 
 struct F3 {
     float x, y, z;
@@ -314,36 +314,6 @@ __global__ void __launch_bounds__(kWave) cut_search_kernel(CutArgs a) {
     const uint32_t bits = (__ballot(err & NXG_CLASSIC_NONFINITE_AMOUNT) ? NXG_CLASSIC_NONFINITE_AMOUNT : 0u) |
                           (__ballot(err & NXG_CLASSIC_NONFINITE_WIDTH) ? NXG_CLASSIC_NONFINITE_WIDTH : 0u);
     if (lane == 0u) a.tiles[t] = static_cast<uint32_t>(__popcll(kept)) | (bits << kErrShift);
-}
-
-// Exclusive scan of one value per thread of a 256-thread block, and the
-// block's total and OR of `flags`.
-__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t flags, uint32_t &total, uint32_t &all_flags) {
-    __shared__ uint32_t sums[kScanThreads / 64u], ors[kScanThreads / 64u];
-    const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
-    uint32_t inc = v, fl = flags;
-#pragma unroll
-    for (uint32_t d = 1u; d < 64u; d <<= 1) {
-        const uint32_t y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-        fl |= __shfl_xor(fl, d, 64);
-    }
-    if (lane == 63u) {
-        sums[wave] = inc;
-        ors[wave] = fl;
-    }
-    __syncthreads();
-    uint32_t before = 0u;
-    total = 0u;
-    all_flags = 0u;
-#pragma unroll
-    for (uint32_t w = 0u; w < kScanThreads / 64u; ++w) {
-        if (w < wave) before += sums[w];
-        total += sums[w];
-        all_flags |= ors[w];
-    }
-    __syncthreads();
-    return before + inc - v;
 }
 
 __global__ void __launch_bounds__(kScanThreads) cut_scan_tiles_kernel(CutArgs a) {

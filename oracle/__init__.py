@@ -124,6 +124,11 @@ def ref():
                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                             C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.c_char_p, C.c_uint64]
         L.ref_validate_curve_cache.argtypes = [C.c_void_p, C.c_uint64, C.c_char_p, C.c_uint64]
+        L.ref_curve_payload.restype = C.c_uint32
+        L.ref_curve_payload.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                        C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_char_p, C.c_uint64]
         L.ref_classic_endpoints.restype = C.c_uint32
         L.ref_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
@@ -460,3 +465,53 @@ def ref_validate_curve_cache(blob):
     msg = C.create_string_buffer(512)
     ref().ref_validate_curve_cache(blob.ctypes.data, blob.nbytes, msg, 512)
     return msg.value.decode()
+
+
+IDENTITY = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+
+
+def ref_curve_payload(base, motion=(), *, max_strands=65536, resample=0, radius_scale=1.0, transform=IDENTITY):
+    """The reference's build_curve_batches.  base / each motion sample: dict(points [P, 3], counts=None|[N],
+    fixed=0, widths=None|[P], constant_width=0.0[, time][, root_uvs]).  Returns dict(points, point_counts,
+    root_uvs, motion_points, batch_first) with the batches concatenated, or (None, message)."""
+    sets = [base] + list(motion)
+    count_of = lambda s: len(s["counts"]) if s.get("counts") is not None else len(s["points"]) // s["fixed"]
+    n = count_of(base)
+    strand_counts = np.array([count_of(s) for s in sets], np.uint32)
+    keep = []
+
+    def arr(x, dt):
+        if x is None:
+            return None
+        a = np.ascontiguousarray(x, dt)
+        keep.append(a)
+        return a
+
+    counts = [arr(s.get("counts"), np.uint32) for s in sets]
+    pts = [arr(s["points"], np.float32) for s in sets]
+    wid = [arr(s.get("widths"), np.float32) for s in sets]
+    P = lambda a: C.c_void_p(0) if a is None else C.c_void_p(a.ctypes.data)
+    cp = (C.c_void_p * len(sets))(*[P(c) for c in counts])
+    pp = (C.c_void_p * len(sets))(*[P(x) for x in pts])
+    wp = (C.c_void_p * len(sets))(*[P(w) for w in wid])
+    fixed = np.array([s.get("fixed", 0) for s in sets], np.uint32)
+    npts = np.array([len(x) // 3 if x.ndim == 1 else len(x) for x in pts], np.uint64)
+    cw = np.array([s.get("constant_width", 0.0) for s in sets], np.float32)
+    times = np.array([s.get("time", 0.0) for s in motion] or [0.0], np.float32)
+    uvs = arr(base.get("root_uvs"), np.float32)
+    total = n * resample if resample else int(npts[0])
+    out_p = np.zeros((max(total, 1), 4), np.float32)
+    out_c = np.zeros(max(n, 1), np.uint32)
+    out_u = np.zeros((max(n, 1), 2), np.float32)
+    out_m = np.zeros((max(total * len(motion), 1), 3), np.float32)
+    first = np.zeros(n + 1, np.uint32)
+    err = C.create_string_buffer(512)
+    tr = np.array(transform, np.float32)
+    b = ref().ref_curve_payload(strand_counts.ctypes.data, len(sets), cp, fixed.ctypes.data, pp, npts.ctypes.data, wp, cw.ctypes.data,
+                                times.ctypes.data, abi.ptr(uvs), max_strands, resample, radius_scale, tr.ctypes.data,
+                                out_p.ctypes.data, out_c.ctypes.data, out_u.ctypes.data, out_m.ctypes.data,
+                                first.ctypes.data, err, 512)
+    if b == 0xFFFFFFFF:
+        return None, err.value.decode()
+    return dict(points=out_p[:total], point_counts=out_c[:n], root_uvs=out_u[:n] if uvs is not None else None,
+                motion_points=out_m[:total * len(motion)], batch_first=first[:b]), ""

@@ -4,6 +4,7 @@
 // write golden fixtures; nothing here ships or runs on the GPU.
 #include "nanoxgen/asset.h"
 #include "nanoxgen/curve_cache.h"
+#include "nanoxgen/curve_payload.h"
 #include "nanoxgen/generate.h"
 #include "nanoxgen/xgen_classic_runtime.h"
 #include "nanoxgen/xgen_expression.h"
@@ -379,6 +380,67 @@ int ref_build_curve_cache(const uint32_t *counts, uint32_t strands, const float 
 void ref_validate_curve_cache(const void *bytes, uint64_t size, char *msg, uint64_t cap) {
     const std::string e = validate_curve_cache({static_cast<const std::byte *>(bytes), size});
     std::snprintf(msg, cap, "%s", e.c_str());
+}
+
+// build_curve_batches (curve_payload.cpp:150-301).  Curve set k (0: base,
+// 1..S: motion samples) has strand_counts[k] curves, counts[k] (NULL:
+// fixed[k] CVs each), points[k]
+// (npoints[k] Vec3), widths[k] (optional), constant_width[k]; the base may
+// have root_uvs.  The batches are concatenated into out_points (packed),
+// out_counts, out_uvs and out_motion (each batch's motion block in turn);
+// batch_first[b] = first strand.  Returns the batch count, or ~0u with the
+// message in err.
+uint32_t ref_curve_payload(const uint32_t *strand_counts, uint32_t sets, const uint32_t *const *counts,
+                           const uint32_t *fixed,
+                           const float *const *points, const uint64_t *npoints, const float *const *widths,
+                           const float *constant_width, const float *times, const float *root_uvs,
+                           uint32_t max_strands, uint32_t resample, float radius_scale, const float *transform,
+                           float *out_points, uint32_t *out_counts, float *out_uvs, float *out_motion,
+                           uint32_t *batch_first, char *err, uint64_t cap) {
+    try {
+        const uint32_t strands = strand_counts[0];
+        auto view = [&](uint32_t k) {
+            CurveSetView v{};
+            v.strand_count = strand_counts[k];
+            v.fixed_cvs_per_strand = fixed[k];
+            if (counts[k]) v.point_counts = {counts[k], strand_counts[k]};
+            v.points = {reinterpret_cast<const Vec3 *>(points[k]), npoints[k]};
+            if (widths[k]) v.widths = {widths[k], npoints[k]};
+            v.constant_width = constant_width[k];
+            return v;
+        };
+        CurveSetView base = view(0u);
+        if (root_uvs) base.root_uvs = {reinterpret_cast<const Vec2 *>(root_uvs), strands};
+        std::vector<MotionSampleView> motion;
+        for (uint32_t k = 1; k < sets; ++k) motion.push_back({times[k - 1u], view(k)});
+        CurvePayloadOptions options{};
+        options.max_strands_per_batch = max_strands;
+        options.resample_cv_count = resample;
+        options.radius_scale = radius_scale;
+        for (int i = 0; i < 12; ++i) options.transform.values[i] = transform[i];
+        const std::vector<CurveBatch> batches = build_curve_batches(base, motion, {}, {}, options);
+        std::size_t p = 0u, c = 0u, u = 0u, m = 0u;
+        for (std::size_t b = 0; b < batches.size(); ++b) {
+            const CurveBatch &batch = batches[b];
+            batch_first[b] = batch.first_strand;
+            std::memcpy(out_points + 4u * p, batch.points.data(), batch.points.size() * sizeof(PackedCurvePoint));
+            p += batch.points.size();
+            std::memcpy(out_counts + c, batch.point_counts.data(), batch.point_counts.size() * 4u);
+            c += batch.point_counts.size();
+            if (out_uvs && !batch.root_uvs.empty()) {
+                std::memcpy(out_uvs + 2u * u, batch.root_uvs.data(), batch.root_uvs.size() * sizeof(Vec2));
+                u += batch.root_uvs.size();
+            }
+            if (out_motion && !batch.motion_points.empty()) {
+                std::memcpy(out_motion + 3u * m, batch.motion_points.data(), batch.motion_points.size() * sizeof(Vec3));
+                m += batch.motion_points.size();
+            }
+        }
+        return static_cast<uint32_t>(batches.size());
+    } catch (const std::exception &e) {
+        std::snprintf(err, cap, "%s", e.what());
+        return ~0u;
+    }
 }
 
 uint32_t ref_runtime_hash_component(float x) { return xgen_runtime_hash_component(x); }

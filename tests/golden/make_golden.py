@@ -17,7 +17,10 @@ never leaves this container.
   * one constant-expression ClumpingFX module of the same runtime over
     seeded clump guides and strands (reference_clump.npz).
 
-    python tests/golden/make_golden.py [generation] [expressions] [classic] [clump]
+  * build_curve_batches over seeded curve sets (reference_payload.npz; the
+    inputs are regenerated from payload_cases()).
+
+    python tests/golden/make_golden.py [generation] [expressions] [classic] [clump] [payload]
 """
 import os
 import sys
@@ -363,8 +366,59 @@ def main_clump():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def payload_cases():
+    """Seeded curve sets for build_curve_batches: (name, base, motion samples, options)."""
+    rng = np.random.default_rng(71)
+    cases = []
+    # the reference's own test (tests/test_main.cpp:357-415)
+    base = dict(counts=np.uint32([2, 3]), points=np.float32([[0, 0, 0], [0, 1, 0], [1, 0, 0], [1, 1, 0], [1, 2, 0]]),
+                widths=np.float32([0.02, 0.04, 0.06, 0.04, 0.02]), root_uvs=np.float32([[0.25, 0.5], [0.75, 0.125]]))
+    mot = dict(counts=np.uint32([3, 2]), points=np.float32([[0, .5, 0], [0, 1, 0], [0, 1.5, 0], [1, .5, 0], [1, 2.5, 0]]),
+               constant_width=0.02, time=0.5)
+    tr = [1.0, 0, 0, 10.0, 0, 1.0, 0, 0, 0, 0, 1.0, 0]
+    cases.append(("reftest", base, [mot], dict(max_strands=1, resample=4, radius_scale=2.0, transform=tr)))
+    # fixed counts over several batches and a tail, constant width
+    n = 1001
+    pts = rng.normal(size=(n * 2, 3)).astype(np.float32)
+    cases.append(("fixed", dict(fixed=2, points=pts, constant_width=0.02,
+                                root_uvs=rng.uniform(size=(n, 2)).astype(np.float32)), [],
+                  dict(max_strands=100, resample=0, radius_scale=1.0)))
+    # variable counts with widths, two motion samples of the same topology
+    n = 700
+    counts = rng.integers(2, 12, size=n).astype(np.uint32)
+    p = int(counts.sum())
+    base = dict(counts=counts, points=rng.normal(size=(p, 3)).astype(np.float32),
+                widths=rng.uniform(0.001, 0.05, size=p).astype(np.float32))
+    motion = [dict(counts=counts, points=(base["points"] + np.float32(0.1 * (k + 1))).astype(np.float32),
+                   time=0.25 * (k + 1)) for k in range(2)]
+    cases.append(("variable", base, motion, dict(max_strands=256, resample=0, radius_scale=0.5)))
+    # resampling with a transform; a motion sample with other counts
+    counts2 = rng.integers(2, 9, size=n).astype(np.uint32)
+    motion2 = [dict(counts=counts2, points=rng.normal(size=(int(counts2.sum()), 3)).astype(np.float32), time=1.0)]
+    tr2 = [0.5, 0.1, 0.0, 1.0, -0.2, 0.9, 0.3, -2.0, 0.0, 0.4, 1.1, 0.5]
+    cases.append(("resample", dict(base, widths=None, constant_width=0.03), motion2,
+                  dict(max_strands=300, resample=7, radius_scale=1.5, transform=tr2)))
+    return cases
+
+
+def main_payload():
+    oracle.build()
+    if oracle.ref() is None:
+        raise SystemExit("reference core not built (needs /root/reference)")
+    fx = {}
+    for name, base, motion, opts in payload_cases():
+        r, err = oracle.ref_curve_payload(base, motion, **opts)
+        assert r is not None, err
+        for k, v in r.items():
+            if v is not None:
+                fx[f"{name}_out_{k}"] = v
+    path = os.path.join(HERE, "reference_payload.npz")
+    np.savez_compressed(path, **fx)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["generation", "expressions", "classic", "clump"]
+    which = sys.argv[1:] or ["generation", "expressions", "classic", "clump", "payload"]
     if "generation" in which:
         main()
     if "expressions" in which:
@@ -373,3 +427,5 @@ if __name__ == "__main__":
         main_classic()
     if "clump" in which:
         main_clump()
+    if "payload" in which:
+        main_payload()
