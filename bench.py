@@ -509,6 +509,74 @@ def clump_cpu_baseline(gen, N, Cv, cvs3, recs, target_s):
                       "the reference's apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread pool)"}
 
 
+def bench_transport(args):
+    """Renderer transport over the plane workload's generated curves
+    (2,000,000 strands x 16 CVs, as the reference's SoA GeneratedCurves:
+    Vec3 points + per-point widths + root uvs): build_curve_batches into 64K
+    batches with a transform and one motion sample (nxg_curve_payload_build,
+    the timed step), then the .nxc curve cache of the packed curves assembled
+    on the GPU (reported beside it: one download plus the serial host hash)."""
+    import numpy as np
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import workloads
+    from nanoxgen_amd.cache import build_curve_cache
+    from nanoxgen_amd.payload import curve_payload
+
+    w = workloads.config2()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    blob = nx.build_asset(w.build)
+    asset = nx.DeviceAsset(blob)
+    gen = nx.PackedOutputs(N, Cv)
+    nx.generate_packed(asset, w.params, gen, octaves=w.octaves)
+    pts = gen.points[:, :3].contiguous()
+    widths = (gen.points[:, 3] * 2.0).contiguous()
+    base = dict(fixed=Cv, points=pts, widths=widths, root_uvs=gen.root_uvs)
+    motion = [dict(fixed=Cv, points=(pts + 0.01).contiguous(), time=0.5)]
+    transform = (1.0, 0.0, 0.0, 5.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, -3.0)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        return curve_payload(base, motion, transform=transform, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e in evs:
+        e[0].record(stream)
+        out = step()
+        e[1].record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    call = sorted(e[0].elapsed_time(e[1]) for e in evs)
+    call_mean = sum(call) / len(call)
+    assert int(out["status"].item()) == 0
+    P = N * Cv
+    alg = P * (12 + 4 + 12) + N * 8 + P * (16 + 12) + N * (4 + 8)
+    achieved = alg / (call_mean * 1e-3) / 1e9
+    t1 = time.perf_counter()
+    cache = build_curve_cache(gen.point_counts, gen.points, patch_uvs=gen.root_uvs)
+    cache_ms = (time.perf_counter() - t1) * 1e3
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "payload call: offsets scans + batch_points_kernel + payload_points_kernel + "
+                          "payload_motion_kernel (one C-ABI call)",
+                "kernel_ms_mean": round(call_mean, 4), "kernel_ms_median": round(call[len(call) // 2], 4),
+                "algorithmic_bytes_per_launch": alg, "valu": None,
+                "curve_cache": {"bytes": int(cache.nbytes), "ms": round(cache_ms, 2),
+                                "what": "GPU assembly + one D2H download + host content hash and validation"}}
+    line = {"metric": METRIC, "value": P / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "transport-payload-plane2M", "input": w.name, "strands": N, "cvs_per_strand": Cv,
+                       "batches": len(out["batch_points"]) - 1, "motion_samples": 1, "parallelism": "single-gpu"},
+            "roofline": roofline, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def classic_cpu_baseline(gen, N, Cv, scale, offset, target_s):
     """The reference's own apply_xgen_classic_float_runtime_plan_cpu (built
     from its sources into oracle/_ref) on the first S generated strands, with
@@ -550,14 +618,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch", "classic", "clump"],
+    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch", "classic", "clump", "transport"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
                          "the trained surrogate + sampled exact check against the exact runtime; "
                          "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph; "
                          "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands; "
-                         "clump: ClumpingFX toward the plane's 4,096 guides over the same strands")
+                         "clump: ClumpingFX toward the plane's 4,096 guides over the same strands; "
+                         "transport: 64K renderer batches (transform, motion) and the .nxc cache of the same strands")
     ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
     ap.add_argument("--sample-stride", type=int, default=16, help="surrogate: exact check of every n-th root")
     ap.add_argument("--train-epochs", type=int, default=40)
@@ -584,6 +653,8 @@ def main():
         return bench_classic(args)
     if args.workload == "clump" and args.gpus == 1:
         return bench_clump(args)
+    if args.workload == "transport" and args.gpus == 1:
+        return bench_transport(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(strands are independent: a multi-GPU run shards strand ranges, one process per GPU)", file=sys.stderr)

@@ -105,7 +105,8 @@ __global__ void copy_status_kernel(const uint32_t *word, uint32_t *caller) { *ca
 
 struct MotionSet {
     const nxg_vec3 *points;
-    const uint32_t *offsets;
+    const uint32_t *offsets;  // NULL: `fixed` CVs per curve
+    uint32_t fixed;
 };
 
 struct PayloadArgs {
@@ -113,7 +114,8 @@ struct PayloadArgs {
     const float *widths;
     const nxg_root_sample *roots;
     const nxg_vec2 *root_uvs;
-    const uint32_t *offsets;  // base CV offsets (strands + 1)
+    const uint32_t *offsets;  // base CV offsets (strands + 1); NULL: `fixed` CVs per curve
+    uint32_t fixed;
     MotionSet motion[NXG_PAYLOAD_MAX_MOTION_SAMPLES];
     uint32_t motion_count;
     uint32_t strands, per_batch, batch_count, resample;
@@ -128,9 +130,16 @@ struct PayloadArgs {
     uint64_t *batch_points;
 };
 
+__device__ __forceinline__ uint32_t in_offset(const PayloadArgs &a, uint32_t s) {
+    return a.offsets ? a.offsets[s] : s * a.fixed;
+}
+__device__ __forceinline__ uint32_t motion_offset(const MotionSet &m, uint32_t s) {
+    return m.offsets ? m.offsets[s] : s * m.fixed;
+}
+
 // Output offset of strand s: s * resample, or its input offset.
 __device__ __forceinline__ uint64_t out_offset(const PayloadArgs &a, uint32_t s) {
-    return a.resample ? uint64_t{s} * a.resample : a.offsets[s];
+    return a.resample ? uint64_t{s} * a.resample : in_offset(a, s);
 }
 
 __device__ __forceinline__ nxg_vec3 transform(const PayloadArgs &a, nxg_vec3 p) {
@@ -195,7 +204,7 @@ __global__ void __launch_bounds__(kTile) payload_points_kernel(PayloadArgs a) {
             const nxg_vec3 p = transform(a, a.points[e]);
             out = {p.x, p.y, p.z, (a.widths ? a.widths[e] : a.constant_width) * a.radius_factor};
         } else {  // src/curve_payload.cpp:247-259
-            const uint32_t c0 = a.offsets[s], count = a.offsets[s + 1u] - c0;
+            const uint32_t c0 = in_offset(a, s), count = in_offset(a, s + 1u) - c0;
             const float t = static_cast<float>(cv) / static_cast<float>(a.resample - 1u);
             uint32_t i;
             float f = 0.0f;
@@ -227,7 +236,7 @@ __global__ void __launch_bounds__(kTile) payload_motion_kernel(PayloadArgs a) {
         const uint32_t s = first + r;
         const uint32_t cv = static_cast<uint32_t>(e - offs[r]);
         const uint32_t out_count = static_cast<uint32_t>(offs[r + 1u] - offs[r]);
-        const uint32_t m0 = m.offsets[s], m_count = m.offsets[s + 1u] - m0;
+        const uint32_t m0 = motion_offset(m, s), m_count = motion_offset(m, s + 1u) - m0;
         nxg_vec3 p;
         if (!a.resample && m_count == out_count) {  // src/curve_payload.cpp:270-272
             p = m.points[m0 + cv];
@@ -361,8 +370,14 @@ uint32_t nxg_curve_payload_build(const nxg_curve_set *base, const nxg_curve_set 
     a.widths = base->widths;
     a.roots = base->roots;
     a.root_uvs = base->root_uvs;
-    a.offsets = sa.job[0].offsets;
-    for (uint32_t k = 0; k < motion_count; ++k) a.motion[k] = {motion[k].points, sa.job[k + 1u].offsets};
+    // fixed-count sets need no scan: their offsets are s * fixed
+    bool scan = false;
+    for (uint32_t k = 0; k <= motion_count; ++k) scan |= (k ? motion[k - 1u] : *base).point_counts != nullptr;
+    a.offsets = base->point_counts ? sa.job[0].offsets : nullptr;
+    a.fixed = base->fixed_cvs_per_strand;
+    for (uint32_t k = 0; k < motion_count; ++k)
+        a.motion[k] = {motion[k].points, motion[k].point_counts ? sa.job[k + 1u].offsets : nullptr,
+                       motion[k].fixed_cvs_per_strand};
     a.motion_count = motion_count;
     a.strands = base->strand_count;
     a.per_batch = options->max_strands_per_batch;
@@ -380,11 +395,13 @@ uint32_t nxg_curve_payload_build(const nxg_curve_set *base, const nxg_curve_set 
     a.out_motion = out->motion_points;
     a.batch_points = out->batch_points;
     hipLaunchKernelGGL(clear_status_kernel, dim3(1), dim3(1), 0, stream, sa.status, out->status);
-    hipLaunchKernelGGL(offsets_block_kernel, dim3(sa.block_count, motion_count + 1u), dim3(kScanThreads), 0, stream,
-                       sa);
-    hipLaunchKernelGGL(offsets_groups_kernel, dim3(motion_count + 1u), dim3(kScanThreads), 0, stream, sa);
-    hipLaunchKernelGGL(offsets_add_kernel, dim3(sa.block_count, motion_count + 1u), dim3(kScanThreads), 0, stream,
-                       sa);
+    if (scan) {
+        hipLaunchKernelGGL(offsets_block_kernel, dim3(sa.block_count, motion_count + 1u), dim3(kScanThreads), 0,
+                           stream, sa);
+        hipLaunchKernelGGL(offsets_groups_kernel, dim3(motion_count + 1u), dim3(kScanThreads), 0, stream, sa);
+        hipLaunchKernelGGL(offsets_add_kernel, dim3(sa.block_count, motion_count + 1u), dim3(kScanThreads), 0,
+                           stream, sa);
+    }
     hipLaunchKernelGGL(batch_points_kernel, dim3((batches + kScanThreads) / kScanThreads), dim3(kScanThreads), 0,
                        stream, a);
     const uint32_t tiles = static_cast<uint32_t>((n + kTile - 1u) / kTile);
