@@ -425,6 +425,64 @@ enum { NXG_ATTR_LENGTH = 0, NXG_ATTR_WIDTH = 1, NXG_ATTR_NOISE = 2, NXG_ATTR_CLU
 #define NXG_EXPR_MAX_PROGRAMS 4u
 #define NXG_EXPR_MAX_TEXEL_MAPS 4u
 
+/* The expression compiler (compile_xgen_scalar_expression followed by
+ * make_xgen_float_expression_program, src/xgen_expression.cpp:50-557,
+ * 771-818; XgenExpressionCompileOptions, xgen_expression.h:102-110): XGen /
+ * SeExpr scalar expression text to this float IR, instruction for
+ * instruction the reference's program, with its diagnostics.  Limits of 0
+ * take the reference's defaults.  extensions = 1 also accepts pow(a, b),
+ * mix(a, b, t), noise(x, y, z) and texel0..texel3(f, u, v) (the POW, MIX,
+ * NOISE3 and TEXEL ops). */
+typedef struct nxg_expr_compile_options {
+    const char *expression_name;  /* with object_type, seeds rand() call sites */
+    const char *object_type;
+    uint64_t max_source_bytes;
+    uint64_t max_tokens;
+    uint64_t max_instructions;
+    uint64_t max_locals;
+    uint64_t max_ramp_points;
+    uint32_t extensions;
+    uint32_t reserved;
+} nxg_expr_compile_options;
+
+/* Caller buffers with capacities in elements (input_names: bytes, the
+ * program's $-variable names NUL-terminated back to back, in input order);
+ * every count is written even when a buffer is too small
+ * (NXG_ERROR_BUFFER_TOO_SMALL), so a first call with zero capacities sizes
+ * the second. */
+typedef struct nxg_expr_compiled {
+    nxg_expr_instruction *instructions;
+    uint32_t *operands;
+    nxg_expr_ramp_point *ramp_points;
+    nxg_expr_ramp *ramps;
+    char *input_names;
+    uint32_t instruction_capacity;
+    uint32_t operand_capacity;
+    uint32_t ramp_point_capacity;
+    uint32_t ramp_capacity;
+    uint32_t input_name_capacity;
+    uint32_t instruction_count;
+    uint32_t operand_count;
+    uint32_t ramp_point_count;
+    uint32_t ramp_count;
+    uint32_t input_count;
+    uint32_t input_name_bytes;
+    uint32_t result;
+    uint32_t expression_seed;
+    uint32_t random_call_count;
+} nxg_expr_compiled;
+
+/* NXG_OK, NXG_ERROR_INVALID_PROGRAM (the reference's message in err) or
+ * NXG_ERROR_BUFFER_TOO_SMALL. */
+uint32_t nxg_expr_compile(const char *source, uint64_t source_bytes, const nxg_expr_compile_options *options,
+                          nxg_expr_compiled *out, char *err, uint64_t err_capacity);
+
+/* xgen_seexpr_hash, xgen_string_seed, xgen_face_seed
+ * (src/xgen_expression.cpp:659-661, 738-769). */
+double nxg_seexpr_hash(const double *arguments, uint32_t count);
+uint32_t nxg_string_seed(const char *text);
+double nxg_face_seed(uint32_t description_id, const char *patch_name, uint32_t face_id);
+
 /* Checks one program the way the reference interpreter does while it runs
  * (operand ranges, SSA order, arity, input/ramp indices, result index), plus
  * the runtime limit NXG_EXPR_MAX_VALUES.  NXG_OK or NXG_ERROR_INVALID_PROGRAM. */

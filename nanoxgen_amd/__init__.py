@@ -40,6 +40,10 @@ SYMBOLS = [
     ("nxg_prepare_packed_generation", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_generate_packed_prepared", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_expr_validate", C.c_uint32, [_P]),
+    ("nxg_expr_compile", C.c_uint32, [C.c_char_p, C.c_uint64, _P, _P, C.c_char_p, C.c_uint64]),
+    ("nxg_seexpr_hash", C.c_double, [_P, C.c_uint32]),
+    ("nxg_string_seed", C.c_uint32, [C.c_char_p]),
+    ("nxg_face_seed", C.c_double, [C.c_uint32, C.c_char_p, C.c_uint32]),
     ("nxg_expr_pack", C.c_uint32, [_P, C.c_uint32, _P, C.c_uint64, _P]),
     ("nxg_evaluate_root_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, _P, _P]),
     ("nxg_generate_packed_attributed", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P, _P, _P, _P, C.c_uint64, _P]),

@@ -274,6 +274,24 @@ class ClassicOutput(C.Structure):
                 ("status", C.c_void_p)]
 
 
+class ExprCompileOptions(C.Structure):
+    """nxg_expr_compile_options (XgenExpressionCompileOptions, xgen_expression.h:102-110)."""
+    _fields_ = [("expression_name", C.c_char_p), ("object_type", C.c_char_p), ("max_source_bytes", C.c_uint64),
+                ("max_tokens", C.c_uint64), ("max_instructions", C.c_uint64), ("max_locals", C.c_uint64),
+                ("max_ramp_points", C.c_uint64), ("extensions", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ExprCompiled(C.Structure):
+    """nxg_expr_compiled: caller buffers and the compiled counts."""
+    _fields_ = [("instructions", C.c_void_p), ("operands", C.c_void_p), ("ramp_points", C.c_void_p),
+                ("ramps", C.c_void_p), ("input_names", C.c_void_p), ("instruction_capacity", C.c_uint32),
+                ("operand_capacity", C.c_uint32), ("ramp_point_capacity", C.c_uint32), ("ramp_capacity", C.c_uint32),
+                ("input_name_capacity", C.c_uint32), ("instruction_count", C.c_uint32), ("operand_count", C.c_uint32),
+                ("ramp_point_count", C.c_uint32), ("ramp_count", C.c_uint32), ("input_count", C.c_uint32),
+                ("input_name_bytes", C.c_uint32), ("result", C.c_uint32), ("expression_seed", C.c_uint32),
+                ("random_call_count", C.c_uint32)]
+
+
 PAYLOAD_INVALID_COUNTS = 8
 PAYLOAD_MAX_MOTION_SAMPLES = 16
 

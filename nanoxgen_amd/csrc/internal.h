@@ -2,6 +2,7 @@
 // libnanoxgen_amd.so.  Not part of the public ABI (include/nanoxgen_amd.h).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 #include "../../include/nanoxgen_amd.h"
@@ -83,6 +84,13 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
     l.total = l.table_offset + a(kClassicTableBytes);
     return l;
 }
+
+// SeExpr hashing and XGen string seeds in double, as the reference computes
+// them (expr_compile.cpp).
+uint32_t seexpr_component(double x);
+uint32_t seexpr_finish(uint32_t state);
+double seexpr_hash(const double *args, std::size_t n);
+uint32_t string_seed(const char *text, std::size_t size);
 
 // Curve cache (curve_cache.cpp): layout from counts, the builder's data
 // checks, the content hash, and the finishing step (hash, header, validation).

@@ -196,3 +196,87 @@ def evaluate_root_attributes(asset, params, exprs, maps, attributes, error_word=
         maps.count if maps else 0, C.c_void_p(attributes.data_ptr()), attributes.numel(),
         C.c_void_p(error_word.data_ptr()) if error_word is not None else None, _stream_handle(stream)),
         "nxg_evaluate_root_attributes")
+
+
+_INSTR = np.dtype([("op", "u1"), ("reserved", "u1"), ("count", "<u2"), ("offset", "<u4"), ("aux", "<u4"),
+                   ("imm", "<f4")])
+_RPT = np.dtype([("p", "<f4"), ("v", "<f4"), ("i", "<u4")])
+
+# Default binding of expression $-variables to per-root variables.
+DEFAULT_BINDINGS = {"u": "u", "v": "v", "id": "id", "faceId": "face", "Px": "position_x", "Py": "position_y",
+                    "Pz": "position_z", "Nx": "normal_x", "Ny": "normal_y", "Nz": "normal_z"}
+
+
+class CompiledProgram:
+    """A program compiled from expression text (nxg_expr_compile), usable
+    wherever a Program is (ExprSet, oracle.expr_eval)."""
+
+    def __init__(self, arrays, inputs, result, expression_seed, random_call_count):
+        self.ins, self.ops, self.pts, self.ramps_arr = arrays
+        self.inputs = inputs
+        self.result = result
+        self.expression_seed = expression_seed
+        self.random_call_count = random_call_count
+        self.bindings = list(range(len(inputs)))
+        self._struct = None
+
+    @property
+    def instructions(self):
+        """(op, operand_offset, operand_count, auxiliary, immediate) per instruction, as Program holds them."""
+        return [(int(i["op"]), int(i["offset"]), int(i["count"]), int(i["aux"]), float(i["imm"])) for i in self.ins]
+
+    def bind(self, bindings):
+        """Program input name -> root variable name (abi.ROOT_VARS)."""
+        missing = [n for n in self.inputs if n not in bindings]
+        if missing:
+            raise KeyError(f"no root variable bound to ${', $'.join(missing)}")
+        self.bindings = [abi.ROOT_VAR[bindings[n]] for n in self.inputs]
+        self._struct = None
+        return self
+
+    @property
+    def struct(self):
+        if self._struct is None:
+            self._bind = np.ascontiguousarray(self.bindings or [0], dtype=np.uint32)
+            self._struct = abi.ExprProgram(
+                instructions=self.ins.ctypes.data, operands=self.ops.ctypes.data, ramp_points=self.pts.ctypes.data,
+                ramps=self.ramps_arr.ctypes.data, input_bindings=self._bind.ctypes.data,
+                instruction_count=len(self.ins), operand_count=len(self.ops), ramp_point_count=len(self.pts),
+                ramp_count=len(self.ramps_arr), input_count=len(self.inputs), result=self.result,
+                expression_seed=self.expression_seed, random_call_count=self.random_call_count)
+        return self._struct
+
+
+def compile_expression(source, name="", object_type="", *, extensions=False, limits=(0, 0, 0, 0, 0)):
+    """compile_xgen_scalar_expression + make_xgen_float_expression_program
+    (src/xgen_expression.cpp:771-818) through nxg_expr_compile.  Raises
+    ValueError with the reference's diagnostic."""
+    from . import lib
+    src = source.encode()
+    opt = abi.ExprCompileOptions(expression_name=name.encode(), object_type=object_type.encode(),
+                                 max_source_bytes=limits[0], max_tokens=limits[1], max_instructions=limits[2],
+                                 max_locals=limits[3], max_ramp_points=limits[4], extensions=1 if extensions else 0)
+    out = abi.ExprCompiled()
+    err = C.create_string_buffer(512)
+    st = lib().nxg_expr_compile(src, len(src), C.byref(opt), C.byref(out), err, 512)
+    if st == abi.INVALID_PROGRAM:
+        raise ValueError(err.value.decode())
+    ins = np.zeros(max(1, out.instruction_count), _INSTR)
+    ops = np.zeros(max(1, out.operand_count), np.uint32)
+    pts = np.zeros(max(1, out.ramp_point_count), _RPT)
+    ramps = np.zeros((max(1, out.ramp_count), 2), np.uint32)
+    names = C.create_string_buffer(max(1, out.input_name_bytes))
+    out = abi.ExprCompiled(instructions=ins.ctypes.data, operands=ops.ctypes.data, ramp_points=pts.ctypes.data,
+                           ramps=ramps.ctypes.data, input_names=C.cast(names, C.c_void_p),
+                           instruction_capacity=len(ins), operand_capacity=len(ops), ramp_point_capacity=len(pts),
+                           ramp_capacity=len(ramps), input_name_capacity=len(names))
+    st = lib().nxg_expr_compile(src, len(src), C.byref(opt), C.byref(out), err, 512)
+    if st != 0:
+        raise ValueError(err.value.decode() or lib().nxg_status_message(st).decode())
+    inputs = [x.decode() for x in names.raw[:out.input_name_bytes].split(b"\0")[:-1]] if out.input_count else []
+    arrays = (ins[:out.instruction_count], ops[:max(out.operand_count, 1)] if out.operand_count else ops[:1],
+              pts[:out.ramp_point_count] if out.ramp_point_count else pts[:1],
+              ramps[:out.ramp_count] if out.ramp_count else ramps[:1])
+    cp = CompiledProgram(arrays, inputs, out.result, out.expression_seed, out.random_call_count)
+    cp.counts = (out.instruction_count, out.operand_count, out.ramp_point_count, out.ramp_count)
+    return cp

@@ -129,6 +129,14 @@ def ref():
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                         C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_char_p, C.c_uint64]
+        L.ref_expr_compile.restype = C.c_int
+        L.ref_expr_compile.argtypes = [C.c_char_p, C.c_uint64, C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint32, C.c_void_p,
+                                       C.c_char_p, C.c_uint64]
+        L.ref_seexpr_hash.restype = C.c_double
+        L.ref_seexpr_hash.argtypes = [C.c_void_p, C.c_uint32]
+        L.ref_face_seed.restype = C.c_double
+        L.ref_face_seed.argtypes = [C.c_uint32, C.c_char_p, C.c_uint32]
         L.ref_classic_endpoints.restype = C.c_uint32
         L.ref_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
@@ -515,3 +523,37 @@ def ref_curve_payload(base, motion=(), *, max_strands=65536, resample=0, radius_
         return None, err.value.decode()
     return dict(points=out_p[:total], point_counts=out_c[:n], root_uvs=out_u[:n] if uvs is not None else None,
                 motion_points=out_m[:total * len(motion)], batch_first=first[:b]), ""
+
+
+_INSTR = np.dtype([("op", "u1"), ("reserved", "u1"), ("count", "<u2"), ("offset", "<u4"), ("aux", "<u4"),
+                   ("imm", "<f4")])
+_RPT = np.dtype([("p", "<f4"), ("v", "<f4"), ("i", "<u4")])
+
+
+def ref_expr_compile(source, name="", object_type="", limits=(0, 0, 0, 0, 0), capacity=4096):
+    """The reference's compile + float conversion: dict(instructions, operands,
+    ramp_points, ramps, inputs, result, expression_seed, random_call_count) or
+    (None, message)."""
+    src = source.encode()
+    ins = np.zeros(capacity, _INSTR)
+    ops = np.zeros(capacity, np.uint32)
+    pts = np.zeros(capacity, _RPT)
+    ramps = np.zeros((capacity, 2), np.uint32)
+    names = C.create_string_buffer(65536)
+    counts = np.zeros(8, np.uint32)
+    lim = np.array(limits, np.uint64)
+    err = C.create_string_buffer(512)
+    rc = ref().ref_expr_compile(src, len(src), name.encode(), object_type.encode(), lim.ctypes.data, ins.ctypes.data,
+                                ops.ctypes.data, pts.ctypes.data, ramps.ctypes.data, names, capacity,
+                                counts.ctypes.data, err, 512)
+    if rc:
+        return None, err.value.decode()
+    n_ins, n_ops, n_pts, n_ramps, n_in, result, seed, rands = map(int, counts)
+    raw = names.raw
+    inputs, at = [], 0
+    for _ in range(n_in):
+        end = raw.index(b"\0", at)
+        inputs.append(raw[at:end].decode())
+        at = end + 1
+    return dict(instructions=ins[:n_ins], operands=ops[:n_ops], ramp_points=pts[:n_pts], ramps=ramps[:n_ramps],
+                inputs=inputs, result=result, expression_seed=seed, random_call_count=rands), ""

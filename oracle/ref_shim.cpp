@@ -443,6 +443,61 @@ uint32_t ref_curve_payload(const uint32_t *strand_counts, uint32_t sets, const u
     }
 }
 
+// compile_xgen_scalar_expression + make_xgen_float_expression_program
+// (xgen_expression.cpp:771-818).  The float program goes to caller arrays of
+// at least `cap` entries each (instructions as nxg_expr_instruction, ramp
+// points as nxg_expr_ramp_point, ramps as two u32), input names NUL-separated
+// into `names`; counts[0..7] = instructions, operands, ramp points, ramps,
+// inputs, result, expression seed, rand calls.  Returns 0, or 1 with the
+// message in err.
+int ref_expr_compile(const char *source, uint64_t size, const char *name, const char *type, const uint64_t *limits,
+                     nxg_expr_instruction *ins, uint32_t *operands, nxg_expr_ramp_point *points, uint32_t *ramps,
+                     char *names, uint32_t cap, uint32_t *counts, char *err, uint64_t errcap) {
+    try {
+        XgenExpressionCompileOptions o{};
+        o.expression_name = name;
+        o.object_type = type;
+        if (limits[0]) o.max_source_bytes = limits[0];
+        if (limits[1]) o.max_tokens = limits[1];
+        if (limits[2]) o.max_instructions = limits[2];
+        if (limits[3]) o.max_locals = limits[3];
+        if (limits[4]) o.max_ramp_points = limits[4];
+        const XgenFloatExpressionProgram p =
+            make_xgen_float_expression_program(compile_xgen_scalar_expression({source, size}, o));
+        if (p.instructions.size() > cap || p.operands.size() > cap || p.ramp_points.size() > cap ||
+            p.ramps.size() > cap)
+            throw std::runtime_error("shim capacity");
+        for (std::size_t i = 0; i < p.instructions.size(); ++i) {
+            const XgenFloatScalarInstruction &x = p.instructions[i];
+            ins[i] = {static_cast<uint8_t>(x.op), 0u, x.operand_count, x.operand_offset, x.auxiliary, x.immediate};
+        }
+        std::memcpy(operands, p.operands.data(), p.operands.size() * 4u);
+        for (std::size_t i = 0; i < p.ramp_points.size(); ++i)
+            points[i] = {p.ramp_points[i].position, p.ramp_points[i].value, p.ramp_points[i].interpolation};
+        for (std::size_t i = 0; i < p.ramps.size(); ++i) {
+            ramps[2 * i] = p.ramps[i].point_offset;
+            ramps[2 * i + 1] = p.ramps[i].point_count;
+        }
+        char *n = names;
+        for (const std::string &s : p.inputs) {
+            std::memcpy(n, s.c_str(), s.size() + 1u);
+            n += s.size() + 1u;
+        }
+        const uint32_t c[8] = {static_cast<uint32_t>(p.instructions.size()), static_cast<uint32_t>(p.operands.size()),
+                               static_cast<uint32_t>(p.ramp_points.size()), static_cast<uint32_t>(p.ramps.size()),
+                               static_cast<uint32_t>(p.inputs.size()), p.result, p.expression_seed,
+                               p.random_call_count};
+        std::memcpy(counts, c, sizeof(c));
+        return 0;
+    } catch (const std::exception &e) {
+        std::snprintf(err, errcap, "%s", e.what());
+        return 1;
+    }
+}
+
+double ref_seexpr_hash(const double *args, uint32_t n) { return xgen_seexpr_hash({args, n}); }
+double ref_face_seed(uint32_t id, const char *patch, uint32_t face) { return xgen_face_seed(id, patch, face); }
+
 uint32_t ref_runtime_hash_component(float x) { return xgen_runtime_hash_component(x); }
 uint32_t ref_seexpr_hash_finish(uint32_t s) { return xgen_seexpr_hash_finish(s); }
 float ref_random01(uint32_t seed, uint32_t strand, uint32_t dim) { return random01(seed, strand, dim); }

@@ -147,3 +147,24 @@ def test_end_to_end_expression_workload(torch, orc):
     rerr = np.abs(gpu["points"][:, 3] - pts[:, 3]).max()
     print(f"expression workload: max position err {err:.3e} (tol {1e-5 * diag:.3e}), max radius err {rerr:.3e}")
     assert err <= 1e-5 * diag and rerr <= 1e-7
+
+
+def test_compiled_expression_text(torch, orc):
+    """Programs compiled from expression text (nxg_expr_compile, identical to
+    the reference compiler's programs) bound to root variables: GPU ==
+    oracle, including rand() call-site seeds and rampUI."""
+    sources = [
+        "$a = hash($id + 355) <= .3 ? rand(0.1, 0.7) : rand(0.2, 0);\n$a * 2 + rampUI(0,0,3:0.5,1,3:1,0,3)",
+        "$w = fit($u, 0, 1, 0.5, 1.5); clamp($w * (1 + noise($Px * 4, $Py * 4, $Pz * 4) - 0.5), 0.5, 1.5)",
+        "$k = smoothstep(0.2, 0.8, $v)->gamma(2); mix(0.3, 1.7, $k) + 0 * texel0($faceId, $u, $v)",
+        "rand(3) > .5 ? max(1, $Nx + 1) : min(1, 0.5 + $Ny * $Ny)",
+    ]
+    progs = [expr.compile_expression(s, f"attr{k}", "SplinePrimitive", extensions=True).bind(expr.DEFAULT_BINDINGS)
+             for k, s in enumerate(sources)]
+    w = workloads.config1(strands=20000)
+    blob = nx.build_asset(w.build)
+    tm = expr.texel_map(abi.header_of(blob).triangle_count)
+    gpu, err = gpu_attributes(blob, w.params, progs, [tm])
+    ref, fails = orc.root_attributes(blob, w.params, progs, [tm])
+    assert err == 0 and fails == 0
+    check_attributes(gpu, ref, progs)
