@@ -211,8 +211,11 @@ class CompiledProgram:
     """A program compiled from expression text (nxg_expr_compile), usable
     wherever a Program is (ExprSet, oracle.expr_eval)."""
 
-    def __init__(self, arrays, inputs, result, expression_seed, random_call_count):
+    def __init__(self, arrays, counts, inputs, result, expression_seed, random_call_count):
+        # arrays hold at least one element each so their pointers are valid;
+        # counts are the lengths the program actually has.
         self.ins, self.ops, self.pts, self.ramps_arr = arrays
+        self.counts = counts
         self.inputs = inputs
         self.result = result
         self.expression_seed = expression_seed
@@ -223,7 +226,8 @@ class CompiledProgram:
     @property
     def instructions(self):
         """(op, operand_offset, operand_count, auxiliary, immediate) per instruction, as Program holds them."""
-        return [(int(i["op"]), int(i["offset"]), int(i["count"]), int(i["aux"]), float(i["imm"])) for i in self.ins]
+        return [(int(i["op"]), int(i["offset"]), int(i["count"]), int(i["aux"]), float(i["imm"]))
+                for i in self.ins[:self.counts[0]]]
 
     def bind(self, bindings):
         """Program input name -> root variable name (abi.ROOT_VARS)."""
@@ -241,8 +245,8 @@ class CompiledProgram:
             self._struct = abi.ExprProgram(
                 instructions=self.ins.ctypes.data, operands=self.ops.ctypes.data, ramp_points=self.pts.ctypes.data,
                 ramps=self.ramps_arr.ctypes.data, input_bindings=self._bind.ctypes.data,
-                instruction_count=len(self.ins), operand_count=len(self.ops), ramp_point_count=len(self.pts),
-                ramp_count=len(self.ramps_arr), input_count=len(self.inputs), result=self.result,
+                instruction_count=self.counts[0], operand_count=self.counts[1], ramp_point_count=self.counts[2],
+                ramp_count=self.counts[3], input_count=len(self.inputs), result=self.result,
                 expression_seed=self.expression_seed, random_call_count=self.random_call_count)
         return self._struct
 
@@ -274,9 +278,6 @@ def compile_expression(source, name="", object_type="", *, extensions=False, lim
     if st != 0:
         raise ValueError(err.value.decode() or lib().nxg_status_message(st).decode())
     inputs = [x.decode() for x in names.raw[:out.input_name_bytes].split(b"\0")[:-1]] if out.input_count else []
-    arrays = (ins[:out.instruction_count], ops[:max(out.operand_count, 1)] if out.operand_count else ops[:1],
-              pts[:out.ramp_point_count] if out.ramp_point_count else pts[:1],
-              ramps[:out.ramp_count] if out.ramp_count else ramps[:1])
-    cp = CompiledProgram(arrays, inputs, out.result, out.expression_seed, out.random_call_count)
-    cp.counts = (out.instruction_count, out.operand_count, out.ramp_point_count, out.ramp_count)
-    return cp
+    counts = (out.instruction_count, out.operand_count, out.ramp_point_count, out.ramp_count)
+    arrays = (ins[:max(1, counts[0])], ops[:max(1, counts[1])], pts[:max(1, counts[2])], ramps[:max(1, counts[3])])
+    return CompiledProgram(arrays, counts, inputs, out.result, out.expression_seed, out.random_call_count)

@@ -203,3 +203,17 @@ def test_extensions_and_bindings(orc):
     out = abi.ExprCompiled()
     assert nx.lib().nxg_expr_compile(b"1+$a", 4, None, C.byref(out), None, 0) == abi.BUFFER_TOO_SMALL
     assert (out.instruction_count, out.operand_count, out.input_count, out.input_name_bytes) == (3, 2, 1, 2)
+
+
+def test_compiled_programs_validate_without_ramps():
+    """A compiled program with no ramps, ramp points or operands presents
+    zero counts to nxg_expr_validate / nxg_expr_pack (its padded host arrays
+    are not counted)."""
+    import ctypes as C
+    for source in ["0.5", "$u + 1", "clamp($u, 0, 1)", "rampUI(0,0,3:1,1,3)"]:
+        p = expr.compile_expression(source, "a", "SplinePrimitive").bind(expr.DEFAULT_BINDINGS)
+        st = p.struct
+        assert (st.instruction_count, st.operand_count, st.ramp_point_count, st.ramp_count) == p.counts
+        assert nx.lib().nxg_expr_validate(C.byref(st)) == 0, source
+        info = nx.abi.ExprSet()
+        assert nx.lib().nxg_expr_pack(C.byref(st), 1, None, 0, C.byref(info)) == 0, source
