@@ -867,6 +867,116 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
                                  uint64_t out_point_capacity, uint32_t *status, void *workspace,
                                  uint64_t workspace_bytes, void *stream);
 
+/* ---- Classic primitive expressions --------------------------------------- */
+
+/* apply_xgen_classic_float_runtime_plan_cpu (include/nanoxgen/
+ * xgen_classic_runtime.h:185-203, src/xgen_classic_runtime.cpp:1420-1918)
+ * for a ClassicFloatRuntimePlan (xgen_classic_runtime.h:130-165) of
+ * SplinePrimitive expressions (length, width, taper, taperStart, widthRamp)
+ * and at most one CutFX module (rebuildType 1) whose amount is an
+ * expression.  Per strand, in the reference's order:
+ *   cLength; length expression -> scale_curve about the root, cLength *=
+ *   scale (an error when not finite or negative); raw width from the width
+ *   expression, else 2 radius[0] / radius_scale; culled (skipped) when
+ *   cLength < 1e-4 or width < 1e-4; cWidth = width; CutFX: amount,
+ *   cut_from_tip_and_rebuild, culled when the cut parameter < 1e-4, cLength
+ *   of the rebuilt curve; taper, taperStart; per CV t = cv / (n - 1),
+ *   scale = 1 - taper (t - start) / (1 - start) past the start, times
+ *   widthRamp(t), radius = 0.5 max(cWidth scale, 0) radius_scale;
+ * then the kept strands are compacted in order.  Program inputs bind as the
+ * reference's input_value does (src/xgen_classic_runtime.cpp:504-553):
+ * NXG_CLASSIC_VAR_ID ($id: the strand's primitive id, else its index),
+ * _CLENGTH, _CWIDTH, and NXG_CLASSIC_VAR_COLUMN0 + k for column k of the
+ * per-strand input row (the reference's [PTEX, custom, $Prefg noise]
+ * runtime_inputs, xgen_classic_runtime.h:193-195).  rand() reads the
+ * strand's SeExpr prefix (required when any program calls it), else u, v of
+ * its root and the face seed xgen_runtime_face_seed(description_id, name,
+ * face) (src/xgen_expression.cpp:724-736). */
+#define NXG_CLASSIC_VAR_ID 0u
+#define NXG_CLASSIC_VAR_CLENGTH 1u
+#define NXG_CLASSIC_VAR_CWIDTH 2u
+#define NXG_CLASSIC_VAR_COLUMN0 3u
+#define NXG_CLASSIC_MAX_COLUMNS 183u  /* 61 PTEX + 61 custom + 61 $Prefg-noise inputs */
+#define NXG_CLASSIC_MAX_PROGRAM_INPUTS 32u
+#define NXG_CLASSIC_MAX_PROGRAM_SLOTS 32u
+/* status bits beyond NXG_CLASSIC_NONFINITE_AMOUNT / _WIDTH */
+#define NXG_CLASSIC_NEGATIVE_LENGTH 8u      /* "Classic length expression produced a negative value" */
+#define NXG_CLASSIC_NONFINITE_TAPER 16u     /* "Classic taper expression produced a non-finite value" */
+#define NXG_CLASSIC_NONFINITE_PROFILE 32u   /* "Classic width profile produced a non-finite value" */
+#define NXG_CLASSIC_EXPRESSION_ERROR 64u    /* "XGen float expression produced a non-finite value"
+                                               (or a non-finite input or context) */
+#define NXG_CLASSIC_VARIABLE_COUNTS 128u    /* "Classic runtime requires fixed point counts" */
+
+enum {
+    NXG_CLASSIC_EXPR_LENGTH = 0,
+    NXG_CLASSIC_EXPR_WIDTH,
+    NXG_CLASSIC_EXPR_CUT_AMOUNT,
+    NXG_CLASSIC_EXPR_TAPER,
+    NXG_CLASSIC_EXPR_TAPER_START,
+    NXG_CLASSIC_EXPR_WIDTH_RAMP,
+    NXG_CLASSIC_EXPR_COUNT
+};
+
+/* Host memory.  expressions[k] NULL: absent.  Bindings are NXG_CLASSIC_VAR_*;
+ * programs hold the reference's ops only (no pow/mix/noise3/texel), at most
+ * NXG_CLASSIC_MAX_PROGRAM_INPUTS inputs and NXG_CLASSIC_MAX_PROGRAM_SLOTS live
+ * values. */
+typedef struct nxg_classic_plan {
+    const nxg_expr_program *expressions[NXG_CLASSIC_EXPR_COUNT];
+    uint32_t has_cut;               /* the CutFX module; needs expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT] */
+    uint32_t description_id;        /* descriptionId */
+    uint32_t description_name_seed; /* nxg_string_seed of the description name after its last ':' */
+    uint32_t reserved;
+} nxg_classic_plan;
+
+/* What nxg_classic_plan_pack reports about a plan. */
+typedef struct nxg_classic_plan_info {
+    uint64_t byte_size;           /* of the device image */
+    uint32_t present;             /* bit k: expressions[k] given */
+    uint32_t has_cut;
+    uint32_t columns_required;    /* 1 + the largest input column read (0: none) */
+    uint32_t needs_random_prefix; /* some program calls rand() */
+    uint32_t final_length;        /* taper, taperStart or widthRamp reads cLength */
+    uint32_t description_id;
+    uint32_t description_name_seed;
+    uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
+    uint32_t reserved;
+} nxg_classic_plan_info;
+
+/* Validates and packs a plan into a position-independent image (out NULL:
+ * size only).  NXG_OK, NXG_ERROR_INVALID_PROGRAM, NXG_ERROR_BUFFER_TOO_SMALL
+ * or NXG_ERROR_NULL_ARGUMENT. */
+uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t capacity,
+                               nxg_classic_plan_info *info);
+
+typedef struct nxg_classic_runtime_inputs {
+    const void *plan_image;           /* device copy of the packed plan, 16-byte aligned */
+    const nxg_classic_plan_info *plan;/* host: the packer's report for that image */
+    const uint32_t *primitive_ids;    /* optional, device: $id per strand */
+    const uint32_t *random_prefixes;  /* device; required when plan->needs_random_prefix */
+    const float *columns;             /* device, row-major [strand][column_count]; NULL iff column_count 0 */
+    uint32_t column_count;            /* >= plan->columns_required */
+    float radius_scale;               /* finite, >= 0 */
+} nxg_classic_runtime_inputs;
+
+/* Host-side checks (the reference's argument checks,
+ * src/xgen_classic_runtime.cpp:1432-1512, plus capacities). */
+uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nxg_classic_runtime_inputs *inputs,
+                                      const nxg_classic_output *output);
+
+/* Device workspace for nxg_classic_runtime_apply. */
+uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count);
+
+/* On `stream`, without synchronising or allocating: the chain above, then
+ * the stable compaction into `output` (points, point counts, roots, root
+ * uvs).  Where the reference would throw, nothing is written, *kept_count is
+ * 0 and *status carries the NXG_CLASSIC_* bits of every failing strand.
+ * Positions and radii follow the reference's CPU operation order bit for
+ * bit. */
+uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves, const nxg_classic_runtime_inputs *inputs,
+                                   const nxg_classic_output *output, void *workspace, uint64_t workspace_bytes,
+                                   void *stream);
+
 /* ---- renderer curve payload (include/nanoxgen/curve_payload.h) -------- */
 
 /* build_curve_batches (src/curve_payload.cpp:150-301) on the GPU: curves

@@ -274,6 +274,34 @@ class ClassicOutput(C.Structure):
                 ("status", C.c_void_p)]
 
 
+CLASSIC_VAR_ID, CLASSIC_VAR_CLENGTH, CLASSIC_VAR_CWIDTH, CLASSIC_VAR_COLUMN0 = 0, 1, 2, 3
+CLASSIC_MAX_COLUMNS = 183
+CLASSIC_NEGATIVE_LENGTH, CLASSIC_NONFINITE_TAPER, CLASSIC_NONFINITE_PROFILE = 8, 16, 32
+CLASSIC_EXPRESSION_ERROR, CLASSIC_VARIABLE_COUNTS = 64, 128
+CLASSIC_EXPRS = ("length", "width", "cut_amount", "taper", "taper_start", "width_ramp")  # NXG_CLASSIC_EXPR_*
+
+
+class ClassicPlan(C.Structure):
+    """nxg_classic_plan: the plan's programs (host nxg_expr_program pointers, NULL: absent)."""
+    _fields_ = [("expressions", C.c_void_p * len(CLASSIC_EXPRS)), ("has_cut", C.c_uint32),
+                ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ClassicPlanInfo(C.Structure):
+    """nxg_classic_plan_info: what nxg_classic_plan_pack reports."""
+    _fields_ = [("byte_size", C.c_uint64), ("present", C.c_uint32), ("has_cut", C.c_uint32),
+                ("columns_required", C.c_uint32), ("needs_random_prefix", C.c_uint32), ("final_length", C.c_uint32),
+                ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32),
+                ("program_offset", C.c_uint32 * len(CLASSIC_EXPRS)), ("reserved", C.c_uint32)]
+
+
+class ClassicRuntimeInputs(C.Structure):
+    """nxg_classic_runtime_inputs (device pointers, host plan report)."""
+    _fields_ = [("plan_image", C.c_void_p), ("plan", C.c_void_p), ("primitive_ids", C.c_void_p),
+                ("random_prefixes", C.c_void_p), ("columns", C.c_void_p), ("column_count", C.c_uint32),
+                ("radius_scale", C.c_float)]
+
+
 class ExprCompileOptions(C.Structure):
     """nxg_expr_compile_options (XgenExpressionCompileOptions, xgen_expression.h:102-110)."""
     _fields_ = [("expression_name", C.c_char_p), ("object_type", C.c_char_p), ("max_source_bytes", C.c_uint64),

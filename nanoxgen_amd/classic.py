@@ -1,4 +1,5 @@
-"""Classic CutFX on device curves (host side of nxg_classic_cut_apply).
+"""Classic CutFX and primitive expressions on device curves (host side of
+nxg_classic_cut_apply and nxg_classic_runtime_apply).
 
 The reference runs a Classic description's CutFX module inside its float
 runtime (apply_xgen_classic_float_runtime_plan_cpu,
@@ -9,12 +10,20 @@ expression, then an in-order compaction of the kept strands
 amount is amount_scale * cLength + amount_offset, optionally times a
 per-strand multiplier (e.g. one column of the attribute runtime's output),
 and the whole chain runs on the GPU into separate output buffers.
+
+ClassicPlan / classic_runtime cover the same function with a plan of
+SplinePrimitive expressions (length, width, taper, taperStart, widthRamp)
+and a CutFX amount expression, compiled from their text as the reference's
+plan compiler does (compile_xgen_classic_float_runtime_plan,
+src/xgen_classic_runtime.cpp:1189-1237) and bound to the Classic variables
+($id, $cLength, $cWidth and the per-strand input row).
 """
 import ctypes as C
 
 import numpy as np
 
 from . import abi, check, lib, _stream_handle, _torch
+from . import expr as _expr
 
 
 class ClassicCutOutputs:
@@ -169,3 +178,122 @@ def clump(points, cvs, strand_guides, guides, params, *, amount_multiplier=None,
                                         C.c_void_p(workspace.data_ptr()), workspace.numel(),
                                         _stream_handle(stream)), "nxg_classic_clump_apply")
     return out
+
+
+# plan slot -> (attribute, object type): the compile options the reference
+# compiles each expression under (expression_name, object_type), which seed
+# rand() call sites (src/xgen_classic_runtime.cpp:420-426)
+CLASSIC_ATTRIBUTES = {
+    "length": ("length", "SplinePrimitive"),
+    "width": ("width", "SplinePrimitive"),
+    "cut_amount": ("amount", "CutFXModule"),
+    "taper": ("taper", "SplinePrimitive"),
+    "taper_start": ("taperStart", "SplinePrimitive"),
+    "width_ramp": ("widthRamp", "SplinePrimitive"),
+}
+_CLASSIC_INPUT_PREFIXES = ("__nxg_map_", "__nxg_custom_", "__nxg_pref_noise_")
+
+
+def classic_binding(name, ptex_count=0, custom_count=0):
+    """The NXG_CLASSIC_VAR_* index of a Classic runtime variable, as the
+    reference's input_value binds it (src/xgen_classic_runtime.cpp:504-553):
+    $id, $cLength, $cWidth, then the input row [PTEX, custom, $Prefg noise]."""
+    fixed = {"id": abi.CLASSIC_VAR_ID, "cLength": abi.CLASSIC_VAR_CLENGTH, "cWidth": abi.CLASSIC_VAR_CWIDTH}
+    if name in fixed:
+        return fixed[name]
+    bases = (0, ptex_count, ptex_count + custom_count)
+    for prefix, base in zip(_CLASSIC_INPUT_PREFIXES, bases):
+        if name.startswith(prefix) and name[len(prefix):].isdigit():
+            return abi.CLASSIC_VAR_COLUMN0 + base + int(name[len(prefix):])
+    raise KeyError(f"Classic runtime variable is not bound: ${name}")
+
+
+class ClassicPlan:
+    """A ClassicFloatRuntimePlan (include/nanoxgen/xgen_classic_runtime.h:
+    130-165) of SplinePrimitive expressions and at most one CutFX module,
+    packed for the device (nxg_classic_plan_pack).  Each expression is text
+    (compiled here with its attribute name and object type) or a
+    CompiledProgram; ptex_count and custom_count place the custom and
+    $Prefg-noise inputs in the input row."""
+
+    def __init__(self, *, length=None, width=None, taper=None, taper_start=None, width_ramp=None,
+                 cut_amount=None, description_id=0, description_name="", ptex_count=0, custom_count=0):
+        given = dict(length=length, width=width, cut_amount=cut_amount, taper=taper, taper_start=taper_start,
+                     width_ramp=width_ramp)
+        self.programs = {}
+        for key in abi.CLASSIC_EXPRS:
+            src = given[key]
+            if src is None:
+                continue
+            attribute, object_type = CLASSIC_ATTRIBUTES[key]
+            prog = src if isinstance(src, _expr.CompiledProgram) else _expr.compile_expression(
+                src, attribute, object_type)
+            self.programs[key] = prog.bind_indices(
+                [classic_binding(name, ptex_count, custom_count) for name in prog.inputs])
+        self.has_cut = cut_amount is not None
+        self.description_id = description_id
+        self.name_seed = int(lib().nxg_string_seed(description_name.rsplit(":", 1)[-1].encode()))
+        self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, description_id=description_id,
+                                      description_name_seed=self.name_seed)
+        for k, key in enumerate(abi.CLASSIC_EXPRS):
+            if key in self.programs:
+                self.struct.expressions[k] = C.addressof(self.programs[key].struct)
+        self.info = abi.ClassicPlanInfo()
+        check(lib().nxg_classic_plan_pack(C.byref(self.struct), None, 0, C.byref(self.info)), "nxg_classic_plan_pack")
+        self.image = np.zeros(self.info.byte_size, np.uint8)
+        check(lib().nxg_classic_plan_pack(C.byref(self.struct), self.image.ctypes.data, self.image.nbytes,
+                                          C.byref(self.info)), "nxg_classic_plan_pack")
+        self._device = {}
+
+    def device_image(self, device):
+        """The packed image in device memory (uploaded once per device)."""
+        key = str(device)
+        if key not in self._device:
+            self._device[key] = _torch().from_numpy(self.image).to(device)
+        return self._device[key]
+
+
+def runtime_workspace_bytes(strands):
+    return int(lib().nxg_classic_runtime_workspace_size(strands))
+
+
+def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=None, primitive_ids=None,
+                    random_prefixes=None, columns=None, radius_scale=1.0, outputs=None, workspace=None,
+                    stream=None):
+    """Enqueue the Classic runtime chain of `plan` (a ClassicPlan) on
+    `stream` over device curves: `points` float32 [N * cvs, 4], `roots` int32
+    [N, 12] (RootSample records: the context's u, v and face); optional
+    `point_counts` int32 [N], `root_uvs` float32 [N, 2], `primitive_ids`
+    int32 [N] ($id), `random_prefixes` int32 [N] (SeExpr prefixes, required
+    when a program calls rand()) and `columns` float32 [N, K] (the input
+    row).  Returns the ClassicCutOutputs (its kept(), status() and to_host()
+    synchronise)."""
+    torch = _torch()
+    if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
+        raise ValueError("points must be a contiguous float32 [N * cvs, 4] tensor")
+    n = points.shape[0] // cvs
+    if outputs is None:
+        outputs = ClassicCutOutputs(n, cvs, device=points.device, point_counts=True, roots=True,
+                                    root_uvs=root_uvs is not None)
+    if workspace is None:
+        workspace = torch.empty(runtime_workspace_bytes(n), dtype=torch.uint8, device=points.device)
+    column_count = 0 if columns is None else (columns.shape[1] if columns.dim() == 2 else 0)
+    for name, x, k in (("roots", roots, 12), ("point_counts", point_counts, 1), ("root_uvs", root_uvs, 2),
+                       ("primitive_ids", primitive_ids, 1), ("random_prefixes", random_prefixes, 1),
+                       ("columns", columns, column_count)):
+        if x is not None and (x.numel() < n * k or not x.is_contiguous() or x.element_size() != 4):
+            raise ValueError(f"{name} must be contiguous 32-bit with {n * k} elements")
+    t = lambda x: None if x is None else x.data_ptr()
+    curves = abi.ClassicCurves(points=points.data_ptr(), point_counts=t(point_counts), roots=t(roots),
+                               root_uvs=t(root_uvs), strand_count=n, cvs_per_strand=cvs)
+    image = plan.device_image(points.device)
+    inputs = abi.ClassicRuntimeInputs(plan_image=image.data_ptr(), plan=C.addressof(plan.info),
+                                      primitive_ids=t(primitive_ids), random_prefixes=t(random_prefixes),
+                                      columns=t(columns) if column_count else None, column_count=column_count,
+                                      radius_scale=radius_scale)
+    out = outputs.descriptor()
+    check(lib().nxg_classic_runtime_apply(C.byref(curves), C.byref(inputs), C.byref(out),
+                                          C.c_void_p(workspace.data_ptr()), workspace.numel(),
+                                          _stream_handle(stream)),
+          "nxg_classic_runtime_apply")
+    return outputs

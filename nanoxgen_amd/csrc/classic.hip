@@ -44,9 +44,11 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "../../include/nanoxgen_amd.h"
 #include "block_scan.h"
+#include "expr_device.h"
 #include "internal.h"
 
 namespace nxg {
@@ -65,6 +67,7 @@ __device__ __forceinline__ F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b
 __device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ float length_squared(F3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+__device__ __forceinline__ float max_std(float a, float b) { return a < b ? b : a; }  // std::max
 
 // One spline sample shared by every strand of a call (all have n CVs): the
 // parameter and, for interior samples, the blended span and its weights, as
@@ -100,6 +103,16 @@ struct CutArgs {
     nxg_root_sample *out_roots;
     nxg_vec2 *out_uvs;
     uint32_t *kept, *status;
+    // Classic primitive expressions (nxg_classic_runtime_apply only)
+    const uint8_t *image;        // packed plan, device
+    uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
+    uint32_t present, has_cut, final_length, profile;
+    const uint32_t *ids, *prefixes;
+    const float *columns;
+    uint32_t column_count;
+    float description_id, name_seed;
+    struct RuntimeState *state;  // per strand, from the search to the write
+    uint32_t *err_bits;          // the NXG_CLASSIC_* bits of every failing strand
 };
 
 // A lane's CVs: staged in LDS (CV-major; ROTATED: row c turned by c lanes),
@@ -348,9 +361,12 @@ __global__ void __launch_bounds__(kScanThreads) cut_scan_groups_kernel(CutArgs a
         flags |= all_flags;
     }
     if (threadIdx.x == 0u) {
-        *a.status_word = flags;
-        if (a.status) *a.status = flags;
-        *a.kept = flags ? 0u : carry;
+        // the runtime path marks failing tiles with one flag and keeps the
+        // bits themselves in err_bits
+        const uint32_t bits = flags && a.err_bits ? *a.err_bits : flags;
+        *a.status_word = bits;
+        if (a.status) *a.status = bits;
+        *a.kept = bits ? 0u : carry;
     }
 }
 
@@ -442,6 +458,334 @@ __global__ void __launch_bounds__(kWave) cut_write_held_kernel(CutArgs a) {
     if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
 }
 
+// ---- Classic primitive expressions ---------------------------------------
+// The plan's programs run per strand in the search (length, width, the cut
+// amount, taper, taperStart, and widthRamp at every CV for its error
+// checks) and widthRamp again per CV in the write, which produces the radii.
+
+// What the search hands the write for a strand.
+struct RuntimeState {
+    float length_scale;  // the length expression's value (1 without one)
+    float c_length;      // cLength after the effects, as the post-effect programs read it
+    float c_width;
+    float taper, taper_start;
+    float reserved[3];
+};
+
+constexpr uint32_t kClassicSlots = NXG_CLASSIC_MAX_PROGRAM_SLOTS;
+
+// The per-strand context of the Classic binding (input_value and the float
+// context, src/xgen_classic_runtime.cpp:504-574, 1628-1640).
+struct ClassicContext {
+    float id;
+    ExprContext expr;
+    const float *row;  // the strand's input row (columns)
+};
+
+__device__ __forceinline__ ClassicContext classic_context(const CutArgs &a, uint32_t s) {
+    ClassicContext c;
+    c.id = static_cast<float>(a.ids ? a.ids[s] : s);
+    const nxg_root_sample r = a.roots[s];
+    const uint32_t face = r.surface_face_id == NXG_INVALID_INDEX ? r.triangle_index : r.surface_face_id;
+    c.expr.u = r.uv.x;
+    c.expr.v = r.uv.y;
+    // xgen_runtime_face_seed (src/xgen_expression.cpp:724-736)
+    c.expr.face_seed =
+        hash_to_unit(hash_step(hash_step(hash_step(0u, a.description_id), a.name_seed), static_cast<float>(face)));
+    c.expr.t = 0.0f;
+    c.expr.random_prefix = a.prefixes ? a.prefixes[s] : 0u;
+    c.expr.has_random_prefix = a.prefixes != nullptr;
+    c.row = a.columns ? a.columns + uint64_t{s} * a.column_count : nullptr;
+    return c;
+}
+
+// Program k of the plan for one strand (evaluate_runtime_expression,
+// src/xgen_classic_runtime.cpp:555-574): false where the reference throws
+// (a non-finite context, input or value).  Input i sits in value slot i; the
+// slots are a register array indexed with wave-uniform slot numbers.
+struct Evaluated {
+    float value;
+    bool ok;
+};
+__device__ Evaluated classic_program(const uint8_t *plan_image, uint32_t program_offset, ClassicContext c,
+                                     float c_length, float c_width, float t) {
+    const Image image = (Image)plan_image;
+    ImageOf(PackedProgram) prog = (ImageOf(PackedProgram))(image + program_offset);
+    ImageOf(PackedInstruction) ins = (ImageOf(PackedInstruction))(image + prog->instructions_offset);
+    ImageOf(uint32_t) operand_slots = (ImageOf(uint32_t))(image + prog->operands_offset);
+    ImageOf(nxg_expr_ramp) ramps = (ImageOf(nxg_expr_ramp))(image + prog->ramps_offset);
+    ImageOf(nxg_expr_ramp_point) points = (ImageOf(nxg_expr_ramp_point))(image + prog->ramp_points_offset);
+    ExprContext ctx = c.expr;
+    ctx.t = t;
+    bool bad = !isfinite(ctx.u) || !isfinite(ctx.v) || !isfinite(ctx.face_seed) || !isfinite(t);
+    float regs[kClassicSlots];
+#define SLOT_GET(x) regs[__builtin_amdgcn_readfirstlane(x) & (kClassicSlots - 1u)]
+    const uint32_t inputs = prog->input_count;
+    for (uint32_t i = 0; i < inputs; ++i) {
+        const uint32_t b = prog->input_bindings[i];
+        const float v = b == NXG_CLASSIC_VAR_ID        ? c.id
+                        : b == NXG_CLASSIC_VAR_CLENGTH ? c_length
+                        : b == NXG_CLASSIC_VAR_CWIDTH  ? c_width
+                                                       : c.row[b - NXG_CLASSIC_VAR_COLUMN0];
+        if (!isfinite(v)) bad = true;
+        SLOT_GET(i) = v;
+    }
+    const uint32_t n = prog->instruction_count;
+    PackedInstruction next = load_instruction(ins);
+    for (uint32_t i = 0; i < n; ++i) {
+        const PackedInstruction in = next;
+        next = load_instruction(ins + i + 1u);
+#define OPERAND(e) (((e) & kImmediate) ? (((e) & 1u) ? in.immediate1 : in.immediate) : SLOT_GET(e))
+#define EXPR_OPV(j) OPERAND(uint32_t{in.slot[j]})
+#define EXPR_OPA(j) OPERAND(operand_slots[in.operand_offset + (j)])
+#define EXPR_NOISE3(x, y, z) __builtin_nanf("")  // no extension ops in a Classic plan
+#define EXPR_TEXEL(map, f, u, v) __builtin_nanf("")
+        float value = 0.0f;
+#include "expr_ops.inc"
+#undef EXPR_OPV
+#undef EXPR_OPA
+#undef EXPR_NOISE3
+#undef EXPR_TEXEL
+#undef OPERAND
+        if (!isfinite(value)) bad = true;
+        SLOT_GET(uint32_t{in.dest}) = value;
+    }
+    const float out = SLOT_GET(prog->result);
+#undef SLOT_GET
+    return {out, !bad};
+}
+
+__device__ __forceinline__ bool classic_eval(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
+                                             float c_width, float t, float &out) {
+    const Evaluated e = classic_program(a.image, a.program_offset[k], c, c_length, c_width, t);
+    out = e.value;
+    return e.ok;
+}
+
+// scale_curve (src/xgen_classic_runtime.cpp:576-585) as a view: positions
+// root + (p - root) * scale when `on` (a length expression ran), radii kept.
+template <class Curve>
+struct ScaledCurve {
+    Curve base;
+    F3 root;
+    float scale;
+    bool on;
+    __device__ __forceinline__ float4 operator[](uint32_t c) const {
+        const float4 v = base[c];
+        if (!on) return v;
+        const F3 q = root + (xyz(v) - root) * scale;
+        return make_float4(q.x, q.y, q.z, v.w);
+    }
+};
+
+// The curve cut_from_tip_and_rebuild leaves (src/xgen_classic_runtime.cpp:
+// 729-742) as a view: CV i at parameter i cut / (n - 1), every CV at the
+// root when the walk removed the whole strand (cut 0).
+template <class Curve>
+struct RebuiltCurve {
+    Curve source;
+    uint32_t n;
+    float cut;
+    __device__ __forceinline__ float4 operator[](uint32_t i) const {
+        if (cut <= 0.0f || i == 0u) return source[0];
+        const F3 q = curve_eval(source, n, cut / static_cast<float>(n - 1u) * static_cast<float>(i));
+        return make_float4(q.x, q.y, q.z, 0.0f);
+    }
+};
+
+__device__ __forceinline__ bool has_program(const CutArgs &a, uint32_t k) { return (a.present >> k) & 1u; }
+
+// The taper part of the width profile at t (src/xgen_classic_runtime.cpp:
+// 1842-1848).
+__device__ __forceinline__ float taper_scale(float t, float taper, float taper_start) {
+    float scale = 1.0f;
+    if (t > taper_start && taper_start < 1.0f) scale *= 1.0f - taper * ((t - taper_start) / (1.0f - taper_start));
+    return scale;
+}
+
+// One strand of the runtime chain (src/xgen_classic_runtime.cpp:1619-1858):
+// the cut parameter of a kept strand (1 without a CutFX), -1 for a culled
+// one; *err gets the NXG_CLASSIC_* bit where the reference throws.
+template <class Curve>
+__device__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t s, uint32_t *err) {
+    const uint32_t n = a.cvs;
+    if (a.counts && a.counts[s] != n) {
+        *err = NXG_CLASSIC_VARIABLE_COUNTS;
+        return -1.0f;
+    }
+    const ClassicContext c = classic_context(a, s);
+    float c_length = spline_length(a.length_samples, source, n);
+    float length_scale = 1.0f;
+    if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) {
+        if (!classic_eval(a, NXG_CLASSIC_EXPR_LENGTH, c, c_length, 0.0f, 0.0f, length_scale)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return -1.0f;
+        }
+        if (!isfinite(length_scale) || length_scale < 0.0f) {
+            *err = NXG_CLASSIC_NEGATIVE_LENGTH;
+            return -1.0f;
+        }
+        c_length *= length_scale;
+    }
+    const ScaledCurve<Curve> p{source, xyz(source[0]), length_scale, has_program(a, NXG_CLASSIC_EXPR_LENGTH)};
+    float width;
+    if (has_program(a, NXG_CLASSIC_EXPR_WIDTH)) {
+        if (!classic_eval(a, NXG_CLASSIC_EXPR_WIDTH, c, c_length, 0.0f, 0.0f, width)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return -1.0f;
+        }
+    } else {
+        width = a.radius_scale > 0.0f ? 2.0f * source[0].w / a.radius_scale : 0.0f;
+    }
+    if (!isfinite(width)) {
+        *err = NXG_CLASSIC_NONFINITE_WIDTH;
+        return -1.0f;
+    }
+    if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
+    const float c_width = width;
+    float cut = 1.0f;
+    bool keep = true;
+    if (a.has_cut) {
+        float amount;
+        if (!classic_eval(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0.0f, amount)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return -1.0f;
+        }
+        if (!isfinite(amount)) {
+            *err = NXG_CLASSIC_NONFINITE_AMOUNT;
+            return -1.0f;
+        }
+        cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
+        keep = cut >= 1.0e-4f;
+        // the post-effect programs read the rebuilt curve's cLength
+        if (a.final_length) c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
+    }
+    float taper = 0.0f, taper_start = 0.0f;
+    if (has_program(a, NXG_CLASSIC_EXPR_TAPER) &&
+        !classic_eval(a, NXG_CLASSIC_EXPR_TAPER, c, c_length, c_width, 0.0f, taper)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return -1.0f;
+    }
+    if (has_program(a, NXG_CLASSIC_EXPR_TAPER_START) &&
+        !classic_eval(a, NXG_CLASSIC_EXPR_TAPER_START, c, c_length, c_width, 0.0f, taper_start)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return -1.0f;
+    }
+    if (!isfinite(taper) || !isfinite(taper_start)) {
+        *err = NXG_CLASSIC_NONFINITE_TAPER;
+        return -1.0f;
+    }
+    if (a.profile) {  // the width profile's errors (its values are the write's)
+        for (uint32_t cv = 0u; cv < n; ++cv) {
+            const float t = static_cast<float>(cv) / static_cast<float>(n - 1u);
+            float scale = taper_scale(t, taper, taper_start);
+            if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
+                float ramp;
+                if (!classic_eval(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, c_length, c_width, t, ramp)) {
+                    *err = NXG_CLASSIC_EXPRESSION_ERROR;
+                    return -1.0f;
+                }
+                scale *= ramp;
+            }
+            if (!isfinite(c_width * scale)) {
+                *err = NXG_CLASSIC_NONFINITE_PROFILE;
+                return -1.0f;
+            }
+        }
+    }
+    RuntimeState st{};
+    st.length_scale = length_scale;
+    st.c_length = c_length;
+    st.c_width = c_width;
+    st.taper = taper;
+    st.taper_start = taper_start;
+    a.state[s] = st;
+    return keep ? cut : -1.0f;
+}
+
+template <bool STAGED>
+__global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    if constexpr (STAGED) stage_tile<true>(a, tile, first, count);
+    const uint32_t s = first + lane;
+    float cut = -1.0f;
+    uint32_t err = 0u;
+    if (lane < count) {
+        if constexpr (STAGED)
+            cut = runtime_strand(a, StagedCurve<true>{tile, lane}, s, &err);
+        else
+            cut = runtime_strand(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
+        a.cut[s] = cut;
+    }
+    const uint64_t kept = __ballot(cut >= 0.0f);
+    uint32_t bits = 0u;
+#pragma unroll
+    for (uint32_t b = 0u; b < 8u; ++b)
+        if (__ballot((err >> b) & 1u)) bits |= 1u << b;
+    if (lane == 0u) {
+        a.tiles[t] = static_cast<uint32_t>(__popcll(kept)) | (bits ? 1u << kErrShift : 0u);
+        if (bits) atomicOr(a.err_bits, bits);
+    }
+}
+
+// The kept strands at their compacted slots: the scaled (and, with a CutFX,
+// rebuilt) positions, and radii from the width profile when the plan has
+// one (src/xgen_classic_runtime.cpp:1836-1858), else the input radii.
+template <bool STAGED>
+__global__ void __launch_bounds__(kWave) runtime_write_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    const uint32_t s = first + lane;
+    const float cut = lane < count ? a.cut[s] : -1.0f;
+    const uint64_t kept = __ballot(cut >= 0.0f);
+    if (kept == 0u) return;  // the block is this one wavefront
+    if constexpr (STAGED) stage_tile<false>(a, tile, first, count);
+    if (!(cut >= 0.0f)) return;
+    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
+    const uint32_t slot = a.groups[t / kScanBlock] + a.tiles[t] + rank;
+    const RuntimeState st = a.state[s];
+    const ClassicContext c = classic_context(a, s);
+    float4 *dst = a.out_points + uint64_t{slot} * n;
+    auto write = [&](const auto &source) {
+        using Curve = typename std::decay<decltype(source)>::type;
+        const ScaledCurve<Curve> p{source, xyz(source[0]), st.length_scale, has_program(a, NXG_CLASSIC_EXPR_LENGTH)};
+        const float step = cut / static_cast<float>(n - 1u);
+        for (uint32_t i = 0u; i < n; ++i) {
+            float4 q = p[i];
+            if (a.has_cut && i > 0u) {
+                const F3 r = curve_eval(p, n, step * static_cast<float>(i));
+                q = make_float4(r.x, r.y, r.z, q.w);
+            }
+            if (a.profile) {
+                const float tt = static_cast<float>(i) / static_cast<float>(n - 1u);
+                float scale = taper_scale(tt, st.taper, st.taper_start);
+                if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
+                    float ramp = 0.0f;
+                    classic_eval(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, tt, ramp);
+                    scale *= ramp;
+                }
+                q.w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
+            }
+            dst[i] = q;
+        }
+    };
+    if constexpr (STAGED)
+        write(StagedCurve<false>{tile, lane});
+    else
+        write(MemoryCurve{a.points + uint64_t{s} * n});
+    if (a.out_counts) a.out_counts[slot] = n;
+    if (a.out_roots) a.out_roots[slot] = a.roots[s];
+    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
+}
+
 // add_xgen_classic_renderer_endpoints (src/xgen_classic_runtime.cpp:
 // 1921-1956): one 256-thread block per 64-strand tile, each thread one
 // output point of the tile's contiguous 64 x (n + 2) run (input and output
@@ -522,8 +866,6 @@ __device__ __forceinline__ F3 normalize3(F3 v) {  // generate.h:67-70
     const float l2 = length_squared(v);
     return l2 > 1.0e-20f ? F3{v.x / sqrtf(l2), v.y / sqrtf(l2), v.z / sqrtf(l2)} : F3{0.0f, 1.0f, 0.0f};
 }
-__device__ __forceinline__ float max_std(float a, float b) { return a < b ? b : a; }
-__device__ __forceinline__ float clamp_std(float x, float lo, float hi) { return x < lo ? lo : (hi < x ? hi : x); }
 
 // xgen_classic_noise_float (src/xgen_classic_runtime.cpp:802-837)
 __device__ float classic_noise(float sx, float sy, float sz) {
@@ -792,6 +1134,57 @@ __global__ void __launch_bounds__(kWave) clump_strand_kernel(ClumpArgs a) {
 }  // namespace
 }  // namespace nxg
 
+namespace nxg {
+namespace {
+
+// The cut walk's parameter sequence (1 - step, repeatedly, clamped at 0) up
+// to its first value <= 1e-10, counted in the same float steps.
+uint32_t walk_count(uint32_t cvs) {
+    const float walk_step = 1.0f / (2.0f * static_cast<float>(cvs) + 4.0f);
+    float previous = 1.0f;
+    uint32_t count = 0u;
+    while (count < kClassicTableEntries) {
+        const float lowered = previous - walk_step;
+        previous = lowered < 0.0f ? 0.0f : lowered;
+        ++count;
+        if (previous <= 1.0e-10f) break;
+    }
+    return count;
+}
+
+// The arguments every chain shares: inputs, outputs and workspace regions.
+CutArgs chain_args(const nxg_classic_curves *curves, const nxg_classic_output *output, uint8_t *ws,
+                   const ClassicLayout &layout) {
+    const uint32_t n = curves->strand_count, cvs = curves->cvs_per_strand;
+    CutArgs a{};
+    a.points = reinterpret_cast<const float4 *>(curves->points);
+    a.counts = curves->point_counts;
+    a.roots = curves->roots;
+    a.uvs = curves->root_uvs;
+    a.strands = n;
+    a.cvs = cvs;
+    a.div_magic = static_cast<uint32_t>(0xffffffffu / cvs + 1u);
+    a.cut = reinterpret_cast<float *>(ws + layout.cut_offset);
+    a.tiles = reinterpret_cast<uint32_t *>(ws + layout.tiles_offset);
+    a.groups = reinterpret_cast<uint32_t *>(ws + layout.groups_offset);
+    a.status_word = reinterpret_cast<uint32_t *>(ws + layout.status_offset);
+    a.tile_count = classic_tiles(n);
+    a.group_count = classic_groups(n);
+    a.length_samples = reinterpret_cast<Sample *>(ws + layout.table_offset);
+    a.walk_samples = a.length_samples + kClassicTableEntries;
+    a.walk_count = walk_count(cvs);
+    a.out_points = reinterpret_cast<float4 *>(output->points);
+    a.out_counts = output->point_counts;
+    a.out_roots = output->roots;
+    a.out_uvs = output->root_uvs;
+    a.kept = output->kept_count;
+    a.status = output->status;
+    return a;
+}
+
+}  // namespace
+}  // namespace nxg
+
 extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, const nxg_classic_cut_params *cut,
                                           const nxg_classic_output *output, void *workspace,
                                           uint64_t workspace_bytes, void *stream_ptr) {
@@ -802,47 +1195,12 @@ extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, cons
     if (!workspace || workspace_bytes < layout.total || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
         return NXG_ERROR_WORKSPACE_TOO_SMALL;
     uint8_t *ws = static_cast<uint8_t *>(workspace);
-    CutArgs a{};
-    a.points = reinterpret_cast<const float4 *>(curves->points);
-    a.counts = curves->point_counts;
-    a.roots = curves->roots;
-    a.uvs = curves->root_uvs;
-    a.strands = n;
-    a.cvs = cvs;
-    a.div_magic = static_cast<uint32_t>(0xffffffffu / cvs + 1u);
+    CutArgs a = chain_args(curves, output, ws, layout);
     a.scale = cut->amount_scale;
     a.offset = cut->amount_offset;
     a.radius_scale = cut->radius_scale;
     a.multiplier = cut->multiplier;
     a.multiplier_stride = cut->multiplier_stride;
-    a.cut = reinterpret_cast<float *>(ws + layout.cut_offset);
-    a.tiles = reinterpret_cast<uint32_t *>(ws + layout.tiles_offset);
-    a.groups = reinterpret_cast<uint32_t *>(ws + layout.groups_offset);
-    a.status_word = reinterpret_cast<uint32_t *>(ws + layout.status_offset);
-    a.tile_count = classic_tiles(n);
-    a.group_count = classic_groups(n);
-    a.length_samples = reinterpret_cast<Sample *>(ws + layout.table_offset);
-    a.walk_samples = a.length_samples + kClassicTableEntries;
-    // The walk's parameter sequence (1 - step, repeatedly, clamped at 0) up
-    // to its first value <= 1e-10, counted here in the same float steps.
-    {
-        const float walk_step = 1.0f / (2.0f * static_cast<float>(cvs) + 4.0f);
-        float previous = 1.0f;
-        uint32_t count = 0u;
-        while (count < kClassicTableEntries) {
-            const float lowered = previous - walk_step;
-            previous = lowered < 0.0f ? 0.0f : lowered;
-            ++count;
-            if (previous <= 1.0e-10f) break;
-        }
-        a.walk_count = count;
-    }
-    a.out_points = reinterpret_cast<float4 *>(output->points);
-    a.out_counts = output->point_counts;
-    a.out_roots = output->roots;
-    a.out_uvs = output->root_uvs;
-    a.kept = output->kept_count;
-    a.status = output->status;
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
     const bool staged = cvs <= kStagedCvs;
     const size_t tile_lds = staged ? size_t{kWave} * cvs * sizeof(float4) : 0u;
@@ -867,6 +1225,57 @@ extern "C" uint32_t nxg_classic_cut_apply(const nxg_classic_curves *curves, cons
             hipLaunchKernelGGL(cut_write_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
             hipLaunchKernelGGL(cut_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
+    }
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
+extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
+                                              const nxg_classic_runtime_inputs *inputs,
+                                              const nxg_classic_output *output, void *workspace,
+                                              uint64_t workspace_bytes, void *stream_ptr) {
+    using namespace nxg;
+    if (uint32_t s = nxg_validate_classic_runtime(curves, inputs, output); s != NXG_OK) return s;
+    const uint32_t n = curves->strand_count, cvs = curves->cvs_per_strand;
+    const ClassicLayout layout = classic_layout(n);
+    if (!workspace || workspace_bytes < classic_runtime_bytes(n) || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
+        return NXG_ERROR_WORKSPACE_TOO_SMALL;
+    uint8_t *ws = static_cast<uint8_t *>(workspace);
+    const nxg_classic_plan_info &plan = *inputs->plan;
+    CutArgs a = chain_args(curves, output, ws, layout);
+    a.radius_scale = inputs->radius_scale;
+    a.image = static_cast<const uint8_t *>(inputs->plan_image);
+    for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k) a.program_offset[k] = plan.program_offset[k];
+    a.present = plan.present;
+    a.has_cut = plan.has_cut;
+    a.final_length = plan.final_length;
+    a.profile = (plan.present & ((1u << NXG_CLASSIC_EXPR_WIDTH) | (1u << NXG_CLASSIC_EXPR_TAPER) |
+                                 (1u << NXG_CLASSIC_EXPR_TAPER_START) | (1u << NXG_CLASSIC_EXPR_WIDTH_RAMP))) != 0u;
+    a.ids = inputs->primitive_ids;
+    a.prefixes = inputs->random_prefixes;
+    a.columns = inputs->columns;
+    a.column_count = inputs->column_count;
+    a.description_id = static_cast<float>(plan.description_id);
+    a.name_seed = static_cast<float>(plan.description_name_seed);
+    a.state = reinterpret_cast<RuntimeState *>(ws + classic_runtime_state_offset(n));
+    a.err_bits = a.status_word + 1;
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    const bool staged = cvs <= kStagedCvs;
+    const size_t tile_lds = staged ? size_t{kWave} * cvs * sizeof(float4) : 0u;
+    if (a.tile_count) {
+        hipLaunchKernelGGL(cut_table_kernel, dim3(1), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, a.err_bits);
+        if (staged)
+            hipLaunchKernelGGL(runtime_search_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
+        else
+            hipLaunchKernelGGL(runtime_search_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
+        hipLaunchKernelGGL(cut_scan_tiles_kernel, dim3(a.group_count), dim3(kScanThreads), 0, stream, a);
+    }
+    hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
+    if (a.tile_count) {
+        if (staged)
+            hipLaunchKernelGGL(runtime_write_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
+        else
+            hipLaunchKernelGGL(runtime_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
     }
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }

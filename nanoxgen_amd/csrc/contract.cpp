@@ -246,3 +246,28 @@ uint32_t nxg_validate_classic_cut(const nxg_classic_curves *curves, const nxg_cl
 }
 
 uint64_t nxg_classic_cut_workspace_size(uint32_t strand_count) { return nxg::classic_layout(strand_count).total; }
+
+uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nxg_classic_runtime_inputs *inputs,
+                                      const nxg_classic_output *output) {
+    if (!curves || !inputs || !output || !inputs->plan) return NXG_ERROR_NULL_ARGUMENT;
+    const nxg_classic_plan_info &plan = *inputs->plan;
+    const float rs = inputs->radius_scale;
+    if (!std::isfinite(rs) || rs < 0.0f) return NXG_ERROR_INVALID_RADIUS_SCALE;
+    const nxg_classic_cut_params common{0.0f, 0.0f, rs, 0u, nullptr};
+    if (uint32_t st = nxg_validate_classic_cut(curves, &common, output); st != NXG_OK) return st;
+    if (plan.has_cut && !(plan.present & (1u << NXG_CLASSIC_EXPR_CUT_AMOUNT))) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.present >> NXG_CLASSIC_EXPR_COUNT) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (inputs->column_count > NXG_CLASSIC_MAX_COLUMNS || inputs->column_count < plan.columns_required ||
+        (inputs->column_count != 0u) != (inputs->columns != nullptr))
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (curves->strand_count == 0u) return NXG_OK;
+    // the reference's context reads every strand's root and, when a program
+    // calls rand(), its SeExpr prefix (src/xgen_classic_runtime.cpp:1432-1491)
+    if (!curves->roots) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.needs_random_prefix && !inputs->random_prefixes) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.present && (!inputs->plan_image || reinterpret_cast<uintptr_t>(inputs->plan_image) % 16u != 0u))
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    return NXG_OK;
+}
+
+uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count) { return nxg::classic_runtime_bytes(strand_count); }

@@ -39,12 +39,16 @@ int arity(uint32_t op) {
 
 }  // namespace
 
-bool validate_program(const nxg_expr_program &p) {
+namespace {
+
+// The checks shared by both binding spaces: input bindings below
+// binding_limit.
+bool validate_bounded(const nxg_expr_program &p, uint32_t binding_limit) {
     if (p.instruction_count == 0u || p.instruction_count > NXG_EXPR_MAX_VALUES) return false;
     if (!p.instructions || (p.operand_count && !p.operands)) return false;
     if (p.input_count > kMaxInputs || (p.input_count && !p.input_bindings)) return false;
     for (uint32_t i = 0; i < p.input_count; ++i)
-        if (p.input_bindings[i] >= NXG_ROOT_VAR_COUNT) return false;
+        if (p.input_bindings[i] >= binding_limit) return false;
     if ((p.ramp_count && !p.ramps) || (p.ramp_point_count && !p.ramp_points)) return false;
     for (uint32_t r = 0; r < p.ramp_count; ++r) {
         const nxg_expr_ramp ramp = p.ramps[r];
@@ -70,7 +74,23 @@ bool validate_program(const nxg_expr_program &p) {
     return p.result < p.instruction_count;
 }
 
+}  // namespace
+
+bool validate_program(const nxg_expr_program &p) { return validate_bounded(p, NXG_ROOT_VAR_COUNT); }
+
 namespace {
+
+// A Classic primitive program: NXG_CLASSIC_VAR_* bindings and the
+// reference's ops only (the Classic runtime has no extension ops).
+bool validate_classic_program(const nxg_expr_program &p) {
+    if (!validate_bounded(p, NXG_CLASSIC_VAR_COLUMN0 + NXG_CLASSIC_MAX_COLUMNS)) return false;
+    if (p.input_count > NXG_CLASSIC_MAX_PROGRAM_INPUTS) return false;
+    for (uint32_t i = 0; i < p.instruction_count; ++i) {
+        const uint32_t op = p.instructions[i].op;
+        if (op == NXG_OP_POW || op == NXG_OP_MIX || op == NXG_OP_NOISE3 || op == NXG_OP_TEXEL) return false;
+    }
+    return true;
+}
 
 uint64_t align16(uint64_t v) { return (v + 15u) & ~uint64_t{15u}; }
 
@@ -104,7 +124,10 @@ struct PackPlan {
 
 bool is_finite(float v) { return std::isfinite(v); }
 
-PackPlan plan_program(const nxg_expr_program &p) {
+// classic: input i lives in slot i (the interpreter loads the program's
+// inputs before it runs), so the first input_count slots are reserved;
+// otherwise inputs read the root variables' reserved slots.
+PackPlan plan_program(const nxg_expr_program &p, bool classic) {
     const uint32_t n = p.instruction_count;
     PackPlan plan;
     plan.dispatched.assign(n, true);
@@ -142,7 +165,7 @@ PackPlan plan_program(const nxg_expr_program &p) {
     for (uint32_t i = 0; i < n; ++i) {
         if (ins(i).op == NXG_OP_INPUT) {
             plan.dispatched[i] = false;
-            plan.var_mask |= 1u << p.input_bindings[ins(i).auxiliary];
+            if (!classic) plan.var_mask |= 1u << p.input_bindings[ins(i).auxiliary];
         } else if (inline_const[i]) {
             plan.dispatched[i] = false;
         }
@@ -154,11 +177,11 @@ PackPlan plan_program(const nxg_expr_program &p) {
     last_use[p.result] = n;
     std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> free_slots;
     std::vector<bool> released(n, false);
-    uint32_t next = kVarSlots;
+    uint32_t next = classic ? p.input_count : kVarSlots;
     for (uint32_t i = 0; i < n; ++i) {
         const nxg_expr_instruction &in = ins(i);
         if (in.op == NXG_OP_INPUT) {
-            plan.slot_of[i] = static_cast<uint16_t>(p.input_bindings[in.auxiliary]);
+            plan.slot_of[i] = static_cast<uint16_t>(classic ? in.auxiliary : p.input_bindings[in.auxiliary]);
             continue;
         }
         if (!plan.dispatched[i]) continue;
@@ -206,6 +229,56 @@ PackPlan plan_program(const nxg_expr_program &p) {
     }
     return plan;
 }
+
+// Writes one planned program at byte `at` of the image (its header, the
+// dispatched instructions, operand slots and ramps); returns the offset
+// after it.
+uint64_t write_program(uint8_t *base, uint64_t at, const nxg_expr_program &p, const PackPlan &plan) {
+    PackedProgram pp{};
+    pp.instruction_count = plan.dispatched_count;
+    pp.result = plan.slot_of[p.result];
+    pp.input_count = p.input_count;
+    pp.slot_count = plan.slot_count;
+    pp.var_mask = plan.var_mask;
+    for (uint32_t i = 0; i < p.input_count; ++i) pp.input_bindings[i] = p.input_bindings[i];
+    uint64_t cursor = at + align16(sizeof(PackedProgram));
+    pp.instructions_offset = static_cast<uint32_t>(cursor);
+    cursor += align16(uint64_t{p.instruction_count + 1u} * sizeof(PackedInstruction));
+    pp.operands_offset = static_cast<uint32_t>(cursor);
+    cursor += align16(uint64_t{p.operand_count} * 4u);
+    pp.ramps_offset = static_cast<uint32_t>(cursor);
+    cursor += align16(uint64_t{p.ramp_count} * sizeof(nxg_expr_ramp));
+    pp.ramp_points_offset = static_cast<uint32_t>(cursor);
+    cursor += align16(uint64_t{p.ramp_point_count} * sizeof(nxg_expr_ramp_point));
+    std::memcpy(base + at, &pp, sizeof(pp));
+    PackedInstruction *ins = reinterpret_cast<PackedInstruction *>(base + pp.instructions_offset);
+    uint32_t *operand_slots = reinterpret_cast<uint32_t *>(base + pp.operands_offset);
+    uint32_t d = 0;
+    for (uint32_t i = 0; i < p.instruction_count; ++i) {
+        if (!plan.dispatched[i]) continue;
+        const nxg_expr_instruction &in = p.instructions[i];
+        PackedInstruction &o = ins[d++];
+        o.op = in.op;
+        o.operand_count = in.operand_count;
+        o.dest = plan.slot_of[i];
+        o.operand_offset = in.operand_offset;
+        o.auxiliary = in.auxiliary;
+        o.immediate = plan.imm0[i];
+        o.immediate1 = plan.imm1[i];
+        for (uint32_t j = 0; j < in.operand_count; ++j) {
+            const uint16_t e = plan.operand[i][j];
+            if (in.operand_count <= 4u) o.slot[j] = e;
+            operand_slots[in.operand_offset + j] = e;
+        }
+    }
+    if (p.ramp_count)
+        std::memcpy(base + pp.ramps_offset, p.ramps, uint64_t{p.ramp_count} * sizeof(nxg_expr_ramp));
+    if (p.ramp_point_count)
+        std::memcpy(base + pp.ramp_points_offset, p.ramp_points,
+                    uint64_t{p.ramp_point_count} * sizeof(nxg_expr_ramp_point));
+    return cursor;
+}
+
 }  // namespace
 }  // namespace nxg
 
@@ -231,7 +304,7 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
             if (in.op == NXG_OP_TEXEL && in.auxiliary + 1u > maps_required) maps_required = in.auxiliary + 1u;
         }
         total += program_bytes(programs[k]);
-        plans.push_back(plan_program(programs[k]));
+        plans.push_back(plan_program(programs[k], false));
         if (plans.back().slot_count > max_values) max_values = plans.back().slot_count;
     }
     if (!info) return NXG_ERROR_NULL_ARGUMENT;
@@ -250,55 +323,54 @@ uint32_t nxg_expr_pack(const nxg_expr_program *programs, uint32_t count, void *o
         const nxg_expr_program &p = programs[k];
         const PackPlan &plan = plans[k];
         set.program_offset[k] = static_cast<uint32_t>(at);
-        PackedProgram pp{};
-        pp.instruction_count = plan.dispatched_count;
-        pp.result = plan.slot_of[p.result];
-        pp.input_count = p.input_count;
-        pp.slot_count = plan.slot_count;
-        pp.var_mask = plan.var_mask;
-        for (uint32_t i = 0; i < p.input_count; ++i) pp.input_bindings[i] = p.input_bindings[i];
-        uint64_t cursor = at + align16(sizeof(PackedProgram));
-        pp.instructions_offset = static_cast<uint32_t>(cursor);
-        cursor += align16(uint64_t{p.instruction_count + 1u} * sizeof(PackedInstruction));
-        pp.operands_offset = static_cast<uint32_t>(cursor);
-        cursor += align16(uint64_t{p.operand_count} * 4u);
-        pp.ramps_offset = static_cast<uint32_t>(cursor);
-        cursor += align16(uint64_t{p.ramp_count} * sizeof(nxg_expr_ramp));
-        pp.ramp_points_offset = static_cast<uint32_t>(cursor);
-        cursor += align16(uint64_t{p.ramp_point_count} * sizeof(nxg_expr_ramp_point));
-        std::memcpy(base + at, &pp, sizeof(pp));
-        PackedInstruction *ins = reinterpret_cast<PackedInstruction *>(base + pp.instructions_offset);
-        uint32_t *operand_slots = reinterpret_cast<uint32_t *>(base + pp.operands_offset);
-        uint32_t d = 0;
-        for (uint32_t i = 0; i < p.instruction_count; ++i) {
-            if (!plan.dispatched[i]) continue;
-            const nxg_expr_instruction &in = p.instructions[i];
-            PackedInstruction &o = ins[d++];
-            o.op = in.op;
-            o.operand_count = in.operand_count;
-            o.dest = plan.slot_of[i];
-            o.operand_offset = in.operand_offset;
-            o.auxiliary = in.auxiliary;
-            o.immediate = plan.imm0[i];
-            o.immediate1 = plan.imm1[i];
-            for (uint32_t j = 0; j < in.operand_count; ++j) {
-                const uint16_t e = plan.operand[i][j];
-                if (in.operand_count <= 4u) o.slot[j] = e;
-                operand_slots[in.operand_offset + j] = e;
-            }
-        }
-        if (p.ramp_count)
-            std::memcpy(base + pp.ramps_offset, p.ramps, uint64_t{p.ramp_count} * sizeof(nxg_expr_ramp));
-        if (p.ramp_point_count)
-            std::memcpy(base + pp.ramp_points_offset, p.ramp_points,
-                        uint64_t{p.ramp_point_count} * sizeof(nxg_expr_ramp_point));
-        at = cursor;
+        at = write_program(base, at, p, plan);
     }
     std::memcpy(base, &set, sizeof(set));
     info->program_count = set.program_count;
     info->max_values = set.max_values;
     info->texel_maps_required = maps_required;
     for (uint32_t k = 0; k < NXG_EXPR_MAX_PROGRAMS; ++k) info->program_offset[k] = set.program_offset[k];
+    return NXG_OK;
+}
+
+uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t capacity,
+                               nxg_classic_plan_info *info) {
+    using namespace nxg;
+    if (!plan || !info) return NXG_ERROR_NULL_ARGUMENT;
+    if (plan->has_cut && !plan->expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT]) return NXG_ERROR_INVALID_PROGRAM;
+    nxg_classic_plan_info r{};
+    r.has_cut = plan->has_cut ? 1u : 0u;
+    r.description_id = plan->description_id;
+    r.description_name_seed = plan->description_name_seed;
+    uint64_t total = 0u;
+    PackPlan plans[NXG_CLASSIC_EXPR_COUNT];
+    for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k) {
+        const nxg_expr_program *p = plan->expressions[k];
+        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut)) continue;
+        if (!validate_classic_program(*p)) return NXG_ERROR_INVALID_PROGRAM;
+        plans[k] = plan_program(*p, true);
+        if (plans[k].slot_count > NXG_CLASSIC_MAX_PROGRAM_SLOTS) return NXG_ERROR_INVALID_PROGRAM;
+        r.present |= 1u << k;
+        r.program_offset[k] = static_cast<uint32_t>(total);
+        total += program_bytes(*p);
+        for (uint32_t i = 0; i < p->input_count; ++i) {
+            const uint32_t b = p->input_bindings[i];
+            if (b >= NXG_CLASSIC_VAR_COLUMN0 && b - NXG_CLASSIC_VAR_COLUMN0 + 1u > r.columns_required)
+                r.columns_required = b - NXG_CLASSIC_VAR_COLUMN0 + 1u;
+            if (b == NXG_CLASSIC_VAR_CLENGTH &&
+                (k == NXG_CLASSIC_EXPR_TAPER || k == NXG_CLASSIC_EXPR_TAPER_START || k == NXG_CLASSIC_EXPR_WIDTH_RAMP))
+                r.final_length = 1u;
+        }
+        if (p->random_call_count != 0u) r.needs_random_prefix = 1u;
+    }
+    r.byte_size = total ? total : 16u;
+    *info = r;
+    if (!out) return NXG_OK;
+    if (capacity < r.byte_size) return NXG_ERROR_BUFFER_TOO_SMALL;
+    uint8_t *base = static_cast<uint8_t *>(out);
+    std::memset(base, 0, r.byte_size);
+    for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k)
+        if (r.present & (1u << k)) write_program(base, r.program_offset[k], *plan->expressions[k], plans[k]);
     return NXG_OK;
 }
 

@@ -26,6 +26,7 @@
 #include <cstring>
 
 #include "../../include/nanoxgen_amd.h"
+#include "expr_device.h"
 #include "expr_format.h"
 #include "internal.h"
 
@@ -1049,103 +1050,15 @@ __device__ float gradient_noise_exact(float sx, float sy, float sz, const float4
     return 0.5f * v[0] + 0.5f;
 }
 
-__constant__ uint8_t kPermutation[256] = {
-#include "seexpr_permutation.inc"
-};
-
-// xgen_seexpr_hash_finish (xgen_expression.cpp:637-651)
-__device__ uint32_t seexpr_hash_finish(uint32_t state) {
-    state ^= state >> 11u;
-    state ^= (state << 7u) & 0x9d2c5680u;
-    state ^= (state << 15u) & 0xefc60000u;
-    state ^= state >> 18u;
-    const uint32_t b3 = kPermutation[state & 0xffu];
-    const uint32_t b2 = kPermutation[((state >> 8u) + b3) & 0xffu];
-    const uint32_t b1 = kPermutation[((state >> 16u) + b2) & 0xffu];
-    const uint32_t b0 = kPermutation[((state >> 24u) + b1) & 0xffu];
-    return b0 | (b1 << 8u) | (b2 << 16u) | (b3 << 24u);
-}
-
-// xgen_runtime_hash_component (xgen_expression.cpp:663-709): the top 32
-// fraction bits of frexp(double(x) * 8.539734222673566) xor the exponent,
-// from a 24 x 53-bit integer product (no FP64).
-__device__ uint32_t runtime_hash_component(float argument) {
-    constexpr uint64_t cm = 0x001114580b45d474ull;
-    const uint32_t bits = __float_as_uint(argument);
-    const uint32_t absolute = bits & 0x7fffffffu;
-    if (absolute == 0u) return 0u;
-    const uint32_t fexp = absolute >> 23u;
-    const uint32_t mant = fexp == 0u ? (absolute & 0x7fffffu) : ((absolute & 0x7fffffu) | 0x800000u);
-    const uint64_t p0 = uint64_t{mant} * static_cast<uint32_t>(cm);
-    const uint64_t p1 = uint64_t{mant} * static_cast<uint32_t>(cm >> 32u);
-    const uint64_t low = p0 + (p1 << 32u);
-    const uint64_t high = (p1 >> 32u) + (low < p0 ? 1u : 0u);
-    const uint32_t length = high != 0u ? 128u - static_cast<uint32_t>(__clzll(high))
-                                       : 64u - static_cast<uint32_t>(__clzll(low));
-    const uint32_t shift = length - 32u;
-    const uint32_t top = static_cast<uint32_t>((low >> shift) | (high << (64u - shift)));
-    const uint64_t remainder = low & ((uint64_t{1} << shift) - 1u);
-    const uint64_t ceil_div = ((low >> 32u) | (high << 32u)) + ((low & 0xffffffffu) != 0u ? 1u : 0u);
-    const uint32_t fraction = top - (remainder < ceil_div ? 1u : 0u);
-    const int32_t value_exponent = fexp == 0u ? -149 : static_cast<int32_t>(fexp) - 150;
-    const uint32_t exponent = static_cast<uint32_t>(static_cast<int32_t>(length) + value_exponent - 49);
-    return ((bits >> 31u) != 0u ? 0u - fraction : fraction) ^ exponent;
-}
-
-__device__ __forceinline__ uint32_t hash_step(uint32_t state, float x) {
-    return state * 1664525u + runtime_hash_component(x) + 1013904223u;
-}
-__device__ __forceinline__ float hash_to_unit(uint32_t state) {
-    return static_cast<float>(seexpr_hash_finish(state)) * 0x1p-32f;
-}
-
-__device__ __forceinline__ float clamp_std(float v, float lo, float hi) {  // std::clamp
-    return v < lo ? lo : (hi < v ? hi : v);
-}
-
-// evaluate_ramp_points<float> (xgen_expression.cpp:559-615)
-// Pointers into the packed program image, in the constant address space.
-#define ImageOf(T) const __attribute__((address_space(4))) T *
-typedef const __attribute__((address_space(4))) uint8_t *Image;
-
-// A whole instruction as eight scalar-loaded words (structs cannot be
-// copied out of another address space directly).
-__device__ __forceinline__ PackedInstruction load_instruction(ImageOf(PackedInstruction) p) {
-    ImageOf(uint32_t) w = (ImageOf(uint32_t))p;
-    uint32_t v[sizeof(PackedInstruction) / 4u];
-#pragma unroll
-    for (uint32_t j = 0; j < sizeof(PackedInstruction) / 4u; ++j) v[j] = w[j];
-    PackedInstruction r;
-    __builtin_memcpy(&r, v, sizeof(r));
-    return r;
-}
-
-__device__ float ramp_eval(ImageOf(nxg_expr_ramp_point) pts, uint32_t n, float x) {
-    if (x <= pts[0].position) return clamp_std(pts[0].value, 0.0f, 1.0f);
-    if (x >= pts[n - 1u].position) return clamp_std(pts[n - 1u].value, 0.0f, 1.0f);
-    uint32_t upper = 1u;
-    while (upper < n && x >= pts[upper].position) ++upper;
-    const uint32_t lower = upper - 1u;
-    const float p0 = pts[lower].position, p1 = pts[upper].position;
-    const float k0 = pts[lower].value, k1 = pts[upper].value;
-    const float u = (x - p0) / (p1 - p0);
-    float value = k0;
-    const uint32_t mode = pts[lower].interpolation;
-    if (mode == 1u) {
-        value = k0 + u * (k1 - k0);
-    } else if (mode >= 2u) {
-        value = k0 * (u - 1.0f) * (u - 1.0f) * (2.0f * u + 1.0f) + k1 * u * u * (3.0f - 2.0f * u);
-        if (mode == 3u) {
-            const float scale = 1.0f / (p1 - p0);
-            const float d0 = lower == 0u ? (k1 - k0) / (scale * (2.0f * p1 - 2.0f * p0))
-                                         : (k1 - pts[lower - 1u].value) / (scale * (p1 - pts[lower - 1u].position));
-            const float d1 = upper + 1u == n ? (k1 - k0) / (scale * (2.0f * p1 - 2.0f * p0))
-                                             : (pts[upper + 1u].value - k0) / (scale * (pts[upper + 1u].position - p0));
-            value = k0 * (u - 1.0f) * (u - 1.0f) * (2.0f * u + 1.0f) + k1 * u * u * (3.0f - 2.0f * u) +
-                    d0 * u * (u - 1.0f) * (u - 1.0f) + d1 * u * u * (u - 1.0f);
-        }
-    }
-    return clamp_std(value, 0.0f, 1.0f);
+// A texel map's point sample (the generator's texel op): face
+// clamp(floor(f)), column clamp(floor(u R)), row clamp(floor(v R)).
+__device__ __forceinline__ float texel_sample(const nxg_texel_map &m, float f, float u, float v) {
+    const float R = static_cast<float>(m.resolution);
+    const float ff = clamp_std(floorf(f), 0.0f, static_cast<float>(m.face_count - 1u));
+    const float fu = clamp_std(floorf(u * R), 0.0f, R - 1.0f);
+    const float fv = clamp_std(floorf(v * R), 0.0f, R - 1.0f);
+    return m.texels[(static_cast<uint64_t>(ff) * m.resolution + static_cast<uint32_t>(fv)) * m.resolution +
+                    static_cast<uint32_t>(fu)];
 }
 
 struct AttrArgs {
@@ -1225,18 +1138,20 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
 #pragma unroll
     for (uint32_t q = 0; q < NXG_ROOT_VAR_COUNT; ++q) SLOT_SET(q, vars[q]);
 
+    const ExprContext ctx{r.u, r.v, face_seed, 0.0f, 0u, false};
     float result[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-    bool bad = false;
+    // the reference rejects a non-finite context (xgen_expression.cpp:991-994)
+    bool bad = !isfinite(r.u) || !isfinite(r.v);
     for (uint32_t k = 0; k < a.program_count; ++k) {
         // the image is read-only for the whole launch: reading it through
         // the constant address space lets every (wave-uniform) fetch be a
         // scalar load
         const Image image = (Image)a.programs;
-        const ImageOf(PackedProgram) prog = (ImageOf(PackedProgram))(image + a.program_offset[k]);
-        const ImageOf(PackedInstruction) ins = (ImageOf(PackedInstruction))(image + prog->instructions_offset);
-        const ImageOf(uint32_t) operand_slots = (ImageOf(uint32_t))(image + prog->operands_offset);
-        const ImageOf(nxg_expr_ramp) ramps = (ImageOf(nxg_expr_ramp))(image + prog->ramps_offset);
-        const ImageOf(nxg_expr_ramp_point) points = (ImageOf(nxg_expr_ramp_point))(image + prog->ramp_points_offset);
+        ImageOf(PackedProgram) prog = (ImageOf(PackedProgram))(image + a.program_offset[k]);
+        ImageOf(PackedInstruction) ins = (ImageOf(PackedInstruction))(image + prog->instructions_offset);
+        ImageOf(uint32_t) operand_slots = (ImageOf(uint32_t))(image + prog->operands_offset);
+        ImageOf(nxg_expr_ramp) ramps = (ImageOf(nxg_expr_ramp))(image + prog->ramps_offset);
+        ImageOf(nxg_expr_ramp_point) points = (ImageOf(nxg_expr_ramp_point))(image + prog->ramp_points_offset);
         const uint32_t n = prog->instruction_count;
         // inputs are not dispatched: the root variables the program reads are
         // checked once here (an input is an error when its value is not finite)
@@ -1252,83 +1167,16 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
             next = load_instruction(ins + i + 1u);
 #define OPERAND(e)                                                                          \
     (((e) & kImmediate) ? (((e) & 1u) ? in.immediate1 : in.immediate) : SLOT_GET(e))
-#define OPV(j) OPERAND(uint32_t{in.slot[j]})
-#define OPA(j) OPERAND(operand_slots[in.operand_offset + (j)])
+#define EXPR_OPV(j) OPERAND(uint32_t{in.slot[j]})
+#define EXPR_OPA(j) OPERAND(operand_slots[in.operand_offset + (j)])
+#define EXPR_NOISE3(x, y, z) gradient_noise_exact(x, y, z, grad)
+#define EXPR_TEXEL(map, f, u, v) texel_sample(a.maps[(map) < a.map_count ? (map) : 0u], f, u, v)
             float value = 0.0f;
-            switch (in.op) {
-                case NXG_OP_CONSTANT: value = in.immediate; break;
-                case NXG_OP_INPUT: value = SLOT_GET(in.auxiliary); break;
-                case NXG_OP_NEGATE: value = -OPV(0); break;
-                case NXG_OP_LOGICAL_NOT: value = OPV(0) == 0.0f; break;
-                case NXG_OP_ADD: value = OPV(0) + OPV(1); break;
-                case NXG_OP_SUBTRACT: value = OPV(0) - OPV(1); break;
-                case NXG_OP_MULTIPLY: value = OPV(0) * OPV(1); break;
-                case NXG_OP_DIVIDE: value = OPV(0) / OPV(1); break;
-                case NXG_OP_LESS: value = OPV(0) < OPV(1); break;
-                case NXG_OP_LESS_EQUAL: value = OPV(0) <= OPV(1); break;
-                case NXG_OP_GREATER: value = OPV(0) > OPV(1); break;
-                case NXG_OP_GREATER_EQUAL: value = OPV(0) >= OPV(1); break;
-                case NXG_OP_EQUAL: value = OPV(0) == OPV(1); break;
-                case NXG_OP_NOT_EQUAL: value = OPV(0) != OPV(1); break;
-                case NXG_OP_LOGICAL_AND: value = OPV(0) != 0.0f && OPV(1) != 0.0f; break;
-                case NXG_OP_LOGICAL_OR: value = OPV(0) != 0.0f || OPV(1) != 0.0f; break;
-                case NXG_OP_SELECT: value = OPV(0) != 0.0f ? OPV(1) : OPV(2); break;
-                case NXG_OP_HASH: {
-                    uint32_t state = 0u;
-                    for (uint32_t j = 0; j < in.operand_count; ++j) state = hash_step(state, OPA(j));
-                    value = hash_to_unit(state);
-                    break;
-                }
-                case NXG_OP_RANDOM: {
-                    const bool explicit_seed = in.operand_count == 1u || in.operand_count == 3u;
-                    uint32_t state = hash_step(hash_step(hash_step(hash_step(0u, r.u), r.v), face_seed), in.immediate);
-                    if (explicit_seed) state = hash_step(state, OPA(in.operand_count - 1u));
-                    value = hash_to_unit(state);
-                    if (in.operand_count >= 2u) value = (OPA(1) - OPA(0)) * value + OPA(0);
-                    break;
-                }
-                case NXG_OP_MINIMUM: { const float x = OPV(0), y = OPV(1); value = y < x ? y : x; break; }
-                case NXG_OP_MAXIMUM: { const float x = OPV(0), y = OPV(1); value = x < y ? y : x; break; }
-                case NXG_OP_CLAMP: value = clamp_std(OPV(0), OPV(1), OPV(2)); break;
-                case NXG_OP_FIT: value = OPA(3) + (OPA(0) - OPA(1)) * (OPA(4) - OPA(3)) / (OPA(2) - OPA(1)); break;
-                case NXG_OP_GAMMA: value = powf(OPV(0), 1.0f / OPV(1)); break;
-                case NXG_OP_CONTRAST: {
-                    const float x = OPV(0);
-                    const float exponent = -log2f(x < 0.5f ? 2.0f * x : 2.0f - 2.0f * x);
-                    const float shaped = 0.5f * powf(1.0f - OPV(1), exponent);
-                    value = x < 0.5f ? shaped : 1.0f - shaped;
-                    break;
-                }
-                case NXG_OP_SMOOTHSTEP: {
-                    const float t = clamp_std((OPV(0) - OPV(1)) / (OPV(2) - OPV(1)), 0.0f, 1.0f);
-                    value = t * t * (3.0f - 2.0f * t);
-                    break;
-                }
-                case NXG_OP_RAMP: {
-                    value = ramp_eval(points + ramps[in.auxiliary].point_offset, ramps[in.auxiliary].point_count, 0.0f);
-                    break;
-                }
-                case NXG_OP_POW: value = powf(OPV(0), OPV(1)); break;
-                case NXG_OP_MIX: {
-                    const float x = OPV(0), y = OPV(1), t = OPV(2);
-                    value = x + (t - 0.0f) * (y - x) / (1.0f - 0.0f);
-                    break;
-                }
-                case NXG_OP_NOISE3: value = gradient_noise_exact(OPV(0), OPV(1), OPV(2), grad); break;
-                case NXG_OP_TEXEL: {
-                    const nxg_texel_map m = a.maps[in.auxiliary < a.map_count ? in.auxiliary : 0u];
-                    const float R = static_cast<float>(m.resolution);
-                    const float ff = clamp_std(floorf(OPV(0)), 0.0f, static_cast<float>(m.face_count - 1u));
-                    const float fu = clamp_std(floorf(OPV(1) * R), 0.0f, R - 1.0f);
-                    const float fv = clamp_std(floorf(OPV(2) * R), 0.0f, R - 1.0f);
-                    value = m.texels[(static_cast<uint64_t>(ff) * m.resolution + static_cast<uint32_t>(fv)) *
-                                         m.resolution + static_cast<uint32_t>(fu)];
-                    break;
-                }
-                default: value = __builtin_nanf(""); break;
-            }
-#undef OPV
-#undef OPA
+#include "expr_ops.inc"
+#undef EXPR_OPV
+#undef EXPR_OPA
+#undef EXPR_NOISE3
+#undef EXPR_TEXEL
 #undef OPERAND
             if (!isfinite(value)) bad = true;
             SLOT_SET(uint32_t{in.dest}, value);

@@ -85,6 +85,14 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
     return l;
 }
 
+// Classic runtime workspace: the cut layout, then one 32-byte state per
+// strand (the search's per-strand values for the write); the word after the
+// status word collects the error bits.
+inline uint64_t classic_runtime_state_offset(uint32_t strand_count) { return classic_layout(strand_count).total; }
+inline uint64_t classic_runtime_bytes(uint32_t strand_count) {
+    return classic_runtime_state_offset(strand_count) + ((uint64_t{strand_count} * 32u + 255u) & ~uint64_t{255u});
+}
+
 // SeExpr hashing and XGen string seeds in double, as the reference computes
 // them (expr_compile.cpp).
 uint32_t seexpr_component(double x);

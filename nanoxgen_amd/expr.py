@@ -238,6 +238,15 @@ class CompiledProgram:
         self._struct = None
         return self
 
+    def bind_indices(self, indices):
+        """Input i reads variable indices[i] of the caller's binding space
+        (e.g. NXG_CLASSIC_VAR_* for a Classic plan)."""
+        if len(indices) != len(self.inputs):
+            raise ValueError(f"{len(self.inputs)} inputs, {len(indices)} bindings")
+        self.bindings = [int(i) for i in indices]
+        self._struct = None
+        return self
+
     @property
     def struct(self):
         if self._struct is None:

@@ -82,6 +82,10 @@ def lib():
         L.orc_clump_apply.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_classic_runtime.restype = C.c_uint32
+        L.orc_classic_runtime.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_classic_cut.restype = C.c_uint32
         L.orc_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p,
@@ -139,6 +143,10 @@ def ref():
         L.ref_face_seed.argtypes = [C.c_uint32, C.c_char_p, C.c_uint32]
         L.ref_classic_endpoints.restype = C.c_uint32
         L.ref_classic_endpoints.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.ref_classic_runtime.restype = C.c_uint32
+        L.ref_classic_runtime.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                          C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_float,
+                                          C.c_char_p, C.c_uint64, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
         L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -369,6 +377,77 @@ def ref_classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, ro
     if kept == 0xFFFFFFFF:
         return None
     return dict(points=pts[:kept * cvs], point_counts=counts[:kept], roots=rr[:kept],
+                root_uvs=None if uu is None else uu[:kept], strand_count=kept, seconds=seconds.value)
+
+
+class _ClassicPlan(C.Structure):
+    """orc_classic_plan."""
+    _fields_ = [("length", C.c_void_p), ("width", C.c_void_p), ("cut_amount", C.c_void_p), ("taper", C.c_void_p),
+                ("taper_start", C.c_void_p), ("width_ramp", C.c_void_p), ("has_cut", C.c_int),
+                ("description_id", C.c_uint32), ("name_seed", C.c_uint32)]
+
+
+def _u32(x, n):
+    return None if x is None else np.ascontiguousarray(np.asarray(x).reshape(-1)[:n]).astype(np.uint32)
+
+
+def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitive_ids=None, random_prefixes=None,
+                    columns=None, radius_scale=1.0):
+    """orc_classic_runtime: the Classic runtime chain of `plan` (a
+    nanoxgen_amd.classic.ClassicPlan: its compiled, bound programs) over
+    packed float4 points [N * C, 4].  Returns dict(points, point_counts,
+    roots, root_uvs, keep, strand_count, error_bits); strand_count is None
+    where the reference throws (error_bits says why)."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    rr = np.ascontiguousarray(roots).view(abi.ROOT_DTYPE).reshape(-1)
+    cc = _u32(counts, n)
+    uu = None if uvs is None else np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+    ids, pre = _u32(primitive_ids, n), _u32(random_prefixes, n)
+    cols = None if columns is None else np.ascontiguousarray(columns, np.float32).reshape(n, -1)
+    ncol = 0 if cols is None else cols.shape[1]
+    op = _ClassicPlan(has_cut=1 if plan.has_cut else 0, description_id=plan.description_id,
+                      name_seed=plan.name_seed)
+    for key, prog in plan.programs.items():
+        setattr(op, key, C.addressof(prog.struct))
+    out = np.zeros_like(pts)
+    oc = np.zeros(n, np.uint32)
+    orr = np.zeros(n, abi.ROOT_DTYPE)
+    ou = np.zeros((n, 2), np.float32)
+    keep = np.zeros(n, np.uint8)
+    bits = C.c_uint32()
+    kept = lib().orc_classic_runtime(pts.ctypes.data, abi.ptr(cc), rr.ctypes.data, abi.ptr(uu), n, cvs,
+                                     C.byref(op), abi.ptr(ids), abi.ptr(pre), abi.ptr(cols), ncol, radius_scale,
+                                     out.ctypes.data, oc.ctypes.data, orr.ctypes.data, ou.ctypes.data,
+                                     keep.ctypes.data, C.byref(bits))
+    if kept == 0xFFFFFFFF:
+        return dict(strand_count=None, error_bits=bits.value, keep=keep)
+    return dict(points=out[:kept * cvs], point_counts=oc[:kept], roots=orr[:kept],
+                root_uvs=None if uu is None else ou[:kept], keep=keep, strand_count=kept, error_bits=0)
+
+
+def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None, primitive_ids=None,
+                        random_prefixes=None, columns=None, radius_scale=1.0):
+    """The reference's parse -> compile_xgen_classic_float_runtime_plan ->
+    apply_xgen_classic_float_runtime_plan_cpu over the first description of
+    `xgen_text`.  Returns dict(points, point_counts, roots, root_uvs,
+    strand_count, seconds), or dict(strand_count=None, error=message) where
+    the reference throws."""
+    pts = np.array(points, np.float32).reshape(-1, 4)
+    n = len(pts) // cvs
+    cc = np.full(n, cvs, np.uint32) if counts is None else np.array(counts, np.uint32)
+    rr = np.array(np.ascontiguousarray(roots).view(abi.ROOT_DTYPE).reshape(-1))
+    uu = None if uvs is None else np.array(uvs, np.float32).reshape(-1, 2)
+    ids, pre = _u32(primitive_ids, n), _u32(random_prefixes, n)
+    cols = None if columns is None else np.ascontiguousarray(columns, np.float32).reshape(-1)
+    err = C.create_string_buffer(1024)
+    seconds = C.c_double()
+    kept = ref().ref_classic_runtime(xgen_text.encode(), pts.ctypes.data, cc.ctypes.data, rr.ctypes.data,
+                                     abi.ptr(uu), n, cvs, abi.ptr(ids), abi.ptr(pre), abi.ptr(cols),
+                                     0 if cols is None else cols.size, radius_scale, err, 1024, C.byref(seconds))
+    if kept == 0xFFFFFFFF:
+        return dict(strand_count=None, error=err.value.decode())
+    return dict(points=pts[:kept * cvs], point_counts=cc[:kept], roots=rr[:kept],
                 root_uvs=None if uu is None else uu[:kept], strand_count=kept, seconds=seconds.value)
 
 

@@ -79,6 +79,17 @@ void orc_sample_root_f32(const void *blob, uint32_t seed, uint32_t strand, nxg_r
 uint32_t orc_seexpr_hash_finish(uint32_t state);
 uint32_t orc_runtime_hash_component(float argument);
 float orc_runtime_hash(const float *args, uint32_t n);
+typedef struct orc_expr_ctx {
+    const float *vars;  /* program input i reads vars[input_bindings[i]] */
+    uint32_t var_count;
+    float u, v, face_seed, t;
+    uint32_t random_prefix;
+    int has_random_prefix;
+    const nxg_vec3 *gradients;  /* noise3 (NULL: noise3 is an error) */
+    const nxg_texel_map *maps;
+    uint32_t map_count;
+} orc_expr_ctx;
+int orc_expr_eval_ctx(const nxg_expr_program *prog, const orc_expr_ctx *ctx, float *scratch, float *out);
 int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u, float v, float face_seed,
                   float t, const nxg_vec3 *gradients, const nxg_texel_map *maps, uint32_t map_count,
                   float *scratch, float *out);
@@ -104,6 +115,34 @@ uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, con
                          const float *tangents3, const float *reference3, uint32_t guides,
                          const nxg_vec3 *gradients, const orc_clump_params *p, const float *amount_multiplier,
                          uint32_t multiplier_stride);
+/* Classic primitive expressions and a CutFX amount expression
+ * (oracle_classic.c, orc_classic_runtime).  Program inputs bind to
+ * ORC_CLASSIC_VAR_* (3 + k: column k of the per-strand input row). */
+#define ORC_CLASSIC_VAR_ID 0u
+#define ORC_CLASSIC_VAR_CLENGTH 1u
+#define ORC_CLASSIC_VAR_CWIDTH 2u
+#define ORC_CLASSIC_MAX_COLUMNS 183u
+#define ORC_CLASSIC_NONFINITE_AMOUNT 1u
+#define ORC_CLASSIC_NONFINITE_WIDTH 2u
+#define ORC_CLASSIC_NEGATIVE_LENGTH 8u
+#define ORC_CLASSIC_NONFINITE_TAPER 16u
+#define ORC_CLASSIC_NONFINITE_PROFILE 32u
+#define ORC_CLASSIC_EXPRESSION 64u
+#define ORC_CLASSIC_VARIABLE_COUNTS 128u
+typedef struct orc_classic_plan {
+    const nxg_expr_program *length, *width, *cut_amount, *taper, *taper_start, *width_ramp; /* NULL: absent */
+    int has_cut;              /* one CutFX whose amount is cut_amount */
+    uint32_t description_id;  /* face seed: runtime hash of (id, name seed, face) */
+    uint32_t name_seed;
+} orc_classic_plan;
+/* Returns the kept count (outputs compacted in order), or ~0u with the
+ * ORC_CLASSIC_* bits of every strand that raised one in *error_bits. */
+uint32_t orc_classic_runtime(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
+                             const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs,
+                             const orc_classic_plan *plan, const uint32_t *primitive_ids,
+                             const uint32_t *random_prefixes, const float *columns, uint32_t column_count,
+                             float radius_scale, float *out_points, uint32_t *out_counts,
+                             nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep, uint32_t *error_bits);
 uint32_t orc_classic_cut(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
                          const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, float amount_scale,
                          float amount_offset, const float *multiplier, float radius_scale, float *out_points,

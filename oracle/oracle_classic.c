@@ -186,3 +186,146 @@ void orc_classic_endpoints(const float *points, uint32_t strand_count, uint32_t 
         o[4u * (m - 1u) + 3] = in[4u * (cvs - 1u) + 3];
     }
 }
+
+/* ---- Classic primitive expressions -------------------------------------
+ * apply_xgen_classic_float_runtime_plan_cpu (src/xgen_classic_runtime.cpp:
+ * 1608-1918) for a plan of SplinePrimitive expressions (length, width,
+ * taper, taperStart, widthRamp) and at most one CutFX (rebuildType 1) whose
+ * amount is an expression.  Per strand, in the reference's order:
+ *   fixed point counts                                          :1619-1622
+ *   context: id, u, v, face seed, random prefix, input row      :1628-1647
+ *   cLength; length expression, scale_curve about the root      :1648-1657
+ *   raw width (expression, else 2 radius[0] / radius_scale)     :1658-1665
+ *   cull below 1e-4 (the strand is skipped)                     :1666-1676
+ *   CutFX: amount, cut_from_tip_and_rebuild, keep = cut >= 1e-4,
+ *          cLength of the rebuilt curve                         :1806-1821
+ *   taper, taperStart; per-CV width profile -> radius           :1827-1858
+ *   stable compaction                                           :1893-1918
+ * Expression inputs bind as input_value does (:504-553): id, cLength,
+ * cWidth, then the input row's columns. */
+
+static int classic_eval(const nxg_expr_program *prog, orc_expr_ctx *ctx, float *vars, float c_length,
+                        float c_width, float t, float *out) {
+    float scratch[NXG_EXPR_MAX_VALUES];
+    vars[ORC_CLASSIC_VAR_CLENGTH] = c_length;
+    vars[ORC_CLASSIC_VAR_CWIDTH] = c_width;
+    ctx->t = t;
+    return orc_expr_eval_ctx(prog, ctx, scratch, out);
+}
+
+static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const orc_classic_plan *plan,
+                               const nxg_root_sample *roots, const uint32_t *primitive_ids,
+                               const uint32_t *random_prefixes, const float *columns, uint32_t column_count,
+                               float radius_scale, float *out4, uint8_t *keep) {
+    cv3 p[256], source[256];
+    float vars[3 + ORC_CLASSIC_MAX_COLUMNS];
+    for (uint32_t i = 0; i < cvs; ++i) p[i] = c3(in[4 * i], in[4 * i + 1], in[4 * i + 2]);
+    const nxg_root_sample *root = &roots[s];
+    orc_expr_ctx ctx;
+    memset(&ctx, 0, sizeof(ctx));
+    vars[ORC_CLASSIC_VAR_ID] = (float)(primitive_ids ? primitive_ids[s] : s);
+    vars[ORC_CLASSIC_VAR_CLENGTH] = 0.0f;
+    vars[ORC_CLASSIC_VAR_CWIDTH] = 0.0f;
+    for (uint32_t k = 0; k < column_count; ++k) vars[3u + k] = columns[(uint64_t)s * column_count + k];
+    ctx.vars = vars;
+    ctx.var_count = 3u + column_count;
+    ctx.u = root->uv.x;
+    ctx.v = root->uv.y;
+    {
+        const uint32_t face = root->surface_face_id == NXG_INVALID_INDEX ? root->triangle_index
+                                                                          : root->surface_face_id;
+        const float args[3] = {(float)plan->description_id, (float)plan->name_seed, (float)face};
+        ctx.face_seed = orc_runtime_hash(args, 3u);
+    }
+    ctx.has_random_prefix = random_prefixes != NULL;
+    ctx.random_prefix = random_prefixes ? random_prefixes[s] : 0u;
+
+    *keep = 0u;
+    float c_length = orc_curve_spline_length_v(p, cvs);
+    if (plan->length) {
+        float scale;
+        if (classic_eval(plan->length, &ctx, vars, c_length, 0.0f, 0.0f, &scale)) return ORC_CLASSIC_EXPRESSION;
+        if (!isfinite(scale) || scale < 0.0f) return ORC_CLASSIC_NEGATIVE_LENGTH;
+        const cv3 r = p[0]; /* scale_curve, :576-585 */
+        for (uint32_t i = 0; i < cvs; ++i) p[i] = add3(r, mul3(sub3(p[i], r), scale));
+        c_length *= scale;
+    }
+    float raw_width;
+    if (plan->width) {
+        if (classic_eval(plan->width, &ctx, vars, c_length, 0.0f, 0.0f, &raw_width)) return ORC_CLASSIC_EXPRESSION;
+    } else {
+        raw_width = radius_scale > 0.0f ? 2.0f * in[3] / radius_scale : 0.0f;
+    }
+    if (!isfinite(raw_width)) return ORC_CLASSIC_NONFINITE_WIDTH;
+    if (c_length < 1.0e-4f || raw_width < 1.0e-4f) return 0u;
+    const float c_width = raw_width;
+    uint8_t k = 1u;
+    if (plan->has_cut) {
+        float amount;
+        if (classic_eval(plan->cut_amount, &ctx, vars, c_length, c_width, 0.0f, &amount))
+            return ORC_CLASSIC_EXPRESSION;
+        if (!isfinite(amount)) return ORC_CLASSIC_NONFINITE_AMOUNT;
+        const float cut = orc_cut_from_tip_v(p, cvs, max_std(amount, 0.0f), source);
+        if (cut < 1.0e-4f) k = 0u;
+        c_length = orc_curve_spline_length_v(p, cvs);
+    }
+    float taper = 0.0f, taper_start = 0.0f;
+    if (plan->taper && classic_eval(plan->taper, &ctx, vars, c_length, c_width, 0.0f, &taper))
+        return ORC_CLASSIC_EXPRESSION;
+    if (plan->taper_start && classic_eval(plan->taper_start, &ctx, vars, c_length, c_width, 0.0f, &taper_start))
+        return ORC_CLASSIC_EXPRESSION;
+    if (!isfinite(taper) || !isfinite(taper_start)) return ORC_CLASSIC_NONFINITE_TAPER;
+    const int profile = plan->width || plan->taper || plan->taper_start || plan->width_ramp;
+    for (uint32_t i = 0; i < cvs; ++i) {
+        out4[4 * i] = p[i].x;
+        out4[4 * i + 1] = p[i].y;
+        out4[4 * i + 2] = p[i].z;
+        out4[4 * i + 3] = in[4 * i + 3];
+        if (!profile) continue;
+        const float t = (float)i / (float)(cvs - 1u);
+        float scale = 1.0f;
+        if (t > taper_start && taper_start < 1.0f) scale *= 1.0f - taper * ((t - taper_start) / (1.0f - taper_start));
+        if (plan->width_ramp) {
+            float ramp;
+            if (classic_eval(plan->width_ramp, &ctx, vars, c_length, c_width, t, &ramp)) return ORC_CLASSIC_EXPRESSION;
+            scale *= ramp;
+        }
+        const float raw_diameter = c_width * scale;
+        if (!isfinite(raw_diameter)) return ORC_CLASSIC_NONFINITE_PROFILE;
+        out4[4 * i + 3] = 0.5f * max_std(raw_diameter, 0.0f) * radius_scale;
+    }
+    *keep = k;
+    return 0u;
+}
+
+uint32_t orc_classic_runtime(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
+                             const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs,
+                             const orc_classic_plan *plan, const uint32_t *primitive_ids,
+                             const uint32_t *random_prefixes, const float *columns, uint32_t column_count,
+                             float radius_scale, float *out_points, uint32_t *out_counts,
+                             nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep, uint32_t *error_bits) {
+    uint32_t errors = 0u, write = 0u;
+    float strand[4 * 256];
+    if (cvs < 2u || cvs > 256u || column_count > ORC_CLASSIC_MAX_COLUMNS || !roots) {
+        *error_bits = ~0u;
+        return ~0u;
+    }
+    for (uint32_t s = 0; s < strand_count; ++s) {
+        if (counts && counts[s] != cvs) {
+            errors |= ORC_CLASSIC_VARIABLE_COUNTS;
+            continue;
+        }
+        uint8_t k = 0u;
+        errors |= classic_strand(points + (uint64_t)s * cvs * 4u, s, cvs, plan, roots, primitive_ids,
+                                 random_prefixes, columns, column_count, radius_scale, strand, &k);
+        if (keep) keep[s] = k;
+        if (!k || errors) continue;
+        memcpy(out_points + (uint64_t)write * cvs * 4u, strand, sizeof(float) * 4u * cvs);
+        if (out_counts) out_counts[write] = cvs;
+        if (out_roots) out_roots[write] = roots[s];
+        if (out_uvs && uvs) out_uvs[write] = uvs[s];
+        ++write;
+    }
+    *error_bits = errors;
+    return errors ? ~0u : write;
+}

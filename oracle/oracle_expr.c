@@ -111,10 +111,12 @@ static int ramp_eval(const nxg_expr_ramp_point *pts, uint32_t n, float parameter
     return 0;
 }
 
-/* Evaluate one program for one root.  ctx: root variables (NXG_ROOT_VAR_*),
- * u, v, face_seed, t of XgenExpressionFloatContext (xgen_expression.h:124-132;
- * no random prefix).  Returns 0 and *out, or 1 on the reference's error
- * conditions (arity, SSA order, ranges, non-finite result).
+/* Evaluate one program in one context (XgenExpressionFloatContext,
+ * xgen_expression.h:124-132): program input i reads vars[input_bindings[i]];
+ * u, v, face_seed, t and the optional random prefix as in the reference.
+ * Returns 0 and *out, or 1 on the reference's error conditions (non-finite
+ * context or input, xgen_expression.cpp:991-999; arity, SSA order, ranges,
+ * non-finite result).
  *
  * Extension ops (no reference counterpart):
  *   pow(a, b)      = powf(a, b)
@@ -124,9 +126,11 @@ static int ramp_eval(const nxg_expr_ramp_point *pts, uint32_t n, float parameter
  *   texel(f, u, v) = point sample of texel map `auxiliary`: face
  *                    clamp(floor(f), 0, faces-1), column clamp(floor(u*R), 0,
  *                    R-1), row clamp(floor(v*R), 0, R-1) of an R x R map. */
-int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u, float v, float face_seed,
-                  float t, const nxg_vec3 *gradients, const nxg_texel_map *maps, uint32_t map_count,
-                  float *scratch, float *out) {
+int orc_expr_eval_ctx(const nxg_expr_program *prog, const orc_expr_ctx *ctx, float *scratch, float *out) {
+    const float u = ctx->u, v = ctx->v, face_seed = ctx->face_seed, t = ctx->t;
+    if (!isfinite(u) || !isfinite(v) || !isfinite(face_seed) || !isfinite(t)) return 1;
+    for (uint32_t i = 0; i < prog->input_count; ++i)
+        if (prog->input_bindings[i] >= ctx->var_count || !isfinite(ctx->vars[prog->input_bindings[i]])) return 1;
     uint32_t count = 0;
     for (uint32_t ii = 0; ii < prog->instruction_count; ++ii) {
         const nxg_expr_instruction *in = &prog->instructions[ii];
@@ -141,9 +145,8 @@ int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u,
             case NXG_OP_CONSTANT: ARITY(0); value = in->immediate; break;
             case NXG_OP_INPUT:
                 ARITY(0);
-                if (in->auxiliary >= prog->input_count || prog->input_bindings[in->auxiliary] >= NXG_ROOT_VAR_COUNT)
-                    return 1;
-                value = root_vars[prog->input_bindings[in->auxiliary]];
+                if (in->auxiliary >= prog->input_count) return 1;
+                value = ctx->vars[prog->input_bindings[in->auxiliary]];
                 break;
             case NXG_OP_NEGATE: ARITY(1); value = -A(0); break;
             case NXG_OP_LOGICAL_NOT: ARITY(1); value = A(0) == 0.0f; break;
@@ -170,9 +173,17 @@ int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u,
             case NXG_OP_RANDOM: {
                 if (in->operand_count > 3u) return 1;
                 const int explicit_seed = in->operand_count == 1u || in->operand_count == 3u;
-                float args[5] = {u, v, face_seed, in->immediate, 0.0f};
-                if (explicit_seed) args[4] = A(in->operand_count - 1u);
-                value = orc_runtime_hash(args, explicit_seed ? 5u : 4u);
+                if (ctx->has_random_prefix) { /* xgen_expression.cpp:1066-1075 */
+                    uint32_t state = ctx->random_prefix * 1664525u + in->auxiliary + 1013904223u;
+                    if (explicit_seed)
+                        state = state * 1664525u + orc_runtime_hash_component(A(in->operand_count - 1u)) +
+                                1013904223u;
+                    value = (float)orc_seexpr_hash_finish(state) * 0x1p-32f;
+                } else {
+                    float args[5] = {u, v, face_seed, in->immediate, 0.0f};
+                    if (explicit_seed) args[4] = A(in->operand_count - 1u);
+                    value = orc_runtime_hash(args, explicit_seed ? 5u : 4u);
+                }
                 if (in->operand_count >= 2u) value = (A(1) - A(0)) * value + A(0);
                 break;
             }
@@ -210,11 +221,15 @@ int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u,
                 value = a + (tt - 0.0f) * (b - a) / (1.0f - 0.0f);
                 break;
             }
-            case NXG_OP_NOISE3: ARITY(3); value = orc_gradient_noise_f32(gradients, A(0), A(1), A(2)); break;
+            case NXG_OP_NOISE3:
+                ARITY(3);
+                if (!ctx->gradients) return 1;
+                value = orc_gradient_noise_f32(ctx->gradients, A(0), A(1), A(2));
+                break;
             case NXG_OP_TEXEL: {
                 ARITY(3);
-                if (in->auxiliary >= map_count) return 1;
-                const nxg_texel_map *m = &maps[in->auxiliary];
+                if (in->auxiliary >= ctx->map_count) return 1;
+                const nxg_texel_map *m = &ctx->maps[in->auxiliary];
                 const float R = (float)m->resolution;
                 float ff = floorf(A(0)), fu = floorf(A(1) * R), fv = floorf(A(2) * R);
                 ff = fclampf(ff, 0.0f, (float)(m->face_count - 1u));
@@ -234,6 +249,24 @@ int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u,
     if (prog->result >= count) return 1;
     *out = scratch[prog->result];
     return 0;
+}
+
+/* The generator's form: root variables (NXG_ROOT_VAR_*), no random prefix. */
+int orc_expr_eval(const nxg_expr_program *prog, const float *root_vars, float u, float v, float face_seed,
+                  float t, const nxg_vec3 *gradients, const nxg_texel_map *maps, uint32_t map_count,
+                  float *scratch, float *out) {
+    orc_expr_ctx ctx;
+    memset(&ctx, 0, sizeof(ctx));
+    ctx.vars = root_vars;
+    ctx.var_count = NXG_ROOT_VAR_COUNT;
+    ctx.u = u;
+    ctx.v = v;
+    ctx.face_seed = face_seed;
+    ctx.t = t;
+    ctx.gradients = gradients;
+    ctx.maps = maps;
+    ctx.map_count = map_count;
+    return orc_expr_eval_ctx(prog, &ctx, scratch, out);
 }
 
 typedef struct {

@@ -6,6 +6,7 @@
 #include "nanoxgen/curve_cache.h"
 #include "nanoxgen/curve_payload.h"
 #include "nanoxgen/generate.h"
+#include "nanoxgen/xgen_classic.h"
 #include "nanoxgen/xgen_classic_runtime.h"
 #include "nanoxgen/xgen_expression.h"
 
@@ -15,6 +16,7 @@
 #include <cstring>
 #include <exception>
 #include <span>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -492,6 +494,62 @@ int ref_expr_compile(const char *source, uint64_t size, const char *name, const 
     } catch (const std::exception &e) {
         std::snprintf(err, errcap, "%s", e.what());
         return 1;
+    }
+}
+
+// The Classic float runtime as the reference runs a description: parse the
+// .xgen text (parse_xgen_classic_collection), compile the first
+// description's plan (compile_xgen_classic_float_runtime_plan, which binds
+// map()/custom/$Prefg inputs to the input row) and apply it
+// (apply_xgen_classic_float_runtime_plan_cpu) to packed curves with optional
+// primitive ids, SeExpr prefixes and a row-major input table.  Returns the
+// kept count (outputs compacted in place) or ~0u with the reference's
+// message in err (a plan that needs a fallback reports its reasons).
+uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *counts, nxg_root_sample *roots,
+                             nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, const uint32_t *primitive_ids,
+                             const uint32_t *prefixes, const float *inputs, uint64_t input_count,
+                             float radius_scale, char *err, uint64_t errcap, double *seconds) {
+    try {
+        std::istringstream text(xgen_text);
+        const ClassicCollection collection = parse_xgen_classic_collection(text);
+        if (collection.descriptions.empty()) throw std::runtime_error("no description");
+        const ClassicFloatRuntimePlan plan =
+            compile_xgen_classic_float_runtime_plan(collection.descriptions.front(), collection.palette_attributes);
+        if (!plan.lowering_complete()) {
+            std::string reasons = "fallback:";
+            for (const std::string &r : plan.fallback_reasons) reasons += " " + r + ";";
+            throw std::runtime_error(reasons);
+        }
+        PackedGeneratedCurves curves{};
+        curves.strand_count = strand_count;
+        curves.cvs_per_strand = cvs;
+        curves.point_counts.assign(counts, counts + strand_count);
+        curves.points.resize(std::size_t{strand_count} * cvs);
+        std::memcpy(curves.points.data(), points, curves.points.size() * sizeof(PackedCurvePoint));
+        curves.roots.resize(strand_count);
+        std::memcpy(curves.roots.data(), roots, std::size_t{strand_count} * sizeof(RootSample));
+        if (uvs) {
+            curves.root_uvs.resize(strand_count);
+            std::memcpy(curves.root_uvs.data(), uvs, std::size_t{strand_count} * sizeof(Vec2));
+        }
+        const std::span<const std::uint32_t> ids =
+            primitive_ids ? std::span<const std::uint32_t>(primitive_ids, strand_count)
+                          : std::span<const std::uint32_t>();
+        const std::span<const std::uint32_t> pre =
+            prefixes ? std::span<const std::uint32_t>(prefixes, strand_count) : std::span<const std::uint32_t>();
+        const std::span<const float> rows =
+            inputs ? std::span<const float>(inputs, input_count) : std::span<const float>();
+        const auto start = std::chrono::steady_clock::now();
+        apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale, {}, pre, ids, {}, rows);
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
+        std::memcpy(counts, curves.point_counts.data(), curves.point_counts.size() * sizeof(uint32_t));
+        std::memcpy(roots, curves.roots.data(), curves.roots.size() * sizeof(RootSample));
+        if (uvs) std::memcpy(uvs, curves.root_uvs.data(), curves.root_uvs.size() * sizeof(Vec2));
+        return curves.strand_count;
+    } catch (const std::exception &e) {
+        std::snprintf(err, errcap, "%s", e.what());
+        return ~0u;
     }
 }
 

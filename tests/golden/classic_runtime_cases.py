@@ -1,0 +1,118 @@
+"""Cases of the Classic runtime with primitive expressions, shared by the
+fixture writer (make_golden.py classic_runtime) and the tests.
+
+Each case is a Classic description: SplinePrimitive attributes (length,
+width, taper, taperStart, widthRamp) and an optional CutFX amount, written
+in the authoring form the reference's plan compiler reads (map('path') for
+PTEX inputs, noise($Prefg*f) for reference-pose noise).  `xgen_text` renders
+the description for the reference's parser; `plan_sources` rewrites the
+same expressions into the bound form (map -> $__nxg_map_k in compile order,
+noise($Prefg*f) -> $__nxg_pref_noise_j per distinct frequency), which is what
+compile_xgen_classic_float_runtime_plan hands its compiler
+(src/xgen_classic_runtime.cpp:138-430).  `case_inputs` builds the strands,
+roots, ids, prefixes and input rows of a case from its seed.
+"""
+import re
+import zlib
+
+import numpy as np
+
+# The order the reference compiles a plan's expressions in
+# (src/xgen_classic_runtime.cpp:1211-1229, then the FX modules).
+COMPILE_ORDER = (("length", "length"), ("width", "width"), ("taper", "taper"), ("taper_start", "taperStart"),
+                 ("width_ramp", "widthRamp"), ("cut_amount", "amount"))
+
+CASES = [
+    # the reference's own test_float_runtime_plan (tests/classic_test.cpp:198-246)
+    dict(name="floatRuntime", description_id=7, strands="line3",
+         exprs=dict(length="2", width="0.2", taper="0.5", taper_start="0.5", width_ramp="rampUI(0,1,1:1,0.5,1)",
+                    cut_amount="0.25*$cLength")),
+    # every expression, PTEX inputs, rand() through SeExpr prefixes
+    dict(name="ns:walkAll", description_id=12, strands="walk", ids=True, prefixes=True,
+         exprs=dict(length="0.5 + rand(0.2, 1.5) * map('len.ptx')",
+                    width="map('w.ptx') * 0.05 + $id * 0.00001",
+                    taper="0.3 + 0.4 * map('t.ptx')", taper_start="clamp(rand(), 0.1, 0.6)",
+                    width_ramp="rampUI(0,1,1:0.5,0.6,2:1,0.1,3)",
+                    cut_amount="$cLength * (0.1 + 0.3 * rand(7)) + $cWidth")),
+    # the generator's own strands: width from the radii, a cLength-reading ramp
+    dict(name="icoRamp", description_id=3, strands="ico5", ids=True,
+         exprs=dict(taper="0.8", width_ramp="$cLength > 0.3 ? 1 : 0.5", cut_amount="0.2 * $cLength")),
+    # no CutFX; a $Prefg noise input after a PTEX input; radius scale 2
+    dict(name="icoNoCut", description_id=5, strands="ico13", radius_scale=2.0,
+         exprs=dict(length="smoothstep(0, 1, map('a.ptx'))->gamma(2) + 0.5",
+                    width="0.01 * (1 + noise($Prefg*4))", taper="0.5", taper_start="0.2")),
+    # the culling thresholds (tests/classic_test.cpp:316-336) by $id
+    dict(name="threshold", description_id=9, strands="walk", ids=True,
+         exprs=dict(length="$id < 100 ? 0 : 1", width="$id < 200 ? 0.0001 : 0.000099")),
+    # a cut that removes everything, then a ramp on the collapsed strand
+    dict(name="cutAll", description_id=2, strands="ico5",
+         exprs=dict(cut_amount="$cLength * 2", width_ramp="$cLength + 0.5")),
+    # errors: the reference's messages
+    dict(name="negativeLength", description_id=1, strands="walk", exprs=dict(length="$id == 50 ? -1 : 1")),
+    dict(name="nonfiniteWidth", description_id=1, strands="walk", ids=True, exprs=dict(width="1 / ($id - 5)")),
+]
+
+
+def xgen_text(case):
+    """The case as a Classic .xgen collection (FileVersion 18)."""
+    e = case["exprs"]
+    s = "FileVersion 18\nPalette\n\tname\tpalette\n\tendAttrs\n"
+    s += f"Description\n\tname\t{case['name']}\n\tdescriptionId\t{case['description_id']}\n\tendAttrs\n"
+    s += "SplinePrimitive\n"
+    for key, attribute in COMPILE_ORDER[:5]:
+        if key in e:
+            s += f"\t{attribute}\t{e[key]}\n"
+    s += "\tendAttrs\n"
+    if "cut_amount" in e:
+        s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{e['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
+    return s + "\tActive\tSplinePrimitive\n"
+
+
+_MAP = re.compile(r"\bmap\s*\(\s*(['\"])(.*?)\1\s*\)")
+_PREF = re.compile(r"\bnoise\s*\(\s*\$Prefg\s*\*\s*([-+]?[0-9.]+)\s*\)")
+
+
+def plan_sources(case):
+    """(sources in the bound form, ptex count, pref-noise frequencies)."""
+    out, ptex, freqs = {}, 0, []
+    for key, _ in COMPILE_ORDER:
+        if key not in case["exprs"]:
+            continue
+        src = case["exprs"][key]
+
+        def map_sub(m):
+            nonlocal ptex
+            ptex += 1
+            return f"$__nxg_map_{ptex - 1}"
+
+        src = _MAP.sub(map_sub, src)
+        out[key] = src
+    for key in list(out):
+        def pref_sub(m):
+            f = np.float32(float(m.group(1)))
+            if f not in freqs:
+                freqs.append(f)
+            return f"$__nxg_pref_noise_{freqs.index(f)}"
+
+        out[key] = _PREF.sub(pref_sub, out[key])
+    return out, ptex, freqs
+
+
+def case_inputs(case, strands_of):
+    """Inputs of a case: dict(points, cvs, roots, uvs, ids, prefixes, columns).
+    strands_of(name) -> (packed float4 points, cvs, roots, uvs)."""
+    from nanoxgen_amd import abi
+    if case["strands"] == "line3":
+        pts = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [2, 0, 0, 0]], np.float32)
+        cvs, roots, uvs = 3, np.zeros(1, abi.ROOT_DTYPE), None
+    else:
+        pts, cvs, roots, uvs = strands_of(case["strands"])
+    n = len(pts) // cvs
+    rng = np.random.default_rng(zlib.crc32(case["name"].encode()))
+    _, ptex, freqs = plan_sources(case)
+    k = ptex + len(freqs)
+    columns = rng.uniform(0.05, 1.0, size=(n, k)).astype(np.float32) if k else None
+    ids = (rng.permutation(n).astype(np.uint32) if case.get("ids") else None)
+    prefixes = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32) if case.get("prefixes") else None
+    return dict(points=pts, cvs=cvs, roots=roots, uvs=uvs, ids=ids, prefixes=prefixes, columns=columns,
+                radius_scale=case.get("radius_scale", 1.0))

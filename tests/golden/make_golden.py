@@ -20,7 +20,12 @@ never leaves this container.
   * build_curve_batches over seeded curve sets (reference_payload.npz; the
     inputs are regenerated from payload_cases()).
 
-    python tests/golden/make_golden.py [generation] [expressions] [classic] [clump] [payload]
+  * the Classic runtime with primitive expressions (length, width, taper,
+    taperStart, widthRamp, a CutFX amount) over the cases of
+    classic_runtime_cases.py, through the reference's .xgen parser and plan
+    compiler (reference_classic_runtime.npz, inputs included).
+
+    python tests/golden/make_golden.py [generation] [expressions] [classic] [clump] [payload] [classic_runtime]
 """
 import os
 import sys
@@ -417,8 +422,39 @@ def main_payload():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def main_classic_runtime():
+    """The Classic runtime with primitive expressions: every case of
+    classic_runtime_cases.CASES through the reference's parser, plan compiler
+    and runtime; inputs and outputs (or the reference's message)."""
+    from classic_runtime_cases import CASES, case_inputs, xgen_text
+    oracle.build()
+    if oracle.ref() is None:
+        raise SystemExit("reference core not built (needs /root/reference)")
+    fx = {}
+    for i, case in enumerate(CASES):
+        inp = case_inputs(case, classic_strands)
+        for k, v in inp.items():
+            if v is not None:
+                fx[f"case{i}_in_{k}"] = np.asarray(v)
+        out = oracle.ref_classic_runtime(xgen_text(case), inp["points"], inp["cvs"], roots=inp["roots"],
+                                         uvs=inp["uvs"], primitive_ids=inp["ids"], random_prefixes=inp["prefixes"],
+                                         columns=inp["columns"], radius_scale=inp["radius_scale"])
+        if out["strand_count"] is None:
+            fx[f"case{i}_error"] = np.array(out["error"])
+            print(case["name"], "->", out["error"])
+            continue
+        for k in ("points", "point_counts", "roots", "root_uvs"):
+            if out[k] is not None:
+                fx[f"case{i}_out_{k}"] = out[k]
+        print(case["name"], "->", out["strand_count"], "of", len(inp["points"]) // inp["cvs"], "kept")
+    path = os.path.join(HERE, "reference_classic_runtime.npz")
+    np.savez_compressed(path, **fx)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["generation", "expressions", "classic", "clump", "payload"]
+    sys.path.insert(0, HERE)
+    which = sys.argv[1:] or ["generation", "expressions", "classic", "clump", "payload", "classic_runtime"]
     if "generation" in which:
         main()
     if "expressions" in which:
@@ -429,3 +465,5 @@ if __name__ == "__main__":
         main_clump()
     if "payload" in which:
         main_payload()
+    if "classic_runtime" in which:
+        main_classic_runtime()

@@ -1,0 +1,175 @@
+"""The Classic runtime with primitive expressions on the GPU
+(nxg_classic_runtime_apply) against the reference's parser, plan compiler and
+runtime (tests/golden/reference_classic_runtime.npz) and the oracle
+(orc_classic_runtime, pinned to that fixture), bit for bit in kept strands,
+point counts, roots and every position and radius; where the reference
+throws, nothing is written and the status bits name its message."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import nanoxgen_amd as nx
+from nanoxgen_amd import abi, classic, workloads
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+from classic_runtime_cases import CASES, plan_sources  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+MESSAGES = {
+    abi.CLASSIC_NONFINITE_AMOUNT: "Classic CutFX amount produced a non-finite value",
+    abi.CLASSIC_NONFINITE_WIDTH: "Classic width expression produced a non-finite value",
+    abi.CLASSIC_NEGATIVE_LENGTH: "Classic length expression produced a negative value",
+    abi.CLASSIC_NONFINITE_TAPER: "Classic taper expression produced a non-finite value",
+    abi.CLASSIC_NONFINITE_PROFILE: "Classic width profile produced a non-finite value",
+    abi.CLASSIC_EXPRESSION_ERROR: "XGen float expression produced a non-finite value",
+    abi.CLASSIC_VARIABLE_COUNTS: "Classic runtime requires fixed point counts",
+}
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def fixture():
+    with np.load(os.path.join(HERE, "golden", "reference_classic_runtime.npz"), allow_pickle=False) as d:
+        return {k: d[k] for k in d.keys()}
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint8)
+
+
+def dev(torch, a, view=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a)
+    return torch.from_numpy(a if view is None else a.view(view)).cuda()
+
+
+def make_plan(case):
+    sources, ptex, _ = plan_sources(case)
+    return classic.ClassicPlan(**sources, description_id=case["description_id"], description_name=case["name"],
+                               ptex_count=ptex)
+
+
+def run_gpu(torch, plan, pts, cvs, *, roots, uvs=None, counts=None, ids=None, prefixes=None, columns=None,
+            radius_scale=1.0):
+    out = classic.classic_runtime(dev(torch, np.asarray(pts, np.float32).reshape(-1, 4)), cvs, plan,
+                                  roots=dev(torch, np.ascontiguousarray(roots).view(np.int32).reshape(-1, 12)),
+                                  root_uvs=dev(torch, uvs), point_counts=dev(torch, counts, np.int32),
+                                  primitive_ids=dev(torch, ids, np.int32), random_prefixes=dev(torch, prefixes, np.int32),
+                                  columns=dev(torch, columns), radius_scale=radius_scale)
+    return out.to_host()
+
+
+def same(got, want, label):
+    k = want["strand_count"]
+    assert got["status"] == 0, f"{label}: status {got['status']}"
+    assert got["strand_count"] == k, f"{label}: kept {got['strand_count']} != {k}"
+    assert np.array_equal(bits(got["points"]), bits(want["points"])), f"{label}: points"
+    assert np.array_equal(got["point_counts"], want["point_counts"]), f"{label}: counts"
+    assert np.array_equal(bits(got["roots"]), bits(want["roots"])), f"{label}: roots"
+
+
+@pytest.mark.parametrize("i", range(len(CASES)), ids=[c["name"] for c in CASES])
+def test_reference_fixture(torch, fixture, i):
+    g = lambda k: fixture.get(f"case{i}_in_{k}")
+    got = run_gpu(torch, make_plan(CASES[i]), g("points"), int(g("cvs")), roots=g("roots"), uvs=g("uvs"),
+                  ids=g("ids"), prefixes=g("prefixes"), columns=g("columns"), radius_scale=float(g("radius_scale")))
+    if f"case{i}_error" in fixture:
+        assert got["strand_count"] == 0 and got["status"] != 0
+        assert {MESSAGES[b] for b in MESSAGES if got["status"] & b} == {str(fixture[f"case{i}_error"])}
+        return
+    want = dict(strand_count=len(fixture[f"case{i}_out_point_counts"]), points=fixture[f"case{i}_out_points"],
+                point_counts=fixture[f"case{i}_out_point_counts"], roots=fixture[f"case{i}_out_roots"])
+    same(got, want, CASES[i]["name"])
+
+
+PLANS = [
+    dict(length="1 + 0.5 * $__nxg_map_0", width="$cWidth + 0.004 + 0.01 * $__nxg_map_1", taper="0.6",
+         taper_start="0.25", width_ramp="rampUI(0,1,1:0.4,0.7,2:1,0.2,3)",
+         cut_amount="$cLength * (0.2 + 0.5 * rand(3)) + $cWidth"),
+    dict(width_ramp="$cLength * 0.1 + 0.5", cut_amount="0.3 * $cLength"),
+    dict(length="fit($__nxg_map_0, 0, 1, 0.5, 1.5)", taper="0.9"),
+]
+
+
+@pytest.mark.parametrize("cvs", [2, 5, 16, 33, 64, 80])
+def test_random_walks_oracle(torch, orc, cvs):
+    """Seeded walks at every staging form (staged tiles up to 64 CVs, memory
+    reads above), ragged final tiles, every plan shape; GPU == oracle."""
+    rng = np.random.default_rng(cvs)
+    n = 64 * 3 + 17
+    steps = rng.normal(0, 0.3, size=(n, cvs, 3)).astype(np.float32)
+    pts = np.concatenate([np.cumsum(steps, 1), rng.uniform(0.0, 0.02, (n, cvs, 1)).astype(np.float32)], 2)
+    pts = pts.reshape(-1, 4).astype(np.float32)
+    roots = np.zeros(n, abi.ROOT_DTYPE)
+    roots["uv"] = rng.uniform(0, 1, (n, 2))
+    roots["triangle_index"] = rng.integers(0, 1000, n)
+    roots["surface_face_id"] = np.where(rng.uniform(size=n) < 0.5, abi.INVALID_INDEX, rng.integers(0, 99, n))
+    columns = rng.uniform(0.0, 1.0, (n, 2)).astype(np.float32)
+    ids = rng.permutation(n).astype(np.uint32)
+    prefixes = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    for k, sources in enumerate(PLANS):
+        plan = classic.ClassicPlan(**sources, description_id=11, description_name="walks", ptex_count=2)
+        want = orc.classic_runtime(pts, cvs, plan, roots=roots, primitive_ids=ids, random_prefixes=prefixes,
+                                   columns=columns)
+        assert want["strand_count"] is not None
+        got = run_gpu(torch, plan, pts, cvs, roots=roots, ids=ids, prefixes=prefixes, columns=columns)
+        same(got, want, f"{cvs} CVs, plan {k}")
+
+
+def test_generated_plane(torch, orc):
+    """The generator's own curves (16 CVs, two octaves) through a full plan."""
+    w = workloads.config2(strands=30000, quads=64, guides_per_side=16)
+    blob = nx.build_asset(w.build)
+    gen = nx.PackedOutputs(w.params.strand_count, 16)
+    nx.generate_packed(nx.DeviceAsset(blob), w.params, gen, octaves=w.octaves)
+    host = gen.to_host()
+    plan = classic.ClassicPlan(**PLANS[0], description_id=2, description_name="plane", ptex_count=2)
+    n = w.params.strand_count
+    rng = np.random.default_rng(9)
+    columns = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    prefixes = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    out = classic.classic_runtime(gen.points, 16, plan, roots=gen.roots, root_uvs=gen.root_uvs,
+                                  point_counts=gen.point_counts, random_prefixes=dev(torch, prefixes, np.int32),
+                                  columns=dev(torch, columns), radius_scale=1.0)
+    got = out.to_host()
+    want = orc.classic_runtime(host["points"], 16, plan, roots=host["roots"], counts=host["point_counts"],
+                               uvs=host["root_uvs"], random_prefixes=prefixes, columns=columns)
+    assert 0 < want["strand_count"] <= n
+    same(got, want, "generated plane")
+    assert np.array_equal(got["root_uvs"], want["root_uvs"])
+
+
+def test_variable_counts_and_reuse(torch, orc):
+    """Variable point counts raise the reference's error and write nothing;
+    the same outputs and workspace then serve a valid call."""
+    rng = np.random.default_rng(2)
+    n, cvs = 100, 6
+    pts = rng.normal(size=(n * cvs, 4)).astype(np.float32)
+    pts[:, 3] = 0.01
+    roots = np.zeros(n, abi.ROOT_DTYPE)
+    plan = classic.ClassicPlan(taper="0.5")
+    counts = np.full(n, cvs, np.uint32)
+    counts[37] = cvs - 1
+    T = torch
+    outputs = classic.ClassicCutOutputs(n, cvs, root_uvs=False)
+    ws = T.empty(classic.runtime_workspace_bytes(n), dtype=T.uint8, device="cuda")
+    kw = dict(roots=dev(T, roots.view(np.int32).reshape(-1, 12)), outputs=outputs, workspace=ws)
+    classic.classic_runtime(dev(T, pts), cvs, plan, point_counts=dev(T, counts, np.int32), **kw)
+    assert outputs.kept() == 0 and outputs.status() == abi.CLASSIC_VARIABLE_COUNTS
+    counts[37] = cvs
+    classic.classic_runtime(dev(T, pts), cvs, plan, point_counts=dev(T, counts, np.int32), **kw)
+    got = outputs.to_host()
+    want = orc.classic_runtime(pts, cvs, plan, roots=roots, counts=counts)
+    same(got, want, "after an error")
