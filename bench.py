@@ -387,6 +387,143 @@ def bench_classic(args):
     return 0
 
 
+# The Classic plan of the classic_expr workload: authored form (the
+# reference's .xgen attributes) and bound form (map -> input columns).
+CLASSIC_EXPR_AUTHORED = dict(length="1 + 0.5 * map('len.ptx')", width="0.004 + 0.01 * map('w.ptx')",
+                             taper="0.6", taper_start="0.25", width_ramp="rampUI(0,1,1:0.4,0.7,2:1,0.2,3)",
+                             cut_amount="$cLength * (0.2 + 0.5 * rand(3)) + $cWidth")
+CLASSIC_EXPR_BOUND = dict(CLASSIC_EXPR_AUTHORED, length="1 + 0.5 * $__nxg_map_0",
+                          width="0.004 + 0.01 * $__nxg_map_1")
+CLASSIC_EXPR_ATTRS = (("length", "length"), ("width", "width"), ("taper", "taper"), ("taper_start", "taperStart"),
+                      ("width_ramp", "widthRamp"))
+
+
+def classic_expr_xgen():
+    """The classic_expr plan as a Classic .xgen description."""
+    a = CLASSIC_EXPR_AUTHORED
+    s = ("FileVersion 18\nPalette\n\tname\tpalette\n\tendAttrs\nDescription\n\tname\tplane\n"
+         "\tdescriptionId\t2\n\tendAttrs\nSplinePrimitive\n")
+    s += "".join(f"\t{attr}\t{a[key]}\n" for key, attr in CLASSIC_EXPR_ATTRS) + "\tendAttrs\n"
+    s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{a['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
+    return s + "\tActive\tSplinePrimitive\n"
+
+
+def bench_classic_expr(args):
+    """The Classic runtime with primitive expressions
+    (nxg_classic_runtime_apply) over the plane workload's generated curves:
+    2,000,000 strands x 16 CVs, a length and a width expression over two
+    input columns, taper 0.6 from 0.25, a three-key widthRamp, and a CutFX
+    whose amount calls rand() through per-strand SeExpr prefixes.  One step =
+    the whole chain (sample tables, search with every expression, two scan
+    launches, write with the width profile + compaction); inputs are resident
+    in HBM before the timed region."""
+    import numpy as np
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import workloads
+    from nanoxgen_amd.classic import ClassicCutOutputs, ClassicPlan, classic_runtime, runtime_workspace_bytes
+
+    w = workloads.config2()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    blob = nx.build_asset(w.build)
+    gen = nx.PackedOutputs(N, Cv)
+    nx.generate_packed(nx.DeviceAsset(blob), w.params, gen, octaves=w.octaves)
+    rng = np.random.default_rng(17)
+    columns = rng.uniform(0.0, 1.0, (N, 2)).astype(np.float32)
+    prefixes = rng.integers(0, 2 ** 32, N, dtype=np.uint64).astype(np.uint32)
+    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, description_id=2, description_name="plane", ptex_count=2)
+    cols_d = torch.from_numpy(columns).cuda()
+    pre_d = torch.from_numpy(prefixes.view(np.int32)).cuda()
+    out = ClassicCutOutputs(N, Cv)
+    ws = torch.empty(runtime_workspace_bytes(N), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    log(f"generated {N} x {Cv} curves; plan image {plan.info.byte_size} B")
+
+    def step():
+        classic_runtime(gen.points, Cv, plan, roots=gen.roots, root_uvs=gen.root_uvs, point_counts=gen.point_counts,
+                        random_prefixes=pre_d, columns=cols_d, outputs=out, workspace=ws, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        events[i][0].record(stream)
+        step()
+        events[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    chain_ms = sorted(e[0].elapsed_time(e[1]) for e in events)
+    chain_mean = sum(chain_ms) / len(chain_ms)
+    kept = out.kept()
+    assert out.status() == 0
+    # algorithmic bytes of one chain: every input strand's points, count,
+    # root, uv, two input columns and prefix read once; the 4 B cut parameter
+    # and 32 B state written by the search and read by the write; every kept
+    # strand's points, count, root and uv written
+    per_in = Cv * 16 + 4 + 48 + 8 + 8 + 4
+    per_out = Cv * 16 + 4 + 48 + 8
+    alg = N * per_in + N * 36 * 2 + kept * per_out
+    achieved = alg / (chain_mean * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "runtime chain: cut_table + runtime_search_kernel<true> + cut_scan_tiles + "
+                          "cut_scan_groups + runtime_write_kernel<true> (one C-ABI call)",
+                "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
+                "algorithmic_bytes_per_launch": alg, "valu": None}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, args.cpu_seconds)
+    line = {"metric": METRIC, "value": kept * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "classic-expressions-plane2M", "input": w.name, "strands": N,
+                       "cvs_per_strand": Cv, "kept_strands": kept, "plan": CLASSIC_EXPR_AUTHORED,
+                       "parallelism": "single-gpu"},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s):
+    """The reference's parser, plan compiler and
+    apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread
+    pool) on the first S generated strands; without that build, the oracle's
+    single-threaded restatement."""
+    import oracle
+    from nanoxgen_amd.classic import ClassicPlan
+    host = gen.to_host()
+    pts, counts, roots, uvs = host["points"], host["point_counts"], host["roots"], host["root_uvs"]
+    use_ref = oracle.ref() is not None
+    cores = len(os.sched_getaffinity(0)) if use_ref else 1
+    text = classic_expr_xgen()
+    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, description_id=2, description_name="plane", ptex_count=2)
+
+    def run(s):
+        if use_ref:
+            r = oracle.ref_classic_runtime(text, pts[:s * Cv], Cv, roots=roots[:s], counts=counts[:s], uvs=uvs[:s],
+                                           random_prefixes=prefixes[:s], columns=columns[:s])
+            return r["seconds"]
+        t = time.perf_counter()
+        oracle.classic_runtime(pts[:s * Cv], Cv, plan, roots=roots[:s], counts=counts[:s], uvs=uvs[:s],
+                               random_prefixes=prefixes[:s], columns=columns[:s])
+        return time.perf_counter() - t
+
+    s = min(N, 65536)
+    dt = run(s)
+    s = int(min(N, max(s, s * target_s / max(dt, 1e-6))))
+    times = [run(s)]
+    while s == N and sum(times) < target_s and len(times) < 5:
+        times.append(run(s))
+    dt = sorted(times)[len(times) // 2]
+    what = ("the reference's apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread pool)"
+            if use_ref else "the oracle's restatement (oracle/oracle_classic.c), one thread")
+    return {"value": s * Cv / dt, "unit": "points/s", "cores": cores, "kind": "reference" if use_ref else "port",
+            "sample": f"strands 0..{s - 1} of the generated curves ({s * Cv} points), the same plan, "
+                      f"median {dt:.3f} s of {len(times)} run(s); {what}"}
+
+
 CLUMP_PARAMS = dict(mask=0.8, amount=0.7, clump_scale=0.1, noise=0.3, noise_scale=1.0, noise_frequency=2.0,
                     noise_correlation=20.0)
 
@@ -618,13 +755,16 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="plane", choices=["plane", "expr", "surrogate", "batch", "classic", "clump", "transport"],
+    ap.add_argument("--workload", default="plane",
+                    choices=["plane", "expr", "surrogate", "batch", "classic", "classic_expr", "clump", "transport"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
                          "the trained surrogate + sampled exact check against the exact runtime; "
                          "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph; "
                          "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands; "
+                         "classic_expr: the same strands through a Classic plan of length, width, taper, "
+                         "taperStart, widthRamp and CutFX-amount expressions; "
                          "clump: ClumpingFX toward the plane's 4,096 guides over the same strands; "
                          "transport: 64K renderer batches (transform, motion) and the .nxc cache of the same strands")
     ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
@@ -651,6 +791,8 @@ def main():
         return bench_batch(args)
     if args.workload == "classic" and args.gpus == 1:
         return bench_classic(args)
+    if args.workload == "classic_expr" and args.gpus == 1:
+        return bench_classic_expr(args)
     if args.workload == "clump" and args.gpus == 1:
         return bench_clump(args)
     if args.workload == "transport" and args.gpus == 1:

@@ -940,7 +940,8 @@ typedef struct nxg_classic_plan_info {
     uint32_t description_id;
     uint32_t description_name_seed;
     uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
-    uint32_t reserved;
+    uint32_t uniform;             /* bit k: expressions[k] reads no input and calls no rand(), so its
+                                     value depends on t alone (evaluated once per call) */
 } nxg_classic_plan_info;
 
 /* Validates and packs a plan into a position-independent image (out NULL:

@@ -292,7 +292,7 @@ class ClassicPlanInfo(C.Structure):
     _fields_ = [("byte_size", C.c_uint64), ("present", C.c_uint32), ("has_cut", C.c_uint32),
                 ("columns_required", C.c_uint32), ("needs_random_prefix", C.c_uint32), ("final_length", C.c_uint32),
                 ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32),
-                ("program_offset", C.c_uint32 * len(CLASSIC_EXPRS)), ("reserved", C.c_uint32)]
+                ("program_offset", C.c_uint32 * len(CLASSIC_EXPRS)), ("uniform", C.c_uint32)]
 
 
 class ClassicRuntimeInputs(C.Structure):

@@ -111,6 +111,8 @@ struct CutArgs {
     const float *columns;
     uint32_t column_count;
     float description_id, name_seed;
+    uint32_t uniform;            // programs whose value depends on t alone
+    float2 *uniform_values;      // {value, ok}: [k] program k at t = 0, [8 + cv] widthRamp at CV cv
     struct RuntimeState *state;  // per strand, from the search to the write
     uint32_t *err_bits;          // the NXG_CLASSIC_* bits of every failing strand
 };
@@ -126,7 +128,7 @@ __device__ __forceinline__ uint32_t staged_index(uint32_t c, uint32_t lane) {
 }
 template <bool ROTATED>
 struct StagedCurve {
-    const float4 *tile;
+    float4 *tile;
     uint32_t lane;
     __device__ __forceinline__ float4 operator[](uint32_t c) const { return tile[staged_index<ROTATED>(c, lane)]; }
 };
@@ -562,6 +564,40 @@ __device__ __forceinline__ bool classic_eval(const CutArgs &a, uint32_t k, const
     return e.ok;
 }
 
+// The parameter of CV cv (src/xgen_classic_runtime.cpp:1840-1841).
+__device__ __forceinline__ float cv_parameter(uint32_t cv, uint32_t n) {
+    return static_cast<float>(cv) / static_cast<float>(n - 1u);
+}
+
+// Program k for one strand (widthRamp at CV cv, the others at t = 0): a
+// uniform program's value comes from the call's table, after the context
+// check the reference makes at every evaluation (xgen_expression.cpp:991-994).
+__device__ __forceinline__ bool classic_value(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
+                                              float c_width, uint32_t cv, float &out) {
+    if ((a.uniform >> k) & 1u) {
+        const float2 v = a.uniform_values[k == NXG_CLASSIC_EXPR_WIDTH_RAMP ? 8u + cv : k];
+        out = v.x;
+        return v.y != 0.0f && isfinite(c.expr.u) && isfinite(c.expr.v) && isfinite(c.expr.face_seed);
+    }
+    const float t = k == NXG_CLASSIC_EXPR_WIDTH_RAMP ? cv_parameter(cv, a.cvs) : 0.0f;
+    return classic_eval(a, k, c, c_length, c_width, t, out);
+}
+
+// The plan's uniform programs, once per call: thread j < 8 program j at
+// t = 0, thread 8 + cv widthRamp at CV cv.
+__global__ void __launch_bounds__(256) classic_uniform_kernel(CutArgs a) {
+    for (uint32_t j = threadIdx.x; j < 8u + a.cvs; j += blockDim.x) {
+        const uint32_t k = j < 8u ? j : static_cast<uint32_t>(NXG_CLASSIC_EXPR_WIDTH_RAMP);
+        if (j < 8u && (j >= NXG_CLASSIC_EXPR_COUNT || k == NXG_CLASSIC_EXPR_WIDTH_RAMP)) continue;
+        if (!((a.uniform >> k) & 1u)) continue;
+        ClassicContext c{};
+        const float t = j < 8u ? 0.0f : cv_parameter(j - 8u, a.cvs);
+        float v = 0.0f;
+        const bool ok = classic_eval(a, k, c, 0.0f, 0.0f, t, v);
+        a.uniform_values[j] = make_float2(v, ok ? 1.0f : 0.0f);
+    }
+}
+
 // scale_curve (src/xgen_classic_runtime.cpp:576-585) as a view: positions
 // root + (p - root) * scale when `on` (a length expression ran), radii kept.
 template <class Curve>
@@ -595,6 +631,22 @@ struct RebuiltCurve {
 
 __device__ __forceinline__ bool has_program(const CutArgs &a, uint32_t k) { return (a.present >> k) & 1u; }
 
+// scale_curve applied once to a lane's staged CVs, in place (each lane owns
+// its strand's LDS entries); the same operations as ScaledCurve per read,
+// CV 0 included (a -0 root coordinate becomes +0 there).  Strands read from
+// memory keep the view.
+template <bool ROTATED>
+__device__ __forceinline__ bool scale_in_place(const StagedCurve<ROTATED> &p, uint32_t n, float scale) {
+    const F3 root = xyz(p[0]);
+    for (uint32_t c = 0u; c < n; ++c) {
+        const float4 v = p[c];
+        const F3 q = root + (xyz(v) - root) * scale;
+        p.tile[staged_index<ROTATED>(c, p.lane)] = make_float4(q.x, q.y, q.z, v.w);
+    }
+    return true;
+}
+__device__ __forceinline__ bool scale_in_place(const MemoryCurve &, uint32_t, float) { return false; }
+
 // The taper part of the width profile at t (src/xgen_classic_runtime.cpp:
 // 1842-1848).
 __device__ __forceinline__ float taper_scale(float t, float taper, float taper_start) {
@@ -607,7 +659,7 @@ __device__ __forceinline__ float taper_scale(float t, float taper, float taper_s
 // the cut parameter of a kept strand (1 without a CutFX), -1 for a culled
 // one; *err gets the NXG_CLASSIC_* bit where the reference throws.
 template <class Curve>
-__device__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t s, uint32_t *err) {
+__device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t s, uint32_t *err) {
     const uint32_t n = a.cvs;
     if (a.counts && a.counts[s] != n) {
         *err = NXG_CLASSIC_VARIABLE_COUNTS;
@@ -617,7 +669,7 @@ __device__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t 
     float c_length = spline_length(a.length_samples, source, n);
     float length_scale = 1.0f;
     if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) {
-        if (!classic_eval(a, NXG_CLASSIC_EXPR_LENGTH, c, c_length, 0.0f, 0.0f, length_scale)) {
+        if (!classic_value(a, NXG_CLASSIC_EXPR_LENGTH, c, c_length, 0.0f, 0u, length_scale)) {
             *err = NXG_CLASSIC_EXPRESSION_ERROR;
             return -1.0f;
         }
@@ -627,10 +679,11 @@ __device__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t 
         }
         c_length *= length_scale;
     }
-    const ScaledCurve<Curve> p{source, xyz(source[0]), length_scale, has_program(a, NXG_CLASSIC_EXPR_LENGTH)};
+    const bool scaled = has_program(a, NXG_CLASSIC_EXPR_LENGTH) && !scale_in_place(source, n, length_scale);
+    const ScaledCurve<Curve> p{source, xyz(source[0]), length_scale, scaled};
     float width;
     if (has_program(a, NXG_CLASSIC_EXPR_WIDTH)) {
-        if (!classic_eval(a, NXG_CLASSIC_EXPR_WIDTH, c, c_length, 0.0f, 0.0f, width)) {
+        if (!classic_value(a, NXG_CLASSIC_EXPR_WIDTH, c, c_length, 0.0f, 0u, width)) {
             *err = NXG_CLASSIC_EXPRESSION_ERROR;
             return -1.0f;
         }
@@ -647,7 +700,7 @@ __device__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t 
     bool keep = true;
     if (a.has_cut) {
         float amount;
-        if (!classic_eval(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0.0f, amount)) {
+        if (!classic_value(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0u, amount)) {
             *err = NXG_CLASSIC_EXPRESSION_ERROR;
             return -1.0f;
         }
@@ -662,12 +715,12 @@ __device__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t 
     }
     float taper = 0.0f, taper_start = 0.0f;
     if (has_program(a, NXG_CLASSIC_EXPR_TAPER) &&
-        !classic_eval(a, NXG_CLASSIC_EXPR_TAPER, c, c_length, c_width, 0.0f, taper)) {
+        !classic_value(a, NXG_CLASSIC_EXPR_TAPER, c, c_length, c_width, 0u, taper)) {
         *err = NXG_CLASSIC_EXPRESSION_ERROR;
         return -1.0f;
     }
     if (has_program(a, NXG_CLASSIC_EXPR_TAPER_START) &&
-        !classic_eval(a, NXG_CLASSIC_EXPR_TAPER_START, c, c_length, c_width, 0.0f, taper_start)) {
+        !classic_value(a, NXG_CLASSIC_EXPR_TAPER_START, c, c_length, c_width, 0u, taper_start)) {
         *err = NXG_CLASSIC_EXPRESSION_ERROR;
         return -1.0f;
     }
@@ -677,11 +730,10 @@ __device__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t 
     }
     if (a.profile) {  // the width profile's errors (its values are the write's)
         for (uint32_t cv = 0u; cv < n; ++cv) {
-            const float t = static_cast<float>(cv) / static_cast<float>(n - 1u);
-            float scale = taper_scale(t, taper, taper_start);
+            float scale = taper_scale(cv_parameter(cv, n), taper, taper_start);
             if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
                 float ramp;
-                if (!classic_eval(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, c_length, c_width, t, ramp)) {
+                if (!classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, c_length, c_width, cv, ramp)) {
                     *err = NXG_CLASSIC_EXPRESSION_ERROR;
                     return -1.0f;
                 }
@@ -765,11 +817,10 @@ __global__ void __launch_bounds__(kWave) runtime_write_kernel(CutArgs a) {
                 q = make_float4(r.x, r.y, r.z, q.w);
             }
             if (a.profile) {
-                const float tt = static_cast<float>(i) / static_cast<float>(n - 1u);
-                float scale = taper_scale(tt, st.taper, st.taper_start);
+                float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
                 if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
                     float ramp = 0.0f;
-                    classic_eval(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, tt, ramp);
+                    classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
                     scale *= ramp;
                 }
                 q.w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
@@ -781,6 +832,76 @@ __global__ void __launch_bounds__(kWave) runtime_write_kernel(CutArgs a) {
         write(StagedCurve<false>{tile, lane});
     else
         write(MemoryCurve{a.points + uint64_t{s} * n});
+    if (a.out_counts) a.out_counts[slot] = n;
+    if (a.out_roots) a.out_roots[slot] = a.roots[s];
+    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
+}
+
+// The write with each strand held in registers (n <= MAXN), laid out in
+// output order through the tile's LDS and stored as one contiguous run per
+// wave, as cut_write_held_kernel does.
+template <uint32_t MAXN>
+__global__ void __launch_bounds__(kWave) runtime_write_held_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    const uint32_t s = first + lane;
+    const float cut = lane < count ? a.cut[s] : -1.0f;
+    const bool keep = cut >= 0.0f;
+    const uint64_t kept = __ballot(keep);
+    if (kept == 0u) return;  // the block is this one wavefront
+    stage_tile<false>(a, tile, first, count);
+    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
+    const uint32_t slot0 = a.groups[t / kScanBlock] + a.tiles[t];
+    float4 held[MAXN];
+    RuntimeState st{};
+    if (keep) {  // positions (the input radii ride along)
+        st = a.state[s];
+        const StagedCurve<false> p{tile, lane};
+        if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) scale_in_place(p, n, st.length_scale);
+        const float step = cut / static_cast<float>(n - 1u);
+#pragma unroll
+        for (uint32_t i = 0u; i < MAXN; ++i) {
+            if (i >= n) continue;
+            float4 q = p[i];
+            if (a.has_cut && i > 0u) {
+                const F3 r = curve_eval(p, n, step * static_cast<float>(i));
+                q = make_float4(r.x, r.y, r.z, q.w);
+            }
+            held[i] = q;
+        }
+    }
+    __syncthreads();
+    const uint32_t stride = n + 1u;
+    if (keep) {
+#pragma unroll
+        for (uint32_t i = 0u; i < MAXN; ++i)
+            if (i < n) tile[rank * stride + i] = held[i];
+        if (a.profile) {  // the width profile's radii, in the lane's own LDS row
+            const ClassicContext c = classic_context(a, s);
+            for (uint32_t i = 0u; i < n; ++i) {
+                float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
+                if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
+                    float ramp = 0.0f;
+                    classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
+                    scale *= ramp;
+                }
+                tile[rank * stride + i].w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t total = static_cast<uint32_t>(__popcll(kept)) * n;
+    float4 *dst = a.out_points + uint64_t{slot0} * n;
+    for (uint32_t e = lane; e < total; e += kWave) {
+        const uint32_t r = __umulhi(e, a.div_magic);
+        dst[e] = tile[r * stride + (e - r * n)];
+    }
+    if (!keep) return;
+    const uint32_t slot = slot0 + rank;
     if (a.out_counts) a.out_counts[slot] = n;
     if (a.out_roots) a.out_roots[slot] = a.roots[s];
     if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
@@ -1256,6 +1377,8 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.column_count = inputs->column_count;
     a.description_id = static_cast<float>(plan.description_id);
     a.name_seed = static_cast<float>(plan.description_name_seed);
+    a.uniform = plan.uniform & plan.present;
+    a.uniform_values = reinterpret_cast<float2 *>(ws + classic_runtime_uniform_offset(n));
     a.state = reinterpret_cast<RuntimeState *>(ws + classic_runtime_state_offset(n));
     a.err_bits = a.status_word + 1;
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
@@ -1264,6 +1387,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     if (a.tile_count) {
         hipLaunchKernelGGL(cut_table_kernel, dim3(1), dim3(256), 0, stream, a);
         hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, a.err_bits);
+        if (a.uniform) hipLaunchKernelGGL(classic_uniform_kernel, dim3(1), dim3(256), 0, stream, a);
         if (staged)
             hipLaunchKernelGGL(runtime_search_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
@@ -1272,7 +1396,14 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     }
     hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
     if (a.tile_count) {
-        if (staged)
+        const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);
+        if (cvs <= 8u)
+            hipLaunchKernelGGL(runtime_write_held_kernel<8>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+        else if (cvs <= 16u)
+            hipLaunchKernelGGL(runtime_write_held_kernel<16>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+        else if (cvs <= 32u)
+            hipLaunchKernelGGL(runtime_write_held_kernel<32>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+        else if (staged)
             hipLaunchKernelGGL(runtime_write_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
             hipLaunchKernelGGL(runtime_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);

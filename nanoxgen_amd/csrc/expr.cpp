@@ -362,6 +362,10 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
                 r.final_length = 1u;
         }
         if (p->random_call_count != 0u) r.needs_random_prefix = 1u;
+        bool uniform = p->input_count == 0u;
+        for (uint32_t i = 0; i < p->instruction_count; ++i)
+            if (p->instructions[i].op == NXG_OP_RANDOM) uniform = false;
+        if (uniform) r.uniform |= 1u << k;
     }
     r.byte_size = total ? total : 16u;
     *info = r;

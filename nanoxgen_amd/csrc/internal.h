@@ -85,10 +85,16 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
     return l;
 }
 
-// Classic runtime workspace: the cut layout, then one 32-byte state per
-// strand (the search's per-strand values for the write); the word after the
-// status word collects the error bits.
-inline uint64_t classic_runtime_state_offset(uint32_t strand_count) { return classic_layout(strand_count).total; }
+// Classic runtime workspace: the cut layout, the values of the plan's
+// uniform programs ({value, ok} pairs: one per program at t = 0, then
+// widthRamp at every CV), then one 32-byte state per strand (the search's
+// per-strand values for the write); the word after the status word collects
+// the error bits.
+constexpr uint64_t kClassicUniformBytes = (8u + NXG_CLASSIC_MAX_CVS) * 8u;
+inline uint64_t classic_runtime_uniform_offset(uint32_t strand_count) { return classic_layout(strand_count).total; }
+inline uint64_t classic_runtime_state_offset(uint32_t strand_count) {
+    return classic_runtime_uniform_offset(strand_count) + ((kClassicUniformBytes + 255u) & ~uint64_t{255u});
+}
 inline uint64_t classic_runtime_bytes(uint32_t strand_count) {
     return classic_runtime_state_offset(strand_count) + ((uint64_t{strand_count} * 32u + 255u) & ~uint64_t{255u});
 }

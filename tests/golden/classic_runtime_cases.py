@@ -50,6 +50,9 @@ CASES = [
     # errors: the reference's messages
     dict(name="negativeLength", description_id=1, strands="walk", exprs=dict(length="$id == 50 ? -1 : 1")),
     dict(name="nonfiniteWidth", description_id=1, strands="walk", ids=True, exprs=dict(width="1 / ($id - 5)")),
+    # a widthRamp of t alone that divides by zero at t = 0.5 (CV 2 of 5)
+    dict(name="uniformRampError", description_id=1, strands="ico5",
+         exprs=dict(width_ramp="1 / (0.5 - rampUI(0,0,1:1,1,1))")),
 ]
 
 

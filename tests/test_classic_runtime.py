@@ -141,6 +141,9 @@ def test_plan_report():
     assert info.present == 0b111111 and info.has_cut == 1
     assert info.columns_required == 3 and info.needs_random_prefix == 1 and info.final_length == 0
     assert make_plan(CASES[2]).info.final_length == 1
+    # programs of t alone (no input, no rand) are evaluated once per call
+    assert info.uniform == 1 << 5
+    assert make_plan(CASES[0]).info.uniform == 0b111011
     assert plan.name_seed == nx.lib().nxg_string_seed(b"walkAll")
 
 

@@ -112,6 +112,7 @@ def test_random_walks_oracle(torch, orc, cvs):
     steps = rng.normal(0, 0.3, size=(n, cvs, 3)).astype(np.float32)
     pts = np.concatenate([np.cumsum(steps, 1), rng.uniform(0.0, 0.02, (n, cvs, 1)).astype(np.float32)], 2)
     pts = pts.reshape(-1, 4).astype(np.float32)
+    pts[::7 * cvs, 0] = -0.0  # scale_curve turns a -0 root coordinate into +0
     roots = np.zeros(n, abi.ROOT_DTYPE)
     roots["uv"] = rng.uniform(0, 1, (n, 2))
     roots["triangle_index"] = rng.integers(0, 1000, n)
