@@ -285,6 +285,23 @@ def bench_batch(args):
 
 
 CUT_PMC_SUMMARY = os.path.join(ROOT, "profiles", "cut_search_kernel_plane2M_pmc.json")
+RUNTIME_PMC_SUMMARY = os.path.join(ROOT, "profiles", "runtime_chain_plane2M_classic_expr_pmc.json")
+
+
+def chain_counters(path, chain_ms):
+    """(HBM traffic per chain, VALU roofline dict) from a committed rocprofv3
+    counter summary of the same chain, or (None, None)."""
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        pmc = json.load(f)
+    valu = None
+    insts = pmc.get("valu_instructions_per_chain")
+    if insts:
+        rate = insts / (chain_ms * 1e-3)
+        valu = {"achieved": round(rate / 1e9, 2), "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                "frac": round(rate / 1e9 / VALU_PEAK_GINST, 4), "instructions_per_chain": insts}
+    return pmc.get("hbm_bytes_per_chain"), valu
 
 
 def bench_classic(args):
@@ -336,20 +353,11 @@ def bench_classic(args):
     per_in, per_out = Cv * 16 + 4 + 48 + 8, Cv * 16 + 4 + 48 + 8
     alg = N * per_in + kept * per_out + N * 8
     achieved = alg / (chain_mean * 1e-3) / 1e9
-    valu = traffic = None
-    if os.path.exists(CUT_PMC_SUMMARY):
-        with open(CUT_PMC_SUMMARY) as f:
-            pmc = json.load(f)
-        traffic = pmc.get("hbm_bytes_per_chain")
-        insts = pmc.get("valu_instructions_per_chain")
-        if insts:
-            rate = insts / (chain_mean * 1e-3)
-            valu = {"achieved": round(rate / 1e9, 2), "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
-                    "frac": round(rate / 1e9 / VALU_PEAK_GINST, 4), "instructions_per_chain": insts}
+    traffic, valu = chain_counters(CUT_PMC_SUMMARY, chain_mean)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "cut chain: cut_search_kernel<true> + cut_scan_tiles + cut_scan_groups + "
-                          "cut_write_kernel<true> (one C-ABI call)",
+                "kernel": "cut chain: cut_table + cut_search_kernel<true> + cut_scan_tiles + cut_scan_groups + "
+                          "cut_write_held_kernel<16> (one C-ABI call)",
                 "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
                 "algorithmic_bytes_per_launch": alg, "valu": valu}
     # the renderer endpoint pass on the cut's output, its strand count read on the device
@@ -466,12 +474,13 @@ def bench_classic_expr(args):
     per_out = Cv * 16 + 4 + 48 + 8
     alg = N * per_in + N * 36 * 2 + kept * per_out
     achieved = alg / (chain_mean * 1e-3) / 1e9
+    traffic, valu = chain_counters(RUNTIME_PMC_SUMMARY, chain_mean)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "runtime chain: cut_table + runtime_search_kernel<true> + cut_scan_tiles + "
-                          "cut_scan_groups + runtime_write_kernel<true> (one C-ABI call)",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "runtime chain: cut_table + classic_uniform_kernel + runtime_search_kernel<true> + "
+                          "cut_scan_tiles + cut_scan_groups + runtime_write_held_kernel<16> (one C-ABI call)",
                 "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
-                "algorithmic_bytes_per_launch": alg, "valu": None}
+                "algorithmic_bytes_per_launch": alg, "valu": valu}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, args.cpu_seconds)
