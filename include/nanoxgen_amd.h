@@ -872,17 +872,27 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
 /* apply_xgen_classic_float_runtime_plan_cpu (include/nanoxgen/
  * xgen_classic_runtime.h:185-203, src/xgen_classic_runtime.cpp:1420-1918)
  * for a ClassicFloatRuntimePlan (xgen_classic_runtime.h:130-165) of
- * SplinePrimitive expressions (length, width, taper, taperStart, widthRamp)
- * and at most one CutFX module (rebuildType 1) whose amount is an
- * expression.  Per strand, in the reference's order:
+ * SplinePrimitive expressions (length, width, taper, taperStart, widthRamp),
+ * at most one ClumpingFX module with world-space guides and at most one
+ * CutFX module (rebuildType 1), the clump before the cut.  Per strand, in
+ * the reference's order:
  *   cLength; length expression -> scale_curve about the root, cLength *=
  *   scale (an error when not finite or negative); raw width from the width
  *   expression, else 2 radius[0] / radius_scale; culled (skipped) when
- *   cLength < 1e-4 or width < 1e-4; cWidth = width; CutFX: amount,
- *   cut_from_tip_and_rebuild, culled when the cut parameter < 1e-4, cLength
- *   of the rebuilt curve; taper, taperStart; per CV t = cv / (n - 1),
- *   scale = 1 - taper (t - start) / (1 - start) past the start, times
- *   widthRamp(t), radius = 0.5 max(cWidth scale, 0) radius_scale;
+ *   cLength < 1e-4 or width < 1e-4; cWidth = width;
+ *   ClumpingFX (:1680-1793) for a strand with a guide: mask (clamped to
+ *   [0, 1]; <= 0 leaves the strand alone); noise, noiseFrequency,
+ *   noiseCorrelation and noiseScale(t) in the guide's context (its uv and
+ *   SeExpr prefix, :1041-1186) build the noise axis, of magnitude
+ *   mask noise noiseScale; cut_to_length_and_rebuild to the guide's spline
+ *   length (cLength of the rebuilt curve); clump amount (clamped);
+ *   CVs 1.. blended toward the guide's render axis by
+ *   mask amount (1 - 2 clumpScale(t)), plus the noise axis; cLength again;
+ *   CutFX: amount, cut_from_tip_and_rebuild, culled when the cut parameter
+ *   < 1e-4, cLength of the rebuilt curve; taper, taperStart; per CV t =
+ *   cv / (n - 1), scale = 1 - taper (t - start) / (1 - start) past the
+ *   start, times widthRamp(t), radius = 0.5 max(cWidth scale, 0)
+ *   radius_scale;
  * then the kept strands are compacted in order.  Program inputs bind as the
  * reference's input_value does (src/xgen_classic_runtime.cpp:504-553):
  * NXG_CLASSIC_VAR_ID ($id: the strand's primitive id, else its index),
@@ -906,6 +916,7 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
 #define NXG_CLASSIC_EXPRESSION_ERROR 64u    /* "XGen float expression produced a non-finite value"
                                                (or a non-finite input or context) */
 #define NXG_CLASSIC_VARIABLE_COUNTS 128u    /* "Classic runtime requires fixed point counts" */
+#define NXG_CLASSIC_NONFINITE_CLUMP 256u    /* "Classic ClumpingFX noise magnitude is non-finite" */
 
 enum {
     NXG_CLASSIC_EXPR_LENGTH = 0,
@@ -914,19 +925,29 @@ enum {
     NXG_CLASSIC_EXPR_TAPER,
     NXG_CLASSIC_EXPR_TAPER_START,
     NXG_CLASSIC_EXPR_WIDTH_RAMP,
+    /* the ClumpingFX module (ClassicFloatClumpModule, xgen_classic_runtime.h:43-56) */
+    NXG_CLASSIC_EXPR_CLUMP_MASK,
+    NXG_CLASSIC_EXPR_CLUMP,
+    NXG_CLASSIC_EXPR_CLUMP_SCALE,
+    NXG_CLASSIC_EXPR_CLUMP_NOISE,
+    NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE,
+    NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY,
+    NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION,
     NXG_CLASSIC_EXPR_COUNT
 };
 
 /* Host memory.  expressions[k] NULL: absent.  Bindings are NXG_CLASSIC_VAR_*;
  * programs hold the reference's ops only (no pow/mix/noise3/texel), at most
  * NXG_CLASSIC_MAX_PROGRAM_INPUTS inputs and NXG_CLASSIC_MAX_PROGRAM_SLOTS live
- * values. */
+ * values.  A clump needs all seven of its expressions; its noiseFrequency
+ * and noiseCorrelation may read no input (they are evaluated once per
+ * guide). */
 typedef struct nxg_classic_plan {
     const nxg_expr_program *expressions[NXG_CLASSIC_EXPR_COUNT];
     uint32_t has_cut;               /* the CutFX module; needs expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT] */
+    uint32_t has_clump;             /* the ClumpingFX module (before the CutFX) */
     uint32_t description_id;        /* descriptionId */
     uint32_t description_name_seed; /* nxg_string_seed of the description name after its last ':' */
-    uint32_t reserved;
 } nxg_classic_plan;
 
 /* What nxg_classic_plan_pack reports about a plan. */
@@ -934,14 +955,17 @@ typedef struct nxg_classic_plan_info {
     uint64_t byte_size;           /* of the device image */
     uint32_t present;             /* bit k: expressions[k] given */
     uint32_t has_cut;
+    uint32_t has_clump;
     uint32_t columns_required;    /* 1 + the largest input column read (0: none) */
     uint32_t needs_random_prefix; /* some program calls rand() */
     uint32_t final_length;        /* taper, taperStart or widthRamp reads cLength */
+    uint32_t clump_length;        /* the clump amount or clumpScale reads cLength */
     uint32_t description_id;
     uint32_t description_name_seed;
     uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
     uint32_t uniform;             /* bit k: expressions[k] reads no input and calls no rand(), so its
                                      value depends on t alone (evaluated once per call) */
+    uint32_t reserved;
 } nxg_classic_plan_info;
 
 /* Validates and packs a plan into a position-independent image (out NULL:
@@ -958,6 +982,12 @@ typedef struct nxg_classic_runtime_inputs {
     const float *columns;             /* device, row-major [strand][column_count]; NULL iff column_count 0 */
     uint32_t column_count;            /* >= plan->columns_required */
     float radius_scale;               /* finite, >= 0 */
+    /* the clump's geometry binding (ClassicClumpRuntimeData, xgen_classic_runtime.h:59-94); with a
+     * clump all are required and cvs_per_strand <= NXG_CLUMP_MAX_CVS */
+    const uint32_t *strand_guides;    /* device: guide per strand, NXG_INVALID_INDEX for none */
+    const nxg_clump_guides *guides;   /* host struct of device arrays (world-space guide CVs) */
+    const nxg_vec2 *guide_uvs;        /* device: the guides' uv (the noise expressions' context) */
+    const uint32_t *guide_random_prefixes; /* device: the guides' SeExpr prefixes */
 } nxg_classic_runtime_inputs;
 
 /* Host-side checks (the reference's argument checks,
@@ -965,8 +995,9 @@ typedef struct nxg_classic_runtime_inputs {
 uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nxg_classic_runtime_inputs *inputs,
                                       const nxg_classic_output *output);
 
-/* Device workspace for nxg_classic_runtime_apply. */
-uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count);
+/* Device workspace for nxg_classic_runtime_apply (guide_count 0 without a
+ * clump). */
+uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count, uint32_t cvs_per_strand, uint32_t guide_count);
 
 /* On `stream`, without synchronising or allocating: the chain above, then
  * the stable compaction into `output` (points, point counts, roots, root

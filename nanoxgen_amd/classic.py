@@ -190,7 +190,18 @@ CLASSIC_ATTRIBUTES = {
     "taper": ("taper", "SplinePrimitive"),
     "taper_start": ("taperStart", "SplinePrimitive"),
     "width_ramp": ("widthRamp", "SplinePrimitive"),
+    "clump_mask": ("mask", "ClumpingFXModule"),
+    "clump": ("clump", "ClumpingFXModule"),
+    "clump_scale": ("clumpScale", "ClumpingFXModule"),
+    "clump_noise": ("noise", "ClumpingFXModule"),
+    "clump_noise_scale": ("noiseScale", "ClumpingFXModule"),
+    "clump_noise_frequency": ("noiseFrequency", "ClumpingFXModule"),
+    "clump_noise_correlation": ("noiseCorrelation", "ClumpingFXModule"),
 }
+# ClassicPlan(clump=...) keys, the module's attribute names
+CLUMP_KEYS = {"mask": "clump_mask", "clump": "clump", "clumpScale": "clump_scale", "noise": "clump_noise",
+              "noiseScale": "clump_noise_scale", "noiseFrequency": "clump_noise_frequency",
+              "noiseCorrelation": "clump_noise_correlation"}
 _CLASSIC_INPUT_PREFIXES = ("__nxg_map_", "__nxg_custom_", "__nxg_pref_noise_")
 
 
@@ -210,19 +221,26 @@ def classic_binding(name, ptex_count=0, custom_count=0):
 
 class ClassicPlan:
     """A ClassicFloatRuntimePlan (include/nanoxgen/xgen_classic_runtime.h:
-    130-165) of SplinePrimitive expressions and at most one CutFX module,
-    packed for the device (nxg_classic_plan_pack).  Each expression is text
-    (compiled here with its attribute name and object type) or a
-    CompiledProgram; ptex_count and custom_count place the custom and
-    $Prefg-noise inputs in the input row."""
+    130-165) of SplinePrimitive expressions, at most one ClumpingFX module
+    (`clump`: its seven expressions by attribute name, mask ... noiseCorrelation)
+    and at most one CutFX module, packed for the device
+    (nxg_classic_plan_pack).  Each expression is text (compiled here with its
+    attribute name and object type) or a CompiledProgram; ptex_count and
+    custom_count place the custom and $Prefg-noise inputs in the input row."""
 
     def __init__(self, *, length=None, width=None, taper=None, taper_start=None, width_ramp=None,
-                 cut_amount=None, description_id=0, description_name="", ptex_count=0, custom_count=0):
+                 cut_amount=None, clump=None, description_id=0, description_name="", ptex_count=0,
+                 custom_count=0):
         given = dict(length=length, width=width, cut_amount=cut_amount, taper=taper, taper_start=taper_start,
                      width_ramp=width_ramp)
+        if clump is not None:
+            missing = set(CLUMP_KEYS) - set(clump)
+            if missing:
+                raise ValueError(f"ClumpingFX module needs {sorted(missing)}")
+            given.update({CLUMP_KEYS[k]: v for k, v in clump.items()})
         self.programs = {}
         for key in abi.CLASSIC_EXPRS:
-            src = given[key]
+            src = given.get(key)
             if src is None:
                 continue
             attribute, object_type = CLASSIC_ATTRIBUTES[key]
@@ -231,10 +249,11 @@ class ClassicPlan:
             self.programs[key] = prog.bind_indices(
                 [classic_binding(name, ptex_count, custom_count) for name in prog.inputs])
         self.has_cut = cut_amount is not None
+        self.has_clump = clump is not None
         self.description_id = description_id
         self.name_seed = int(lib().nxg_string_seed(description_name.rsplit(":", 1)[-1].encode()))
-        self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, description_id=description_id,
-                                      description_name_seed=self.name_seed)
+        self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, has_clump=1 if self.has_clump else 0,
+                                      description_id=description_id, description_name_seed=self.name_seed)
         for k, key in enumerate(abi.CLASSIC_EXPRS):
             if key in self.programs:
                 self.struct.expressions[k] = C.addressof(self.programs[key].struct)
@@ -253,30 +272,51 @@ class ClassicPlan:
         return self._device[key]
 
 
-def runtime_workspace_bytes(strands):
-    return int(lib().nxg_classic_runtime_workspace_size(strands))
+def runtime_workspace_bytes(strands, cvs=2, guides=0):
+    return int(lib().nxg_classic_runtime_workspace_size(strands, cvs, guides))
 
 
 def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=None, primitive_ids=None,
-                    random_prefixes=None, columns=None, radius_scale=1.0, outputs=None, workspace=None,
-                    stream=None):
+                    random_prefixes=None, columns=None, radius_scale=1.0, strand_guides=None, guides=None,
+                    guide_uvs=None, guide_random_prefixes=None, outputs=None, workspace=None, stream=None):
     """Enqueue the Classic runtime chain of `plan` (a ClassicPlan) on
     `stream` over device curves: `points` float32 [N * cvs, 4], `roots` int32
     [N, 12] (RootSample records: the context's u, v and face); optional
     `point_counts` int32 [N], `root_uvs` float32 [N, 2], `primitive_ids`
     int32 [N] ($id), `random_prefixes` int32 [N] (SeExpr prefixes, required
     when a program calls rand()) and `columns` float32 [N, K] (the input
-    row).  Returns the ClassicCutOutputs (its kept(), status() and to_host()
-    synchronise)."""
+    row).  A plan with a clump also takes `strand_guides` int32 [N],
+    `guides` (dict of float32 device tensors axes [G * cvs, 3], normals and
+    tangents [G, 3], optional reference [G, 3]), `guide_uvs` float32 [G, 2]
+    and `guide_random_prefixes` int32 [G].  Returns the ClassicCutOutputs
+    (its kept(), status() and to_host() synchronise)."""
     torch = _torch()
     if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
         raise ValueError("points must be a contiguous float32 [N * cvs, 4] tensor")
     n = points.shape[0] // cvs
+    guide_count = 0
+    guide_struct = None
+    if guides is not None:
+        guide_count = guides["normals"].shape[0]
+        for name, x, k in (("axes", guides["axes"], guide_count * cvs * 3), ("normals", guides["normals"], guide_count * 3),
+                           ("tangents", guides["tangents"], guide_count * 3),
+                           ("reference", guides.get("reference"), guide_count * 3),
+                           ("guide_uvs", guide_uvs, guide_count * 2),
+                           ("guide_random_prefixes", guide_random_prefixes, guide_count),
+                           ("strand_guides", strand_guides, n)):
+            if x is not None and (x.numel() < k or not x.is_contiguous() or x.element_size() != 4):
+                raise ValueError(f"{name} must be contiguous 32-bit with {k} elements")
+        ref = guides.get("reference")
+        guide_struct = abi.ClumpGuides(axes=guides["axes"].data_ptr(), normals=guides["normals"].data_ptr(),
+                                       tangents=guides["tangents"].data_ptr(),
+                                       reference_positions=None if ref is None else ref.data_ptr(),
+                                       guide_count=guide_count)
     if outputs is None:
         outputs = ClassicCutOutputs(n, cvs, device=points.device, point_counts=True, roots=True,
                                     root_uvs=root_uvs is not None)
     if workspace is None:
-        workspace = torch.empty(runtime_workspace_bytes(n), dtype=torch.uint8, device=points.device)
+        workspace = torch.empty(runtime_workspace_bytes(n, cvs, guide_count if plan.has_clump else 0),
+                                dtype=torch.uint8, device=points.device)
     column_count = 0 if columns is None else (columns.shape[1] if columns.dim() == 2 else 0)
     for name, x, k in (("roots", roots, 12), ("point_counts", point_counts, 1), ("root_uvs", root_uvs, 2),
                        ("primitive_ids", primitive_ids, 1), ("random_prefixes", random_prefixes, 1),
@@ -290,7 +330,9 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
     inputs = abi.ClassicRuntimeInputs(plan_image=image.data_ptr(), plan=C.addressof(plan.info),
                                       primitive_ids=t(primitive_ids), random_prefixes=t(random_prefixes),
                                       columns=t(columns) if column_count else None, column_count=column_count,
-                                      radius_scale=radius_scale)
+                                      radius_scale=radius_scale, strand_guides=t(strand_guides),
+                                      guides=None if guide_struct is None else C.addressof(guide_struct),
+                                      guide_uvs=t(guide_uvs), guide_random_prefixes=t(guide_random_prefixes))
     out = outputs.descriptor()
     check(lib().nxg_classic_runtime_apply(C.byref(curves), C.byref(inputs), C.byref(out),
                                           C.c_void_p(workspace.data_ptr()), workspace.numel(),

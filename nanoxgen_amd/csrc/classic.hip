@@ -112,9 +112,19 @@ struct CutArgs {
     uint32_t column_count;
     float description_id, name_seed;
     uint32_t uniform;            // programs whose value depends on t alone
-    float2 *uniform_values;      // {value, ok}: [k] program k at t = 0, [8 + cv] widthRamp at CV cv
+    float2 *uniform_values;      // {value, ok} per uniform program (uniform_slot)
     struct RuntimeState *state;  // per strand, from the search to the write
     uint32_t *err_bits;          // the NXG_CLASSIC_* bits of every failing strand
+    // the ClumpingFX module
+    uint32_t has_clump, clump_length, guide_count;
+    const uint32_t *strand_guides;
+    const nxg_vec3 *guide_axes, *guide_normals, *guide_tangents, *guide_reference;
+    const nxg_vec2 *guide_uvs;
+    const uint32_t *guide_prefixes;
+    float4 *clumped;             // [strands x cvs] every clumped (or passed-through) strand
+    float4 *guide_render;        // [guides x cvs] render axes
+    float4 *guide_noise;         // [guides x cvs] unit clump-noise axes
+    float2 *guide_info;          // [guides] {spline length, 1 if its noise frequency/correlation evaluated}
 };
 
 // A lane's CVs: staged in LDS (CV-major; ROTATED: row c turned by c lanes),
@@ -460,453 +470,6 @@ __global__ void __launch_bounds__(kWave) cut_write_held_kernel(CutArgs a) {
     if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
 }
 
-// ---- Classic primitive expressions ---------------------------------------
-// The plan's programs run per strand in the search (length, width, the cut
-// amount, taper, taperStart, and widthRamp at every CV for its error
-// checks) and widthRamp again per CV in the write, which produces the radii.
-
-// What the search hands the write for a strand.
-struct RuntimeState {
-    float length_scale;  // the length expression's value (1 without one)
-    float c_length;      // cLength after the effects, as the post-effect programs read it
-    float c_width;
-    float taper, taper_start;
-    float reserved[3];
-};
-
-constexpr uint32_t kClassicSlots = NXG_CLASSIC_MAX_PROGRAM_SLOTS;
-
-// The per-strand context of the Classic binding (input_value and the float
-// context, src/xgen_classic_runtime.cpp:504-574, 1628-1640).
-struct ClassicContext {
-    float id;
-    ExprContext expr;
-    const float *row;  // the strand's input row (columns)
-};
-
-__device__ __forceinline__ ClassicContext classic_context(const CutArgs &a, uint32_t s) {
-    ClassicContext c;
-    c.id = static_cast<float>(a.ids ? a.ids[s] : s);
-    const nxg_root_sample r = a.roots[s];
-    const uint32_t face = r.surface_face_id == NXG_INVALID_INDEX ? r.triangle_index : r.surface_face_id;
-    c.expr.u = r.uv.x;
-    c.expr.v = r.uv.y;
-    // xgen_runtime_face_seed (src/xgen_expression.cpp:724-736)
-    c.expr.face_seed =
-        hash_to_unit(hash_step(hash_step(hash_step(0u, a.description_id), a.name_seed), static_cast<float>(face)));
-    c.expr.t = 0.0f;
-    c.expr.random_prefix = a.prefixes ? a.prefixes[s] : 0u;
-    c.expr.has_random_prefix = a.prefixes != nullptr;
-    c.row = a.columns ? a.columns + uint64_t{s} * a.column_count : nullptr;
-    return c;
-}
-
-// Program k of the plan for one strand (evaluate_runtime_expression,
-// src/xgen_classic_runtime.cpp:555-574): false where the reference throws
-// (a non-finite context, input or value).  Input i sits in value slot i; the
-// slots are a register array indexed with wave-uniform slot numbers.
-struct Evaluated {
-    float value;
-    bool ok;
-};
-__device__ Evaluated classic_program(const uint8_t *plan_image, uint32_t program_offset, ClassicContext c,
-                                     float c_length, float c_width, float t) {
-    const Image image = (Image)plan_image;
-    ImageOf(PackedProgram) prog = (ImageOf(PackedProgram))(image + program_offset);
-    ImageOf(PackedInstruction) ins = (ImageOf(PackedInstruction))(image + prog->instructions_offset);
-    ImageOf(uint32_t) operand_slots = (ImageOf(uint32_t))(image + prog->operands_offset);
-    ImageOf(nxg_expr_ramp) ramps = (ImageOf(nxg_expr_ramp))(image + prog->ramps_offset);
-    ImageOf(nxg_expr_ramp_point) points = (ImageOf(nxg_expr_ramp_point))(image + prog->ramp_points_offset);
-    ExprContext ctx = c.expr;
-    ctx.t = t;
-    bool bad = !isfinite(ctx.u) || !isfinite(ctx.v) || !isfinite(ctx.face_seed) || !isfinite(t);
-    float regs[kClassicSlots];
-#define SLOT_GET(x) regs[__builtin_amdgcn_readfirstlane(x) & (kClassicSlots - 1u)]
-    const uint32_t inputs = prog->input_count;
-    for (uint32_t i = 0; i < inputs; ++i) {
-        const uint32_t b = prog->input_bindings[i];
-        const float v = b == NXG_CLASSIC_VAR_ID        ? c.id
-                        : b == NXG_CLASSIC_VAR_CLENGTH ? c_length
-                        : b == NXG_CLASSIC_VAR_CWIDTH  ? c_width
-                                                       : c.row[b - NXG_CLASSIC_VAR_COLUMN0];
-        if (!isfinite(v)) bad = true;
-        SLOT_GET(i) = v;
-    }
-    const uint32_t n = prog->instruction_count;
-    PackedInstruction next = load_instruction(ins);
-    for (uint32_t i = 0; i < n; ++i) {
-        const PackedInstruction in = next;
-        next = load_instruction(ins + i + 1u);
-#define OPERAND(e) (((e) & kImmediate) ? (((e) & 1u) ? in.immediate1 : in.immediate) : SLOT_GET(e))
-#define EXPR_OPV(j) OPERAND(uint32_t{in.slot[j]})
-#define EXPR_OPA(j) OPERAND(operand_slots[in.operand_offset + (j)])
-#define EXPR_NOISE3(x, y, z) __builtin_nanf("")  // no extension ops in a Classic plan
-#define EXPR_TEXEL(map, f, u, v) __builtin_nanf("")
-        float value = 0.0f;
-#include "expr_ops.inc"
-#undef EXPR_OPV
-#undef EXPR_OPA
-#undef EXPR_NOISE3
-#undef EXPR_TEXEL
-#undef OPERAND
-        if (!isfinite(value)) bad = true;
-        SLOT_GET(uint32_t{in.dest}) = value;
-    }
-    const float out = SLOT_GET(prog->result);
-#undef SLOT_GET
-    return {out, !bad};
-}
-
-__device__ __forceinline__ bool classic_eval(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
-                                             float c_width, float t, float &out) {
-    const Evaluated e = classic_program(a.image, a.program_offset[k], c, c_length, c_width, t);
-    out = e.value;
-    return e.ok;
-}
-
-// The parameter of CV cv (src/xgen_classic_runtime.cpp:1840-1841).
-__device__ __forceinline__ float cv_parameter(uint32_t cv, uint32_t n) {
-    return static_cast<float>(cv) / static_cast<float>(n - 1u);
-}
-
-// Program k for one strand (widthRamp at CV cv, the others at t = 0): a
-// uniform program's value comes from the call's table, after the context
-// check the reference makes at every evaluation (xgen_expression.cpp:991-994).
-__device__ __forceinline__ bool classic_value(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
-                                              float c_width, uint32_t cv, float &out) {
-    if ((a.uniform >> k) & 1u) {
-        const float2 v = a.uniform_values[k == NXG_CLASSIC_EXPR_WIDTH_RAMP ? 8u + cv : k];
-        out = v.x;
-        return v.y != 0.0f && isfinite(c.expr.u) && isfinite(c.expr.v) && isfinite(c.expr.face_seed);
-    }
-    const float t = k == NXG_CLASSIC_EXPR_WIDTH_RAMP ? cv_parameter(cv, a.cvs) : 0.0f;
-    return classic_eval(a, k, c, c_length, c_width, t, out);
-}
-
-// The plan's uniform programs, once per call: thread j < 8 program j at
-// t = 0, thread 8 + cv widthRamp at CV cv.
-__global__ void __launch_bounds__(256) classic_uniform_kernel(CutArgs a) {
-    for (uint32_t j = threadIdx.x; j < 8u + a.cvs; j += blockDim.x) {
-        const uint32_t k = j < 8u ? j : static_cast<uint32_t>(NXG_CLASSIC_EXPR_WIDTH_RAMP);
-        if (j < 8u && (j >= NXG_CLASSIC_EXPR_COUNT || k == NXG_CLASSIC_EXPR_WIDTH_RAMP)) continue;
-        if (!((a.uniform >> k) & 1u)) continue;
-        ClassicContext c{};
-        const float t = j < 8u ? 0.0f : cv_parameter(j - 8u, a.cvs);
-        float v = 0.0f;
-        const bool ok = classic_eval(a, k, c, 0.0f, 0.0f, t, v);
-        a.uniform_values[j] = make_float2(v, ok ? 1.0f : 0.0f);
-    }
-}
-
-// scale_curve (src/xgen_classic_runtime.cpp:576-585) as a view: positions
-// root + (p - root) * scale when `on` (a length expression ran), radii kept.
-template <class Curve>
-struct ScaledCurve {
-    Curve base;
-    F3 root;
-    float scale;
-    bool on;
-    __device__ __forceinline__ float4 operator[](uint32_t c) const {
-        const float4 v = base[c];
-        if (!on) return v;
-        const F3 q = root + (xyz(v) - root) * scale;
-        return make_float4(q.x, q.y, q.z, v.w);
-    }
-};
-
-// The curve cut_from_tip_and_rebuild leaves (src/xgen_classic_runtime.cpp:
-// 729-742) as a view: CV i at parameter i cut / (n - 1), every CV at the
-// root when the walk removed the whole strand (cut 0).
-template <class Curve>
-struct RebuiltCurve {
-    Curve source;
-    uint32_t n;
-    float cut;
-    __device__ __forceinline__ float4 operator[](uint32_t i) const {
-        if (cut <= 0.0f || i == 0u) return source[0];
-        const F3 q = curve_eval(source, n, cut / static_cast<float>(n - 1u) * static_cast<float>(i));
-        return make_float4(q.x, q.y, q.z, 0.0f);
-    }
-};
-
-__device__ __forceinline__ bool has_program(const CutArgs &a, uint32_t k) { return (a.present >> k) & 1u; }
-
-// scale_curve applied once to a lane's staged CVs, in place (each lane owns
-// its strand's LDS entries); the same operations as ScaledCurve per read,
-// CV 0 included (a -0 root coordinate becomes +0 there).  Strands read from
-// memory keep the view.
-template <bool ROTATED>
-__device__ __forceinline__ bool scale_in_place(const StagedCurve<ROTATED> &p, uint32_t n, float scale) {
-    const F3 root = xyz(p[0]);
-    for (uint32_t c = 0u; c < n; ++c) {
-        const float4 v = p[c];
-        const F3 q = root + (xyz(v) - root) * scale;
-        p.tile[staged_index<ROTATED>(c, p.lane)] = make_float4(q.x, q.y, q.z, v.w);
-    }
-    return true;
-}
-__device__ __forceinline__ bool scale_in_place(const MemoryCurve &, uint32_t, float) { return false; }
-
-// The taper part of the width profile at t (src/xgen_classic_runtime.cpp:
-// 1842-1848).
-__device__ __forceinline__ float taper_scale(float t, float taper, float taper_start) {
-    float scale = 1.0f;
-    if (t > taper_start && taper_start < 1.0f) scale *= 1.0f - taper * ((t - taper_start) / (1.0f - taper_start));
-    return scale;
-}
-
-// One strand of the runtime chain (src/xgen_classic_runtime.cpp:1619-1858):
-// the cut parameter of a kept strand (1 without a CutFX), -1 for a culled
-// one; *err gets the NXG_CLASSIC_* bit where the reference throws.
-template <class Curve>
-__device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t s, uint32_t *err) {
-    const uint32_t n = a.cvs;
-    if (a.counts && a.counts[s] != n) {
-        *err = NXG_CLASSIC_VARIABLE_COUNTS;
-        return -1.0f;
-    }
-    const ClassicContext c = classic_context(a, s);
-    float c_length = spline_length(a.length_samples, source, n);
-    float length_scale = 1.0f;
-    if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) {
-        if (!classic_value(a, NXG_CLASSIC_EXPR_LENGTH, c, c_length, 0.0f, 0u, length_scale)) {
-            *err = NXG_CLASSIC_EXPRESSION_ERROR;
-            return -1.0f;
-        }
-        if (!isfinite(length_scale) || length_scale < 0.0f) {
-            *err = NXG_CLASSIC_NEGATIVE_LENGTH;
-            return -1.0f;
-        }
-        c_length *= length_scale;
-    }
-    const bool scaled = has_program(a, NXG_CLASSIC_EXPR_LENGTH) && !scale_in_place(source, n, length_scale);
-    const ScaledCurve<Curve> p{source, xyz(source[0]), length_scale, scaled};
-    float width;
-    if (has_program(a, NXG_CLASSIC_EXPR_WIDTH)) {
-        if (!classic_value(a, NXG_CLASSIC_EXPR_WIDTH, c, c_length, 0.0f, 0u, width)) {
-            *err = NXG_CLASSIC_EXPRESSION_ERROR;
-            return -1.0f;
-        }
-    } else {
-        width = a.radius_scale > 0.0f ? 2.0f * source[0].w / a.radius_scale : 0.0f;
-    }
-    if (!isfinite(width)) {
-        *err = NXG_CLASSIC_NONFINITE_WIDTH;
-        return -1.0f;
-    }
-    if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
-    const float c_width = width;
-    float cut = 1.0f;
-    bool keep = true;
-    if (a.has_cut) {
-        float amount;
-        if (!classic_value(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0u, amount)) {
-            *err = NXG_CLASSIC_EXPRESSION_ERROR;
-            return -1.0f;
-        }
-        if (!isfinite(amount)) {
-            *err = NXG_CLASSIC_NONFINITE_AMOUNT;
-            return -1.0f;
-        }
-        cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
-        keep = cut >= 1.0e-4f;
-        // the post-effect programs read the rebuilt curve's cLength
-        if (a.final_length) c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
-    }
-    float taper = 0.0f, taper_start = 0.0f;
-    if (has_program(a, NXG_CLASSIC_EXPR_TAPER) &&
-        !classic_value(a, NXG_CLASSIC_EXPR_TAPER, c, c_length, c_width, 0u, taper)) {
-        *err = NXG_CLASSIC_EXPRESSION_ERROR;
-        return -1.0f;
-    }
-    if (has_program(a, NXG_CLASSIC_EXPR_TAPER_START) &&
-        !classic_value(a, NXG_CLASSIC_EXPR_TAPER_START, c, c_length, c_width, 0u, taper_start)) {
-        *err = NXG_CLASSIC_EXPRESSION_ERROR;
-        return -1.0f;
-    }
-    if (!isfinite(taper) || !isfinite(taper_start)) {
-        *err = NXG_CLASSIC_NONFINITE_TAPER;
-        return -1.0f;
-    }
-    if (a.profile) {  // the width profile's errors (its values are the write's)
-        for (uint32_t cv = 0u; cv < n; ++cv) {
-            float scale = taper_scale(cv_parameter(cv, n), taper, taper_start);
-            if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
-                float ramp;
-                if (!classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, c_length, c_width, cv, ramp)) {
-                    *err = NXG_CLASSIC_EXPRESSION_ERROR;
-                    return -1.0f;
-                }
-                scale *= ramp;
-            }
-            if (!isfinite(c_width * scale)) {
-                *err = NXG_CLASSIC_NONFINITE_PROFILE;
-                return -1.0f;
-            }
-        }
-    }
-    RuntimeState st{};
-    st.length_scale = length_scale;
-    st.c_length = c_length;
-    st.c_width = c_width;
-    st.taper = taper;
-    st.taper_start = taper_start;
-    a.state[s] = st;
-    return keep ? cut : -1.0f;
-}
-
-template <bool STAGED>
-__global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
-    extern __shared__ float4 tile[];
-    const uint32_t n = a.cvs;
-    const uint32_t t = blockIdx.x, lane = threadIdx.x;
-    const uint32_t first = t * kWave;
-    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
-    if constexpr (STAGED) stage_tile<true>(a, tile, first, count);
-    const uint32_t s = first + lane;
-    float cut = -1.0f;
-    uint32_t err = 0u;
-    if (lane < count) {
-        if constexpr (STAGED)
-            cut = runtime_strand(a, StagedCurve<true>{tile, lane}, s, &err);
-        else
-            cut = runtime_strand(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
-        a.cut[s] = cut;
-    }
-    const uint64_t kept = __ballot(cut >= 0.0f);
-    uint32_t bits = 0u;
-#pragma unroll
-    for (uint32_t b = 0u; b < 8u; ++b)
-        if (__ballot((err >> b) & 1u)) bits |= 1u << b;
-    if (lane == 0u) {
-        a.tiles[t] = static_cast<uint32_t>(__popcll(kept)) | (bits ? 1u << kErrShift : 0u);
-        if (bits) atomicOr(a.err_bits, bits);
-    }
-}
-
-// The kept strands at their compacted slots: the scaled (and, with a CutFX,
-// rebuilt) positions, and radii from the width profile when the plan has
-// one (src/xgen_classic_runtime.cpp:1836-1858), else the input radii.
-template <bool STAGED>
-__global__ void __launch_bounds__(kWave) runtime_write_kernel(CutArgs a) {
-    extern __shared__ float4 tile[];
-    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
-    const uint32_t n = a.cvs;
-    const uint32_t t = blockIdx.x, lane = threadIdx.x;
-    const uint32_t first = t * kWave;
-    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
-    const uint32_t s = first + lane;
-    const float cut = lane < count ? a.cut[s] : -1.0f;
-    const uint64_t kept = __ballot(cut >= 0.0f);
-    if (kept == 0u) return;  // the block is this one wavefront
-    if constexpr (STAGED) stage_tile<false>(a, tile, first, count);
-    if (!(cut >= 0.0f)) return;
-    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
-    const uint32_t slot = a.groups[t / kScanBlock] + a.tiles[t] + rank;
-    const RuntimeState st = a.state[s];
-    const ClassicContext c = classic_context(a, s);
-    float4 *dst = a.out_points + uint64_t{slot} * n;
-    auto write = [&](const auto &source) {
-        using Curve = typename std::decay<decltype(source)>::type;
-        const ScaledCurve<Curve> p{source, xyz(source[0]), st.length_scale, has_program(a, NXG_CLASSIC_EXPR_LENGTH)};
-        const float step = cut / static_cast<float>(n - 1u);
-        for (uint32_t i = 0u; i < n; ++i) {
-            float4 q = p[i];
-            if (a.has_cut && i > 0u) {
-                const F3 r = curve_eval(p, n, step * static_cast<float>(i));
-                q = make_float4(r.x, r.y, r.z, q.w);
-            }
-            if (a.profile) {
-                float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
-                if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
-                    float ramp = 0.0f;
-                    classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
-                    scale *= ramp;
-                }
-                q.w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
-            }
-            dst[i] = q;
-        }
-    };
-    if constexpr (STAGED)
-        write(StagedCurve<false>{tile, lane});
-    else
-        write(MemoryCurve{a.points + uint64_t{s} * n});
-    if (a.out_counts) a.out_counts[slot] = n;
-    if (a.out_roots) a.out_roots[slot] = a.roots[s];
-    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
-}
-
-// The write with each strand held in registers (n <= MAXN), laid out in
-// output order through the tile's LDS and stored as one contiguous run per
-// wave, as cut_write_held_kernel does.
-template <uint32_t MAXN>
-__global__ void __launch_bounds__(kWave) runtime_write_held_kernel(CutArgs a) {
-    extern __shared__ float4 tile[];
-    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
-    const uint32_t n = a.cvs;
-    const uint32_t t = blockIdx.x, lane = threadIdx.x;
-    const uint32_t first = t * kWave;
-    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
-    const uint32_t s = first + lane;
-    const float cut = lane < count ? a.cut[s] : -1.0f;
-    const bool keep = cut >= 0.0f;
-    const uint64_t kept = __ballot(keep);
-    if (kept == 0u) return;  // the block is this one wavefront
-    stage_tile<false>(a, tile, first, count);
-    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
-    const uint32_t slot0 = a.groups[t / kScanBlock] + a.tiles[t];
-    float4 held[MAXN];
-    RuntimeState st{};
-    if (keep) {  // positions (the input radii ride along)
-        st = a.state[s];
-        const StagedCurve<false> p{tile, lane};
-        if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) scale_in_place(p, n, st.length_scale);
-        const float step = cut / static_cast<float>(n - 1u);
-#pragma unroll
-        for (uint32_t i = 0u; i < MAXN; ++i) {
-            if (i >= n) continue;
-            float4 q = p[i];
-            if (a.has_cut && i > 0u) {
-                const F3 r = curve_eval(p, n, step * static_cast<float>(i));
-                q = make_float4(r.x, r.y, r.z, q.w);
-            }
-            held[i] = q;
-        }
-    }
-    __syncthreads();
-    const uint32_t stride = n + 1u;
-    if (keep) {
-#pragma unroll
-        for (uint32_t i = 0u; i < MAXN; ++i)
-            if (i < n) tile[rank * stride + i] = held[i];
-        if (a.profile) {  // the width profile's radii, in the lane's own LDS row
-            const ClassicContext c = classic_context(a, s);
-            for (uint32_t i = 0u; i < n; ++i) {
-                float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
-                if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
-                    float ramp = 0.0f;
-                    classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
-                    scale *= ramp;
-                }
-                tile[rank * stride + i].w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
-            }
-        }
-    }
-    __syncthreads();
-    const uint32_t total = static_cast<uint32_t>(__popcll(kept)) * n;
-    float4 *dst = a.out_points + uint64_t{slot0} * n;
-    for (uint32_t e = lane; e < total; e += kWave) {
-        const uint32_t r = __umulhi(e, a.div_magic);
-        dst[e] = tile[r * stride + (e - r * n)];
-    }
-    if (!keep) return;
-    const uint32_t slot = slot0 + rank;
-    if (a.out_counts) a.out_counts[slot] = n;
-    if (a.out_roots) a.out_roots[slot] = a.roots[s];
-    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
-}
-
 // add_xgen_classic_renderer_endpoints (src/xgen_classic_runtime.cpp:
 // 1921-1956): one 256-thread block per 64-strand tile, each thread one
 // output point of the tile's contiguous 64 x (n + 2) run (input and output
@@ -1037,13 +600,15 @@ __device__ F3 rotate_between(F3 value, F3 from, F3 to, float fraction) {
 // axis and spline length (src/xgen_classic_runtime.cpp:839-875), then the
 // guide's clump-noise axis (build_clump_noise_axis, 1041-1186), shared by
 // every strand the guide clumps since the module's expressions are constants.
-__global__ void __launch_bounds__(64) clump_guide_kernel(ClumpArgs a) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.guides) return;
-    const uint32_t n = a.cvs;
-    const Vec3Curve src{a.axes + uint64_t{g} * n};
-    float4 *render = a.render + uint64_t{g} * n;
-    float4 *noise = a.noise + uint64_t{g} * n;
+// One guide's render axis (CVs at cv / (n - 1) of its spline), spline
+// length and clump-noise axis (build_clump_noise_axis, src/xgen_classic_
+// runtime.cpp:1041-1186) scaled by `magnitude` (1: the unit axis, which a
+// caller scales per strand in the reference's order); magnitude 0 writes
+// the zero axis without building the frame.
+__device__ void clump_guide_axes(const nxg_vec3 *axes, uint32_t n, nxg_vec3 domain_point, nxg_vec3 raw_normal,
+                                 nxg_vec3 raw_tangent, float frequency, float correlation, float magnitude,
+                                 float4 *render, float4 *noise, float *length) {
+    const Vec3Curve src{axes};
     render[0] = src[0];
     for (uint32_t cv = 1u; cv + 1u < n; ++cv) {
         const F3 q = curve_eval(src, n, static_cast<float>(cv) / static_cast<float>(n - 1u));
@@ -1061,19 +626,18 @@ __global__ void __launch_bounds__(64) clump_guide_kernel(ClumpArgs a) {
             previous = current;
         }
         result += sqrtf(length_squared(xyz(src[n - 1u]) - previous));
-        a.lengths[g] = result;
+        *length = result;
     }
     for (uint32_t cv = 0u; cv < n; ++cv) noise[cv] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (a.magnitude == 0.0f) return;  // the noise or mask test failed: a zero axis
+    if (magnitude == 0.0f) return;  // the noise or mask test failed: a zero axis
     float polyline = 0.0f;
     for (uint32_t cv = 1u; cv < n; ++cv) polyline += sqrtf(length_squared(xyz(src[cv]) - xyz(src[cv - 1u])));
-    const float effective = polyline > 0.0f ? max_std(0.5f / polyline, a.frequency) : a.frequency;
-    const float decorrelation = 1.0f - a.correlation;
+    const float effective = polyline > 0.0f ? max_std(0.5f / polyline, frequency) : frequency;
+    const float decorrelation = 1.0f - correlation;
     const float domain_scale = 100.0f * decorrelation * decorrelation;
-    const nxg_vec3 dp = a.reference ? a.reference[g] : a.axes[uint64_t{g} * n];
-    const F3 domain = (F3{dp.x, dp.y, dp.z} + F3{0.419276f, 0.184247f, 0.805721f}) * domain_scale;
-    const nxg_vec3 rn = a.normals[g], rt = a.tangents[g];
-    F3 normal{rn.x, rn.y, rn.z}, tangent{rt.x, rt.y, rt.z};
+    const F3 domain = (F3{domain_point.x, domain_point.y, domain_point.z} + F3{0.419276f, 0.184247f, 0.805721f}) *
+                      domain_scale;
+    F3 normal{raw_normal.x, raw_normal.y, raw_normal.z}, tangent{raw_tangent.x, raw_tangent.y, raw_tangent.z};
     normal = !(length_squared(normal) > 1.0e-20f) ? F3{0.0f, 1.0f, 0.0f} : normalize3(normal);
     if (!(length_squared(tangent) > 1.0e-20f)) {  // fallback_surface_u (generate.h:312-316)
         const F3 axis = normal.z > -0.999f && normal.z < 0.999f ? F3{0.0f, 0.0f, 1.0f} : F3{0.0f, 1.0f, 0.0f};
@@ -1109,7 +673,7 @@ __global__ void __launch_bounds__(64) clump_guide_kernel(ClumpArgs a) {
         const float distance = travelled * effective;
         const float first = classic_noise(domain.x + distance, domain.y, domain.z) - 0.5f;
         const float second = classic_noise(domain.x, domain.y, domain.z + distance) - 0.5f;
-        const F3 d = (su * first + sv * second) * a.magnitude;
+        const F3 d = (su * first + sv * second) * magnitude;
         noise[cv] = make_float4(d.x, d.y, d.z, 0.0f);
         if (advance) {
             tu = rotate_between(tu, current, next, 1.0f);
@@ -1117,6 +681,45 @@ __global__ void __launch_bounds__(64) clump_guide_kernel(ClumpArgs a) {
             current = next;
         }
     }
+}
+
+__global__ void __launch_bounds__(64) clump_guide_kernel(ClumpArgs a) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.guides) return;
+    const uint32_t n = a.cvs;
+    clump_guide_axes(a.axes + uint64_t{g} * n, n, a.reference ? a.reference[g] : a.axes[uint64_t{g} * n],
+                     a.normals[g], a.tangents[g], a.frequency, a.correlation, a.magnitude,
+                     a.render + uint64_t{g} * n, a.noise + uint64_t{g} * n, a.lengths + g);
+}
+
+// The search half of cut_to_length_and_rebuild (src/xgen_classic_runtime.cpp:
+// 750-799) over the shared cLength samples: the rebuild parameter that keeps
+// `target` of spline length measured from the root (1 when the strand is
+// shorter), -1 when every CV collapses onto the root.
+template <class Curve>
+__device__ __forceinline__ float cut_to_length(const Sample *samples_table, const Curve &p, uint32_t n, float target) {
+    if (target <= 1.0e-10f) return -1.0f;
+    float previous_t = 0.0f, accumulated = 0.0f;
+    F3 previous = xyz(p[0]);
+    const uint32_t samples = 2u * n + 3u;
+    for (uint32_t k = 0u; k <= samples; ++k) {
+        float t;
+        F3 current;
+        if (k < samples) {
+            const Sample smp = load_sample(samples_table, k);
+            t = smp.t;
+            current = blend(p, n, smp);
+        } else {
+            t = 1.0f;
+            current = xyz(p[n - 1u]);
+        }
+        const float segment = sqrtf(length_squared(current - previous));
+        accumulated += segment;
+        if (accumulated >= target && segment > 0.0f) return t - ((accumulated - target) / segment) * (t - previous_t);
+        previous_t = t;
+        previous = current;
+    }
+    return 1.0f;
 }
 
 // The clump step of one strand (src/xgen_classic_runtime.cpp:1752-1792),
@@ -1140,36 +743,7 @@ __device__ __forceinline__ bool clump_strand(const ClumpArgs &a, const StagedCur
         return false;
     }
     const float amount = clamp_std(raw, 0.0f, 1.0f);
-    // cut_to_length_and_rebuild (src/xgen_classic_runtime.cpp:750-799)
-    const float target = a.lengths[g];
-    float cut = 1.0f;
-    if (target <= 1.0e-10f) {
-        cut = -1.0f;  // every CV at the root
-    } else {
-        float previous_t = 0.0f, accumulated = 0.0f;
-        F3 previous = xyz(p[0]);
-        const uint32_t samples = 2u * n + 3u;
-        for (uint32_t k = 0u; k <= samples; ++k) {
-            float t;
-            F3 current;
-            if (k < samples) {
-                const Sample smp = load_sample(a.length_samples, k);
-                t = smp.t;
-                current = blend(p, n, smp);
-            } else {
-                t = 1.0f;
-                current = xyz(p[n - 1u]);
-            }
-            const float segment = sqrtf(length_squared(current - previous));
-            accumulated += segment;
-            if (accumulated >= target && segment > 0.0f) {
-                cut = t - ((accumulated - target) / segment) * (t - previous_t);
-                break;
-            }
-            previous_t = t;
-            previous = current;
-        }
-    }
+    const float cut = cut_to_length(a.length_samples, p, n, a.lengths[g]);
     const float factor = a.mask * amount * (1.0f - 2.0f * a.clump_scale);
     const float4 *render = a.render + uint64_t{g} * n;
     const float4 *noise = a.noise + uint64_t{g} * n;
@@ -1250,6 +824,629 @@ __global__ void __launch_bounds__(kWave) clump_strand_kernel(ClumpArgs a) {
         }
     }
     clump_report(a, err);
+}
+
+// ---- Classic primitive expressions ---------------------------------------
+// The plan's programs run per strand in the search (length, width, the cut
+// amount, taper, taperStart, and widthRamp at every CV for its error
+// checks) and widthRamp again per CV in the write, which produces the radii.
+
+// What the search hands the write for a strand.
+struct RuntimeState {
+    float length_scale;  // the length expression's value (1 without one)
+    float c_length;      // cLength after the effects, as the post-effect programs read it
+    float c_width;
+    float taper, taper_start;
+    float reserved[3];
+};
+
+constexpr uint32_t kClassicSlots = NXG_CLASSIC_MAX_PROGRAM_SLOTS;
+
+// The per-strand context of the Classic binding (input_value and the float
+// context, src/xgen_classic_runtime.cpp:504-574, 1628-1640).
+struct ClassicContext {
+    float id;
+    ExprContext expr;
+    const float *row;  // the strand's input row (columns)
+};
+
+__device__ __forceinline__ ClassicContext classic_context(const CutArgs &a, uint32_t s) {
+    ClassicContext c;
+    c.id = static_cast<float>(a.ids ? a.ids[s] : s);
+    const nxg_root_sample r = a.roots[s];
+    const uint32_t face = r.surface_face_id == NXG_INVALID_INDEX ? r.triangle_index : r.surface_face_id;
+    c.expr.u = r.uv.x;
+    c.expr.v = r.uv.y;
+    // xgen_runtime_face_seed (src/xgen_expression.cpp:724-736)
+    c.expr.face_seed =
+        hash_to_unit(hash_step(hash_step(hash_step(0u, a.description_id), a.name_seed), static_cast<float>(face)));
+    c.expr.t = 0.0f;
+    c.expr.random_prefix = a.prefixes ? a.prefixes[s] : 0u;
+    c.expr.has_random_prefix = a.prefixes != nullptr;
+    c.row = a.columns ? a.columns + uint64_t{s} * a.column_count : nullptr;
+    return c;
+}
+
+// Program k of the plan for one strand (evaluate_runtime_expression,
+// src/xgen_classic_runtime.cpp:555-574): false where the reference throws
+// (a non-finite context, input or value).  Input i sits in value slot i; the
+// slots are a register array indexed with wave-uniform slot numbers.
+struct Evaluated {
+    float value;
+    bool ok;
+};
+__device__ Evaluated classic_program(const uint8_t *plan_image, uint32_t program_offset, ClassicContext c,
+                                     float c_length, float c_width, float t) {
+    const Image image = (Image)plan_image;
+    ImageOf(PackedProgram) prog = (ImageOf(PackedProgram))(image + program_offset);
+    ImageOf(PackedInstruction) ins = (ImageOf(PackedInstruction))(image + prog->instructions_offset);
+    ImageOf(uint32_t) operand_slots = (ImageOf(uint32_t))(image + prog->operands_offset);
+    ImageOf(nxg_expr_ramp) ramps = (ImageOf(nxg_expr_ramp))(image + prog->ramps_offset);
+    ImageOf(nxg_expr_ramp_point) points = (ImageOf(nxg_expr_ramp_point))(image + prog->ramp_points_offset);
+    ExprContext ctx = c.expr;
+    ctx.t = t;
+    bool bad = !isfinite(ctx.u) || !isfinite(ctx.v) || !isfinite(ctx.face_seed) || !isfinite(t);
+    float regs[kClassicSlots];
+#define SLOT_GET(x) regs[__builtin_amdgcn_readfirstlane(x) & (kClassicSlots - 1u)]
+    const uint32_t inputs = prog->input_count;
+    for (uint32_t i = 0; i < inputs; ++i) {
+        const uint32_t b = prog->input_bindings[i];
+        const float v = b == NXG_CLASSIC_VAR_ID        ? c.id
+                        : b == NXG_CLASSIC_VAR_CLENGTH ? c_length
+                        : b == NXG_CLASSIC_VAR_CWIDTH  ? c_width
+                                                       : c.row[b - NXG_CLASSIC_VAR_COLUMN0];
+        if (!isfinite(v)) bad = true;
+        SLOT_GET(i) = v;
+    }
+    const uint32_t n = prog->instruction_count;
+    PackedInstruction next = load_instruction(ins);
+    for (uint32_t i = 0; i < n; ++i) {
+        const PackedInstruction in = next;
+        next = load_instruction(ins + i + 1u);
+#define OPERAND(e) (((e) & kImmediate) ? (((e) & 1u) ? in.immediate1 : in.immediate) : SLOT_GET(e))
+#define EXPR_OPV(j) OPERAND(uint32_t{in.slot[j]})
+#define EXPR_OPA(j) OPERAND(operand_slots[in.operand_offset + (j)])
+#define EXPR_NOISE3(x, y, z) __builtin_nanf("")  // no extension ops in a Classic plan
+#define EXPR_TEXEL(map, f, u, v) __builtin_nanf("")
+        float value = 0.0f;
+#include "expr_ops.inc"
+#undef EXPR_OPV
+#undef EXPR_OPA
+#undef EXPR_NOISE3
+#undef EXPR_TEXEL
+#undef OPERAND
+        if (!isfinite(value)) bad = true;
+        SLOT_GET(uint32_t{in.dest}) = value;
+    }
+    const float out = SLOT_GET(prog->result);
+#undef SLOT_GET
+    return {out, !bad};
+}
+
+__device__ __forceinline__ bool classic_eval(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
+                                             float c_width, float t, float &out) {
+    const Evaluated e = classic_program(a.image, a.program_offset[k], c, c_length, c_width, t);
+    out = e.value;
+    return e.ok;
+}
+
+// The parameter of CV cv (src/xgen_classic_runtime.cpp:1840-1841).
+__device__ __forceinline__ float cv_parameter(uint32_t cv, uint32_t n) {
+    return static_cast<float>(cv) / static_cast<float>(n - 1u);
+}
+
+// The programs evaluated per CV (at t = cv / (n - 1)); the others run at t = 0.
+__device__ __forceinline__ int cv_table(uint32_t k) {
+    return k == NXG_CLASSIC_EXPR_WIDTH_RAMP ? 0 : k == NXG_CLASSIC_EXPR_CLUMP_SCALE ? 1
+         : k == NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE ? 2 : -1;
+}
+// The programs a clump leaves t at 1 for: the reference's context keeps the
+// t of the clump's last CV (src/xgen_classic_runtime.cpp:1761-1764), which
+// the cut amount, taper and taperStart then read.  Their values at t = 1
+// take the table's spare scalar slots.
+__device__ __forceinline__ int after_clump_slot(uint32_t k) {
+    return k == NXG_CLASSIC_EXPR_CUT_AMOUNT ? 0 : k == NXG_CLASSIC_EXPR_TAPER ? 1
+         : k == NXG_CLASSIC_EXPR_TAPER_START ? 2 : -1;
+}
+static_assert(NXG_CLASSIC_EXPR_COUNT + 3u <= kClassicScalarSlots, "spare scalar slots for t = 1");
+
+// Where a uniform program's value lives in the call's table.
+__device__ __forceinline__ uint32_t uniform_slot(uint32_t k, uint32_t cv, bool at_one) {
+    const int table = cv_table(k);
+    if (table >= 0) return kClassicScalarSlots + static_cast<uint32_t>(table) * NXG_CLASSIC_MAX_CVS + cv;
+    return at_one ? NXG_CLASSIC_EXPR_COUNT + static_cast<uint32_t>(after_clump_slot(k)) : k;
+}
+
+// Program k for one strand (per-CV programs at CV cv, the others at t = 0,
+// or at t = 1 after a clump when at_one): a uniform program's value comes
+// from the call's table, after the context check the reference makes at
+// every evaluation (xgen_expression.cpp:991-994).
+__device__ __forceinline__ bool classic_value(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
+                                              float c_width, uint32_t cv, float &out, bool at_one = false) {
+    if ((a.uniform >> k) & 1u) {
+        const float2 v = a.uniform_values[uniform_slot(k, cv, at_one)];
+        out = v.x;
+        return v.y != 0.0f && isfinite(c.expr.u) && isfinite(c.expr.v) && isfinite(c.expr.face_seed);
+    }
+    const float t = cv_table(k) >= 0 ? cv_parameter(cv, a.cvs) : (at_one ? 1.0f : 0.0f);
+    return classic_eval(a, k, c, c_length, c_width, t, out);
+}
+
+// The plan's uniform programs, once per call: entry j < kClassicScalarSlots
+// is program j at t = 0 (and the cut amount, taper and taperStart at t = 1
+// in the spare slots), then each per-CV program at every CV.
+__global__ void __launch_bounds__(256) classic_uniform_kernel(CutArgs a) {
+    const uint32_t entries = kClassicScalarSlots + kClassicCvTables * NXG_CLASSIC_MAX_CVS;
+    for (uint32_t j = threadIdx.x; j < entries; j += blockDim.x) {
+        uint32_t k, cv = 0u;
+        float t = 0.0f;
+        if (j < NXG_CLASSIC_EXPR_COUNT) {
+            k = j;
+            if (cv_table(k) >= 0) continue;
+        } else if (j < NXG_CLASSIC_EXPR_COUNT + 3u) {
+            const uint32_t spare = j - NXG_CLASSIC_EXPR_COUNT;
+            k = spare == 0u ? NXG_CLASSIC_EXPR_CUT_AMOUNT : spare == 1u ? NXG_CLASSIC_EXPR_TAPER
+                                                                        : NXG_CLASSIC_EXPR_TAPER_START;
+            t = 1.0f;
+        } else if (j < kClassicScalarSlots) {
+            continue;
+        } else {
+            const uint32_t table = (j - kClassicScalarSlots) / NXG_CLASSIC_MAX_CVS;
+            cv = (j - kClassicScalarSlots) % NXG_CLASSIC_MAX_CVS;
+            if (cv >= a.cvs) continue;
+            k = table == 0u ? NXG_CLASSIC_EXPR_WIDTH_RAMP
+                : table == 1u ? NXG_CLASSIC_EXPR_CLUMP_SCALE : NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE;
+            t = cv_parameter(cv, a.cvs);
+        }
+        if (!((a.uniform >> k) & 1u)) continue;
+        ClassicContext c{};
+        float v = 0.0f;
+        const bool ok = classic_eval(a, k, c, 0.0f, 0.0f, t, v);
+        a.uniform_values[j] = make_float2(v, ok ? 1.0f : 0.0f);
+    }
+}
+
+// The context the clump's noise expressions see (build_clump_noise_axis,
+// src/xgen_classic_runtime.cpp:1730-1737, 1057-1059): the strand's, with the
+// guide's uv and its SeExpr prefix.  The face seed feeds only a rand()
+// without a prefix, which this context never evaluates.
+__device__ __forceinline__ ClassicContext guide_context(const CutArgs &a, const ClassicContext &strand, uint32_t g) {
+    ClassicContext c = strand;
+    const nxg_vec2 uv = a.guide_uvs[g];
+    c.expr.u = uv.x;
+    c.expr.v = uv.y;
+    c.expr.face_seed = 0.0f;
+    c.expr.random_prefix = a.guide_prefixes[g];
+    c.expr.has_random_prefix = true;
+    return c;
+}
+
+// Per guide, once per call: render axis, spline length and the unit
+// clump-noise axis with the guide's noise frequency and correlation
+// (programs that read no input, so one value per guide).
+__global__ void __launch_bounds__(64) runtime_guide_kernel(CutArgs a) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.guide_count) return;
+    const uint32_t n = a.cvs;
+    const ClassicContext gc = guide_context(a, ClassicContext{}, g);
+    float raw_frequency = 0.0f, raw_correlation = 0.0f;
+    const bool ok = classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY, gc, 0.0f, 0.0f, 0u, raw_frequency) &&
+                    classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION, gc, 0.0f, 0.0f, 0u, raw_correlation);
+    const float frequency = max_std(raw_frequency, 0.0f);
+    const float correlation = clamp_std(raw_correlation * 0.01f, 0.0f, 1.0f);
+    const nxg_vec3 *axes = a.guide_axes + uint64_t{g} * n;
+    float length = 0.0f;
+    clump_guide_axes(axes, n, a.guide_reference ? a.guide_reference[g] : axes[0], a.guide_normals[g],
+                     a.guide_tangents[g], frequency, correlation, ok ? 1.0f : 0.0f, a.guide_render + uint64_t{g} * n,
+                     a.guide_noise + uint64_t{g} * n, &length);
+    a.guide_info[g] = make_float2(length, ok ? 1.0f : 0.0f);
+}
+
+// scale_curve (src/xgen_classic_runtime.cpp:576-585) as a view: positions
+// root + (p - root) * scale when `on` (a length expression ran), radii kept.
+template <class Curve>
+struct ScaledCurve {
+    Curve base;
+    F3 root;
+    float scale;
+    bool on;
+    __device__ __forceinline__ float4 operator[](uint32_t c) const {
+        const float4 v = base[c];
+        if (!on) return v;
+        const F3 q = root + (xyz(v) - root) * scale;
+        return make_float4(q.x, q.y, q.z, v.w);
+    }
+};
+
+// The curve cut_from_tip_and_rebuild leaves (src/xgen_classic_runtime.cpp:
+// 729-742) as a view: CV i at parameter i cut / (n - 1), every CV at the
+// root when the walk removed the whole strand (cut 0).
+template <class Curve>
+struct RebuiltCurve {
+    Curve source;
+    uint32_t n;
+    float cut;
+    __device__ __forceinline__ float4 operator[](uint32_t i) const {
+        if (cut <= 0.0f || i == 0u) return source[0];
+        const F3 q = curve_eval(source, n, cut / static_cast<float>(n - 1u) * static_cast<float>(i));
+        return make_float4(q.x, q.y, q.z, 0.0f);
+    }
+};
+
+__device__ __forceinline__ bool has_program(const CutArgs &a, uint32_t k) { return (a.present >> k) & 1u; }
+
+// scale_curve applied once to a lane's staged CVs, in place (each lane owns
+// its strand's LDS entries); the same operations as ScaledCurve per read,
+// CV 0 included (a -0 root coordinate becomes +0 there).  Strands read from
+// memory keep the view.
+template <bool ROTATED>
+__device__ __forceinline__ bool scale_in_place(const StagedCurve<ROTATED> &p, uint32_t n, float scale) {
+    const F3 root = xyz(p[0]);
+    for (uint32_t c = 0u; c < n; ++c) {
+        const float4 v = p[c];
+        const F3 q = root + (xyz(v) - root) * scale;
+        p.tile[staged_index<ROTATED>(c, p.lane)] = make_float4(q.x, q.y, q.z, v.w);
+    }
+    return true;
+}
+__device__ __forceinline__ bool scale_in_place(const MemoryCurve &, uint32_t, float) { return false; }
+
+// The taper part of the width profile at t (src/xgen_classic_runtime.cpp:
+// 1842-1848).
+__device__ __forceinline__ float taper_scale(float t, float taper, float taper_start) {
+    float scale = 1.0f;
+    if (t > taper_start && taper_start < 1.0f) scale *= 1.0f - taper * ((t - taper_start) / (1.0f - taper_start));
+    return scale;
+}
+
+// The ClumpingFX step of one strand (src/xgen_classic_runtime.cpp:1680-1793)
+// over its staged CVs: the strand's CVs after the step (unchanged when it
+// has no guide or a zero mask) go to a.clumped and back into its LDS row;
+// c_length follows the reference's context.  False (with *err) where the
+// reference throws.
+__device__ __forceinline__ bool runtime_clump(const CutArgs &a, const ClassicContext &c,
+                                              const StagedCurve<true> &p, uint32_t s, float &c_length,
+                                              float c_width, bool &clumped, uint32_t *err) {
+    const uint32_t n = a.cvs;
+    float4 *out = a.clumped + uint64_t{s} * n;
+    const uint32_t g = a.strand_guides[s];
+    float mask = 0.0f;
+    if (g != NXG_INVALID_INDEX) {
+        if (g >= a.guide_count) {
+            *err = NXG_CLASSIC_INVALID_GUIDE;
+            return false;
+        }
+        float raw_mask;
+        if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_MASK, c, c_length, c_width, 0u, raw_mask)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return false;
+        }
+        mask = clamp_std(raw_mask, 0.0f, 1.0f);
+    }
+    if (!(mask > 0.0f)) {  // the strand passes through
+        for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = p[cv];
+        return true;
+    }
+    // the noise axis's parameters, in the guide's context (:1041-1186)
+    const ClassicContext gc = guide_context(a, c, g);
+    float raw_noise;
+    if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE, gc, c_length, c_width, 0u, raw_noise)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return false;
+    }
+    const float noise = max_std(raw_noise, 0.0f);
+    const bool noisy = noise > 1.0e-5f && mask > 1.0e-4f;
+    const float2 info = a.guide_info[g];
+    if (noisy) {
+        if (info.y == 0.0f) {  // the guide's noise frequency or correlation failed
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return false;
+        }
+        for (uint32_t cv = 1u; cv < n; ++cv) {
+            float raw_scale;
+            if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE, gc, c_length, c_width, cv, raw_scale)) {
+                *err = NXG_CLASSIC_EXPRESSION_ERROR;
+                return false;
+            }
+            if (!isfinite(mask * noise * max_std(raw_scale, 0.0f))) {
+                *err = NXG_CLASSIC_NONFINITE_CLUMP;
+                return false;
+            }
+        }
+    }
+    // cut_to_length_and_rebuild to the guide's length, then the amount
+    const float cut = cut_to_length(a.length_samples, p, n, info.x);
+    const float noise_length = c_length;  // the noise expressions' cLength
+    if (a.clump_length) c_length = spline_length(a.length_samples, RebuiltCurve<StagedCurve<true>>{p, n, cut}, n);
+    float raw_amount;
+    if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP, c, c_length, c_width, 0u, raw_amount)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return false;
+    }
+    const float amount = clamp_std(raw_amount, 0.0f, 1.0f);
+    const float4 *render = a.guide_render + uint64_t{g} * n;
+    const float4 *unit = a.guide_noise + uint64_t{g} * n;
+    const float step = cut / static_cast<float>(n - 1u);
+    out[0] = p[0];
+    for (uint32_t cv = 1u; cv < n; ++cv) {
+        F3 q = cut < 0.0f ? xyz(p[0]) : curve_eval(p, n, step * static_cast<float>(cv));
+        float scale;
+        if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_SCALE, c, c_length, c_width, cv, scale)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return false;
+        }
+        q = q + (xyz(render[cv]) - q) * (mask * amount * (1.0f - 2.0f * scale));
+        // the noise axis is added to every CV, zero when it is off (so -0 becomes +0)
+        F3 d{0.0f, 0.0f, 0.0f};
+        if (noisy) {
+            float raw_scale = 0.0f;
+            classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE, gc, noise_length, c_width, cv, raw_scale);
+            d = xyz(unit[cv]) * (mask * noise * max_std(raw_scale, 0.0f));
+        }
+        q = q + d;
+        out[cv] = make_float4(q.x, q.y, q.z, p[cv].w);
+    }
+    for (uint32_t cv = 0u; cv < n; ++cv) p.tile[staged_index<true>(cv, p.lane)] = out[cv];
+    c_length = spline_length(a.length_samples, p, n);
+    clumped = true;
+    return true;
+}
+
+// One strand of the runtime chain (src/xgen_classic_runtime.cpp:1619-1858):
+// the cut parameter of a kept strand (1 without a CutFX), -1 for a culled
+// one; *err gets the NXG_CLASSIC_* bit where the reference throws.
+template <class Curve>
+__device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t s, uint32_t *err) {
+    const uint32_t n = a.cvs;
+    if (a.counts && a.counts[s] != n) {
+        *err = NXG_CLASSIC_VARIABLE_COUNTS;
+        return -1.0f;
+    }
+    const ClassicContext c = classic_context(a, s);
+    float c_length = spline_length(a.length_samples, source, n);
+    float length_scale = 1.0f;
+    if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) {
+        if (!classic_value(a, NXG_CLASSIC_EXPR_LENGTH, c, c_length, 0.0f, 0u, length_scale)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return -1.0f;
+        }
+        if (!isfinite(length_scale) || length_scale < 0.0f) {
+            *err = NXG_CLASSIC_NEGATIVE_LENGTH;
+            return -1.0f;
+        }
+        c_length *= length_scale;
+    }
+    const bool scaled = has_program(a, NXG_CLASSIC_EXPR_LENGTH) && !scale_in_place(source, n, length_scale);
+    const ScaledCurve<Curve> p{source, xyz(source[0]), length_scale, scaled};
+    float width;
+    if (has_program(a, NXG_CLASSIC_EXPR_WIDTH)) {
+        if (!classic_value(a, NXG_CLASSIC_EXPR_WIDTH, c, c_length, 0.0f, 0u, width)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return -1.0f;
+        }
+    } else {
+        width = a.radius_scale > 0.0f ? 2.0f * source[0].w / a.radius_scale : 0.0f;
+    }
+    if (!isfinite(width)) {
+        *err = NXG_CLASSIC_NONFINITE_WIDTH;
+        return -1.0f;
+    }
+    if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
+    const float c_width = width;
+    bool clumped = false;  // the clump ran its CV loop, leaving the context's t at 1
+    if constexpr (std::is_same<Curve, StagedCurve<true>>::value) {
+        // the clump (strands of at most 64 CVs, always staged)
+        if (a.has_clump && !runtime_clump(a, c, source, s, c_length, c_width, clumped, err)) return -1.0f;
+    }
+    float cut = 1.0f;
+    bool keep = true;
+    if (a.has_cut) {
+        float amount;
+        if (!classic_value(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0u, amount, clumped)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return -1.0f;
+        }
+        if (!isfinite(amount)) {
+            *err = NXG_CLASSIC_NONFINITE_AMOUNT;
+            return -1.0f;
+        }
+        cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
+        keep = cut >= 1.0e-4f;
+        // the post-effect programs read the rebuilt curve's cLength
+        if (a.final_length) c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
+    }
+    float taper = 0.0f, taper_start = 0.0f;
+    if (has_program(a, NXG_CLASSIC_EXPR_TAPER) &&
+        !classic_value(a, NXG_CLASSIC_EXPR_TAPER, c, c_length, c_width, 0u, taper, clumped)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return -1.0f;
+    }
+    if (has_program(a, NXG_CLASSIC_EXPR_TAPER_START) &&
+        !classic_value(a, NXG_CLASSIC_EXPR_TAPER_START, c, c_length, c_width, 0u, taper_start, clumped)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return -1.0f;
+    }
+    if (!isfinite(taper) || !isfinite(taper_start)) {
+        *err = NXG_CLASSIC_NONFINITE_TAPER;
+        return -1.0f;
+    }
+    if (a.profile) {  // the width profile's errors (its values are the write's)
+        for (uint32_t cv = 0u; cv < n; ++cv) {
+            float scale = taper_scale(cv_parameter(cv, n), taper, taper_start);
+            if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
+                float ramp;
+                if (!classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, c_length, c_width, cv, ramp)) {
+                    *err = NXG_CLASSIC_EXPRESSION_ERROR;
+                    return -1.0f;
+                }
+                scale *= ramp;
+            }
+            if (!isfinite(c_width * scale)) {
+                *err = NXG_CLASSIC_NONFINITE_PROFILE;
+                return -1.0f;
+            }
+        }
+    }
+    RuntimeState st{};
+    st.length_scale = length_scale;
+    st.c_length = c_length;
+    st.c_width = c_width;
+    st.taper = taper;
+    st.taper_start = taper_start;
+    a.state[s] = st;
+    return keep ? cut : -1.0f;
+}
+
+template <bool STAGED>
+__global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    if constexpr (STAGED) stage_tile<true>(a, tile, first, count);
+    const uint32_t s = first + lane;
+    float cut = -1.0f;
+    uint32_t err = 0u;
+    if (lane < count) {
+        if constexpr (STAGED)
+            cut = runtime_strand(a, StagedCurve<true>{tile, lane}, s, &err);
+        else
+            cut = runtime_strand(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
+        a.cut[s] = cut;
+    }
+    const uint64_t kept = __ballot(cut >= 0.0f);
+    uint32_t bits = 0u;  // the wave's NXG_CLASSIC_* bits (all below 1 << 16)
+#pragma unroll
+    for (uint32_t b = 0u; b < 16u; ++b)
+        if (__ballot((err >> b) & 1u)) bits |= 1u << b;
+    if (lane == 0u) {
+        a.tiles[t] = static_cast<uint32_t>(__popcll(kept)) | (bits ? 1u << kErrShift : 0u);
+        if (bits) atomicOr(a.err_bits, bits);
+    }
+}
+
+// The kept strands at their compacted slots: the scaled (and, with a CutFX,
+// rebuilt) positions, and radii from the width profile when the plan has
+// one (src/xgen_classic_runtime.cpp:1836-1858), else the input radii.
+template <bool STAGED>
+__global__ void __launch_bounds__(kWave) runtime_write_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    const uint32_t s = first + lane;
+    const float cut = lane < count ? a.cut[s] : -1.0f;
+    const uint64_t kept = __ballot(cut >= 0.0f);
+    if (kept == 0u) return;  // the block is this one wavefront
+    if constexpr (STAGED) stage_tile<false>(a, tile, first, count);
+    if (!(cut >= 0.0f)) return;
+    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
+    const uint32_t slot = a.groups[t / kScanBlock] + a.tiles[t] + rank;
+    const RuntimeState st = a.state[s];
+    const ClassicContext c = classic_context(a, s);
+    float4 *dst = a.out_points + uint64_t{slot} * n;
+    auto write = [&](const auto &source) {
+        using Curve = typename std::decay<decltype(source)>::type;
+        const ScaledCurve<Curve> p{source, xyz(source[0]), st.length_scale, has_program(a, NXG_CLASSIC_EXPR_LENGTH)};
+        const float step = cut / static_cast<float>(n - 1u);
+        for (uint32_t i = 0u; i < n; ++i) {
+            float4 q = p[i];
+            if (a.has_cut && i > 0u) {
+                const F3 r = curve_eval(p, n, step * static_cast<float>(i));
+                q = make_float4(r.x, r.y, r.z, q.w);
+            }
+            if (a.profile) {
+                float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
+                if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
+                    float ramp = 0.0f;
+                    classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
+                    scale *= ramp;
+                }
+                q.w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
+            }
+            dst[i] = q;
+        }
+    };
+    if constexpr (STAGED)
+        write(StagedCurve<false>{tile, lane});
+    else
+        write(MemoryCurve{a.points + uint64_t{s} * n});
+    if (a.out_counts) a.out_counts[slot] = n;
+    if (a.out_roots) a.out_roots[slot] = a.roots[s];
+    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
+}
+
+// The write with each strand held in registers (n <= MAXN), laid out in
+// output order through the tile's LDS and stored as one contiguous run per
+// wave, as cut_write_held_kernel does.
+template <uint32_t MAXN>
+__global__ void __launch_bounds__(kWave) runtime_write_held_kernel(CutArgs a) {
+    extern __shared__ float4 tile[];
+    if (*a.status_word != 0u) return;  // the reference throws: nothing is written
+    const uint32_t n = a.cvs;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
+    const uint32_t first = t * kWave;
+    const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
+    const uint32_t s = first + lane;
+    const float cut = lane < count ? a.cut[s] : -1.0f;
+    const bool keep = cut >= 0.0f;
+    const uint64_t kept = __ballot(keep);
+    if (kept == 0u) return;  // the block is this one wavefront
+    stage_tile<false>(a, tile, first, count);
+    const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
+    const uint32_t slot0 = a.groups[t / kScanBlock] + a.tiles[t];
+    float4 held[MAXN];
+    RuntimeState st{};
+    if (keep) {  // positions (the input radii ride along)
+        st = a.state[s];
+        const StagedCurve<false> p{tile, lane};
+        if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) scale_in_place(p, n, st.length_scale);
+        const float step = cut / static_cast<float>(n - 1u);
+#pragma unroll
+        for (uint32_t i = 0u; i < MAXN; ++i) {
+            if (i >= n) continue;
+            float4 q = p[i];
+            if (a.has_cut && i > 0u) {
+                const F3 r = curve_eval(p, n, step * static_cast<float>(i));
+                q = make_float4(r.x, r.y, r.z, q.w);
+            }
+            held[i] = q;
+        }
+    }
+    __syncthreads();
+    const uint32_t stride = n + 1u;
+    if (keep) {
+#pragma unroll
+        for (uint32_t i = 0u; i < MAXN; ++i)
+            if (i < n) tile[rank * stride + i] = held[i];
+        if (a.profile) {  // the width profile's radii, in the lane's own LDS row
+            const ClassicContext c = classic_context(a, s);
+            for (uint32_t i = 0u; i < n; ++i) {
+                float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
+                if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
+                    float ramp = 0.0f;
+                    classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
+                    scale *= ramp;
+                }
+                tile[rank * stride + i].w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t total = static_cast<uint32_t>(__popcll(kept)) * n;
+    float4 *dst = a.out_points + uint64_t{slot0} * n;
+    for (uint32_t e = lane; e < total; e += kWave) {
+        const uint32_t r = __umulhi(e, a.div_magic);
+        dst[e] = tile[r * stride + (e - r * n)];
+    }
+    if (!keep) return;
+    const uint32_t slot = slot0 + rank;
+    if (a.out_counts) a.out_counts[slot] = n;
+    if (a.out_roots) a.out_roots[slot] = a.roots[s];
+    if (a.out_uvs) a.out_uvs[slot] = a.uvs[s];
 }
 
 }  // namespace
@@ -1357,11 +1554,13 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     using namespace nxg;
     if (uint32_t s = nxg_validate_classic_runtime(curves, inputs, output); s != NXG_OK) return s;
     const uint32_t n = curves->strand_count, cvs = curves->cvs_per_strand;
+    const nxg_classic_plan_info &plan = *inputs->plan;
+    const uint32_t guides = plan.has_clump ? inputs->guides->guide_count : 0u;
     const ClassicLayout layout = classic_layout(n);
-    if (!workspace || workspace_bytes < classic_runtime_bytes(n) || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
+    const ClassicRuntimeLayout rt = classic_runtime_layout(n, cvs, guides);
+    if (!workspace || workspace_bytes < rt.total || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
         return NXG_ERROR_WORKSPACE_TOO_SMALL;
     uint8_t *ws = static_cast<uint8_t *>(workspace);
-    const nxg_classic_plan_info &plan = *inputs->plan;
     CutArgs a = chain_args(curves, output, ws, layout);
     a.radius_scale = inputs->radius_scale;
     a.image = static_cast<const uint8_t *>(inputs->plan_image);
@@ -1378,9 +1577,26 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.description_id = static_cast<float>(plan.description_id);
     a.name_seed = static_cast<float>(plan.description_name_seed);
     a.uniform = plan.uniform & plan.present;
-    a.uniform_values = reinterpret_cast<float2 *>(ws + classic_runtime_uniform_offset(n));
-    a.state = reinterpret_cast<RuntimeState *>(ws + classic_runtime_state_offset(n));
+    a.uniform_values = reinterpret_cast<float2 *>(ws + rt.uniform_offset);
+    a.state = reinterpret_cast<RuntimeState *>(ws + rt.state_offset);
     a.err_bits = a.status_word + 1;
+    if (plan.has_clump) {
+        const nxg_clump_guides &g = *inputs->guides;
+        a.has_clump = 1u;
+        a.clump_length = plan.clump_length;
+        a.guide_count = g.guide_count;
+        a.strand_guides = inputs->strand_guides;
+        a.guide_axes = g.axes;
+        a.guide_normals = g.normals;
+        a.guide_tangents = g.tangents;
+        a.guide_reference = g.reference_positions;
+        a.guide_uvs = inputs->guide_uvs;
+        a.guide_prefixes = inputs->guide_random_prefixes;
+        a.clumped = reinterpret_cast<float4 *>(ws + rt.clumped_offset);
+        a.guide_render = reinterpret_cast<float4 *>(ws + rt.render_offset);
+        a.guide_noise = reinterpret_cast<float4 *>(ws + rt.noise_offset);
+        a.guide_info = reinterpret_cast<float2 *>(ws + rt.guide_offset);
+    }
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
     const bool staged = cvs <= kStagedCvs;
     const size_t tile_lds = staged ? size_t{kWave} * cvs * sizeof(float4) : 0u;
@@ -1388,6 +1604,8 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
         hipLaunchKernelGGL(cut_table_kernel, dim3(1), dim3(256), 0, stream, a);
         hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, a.err_bits);
         if (a.uniform) hipLaunchKernelGGL(classic_uniform_kernel, dim3(1), dim3(256), 0, stream, a);
+        if (a.has_clump)
+            hipLaunchKernelGGL(runtime_guide_kernel, dim3((a.guide_count + 63u) / 64u), dim3(64), 0, stream, a);
         if (staged)
             hipLaunchKernelGGL(runtime_search_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
@@ -1396,17 +1614,23 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     }
     hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
     if (a.tile_count) {
+        // after a clump the write starts from the clumped strands, already scaled
+        CutArgs w = a;
+        if (a.has_clump) {
+            w.points = a.clumped;
+            w.present &= ~(1u << NXG_CLASSIC_EXPR_LENGTH);
+        }
         const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);
         if (cvs <= 8u)
-            hipLaunchKernelGGL(runtime_write_held_kernel<8>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+            hipLaunchKernelGGL(runtime_write_held_kernel<8>, dim3(a.tile_count), dim3(kWave), held_lds, stream, w);
         else if (cvs <= 16u)
-            hipLaunchKernelGGL(runtime_write_held_kernel<16>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+            hipLaunchKernelGGL(runtime_write_held_kernel<16>, dim3(a.tile_count), dim3(kWave), held_lds, stream, w);
         else if (cvs <= 32u)
-            hipLaunchKernelGGL(runtime_write_held_kernel<32>, dim3(a.tile_count), dim3(kWave), held_lds, stream, a);
+            hipLaunchKernelGGL(runtime_write_held_kernel<32>, dim3(a.tile_count), dim3(kWave), held_lds, stream, w);
         else if (staged)
-            hipLaunchKernelGGL(runtime_write_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
+            hipLaunchKernelGGL(runtime_write_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, w);
         else
-            hipLaunchKernelGGL(runtime_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
+            hipLaunchKernelGGL(runtime_write_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, w);
     }
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }

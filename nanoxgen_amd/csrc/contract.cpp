@@ -257,6 +257,14 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
     if (uint32_t st = nxg_validate_classic_cut(curves, &common, output); st != NXG_OK) return st;
     if (plan.has_cut && !(plan.present & (1u << NXG_CLASSIC_EXPR_CUT_AMOUNT))) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     if (plan.present >> NXG_CLASSIC_EXPR_COUNT) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.has_clump) {
+        const uint32_t all = ((1u << NXG_CLASSIC_EXPR_COUNT) - 1u) & ~((1u << NXG_CLASSIC_EXPR_CLUMP_MASK) - 1u);
+        const nxg_clump_guides *g = inputs->guides;
+        if ((plan.present & all) != all || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS || !g ||
+            g->guide_count == 0u || !g->axes || !g->normals || !g->tangents || !inputs->guide_uvs ||
+            !inputs->guide_random_prefixes || (curves->strand_count && !inputs->strand_guides))
+            return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    }
     if (inputs->column_count > NXG_CLASSIC_MAX_COLUMNS || inputs->column_count < plan.columns_required ||
         (inputs->column_count != 0u) != (inputs->columns != nullptr))
         return NXG_ERROR_INVALID_CLASSIC_REQUEST;
@@ -270,4 +278,6 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
     return NXG_OK;
 }
 
-uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count) { return nxg::classic_runtime_bytes(strand_count); }
+uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count, uint32_t cvs_per_strand, uint32_t guide_count) {
+    return nxg::classic_runtime_layout(strand_count, cvs_per_strand, guide_count).total;
+}

@@ -338,16 +338,25 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
     using namespace nxg;
     if (!plan || !info) return NXG_ERROR_NULL_ARGUMENT;
     if (plan->has_cut && !plan->expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT]) return NXG_ERROR_INVALID_PROGRAM;
+    if (plan->has_clump)
+        for (uint32_t k = NXG_CLASSIC_EXPR_CLUMP_MASK; k < NXG_CLASSIC_EXPR_COUNT; ++k)
+            if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
     nxg_classic_plan_info r{};
     r.has_cut = plan->has_cut ? 1u : 0u;
+    r.has_clump = plan->has_clump ? 1u : 0u;
     r.description_id = plan->description_id;
     r.description_name_seed = plan->description_name_seed;
     uint64_t total = 0u;
     PackPlan plans[NXG_CLASSIC_EXPR_COUNT];
     for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k) {
         const nxg_expr_program *p = plan->expressions[k];
-        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut)) continue;
+        const bool clump = k >= NXG_CLASSIC_EXPR_CLUMP_MASK;
+        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut) || (clump && !plan->has_clump)) continue;
         if (!validate_classic_program(*p)) return NXG_ERROR_INVALID_PROGRAM;
+        // the noise frequency and correlation are evaluated once per guide
+        if ((k == NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY || k == NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION) &&
+            p->input_count != 0u)
+            return NXG_ERROR_INVALID_PROGRAM;
         plans[k] = plan_program(*p, true);
         if (plans[k].slot_count > NXG_CLASSIC_MAX_PROGRAM_SLOTS) return NXG_ERROR_INVALID_PROGRAM;
         r.present |= 1u << k;
@@ -360,6 +369,8 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
             if (b == NXG_CLASSIC_VAR_CLENGTH &&
                 (k == NXG_CLASSIC_EXPR_TAPER || k == NXG_CLASSIC_EXPR_TAPER_START || k == NXG_CLASSIC_EXPR_WIDTH_RAMP))
                 r.final_length = 1u;
+            if (b == NXG_CLASSIC_VAR_CLENGTH && (k == NXG_CLASSIC_EXPR_CLUMP || k == NXG_CLASSIC_EXPR_CLUMP_SCALE))
+                r.clump_length = 1u;
         }
         if (p->random_call_count != 0u) r.needs_random_prefix = 1u;
         bool uniform = p->input_count == 0u;

@@ -85,18 +85,33 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
     return l;
 }
 
-// Classic runtime workspace: the cut layout, the values of the plan's
-// uniform programs ({value, ok} pairs: one per program at t = 0, then
-// widthRamp at every CV), then one 32-byte state per strand (the search's
-// per-strand values for the write); the word after the status word collects
-// the error bits.
-constexpr uint64_t kClassicUniformBytes = (8u + NXG_CLASSIC_MAX_CVS) * 8u;
-inline uint64_t classic_runtime_uniform_offset(uint32_t strand_count) { return classic_layout(strand_count).total; }
-inline uint64_t classic_runtime_state_offset(uint32_t strand_count) {
-    return classic_runtime_uniform_offset(strand_count) + ((kClassicUniformBytes + 255u) & ~uint64_t{255u});
-}
-inline uint64_t classic_runtime_bytes(uint32_t strand_count) {
-    return classic_runtime_state_offset(strand_count) + ((uint64_t{strand_count} * 32u + 255u) & ~uint64_t{255u});
+// Classic runtime workspace, after the cut layout (each region 256-byte
+// aligned):
+//   uniform values  {value, ok} pairs of the plan's programs of t alone:
+//                   [k] program k at t = 0, then widthRamp, clumpScale and
+//                   noiseScale at every CV (kClassicCvTables tables)
+//   strand state    32 B per strand: the search's values for the write
+//   clump only:     the clumped strands (strand_count x cvs float4), and per
+//                   guide its render axis and unit noise axis (cvs float4
+//                   each) and {spline length, noise frequency-correlation ok}
+constexpr uint32_t kClassicScalarSlots = 16u;
+constexpr uint32_t kClassicCvTables = 3u;
+constexpr uint64_t kClassicUniformBytes = (kClassicScalarSlots + kClassicCvTables * NXG_CLASSIC_MAX_CVS) * 8u;
+struct ClassicRuntimeLayout {
+    uint64_t uniform_offset, state_offset, clumped_offset, render_offset, noise_offset, guide_offset, total;
+};
+inline ClassicRuntimeLayout classic_runtime_layout(uint32_t strand_count, uint32_t cvs, uint32_t guides) {
+    auto a = [](uint64_t v) { return (v + 255u) & ~uint64_t{255u}; };
+    ClassicRuntimeLayout l{};
+    l.uniform_offset = classic_layout(strand_count).total;
+    l.state_offset = l.uniform_offset + a(kClassicUniformBytes);
+    l.clumped_offset = l.state_offset + a(uint64_t{strand_count} * 32u);
+    const uint64_t clumped = guides ? uint64_t{strand_count} * cvs * 16u : 0u;
+    l.render_offset = l.clumped_offset + a(clumped);
+    l.noise_offset = l.render_offset + a(uint64_t{guides} * cvs * 16u);
+    l.guide_offset = l.noise_offset + a(uint64_t{guides} * cvs * 16u);
+    l.total = l.guide_offset + a(uint64_t{guides} * 8u);
+    return l;
 }
 
 // SeExpr hashing and XGen string seeds in double, as the reference computes

@@ -82,6 +82,8 @@ def lib():
         L.orc_clump_apply.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_uint32]
+        L.orc_seexpr_gradients.restype = None
+        L.orc_seexpr_gradients.argtypes = [C.c_void_p]
         L.orc_classic_runtime.restype = C.c_uint32
         L.orc_classic_runtime.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
@@ -146,7 +148,8 @@ def ref():
         L.ref_classic_runtime.restype = C.c_uint32
         L.ref_classic_runtime.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                           C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_float,
-                                          C.c_char_p, C.c_uint64, C.c_void_p]
+                                          C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_uint32, C.c_void_p, C.c_char_p, C.c_uint64, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
         L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -380,11 +383,27 @@ def ref_classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, ro
                 root_uvs=None if uu is None else uu[:kept], strand_count=kept, seconds=seconds.value)
 
 
+class _ClumpBinding(C.Structure):
+    """orc_clump_binding."""
+    _fields_ = [("axes3", C.c_void_p), ("render3", C.c_void_p), ("lengths", C.c_void_p), ("normals3", C.c_void_p),
+                ("tangents3", C.c_void_p), ("reference3", C.c_void_p), ("guide_uvs", C.c_void_p),
+                ("guide_prefixes", C.c_void_p), ("strand_guides", C.c_void_p), ("guide_count", C.c_uint32),
+                ("gradients", C.c_void_p)]
+
+
 class _ClassicPlan(C.Structure):
     """orc_classic_plan."""
     _fields_ = [("length", C.c_void_p), ("width", C.c_void_p), ("cut_amount", C.c_void_p), ("taper", C.c_void_p),
                 ("taper_start", C.c_void_p), ("width_ramp", C.c_void_p), ("has_cut", C.c_int),
-                ("description_id", C.c_uint32), ("name_seed", C.c_uint32)]
+                ("description_id", C.c_uint32), ("name_seed", C.c_uint32), ("has_clump", C.c_int),
+                ("clump", C.c_void_p * 7), ("binding", C.c_void_p)]
+
+
+def seexpr_gradients():
+    """The SeExpr gradient table, float32 [256, 3]."""
+    g = np.zeros((256, 3), np.float32)
+    lib().orc_seexpr_gradients(g.ctypes.data)
+    return g
 
 
 def _u32(x, n):
@@ -392,10 +411,13 @@ def _u32(x, n):
 
 
 def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitive_ids=None, random_prefixes=None,
-                    columns=None, radius_scale=1.0):
+                    columns=None, radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None,
+                    guide_random_prefixes=None):
     """orc_classic_runtime: the Classic runtime chain of `plan` (a
     nanoxgen_amd.classic.ClassicPlan: its compiled, bound programs) over
-    packed float4 points [N * C, 4].  Returns dict(points, point_counts,
+    packed float4 points [N * C, 4]; a plan with a clump also takes
+    strand_guides, guides (dict(axes, normals, tangents[, reference])),
+    guide_uvs and guide_random_prefixes.  Returns dict(points, point_counts,
     roots, root_uvs, keep, strand_count, error_bits); strand_count is None
     where the reference throws (error_bits says why)."""
     pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
@@ -409,7 +431,26 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
     op = _ClassicPlan(has_cut=1 if plan.has_cut else 0, description_id=plan.description_id,
                       name_seed=plan.name_seed)
     for key, prog in plan.programs.items():
-        setattr(op, key, C.addressof(prog.struct))
+        if key.startswith("clump"):
+            op.clump[abi.CLASSIC_CLUMP_EXPRS.index(key)] = C.addressof(prog.struct)
+        else:
+            setattr(op, key, C.addressof(prog.struct))
+    keepalive = []
+    if plan.has_clump:
+        ax = _f3(guides["axes"], -1)
+        g = len(ax) // cvs
+        render, lengths = clump_prepare(ax, cvs)
+        arrays = dict(axes3=ax, render3=render, lengths=lengths, normals3=_f3(guides["normals"], g),
+                      tangents3=_f3(guides["tangents"], g),
+                      reference3=None if guides.get("reference") is None else _f3(guides["reference"], g),
+                      guide_uvs=np.ascontiguousarray(guide_uvs, np.float32).reshape(g, 2),
+                      guide_prefixes=_u32(guide_random_prefixes, g), strand_guides=_u32(strand_guides, n),
+                      gradients=seexpr_gradients())
+        keepalive.append(arrays)
+        binding = _ClumpBinding(guide_count=g, **{k: abi.ptr(v) for k, v in arrays.items()})
+        keepalive.append(binding)
+        op.has_clump = 1
+        op.binding = C.addressof(binding)
     out = np.zeros_like(pts)
     oc = np.zeros(n, np.uint32)
     orr = np.zeros(n, abi.ROOT_DTYPE)
@@ -427,7 +468,8 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
 
 
 def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None, primitive_ids=None,
-                        random_prefixes=None, columns=None, radius_scale=1.0):
+                        random_prefixes=None, columns=None, radius_scale=1.0, clump_name="clump", strand_guides=None,
+                        guides=None, guide_uvs=None, guide_random_prefixes=None):
     """The reference's parse -> compile_xgen_classic_float_runtime_plan ->
     apply_xgen_classic_float_runtime_plan_cpu over the first description of
     `xgen_text`.  Returns dict(points, point_counts, roots, root_uvs,
@@ -442,9 +484,20 @@ def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None,
     cols = None if columns is None else np.ascontiguousarray(columns, np.float32).reshape(-1)
     err = C.create_string_buffer(1024)
     seconds = C.c_double()
+    g = 0
+    ga = gn = gt = gr = gu = gp = sg = None
+    if guides is not None:
+        ga = _f3(guides["axes"], -1)
+        g = len(ga) // cvs
+        gn, gt = _f3(guides["normals"], g), _f3(guides["tangents"], g)
+        gr = None if guides.get("reference") is None else _f3(guides["reference"], g)
+        gu = np.ascontiguousarray(guide_uvs, np.float32).reshape(g, 2)
+        gp, sg = _u32(guide_random_prefixes, g), _u32(strand_guides, n)
     kept = ref().ref_classic_runtime(xgen_text.encode(), pts.ctypes.data, cc.ctypes.data, rr.ctypes.data,
                                      abi.ptr(uu), n, cvs, abi.ptr(ids), abi.ptr(pre), abi.ptr(cols),
-                                     0 if cols is None else cols.size, radius_scale, err, 1024, C.byref(seconds))
+                                     0 if cols is None else cols.size, radius_scale, clump_name.encode(),
+                                     abi.ptr(ga), abi.ptr(gn), abi.ptr(gt), abi.ptr(gr), abi.ptr(gu), abi.ptr(gp), g,
+                                     abi.ptr(sg), err, 1024, C.byref(seconds))
     if kept == 0xFFFFFFFF:
         return dict(strand_count=None, error=err.value.decode())
     return dict(points=pts[:kept * cvs], point_counts=cc[:kept], roots=rr[:kept],

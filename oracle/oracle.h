@@ -129,12 +129,33 @@ uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, con
 #define ORC_CLASSIC_NONFINITE_PROFILE 32u
 #define ORC_CLASSIC_EXPRESSION 64u
 #define ORC_CLASSIC_VARIABLE_COUNTS 128u
+#define ORC_CLASSIC_INVALID_GUIDE 4u
+#define ORC_CLASSIC_NONFINITE_CLUMP 256u
+/* The clump's geometry: world-space guides, render axes and spline lengths
+ * (orc_clump_prepare), the guides' uv and SeExpr prefixes, a guide per
+ * strand (0xffffffff: none). */
+typedef struct orc_clump_binding {
+    const float *axes3, *render3, *lengths, *normals3, *tangents3, *reference3; /* reference3 optional */
+    const nxg_vec2 *guide_uvs;
+    const uint32_t *guide_prefixes;
+    const uint32_t *strand_guides;
+    uint32_t guide_count;
+    const nxg_vec3 *gradients;
+} orc_clump_binding;
 typedef struct orc_classic_plan {
     const nxg_expr_program *length, *width, *cut_amount, *taper, *taper_start, *width_ramp; /* NULL: absent */
     int has_cut;              /* one CutFX whose amount is cut_amount */
     uint32_t description_id;  /* face seed: runtime hash of (id, name seed, face) */
     uint32_t name_seed;
+    int has_clump;            /* one ClumpingFX before the CutFX */
+    const nxg_expr_program *clump[7]; /* mask, clump, clumpScale, noise, noiseScale, noiseFrequency, noiseCorrelation */
+    const orc_clump_binding *binding;
 } orc_classic_plan;
+void orc_seexpr_gradients(float *out); /* 256 x 3 */
+void orc_clump_noise_axis(const float *axis3, const float *distance3, uint32_t n, float raw_frequency,
+                          float raw_correlation, const float *domain3, const float *normal3, const float *tangent3,
+                          const float *magnitudes, const nxg_vec3 *gradients, float *out3);
+void orc_cut_to_length(float *points3, uint32_t n, float target);
 /* Returns the kept count (outputs compacted in order), or ~0u with the
  * ORC_CLASSIC_* bits of every strand that raised one in *error_bits. */
 uint32_t orc_classic_runtime(const float *points, const uint32_t *counts, const nxg_root_sample *roots,

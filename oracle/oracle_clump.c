@@ -99,14 +99,17 @@ static v3 rotate_between(v3 value, v3 from, v3 to, float fraction) {
     return add(add(mul(value, c), mul(cross(axis, value), s)), mul(axis, dot(axis, value) * (1.0f - c)));
 }
 
-/* build_clump_noise_axis with constant noise expressions: out[0..n) */
-static void noise_axis(const v3 *axis, const v3 *distance_axis, uint32_t n, const orc_clump_params *p, v3 domain_position,
-                       v3 normal, v3 tangent, float mask, const nxg_vec3 *gradients, v3 *out) {
+/* build_clump_noise_axis (src/xgen_classic_runtime.cpp:1041-1186) from the
+ * evaluated noise frequency and correlation and the per-CV magnitudes
+ * mask * noise * max(noiseScale(t), 0); magnitudes NULL: the zero axis (the
+ * noise or mask test failed).  out[0..n). */
+static void noise_axis_general(const v3 *axis, const v3 *distance_axis, uint32_t n, float raw_frequency,
+                               float raw_correlation, v3 domain_position, v3 normal, v3 tangent,
+                               const float *magnitudes, const nxg_vec3 *gradients, v3 *out) {
     for (uint32_t i = 0; i < n; ++i) out[i] = mk(0.0f, 0.0f, 0.0f);
-    const float noise = max_std(p->noise, 0.0f);
-    if (!(noise > 1.0e-5f) || !(mask > 1.0e-4f)) return;
-    const float frequency = max_std(p->noise_frequency, 0.0f);
-    const float correlation = clamp_std(p->noise_correlation * 0.01f, 0.0f, 1.0f);
+    if (!magnitudes) return;
+    const float frequency = max_std(raw_frequency, 0.0f);
+    const float correlation = clamp_std(raw_correlation * 0.01f, 0.0f, 1.0f);
     float polyline = 0.0f;
     for (uint32_t cv = 1u; cv < n; ++cv) polyline += sqrtf(len2(sub(distance_axis[cv], distance_axis[cv - 1u])));
     const float effective = polyline > 0.0f ? max_std(0.5f / polyline, frequency) : frequency;
@@ -141,8 +144,7 @@ static void noise_axis(const v3 *axis, const v3 *distance_axis, uint32_t n, cons
                 }
             }
         }
-        const float scale = max_std(p->noise_scale, 0.0f);
-        const float magnitude = mask * noise * scale;
+        const float magnitude = magnitudes[cv];
         const float distance = travelled * effective;
         const float first = orc_gradient_noise_f32(gradients, domain.x + distance, domain.y, domain.z) - 0.5f;
         const float second = orc_gradient_noise_f32(gradients, domain.x, domain.y, domain.z + distance) - 0.5f;
@@ -155,7 +157,29 @@ static void noise_axis(const v3 *axis, const v3 *distance_axis, uint32_t n, cons
     }
 }
 
+/* build_clump_noise_axis with constant noise expressions: out[0..n) */
+static void noise_axis(const v3 *axis, const v3 *distance_axis, uint32_t n, const orc_clump_params *p, v3 domain_position,
+                       v3 normal, v3 tangent, float mask, const nxg_vec3 *gradients, v3 *out) {
+    float magnitudes[256];
+    const float noise = max_std(p->noise, 0.0f);
+    const int on = noise > 1.0e-5f && mask > 1.0e-4f;
+    for (uint32_t cv = 0; cv < n; ++cv) magnitudes[cv] = mask * noise * max_std(p->noise_scale, 0.0f);
+    noise_axis_general(axis, distance_axis, n, p->noise_frequency, p->noise_correlation, domain_position, normal,
+                       tangent, on ? magnitudes : NULL, gradients, out);
+}
+
+void orc_clump_noise_axis(const float *axis3, const float *distance3, uint32_t n, float raw_frequency,
+                          float raw_correlation, const float *domain3, const float *normal3, const float *tangent3,
+                          const float *magnitudes, const nxg_vec3 *gradients, float *out3) {
+    noise_axis_general((const v3 *)axis3, (const v3 *)distance3, n, raw_frequency, raw_correlation,
+                       *(const v3 *)domain3, *(const v3 *)normal3, *(const v3 *)tangent3, magnitudes, gradients,
+                       (v3 *)out3);
+}
+
 /* cut_to_length_and_rebuild: points in place, n <= 256 */
+static void cut_to_length(v3 *points, uint32_t n, float target);
+void orc_cut_to_length(float *points3, uint32_t n, float target) { cut_to_length((v3 *)points3, n, target); }
+
 static void cut_to_length(v3 *points, uint32_t n, float target) {
     v3 source[256];
     memcpy(source, points, sizeof(v3) * n);

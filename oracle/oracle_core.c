@@ -15,6 +15,17 @@
 
 #include "oracle.h"
 
+/* The SeExpr gradient table (256 x 3 floats) every asset carries
+ * (seexpr_noise_table.h of the reference). */
+void orc_seexpr_gradients(float *out) {
+    static const float grads[256][3] = {
+#define NXG_GRADIENT(x, y, z) {x, y, z},
+#include "../nanoxgen_amd/csrc/seexpr_gradients.inc"
+#undef NXG_GRADIENT
+    };
+    memcpy(out, grads, sizeof(grads));
+}
+
 /* generate.h:72-79 */
 uint32_t orc_hash32(uint32_t x) {
     x ^= x >> 16u;
@@ -271,12 +282,7 @@ int orc_build_asset(const nxg_asset_build_input *in, void **out_blob, uint64_t *
     memcpy(blob + h.guide_cvs_offset, in->guide_cvs, (size_t)total_cvs * 12u);
     memcpy(blob + h.triangle_guides_offset, stencil, (size_t)T * 32u);
     {
-        static const float grads[256][3] = {
-#define NXG_GRADIENT(x, y, z) {x, y, z},
-#include "../nanoxgen_amd/csrc/seexpr_gradients.inc"
-#undef NXG_GRADIENT
-        };
-        memcpy(blob + h.noise_gradients_offset, grads, sizeof(grads));
+        orc_seexpr_gradients((float *)(blob + h.noise_gradients_offset));
     }
     h.content_hash = orc_fnv1a(blob + sizeof(h), size - sizeof(h));
     memcpy(blob, &h, sizeof(h));

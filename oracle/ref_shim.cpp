@@ -505,10 +505,16 @@ int ref_expr_compile(const char *source, uint64_t size, const char *name, const 
 // primitive ids, SeExpr prefixes and a row-major input table.  Returns the
 // kept count (outputs compacted in place) or ~0u with the reference's
 // message in err (a plan that needs a fallback reports its reasons).
+// guide_count > 0 binds the description's ClumpingFX module `clump_name` to
+// world-space guides (ClassicClumpRuntimeData: axes, normals, tangents,
+// optional reference positions, uvs and SeExpr prefixes, a guide per strand).
 uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *counts, nxg_root_sample *roots,
                              nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, const uint32_t *primitive_ids,
                              const uint32_t *prefixes, const float *inputs, uint64_t input_count,
-                             float radius_scale, char *err, uint64_t errcap, double *seconds) {
+                             float radius_scale, const char *clump_name, const float *guide_axes3,
+                             const float *guide_normals3, const float *guide_tangents3, const float *guide_reference3,
+                             const float *guide_uvs2, const uint32_t *guide_prefixes, uint32_t guide_count,
+                             const uint32_t *strand_guides, char *err, uint64_t errcap, double *seconds) {
     try {
         std::istringstream text(xgen_text);
         const ClassicCollection collection = parse_xgen_classic_collection(text);
@@ -539,8 +545,31 @@ uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *cou
             prefixes ? std::span<const std::uint32_t>(prefixes, strand_count) : std::span<const std::uint32_t>();
         const std::span<const float> rows =
             inputs ? std::span<const float>(inputs, input_count) : std::span<const float>();
+        std::vector<ClassicClumpRuntimeData> clumps;
+        if (guide_count) {
+            ClassicClumpRuntimeData d{};
+            d.module_name = clump_name;
+            d.cvs_per_guide = cvs;
+            const std::size_t axis_count = std::size_t{guide_count} * cvs;
+            d.guide_axes.resize(axis_count);
+            std::memcpy(d.guide_axes.data(), guide_axes3, axis_count * sizeof(Vec3));
+            d.guide_normals.resize(guide_count);
+            std::memcpy(d.guide_normals.data(), guide_normals3, guide_count * sizeof(Vec3));
+            d.guide_tangents.resize(guide_count);
+            std::memcpy(d.guide_tangents.data(), guide_tangents3, guide_count * sizeof(Vec3));
+            if (guide_reference3) {
+                d.guide_reference_positions.resize(guide_count);
+                std::memcpy(d.guide_reference_positions.data(), guide_reference3, guide_count * sizeof(Vec3));
+            }
+            d.guide_uvs.resize(guide_count);
+            std::memcpy(d.guide_uvs.data(), guide_uvs2, guide_count * sizeof(Vec2));
+            d.guide_face_ids.assign(guide_count, 0u);
+            d.guide_random_prefixes.assign(guide_prefixes, guide_prefixes + guide_count);
+            d.strand_guide_indices.assign(strand_guides, strand_guides + strand_count);
+            clumps.push_back(std::move(d));
+        }
         const auto start = std::chrono::steady_clock::now();
-        apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale, {}, pre, ids, {}, rows);
+        apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale, {}, pre, ids, clumps, rows);
         if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
         std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
         std::memcpy(counts, curves.point_counts.data(), curves.point_counts.size() * sizeof(uint32_t));

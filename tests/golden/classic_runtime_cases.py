@@ -18,9 +18,12 @@ import zlib
 import numpy as np
 
 # The order the reference compiles a plan's expressions in
-# (src/xgen_classic_runtime.cpp:1211-1229, then the FX modules).
-COMPILE_ORDER = (("length", "length"), ("width", "width"), ("taper", "taper"), ("taper_start", "taperStart"),
-                 ("width_ramp", "widthRamp"), ("cut_amount", "amount"))
+# (src/xgen_classic_runtime.cpp:1211-1229, then the FX modules in object
+# order: the ClumpingFX module before the CutFX).
+PRIMITIVE = (("length", "length"), ("width", "width"), ("taper", "taper"), ("taper_start", "taperStart"),
+             ("width_ramp", "widthRamp"))
+CLUMP = ("mask", "clump", "clumpScale", "noise", "noiseScale", "noiseFrequency", "noiseCorrelation")
+COMPILE_ORDER = PRIMITIVE + tuple((f"clump.{a}", a) for a in CLUMP) + (("cut_amount", "amount"),)
 
 CASES = [
     # the reference's own test_float_runtime_plan (tests/classic_test.cpp:198-246)
@@ -53,6 +56,27 @@ CASES = [
     # a widthRamp of t alone that divides by zero at t = 0.5 (CV 2 of 5)
     dict(name="uniformRampError", description_id=1, strands="ico5",
          exprs=dict(width_ramp="1 / (0.5 - rampUI(0,0,1:1,1,1))")),
+    # ClumpingFX before the CutFX: expression-driven mask (PTEX input), amount
+    # (strand prefix), per-CV clumpScale, noise through the guides' prefixes,
+    # a per-strand noiseScale; the cut amount and taper read t, which the clump
+    # leaves at 1 for the strands it touched
+    dict(name="ns:clumpCut", description_id=21, strands="walk", ids=True, prefixes=True, guides=12,
+         exprs=dict(width="0.01 + $cWidth", taper="rampUI(0,0.2,1:1,0.7,1)",
+                    cut_amount="$cLength * 0.2 + rampUI(0,0,1:1,0.1,1)"),
+         clump=dict(mask="0.3 + 0.7 * map('m.ptx')", clump="0.6 + 0.2 * rand()",
+                    clumpScale="rampUI(0,0,1:1,0.3,1)", noise="0.2 * rand(5)", noiseScale="1 - $cWidth * 10",
+                    noiseFrequency="2 + rand()", noiseCorrelation="30")),
+    # the generator's strands, a clump reading cLength, no noise, no cut
+    dict(name="icoClump", description_id=22, strands="ico5", guides=9,
+         exprs=dict(width_ramp="$cLength"),
+         clump=dict(mask="1", clump="clamp($cLength, 0, 1)", clumpScale="0.1 * $cLength", noise="0",
+                    noiseScale="1", noiseFrequency="1", noiseCorrelation="0")),
+    dict(name="clumpBadGuide", description_id=23, strands="walk", guides=4, bad_guide=True,
+         clump=dict(mask="1", clump="0.5", clumpScale="0", noise="0", noiseScale="1", noiseFrequency="1",
+                    noiseCorrelation="0"), exprs=dict()),
+    dict(name="clumpMagnitude", description_id=24, strands="walk", guides=4,
+         clump=dict(mask="0.5", clump="0.5", clumpScale="0", noise="1e30", noiseScale="1e30", noiseFrequency="1",
+                    noiseCorrelation="0"), exprs=dict()),
 ]
 
 
@@ -62,10 +86,13 @@ def xgen_text(case):
     s = "FileVersion 18\nPalette\n\tname\tpalette\n\tendAttrs\n"
     s += f"Description\n\tname\t{case['name']}\n\tdescriptionId\t{case['description_id']}\n\tendAttrs\n"
     s += "SplinePrimitive\n"
-    for key, attribute in COMPILE_ORDER[:5]:
+    for key, attribute in PRIMITIVE:
         if key in e:
             s += f"\t{attribute}\t{e[key]}\n"
     s += "\tendAttrs\n"
+    if "clump" in case:
+        s += "ClumpingFXModule\n\tactive\ttrue\n\tname\tclump\n"
+        s += "".join(f"\t{a}\t{case['clump'][a]}\n" for a in CLUMP) + "\tendAttrs\n"
     if "cut_amount" in e:
         s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{e['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
     return s + "\tActive\tSplinePrimitive\n"
@@ -76,12 +103,18 @@ _PREF = re.compile(r"\bnoise\s*\(\s*\$Prefg\s*\*\s*([-+]?[0-9.]+)\s*\)")
 
 
 def plan_sources(case):
-    """(sources in the bound form, ptex count, pref-noise frequencies)."""
+    """(sources in the bound form, ptex count, pref-noise frequencies); clump
+    expressions under the "clump" key by attribute name."""
     out, ptex, freqs = {}, 0, []
     for key, _ in COMPILE_ORDER:
-        if key not in case["exprs"]:
+        if key.startswith("clump."):
+            if "clump" not in case:
+                continue
+            src = case["clump"][key[6:]]
+        elif key not in case["exprs"]:
             continue
-        src = case["exprs"][key]
+        else:
+            src = case["exprs"][key]
 
         def map_sub(m):
             nonlocal ptex
@@ -98,7 +131,29 @@ def plan_sources(case):
             return f"$__nxg_pref_noise_{freqs.index(f)}"
 
         out[key] = _PREF.sub(pref_sub, out[key])
+    clump = {k[6:]: out.pop(k) for k in list(out) if k.startswith("clump.")}
+    if clump:
+        out["clump"] = clump
     return out, ptex, freqs
+
+
+def clump_guides(count, cvs, rng):
+    """Seeded world-space clump guides (curled verticals over [-5, 5]^2), their
+    uvs and SeExpr prefixes."""
+    roots = np.stack([rng.uniform(-5, 5, count), np.zeros(count), rng.uniform(-5, 5, count)], 1)
+    t = np.linspace(0, 1, cvs)
+    length = rng.uniform(0.5, 2.0, count)
+    phase = rng.uniform(0, 6.28, count)
+    axes = np.zeros((count, cvs, 3))
+    axes[:, :, 0] = roots[:, None, 0] + 0.2 * np.sin(phase[:, None] + 3 * t[None])
+    axes[:, :, 1] = length[:, None] * t[None]
+    axes[:, :, 2] = roots[:, None, 2] + 0.2 * np.cos(phase[:, None] + 3 * t[None])
+    normals = np.tile([0.0, 1.0, 0.0], (count, 1)) + rng.normal(0, 0.1, (count, 3))
+    tangents = np.tile([1.0, 0.0, 0.0], (count, 1))
+    return dict(axes=axes.reshape(-1, 3).astype(np.float32), normals=normals.astype(np.float32),
+                tangents=tangents.astype(np.float32), reference=(roots + 0.01).astype(np.float32),
+                uvs=rng.uniform(0, 1, (count, 2)).astype(np.float32),
+                prefixes=rng.integers(0, 2 ** 32, count, dtype=np.uint64).astype(np.uint32))
 
 
 def case_inputs(case, strands_of):
@@ -117,5 +172,25 @@ def case_inputs(case, strands_of):
     columns = rng.uniform(0.05, 1.0, size=(n, k)).astype(np.float32) if k else None
     ids = (rng.permutation(n).astype(np.uint32) if case.get("ids") else None)
     prefixes = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32) if case.get("prefixes") else None
-    return dict(points=pts, cvs=cvs, roots=roots, uvs=uvs, ids=ids, prefixes=prefixes, columns=columns,
-                radius_scale=case.get("radius_scale", 1.0))
+    out = dict(points=pts, cvs=cvs, roots=roots, uvs=uvs, ids=ids, prefixes=prefixes, columns=columns,
+               radius_scale=case.get("radius_scale", 1.0))
+    if "guides" in case:
+        g = clump_guides(case["guides"], cvs, rng)
+        strand_guides = rng.integers(0, case["guides"], n).astype(np.uint32)
+        strand_guides[::7] = 0xFFFFFFFF
+        if case.get("bad_guide"):
+            strand_guides[n // 2] = case["guides"]
+        out.update(guide_axes=g["axes"], guide_normals=g["normals"], guide_tangents=g["tangents"],
+                   guide_reference=g["reference"], guide_uvs=g["uvs"], guide_prefixes=g["prefixes"],
+                   strand_guides=strand_guides)
+    return out
+
+
+def guide_args(inp):
+    """The clump keyword arguments of case_inputs' output (empty without guides)."""
+    if "guide_axes" not in inp:
+        return {}
+    return dict(strand_guides=inp["strand_guides"], guide_uvs=inp["guide_uvs"],
+                guide_random_prefixes=inp["guide_prefixes"],
+                guides=dict(axes=inp["guide_axes"], normals=inp["guide_normals"], tangents=inp["guide_tangents"],
+                            reference=inp["guide_reference"]))

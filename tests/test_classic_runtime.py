@@ -17,7 +17,7 @@ from nanoxgen_amd import abi, classic
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
-from classic_runtime_cases import CASES, plan_sources  # noqa: E402
+from classic_runtime_cases import CASES, guide_args, plan_sources  # noqa: E402
 
 FIXTURE = os.path.join(HERE, "golden", "reference_classic_runtime.npz")
 
@@ -30,6 +30,8 @@ MESSAGES = {
     abi.CLASSIC_NONFINITE_PROFILE: "Classic width profile produced a non-finite value",
     abi.CLASSIC_EXPRESSION_ERROR: "XGen float expression produced a non-finite value",
     abi.CLASSIC_VARIABLE_COUNTS: "Classic runtime requires fixed point counts",
+    abi.CLASSIC_INVALID_GUIDE: "Classic ClumpingFX guide index is invalid",
+    abi.CLASSIC_NONFINITE_CLUMP: "Classic ClumpingFX noise magnitude is non-finite",
 }
 
 
@@ -48,7 +50,8 @@ def make_plan(case):
 def case_args(fx, i):
     g = lambda k: fx.get(f"case{i}_in_{k}")
     return dict(points=g("points"), cvs=int(g("cvs")), roots=g("roots"), uvs=g("uvs"), ids=g("ids"),
-                prefixes=g("prefixes"), columns=g("columns"), radius_scale=float(g("radius_scale")))
+                prefixes=g("prefixes"), columns=g("columns"), radius_scale=float(g("radius_scale")),
+                guides=guide_args({k[len(f"case{i}_in_"):]: v for k, v in fx.items() if k.startswith(f"case{i}_in_")}))
 
 
 def bits(a):
@@ -93,7 +96,7 @@ def test_oracle_matches_reference_fixture(orc, fixture, i):
     a = case_args(fixture, i)
     r = orc.classic_runtime(a["points"], a["cvs"], make_plan(CASES[i]), roots=a["roots"], uvs=a["uvs"],
                             primitive_ids=a["ids"], random_prefixes=a["prefixes"], columns=a["columns"],
-                            radius_scale=a["radius_scale"])
+                            radius_scale=a["radius_scale"], **a["guides"])
     if f"case{i}_error" in fixture:
         assert r["strand_count"] is None
         assert {MESSAGES[b] for b in MESSAGES if r["error_bits"] & b} == {str(fixture[f"case{i}_error"])}
