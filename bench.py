@@ -286,6 +286,7 @@ def bench_batch(args):
 
 CUT_PMC_SUMMARY = os.path.join(ROOT, "profiles", "cut_search_kernel_plane2M_pmc.json")
 RUNTIME_PMC_SUMMARY = os.path.join(ROOT, "profiles", "runtime_chain_plane2M_classic_expr_pmc.json")
+PLAN_PMC_SUMMARY = os.path.join(ROOT, "profiles", "runtime_chain_plane2M_classic_plan_pmc.json")
 
 
 def chain_counters(path, chain_ms):
@@ -406,50 +407,78 @@ CLASSIC_EXPR_ATTRS = (("length", "length"), ("width", "width"), ("taper", "taper
                       ("width_ramp", "widthRamp"))
 
 
-def classic_expr_xgen():
-    """The classic_expr plan as a Classic .xgen description."""
+# The classic_plan workload's ClumpingFX module (toward the plane's guides).
+CLASSIC_PLAN_CLUMP = dict(mask="1", clump="0.6 + 0.3 * rand()", clumpScale="rampUI(0,0,1:1,0.2,1)", noise="0.3",
+                          noiseScale="1", noiseFrequency="2", noiseCorrelation="20")
+
+
+def classic_expr_xgen(with_clump=False):
+    """The classic_expr plan (and, with_clump, the classic_plan clump before
+    the cut) as a Classic .xgen description."""
     a = CLASSIC_EXPR_AUTHORED
     s = ("FileVersion 18\nPalette\n\tname\tpalette\n\tendAttrs\nDescription\n\tname\tplane\n"
          "\tdescriptionId\t2\n\tendAttrs\nSplinePrimitive\n")
     s += "".join(f"\t{attr}\t{a[key]}\n" for key, attr in CLASSIC_EXPR_ATTRS) + "\tendAttrs\n"
+    if with_clump:
+        s += "ClumpingFXModule\n\tactive\ttrue\n\tname\tclump\n"
+        s += "".join(f"\t{k}\t{v}\n" for k, v in CLASSIC_PLAN_CLUMP.items()) + "\tendAttrs\n"
     s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{a['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
     return s + "\tActive\tSplinePrimitive\n"
 
 
-def bench_classic_expr(args):
+def bench_classic_expr(args, with_clump=False):
     """The Classic runtime with primitive expressions
     (nxg_classic_runtime_apply) over the plane workload's generated curves:
     2,000,000 strands x 16 CVs, a length and a width expression over two
     input columns, taper 0.6 from 0.25, a three-key widthRamp, and a CutFX
-    whose amount calls rand() through per-strand SeExpr prefixes.  One step =
-    the whole chain (sample tables, search with every expression, two scan
-    launches, write with the width profile + compaction); inputs are resident
-    in HBM before the timed region."""
+    whose amount calls rand() through per-strand SeExpr prefixes; with_clump
+    (classic_plan) adds a ClumpingFX module with expressions before the cut,
+    each strand clumped to its stencil's slot-0 guide among the
+    description's 4,096 guides.  One step = the whole chain (sample tables,
+    uniform programs, guide pass, search with every expression, two scan
+    launches, write with the width profile + compaction); inputs are
+    resident in HBM before the timed region."""
     import numpy as np
     import torch
     import nanoxgen_amd as nx
-    from nanoxgen_amd import workloads
+    from nanoxgen_amd import abi, workloads
     from nanoxgen_amd.classic import ClassicCutOutputs, ClassicPlan, classic_runtime, runtime_workspace_bytes
 
     w = workloads.config2()
     N, Cv = w.params.strand_count, w.params.cvs_per_strand
     blob = nx.build_asset(w.build)
-    gen = nx.PackedOutputs(N, Cv)
+    gen = nx.PackedOutputs(N, Cv, stencil=with_clump)
     nx.generate_packed(nx.DeviceAsset(blob), w.params, gen, octaves=w.octaves)
     rng = np.random.default_rng(17)
     columns = rng.uniform(0.0, 1.0, (N, 2)).astype(np.float32)
     prefixes = rng.integers(0, 2 ** 32, N, dtype=np.uint64).astype(np.uint32)
-    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, description_id=2, description_name="plane", ptex_count=2)
+    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, clump=CLASSIC_PLAN_CLUMP if with_clump else None, description_id=2,
+                       description_name="plane", ptex_count=2)
     cols_d = torch.from_numpy(columns).cuda()
     pre_d = torch.from_numpy(prefixes.view(np.int32)).cuda()
+    clump_kw, guide_host, guide_count = {}, None, 0
+    if with_clump:
+        h = abi.header_of(blob)
+        guide_count = h.guide_count
+        recs = blob[h.guides_offset:h.guides_offset + 56 * guide_count].view(abi.GUIDE_DTYPE)
+        cvs3 = blob[h.guide_cvs_offset:h.guide_cvs_offset + 12 * h.guide_cv_count].view(np.float32).reshape(-1, 3)
+        guide_host = dict(axes=cvs3.copy(), normals=recs["root_normal"].astype(np.float32),
+                          tangents=np.zeros((guide_count, 3), np.float32))
+        d = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+        guide_uvs = rng.uniform(0, 1, (guide_count, 2)).astype(np.float32)
+        guide_prefixes = rng.integers(0, 2 ** 32, guide_count, dtype=np.uint64).astype(np.uint32)
+        clump_kw = dict(strand_guides=gen.stencil_indices[:, 0].contiguous(),
+                        guides={k: d(v) for k, v in guide_host.items()}, guide_uvs=d(guide_uvs),
+                        guide_random_prefixes=torch.from_numpy(guide_prefixes.view(np.int32)).cuda())
+        guide_host.update(uvs=guide_uvs, prefixes=guide_prefixes)
     out = ClassicCutOutputs(N, Cv)
-    ws = torch.empty(runtime_workspace_bytes(N), dtype=torch.uint8, device="cuda")
+    ws = torch.empty(runtime_workspace_bytes(N, Cv, guide_count), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
-    log(f"generated {N} x {Cv} curves; plan image {plan.info.byte_size} B")
+    log(f"generated {N} x {Cv} curves; plan image {plan.info.byte_size} B; {guide_count} clump guides")
 
     def step():
         classic_runtime(gen.points, Cv, plan, roots=gen.roots, root_uvs=gen.root_uvs, point_counts=gen.point_counts,
-                        random_prefixes=pre_d, columns=cols_d, outputs=out, workspace=ws, stream=stream)
+                        random_prefixes=pre_d, columns=cols_d, outputs=out, workspace=ws, stream=stream, **clump_kw)
 
     for _ in range(args.warmup):
         step()
@@ -473,50 +502,62 @@ def bench_classic_expr(args):
     per_in = Cv * 16 + 4 + 48 + 8 + 8 + 4
     per_out = Cv * 16 + 4 + 48 + 8
     alg = N * per_in + N * 36 * 2 + kept * per_out
+    if with_clump:  # the strand's guide index, its guide's axes, the clumped copy written and read back
+        alg += N * 4 + guide_count * Cv * (12 + 32) + N * Cv * 16 * 2
     achieved = alg / (chain_mean * 1e-3) / 1e9
-    traffic, valu = chain_counters(RUNTIME_PMC_SUMMARY, chain_mean)
+    traffic, valu = chain_counters(PLAN_PMC_SUMMARY if with_clump else RUNTIME_PMC_SUMMARY, chain_mean)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "runtime chain: cut_table + classic_uniform_kernel + runtime_search_kernel<true> + "
+                "kernel": "runtime chain: cut_table + classic_uniform_kernel + "
+                          + ("runtime_guide_kernel + " if with_clump else "") + "runtime_search_kernel<true> + "
                           "cut_scan_tiles + cut_scan_groups + runtime_write_held_kernel<16> (one C-ABI call)",
                 "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
                 "algorithmic_bytes_per_launch": alg, "valu": valu}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, args.cpu_seconds)
+        cpu = classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, args.cpu_seconds, guide_host)
+    plan_text = dict(CLASSIC_EXPR_AUTHORED, **({"clump": CLASSIC_PLAN_CLUMP} if with_clump else {}))
     line = {"metric": METRIC, "value": kept * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "classic-expressions-plane2M", "input": w.name, "strands": N,
-                       "cvs_per_strand": Cv, "kept_strands": kept, "plan": CLASSIC_EXPR_AUTHORED,
-                       "parallelism": "single-gpu"},
+            "config": {"workload": "classic-plan-plane2M" if with_clump else "classic-expressions-plane2M",
+                       "input": w.name, "strands": N, "cvs_per_strand": Cv, "kept_strands": kept,
+                       "clump_guides": guide_count, "plan": plan_text, "parallelism": "single-gpu"},
             "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     return 0
 
 
-def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s):
+def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s, guides=None):
     """The reference's parser, plan compiler and
     apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread
     pool) on the first S generated strands; without that build, the oracle's
-    single-threaded restatement."""
+    single-threaded restatement.  guides (host arrays) adds the clump."""
     import oracle
     from nanoxgen_amd.classic import ClassicPlan
     host = gen.to_host()
     pts, counts, roots, uvs = host["points"], host["point_counts"], host["roots"], host["root_uvs"]
     use_ref = oracle.ref() is not None
     cores = len(os.sched_getaffinity(0)) if use_ref else 1
-    text = classic_expr_xgen()
-    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, description_id=2, description_name="plane", ptex_count=2)
+    text = classic_expr_xgen(guides is not None)
+    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, clump=CLASSIC_PLAN_CLUMP if guides is not None else None,
+                       description_id=2, description_name="plane", ptex_count=2)
+
+    def clump_kw(s):
+        if guides is None:
+            return {}
+        return dict(strand_guides=host["stencil_indices"][:s, 0], guide_uvs=guides["uvs"],
+                    guide_random_prefixes=guides["prefixes"],
+                    guides=dict(axes=guides["axes"], normals=guides["normals"], tangents=guides["tangents"]))
 
     def run(s):
         if use_ref:
             r = oracle.ref_classic_runtime(text, pts[:s * Cv], Cv, roots=roots[:s], counts=counts[:s], uvs=uvs[:s],
-                                           random_prefixes=prefixes[:s], columns=columns[:s])
+                                           random_prefixes=prefixes[:s], columns=columns[:s], **clump_kw(s))
             return r["seconds"]
         t = time.perf_counter()
         oracle.classic_runtime(pts[:s * Cv], Cv, plan, roots=roots[:s], counts=counts[:s], uvs=uvs[:s],
-                               random_prefixes=prefixes[:s], columns=columns[:s])
+                               random_prefixes=prefixes[:s], columns=columns[:s], **clump_kw(s))
         return time.perf_counter() - t
 
     s = min(N, 65536)
@@ -765,7 +806,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="plane",
-                    choices=["plane", "expr", "surrogate", "batch", "classic", "classic_expr", "clump", "transport"],
+                    choices=["plane", "expr", "surrogate", "batch", "classic", "classic_expr", "classic_plan", "clump",
+                             "transport"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
@@ -774,6 +816,7 @@ def main():
                          "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands; "
                          "classic_expr: the same strands through a Classic plan of length, width, taper, "
                          "taperStart, widthRamp and CutFX-amount expressions; "
+                         "classic_plan: the same plan with an expression ClumpingFX toward the guides before the cut; "
                          "clump: ClumpingFX toward the plane's 4,096 guides over the same strands; "
                          "transport: 64K renderer batches (transform, motion) and the .nxc cache of the same strands")
     ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
@@ -802,6 +845,8 @@ def main():
         return bench_classic(args)
     if args.workload == "classic_expr" and args.gpus == 1:
         return bench_classic_expr(args)
+    if args.workload == "classic_plan" and args.gpus == 1:
+        return bench_classic_expr(args, with_clump=True)
     if args.workload == "clump" and args.gpus == 1:
         return bench_clump(args)
     if args.workload == "transport" and args.gpus == 1:
