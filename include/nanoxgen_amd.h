@@ -888,6 +888,7 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
  *   length (cLength of the rebuilt curve); clump amount (clamped);
  *   CVs 1.. blended toward the guide's render axis by
  *   mask amount (1 - 2 clumpScale(t)), plus the noise axis; cLength again;
+ *   (each ClumpingFX module in turn, on the previous one's strand);
  *   CutFX: amount, cut_from_tip_and_rebuild, culled when the cut parameter
  *   < 1e-4, cLength of the rebuilt curve; taper, taperStart; per CV t =
  *   cv / (n - 1), scale = 1 - taper (t - start) / (1 - start) past the
@@ -918,6 +919,12 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
 #define NXG_CLASSIC_VARIABLE_COUNTS 128u    /* "Classic runtime requires fixed point counts" */
 #define NXG_CLASSIC_NONFINITE_CLUMP 256u    /* "Classic ClumpingFX noise magnitude is non-finite" */
 
+/* Hierarchical clumping: up to this many ClumpingFX modules in one plan, applied
+ * in object order before the CutFX (the reference's plan.effects, each with its
+ * own ClassicClumpRuntimeData, src/xgen_classic_runtime.cpp:1677-1793). */
+#define NXG_CLASSIC_MAX_CLUMPS 3u
+#define NXG_CLASSIC_CLUMP_EXPRS 7u
+
 enum {
     NXG_CLASSIC_EXPR_LENGTH = 0,
     NXG_CLASSIC_EXPR_WIDTH,
@@ -925,7 +932,8 @@ enum {
     NXG_CLASSIC_EXPR_TAPER,
     NXG_CLASSIC_EXPR_TAPER_START,
     NXG_CLASSIC_EXPR_WIDTH_RAMP,
-    /* the ClumpingFX module (ClassicFloatClumpModule, xgen_classic_runtime.h:43-56) */
+    /* the first ClumpingFX module (ClassicFloatClumpModule, xgen_classic_runtime.h:43-56); module m's
+     * programs follow at NXG_CLASSIC_EXPR_CLUMP_OF(m, k) */
     NXG_CLASSIC_EXPR_CLUMP_MASK,
     NXG_CLASSIC_EXPR_CLUMP,
     NXG_CLASSIC_EXPR_CLUMP_SCALE,
@@ -933,19 +941,22 @@ enum {
     NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE,
     NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY,
     NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION,
-    NXG_CLASSIC_EXPR_COUNT
+    NXG_CLASSIC_EXPR_COUNT = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_MAX_CLUMPS * NXG_CLASSIC_CLUMP_EXPRS
 };
+/* Program k (one of NXG_CLASSIC_EXPR_CLUMP_MASK .. _NOISE_CORRELATION) of ClumpingFX module m. */
+#define NXG_CLASSIC_EXPR_CLUMP_OF(m, k) ((k) + NXG_CLASSIC_CLUMP_EXPRS * (m))
 
 /* Host memory.  expressions[k] NULL: absent.  Bindings are NXG_CLASSIC_VAR_*;
  * programs hold the reference's ops only (no pow/mix/noise3/texel), at most
  * NXG_CLASSIC_MAX_PROGRAM_INPUTS inputs and NXG_CLASSIC_MAX_PROGRAM_SLOTS live
- * values.  A clump needs all seven of its expressions; its noiseFrequency
+ * values.  Each clump needs all seven of its expressions; its noiseFrequency
  * and noiseCorrelation may read no input (they are evaluated once per
  * guide). */
 typedef struct nxg_classic_plan {
     const nxg_expr_program *expressions[NXG_CLASSIC_EXPR_COUNT];
     uint32_t has_cut;               /* the CutFX module; needs expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT] */
-    uint32_t has_clump;             /* the ClumpingFX module (before the CutFX) */
+    uint32_t clump_count;           /* ClumpingFX modules, applied in order before the CutFX
+                                       (0..NXG_CLASSIC_MAX_CLUMPS) */
     uint32_t description_id;        /* descriptionId */
     uint32_t description_name_seed; /* nxg_string_seed of the description name after its last ':' */
 } nxg_classic_plan;
@@ -955,11 +966,11 @@ typedef struct nxg_classic_plan_info {
     uint64_t byte_size;           /* of the device image */
     uint32_t present;             /* bit k: expressions[k] given */
     uint32_t has_cut;
-    uint32_t has_clump;
+    uint32_t clump_count;
     uint32_t columns_required;    /* 1 + the largest input column read (0: none) */
     uint32_t needs_random_prefix; /* some program calls rand() */
     uint32_t final_length;        /* taper, taperStart or widthRamp reads cLength */
-    uint32_t clump_length;        /* the clump amount or clumpScale reads cLength */
+    uint32_t clump_length;        /* bit m: module m's clump amount or clumpScale reads cLength */
     uint32_t description_id;
     uint32_t description_name_seed;
     uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
@@ -974,6 +985,15 @@ typedef struct nxg_classic_plan_info {
 uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t capacity,
                                nxg_classic_plan_info *info);
 
+/* One ClumpingFX module's geometry binding (ClassicClumpRuntimeData,
+ * xgen_classic_runtime.h:59-94); every field is required. */
+typedef struct nxg_classic_clump_binding {
+    const uint32_t *strand_guides;    /* device: guide per strand, NXG_INVALID_INDEX for none */
+    const nxg_clump_guides *guides;   /* host struct of device arrays (world-space guide CVs) */
+    const nxg_vec2 *guide_uvs;        /* device: the guides' uv (the noise expressions' context) */
+    const uint32_t *guide_random_prefixes; /* device: the guides' SeExpr prefixes */
+} nxg_classic_clump_binding;
+
 typedef struct nxg_classic_runtime_inputs {
     const void *plan_image;           /* device copy of the packed plan, 16-byte aligned */
     const nxg_classic_plan_info *plan;/* host: the packer's report for that image */
@@ -982,12 +1002,8 @@ typedef struct nxg_classic_runtime_inputs {
     const float *columns;             /* device, row-major [strand][column_count]; NULL iff column_count 0 */
     uint32_t column_count;            /* >= plan->columns_required */
     float radius_scale;               /* finite, >= 0 */
-    /* the clump's geometry binding (ClassicClumpRuntimeData, xgen_classic_runtime.h:59-94); with a
-     * clump all are required and cvs_per_strand <= NXG_CLUMP_MAX_CVS */
-    const uint32_t *strand_guides;    /* device: guide per strand, NXG_INVALID_INDEX for none */
-    const nxg_clump_guides *guides;   /* host struct of device arrays (world-space guide CVs) */
-    const nxg_vec2 *guide_uvs;        /* device: the guides' uv (the noise expressions' context) */
-    const uint32_t *guide_random_prefixes; /* device: the guides' SeExpr prefixes */
+    const nxg_classic_clump_binding *clumps; /* host array, one per ClumpingFX module (plan->clump_count);
+                                                with a clump cvs_per_strand <= NXG_CLUMP_MAX_CVS */
 } nxg_classic_runtime_inputs;
 
 /* Host-side checks (the reference's argument checks,
@@ -995,8 +1011,8 @@ typedef struct nxg_classic_runtime_inputs {
 uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nxg_classic_runtime_inputs *inputs,
                                       const nxg_classic_output *output);
 
-/* Device workspace for nxg_classic_runtime_apply (guide_count 0 without a
- * clump). */
+/* Device workspace for nxg_classic_runtime_apply; guide_count is the sum of
+ * the modules' guide counts (0 without a clump). */
 uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count, uint32_t cvs_per_strand, uint32_t guide_count);
 
 /* On `stream`, without synchronising or allocating: the chain above, then

@@ -280,30 +280,45 @@ CLASSIC_NEGATIVE_LENGTH, CLASSIC_NONFINITE_TAPER, CLASSIC_NONFINITE_PROFILE = 8,
 CLASSIC_EXPRESSION_ERROR, CLASSIC_VARIABLE_COUNTS, CLASSIC_NONFINITE_CLUMP = 64, 128, 256
 CLASSIC_CLUMP_EXPRS = ("clump_mask", "clump", "clump_scale", "clump_noise", "clump_noise_scale",
                        "clump_noise_frequency", "clump_noise_correlation")
-CLASSIC_EXPRS = ("length", "width", "cut_amount", "taper", "taper_start", "width_ramp") + CLASSIC_CLUMP_EXPRS
+CLASSIC_MAX_CLUMPS = 3
+
+
+def clump_key(module, key):
+    """The CLASSIC_EXPRS name of clump program `key` of ClumpingFX module `module`."""
+    return key if module == 0 else f"{key}@{module}"
+
+
+# NXG_CLASSIC_EXPR_*: module m's clump programs at 6 + 7 m + k
+CLASSIC_EXPRS = (("length", "width", "cut_amount", "taper", "taper_start", "width_ramp")
+                 + tuple(clump_key(m, k) for m in range(CLASSIC_MAX_CLUMPS) for k in CLASSIC_CLUMP_EXPRS))
 
 
 class ClassicPlan(C.Structure):
     """nxg_classic_plan: the plan's programs (host nxg_expr_program pointers, NULL: absent)."""
     _fields_ = [("expressions", C.c_void_p * len(CLASSIC_EXPRS)), ("has_cut", C.c_uint32),
-                ("has_clump", C.c_uint32), ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32)]
+                ("clump_count", C.c_uint32), ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32)]
 
 
 class ClassicPlanInfo(C.Structure):
     """nxg_classic_plan_info: what nxg_classic_plan_pack reports."""
     _fields_ = [("byte_size", C.c_uint64), ("present", C.c_uint32), ("has_cut", C.c_uint32),
-                ("has_clump", C.c_uint32), ("columns_required", C.c_uint32), ("needs_random_prefix", C.c_uint32),
+                ("clump_count", C.c_uint32), ("columns_required", C.c_uint32), ("needs_random_prefix", C.c_uint32),
                 ("final_length", C.c_uint32), ("clump_length", C.c_uint32), ("description_id", C.c_uint32),
                 ("description_name_seed", C.c_uint32), ("program_offset", C.c_uint32 * len(CLASSIC_EXPRS)),
                 ("uniform", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class ClassicClumpBinding(C.Structure):
+    """nxg_classic_clump_binding: one ClumpingFX module's device arrays and host guide struct."""
+    _fields_ = [("strand_guides", C.c_void_p), ("guides", C.c_void_p), ("guide_uvs", C.c_void_p),
+                ("guide_random_prefixes", C.c_void_p)]
+
+
 class ClassicRuntimeInputs(C.Structure):
-    """nxg_classic_runtime_inputs (device pointers, host plan report, host guide struct)."""
+    """nxg_classic_runtime_inputs (device pointers, host plan report, host clump bindings)."""
     _fields_ = [("plan_image", C.c_void_p), ("plan", C.c_void_p), ("primitive_ids", C.c_void_p),
                 ("random_prefixes", C.c_void_p), ("columns", C.c_void_p), ("column_count", C.c_uint32),
-                ("radius_scale", C.c_float), ("strand_guides", C.c_void_p), ("guides", C.c_void_p),
-                ("guide_uvs", C.c_void_p), ("guide_random_prefixes", C.c_void_p)]
+                ("radius_scale", C.c_float), ("clumps", C.c_void_p)]
 
 
 class ExprCompileOptions(C.Structure):

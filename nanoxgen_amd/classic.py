@@ -99,7 +99,6 @@ def classic_cut(points, cvs, amount_scale, amount_offset, *, point_counts=None, 
     for name, x, k in (("point_counts", point_counts, 1), ("roots", roots, 12), ("root_uvs", root_uvs, 2)):
         if x is not None and (x.numel() < n * k or not x.is_contiguous()):
             raise ValueError(f"{name} must be contiguous with {n * k} elements")
-    t = lambda x: None if x is None else x.data_ptr()
     curves = abi.ClassicCurves(points=points.data_ptr(), point_counts=t(point_counts), roots=t(roots),
                                root_uvs=t(root_uvs), strand_count=n, cvs_per_strand=cvs)
     cut = abi.ClassicCutParams(amount_scale=amount_scale, amount_offset=amount_offset, radius_scale=radius_scale,
@@ -221,9 +220,10 @@ def classic_binding(name, ptex_count=0, custom_count=0):
 
 class ClassicPlan:
     """A ClassicFloatRuntimePlan (include/nanoxgen/xgen_classic_runtime.h:
-    130-165) of SplinePrimitive expressions, at most one ClumpingFX module
-    (`clump`: its seven expressions by attribute name, mask ... noiseCorrelation)
-    and at most one CutFX module, packed for the device
+    130-165) of SplinePrimitive expressions, up to three ClumpingFX modules
+    applied in order (`clump`: a dict of one module's seven expressions by
+    attribute name, mask ... noiseCorrelation, or a list of such dicts) and at
+    most one CutFX module after them, packed for the device
     (nxg_classic_plan_pack).  Each expression is text (compiled here with its
     attribute name and object type) or a CompiledProgram; ptex_count and
     custom_count place the custom and $Prefg-noise inputs in the input row."""
@@ -233,26 +233,29 @@ class ClassicPlan:
                  custom_count=0):
         given = dict(length=length, width=width, cut_amount=cut_amount, taper=taper, taper_start=taper_start,
                      width_ramp=width_ramp)
-        if clump is not None:
-            missing = set(CLUMP_KEYS) - set(clump)
+        modules = [] if clump is None else [clump] if isinstance(clump, dict) else list(clump)
+        if len(modules) > abi.CLASSIC_MAX_CLUMPS:
+            raise ValueError(f"at most {abi.CLASSIC_MAX_CLUMPS} ClumpingFX modules")
+        for m, module in enumerate(modules):
+            missing = set(CLUMP_KEYS) - set(module)
             if missing:
-                raise ValueError(f"ClumpingFX module needs {sorted(missing)}")
-            given.update({CLUMP_KEYS[k]: v for k, v in clump.items()})
+                raise ValueError(f"ClumpingFX module {m} needs {sorted(missing)}")
+            given.update({abi.clump_key(m, CLUMP_KEYS[k]): v for k, v in module.items()})
         self.programs = {}
         for key in abi.CLASSIC_EXPRS:
             src = given.get(key)
             if src is None:
                 continue
-            attribute, object_type = CLASSIC_ATTRIBUTES[key]
+            attribute, object_type = CLASSIC_ATTRIBUTES[key.split("@")[0]]
             prog = src if isinstance(src, _expr.CompiledProgram) else _expr.compile_expression(
                 src, attribute, object_type)
             self.programs[key] = prog.bind_indices(
                 [classic_binding(name, ptex_count, custom_count) for name in prog.inputs])
         self.has_cut = cut_amount is not None
-        self.has_clump = clump is not None
+        self.clump_count = len(modules)
         self.description_id = description_id
         self.name_seed = int(lib().nxg_string_seed(description_name.rsplit(":", 1)[-1].encode()))
-        self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, has_clump=1 if self.has_clump else 0,
+        self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, clump_count=self.clump_count,
                                       description_id=description_id, description_name_seed=self.name_seed)
         for k, key in enumerate(abi.CLASSIC_EXPRS):
             if key in self.programs:
@@ -278,44 +281,57 @@ def runtime_workspace_bytes(strands, cvs=2, guides=0):
 
 def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=None, primitive_ids=None,
                     random_prefixes=None, columns=None, radius_scale=1.0, strand_guides=None, guides=None,
-                    guide_uvs=None, guide_random_prefixes=None, outputs=None, workspace=None, stream=None):
+                    guide_uvs=None, guide_random_prefixes=None, clumps=None, outputs=None, workspace=None,
+                    stream=None):
     """Enqueue the Classic runtime chain of `plan` (a ClassicPlan) on
     `stream` over device curves: `points` float32 [N * cvs, 4], `roots` int32
     [N, 12] (RootSample records: the context's u, v and face); optional
     `point_counts` int32 [N], `root_uvs` float32 [N, 2], `primitive_ids`
     int32 [N] ($id), `random_prefixes` int32 [N] (SeExpr prefixes, required
     when a program calls rand()) and `columns` float32 [N, K] (the input
-    row).  A plan with a clump also takes `strand_guides` int32 [N],
-    `guides` (dict of float32 device tensors axes [G * cvs, 3], normals and
-    tangents [G, 3], optional reference [G, 3]), `guide_uvs` float32 [G, 2]
-    and `guide_random_prefixes` int32 [G].  Returns the ClassicCutOutputs
-    (its kept(), status() and to_host() synchronise)."""
+    row).  Each ClumpingFX module of the plan takes a binding: `clumps`, a
+    list of dicts with `strand_guides` int32 [N], `guides` (dict of float32
+    device tensors axes [G * cvs, 3], normals and tangents [G, 3], optional
+    reference [G, 3]), `guide_uvs` float32 [G, 2] and `guide_random_prefixes`
+    int32 [G]; for a single module those four may be passed as keywords
+    instead.  Returns the ClassicCutOutputs (its kept(), status() and
+    to_host() synchronise)."""
     torch = _torch()
     if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
         raise ValueError("points must be a contiguous float32 [N * cvs, 4] tensor")
     n = points.shape[0] // cvs
+    if clumps is None and guides is not None:
+        clumps = [dict(strand_guides=strand_guides, guides=guides, guide_uvs=guide_uvs,
+                       guide_random_prefixes=guide_random_prefixes)]
+    clumps = list(clumps or [])
+    t = lambda x: None if x is None else x.data_ptr()
     guide_count = 0
-    guide_struct = None
-    if guides is not None:
-        guide_count = guides["normals"].shape[0]
-        for name, x, k in (("axes", guides["axes"], guide_count * cvs * 3), ("normals", guides["normals"], guide_count * 3),
-                           ("tangents", guides["tangents"], guide_count * 3),
-                           ("reference", guides.get("reference"), guide_count * 3),
-                           ("guide_uvs", guide_uvs, guide_count * 2),
-                           ("guide_random_prefixes", guide_random_prefixes, guide_count),
-                           ("strand_guides", strand_guides, n)):
+    keep = []  # the ctypes structs the call points at
+    bindings = (abi.ClassicClumpBinding * max(len(clumps), 1))()
+    for m, binding in enumerate(clumps):
+        gs = binding["guides"]
+        g = gs["normals"].shape[0]
+        for name, x, k in (("axes", gs["axes"], g * cvs * 3), ("normals", gs["normals"], g * 3),
+                           ("tangents", gs["tangents"], g * 3), ("reference", gs.get("reference"), g * 3),
+                           ("guide_uvs", binding["guide_uvs"], g * 2),
+                           ("guide_random_prefixes", binding["guide_random_prefixes"], g),
+                           ("strand_guides", binding["strand_guides"], n)):
             if x is not None and (x.numel() < k or not x.is_contiguous() or x.element_size() != 4):
-                raise ValueError(f"{name} must be contiguous 32-bit with {k} elements")
-        ref = guides.get("reference")
-        guide_struct = abi.ClumpGuides(axes=guides["axes"].data_ptr(), normals=guides["normals"].data_ptr(),
-                                       tangents=guides["tangents"].data_ptr(),
-                                       reference_positions=None if ref is None else ref.data_ptr(),
-                                       guide_count=guide_count)
+                raise ValueError(f"clump {m}: {name} must be contiguous 32-bit with {k} elements")
+        gd = abi.ClumpGuides(axes=gs["axes"].data_ptr(), normals=gs["normals"].data_ptr(),
+                             tangents=gs["tangents"].data_ptr(), reference_positions=t(gs.get("reference")),
+                             guide_count=g)
+        keep.append(gd)
+        bindings[m] = abi.ClassicClumpBinding(strand_guides=t(binding["strand_guides"]), guides=C.addressof(gd),
+                                              guide_uvs=t(binding["guide_uvs"]),
+                                              guide_random_prefixes=t(binding["guide_random_prefixes"]))
+        if m < plan.clump_count:
+            guide_count += g
     if outputs is None:
         outputs = ClassicCutOutputs(n, cvs, device=points.device, point_counts=True, roots=True,
                                     root_uvs=root_uvs is not None)
     if workspace is None:
-        workspace = torch.empty(runtime_workspace_bytes(n, cvs, guide_count if plan.has_clump else 0),
+        workspace = torch.empty(runtime_workspace_bytes(n, cvs, guide_count),
                                 dtype=torch.uint8, device=points.device)
     column_count = 0 if columns is None else (columns.shape[1] if columns.dim() == 2 else 0)
     for name, x, k in (("roots", roots, 12), ("point_counts", point_counts, 1), ("root_uvs", root_uvs, 2),
@@ -323,16 +339,14 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
                        ("columns", columns, column_count)):
         if x is not None and (x.numel() < n * k or not x.is_contiguous() or x.element_size() != 4):
             raise ValueError(f"{name} must be contiguous 32-bit with {n * k} elements")
-    t = lambda x: None if x is None else x.data_ptr()
     curves = abi.ClassicCurves(points=points.data_ptr(), point_counts=t(point_counts), roots=t(roots),
                                root_uvs=t(root_uvs), strand_count=n, cvs_per_strand=cvs)
     image = plan.device_image(points.device)
     inputs = abi.ClassicRuntimeInputs(plan_image=image.data_ptr(), plan=C.addressof(plan.info),
                                       primitive_ids=t(primitive_ids), random_prefixes=t(random_prefixes),
                                       columns=t(columns) if column_count else None, column_count=column_count,
-                                      radius_scale=radius_scale, strand_guides=t(strand_guides),
-                                      guides=None if guide_struct is None else C.addressof(guide_struct),
-                                      guide_uvs=t(guide_uvs), guide_random_prefixes=t(guide_random_prefixes))
+                                      radius_scale=radius_scale,
+                                      clumps=C.addressof(bindings) if clumps else None)
     out = outputs.descriptor()
     check(lib().nxg_classic_runtime_apply(C.byref(curves), C.byref(inputs), C.byref(out),
                                           C.c_void_p(workspace.data_ptr()), workspace.numel(),

@@ -82,6 +82,16 @@ struct Sample {
 static_assert(sizeof(Sample) == 32, "one 32-byte scalar load per sample");
 constexpr uint32_t kAtRoot = 1u, kAtTip = 2u, kReflectLow = 4u, kReflectHigh = 8u;
 
+// One ClumpingFX module's geometry binding; its guides own entries
+// guide_base .. guide_base + guide_count - 1 of the per-guide workspace arrays.
+struct ClumpModule {
+    uint32_t guide_count, guide_base;
+    const uint32_t *strand_guides;
+    const nxg_vec3 *axes, *normals, *tangents, *reference;
+    const nxg_vec2 *uvs;
+    const uint32_t *prefixes;
+};
+
 struct CutArgs {
     const float4 *points;
     const uint32_t *counts;
@@ -115,16 +125,14 @@ struct CutArgs {
     float2 *uniform_values;      // {value, ok} per uniform program (uniform_slot)
     struct RuntimeState *state;  // per strand, from the search to the write
     uint32_t *err_bits;          // the NXG_CLASSIC_* bits of every failing strand
-    // the ClumpingFX module
-    uint32_t has_clump, clump_length, guide_count;
-    const uint32_t *strand_guides;
-    const nxg_vec3 *guide_axes, *guide_normals, *guide_tangents, *guide_reference;
-    const nxg_vec2 *guide_uvs;
-    const uint32_t *guide_prefixes;
-    float4 *clumped;             // [strands x cvs] every clumped (or passed-through) strand
-    float4 *guide_render;        // [guides x cvs] render axes
-    float4 *guide_noise;         // [guides x cvs] unit clump-noise axes
-    float2 *guide_info;          // [guides] {spline length, 1 if its noise frequency/correlation evaluated}
+    // the ClumpingFX modules, in order
+    uint32_t clump_count;
+    uint32_t clump_length;       // bit m: module m's amount or clumpScale reads cLength
+    ClumpModule clumps[NXG_CLASSIC_MAX_CLUMPS];
+    float4 *clumped;             // [strands x cvs] every strand after the last module
+    float4 *guide_render;        // [all modules' guides x cvs] render axes
+    float4 *guide_noise;         // [all modules' guides x cvs] unit clump-noise axes
+    float2 *guide_info;          // [all modules' guides] {spline length, 1 if its noise frequency/correlation evaluated}
 };
 
 // A lane's CVs: staged in LDS (CV-major; ROTATED: row c turned by c lanes),
@@ -935,10 +943,24 @@ __device__ __forceinline__ float cv_parameter(uint32_t cv, uint32_t n) {
     return static_cast<float>(cv) / static_cast<float>(n - 1u);
 }
 
-// The programs evaluated per CV (at t = cv / (n - 1)); the others run at t = 0.
+// The programs evaluated per CV (at t = cv / (n - 1)): widthRamp (table 0)
+// and each clump module's clumpScale and noiseScale (tables 1 + 2m, 2 + 2m);
+// the others run at t = 0.
 __device__ __forceinline__ int cv_table(uint32_t k) {
-    return k == NXG_CLASSIC_EXPR_WIDTH_RAMP ? 0 : k == NXG_CLASSIC_EXPR_CLUMP_SCALE ? 1
-         : k == NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE ? 2 : -1;
+    if (k == NXG_CLASSIC_EXPR_WIDTH_RAMP) return 0;
+    if (k < NXG_CLASSIC_EXPR_CLUMP_MASK) return -1;
+    const uint32_t m = (k - NXG_CLASSIC_EXPR_CLUMP_MASK) / NXG_CLASSIC_CLUMP_EXPRS;
+    const uint32_t role = k - NXG_CLASSIC_CLUMP_EXPRS * m;
+    return role == NXG_CLASSIC_EXPR_CLUMP_SCALE         ? static_cast<int>(1u + 2u * m)
+           : role == NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE ? static_cast<int>(2u + 2u * m)
+                                                        : -1;
+}
+// The program behind CV table `table`.
+__device__ __forceinline__ uint32_t cv_table_program(uint32_t table) {
+    if (table == 0u) return NXG_CLASSIC_EXPR_WIDTH_RAMP;
+    const uint32_t m = (table - 1u) / 2u;
+    return NXG_CLASSIC_EXPR_CLUMP_OF(m, (table - 1u) % 2u == 0u ? NXG_CLASSIC_EXPR_CLUMP_SCALE
+                                                                : NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE);
 }
 // The programs a clump leaves t at 1 for: the reference's context keeps the
 // t of the clump's last CV (src/xgen_classic_runtime.cpp:1761-1764), which
@@ -994,8 +1016,7 @@ __global__ void __launch_bounds__(256) classic_uniform_kernel(CutArgs a) {
             const uint32_t table = (j - kClassicScalarSlots) / NXG_CLASSIC_MAX_CVS;
             cv = (j - kClassicScalarSlots) % NXG_CLASSIC_MAX_CVS;
             if (cv >= a.cvs) continue;
-            k = table == 0u ? NXG_CLASSIC_EXPR_WIDTH_RAMP
-                : table == 1u ? NXG_CLASSIC_EXPR_CLUMP_SCALE : NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE;
+            k = cv_table_program(table);
             t = cv_parameter(cv, a.cvs);
         }
         if (!((a.uniform >> k) & 1u)) continue;
@@ -1006,40 +1027,45 @@ __global__ void __launch_bounds__(256) classic_uniform_kernel(CutArgs a) {
     }
 }
 
-// The context the clump's noise expressions see (build_clump_noise_axis,
+// The context module m's noise expressions see (build_clump_noise_axis,
 // src/xgen_classic_runtime.cpp:1730-1737, 1057-1059): the strand's, with the
 // guide's uv and its SeExpr prefix.  The face seed feeds only a rand()
 // without a prefix, which this context never evaluates.
-__device__ __forceinline__ ClassicContext guide_context(const CutArgs &a, const ClassicContext &strand, uint32_t g) {
+__device__ __forceinline__ ClassicContext guide_context(const ClumpModule &cm, const ClassicContext &strand,
+                                                        uint32_t g) {
     ClassicContext c = strand;
-    const nxg_vec2 uv = a.guide_uvs[g];
+    const nxg_vec2 uv = cm.uvs[g];
     c.expr.u = uv.x;
     c.expr.v = uv.y;
     c.expr.face_seed = 0.0f;
-    c.expr.random_prefix = a.guide_prefixes[g];
+    c.expr.random_prefix = cm.prefixes[g];
     c.expr.has_random_prefix = true;
     return c;
 }
 
-// Per guide, once per call: render axis, spline length and the unit
-// clump-noise axis with the guide's noise frequency and correlation
+// Per guide of module m, once per call: render axis, spline length and the
+// unit clump-noise axis with the guide's noise frequency and correlation
 // (programs that read no input, so one value per guide).
-__global__ void __launch_bounds__(64) runtime_guide_kernel(CutArgs a) {
+__global__ void __launch_bounds__(64) runtime_guide_kernel(CutArgs a, uint32_t m) {
+    const ClumpModule &cm = a.clumps[m];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.guide_count) return;
+    if (g >= cm.guide_count) return;
     const uint32_t n = a.cvs;
-    const ClassicContext gc = guide_context(a, ClassicContext{}, g);
+    const uint64_t slot = uint64_t{cm.guide_base} + g;
+    const ClassicContext gc = guide_context(cm, ClassicContext{}, g);
     float raw_frequency = 0.0f, raw_correlation = 0.0f;
-    const bool ok = classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY, gc, 0.0f, 0.0f, 0u, raw_frequency) &&
-                    classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION, gc, 0.0f, 0.0f, 0u, raw_correlation);
+    const bool ok =
+        classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY), gc, 0.0f, 0.0f, 0u,
+                      raw_frequency) &&
+        classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION), gc, 0.0f, 0.0f, 0u,
+                      raw_correlation);
     const float frequency = max_std(raw_frequency, 0.0f);
     const float correlation = clamp_std(raw_correlation * 0.01f, 0.0f, 1.0f);
-    const nxg_vec3 *axes = a.guide_axes + uint64_t{g} * n;
+    const nxg_vec3 *axes = cm.axes + uint64_t{g} * n;
     float length = 0.0f;
-    clump_guide_axes(axes, n, a.guide_reference ? a.guide_reference[g] : axes[0], a.guide_normals[g],
-                     a.guide_tangents[g], frequency, correlation, ok ? 1.0f : 0.0f, a.guide_render + uint64_t{g} * n,
-                     a.guide_noise + uint64_t{g} * n, &length);
-    a.guide_info[g] = make_float2(length, ok ? 1.0f : 0.0f);
+    clump_guide_axes(axes, n, cm.reference ? cm.reference[g] : axes[0], cm.normals[g], cm.tangents[g], frequency,
+                     correlation, ok ? 1.0f : 0.0f, a.guide_render + slot * n, a.guide_noise + slot * n, &length);
+    a.guide_info[slot] = make_float2(length, ok ? 1.0f : 0.0f);
 }
 
 // scale_curve (src/xgen_classic_runtime.cpp:576-585) as a view: positions
@@ -1099,44 +1125,46 @@ __device__ __forceinline__ float taper_scale(float t, float taper, float taper_s
     return scale;
 }
 
-// The ClumpingFX step of one strand (src/xgen_classic_runtime.cpp:1680-1793)
-// over its staged CVs: the strand's CVs after the step (unchanged when it
-// has no guide or a zero mask) go to a.clumped and back into its LDS row;
-// c_length follows the reference's context.  False (with *err) where the
-// reference throws.
-__device__ __forceinline__ bool runtime_clump(const CutArgs &a, const ClassicContext &c,
+// ClumpingFX module m on one strand (src/xgen_classic_runtime.cpp:1680-1793)
+// over its staged CVs.  A strand the module clumps goes to a.clumped and
+// back into its LDS row (the next module, the cut and the write read it from
+// there); c_length follows the reference's context.  at_one tracks the
+// context's t: a module with a valid guide sets it to 0 for its mask and,
+// when it clumps, leaves the t of its last CV, 1.  False (with *err) where
+// the reference throws.
+__device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, const ClassicContext &c,
                                               const StagedCurve<true> &p, uint32_t s, float &c_length,
-                                              float c_width, bool &clumped, uint32_t *err) {
+                                              float c_width, bool &at_one, bool &clumped, uint32_t *err) {
     const uint32_t n = a.cvs;
-    float4 *out = a.clumped + uint64_t{s} * n;
-    const uint32_t g = a.strand_guides[s];
-    float mask = 0.0f;
-    if (g != NXG_INVALID_INDEX) {
-        if (g >= a.guide_count) {
-            *err = NXG_CLASSIC_INVALID_GUIDE;
-            return false;
-        }
-        float raw_mask;
-        if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_MASK, c, c_length, c_width, 0u, raw_mask)) {
-            *err = NXG_CLASSIC_EXPRESSION_ERROR;
-            return false;
-        }
-        mask = clamp_std(raw_mask, 0.0f, 1.0f);
+    const ClumpModule &cm = a.clumps[m];
+    const uint32_t g = cm.strand_guides[s];
+    if (g == NXG_INVALID_INDEX) return true;
+    if (g >= cm.guide_count) {
+        *err = NXG_CLASSIC_INVALID_GUIDE;
+        return false;
     }
-    if (!(mask > 0.0f)) {  // the strand passes through
-        for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = p[cv];
-        return true;
+    at_one = false;
+    float raw_mask;
+    if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_MASK), c, c_length, c_width, 0u,
+                       raw_mask)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return false;
     }
+    const float mask = clamp_std(raw_mask, 0.0f, 1.0f);
+    if (!(mask > 0.0f)) return true;  // the strand passes through
     // the noise axis's parameters, in the guide's context (:1041-1186)
-    const ClassicContext gc = guide_context(a, c, g);
+    const ClassicContext gc = guide_context(cm, c, g);
+    const uint32_t noise_scale = NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE);
     float raw_noise;
-    if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE, gc, c_length, c_width, 0u, raw_noise)) {
+    if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_NOISE), gc, c_length, c_width, 0u,
+                       raw_noise)) {
         *err = NXG_CLASSIC_EXPRESSION_ERROR;
         return false;
     }
     const float noise = max_std(raw_noise, 0.0f);
     const bool noisy = noise > 1.0e-5f && mask > 1.0e-4f;
-    const float2 info = a.guide_info[g];
+    const uint64_t slot = uint64_t{cm.guide_base} + g;
+    const float2 info = a.guide_info[slot];
     if (noisy) {
         if (info.y == 0.0f) {  // the guide's noise frequency or correlation failed
             *err = NXG_CLASSIC_EXPRESSION_ERROR;
@@ -1144,7 +1172,7 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, const ClassicCon
         }
         for (uint32_t cv = 1u; cv < n; ++cv) {
             float raw_scale;
-            if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE, gc, c_length, c_width, cv, raw_scale)) {
+            if (!classic_value(a, noise_scale, gc, c_length, c_width, cv, raw_scale)) {
                 *err = NXG_CLASSIC_EXPRESSION_ERROR;
                 return false;
             }
@@ -1157,21 +1185,25 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, const ClassicCon
     // cut_to_length_and_rebuild to the guide's length, then the amount
     const float cut = cut_to_length(a.length_samples, p, n, info.x);
     const float noise_length = c_length;  // the noise expressions' cLength
-    if (a.clump_length) c_length = spline_length(a.length_samples, RebuiltCurve<StagedCurve<true>>{p, n, cut}, n);
+    if ((a.clump_length >> m) & 1u)
+        c_length = spline_length(a.length_samples, RebuiltCurve<StagedCurve<true>>{p, n, cut}, n);
     float raw_amount;
-    if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP, c, c_length, c_width, 0u, raw_amount)) {
+    if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP), c, c_length, c_width, 0u,
+                       raw_amount)) {
         *err = NXG_CLASSIC_EXPRESSION_ERROR;
         return false;
     }
     const float amount = clamp_std(raw_amount, 0.0f, 1.0f);
-    const float4 *render = a.guide_render + uint64_t{g} * n;
-    const float4 *unit = a.guide_noise + uint64_t{g} * n;
+    float4 *out = a.clumped + uint64_t{s} * n;
+    const float4 *render = a.guide_render + slot * n;
+    const float4 *unit = a.guide_noise + slot * n;
     const float step = cut / static_cast<float>(n - 1u);
     out[0] = p[0];
     for (uint32_t cv = 1u; cv < n; ++cv) {
         F3 q = cut < 0.0f ? xyz(p[0]) : curve_eval(p, n, step * static_cast<float>(cv));
         float scale;
-        if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_SCALE, c, c_length, c_width, cv, scale)) {
+        if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_SCALE), c, c_length, c_width, cv,
+                           scale)) {
             *err = NXG_CLASSIC_EXPRESSION_ERROR;
             return false;
         }
@@ -1180,7 +1212,7 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, const ClassicCon
         F3 d{0.0f, 0.0f, 0.0f};
         if (noisy) {
             float raw_scale = 0.0f;
-            classic_value(a, NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE, gc, noise_length, c_width, cv, raw_scale);
+            classic_value(a, noise_scale, gc, noise_length, c_width, cv, raw_scale);
             d = xyz(unit[cv]) * (mask * noise * max_std(raw_scale, 0.0f));
         }
         q = q + d;
@@ -1188,6 +1220,7 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, const ClassicCon
     }
     for (uint32_t cv = 0u; cv < n; ++cv) p.tile[staged_index<true>(cv, p.lane)] = out[cv];
     c_length = spline_length(a.length_samples, p, n);
+    at_one = true;
     clumped = true;
     return true;
 }
@@ -1233,10 +1266,19 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     }
     if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
     const float c_width = width;
-    bool clumped = false;  // the clump ran its CV loop, leaving the context's t at 1
+    bool clumped = false;  // the context's t is 1: the last module with a guide clumped the strand
     if constexpr (std::is_same<Curve, StagedCurve<true>>::value) {
-        // the clump (strands of at most 64 CVs, always staged)
-        if (a.has_clump && !runtime_clump(a, c, source, s, c_length, c_width, clumped, err)) return -1.0f;
+        // the clump modules in order (strands of at most 64 CVs, always staged)
+        if (a.clump_count) {
+            bool any = false;
+#pragma unroll 1
+            for (uint32_t m = 0; m < a.clump_count; ++m)
+                if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, any, err)) return -1.0f;
+            if (!any) {  // the write reads every strand from a.clumped
+                float4 *out = a.clumped + uint64_t{s} * n;
+                for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = source[cv];
+            }
+        }
     }
     float cut = 1.0f;
     bool keep = true;
@@ -1555,7 +1597,8 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     if (uint32_t s = nxg_validate_classic_runtime(curves, inputs, output); s != NXG_OK) return s;
     const uint32_t n = curves->strand_count, cvs = curves->cvs_per_strand;
     const nxg_classic_plan_info &plan = *inputs->plan;
-    const uint32_t guides = plan.has_clump ? inputs->guides->guide_count : 0u;
+    uint32_t guides = 0u;
+    for (uint32_t m = 0; m < plan.clump_count; ++m) guides += inputs->clumps[m].guides->guide_count;
     const ClassicLayout layout = classic_layout(n);
     const ClassicRuntimeLayout rt = classic_runtime_layout(n, cvs, guides);
     if (!workspace || workspace_bytes < rt.total || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
@@ -1580,18 +1623,25 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.uniform_values = reinterpret_cast<float2 *>(ws + rt.uniform_offset);
     a.state = reinterpret_cast<RuntimeState *>(ws + rt.state_offset);
     a.err_bits = a.status_word + 1;
-    if (plan.has_clump) {
-        const nxg_clump_guides &g = *inputs->guides;
-        a.has_clump = 1u;
+    if (plan.clump_count) {
+        a.clump_count = plan.clump_count;
         a.clump_length = plan.clump_length;
-        a.guide_count = g.guide_count;
-        a.strand_guides = inputs->strand_guides;
-        a.guide_axes = g.axes;
-        a.guide_normals = g.normals;
-        a.guide_tangents = g.tangents;
-        a.guide_reference = g.reference_positions;
-        a.guide_uvs = inputs->guide_uvs;
-        a.guide_prefixes = inputs->guide_random_prefixes;
+        uint32_t base = 0u;
+        for (uint32_t m = 0; m < plan.clump_count; ++m) {
+            const nxg_classic_clump_binding &b = inputs->clumps[m];
+            const nxg_clump_guides &g = *b.guides;
+            ClumpModule &cm = a.clumps[m];
+            cm.guide_count = g.guide_count;
+            cm.guide_base = base;
+            base += g.guide_count;
+            cm.strand_guides = b.strand_guides;
+            cm.axes = g.axes;
+            cm.normals = g.normals;
+            cm.tangents = g.tangents;
+            cm.reference = g.reference_positions;
+            cm.uvs = b.guide_uvs;
+            cm.prefixes = b.guide_random_prefixes;
+        }
         a.clumped = reinterpret_cast<float4 *>(ws + rt.clumped_offset);
         a.guide_render = reinterpret_cast<float4 *>(ws + rt.render_offset);
         a.guide_noise = reinterpret_cast<float4 *>(ws + rt.noise_offset);
@@ -1604,8 +1654,9 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
         hipLaunchKernelGGL(cut_table_kernel, dim3(1), dim3(256), 0, stream, a);
         hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, a.err_bits);
         if (a.uniform) hipLaunchKernelGGL(classic_uniform_kernel, dim3(1), dim3(256), 0, stream, a);
-        if (a.has_clump)
-            hipLaunchKernelGGL(runtime_guide_kernel, dim3((a.guide_count + 63u) / 64u), dim3(64), 0, stream, a);
+        for (uint32_t m = 0; m < a.clump_count; ++m)
+            hipLaunchKernelGGL(runtime_guide_kernel, dim3((a.clumps[m].guide_count + 63u) / 64u), dim3(64), 0,
+                               stream, a, m);
         if (staged)
             hipLaunchKernelGGL(runtime_search_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
         else
@@ -1616,7 +1667,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     if (a.tile_count) {
         // after a clump the write starts from the clumped strands, already scaled
         CutArgs w = a;
-        if (a.has_clump) {
+        if (a.clump_count) {
             w.points = a.clumped;
             w.present &= ~(1u << NXG_CLASSIC_EXPR_LENGTH);
         }

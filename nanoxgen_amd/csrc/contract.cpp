@@ -257,13 +257,24 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
     if (uint32_t st = nxg_validate_classic_cut(curves, &common, output); st != NXG_OK) return st;
     if (plan.has_cut && !(plan.present & (1u << NXG_CLASSIC_EXPR_CUT_AMOUNT))) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     if (plan.present >> NXG_CLASSIC_EXPR_COUNT) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
-    if (plan.has_clump) {
-        const uint32_t all = ((1u << NXG_CLASSIC_EXPR_COUNT) - 1u) & ~((1u << NXG_CLASSIC_EXPR_CLUMP_MASK) - 1u);
-        const nxg_clump_guides *g = inputs->guides;
-        if ((plan.present & all) != all || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS || !g ||
-            g->guide_count == 0u || !g->axes || !g->normals || !g->tangents || !inputs->guide_uvs ||
-            !inputs->guide_random_prefixes || (curves->strand_count && !inputs->strand_guides))
+    if (plan.clump_count > NXG_CLASSIC_MAX_CLUMPS) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.clump_count) {
+        const uint32_t first = NXG_CLASSIC_EXPR_CLUMP_MASK;
+        const uint32_t used = first + NXG_CLASSIC_CLUMP_EXPRS * plan.clump_count;
+        const uint32_t all = ((1u << used) - 1u) & ~((1u << first) - 1u);
+        if ((plan.present & all) != all || (plan.present >> used) || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS ||
+            !inputs->clumps)
             return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+        uint64_t guides = 0u;
+        for (uint32_t m = 0; m < plan.clump_count; ++m) {
+            const nxg_classic_clump_binding &b = inputs->clumps[m];
+            const nxg_clump_guides *g = b.guides;
+            if (!g || g->guide_count == 0u || !g->axes || !g->normals || !g->tangents || !b.guide_uvs ||
+                !b.guide_random_prefixes || (curves->strand_count && !b.strand_guides))
+                return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+            guides += g->guide_count;
+        }
+        if (guides > 0xFFFFFFFFu) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     }
     if (inputs->column_count > NXG_CLASSIC_MAX_COLUMNS || inputs->column_count < plan.columns_required ||
         (inputs->column_count != 0u) != (inputs->columns != nullptr))

@@ -338,23 +338,26 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
     using namespace nxg;
     if (!plan || !info) return NXG_ERROR_NULL_ARGUMENT;
     if (plan->has_cut && !plan->expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT]) return NXG_ERROR_INVALID_PROGRAM;
-    if (plan->has_clump)
-        for (uint32_t k = NXG_CLASSIC_EXPR_CLUMP_MASK; k < NXG_CLASSIC_EXPR_COUNT; ++k)
-            if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
+    if (plan->clump_count > NXG_CLASSIC_MAX_CLUMPS) return NXG_ERROR_INVALID_PROGRAM;
+    const uint32_t used = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_CLUMP_EXPRS * plan->clump_count;
+    for (uint32_t k = NXG_CLASSIC_EXPR_CLUMP_MASK; k < used; ++k)
+        if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
     nxg_classic_plan_info r{};
     r.has_cut = plan->has_cut ? 1u : 0u;
-    r.has_clump = plan->has_clump ? 1u : 0u;
+    r.clump_count = plan->clump_count;
     r.description_id = plan->description_id;
     r.description_name_seed = plan->description_name_seed;
     uint64_t total = 0u;
     PackPlan plans[NXG_CLASSIC_EXPR_COUNT];
     for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k) {
         const nxg_expr_program *p = plan->expressions[k];
-        const bool clump = k >= NXG_CLASSIC_EXPR_CLUMP_MASK;
-        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut) || (clump && !plan->has_clump)) continue;
+        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut) || k >= used) continue;
+        // module and role of a clump program (role = its module-0 index)
+        const uint32_t module = k >= NXG_CLASSIC_EXPR_CLUMP_MASK ? (k - NXG_CLASSIC_EXPR_CLUMP_MASK) / NXG_CLASSIC_CLUMP_EXPRS : 0u;
+        const uint32_t role = k - NXG_CLASSIC_CLUMP_EXPRS * module;
         if (!validate_classic_program(*p)) return NXG_ERROR_INVALID_PROGRAM;
         // the noise frequency and correlation are evaluated once per guide
-        if ((k == NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY || k == NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION) &&
+        if ((role == NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY || role == NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION) &&
             p->input_count != 0u)
             return NXG_ERROR_INVALID_PROGRAM;
         plans[k] = plan_program(*p, true);
@@ -369,8 +372,9 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
             if (b == NXG_CLASSIC_VAR_CLENGTH &&
                 (k == NXG_CLASSIC_EXPR_TAPER || k == NXG_CLASSIC_EXPR_TAPER_START || k == NXG_CLASSIC_EXPR_WIDTH_RAMP))
                 r.final_length = 1u;
-            if (b == NXG_CLASSIC_VAR_CLENGTH && (k == NXG_CLASSIC_EXPR_CLUMP || k == NXG_CLASSIC_EXPR_CLUMP_SCALE))
-                r.clump_length = 1u;
+            if (b == NXG_CLASSIC_VAR_CLENGTH && k >= NXG_CLASSIC_EXPR_CLUMP_MASK &&
+                (role == NXG_CLASSIC_EXPR_CLUMP || role == NXG_CLASSIC_EXPR_CLUMP_SCALE))
+                r.clump_length |= 1u << module;
         }
         if (p->random_call_count != 0u) r.needs_random_prefix = 1u;
         bool uniform = p->input_count == 0u;

@@ -94,8 +94,8 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
 //   clump only:     the clumped strands (strand_count x cvs float4), and per
 //                   guide its render axis and unit noise axis (cvs float4
 //                   each) and {spline length, noise frequency-correlation ok}
-constexpr uint32_t kClassicScalarSlots = 16u;
-constexpr uint32_t kClassicCvTables = 3u;
+constexpr uint32_t kClassicScalarSlots = 32u;
+constexpr uint32_t kClassicCvTables = 1u + 2u * NXG_CLASSIC_MAX_CLUMPS;  // widthRamp; clumpScale, noiseScale per module
 constexpr uint64_t kClassicUniformBytes = (kClassicScalarSlots + kClassicCvTables * NXG_CLASSIC_MAX_CVS) * 8u;
 struct ClassicRuntimeLayout {
     uint64_t uniform_offset, state_offset, clumped_offset, render_offset, noise_offset, guide_offset, total;

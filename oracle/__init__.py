@@ -148,8 +148,9 @@ def ref():
         L.ref_classic_runtime.restype = C.c_uint32
         L.ref_classic_runtime.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                           C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_float,
-                                          C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                          C.c_void_p, C.c_uint32, C.c_void_p, C.c_char_p, C.c_uint64, C.c_void_p]
+                                          C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint64,
+                                          C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
         L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -395,8 +396,8 @@ class _ClassicPlan(C.Structure):
     """orc_classic_plan."""
     _fields_ = [("length", C.c_void_p), ("width", C.c_void_p), ("cut_amount", C.c_void_p), ("taper", C.c_void_p),
                 ("taper_start", C.c_void_p), ("width_ramp", C.c_void_p), ("has_cut", C.c_int),
-                ("description_id", C.c_uint32), ("name_seed", C.c_uint32), ("has_clump", C.c_int),
-                ("clump", C.c_void_p * 7), ("binding", C.c_void_p)]
+                ("description_id", C.c_uint32), ("name_seed", C.c_uint32), ("clump_count", C.c_uint32),
+                ("clump", (C.c_void_p * 7) * 3), ("binding", C.c_void_p * 3)]
 
 
 def seexpr_gradients():
@@ -410,14 +411,22 @@ def _u32(x, n):
     return None if x is None else np.ascontiguousarray(np.asarray(x).reshape(-1)[:n]).astype(np.uint32)
 
 
+def _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes):
+    if clumps is None and guides is not None:
+        return [dict(strand_guides=strand_guides, guides=guides, guide_uvs=guide_uvs,
+                     guide_random_prefixes=guide_random_prefixes)]
+    return list(clumps or [])
+
+
 def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitive_ids=None, random_prefixes=None,
                     columns=None, radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None,
-                    guide_random_prefixes=None):
+                    guide_random_prefixes=None, clumps=None):
     """orc_classic_runtime: the Classic runtime chain of `plan` (a
     nanoxgen_amd.classic.ClassicPlan: its compiled, bound programs) over
-    packed float4 points [N * C, 4]; a plan with a clump also takes
-    strand_guides, guides (dict(axes, normals, tangents[, reference])),
-    guide_uvs and guide_random_prefixes.  Returns dict(points, point_counts,
+    packed float4 points [N * C, 4]; each clump module of the plan takes a
+    binding in `clumps` (dicts of strand_guides, guides (dict(axes, normals,
+    tangents[, reference])), guide_uvs and guide_random_prefixes), or, for
+    one module, those four as keywords.  Returns dict(points, point_counts,
     roots, root_uvs, keep, strand_count, error_bits); strand_count is None
     where the reference throws (error_bits says why)."""
     pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
@@ -429,28 +438,31 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
     cols = None if columns is None else np.ascontiguousarray(columns, np.float32).reshape(n, -1)
     ncol = 0 if cols is None else cols.shape[1]
     op = _ClassicPlan(has_cut=1 if plan.has_cut else 0, description_id=plan.description_id,
-                      name_seed=plan.name_seed)
+                      name_seed=plan.name_seed, clump_count=plan.clump_count)
     for key, prog in plan.programs.items():
         if key.startswith("clump"):
-            op.clump[abi.CLASSIC_CLUMP_EXPRS.index(key)] = C.addressof(prog.struct)
+            name, _, module = key.partition("@")
+            op.clump[int(module or 0)][abi.CLASSIC_CLUMP_EXPRS.index(name)] = C.addressof(prog.struct)
         else:
             setattr(op, key, C.addressof(prog.struct))
     keepalive = []
-    if plan.has_clump:
-        ax = _f3(guides["axes"], -1)
+    bindings = _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes)
+    for m in range(plan.clump_count):
+        b = bindings[m]
+        gs = b["guides"]
+        ax = _f3(gs["axes"], -1)
         g = len(ax) // cvs
         render, lengths = clump_prepare(ax, cvs)
-        arrays = dict(axes3=ax, render3=render, lengths=lengths, normals3=_f3(guides["normals"], g),
-                      tangents3=_f3(guides["tangents"], g),
-                      reference3=None if guides.get("reference") is None else _f3(guides["reference"], g),
-                      guide_uvs=np.ascontiguousarray(guide_uvs, np.float32).reshape(g, 2),
-                      guide_prefixes=_u32(guide_random_prefixes, g), strand_guides=_u32(strand_guides, n),
+        arrays = dict(axes3=ax, render3=render, lengths=lengths, normals3=_f3(gs["normals"], g),
+                      tangents3=_f3(gs["tangents"], g),
+                      reference3=None if gs.get("reference") is None else _f3(gs["reference"], g),
+                      guide_uvs=np.ascontiguousarray(b["guide_uvs"], np.float32).reshape(g, 2),
+                      guide_prefixes=_u32(b["guide_random_prefixes"], g), strand_guides=_u32(b["strand_guides"], n),
                       gradients=seexpr_gradients())
         keepalive.append(arrays)
         binding = _ClumpBinding(guide_count=g, **{k: abi.ptr(v) for k, v in arrays.items()})
         keepalive.append(binding)
-        op.has_clump = 1
-        op.binding = C.addressof(binding)
+        op.binding[m] = C.addressof(binding)
     out = np.zeros_like(pts)
     oc = np.zeros(n, np.uint32)
     orr = np.zeros(n, abi.ROOT_DTYPE)
@@ -469,10 +481,12 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
 
 def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None, primitive_ids=None,
                         random_prefixes=None, columns=None, radius_scale=1.0, clump_name="clump", strand_guides=None,
-                        guides=None, guide_uvs=None, guide_random_prefixes=None):
+                        guides=None, guide_uvs=None, guide_random_prefixes=None, clumps=None, clump_names=None):
     """The reference's parse -> compile_xgen_classic_float_runtime_plan ->
     apply_xgen_classic_float_runtime_plan_cpu over the first description of
-    `xgen_text`.  Returns dict(points, point_counts, roots, root_uvs,
+    `xgen_text`, with one geometry binding per ClumpingFX module (`clumps`
+    and their module names `clump_names`, or one module's binding as
+    keywords).  Returns dict(points, point_counts, roots, root_uvs,
     strand_count, seconds), or dict(strand_count=None, error=message) where
     the reference throws."""
     pts = np.array(points, np.float32).reshape(-1, 4)
@@ -484,20 +498,29 @@ def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None,
     cols = None if columns is None else np.ascontiguousarray(columns, np.float32).reshape(-1)
     err = C.create_string_buffer(1024)
     seconds = C.c_double()
-    g = 0
-    ga = gn = gt = gr = gu = gp = sg = None
-    if guides is not None:
-        ga = _f3(guides["axes"], -1)
-        g = len(ga) // cvs
-        gn, gt = _f3(guides["normals"], g), _f3(guides["tangents"], g)
-        gr = None if guides.get("reference") is None else _f3(guides["reference"], g)
-        gu = np.ascontiguousarray(guide_uvs, np.float32).reshape(g, 2)
-        gp, sg = _u32(guide_random_prefixes, g), _u32(strand_guides, n)
+    bindings = _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes)
+    names = list(clump_names) if clump_names is not None else [clump_name] * len(bindings)
+    m = len(bindings)
+
+    def cat(f, w):
+        return np.ascontiguousarray(np.concatenate([f(b) for b in bindings]).reshape(-1, w)) if m else None
+
+    guide_counts = np.array([len(_f3(b["guides"]["axes"], -1)) // cvs for b in bindings] or [0], np.uint32)
+    ga = cat(lambda b: _f3(b["guides"]["axes"], -1), 3)
+    gn = cat(lambda b: _f3(b["guides"]["normals"], -1), 3)
+    gt = cat(lambda b: _f3(b["guides"]["tangents"], -1), 3)
+    with_reference = m and all(b["guides"].get("reference") is not None for b in bindings)
+    gr = cat(lambda b: _f3(b["guides"]["reference"], -1), 3) if with_reference else None
+    gu = cat(lambda b: np.asarray(b["guide_uvs"], np.float32).reshape(-1, 2), 2)
+    gp = None if not m else np.concatenate([np.asarray(b["guide_random_prefixes"]).reshape(-1).astype(np.uint32)
+                                            for b in bindings])
+    sg = None if not m else np.concatenate([_u32(b["strand_guides"], n) for b in bindings])
+    name_ptrs = (C.c_char_p * max(m, 1))(*[x.encode() for x in names])
     kept = ref().ref_classic_runtime(xgen_text.encode(), pts.ctypes.data, cc.ctypes.data, rr.ctypes.data,
                                      abi.ptr(uu), n, cvs, abi.ptr(ids), abi.ptr(pre), abi.ptr(cols),
-                                     0 if cols is None else cols.size, radius_scale, clump_name.encode(),
-                                     abi.ptr(ga), abi.ptr(gn), abi.ptr(gt), abi.ptr(gr), abi.ptr(gu), abi.ptr(gp), g,
-                                     abi.ptr(sg), err, 1024, C.byref(seconds))
+                                     0 if cols is None else cols.size, radius_scale, m, name_ptrs,
+                                     guide_counts.ctypes.data, abi.ptr(ga), abi.ptr(gn), abi.ptr(gt), abi.ptr(gr),
+                                     abi.ptr(gu), abi.ptr(gp), abi.ptr(sg), err, 1024, C.byref(seconds))
     if kept == 0xFFFFFFFF:
         return dict(strand_count=None, error=err.value.decode())
     return dict(points=pts[:kept * cvs], point_counts=cc[:kept], roots=rr[:kept],

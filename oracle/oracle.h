@@ -147,9 +147,10 @@ typedef struct orc_classic_plan {
     int has_cut;              /* one CutFX whose amount is cut_amount */
     uint32_t description_id;  /* face seed: runtime hash of (id, name seed, face) */
     uint32_t name_seed;
-    int has_clump;            /* one ClumpingFX before the CutFX */
-    const nxg_expr_program *clump[7]; /* mask, clump, clumpScale, noise, noiseScale, noiseFrequency, noiseCorrelation */
-    const orc_clump_binding *binding;
+    uint32_t clump_count;     /* ClumpingFX modules (at most 3), in order, before the CutFX */
+    /* per module: mask, clump, clumpScale, noise, noiseScale, noiseFrequency, noiseCorrelation */
+    const nxg_expr_program *clump[3][7];
+    const orc_clump_binding *binding[3];
 } orc_classic_plan;
 void orc_seexpr_gradients(float *out); /* 256 x 3 */
 void orc_clump_noise_axis(const float *axis3, const float *distance3, uint32_t n, float raw_frequency,

@@ -259,69 +259,65 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
     if (!isfinite(raw_width)) return ORC_CLASSIC_NONFINITE_WIDTH;
     if (c_length < 1.0e-4f || raw_width < 1.0e-4f) return 0u;
     const float c_width = raw_width;
-    /* the context's t after the effects: the clump leaves it at its last CV's
-     * (1), which the cut amount, taper and taperStart then read */
+    /* the context's t after the effects: a module with a guide sets it to 0
+     * for its mask, and one that clumps leaves it at its last CV's (1); the
+     * cut amount, taper and taperStart then read it */
     float t_after = 0.0f;
-    if (plan->has_clump) { /* :1680-1793 */
-        const orc_clump_binding *b = plan->binding;
+    for (uint32_t m = 0; m < plan->clump_count; ++m) { /* :1680-1793, modules in object order */
+        const orc_clump_binding *b = plan->binding[m];
+        const nxg_expr_program *const *e = plan->clump[m];
         const uint32_t g = b->strand_guides[s];
-        if (g != NXG_INVALID_INDEX) {
-            if (g >= b->guide_count) return ORC_CLASSIC_INVALID_GUIDE;
-            float raw_mask;
-            if (classic_eval(plan->clump[0], &ctx, vars, c_length, c_width, 0.0f, &raw_mask))
+        if (g == NXG_INVALID_INDEX) continue;
+        if (g >= b->guide_count) return ORC_CLASSIC_INVALID_GUIDE;
+        t_after = 0.0f;
+        float raw_mask;
+        if (classic_eval(e[0], &ctx, vars, c_length, c_width, 0.0f, &raw_mask)) return ORC_CLASSIC_EXPRESSION;
+        const float mask = raw_mask < 0.0f ? 0.0f : (1.0f < raw_mask ? 1.0f : raw_mask);
+        if (!(mask > 0.0f)) continue;
+        /* build_clump_noise_axis in the guide's context (:1041-1186) */
+        orc_expr_ctx gctx = ctx;
+        gctx.u = b->guide_uvs[g].x;
+        gctx.v = b->guide_uvs[g].y;
+        gctx.face_seed = 0.0f; /* read only by rand() without a prefix */
+        gctx.random_prefix = b->guide_prefixes[g];
+        gctx.has_random_prefix = 1;
+        float raw_noise, magnitudes[256], noise3[3 * 256];
+        if (classic_eval(e[3], &gctx, vars, c_length, c_width, 0.0f, &raw_noise)) return ORC_CLASSIC_EXPRESSION;
+        const float noise = max_std(raw_noise, 0.0f);
+        const int noisy = noise > 1.0e-5f && mask > 1.0e-4f;
+        float frequency = 0.0f, correlation = 0.0f;
+        if (noisy) {
+            if (classic_eval(e[5], &gctx, vars, c_length, c_width, 0.0f, &frequency) ||
+                classic_eval(e[6], &gctx, vars, c_length, c_width, 0.0f, &correlation))
                 return ORC_CLASSIC_EXPRESSION;
-            const float mask = raw_mask < 0.0f ? 0.0f : (1.0f < raw_mask ? 1.0f : raw_mask);
-            if (mask > 0.0f) {
-                /* build_clump_noise_axis in the guide's context (:1041-1186) */
-                orc_expr_ctx gctx = ctx;
-                gctx.u = b->guide_uvs[g].x;
-                gctx.v = b->guide_uvs[g].y;
-                gctx.face_seed = 0.0f; /* read only by rand() without a prefix */
-                gctx.random_prefix = b->guide_prefixes[g];
-                gctx.has_random_prefix = 1;
-                float raw_noise, magnitudes[256], noise3[3 * 256];
-                if (classic_eval(plan->clump[3], &gctx, vars, c_length, c_width, 0.0f, &raw_noise))
-                    return ORC_CLASSIC_EXPRESSION;
-                const float noise = max_std(raw_noise, 0.0f);
-                const int noisy = noise > 1.0e-5f && mask > 1.0e-4f;
-                float frequency = 0.0f, correlation = 0.0f;
-                if (noisy) {
-                    if (classic_eval(plan->clump[5], &gctx, vars, c_length, c_width, 0.0f, &frequency) ||
-                        classic_eval(plan->clump[6], &gctx, vars, c_length, c_width, 0.0f, &correlation))
-                        return ORC_CLASSIC_EXPRESSION;
-                    for (uint32_t cv = 1u; cv < cvs; ++cv) {
-                        float raw_scale;
-                        const float t = (float)cv / (float)(cvs - 1u);
-                        if (classic_eval(plan->clump[4], &gctx, vars, c_length, c_width, t, &raw_scale))
-                            return ORC_CLASSIC_EXPRESSION;
-                        magnitudes[cv] = mask * noise * max_std(raw_scale, 0.0f);
-                        if (!isfinite(magnitudes[cv])) return ORC_CLASSIC_NONFINITE_CLUMP;
-                    }
-                }
-                const float *axes = b->axes3 + (uint64_t)g * cvs * 3u;
-                const float *render = b->render3 + (uint64_t)g * cvs * 3u;
-                orc_clump_noise_axis(render, axes, cvs, frequency, correlation,
-                                     b->reference3 ? b->reference3 + 3u * g : axes, b->normals3 + 3u * g,
-                                     b->tangents3 + 3u * g, noisy ? magnitudes : NULL, b->gradients, noise3);
-                orc_cut_to_length(&p[0].x, cvs, b->lengths[g]);
-                c_length = orc_curve_spline_length_v(p, cvs);
-                float raw_amount;
-                if (classic_eval(plan->clump[1], &ctx, vars, c_length, c_width, 0.0f, &raw_amount))
-                    return ORC_CLASSIC_EXPRESSION;
-                const float amount = raw_amount < 0.0f ? 0.0f : (1.0f < raw_amount ? 1.0f : raw_amount);
-                for (uint32_t cv = 1u; cv < cvs; ++cv) {
-                    float scale;
-                    t_after = (float)cv / (float)(cvs - 1u);
-                    if (classic_eval(plan->clump[2], &ctx, vars, c_length, c_width, t_after, &scale))
-                        return ORC_CLASSIC_EXPRESSION;
-                    const cv3 goal = sub3(c3(render[3 * cv], render[3 * cv + 1], render[3 * cv + 2]), c3(0.0f, 0.0f, 0.0f));
-                    p[cv] = add3(p[cv], mul3(sub3(goal, p[cv]), mask * amount * (1.0f - 2.0f * scale)));
-                }
-                for (uint32_t cv = 1u; cv < cvs; ++cv)
-                    p[cv] = add3(p[cv], c3(noise3[3 * cv], noise3[3 * cv + 1], noise3[3 * cv + 2]));
-                c_length = orc_curve_spline_length_v(p, cvs);
+            for (uint32_t cv = 1u; cv < cvs; ++cv) {
+                float raw_scale;
+                const float t = (float)cv / (float)(cvs - 1u);
+                if (classic_eval(e[4], &gctx, vars, c_length, c_width, t, &raw_scale)) return ORC_CLASSIC_EXPRESSION;
+                magnitudes[cv] = mask * noise * max_std(raw_scale, 0.0f);
+                if (!isfinite(magnitudes[cv])) return ORC_CLASSIC_NONFINITE_CLUMP;
             }
         }
+        const float *axes = b->axes3 + (uint64_t)g * cvs * 3u;
+        const float *render = b->render3 + (uint64_t)g * cvs * 3u;
+        orc_clump_noise_axis(render, axes, cvs, frequency, correlation,
+                             b->reference3 ? b->reference3 + 3u * g : axes, b->normals3 + 3u * g,
+                             b->tangents3 + 3u * g, noisy ? magnitudes : NULL, b->gradients, noise3);
+        orc_cut_to_length(&p[0].x, cvs, b->lengths[g]);
+        c_length = orc_curve_spline_length_v(p, cvs);
+        float raw_amount;
+        if (classic_eval(e[1], &ctx, vars, c_length, c_width, 0.0f, &raw_amount)) return ORC_CLASSIC_EXPRESSION;
+        const float amount = raw_amount < 0.0f ? 0.0f : (1.0f < raw_amount ? 1.0f : raw_amount);
+        for (uint32_t cv = 1u; cv < cvs; ++cv) {
+            float scale;
+            t_after = (float)cv / (float)(cvs - 1u);
+            if (classic_eval(e[2], &ctx, vars, c_length, c_width, t_after, &scale)) return ORC_CLASSIC_EXPRESSION;
+            const cv3 goal = sub3(c3(render[3 * cv], render[3 * cv + 1], render[3 * cv + 2]), c3(0.0f, 0.0f, 0.0f));
+            p[cv] = add3(p[cv], mul3(sub3(goal, p[cv]), mask * amount * (1.0f - 2.0f * scale)));
+        }
+        for (uint32_t cv = 1u; cv < cvs; ++cv)
+            p[cv] = add3(p[cv], c3(noise3[3 * cv], noise3[3 * cv + 1], noise3[3 * cv + 2]));
+        c_length = orc_curve_spline_length_v(p, cvs);
     }
     uint8_t k = 1u;
     if (plan->has_cut) {

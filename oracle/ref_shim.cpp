@@ -511,10 +511,11 @@ int ref_expr_compile(const char *source, uint64_t size, const char *name, const 
 uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *counts, nxg_root_sample *roots,
                              nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, const uint32_t *primitive_ids,
                              const uint32_t *prefixes, const float *inputs, uint64_t input_count,
-                             float radius_scale, const char *clump_name, const float *guide_axes3,
-                             const float *guide_normals3, const float *guide_tangents3, const float *guide_reference3,
-                             const float *guide_uvs2, const uint32_t *guide_prefixes, uint32_t guide_count,
-                             const uint32_t *strand_guides, char *err, uint64_t errcap, double *seconds) {
+                             float radius_scale, uint32_t clump_count, const char *const *clump_names,
+                             const uint32_t *guide_counts, const float *guide_axes3, const float *guide_normals3,
+                             const float *guide_tangents3, const float *guide_reference3, const float *guide_uvs2,
+                             const uint32_t *guide_prefixes, const uint32_t *strand_guides, char *err,
+                             uint64_t errcap, double *seconds) {
     try {
         std::istringstream text(xgen_text);
         const ClassicCollection collection = parse_xgen_classic_collection(text);
@@ -545,28 +546,34 @@ uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *cou
             prefixes ? std::span<const std::uint32_t>(prefixes, strand_count) : std::span<const std::uint32_t>();
         const std::span<const float> rows =
             inputs ? std::span<const float>(inputs, input_count) : std::span<const float>();
+        // one binding per module, the guide arrays of the modules back to back
         std::vector<ClassicClumpRuntimeData> clumps;
-        if (guide_count) {
+        std::size_t first = 0u;
+        for (uint32_t m = 0; m < clump_count; ++m) {
+            const uint32_t guide_count = guide_counts[m];
             ClassicClumpRuntimeData d{};
-            d.module_name = clump_name;
+            d.module_name = clump_names[m];
             d.cvs_per_guide = cvs;
             const std::size_t axis_count = std::size_t{guide_count} * cvs;
             d.guide_axes.resize(axis_count);
-            std::memcpy(d.guide_axes.data(), guide_axes3, axis_count * sizeof(Vec3));
+            std::memcpy(d.guide_axes.data(), guide_axes3 + 3u * first * cvs, axis_count * sizeof(Vec3));
             d.guide_normals.resize(guide_count);
-            std::memcpy(d.guide_normals.data(), guide_normals3, guide_count * sizeof(Vec3));
+            std::memcpy(d.guide_normals.data(), guide_normals3 + 3u * first, guide_count * sizeof(Vec3));
             d.guide_tangents.resize(guide_count);
-            std::memcpy(d.guide_tangents.data(), guide_tangents3, guide_count * sizeof(Vec3));
+            std::memcpy(d.guide_tangents.data(), guide_tangents3 + 3u * first, guide_count * sizeof(Vec3));
             if (guide_reference3) {
                 d.guide_reference_positions.resize(guide_count);
-                std::memcpy(d.guide_reference_positions.data(), guide_reference3, guide_count * sizeof(Vec3));
+                std::memcpy(d.guide_reference_positions.data(), guide_reference3 + 3u * first,
+                            guide_count * sizeof(Vec3));
             }
             d.guide_uvs.resize(guide_count);
-            std::memcpy(d.guide_uvs.data(), guide_uvs2, guide_count * sizeof(Vec2));
+            std::memcpy(d.guide_uvs.data(), guide_uvs2 + 2u * first, guide_count * sizeof(Vec2));
             d.guide_face_ids.assign(guide_count, 0u);
-            d.guide_random_prefixes.assign(guide_prefixes, guide_prefixes + guide_count);
-            d.strand_guide_indices.assign(strand_guides, strand_guides + strand_count);
+            d.guide_random_prefixes.assign(guide_prefixes + first, guide_prefixes + first + guide_count);
+            d.strand_guide_indices.assign(strand_guides + std::size_t{m} * strand_count,
+                                          strand_guides + std::size_t{m + 1u} * strand_count);
             clumps.push_back(std::move(d));
+            first += guide_count;
         }
         const auto start = std::chrono::steady_clock::now();
         apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale, {}, pre, ids, clumps, rows);

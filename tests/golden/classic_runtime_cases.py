@@ -2,7 +2,8 @@
 fixture writer (make_golden.py classic_runtime) and the tests.
 
 Each case is a Classic description: SplinePrimitive attributes (length,
-width, taper, taperStart, widthRamp) and an optional CutFX amount, written
+width, taper, taperStart, widthRamp), ClumpingFX modules (`clump`: one,
+`clumps`: several, applied in order) and an optional CutFX amount, written
 in the authoring form the reference's plan compiler reads (map('path') for
 PTEX inputs, noise($Prefg*f) for reference-pose noise).  `xgen_text` renders
 the description for the reference's parser; `plan_sources` rewrites the
@@ -19,11 +20,11 @@ import numpy as np
 
 # The order the reference compiles a plan's expressions in
 # (src/xgen_classic_runtime.cpp:1211-1229, then the FX modules in object
-# order: the ClumpingFX module before the CutFX).
+# order: the ClumpingFX modules before the CutFX).
 PRIMITIVE = (("length", "length"), ("width", "width"), ("taper", "taper"), ("taper_start", "taperStart"),
              ("width_ramp", "widthRamp"))
 CLUMP = ("mask", "clump", "clumpScale", "noise", "noiseScale", "noiseFrequency", "noiseCorrelation")
-COMPILE_ORDER = PRIMITIVE + tuple((f"clump.{a}", a) for a in CLUMP) + (("cut_amount", "amount"),)
+
 
 CASES = [
     # the reference's own test_float_runtime_plan (tests/classic_test.cpp:198-246)
@@ -77,7 +78,46 @@ CASES = [
     dict(name="clumpMagnitude", description_id=24, strands="walk", guides=4,
          clump=dict(mask="0.5", clump="0.5", clumpScale="0", noise="1e30", noiseScale="1e30", noiseFrequency="1",
                     noiseCorrelation="0"), exprs=dict()),
+    # hierarchical clumping: a coarse module, then a fine one with PTEX mask,
+    # guide-prefix noise and a cLength-reading amount, then the cut; the cut
+    # amount and taper read the t the last module with a guide leaves
+    dict(name="ns:hierarchy", description_id=31, strands="walk", ids=True, prefixes=True, guides=[6, 14],
+         exprs=dict(taper="rampUI(0,0.1,1:1,0.8,1)", cut_amount="$cLength * 0.15 + rampUI(0,0,1:1,0.05,1)"),
+         clumps=[dict(mask="1", clump="0.5 + 0.3 * rand()", clumpScale="rampUI(0,0,1:1,0.25,1)", noise="0",
+                      noiseScale="1", noiseFrequency="1", noiseCorrelation="0"),
+                 dict(mask="0.2 + 0.8 * map('fine.ptx')", clump="clamp($cLength * 0.4, 0, 1)",
+                      clumpScale="0.1", noise="0.05 + 0.1 * rand(3)", noiseScale="1 - 0.5 * $cWidth",
+                      noiseFrequency="3 + rand()", noiseCorrelation="50")]),
+    # three levels on the generator's strands; the middle one masks a third of
+    # the strands out (their context's t goes back to 0)
+    dict(name="threeLevels", description_id=32, strands="ico5", ids=True, guides=[5, 9, 13],
+         exprs=dict(width_ramp="1 - 0.5 * $cLength", cut_amount="rampUI(0,0,1:1,0.02,1)"),
+         clumps=[dict(mask="0.9", clump="0.4", clumpScale="0", noise="0", noiseScale="1", noiseFrequency="1",
+                      noiseCorrelation="0"),
+                 dict(mask="$id < 100 ? 0 : 1", clump="0.3 + 0.1 * $cLength", clumpScale="rampUI(0,0.1,1:1,0,1)",
+                      noise="0", noiseScale="1", noiseFrequency="2", noiseCorrelation="10"),
+                 dict(mask="0.5", clump="0.6", clumpScale="0.05 * $cLength", noise="0", noiseScale="1",
+                      noiseFrequency="1", noiseCorrelation="0")]),
 ]
+
+
+def case_clumps(case):
+    """The case's ClumpingFX modules (attribute dicts), in object order."""
+    if "clumps" in case:
+        return case["clumps"]
+    return [case["clump"]] if "clump" in case else []
+
+
+def clump_names(case):
+    """The modules' names as xgen_text writes them."""
+    return ["clump" if m == 0 else f"clump{m}" for m in range(len(case_clumps(case)))]
+
+
+def compile_order(case):
+    """(key, attribute) in the reference's compile order; clump keys are
+    "clump<m>.<attribute>"."""
+    return (PRIMITIVE + tuple((f"clump{m}.{a}", a) for m in range(len(case_clumps(case))) for a in CLUMP)
+            + (("cut_amount", "amount"),))
 
 
 def xgen_text(case):
@@ -90,9 +130,9 @@ def xgen_text(case):
         if key in e:
             s += f"\t{attribute}\t{e[key]}\n"
     s += "\tendAttrs\n"
-    if "clump" in case:
-        s += "ClumpingFXModule\n\tactive\ttrue\n\tname\tclump\n"
-        s += "".join(f"\t{a}\t{case['clump'][a]}\n" for a in CLUMP) + "\tendAttrs\n"
+    for module, name in zip(case_clumps(case), clump_names(case)):
+        s += f"ClumpingFXModule\n\tactive\ttrue\n\tname\t{name}\n"
+        s += "".join(f"\t{a}\t{module[a]}\n" for a in CLUMP) + "\tendAttrs\n"
     if "cut_amount" in e:
         s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{e['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
     return s + "\tActive\tSplinePrimitive\n"
@@ -104,13 +144,14 @@ _PREF = re.compile(r"\bnoise\s*\(\s*\$Prefg\s*\*\s*([-+]?[0-9.]+)\s*\)")
 
 def plan_sources(case):
     """(sources in the bound form, ptex count, pref-noise frequencies); clump
-    expressions under the "clump" key by attribute name."""
+    expressions under the "clump" key by attribute name (a list of such
+    dicts for several modules)."""
     out, ptex, freqs = {}, 0, []
-    for key, _ in COMPILE_ORDER:
-        if key.startswith("clump."):
-            if "clump" not in case:
-                continue
-            src = case["clump"][key[6:]]
+    modules = case_clumps(case)
+    for key, _ in compile_order(case):
+        if key.startswith("clump"):
+            m, attribute = key[5:].split(".")
+            src = modules[int(m)][attribute]
         elif key not in case["exprs"]:
             continue
         else:
@@ -131,9 +172,9 @@ def plan_sources(case):
             return f"$__nxg_pref_noise_{freqs.index(f)}"
 
         out[key] = _PREF.sub(pref_sub, out[key])
-    clump = {k[6:]: out.pop(k) for k in list(out) if k.startswith("clump.")}
-    if clump:
-        out["clump"] = clump
+    clumps = [{a: out.pop(f"clump{m}.{a}") for a in CLUMP} for m in range(len(modules))]
+    if clumps:
+        out["clump"] = clumps if "clumps" in case else clumps[0]
     return out, ptex, freqs
 
 
@@ -174,23 +215,29 @@ def case_inputs(case, strands_of):
     prefixes = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32) if case.get("prefixes") else None
     out = dict(points=pts, cvs=cvs, roots=roots, uvs=uvs, ids=ids, prefixes=prefixes, columns=columns,
                radius_scale=case.get("radius_scale", 1.0))
-    if "guides" in case:
-        g = clump_guides(case["guides"], cvs, rng)
-        strand_guides = rng.integers(0, case["guides"], n).astype(np.uint32)
-        strand_guides[::7] = 0xFFFFFFFF
+    counts = case.get("guides")
+    for m, count in enumerate([] if counts is None else counts if isinstance(counts, list) else [counts]):
+        g = clump_guides(count, cvs, rng)
+        strand_guides = rng.integers(0, count, n).astype(np.uint32)
+        strand_guides[m::7 - m] = 0xFFFFFFFF
         if case.get("bad_guide"):
-            strand_guides[n // 2] = case["guides"]
-        out.update(guide_axes=g["axes"], guide_normals=g["normals"], guide_tangents=g["tangents"],
-                   guide_reference=g["reference"], guide_uvs=g["uvs"], guide_prefixes=g["prefixes"],
-                   strand_guides=strand_guides)
+            strand_guides[n // 2] = count
+        suffix = "" if m == 0 else f"_{m}"
+        out.update({f"{k}{suffix}": v for k, v in dict(
+            guide_axes=g["axes"], guide_normals=g["normals"], guide_tangents=g["tangents"],
+            guide_reference=g["reference"], guide_uvs=g["uvs"], guide_prefixes=g["prefixes"],
+            strand_guides=strand_guides).items()})
     return out
 
 
 def guide_args(inp):
-    """The clump keyword arguments of case_inputs' output (empty without guides)."""
-    if "guide_axes" not in inp:
-        return {}
-    return dict(strand_guides=inp["strand_guides"], guide_uvs=inp["guide_uvs"],
-                guide_random_prefixes=inp["guide_prefixes"],
-                guides=dict(axes=inp["guide_axes"], normals=inp["guide_normals"], tangents=inp["guide_tangents"],
-                            reference=inp["guide_reference"]))
+    """The `clumps` keyword argument of case_inputs' output: one binding per
+    module (empty without guides)."""
+    clumps = []
+    while ("guide_axes" + ("" if not clumps else f"_{len(clumps)}")) in inp:
+        x = lambda k: inp[k + ("" if not clumps else f"_{len(clumps)}")]
+        clumps.append(dict(strand_guides=x("strand_guides"), guide_uvs=x("guide_uvs"),
+                           guide_random_prefixes=x("guide_prefixes"),
+                           guides=dict(axes=x("guide_axes"), normals=x("guide_normals"),
+                                       tangents=x("guide_tangents"), reference=x("guide_reference"))))
+    return dict(clumps=clumps) if clumps else {}

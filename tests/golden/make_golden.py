@@ -426,7 +426,7 @@ def main_classic_runtime():
     """The Classic runtime with primitive expressions: every case of
     classic_runtime_cases.CASES through the reference's parser, plan compiler
     and runtime; inputs and outputs (or the reference's message)."""
-    from classic_runtime_cases import CASES, case_inputs, guide_args, xgen_text
+    from classic_runtime_cases import CASES, case_inputs, clump_names, guide_args, xgen_text
     oracle.build()
     if oracle.ref() is None:
         raise SystemExit("reference core not built (needs /root/reference)")
@@ -439,7 +439,7 @@ def main_classic_runtime():
         out = oracle.ref_classic_runtime(xgen_text(case), inp["points"], inp["cvs"], roots=inp["roots"],
                                          uvs=inp["uvs"], primitive_ids=inp["ids"], random_prefixes=inp["prefixes"],
                                          columns=inp["columns"], radius_scale=inp["radius_scale"],
-                                         **guide_args(inp))
+                                         clump_names=clump_names(case), **guide_args(inp))
         if out["strand_count"] is None:
             fx[f"case{i}_error"] = np.array(out["error"])
             print(case["name"], "->", out["error"])

@@ -15,7 +15,7 @@ from nanoxgen_amd import abi, classic, workloads
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
-from classic_runtime_cases import CASES, clump_guides, guide_args, plan_sources  # noqa: E402
+from classic_runtime_cases import CASES, case_clumps, clump_guides, guide_args, plan_sources  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -64,16 +64,20 @@ def make_plan(case):
 
 
 def run_gpu(torch, plan, pts, cvs, *, roots, uvs=None, counts=None, ids=None, prefixes=None, columns=None,
-            radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None, guide_random_prefixes=None):
-    gd = None if guides is None else {k: dev(torch, v) for k, v in guides.items() if v is not None}
+            radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None, guide_random_prefixes=None,
+            clumps=None):
+    if clumps is None and guides is not None:
+        clumps = [dict(strand_guides=strand_guides, guides=guides, guide_uvs=guide_uvs,
+                       guide_random_prefixes=guide_random_prefixes)]
+    bindings = [dict(strand_guides=dev(torch, b["strand_guides"], np.int32), guide_uvs=dev(torch, b["guide_uvs"]),
+                     guide_random_prefixes=dev(torch, b["guide_random_prefixes"], np.int32),
+                     guides={k: dev(torch, v) for k, v in b["guides"].items() if v is not None})
+                for b in clumps or []]
     out = classic.classic_runtime(dev(torch, np.asarray(pts, np.float32).reshape(-1, 4)), cvs, plan,
                                   roots=dev(torch, np.ascontiguousarray(roots).view(np.int32).reshape(-1, 12)),
                                   root_uvs=dev(torch, uvs), point_counts=dev(torch, counts, np.int32),
                                   primitive_ids=dev(torch, ids, np.int32), random_prefixes=dev(torch, prefixes, np.int32),
-                                  columns=dev(torch, columns), radius_scale=radius_scale,
-                                  strand_guides=dev(torch, strand_guides, np.int32), guides=gd,
-                                  guide_uvs=dev(torch, guide_uvs),
-                                  guide_random_prefixes=dev(torch, guide_random_prefixes, np.int32))
+                                  columns=dev(torch, columns), radius_scale=radius_scale, clumps=bindings)
     return out.to_host()
 
 
@@ -110,7 +114,7 @@ def test_reference_fixture(torch, fixture, i):
         return
     want = dict(strand_count=len(fixture[f"case{i}_out_point_counts"]), points=fixture[f"case{i}_out_points"],
                 point_counts=fixture[f"case{i}_out_point_counts"], roots=fixture[f"case{i}_out_roots"])
-    same(got, want, CASES[i]["name"], noisy_clump=CASES[i].get("clump", {}).get("noise", "0") != "0")
+    same(got, want, CASES[i]["name"], noisy_clump=any(c["noise"] != "0" for c in case_clumps(CASES[i])))
 
 
 PLANS = [
@@ -187,6 +191,44 @@ def test_clump_walks_oracle(torch, orc, cvs):
         assert want["strand_count"] is not None, want["error_bits"]
         got = run_gpu(torch, plan, pts, cvs, roots=roots, ids=ids, prefixes=prefixes, columns=columns, **kw)
         same(got, want, f"{cvs} CVs, clump plan {k}", noisy_clump=True)
+
+
+@pytest.mark.parametrize("cvs", [3, 16, 64])
+def test_hierarchy_walks_oracle(torch, orc, cvs):
+    """Hierarchical clumping: two and three ClumpingFX modules, each with its
+    own guides (different counts, guideless strands, the second module's
+    mask zero for some strands), then a CutFX reading t; GPU == oracle."""
+    rng = np.random.default_rng(300 + cvs)
+    n = 64 * 2 + 37
+    steps = rng.normal(0, 0.3, size=(n, cvs, 3)).astype(np.float32)
+    pts = np.concatenate([np.cumsum(steps, 1), rng.uniform(0.001, 0.02, (n, cvs, 1)).astype(np.float32)], 2)
+    pts = pts.reshape(-1, 4).astype(np.float32)
+    roots = np.zeros(n, abi.ROOT_DTYPE)
+    roots["uv"] = rng.uniform(0, 1, (n, 2))
+    columns = rng.uniform(0.0, 1.0, (n, 2)).astype(np.float32)
+    ids = rng.permutation(n).astype(np.uint32)
+    prefixes = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    clumps = []
+    for m, count in enumerate((7, 19, 4)):
+        g = clump_guides(count, cvs, rng)
+        sg = rng.integers(0, count, n).astype(np.uint32)
+        sg[m::4 + m] = 0xFFFFFFFF
+        clumps.append(dict(strand_guides=sg, guide_uvs=g["uvs"], guide_random_prefixes=g["prefixes"],
+                           guides=dict(axes=g["axes"], normals=g["normals"], tangents=g["tangents"],
+                                       reference=g["reference"] if m != 1 else None)))
+    levels = [CLUMP_PLANS[0]["clump"], CLUMP_PLANS[1]["clump"],
+              dict(mask="0.7", clump="0.5 * $__nxg_map_1", clumpScale="rampUI(0,0,1:1,0.2,1)", noise="0.1",
+                   noiseScale="1", noiseFrequency="2", noiseCorrelation="20")]
+    for count in (2, 3):
+        plan = classic.ClassicPlan(clump=levels[:count], cut_amount="$cLength * 0.2 + rampUI(0,0,1:1,0.1,1)",
+                                   taper="rampUI(0,0.3,1:1,0.6,1)", description_id=6, description_name="levels",
+                                   ptex_count=2)
+        want = orc.classic_runtime(pts, cvs, plan, roots=roots, primitive_ids=ids, random_prefixes=prefixes,
+                                   columns=columns, clumps=clumps[:count])
+        assert want["strand_count"] is not None, want["error_bits"]
+        got = run_gpu(torch, plan, pts, cvs, roots=roots, ids=ids, prefixes=prefixes, columns=columns,
+                      clumps=clumps[:count])
+        same(got, want, f"{cvs} CVs, {count} clump levels", noisy_clump=True)
 
 
 def test_generated_plane(torch, orc):
