@@ -147,6 +147,34 @@ def bench_surrogate(args):
                                                               args.sample_stride, flags, stats, stream=stream)))
     mlp_ms = attr_ms(lambda: surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream))
     counts_exact = bool((out.point_counts == Cv).all())
+    # Where the surrogate's error comes from, on the first 65,536 roots: the
+    # trained weights in float64 arithmetic (no bf16 rounding) against the
+    # exact attributes, and the bf16 device network against that float64
+    # network (the rounding floor of bf16 weights and activations).
+    k = min(N, 65536)
+    feats = surrogate.root_features(asset, w.params, maps)[:k].contiguous()
+    flat = torch.from_numpy(net.params()).double().cuda()
+    norm = torch.from_numpy(net.norm).double().cuda()
+    h = torch.zeros((k, surrogate.IN_PAD), dtype=torch.float64, device="cuda")
+    h[:, :surrogate.INPUTS] = (feats.double() - norm[:10]) * norm[10:20]
+    at = 0
+    for i, (r, c) in enumerate(surrogate.layer_shapes()):
+        W = flat[at:at + r * c].view(r, c)
+        b = flat[at + r * c:at + r * c + c]
+        at += r * c + c
+        z = h @ W + b
+        h = z * torch.sigmoid(z) if i < surrogate.LAYERS else z
+    y64 = h[:, :surrogate.OUTPUTS] * norm[20:24] + norm[24:28]
+    yb16 = net.forward(feats).double()
+    ex = exact_attrs[:k].double()
+    rel64 = ((y64 - ex).abs() / ex.abs().clamp_min(1e-6)).max(1).values
+    relb16 = ((yb16 - y64).abs() / y64.abs().clamp_min(1e-6)).max(1).values
+    q = torch.tensor([0.5, 0.99], dtype=torch.float64, device="cuda")
+    error_sources = {"roots": k,
+                     "float64_network_vs_exact_rel_p50_p99": [float(v) for v in torch.quantile(rel64, q)],
+                     "bf16_network_vs_float64_network_rel_p50_p99": [float(v) for v in torch.quantile(relb16, q)],
+                     "float64_network_roots_within_threshold": float((rel64 <= args.threshold).double().mean()),
+                     "bf16_rounding_roots_within_threshold": float((relb16 <= args.threshold).double().mean())}
     points = N * Cv
     line = {"metric": METRIC, "value": points / (sur_ms * 1e-3), "unit": "points/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sur_ms, 4),
@@ -167,7 +195,8 @@ def bench_surrogate(args):
                           "roots_over_threshold": int((rel > args.threshold).sum().item()),
                           "point_counts_exact": counts_exact,
                           "train_pairs": 1 << 20, "train_seconds": round(train_s, 2),
-                          "train_loss_first_last": [hist[0][1], hist[-1][1]]},
+                          "train_loss_first_last": [hist[0][1], hist[-1][1]],
+                          "error_sources": error_sources},
             "roofline": {"bound": "mfma", "kernel": "surrogate_kernel (root features + MLP)",
                          "achieved": round(26368.0 * N / (mlp_ms * 1e-3) / 1e12, 3), "peak": 2500.0,
                          "unit": "TFLOP/s", "frac": round(26368.0 * N / (mlp_ms * 1e-3) / 1e12 / 2500.0, 6),
