@@ -920,8 +920,10 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
 #define NXG_CLASSIC_NONFINITE_CLUMP 256u    /* "Classic ClumpingFX noise magnitude is non-finite" */
 
 /* Hierarchical clumping: up to this many ClumpingFX modules in one plan, applied
- * in object order before the CutFX (the reference's plan.effects, each with its
- * own ClassicClumpRuntimeData, src/xgen_classic_runtime.cpp:1677-1793). */
+ * in object order, each with its own ClassicClumpRuntimeData; the CutFX may sit
+ * between them (the reference's plan.effects, src/xgen_classic_runtime.cpp:
+ * 1677-1823).  A CutFX followed by a clump rebuilds the strand at once; a
+ * strand it culls still runs the later modules (their errors count). */
 #define NXG_CLASSIC_MAX_CLUMPS 3u
 #define NXG_CLASSIC_CLUMP_EXPRS 7u
 
@@ -955,10 +957,11 @@ enum {
 typedef struct nxg_classic_plan {
     const nxg_expr_program *expressions[NXG_CLASSIC_EXPR_COUNT];
     uint32_t has_cut;               /* the CutFX module; needs expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT] */
-    uint32_t clump_count;           /* ClumpingFX modules, applied in order before the CutFX
-                                       (0..NXG_CLASSIC_MAX_CLUMPS) */
+    uint32_t clump_count;           /* ClumpingFX modules, applied in order (0..NXG_CLASSIC_MAX_CLUMPS) */
     uint32_t description_id;        /* descriptionId */
     uint32_t description_name_seed; /* nxg_string_seed of the description name after its last ':' */
+    uint32_t clumps_after_cut;      /* the last this many ClumpingFX modules follow the CutFX in object
+                                       order (0: the CutFX comes last) */
 } nxg_classic_plan;
 
 /* What nxg_classic_plan_pack reports about a plan. */
@@ -976,7 +979,7 @@ typedef struct nxg_classic_plan_info {
     uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
     uint32_t uniform;             /* bit k: expressions[k] reads no input and calls no rand(), so its
                                      value depends on t alone (evaluated once per call) */
-    uint32_t reserved;
+    uint32_t clumps_after_cut;
 } nxg_classic_plan_info;
 
 /* Validates and packs a plan into a position-independent image (out NULL:

@@ -296,7 +296,8 @@ CLASSIC_EXPRS = (("length", "width", "cut_amount", "taper", "taper_start", "widt
 class ClassicPlan(C.Structure):
     """nxg_classic_plan: the plan's programs (host nxg_expr_program pointers, NULL: absent)."""
     _fields_ = [("expressions", C.c_void_p * len(CLASSIC_EXPRS)), ("has_cut", C.c_uint32),
-                ("clump_count", C.c_uint32), ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32)]
+                ("clump_count", C.c_uint32), ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32),
+                ("clumps_after_cut", C.c_uint32)]
 
 
 class ClassicPlanInfo(C.Structure):
@@ -305,7 +306,7 @@ class ClassicPlanInfo(C.Structure):
                 ("clump_count", C.c_uint32), ("columns_required", C.c_uint32), ("needs_random_prefix", C.c_uint32),
                 ("final_length", C.c_uint32), ("clump_length", C.c_uint32), ("description_id", C.c_uint32),
                 ("description_name_seed", C.c_uint32), ("program_offset", C.c_uint32 * len(CLASSIC_EXPRS)),
-                ("uniform", C.c_uint32), ("reserved", C.c_uint32)]
+                ("uniform", C.c_uint32), ("clumps_after_cut", C.c_uint32)]
 
 
 class ClassicClumpBinding(C.Structure):

@@ -99,6 +99,7 @@ def classic_cut(points, cvs, amount_scale, amount_offset, *, point_counts=None, 
     for name, x, k in (("point_counts", point_counts, 1), ("roots", roots, 12), ("root_uvs", root_uvs, 2)):
         if x is not None and (x.numel() < n * k or not x.is_contiguous()):
             raise ValueError(f"{name} must be contiguous with {n * k} elements")
+    t = lambda x: None if x is None else x.data_ptr()
     curves = abi.ClassicCurves(points=points.data_ptr(), point_counts=t(point_counts), roots=t(roots),
                                root_uvs=t(root_uvs), strand_count=n, cvs_per_strand=cvs)
     cut = abi.ClassicCutParams(amount_scale=amount_scale, amount_offset=amount_offset, radius_scale=radius_scale,
@@ -223,14 +224,15 @@ class ClassicPlan:
     130-165) of SplinePrimitive expressions, up to three ClumpingFX modules
     applied in order (`clump`: a dict of one module's seven expressions by
     attribute name, mask ... noiseCorrelation, or a list of such dicts) and at
-    most one CutFX module after them, packed for the device
+    most one CutFX module (after them, or before the last `clumps_after_cut`
+    of them), packed for the device
     (nxg_classic_plan_pack).  Each expression is text (compiled here with its
     attribute name and object type) or a CompiledProgram; ptex_count and
     custom_count place the custom and $Prefg-noise inputs in the input row."""
 
     def __init__(self, *, length=None, width=None, taper=None, taper_start=None, width_ramp=None,
                  cut_amount=None, clump=None, description_id=0, description_name="", ptex_count=0,
-                 custom_count=0):
+                 custom_count=0, clumps_after_cut=0):
         given = dict(length=length, width=width, cut_amount=cut_amount, taper=taper, taper_start=taper_start,
                      width_ramp=width_ramp)
         modules = [] if clump is None else [clump] if isinstance(clump, dict) else list(clump)
@@ -253,9 +255,11 @@ class ClassicPlan:
                 [classic_binding(name, ptex_count, custom_count) for name in prog.inputs])
         self.has_cut = cut_amount is not None
         self.clump_count = len(modules)
+        self.clumps_after_cut = clumps_after_cut
         self.description_id = description_id
         self.name_seed = int(lib().nxg_string_seed(description_name.rsplit(":", 1)[-1].encode()))
         self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, clump_count=self.clump_count,
+                                      clumps_after_cut=clumps_after_cut,
                                       description_id=description_id, description_name_seed=self.name_seed)
         for k, key in enumerate(abi.CLASSIC_EXPRS):
             if key in self.programs:

@@ -127,6 +127,7 @@ struct CutArgs {
     uint32_t *err_bits;          // the NXG_CLASSIC_* bits of every failing strand
     // the ClumpingFX modules, in order
     uint32_t clump_count;
+    uint32_t clumps_after_cut;   // the last modules, which follow the CutFX
     uint32_t clump_length;       // bit m: module m's amount or clumpScale reads cLength
     ClumpModule clumps[NXG_CLASSIC_MAX_CLUMPS];
     float4 *clumped;             // [strands x cvs] every strand after the last module
@@ -1225,6 +1226,23 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
     return true;
 }
 
+// The rebuild half of cut_from_tip_and_rebuild (src/xgen_classic_runtime.cpp:
+// 729-742) for a CutFX that later modules follow: n CVs uniformly on
+// [0, cut] (all at the root when cut is 0), radii kept, into a.clumped and
+// back into the lane's LDS row.
+__device__ __forceinline__ void rebuild_staged(const CutArgs &a, const StagedCurve<true> &p, uint32_t s, float cut) {
+    const uint32_t n = a.cvs;
+    float4 *out = a.clumped + uint64_t{s} * n;
+    const float step = cut / static_cast<float>(n - 1u);
+    const F3 root = xyz(p[0]);
+    out[0] = p[0];
+    for (uint32_t cv = 1u; cv < n; ++cv) {
+        const F3 q = cut > 0.0f ? curve_eval(p, n, step * static_cast<float>(cv)) : root;
+        out[cv] = make_float4(q.x, q.y, q.z, p[cv].w);
+    }
+    for (uint32_t cv = 0u; cv < n; ++cv) p.tile[staged_index<true>(cv, p.lane)] = out[cv];
+}
+
 // One strand of the runtime chain (src/xgen_classic_runtime.cpp:1619-1858):
 // the cut parameter of a kept strand (1 without a CutFX), -1 for a culled
 // one; *err gets the NXG_CLASSIC_* bit where the reference throws.
@@ -1267,18 +1285,15 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
     const float c_width = width;
     bool clumped = false;  // the context's t is 1: the last module with a guide clumped the strand
-    if constexpr (std::is_same<Curve, StagedCurve<true>>::value) {
-        // the clump modules in order (strands of at most 64 CVs, always staged)
-        if (a.clump_count) {
-            bool any = false;
+    bool written = false;  // a.clumped holds the strand's current CVs
+    constexpr bool kStaged = std::is_same<Curve, StagedCurve<true>>::value;
+    // the clump modules before the CutFX, in order (strands of at most 64
+    // CVs, always staged)
+    const uint32_t before = a.clump_count - a.clumps_after_cut;
+    if constexpr (kStaged) {
 #pragma unroll 1
-            for (uint32_t m = 0; m < a.clump_count; ++m)
-                if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, any, err)) return -1.0f;
-            if (!any) {  // the write reads every strand from a.clumped
-                float4 *out = a.clumped + uint64_t{s} * n;
-                for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = source[cv];
-            }
-        }
+        for (uint32_t m = 0; m < before; ++m)
+            if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
     }
     float cut = 1.0f;
     bool keep = true;
@@ -1294,8 +1309,28 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
         }
         cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
         keep = cut >= 1.0e-4f;
+        if constexpr (kStaged) {
+            if (a.clumps_after_cut) {  // later modules read the cut strand: rebuild it now
+                rebuild_staged(a, source, s, cut);
+                written = true;
+                cut = 1.0f;  // nothing left for the write to resample
+                c_length = spline_length(a.length_samples, source, n);
+            }
+        }
         // the post-effect programs read the rebuilt curve's cLength
-        if (a.final_length) c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
+        if (!a.clumps_after_cut && a.final_length)
+            c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
+    }
+    if constexpr (kStaged) {
+        if (a.clump_count) {
+#pragma unroll 1
+            for (uint32_t m = before; m < a.clump_count; ++m)
+                if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
+            if (!written) {  // the write reads every strand from a.clumped
+                float4 *out = a.clumped + uint64_t{s} * n;
+                for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = source[cv];
+            }
+        }
     }
     float taper = 0.0f, taper_start = 0.0f;
     if (has_program(a, NXG_CLASSIC_EXPR_TAPER) &&
@@ -1625,6 +1660,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.err_bits = a.status_word + 1;
     if (plan.clump_count) {
         a.clump_count = plan.clump_count;
+        a.clumps_after_cut = plan.clumps_after_cut;
         a.clump_length = plan.clump_length;
         uint32_t base = 0u;
         for (uint32_t m = 0; m < plan.clump_count; ++m) {
@@ -1670,6 +1706,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
         if (a.clump_count) {
             w.points = a.clumped;
             w.present &= ~(1u << NXG_CLASSIC_EXPR_LENGTH);
+            if (a.clumps_after_cut) w.has_cut = 0u;  // the search rebuilt the cut strands
         }
         const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);
         if (cvs <= 8u)

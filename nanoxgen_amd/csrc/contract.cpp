@@ -257,7 +257,9 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
     if (uint32_t st = nxg_validate_classic_cut(curves, &common, output); st != NXG_OK) return st;
     if (plan.has_cut && !(plan.present & (1u << NXG_CLASSIC_EXPR_CUT_AMOUNT))) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     if (plan.present >> NXG_CLASSIC_EXPR_COUNT) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
-    if (plan.clump_count > NXG_CLASSIC_MAX_CLUMPS) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.clump_count > NXG_CLASSIC_MAX_CLUMPS || plan.clumps_after_cut > plan.clump_count ||
+        (plan.clumps_after_cut && !plan.has_cut))
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     if (plan.clump_count) {
         const uint32_t first = NXG_CLASSIC_EXPR_CLUMP_MASK;
         const uint32_t used = first + NXG_CLASSIC_CLUMP_EXPRS * plan.clump_count;

@@ -339,12 +339,15 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
     if (!plan || !info) return NXG_ERROR_NULL_ARGUMENT;
     if (plan->has_cut && !plan->expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT]) return NXG_ERROR_INVALID_PROGRAM;
     if (plan->clump_count > NXG_CLASSIC_MAX_CLUMPS) return NXG_ERROR_INVALID_PROGRAM;
+    if (plan->clumps_after_cut > plan->clump_count || (plan->clumps_after_cut && !plan->has_cut))
+        return NXG_ERROR_INVALID_PROGRAM;
     const uint32_t used = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_CLUMP_EXPRS * plan->clump_count;
     for (uint32_t k = NXG_CLASSIC_EXPR_CLUMP_MASK; k < used; ++k)
         if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
     nxg_classic_plan_info r{};
     r.has_cut = plan->has_cut ? 1u : 0u;
     r.clump_count = plan->clump_count;
+    r.clumps_after_cut = plan->clumps_after_cut;
     r.description_id = plan->description_id;
     r.description_name_seed = plan->description_name_seed;
     uint64_t total = 0u;

@@ -397,7 +397,7 @@ class _ClassicPlan(C.Structure):
     _fields_ = [("length", C.c_void_p), ("width", C.c_void_p), ("cut_amount", C.c_void_p), ("taper", C.c_void_p),
                 ("taper_start", C.c_void_p), ("width_ramp", C.c_void_p), ("has_cut", C.c_int),
                 ("description_id", C.c_uint32), ("name_seed", C.c_uint32), ("clump_count", C.c_uint32),
-                ("clump", (C.c_void_p * 7) * 3), ("binding", C.c_void_p * 3)]
+                ("clump", (C.c_void_p * 7) * 3), ("binding", C.c_void_p * 3), ("clumps_after_cut", C.c_uint32)]
 
 
 def seexpr_gradients():
@@ -438,7 +438,8 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
     cols = None if columns is None else np.ascontiguousarray(columns, np.float32).reshape(n, -1)
     ncol = 0 if cols is None else cols.shape[1]
     op = _ClassicPlan(has_cut=1 if plan.has_cut else 0, description_id=plan.description_id,
-                      name_seed=plan.name_seed, clump_count=plan.clump_count)
+                      name_seed=plan.name_seed, clump_count=plan.clump_count,
+                      clumps_after_cut=plan.clumps_after_cut)
     for key, prog in plan.programs.items():
         if key.startswith("clump"):
             name, _, module = key.partition("@")

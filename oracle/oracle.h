@@ -151,6 +151,7 @@ typedef struct orc_classic_plan {
     /* per module: mask, clump, clumpScale, noise, noiseScale, noiseFrequency, noiseCorrelation */
     const nxg_expr_program *clump[3][7];
     const orc_clump_binding *binding[3];
+    uint32_t clumps_after_cut; /* the last modules, which follow the CutFX */
 } orc_classic_plan;
 void orc_seexpr_gradients(float *out); /* 256 x 3 */
 void orc_clump_noise_axis(const float *axis3, const float *distance3, uint32_t n, float raw_frequency,

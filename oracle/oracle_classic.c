@@ -263,7 +263,19 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
      * for its mask, and one that clumps leaves it at its last CV's (1); the
      * cut amount, taper and taperStart then read it */
     float t_after = 0.0f;
-    for (uint32_t m = 0; m < plan->clump_count; ++m) { /* :1680-1793, modules in object order */
+    uint8_t k = 1u;
+    const uint32_t before = plan->clump_count - plan->clumps_after_cut;
+    for (uint32_t m = 0; m <= plan->clump_count; ++m) { /* :1680-1823, effects in object order */
+        if (m == before && plan->has_cut) { /* the CutFX (:1806-1821) */
+            float amount;
+            if (classic_eval(plan->cut_amount, &ctx, vars, c_length, c_width, t_after, &amount))
+                return ORC_CLASSIC_EXPRESSION;
+            if (!isfinite(amount)) return ORC_CLASSIC_NONFINITE_AMOUNT;
+            const float cut = orc_cut_from_tip_v(p, cvs, max_std(amount, 0.0f), source);
+            if (cut < 1.0e-4f) k = 0u;
+            c_length = orc_curve_spline_length_v(p, cvs);
+        }
+        if (m == plan->clump_count) break;
         const orc_clump_binding *b = plan->binding[m];
         const nxg_expr_program *const *e = plan->clump[m];
         const uint32_t g = b->strand_guides[s];
@@ -317,16 +329,6 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
         }
         for (uint32_t cv = 1u; cv < cvs; ++cv)
             p[cv] = add3(p[cv], c3(noise3[3 * cv], noise3[3 * cv + 1], noise3[3 * cv + 2]));
-        c_length = orc_curve_spline_length_v(p, cvs);
-    }
-    uint8_t k = 1u;
-    if (plan->has_cut) {
-        float amount;
-        if (classic_eval(plan->cut_amount, &ctx, vars, c_length, c_width, t_after, &amount))
-            return ORC_CLASSIC_EXPRESSION;
-        if (!isfinite(amount)) return ORC_CLASSIC_NONFINITE_AMOUNT;
-        const float cut = orc_cut_from_tip_v(p, cvs, max_std(amount, 0.0f), source);
-        if (cut < 1.0e-4f) k = 0u;
         c_length = orc_curve_spline_length_v(p, cvs);
     }
     float taper = 0.0f, taper_start = 0.0f;

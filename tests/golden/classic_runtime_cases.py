@@ -98,6 +98,16 @@ CASES = [
                       noise="0", noiseScale="1", noiseFrequency="2", noiseCorrelation="10"),
                  dict(mask="0.5", clump="0.6", clumpScale="0.05 * $cLength", noise="0", noiseScale="1",
                       noiseFrequency="1", noiseCorrelation="0")]),
+    # the CutFX between two modules: the second clumps the cut strands (a
+    # strand the cut culls still runs it); the cut amount reads a PTEX input
+    # numbered in object order, after the first module's
+    dict(name="ns:cutBetween", description_id=33, strands="walk", ids=True, prefixes=True, guides=[8, 11],
+         clumps_after_cut=1,
+         exprs=dict(taper="rampUI(0,0.2,1:1,0.5,1)", cut_amount="$cLength * (0.2 + 0.6 * map('cut.ptx'))"),
+         clumps=[dict(mask="map('a.ptx')", clump="0.4 + 0.4 * rand()", clumpScale="0", noise="0.05",
+                      noiseScale="1", noiseFrequency="2", noiseCorrelation="30"),
+                 dict(mask="1", clump="clamp($cLength, 0, 0.8)", clumpScale="rampUI(0,0,1:1,0.3,1)", noise="0",
+                      noiseScale="1", noiseFrequency="1", noiseCorrelation="0")]),
 ]
 
 
@@ -115,9 +125,12 @@ def clump_names(case):
 
 def compile_order(case):
     """(key, attribute) in the reference's compile order; clump keys are
-    "clump<m>.<attribute>"."""
-    return (PRIMITIVE + tuple((f"clump{m}.{a}", a) for m in range(len(case_clumps(case))) for a in CLUMP)
-            + (("cut_amount", "amount"),))
+    "clump<m>.<attribute>"; the CutFX sits before the last
+    case["clumps_after_cut"] modules."""
+    count = len(case_clumps(case))
+    before = count - case.get("clumps_after_cut", 0)
+    clump = lambda ms: tuple((f"clump{m}.{a}", a) for m in ms for a in CLUMP)
+    return PRIMITIVE + clump(range(before)) + (("cut_amount", "amount"),) + clump(range(before, count))
 
 
 def xgen_text(case):
@@ -130,11 +143,15 @@ def xgen_text(case):
         if key in e:
             s += f"\t{attribute}\t{e[key]}\n"
     s += "\tendAttrs\n"
-    for module, name in zip(case_clumps(case), clump_names(case)):
-        s += f"ClumpingFXModule\n\tactive\ttrue\n\tname\t{name}\n"
-        s += "".join(f"\t{a}\t{module[a]}\n" for a in CLUMP) + "\tendAttrs\n"
-    if "cut_amount" in e:
-        s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{e['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
+    modules = list(zip(case_clumps(case), clump_names(case)))
+    before = len(modules) - case.get("clumps_after_cut", 0)
+    for i, module in enumerate(modules + [None]):
+        if i == before and "cut_amount" in e:
+            s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{e['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
+        if module is None:
+            break
+        s += f"ClumpingFXModule\n\tactive\ttrue\n\tname\t{module[1]}\n"
+        s += "".join(f"\t{a}\t{module[0][a]}\n" for a in CLUMP) + "\tendAttrs\n"
     return s + "\tActive\tSplinePrimitive\n"
 
 
@@ -175,6 +192,8 @@ def plan_sources(case):
     clumps = [{a: out.pop(f"clump{m}.{a}") for a in CLUMP} for m in range(len(modules))]
     if clumps:
         out["clump"] = clumps if "clumps" in case else clumps[0]
+    if case.get("clumps_after_cut"):
+        out["clumps_after_cut"] = case["clumps_after_cut"]
     return out, ptex, freqs
 
 

@@ -197,7 +197,8 @@ def test_clump_walks_oracle(torch, orc, cvs):
 def test_hierarchy_walks_oracle(torch, orc, cvs):
     """Hierarchical clumping: two and three ClumpingFX modules, each with its
     own guides (different counts, guideless strands, the second module's
-    mask zero for some strands), then a CutFX reading t; GPU == oracle."""
+    mask zero for some strands), and a CutFX reading t after them or
+    between them; GPU == oracle."""
     rng = np.random.default_rng(300 + cvs)
     n = 64 * 2 + 37
     steps = rng.normal(0, 0.3, size=(n, cvs, 3)).astype(np.float32)
@@ -219,16 +220,16 @@ def test_hierarchy_walks_oracle(torch, orc, cvs):
     levels = [CLUMP_PLANS[0]["clump"], CLUMP_PLANS[1]["clump"],
               dict(mask="0.7", clump="0.5 * $__nxg_map_1", clumpScale="rampUI(0,0,1:1,0.2,1)", noise="0.1",
                    noiseScale="1", noiseFrequency="2", noiseCorrelation="20")]
-    for count in (2, 3):
+    for count, after in ((2, 0), (3, 0), (2, 1), (3, 2)):
         plan = classic.ClassicPlan(clump=levels[:count], cut_amount="$cLength * 0.2 + rampUI(0,0,1:1,0.1,1)",
                                    taper="rampUI(0,0.3,1:1,0.6,1)", description_id=6, description_name="levels",
-                                   ptex_count=2)
+                                   ptex_count=2, clumps_after_cut=after)
         want = orc.classic_runtime(pts, cvs, plan, roots=roots, primitive_ids=ids, random_prefixes=prefixes,
                                    columns=columns, clumps=clumps[:count])
         assert want["strand_count"] is not None, want["error_bits"]
         got = run_gpu(torch, plan, pts, cvs, roots=roots, ids=ids, prefixes=prefixes, columns=columns,
                       clumps=clumps[:count])
-        same(got, want, f"{cvs} CVs, {count} clump levels", noisy_clump=True)
+        same(got, want, f"{cvs} CVs, {count} clump levels, {after} after the cut", noisy_clump=True)
 
 
 def test_generated_plane(torch, orc):
