@@ -5,7 +5,7 @@
 // and the sampled exact check that sends failing tiles of roots back to the
 // exact expression runtime.  No reference counterpart: the reference only
 // has the exact runtime (src/xgen_expression.cpp:984-1148), which stays the
-// fallback and the judge.
+// fallback and the arbiter.
 //
 // A block of 256 lanes (four wavefronts) owns a tile of 64 samples; every
 // layer is a 64 x N x K product split into 16 x 16 output tiles, four per

@@ -5,7 +5,7 @@ Surrogate: a 10 -> 64 -> 64 -> 64 -> 64 -> 4 SiLU MLP with BF16 weights and
 activations and FP32 accumulation, trained on the GPU by the library's own
 forward/backward/Adam kernels on (root features -> attributes) pairs that
 the exact expression runtime generates for roots drawn with a disjoint seed.
-At inference the exact runtime stays the judge: nxg_verify_attributes
+At inference the exact runtime stays the arbiter: nxg_verify_attributes
 re-evaluates a sample of roots exactly and sends every 64-root tile with a
 sample outside the relative threshold back to the exact runtime.
 """
