@@ -501,7 +501,7 @@ def bench_classic_expr(args, with_clump=False):
                         guide_random_prefixes=torch.from_numpy(guide_prefixes.view(np.int32)).cuda())
         guide_host.update(uvs=guide_uvs, prefixes=guide_prefixes)
     out = ClassicCutOutputs(N, Cv)
-    ws = torch.empty(runtime_workspace_bytes(N, Cv, guide_count), dtype=torch.uint8, device="cuda")
+    ws = torch.empty(runtime_workspace_bytes(N, Cv, guide_count, plan), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     log(f"generated {N} x {Cv} curves; plan image {plan.info.byte_size} B; {guide_count} clump guides")
 

@@ -889,6 +889,11 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
  *   CVs 1.. blended toward the guide's render axis by
  *   mask amount (1 - 2 clumpScale(t)), plus the noise axis; cLength again;
  *   (each ClumpingFX module in turn, on the previous one's strand);
+ *   NoiseFX (apply_noise, :897-1039): mask (> 1e-6), frequency, correlation
+ *   and preserveLength at the context's t; the transported frame of the
+ *   strand from its surface normal and tangent; per CV magnitude(t) *
+ *   magnitudeScale(t) * mask times the classic noise of the domain moved
+ *   along each axis; the preserve-length rescale; cLength again;
  *   CutFX: amount, cut_from_tip_and_rebuild, culled when the cut parameter
  *   < 1e-4, cLength of the rebuilt curve; taper, taperStart; per CV t =
  *   cv / (n - 1), scale = 1 - taper (t - start) / (1 - start) past the
@@ -918,6 +923,8 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
                                                (or a non-finite input or context) */
 #define NXG_CLASSIC_VARIABLE_COUNTS 128u    /* "Classic runtime requires fixed point counts" */
 #define NXG_CLASSIC_NONFINITE_CLUMP 256u    /* "Classic ClumpingFX noise magnitude is non-finite" */
+#define NXG_CLASSIC_NONFINITE_NOISE_PARAMETER 512u /* "Classic NoiseFX produced a non-finite parameter" */
+#define NXG_CLASSIC_NONFINITE_NOISE 1024u   /* "Classic NoiseFX magnitude is non-finite" */
 
 /* Hierarchical clumping: up to this many ClumpingFX modules in one plan, applied
  * in object order, each with its own ClassicClumpRuntimeData; the CutFX may sit
@@ -943,7 +950,14 @@ enum {
     NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE,
     NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY,
     NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION,
-    NXG_CLASSIC_EXPR_COUNT = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_MAX_CLUMPS * NXG_CLASSIC_CLUMP_EXPRS
+    /* the NoiseFX module (ClassicFloatNoiseModule, xgen_classic_runtime.h:32-40), after the clumps */
+    NXG_CLASSIC_EXPR_NOISE_MASK = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_MAX_CLUMPS * NXG_CLASSIC_CLUMP_EXPRS,
+    NXG_CLASSIC_EXPR_NOISE_MAGNITUDE,
+    NXG_CLASSIC_EXPR_NOISE_MAGNITUDE_SCALE,
+    NXG_CLASSIC_EXPR_NOISE_FREQUENCY,
+    NXG_CLASSIC_EXPR_NOISE_CORRELATION,
+    NXG_CLASSIC_EXPR_NOISE_PRESERVE_LENGTH,
+    NXG_CLASSIC_EXPR_COUNT
 };
 /* Program k (one of NXG_CLASSIC_EXPR_CLUMP_MASK .. _NOISE_CORRELATION) of ClumpingFX module m. */
 #define NXG_CLASSIC_EXPR_CLUMP_OF(m, k) ((k) + NXG_CLASSIC_CLUMP_EXPRS * (m))
@@ -962,12 +976,14 @@ typedef struct nxg_classic_plan {
     uint32_t description_name_seed; /* nxg_string_seed of the description name after its last ':' */
     uint32_t clumps_after_cut;      /* the last this many ClumpingFX modules follow the CutFX in object
                                        order (0: the CutFX comes last) */
+    uint32_t has_noise;             /* one NoiseFX module (mode 0) after the clumps, before the CutFX;
+                                       needs clumps_after_cut 0 and all six of its expressions */
 } nxg_classic_plan;
 
 /* What nxg_classic_plan_pack reports about a plan. */
 typedef struct nxg_classic_plan_info {
     uint64_t byte_size;           /* of the device image */
-    uint32_t present;             /* bit k: expressions[k] given */
+    uint64_t present;             /* bit k: expressions[k] given */
     uint32_t has_cut;
     uint32_t clump_count;
     uint32_t columns_required;    /* 1 + the largest input column read (0: none) */
@@ -980,6 +996,8 @@ typedef struct nxg_classic_plan_info {
     uint32_t uniform;             /* bit k: expressions[k] reads no input and calls no rand(), so its
                                      value depends on t alone (evaluated once per call) */
     uint32_t clumps_after_cut;
+    uint32_t has_noise;
+    uint32_t reserved;
 } nxg_classic_plan_info;
 
 /* Validates and packs a plan into a position-independent image (out NULL:
@@ -1007,6 +1025,11 @@ typedef struct nxg_classic_runtime_inputs {
     float radius_scale;               /* finite, >= 0 */
     const nxg_classic_clump_binding *clumps; /* host array, one per ClumpingFX module (plan->clump_count);
                                                 with a clump cvs_per_strand <= NXG_CLUMP_MAX_CVS */
+    /* the NoiseFX binding: one surface tangent per strand (required with a NoiseFX, which also needs
+     * cvs_per_strand <= NXG_CLUMP_MAX_CVS) and optional per-strand noise domain positions (else the
+     * strand's current root) */
+    const nxg_vec3 *surface_tangents;
+    const nxg_vec3 *noise_domain_positions;
 } nxg_classic_runtime_inputs;
 
 /* Host-side checks (the reference's argument checks,
@@ -1014,9 +1037,11 @@ typedef struct nxg_classic_runtime_inputs {
 uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nxg_classic_runtime_inputs *inputs,
                                       const nxg_classic_output *output);
 
-/* Device workspace for nxg_classic_runtime_apply; guide_count is the sum of
- * the modules' guide counts (0 without a clump). */
-uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count, uint32_t cvs_per_strand, uint32_t guide_count);
+/* Device workspace for nxg_classic_runtime_apply of a plan (the packer's
+ * report; NULL: a plan without ClumpingFX or NoiseFX); guide_count is the
+ * sum of the clump modules' guide counts. */
+uint64_t nxg_classic_runtime_workspace_size(const nxg_classic_plan_info *plan, uint32_t strand_count,
+                                            uint32_t cvs_per_strand, uint32_t guide_count);
 
 /* On `stream`, without synchronising or allocating: the chain above, then
  * the stable compaction into `output` (points, point counts, roots, root

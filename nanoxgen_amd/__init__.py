@@ -76,7 +76,7 @@ SYMBOLS = [
                                              C.c_uint64, _P]),
     ("nxg_classic_plan_pack", C.c_uint32, [_P, _P, C.c_uint64, _P]),
     ("nxg_validate_classic_runtime", C.c_uint32, [_P, _P, _P]),
-    ("nxg_classic_runtime_workspace_size", C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("nxg_classic_runtime_workspace_size", C.c_uint64, [_P, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("nxg_classic_runtime_apply", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_curve_payload_plan", C.c_uint32, [_P, _P, C.c_uint32, _P, _P, _P, C.c_char_p, C.c_uint64]),
     ("nxg_curve_payload_workspace_size", C.c_uint64, [C.c_uint32, C.c_uint32]),

@@ -289,24 +289,29 @@ def clump_key(module, key):
 
 
 # NXG_CLASSIC_EXPR_*: module m's clump programs at 6 + 7 m + k
+CLASSIC_NOISE_EXPRS = ("noise_mask", "noise_magnitude", "noise_magnitude_scale", "noise_frequency",
+                       "noise_correlation", "noise_preserve_length")
 CLASSIC_EXPRS = (("length", "width", "cut_amount", "taper", "taper_start", "width_ramp")
-                 + tuple(clump_key(m, k) for m in range(CLASSIC_MAX_CLUMPS) for k in CLASSIC_CLUMP_EXPRS))
+                 + tuple(clump_key(m, k) for m in range(CLASSIC_MAX_CLUMPS) for k in CLASSIC_CLUMP_EXPRS)
+                 + CLASSIC_NOISE_EXPRS)
+CLASSIC_NONFINITE_NOISE_PARAMETER, CLASSIC_NONFINITE_NOISE = 512, 1024
 
 
 class ClassicPlan(C.Structure):
     """nxg_classic_plan: the plan's programs (host nxg_expr_program pointers, NULL: absent)."""
     _fields_ = [("expressions", C.c_void_p * len(CLASSIC_EXPRS)), ("has_cut", C.c_uint32),
                 ("clump_count", C.c_uint32), ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32),
-                ("clumps_after_cut", C.c_uint32)]
+                ("clumps_after_cut", C.c_uint32), ("has_noise", C.c_uint32)]
 
 
 class ClassicPlanInfo(C.Structure):
     """nxg_classic_plan_info: what nxg_classic_plan_pack reports."""
-    _fields_ = [("byte_size", C.c_uint64), ("present", C.c_uint32), ("has_cut", C.c_uint32),
+    _fields_ = [("byte_size", C.c_uint64), ("present", C.c_uint64), ("has_cut", C.c_uint32),
                 ("clump_count", C.c_uint32), ("columns_required", C.c_uint32), ("needs_random_prefix", C.c_uint32),
                 ("final_length", C.c_uint32), ("clump_length", C.c_uint32), ("description_id", C.c_uint32),
                 ("description_name_seed", C.c_uint32), ("program_offset", C.c_uint32 * len(CLASSIC_EXPRS)),
-                ("uniform", C.c_uint32), ("clumps_after_cut", C.c_uint32)]
+                ("uniform", C.c_uint32), ("clumps_after_cut", C.c_uint32), ("has_noise", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class ClassicClumpBinding(C.Structure):
@@ -319,7 +324,8 @@ class ClassicRuntimeInputs(C.Structure):
     """nxg_classic_runtime_inputs (device pointers, host plan report, host clump bindings)."""
     _fields_ = [("plan_image", C.c_void_p), ("plan", C.c_void_p), ("primitive_ids", C.c_void_p),
                 ("random_prefixes", C.c_void_p), ("columns", C.c_void_p), ("column_count", C.c_uint32),
-                ("radius_scale", C.c_float), ("clumps", C.c_void_p)]
+                ("radius_scale", C.c_float), ("clumps", C.c_void_p), ("surface_tangents", C.c_void_p),
+                ("noise_domain_positions", C.c_void_p)]
 
 
 class ExprCompileOptions(C.Structure):

@@ -197,7 +197,17 @@ CLASSIC_ATTRIBUTES = {
     "clump_noise_scale": ("noiseScale", "ClumpingFXModule"),
     "clump_noise_frequency": ("noiseFrequency", "ClumpingFXModule"),
     "clump_noise_correlation": ("noiseCorrelation", "ClumpingFXModule"),
+    "noise_mask": ("mask", "NoiseFXModule"),
+    "noise_magnitude": ("magnitude", "NoiseFXModule"),
+    "noise_magnitude_scale": ("magnitudeScale", "NoiseFXModule"),
+    "noise_frequency": ("frequency", "NoiseFXModule"),
+    "noise_correlation": ("correlation", "NoiseFXModule"),
+    "noise_preserve_length": ("preserveLength", "NoiseFXModule"),
 }
+# ClassicPlan(noise=...) keys, the NoiseFX module's attribute names
+NOISE_KEYS = {"mask": "noise_mask", "magnitude": "noise_magnitude", "magnitudeScale": "noise_magnitude_scale",
+              "frequency": "noise_frequency", "correlation": "noise_correlation",
+              "preserveLength": "noise_preserve_length"}
 # ClassicPlan(clump=...) keys, the module's attribute names
 CLUMP_KEYS = {"mask": "clump_mask", "clump": "clump", "clumpScale": "clump_scale", "noise": "clump_noise",
               "noiseScale": "clump_noise_scale", "noiseFrequency": "clump_noise_frequency",
@@ -225,14 +235,16 @@ class ClassicPlan:
     applied in order (`clump`: a dict of one module's seven expressions by
     attribute name, mask ... noiseCorrelation, or a list of such dicts) and at
     most one CutFX module (after them, or before the last `clumps_after_cut`
-    of them), packed for the device
+    of them), with an optional NoiseFX module (`noise`: its six expressions
+    by attribute name) after the clumps and before the cut, packed for the
+    device
     (nxg_classic_plan_pack).  Each expression is text (compiled here with its
     attribute name and object type) or a CompiledProgram; ptex_count and
     custom_count place the custom and $Prefg-noise inputs in the input row."""
 
     def __init__(self, *, length=None, width=None, taper=None, taper_start=None, width_ramp=None,
                  cut_amount=None, clump=None, description_id=0, description_name="", ptex_count=0,
-                 custom_count=0, clumps_after_cut=0):
+                 custom_count=0, clumps_after_cut=0, noise=None):
         given = dict(length=length, width=width, cut_amount=cut_amount, taper=taper, taper_start=taper_start,
                      width_ramp=width_ramp)
         modules = [] if clump is None else [clump] if isinstance(clump, dict) else list(clump)
@@ -243,6 +255,11 @@ class ClassicPlan:
             if missing:
                 raise ValueError(f"ClumpingFX module {m} needs {sorted(missing)}")
             given.update({abi.clump_key(m, CLUMP_KEYS[k]): v for k, v in module.items()})
+        if noise is not None:
+            missing = set(NOISE_KEYS) - set(noise)
+            if missing:
+                raise ValueError(f"NoiseFX module needs {sorted(missing)}")
+            given.update({NOISE_KEYS[k]: v for k, v in noise.items()})
         self.programs = {}
         for key in abi.CLASSIC_EXPRS:
             src = given.get(key)
@@ -256,10 +273,11 @@ class ClassicPlan:
         self.has_cut = cut_amount is not None
         self.clump_count = len(modules)
         self.clumps_after_cut = clumps_after_cut
+        self.has_noise = noise is not None
         self.description_id = description_id
         self.name_seed = int(lib().nxg_string_seed(description_name.rsplit(":", 1)[-1].encode()))
         self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, clump_count=self.clump_count,
-                                      clumps_after_cut=clumps_after_cut,
+                                      clumps_after_cut=clumps_after_cut, has_noise=1 if noise is not None else 0,
                                       description_id=description_id, description_name_seed=self.name_seed)
         for k, key in enumerate(abi.CLASSIC_EXPRS):
             if key in self.programs:
@@ -279,14 +297,15 @@ class ClassicPlan:
         return self._device[key]
 
 
-def runtime_workspace_bytes(strands, cvs=2, guides=0):
-    return int(lib().nxg_classic_runtime_workspace_size(strands, cvs, guides))
+def runtime_workspace_bytes(strands, cvs=2, guides=0, plan=None):
+    info = None if plan is None else C.byref(plan.info)
+    return int(lib().nxg_classic_runtime_workspace_size(info, strands, cvs, guides))
 
 
 def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=None, primitive_ids=None,
                     random_prefixes=None, columns=None, radius_scale=1.0, strand_guides=None, guides=None,
-                    guide_uvs=None, guide_random_prefixes=None, clumps=None, outputs=None, workspace=None,
-                    stream=None):
+                    guide_uvs=None, guide_random_prefixes=None, clumps=None, surface_tangents=None,
+                    noise_domain_positions=None, outputs=None, workspace=None, stream=None):
     """Enqueue the Classic runtime chain of `plan` (a ClassicPlan) on
     `stream` over device curves: `points` float32 [N * cvs, 4], `roots` int32
     [N, 12] (RootSample records: the context's u, v and face); optional
@@ -298,7 +317,8 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
     device tensors axes [G * cvs, 3], normals and tangents [G, 3], optional
     reference [G, 3]), `guide_uvs` float32 [G, 2] and `guide_random_prefixes`
     int32 [G]; for a single module those four may be passed as keywords
-    instead.  Returns the ClassicCutOutputs (its kept(), status() and
+    instead.  A NoiseFX takes `surface_tangents` float32 [N, 3] and optional
+    `noise_domain_positions` float32 [N, 3].  Returns the ClassicCutOutputs (its kept(), status() and
     to_host() synchronise)."""
     torch = _torch()
     if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
@@ -335,7 +355,7 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
         outputs = ClassicCutOutputs(n, cvs, device=points.device, point_counts=True, roots=True,
                                     root_uvs=root_uvs is not None)
     if workspace is None:
-        workspace = torch.empty(runtime_workspace_bytes(n, cvs, guide_count),
+        workspace = torch.empty(runtime_workspace_bytes(n, cvs, guide_count, plan),
                                 dtype=torch.uint8, device=points.device)
     column_count = 0 if columns is None else (columns.shape[1] if columns.dim() == 2 else 0)
     for name, x, k in (("roots", roots, 12), ("point_counts", point_counts, 1), ("root_uvs", root_uvs, 2),
@@ -350,7 +370,9 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
                                       primitive_ids=t(primitive_ids), random_prefixes=t(random_prefixes),
                                       columns=t(columns) if column_count else None, column_count=column_count,
                                       radius_scale=radius_scale,
-                                      clumps=C.addressof(bindings) if clumps else None)
+                                      clumps=C.addressof(bindings) if clumps else None,
+                                      surface_tangents=t(surface_tangents),
+                                      noise_domain_positions=t(noise_domain_positions))
     out = outputs.descriptor()
     check(lib().nxg_classic_runtime_apply(C.byref(curves), C.byref(inputs), C.byref(out),
                                           C.c_void_p(workspace.data_ptr()), workspace.numel(),

@@ -116,7 +116,8 @@ struct CutArgs {
     // Classic primitive expressions (nxg_classic_runtime_apply only)
     const uint8_t *image;        // packed plan, device
     uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
-    uint32_t present, has_cut, final_length, profile;
+    uint64_t present;            // bit k: program k given
+    uint32_t has_cut, final_length, profile;
     const uint32_t *ids, *prefixes;
     const float *columns;
     uint32_t column_count;
@@ -128,6 +129,8 @@ struct CutArgs {
     // the ClumpingFX modules, in order
     uint32_t clump_count;
     uint32_t clumps_after_cut;   // the last modules, which follow the CutFX
+    uint32_t has_noise;          // the NoiseFX module, after the clumps
+    const nxg_vec3 *surface_tangents, *noise_domain;
     uint32_t clump_length;       // bit m: module m's amount or clumpScale reads cLength
     ClumpModule clumps[NXG_CLASSIC_MAX_CLUMPS];
     float4 *clumped;             // [strands x cvs] every strand after the last module
@@ -949,7 +952,7 @@ __device__ __forceinline__ float cv_parameter(uint32_t cv, uint32_t n) {
 // the others run at t = 0.
 __device__ __forceinline__ int cv_table(uint32_t k) {
     if (k == NXG_CLASSIC_EXPR_WIDTH_RAMP) return 0;
-    if (k < NXG_CLASSIC_EXPR_CLUMP_MASK) return -1;
+    if (k < NXG_CLASSIC_EXPR_CLUMP_MASK || k >= NXG_CLASSIC_EXPR_NOISE_MASK) return -1;
     const uint32_t m = (k - NXG_CLASSIC_EXPR_CLUMP_MASK) / NXG_CLASSIC_CLUMP_EXPRS;
     const uint32_t role = k - NXG_CLASSIC_CLUMP_EXPRS * m;
     return role == NXG_CLASSIC_EXPR_CLUMP_SCALE         ? static_cast<int>(1u + 2u * m)
@@ -986,7 +989,7 @@ __device__ __forceinline__ uint32_t uniform_slot(uint32_t k, uint32_t cv, bool a
 // every evaluation (xgen_expression.cpp:991-994).
 __device__ __forceinline__ bool classic_value(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
                                               float c_width, uint32_t cv, float &out, bool at_one = false) {
-    if ((a.uniform >> k) & 1u) {
+    if (k < 32u && ((a.uniform >> k) & 1u)) {
         const float2 v = a.uniform_values[uniform_slot(k, cv, at_one)];
         out = v.x;
         return v.y != 0.0f && isfinite(c.expr.u) && isfinite(c.expr.v) && isfinite(c.expr.face_seed);
@@ -1020,7 +1023,7 @@ __global__ void __launch_bounds__(256) classic_uniform_kernel(CutArgs a) {
             k = cv_table_program(table);
             t = cv_parameter(cv, a.cvs);
         }
-        if (!((a.uniform >> k) & 1u)) continue;
+        if (k >= 32u || !((a.uniform >> k) & 1u)) continue;  // NoiseFX programs are never uniform
         ClassicContext c{};
         float v = 0.0f;
         const bool ok = classic_eval(a, k, c, 0.0f, 0.0f, t, v);
@@ -1226,6 +1229,136 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
     return true;
 }
 
+// fallback_surface_u (generate.h:312-316)
+__device__ __forceinline__ F3 fallback_surface_u(F3 normal) {
+    const F3 axis = normal.z > -0.999f && normal.z < 0.999f ? F3{0.0f, 0.0f, 1.0f} : F3{0.0f, 1.0f, 0.0f};
+    return normalize3(cross3(normal, axis));
+}
+
+// The NoiseFX step (apply_noise, src/xgen_classic_runtime.cpp:897-1039) on
+// one strand's staged CVs: mask, frequency, correlation and preserveLength
+// at the context's t; the frame transported from the surface normal along
+// the segments (acos, cos and sin as the reference evaluates them); per CV
+// the magnitude at its t and the classic noise of the domain moved along
+// each axis; the preserve-length rescale.  The result goes to a.clumped and
+// back into the lane's LDS row; c_length follows.  False (with *err) where
+// the reference throws.
+__device__ __forceinline__ bool runtime_noise(const CutArgs &a, const ClassicContext &c, const StagedCurve<true> &p,
+                                              uint32_t s, float &c_length, float c_width, bool &at_one,
+                                              bool &written, uint32_t *err) {
+    const uint32_t n = a.cvs;
+    const float t0 = at_one ? 1.0f : 0.0f;
+    float raw_mask, raw_frequency, raw_correlation, raw_preserve;
+    if (!classic_eval(a, NXG_CLASSIC_EXPR_NOISE_MASK, c, c_length, c_width, t0, raw_mask)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return false;
+    }
+    const float mask = clamp_std(raw_mask, 0.0f, 1.0f);
+    if (!(mask > 1.0e-6f)) return true;
+    if (!classic_eval(a, NXG_CLASSIC_EXPR_NOISE_FREQUENCY, c, c_length, c_width, t0, raw_frequency) ||
+        !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_CORRELATION, c, c_length, c_width, t0, raw_correlation) ||
+        !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_PRESERVE_LENGTH, c, c_length, c_width, t0, raw_preserve)) {
+        *err = NXG_CLASSIC_EXPRESSION_ERROR;
+        return false;
+    }
+    const float frequency = max_std(raw_frequency, 0.0f);
+    const float correlation = clamp_std(raw_correlation * 0.01f, 0.0f, 1.0f);
+    const float preserve = clamp_std(raw_preserve * 0.01f, 0.0f, 1.0f);
+    if (!isfinite(frequency) || !isfinite(correlation) || !isfinite(preserve)) {
+        *err = NXG_CLASSIC_NONFINITE_NOISE_PARAMETER;
+        return false;
+    }
+    float original = 0.0f;  // curve_polyline_length
+    for (uint32_t cv = 1u; cv < n; ++cv) original += sqrtf(length_squared(xyz(p[cv]) - xyz(p[cv - 1u])));
+    const float effective = original > 0.0f ? max_std(0.5f / original, frequency) : frequency;
+    const float decorrelation = 1.0f - correlation;
+    const float domain_scale = 100.0f * decorrelation * decorrelation;
+    const F3 root = xyz(p[0]);
+    F3 domain = root;
+    if (a.noise_domain) domain = F3{a.noise_domain[s].x, a.noise_domain[s].y, a.noise_domain[s].z};
+    domain = (domain + F3{0.419276f, 0.184247f, 0.805721f}) * domain_scale;
+    const nxg_vec3 rn = a.roots[s].normal, rt = a.surface_tangents[s];
+    const F3 surface_normal{rn.x, rn.y, rn.z};
+    F3 surface_tangent{rt.x, rt.y, rt.z};
+    surface_tangent = !(length_squared(surface_tangent) >= 1.0e-12f) ? fallback_surface_u(surface_normal)
+                                                                     : normalize3(surface_tangent);
+    F3 previous_base = root, current_base = root, next_base = xyz(p[1]);
+    F3 prior = length_squared(surface_normal) >= 1.0e-12f ? normalize3(surface_normal) : F3{0.0f, 1.0f, 0.0f};
+    F3 transported = surface_tangent;
+    float travelled = 0.0f;
+    float4 *out = a.clumped + uint64_t{s} * n;
+    out[0] = p[0];
+    for (uint32_t cv = 0u; cv < n; ++cv) {
+        if (cv > 0u) travelled += sqrtf(length_squared(current_base - previous_base));
+        F3 next = prior;
+        if (cv + 1u < n) {
+            const F3 segment = next_base - current_base;
+            if (length_squared(segment) >= 1.0e-12f) next = normalize3(segment);
+        }
+        const F3 axis = cross3(prior, next);
+        const float als = length_squared(axis);
+        const float tangent_dot = clamp_std(dot3(prior, next), -1.0f, 1.0f);
+        if (als >= 1.0e-12f) {
+            const F3 ra = axis * (1.0f / sqrtf(als));
+            const float angle = acosf(tangent_dot);
+            const float cosine = cosf(angle), sine = sinf(angle);
+            const F3 rotated = transported * cosine + cross3(ra, transported) * sine +
+                               ra * (dot3(ra, transported) * (1.0f - cosine));
+            transported = length_squared(rotated) >= 1.0e-12f ? normalize3(rotated) : F3{0.0f, 0.0f, 0.0f};
+        }
+        const F3 normal = transported;
+        F3 binormal = cross3(normal, next);
+        binormal = length_squared(binormal) >= 1.0e-12f ? normalize3(binormal) : F3{0.0f, 0.0f, 0.0f};
+        const F3 tangent = cross3(binormal, normal);
+        const float t = cv_parameter(cv, n);
+        float raw_magnitude, raw_scale;
+        if (!classic_eval(a, NXG_CLASSIC_EXPR_NOISE_MAGNITUDE, c, c_length, c_width, t, raw_magnitude) ||
+            !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_MAGNITUDE_SCALE, c, c_length, c_width, t, raw_scale)) {
+            *err = NXG_CLASSIC_EXPRESSION_ERROR;
+            return false;
+        }
+        const float magnitude = max_std(raw_magnitude, 0.0f) * max_std(raw_scale, 0.0f) * mask;
+        if (!isfinite(magnitude)) {
+            *err = NXG_CLASSIC_NONFINITE_NOISE;
+            return false;
+        }
+        if (cv > 0u) {
+            const float distance = travelled * effective;
+            const F3 local{(classic_noise(domain.x + distance, domain.y, domain.z) - 0.5f) * magnitude,
+                           (classic_noise(domain.x, domain.y + distance, domain.z) - 0.5f) * magnitude,
+                           (classic_noise(domain.x, domain.y, domain.z + distance) - 0.5f) * magnitude};
+            const F3 q = current_base + normal * local.x + binormal * local.y + tangent * local.z;
+            out[cv] = make_float4(q.x, q.y, q.z, p[cv].w);
+        }
+        prior = next;
+        previous_base = current_base;
+        if (cv + 1u < n) {
+            current_base = next_base;
+            if (cv + 2u < n) next_base = xyz(p[cv + 2u]);
+        }
+    }
+    if (preserve > 0.001f) {
+        float noisy = 0.0f;
+        for (uint32_t cv = 1u; cv < n; ++cv) noisy += sqrtf(length_squared(xyz(out[cv]) - xyz(out[cv - 1u])));
+        if (noisy > 0.0f) {
+            const float target = original * preserve + noisy * (1.0f - preserve);
+            if (fabsf(noisy - target) >= 0.0001f) {  // scale_curve (:576-585)
+                const float scale = target / noisy;
+                const F3 r0 = xyz(out[0]);
+                for (uint32_t cv = 0u; cv < n; ++cv) {
+                    const F3 q = r0 + (xyz(out[cv]) - r0) * scale;
+                    out[cv] = make_float4(q.x, q.y, q.z, out[cv].w);
+                }
+            }
+        }
+    }
+    for (uint32_t cv = 0u; cv < n; ++cv) p.tile[staged_index<true>(cv, p.lane)] = out[cv];
+    c_length = spline_length(a.length_samples, p, n);
+    at_one = true;
+    written = true;
+    return true;
+}
+
 // The rebuild half of cut_from_tip_and_rebuild (src/xgen_classic_runtime.cpp:
 // 729-742) for a CutFX that later modules follow: n CVs uniformly on
 // [0, cut] (all at the root when cut is 0), radii kept, into a.clumped and
@@ -1295,6 +1428,11 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
         for (uint32_t m = 0; m < before; ++m)
             if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
     }
+    if constexpr (kStaged) {
+        if (a.has_noise) {  // after the clumps, before the CutFX
+            if (!runtime_noise(a, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
+        }
+    }
     float cut = 1.0f;
     bool keep = true;
     if (a.has_cut) {
@@ -1322,7 +1460,7 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
             c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
     }
     if constexpr (kStaged) {
-        if (a.clump_count) {
+        if (a.clump_count || a.has_noise) {
 #pragma unroll 1
             for (uint32_t m = before; m < a.clump_count; ++m)
                 if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
@@ -1635,7 +1773,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     uint32_t guides = 0u;
     for (uint32_t m = 0; m < plan.clump_count; ++m) guides += inputs->clumps[m].guides->guide_count;
     const ClassicLayout layout = classic_layout(n);
-    const ClassicRuntimeLayout rt = classic_runtime_layout(n, cvs, guides);
+    const ClassicRuntimeLayout rt = classic_runtime_layout(n, cvs, guides, plan.clump_count || plan.has_noise);
     if (!workspace || workspace_bytes < rt.total || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
         return NXG_ERROR_WORKSPACE_TOO_SMALL;
     uint8_t *ws = static_cast<uint8_t *>(workspace);
@@ -1654,10 +1792,16 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.column_count = inputs->column_count;
     a.description_id = static_cast<float>(plan.description_id);
     a.name_seed = static_cast<float>(plan.description_name_seed);
-    a.uniform = plan.uniform & plan.present;
+    a.uniform = plan.uniform & static_cast<uint32_t>(plan.present);
+    a.has_noise = plan.has_noise;
+    a.surface_tangents = inputs->surface_tangents;
+    a.noise_domain = inputs->noise_domain_positions;
     a.uniform_values = reinterpret_cast<float2 *>(ws + rt.uniform_offset);
     a.state = reinterpret_cast<RuntimeState *>(ws + rt.state_offset);
     a.err_bits = a.status_word + 1;
+    if (plan.clump_count || plan.has_noise) {
+        a.clumped = reinterpret_cast<float4 *>(ws + rt.clumped_offset);
+    }
     if (plan.clump_count) {
         a.clump_count = plan.clump_count;
         a.clumps_after_cut = plan.clumps_after_cut;
@@ -1703,7 +1847,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     if (a.tile_count) {
         // after a clump the write starts from the clumped strands, already scaled
         CutArgs w = a;
-        if (a.clump_count) {
+        if (a.clump_count || a.has_noise) {
             w.points = a.clumped;
             w.present &= ~(1u << NXG_CLASSIC_EXPR_LENGTH);
             if (a.clumps_after_cut) w.has_cut = 0u;  // the search rebuilt the cut strands

@@ -255,16 +255,26 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
     if (!std::isfinite(rs) || rs < 0.0f) return NXG_ERROR_INVALID_RADIUS_SCALE;
     const nxg_classic_cut_params common{0.0f, 0.0f, rs, 0u, nullptr};
     if (uint32_t st = nxg_validate_classic_cut(curves, &common, output); st != NXG_OK) return st;
-    if (plan.has_cut && !(plan.present & (1u << NXG_CLASSIC_EXPR_CUT_AMOUNT))) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.has_cut && !((plan.present >> NXG_CLASSIC_EXPR_CUT_AMOUNT) & 1u)) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     if (plan.present >> NXG_CLASSIC_EXPR_COUNT) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.has_noise) {
+        const uint64_t all = ((uint64_t{1} << NXG_CLASSIC_EXPR_COUNT) - 1u) &
+                             ~((uint64_t{1} << NXG_CLASSIC_EXPR_NOISE_MASK) - 1u);
+        if ((plan.present & all) != all || plan.clumps_after_cut || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS ||
+            (curves->strand_count && !inputs->surface_tangents))
+            return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    } else if (plan.present >> NXG_CLASSIC_EXPR_NOISE_MASK) {
+        return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    }
     if (plan.clump_count > NXG_CLASSIC_MAX_CLUMPS || plan.clumps_after_cut > plan.clump_count ||
         (plan.clumps_after_cut && !plan.has_cut))
         return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     if (plan.clump_count) {
         const uint32_t first = NXG_CLASSIC_EXPR_CLUMP_MASK;
         const uint32_t used = first + NXG_CLASSIC_CLUMP_EXPRS * plan.clump_count;
-        const uint32_t all = ((1u << used) - 1u) & ~((1u << first) - 1u);
-        if ((plan.present & all) != all || (plan.present >> used) || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS ||
+        const uint64_t all = ((uint64_t{1} << used) - 1u) & ~((uint64_t{1} << first) - 1u);
+        const uint64_t clumps = (plan.present & ((uint64_t{1} << NXG_CLASSIC_EXPR_NOISE_MASK) - 1u));
+        if ((plan.present & all) != all || (clumps >> used) || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS ||
             !inputs->clumps)
             return NXG_ERROR_INVALID_CLASSIC_REQUEST;
         uint64_t guides = 0u;
@@ -291,6 +301,8 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
     return NXG_OK;
 }
 
-uint64_t nxg_classic_runtime_workspace_size(uint32_t strand_count, uint32_t cvs_per_strand, uint32_t guide_count) {
-    return nxg::classic_runtime_layout(strand_count, cvs_per_strand, guide_count).total;
+uint64_t nxg_classic_runtime_workspace_size(const nxg_classic_plan_info *plan, uint32_t strand_count,
+                                            uint32_t cvs_per_strand, uint32_t guide_count) {
+    const bool rewrites = plan && (plan->clump_count || plan->has_noise);
+    return nxg::classic_runtime_layout(strand_count, cvs_per_strand, guide_count, rewrites).total;
 }

@@ -341,6 +341,10 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
     if (plan->clump_count > NXG_CLASSIC_MAX_CLUMPS) return NXG_ERROR_INVALID_PROGRAM;
     if (plan->clumps_after_cut > plan->clump_count || (plan->clumps_after_cut && !plan->has_cut))
         return NXG_ERROR_INVALID_PROGRAM;
+    if (plan->has_noise && plan->clumps_after_cut) return NXG_ERROR_INVALID_PROGRAM;
+    if (plan->has_noise)
+        for (uint32_t k = NXG_CLASSIC_EXPR_NOISE_MASK; k < NXG_CLASSIC_EXPR_COUNT; ++k)
+            if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
     const uint32_t used = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_CLUMP_EXPRS * plan->clump_count;
     for (uint32_t k = NXG_CLASSIC_EXPR_CLUMP_MASK; k < used; ++k)
         if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
@@ -348,24 +352,30 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
     r.has_cut = plan->has_cut ? 1u : 0u;
     r.clump_count = plan->clump_count;
     r.clumps_after_cut = plan->clumps_after_cut;
+    r.has_noise = plan->has_noise ? 1u : 0u;
     r.description_id = plan->description_id;
     r.description_name_seed = plan->description_name_seed;
     uint64_t total = 0u;
     PackPlan plans[NXG_CLASSIC_EXPR_COUNT];
     for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k) {
         const nxg_expr_program *p = plan->expressions[k];
-        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut) || k >= used) continue;
+        const bool noise = k >= NXG_CLASSIC_EXPR_NOISE_MASK;
+        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut) || (!noise && k >= used) ||
+            (noise && !plan->has_noise))
+            continue;
         // module and role of a clump program (role = its module-0 index)
-        const uint32_t module = k >= NXG_CLASSIC_EXPR_CLUMP_MASK ? (k - NXG_CLASSIC_EXPR_CLUMP_MASK) / NXG_CLASSIC_CLUMP_EXPRS : 0u;
+        const uint32_t module = k >= NXG_CLASSIC_EXPR_CLUMP_MASK && !noise
+                                    ? (k - NXG_CLASSIC_EXPR_CLUMP_MASK) / NXG_CLASSIC_CLUMP_EXPRS : 0u;
         const uint32_t role = k - NXG_CLASSIC_CLUMP_EXPRS * module;
         if (!validate_classic_program(*p)) return NXG_ERROR_INVALID_PROGRAM;
         // the noise frequency and correlation are evaluated once per guide
-        if ((role == NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY || role == NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION) &&
+        if (!noise &&
+            (role == NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY || role == NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION) &&
             p->input_count != 0u)
             return NXG_ERROR_INVALID_PROGRAM;
         plans[k] = plan_program(*p, true);
         if (plans[k].slot_count > NXG_CLASSIC_MAX_PROGRAM_SLOTS) return NXG_ERROR_INVALID_PROGRAM;
-        r.present |= 1u << k;
+        r.present |= uint64_t{1} << k;
         r.program_offset[k] = static_cast<uint32_t>(total);
         total += program_bytes(*p);
         for (uint32_t i = 0; i < p->input_count; ++i) {
@@ -375,7 +385,7 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
             if (b == NXG_CLASSIC_VAR_CLENGTH &&
                 (k == NXG_CLASSIC_EXPR_TAPER || k == NXG_CLASSIC_EXPR_TAPER_START || k == NXG_CLASSIC_EXPR_WIDTH_RAMP))
                 r.final_length = 1u;
-            if (b == NXG_CLASSIC_VAR_CLENGTH && k >= NXG_CLASSIC_EXPR_CLUMP_MASK &&
+            if (b == NXG_CLASSIC_VAR_CLENGTH && k >= NXG_CLASSIC_EXPR_CLUMP_MASK && !noise &&
                 (role == NXG_CLASSIC_EXPR_CLUMP || role == NXG_CLASSIC_EXPR_CLUMP_SCALE))
                 r.clump_length |= 1u << module;
         }
@@ -383,7 +393,7 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
         bool uniform = p->input_count == 0u;
         for (uint32_t i = 0; i < p->instruction_count; ++i)
             if (p->instructions[i].op == NXG_OP_RANDOM) uniform = false;
-        if (uniform) r.uniform |= 1u << k;
+        if (uniform && !noise) r.uniform |= 1u << k;  // NoiseFX programs run per strand
     }
     r.byte_size = total ? total : 16u;
     *info = r;
@@ -392,7 +402,7 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
     uint8_t *base = static_cast<uint8_t *>(out);
     std::memset(base, 0, r.byte_size);
     for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k)
-        if (r.present & (1u << k)) write_program(base, r.program_offset[k], *plan->expressions[k], plans[k]);
+        if ((r.present >> k) & 1u) write_program(base, r.program_offset[k], *plan->expressions[k], plans[k]);
     return NXG_OK;
 }
 

@@ -94,19 +94,21 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
 //   clump only:     the clumped strands (strand_count x cvs float4), and per
 //                   guide its render axis and unit noise axis (cvs float4
 //                   each) and {spline length, noise frequency-correlation ok}
-constexpr uint32_t kClassicScalarSlots = 32u;
+constexpr uint32_t kClassicScalarSlots = 40u;
 constexpr uint32_t kClassicCvTables = 1u + 2u * NXG_CLASSIC_MAX_CLUMPS;  // widthRamp; clumpScale, noiseScale per module
 constexpr uint64_t kClassicUniformBytes = (kClassicScalarSlots + kClassicCvTables * NXG_CLASSIC_MAX_CVS) * 8u;
 struct ClassicRuntimeLayout {
     uint64_t uniform_offset, state_offset, clumped_offset, render_offset, noise_offset, guide_offset, total;
 };
-inline ClassicRuntimeLayout classic_runtime_layout(uint32_t strand_count, uint32_t cvs, uint32_t guides) {
+inline ClassicRuntimeLayout classic_runtime_layout(uint32_t strand_count, uint32_t cvs, uint32_t guides,
+                                                   bool rewrites = false) {
     auto a = [](uint64_t v) { return (v + 255u) & ~uint64_t{255u}; };
     ClassicRuntimeLayout l{};
     l.uniform_offset = classic_layout(strand_count).total;
     l.state_offset = l.uniform_offset + a(kClassicUniformBytes);
     l.clumped_offset = l.state_offset + a(uint64_t{strand_count} * 32u);
-    const uint64_t clumped = guides ? uint64_t{strand_count} * cvs * 16u : 0u;
+    // the strands a clump or a NoiseFX rewrites
+    const uint64_t clumped = guides || rewrites ? uint64_t{strand_count} * cvs * 16u : 0u;
     l.render_offset = l.clumped_offset + a(clumped);
     l.noise_offset = l.render_offset + a(uint64_t{guides} * cvs * 16u);
     l.guide_offset = l.noise_offset + a(uint64_t{guides} * cvs * 16u);

@@ -149,8 +149,8 @@ def ref():
         L.ref_classic_runtime.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                           C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_float,
                                           C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_uint64,
-                                          C.c_void_p]
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_char_p, C.c_uint64, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
         L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -397,7 +397,9 @@ class _ClassicPlan(C.Structure):
     _fields_ = [("length", C.c_void_p), ("width", C.c_void_p), ("cut_amount", C.c_void_p), ("taper", C.c_void_p),
                 ("taper_start", C.c_void_p), ("width_ramp", C.c_void_p), ("has_cut", C.c_int),
                 ("description_id", C.c_uint32), ("name_seed", C.c_uint32), ("clump_count", C.c_uint32),
-                ("clump", (C.c_void_p * 7) * 3), ("binding", C.c_void_p * 3), ("clumps_after_cut", C.c_uint32)]
+                ("clump", (C.c_void_p * 7) * 3), ("binding", C.c_void_p * 3), ("clumps_after_cut", C.c_uint32),
+                ("has_noise", C.c_int), ("noise", C.c_void_p * 6), ("surface_tangents3", C.c_void_p),
+                ("noise_domain3", C.c_void_p), ("gradients", C.c_void_p)]
 
 
 def seexpr_gradients():
@@ -420,7 +422,7 @@ def _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes)
 
 def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitive_ids=None, random_prefixes=None,
                     columns=None, radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None,
-                    guide_random_prefixes=None, clumps=None):
+                    guide_random_prefixes=None, clumps=None, surface_tangents=None, noise_domain_positions=None):
     """orc_classic_runtime: the Classic runtime chain of `plan` (a
     nanoxgen_amd.classic.ClassicPlan: its compiled, bound programs) over
     packed float4 points [N * C, 4]; each clump module of the plan takes a
@@ -441,12 +443,20 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
                       name_seed=plan.name_seed, clump_count=plan.clump_count,
                       clumps_after_cut=plan.clumps_after_cut)
     for key, prog in plan.programs.items():
-        if key.startswith("clump"):
+        if key.startswith("noise_"):
+            op.noise[abi.CLASSIC_NOISE_EXPRS.index(key)] = C.addressof(prog.struct)
+        elif key.startswith("clump"):
             name, _, module = key.partition("@")
             op.clump[int(module or 0)][abi.CLASSIC_CLUMP_EXPRS.index(name)] = C.addressof(prog.struct)
         else:
             setattr(op, key, C.addressof(prog.struct))
     keepalive = []
+    if plan.has_noise:
+        noise_arrays = [_f3(surface_tangents, n), None if noise_domain_positions is None
+                        else _f3(noise_domain_positions, n), seexpr_gradients()]
+        keepalive.append(noise_arrays)
+        op.has_noise = 1
+        op.surface_tangents3, op.noise_domain3, op.gradients = [abi.ptr(x) for x in noise_arrays]
     bindings = _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes)
     for m in range(plan.clump_count):
         b = bindings[m]
@@ -482,7 +492,8 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
 
 def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None, primitive_ids=None,
                         random_prefixes=None, columns=None, radius_scale=1.0, clump_name="clump", strand_guides=None,
-                        guides=None, guide_uvs=None, guide_random_prefixes=None, clumps=None, clump_names=None):
+                        guides=None, guide_uvs=None, guide_random_prefixes=None, clumps=None, clump_names=None,
+                        surface_tangents=None, noise_domain_positions=None):
     """The reference's parse -> compile_xgen_classic_float_runtime_plan ->
     apply_xgen_classic_float_runtime_plan_cpu over the first description of
     `xgen_text`, with one geometry binding per ClumpingFX module (`clumps`
@@ -521,7 +532,10 @@ def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None,
                                      abi.ptr(uu), n, cvs, abi.ptr(ids), abi.ptr(pre), abi.ptr(cols),
                                      0 if cols is None else cols.size, radius_scale, m, name_ptrs,
                                      guide_counts.ctypes.data, abi.ptr(ga), abi.ptr(gn), abi.ptr(gt), abi.ptr(gr),
-                                     abi.ptr(gu), abi.ptr(gp), abi.ptr(sg), err, 1024, C.byref(seconds))
+                                     abi.ptr(gu), abi.ptr(gp), abi.ptr(sg),
+                                     abi.ptr(None if surface_tangents is None else _f3(surface_tangents, n)),
+                                     abi.ptr(None if noise_domain_positions is None else _f3(noise_domain_positions, n)),
+                                     err, 1024, C.byref(seconds))
     if kept == 0xFFFFFFFF:
         return dict(strand_count=None, error=err.value.decode())
     return dict(points=pts[:kept * cvs], point_counts=cc[:kept], roots=rr[:kept],

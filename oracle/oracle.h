@@ -131,6 +131,8 @@ uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, con
 #define ORC_CLASSIC_VARIABLE_COUNTS 128u
 #define ORC_CLASSIC_INVALID_GUIDE 4u
 #define ORC_CLASSIC_NONFINITE_CLUMP 256u
+#define ORC_CLASSIC_NONFINITE_NOISE_PARAMETER 512u
+#define ORC_CLASSIC_NONFINITE_NOISE 1024u
 /* The clump's geometry: world-space guides, render axes and spline lengths
  * (orc_clump_prepare), the guides' uv and SeExpr prefixes, a guide per
  * strand (0xffffffff: none). */
@@ -152,7 +154,15 @@ typedef struct orc_classic_plan {
     const nxg_expr_program *clump[3][7];
     const orc_clump_binding *binding[3];
     uint32_t clumps_after_cut; /* the last modules, which follow the CutFX */
+    int has_noise;             /* one NoiseFX after the clumps, before the CutFX */
+    /* mask, magnitude, magnitudeScale, frequency, correlation, preserveLength */
+    const nxg_expr_program *noise[6];
+    const float *surface_tangents3, *noise_domain3; /* per strand; noise_domain3 optional */
+    const nxg_vec3 *gradients;                      /* the SeExpr gradient table (with a NoiseFX) */
 } orc_classic_plan;
+void orc_noise_fx(float *points3, uint32_t n, float frequency, float correlation, float preserve,
+                  const float *domain_position3, const float *surface_normal3, const float *surface_tangent3,
+                  const float *magnitudes, const nxg_vec3 *gradients);
 void orc_seexpr_gradients(float *out); /* 256 x 3 */
 void orc_clump_noise_axis(const float *axis3, const float *distance3, uint32_t n, float raw_frequency,
                           float raw_correlation, const float *domain3, const float *normal3, const float *tangent3,

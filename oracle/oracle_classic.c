@@ -266,6 +266,40 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
     uint8_t k = 1u;
     const uint32_t before = plan->clump_count - plan->clumps_after_cut;
     for (uint32_t m = 0; m <= plan->clump_count; ++m) { /* :1680-1823, effects in object order */
+        if (m == before && plan->has_noise) { /* the NoiseFX (apply_noise, :897-1039) */
+            const nxg_expr_program *const *e = plan->noise;
+            float raw_mask, f, c, pl, magnitudes[256];
+            if (classic_eval(e[0], &ctx, vars, c_length, c_width, t_after, &raw_mask)) return ORC_CLASSIC_EXPRESSION;
+            const float mask = raw_mask < 0.0f ? 0.0f : (1.0f < raw_mask ? 1.0f : raw_mask);
+            if (mask > 1.0e-6f) {
+                if (classic_eval(e[3], &ctx, vars, c_length, c_width, t_after, &f) ||
+                    classic_eval(e[4], &ctx, vars, c_length, c_width, t_after, &c) ||
+                    classic_eval(e[5], &ctx, vars, c_length, c_width, t_after, &pl))
+                    return ORC_CLASSIC_EXPRESSION;
+                const float frequency = max_std(f, 0.0f);
+                const float cc = c * 0.01f, pp = pl * 0.01f;
+                const float correlation = cc < 0.0f ? 0.0f : (1.0f < cc ? 1.0f : cc);
+                const float preserve = pp < 0.0f ? 0.0f : (1.0f < pp ? 1.0f : pp);
+                if (!isfinite(frequency) || !isfinite(correlation) || !isfinite(preserve))
+                    return ORC_CLASSIC_NONFINITE_NOISE_PARAMETER;
+                /* the magnitudes depend on t alone along the strand; the
+                 * reference evaluates them in CV order inside its loop */
+                for (uint32_t cv = 0u; cv < cvs; ++cv) {
+                    float mg, ms;
+                    t_after = (float)cv / (float)(cvs - 1u);
+                    if (classic_eval(e[1], &ctx, vars, c_length, c_width, t_after, &mg) ||
+                        classic_eval(e[2], &ctx, vars, c_length, c_width, t_after, &ms))
+                        return ORC_CLASSIC_EXPRESSION;
+                    magnitudes[cv] = max_std(mg, 0.0f) * max_std(ms, 0.0f) * mask;
+                    if (!isfinite(magnitudes[cv])) return ORC_CLASSIC_NONFINITE_NOISE;
+                }
+                const float normal3[3] = {root->normal.x, root->normal.y, root->normal.z};
+                orc_noise_fx(&p[0].x, cvs, frequency, correlation, preserve,
+                             plan->noise_domain3 ? plan->noise_domain3 + 3u * s : NULL, normal3,
+                             plan->surface_tangents3 + 3u * s, magnitudes, plan->gradients);
+                c_length = orc_curve_spline_length_v(p, cvs);
+            }
+        }
         if (m == before && plan->has_cut) { /* the CutFX (:1806-1821) */
             float amount;
             if (classic_eval(plan->cut_amount, &ctx, vars, c_length, c_width, t_after, &amount))

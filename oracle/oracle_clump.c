@@ -247,3 +247,79 @@ uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, con
     }
     return 0u;
 }
+
+/* apply_noise's geometry (src/xgen_classic_runtime.cpp:914-1039) on n points
+ * (xyz, in place) once its expressions are evaluated: the frame transported
+ * from the surface normal along the segments, per CV the displacement of
+ * magnitudes[cv] along normal, binormal and tangent by the classic noise of
+ * the domain moved along each axis, then the preserve-length rescale. */
+void orc_noise_fx(float *points3, uint32_t n, float frequency, float correlation, float preserve,
+                  const float *domain_position3, const float *surface_normal3, const float *surface_tangent3,
+                  const float *magnitudes, const nxg_vec3 *gradients) {
+    v3 *pts = (v3 *)points3;
+    float original = 0.0f; /* curve_polyline_length */
+    for (uint32_t cv = 1u; cv < n; ++cv) original += sqrtf(len2(sub(pts[cv], pts[cv - 1u])));
+    const float effective = original > 0.0f ? max_std(0.5f / original, frequency) : frequency;
+    const float decorrelation = 1.0f - correlation;
+    const float domain_scale = 100.0f * decorrelation * decorrelation;
+    const v3 root = pts[0];
+    const v3 dp = domain_position3 ? mk(domain_position3[0], domain_position3[1], domain_position3[2]) : root;
+    const v3 domain = mul(add(dp, mk(0.419276f, 0.184247f, 0.805721f)), domain_scale);
+    const v3 surface_normal = mk(surface_normal3[0], surface_normal3[1], surface_normal3[2]);
+    v3 surface_tangent = mk(surface_tangent3[0], surface_tangent3[1], surface_tangent3[2]);
+    surface_tangent = !(len2(surface_tangent) >= 1.0e-12f) ? fallback_surface_u(surface_normal)
+                                                           : normalize(surface_tangent);
+    v3 previous_base = root, current_base = root, next_base = pts[1];
+    v3 prior = len2(surface_normal) >= 1.0e-12f ? normalize(surface_normal) : mk(0.0f, 1.0f, 0.0f);
+    v3 transported = surface_tangent;
+    float travelled = 0.0f;
+    for (uint32_t cv = 0u; cv < n; ++cv) {
+        if (cv > 0u) travelled += sqrtf(len2(sub(current_base, previous_base)));
+        v3 next = prior;
+        if (cv + 1u < n) {
+            const v3 segment = sub(next_base, current_base);
+            if (len2(segment) >= 1.0e-12f) next = normalize(segment);
+        }
+        const v3 axis = cross(prior, next);
+        const float als = len2(axis);
+        const float tangent_dot = clamp_std(dot(prior, next), -1.0f, 1.0f);
+        if (als >= 1.0e-12f) {
+            const v3 ra = mul(axis, 1.0f / sqrtf(als));
+            const float angle = acosf(tangent_dot);
+            const float cosine = cosf(angle), sine = sinf(angle);
+            const v3 rotated = add(add(mul(transported, cosine), mul(cross(ra, transported), sine)),
+                                   mul(ra, dot(ra, transported) * (1.0f - cosine)));
+            transported = len2(rotated) >= 1.0e-12f ? normalize(rotated) : mk(0.0f, 0.0f, 0.0f);
+        }
+        const v3 normal = transported;
+        v3 binormal = cross(normal, next);
+        binormal = len2(binormal) >= 1.0e-12f ? normalize(binormal) : mk(0.0f, 0.0f, 0.0f);
+        const v3 tangent = cross(binormal, normal);
+        if (cv > 0u) {
+            const float distance = travelled * effective;
+            const float m = magnitudes[cv];
+            const v3 local = mk((orc_gradient_noise_f32(gradients, domain.x + distance, domain.y, domain.z) - 0.5f) * m,
+                                (orc_gradient_noise_f32(gradients, domain.x, domain.y + distance, domain.z) - 0.5f) * m,
+                                (orc_gradient_noise_f32(gradients, domain.x, domain.y, domain.z + distance) - 0.5f) * m);
+            pts[cv] = add(add(add(current_base, mul(normal, local.x)), mul(binormal, local.y)), mul(tangent, local.z));
+        }
+        prior = next;
+        previous_base = current_base;
+        if (cv + 1u < n) {
+            current_base = next_base;
+            if (cv + 2u < n) next_base = pts[cv + 2u];
+        }
+    }
+    if (preserve > 0.001f) {
+        float noisy = 0.0f;
+        for (uint32_t cv = 1u; cv < n; ++cv) noisy += sqrtf(len2(sub(pts[cv], pts[cv - 1u])));
+        if (noisy > 0.0f) {
+            const float target = original * preserve + noisy * (1.0f - preserve);
+            if (fabsf(noisy - target) >= 0.0001f) { /* scale_curve (:576-585) */
+                const float scale = target / noisy;
+                const v3 r0 = pts[0];
+                for (uint32_t cv = 0u; cv < n; ++cv) pts[cv] = add(r0, mul(sub(pts[cv], r0), scale));
+            }
+        }
+    }
+}

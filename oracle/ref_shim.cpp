@@ -514,7 +514,8 @@ uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *cou
                              float radius_scale, uint32_t clump_count, const char *const *clump_names,
                              const uint32_t *guide_counts, const float *guide_axes3, const float *guide_normals3,
                              const float *guide_tangents3, const float *guide_reference3, const float *guide_uvs2,
-                             const uint32_t *guide_prefixes, const uint32_t *strand_guides, char *err,
+                             const uint32_t *guide_prefixes, const uint32_t *strand_guides,
+                             const float *surface_tangents3, const float *noise_domain3, char *err,
                              uint64_t errcap, double *seconds) {
     try {
         std::istringstream text(xgen_text);
@@ -576,7 +577,17 @@ uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *cou
             first += guide_count;
         }
         const auto start = std::chrono::steady_clock::now();
-        apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale, {}, pre, ids, clumps, rows);
+        std::vector<Vec3> tangents, domain;
+        if (surface_tangents3) {
+            tangents.resize(strand_count);
+            std::memcpy(tangents.data(), surface_tangents3, std::size_t{strand_count} * sizeof(Vec3));
+        }
+        if (noise_domain3) {
+            domain.resize(strand_count);
+            std::memcpy(domain.data(), noise_domain3, std::size_t{strand_count} * sizeof(Vec3));
+        }
+        apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale, tangents, pre, ids, clumps, rows,
+                                                  domain);
         if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
         std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
         std::memcpy(counts, curves.point_counts.data(), curves.point_counts.size() * sizeof(uint32_t));

@@ -24,6 +24,7 @@ import numpy as np
 PRIMITIVE = (("length", "length"), ("width", "width"), ("taper", "taper"), ("taper_start", "taperStart"),
              ("width_ramp", "widthRamp"))
 CLUMP = ("mask", "clump", "clumpScale", "noise", "noiseScale", "noiseFrequency", "noiseCorrelation")
+NOISE = ("mask", "magnitude", "magnitudeScale", "frequency", "correlation", "preserveLength")
 
 
 CASES = [
@@ -108,6 +109,25 @@ CASES = [
                       noiseScale="1", noiseFrequency="2", noiseCorrelation="30"),
                  dict(mask="1", clump="clamp($cLength, 0, 0.8)", clumpScale="rampUI(0,0,1:1,0.3,1)", noise="0",
                       noiseScale="1", noiseFrequency="1", noiseCorrelation="0")]),
+    # NoiseFX: PTEX mask, rand() magnitude through the strand's prefix, a
+    # per-CV magnitude scale, preserve length 50%; the cut and taper after it
+    # read the t of its last CV
+    dict(name="ns:noiseCut", description_id=41, strands="walk", ids=True, prefixes=True,
+         exprs=dict(taper="rampUI(0,0.1,1:1,0.6,1)", cut_amount="$cLength * 0.1 + rampUI(0,0,1:1,0.05,1)"),
+         noise=dict(mask="0.5 + 0.5 * map('n.ptx')", magnitude="0.2 + 0.1 * rand()",
+                    magnitudeScale="rampUI(0,0,1:1,1,1)", frequency="3", correlation="20", preserveLength="50")),
+    # a clump, then a NoiseFX with domain positions, masked out for some
+    # strands (their t stays where the clump left it); the generator's strands
+    dict(name="clumpNoise", description_id=42, strands="ico5", ids=True, guides=9, noise_domain=True,
+         exprs=dict(width_ramp="1 - 0.3 * $cLength"),
+         clump=dict(mask="0.8", clump="0.5", clumpScale="0", noise="0", noiseScale="1", noiseFrequency="1",
+                    noiseCorrelation="0"),
+         noise=dict(mask="$id < 80 ? 0 : 1", magnitude="0.05 * $cLength", magnitudeScale="1", frequency="1.5",
+                    correlation="0", preserveLength="0")),
+    # the NoiseFX magnitude error
+    dict(name="noiseMagnitude", description_id=43, strands="walk", exprs=dict(),
+         noise=dict(mask="1", magnitude="1e30", magnitudeScale="1e30", frequency="1", correlation="0",
+                    preserveLength="0")),
 ]
 
 
@@ -130,7 +150,8 @@ def compile_order(case):
     count = len(case_clumps(case))
     before = count - case.get("clumps_after_cut", 0)
     clump = lambda ms: tuple((f"clump{m}.{a}", a) for m in ms for a in CLUMP)
-    return PRIMITIVE + clump(range(before)) + (("cut_amount", "amount"),) + clump(range(before, count))
+    noise = tuple((f"noise.{a}", a) for a in NOISE) if "noise" in case else ()
+    return PRIMITIVE + clump(range(before)) + noise + (("cut_amount", "amount"),) + clump(range(before, count))
 
 
 def xgen_text(case):
@@ -146,6 +167,9 @@ def xgen_text(case):
     modules = list(zip(case_clumps(case), clump_names(case)))
     before = len(modules) - case.get("clumps_after_cut", 0)
     for i, module in enumerate(modules + [None]):
+        if i == before and "noise" in case:
+            s += "NoiseFXModule\n\tactive\ttrue\n\tname\tnoise\n\tmode\t0\n"
+            s += "".join(f"\t{a}\t{case['noise'][a]}\n" for a in NOISE) + "\tendAttrs\n"
         if i == before and "cut_amount" in e:
             s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{e['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
         if module is None:
@@ -166,7 +190,9 @@ def plan_sources(case):
     out, ptex, freqs = {}, 0, []
     modules = case_clumps(case)
     for key, _ in compile_order(case):
-        if key.startswith("clump"):
+        if key.startswith("noise."):
+            src = case["noise"][key[6:]]
+        elif key.startswith("clump"):
             m, attribute = key[5:].split(".")
             src = modules[int(m)][attribute]
         elif key not in case["exprs"]:
@@ -194,6 +220,8 @@ def plan_sources(case):
         out["clump"] = clumps if "clumps" in case else clumps[0]
     if case.get("clumps_after_cut"):
         out["clumps_after_cut"] = case["clumps_after_cut"]
+    if "noise" in case:
+        out["noise"] = {a: out.pop(f"noise.{a}") for a in NOISE}
     return out, ptex, freqs
 
 
@@ -246,6 +274,12 @@ def case_inputs(case, strands_of):
             guide_axes=g["axes"], guide_normals=g["normals"], guide_tangents=g["tangents"],
             guide_reference=g["reference"], guide_uvs=g["uvs"], guide_prefixes=g["prefixes"],
             strand_guides=strand_guides).items()})
+    if "noise" in case:  # surface tangents (every 9th zero: the fallback frame), domain positions
+        tangents = rng.normal(0, 1, (n, 3)).astype(np.float32)
+        tangents[::9] = 0.0
+        out["surface_tangents"] = tangents
+        if case.get("noise_domain"):
+            out["noise_domain"] = rng.uniform(-2, 2, (n, 3)).astype(np.float32)
     return out
 
 
@@ -259,4 +293,8 @@ def guide_args(inp):
                            guide_random_prefixes=x("guide_prefixes"),
                            guides=dict(axes=x("guide_axes"), normals=x("guide_normals"),
                                        tangents=x("guide_tangents"), reference=x("guide_reference"))))
-    return dict(clumps=clumps) if clumps else {}
+    out = dict(clumps=clumps) if clumps else {}
+    if "surface_tangents" in inp:
+        out["surface_tangents"] = inp["surface_tangents"]
+        out["noise_domain_positions"] = inp.get("noise_domain")
+    return out

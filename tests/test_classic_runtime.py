@@ -32,6 +32,8 @@ MESSAGES = {
     abi.CLASSIC_VARIABLE_COUNTS: "Classic runtime requires fixed point counts",
     abi.CLASSIC_INVALID_GUIDE: "Classic ClumpingFX guide index is invalid",
     abi.CLASSIC_NONFINITE_CLUMP: "Classic ClumpingFX noise magnitude is non-finite",
+    abi.CLASSIC_NONFINITE_NOISE_PARAMETER: "Classic NoiseFX produced a non-finite parameter",
+    abi.CLASSIC_NONFINITE_NOISE: "Classic NoiseFX magnitude is non-finite",
 }
 
 

@@ -29,6 +29,8 @@ MESSAGES = {
     abi.CLASSIC_VARIABLE_COUNTS: "Classic runtime requires fixed point counts",
     abi.CLASSIC_INVALID_GUIDE: "Classic ClumpingFX guide index is invalid",
     abi.CLASSIC_NONFINITE_CLUMP: "Classic ClumpingFX noise magnitude is non-finite",
+    abi.CLASSIC_NONFINITE_NOISE_PARAMETER: "Classic NoiseFX produced a non-finite parameter",
+    abi.CLASSIC_NONFINITE_NOISE: "Classic NoiseFX magnitude is non-finite",
 }
 
 
@@ -65,7 +67,7 @@ def make_plan(case):
 
 def run_gpu(torch, plan, pts, cvs, *, roots, uvs=None, counts=None, ids=None, prefixes=None, columns=None,
             radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None, guide_random_prefixes=None,
-            clumps=None):
+            clumps=None, surface_tangents=None, noise_domain_positions=None):
     if clumps is None and guides is not None:
         clumps = [dict(strand_guides=strand_guides, guides=guides, guide_uvs=guide_uvs,
                        guide_random_prefixes=guide_random_prefixes)]
@@ -77,7 +79,9 @@ def run_gpu(torch, plan, pts, cvs, *, roots, uvs=None, counts=None, ids=None, pr
                                   roots=dev(torch, np.ascontiguousarray(roots).view(np.int32).reshape(-1, 12)),
                                   root_uvs=dev(torch, uvs), point_counts=dev(torch, counts, np.int32),
                                   primitive_ids=dev(torch, ids, np.int32), random_prefixes=dev(torch, prefixes, np.int32),
-                                  columns=dev(torch, columns), radius_scale=radius_scale, clumps=bindings)
+                                  columns=dev(torch, columns), radius_scale=radius_scale, clumps=bindings,
+                                  surface_tangents=dev(torch, surface_tangents),
+                                  noise_domain_positions=dev(torch, noise_domain_positions))
     return out.to_host()
 
 
@@ -114,7 +118,8 @@ def test_reference_fixture(torch, fixture, i):
         return
     want = dict(strand_count=len(fixture[f"case{i}_out_point_counts"]), points=fixture[f"case{i}_out_points"],
                 point_counts=fixture[f"case{i}_out_point_counts"], roots=fixture[f"case{i}_out_roots"])
-    same(got, want, CASES[i]["name"], noisy_clump=any(c["noise"] != "0" for c in case_clumps(CASES[i])))
+    same(got, want, CASES[i]["name"],
+         noisy_clump="noise" in CASES[i] or any(c["noise"] != "0" for c in case_clumps(CASES[i])))
 
 
 PLANS = [
@@ -230,6 +235,43 @@ def test_hierarchy_walks_oracle(torch, orc, cvs):
         got = run_gpu(torch, plan, pts, cvs, roots=roots, ids=ids, prefixes=prefixes, columns=columns,
                       clumps=clumps[:count])
         same(got, want, f"{cvs} CVs, {count} clump levels, {after} after the cut", noisy_clump=True)
+
+
+@pytest.mark.parametrize("cvs", [2, 5, 16, 33, 64])
+def test_noise_walks_oracle(torch, orc, cvs):
+    """NoiseFX (after a clump, before a CutFX) on seeded walks: PTEX mask,
+    rand() magnitude, per-CV magnitude scale, preserve length, zero surface
+    tangents (the fallback frame), with and without domain positions;
+    GPU == oracle within the noise-frame tolerance."""
+    rng = np.random.default_rng(500 + cvs)
+    n = 64 * 2 + 21
+    steps = rng.normal(0, 0.3, size=(n, cvs, 3)).astype(np.float32)
+    pts = np.concatenate([np.cumsum(steps, 1), rng.uniform(0.001, 0.02, (n, cvs, 1)).astype(np.float32)], 2)
+    pts = pts.reshape(-1, 4).astype(np.float32)
+    roots = np.zeros(n, abi.ROOT_DTYPE)
+    roots["uv"] = rng.uniform(0, 1, (n, 2))
+    roots["normal"] = rng.normal(0, 1, (n, 3))
+    columns = rng.uniform(0.0, 1.0, (n, 2)).astype(np.float32)
+    prefixes = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    tangents = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    tangents[::7] = 0.0
+    domain = rng.uniform(-3, 3, (n, 3)).astype(np.float32)
+    g = clump_guides(6, cvs, rng)
+    sg = rng.integers(0, 6, n).astype(np.uint32)
+    clumps = [dict(strand_guides=sg, guide_uvs=g["uvs"], guide_random_prefixes=g["prefixes"],
+                   guides=dict(axes=g["axes"], normals=g["normals"], tangents=g["tangents"], reference=None))]
+    noise = dict(mask="0.3 + 0.7 * $__nxg_map_0", magnitude="0.1 + 0.2 * rand()", magnitudeScale="rampUI(0,0.2,1:1,1,1)",
+                 frequency="2 + $__nxg_map_1", correlation="30", preserveLength="40")
+    for k, with_clump in enumerate((False, True)):
+        plan = classic.ClassicPlan(noise=noise, cut_amount="$cLength * 0.15",
+                                   clump=CLUMP_PLANS[1]["clump"] if with_clump else None,
+                                   description_id=7, description_name="noise", ptex_count=2)
+        kw = dict(clumps=clumps if with_clump else None, surface_tangents=tangents,
+                  noise_domain_positions=domain if k else None)
+        want = orc.classic_runtime(pts, cvs, plan, roots=roots, random_prefixes=prefixes, columns=columns, **kw)
+        assert want["strand_count"] is not None, want["error_bits"]
+        got = run_gpu(torch, plan, pts, cvs, roots=roots, prefixes=prefixes, columns=columns, **kw)
+        same(got, want, f"{cvs} CVs, NoiseFX plan {k}", noisy_clump=True)
 
 
 def test_generated_plane(torch, orc):
