@@ -1379,7 +1379,9 @@ __device__ __forceinline__ void rebuild_staged(const CutArgs &a, const StagedCur
 // One strand of the runtime chain (src/xgen_classic_runtime.cpp:1619-1858):
 // the cut parameter of a kept strand (1 without a CutFX), -1 for a culled
 // one; *err gets the NXG_CLASSIC_* bit where the reference throws.
-template <class Curve>
+// NOISE compiles the NoiseFX step in: its frame state raises the kernel's
+// register use, so plans without a NoiseFX run a variant without it.
+template <bool NOISE, class Curve>
 __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &source, uint32_t s, uint32_t *err) {
     const uint32_t n = a.cvs;
     if (a.counts && a.counts[s] != n) {
@@ -1420,54 +1422,53 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     bool clumped = false;  // the context's t is 1: the last module with a guide clumped the strand
     bool written = false;  // a.clumped holds the strand's current CVs
     constexpr bool kStaged = std::is_same<Curve, StagedCurve<true>>::value;
-    // the clump modules before the CutFX, in order (strands of at most 64
-    // CVs, always staged)
+    // the effects in object order: the clump modules (strands of at most 64
+    // CVs, always staged), with the NoiseFX and the CutFX after the first
+    // `before` of them; one loop, so the clump step is compiled once
     const uint32_t before = a.clump_count - a.clumps_after_cut;
-    if constexpr (kStaged) {
-#pragma unroll 1
-        for (uint32_t m = 0; m < before; ++m)
-            if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
-    }
-    if constexpr (kStaged) {
-        if (a.has_noise) {  // after the clumps, before the CutFX
-            if (!runtime_noise(a, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
-        }
-    }
     float cut = 1.0f;
     bool keep = true;
-    if (a.has_cut) {
-        float amount;
-        if (!classic_value(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0u, amount, clumped)) {
-            *err = NXG_CLASSIC_EXPRESSION_ERROR;
-            return -1.0f;
-        }
-        if (!isfinite(amount)) {
-            *err = NXG_CLASSIC_NONFINITE_AMOUNT;
-            return -1.0f;
-        }
-        cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
-        keep = cut >= 1.0e-4f;
-        if constexpr (kStaged) {
-            if (a.clumps_after_cut) {  // later modules read the cut strand: rebuild it now
-                rebuild_staged(a, source, s, cut);
-                written = true;
-                cut = 1.0f;  // nothing left for the write to resample
-                c_length = spline_length(a.length_samples, source, n);
+#pragma unroll 1
+    for (uint32_t m = 0; m <= a.clump_count; ++m) {
+        if (m == before) {
+            if constexpr (kStaged && NOISE) {
+                if (a.has_noise && !runtime_noise(a, c, source, s, c_length, c_width, clumped, written, err))
+                    return -1.0f;
+            }
+            if (a.has_cut) {
+                float amount;
+                if (!classic_value(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0u, amount, clumped)) {
+                    *err = NXG_CLASSIC_EXPRESSION_ERROR;
+                    return -1.0f;
+                }
+                if (!isfinite(amount)) {
+                    *err = NXG_CLASSIC_NONFINITE_AMOUNT;
+                    return -1.0f;
+                }
+                cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
+                keep = cut >= 1.0e-4f;
+                if constexpr (kStaged) {
+                    if (a.clumps_after_cut) {  // later modules read the cut strand: rebuild it now
+                        rebuild_staged(a, source, s, cut);
+                        written = true;
+                        cut = 1.0f;  // nothing left for the write to resample
+                        c_length = spline_length(a.length_samples, source, n);
+                    }
+                }
+                // the post-effect programs read the rebuilt curve's cLength
+                if (!a.clumps_after_cut && a.final_length)
+                    c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
             }
         }
-        // the post-effect programs read the rebuilt curve's cLength
-        if (!a.clumps_after_cut && a.final_length)
-            c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
+        if constexpr (kStaged) {
+            if (m < a.clump_count && !runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err))
+                return -1.0f;
+        }
     }
     if constexpr (kStaged) {
-        if (a.clump_count || a.has_noise) {
-#pragma unroll 1
-            for (uint32_t m = before; m < a.clump_count; ++m)
-                if (!runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
-            if (!written) {  // the write reads every strand from a.clumped
-                float4 *out = a.clumped + uint64_t{s} * n;
-                for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = source[cv];
-            }
+        if ((a.clump_count || a.has_noise) && !written) {  // the write reads every strand from a.clumped
+            float4 *out = a.clumped + uint64_t{s} * n;
+            for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = source[cv];
         }
     }
     float taper = 0.0f, taper_start = 0.0f;
@@ -1512,7 +1513,7 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     return keep ? cut : -1.0f;
 }
 
-template <bool STAGED>
+template <bool STAGED, bool NOISE>
 __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
     extern __shared__ float4 tile[];
     const uint32_t n = a.cvs;
@@ -1525,9 +1526,9 @@ __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
     uint32_t err = 0u;
     if (lane < count) {
         if constexpr (STAGED)
-            cut = runtime_strand(a, StagedCurve<true>{tile, lane}, s, &err);
+            cut = runtime_strand<NOISE>(a, StagedCurve<true>{tile, lane}, s, &err);
         else
-            cut = runtime_strand(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
+            cut = runtime_strand<false>(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
         a.cut[s] = cut;
     }
     const uint64_t kept = __ballot(cut >= 0.0f);
@@ -1838,9 +1839,16 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
             hipLaunchKernelGGL(runtime_guide_kernel, dim3((a.clumps[m].guide_count + 63u) / 64u), dim3(64), 0,
                                stream, a, m);
         if (staged)
-            hipLaunchKernelGGL(runtime_search_kernel<true>, dim3(a.tile_count), dim3(kWave), tile_lds, stream, a);
+        {
+            if (a.has_noise)
+                hipLaunchKernelGGL((runtime_search_kernel<true, true>), dim3(a.tile_count), dim3(kWave), tile_lds,
+                                   stream, a);
+            else
+                hipLaunchKernelGGL((runtime_search_kernel<true, false>), dim3(a.tile_count), dim3(kWave), tile_lds,
+                                   stream, a);
+        }
         else
-            hipLaunchKernelGGL(runtime_search_kernel<false>, dim3(a.tile_count), dim3(kWave), 0, stream, a);
+            hipLaunchKernelGGL((runtime_search_kernel<false, false>), dim3(a.tile_count), dim3(kWave), 0, stream, a);
         hipLaunchKernelGGL(cut_scan_tiles_kernel, dim3(a.group_count), dim3(kScanThreads), 0, stream, a);
     }
     hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
