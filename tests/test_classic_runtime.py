@@ -205,3 +205,48 @@ def test_validate_classic_runtime():
     assert status(plan_image=fake + (1 << 19) + 4) == abi.STATUS.index("INVALID_CLASSIC_REQUEST")
     curves.roots = None
     assert status() == abi.STATUS.index("INVALID_CLASSIC_REQUEST")
+
+
+NOISE = dict(mask="1", magnitude="0.1", magnitudeScale="1", frequency="2", correlation="0", preserveLength="0")
+CLUMP = dict(mask="1", clump="0.5", clumpScale="0", noise="0", noiseScale="1", noiseFrequency="1",
+             noiseCorrelation="0")
+
+
+def test_effect_plan_rules():
+    """Plan packing: a NoiseFX needs its six expressions and sits after the
+    clumps and before the CutFX; the CutFX can precede only as many modules
+    as there are, and only when it exists."""
+    plan = classic.ClassicPlan(noise=NOISE, clump=[CLUMP, CLUMP], cut_amount="0.1")
+    assert plan.info.has_noise == 1 and plan.info.clump_count == 2
+    assert (plan.info.present >> 27) & 0x3F == 0x3F  # the six NoiseFX programs
+    assert plan.info.uniform >> 27 == 0  # NoiseFX programs run per strand
+    with pytest.raises(ValueError):
+        classic.ClassicPlan(noise={k: v for k, v in NOISE.items() if k != "preserveLength"})
+    for kw in (dict(clump=[CLUMP], cut_amount="0.1", clumps_after_cut=2),  # more than the modules
+               dict(clump=[CLUMP], clumps_after_cut=1),  # no CutFX to precede them
+               dict(clump=[CLUMP], cut_amount="0.1", clumps_after_cut=1, noise=NOISE)):  # NoiseFX after the cut
+        with pytest.raises(nx.NxgError):
+            classic.ClassicPlan(**kw)
+    assert classic.ClassicPlan(clump=[CLUMP, CLUMP, CLUMP], cut_amount="0.1", clumps_after_cut=3).info.clumps_after_cut == 3
+
+
+def test_noise_validation():
+    """A NoiseFX needs one surface tangent per strand and at most 64 CVs."""
+    L = nx.lib()
+    plan = classic.ClassicPlan(noise=NOISE)
+    n = 10
+    fake = 1 << 20  # never dereferenced by the host checks
+    out = abi.ClassicOutput(points=fake + (1 << 16), point_capacity=n * 80, kept_count=fake + (1 << 18))
+    good = dict(plan_image=fake + (1 << 19), plan=C.addressof(plan.info), radius_scale=1.0,
+                surface_tangents=fake + 8192)
+
+    def status(cvs=7, **kw):
+        curves = abi.ClassicCurves(points=fake, roots=fake + 4096, strand_count=n, cvs_per_strand=cvs)
+        d = dict(good)
+        d.update(kw)
+        return L.nxg_validate_classic_runtime(C.byref(curves), C.byref(abi.ClassicRuntimeInputs(**d)), C.byref(out))
+
+    assert status() == 0
+    assert status(surface_tangents=None) == abi.STATUS.index("INVALID_CLASSIC_REQUEST")
+    assert status(cvs=65) == abi.STATUS.index("INVALID_CLASSIC_REQUEST")
+    assert status(noise_domain_positions=fake + 12288) == 0
