@@ -538,7 +538,7 @@ def bench_classic_expr(args, with_clump=False):
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "runtime chain: cut_table + classic_uniform_kernel + "
-                          + ("runtime_guide_kernel + " if with_clump else "") + "runtime_search_kernel<true> + "
+                          + ("runtime_guide_kernel + " if with_clump else "") + "runtime_search_kernel<true, false> + "
                           "cut_scan_tiles + cut_scan_groups + runtime_write_held_kernel<16> (one C-ABI call)",
                 "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
                 "algorithmic_bytes_per_launch": alg, "valu": valu}
