@@ -441,7 +441,12 @@ CLASSIC_PLAN_CLUMP = dict(mask="1", clump="0.6 + 0.3 * rand()", clumpScale="ramp
                           noiseScale="1", noiseFrequency="2", noiseCorrelation="20")
 
 
-def classic_expr_xgen(with_clump=False):
+# The classic_noise workload's NoiseFX module (after the clump, before the cut).
+CLASSIC_NOISE = dict(mask="1", magnitude="0.05 + 0.05 * rand()", magnitudeScale="rampUI(0,0,1:1,1,1)",
+                     frequency="3", correlation="20", preserveLength="50")
+
+
+def classic_expr_xgen(with_clump=False, with_noise=False):
     """The classic_expr plan (and, with_clump, the classic_plan clump before
     the cut) as a Classic .xgen description."""
     a = CLASSIC_EXPR_AUTHORED
@@ -451,11 +456,14 @@ def classic_expr_xgen(with_clump=False):
     if with_clump:
         s += "ClumpingFXModule\n\tactive\ttrue\n\tname\tclump\n"
         s += "".join(f"\t{k}\t{v}\n" for k, v in CLASSIC_PLAN_CLUMP.items()) + "\tendAttrs\n"
+    if with_noise:
+        s += "NoiseFXModule\n\tactive\ttrue\n\tname\tnoise\n\tmode\t0\n"
+        s += "".join(f"\t{k}\t{v}\n" for k, v in CLASSIC_NOISE.items()) + "\tendAttrs\n"
     s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{a['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
     return s + "\tActive\tSplinePrimitive\n"
 
 
-def bench_classic_expr(args, with_clump=False):
+def bench_classic_expr(args, with_clump=False, with_noise=False):
     """The Classic runtime with primitive expressions
     (nxg_classic_runtime_apply) over the plane workload's generated curves:
     2,000,000 strands x 16 CVs, a length and a width expression over two
@@ -481,7 +489,8 @@ def bench_classic_expr(args, with_clump=False):
     rng = np.random.default_rng(17)
     columns = rng.uniform(0.0, 1.0, (N, 2)).astype(np.float32)
     prefixes = rng.integers(0, 2 ** 32, N, dtype=np.uint64).astype(np.uint32)
-    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, clump=CLASSIC_PLAN_CLUMP if with_clump else None, description_id=2,
+    plan = ClassicPlan(**CLASSIC_EXPR_BOUND, clump=CLASSIC_PLAN_CLUMP if with_clump else None,
+                       noise=CLASSIC_NOISE if with_noise else None, description_id=2,
                        description_name="plane", ptex_count=2)
     cols_d = torch.from_numpy(columns).cuda()
     pre_d = torch.from_numpy(prefixes.view(np.int32)).cuda()
@@ -504,6 +513,10 @@ def bench_classic_expr(args, with_clump=False):
     ws = torch.empty(runtime_workspace_bytes(N, Cv, guide_count, plan), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     log(f"generated {N} x {Cv} curves; plan image {plan.info.byte_size} B; {guide_count} clump guides")
+
+    tangents = np.tile(np.float32([1.0, 0.0, 0.0]), (N, 1))  # per-strand surface tangents of the plane
+    if with_noise:
+        clump_kw["surface_tangents"] = torch.from_numpy(tangents).cuda()
 
     def step():
         classic_runtime(gen.points, Cv, plan, roots=gen.roots, root_uvs=gen.root_uvs, point_counts=gen.point_counts,
@@ -534,22 +547,27 @@ def bench_classic_expr(args, with_clump=False):
     if with_clump:  # the strand's guide index, its guide's axes, the clumped copy written and read back
         alg += N * 4 + guide_count * Cv * (12 + 32) + N * Cv * 16 * 2
     achieved = alg / (chain_mean * 1e-3) / 1e9
-    traffic, valu = chain_counters(PLAN_PMC_SUMMARY if with_clump else RUNTIME_PMC_SUMMARY, chain_mean)
+    # (the counter summaries cover the variant without the NoiseFX step)
+    traffic, valu = (None, None) if with_noise else chain_counters(
+        PLAN_PMC_SUMMARY if with_clump else RUNTIME_PMC_SUMMARY, chain_mean)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "runtime chain: cut_table + classic_uniform_kernel + "
-                          + ("runtime_guide_kernel + " if with_clump else "") + "runtime_search_kernel<true, false> + "
+                          + ("runtime_guide_kernel + " if with_clump else "") + f"runtime_search_kernel<true, {'true' if with_noise else 'false'}> + "
                           "cut_scan_tiles + cut_scan_groups + runtime_write_held_kernel<16> (one C-ABI call)",
                 "kernel_ms_mean": round(chain_mean, 4), "kernel_ms_median": round(chain_ms[len(chain_ms) // 2], 4),
                 "algorithmic_bytes_per_launch": alg, "valu": valu}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, args.cpu_seconds, guide_host)
-    plan_text = dict(CLASSIC_EXPR_AUTHORED, **({"clump": CLASSIC_PLAN_CLUMP} if with_clump else {}))
+        cpu = classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, args.cpu_seconds, guide_host,
+                                        tangents if with_noise else None)
+    plan_text = dict(CLASSIC_EXPR_AUTHORED, **({"clump": CLASSIC_PLAN_CLUMP} if with_clump else {}),
+                     **({"noise": CLASSIC_NOISE} if with_noise else {}))
     line = {"metric": METRIC, "value": kept * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "classic-plan-plane2M" if with_clump else "classic-expressions-plane2M",
+            "config": {"workload": "classic-noise-plane2M" if with_noise else
+                       "classic-plan-plane2M" if with_clump else "classic-expressions-plane2M",
                        "input": w.name, "strands": N, "cvs_per_strand": Cv, "kept_strands": kept,
                        "clump_guides": guide_count, "plan": plan_text, "parallelism": "single-gpu"},
             "roofline": roofline, "cpu_baseline": cpu}
@@ -557,7 +575,7 @@ def bench_classic_expr(args, with_clump=False):
     return 0
 
 
-def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s, guides=None):
+def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s, guides=None, tangents=None):
     """The reference's parser, plan compiler and
     apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread
     pool) on the first S generated strands; without that build, the oracle's
@@ -568,14 +586,16 @@ def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s, guides=No
     pts, counts, roots, uvs = host["points"], host["point_counts"], host["roots"], host["root_uvs"]
     use_ref = oracle.ref() is not None
     cores = len(os.sched_getaffinity(0)) if use_ref else 1
-    text = classic_expr_xgen(guides is not None)
+    text = classic_expr_xgen(guides is not None, tangents is not None)
     plan = ClassicPlan(**CLASSIC_EXPR_BOUND, clump=CLASSIC_PLAN_CLUMP if guides is not None else None,
+                       noise=CLASSIC_NOISE if tangents is not None else None,
                        description_id=2, description_name="plane", ptex_count=2)
 
     def clump_kw(s):
+        kw = {} if tangents is None else dict(surface_tangents=tangents[:s])
         if guides is None:
-            return {}
-        return dict(strand_guides=host["stencil_indices"][:s, 0], guide_uvs=guides["uvs"],
+            return kw
+        return dict(kw, strand_guides=host["stencil_indices"][:s, 0], guide_uvs=guides["uvs"],
                     guide_random_prefixes=guides["prefixes"],
                     guides=dict(axes=guides["axes"], normals=guides["normals"], tangents=guides["tangents"]))
 
@@ -835,7 +855,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="plane",
-                    choices=["plane", "expr", "surrogate", "batch", "classic", "classic_expr", "classic_plan", "clump",
+                    choices=["plane", "expr", "surrogate", "batch", "classic", "classic_expr", "classic_plan", "classic_noise", "clump",
                              "transport"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
@@ -876,6 +896,8 @@ def main():
         return bench_classic_expr(args)
     if args.workload == "classic_plan" and args.gpus == 1:
         return bench_classic_expr(args, with_clump=True)
+    if args.workload == "classic_noise" and args.gpus == 1:
+        return bench_classic_expr(args, with_clump=True, with_noise=True)
     if args.workload == "clump" and args.gpus == 1:
         return bench_clump(args)
     if args.workload == "transport" and args.gpus == 1:
