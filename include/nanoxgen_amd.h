@@ -1048,7 +1048,9 @@ uint64_t nxg_classic_runtime_workspace_size(const nxg_classic_plan_info *plan, u
  * uvs).  Where the reference would throw, nothing is written, *kept_count is
  * 0 and *status carries the NXG_CLASSIC_* bits of every failing strand.
  * Positions and radii follow the reference's CPU operation order bit for
- * bit. */
+ * bit, except where a clump's noise axis or a NoiseFX frame evaluates
+ * acos, cos and sin (the device's acosf, cosf and sinf): there positions
+ * agree within 1e-5 of the strands' bounding-box diagonal. */
 uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves, const nxg_classic_runtime_inputs *inputs,
                                    const nxg_classic_output *output, void *workspace, uint64_t workspace_bytes,
                                    void *stream);
