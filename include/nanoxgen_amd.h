@@ -222,9 +222,9 @@ enum {
  * get messages of their own.  Never returns NULL. */
 const char *nxg_status_message(uint32_t status);
 
-/* ---- host asset builder (asset.h:304-334, 409-418; asset.cpp:229-378) --- */
+/* ---- host asset builder (asset.h:14-44, 119-125; asset.cpp:229-378) --- */
 
-/* Flattened AssetBuildInput/GuideInput (asset.h:304-334).  Optional arrays
+/* Flattened AssetBuildInput/GuideInput (asset.h:14-44).  Optional arrays
  * may be NULL.  Guide i owns guide_cv_counts[i] consecutive CVs of
  * guide_cvs; its first CV is its root. */
 typedef struct nxg_asset_build_input {

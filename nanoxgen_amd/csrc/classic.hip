@@ -1254,7 +1254,13 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, const ClassicCon
         return false;
     }
     const float mask = clamp_std(raw_mask, 0.0f, 1.0f);
-    if (!(mask > 1.0e-6f)) return true;
+    if (!(mask > 1.0e-6f)) {
+        // the curve is unchanged, but cLength is re-measured after the module
+        // whatever its mask (:1805), so a length expression's scaled value
+        // gives way to the spline length of the scaled curve
+        c_length = spline_length(a.length_samples, p, n);
+        return true;
+    }
     if (!classic_eval(a, NXG_CLASSIC_EXPR_NOISE_FREQUENCY, c, c_length, c_width, t0, raw_frequency) ||
         !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_CORRELATION, c, c_length, c_width, t0, raw_correlation) ||
         !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_PRESERVE_LENGTH, c, c_length, c_width, t0, raw_preserve)) {
@@ -1785,8 +1791,9 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.present = plan.present;
     a.has_cut = plan.has_cut;
     a.final_length = plan.final_length;
-    a.profile = (plan.present & ((1u << NXG_CLASSIC_EXPR_WIDTH) | (1u << NXG_CLASSIC_EXPR_TAPER) |
-                                 (1u << NXG_CLASSIC_EXPR_TAPER_START) | (1u << NXG_CLASSIC_EXPR_WIDTH_RAMP))) != 0u;
+    constexpr uint64_t one = 1u;
+    a.profile = (plan.present & ((one << NXG_CLASSIC_EXPR_WIDTH) | (one << NXG_CLASSIC_EXPR_TAPER) |
+                                 (one << NXG_CLASSIC_EXPR_TAPER_START) | (one << NXG_CLASSIC_EXPR_WIDTH_RAMP))) != 0u;
     a.ids = inputs->primitive_ids;
     a.prefixes = inputs->random_prefixes;
     a.columns = inputs->columns;
@@ -1857,7 +1864,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
         CutArgs w = a;
         if (a.clump_count || a.has_noise) {
             w.points = a.clumped;
-            w.present &= ~(1u << NXG_CLASSIC_EXPR_LENGTH);
+            w.present &= ~(uint64_t{1} << NXG_CLASSIC_EXPR_LENGTH);
             if (a.clumps_after_cut) w.has_cut = 0u;  // the search rebuilt the cut strands
         }
         const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);

@@ -93,10 +93,13 @@ class Program:
 ATTRIBUTE_RANGES = {"length": (0.6, 1.4), "width": (0.5, 1.5), "noise": (0.3, 1.7), "clump": (0.0, 1.0)}
 
 
-def random_attribute_program(seed, lo, hi, n_ops=40, noise_calls=4, texel_reads=2, position_extent=1.1):
+def random_attribute_program(seed, lo, hi, n_ops=40, noise_calls=4, texel_reads=2, position_extent=1.1,
+                             frequency_scale=1.0):
     """A seeded program of exactly n_ops ops (noise3 and texel calls and the
     arithmetic that feeds them included); constants and inputs are not ops.
-    Every value stays in [0, 2]; the result is mix(lo, hi, clamp(x, 0, 1))."""
+    Every value stays in [0, 2]; the result is mix(lo, hi, clamp(x, 0, 1)).
+    frequency_scale multiplies the noise3 argument scales (drawn from
+    U(0.8, 3.0)); at 1.0 the program is unchanged bit for bit."""
     rng = np.random.default_rng(seed)
     p = Program()
     pool = []  # (value index, max)
@@ -111,7 +114,7 @@ def random_attribute_program(seed, lo, hi, n_ops=40, noise_calls=4, texel_reads=
         args = []
         for axis in range(3):
             src = p.input(sources[axis] if rng.random() < 0.7 else sources[rng.integers(0, 6)])
-            args.append(p.op("multiply", src, p.const(rng.uniform(0.8, 3.0))))
+            args.append(p.op("multiply", src, p.const(rng.uniform(0.8, 3.0) * frequency_scale)))
         pool.append((p.op("noise3", *args), 1.0))
     # texcoords and normalised positions as further [0,1] leaves
     for name in ("u", "v"):
@@ -146,11 +149,12 @@ def random_attribute_program(seed, lo, hi, n_ops=40, noise_calls=4, texel_reads=
     return p.finish(p.op("mix", p.const(lo), p.const(hi), x))
 
 
-def attribute_programs(seed=3, n_ops=40, clump=False):
+def attribute_programs(seed=3, n_ops=40, clump=False, frequency_scale=1.0):
     """The per-root length, width and noise-magnitude programs (and, with
     clump=True, the clump weight) of the expression-attribute workloads."""
     names = ["length", "width", "noise"] + (["clump"] if clump else [])
-    return [random_attribute_program(seed * 1000 + k, *ATTRIBUTE_RANGES[n], n_ops=n_ops) for k, n in enumerate(names)]
+    return [random_attribute_program(seed * 1000 + k, *ATTRIBUTE_RANGES[n], n_ops=n_ops,
+                                     frequency_scale=frequency_scale) for k, n in enumerate(names)]
 
 
 def texel_map(faces, resolution=16, seed=4):

@@ -297,8 +297,8 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
                 orc_noise_fx(&p[0].x, cvs, frequency, correlation, preserve,
                              plan->noise_domain3 ? plan->noise_domain3 + 3u * s : NULL, normal3,
                              plan->surface_tangents3 + 3u * s, magnitudes, plan->gradients);
-                c_length = orc_curve_spline_length_v(p, cvs);
             }
+            c_length = orc_curve_spline_length_v(p, cvs); /* whatever the mask (:1805) */
         }
         if (m == before && plan->has_cut) { /* the CutFX (:1806-1821) */
             float amount;

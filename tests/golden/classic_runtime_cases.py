@@ -124,6 +124,14 @@ CASES = [
                     noiseCorrelation="0"),
          noise=dict(mask="$id < 80 ? 0 : 1", magnitude="0.05 * $cLength", magnitudeScale="1", frequency="1.5",
                     correlation="0", preserveLength="0")),
+    # a length expression, then a NoiseFX with no clump before it, masked
+    # out for the first 130 strands: cLength is re-measured after the module
+    # whatever its mask (the scaled value gives way to the spline length),
+    # and the cut amount reads it
+    dict(name="noiseMaskedLength", description_id=44, strands="walk", ids=True,
+         exprs=dict(length="1.37", taper="0.5", cut_amount="$cLength * 0.3"),
+         noise=dict(mask="$id < 130 ? 0 : 1", magnitude="0.04", magnitudeScale="1", frequency="2",
+                    correlation="0", preserveLength="0")),
     # the NoiseFX magnitude error
     dict(name="noiseMagnitude", description_id=43, strands="walk", exprs=dict(),
          noise=dict(mask="1", magnitude="1e30", magnitudeScale="1e30", frequency="1", correlation="0",
