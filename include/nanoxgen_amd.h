@@ -215,7 +215,8 @@ enum {
     NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL = 31,
     NXG_ERROR_INVALID_MLP_REQUEST = 32,
     NXG_ERROR_INVALID_BATCH = 33,
-    NXG_ERROR_INVALID_CLASSIC_REQUEST = 34
+    NXG_ERROR_INVALID_CLASSIC_REQUEST = 34,
+    NXG_ERROR_SCRATCH_LIMIT = 35       /* request exceeds the static scratch of the no-alloc entry point */
 };
 
 /* device_generation_error_message (device_contract.h:82-116); extension codes
@@ -305,6 +306,30 @@ uint32_t nxg_generate_packed(const nxg_device_asset *asset,
                              const nxg_launch_config *config,
                              void *workspace, uint64_t workspace_bytes,
                              void *stream);
+
+/* Bytes of the library's static device scratch, per GPU (32 MiB): the
+ * largest nxg_packed_generation_workspace_size nxg_generate_packed_noalloc
+ * accepts. */
+#define NXG_STATIC_SCRATCH_BYTES (32ull << 20)
+
+/* nxg_generate_packed without a workspace argument and without any device
+ * allocation: the workspace is the library's static device scratch of the
+ * current GPU (a file-scope device array of NXG_STATIC_SCRATCH_BYTES, held
+ * until the process ends).  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, serving loops
+ * that forbid hipMalloc).  Requests whose workspace would exceed the scratch
+ * return NXG_ERROR_SCRATCH_LIMIT before any launch.  Every call on one GPU
+ * shares its scratch, and a call leaves its contents there: calls that
+ * overlap on one GPU (different streams without ordering between them) race.
+ * The first call on a device looks up that device's scratch address. */
+uint32_t nxg_generate_packed_noalloc(const nxg_device_asset *asset,
+                                     const nxg_deformed_geometry *deformed,
+                                     const nxg_generation_params *params,
+                                     const nxg_noise_octaves *octaves,
+                                     const nxg_packed_output *output,
+                                     const nxg_stencil_output *stencil,
+                                     const nxg_launch_config *config,
+                                     void *stream);
 
 /* The two stream-ordered halves of nxg_generate_packed.  prepare fills the
  * workspace with the per-guide state and the guide offset table, which

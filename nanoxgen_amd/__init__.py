@@ -36,6 +36,7 @@ SYMBOLS = [
     ("nxg_validate_packed_generation", C.c_uint32, [_P, _P, _P, _P, _P]),
     ("nxg_packed_generation_workspace_size", C.c_uint64, [_P, _P]),
     ("nxg_generate_packed", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_generate_packed_noalloc", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P]),
     ("nxg_generate_packed_from_roots", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_prepare_packed_generation", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_generate_packed_prepared", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),

@@ -14,6 +14,7 @@ GUIDE_STENCIL_SIZE = 8
 NOISE_GRADIENT_COUNT = 256
 INVALID_INDEX = 0xFFFFFFFF
 MAX_NOISE_OCTAVES = 4
+STATIC_SCRATCH_BYTES = 32 << 20  # NXG_STATIC_SCRATCH_BYTES
 HAS_NORMALS = 1
 HAS_TEXCOORDS = 2
 
@@ -443,7 +444,7 @@ STATUS = [
     "OUTPUT_SIZE_OVERFLOW", "HIP", "WORKSPACE_TOO_SMALL", "INVALID_OCTAVES", "NULL_ARGUMENT",
     "STENCIL_CAPACITY_TOO_SMALL", "INVALID_ROOTS", "INVALID_BUILD_INPUT", "BUFFER_TOO_SMALL",
     "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL", "INVALID_MLP_REQUEST", "INVALID_BATCH",
-    "INVALID_CLASSIC_REQUEST",
+    "INVALID_CLASSIC_REQUEST", "SCRATCH_LIMIT",
 ]
 globals().update({name: i for i, name in enumerate(STATUS)})
 

@@ -158,6 +158,8 @@ const char *nxg_status_message(uint32_t status) {
         case NXG_ERROR_INVALID_BATCH: return "batch item count or tuning is invalid";
         case NXG_ERROR_INVALID_CLASSIC_REQUEST:
             return "Classic cut curves, parameters or outputs are invalid (CV count, alignment, overlap)";
+        case NXG_ERROR_SCRATCH_LIMIT:
+            return "request needs more workspace than the library's static scratch (NXG_STATIC_SCRATCH_BYTES)";
     }
     return "unknown device generation error";
 }

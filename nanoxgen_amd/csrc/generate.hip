@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/nanoxgen_amd.h"
 #include "expr_device.h"
@@ -1677,6 +1678,37 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
 }
 
 }  // namespace
+// The static scratch of nxg_generate_packed_noalloc: a file-scope device
+// array, one copy per GPU at that GPU's own address.  Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in for the
+// symbol); it uses the address hipGetSymbolAddress returns for the current
+// device, looked up once per device on its first call.
+constexpr uint64_t kStaticScratchBytes = NXG_STATIC_SCRATCH_BYTES;
+__device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+void *scratch_ptr[kMaxDevices];
+
+// The current device's scratch address, or nullptr with *status set.
+void *device_scratch(uint32_t *status) {
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) {
+        *status = NXG_ERROR_HIP;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            *status = NXG_ERROR_HIP;
+            return nullptr;
+        }
+        scratch_ptr[device] = ptr;
+    }
+    return scratch_ptr[device];
+}
+
 }  // namespace nxg
 
 extern "C" {
@@ -1728,6 +1760,23 @@ uint32_t nxg_generate_packed(const nxg_device_asset *asset, const nxg_deformed_g
                              uint64_t workspace_bytes, void *stream) {
     return nxg::launch(asset, deformed, params, octaves, nullptr, nullptr, 0, output, stencil, config, workspace,
                        workspace_bytes, stream, nxg::kPrepare | nxg::kStrands);
+}
+
+uint32_t nxg_generate_packed_noalloc(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                                     const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                                     const nxg_packed_output *output, const nxg_stencil_output *stencil,
+                                     const nxg_launch_config *config, void *stream) {
+    // the request is checked in full before the scratch is looked up or used
+    if (uint32_t s = nxg_validate_packed_generation(asset, deformed, params, output, config); s != NXG_OK) return s;
+    const uint64_t need = nxg_packed_generation_workspace_size(&asset->header, params);
+    if (need > nxg::kStaticScratchBytes) return NXG_ERROR_SCRATCH_LIMIT;
+    uint32_t status = NXG_OK;
+    void *ws = nxg::device_scratch(&status);
+    if (!ws) return status;
+    // the packed workspace needs no clearing: the guide pass and the strand
+    // order overwrite every word the strand kernel reads
+    return nxg::launch(asset, deformed, params, octaves, nullptr, nullptr, 0, output, stencil, config, ws,
+                       nxg::kStaticScratchBytes, stream, nxg::kPrepare | nxg::kStrands);
 }
 
 uint32_t nxg_prepare_packed_generation(const nxg_device_asset *asset,

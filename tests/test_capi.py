@@ -180,6 +180,22 @@ def test_generate_rejects_before_any_device_work():
                                           ws.ctypes.data, ws.nbytes, None)
     assert st == abi.INVALID_ROOTS
     assert L.nxg_packed_generation_workspace_size(C.byref(r.desc.header), C.byref(r.params)) > 16
+    # the no-alloc entry point: the contract first, then the static scratch limit
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.point_capacity = 34
+    st = L.nxg_generate_packed_noalloc(C.byref(r.desc), None, C.byref(r.params), None, C.byref(out), None, None,
+                                       None)
+    assert st == abi.POINT_CAPACITY_TOO_SMALL
+    huge = r.params.copy(strand_count=20_000_000)
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.point_capacity = huge.strand_count * huge.cvs_per_strand
+    out.root_capacity = out.root_uv_capacity = out.point_count_capacity = huge.strand_count
+    assert L.nxg_packed_generation_workspace_size(C.byref(r.desc.header), C.byref(huge)) > abi.STATIC_SCRATCH_BYTES
+    st = L.nxg_generate_packed_noalloc(C.byref(r.desc), None, C.byref(huge), None, C.byref(out), None, None, None)
+    assert st == abi.SCRATCH_LIMIT
+    with open(os.path.join(ROOT, "include", "nanoxgen_amd.h")) as f:
+        m = re.search(r"#define NXG_STATIC_SCRATCH_BYTES \((\d+)ull << (\d+)\)", f.read())
+    assert m and int(m.group(1)) << int(m.group(2)) == abi.STATIC_SCRATCH_BYTES
 
 
 def test_status_messages():
