@@ -14,6 +14,7 @@ from nanoxgen_amd import abi, workloads
 
 pytestmark = pytest.mark.gpu
 POSITION_TOL = 1e-5  # fraction of the groom bounding-box diagonal
+WEIGHT_ULPS = 2      # stencil weights: pow from the device math library
 
 
 @pytest.fixture(scope="module")
@@ -56,6 +57,16 @@ def compare(gpu, orc_out, f64=None, label=""):
     assert np.array_equal(gpu["point_counts"], orc_out["point_counts"])
     if "stencil_indices" in gpu and "stencil_indices" in orc_out:
         assert np.array_equal(gpu["stencil_indices"], orc_out["stencil_indices"]), "stencil slots differ"
+    if "stencil_weights" in gpu and "stencil_weights" in orc_out:
+        # w = pow(1 - sqrt(d2) / r, p) (generate.h:232-240): every operand is
+        # bit-exact, pow comes from the device math library, so the bar is a
+        # distance in ulps (weights are >= 0: the bit patterns order them)
+        gw = gpu["stencil_weights"].view(np.int32).astype(np.int64)
+        ow = orc_out["stencil_weights"].view(np.int32).astype(np.int64)
+        ulps = np.abs(gw - ow)
+        print(f"[{label}] stencil weights: {int((ulps == 0).sum())} of {ulps.size} bit-exact, "
+              f"max {int(ulps.max()) if ulps.size else 0} ulp")
+        assert ulps.size == 0 or ulps.max() <= WEIGHT_ULPS, "stencil weights beyond the ulp bound"
     assert np.array_equal(gpu["points"][:, 3].view(np.uint32), pts[:, 3].view(np.uint32)), "radii differ"
     err = np.abs(gpu["points"][:, :3].astype(np.float64) - pts[:, :3]).max() if len(pts) else 0.0
     msg = f"[{label}] max |gpu - oracle| = {err:.3e} (tol {POSITION_TOL * diag:.3e}, diag {diag:.3f})"
