@@ -10,8 +10,11 @@ kernel, over one synthetic description resident in HBM):
 
 Prints one JSON line: points/s, the strand kernel's HBM roofline fraction
 (algorithmic bytes / kernel time, timed with HIP events on the launch
-stream), and the CPU oracle's points/s on a bounded sample of the same
-workload on this host's cores.
+stream), the CPU oracle's points/s on a bounded sample of the same workload
+on this host's cores, and the metric's surrogate clause: the learned
+surrogate's pipeline against the exact expression runtime's on the
+1,000,000-root, 24-CV plane workload at the 1e-3 relative threshold
+("surrogate": trained outside the timed region).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -68,33 +71,48 @@ def cpu_baseline(blob, w, target_s, threads):
                       f"(oracle/), {threads} threads"}
 
 
-def bench_surrogate(args):
-    """Learned-surrogate workload: train on 2^20 exact-runtime pairs from
-    seed-disjoint roots (outside the timed region), then time the exact
-    pipeline (exact attributes -> generation) and the surrogate pipeline
-    (fused features + MLP -> sampled exact check with per-tile fallback ->
-    generation) on the same 1,000,000 roots."""
-    import numpy as np
+SURROGATE_PMC_SUMMARY = os.path.join(ROOT, "profiles", "surrogate_kernel_plane1M_pmc.json")
+SURROGATE_FLOP_PER_ROOT = 26368  # 2 x (10x64 + 3 x 64x64 + 64x4) multiply-adds of the network
+
+
+def surrogate_record(args, w=None, threshold=None, stride=None, epochs=None, steps=None, quiet=False):
+    """The learned-surrogate workload (1,000,000 roots x 24 CVs on the plane,
+    four 40-op attribute programs): train on 2^20 exact-runtime pairs from
+    seed-disjoint roots (outside the timed region), then time two pipelines
+    over the same roots, each preparing the description once:
+      exact:     prepare -> exact attributes -> strands
+      surrogate: prepare -> fused root features + MLP (triangle order) ->
+                 exact check of every stride-th root, exact re-evaluation of
+                 each 64-root tile that misses the threshold -> strands.
+    Returns the record (pipelines, kernel times, served tiles, errors of the
+    served roots against the exact runtime, training)."""
     import torch
     import nanoxgen_amd as nx
     from nanoxgen_amd import expr, surrogate, workloads
 
-    w = workloads.config4()
+    w = w or workloads.config4()
+    threshold = args.threshold if threshold is None else threshold
+    stride = args.sample_stride if stride is None else stride
+    epochs = args.train_epochs if epochs is None else epochs
+    steps = args.steps if steps is None else steps
     N, Cv = w.params.strand_count, w.params.cvs_per_strand
     blob = nx.build_asset(w.build)
     asset = nx.DeviceAsset(blob)
     es, maps = expr.ExprSet(w.info["programs"]), expr.TexelMaps(w.info["maps"])
-    train_p = w.params.copy(seed=w.info["train_seed"], strand_count=1 << 20)
+    pairs = 1 << 20
+    train_p = w.params.copy(seed=w.info["train_seed"], strand_count=pairs)
     t0 = time.perf_counter()
     feats = surrogate.root_features(asset, train_p, maps)
-    targets = torch.empty((1 << 20, 4), device="cuda")
+    targets = torch.empty((pairs, 4), device="cuda")
     expr.evaluate_root_attributes(asset, train_p, es, maps, targets)
     net = surrogate.Surrogate(seed=1)
     net.fit_norm(feats, targets)
-    hist = net.train(feats, targets, epochs=args.train_epochs, batch=4096, lr=3e-3, log_every=256)
+    hist = net.train(feats, targets, epochs=epochs, batch=4096, lr=3e-3, log_every=256)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    log(f"trained {len(hist)} log points, loss {hist[0][1]:.4f} -> {hist[-1][1]:.5f} in {train_s:.1f} s")
+    del feats, targets
+    if not quiet:
+        log(f"trained on {pairs} pairs, {epochs} epochs: loss {hist[0][1]:.4f} -> {hist[-1][1]:.2e} in {train_s:.1f} s")
 
     out = nx.PackedOutputs(N, Cv, roots=False, root_uvs=False, point_counts=True)
     ws = torch.empty(asset.workspace_bytes(w.params), dtype=torch.uint8, device="cuda")
@@ -103,106 +121,135 @@ def bench_surrogate(args):
     stats = torch.zeros(2, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
 
-    def exact_step():
+    def exact_attrs():
         expr.evaluate_root_attributes(asset, w.params, es, maps, attrs, stream=stream)
-        nx.generate_packed(asset, w.params, out, octaves=w.octaves, workspace=ws, stream=stream, attributes=attrs)
 
-    def surrogate_step():
-        surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream)
-        surrogate.verify_attributes(asset, w.params, es, maps, attrs, args.threshold, args.sample_stride, flags,
-                                    stats, stream=stream)
-        nx.generate_packed(asset, w.params, out, octaves=w.octaves, workspace=ws, stream=stream, attributes=attrs)
+    def surrogate_attrs():
+        surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream, prepared=ws)
+
+    def verify():
+        surrogate.verify_attributes(asset, w.params, es, maps, attrs, threshold, stride, flags, stats, stream=stream)
+
+    def pipeline(fns):
+        def step():
+            nx.prepare_packed(asset, w.params, ws, stream=stream)
+            for fn in fns:
+                fn()
+            nx.generate_packed_prepared(asset, w.params, out, ws, octaves=w.octaves, stream=stream, attributes=attrs)
+        return step
 
     def timed(fn):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             fn()
         torch.cuda.synchronize()
-        return (time.perf_counter() - t) * 1e3 / args.steps
+        return (time.perf_counter() - t) * 1e3 / steps
 
-    def attr_ms(fn_attr):
+    def event_ms(fn, reps=7):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ms = []
-        for _ in range(5):
+        for _ in range(reps):
             ev[0].record(stream)
-            fn_attr()
+            fn()
             ev[1].record(stream)
             torch.cuda.synchronize()
             ms.append(ev[0].elapsed_time(ev[1]))
-        return sorted(ms)[2]
+        return sorted(ms)[reps // 2]
 
-    exact_ms = timed(exact_step)
-    exact_attrs = attrs.clone()
-    exact_points = out.points.clone()
-    sur_ms = timed(surrogate_step)
+    exact_ms = timed(pipeline([exact_attrs]))
+    exact = attrs.clone()
+    sur_ms = timed(pipeline([surrogate_attrs, verify]))
     torch.cuda.synchronize()
-    rel = ((attrs - exact_attrs).abs() / exact_attrs.abs().clamp_min(1e-6)).max(1).values
     tiles_redone, samples = stats.tolist()
-    ex_attr_ms = attr_ms(lambda: expr.evaluate_root_attributes(asset, w.params, es, maps, attrs, stream=stream))
-    su_attr_ms = attr_ms(lambda: (surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream),
-                                  surrogate.verify_attributes(asset, w.params, es, maps, attrs, args.threshold,
-                                                              args.sample_stride, flags, stats, stream=stream)))
-    mlp_ms = attr_ms(lambda: surrogate.surrogate_attributes(asset, w.params, net, maps, attrs, stream=stream))
     counts_exact = bool((out.point_counts == Cv).all())
-    # Where the surrogate's error comes from, on the first 65,536 roots: the
-    # trained weights in float64 arithmetic (no bf16 rounding) against the
-    # exact attributes, and the bf16 device network against that float64
-    # network (the rounding floor of bf16 weights and activations).
-    k = min(N, 65536)
-    feats = surrogate.root_features(asset, w.params, maps)[:k].contiguous()
-    flat = torch.from_numpy(net.params()).double().cuda()
-    norm = torch.from_numpy(net.norm).double().cuda()
-    h = torch.zeros((k, surrogate.IN_PAD), dtype=torch.float64, device="cuda")
-    h[:, :surrogate.INPUTS] = (feats.double() - norm[:10]) * norm[10:20]
-    at = 0
-    for i, (r, c) in enumerate(surrogate.layer_shapes()):
-        W = flat[at:at + r * c].view(r, c)
-        b = flat[at + r * c:at + r * c + c]
-        at += r * c + c
-        z = h @ W + b
-        h = z * torch.sigmoid(z) if i < surrogate.LAYERS else z
-    y64 = h[:, :surrogate.OUTPUTS] * norm[20:24] + norm[24:28]
-    yb16 = net.forward(feats).double()
-    ex = exact_attrs[:k].double()
-    rel64 = ((y64 - ex).abs() / ex.abs().clamp_min(1e-6)).max(1).values
-    relb16 = ((yb16 - y64).abs() / y64.abs().clamp_min(1e-6)).max(1).values
-    q = torch.tensor([0.5, 0.99], dtype=torch.float64, device="cuda")
-    error_sources = {"roots": k,
-                     "float64_network_vs_exact_rel_p50_p99": [float(v) for v in torch.quantile(rel64, q)],
-                     "bf16_network_vs_float64_network_rel_p50_p99": [float(v) for v in torch.quantile(relb16, q)],
-                     "float64_network_roots_within_threshold": float((rel64 <= args.threshold).double().mean()),
-                     "bf16_rounding_roots_within_threshold": float((relb16 <= args.threshold).double().mean())}
+    rel = ((attrs - exact).abs() / exact.abs().clamp_min(1e-6)).max(1).values
+    served = torch.repeat_interleave(flags == 0, 64)[:N]
+    rel_served = rel[served]
+    # the attribute phase alone, with the workspace prepared
+    nx.prepare_packed(asset, w.params, ws, stream=stream)
+    ex_attr_ms = event_ms(exact_attrs)
+    mlp_ms = event_ms(surrogate_attrs)
+    surrogate_attrs()
+    verify_ms = event_ms(lambda: (surrogate_attrs(), verify())) - mlp_ms
+    # before the gate: the network's own error over every root
+    surrogate_attrs()
+    torch.cuda.synchronize()
+    rel_net = ((attrs - exact).abs() / exact.abs().clamp_min(1e-6)).max(1).values
+    q = torch.tensor([0.5, 0.99], device="cuda")
     points = N * Cv
-    line = {"metric": METRIC, "value": points / (sur_ms * 1e-3), "unit": "points/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sur_ms, 4),
+    rec = {"workload": w.name, "frequency_scale": w.info.get("frequency_scale"), "threshold": threshold,
+           "sample_stride": stride,
+           "exact_pipeline_points_per_s": points / (exact_ms * 1e-3), "exact_pipeline_ms": round(exact_ms, 4),
+           "surrogate_pipeline_points_per_s": points / (sur_ms * 1e-3), "surrogate_pipeline_ms": round(sur_ms, 4),
+           "speedup_vs_exact": round(exact_ms / sur_ms, 4),
+           "attributes_ms_exact": round(ex_attr_ms, 4), "surrogate_kernel_ms": round(mlp_ms, 4),
+           "check_and_fallback_ms": round(verify_ms, 4),
+           "tiles": int(flags.numel()), "tiles_served": int(flags.numel()) - tiles_redone,
+           "tiles_reevaluated": tiles_redone, "samples_checked": samples,
+           "served_roots": int(served.sum().item()),
+           "served_roots_within_threshold": float((rel_served <= threshold).float().mean().item())
+           if rel_served.numel() else None,
+           "served_roots_max_rel_error": float(rel_served.max().item()) if rel_served.numel() else None,
+           "network_rel_error_p50_p99": [float(v) for v in torch.quantile(rel_net[:1 << 20], q)],
+           "network_roots_within_threshold": float((rel_net <= threshold).float().mean().item()),
+           "point_counts_exact": counts_exact,
+           "train_pairs": pairs, "train_epochs": epochs, "train_seconds": round(train_s, 2),
+           "train_loss_first_last": [hist[0][1], hist[-1][1]]}
+    return rec
+
+
+def bench_surrogate(args):
+    """The learned-surrogate workload's own line: value is the surrogate
+    pipeline's points/s; the roofline object is the surrogate kernel's
+    (FLOPs of the network at the BF16 MFMA peak, with the VALU issue rate of
+    its SiLU activations beside it from the profiled counters)."""
+    rec = surrogate_record(args)
+    N = 1_000_000
+    mlp_ms = rec["surrogate_kernel_ms"]
+    flops = SURROGATE_FLOP_PER_ROOT * N / (mlp_ms * 1e-3) / 1e12
+    valu = None
+    if os.path.exists(SURROGATE_PMC_SUMMARY):
+        with open(SURROGATE_PMC_SUMMARY) as f:
+            insts = json.load(f).get("sq", {}).get("SQ_INSTS_VALU")
+        if insts:
+            rate = insts / (mlp_ms * 1e-3) / 1e9
+            valu = {"achieved": round(rate, 2), "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                    "frac": round(rate / VALU_PEAK_GINST, 4), "instructions_per_launch": insts}
+    line = {"metric": METRIC, "value": rec["surrogate_pipeline_points_per_s"], "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": rec["surrogate_pipeline_ms"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (surrogate: bf16 mfma)",
             "data": "synthetic",
-            "config": {"workload": w.name, "strands": N, "cvs_per_strand": Cv, "attribute_programs": 4,
-                       "threshold": args.threshold, "sample_stride": args.sample_stride,
+            "config": {"workload": rec["workload"], "strands": N, "cvs_per_strand": 24, "attribute_programs": 4,
+                       "threshold": rec["threshold"], "sample_stride": rec["sample_stride"],
                        "parallelism": "single-gpu"},
-            "surrogate": {"exact_pipeline_points_per_s": points / (exact_ms * 1e-3),
-                          "exact_pipeline_ms": round(exact_ms, 4),
-                          "speedup_vs_exact": round(exact_ms / sur_ms, 4),
-                          "attributes_ms_exact": round(ex_attr_ms, 4),
-                          "attributes_ms_surrogate_plus_check": round(su_attr_ms, 4),
-                          "mlp_inference_ms": round(mlp_ms, 4),
-                          "tiles_reevaluated": tiles_redone, "tiles": int(flags.numel()),
-                          "samples_checked": samples,
-                          "max_rel_attribute_error": float(rel.max().item()),
-                          "roots_over_threshold": int((rel > args.threshold).sum().item()),
-                          "point_counts_exact": counts_exact,
-                          "train_pairs": 1 << 20, "train_seconds": round(train_s, 2),
-                          "train_loss_first_last": [hist[0][1], hist[-1][1]],
-                          "error_sources": error_sources},
+            "surrogate": rec,
             "roofline": {"bound": "mfma", "kernel": "surrogate_kernel (root features + MLP)",
-                         "achieved": round(26368.0 * N / (mlp_ms * 1e-3) / 1e12, 3), "peak": 2500.0,
-                         "unit": "TFLOP/s", "frac": round(26368.0 * N / (mlp_ms * 1e-3) / 1e12 / 2500.0, 6),
-                         "traffic": None, "flop_per_root": 26368},
+                         "achieved": round(flops, 3), "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(flops / 2500.0, 6), "traffic": None,
+                         "flop_per_root": SURROGATE_FLOP_PER_ROOT, "kernel_ms": mlp_ms, "valu": valu},
             "cpu_baseline": None}
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_surrogate_sweep(args):
+    """Acceptance and points/s of the surrogate pipeline against the exact
+    one over relative thresholds 1e-3 .. 3e-2 and the programs' noise
+    frequency scale 0.03 .. 1 (one trained network per scale); one JSON line
+    per (scale, threshold)."""
+    from nanoxgen_amd import workloads
+    for fs in (0.03, 0.1, 0.3, 1.0):
+        w = workloads.config4(frequency_scale=fs)
+        for thr in (1e-3, 3e-3, 1e-2, 3e-2):
+            rec = surrogate_record(args, w=w, threshold=thr, stride=64, steps=5, quiet=True)
+            keep = ("frequency_scale", "threshold", "speedup_vs_exact", "exact_pipeline_points_per_s",
+                    "surrogate_pipeline_points_per_s", "tiles_served", "tiles", "served_roots_within_threshold",
+                    "served_roots_max_rel_error", "network_roots_within_threshold", "network_rel_error_p50_p99",
+                    "train_loss_first_last")
+            print(json.dumps({k: rec[k] for k in keep}), flush=True)
     return 0
 
 
@@ -855,12 +902,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="plane",
-                    choices=["plane", "expr", "surrogate", "batch", "classic", "classic_expr", "classic_plan", "classic_noise", "clump",
+                    choices=["plane", "expr", "surrogate", "surrogate_sweep", "batch", "classic", "classic_expr", "classic_plan", "classic_noise", "clump",
                              "transport"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
                          "the trained surrogate + sampled exact check against the exact runtime; "
+                         "surrogate_sweep: that comparison over thresholds and program noise frequencies; "
                          "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph; "
                          "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands; "
                          "classic_expr: the same strands through a Classic plan of length, width, taper, "
@@ -869,8 +917,9 @@ def main():
                          "clump: ClumpingFX toward the plane's 4,096 guides over the same strands; "
                          "transport: 64K renderer batches (transform, motion) and the .nxc cache of the same strands")
     ap.add_argument("--threshold", type=float, default=1e-3, help="surrogate: relative attribute error bound")
-    ap.add_argument("--sample-stride", type=int, default=16, help="surrogate: exact check of every n-th root")
-    ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--sample-stride", type=int, default=64, help="surrogate: exact check of every n-th root")
+    ap.add_argument("--train-epochs", type=int, default=200, help="surrogate: epochs over the 2^20 training pairs")
+    ap.add_argument("--no-surrogate", action="store_true", help="plane: leave out the surrogate record")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="instead of the benchmark, run a plain device copy of this many MiB "
@@ -888,6 +937,8 @@ def main():
         return 0
     if args.workload == "surrogate" and args.gpus == 1:
         return bench_surrogate(args)
+    if args.workload == "surrogate_sweep" and args.gpus == 1:
+        return bench_surrogate_sweep(args)
     if args.workload == "batch" and args.gpus == 1:
         return bench_batch(args)
     if args.workload == "classic" and args.gpus == 1:
@@ -988,6 +1039,12 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(blob, w, args.cpu_seconds, threads)
+    # the metric's surrogate clause: the learned-surrogate workload beside the
+    # exact runtime at the stated threshold (its own timed steps)
+    sur = None
+    if args.workload == "plane" and not args.no_surrogate:
+        del out, ws
+        sur = surrogate_record(args)
     line = {"metric": METRIC, "value": points / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -998,7 +1055,7 @@ def main():
                        "octaves": ([[w.octaves.amplitude[i], w.octaves.frequency[i]] for i in range(w.octaves.count)]
                                    if w.octaves else [[w.params.noise_amplitude, w.params.noise_frequency]]),
                        "attribute_programs": len(w.info.get("programs", [])), "parallelism": "single-gpu"},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu, "surrogate": sur}
     print(json.dumps(line), flush=True)
     return 0
 

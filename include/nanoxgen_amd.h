@@ -558,18 +558,34 @@ uint32_t nxg_generate_packed_attributed(const nxg_device_asset *asset,
                                         void *workspace, uint64_t workspace_bytes,
                                         void *stream);
 
+/* The strand half of nxg_generate_packed_attributed, from a workspace that
+ * nxg_prepare_packed_generation filled (the same rule as
+ * nxg_generate_packed_prepared). */
+uint32_t nxg_generate_packed_attributed_prepared(const nxg_device_asset *asset,
+                                                 const nxg_deformed_geometry *deformed,
+                                                 const nxg_generation_params *params,
+                                                 const nxg_noise_octaves *octaves,
+                                                 const float *attributes, uint64_t attribute_capacity,
+                                                 const nxg_packed_output *output,
+                                                 const nxg_stencil_output *stencil,
+                                                 const nxg_launch_config *config,
+                                                 void *workspace, uint64_t workspace_bytes,
+                                                 void *stream);
+
 /* ---- attribute surrogate (extension, no reference counterpart) --------- */
 
 /* A 10 -> 64 -> 64 -> 64 -> 64 -> 4 MLP with SiLU hidden activations, BF16
- * weights and activations, FP32 accumulation (gfx950 MFMA).  Inputs are the
- * root features written by nxg_root_features; outputs are the NXG_ATTR_*
- * attributes.  All state lives in one caller-owned device buffer laid out by
- * nxg_mlp_get_layout: FP32 master parameters (input-major padded weights
- * 16x64, 64x64 x3, 64x16, each followed by its bias), Adam moments, the BF16
- * parameter copy, normalisation (input mean[10], input 1/std[10], output
- * scale[4], output offset[4]), the last loss and per-block gradient
- * partials.  The caller uploads parameters and normalisation, then calls
- * nxg_mlp_sync. */
+ * weights, BF16 activations carried as two terms (hi = bf16(h),
+ * lo = bf16(h - hi), two MFMAs per product), FP32 accumulation (gfx950
+ * v_mfma_f32_16x16x32_bf16).  Inputs are the root features written by
+ * nxg_root_features; outputs are the NXG_ATTR_* attributes.  All state lives
+ * in one caller-owned device buffer laid out by nxg_mlp_get_layout: FP32
+ * master parameters (input-major padded weights 16x64, 64x64 x3, 64x16, each
+ * followed by its bias), Adam moments, the BF16 parameter copy, the BF16
+ * weights in MFMA operand order, normalisation (input mean[10], input
+ * 1/std[10], output scale[4], output offset[4]), the last loss and per-block
+ * gradient partials.  The caller uploads parameters and normalisation, then
+ * calls nxg_mlp_sync; every training step refreshes both BF16 copies. */
 #define NXG_MLP_INPUTS 10u
 #define NXG_MLP_HIDDEN 64u
 #define NXG_MLP_HIDDEN_LAYERS 4u
@@ -582,6 +598,7 @@ typedef struct nxg_mlp_layout {
     uint64_t adam_m_offset;
     uint64_t adam_v_offset;
     uint64_t params_bf16_offset;
+    uint64_t fragments_offset; /* BF16 weights in MFMA operand order (inference) */
     uint64_t norm_offset;
     uint64_t loss_offset;
     uint64_t partials_offset; /* [batch/64][param_count + 1] FP32 */
@@ -591,7 +608,8 @@ typedef struct nxg_mlp_layout {
 
 void nxg_mlp_get_layout(nxg_mlp_layout *layout);
 
-/* Refresh the BF16 parameter copy from the FP32 master parameters. */
+/* Refresh both BF16 copies (flat and MFMA operand order) from the FP32
+ * master parameters. */
 uint32_t nxg_mlp_sync(void *state, void *stream);
 
 /* Forward + backward on samples [batch_first, batch_first + batch_size)
@@ -619,10 +637,16 @@ uint32_t nxg_root_features(const nxg_device_asset *asset, const nxg_generation_p
                            const nxg_texel_map *map, float *features, uint64_t capacity,
                            void *stream);
 
-/* Fused root features + MLP: attributes[4*s + k] predicted for strand s. */
+/* Fused root features + MLP: attributes[4*s + k] predicted for strand s.
+ * prepared_workspace (optional, with its byte size): a workspace that
+ * nxg_prepare_packed_generation filled for the same asset, seed and
+ * strand_count; the roots are then processed in its order (grouped by root
+ * triangle, so a wavefront's feature gathers share cache lines), which
+ * changes no value. */
 uint32_t nxg_surrogate_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
                                   const void *state, const nxg_texel_map *map, float *attributes,
-                                  uint64_t attribute_capacity, void *stream);
+                                  uint64_t attribute_capacity, const void *prepared_workspace,
+                                  uint64_t workspace_bytes, void *stream);
 
 /* The exact runtime as judge: evaluates the set's programs exactly for every
  * sample_stride-th root (strands s with s % stride == stride / 2); a 64-root

@@ -48,13 +48,15 @@ SYMBOLS = [
     ("nxg_expr_pack", C.c_uint32, [_P, C.c_uint32, _P, C.c_uint64, _P]),
     ("nxg_evaluate_root_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, _P, _P]),
     ("nxg_generate_packed_attributed", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_generate_packed_attributed_prepared", C.c_uint32,
+     [_P, _P, _P, _P, _P, C.c_uint64, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_mlp_get_layout", None, [_P]),
     ("nxg_mlp_sync", C.c_uint32, [_P, _P]),
     ("nxg_mlp_gradients", C.c_uint32, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, _P]),
     ("nxg_mlp_train_step", C.c_uint32, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint32, C.c_float, C.c_uint32, _P]),
     ("nxg_mlp_forward", C.c_uint32, [_P, _P, _P, C.c_uint64, _P]),
     ("nxg_root_features", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
-    ("nxg_surrogate_attributes", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_surrogate_attributes", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P, C.c_uint64, _P]),
     ("nxg_verify_attributes", C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, C.c_uint64, C.c_float, C.c_uint32,
                                            _P, C.c_uint64, _P, _P]),
     ("nxg_validate_motion_generation", C.c_uint32, [_P, _P, C.c_uint32, _P, _P, _P]),
@@ -312,10 +314,18 @@ def prepare_packed(asset, params, workspace, *, deformed=None, stream=None):
                                               _stream_handle(stream)), "nxg_prepare_packed_generation")
 
 
-def generate_packed_prepared(asset, params, outputs, workspace, *, octaves=None, deformed=None, stream=None):
-    """Second half of generate_packed: the strand kernel from a prepared workspace."""
+def generate_packed_prepared(asset, params, outputs, workspace, *, octaves=None, deformed=None, stream=None,
+                             attributes=None):
+    """Second half of generate_packed: the strand kernel from a prepared
+    workspace (`attributes`: the per-root attribute multipliers)."""
     out = outputs.descriptor()
     st = outputs.stencil_descriptor()
+    if attributes is not None:
+        check(lib().nxg_generate_packed_attributed_prepared(
+            C.byref(asset.desc), _ref(deformed), C.byref(params), _ref(octaves), C.c_void_p(attributes.data_ptr()),
+            attributes.numel(), C.byref(out), _ref(st), None, C.c_void_p(workspace.data_ptr()), workspace.numel(),
+            _stream_handle(stream)), "nxg_generate_packed_attributed_prepared")
+        return outputs
     check(lib().nxg_generate_packed_prepared(C.byref(asset.desc), _ref(deformed), C.byref(params), _ref(octaves),
                                              C.byref(out), _ref(st), None, C.c_void_p(workspace.data_ptr()),
                                              workspace.numel(), _stream_handle(stream)),

@@ -1081,6 +1081,7 @@ struct AttrArgs {
     float threshold;        // kAttrCheck: relative error bound
     uint32_t *tile_flags;   // one word per 64-root tile
     uint32_t *stats;        // [0] tiles re-evaluated, [1] samples checked
+    uint32_t split;         // one program per grid row (blockIdx.y), only its attribute written
 };
 
 enum AttrMode : uint32_t { kAttrAll = 0u, kAttrCheck = 1u, kAttrFallback = 2u };
@@ -1143,7 +1144,11 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
     float result[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     // the reference rejects a non-finite context (xgen_expression.cpp:991-994)
     bool bad = !isfinite(r.u) || !isfinite(r.v);
-    for (uint32_t k = 0; k < a.program_count; ++k) {
+    // the programs of this launch row: all of them, or with `split` the one
+    // of blockIdx.y (the verification passes run few roots; one program per
+    // wavefront shortens each wavefront's interpreter pass)
+    const uint32_t k_first = a.split ? blockIdx.y : 0u, k_end = a.split ? blockIdx.y + 1u : a.program_count;
+    for (uint32_t k = k_first; k < k_end; ++k) {
         // the image is read-only for the whole launch: reading it through
         // the constant address space lets every (wave-uniform) fetch be a
         // scalar load
@@ -1201,15 +1206,22 @@ __global__ void __launch_bounds__(256) attribute_kernel(AttrArgs a) {
         bool fail = bad;
 #pragma unroll
         for (uint32_t k = 0; k < NXG_EXPR_MAX_PROGRAMS; ++k) {
-            if (k >= a.program_count) break;
+            if (k < k_first || k >= k_end) continue;
             const float e = result[k], d = guess[k] - e;
             const float mag = (e < 0.0f ? -e : e) > 1.0e-6f ? (e < 0.0f ? -e : e) : 1.0e-6f;
             if (!((d < 0.0f ? -d : d) <= a.threshold * mag)) fail = true;
         }
-        if (a.stats) atomicAdd(&a.stats[1], 1u);
+        if (a.stats && k_first == 0u) atomicAdd(&a.stats[1], 1u);
         if (fail && atomicOr(&a.tile_flags[strand / 64u], 1u) == 0u && a.stats) atomicAdd(&a.stats[0], 1u);
     }
-    a.attributes[strand] = make_float4(result[0], result[1], result[2], result[3]);
+    if (a.split) {
+        float *row = reinterpret_cast<float *>(a.attributes + strand);
+#pragma unroll
+        for (uint32_t k = 0; k < NXG_EXPR_MAX_PROGRAMS; ++k)
+            if (k == k_first) row[k] = result[k];
+    } else {
+        a.attributes[strand] = make_float4(result[0], result[1], result[2], result[3]);
+    }
 }
 
 __global__ void clear_words_kernel(uint32_t *words, uint64_t count, uint32_t *stats) {
@@ -1489,6 +1501,7 @@ uint32_t make_attr_launch(const nxg_device_asset *asset, const nxg_generation_pa
     out.registers = set->max_values <= kRegSlots;
     out.lds = (out.registers ? 0u : set->max_values * block * 4u) + 4u * 4u * NXG_NOISE_GRADIENT_COUNT;
     out.lanes = mode == kAttrCheck ? (uint64_t{params->strand_count} + stride - 1u) / stride : params->strand_count;
+    a.split = mode != kAttrAll ? 1u : 0u;
     return NXG_OK;
 }
 
@@ -1498,7 +1511,8 @@ uint32_t launch_attributes(const AttrLaunch &l, hipStream_t stream) {
         hipFuncSetAttribute(reinterpret_cast<const void *>(&attribute_kernel<false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(l.lds)) != hipSuccess)
         return NXG_ERROR_HIP;
-    const dim3 grid(static_cast<uint32_t>((l.lanes + l.block - 1u) / l.block)), block(l.block);
+    const dim3 grid(static_cast<uint32_t>((l.lanes + l.block - 1u) / l.block), l.a.split ? l.a.program_count : 1u),
+        block(l.block);
     if (l.registers) hipLaunchKernelGGL(attribute_kernel<true>, grid, block, l.lds, stream, l.a);
     else hipLaunchKernelGGL(attribute_kernel<false>, grid, block, l.lds, stream, l.a);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
@@ -1751,6 +1765,19 @@ uint32_t nxg_generate_packed_attributed(const nxg_device_asset *asset, const nxg
     if (!attributes) return NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL;
     return nxg::launch(asset, deformed, params, octaves, nullptr, attributes, attribute_capacity, output,
                        stencil, config, workspace, workspace_bytes, stream, nxg::kPrepare | nxg::kStrands);
+}
+
+uint32_t nxg_generate_packed_attributed_prepared(const nxg_device_asset *asset,
+                                                 const nxg_deformed_geometry *deformed,
+                                                 const nxg_generation_params *params,
+                                                 const nxg_noise_octaves *octaves, const float *attributes,
+                                                 uint64_t attribute_capacity, const nxg_packed_output *output,
+                                                 const nxg_stencil_output *stencil,
+                                                 const nxg_launch_config *config, void *workspace,
+                                                 uint64_t workspace_bytes, void *stream) {
+    if (!attributes) return NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL;
+    return nxg::launch(asset, deformed, params, octaves, nullptr, attributes, attribute_capacity, output, stencil,
+                       config, workspace, workspace_bytes, stream, nxg::kStrands);
 }
 
 uint32_t nxg_generate_packed(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,

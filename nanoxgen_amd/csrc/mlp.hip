@@ -1,5 +1,5 @@
 // Attribute surrogate: a 10 -> 64 -> 64 -> 64 -> 64 -> 4 SiLU MLP on gfx950
-// MFMA (v_mfma_f32_16x16x16_bf16: BF16 operands, FP32 accumulation), with
+// MFMA (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation), with
 // its training step (forward, backward, deterministic gradient reduction,
 // Adam on FP32 master weights), the fused root-feature + inference kernel,
 // and the sampled exact check that sends failing tiles of roots back to the
@@ -7,9 +7,24 @@
 // has the exact runtime (src/xgen_expression.cpp:984-1148), which stays the
 // fallback and the arbiter.
 //
-// A block of 256 lanes (four wavefronts) owns a tile of 64 samples; every
-// layer is a 64 x N x K product split into 16 x 16 output tiles, four per
-// wavefront.
+// Numerics.  Weights are BF16 (rounded from the FP32 masters; training sees
+// the rounded weights, so it learns around their rounding).  Inputs and
+// hidden activations are carried as a pair of BF16 terms, hi = bf16(h) and
+// lo = bf16(h - hi), and every product is two MFMAs, W.hi + W.lo, into one
+// FP32 accumulator: the activations keep about 16 significant bits instead
+// of 8 (Henry, Tang & Heinecke 2019, "Leveraging the bfloat16 artificial
+// intelligence datatype for higher-precision computations").  Biases are
+// FP32.
+//
+// Inference (surrogate_kernel, mlp_forward_kernel).  The network runs
+// transposed, Z^T = W^T H^T: the weights are the MFMA A operand, 16 output
+// neurons by 32 inputs per fragment, pre-swizzled into the per-lane order of
+// the instruction and staged once per workgroup in LDS; 16 roots are the
+// columns.  A lane of a 16 x 16 result holds 4 consecutive neurons of one
+// root; the next layer's B operand wants 8 consecutive inputs of one root in
+// the same lane, so the inputs of every layer after the first are numbered
+// in the order the previous layer leaves them (the weight fragments carry
+// that permutation) and activations never leave registers.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -39,108 +54,305 @@ constexpr int kParams = w_off(kLayers);
 static_assert(kParams == 14608, "parameter count");
 static_assert(NXG_MLP_MAX_BATCH % 64u == 0u, "batch tiles");
 
-typedef short v4s __attribute__((ext_vector_type(4)));
+// ---- MFMA fragments --------------------------------------------------------
+// v_mfma_f32_16x16x32_bf16, lane L, g = L / 16, r = L % 16:
+//   A (16 x 32): A[r][8g + j], j = 0..7;  B (32 x 16): B[8g + j][r];
+//   D (16 x 16): D[4g + j][r], j = 0..3.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma(const bf16x8 &a, const bf16x8 &b, const v4f &c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Fragment (layer l, output tile tm, input chunk kc): 16 output neurons by 32
+// inputs, 64 lanes x 8 BF16.  Layer 0 has one chunk (16 padded inputs, the
+// upper 16 slots zero); the output layer one tile (16 padded outputs).
+__host__ __device__ constexpr int frag_index(int l, int tm, int kc) {
+    return l == 0 ? tm : (l < kLayers - 1 ? 4 + (l - 1) * 8 + tm * 2 + kc : 28 + kc);
+}
+constexpr int kFrags = 30;
+constexpr int kFragElems = kFrags * 64 * 8;  // BF16 elements
+static_assert(frag_index(kLayers - 1, 0, 1) == kFrags - 1, "fragment count");
+
+// Where W_l[k][n] (input-major master) sits among the fragments.  Input slot
+// of a hidden or output layer: the previous layer leaves neuron
+// 16 T + 4 g + jj (T = output tile, jj = 0..3) in lane group g, so chunk kc
+// slot j of lane group g holds neuron 16 (2 kc + j / 4) + 4 g + j % 4.
+__host__ __device__ inline int frag_position(int l, int k, int n) {
+    const int tm = n / 16, r = n % 16;
+    int g, j, kc;
+    if (l == 0) {
+        kc = 0;
+        g = k / 8;
+        j = k % 8;
+    } else {
+        const int T = k / 16;
+        g = (k % 16) / 4;
+        kc = T / 2;
+        j = (T % 2) * 4 + k % 4;
+    }
+    return (frag_index(l, tm, kc) * 64 + g * 16 + r) * 8 + j;
+}
 
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16u) & 1u);
     return static_cast<uint16_t>(u >> 16u);
 }
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t{b} << 16u); }
 
-// One 16 x 16 output tile of C = A(64 x K) * B(K x N): a(m, k) and b(k, n)
-// return BF16 bits.  Result: acc[j] = C[tm*16 + 4*(lane/16) + j][tn*16 + lane%16].
-template <class FA, class FB>
-__device__ __forceinline__ v4f mfma_tile(int tm, int tn, int K, const FA &a, const FB &b, int lane) {
-    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    const int r = lane & 15, q = lane >> 4;
-    for (int kk = 0; kk < K; kk += 16) {
-        v4s av, bv;
+// SiLU with the hardware exp2 and reciprocal (used alike by training and
+// inference); its derivative for the backward pass.
+__device__ __forceinline__ float sigmoid(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+
+// The two BF16 terms of 8 FP32 values h as MFMA operands, from nh = -h:
+// hi = bf16(h); the dot product of the hi pair with a unit vector plus nh is
+// hi - h exactly (one rounding of an exactly representable sum), so
+// lo = bf16(h - hi) is its negation -- two instructions per pair instead of
+// widening hi back to FP32 and subtracting.  The unit vectors are held in
+// scalar registers: written as constants, the compiler encodes (1, 0) as the
+// inline constant 1.0, which the hardware reads as the pair (0, 1).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split8_neg(const float (&nh)[8], bf16x8 &hi, bf16x8 &lo) {
+    uint32_t u0 = 0x00003f80u, u1 = 0x3f800000u;  // (1, 0) and (0, 1)
+    asm volatile("" : "+s"(u0));
+    asm volatile("" : "+s"(u1));
+    const bf16x2 e0 = __builtin_bit_cast(bf16x2, u0);
+    const bf16x2 e1 = __builtin_bit_cast(bf16x2, u1);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            av[j] = static_cast<short>(a(tm * 16 + r, kk + 4 * q + j));
-            bv[j] = static_cast<short>(b(kk + 4 * q + j, tn * 16 + r));
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    for (int j = 0; j < 8; j += 2) {
+        bf16x2 p;
+        p[0] = static_cast<__bf16>(-nh[j]);
+        p[1] = static_cast<__bf16>(-nh[j + 1]);
+        const float d0 = __builtin_amdgcn_fdot2_f32_bf16(p, e0, nh[j], false);
+        const float d1 = __builtin_amdgcn_fdot2_f32_bf16(p, e1, nh[j + 1], false);
+        hi[j] = p[0];
+        hi[j + 1] = p[1];
+        lo[j] = static_cast<__bf16>(-d0);
+        lo[j + 1] = static_cast<__bf16>(-d1);
     }
-    return acc;
+}
+
+// Orders this wavefront's LDS traffic: every lane's earlier LDS writes are
+// visible to every lane's later reads (the feature staging is per
+// wavefront, and wavefronts of a workgroup run different numbers of tiles,
+// so no workgroup barrier is possible there).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 struct MlpState {
     float *params;          // FP32 master parameters
     float *adam_m, *adam_v;
-    uint16_t *params_bf16;  // BF16 copy used by the kernels
+    uint16_t *params_bf16;  // BF16 copy of the flat layout (training)
+    uint16_t *frags;        // BF16 weight fragments (inference)
     float *norm;            // in_mean[10], in_inv_std[10], out_scale[4], out_offset[4]
     float *partials;        // [max_blocks][kParams + 1] per-block gradients and loss
     float *loss;            // mean squared error of the last step (normalised targets)
 };
 
-// Forward pass of one 64-sample tile held in LDS; h0: [64][kInPad] BF16
-// normalised inputs, y: [64][kOutPad] normalised outputs.  The training form
-// keeps every hidden layer's z (FP32, for SiLU') and h (BF16); the
-// inference form ping-pongs two activation buffers.
-struct TileBuffers {
-    uint16_t h0[kTile * kInPad];
-    uint16_t h[NXG_MLP_HIDDEN_LAYERS][kTile * kHid];
-    float z[NXG_MLP_HIDDEN_LAYERS][kTile * kHid];
-    float y[kTile * kOutPad];
-};
-struct InferBuffers {
-    uint16_t h0[kTile * kInPad];
-    uint16_t h[2][kTile * kHid];
-    float y[kTile * kOutPad];
+// ---- inference --------------------------------------------------------------
+
+// Workgroup shape of the inference kernels: kInferWaves wavefronts share one
+// staged copy of the network; each runs the network kInferBlocks column
+// blocks of 16 roots at a time.
+#ifndef NXG_INFER_WAVES
+#define NXG_INFER_WAVES 4
+#endif
+#ifndef NXG_INFER_BLOCKS
+#define NXG_INFER_BLOCKS 2
+#endif
+constexpr int kInferWaves = NXG_INFER_WAVES, kInferBlocks = NXG_INFER_BLOCKS;
+constexpr int kFeatRow = 12;  // floats per staged feature row (10 features, zero padding)
+static_assert(kTile % (16 * kInferBlocks) == 0, "column blocks per tile");
+
+// What a workgroup stages once: the weight fragments, the biases and the
+// normalisation; and per wavefront the normalised features of its tile.
+struct InferShared {
+    bf16x8 frag[kFrags * 64];
+    float bias[NXG_MLP_HIDDEN_LAYERS * kHid + kOutPad];
+    float norm[28];
+    float feat[kInferWaves][kTile][kFeatRow];
 };
 
-template <bool Train, class Buf>
-__device__ void tile_forward(const uint16_t *W, const float *P, Buf &t) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int l = 0; l < kLayers; ++l) {
-        const int K = rows_of(l), N = cols_of(l);
-        const uint16_t *Wl = W + w_off(l);
-        const float *bl = P + b_off(l);
-        const uint16_t *hin = l == 0 ? t.h0 : t.h[Train ? l - 1 : (l - 1) & 1];
-        uint16_t *hout = l < kLayers - 1 ? t.h[Train ? l : l & 1] : nullptr;
-        const int ldin = l == 0 ? kInPad : kHid;
-        const int tiles = (kTile / 16) * (N / 16);
-        for (int tt = wave; tt < tiles; tt += 4) {
-            const int tm = tt / (N / 16), tn = tt % (N / 16);
-            const v4f acc = mfma_tile(
-                tm, tn, K, [&](int m, int k) { return hin[m * ldin + k]; },
-                [&](int k, int n) { return Wl[k * N + n]; }, lane);
+__device__ void stage_network(const MlpState &s, InferShared &sh) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(s.frags);
+    uint4 *dst = reinterpret_cast<uint4 *>(sh.frag);
+    for (int i = threadIdx.x; i < kFrags * 64; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < NXG_MLP_HIDDEN_LAYERS * kHid + kOutPad; i += blockDim.x) {
+        const int l = i / kHid, n = i % kHid;
+        sh.bias[i] = s.params[b_off(l) + n];
+    }
+    for (int i = threadIdx.x; i < 28; i += blockDim.x) sh.norm[i] = s.norm[i];
+    for (int i = threadIdx.x; i < kInferWaves * kTile; i += blockDim.x)
+        for (int k = kIn; k < kFeatRow; ++k) sh.feat[i / kTile][i % kTile][k] = 0.0f;
+    __syncthreads();
+}
+
+// A weight fragment from LDS.
+__device__ __forceinline__ bf16x8 frag_at(const InferShared &sh, int index, int lane) {
+    return sh.frag[index * 64 + lane];
+}
+
+// A compiler-only barrier at each layer: it keeps a layer's fragment reads
+// in that layer.  Hoisted out of the tile loop, the 30 fragments would hold
+// 120 registers and leave one wavefront per SIMD.
+__device__ __forceinline__ void layer_fence() { asm volatile("" ::: "memory"); }
+
+// The network over CB column blocks of 16 roots whose normalised features
+// sit in f[root][0..9] (LDS rows of kFeatRow floats, zero from 10 on);
+// returns in lanes 0..15 of y[cb] the 4 normalised outputs of root
+// cb*16 + lane.
+template <int CB>
+__device__ __forceinline__ void network(const InferShared &sh, const float (*f)[kFeatRow], v4f (&y)[CB]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    bf16x8 hi[CB][2], lo[CB][2];  // [column block][input chunk]
+    // layer 0: inputs 8g..8g+7 of root r -- features 0..7 for lane group 0,
+    // 8..9 and zeros for group 1, zeros for groups 2 and 3
+    layer_fence();
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int m = tm * 16 + 4 * (lane >> 4) + j, n = tn * 16 + (lane & 15);
-                const float z = acc[j] + bl[n];
-                if (l < kLayers - 1) {
-                    if constexpr (Train) t.z[l][m * kHid + n] = z;
-                    hout[m * kHid + n] = f2bf(z * sigmoid(z));
-                } else {
-                    t.y[m * kOutPad + n] = z;
+    for (int cb = 0; cb < CB; ++cb) {
+        const float4 *row = reinterpret_cast<const float4 *>(f[cb * 16 + r]);
+        const float4 p = row[g & 1 ? 2 : 0], q = row[1];
+        const float kp = g < 2 ? 1.0f : 0.0f, kq = g == 0 ? 1.0f : 0.0f;
+        const float nx[8] = {-p.x * kp, -p.y * kp, -p.z * kp, -p.w * kp,
+                             -q.x * kq, -q.y * kq, -q.z * kq, -q.w * kq};
+        split8_neg(nx, hi[cb][0], lo[cb][0]);
+    }
+    v4f acc[CB][4];
+#pragma unroll
+    for (int tm = 0; tm < 4; ++tm) {
+        const bf16x8 a = frag_at(sh, frag_index(0, tm, 0), lane);
+        const float4 b4 = *reinterpret_cast<const float4 *>(&sh.bias[16 * tm + 4 * g]);
+        const v4f c = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) acc[cb][tm] = mfma(a, lo[cb][0], mfma(a, hi[cb][0], c));
+    }
+#pragma unroll
+    for (int l = 1; l < kLayers; ++l) {
+        layer_fence();
+        // SiLU of the previous layer, in the next layer's input order
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc) {
+                float nh[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const float z = acc[cb][2 * kc + j / 4][j % 4];
+                    nh[j] = -z * sigmoid(z);
                 }
+                split8_neg(nh, hi[cb][kc], lo[cb][kc]);
             }
         }
-        __syncthreads();
+        const int tiles = l < kLayers - 1 ? 4 : 1;
+#pragma unroll
+        for (int tm = 0; tm < tiles; ++tm) {
+            const float4 b4 = *reinterpret_cast<const float4 *>(&sh.bias[l * kHid + 16 * tm + 4 * g]);
+            const v4f c = {b4.x, b4.y, b4.z, b4.w};
+            const bf16x8 a0 = frag_at(sh, frag_index(l, tm, 0), lane);
+            const bf16x8 a1 = frag_at(sh, frag_index(l, tm, 1), lane);
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+                acc[cb][tm] = mfma(a1, lo[cb][1], mfma(a1, hi[cb][1], mfma(a0, lo[cb][0], mfma(a0, hi[cb][0], c))));
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) y[cb] = acc[cb][0];
+}
+
+// Outputs of a 64-root tile: lanes 0..15 of each column block hold the 4
+// normalised outputs of one root.
+template <class Store>
+__device__ __forceinline__ void network_tile(const InferShared &sh, const float (*f)[kFeatRow], const Store &store) {
+    const int lane = threadIdx.x & 63;
+    constexpr int kRoots = 16 * kInferBlocks;
+#pragma unroll 1
+    for (int pass = 0; pass < kTile / kRoots; ++pass) {
+        v4f y[kInferBlocks];
+        network<kInferBlocks>(sh, f + pass * kRoots, y);
+        if (lane < 16) {
+#pragma unroll
+            for (int cb = 0; cb < kInferBlocks; ++cb) {
+                float o[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) o[j] = y[cb][j] * sh.norm[2 * kIn + j] + sh.norm[2 * kIn + kOut + j];
+                store(pass * kRoots + cb * 16 + lane, make_float4(o[0], o[1], o[2], o[3]));
+            }
+        }
+    }
+    wave_lds_sync();  // the staged features may be overwritten from here on
+}
+
+// Inference over a feature array: outputs are attributes (denormalised).
+// Each wavefront takes 64-row tiles in turn.
+__global__ void __launch_bounds__(64 * kInferWaves) mlp_forward_kernel(MlpState s, const float *features,
+                                                                       float *outputs, uint64_t count) {
+    extern __shared__ uint8_t smem[];
+    InferShared &sh = *reinterpret_cast<InferShared *>(smem);
+    stage_network(s, sh);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t tiles = (count + kTile - 1) / kTile;
+    for (uint64_t t = uint64_t{blockIdx.x} * kInferWaves + wave; t < tiles; t += uint64_t{gridDim.x} * kInferWaves) {
+        const uint64_t first = t * kTile;
+        const uint64_t row = first + lane < count ? first + lane : count - 1;
+        for (int k = 0; k < kIn; ++k)
+            sh.feat[wave][lane][k] = (features[row * kIn + k] - sh.norm[k]) * sh.norm[kIn + k];
+        wave_lds_sync();
+        network_tile(sh, sh.feat[wave], [&](int m, float4 o) {
+            if (first + m < count) {
+                float *dst = outputs + (first + m) * kOut;
+                dst[0] = o.x;
+                dst[1] = o.y;
+                dst[2] = o.z;
+                dst[3] = o.w;
+            }
+        });
     }
 }
 
+// ---- training ---------------------------------------------------------------
+
+// One 16 x 16 output tile of C = A(M x K) * B(K x N), K a multiple of 32,
+// with operands gathered element by element: a(m, k) and b(k, n) return
+// BF16 bits.  acc[j] = C[tm*16 + 4*(lane/16) + j][tn*16 + lane%16].
+template <class FA, class FB>
+__device__ __forceinline__ v4f mfma_gather(int tm, int tn, int K, const FA &a, const FB &b, int lane, v4f acc) {
+    const int r = lane & 15, q = lane >> 4;
+    for (int kk = 0; kk < K; kk += 32) {
+        bf16x8 av, bv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            av[j] = __builtin_bit_cast(__bf16, a(tm * 16 + r, kk + 8 * q + j));
+            bv[j] = __builtin_bit_cast(__bf16, b(kk + 8 * q + j, tn * 16 + r));
+        }
+        acc = mfma(av, bv, acc);
+    }
+    return acc;
+}
+
+// A 64-sample training tile in LDS.  The forward keeps every layer's
+// pre-activation z (FP32); the hidden activations are recomputed from z when
+// the backward pass needs them (the same function of the same value, so the
+// same BF16 terms).
 struct TrainBuffers {
-    TileBuffers f;
+    uint16_t x_hi[kTile * kInPad], x_lo[kTile * kInPad];  // normalised inputs
+    uint16_t h_hi[2][kTile * kHid], h_lo[2][kTile * kHid];  // activations (forward ping-pong; backward input)
+    float z[NXG_MLP_HIDDEN_LAYERS][kTile * kHid];
+    float y[kTile * kOutPad];
     uint16_t dz[kTile * kHid];  // BF16 gradient of the current layer's pre-activation
     float dh[kTile * kHid];     // FP32 gradient of the previous layer's output
 };
 
-// Normalised features of sample rows into h0 (BF16), zero padding.
-__device__ void load_inputs(const float *features, uint64_t first, uint64_t count, const float *norm,
-                            uint16_t *h0) {
-    for (int i = threadIdx.x; i < kTile * kInPad; i += blockDim.x) {
-        const int m = i / kInPad, k = i % kInPad;
-        const uint64_t s = first + m;
-        float v = 0.0f;
-        if (k < kIn && s < count) v = (features[s * kIn + k] - norm[k]) * norm[kIn + k];
-        h0[i] = f2bf(v);
-    }
-    __syncthreads();
+__device__ __forceinline__ void split_store(float h, uint16_t &hi, uint16_t &lo) {
+    hi = f2bf(h);
+    lo = f2bf(h - bf2f(hi));
 }
 
 // One training step over samples [first, first + batch) (wrapping modulo
@@ -153,16 +365,47 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t tile_first = first + uint64_t{blockIdx.x} * kTile;
     const uint32_t valid = batch - blockIdx.x * kTile < static_cast<uint32_t>(kTile) ? batch - blockIdx.x * kTile : kTile;
-    // rows of this tile (sample indices wrap around the data set)
     for (int i = threadIdx.x; i < kTile * kInPad; i += blockDim.x) {
         const int m = i / kInPad, k = i % kInPad;
         const uint64_t sidx = (tile_first + m) % count;
         float v = 0.0f;
         if (k < kIn && static_cast<uint32_t>(m) < valid) v = (features[sidx * kIn + k] - s.norm[k]) * s.norm[kIn + k];
-        t.f.h0[i] = f2bf(v);
+        split_store(v, t.x_hi[i], t.x_lo[i]);
     }
     __syncthreads();
-    tile_forward<true>(s.params_bf16, s.params, t.f);
+    const uint16_t *W = s.params_bf16;
+    // forward
+    for (int l = 0; l < kLayers; ++l) {
+        const int K = rows_of(l), N = cols_of(l), ldin = l == 0 ? kInPad : kHid;
+        const uint16_t *Wl = W + w_off(l);
+        const float *bl = s.params + b_off(l);
+        const uint16_t *in_hi = l == 0 ? t.x_hi : t.h_hi[(l - 1) & 1];
+        const uint16_t *in_lo = l == 0 ? t.x_lo : t.h_lo[(l - 1) & 1];
+        const int tiles = (kTile / 16) * (N / 16);
+        for (int tt = wave; tt < tiles; tt += 4) {
+            const int tm = tt / (N / 16), tn = tt % (N / 16);
+            auto wb = [&](int k, int n) -> uint16_t { return k < K ? Wl[k * N + n] : uint16_t{0}; };
+            v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma_gather(tm, tn, 32 * ((K + 31) / 32),
+                              [&](int m, int k) -> uint16_t { return k < K ? in_hi[m * ldin + k] : uint16_t{0}; },
+                              wb, lane, acc);
+            acc = mfma_gather(tm, tn, 32 * ((K + 31) / 32),
+                              [&](int m, int k) -> uint16_t { return k < K ? in_lo[m * ldin + k] : uint16_t{0}; },
+                              wb, lane, acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = tm * 16 + 4 * (lane >> 4) + j, n = tn * 16 + (lane & 15);
+                const float z = acc[j] + bl[n];
+                if (l < kLayers - 1) {
+                    t.z[l][m * kHid + n] = z;
+                    split_store(silu(z), t.h_hi[l & 1][m * kHid + n], t.h_lo[l & 1][m * kHid + n]);
+                } else {
+                    t.y[m * kOutPad + n] = z;
+                }
+            }
+        }
+        __syncthreads();
+    }
     float *part = s.partials + uint64_t{blockIdx.x} * (kParams + 1);
     // output gradient dy = 2 (y - t) / (batch * 4) on valid rows; loss partial
     float loss = 0.0f;
@@ -172,11 +415,11 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
         if (n < kOut && static_cast<uint32_t>(m) < valid) {
             const uint64_t sidx = (tile_first + m) % count;
             const float tn = (targets[sidx * kOut + n] - s.norm[2 * kIn + kOut + n]) / s.norm[2 * kIn + n];
-            const float diff = t.f.y[i] - tn;
+            const float diff = t.y[i] - tn;
             loss += diff * diff;
             d = 2.0f * diff / (static_cast<float>(batch) * kOut);
         }
-        t.dh[m * kHid + n] = d;  // reuse dh as the FP32 output gradient
+        t.dh[m * kHid + n] = d;  // dh holds the FP32 output gradient
     }
     // block loss reduction (fixed order: per-wave shuffle, then wave 0)
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
@@ -186,18 +429,23 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
     if (threadIdx.x == 0) part[kParams] = wave_loss[0] + wave_loss[1] + wave_loss[2] + wave_loss[3];
 
     for (int l = kLayers - 1; l >= 0; --l) {
-        const int K = rows_of(l), N = cols_of(l);
-        // dz = dh * silu'(z) (identity on the output layer), FP32 -> BF16
-        for (int i = threadIdx.x; i < kTile * N; i += blockDim.x) {
-            const int m = i / N, n = i % N;
-            float d = t.dh[m * kHid + n];
-            if (l < kLayers - 1) {
-                const float z = t.f.z[l][m * kHid + n];
-                const float sg = sigmoid(z);
-                d *= sg * (1.0f + z * (1.0f - sg));
+        const int K = rows_of(l), N = cols_of(l), ldin = l == 0 ? kInPad : kHid;
+        // dz = dh * silu'(z) (identity on the output layer), FP32 -> BF16;
+        // the layer's input activations (hi, lo) recomputed from z into
+        // ping-pong slot 0
+        for (int i = threadIdx.x; i < kTile * kHid; i += blockDim.x) {
+            const int m = i / kHid, n = i % kHid;
+            if (n < N) {
+                float d = t.dh[m * kHid + n];
+                if (l < kLayers - 1) {
+                    const float z = t.z[l][m * kHid + n];
+                    const float sg = sigmoid(z);
+                    d *= sg * (1.0f + z * (1.0f - sg));
+                }
+                t.dz[m * kHid + n] = f2bf(d);
+                t.dh[m * kHid + n] = d;  // keep the FP32 dz for the bias gradient
             }
-            t.dz[m * kHid + n] = f2bf(d);
-            t.dh[m * kHid + n] = d;  // keep the FP32 dz for the bias gradient
+            if (l > 0) split_store(silu(t.z[l - 1][i]), t.h_hi[0][i], t.h_lo[0][i]);
         }
         __syncthreads();
         // bias gradient: column sums of FP32 dz, fixed order
@@ -206,16 +454,18 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
             for (int m = 0; m < kTile; ++m) acc += t.dh[m * kHid + n];
             part[b_off(l) + n] = acc;
         }
-        // weight gradient dW[k][n] = sum_m h_in[m][k] dz[m][n]  (K x N, contraction over samples)
-        const uint16_t *hin = l == 0 ? t.f.h0 : t.f.h[l - 1];
-        const int ldin = l == 0 ? kInPad : kHid;
+        // weight gradient dW[k][n] = sum_m (hi + lo)[m][k] dz[m][n]
+        const uint16_t *in_hi = l == 0 ? t.x_hi : t.h_hi[0];
+        const uint16_t *in_lo = l == 0 ? t.x_lo : t.h_lo[0];
         const int wt = (K / 16) * (N / 16);
         for (int tt = wave; tt < wt; tt += 4) {
             const int tk = tt / (N / 16), tn = tt % (N / 16);
-            // A(row=k, col=m) = h_in[m][k]; B(row=m, col=n) = dz[m][n]; contraction length 64
-            const v4f acc = mfma_tile(
-                tk, tn, kTile, [&](int k, int m) { return hin[m * ldin + k]; },
-                [&](int m, int n) { return t.dz[m * kHid + n]; }, lane);
+            auto dzb = [&](int m, int n) -> uint16_t { return t.dz[m * kHid + n]; };
+            v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma_gather(tk, tn, kTile, [&](int k, int m) -> uint16_t { return in_hi[m * ldin + k]; }, dzb,
+                              lane, acc);
+            acc = mfma_gather(tk, tn, kTile, [&](int k, int m) -> uint16_t { return in_lo[m * ldin + k]; }, dzb,
+                              lane, acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int k = tk * 16 + 4 * (lane >> 4) + j, n = tn * 16 + (lane & 15);
@@ -225,13 +475,15 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
         __syncthreads();
         if (l == 0) break;
         // dh_prev[m][k] = sum_n dz[m][n] W[k][n]  (64 x K, contraction over N)
-        const uint16_t *Wl = s.params_bf16 + w_off(l);
+        const uint16_t *Wl = W + w_off(l);
         const int ht = (kTile / 16) * (K / 16);
         for (int tt = wave; tt < ht; tt += 4) {
             const int tm = tt / (K / 16), tk = tt % (K / 16);
-            const v4f acc = mfma_tile(
-                tm, tk, N, [&](int m, int n) { return t.dz[m * kHid + n]; },
-                [&](int n, int k) { return Wl[k * N + n]; }, lane);
+            v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma_gather(
+                tm, tk, 32 * ((N + 31) / 32),
+                [&](int m, int n) -> uint16_t { return n < N ? t.dz[m * kHid + n] : uint16_t{0}; },
+                [&](int n, int k) -> uint16_t { return n < N ? Wl[k * N + n] : uint16_t{0}; }, lane, acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int m = tm * 16 + 4 * (lane >> 4) + j, k = tk * 16 + (lane & 15);
@@ -242,19 +494,45 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(MlpState s, const float 
     }
 }
 
+// The BF16 copies of parameter i: the flat copy and, for a weight, its slot
+// among the inference fragments.
+__device__ __forceinline__ void store_bf16(const MlpState &s, int i, float p) {
+    const uint16_t b = f2bf(p);
+    s.params_bf16[i] = b;
+    for (int l = 0; l < kLayers; ++l) {
+        if (i >= w_off(l) && i < b_off(l)) {
+            const int e = i - w_off(l), k = e / cols_of(l), n = e % cols_of(l);
+            s.frags[frag_position(l, k, n)] = b;
+        }
+    }
+}
+
 // Reduce the per-block partials in block order, then Adam on the FP32
-// master parameters and refresh of the BF16 copy.
+// master parameters and refresh of the BF16 copies.  A block owns 64
+// parameters; its four wavefronts sum interleaved quarters of the partials
+// (independent loads in flight), then the quarters are added in order.
 __global__ void __launch_bounds__(256) mlp_adam_kernel(MlpState s, uint32_t blocks, uint32_t batch, uint32_t step,
                                                        float lr, float beta1, float beta2, float eps) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        float loss = 0.0f;
-        for (uint32_t b = 0; b < blocks; ++b) loss += s.partials[uint64_t{b} * (kParams + 1) + kParams];
-        s.loss[0] = loss / (static_cast<float>(batch) * kOut);
+    __shared__ float quarter[4][64];
+    __shared__ float loss_quarter[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const uint64_t stride = kParams + 1;
+    float g = 0.0f, lq = 0.0f;
+#pragma unroll 8
+    for (uint32_t b = wave; b < blocks; b += 4) {
+        if (i < kParams) g += s.partials[uint64_t{b} * stride + i];
+        if (blockIdx.x == 0 && lane == 0) lq += s.partials[uint64_t{b} * stride + kParams];
     }
-    if (i >= static_cast<uint32_t>(kParams)) return;
-    float g = 0.0f;
-    for (uint32_t b = 0; b < blocks; ++b) g += s.partials[uint64_t{b} * (kParams + 1) + i];
+    quarter[wave][lane] = g;
+    if (lane == 0) loss_quarter[wave] = lq;
+    __syncthreads();
+    if (wave != 0) return;
+    if (blockIdx.x == 0 && lane == 0)
+        s.loss[0] = (((loss_quarter[0] + loss_quarter[1]) + loss_quarter[2]) + loss_quarter[3]) /
+                    (static_cast<float>(batch) * kOut);
+    if (i >= kParams) return;
+    g = ((quarter[0][lane] + quarter[1][lane]) + quarter[2][lane]) + quarter[3][lane];
     const float m = beta1 * s.adam_m[i] + (1.0f - beta1) * g;
     const float v = beta2 * s.adam_v[i] + (1.0f - beta2) * g * g;
     s.adam_m[i] = m;
@@ -263,26 +541,17 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(MlpState s, uint32_t bloc
     const float vh = v / (1.0f - powf(beta2, static_cast<float>(step)));
     const float p = s.params[i] - lr * mh / (sqrtf(vh) + eps);
     s.params[i] = p;
-    s.params_bf16[i] = f2bf(p);
+    store_bf16(s, i, p);
 }
 
+// Both BF16 copies from the masters; also zeroes the fragment slots no
+// parameter maps to (layer 0's inputs 16..31).
 __global__ void mlp_sync_kernel(MlpState s) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < static_cast<uint32_t>(kParams)) s.params_bf16[i] = f2bf(s.params[i]);
-}
-
-// Inference over a feature array: outputs are attributes (denormalised).
-__global__ void __launch_bounds__(256) mlp_forward_kernel(MlpState s, const float *features, float *outputs,
-                                                          uint64_t count) {
-    extern __shared__ uint8_t smem[];
-    InferBuffers &t = *reinterpret_cast<InferBuffers *>(smem);
-    const uint64_t first = uint64_t{blockIdx.x} * kTile;
-    load_inputs(features, first, count, s.norm, t.h0);
-    tile_forward<false>(s.params_bf16, s.params, t);
-    for (int i = threadIdx.x; i < kTile * kOut; i += blockDim.x) {
-        const int m = i / kOut, n = i % kOut;
-        if (first + m < count)
-            outputs[(first + m) * kOut + n] = t.y[m * kOutPad + n] * s.norm[2 * kIn + n] + s.norm[2 * kIn + kOut + n];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kParams) store_bf16(s, i, s.params[i]);
+    if (i < 4 * 32 * 8) {  // tile, lane group 2..3 x 16 lanes, slot
+        const int tm = i / 256, lane = 32 + (i / 8) % 32, j = i % 8;
+        s.frags[(frag_index(0, tm, 0) * 64 + lane) * 8 + j] = 0u;
     }
 }
 
@@ -390,26 +659,52 @@ __global__ void __launch_bounds__(256) features_kernel(FeatureArgs a, float *out
     for (int k = 0; k < kIn; ++k) out[s * kIn + k] = f[k];
 }
 
-// Fused: features of 64 roots, normalisation, MLP, attributes.
-__global__ void __launch_bounds__(256) surrogate_kernel(FeatureArgs a, MlpState s, float4 *attributes) {
+// Fused: root features of a wavefront's 64 roots (one per lane), their
+// normalisation, the network, attributes.  Each wavefront takes 64-root
+// tiles in turn; the workgroup stages the network once.  With `order`
+// (strand indices grouped by root triangle) position p of the processing
+// order is strand order[p], and its attributes go to that strand's slot.
+__global__ void __launch_bounds__(64 * kInferWaves) surrogate_kernel(FeatureArgs a, MlpState s, const uint32_t *order,
+                                                                     float4 *attributes) {
     extern __shared__ uint8_t smem[];
-    InferBuffers &t = *reinterpret_cast<InferBuffers *>(smem);
-    const uint64_t first = uint64_t{blockIdx.x} * kTile;
-    if (threadIdx.x < kTile) {
-        const uint64_t sidx = first + threadIdx.x;
+    InferShared &sh = *reinterpret_cast<InferShared *>(smem);
+    __shared__ uint32_t strand_of[kInferWaves][kTile];
+    stage_network(s, sh);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t tiles = (a.strand_count + kTile - 1) / kTile;
+    for (uint32_t t = blockIdx.x * kInferWaves + wave; t < tiles; t += gridDim.x * kInferWaves) {
+        const uint32_t first = t * kTile;
+        const uint32_t p = first + lane < a.strand_count ? first + lane : a.strand_count - 1u;
+        const uint32_t strand = order ? order[p] : p;
         float f[kIn];
-        root_features(a, static_cast<uint32_t>(sidx < a.strand_count ? sidx : a.strand_count - 1u), f);
-        for (int k = 0; k < kInPad; ++k)
-            t.h0[threadIdx.x * kInPad + k] = k < kIn ? f2bf((f[k] - s.norm[k]) * s.norm[kIn + k]) : 0;
+        root_features(a, strand, f);
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) sh.feat[wave][lane][k] = (f[k] - sh.norm[k]) * sh.norm[kIn + k];
+        strand_of[wave][lane] = strand;
+        wave_lds_sync();
+        network_tile(sh, sh.feat[wave], [&](int m, float4 o) {
+            if (first + m < a.strand_count) attributes[strand_of[wave][m]] = o;
+        });
     }
-    __syncthreads();
-    tile_forward<false>(s.params_bf16, s.params, t);
-    if (threadIdx.x < kTile && first + threadIdx.x < a.strand_count) {
-        const int m = threadIdx.x;
-        float o[kOut];
-        for (int n = 0; n < kOut; ++n) o[n] = t.y[m * kOutPad + n] * s.norm[2 * kIn + n] + s.norm[2 * kIn + kOut + n];
-        attributes[first + m] = make_float4(o[0], o[1], o[2], o[3]);
-    }
+}
+
+// Workgroups of a persistent inference launch: enough to fill every CU
+// (as many as fit at once), at most one per kInferWaves tiles.
+uint32_t infer_grid(const void *kernel, uint64_t tiles) {
+    static int ctas = [] {
+        int device = 0, cus = 256;
+        if (hipGetDevice(&device) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        return cus;
+    }();
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kInferWaves, sizeof(InferShared)) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint64_t want = (tiles + kInferWaves - 1u) / kInferWaves;
+    const uint64_t most = uint64_t{static_cast<uint32_t>(ctas)} * static_cast<uint32_t>(per_cu);
+    return static_cast<uint32_t>(want < most ? want : most);
 }
 
 }  // namespace
@@ -432,6 +727,7 @@ MlpState mlp_state_of(void *base) {
     s.adam_m = reinterpret_cast<float *>(b + L.adam_m_offset);
     s.adam_v = reinterpret_cast<float *>(b + L.adam_v_offset);
     s.params_bf16 = reinterpret_cast<uint16_t *>(b + L.params_bf16_offset);
+    s.frags = reinterpret_cast<uint16_t *>(b + L.fragments_offset);
     s.norm = reinterpret_cast<float *>(b + L.norm_offset);
     s.partials = reinterpret_cast<float *>(b + L.partials_offset);
     s.loss = reinterpret_cast<float *>(b + L.loss_offset);
@@ -469,7 +765,8 @@ void nxg_mlp_get_layout(nxg_mlp_layout *out) {
     L.adam_m_offset = align(L.params_offset + P * 4u);
     L.adam_v_offset = align(L.adam_m_offset + P * 4u);
     L.params_bf16_offset = align(L.adam_v_offset + P * 4u);
-    L.norm_offset = align(L.params_bf16_offset + P * 2u);
+    L.fragments_offset = align(L.params_bf16_offset + P * 2u);
+    L.norm_offset = align(L.fragments_offset + uint64_t{nxg::kFragElems} * 2u);
     L.loss_offset = align(L.norm_offset + 28u * 4u);
     L.partials_offset = align(L.loss_offset + 4u);
     L.max_batch = NXG_MLP_MAX_BATCH;
@@ -498,7 +795,7 @@ uint32_t nxg_mlp_train_step(void *state, const float *features, const float *tar
     hipLaunchKernelGGL(nxg::mlp_train_kernel, dim3(blocks), dim3(256), sizeof(nxg::TrainBuffers), s, st, features,
                        targets, sample_count, batch_first % sample_count, batch_size);
     if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
-    hipLaunchKernelGGL(nxg::mlp_adam_kernel, dim3((nxg::kParams + 255) / 256), dim3(256), 0, s, st, blocks,
+    hipLaunchKernelGGL(nxg::mlp_adam_kernel, dim3((nxg::kParams + 63) / 64), dim3(256), 0, s, st, blocks,
                        batch_size, step, learning_rate, 0.9f, 0.999f, 1.0e-8f);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
@@ -518,8 +815,10 @@ uint32_t nxg_mlp_gradients(void *state, const float *features, const float *targ
 uint32_t nxg_mlp_forward(const void *state, const float *features, float *outputs, uint64_t count, void *stream) {
     if (!state || !features || !outputs) return NXG_ERROR_NULL_ARGUMENT;
     if (count == 0u) return NXG_OK;
-    hipLaunchKernelGGL(nxg::mlp_forward_kernel, dim3(static_cast<uint32_t>((count + 63u) / 64u)), dim3(256),
-                       sizeof(nxg::InferBuffers), static_cast<hipStream_t>(stream),
+    const uint64_t tiles = (count + 63u) / 64u;
+    hipLaunchKernelGGL(nxg::mlp_forward_kernel,
+                       dim3(nxg::infer_grid(reinterpret_cast<const void *>(&nxg::mlp_forward_kernel), tiles)),
+                       dim3(64 * nxg::kInferWaves), sizeof(nxg::InferShared), static_cast<hipStream_t>(stream),
                        nxg::mlp_state_of(const_cast<void *>(state)), features, outputs, count);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
@@ -537,16 +836,26 @@ uint32_t nxg_root_features(const nxg_device_asset *asset, const nxg_generation_p
 
 uint32_t nxg_surrogate_attributes(const nxg_device_asset *asset, const nxg_generation_params *params,
                                   const void *state, const nxg_texel_map *map, float *attributes,
-                                  uint64_t attribute_capacity, void *stream) {
+                                  uint64_t attribute_capacity, const void *prepared_workspace,
+                                  uint64_t workspace_bytes, void *stream) {
     if (uint32_t s = nxg::validate_common(asset, nullptr, params, nullptr); s != NXG_OK) return s;
     if (!state || !map || !map->texels || map->face_count == 0u || map->resolution == 0u)
         return NXG_ERROR_NULL_ARGUMENT;
     if (!attributes || attribute_capacity < uint64_t{params->strand_count} * NXG_ATTR_COUNT)
         return NXG_ERROR_ATTRIBUTE_CAPACITY_TOO_SMALL;
-    hipLaunchKernelGGL(nxg::surrogate_kernel, dim3((params->strand_count + 63u) / 64u), dim3(256),
-                       sizeof(nxg::InferBuffers), static_cast<hipStream_t>(stream),
+    const uint32_t *order = nullptr;
+    if (prepared_workspace) {
+        const nxg::WorkspaceLayout L = nxg::workspace_layout(asset->header.guide_count, params->cvs_per_strand,
+                                                             params->strand_count, asset->header.triangle_count);
+        if (workspace_bytes < L.total) return NXG_ERROR_WORKSPACE_TOO_SMALL;
+        order = reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(prepared_workspace) + L.order_offset);
+    }
+    const uint64_t tiles = (uint64_t{params->strand_count} + 63u) / 64u;
+    hipLaunchKernelGGL(nxg::surrogate_kernel,
+                       dim3(nxg::infer_grid(reinterpret_cast<const void *>(&nxg::surrogate_kernel), tiles)),
+                       dim3(64 * nxg::kInferWaves), sizeof(nxg::InferShared), static_cast<hipStream_t>(stream),
                        nxg::feature_args(*asset, *params, *map), nxg::mlp_state_of(const_cast<void *>(state)),
-                       reinterpret_cast<float4 *>(attributes));
+                       order, reinterpret_cast<float4 *>(attributes));
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 

@@ -21,7 +21,7 @@ IN_PAD, OUT_PAD = 16, 16
 
 class Layout(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("param_count", "params_offset", "adam_m_offset", "adam_v_offset",
-                                          "params_bf16_offset", "norm_offset", "loss_offset", "partials_offset",
+                                          "params_bf16_offset", "fragments_offset", "norm_offset", "loss_offset", "partials_offset",
                                           "max_batch", "total_bytes")]
 
 
@@ -82,6 +82,11 @@ class Surrogate:
 
     def params_bf16(self):
         return self._view(self.L.params_bf16_offset, self.L.param_count, np.uint16).cpu().numpy()
+
+    def fragments(self):
+        """The BF16 weights in MFMA operand order (30 fragments x 64 lanes x 8)."""
+        n = (self.L.norm_offset - self.L.fragments_offset) // 2
+        return self._view(self.L.fragments_offset, min(n, 30 * 64 * 8), np.uint16).cpu().numpy()
 
     def set_norm(self, in_mean, in_std, out_scale, out_offset):
         norm = np.concatenate([in_mean, 1.0 / np.maximum(np.asarray(in_std, np.float64), 1e-6),
@@ -145,9 +150,14 @@ def root_features(asset, params, maps, out=None, stream=None):
     return out
 
 
-def surrogate_attributes(asset, params, net, maps, out, stream=None):
+def surrogate_attributes(asset, params, net, maps, out, stream=None, prepared=None):
+    """Enqueue the fused surrogate; `prepared` (optional) is a workspace that
+    nanoxgen_amd.prepare_packed filled for the same asset, seed
+    and strand count (the roots then run grouped by triangle)."""
+    ws = C.c_void_p(prepared.data_ptr()) if prepared is not None else None
     check(lib().nxg_surrogate_attributes(C.byref(asset.desc), C.byref(params), net.ptr(), C.byref(maps.structs[0]),
-                                         C.c_void_p(out.data_ptr()), out.numel(), _stream_handle(stream)),
+                                         C.c_void_p(out.data_ptr()), out.numel(), ws,
+                                         prepared.numel() if prepared is not None else 0, _stream_handle(stream)),
           "nxg_surrogate_attributes")
     return out
 

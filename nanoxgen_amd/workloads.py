@@ -215,20 +215,34 @@ def config3(seed=1, strands=1_000_000, program_seed=3, map_seed=4):
                     info=dict(programs=programs, maps=maps))
 
 
-def config4(seed=2, strands=1_000_000, quads=512, guides_per_side=64, program_seed=5, map_seed=6):
+# Noise-frequency scale of the surrogate workload's programs.  Its four
+# programs are built like the expression workload's (40 ops, 4 noise3 calls
+# of scaled root positions and normals, 2 texel reads), with the noise3
+# argument scales multiplied by this factor: 0.08-0.3 cycles per unit, under
+# one and a half cycles across the 4-unit plane, the smoothly varying
+# per-root attributes a groom's length, width, noise and clump maps carry.
+# At the expression workload's scales (factor 1: up to 14 cycles across) a
+# 10 -> 64x4 -> 4 network cannot meet a 1e-3 relative bound anywhere.
+SURROGATE_FREQUENCY_SCALE = 0.1
+
+
+def config4(seed=2, strands=1_000_000, quads=512, guides_per_side=64, program_seed=5, map_seed=6,
+            frequency_scale=SURROGATE_FREQUENCY_SCALE):
     """Learned-surrogate form of the plane description: 1,000,000 roots x 24
     CVs, two octaves, per-root length/width/noise-magnitude/clump-weight
-    programs of 40 ops (4 noise3, 2 texel reads of a 16x16 per-face map).
-    The surrogate trains on roots drawn with seed + 1000 (disjoint)."""
+    programs of 40 ops (4 noise3, 2 texel reads of a 16x16 per-face map),
+    noise3 argument scales multiplied by frequency_scale.  The surrogate
+    trains on roots drawn with seed + 1000 (disjoint)."""
     from . import expr
     build = plane_description(seed=seed, quads=quads, guides_per_side=guides_per_side)
     params = GenerationParams(strand_count=strands, cvs_per_strand=24, seed=seed, root_width=0.01,
                               tip_width=0.001, noise_amplitude=0.1, noise_frequency=4.0)
     octaves = NoiseOctaves.of([(0.1, 4.0), (0.05, 8.0)])
-    programs = expr.attribute_programs(seed=program_seed, clump=True)
+    programs = expr.attribute_programs(seed=program_seed, clump=True, frequency_scale=frequency_scale)
     maps = [expr.texel_map(len(build.triangles), 16, seed=map_seed)]
-    return Workload(f"plane-{2 * quads * quads}tri-{strands}-strands-24cv-2oct-4x40op-surrogate", build, params,
-                    octaves, info=dict(programs=programs, maps=maps, train_seed=seed + 1000))
+    return Workload(f"plane-{2 * quads * quads}tri-{strands}-strands-24cv-2oct-4x40op-f{frequency_scale:g}-surrogate",
+                    build, params, octaves,
+                    info=dict(programs=programs, maps=maps, train_seed=seed + 1000, frequency_scale=frequency_scale))
 
 
 # Batched uneven form: (strands, CVs per strand, mesh, stencil K, octaves,

@@ -6,11 +6,15 @@ Network (no reference counterpart; the generator's own learned form):
   -> output layer: y = h4 @ W5 + b5 (4 values) -> attribute = y * scale + offset
 Loss: mean over batch and outputs of (y - t_norm)^2 with t_norm = (t - offset) / scale.
 
-The device kernels compute in BF16 operands with FP32 accumulation: the
-rounding points are the normalised input, every hidden activation and the
-weights.  `forward(..., bf16=True)` applies the same rounding so the device
+The device kernels compute in BF16 operands with FP32 accumulation.  The
+weights are rounded to BF16; the normalised input and every hidden
+activation are carried as two BF16 terms, hi = bf16(h) and lo = bf16(h - hi)
+(so their value is hi + lo, about 16 significant bits); in the backward pass
+the pre-activation gradient dz is rounded to BF16 for both products and the
+bias gradient sums the unrounded dz.  `forward(..., bf16_round=True)` and
+`loss_and_grads(..., bf16_round=True)` apply the same rounding so the device
 results can be compared to FP64 arithmetic at the same rounding points;
-`bf16=False` is the plain FP64 network, pinned by finite differences.
+`bf16_round=False` is the plain FP64 network, pinned by finite differences.
 """
 import numpy as np
 
@@ -60,6 +64,13 @@ def bf16(x):
     return rounded.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def split(x):
+    """hi + lo of the two BF16 terms an activation is carried as."""
+    x = np.asarray(x, np.float64)
+    hi = bf16(x)
+    return hi + bf16(x - hi)
+
+
 def silu(z):
     return z / (1.0 + np.exp(-z))
 
@@ -76,18 +87,19 @@ def normalise(x, norm):
 
 def forward(flat, x, norm, bf16_round=False):
     """Returns (y [B,4] normalised outputs, cache for backward)."""
-    r = bf16 if bf16_round else (lambda v: np.asarray(v, np.float64))
+    plain = lambda v: np.asarray(v, np.float64)
+    rw, ra = (bf16, split) if bf16_round else (plain, plain)
     layers = unpack(np.asarray(flat, np.float32 if bf16_round else np.float64))
     h = np.zeros((len(x), IN_PAD))
     h[:, :INPUTS] = normalise(x, norm)
-    h = r(h)
+    h = ra(h)
     cache = []
     for i, (W, b) in enumerate(layers):
-        Wr = r(W)
+        Wr = rw(W)
         z = h @ Wr + b.astype(np.float64)
         cache.append((h, Wr, z))
         if i < LAYERS:
-            h = r(silu(z))
+            h = ra(silu(z))
         else:
             h = z
     return h[:, :OUTPUTS], cache
