@@ -355,22 +355,52 @@ __device__ __forceinline__ Blend make_blend(const Args &a, const Root &r) {
     return b;
 }
 
-// evaluate_base_strand_cv (generate.h:246-267, 300-310) from the offset
-// table, fast form (positions only): the weighted mean multiplies by 1/sum.
-// At cv 0 every offset is exactly zero, so the first CV is the root.
-__device__ __forceinline__ V3 base_cv(const Args &a, const Root &r, const Blend &b, uint32_t cv) {
-    V3 off = v3(0.0f, 0.0f, 0.0f);
+// The blend as the CV loop reads it: per stencil slot the byte offset of
+// the guide's row in the offset table and the slot's weight, an empty slot
+// reading guide 0's row with weight 0 (it adds +-0), so that the eight
+// gathers of a CV are unconditional and independent; the nearest-guide
+// fallback (no weight in reach) is one slot of weight 1 and scale 1, and a
+// strand with no guide at all follows its normal.
+struct BlendRows {
+    uint32_t off[8];
+    float w[8];
+    float scale;         // 1 / sum, or 1 for the nearest guide
+    float length_scale;  // length_scale * length attribute
+    bool along_normal;   // no guide in reach and no nearest guide
+};
+
+__device__ __forceinline__ BlendRows blend_rows(const Args &a, const Blend &b) {
+    BlendRows br;
+    const uint32_t row = a.cvs * 16u;
+    const bool weighted = b.sum > 1.0e-12f;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        if (b.idx[j] == NXG_INVALID_INDEX) continue;
-        const float4 o = a.guide_offsets[uint64_t{b.idx[j]} * a.cvs + cv];
-        off = fma_v(v3(o.x, o.y, o.z), b.w[j], off);
+        const bool use = weighted ? b.idx[j] != NXG_INVALID_INDEX : (j == 0 && b.nearest != NXG_INVALID_INDEX);
+        const uint32_t g = weighted ? b.idx[j] : b.nearest;
+        br.off[j] = use ? g * row : 0u;
+        br.w[j] = use ? (weighted ? b.w[j] : 1.0f) : 0.0f;
     }
-    if (b.sum > 1.0e-12f) {
-        off = off * b.inv_sum;
-    } else if (b.nearest != NXG_INVALID_INDEX) {
-        const float4 o = a.guide_offsets[uint64_t{b.nearest} * a.cvs + cv];
-        off = v3(o.x, o.y, o.z);
+    br.scale = weighted ? b.inv_sum : 1.0f;
+    br.length_scale = b.length_scale;
+    br.along_normal = !weighted && b.nearest == NXG_INVALID_INDEX;
+    return br;
+}
+
+// evaluate_base_strand_cv (generate.h:246-267, 300-310) from the offset
+// table, fast form (positions only): the weighted mean multiplies by 1/sum.
+// At cv 0 every offset is exactly zero, so the first CV is the root.  The
+// row base of the CV is wave-uniform, the slot offsets per lane.
+__device__ __forceinline__ V3 base_cv(const Args &a, const Root &r, const BlendRows &b, uint32_t cv) {
+    V3 off;
+    if (!b.along_normal) {
+        const uint8_t *rows = reinterpret_cast<const uint8_t *>(a.guide_offsets) + cv * 16u;
+        float4 o[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = *reinterpret_cast<const float4 *>(rows + b.off[j]);
+        off = v3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) off = fma_v(v3(o[j].x, o[j].y, o[j].z), b.w[j], off);
+        off = off * b.scale;
     } else {
         const float t = static_cast<float>(cv) / static_cast<float>(a.cvs - 1u);
         off = r.normal * t;
@@ -419,7 +449,7 @@ struct NoiseState {
 // evaluate_and_advance_noise_cursor (generate.h:459-515); OCT octaves share
 // the transported frame and the arc length and their displacements add.
 template <int OCT, bool RT = false>
-__device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend &b,
+__device__ __forceinline__ V3 advance(const Args &a, const Root &r, const BlendRows &b,
                                       NoiseState<OCT> &ns, Cursor &c, const float4 *grad) {
     if (c.cv > 0u) c.arc += c.step;
     V3 nt = c.tangent;
@@ -462,7 +492,7 @@ __device__ __forceinline__ V3 advance(const Args &a, const Root &r, const Blend 
 }
 
 template <int OCT>
-__device__ __forceinline__ Cursor make_cursor(const Args &a, const Root &r, const Blend &b, V3 surface_u) {
+__device__ __forceinline__ Cursor make_cursor(const Args &a, const Root &r, const BlendRows &b, V3 surface_u) {
     Cursor c;
     c.cv = 0u;
     c.cur = base_cv(a, r, b, 0u);
@@ -663,15 +693,16 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
         }
     }
 
+    const BlendRows br = blend_rows(a, b);
     NoiseState<OCT> ns;
     V3 surface_u = v3(0.0f, 0.0f, 0.0f);
     float base_length = 0.0f;
     if constexpr (NOISE) {
         surface_u = root_surface_u(a, r);
         // base_strand_length (generate.h:420-433)
-        V3 prev = base_cv(a, r, b, 0u);
+        V3 prev = base_cv(a, r, br, 0u);
         for (uint32_t cv = 1u; cv < a.cvs; ++cv) {
-            const V3 c = base_cv(a, r, b, cv);
+            const V3 c = base_cv(a, r, br, cv);
             base_length += sqrtf(len2(c - prev));
             prev = c;
         }
@@ -698,7 +729,7 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
         const bool emit = pass + 1u == passes;
         Cursor cur;
         if constexpr (NOISE) {
-            cur = make_cursor<OCT>(a, r, b, surface_u);
+            cur = make_cursor<OCT>(a, r, br, surface_u);
 #pragma unroll
             for (int k = 0; k < OCT; ++k)
 #pragma unroll
@@ -720,9 +751,9 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
                 const uint32_t cv = cv0 + k;
                 V3 p;
                 if constexpr (NOISE) {
-                    p = advance<OCT, RT>(a, r, b, ns, cur, grad);
+                    p = advance<OCT, RT>(a, r, br, ns, cur, grad);
                 } else {
-                    p = base_cv(a, r, b, cv);
+                    p = base_cv(a, r, br, cv);
                 }
                 if (!emit) {
                     if (cv > 0u) noisy += sqrtf(len2(p - last));
@@ -1278,6 +1309,9 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
     const WorkspaceLayout L =
         workspace_layout(h.guide_count, params->cvs_per_strand, params->strand_count, h.triangle_count);
     if (!workspace || workspace_bytes < L.total) return NXG_ERROR_WORKSPACE_TOO_SMALL;
+    // the strand kernel addresses the guide-offset table with 32-bit byte offsets
+    if (uint64_t{h.guide_count} * params->cvs_per_strand * 16u > 0xFFFFFFFFull)
+        return NXG_ERROR_INVALID_GENERATION_PARAMETERS;
 
     const nxg_vec3 *d_pos = deformed ? deformed->positions : nullptr;
     const nxg_vec3 *d_nrm = deformed ? deformed->normals : nullptr;
