@@ -577,8 +577,8 @@ __device__ float classic_noise(float sx, float sy, float sz) {
         seed = seed * 1664525u + static_cast<uint32_t>(iy + oy) + 1013904223u;
         seed = seed * 1664525u + static_cast<uint32_t>(iz + oz) + 1013904223u;
         seed ^= seed >> 11u;
-        seed ^= (seed << 7u) & 0x9d2c5680u;
-        seed ^= (seed << 15u) & 0xefc60000u;
+        seed = xor_and(seed, seed << 7u, 0x9d2c5680u);
+        seed = xor_and(seed, seed << 15u, 0xefc60000u);
         seed ^= seed >> 18u;
         const float4 g = kClumpGradients[(((seed & 0x00ff0000u) >> 4u) + (seed & 0xffu)) & 0xffu];
         v[c] = g.x * (wx - static_cast<float>(ox)) + g.y * (wy - static_cast<float>(oy)) +

@@ -20,11 +20,16 @@ __constant__ uint8_t kPermutation[256] = {
 #include "seexpr_permutation.inc"
 };
 
+// x ^ (y & m) in one v_bitop3 (truth table 0x78, indexed x*4 + y*2 + m).
+__device__ __forceinline__ uint32_t xor_and(uint32_t x, uint32_t y, uint32_t m) {
+    return __builtin_amdgcn_bitop3_b32(x, y, m, 0x78);
+}
+
 // xgen_seexpr_hash_finish (xgen_expression.cpp:637-651)
 __device__ uint32_t seexpr_hash_finish(uint32_t state) {
     state ^= state >> 11u;
-    state ^= (state << 7u) & 0x9d2c5680u;
-    state ^= (state << 15u) & 0xefc60000u;
+    state = xor_and(state, state << 7u, 0x9d2c5680u);
+    state = xor_and(state, state << 15u, 0xefc60000u);
     state ^= state >> 18u;
     const uint32_t b3 = kPermutation[state & 0xffu];
     const uint32_t b2 = kPermutation[((state >> 8u) + b3) & 0xffu];

@@ -106,8 +106,8 @@ constexpr uint32_t kLcgK = kLcgC * kLcgA2 + kLcgC * kLcgA + kLcgC;
 
 __device__ __forceinline__ uint32_t temper_index(uint32_t seed) {
     seed ^= seed >> 11u;
-    seed ^= (seed << 7u) & 0x9d2c5680u;
-    seed ^= (seed << 15u) & 0xefc60000u;
+    seed = xor_and(seed, seed << 7u, 0x9d2c5680u);
+    seed = xor_and(seed, seed << 15u, 0xefc60000u);
     seed ^= seed >> 18u;
     return (((seed & 0x00ff0000u) >> 4u) + (seed & 0xffu)) & 0xffu;
 }
@@ -250,6 +250,7 @@ struct Args {
     float noise_mask, decorrelation_scale, preserve, inv_span;
     float amp[NXG_MAX_NOISE_OCTAVES];
     float freq[NXG_MAX_NOISE_OCTAVES];
+    float min_freq;  // lowest octave frequency
     nxg_packed_point *points;
     nxg_root_sample *roots;
     nxg_vec2 *root_uvs;
@@ -699,12 +700,22 @@ __device__ __forceinline__ void strand_tile(const Args &a, uint64_t wave_first, 
     float base_length = 0.0f;
     if constexpr (NOISE) {
         surface_u = root_surface_u(a, r);
-        // base_strand_length (generate.h:420-433)
-        V3 prev = base_cv(a, r, br, 0u);
-        for (uint32_t cv = 1u; cv < a.cvs; ++cv) {
-            const V3 c = base_cv(a, r, br, cv);
-            base_length += sqrtf(len2(c - prev));
-            prev = c;
+        // base_strand_length (generate.h:420-433) sets the effective
+        // frequency max(0.5 / length, f) and the preserve-length target.
+        // Without preserve-length the length is needed only when 0.5 / length
+        // could reach the lowest octave frequency: the chord from the root to
+        // the last base CV bounds the polyline's length from below, so a
+        // chord past 0.5 / f_min (with a margin for the float sums) fixes
+        // every frequency at f and the pass over the base CVs is skipped.
+        const V3 tip = base_cv(a, r, br, a.cvs - 1u);
+        const float chord = sqrtf(len2(tip - r.position));
+        if (a.preserve > 0.001f || !(a.min_freq > 0.0f) || !(chord * a.min_freq > 0.5f * 1.001f)) {
+            V3 prev = base_cv(a, r, br, 0u);
+            for (uint32_t cv = 1u; cv < a.cvs; ++cv) {
+                const V3 c = base_cv(a, r, br, cv);
+                base_length += sqrtf(len2(c - prev));
+                prev = c;
+            }
         }
 #pragma unroll
         for (int k = 0; k < OCT; ++k) {
@@ -1370,6 +1381,8 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
         a.amp[0] = params->noise_amplitude;
         a.freq[0] = params->noise_frequency;
     }
+    a.min_freq = a.freq[0];
+    for (uint32_t k = 1; k < a.octave_count; ++k) a.min_freq = a.freq[k] < a.min_freq ? a.freq[k] : a.min_freq;
     a.noise = plan.noise ? 1u : 0u;
     if (output) {
         a.points = output->points;
