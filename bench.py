@@ -37,6 +37,39 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host's CPU model, its logical CPU count and this process's
+    affinity set: the facts a CPU baseline is read against."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def baseline_threads():
+    """Worker threads for a CPU baseline: the affinity set, at most 16."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def sampled_seconds(run, N, target_s, first=65536):
+    """Time run(s) (seconds for the first s strands) on a sample sized to
+    about target_s of work, capped at N; whole-workload runs repeat (median
+    of up to 5).  Returns (s, median seconds, runs)."""
+    s = min(N, first)
+    dt = run(s)
+    s = int(min(N, max(s, s * target_s / max(dt, 1e-6))))
+    times = [run(s)]
+    while s == N and sum(times) < target_s and len(times) < 5:
+        times.append(run(s))
+    return s, sorted(times)[len(times) // 2], len(times)
+
+
 def cpu_baseline(blob, w, target_s, threads):
     """The oracle's multithreaded CPU path (128-strand tiles over an atomic
     counter, asset.cpp:160-202) on the same workload: per-root attribute
@@ -65,10 +98,10 @@ def cpu_baseline(blob, w, target_s, threads):
     dt = sorted(times)[len(times) // 2]
     pts = s * w.params.cvs_per_strand
     what = "attribute programs + generation" if programs else "generation"
-    return {"value": pts / dt, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"strands 0..{s - 1} of the same description ({pts} points), {what}, median "
-                      f"{dt:.3f} s of {len(times)} run(s); float restatement of the reference CPU path "
-                      f"(oracle/), {threads} threads"}
+    return dict({"value": pts / dt, "unit": "points/s", "cores": threads, "kind": "port",
+                 "sample": f"strands 0..{s - 1} of the same description ({pts} points), {what}, median "
+                           f"{dt:.3f} s of {len(times)} run(s); float restatement of the reference CPU path "
+                           f"(oracle/), {threads} threads"}, **host_cpu())
 
 
 SURROGATE_PMC_SUMMARY = os.path.join(ROOT, "profiles", "surrogate_kernel_plane1M_pmc.json")
@@ -332,7 +365,7 @@ def bench_batch(args):
     if not args.no_cpu_baseline:
         import oracle
         oracle.lib()
-        threads = min(16, os.cpu_count() or 1)
+        threads = baseline_threads()
         frac = 0.002
         cpu_pts, cpu_s, tries = 0, 0.0, 0
         while tries < 4:
@@ -349,10 +382,10 @@ def bench_batch(args):
                 break
             frac = min(1.0, frac * args.cpu_seconds / max(cpu_s, 1e-3))
             tries += 1
-        cpu = {"value": cpu_pts / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
-               "sample": f"the first {frac:.4f} of every description's strands ({cpu_pts} points), attribute "
-                         f"programs + generation, {cpu_s:.2f} s; float restatement of the reference CPU path "
-                         f"(oracle/), {threads} threads"}
+        cpu = dict({"value": cpu_pts / cpu_s, "unit": "points/s", "cores": threads, "kind": "port",
+                    "sample": f"the first {frac:.4f} of every description's strands ({cpu_pts} points), attribute "
+                              f"programs + generation, {cpu_s:.2f} s; float restatement of the reference CPU path "
+                              f"(oracle/), {threads} threads"}, **host_cpu())
     line = {"metric": METRIC, "value": points / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -467,7 +500,7 @@ def bench_classic(args):
                              "frac": round(end_bytes / (end_mean * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = classic_cpu_baseline(gen, N, Cv, scale, offset, args.cpu_seconds)
+        cpu = classic_cpu_baseline(gen.to_host(), Cv, scale, offset, args.cpu_seconds, baseline_threads())
     line = {"metric": METRIC, "value": kept * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -613,8 +646,8 @@ def bench_classic_expr(args, with_clump=False, with_noise=False):
                 "algorithmic_bytes_per_launch": alg, "valu": valu}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, args.cpu_seconds, guide_host,
-                                        tangents if with_noise else None)
+        cpu = classic_expr_cpu_baseline(gen.to_host(), Cv, columns, prefixes, args.cpu_seconds, baseline_threads(),
+                                        guide_host, tangents if with_noise else None)
     plan_text = dict(CLASSIC_EXPR_AUTHORED, **({"clump": CLASSIC_PLAN_CLUMP} if with_clump else {}),
                      **({"noise": CLASSIC_NOISE} if with_noise else {}))
     line = {"metric": METRIC, "value": kept * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
@@ -629,17 +662,18 @@ def bench_classic_expr(args, with_clump=False, with_noise=False):
     return 0
 
 
-def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s, guides=None, tangents=None):
-    """The reference's parser, plan compiler and
-    apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread
-    pool) on the first S generated strands; without that build, the oracle's
-    single-threaded restatement.  guides (host arrays) adds the clump."""
+def classic_expr_cpu_baseline(host, Cv, columns, prefixes, target_s, threads, guides=None, tangents=None,
+                              use_ref=False):
+    """The oracle's restatement of the Classic runtime chain
+    (oracle/oracle_classic.c, strands over `threads` workers, then the stable
+    compaction) on the first S strands of `host` (the generated curves as
+    host arrays) with the same plan; use_ref times the reference's parser,
+    plan compiler and apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref,
+    its own thread pool) instead.  guides (host arrays) adds the clump."""
     import oracle
     from nanoxgen_amd.classic import ClassicPlan
-    host = gen.to_host()
     pts, counts, roots, uvs = host["points"], host["point_counts"], host["roots"], host["root_uvs"]
-    use_ref = oracle.ref() is not None
-    cores = len(os.sched_getaffinity(0)) if use_ref else 1
+    N = len(counts)
     text = classic_expr_xgen(guides is not None, tangents is not None)
     plan = ClassicPlan(**CLASSIC_EXPR_BOUND, clump=CLASSIC_PLAN_CLUMP if guides is not None else None,
                        noise=CLASSIC_NOISE if tangents is not None else None,
@@ -659,22 +693,29 @@ def classic_expr_cpu_baseline(gen, N, Cv, columns, prefixes, target_s, guides=No
                                            random_prefixes=prefixes[:s], columns=columns[:s], **clump_kw(s))
             return r["seconds"]
         t = time.perf_counter()
-        oracle.classic_runtime(pts[:s * Cv], Cv, plan, roots=roots[:s], counts=counts[:s], uvs=uvs[:s],
-                               random_prefixes=prefixes[:s], columns=columns[:s], **clump_kw(s))
-        return time.perf_counter() - t
+        r = oracle.classic_runtime(pts[:s * Cv], Cv, plan, roots=roots[:s], counts=counts[:s], uvs=uvs[:s],
+                                   random_prefixes=prefixes[:s], columns=columns[:s], workers=threads,
+                                   **clump_kw(s))
+        dt = time.perf_counter() - t
+        assert r["strand_count"] is not None, r["error_bits"]
+        return dt
 
-    s = min(N, 65536)
-    dt = run(s)
-    s = int(min(N, max(s, s * target_s / max(dt, 1e-6))))
-    times = [run(s)]
-    while s == N and sum(times) < target_s and len(times) < 5:
-        times.append(run(s))
-    dt = sorted(times)[len(times) // 2]
-    what = ("the reference's apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread pool)"
-            if use_ref else "the oracle's restatement (oracle/oracle_classic.c), one thread")
-    return {"value": s * Cv / dt, "unit": "points/s", "cores": cores, "kind": "reference" if use_ref else "port",
-            "sample": f"strands 0..{s - 1} of the generated curves ({s * Cv} points), the same plan, "
-                      f"median {dt:.3f} s of {len(times)} run(s); {what}"}
+    s, dt, runs = sampled_seconds(run, N, target_s)
+    return baseline_record(s, Cv, dt, runs, threads, use_ref, "the same plan",
+                           "apply_xgen_classic_float_runtime_plan_cpu")
+
+
+def baseline_record(s, Cv, dt, runs, threads, use_ref, what, ref_entry):
+    """A Classic workload's cpu_baseline object: the restatement ("port", on
+    `threads` workers) or the reference build ("reference", its own pool)."""
+    who = (f"the reference's {ref_entry} (oracle/_ref, its own thread pool, seconds measured inside the call)"
+           if use_ref else f"the oracle's restatement (oracle/oracle_classic.c, oracle_clump.c), {threads} threads; "
+                           "oracle/standin_check.py times it against the reference build on the same host")
+    return dict({"value": s * Cv / dt, "unit": "points/s",
+                 "cores": len(os.sched_getaffinity(0)) if use_ref else threads,
+                 "kind": "reference" if use_ref else "port",
+                 "sample": f"strands 0..{s - 1} of the generated curves ({s * Cv} points), {what}, "
+                           f"median {dt:.3f} s of {runs} run(s); {who}"}, **host_cpu())
 
 
 CLUMP_PARAMS = dict(mask=0.8, amount=0.7, clump_scale=0.1, noise=0.3, noise_scale=1.0, noise_frequency=2.0,
@@ -753,7 +794,7 @@ def bench_clump(args):
                 "algorithmic_bytes_per_launch": alg, "valu": valu}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = clump_cpu_baseline(gen, N, Cv, cvs3, recs, args.cpu_seconds)
+        cpu = clump_cpu_baseline(gen.to_host(), Cv, cvs3, recs, args.cpu_seconds, baseline_threads())
     line = {"metric": METRIC, "value": N * Cv / (ms_per_step * 1e-3), "unit": "points/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -764,39 +805,33 @@ def bench_clump(args):
     return 0
 
 
-def clump_cpu_baseline(gen, N, Cv, cvs3, recs, target_s):
-    """The reference's own float runtime with the same ClumpingFX module
-    (oracle/_ref, its own thread pool) on the first S generated strands; the
-    oracle's single-threaded restatement without that build."""
+def clump_cpu_baseline(host, Cv, cvs3, recs, target_s, threads, use_ref=False):
+    """The oracle's restatement of ClumpingFX with constant module
+    expressions (oracle/oracle_clump.c, strands over `threads` workers) on
+    the first S strands of `host`, each clumped to its stencil's slot-0
+    guide; use_ref times the reference's float runtime with the same module
+    (oracle/_ref, its own thread pool; its time includes the shim's copies)."""
     import numpy as np
     import oracle
-    host = gen.to_host()
     pts = host["points"]
+    N = len(host["point_counts"])
     sg = host["stencil_indices"][:, 0].copy()
     guides = dict(axes=cvs3, normals=np.ascontiguousarray(recs["root_normal"], np.float32),
                   tangents=np.zeros((len(recs), 3), np.float32))
-    use_ref = oracle.ref() is not None
-    cores = len(os.sched_getaffinity(0)) if use_ref else 1
+    gradients = oracle.seexpr_gradients()
 
     def run(s):
         t = time.perf_counter()
         if use_ref:
             oracle.ref_clump_apply(pts[:s * Cv], Cv, sg[:s], guides, CLUMP_PARAMS)
         else:
-            raise RuntimeError("no CPU baseline without the reference build")
+            r = oracle.clump_apply(pts[:s * Cv], Cv, sg[:s], guides, CLUMP_PARAMS, gradients, workers=threads)
+            assert r is not None
         return time.perf_counter() - t
 
-    s = min(N, 65536)
-    dt = run(s)
-    s = int(min(N, max(s, s * target_s / max(dt, 1e-6))))
-    times = [run(s)]
-    while s == N and sum(times) < target_s and len(times) < 5:
-        times.append(run(s))
-    dt = sorted(times)[len(times) // 2]
-    return {"value": s * Cv / dt, "unit": "points/s", "cores": cores, "kind": "reference",
-            "sample": f"strands 0..{s - 1} of the generated curves ({s * Cv} points), one ClumpingFX module, "
-                      f"median {dt:.3f} s of {len(times)} run(s) including the shim's copies in and out; "
-                      "the reference's apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread pool)"}
+    s, dt, runs = sampled_seconds(run, N, target_s)
+    return baseline_record(s, Cv, dt, runs, threads, use_ref, "one ClumpingFX module",
+                           "float runtime with one ClumpingFX module")
 
 
 def bench_transport(args):
@@ -867,17 +902,14 @@ def bench_transport(args):
     return 0
 
 
-def classic_cpu_baseline(gen, N, Cv, scale, offset, target_s):
-    """The reference's own apply_xgen_classic_float_runtime_plan_cpu (built
-    from its sources into oracle/_ref) on the first S generated strands, with
-    the worker count it picks itself (the process's CPU affinity); without
-    that build, the oracle's single-threaded restatement."""
-    import numpy as np
+def classic_cpu_baseline(host, Cv, scale, offset, target_s, threads, use_ref=False):
+    """The oracle's restatement of the CutFX chain (oracle/oracle_classic.c,
+    strands over `threads` workers, then the stable compaction) on the first
+    S strands of `host`; use_ref times the reference's own
+    apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own pool)."""
     import oracle
-    host = gen.to_host()
     pts, counts, roots, uvs = host["points"], host["point_counts"], host["roots"], host["root_uvs"]
-    use_ref = oracle.ref() is not None
-    cores = len(os.sched_getaffinity(0)) if use_ref else 1
+    N = len(counts)
 
     def run(s):
         if use_ref:
@@ -885,21 +917,15 @@ def classic_cpu_baseline(gen, N, Cv, scale, offset, target_s):
                                        uvs=uvs[:s])
             return r["seconds"]
         t = time.perf_counter()
-        oracle.classic_cut(pts[:s * Cv], Cv, scale, offset)
-        return time.perf_counter() - t
+        r = oracle.classic_cut(pts[:s * Cv], Cv, scale, offset, counts=counts[:s], roots=roots[:s], uvs=uvs[:s],
+                               workers=threads)
+        dt = time.perf_counter() - t
+        assert r is not None
+        return dt
 
-    s = min(N, 65536)
-    dt = run(s)
-    s = int(min(N, max(s, s * target_s / max(dt, 1e-6))))
-    times = [run(s)]
-    while s == N and sum(times) < target_s and len(times) < 5:
-        times.append(run(s))
-    dt = sorted(times)[len(times) // 2]
-    what = ("the reference's apply_xgen_classic_float_runtime_plan_cpu (oracle/_ref, its own thread pool)"
-            if use_ref else "the oracle's restatement (oracle/oracle_classic.c), one thread")
-    return {"value": s * Cv / dt, "unit": "points/s", "cores": cores, "kind": "reference" if use_ref else "port",
-            "sample": f"strands 0..{s - 1} of the generated curves ({s * Cv} points), one CutFX at "
-                      f"{scale} * cLength, median {dt:.3f} s of {len(times)} run(s); {what}"}
+    s, dt, runs = sampled_seconds(run, N, target_s)
+    return baseline_record(s, Cv, dt, runs, threads, use_ref, f"one CutFX at {scale} * cLength",
+                           "apply_xgen_classic_float_runtime_plan_cpu")
 
 
 def main():
@@ -1044,8 +1070,7 @@ def main():
                 "algorithmic_bytes_per_launch": alg_bytes, "valu": valu}
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(blob, w, args.cpu_seconds, threads)
+        cpu = cpu_baseline(blob, w, args.cpu_seconds, baseline_threads())
     # the metric's surrogate clause: the learned-surrogate workload beside the
     # exact runtime at the stated threshold (its own timed steps)
     sur = None

@@ -81,17 +81,18 @@ def lib():
         L.orc_clump_apply.restype = C.c_uint32
         L.orc_clump_apply.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
-                                      C.c_void_p, C.c_void_p, C.c_uint32]
+                                      C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
         L.orc_seexpr_gradients.restype = None
         L.orc_seexpr_gradients.argtypes = [C.c_void_p]
         L.orc_classic_runtime.restype = C.c_uint32
         L.orc_classic_runtime.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
-                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_uint32]
         L.orc_classic_cut.restype = C.c_uint32
         L.orc_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p,
-                                      C.c_void_p, C.c_void_p, C.c_void_p]
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
         _LIB = L
     return _LIB
 
@@ -342,7 +343,7 @@ def curve_spline_length(points):
 
 
 def classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, roots=None, uvs=None, multiplier=None,
-                radius_scale=1.0):
+                radius_scale=1.0, workers=0):
     """The CutFX chain (cLength, culling, cut from tip, compaction) over
     packed float4 points [N * C, 4]: returns dict(points, point_counts,
     roots, root_uvs, keep, strand_count), or None where the reference
@@ -358,7 +359,7 @@ def classic_cut(points, cvs, amount_scale, amount_offset, *, counts=None, roots=
     m = None if multiplier is None else np.ascontiguousarray(multiplier, np.float32)
     kept = lib().orc_classic_cut(pts.ctypes.data, counts.ctypes.data, abi.ptr(roots), abi.ptr(uvs), n, cvs,
                                  amount_scale, amount_offset, abi.ptr(m), radius_scale, out.ctypes.data,
-                                 oc.ctypes.data, abi.ptr(orr), abi.ptr(ou), keep.ctypes.data)
+                                 oc.ctypes.data, abi.ptr(orr), abi.ptr(ou), keep.ctypes.data, workers)
     if kept == 0xFFFFFFFF:
         return None
     return dict(points=out[:kept * cvs], point_counts=oc[:kept], roots=None if orr is None else orr[:kept],
@@ -422,7 +423,8 @@ def _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes)
 
 def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitive_ids=None, random_prefixes=None,
                     columns=None, radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None,
-                    guide_random_prefixes=None, clumps=None, surface_tangents=None, noise_domain_positions=None):
+                    guide_random_prefixes=None, clumps=None, surface_tangents=None, noise_domain_positions=None,
+                    workers=0):
     """orc_classic_runtime: the Classic runtime chain of `plan` (a
     nanoxgen_amd.classic.ClassicPlan: its compiled, bound programs) over
     packed float4 points [N * C, 4]; each clump module of the plan takes a
@@ -483,7 +485,7 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
     kept = lib().orc_classic_runtime(pts.ctypes.data, abi.ptr(cc), rr.ctypes.data, abi.ptr(uu), n, cvs,
                                      C.byref(op), abi.ptr(ids), abi.ptr(pre), abi.ptr(cols), ncol, radius_scale,
                                      out.ctypes.data, oc.ctypes.data, orr.ctypes.data, ou.ctypes.data,
-                                     keep.ctypes.data, C.byref(bits))
+                                     keep.ctypes.data, C.byref(bits), workers)
     if kept == 0xFFFFFFFF:
         return dict(strand_count=None, error_bits=bits.value, keep=keep)
     return dict(points=out[:kept * cvs], point_counts=oc[:kept], roots=orr[:kept],
@@ -576,7 +578,8 @@ def clump_prepare(axes, cvs):
     return render, lengths
 
 
-def clump_apply(points, cvs, strand_guides, guides, params, gradients, amount_multiplier=None, multiplier_stride=1):
+def clump_apply(points, cvs, strand_guides, guides, params, gradients, amount_multiplier=None, multiplier_stride=1,
+                workers=0):
     """ClumpingFX with constant expressions (params: dict of CLUMP_KEYS) on
     packed strands; guides: dict(axes, normals, tangents[, reference_positions]);
     gradients: the 256 x 3 SeExpr table.  Returns the new points, or None
@@ -595,7 +598,7 @@ def clump_apply(points, cvs, strand_guides, guides, params, gradients, amount_mu
     m = None if amount_multiplier is None else np.ascontiguousarray(amount_multiplier, np.float32)
     rc = lib().orc_clump_apply(pts.ctypes.data, n, cvs, sg.ctypes.data, ax.ctypes.data, render.ctypes.data,
                                lengths.ctypes.data, nrm.ctypes.data, tan.ctypes.data, abi.ptr(ref3), g,
-                               grad.ctypes.data, k, abi.ptr(m), multiplier_stride)
+                               grad.ctypes.data, k, abi.ptr(m), multiplier_stride, workers)
     return None if rc else pts
 
 

@@ -114,7 +114,7 @@ uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, con
                          const float *axes3, const float *render3, const float *lengths, const float *normals3,
                          const float *tangents3, const float *reference3, uint32_t guides,
                          const nxg_vec3 *gradients, const orc_clump_params *p, const float *amount_multiplier,
-                         uint32_t multiplier_stride);
+                         uint32_t multiplier_stride, uint32_t workers);
 /* Classic primitive expressions and a CutFX amount expression
  * (oracle_classic.c, orc_classic_runtime).  Program inputs bind to
  * ORC_CLASSIC_VAR_* (3 + k: column k of the per-strand input row). */
@@ -169,17 +169,21 @@ void orc_clump_noise_axis(const float *axis3, const float *distance3, uint32_t n
                           const float *magnitudes, const nxg_vec3 *gradients, float *out3);
 void orc_cut_to_length(float *points3, uint32_t n, float target);
 /* Returns the kept count (outputs compacted in order), or ~0u with the
- * ORC_CLASSIC_* bits of every strand that raised one in *error_bits. */
+ * ORC_CLASSIC_* bits of every strand that raised one in *error_bits.
+ * out_points and keep hold strand_count entries; strands run in parallel
+ * over `workers` threads (0 = all cores), with identical results. */
 uint32_t orc_classic_runtime(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
                              const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs,
                              const orc_classic_plan *plan, const uint32_t *primitive_ids,
                              const uint32_t *random_prefixes, const float *columns, uint32_t column_count,
                              float radius_scale, float *out_points, uint32_t *out_counts,
-                             nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep, uint32_t *error_bits);
+                             nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep, uint32_t *error_bits,
+                             uint32_t workers);
 uint32_t orc_classic_cut(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
                          const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, float amount_scale,
                          float amount_offset, const float *multiplier, float radius_scale, float *out_points,
-                         uint32_t *out_counts, nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep);
+                         uint32_t *out_counts, nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep,
+                         uint32_t workers);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "oracle.h"
@@ -124,48 +125,85 @@ float orc_curve_spline_length(const float *points4, uint32_t n) {
     return orc_curve_spline_length_v(p, n);
 }
 
-/* The cut chain over `strand_count` packed strands of `cvs` points (float4
- * position + radius), compacting the kept strands, in order, into the
- * outputs; returns the kept count (or ~0u on a non-finite width or amount,
- * which the reference reports as an error).  multiplier may be NULL. */
-uint32_t orc_classic_cut(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
-                         const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, float amount_scale,
-                         float amount_offset, const float *multiplier, float radius_scale, float *out_points,
-                         uint32_t *out_counts, nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep) {
-    cv3 p[256], source[256];
-    if (cvs < 2u || cvs > 256u) return ~0u;
+/* Stable compaction of the strands whose flags[s] is 1 (kept, no error):
+ * strand s's points were written at slot s of out_points, and each kept one
+ * moves to the next free slot in strand order (src/xgen_classic_runtime.cpp:
+ * 1893-1918); counts, roots and uvs follow it.  Returns the kept count. */
+static uint32_t compact_slots(uint32_t strand_count, uint32_t cvs, const uint8_t *flags, const uint32_t *counts,
+                              const nxg_root_sample *roots, const nxg_vec2 *uvs, float *out_points,
+                              uint32_t *out_counts, nxg_root_sample *out_roots, nxg_vec2 *out_uvs) {
     uint32_t write = 0u;
     for (uint32_t s = 0; s < strand_count; ++s) {
-        const float *in = points + (uint64_t)s * cvs * 4u;
-        for (uint32_t i = 0; i < cvs; ++i) p[i] = c3(in[4 * i], in[4 * i + 1], in[4 * i + 2]);
-        const float c_length = orc_curve_spline_length_v(p, cvs);
-        const float raw_width = radius_scale > 0.0f ? 2.0f * in[3] / radius_scale : 0.0f;
-        if (!isfinite(raw_width)) return ~0u;
-        uint8_t k = 1u;
-        if (c_length < 1.0e-4f || raw_width < 1.0e-4f) {
-            k = 0u;
-        } else {
-            float amount = amount_scale * c_length + amount_offset;
-            if (multiplier) amount = amount * multiplier[s];
-            if (!isfinite(amount)) return ~0u;
-            const float cut = orc_cut_from_tip_v(p, cvs, max_std(amount, 0.0f), source);
-            if (cut < 1.0e-4f) k = 0u;
-        }
-        if (keep) keep[s] = k;
-        if (!k) continue;
-        float *out = out_points + (uint64_t)write * cvs * 4u;
-        for (uint32_t i = 0; i < cvs; ++i) {
-            out[4 * i] = p[i].x;
-            out[4 * i + 1] = p[i].y;
-            out[4 * i + 2] = p[i].z;
-            out[4 * i + 3] = in[4 * i + 3];
-        }
+        if (flags[s] != 1u) continue;
+        if (write != s)
+            memmove(out_points + (uint64_t)write * cvs * 4u, out_points + (uint64_t)s * cvs * 4u,
+                    sizeof(float) * 4u * cvs);
         if (out_counts) out_counts[write] = counts ? counts[s] : cvs;
         if (out_roots && roots) out_roots[write] = roots[s];
         if (out_uvs && uvs) out_uvs[write] = uvs[s];
         ++write;
     }
     return write;
+}
+
+typedef struct {
+    const float *points, *multiplier;
+    uint32_t cvs;
+    float amount_scale, amount_offset, radius_scale;
+    float *out_points;
+    uint8_t *flags; /* 0 culled, 1 kept, 2 non-finite width or amount */
+} cut_ctx;
+
+static void cut_one(void *c, uint32_t s) {
+    const cut_ctx *x = (const cut_ctx *)c;
+    cv3 p[256], source[256];
+    const uint32_t cvs = x->cvs;
+    const float *in = x->points + (uint64_t)s * cvs * 4u;
+    for (uint32_t i = 0; i < cvs; ++i) p[i] = c3(in[4 * i], in[4 * i + 1], in[4 * i + 2]);
+    const float c_length = orc_curve_spline_length_v(p, cvs);
+    const float raw_width = x->radius_scale > 0.0f ? 2.0f * in[3] / x->radius_scale : 0.0f;
+    x->flags[s] = 0u;
+    if (!isfinite(raw_width)) {
+        x->flags[s] = 2u;
+        return;
+    }
+    if (c_length < 1.0e-4f || raw_width < 1.0e-4f) return;
+    float amount = x->amount_scale * c_length + x->amount_offset;
+    if (x->multiplier) amount = amount * x->multiplier[s];
+    if (!isfinite(amount)) {
+        x->flags[s] = 2u;
+        return;
+    }
+    const float cut = orc_cut_from_tip_v(p, cvs, max_std(amount, 0.0f), source);
+    if (cut < 1.0e-4f) return;
+    float *out = x->out_points + (uint64_t)s * cvs * 4u;
+    for (uint32_t i = 0; i < cvs; ++i) {
+        out[4 * i] = p[i].x;
+        out[4 * i + 1] = p[i].y;
+        out[4 * i + 2] = p[i].z;
+        out[4 * i + 3] = in[4 * i + 3];
+    }
+    x->flags[s] = 1u;
+}
+
+/* The cut chain over `strand_count` packed strands of `cvs` points (float4
+ * position + radius), compacting the kept strands, in order, into the
+ * outputs; returns the kept count (or ~0u on a non-finite width or amount,
+ * which the reference reports as an error).  multiplier may be NULL.
+ * out_points holds strand_count * cvs float4s (each strand is computed in
+ * its own slot, in parallel over `workers` threads, 0 = all cores, before
+ * the stable compaction); keep (strand_count flags) is required. */
+uint32_t orc_classic_cut(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
+                         const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs, float amount_scale,
+                         float amount_offset, const float *multiplier, float radius_scale, float *out_points,
+                         uint32_t *out_counts, nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep,
+                         uint32_t workers) {
+    if (cvs < 2u || cvs > 256u || !keep) return ~0u;
+    cut_ctx c = {points, multiplier, cvs, amount_scale, amount_offset, radius_scale, out_points, keep};
+    orc_run_tiles(strand_count, workers, 256u, cut_one, &c);
+    for (uint32_t s = 0; s < strand_count; ++s)
+        if (keep[s] == 2u) return ~0u;
+    return compact_slots(strand_count, cvs, keep, counts, roots, uvs, out_points, out_counts, out_roots, out_uvs);
 }
 
 /* add_xgen_classic_renderer_endpoints (src/xgen_classic_runtime.cpp:1921-
@@ -394,34 +432,62 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
     return 0u;
 }
 
+typedef struct {
+    const float *points;
+    const uint32_t *counts;
+    const nxg_root_sample *roots;
+    uint32_t cvs;
+    const orc_classic_plan *plan;
+    const uint32_t *primitive_ids, *random_prefixes;
+    const float *columns;
+    uint32_t column_count;
+    float radius_scale;
+    float *out_points;
+    uint8_t *flags; /* 0 culled, 1 kept; keep[] values */
+    uint32_t *errors; /* per strand ORC_CLASSIC_* bits */
+} runtime_ctx;
+
+static void runtime_one(void *c, uint32_t s) {
+    const runtime_ctx *x = (const runtime_ctx *)c;
+    const uint32_t cvs = x->cvs;
+    x->flags[s] = 0u;
+    if (x->counts && x->counts[s] != cvs) {
+        x->errors[s] = ORC_CLASSIC_VARIABLE_COUNTS;
+        return;
+    }
+    uint8_t k = 0u;
+    x->errors[s] = classic_strand(x->points + (uint64_t)s * cvs * 4u, s, cvs, x->plan, x->roots, x->primitive_ids,
+                                  x->random_prefixes, x->columns, x->column_count, x->radius_scale,
+                                  x->out_points + (uint64_t)s * cvs * 4u, &k);
+    x->flags[s] = k;
+}
+
+/* out_points holds strand_count * cvs float4s: each strand is computed in
+ * its own slot, in parallel over `workers` threads (0 = all cores), then
+ * the kept ones are compacted in strand order. */
 uint32_t orc_classic_runtime(const float *points, const uint32_t *counts, const nxg_root_sample *roots,
                              const nxg_vec2 *uvs, uint32_t strand_count, uint32_t cvs,
                              const orc_classic_plan *plan, const uint32_t *primitive_ids,
                              const uint32_t *random_prefixes, const float *columns, uint32_t column_count,
                              float radius_scale, float *out_points, uint32_t *out_counts,
-                             nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep, uint32_t *error_bits) {
-    uint32_t errors = 0u, write = 0u;
-    float strand[4 * 256];
-    if (cvs < 2u || cvs > 256u || column_count > ORC_CLASSIC_MAX_COLUMNS || !roots) {
+                             nxg_root_sample *out_roots, nxg_vec2 *out_uvs, uint8_t *keep, uint32_t *error_bits,
+                             uint32_t workers) {
+    if (cvs < 2u || cvs > 256u || column_count > ORC_CLASSIC_MAX_COLUMNS || !roots || !keep) {
         *error_bits = ~0u;
         return ~0u;
     }
-    for (uint32_t s = 0; s < strand_count; ++s) {
-        if (counts && counts[s] != cvs) {
-            errors |= ORC_CLASSIC_VARIABLE_COUNTS;
-            continue;
-        }
-        uint8_t k = 0u;
-        errors |= classic_strand(points + (uint64_t)s * cvs * 4u, s, cvs, plan, roots, primitive_ids,
-                                 random_prefixes, columns, column_count, radius_scale, strand, &k);
-        if (keep) keep[s] = k;
-        if (!k || errors) continue;
-        memcpy(out_points + (uint64_t)write * cvs * 4u, strand, sizeof(float) * 4u * cvs);
-        if (out_counts) out_counts[write] = cvs;
-        if (out_roots) out_roots[write] = roots[s];
-        if (out_uvs && uvs) out_uvs[write] = uvs[s];
-        ++write;
+    uint32_t *errors = (uint32_t *)calloc(strand_count ? strand_count : 1u, sizeof(uint32_t));
+    if (!errors) {
+        *error_bits = ~0u;
+        return ~0u;
     }
-    *error_bits = errors;
-    return errors ? ~0u : write;
+    runtime_ctx c = {points, counts, roots, cvs, plan, primitive_ids, random_prefixes, columns, column_count,
+                     radius_scale, out_points, keep, errors};
+    orc_run_tiles(strand_count, workers, 64u, runtime_one, &c);
+    uint32_t bits = 0u;
+    for (uint32_t s = 0; s < strand_count; ++s) bits |= errors[s];
+    free(errors);
+    *error_bits = bits;
+    if (bits) return ~0u;
+    return compact_slots(strand_count, cvs, keep, NULL, roots, uvs, out_points, out_counts, out_roots, out_uvs);
 }

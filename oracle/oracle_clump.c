@@ -13,6 +13,7 @@
  * acosf/cosf/sinf are the C library's, as the reference's std:: calls are.
  */
 #include <math.h>
+#include <stdlib.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -208,44 +209,79 @@ static void cut_to_length(v3 *points, uint32_t n, float target) {
     for (uint32_t i = 1u; i < n; ++i) points[i] = curve_eval(source, n, rebuild * (float)i);
 }
 
+typedef struct {
+    float *points;
+    uint32_t cvs;
+    const uint32_t *strand_guides;
+    const float *axes3, *render3, *lengths, *normals3, *tangents3, *reference3;
+    uint32_t guides;
+    const nxg_vec3 *gradients;
+    const orc_clump_params *p;
+    const float *amount_multiplier;
+    uint32_t multiplier_stride;
+    uint8_t *failed;
+} clump_ctx;
+
+static void clump_one(void *c, uint32_t s) {
+    const clump_ctx *x = (const clump_ctx *)c;
+    const orc_clump_params *p = x->p;
+    const uint32_t cvs = x->cvs;
+    v3 pts[256], noise[256];
+    x->failed[s] = 0u;
+    const uint32_t g = x->strand_guides[s];
+    if (g == 0xffffffffu) return;
+    if (g >= x->guides) {
+        x->failed[s] = 1u;
+        return;
+    }
+    const float mask = clamp_std(p->mask, 0.0f, 1.0f);
+    if (mask <= 0.0f) return;
+    float *io = x->points + (uint64_t)s * cvs * 4u;
+    const v3 *render = (const v3 *)(x->render3 + (uint64_t)g * cvs * 3u);
+    const v3 *source = (const v3 *)(x->axes3 + (uint64_t)g * cvs * 3u);
+    const v3 root = source[0];
+    const v3 domain_position = x->reference3 ? *(const v3 *)(x->reference3 + 3u * g) : root;
+    noise_axis(render, source, cvs, p, domain_position, *(const v3 *)(x->normals3 + 3u * g),
+               *(const v3 *)(x->tangents3 + 3u * g), mask, x->gradients, noise);
+    for (uint32_t i = 0; i < cvs; ++i) pts[i] = mk(io[4 * i], io[4 * i + 1], io[4 * i + 2]);
+    cut_to_length(pts, cvs, x->lengths[g]);
+    float raw_amount = p->amount;
+    if (x->amount_multiplier) raw_amount = raw_amount * x->amount_multiplier[(uint64_t)s * x->multiplier_stride];
+    if (!isfinite(raw_amount)) {
+        x->failed[s] = 1u;
+        return;
+    }
+    const float amount = clamp_std(raw_amount, 0.0f, 1.0f);
+    for (uint32_t cv = 1u; cv < cvs; ++cv) {
+        const v3 goal = sub(render[cv], mk(0.0f, 0.0f, 0.0f));
+        pts[cv] = add(pts[cv], mul(sub(goal, pts[cv]), mask * amount * (1.0f - 2.0f * p->clump_scale)));
+    }
+    for (uint32_t cv = 1u; cv < cvs; ++cv) pts[cv] = add(pts[cv], noise[cv]);
+    for (uint32_t i = 0; i < cvs; ++i) {
+        io[4 * i] = pts[i].x;
+        io[4 * i + 1] = pts[i].y;
+        io[4 * i + 2] = pts[i].z;
+    }
+}
+
+/* Strands are independent: they run in parallel over `workers` threads
+ * (0 = all cores).  Returns ~0u (points then undefined) where the
+ * reference throws: a guide index out of range or a non-finite amount. */
 uint32_t orc_clump_apply(float *points, uint32_t strand_count, uint32_t cvs, const uint32_t *strand_guides,
                          const float *axes3, const float *render3, const float *lengths, const float *normals3,
                          const float *tangents3, const float *reference3, uint32_t guides,
                          const nxg_vec3 *gradients, const orc_clump_params *p, const float *amount_multiplier,
-                         uint32_t multiplier_stride) {
-    v3 pts[256], noise[256];
+                         uint32_t multiplier_stride, uint32_t workers) {
     if (cvs < 3u || cvs > 256u) return ~0u;
-    for (uint32_t s = 0; s < strand_count; ++s) {
-        const uint32_t g = strand_guides[s];
-        if (g == 0xffffffffu) continue;
-        if (g >= guides) return ~0u;
-        const float mask = clamp_std(p->mask, 0.0f, 1.0f);
-        if (mask <= 0.0f) continue;
-        float *io = points + (uint64_t)s * cvs * 4u;
-        const v3 *render = (const v3 *)(render3 + (uint64_t)g * cvs * 3u);
-        const v3 *source = (const v3 *)(axes3 + (uint64_t)g * cvs * 3u);
-        const v3 root = source[0];
-        const v3 domain_position = reference3 ? *(const v3 *)(reference3 + 3u * g) : root;
-        noise_axis(render, source, cvs, p, domain_position, *(const v3 *)(normals3 + 3u * g),
-                   *(const v3 *)(tangents3 + 3u * g), mask, gradients, noise);
-        for (uint32_t i = 0; i < cvs; ++i) pts[i] = mk(io[4 * i], io[4 * i + 1], io[4 * i + 2]);
-        cut_to_length(pts, cvs, lengths[g]);
-        float raw_amount = p->amount;
-        if (amount_multiplier) raw_amount = raw_amount * amount_multiplier[(uint64_t)s * multiplier_stride];
-        if (!isfinite(raw_amount)) return ~0u;
-        const float amount = clamp_std(raw_amount, 0.0f, 1.0f);
-        for (uint32_t cv = 1u; cv < cvs; ++cv) {
-            const v3 goal = sub(render[cv], mk(0.0f, 0.0f, 0.0f));
-            pts[cv] = add(pts[cv], mul(sub(goal, pts[cv]), mask * amount * (1.0f - 2.0f * p->clump_scale)));
-        }
-        for (uint32_t cv = 1u; cv < cvs; ++cv) pts[cv] = add(pts[cv], noise[cv]);
-        for (uint32_t i = 0; i < cvs; ++i) {
-            io[4 * i] = pts[i].x;
-            io[4 * i + 1] = pts[i].y;
-            io[4 * i + 2] = pts[i].z;
-        }
-    }
-    return 0u;
+    uint8_t *failed = (uint8_t *)calloc(strand_count ? strand_count : 1u, 1u);
+    if (!failed) return ~0u;
+    clump_ctx c = {points, cvs, strand_guides, axes3, render3, lengths, normals3, tangents3, reference3,
+                   guides, gradients, p, amount_multiplier, multiplier_stride, failed};
+    orc_run_tiles(strand_count, workers, 64u, clump_one, &c);
+    uint32_t bad = 0u;
+    for (uint32_t s = 0; s < strand_count; ++s) bad |= failed[s];
+    free(failed);
+    return bad ? ~0u : 0u;
 }
 
 /* apply_noise's geometry (src/xgen_classic_runtime.cpp:914-1039) on n points
