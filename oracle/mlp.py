@@ -105,29 +105,55 @@ def forward(flat, x, norm, bf16_round=False):
     return h[:, :OUTPUTS], cache
 
 
-def loss_and_grads(flat, x, targets_norm, norm, bf16_round=False):
-    """Mean-squared loss and its gradient w.r.t. the flat parameters."""
+# Deliberate mistakes loss_and_grads can plant, so that tests can show what
+# a gradient check at a given tolerance detects (each at layer `layer`):
+#   "dsilu_term"  silu's derivative without its z (1 - s) term (hidden layer)
+#   "bias"        the layer's bias gradient left at zero
+#   "unit"        the gradient through hidden unit 0 of the layer's input
+#                 dropped (layers 1..4)
+#   "transposed"  dh = dz @ W instead of dz @ W^T (square layers 1..3)
+#   "example"     the last example's output gradient dropped (any layer)
+PLANTS = ("dsilu_term", "bias", "unit", "transposed", "example")
+
+
+def loss_and_grads(flat, x, targets_norm, norm, bf16_round=False, plant=None, layer=0, dtype=np.float64):
+    """Mean-squared loss and its gradient w.r.t. the flat parameters.
+    plant/layer inject one of PLANTS (None: the correct gradient); dtype
+    float32 evaluates the backward pass in FP32 arithmetic (another valid
+    accumulation order, for the same tests)."""
+    assert plant is None or plant in PLANTS
     y, cache = forward(flat, x, norm, bf16_round)
     B = len(x)
     diff = y - targets_norm
     loss = float(np.mean(diff ** 2))
-    dy = np.zeros((B, OUT_PAD))
+    dy = np.zeros((B, OUT_PAD), dtype)
     dy[:, :OUTPUTS] = 2.0 * diff / (B * OUTPUTS)
+    if plant == "example":
+        dy[-1] = 0.0
     grads = []
     dz = dy
     for i in range(LAYERS, -1, -1):
-        h, Wr, z = cache[i]
+        h, Wr, z = (np.asarray(a, dtype) for a in cache[i])
         # the device feeds BF16 copies of dz to both products and sums the
         # FP32 dz for the bias
-        dzr = bf16(dz) if bf16_round else dz
+        dzr = np.asarray(bf16(dz), dtype) if bf16_round else dz
         dW = h.T @ dzr
         db = dz.sum(0)
+        if plant == "bias" and i == layer:
+            db = np.zeros_like(db)
         grads.append((dW, db))
         if i > 0:
-            dh = dzr @ Wr.T
-            dz = dh * dsilu(cache[i - 1][2])
+            dh = (dzr @ Wr) if (plant == "transposed" and i == layer) else dzr @ Wr.T
+            if plant == "unit" and i == layer:
+                dh[:, 0] = 0.0
+            zp = np.asarray(cache[i - 1][2], dtype)
+            if plant == "dsilu_term" and i - 1 == layer:
+                d = 1.0 / (1.0 + np.exp(-zp))
+            else:
+                d = dsilu(zp)
+            dz = dh * d
     grads.reverse()
-    g = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in grads])
+    g = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in grads]).astype(np.float64)
     # padding entries carry no gradient in the packed layout
     mask = np.concatenate([np.concatenate([_pad_mask(i).ravel(), _bias_mask(i)]) for i in range(LAYERS + 1)])
     return loss, g * mask

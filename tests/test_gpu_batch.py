@@ -155,3 +155,46 @@ def test_single_item_and_tiny_descriptions(torch, orc):
     torch.cuda.synchronize()
     for p, o in zip(params, outs):
         compare(o.to_host(), orc.generate(blob, p, stencil=True), label=f"{p.strand_count} strands")
+
+
+def test_config5_full_size(torch, orc):
+    """The batched uneven workload at full size: nine descriptions, 2,456,139
+    strands, 104,492,896 points, one batched launch.  Per description: the
+    batch's outputs equal that description's single launch bit for bit; its
+    attributes equal the expression runtime's and, on every root, meet the
+    oracle's attribute bar; the oracle generates a 4,096-strand sample from
+    the batch's attributes to the generator's bar; counts hold everywhere."""
+    from test_gpu_expr import check_attributes
+    ws = workloads.config5()
+    assert sum(w.params.strand_count for w in ws) == 2_456_139
+    assert sum(w.params.strand_count * w.params.cvs_per_strand for w in ws) == 104_492_896
+    blobs = [nx.build_asset(w.build) for w in ws]
+    assets = [nx.DeviceAsset(b) for b in blobs]
+    b, outs = make_batch(torch, ws, assets)
+    b.generate()
+    torch.cuda.synchronize()
+    for w, blob, a, e, out in zip(ws, blobs, assets, b.entries, outs):
+        n, c = w.params.strand_count, w.params.cvs_per_strand
+        single = nx.PackedOutputs(n, c, stencil=True)
+        nx.generate_packed(a, w.params, single, octaves=w.octaves, attributes=e["attributes"])
+        attrs = torch.empty_like(e["attributes"])
+        expr.evaluate_root_attributes(e["asset"], w.params, e["exprs"], e["maps"], attrs)
+        torch.cuda.synchronize()
+        assert same(torch, out, single), f"{w.name}: batch differs from the single-description launch"
+        assert torch.equal(attrs, e["attributes"]), f"{w.name}: batch attributes differ"
+        del single, attrs
+        gpu_attrs = e["attributes"].cpu().numpy()
+        ref_attrs, fails = orc.root_attributes(blob, w.params, w.info["programs"], w.info["maps"])
+        check_attributes(gpu_attrs, ref_attrs, w.info["programs"])
+        gpu = out.to_host()
+        assert np.all(gpu["point_counts"] == c), w.name
+        sample = np.unique(np.linspace(0, n - 1, min(n, 4096)).astype(np.int64))
+        ref = orc.generate(blob, w.params, octaves=w.octaves, attributes=gpu_attrs[sample], strands=sample,
+                           stencil=True)
+        idx = (sample[:, None] * c + np.arange(c)[None, :]).ravel()
+        sub = dict(points=gpu["points"][idx], roots=gpu["roots"][sample], root_uvs=gpu["root_uvs"][sample],
+                   point_counts=gpu["point_counts"][sample], stencil_indices=gpu["stencil_indices"][sample],
+                   stencil_weights=gpu["stencil_weights"][sample])
+        print(f"{w.name}: {fails} oracle attribute errors")
+        compare(sub, ref, label=w.name)
+        del gpu

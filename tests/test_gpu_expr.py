@@ -149,6 +149,45 @@ def test_end_to_end_expression_workload(torch, orc):
     assert err <= 1e-5 * diag and rerr <= 1e-7
 
 
+def test_config3_full_size(torch, orc):
+    """The expression workload at full size: 1,000,000 roots x 24 CVs of the
+    icosphere with the three 40-op programs and their texel map.  The oracle
+    evaluates every root's attributes (the attribute bar on all of them) and
+    generates every strand from the GPU's attributes (the generator's bar on
+    all of them: root records, stencil slots and weights, counts and radii,
+    positions within 1e-5 of the diagonal); size-independent properties on
+    the GPU outputs: determinism, the first CV is the root, counts."""
+    w = workloads.config3()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    assert (N, Cv) == (1_000_000, 24)
+    blob = nx.build_asset(w.build)
+    progs, maps = w.info["programs"], w.info["maps"]
+    assert len(progs) == 3 and all(len(p.instructions) >= 40 for p in progs)
+    asset = nx.DeviceAsset(blob)
+    es, tmaps = expr.ExprSet(progs), expr.TexelMaps(maps)
+    attrs = torch.empty((N, 4), dtype=torch.float32, device="cuda")
+    err = torch.zeros(1, dtype=torch.int32, device="cuda")
+    expr.evaluate_root_attributes(asset, w.params, es, tmaps, attrs, err)
+    out = nx.PackedOutputs(N, Cv, stencil=True)
+    nx.generate_packed(asset, w.params, out, attributes=attrs)
+    again = nx.PackedOutputs(N, Cv, stencil=False)
+    nx.generate_packed(asset, w.params, again, attributes=attrs)
+    torch.cuda.synchronize()
+    assert torch.equal(out.points, again.points), "generation must be deterministic"
+    del again
+    gpu_attrs = attrs.cpu().numpy()
+    ref_attrs, fails = orc.root_attributes(blob, w.params, progs, maps)
+    print(f"config3: {fails} of {N} roots raise in the oracle; GPU error word {int(err.item())}")
+    assert (fails == 0) == (int(err.item()) == 0)
+    check_attributes(gpu_attrs, ref_attrs, progs)
+    gpu = out.to_host()
+    assert np.all(gpu["point_counts"] == Cv)
+    first = gpu["points"].reshape(N, Cv, 4)[:, 0, :3]
+    assert np.array_equal(first.view(np.uint32), gpu["roots"]["position"].view(np.uint32)), "first CV is the root"
+    ref = orc.generate(blob, w.params, attributes=gpu_attrs, stencil=True)
+    compare(gpu, ref, label="config3 full size")
+
+
 def test_compiled_expression_text(torch, orc):
     """Programs compiled from expression text (nxg_expr_compile, identical to
     the reference compiler's programs) bound to root variables: GPU ==

@@ -1,8 +1,12 @@
 """The attribute surrogate on the CPU: the FP64 oracle's gradients against
-finite differences, the device layout query, host-side refusals."""
+finite differences, the sensitivity of the GPU test's per-layer gradient
+check (mistakes planted in the oracle's backward pass are flagged; another
+valid accumulation order is not), the device layout query, host-side
+refusals."""
 import ctypes as C
 
 import numpy as np
+import pytest
 
 import nanoxgen_amd as nx
 from nanoxgen_amd import abi, surrogate
@@ -59,3 +63,45 @@ def test_refusals_before_device_work():
     assert L.nxg_mlp_gradients(p, p, p, 0, 0, 64, None) == abi.INVALID_MLP_REQUEST
     assert L.nxg_verify_attributes(None, None, None, None, 0, None, 0, 1e-3, 16, None, 0, None, None) == \
         abi.NULL_ARGUMENT
+
+
+def _problem():
+    """Random nonzero parameters (biases too), quantised to float32, and a
+    320-example batch like the GPU gradient test's."""
+    rng = np.random.default_rng(8)
+    flat = mlp.init_params(8)
+    flat += (rng.normal(size=flat.size) * 0.05).astype(np.float32)
+    x = (rng.normal(size=(320, 10)) * 2.0 + 0.5).astype(np.float32)
+    t = (rng.normal(size=(320, 4)) * 0.3 + 1.0).astype(np.float32)
+    norm = np.concatenate([np.zeros(10), np.full(10, 2.0), np.full(4, 0.3), np.ones(4)])
+    return flat, x, (t - norm[24:28]) / norm[20:24], norm
+
+
+PLANTED = [("dsilu_term", l) for l in range(4)] + [("bias", l) for l in range(5)] + \
+          [("unit", l) for l in range(1, 5)] + [("transposed", l) for l in range(1, 4)] + [("example", 0)]
+
+
+@pytest.mark.parametrize("plant,layer", PLANTED)
+def test_gradient_check_detects_planted_mistakes(plant, layer):
+    """The GPU gradient test's per-layer check (relative error in norm per
+    layer and parameter kind, at the same BF16 rounding points) flags every
+    planted mistake with at least twice its tolerance, down to one missing
+    example of 320 (6.6e-3) or one hidden unit's path of 64 (6.6e-3 at
+    layer 3)."""
+    from test_gpu_surrogate import GRAD_TOL, per_layer_errors
+    flat, x, tn, norm = _problem()
+    _, good = mlp.loss_and_grads(flat, x, tn, norm, bf16_round=True)
+    _, bad = mlp.loss_and_grads(flat, x, tn, norm, bf16_round=True, plant=plant, layer=layer)
+    worst = max(e for _, _, e in per_layer_errors(bad, good))
+    assert worst > 2 * GRAD_TOL, f"{plant} at layer {layer}: worst per-layer error {worst:.2e}"
+
+
+def test_gradient_check_passes_another_accumulation_order():
+    """The same gradients evaluated in FP32 arithmetic (a valid, different
+    rounding of the accumulations) stay well inside the per-layer bar."""
+    from test_gpu_surrogate import GRAD_TOL, per_layer_errors
+    flat, x, tn, norm = _problem()
+    _, good = mlp.loss_and_grads(flat, x, tn, norm, bf16_round=True)
+    _, f32 = mlp.loss_and_grads(flat, x, tn, norm, bf16_round=True, dtype=np.float32)
+    worst = max(e for _, _, e in per_layer_errors(f32, good))
+    assert worst < 0.1 * GRAD_TOL, worst
