@@ -216,7 +216,8 @@ enum {
     NXG_ERROR_INVALID_MLP_REQUEST = 32,
     NXG_ERROR_INVALID_BATCH = 33,
     NXG_ERROR_INVALID_CLASSIC_REQUEST = 34,
-    NXG_ERROR_SCRATCH_LIMIT = 35       /* request exceeds the static scratch of the no-alloc entry point */
+    NXG_ERROR_SCRATCH_LIMIT = 35,      /* request exceeds the static scratch of the no-alloc entry point */
+    NXG_ERROR_INVALID_SAMPLER = 36     /* inverse-CDF sampler buffer missing, misaligned or too small */
 };
 
 /* device_generation_error_message (device_contract.h:82-116); extension codes
@@ -355,6 +356,36 @@ uint32_t nxg_generate_packed_prepared(const nxg_device_asset *asset,
                                       const nxg_launch_config *config,
                                       void *workspace, uint64_t workspace_bytes,
                                       void *stream);
+
+/* ---- opt-in inverse-CDF root sampler ------------------------------------
+ * No reference counterpart: an alternative to the alias-table triangle draw
+ * of sample_root (generate.h:107-112) that samples the same distribution
+ * through a prefix sum and a binary search.  It sends a strand to a
+ * different triangle than the alias draw (the barycentric draw and
+ * everything after it are the alias mode's), so its outputs match the
+ * reference only in distribution; use it only when that is what is wanted.
+ * The distribution is read off the blob's alias table in integers: bucket j
+ * keeps ceil(prob_j 2^24) 2^8 and hands the rest of its 2^32 to its alias;
+ * the inclusive prefix sums of those masses (u64, T of them, summing to
+ * T 2^32) are the table.  A strand's target is floor((h0 2^32 + h1) T / 2^32)
+ * with h0, h1 the hashes of random01's dimensions 0 and 1, and its triangle
+ * the first whose prefix sum exceeds the target.
+ *
+ * nxg_cdf_sampler_size: bytes of the sampler buffer for an asset (0 for a
+ * null header or no triangles).  The buffer (device memory, 256-byte
+ * aligned) starts with the T prefix sums.
+ * nxg_cdf_sampler_build: fills the buffer from the device asset's alias
+ * table (five kernels on `stream`).  It depends on the asset only.
+ * nxg_generate_packed_cdf: nxg_generate_packed with the roots drawn from the
+ * built sampler; output->roots is required and receives them. */
+uint64_t nxg_cdf_sampler_size(const nxg_asset_header *header);
+uint32_t nxg_cdf_sampler_build(const nxg_device_asset *asset, void *sampler, uint64_t sampler_bytes,
+                               void *stream);
+uint32_t nxg_generate_packed_cdf(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                                 const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                                 const void *sampler, uint64_t sampler_bytes, const nxg_packed_output *output,
+                                 const nxg_stencil_output *stencil, const nxg_launch_config *config,
+                                 void *workspace, uint64_t workspace_bytes, void *stream);
 
 /* generate_packed_roots_cpu's device equivalent (asset.cpp:451-501,
  * generate.h:644-676): strands start from caller-supplied device roots

@@ -38,6 +38,9 @@ SYMBOLS = [
     ("nxg_generate_packed", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_generate_packed_noalloc", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P]),
     ("nxg_generate_packed_from_roots", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
+    ("nxg_cdf_sampler_size", C.c_uint64, [_P]),
+    ("nxg_cdf_sampler_build", C.c_uint32, [_P, _P, C.c_uint64, _P]),
+    ("nxg_generate_packed_cdf", C.c_uint32, [_P, _P, _P, _P, _P, C.c_uint64, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_prepare_packed_generation", C.c_uint32, [_P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_generate_packed_prepared", C.c_uint32, [_P, _P, _P, _P, _P, _P, _P, _P, C.c_uint64, _P]),
     ("nxg_expr_validate", C.c_uint32, [_P]),
@@ -206,6 +209,30 @@ class DeviceAsset:
         return lib().nxg_packed_generation_workspace_size(C.byref(self.desc.header), C.byref(params))
 
 
+class CdfSampler:
+    """The opt-in inverse-CDF root sampler of a device asset
+    (nxg_cdf_sampler_build): the alias table's distribution as u64 prefix
+    sums, drawn by binary search.  Roots drawn through it differ from the
+    alias draw's (generate.h:107-112) and match it in distribution only."""
+
+    def __init__(self, asset, stream=None):
+        torch = _torch()
+        self.asset = asset
+        self.nbytes = lib().nxg_cdf_sampler_size(C.byref(asset.desc.header))
+        if self.nbytes == 0:
+            raise ValueError("asset has no triangles")
+        raw = torch.empty(self.nbytes + 256, dtype=torch.uint8, device=asset.tensor.device)
+        off = (-raw.data_ptr()) % 256
+        self.storage, self.tensor = raw, raw[off:off + self.nbytes]
+        check(lib().nxg_cdf_sampler_build(C.byref(asset.desc), C.c_void_p(self.tensor.data_ptr()), self.nbytes,
+                                          _stream_handle(stream)), "nxg_cdf_sampler_build")
+
+    def prefix_sums(self):
+        """The T inclusive prefix sums (numpy uint64; synchronises)."""
+        t = self.asset.header.triangle_count
+        return self.tensor[:8 * t].cpu().numpy().view(np.uint64)
+
+
 class PackedOutputs:
     """Caller-owned device outputs of one request (torch tensors)."""
 
@@ -259,12 +286,13 @@ def _ref(x):
 
 
 def generate_packed(asset, params, outputs, *, octaves=None, deformed=None, workspace=None,
-                    stream=None, roots=None, attributes=None):
+                    stream=None, roots=None, attributes=None, sampler=None):
     """Enqueue generate_packed (generate.h:612-676) for every strand on
     `stream` (torch stream; default the current one).  `roots` (optional
     device tensor of RootSample records) selects generate-from-roots;
     `attributes` (optional float32 [N, 4] device tensor) the per-root
-    attribute multipliers."""
+    attribute multipliers; `sampler` (a CdfSampler of the asset) the
+    opt-in inverse-CDF root draw, whose roots land in outputs.roots."""
     torch = _torch()
     L = lib()
     if workspace is None:
@@ -276,6 +304,11 @@ def generate_packed(asset, params, outputs, *, octaves=None, deformed=None, work
         status = L.nxg_generate_packed_attributed(
             C.byref(asset.desc), _ref(deformed), C.byref(params), _ref(octaves),
             C.c_void_p(attributes.data_ptr()), attributes.numel(), C.byref(out), _ref(st), None,
+            C.c_void_p(workspace.data_ptr()), workspace.numel(), _stream_handle(stream))
+    elif sampler is not None:
+        status = L.nxg_generate_packed_cdf(
+            C.byref(asset.desc), _ref(deformed), C.byref(params), _ref(octaves),
+            C.c_void_p(sampler.tensor.data_ptr()), sampler.nbytes, C.byref(out), _ref(st), None,
             C.c_void_p(workspace.data_ptr()), workspace.numel(), _stream_handle(stream))
     elif roots is not None:
         status = L.nxg_generate_packed_from_roots(

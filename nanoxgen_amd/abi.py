@@ -444,7 +444,7 @@ STATUS = [
     "OUTPUT_SIZE_OVERFLOW", "HIP", "WORKSPACE_TOO_SMALL", "INVALID_OCTAVES", "NULL_ARGUMENT",
     "STENCIL_CAPACITY_TOO_SMALL", "INVALID_ROOTS", "INVALID_BUILD_INPUT", "BUFFER_TOO_SMALL",
     "INVALID_ASSET", "INVALID_PROGRAM", "ATTRIBUTE_CAPACITY_TOO_SMALL", "INVALID_MLP_REQUEST", "INVALID_BATCH",
-    "INVALID_CLASSIC_REQUEST", "SCRATCH_LIMIT",
+    "INVALID_CLASSIC_REQUEST", "SCRATCH_LIMIT", "INVALID_SAMPLER",
 ]
 globals().update({name: i for i, name in enumerate(STATUS)})
 

@@ -43,9 +43,8 @@ bool valid_octaves(const nxg_noise_octaves *o) {
     return true;
 }
 
-uint32_t validate_common(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
-                         const nxg_generation_params *params, const nxg_launch_config *config) {
-    if (!asset || !params) return NXG_ERROR_NULL_ARGUMENT;
+uint32_t validate_asset_view(const nxg_device_asset *asset) {
+    if (!asset) return NXG_ERROR_NULL_ARGUMENT;
     if (!asset->data) return NXG_ERROR_NULL_ASSET;
     if (reinterpret_cast<uintptr_t>(asset->data) % NXG_BLOB_ALIGNMENT != 0u)
         return NXG_ERROR_MISALIGNED_ASSET;
@@ -59,6 +58,14 @@ uint32_t validate_common(const nxg_device_asset *asset, const nxg_deformed_geome
         uint64_t{h.noise_gradient_count} * sizeof(nxg_vec3) > h.byte_size - h.noise_gradients_offset)
         return NXG_ERROR_INVALID_ASSET_METADATA;
     if (asset->byte_capacity < h.byte_size) return NXG_ERROR_ASSET_CAPACITY_TOO_SMALL;
+    return NXG_OK;
+}
+
+uint32_t validate_common(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                         const nxg_generation_params *params, const nxg_launch_config *config) {
+    if (!asset || !params) return NXG_ERROR_NULL_ARGUMENT;
+    if (uint32_t s = validate_asset_view(asset); s != NXG_OK) return s;
+    const nxg_asset_header &h = asset->header;
     if (!valid_generation_parameters(*params)) return NXG_ERROR_INVALID_GENERATION_PARAMETERS;
     const uint32_t block = config ? config->block_size : 128u;
     if (block == 0u || block > 1024u) return NXG_ERROR_INVALID_BLOCK_SIZE;
@@ -160,6 +167,8 @@ const char *nxg_status_message(uint32_t status) {
             return "Classic cut curves, parameters or outputs are invalid (CV count, alignment, overlap)";
         case NXG_ERROR_SCRATCH_LIMIT:
             return "request needs more workspace than the library's static scratch (NXG_STATIC_SCRATCH_BYTES)";
+        case NXG_ERROR_INVALID_SAMPLER:
+            return "inverse-CDF sampler buffer is missing, misaligned or smaller than nxg_cdf_sampler_size";
     }
     return "unknown device generation error";
 }

@@ -289,8 +289,9 @@ __device__ __forceinline__ uint32_t sample_triangle(const nxg_alias_entry *alias
     return tri < triangle_count ? tri : bucket;
 }
 
-__device__ __forceinline__ Root sample_root(const Args &a, uint32_t strand) {
-    const uint32_t tri = sample_triangle(a.alias, a.seed, a.triangle_count, strand);
+// sample_root's barycentric draw and surface interpolation on triangle tri
+// (generate.h:113-135); the alias and the inverse-CDF samplers share it
+__device__ __forceinline__ Root root_on_triangle(const Args &a, uint32_t strand, uint32_t tri) {
     const float r1 = random01(a.seed, strand, 2u);
     const float r2 = random01(a.seed, strand, 3u);
     const float s = sqrtf(r1);
@@ -317,6 +318,10 @@ __device__ __forceinline__ Root sample_root(const Args &a, uint32_t strand) {
     r.b1 = b1;
     r.b2 = b2;
     return r;
+}
+
+__device__ __forceinline__ Root sample_root(const Args &a, uint32_t strand) {
+    return root_on_triangle(a, strand, sample_triangle(a.alias, a.seed, a.triangle_count, strand));
 }
 
 // make_guide_blend_state (generate.h:214-244) over the per-guide state
@@ -906,10 +911,19 @@ __global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 && CHUNK < 16 ? 3 : 2
 
 struct OrderArgs {
     const nxg_alias_entry *alias;
+    const nxg_root_sample *roots;  // roots already sampled (their triangles), or null: the alias draw
     uint32_t seed, triangle_count, strand_count, cvs;
     uint32_t shift, tiles;
     uint32_t *counts, *partials, *order;
 };
+
+__device__ __forceinline__ uint32_t order_triangle(const OrderArgs &o, uint32_t s) {
+    if (o.roots) {
+        const uint32_t t = o.roots[s].triangle_index;
+        return t < o.triangle_count ? t : o.triangle_count - 1u;
+    }
+    return sample_triangle(o.alias, o.seed, o.triangle_count, s);
+}
 
 __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t tile) {
     __shared__ uint32_t hist[kOrderBins];
@@ -919,7 +933,7 @@ __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t ti
     for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
         const uint32_t s = first + i;
         if (s >= o.strand_count) break;
-        atomicAdd(&hist[sample_triangle(o.alias, o.seed, o.triangle_count, s) >> o.shift], 1u);
+        atomicAdd(&hist[order_triangle(o, s) >> o.shift], 1u);
     }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) o.counts[b * o.tiles + tile] = hist[b];
@@ -933,7 +947,7 @@ __device__ __forceinline__ void order_scatter_tile(const OrderArgs &o, uint32_t 
     for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
         const uint32_t s = first + i;
         if (s >= o.strand_count) break;
-        const uint32_t bin = sample_triangle(o.alias, o.seed, o.triangle_count, s) >> o.shift;
+        const uint32_t bin = order_triangle(o, s) >> o.shift;
         o.order[atomicAdd(&next[bin], 1u)] = s;
     }
 }
@@ -1395,7 +1409,7 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
     // explicit roots keep the caller's order; sampled roots run grouped by triangle
     plan.ordered = explicit_roots == nullptr;
     uint8_t *ws = static_cast<uint8_t *>(workspace);
-    plan.order = OrderArgs{a.alias, a.seed, a.triangle_count, a.strand_count, a.cvs,
+    plan.order = OrderArgs{a.alias, nullptr, a.seed, a.triangle_count, a.strand_count, a.cvs,
                            order_shift(a.triangle_count), order_tiles(a.strand_count),
                            reinterpret_cast<uint32_t *>(ws + L.counts_offset),
                            reinterpret_cast<uint32_t *>(ws + L.partials_offset),
@@ -1738,6 +1752,276 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
+// ---- Opt-in inverse-CDF root sampler (no reference counterpart) ----------
+// The reference draws a root triangle from the blob's Walker alias table
+// (generate.h:107-112).  This sampler draws from the same distribution
+// through a prefix sum and a binary search instead; it sends a strand to a
+// different triangle than the alias draw, so it is used only when the caller
+// asks for it.  The distribution is read off the alias table in integers,
+// in units of 2^-32 of one bucket's probability mass:
+//   bucket j keeps keep_j = ceil(prob_j * 2^24) * 2^8, the probability that
+//   random01(seed, strand, 1) (one of 2^24 equally likely values k / 2^24)
+//   falls below prob_j, and hands 2^32 - keep_j to its alias (to itself when
+//   the alias is out of range, as sample_triangle does).
+// The masses sum to T * 2^32 exactly and their prefix sums are integers, so
+// every summation order gives the same table.  A strand draws
+// r = h0 * 2^32 + h1 from the hashes of random01's dimensions 0 and 1,
+// target = floor(r * T / 2^32) in [0, T * 2^32), and takes the first triangle
+// whose inclusive prefix sum exceeds target; its barycentrics come from
+// dimensions 2 and 3, as in the alias draw.  The sampler buffer holds the
+// prefix sums (u64 x T), the scan's block sums, and a coarse table of every
+// S-th prefix sum (at most kCdfCoarseMax entries) that each block keeps in
+// LDS: the search runs over the coarse table in LDS, then over one S-entry
+// segment of the prefix sums in memory.
+
+constexpr uint32_t kCdfCoarseMax = 4096u;   // coarse entries in LDS (32 KB)
+constexpr uint32_t kCdfBlockItems = 4096u;  // prefix-sum elements per 256-thread scan block
+
+struct CdfLayout {
+    uint64_t sums_offset, coarse_offset, total_bytes;
+    uint32_t stride_log2, coarse_count, block_count;
+};
+
+CdfLayout cdf_layout(uint32_t triangle_count) {
+    CdfLayout l{};
+    auto align = [](uint64_t x) { return (x + 255u) & ~uint64_t{255u}; };
+    l.stride_log2 = 6u;
+    while ((uint64_t{triangle_count} + (uint64_t{1} << l.stride_log2) - 1u) >> l.stride_log2 > kCdfCoarseMax)
+        ++l.stride_log2;
+    l.coarse_count = static_cast<uint32_t>((uint64_t{triangle_count} + (uint64_t{1} << l.stride_log2) - 1u) >>
+                                           l.stride_log2);
+    l.block_count = (triangle_count + kCdfBlockItems - 1u) / kCdfBlockItems;
+    l.sums_offset = align(uint64_t{triangle_count} * 8u);
+    l.coarse_offset = l.sums_offset + align(uint64_t{l.block_count} * 8u);
+    l.total_bytes = l.coarse_offset + align(uint64_t{l.coarse_count} * 8u);
+    return l;
+}
+
+struct CdfArgs {
+    const nxg_alias_entry *alias;
+    uint64_t *prefix, *sums, *coarse;
+    uint32_t triangle_count, stride_log2, coarse_count, block_count;
+};
+
+__device__ __forceinline__ uint64_t alias_keep(const nxg_alias_entry &e) {
+    const float p = e.probability;
+    uint32_t k = 0u;  // also for a NaN probability, which never keeps
+    if (p >= 1.0f) k = 1u << 24;
+    else if (p > 0.0f) k = static_cast<uint32_t>(ceilf(p * 16777216.0f));
+    return uint64_t{k} << 8;
+}
+
+__global__ void __launch_bounds__(256) cdf_keep_kernel(CdfArgs c) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j < c.triangle_count) c.prefix[j] = alias_keep(c.alias[j]);
+}
+
+__global__ void __launch_bounds__(256) cdf_hand_over_kernel(CdfArgs c) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= c.triangle_count) return;
+    const nxg_alias_entry e = c.alias[j];
+    const uint64_t give = (uint64_t{1} << 32) - alias_keep(e);
+    if (give) {
+        const uint32_t to = e.alias < c.triangle_count ? e.alias : j;
+        atomicAdd(reinterpret_cast<unsigned long long *>(&c.prefix[to]), static_cast<unsigned long long>(give));
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, uint32_t d) {
+    const uint32_t lo = __shfl_up(static_cast<uint32_t>(v), d, kWave);
+    const uint32_t hi = __shfl_up(static_cast<uint32_t>(v >> 32), d, kWave);
+    return (uint64_t{hi} << 32) | lo;
+}
+
+// Exclusive scan of one u64 per thread of a 256-thread block; total is the
+// block's sum.
+__device__ __forceinline__ uint64_t block_scan_u64(uint64_t v, uint64_t &total) {
+    __shared__ uint64_t wave_totals[256u / kWave];
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    uint64_t inc = v;
+#pragma unroll
+    for (uint32_t d = 1; d < static_cast<uint32_t>(kWave); d <<= 1) {
+        const uint64_t y = shfl_up_u64(inc, d);
+        if (lane >= d) inc += y;
+    }
+    if (lane == kWave - 1) wave_totals[wave] = inc;
+    __syncthreads();
+    uint64_t before = 0u;
+    total = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < 256u / kWave; ++w) {
+        if (w < wave) before += wave_totals[w];
+        total += wave_totals[w];
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+// Sum of each kCdfBlockItems-element block of the masses.
+__global__ void __launch_bounds__(256) cdf_block_sums_kernel(CdfArgs c) {
+    const uint64_t base = uint64_t{blockIdx.x} * kCdfBlockItems;
+    uint64_t v = 0u;
+    for (uint32_t i = threadIdx.x; i < kCdfBlockItems; i += 256u)
+        if (base + i < c.triangle_count) v += c.prefix[base + i];
+    uint64_t total;
+    block_scan_u64(v, total);
+    if (threadIdx.x == 0u) c.sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block sums in place (one block, 256 at a time).
+__global__ void __launch_bounds__(256) cdf_scan_sums_kernel(CdfArgs c) {
+    uint64_t carry = 0u;
+    for (uint32_t first = 0u; first < c.block_count; first += 256u) {
+        const uint32_t i = first + threadIdx.x;
+        const uint64_t v = i < c.block_count ? c.sums[i] : 0u;
+        uint64_t total;
+        const uint64_t ex = block_scan_u64(v, total);
+        if (i < c.block_count) c.sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// Inclusive prefix sums of one block's masses in place, offset by the
+// block's scanned sum, and the coarse entries (every S-th sum, and the last)
+// that fall in the block.  Each thread scans 16 consecutive elements staged
+// in LDS (row pitch 17 to spread the banks).
+__global__ void __launch_bounds__(256) cdf_scan_kernel(CdfArgs c) {
+    __shared__ uint64_t tile[256u * 17u];
+    const uint64_t base = uint64_t{blockIdx.x} * kCdfBlockItems;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < kCdfBlockItems; i += 256u)
+        tile[(i / 16u) * 17u + i % 16u] = base + i < c.triangle_count ? c.prefix[base + i] : 0u;
+    __syncthreads();
+    uint64_t run = 0u;
+    for (uint32_t k = 0; k < 16u; ++k) run += tile[t * 17u + k];
+    uint64_t total;
+    uint64_t before = block_scan_u64(run, total) + c.sums[blockIdx.x];
+    for (uint32_t k = 0; k < 16u; ++k) {
+        before += tile[t * 17u + k];
+        tile[t * 17u + k] = before;
+    }
+    __syncthreads();
+    const uint64_t last = uint64_t{c.triangle_count} - 1u;
+    const uint64_t stride_mask = (uint64_t{1} << c.stride_log2) - 1u;
+    for (uint32_t i = t; i < kCdfBlockItems; i += 256u) {
+        const uint64_t g = base + i;
+        if (g > last) break;
+        const uint64_t value = tile[(i / 16u) * 17u + i % 16u];
+        c.prefix[g] = value;
+        if ((g & stride_mask) == stride_mask || g == last) c.coarse[g >> c.stride_log2] = value;
+    }
+}
+
+__device__ __forceinline__ uint32_t cdf_hash(uint32_t seed, uint32_t strand, uint32_t dim) {
+    return hash32(seed ^ hash32(strand + 0x9e3779b9u * (dim + 1u)));
+}
+
+// One lane per strand (grid-stride over 256-strand chunks): the triangle by
+// the two-level search, then the alias mode's root on it, as a RootSample.
+__global__ void __launch_bounds__(256) cdf_roots_kernel(Args a, CdfArgs c, nxg_root_sample *roots) {
+    __shared__ uint64_t coarse[kCdfCoarseMax];
+    for (uint32_t k = threadIdx.x; k < c.coarse_count; k += 256u) coarse[k] = c.coarse[k];
+    __syncthreads();
+    const uint64_t T = c.triangle_count;
+    for (uint64_t s = uint64_t{blockIdx.x} * 256u + threadIdx.x; s < a.strand_count;
+         s += uint64_t{gridDim.x} * 256u) {
+        const uint32_t strand = static_cast<uint32_t>(s);
+        const uint64_t h0 = cdf_hash(a.seed, strand, 0u), h1 = cdf_hash(a.seed, strand, 1u);
+        const uint64_t target = h0 * T + ((h1 * T) >> 32);  // floor((h0 2^32 + h1) T / 2^32)
+        uint32_t lo = 0u, hi = c.coarse_count - 1u;  // coarse[last] = T 2^32 > target
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (coarse[mid] > target) hi = mid;
+            else lo = mid + 1u;
+        }
+        // the first triangle with a prefix sum above target lies in coarse
+        // segment lo: [lo S, min((lo + 1) S, T) - 1]
+        const uint64_t seg = uint64_t{lo} << c.stride_log2;
+        const uint64_t end = seg + (uint64_t{1} << c.stride_log2) < T ? seg + (uint64_t{1} << c.stride_log2) : T;
+        uint32_t first = static_cast<uint32_t>(seg), last = static_cast<uint32_t>(end - 1u);
+        while (first < last) {
+            const uint32_t mid = (first + last) >> 1;
+            if (c.prefix[mid] > target) last = mid;
+            else first = mid + 1u;
+        }
+        const Root r = root_on_triangle(a, strand, first);
+        nxg_root_sample o;
+        o.position = {r.position.x, r.position.y, r.position.z};
+        o.normal = {r.normal.x, r.normal.y, r.normal.z};
+        o.uv = {r.u, r.v};
+        o.triangle_index = r.tri;
+        o.barycentric = {r.b1, r.b2};
+        o.surface_face_id = r.tri;
+        roots[s] = o;
+    }
+}
+
+uint32_t cdf_args(const nxg_device_asset *asset, const void *sampler, uint64_t sampler_bytes, CdfArgs &c) {
+    if (uint32_t s = validate_asset_view(asset); s != NXG_OK) return s;
+    const nxg_asset_header &h = asset->header;
+    const CdfLayout l = cdf_layout(h.triangle_count);
+    if (!sampler || reinterpret_cast<uintptr_t>(sampler) % 256u != 0u || sampler_bytes < l.total_bytes)
+        return NXG_ERROR_INVALID_SAMPLER;
+    uint8_t *base = static_cast<uint8_t *>(const_cast<void *>(sampler));
+    c.alias = section<nxg_alias_entry>(*asset, h.alias_table_offset);
+    c.prefix = reinterpret_cast<uint64_t *>(base);
+    c.sums = reinterpret_cast<uint64_t *>(base + l.sums_offset);
+    c.coarse = reinterpret_cast<uint64_t *>(base + l.coarse_offset);
+    c.triangle_count = h.triangle_count;
+    c.stride_log2 = l.stride_log2;
+    c.coarse_count = l.coarse_count;
+    c.block_count = l.block_count;
+    return NXG_OK;
+}
+
+uint32_t cdf_build(const nxg_device_asset *asset, void *sampler, uint64_t sampler_bytes, hipStream_t stream) {
+    CdfArgs c;
+    if (uint32_t s = cdf_args(asset, sampler, sampler_bytes, c); s != NXG_OK) return s;
+    const dim3 lanes((c.triangle_count + 255u) / 256u);
+    hipLaunchKernelGGL(cdf_keep_kernel, lanes, dim3(256), 0, stream, c);
+    hipLaunchKernelGGL(cdf_hand_over_kernel, lanes, dim3(256), 0, stream, c);
+    hipLaunchKernelGGL(cdf_block_sums_kernel, dim3(c.block_count), dim3(256), 0, stream, c);
+    hipLaunchKernelGGL(cdf_scan_sums_kernel, dim3(1), dim3(256), 0, stream, c);
+    hipLaunchKernelGGL(cdf_scan_kernel, dim3(c.block_count), dim3(256), 0, stream, c);
+    return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
+}
+
+// Generation with inverse-CDF roots: the roots are sampled into
+// output->roots (required), the strands then start from them, processed
+// grouped by their triangles as the alias mode's are.
+uint32_t generate_cdf(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                      const nxg_generation_params *params, const nxg_noise_octaves *octaves, const void *sampler,
+                      uint64_t sampler_bytes, const nxg_packed_output *output, const nxg_stencil_output *stencil,
+                      const nxg_launch_config *config, void *workspace, uint64_t workspace_bytes,
+                      hipStream_t stream) {
+    if (uint32_t s = validate_common(asset, deformed, params, config); s != NXG_OK) return s;
+    if (uint32_t s = validate_packed_output(*params, output); s != NXG_OK) return s;
+    if (!output->roots) return NXG_ERROR_ROOT_CAPACITY_TOO_SMALL;
+    CdfArgs c;
+    if (uint32_t s = cdf_args(asset, sampler, sampler_bytes, c); s != NXG_OK) return s;
+    Plan plan;
+    if (uint32_t s = make_plan(asset, deformed, params, octaves, output->roots, nullptr, 0u, output, stencil, config,
+                               workspace, workspace_bytes, kPrepare | kStrands, plan);
+        s != NXG_OK)
+        return s;
+    plan.a.roots = nullptr;  // the sampler has written them
+    plan.ordered = true;
+    plan.order.roots = output->roots;
+    plan.a.order = plan.order.order;
+    int device = 0, cus = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        return NXG_ERROR_HIP;
+    const uint64_t blocks = std::min<uint64_t>((uint64_t{params->strand_count} + 255u) / 256u,
+                                               uint64_t{static_cast<uint32_t>(cus)} * 4u);
+    if (blocks) {
+        hipLaunchKernelGGL(cdf_roots_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream, plan.a, c,
+                           output->roots);
+        if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
+    }
+    return launch_plan(plan, *params, stream, kPrepare | kStrands);
+}
+
 }  // namespace
 // The static scratch of nxg_generate_packed_noalloc: a file-scope device
 // array, one copy per GPU at that GPU's own address.  Host code never passes
@@ -1883,6 +2167,25 @@ uint32_t nxg_generate_packed_from_roots(const nxg_device_asset *asset,
     if (!roots) return NXG_ERROR_INVALID_ROOTS;
     return nxg::launch(asset, nullptr, params, octaves, roots, nullptr, 0, output, nullptr, config, workspace,
                        workspace_bytes, stream, nxg::kPrepare | nxg::kStrands);
+}
+
+uint64_t nxg_cdf_sampler_size(const nxg_asset_header *header) {
+    if (!header || header->triangle_count == 0u) return 0u;
+    return nxg::cdf_layout(header->triangle_count).total_bytes;
+}
+
+uint32_t nxg_cdf_sampler_build(const nxg_device_asset *asset, void *sampler, uint64_t sampler_bytes,
+                               void *stream) {
+    return nxg::cdf_build(asset, sampler, sampler_bytes, static_cast<hipStream_t>(stream));
+}
+
+uint32_t nxg_generate_packed_cdf(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
+                                 const nxg_generation_params *params, const nxg_noise_octaves *octaves,
+                                 const void *sampler, uint64_t sampler_bytes, const nxg_packed_output *output,
+                                 const nxg_stencil_output *stencil, const nxg_launch_config *config,
+                                 void *workspace, uint64_t workspace_bytes, void *stream) {
+    return nxg::generate_cdf(asset, deformed, params, octaves, sampler, sampler_bytes, output, stencil, config,
+                             workspace, workspace_bytes, static_cast<hipStream_t>(stream));
 }
 
 void nxg_batch_default_tuning(nxg_batch_tuning *tuning) {

@@ -9,6 +9,9 @@
 
 namespace nxg {
 
+// The asset view's checks of validate_common (pointer, alignment, header
+// metadata, capacity).
+uint32_t validate_asset_view(const nxg_device_asset *asset);
 // Shared checks of device_contract.h:118-181 (validate_common_device_input).
 uint32_t validate_common(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
                          const nxg_generation_params *params, const nxg_launch_config *config);

@@ -282,6 +282,26 @@ def ref_generate(blob, params, *, deformed=None, roots=None, radius_scale=1.0, w
     return dict(points=points, roots=root_arr, root_uvs=uvs, point_counts=counts)
 
 
+def cdf_prefix(blob):
+    """The inverse-CDF sampler's prefix sums (uint64 [T]) for a blob."""
+    L = lib()
+    t = abi.header_of(blob).triangle_count
+    out = np.zeros(t, np.uint64)
+    L.orc_cdf_prefix(C.c_void_p(blob.ctypes.data), C.c_void_p(out.ctypes.data))
+    return out
+
+
+def cdf_roots(blob, seed, strand_count, prefix=None):
+    """The inverse-CDF sampler's roots of strands 0..strand_count-1 (rest
+    pose) as ROOT_DTYPE records."""
+    L = lib()
+    prefix = cdf_prefix(blob) if prefix is None else np.ascontiguousarray(prefix, np.uint64)
+    out = np.zeros(strand_count, abi.ROOT_DTYPE)
+    L.orc_cdf_roots(C.c_void_p(blob.ctypes.data), C.c_void_p(prefix.ctypes.data), C.c_uint32(seed),
+                    C.c_uint32(strand_count), C.c_void_p(out.ctypes.data))
+    return out
+
+
 def gradients_of(blob):
     h = abi.header_of(blob)
     return blob[h.noise_gradients_offset:h.noise_gradients_offset + 256 * 12]

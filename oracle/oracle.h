@@ -74,6 +74,14 @@ void orc_free(void *p);
 int orc_validate_asset(const void *blob, uint64_t size, char *msg, uint64_t msg_capacity);
 
 void orc_sample_root_f32(const void *blob, uint32_t seed, uint32_t strand, nxg_root_sample *out);
+void orc_root_on_triangle_f32(const void *blob, uint32_t seed, uint32_t strand, uint32_t tri,
+                              nxg_root_sample *out);
+/* The opt-in inverse-CDF sampler (oracle_cdf.c): prefix sums (T u64), one
+ * strand's triangle, and the roots of strands 0..strand_count-1. */
+void orc_cdf_prefix(const void *blob, uint64_t *prefix);
+uint32_t orc_cdf_triangle(const uint64_t *prefix, uint32_t T, uint32_t seed, uint32_t strand);
+void orc_cdf_roots(const void *blob, const uint64_t *prefix, uint32_t seed, uint32_t strand_count,
+                   nxg_root_sample *roots);
 
 /* float expression runtime (xgen_expression.cpp:637-722, 984-1148) */
 uint32_t orc_seexpr_hash_finish(uint32_t state);

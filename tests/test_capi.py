@@ -198,6 +198,35 @@ def test_generate_rejects_before_any_device_work():
     assert m and int(m.group(1)) << int(m.group(2)) == abi.STATIC_SCRATCH_BYTES
 
 
+def test_cdf_sampler_rejects_before_any_device_work():
+    """The inverse-CDF entry points refuse a missing, misaligned or short
+    sampler buffer, and output without roots, on the host."""
+    r = Request()
+    L = nx.lib()
+    need = L.nxg_cdf_sampler_size(C.byref(r.desc.header))
+    t = r.desc.header.triangle_count
+    assert need >= 8 * t and need % 256 == 0
+    assert L.nxg_cdf_sampler_size(None) == 0
+    buf = np.zeros(need + 512, np.uint8)
+    aligned = buf.ctypes.data + (-buf.ctypes.data) % 256
+    assert L.nxg_cdf_sampler_build(C.byref(r.desc), None, need, None) == abi.INVALID_SAMPLER
+    assert L.nxg_cdf_sampler_build(C.byref(r.desc), aligned, need - 1, None) == abi.INVALID_SAMPLER
+    assert L.nxg_cdf_sampler_build(C.byref(r.desc), aligned + 64, need, None) == abi.INVALID_SAMPLER
+    null = abi.DeviceAsset.from_buffer_copy(r.desc)
+    null.data = None
+    assert L.nxg_cdf_sampler_build(C.byref(null), aligned, need, None) == abi.NULL_ASSET
+    ws = np.zeros(16, np.uint8)
+    gen = lambda out, sampler, nbytes: L.nxg_generate_packed_cdf(
+        C.byref(r.desc), None, C.byref(r.params), None, sampler, nbytes, C.byref(out), None, None,
+        ws.ctypes.data, ws.nbytes, None)
+    assert gen(r.out, None, need) == abi.INVALID_SAMPLER
+    assert gen(r.out, aligned, need - 8) == abi.INVALID_SAMPLER
+    out = abi.PackedOutput.from_buffer_copy(r.out)
+    out.roots, out.root_capacity = None, 0
+    assert gen(out, aligned, need) == abi.ROOT_CAPACITY_TOO_SMALL
+    assert gen(r.out, aligned, need) == abi.WORKSPACE_TOO_SMALL
+
+
 def test_status_messages():
     L = nx.lib()
     assert L.nxg_status_message(0) == b""
