@@ -405,6 +405,96 @@ RUNTIME_PMC_SUMMARY = os.path.join(ROOT, "profiles", "runtime_chain_plane2M_clas
 PLAN_PMC_SUMMARY = os.path.join(ROOT, "profiles", "runtime_chain_plane2M_classic_plan_pmc.json")
 
 
+def bench_cdf(args):
+    """The plane workload (2,000,000 strands x 16 CVs, two octaves) with the
+    opt-in inverse-CDF root sampler.  The sampler is built once from the
+    asset (timed apart); one step = nxg_generate_packed_cdf: the CDF roots
+    kernel (two-level search, roots written to the output), guide prep, the
+    strand order by the sampled triangles, the strand kernel.  The alias
+    mode with the same outputs (roots on) is timed in the same process."""
+    import numpy as np
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import workloads
+
+    w = workloads.config2()
+    N, Cv = w.params.strand_count, w.params.cvs_per_strand
+    blob = nx.build_asset(w.build)
+    asset = nx.DeviceAsset(blob)
+    stream = torch.cuda.current_stream()
+    nx.CdfSampler(asset, stream=stream)  # first build: allocation and module load
+    build_ms = []
+    for _ in range(5):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e[0].record(stream)
+        sampler = nx.CdfSampler(asset, stream=stream)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        build_ms.append(e[0].elapsed_time(e[1]))
+    out = nx.PackedOutputs(N, Cv, roots=True, root_uvs=False, point_counts=True)
+    ws = torch.empty(asset.workspace_bytes(w.params), dtype=torch.uint8, device="cuda")
+
+    def run(use_cdf):
+        def step():
+            nx.generate_packed(asset, w.params, out, octaves=w.octaves, workspace=ws, stream=stream,
+                               sampler=sampler if use_cdf else None)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for e in ev:
+            e[0].record(stream)
+            step()
+            e[1].record(stream)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / args.steps
+        ms = sorted(e[0].elapsed_time(e[1]) for e in ev)
+        return wall, sum(ms) / len(ms), ms[len(ms) // 2]
+
+    alias_wall, alias_mean, _ = run(False)
+    wall, call_mean, call_median = run(True)
+    points = N * Cv
+    T = asset.header.triangle_count
+    # outputs + counts + roots written by the sampler and read back by the
+    # strand kernel + the blob + the prefix sums read once
+    alg = points * 16 + N * 4 + 2 * N * 48 + blob.nbytes + T * 8
+    achieved = alg / (call_mean * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "the whole call: cdf_roots_kernel + guide_prep_kernel + strand order + strand_kernel<true, 2>",
+                "kernel_ms_mean": round(call_mean, 4), "kernel_ms_median": round(call_median, 4),
+                "algorithmic_bytes_per_launch": alg}
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        pre = oracle.cdf_prefix(blob)
+        threads = baseline_threads()
+
+        def cpu_run(s):
+            t = time.perf_counter()
+            roots = oracle.cdf_roots(blob, w.params.seed, s, pre)
+            oracle.generate(blob, w.params.copy(strand_count=s), octaves=w.octaves, roots=roots, workers=threads)
+            return time.perf_counter() - t
+
+        s, dt, runs = sampled_seconds(cpu_run, N, args.cpu_seconds)
+        cpu = dict({"value": s * Cv / dt, "unit": "points/s", "cores": threads, "kind": "port",
+                    "sample": f"strands 0..{s - 1} ({s * Cv} points): the restatement's CDF roots "
+                              f"(oracle/oracle_cdf.c, one thread) + generation from them ({threads} threads), "
+                              f"median {dt:.3f} s of {runs} run(s)"}, **host_cpu())
+    line = {"metric": METRIC, "value": points / (wall * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "plane-cdf-sampler", "input": w.name, "strands": N, "cvs_per_strand": Cv,
+                       "triangles": T, "sampler": "inverse CDF (opt-in)", "parallelism": "single-gpu"},
+            "sampler_build_ms_median": round(sorted(build_ms)[len(build_ms) // 2], 4),
+            "alias_same_outputs": {"ms_per_step": round(alias_wall, 4), "call_ms_mean": round(alias_mean, 4),
+                                   "points_per_s": points / (alias_wall * 1e-3)},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def chain_counters(path, chain_ms):
     """(HBM traffic per chain, VALU roofline dict) from a committed rocprofv3
     counter summary of the same chain, or (None, None)."""
@@ -935,9 +1025,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="plane",
-                    choices=["plane", "expr", "surrogate", "surrogate_sweep", "batch", "classic", "classic_expr", "classic_plan", "classic_noise", "clump",
-                             "transport"],
+                    choices=["plane", "plane_cdf", "expr", "surrogate", "surrogate_sweep", "batch", "classic",
+                             "classic_expr", "classic_plan", "classic_noise", "clump", "transport"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
+                         "plane_cdf: the same with the opt-in inverse-CDF root sampler; "
                          "expr: 1,000,000 icosphere roots x 24 CVs with three 40-op attribute programs; "
                          "surrogate: the plane with 1,000,000 roots x 24 CVs and four 40-op attribute programs, "
                          "the trained surrogate + sampled exact check against the exact runtime; "
@@ -968,6 +1059,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
         return 0
+    if args.workload == "plane_cdf" and args.gpus == 1:
+        return bench_cdf(args)
     if args.workload == "surrogate" and args.gpus == 1:
         return bench_surrogate(args)
     if args.workload == "surrogate_sweep" and args.gpus == 1:

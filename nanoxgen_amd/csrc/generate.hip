@@ -324,6 +324,16 @@ __device__ __forceinline__ Root sample_root(const Args &a, uint32_t strand) {
     return root_on_triangle(a, strand, sample_triangle(a.alias, a.seed, a.triangle_count, strand));
 }
 
+// The opt-in inverse-CDF sampler's target for a strand: floor(r T / 2^32)
+// with r = h0 2^32 + h1, h0 and h1 the hashes behind random01's dimensions
+// 0 and 1; it lies in [0, T 2^32), and target / 2^32 is the strand's place
+// along the triangles in units of one triangle's mean mass.
+__device__ __forceinline__ uint64_t cdf_target(uint32_t seed, uint32_t strand, uint32_t triangle_count) {
+    const uint64_t h0 = hash32(seed ^ hash32(strand + 0x9e3779b9u * 1u));
+    const uint64_t h1 = hash32(seed ^ hash32(strand + 0x9e3779b9u * 2u));
+    return h0 * triangle_count + ((h1 * triangle_count) >> 32);
+}
+
 // make_guide_blend_state (generate.h:214-244) over the per-guide state
 __device__ __forceinline__ Blend make_blend(const Args &a, const Root &r) {
     Blend b;
@@ -915,9 +925,11 @@ struct OrderArgs {
     uint32_t seed, triangle_count, strand_count, cvs;
     uint32_t shift, tiles;
     uint32_t *counts, *partials, *order;
+    uint32_t cdf_key;  // sort by the inverse-CDF target / 2^32 instead (monotone in the triangle drawn)
 };
 
 __device__ __forceinline__ uint32_t order_triangle(const OrderArgs &o, uint32_t s) {
+    if (o.cdf_key) return static_cast<uint32_t>(cdf_target(o.seed, s, o.triangle_count) >> 32);
     if (o.roots) {
         const uint32_t t = o.roots[s].triangle_index;
         return t < o.triangle_count ? t : o.triangle_count - 1u;
@@ -1912,22 +1924,19 @@ __global__ void __launch_bounds__(256) cdf_scan_kernel(CdfArgs c) {
     }
 }
 
-__device__ __forceinline__ uint32_t cdf_hash(uint32_t seed, uint32_t strand, uint32_t dim) {
-    return hash32(seed ^ hash32(strand + 0x9e3779b9u * (dim + 1u)));
-}
-
-// One lane per strand (grid-stride over 256-strand chunks): the triangle by
-// the two-level search, then the alias mode's root on it, as a RootSample.
+// One lane per strand (grid-stride over 256-strand chunks of the processing
+// order, sorted by target, so that a wavefront's searches and vertex reads
+// share cache lines): the triangle by the two-level search, then the alias
+// mode's root on it, as a RootSample.
 __global__ void __launch_bounds__(256) cdf_roots_kernel(Args a, CdfArgs c, nxg_root_sample *roots) {
     __shared__ uint64_t coarse[kCdfCoarseMax];
     for (uint32_t k = threadIdx.x; k < c.coarse_count; k += 256u) coarse[k] = c.coarse[k];
     __syncthreads();
     const uint64_t T = c.triangle_count;
-    for (uint64_t s = uint64_t{blockIdx.x} * 256u + threadIdx.x; s < a.strand_count;
-         s += uint64_t{gridDim.x} * 256u) {
-        const uint32_t strand = static_cast<uint32_t>(s);
-        const uint64_t h0 = cdf_hash(a.seed, strand, 0u), h1 = cdf_hash(a.seed, strand, 1u);
-        const uint64_t target = h0 * T + ((h1 * T) >> 32);  // floor((h0 2^32 + h1) T / 2^32)
+    for (uint64_t slot = uint64_t{blockIdx.x} * 256u + threadIdx.x; slot < a.strand_count;
+         slot += uint64_t{gridDim.x} * 256u) {
+        const uint32_t strand = a.order ? a.order[slot] : static_cast<uint32_t>(slot);
+        const uint64_t target = cdf_target(a.seed, strand, c.triangle_count);
         uint32_t lo = 0u, hi = c.coarse_count - 1u;  // coarse[last] = T 2^32 > target
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -1952,7 +1961,7 @@ __global__ void __launch_bounds__(256) cdf_roots_kernel(Args a, CdfArgs c, nxg_r
         o.triangle_index = r.tri;
         o.barycentric = {r.b1, r.b2};
         o.surface_face_id = r.tri;
-        roots[s] = o;
+        roots[strand] = o;
     }
 }
 
@@ -1986,9 +1995,9 @@ uint32_t cdf_build(const nxg_device_asset *asset, void *sampler, uint64_t sample
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
-// Generation with inverse-CDF roots: the roots are sampled into
-// output->roots (required), the strands then start from them, processed
-// grouped by their triangles as the alias mode's are.
+// Generation with inverse-CDF roots: guide prep and the processing order
+// (by target), the roots sampled into output->roots (required) in that
+// order, then the strands from them in the same order.
 uint32_t generate_cdf(const nxg_device_asset *asset, const nxg_deformed_geometry *deformed,
                       const nxg_generation_params *params, const nxg_noise_octaves *octaves, const void *sampler,
                       uint64_t sampler_bytes, const nxg_packed_output *output, const nxg_stencil_output *stencil,
@@ -2004,10 +2013,13 @@ uint32_t generate_cdf(const nxg_device_asset *asset, const nxg_deformed_geometry
                                workspace, workspace_bytes, kPrepare | kStrands, plan);
         s != NXG_OK)
         return s;
-    plan.a.roots = nullptr;  // the sampler has written them
+    // one order for both kernels: by target / 2^32, which is monotone in the
+    // triangle drawn, so the strands also run grouped by triangle
+    plan.a.roots = nullptr;  // the sampler writes them
     plan.ordered = true;
-    plan.order.roots = output->roots;
+    plan.order.cdf_key = 1u;
     plan.a.order = plan.order.order;
+    if (uint32_t s = launch_plan(plan, *params, stream, kPrepare); s != NXG_OK) return s;
     int device = 0, cus = 0;
     if (hipGetDevice(&device) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
@@ -2019,7 +2031,7 @@ uint32_t generate_cdf(const nxg_device_asset *asset, const nxg_deformed_geometry
                            output->roots);
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     }
-    return launch_plan(plan, *params, stream, kPrepare | kStrands);
+    return launch_plan(plan, *params, stream, kStrands);
 }
 
 }  // namespace
