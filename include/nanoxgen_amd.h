@@ -952,14 +952,17 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
 /* apply_xgen_classic_float_runtime_plan_cpu (include/nanoxgen/
  * xgen_classic_runtime.h:185-203, src/xgen_classic_runtime.cpp:1420-1918)
  * for a ClassicFloatRuntimePlan (xgen_classic_runtime.h:130-165) of
- * SplinePrimitive expressions (length, width, taper, taperStart, widthRamp),
- * at most one ClumpingFX module with world-space guides and at most one
- * CutFX module (rebuildType 1), the clump before the cut.  Per strand, in
- * the reference's order:
+ * SplinePrimitive expressions (length, width, taper, taperStart, widthRamp)
+ * and an ordered list of FX effects (the reference's plan.effects,
+ * xgen_classic_runtime.h:98-106): ClumpingFX, NoiseFX (mode 0) and CutFX
+ * (rebuildType 1) modules in any order, up to NXG_CLASSIC_MAX_CLUMPS,
+ * _MAX_NOISES and _MAX_CUTS of them.  Per strand, in the reference's order
+ * (src/xgen_classic_runtime.cpp:1619-1858):
  *   cLength; length expression -> scale_curve about the root, cLength *=
  *   scale (an error when not finite or negative); raw width from the width
  *   expression, else 2 radius[0] / radius_scale; culled (skipped) when
  *   cLength < 1e-4 or width < 1e-4; cWidth = width;
+ * then each effect in list order:
  *   ClumpingFX (:1680-1793) for a strand with a guide: mask (clamped to
  *   [0, 1]; <= 0 leaves the strand alone); noise, noiseFrequency,
  *   noiseCorrelation and noiseScale(t) in the guide's context (its uv and
@@ -967,15 +970,19 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
  *   mask noise noiseScale; cut_to_length_and_rebuild to the guide's spline
  *   length (cLength of the rebuilt curve); clump amount (clamped);
  *   CVs 1.. blended toward the guide's render axis by
- *   mask amount (1 - 2 clumpScale(t)), plus the noise axis; cLength again;
- *   (each ClumpingFX module in turn, on the previous one's strand);
+ *   mask amount (1 - 2 clumpScale(t)), plus the noise axis; cLength again
+ *   (with root_relative the goal is taken about the strand's root,
+ *   :1627-1628, 1774-1776, and a module with local guide axes clumps
+ *   toward their render axis moved to the guide's root);
  *   NoiseFX (apply_noise, :897-1039): mask (> 1e-6), frequency, correlation
  *   and preserveLength at the context's t; the transported frame of the
  *   strand from its surface normal and tangent; per CV magnitude(t) *
  *   magnitudeScale(t) * mask times the classic noise of the domain moved
  *   along each axis; the preserve-length rescale; cLength again;
  *   CutFX: amount, cut_from_tip_and_rebuild, culled when the cut parameter
- *   < 1e-4, cLength of the rebuilt curve; taper, taperStart; per CV t =
+ *   < 1e-4 (a culled strand still runs the later effects, whose errors
+ *   count), cLength of the rebuilt curve;
+ * taper, taperStart; per CV t =
  *   cv / (n - 1), scale = 1 - taper (t - start) / (1 - start) past the
  *   start, times widthRamp(t), radius = 0.5 max(cWidth scale, 0)
  *   radius_scale;
@@ -1006,13 +1013,27 @@ uint32_t nxg_classic_clump_apply(const nxg_packed_point *points, uint32_t strand
 #define NXG_CLASSIC_NONFINITE_NOISE_PARAMETER 512u /* "Classic NoiseFX produced a non-finite parameter" */
 #define NXG_CLASSIC_NONFINITE_NOISE 1024u   /* "Classic NoiseFX magnitude is non-finite" */
 
-/* Hierarchical clumping: up to this many ClumpingFX modules in one plan, applied
- * in object order, each with its own ClassicClumpRuntimeData; the CutFX may sit
- * between them (the reference's plan.effects, src/xgen_classic_runtime.cpp:
- * 1677-1823).  A CutFX followed by a clump rebuilds the strand at once; a
- * strand it culls still runs the later modules (their errors count). */
+/* Modules of one plan: ClumpingFX modules (each with its own
+ * ClassicClumpRuntimeData), NoiseFX modules and CutFX modules, applied in the
+ * order of the plan's effect list (the reference's plan.effects,
+ * src/xgen_classic_runtime.cpp:1679-1826).  A CutFX that other effects follow
+ * rebuilds the strand at once; a strand it culls still runs them. */
 #define NXG_CLASSIC_MAX_CLUMPS 3u
+#define NXG_CLASSIC_MAX_NOISES 2u
+#define NXG_CLASSIC_MAX_CUTS 2u
+#define NXG_CLASSIC_MAX_EFFECTS 8u
 #define NXG_CLASSIC_CLUMP_EXPRS 7u
+#define NXG_CLASSIC_NOISE_EXPRS 6u
+
+/* ClassicFloatEffectType (xgen_classic_runtime.h:98-102), same numbering. */
+enum { NXG_CLASSIC_EFFECT_CLUMP = 0, NXG_CLASSIC_EFFECT_NOISE = 1, NXG_CLASSIC_EFFECT_CUT = 2 };
+
+/* ClassicFloatEffect (xgen_classic_runtime.h:104-107): an effect type and
+ * the index of its module among the plan's modules of that type. */
+typedef struct nxg_classic_effect {
+    uint32_t type;
+    uint32_t module;
+} nxg_classic_effect;
 
 enum {
     NXG_CLASSIC_EXPR_LENGTH = 0,
@@ -1030,34 +1051,53 @@ enum {
     NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE,
     NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY,
     NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION,
-    /* the NoiseFX module (ClassicFloatNoiseModule, xgen_classic_runtime.h:32-40), after the clumps */
+    /* the first NoiseFX module (ClassicFloatNoiseModule, xgen_classic_runtime.h:32-40); module n's
+     * programs follow at NXG_CLASSIC_EXPR_NOISE_OF(n, k) */
     NXG_CLASSIC_EXPR_NOISE_MASK = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_MAX_CLUMPS * NXG_CLASSIC_CLUMP_EXPRS,
     NXG_CLASSIC_EXPR_NOISE_MAGNITUDE,
     NXG_CLASSIC_EXPR_NOISE_MAGNITUDE_SCALE,
     NXG_CLASSIC_EXPR_NOISE_FREQUENCY,
     NXG_CLASSIC_EXPR_NOISE_CORRELATION,
     NXG_CLASSIC_EXPR_NOISE_PRESERVE_LENGTH,
-    NXG_CLASSIC_EXPR_COUNT
+    /* the amounts of CutFX modules 1 .. NXG_CLASSIC_MAX_CUTS - 1 (module 0's is NXG_CLASSIC_EXPR_CUT_AMOUNT) */
+    NXG_CLASSIC_EXPR_CUT_AMOUNT_1 = NXG_CLASSIC_EXPR_NOISE_MASK + NXG_CLASSIC_MAX_NOISES * NXG_CLASSIC_NOISE_EXPRS,
+    NXG_CLASSIC_EXPR_COUNT = NXG_CLASSIC_EXPR_CUT_AMOUNT_1 + NXG_CLASSIC_MAX_CUTS - 1
 };
 /* Program k (one of NXG_CLASSIC_EXPR_CLUMP_MASK .. _NOISE_CORRELATION) of ClumpingFX module m. */
 #define NXG_CLASSIC_EXPR_CLUMP_OF(m, k) ((k) + NXG_CLASSIC_CLUMP_EXPRS * (m))
+/* Program k (one of NXG_CLASSIC_EXPR_NOISE_MASK .. _NOISE_PRESERVE_LENGTH) of NoiseFX module n. */
+#define NXG_CLASSIC_EXPR_NOISE_OF(n, k) ((k) + NXG_CLASSIC_NOISE_EXPRS * (n))
+/* The amount program of CutFX module c. */
+#define NXG_CLASSIC_EXPR_CUT_AMOUNT_OF(c) ((c) == 0u ? (uint32_t)NXG_CLASSIC_EXPR_CUT_AMOUNT \
+                                                    : (uint32_t)NXG_CLASSIC_EXPR_CUT_AMOUNT_1 + (c) - 1u)
 
 /* Host memory.  expressions[k] NULL: absent.  Bindings are NXG_CLASSIC_VAR_*;
  * programs hold the reference's ops only (no pow/mix/noise3/texel), at most
  * NXG_CLASSIC_MAX_PROGRAM_INPUTS inputs and NXG_CLASSIC_MAX_PROGRAM_SLOTS live
  * values.  Each clump needs all seven of its expressions; its noiseFrequency
  * and noiseCorrelation may read no input (they are evaluated once per
- * guide). */
+ * guide).  Each NoiseFX needs all six of its expressions, each CutFX its
+ * amount.
+ *
+ * The effects: with effect_count > 0, effects[0 .. effect_count) in order,
+ * naming every module of the plan (clump_count ClumpingFX, noise_count
+ * NoiseFX, cut_count CutFX modules) exactly once.  With effect_count 0 the
+ * short form applies: clump_count clumps in order, the NoiseFX (has_noise)
+ * and then the CutFX (has_cut) after the first clump_count -
+ * clumps_after_cut of them (noise_count and cut_count must be 0 then). */
 typedef struct nxg_classic_plan {
     const nxg_expr_program *expressions[NXG_CLASSIC_EXPR_COUNT];
-    uint32_t has_cut;               /* the CutFX module; needs expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT] */
-    uint32_t clump_count;           /* ClumpingFX modules, applied in order (0..NXG_CLASSIC_MAX_CLUMPS) */
+    uint32_t has_cut;               /* short form: one CutFX; needs expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT] */
+    uint32_t clump_count;           /* ClumpingFX modules (0..NXG_CLASSIC_MAX_CLUMPS) */
     uint32_t description_id;        /* descriptionId */
     uint32_t description_name_seed; /* nxg_string_seed of the description name after its last ':' */
-    uint32_t clumps_after_cut;      /* the last this many ClumpingFX modules follow the CutFX in object
-                                       order (0: the CutFX comes last) */
-    uint32_t has_noise;             /* one NoiseFX module (mode 0) after the clumps, before the CutFX;
-                                       needs clumps_after_cut 0 and all six of its expressions */
+    uint32_t clumps_after_cut;      /* short form: the last this many clumps follow the CutFX */
+    uint32_t has_noise;             /* short form: one NoiseFX after the clumps, before the CutFX (needs
+                                       clumps_after_cut 0) */
+    uint32_t noise_count;           /* NoiseFX modules of the effect list (0..NXG_CLASSIC_MAX_NOISES) */
+    uint32_t cut_count;             /* CutFX modules of the effect list (0..NXG_CLASSIC_MAX_CUTS) */
+    uint32_t effect_count;          /* 0: the short form */
+    nxg_classic_effect effects[NXG_CLASSIC_MAX_EFFECTS];
 } nxg_classic_plan;
 
 /* What nxg_classic_plan_pack reports about a plan. */
@@ -1075,9 +1115,15 @@ typedef struct nxg_classic_plan_info {
     uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
     uint32_t uniform;             /* bit k: expressions[k] reads no input and calls no rand(), so its
                                      value depends on t alone (evaluated once per call) */
-    uint32_t clumps_after_cut;
-    uint32_t has_noise;
-    uint32_t reserved;
+    uint32_t clumps_after_cut;    /* the short form's (0 for an effect list) */
+    uint32_t has_noise;           /* some NoiseFX module */
+    uint32_t noise_count;
+    uint32_t cut_count;
+    uint32_t effect_count;        /* the effects, short form expanded */
+    nxg_classic_effect effects[NXG_CLASSIC_MAX_EFFECTS];
+    uint32_t trailing_cut;        /* the last effect is a CutFX (the write resamples the cut strands) */
+    uint32_t rewrites;            /* some effect other than a trailing CutFX changes the CVs (they need
+                                     cvs_per_strand <= NXG_CLUMP_MAX_CVS and the workspace copy) */
 } nxg_classic_plan_info;
 
 /* Validates and packs a plan into a position-independent image (out NULL:
@@ -1093,6 +1139,10 @@ typedef struct nxg_classic_clump_binding {
     const nxg_clump_guides *guides;   /* host struct of device arrays (world-space guide CVs) */
     const nxg_vec2 *guide_uvs;        /* device: the guides' uv (the noise expressions' context) */
     const uint32_t *guide_random_prefixes; /* device: the guides' SeExpr prefixes */
+    const nxg_vec3 *local_axes;       /* optional, device: translation-free copies of the guide CVs
+                                         (guide_local_axes, xgen_classic_runtime.h:65-70), guide-major;
+                                         they set the guides' spline lengths and, with root_relative,
+                                         their render and noise axes */
 } nxg_classic_clump_binding;
 
 typedef struct nxg_classic_runtime_inputs {
@@ -1110,6 +1160,9 @@ typedef struct nxg_classic_runtime_inputs {
      * strand's current root) */
     const nxg_vec3 *surface_tangents;
     const nxg_vec3 *noise_domain_positions;
+    uint32_t root_relative;           /* the reference's root_relative (xgen_classic_runtime.h:203): clump
+                                         goals about each strand's root position */
+    uint32_t reserved;
 } nxg_classic_runtime_inputs;
 
 /* Host-side checks (the reference's argument checks,

@@ -282,6 +282,12 @@ CLASSIC_EXPRESSION_ERROR, CLASSIC_VARIABLE_COUNTS, CLASSIC_NONFINITE_CLUMP = 64,
 CLASSIC_CLUMP_EXPRS = ("clump_mask", "clump", "clump_scale", "clump_noise", "clump_noise_scale",
                        "clump_noise_frequency", "clump_noise_correlation")
 CLASSIC_MAX_CLUMPS = 3
+CLASSIC_MAX_NOISES = 2
+CLASSIC_MAX_CUTS = 2
+CLASSIC_MAX_EFFECTS = 8
+# ClassicFloatEffectType numbering (NXG_CLASSIC_EFFECT_*)
+CLASSIC_EFFECT_CLUMP, CLASSIC_EFFECT_NOISE, CLASSIC_EFFECT_CUT = 0, 1, 2
+CLASSIC_EFFECT_TYPES = {"clump": CLASSIC_EFFECT_CLUMP, "noise": CLASSIC_EFFECT_NOISE, "cut": CLASSIC_EFFECT_CUT}
 
 
 def clump_key(module, key):
@@ -289,20 +295,39 @@ def clump_key(module, key):
     return key if module == 0 else f"{key}@{module}"
 
 
-# NXG_CLASSIC_EXPR_*: module m's clump programs at 6 + 7 m + k
+def noise_key(module, key):
+    """The CLASSIC_EXPRS name of NoiseFX program `key` of NoiseFX module `module`."""
+    return key if module == 0 else f"{key}@{module}"
+
+
+def cut_key(module):
+    """The CLASSIC_EXPRS name of CutFX module `module`'s amount."""
+    return "cut_amount" if module == 0 else f"cut_amount@{module}"
+
+
+# NXG_CLASSIC_EXPR_*: module m's clump programs at 6 + 7 m + k, NoiseFX module n's
+# at 27 + 6 n + k, the amounts of CutFX modules 1.. at the end
 CLASSIC_NOISE_EXPRS = ("noise_mask", "noise_magnitude", "noise_magnitude_scale", "noise_frequency",
                        "noise_correlation", "noise_preserve_length")
 CLASSIC_EXPRS = (("length", "width", "cut_amount", "taper", "taper_start", "width_ramp")
                  + tuple(clump_key(m, k) for m in range(CLASSIC_MAX_CLUMPS) for k in CLASSIC_CLUMP_EXPRS)
-                 + CLASSIC_NOISE_EXPRS)
+                 + tuple(noise_key(n, k) for n in range(CLASSIC_MAX_NOISES) for k in CLASSIC_NOISE_EXPRS)
+                 + tuple(cut_key(c) for c in range(1, CLASSIC_MAX_CUTS)))
 CLASSIC_NONFINITE_NOISE_PARAMETER, CLASSIC_NONFINITE_NOISE = 512, 1024
+
+
+class ClassicEffect(C.Structure):
+    """nxg_classic_effect: an effect type (CLASSIC_EFFECT_*) and its module index."""
+    _fields_ = [("type", C.c_uint32), ("module", C.c_uint32)]
 
 
 class ClassicPlan(C.Structure):
     """nxg_classic_plan: the plan's programs (host nxg_expr_program pointers, NULL: absent)."""
     _fields_ = [("expressions", C.c_void_p * len(CLASSIC_EXPRS)), ("has_cut", C.c_uint32),
                 ("clump_count", C.c_uint32), ("description_id", C.c_uint32), ("description_name_seed", C.c_uint32),
-                ("clumps_after_cut", C.c_uint32), ("has_noise", C.c_uint32)]
+                ("clumps_after_cut", C.c_uint32), ("has_noise", C.c_uint32), ("noise_count", C.c_uint32),
+                ("cut_count", C.c_uint32), ("effect_count", C.c_uint32),
+                ("effects", ClassicEffect * CLASSIC_MAX_EFFECTS)]
 
 
 class ClassicPlanInfo(C.Structure):
@@ -312,13 +337,15 @@ class ClassicPlanInfo(C.Structure):
                 ("final_length", C.c_uint32), ("clump_length", C.c_uint32), ("description_id", C.c_uint32),
                 ("description_name_seed", C.c_uint32), ("program_offset", C.c_uint32 * len(CLASSIC_EXPRS)),
                 ("uniform", C.c_uint32), ("clumps_after_cut", C.c_uint32), ("has_noise", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("noise_count", C.c_uint32), ("cut_count", C.c_uint32), ("effect_count", C.c_uint32),
+                ("effects", ClassicEffect * CLASSIC_MAX_EFFECTS), ("trailing_cut", C.c_uint32),
+                ("rewrites", C.c_uint32)]
 
 
 class ClassicClumpBinding(C.Structure):
     """nxg_classic_clump_binding: one ClumpingFX module's device arrays and host guide struct."""
     _fields_ = [("strand_guides", C.c_void_p), ("guides", C.c_void_p), ("guide_uvs", C.c_void_p),
-                ("guide_random_prefixes", C.c_void_p)]
+                ("guide_random_prefixes", C.c_void_p), ("local_axes", C.c_void_p)]
 
 
 class ClassicRuntimeInputs(C.Structure):
@@ -326,7 +353,7 @@ class ClassicRuntimeInputs(C.Structure):
     _fields_ = [("plan_image", C.c_void_p), ("plan", C.c_void_p), ("primitive_ids", C.c_void_p),
                 ("random_prefixes", C.c_void_p), ("columns", C.c_void_p), ("column_count", C.c_uint32),
                 ("radius_scale", C.c_float), ("clumps", C.c_void_p), ("surface_tangents", C.c_void_p),
-                ("noise_domain_positions", C.c_void_p)]
+                ("noise_domain_positions", C.c_void_p), ("root_relative", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ExprCompileOptions(C.Structure):

@@ -231,35 +231,60 @@ def classic_binding(name, ptex_count=0, custom_count=0):
 
 class ClassicPlan:
     """A ClassicFloatRuntimePlan (include/nanoxgen/xgen_classic_runtime.h:
-    130-165) of SplinePrimitive expressions, up to three ClumpingFX modules
-    applied in order (`clump`: a dict of one module's seven expressions by
-    attribute name, mask ... noiseCorrelation, or a list of such dicts) and at
-    most one CutFX module (after them, or before the last `clumps_after_cut`
-    of them), with an optional NoiseFX module (`noise`: its six expressions
-    by attribute name) after the clumps and before the cut, packed for the
-    device
-    (nxg_classic_plan_pack).  Each expression is text (compiled here with its
-    attribute name and object type) or a CompiledProgram; ptex_count and
-    custom_count place the custom and $Prefg-noise inputs in the input row."""
+    130-165) of SplinePrimitive expressions and FX modules, packed for the
+    device (nxg_classic_plan_pack).
+
+    `effects` is the plan's effect list in object order (the reference's
+    plan.effects): ("clump", dict of the module's seven expressions by
+    attribute name, mask ... noiseCorrelation), ("noise", dict of the six
+    NoiseFX expressions, mask ... preserveLength) or ("cut", the CutFX amount
+    expression), up to three clumps, two NoiseFX and two CutFX.  Without it
+    the short form applies: `clump` (one module's dict or a list of them),
+    an optional `noise` module and an optional `cut_amount`, the NoiseFX and
+    the CutFX after all clumps but the last `clumps_after_cut`.
+
+    Each expression is text (compiled here with its attribute name and
+    object type) or a CompiledProgram; ptex_count and custom_count place the
+    custom and $Prefg-noise inputs in the input row."""
 
     def __init__(self, *, length=None, width=None, taper=None, taper_start=None, width_ramp=None,
                  cut_amount=None, clump=None, description_id=0, description_name="", ptex_count=0,
-                 custom_count=0, clumps_after_cut=0, noise=None):
-        given = dict(length=length, width=width, cut_amount=cut_amount, taper=taper, taper_start=taper_start,
-                     width_ramp=width_ramp)
-        modules = [] if clump is None else [clump] if isinstance(clump, dict) else list(clump)
-        if len(modules) > abi.CLASSIC_MAX_CLUMPS:
-            raise ValueError(f"at most {abi.CLASSIC_MAX_CLUMPS} ClumpingFX modules")
-        for m, module in enumerate(modules):
+                 custom_count=0, clumps_after_cut=0, noise=None, effects=None):
+        given = dict(length=length, width=width, taper=taper, taper_start=taper_start, width_ramp=width_ramp)
+        if effects is not None:
+            if clump is not None or noise is not None or cut_amount is not None or clumps_after_cut:
+                raise ValueError("effects= replaces clump=, noise=, cut_amount= and clumps_after_cut=")
+            modules = dict(clump=[], noise=[], cut=[])
+            order = []
+            for kind, module in effects:
+                if kind not in modules:
+                    raise ValueError(f"unknown effect type {kind!r}")
+                order.append((kind, len(modules[kind])))
+                modules[kind].append(module)
+        else:
+            clumps = [] if clump is None else [clump] if isinstance(clump, dict) else list(clump)
+            before = len(clumps) - clumps_after_cut
+            order = ([("clump", m) for m in range(before)] + ([("noise", 0)] if noise is not None else [])
+                     + ([("cut", 0)] if cut_amount is not None else [])
+                     + [("clump", m) for m in range(before, len(clumps))])
+            modules = dict(clump=clumps, noise=[] if noise is None else [noise],
+                           cut=[] if cut_amount is None else [cut_amount])
+        limits = dict(clump=abi.CLASSIC_MAX_CLUMPS, noise=abi.CLASSIC_MAX_NOISES, cut=abi.CLASSIC_MAX_CUTS)
+        for kind, limit in limits.items():
+            if len(modules[kind]) > limit:
+                raise ValueError(f"at most {limit} {kind} modules")
+        for m, module in enumerate(modules["clump"]):
             missing = set(CLUMP_KEYS) - set(module)
             if missing:
                 raise ValueError(f"ClumpingFX module {m} needs {sorted(missing)}")
             given.update({abi.clump_key(m, CLUMP_KEYS[k]): v for k, v in module.items()})
-        if noise is not None:
-            missing = set(NOISE_KEYS) - set(noise)
+        for n, module in enumerate(modules["noise"]):
+            missing = set(NOISE_KEYS) - set(module)
             if missing:
-                raise ValueError(f"NoiseFX module needs {sorted(missing)}")
-            given.update({NOISE_KEYS[k]: v for k, v in noise.items()})
+                raise ValueError(f"NoiseFX module {n} needs {sorted(missing)}")
+            given.update({abi.noise_key(n, NOISE_KEYS[k]): v for k, v in module.items()})
+        for c, amount in enumerate(modules["cut"]):
+            given[abi.cut_key(c)] = amount
         self.programs = {}
         for key in abi.CLASSIC_EXPRS:
             src = given.get(key)
@@ -270,15 +295,25 @@ class ClassicPlan:
                 src, attribute, object_type)
             self.programs[key] = prog.bind_indices(
                 [classic_binding(name, ptex_count, custom_count) for name in prog.inputs])
-        self.has_cut = cut_amount is not None
-        self.clump_count = len(modules)
+        self.effects = order  # [(kind, module)] in application order
+        self.clump_count = len(modules["clump"])
+        self.noise_count = len(modules["noise"])
+        self.cut_count = len(modules["cut"])
+        self.has_cut = self.cut_count > 0
+        self.has_noise = self.noise_count > 0
         self.clumps_after_cut = clumps_after_cut
-        self.has_noise = noise is not None
         self.description_id = description_id
         self.name_seed = int(lib().nxg_string_seed(description_name.rsplit(":", 1)[-1].encode()))
-        self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, clump_count=self.clump_count,
-                                      clumps_after_cut=clumps_after_cut, has_noise=1 if noise is not None else 0,
-                                      description_id=description_id, description_name_seed=self.name_seed)
+        if effects is not None:
+            self.struct = abi.ClassicPlan(clump_count=self.clump_count, noise_count=self.noise_count,
+                                          cut_count=self.cut_count, effect_count=len(order),
+                                          description_id=description_id, description_name_seed=self.name_seed)
+            for e, (kind, module) in enumerate(order):
+                self.struct.effects[e] = abi.ClassicEffect(abi.CLASSIC_EFFECT_TYPES[kind], module)
+        else:
+            self.struct = abi.ClassicPlan(has_cut=1 if self.has_cut else 0, clump_count=self.clump_count,
+                                          clumps_after_cut=clumps_after_cut, has_noise=1 if self.has_noise else 0,
+                                          description_id=description_id, description_name_seed=self.name_seed)
         for k, key in enumerate(abi.CLASSIC_EXPRS):
             if key in self.programs:
                 self.struct.expressions[k] = C.addressof(self.programs[key].struct)
@@ -305,7 +340,7 @@ def runtime_workspace_bytes(strands, cvs=2, guides=0, plan=None):
 def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=None, primitive_ids=None,
                     random_prefixes=None, columns=None, radius_scale=1.0, strand_guides=None, guides=None,
                     guide_uvs=None, guide_random_prefixes=None, clumps=None, surface_tangents=None,
-                    noise_domain_positions=None, outputs=None, workspace=None, stream=None):
+                    noise_domain_positions=None, root_relative=False, outputs=None, workspace=None, stream=None):
     """Enqueue the Classic runtime chain of `plan` (a ClassicPlan) on
     `stream` over device curves: `points` float32 [N * cvs, 4], `roots` int32
     [N, 12] (RootSample records: the context's u, v and face); optional
@@ -316,10 +351,12 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
     list of dicts with `strand_guides` int32 [N], `guides` (dict of float32
     device tensors axes [G * cvs, 3], normals and tangents [G, 3], optional
     reference [G, 3]), `guide_uvs` float32 [G, 2] and `guide_random_prefixes`
-    int32 [G]; for a single module those four may be passed as keywords
+    int32 [G], optionally `local_axes` float32 [G * cvs, 3] (translation-free
+    guide CVs); for a single module the first four may be passed as keywords
     instead.  A NoiseFX takes `surface_tangents` float32 [N, 3] and optional
-    `noise_domain_positions` float32 [N, 3].  Returns the ClassicCutOutputs (its kept(), status() and
-    to_host() synchronise)."""
+    `noise_domain_positions` float32 [N, 3].  `root_relative` takes clump
+    goals about each strand's root.  Returns the ClassicCutOutputs (its
+    kept(), status() and to_host() synchronise)."""
     torch = _torch()
     if points.dtype != torch.float32 or points.dim() != 2 or points.shape[1] != 4 or not points.is_contiguous():
         raise ValueError("points must be a contiguous float32 [N * cvs, 4] tensor")
@@ -339,7 +376,8 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
                            ("tangents", gs["tangents"], g * 3), ("reference", gs.get("reference"), g * 3),
                            ("guide_uvs", binding["guide_uvs"], g * 2),
                            ("guide_random_prefixes", binding["guide_random_prefixes"], g),
-                           ("strand_guides", binding["strand_guides"], n)):
+                           ("strand_guides", binding["strand_guides"], n),
+                           ("local_axes", binding.get("local_axes"), g * cvs * 3)):
             if x is not None and (x.numel() < k or not x.is_contiguous() or x.element_size() != 4):
                 raise ValueError(f"clump {m}: {name} must be contiguous 32-bit with {k} elements")
         gd = abi.ClumpGuides(axes=gs["axes"].data_ptr(), normals=gs["normals"].data_ptr(),
@@ -348,7 +386,8 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
         keep.append(gd)
         bindings[m] = abi.ClassicClumpBinding(strand_guides=t(binding["strand_guides"]), guides=C.addressof(gd),
                                               guide_uvs=t(binding["guide_uvs"]),
-                                              guide_random_prefixes=t(binding["guide_random_prefixes"]))
+                                              guide_random_prefixes=t(binding["guide_random_prefixes"]),
+                                              local_axes=t(binding.get("local_axes")))
         if m < plan.clump_count:
             guide_count += g
     if outputs is None:
@@ -372,7 +411,8 @@ def classic_runtime(points, cvs, plan, *, roots, point_counts=None, root_uvs=Non
                                       radius_scale=radius_scale,
                                       clumps=C.addressof(bindings) if clumps else None,
                                       surface_tangents=t(surface_tangents),
-                                      noise_domain_positions=t(noise_domain_positions))
+                                      noise_domain_positions=t(noise_domain_positions),
+                                      root_relative=1 if root_relative else 0)
     out = outputs.descriptor()
     check(lib().nxg_classic_runtime_apply(C.byref(curves), C.byref(inputs), C.byref(out),
                                           C.c_void_p(workspace.data_ptr()), workspace.numel(),

@@ -90,6 +90,8 @@ struct ClumpModule {
     const nxg_vec3 *axes, *normals, *tangents, *reference;
     const nxg_vec2 *uvs;
     const uint32_t *prefixes;
+    const nxg_vec3 *local_axes;  // optional translation-free guide CVs (their spline lengths)
+    uint32_t local_frame;        // root_relative with local_axes: render and noise axes from them
 };
 
 struct CutArgs {
@@ -117,7 +119,7 @@ struct CutArgs {
     const uint8_t *image;        // packed plan, device
     uint32_t program_offset[NXG_CLASSIC_EXPR_COUNT];
     uint64_t present;            // bit k: program k given
-    uint32_t has_cut, final_length, profile;
+    uint32_t has_cut, final_length, profile;  // has_cut: the write resamples (a trailing CutFX)
     const uint32_t *ids, *prefixes;
     const float *columns;
     uint32_t column_count;
@@ -126,10 +128,12 @@ struct CutArgs {
     float2 *uniform_values;      // {value, ok} per uniform program (uniform_slot)
     struct RuntimeState *state;  // per strand, from the search to the write
     uint32_t *err_bits;          // the NXG_CLASSIC_* bits of every failing strand
-    // the ClumpingFX modules, in order
+    // the effects in order: type << 8 | module (NXG_CLASSIC_EFFECT_*)
+    uint32_t effect_count;
+    uint32_t effects[NXG_CLASSIC_MAX_EFFECTS];
+    uint32_t rewrites;           // some effect other than a trailing CutFX changes the CVs
+    uint32_t root_relative;      // clump goals about each strand's root
     uint32_t clump_count;
-    uint32_t clumps_after_cut;   // the last modules, which follow the CutFX
-    uint32_t has_noise;          // the NoiseFX module, after the clumps
     const nxg_vec3 *surface_tangents, *noise_domain;
     uint32_t clump_length;       // bit m: module m's amount or clumpScale reads cLength
     ClumpModule clumps[NXG_CLASSIC_MAX_CLUMPS];
@@ -617,6 +621,21 @@ __device__ F3 rotate_between(F3 value, F3 from, F3 to, float fraction) {
 // runtime.cpp:1041-1186) scaled by `magnitude` (1: the unit axis, which a
 // caller scales per strand in the reference's order); magnitude 0 writes
 // the zero axis without building the frame.
+// curve_spline_length (src/xgen_classic_runtime.cpp:665-690) of n guide CVs
+__device__ float axes_spline_length(const nxg_vec3 *axes, uint32_t n) {
+    const Vec3Curve src{axes};
+    const uint32_t intervals = n * 2u + 4u;
+    const float step = 1.0f / static_cast<float>(intervals);
+    F3 previous = xyz(src[0]);
+    float result = 0.0f;
+    for (uint32_t k = 1u; k < intervals; ++k) {
+        const F3 current = curve_eval(src, n, step * static_cast<float>(k));
+        result += sqrtf(length_squared(current - previous));
+        previous = current;
+    }
+    return result + sqrtf(length_squared(xyz(src[n - 1u]) - previous));
+}
+
 __device__ void clump_guide_axes(const nxg_vec3 *axes, uint32_t n, nxg_vec3 domain_point, nxg_vec3 raw_normal,
                                  nxg_vec3 raw_tangent, float frequency, float correlation, float magnitude,
                                  float4 *render, float4 *noise, float *length) {
@@ -627,19 +646,7 @@ __device__ void clump_guide_axes(const nxg_vec3 *axes, uint32_t n, nxg_vec3 doma
         render[cv] = make_float4(q.x, q.y, q.z, 0.0f);
     }
     render[n - 1u] = src[n - 1u];
-    {  // curve_spline_length (src/xgen_classic_runtime.cpp:665-690)
-        const uint32_t intervals = n * 2u + 4u;
-        const float step = 1.0f / static_cast<float>(intervals);
-        F3 previous = xyz(src[0]);
-        float result = 0.0f;
-        for (uint32_t k = 1u; k < intervals; ++k) {
-            const F3 current = curve_eval(src, n, step * static_cast<float>(k));
-            result += sqrtf(length_squared(current - previous));
-            previous = current;
-        }
-        result += sqrtf(length_squared(xyz(src[n - 1u]) - previous));
-        *length = result;
-    }
+    *length = axes_spline_length(axes, n);
     for (uint32_t cv = 0u; cv < n; ++cv) noise[cv] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (magnitude == 0.0f) return;  // the noise or mask test failed: a zero axis
     float polyline = 0.0f;
@@ -1066,9 +1073,16 @@ __global__ void __launch_bounds__(64) runtime_guide_kernel(CutArgs a, uint32_t m
     const float frequency = max_std(raw_frequency, 0.0f);
     const float correlation = clamp_std(raw_correlation * 0.01f, 0.0f, 1.0f);
     const nxg_vec3 *axes = cm.axes + uint64_t{g} * n;
+    const nxg_vec3 *local = cm.local_axes ? cm.local_axes + uint64_t{g} * n : nullptr;
+    // with root_relative and local axes the render and noise axes come from
+    // the local copy; the noise domain stays the world root or reference
+    // position, and the spline length is the local copy's whenever there is
+    // one (prepare_xgen_classic_clump_runtime_data, :839-890, 1697-1721)
     float length = 0.0f;
-    clump_guide_axes(axes, n, cm.reference ? cm.reference[g] : axes[0], cm.normals[g], cm.tangents[g], frequency,
-                     correlation, ok ? 1.0f : 0.0f, a.guide_render + slot * n, a.guide_noise + slot * n, &length);
+    clump_guide_axes(cm.local_frame ? local : axes, n, cm.reference ? cm.reference[g] : axes[0], cm.normals[g],
+                     cm.tangents[g], frequency, correlation, ok ? 1.0f : 0.0f, a.guide_render + slot * n,
+                     a.guide_noise + slot * n, &length);
+    if (local && !cm.local_frame) length = axes_spline_length(local, n);
     a.guide_info[slot] = make_float2(length, ok ? 1.0f : 0.0f);
 }
 
@@ -1201,6 +1215,21 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
     float4 *out = a.clumped + uint64_t{s} * n;
     const float4 *render = a.guide_render + slot * n;
     const float4 *unit = a.guide_noise + slot * n;
+    // the goal's offset (:1627-1628, 1773-1776): render - origin, or for a
+    // local guide render + (guide root - origin); origin is the strand's
+    // root with root_relative, else 0 (render - 0 is render, -0 included)
+    F3 shift{0.0f, 0.0f, 0.0f};
+    const bool offset = a.root_relative || cm.local_frame;
+    if (offset) {
+        const nxg_vec3 r = a.roots[s].position;
+        const F3 origin = a.root_relative ? F3{r.x, r.y, r.z} : F3{0.0f, 0.0f, 0.0f};
+        if (cm.local_frame) {
+            const nxg_vec3 gr = cm.axes[uint64_t{g} * n];
+            shift = F3{gr.x, gr.y, gr.z} - origin;
+        } else {
+            shift = origin;
+        }
+    }
     const float step = cut / static_cast<float>(n - 1u);
     out[0] = p[0];
     for (uint32_t cv = 1u; cv < n; ++cv) {
@@ -1211,7 +1240,9 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
             *err = NXG_CLASSIC_EXPRESSION_ERROR;
             return false;
         }
-        q = q + (xyz(render[cv]) - q) * (mask * amount * (1.0f - 2.0f * scale));
+        F3 goal = xyz(render[cv]);
+        if (offset) goal = cm.local_frame ? goal + shift : goal - shift;
+        q = q + (goal - q) * (mask * amount * (1.0f - 2.0f * scale));
         // the noise axis is added to every CV, zero when it is off (so -0 becomes +0)
         F3 d{0.0f, 0.0f, 0.0f};
         if (noisy) {
@@ -1243,13 +1274,14 @@ __device__ __forceinline__ F3 fallback_surface_u(F3 normal) {
 // each axis; the preserve-length rescale.  The result goes to a.clumped and
 // back into the lane's LDS row; c_length follows.  False (with *err) where
 // the reference throws.
-__device__ __forceinline__ bool runtime_noise(const CutArgs &a, const ClassicContext &c, const StagedCurve<true> &p,
-                                              uint32_t s, float &c_length, float c_width, bool &at_one,
-                                              bool &written, uint32_t *err) {
+__device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module, const ClassicContext &c,
+                                              const StagedCurve<true> &p, uint32_t s, float &c_length, float c_width,
+                                              bool &at_one, bool &written, uint32_t *err) {
     const uint32_t n = a.cvs;
+#define NOISE_PROGRAM(k) NXG_CLASSIC_EXPR_NOISE_OF(module, NXG_CLASSIC_EXPR_NOISE_##k)
     const float t0 = at_one ? 1.0f : 0.0f;
     float raw_mask, raw_frequency, raw_correlation, raw_preserve;
-    if (!classic_eval(a, NXG_CLASSIC_EXPR_NOISE_MASK, c, c_length, c_width, t0, raw_mask)) {
+    if (!classic_eval(a, NOISE_PROGRAM(MASK), c, c_length, c_width, t0, raw_mask)) {
         *err = NXG_CLASSIC_EXPRESSION_ERROR;
         return false;
     }
@@ -1261,9 +1293,9 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, const ClassicCon
         c_length = spline_length(a.length_samples, p, n);
         return true;
     }
-    if (!classic_eval(a, NXG_CLASSIC_EXPR_NOISE_FREQUENCY, c, c_length, c_width, t0, raw_frequency) ||
-        !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_CORRELATION, c, c_length, c_width, t0, raw_correlation) ||
-        !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_PRESERVE_LENGTH, c, c_length, c_width, t0, raw_preserve)) {
+    if (!classic_eval(a, NOISE_PROGRAM(FREQUENCY), c, c_length, c_width, t0, raw_frequency) ||
+        !classic_eval(a, NOISE_PROGRAM(CORRELATION), c, c_length, c_width, t0, raw_correlation) ||
+        !classic_eval(a, NOISE_PROGRAM(PRESERVE_LENGTH), c, c_length, c_width, t0, raw_preserve)) {
         *err = NXG_CLASSIC_EXPRESSION_ERROR;
         return false;
     }
@@ -1318,8 +1350,8 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, const ClassicCon
         const F3 tangent = cross3(binormal, normal);
         const float t = cv_parameter(cv, n);
         float raw_magnitude, raw_scale;
-        if (!classic_eval(a, NXG_CLASSIC_EXPR_NOISE_MAGNITUDE, c, c_length, c_width, t, raw_magnitude) ||
-            !classic_eval(a, NXG_CLASSIC_EXPR_NOISE_MAGNITUDE_SCALE, c, c_length, c_width, t, raw_scale)) {
+        if (!classic_eval(a, NOISE_PROGRAM(MAGNITUDE), c, c_length, c_width, t, raw_magnitude) ||
+            !classic_eval(a, NOISE_PROGRAM(MAGNITUDE_SCALE), c, c_length, c_width, t, raw_scale)) {
             *err = NXG_CLASSIC_EXPRESSION_ERROR;
             return false;
         }
@@ -1363,6 +1395,7 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, const ClassicCon
     at_one = true;
     written = true;
     return true;
+#undef NOISE_PROGRAM
 }
 
 // The rebuild half of cut_from_tip_and_rebuild (src/xgen_classic_runtime.cpp:
@@ -1425,54 +1458,53 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     }
     if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
     const float c_width = width;
-    bool clumped = false;  // the context's t is 1: the last module with a guide clumped the strand
+    bool clumped = false;  // the context's t is 1: the last effect that set it ran its CV loop
     bool written = false;  // a.clumped holds the strand's current CVs
     constexpr bool kStaged = std::is_same<Curve, StagedCurve<true>>::value;
-    // the effects in object order: the clump modules (strands of at most 64
-    // CVs, always staged), with the NoiseFX and the CutFX after the first
-    // `before` of them; one loop, so the clump step is compiled once
-    const uint32_t before = a.clump_count - a.clumps_after_cut;
+    // the effects in list order (:1679-1826); effects that change the CVs
+    // run on staged strands (at most 64 CVs), a trailing CutFX also on
+    // strands read from memory (the write resamples them)
     float cut = 1.0f;
     bool keep = true;
 #pragma unroll 1
-    for (uint32_t m = 0; m <= a.clump_count; ++m) {
-        if (m == before) {
-            if constexpr (kStaged && NOISE) {
-                if (a.has_noise && !runtime_noise(a, c, source, s, c_length, c_width, clumped, written, err))
-                    return -1.0f;
-            }
-            if (a.has_cut) {
-                float amount;
-                if (!classic_value(a, NXG_CLASSIC_EXPR_CUT_AMOUNT, c, c_length, c_width, 0u, amount, clumped)) {
-                    *err = NXG_CLASSIC_EXPRESSION_ERROR;
-                    return -1.0f;
-                }
-                if (!isfinite(amount)) {
-                    *err = NXG_CLASSIC_NONFINITE_AMOUNT;
-                    return -1.0f;
-                }
-                cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
-                keep = cut >= 1.0e-4f;
-                if constexpr (kStaged) {
-                    if (a.clumps_after_cut) {  // later modules read the cut strand: rebuild it now
-                        rebuild_staged(a, source, s, cut);
-                        written = true;
-                        cut = 1.0f;  // nothing left for the write to resample
-                        c_length = spline_length(a.length_samples, source, n);
-                    }
-                }
-                // the post-effect programs read the rebuilt curve's cLength
-                if (!a.clumps_after_cut && a.final_length)
-                    c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
-            }
-        }
-        if constexpr (kStaged) {
-            if (m < a.clump_count && !runtime_clump(a, m, c, source, s, c_length, c_width, clumped, written, err))
+    for (uint32_t e = 0; e < a.effect_count; ++e) {
+        const uint32_t type = a.effects[e] >> 8u, module = a.effects[e] & 0xffu;
+        if (type == NXG_CLASSIC_EFFECT_CUT) {
+            float amount;
+            if (!classic_value(a, NXG_CLASSIC_EXPR_CUT_AMOUNT_OF(module), c, c_length, c_width, 0u, amount,
+                               clumped)) {
+                *err = NXG_CLASSIC_EXPRESSION_ERROR;
                 return -1.0f;
+            }
+            if (!isfinite(amount)) {
+                *err = NXG_CLASSIC_NONFINITE_AMOUNT;
+                return -1.0f;
+            }
+            cut = cut_parameter(a.walk_samples, a.walk_count, p, n, amount < 0.0f ? 0.0f : amount);
+            if (cut < 1.0e-4f) keep = false;  // culled, but the later effects still run
+            if (e + 1u < a.effect_count) {
+                if constexpr (kStaged) {  // later effects read the cut strand: rebuild it now
+                    rebuild_staged(a, source, s, cut);
+                    written = true;
+                    cut = 1.0f;  // nothing left for the write to resample
+                    c_length = spline_length(a.length_samples, source, n);
+                }
+            } else if (a.final_length) {  // the post-effect programs read the rebuilt curve's cLength
+                c_length = spline_length(a.length_samples, RebuiltCurve<ScaledCurve<Curve>>{p, n, cut}, n);
+            }
+        } else if (type == NXG_CLASSIC_EFFECT_NOISE) {
+            if constexpr (kStaged && NOISE) {
+                if (!runtime_noise(a, module, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
+            }
+        } else {
+            if constexpr (kStaged) {
+                if (!runtime_clump(a, module, c, source, s, c_length, c_width, clumped, written, err))
+                    return -1.0f;
+            }
         }
     }
     if constexpr (kStaged) {
-        if ((a.clump_count || a.has_noise) && !written) {  // the write reads every strand from a.clumped
+        if (a.rewrites && !written) {  // the write reads every strand from a.clumped
             float4 *out = a.clumped + uint64_t{s} * n;
             for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = source[cv];
         }
@@ -1780,7 +1812,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     uint32_t guides = 0u;
     for (uint32_t m = 0; m < plan.clump_count; ++m) guides += inputs->clumps[m].guides->guide_count;
     const ClassicLayout layout = classic_layout(n);
-    const ClassicRuntimeLayout rt = classic_runtime_layout(n, cvs, guides, plan.clump_count || plan.has_noise);
+    const ClassicRuntimeLayout rt = classic_runtime_layout(n, cvs, guides, plan.rewrites != 0u);
     if (!workspace || workspace_bytes < rt.total || reinterpret_cast<uintptr_t>(workspace) % 256u != 0u)
         return NXG_ERROR_WORKSPACE_TOO_SMALL;
     uint8_t *ws = static_cast<uint8_t *>(workspace);
@@ -1789,7 +1821,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.image = static_cast<const uint8_t *>(inputs->plan_image);
     for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k) a.program_offset[k] = plan.program_offset[k];
     a.present = plan.present;
-    a.has_cut = plan.has_cut;
+    a.has_cut = plan.trailing_cut;
     a.final_length = plan.final_length;
     constexpr uint64_t one = 1u;
     a.profile = (plan.present & ((one << NXG_CLASSIC_EXPR_WIDTH) | (one << NXG_CLASSIC_EXPR_TAPER) |
@@ -1801,18 +1833,18 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     a.description_id = static_cast<float>(plan.description_id);
     a.name_seed = static_cast<float>(plan.description_name_seed);
     a.uniform = plan.uniform & static_cast<uint32_t>(plan.present);
-    a.has_noise = plan.has_noise;
+    a.effect_count = plan.effect_count;
+    for (uint32_t e = 0; e < plan.effect_count; ++e) a.effects[e] = plan.effects[e].type << 8u | plan.effects[e].module;
+    a.rewrites = plan.rewrites;
+    a.root_relative = inputs->root_relative ? 1u : 0u;
     a.surface_tangents = inputs->surface_tangents;
     a.noise_domain = inputs->noise_domain_positions;
     a.uniform_values = reinterpret_cast<float2 *>(ws + rt.uniform_offset);
     a.state = reinterpret_cast<RuntimeState *>(ws + rt.state_offset);
     a.err_bits = a.status_word + 1;
-    if (plan.clump_count || plan.has_noise) {
-        a.clumped = reinterpret_cast<float4 *>(ws + rt.clumped_offset);
-    }
+    if (plan.rewrites) a.clumped = reinterpret_cast<float4 *>(ws + rt.clumped_offset);
     if (plan.clump_count) {
         a.clump_count = plan.clump_count;
-        a.clumps_after_cut = plan.clumps_after_cut;
         a.clump_length = plan.clump_length;
         uint32_t base = 0u;
         for (uint32_t m = 0; m < plan.clump_count; ++m) {
@@ -1829,6 +1861,8 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
             cm.reference = g.reference_positions;
             cm.uvs = b.guide_uvs;
             cm.prefixes = b.guide_random_prefixes;
+            cm.local_axes = b.local_axes;
+            cm.local_frame = inputs->root_relative && b.local_axes ? 1u : 0u;
         }
         a.clumped = reinterpret_cast<float4 *>(ws + rt.clumped_offset);
         a.guide_render = reinterpret_cast<float4 *>(ws + rt.render_offset);
@@ -1847,7 +1881,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
                                stream, a, m);
         if (staged)
         {
-            if (a.has_noise)
+            if (plan.noise_count)
                 hipLaunchKernelGGL((runtime_search_kernel<true, true>), dim3(a.tile_count), dim3(kWave), tile_lds,
                                    stream, a);
             else
@@ -1860,12 +1894,12 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     }
     hipLaunchKernelGGL(cut_scan_groups_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
     if (a.tile_count) {
-        // after a clump the write starts from the clumped strands, already scaled
+        // after an effect that changes the CVs the write starts from the
+        // search's copy, already scaled (and rebuilt by every non-trailing cut)
         CutArgs w = a;
-        if (a.clump_count || a.has_noise) {
+        if (a.rewrites) {
             w.points = a.clumped;
             w.present &= ~(uint64_t{1} << NXG_CLASSIC_EXPR_LENGTH);
-            if (a.clumps_after_cut) w.has_cut = 0u;  // the search rebuilt the cut strands
         }
         const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);
         if (cvs <= 8u)

@@ -266,28 +266,47 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
     if (!std::isfinite(rs) || rs < 0.0f) return NXG_ERROR_INVALID_RADIUS_SCALE;
     const nxg_classic_cut_params common{0.0f, 0.0f, rs, 0u, nullptr};
     if (uint32_t st = nxg_validate_classic_cut(curves, &common, output); st != NXG_OK) return st;
-    if (plan.has_cut && !((plan.present >> NXG_CLASSIC_EXPR_CUT_AMOUNT) & 1u)) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
-    if (plan.present >> NXG_CLASSIC_EXPR_COUNT) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
-    if (plan.has_noise) {
-        const uint64_t all = ((uint64_t{1} << NXG_CLASSIC_EXPR_COUNT) - 1u) &
-                             ~((uint64_t{1} << NXG_CLASSIC_EXPR_NOISE_MASK) - 1u);
-        if ((plan.present & all) != all || plan.clumps_after_cut || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS ||
-            (curves->strand_count && !inputs->surface_tangents))
-            return NXG_ERROR_INVALID_CLASSIC_REQUEST;
-    } else if (plan.present >> NXG_CLASSIC_EXPR_NOISE_MASK) {
+    // the effect list (the packer's report; the device indexes modules by it)
+    if (plan.effect_count > NXG_CLASSIC_MAX_EFFECTS || plan.clump_count > NXG_CLASSIC_MAX_CLUMPS ||
+        plan.noise_count > NXG_CLASSIC_MAX_NOISES || plan.cut_count > NXG_CLASSIC_MAX_CUTS ||
+        plan.has_cut != (plan.cut_count ? 1u : 0u) || plan.has_noise != (plan.noise_count ? 1u : 0u))
         return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    {
+        const uint32_t counts[3] = {plan.clump_count, plan.noise_count, plan.cut_count};
+        uint32_t seen[3] = {0u, 0u, 0u};
+        for (uint32_t e = 0; e < plan.effect_count; ++e) {
+            const nxg_classic_effect &x = plan.effects[e];
+            if (x.type > NXG_CLASSIC_EFFECT_CUT || x.module >= counts[x.type] || ((seen[x.type] >> x.module) & 1u))
+                return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+            seen[x.type] |= 1u << x.module;
+        }
+        for (uint32_t t = 0; t < 3u; ++t)
+            if (seen[t] != (1u << counts[t]) - 1u) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+        const uint32_t trailing =
+            plan.effect_count && plan.effects[plan.effect_count - 1u].type == NXG_CLASSIC_EFFECT_CUT ? 1u : 0u;
+        if (plan.trailing_cut != trailing || plan.rewrites != (plan.effect_count > trailing ? 1u : 0u))
+            return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     }
-    if (plan.clump_count > NXG_CLASSIC_MAX_CLUMPS || plan.clumps_after_cut > plan.clump_count ||
-        (plan.clumps_after_cut && !plan.has_cut))
+    // every module's programs present, none of an absent module
+    {
+        uint64_t need = 0u, allowed = (uint64_t{1} << NXG_CLASSIC_EXPR_CLUMP_MASK) - 1u;
+        allowed &= ~(uint64_t{1} << NXG_CLASSIC_EXPR_CUT_AMOUNT);
+        for (uint32_t m = 0; m < plan.clump_count; ++m)
+            for (uint32_t k = 0; k < NXG_CLASSIC_CLUMP_EXPRS; ++k)
+                need |= uint64_t{1} << NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_MASK + k);
+        for (uint32_t n = 0; n < plan.noise_count; ++n)
+            for (uint32_t k = 0; k < NXG_CLASSIC_NOISE_EXPRS; ++k)
+                need |= uint64_t{1} << NXG_CLASSIC_EXPR_NOISE_OF(n, NXG_CLASSIC_EXPR_NOISE_MASK + k);
+        for (uint32_t c = 0; c < plan.cut_count; ++c) need |= uint64_t{1} << NXG_CLASSIC_EXPR_CUT_AMOUNT_OF(c);
+        allowed |= need;
+        if ((plan.present & need) != need || (plan.present & ~allowed)) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    }
+    // effects that change the CVs run on staged strands
+    if (plan.rewrites && curves->cvs_per_strand > NXG_CLUMP_MAX_CVS) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+    if (plan.noise_count && curves->strand_count && !inputs->surface_tangents)
         return NXG_ERROR_INVALID_CLASSIC_REQUEST;
     if (plan.clump_count) {
-        const uint32_t first = NXG_CLASSIC_EXPR_CLUMP_MASK;
-        const uint32_t used = first + NXG_CLASSIC_CLUMP_EXPRS * plan.clump_count;
-        const uint64_t all = ((uint64_t{1} << used) - 1u) & ~((uint64_t{1} << first) - 1u);
-        const uint64_t clumps = (plan.present & ((uint64_t{1} << NXG_CLASSIC_EXPR_NOISE_MASK) - 1u));
-        if ((plan.present & all) != all || (clumps >> used) || curves->cvs_per_strand > NXG_CLUMP_MAX_CVS ||
-            !inputs->clumps)
-            return NXG_ERROR_INVALID_CLASSIC_REQUEST;
+        if (!inputs->clumps) return NXG_ERROR_INVALID_CLASSIC_REQUEST;
         uint64_t guides = 0u;
         for (uint32_t m = 0; m < plan.clump_count; ++m) {
             const nxg_classic_clump_binding &b = inputs->clumps[m];
@@ -314,6 +333,5 @@ uint32_t nxg_validate_classic_runtime(const nxg_classic_curves *curves, const nx
 
 uint64_t nxg_classic_runtime_workspace_size(const nxg_classic_plan_info *plan, uint32_t strand_count,
                                             uint32_t cvs_per_strand, uint32_t guide_count) {
-    const bool rewrites = plan && (plan->clump_count || plan->has_noise);
-    return nxg::classic_runtime_layout(strand_count, cvs_per_strand, guide_count, rewrites).total;
+    return nxg::classic_runtime_layout(strand_count, cvs_per_strand, guide_count, plan && plan->rewrites).total;
 }

@@ -337,39 +337,96 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
                                nxg_classic_plan_info *info) {
     using namespace nxg;
     if (!plan || !info) return NXG_ERROR_NULL_ARGUMENT;
-    if (plan->has_cut && !plan->expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT]) return NXG_ERROR_INVALID_PROGRAM;
     if (plan->clump_count > NXG_CLASSIC_MAX_CLUMPS) return NXG_ERROR_INVALID_PROGRAM;
-    if (plan->clumps_after_cut > plan->clump_count || (plan->clumps_after_cut && !plan->has_cut))
-        return NXG_ERROR_INVALID_PROGRAM;
-    if (plan->has_noise && plan->clumps_after_cut) return NXG_ERROR_INVALID_PROGRAM;
-    if (plan->has_noise)
-        for (uint32_t k = NXG_CLASSIC_EXPR_NOISE_MASK; k < NXG_CLASSIC_EXPR_COUNT; ++k)
-            if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
-    const uint32_t used = NXG_CLASSIC_EXPR_CLUMP_MASK + NXG_CLASSIC_CLUMP_EXPRS * plan->clump_count;
-    for (uint32_t k = NXG_CLASSIC_EXPR_CLUMP_MASK; k < used; ++k)
-        if (!plan->expressions[k]) return NXG_ERROR_INVALID_PROGRAM;
     nxg_classic_plan_info r{};
-    r.has_cut = plan->has_cut ? 1u : 0u;
+    uint32_t noises = 0u, cuts = 0u;
+    auto push = [&](uint32_t type, uint32_t module) { r.effects[r.effect_count++] = nxg_classic_effect{type, module}; };
+    if (plan->effect_count == 0u) {
+        // the short form: clumps, then the NoiseFX and the CutFX before the last clumps_after_cut clumps
+        if (plan->noise_count || plan->cut_count) return NXG_ERROR_INVALID_PROGRAM;
+        if (plan->clumps_after_cut > plan->clump_count || (plan->clumps_after_cut && !plan->has_cut))
+            return NXG_ERROR_INVALID_PROGRAM;
+        if (plan->has_noise && plan->clumps_after_cut) return NXG_ERROR_INVALID_PROGRAM;
+        const uint32_t before = plan->clump_count - plan->clumps_after_cut;
+        for (uint32_t m = 0; m < before; ++m) push(NXG_CLASSIC_EFFECT_CLUMP, m);
+        if (plan->has_noise) push(NXG_CLASSIC_EFFECT_NOISE, 0u);
+        if (plan->has_cut) push(NXG_CLASSIC_EFFECT_CUT, 0u);
+        for (uint32_t m = before; m < plan->clump_count; ++m) push(NXG_CLASSIC_EFFECT_CLUMP, m);
+        noises = plan->has_noise ? 1u : 0u;
+        cuts = plan->has_cut ? 1u : 0u;
+        r.clumps_after_cut = plan->clumps_after_cut;
+    } else {
+        // an effect list naming every module exactly once
+        if (plan->effect_count > NXG_CLASSIC_MAX_EFFECTS || plan->noise_count > NXG_CLASSIC_MAX_NOISES ||
+            plan->cut_count > NXG_CLASSIC_MAX_CUTS || plan->has_cut || plan->has_noise || plan->clumps_after_cut)
+            return NXG_ERROR_INVALID_PROGRAM;
+        const uint32_t counts[3] = {plan->clump_count, plan->noise_count, plan->cut_count};
+        uint32_t seen[3] = {0u, 0u, 0u};
+        for (uint32_t e = 0; e < plan->effect_count; ++e) {
+            const nxg_classic_effect &x = plan->effects[e];
+            if (x.type > NXG_CLASSIC_EFFECT_CUT || x.module >= counts[x.type] || ((seen[x.type] >> x.module) & 1u))
+                return NXG_ERROR_INVALID_PROGRAM;
+            seen[x.type] |= 1u << x.module;
+            push(x.type, x.module);
+        }
+        for (uint32_t t = 0; t < 3u; ++t)
+            if (seen[t] != (1u << counts[t]) - 1u) return NXG_ERROR_INVALID_PROGRAM;
+        noises = plan->noise_count;
+        cuts = plan->cut_count;
+    }
+    // every module's programs
+    for (uint32_t m = 0; m < plan->clump_count; ++m)
+        for (uint32_t k = 0; k < NXG_CLASSIC_CLUMP_EXPRS; ++k)
+            if (!plan->expressions[NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_MASK + k)])
+                return NXG_ERROR_INVALID_PROGRAM;
+    for (uint32_t n = 0; n < noises; ++n)
+        for (uint32_t k = 0; k < NXG_CLASSIC_NOISE_EXPRS; ++k)
+            if (!plan->expressions[NXG_CLASSIC_EXPR_NOISE_OF(n, NXG_CLASSIC_EXPR_NOISE_MASK + k)])
+                return NXG_ERROR_INVALID_PROGRAM;
+    for (uint32_t c = 0; c < cuts; ++c)
+        if (!plan->expressions[NXG_CLASSIC_EXPR_CUT_AMOUNT_OF(c)]) return NXG_ERROR_INVALID_PROGRAM;
+    r.has_cut = cuts ? 1u : 0u;
+    r.cut_count = cuts;
     r.clump_count = plan->clump_count;
-    r.clumps_after_cut = plan->clumps_after_cut;
-    r.has_noise = plan->has_noise ? 1u : 0u;
+    r.has_noise = noises ? 1u : 0u;
+    r.noise_count = noises;
+    r.trailing_cut = r.effect_count && r.effects[r.effect_count - 1u].type == NXG_CLASSIC_EFFECT_CUT ? 1u : 0u;
+    r.rewrites = r.effect_count > r.trailing_cut ? 1u : 0u;
     r.description_id = plan->description_id;
     r.description_name_seed = plan->description_name_seed;
+    // the module and role of program k (role: its index in module 0's block), and whether the plan uses it
+    auto role_of = [](uint32_t k, uint32_t &module) -> uint32_t {
+        module = 0u;
+        if (k >= NXG_CLASSIC_EXPR_CUT_AMOUNT_1) {
+            module = k - NXG_CLASSIC_EXPR_CUT_AMOUNT_1 + 1u;
+            return NXG_CLASSIC_EXPR_CUT_AMOUNT;
+        }
+        if (k >= NXG_CLASSIC_EXPR_NOISE_MASK) {
+            module = (k - NXG_CLASSIC_EXPR_NOISE_MASK) / NXG_CLASSIC_NOISE_EXPRS;
+            return k - NXG_CLASSIC_NOISE_EXPRS * module;
+        }
+        if (k >= NXG_CLASSIC_EXPR_CLUMP_MASK) {
+            module = (k - NXG_CLASSIC_EXPR_CLUMP_MASK) / NXG_CLASSIC_CLUMP_EXPRS;
+            return k - NXG_CLASSIC_CLUMP_EXPRS * module;
+        }
+        return k;
+    };
     uint64_t total = 0u;
     PackPlan plans[NXG_CLASSIC_EXPR_COUNT];
     for (uint32_t k = 0; k < NXG_CLASSIC_EXPR_COUNT; ++k) {
         const nxg_expr_program *p = plan->expressions[k];
-        const bool noise = k >= NXG_CLASSIC_EXPR_NOISE_MASK;
-        if (!p || (k == NXG_CLASSIC_EXPR_CUT_AMOUNT && !plan->has_cut) || (!noise && k >= used) ||
-            (noise && !plan->has_noise))
-            continue;
-        // module and role of a clump program (role = its module-0 index)
-        const uint32_t module = k >= NXG_CLASSIC_EXPR_CLUMP_MASK && !noise
-                                    ? (k - NXG_CLASSIC_EXPR_CLUMP_MASK) / NXG_CLASSIC_CLUMP_EXPRS : 0u;
-        const uint32_t role = k - NXG_CLASSIC_CLUMP_EXPRS * module;
+        uint32_t module;
+        const uint32_t role = role_of(k, module);
+        const bool noise = role >= NXG_CLASSIC_EXPR_NOISE_MASK;
+        const bool clump = !noise && role >= NXG_CLASSIC_EXPR_CLUMP_MASK;
+        const bool used = role == NXG_CLASSIC_EXPR_CUT_AMOUNT ? module < cuts
+                          : noise                            ? module < noises
+                          : clump                            ? module < plan->clump_count
+                                                             : true;
+        if (!p || !used) continue;
         if (!validate_classic_program(*p)) return NXG_ERROR_INVALID_PROGRAM;
         // the noise frequency and correlation are evaluated once per guide
-        if (!noise &&
+        if (clump &&
             (role == NXG_CLASSIC_EXPR_CLUMP_NOISE_FREQUENCY || role == NXG_CLASSIC_EXPR_CLUMP_NOISE_CORRELATION) &&
             p->input_count != 0u)
             return NXG_ERROR_INVALID_PROGRAM;
@@ -385,7 +442,7 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
             if (b == NXG_CLASSIC_VAR_CLENGTH &&
                 (k == NXG_CLASSIC_EXPR_TAPER || k == NXG_CLASSIC_EXPR_TAPER_START || k == NXG_CLASSIC_EXPR_WIDTH_RAMP))
                 r.final_length = 1u;
-            if (b == NXG_CLASSIC_VAR_CLENGTH && k >= NXG_CLASSIC_EXPR_CLUMP_MASK && !noise &&
+            if (b == NXG_CLASSIC_VAR_CLENGTH && clump &&
                 (role == NXG_CLASSIC_EXPR_CLUMP || role == NXG_CLASSIC_EXPR_CLUMP_SCALE))
                 r.clump_length |= 1u << module;
         }
@@ -393,7 +450,8 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
         bool uniform = p->input_count == 0u;
         for (uint32_t i = 0; i < p->instruction_count; ++i)
             if (p->instructions[i].op == NXG_OP_RANDOM) uniform = false;
-        if (uniform && !noise) r.uniform |= 1u << k;  // NoiseFX programs run per strand
+        // NoiseFX programs run per strand; the table holds one CutFX amount (module 0's)
+        if (uniform && !noise && k < 32u) r.uniform |= 1u << k;
     }
     r.byte_size = total ? total : 16u;
     *info = r;

@@ -97,7 +97,7 @@ inline ClassicLayout classic_layout(uint32_t strand_count) {
 //   clump only:     the clumped strands (strand_count x cvs float4), and per
 //                   guide its render axis and unit noise axis (cvs float4
 //                   each) and {spline length, noise frequency-correlation ok}
-constexpr uint32_t kClassicScalarSlots = 40u;
+constexpr uint32_t kClassicScalarSlots = 48u;  // every program at t = 0, three spares at t = 1
 constexpr uint32_t kClassicCvTables = 1u + 2u * NXG_CLASSIC_MAX_CLUMPS;  // widthRamp; clumpScale, noiseScale per module
 constexpr uint64_t kClassicUniformBytes = (kClassicScalarSlots + kClassicCvTables * NXG_CLASSIC_MAX_CVS) * 8u;
 struct ClassicRuntimeLayout {

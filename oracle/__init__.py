@@ -151,7 +151,7 @@ def ref():
                                           C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_float,
                                           C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                          C.c_char_p, C.c_uint64, C.c_void_p]
+                                          C.c_void_p, C.c_void_p, C.c_int, C.c_char_p, C.c_uint64, C.c_void_p]
         L.ref_classic_cut.restype = C.c_uint32
         L.ref_classic_cut.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                       C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -410,16 +410,21 @@ class _ClumpBinding(C.Structure):
     _fields_ = [("axes3", C.c_void_p), ("render3", C.c_void_p), ("lengths", C.c_void_p), ("normals3", C.c_void_p),
                 ("tangents3", C.c_void_p), ("reference3", C.c_void_p), ("guide_uvs", C.c_void_p),
                 ("guide_prefixes", C.c_void_p), ("strand_guides", C.c_void_p), ("guide_count", C.c_uint32),
-                ("gradients", C.c_void_p)]
+                ("gradients", C.c_void_p), ("local3", C.c_void_p), ("local_render3", C.c_void_p)]
+
+
+_MAX_EFFECTS = 8
+_EFFECT_TYPES = {"clump": 0, "noise": 1, "cut": 2}
 
 
 class _ClassicPlan(C.Structure):
     """orc_classic_plan."""
-    _fields_ = [("length", C.c_void_p), ("width", C.c_void_p), ("cut_amount", C.c_void_p), ("taper", C.c_void_p),
-                ("taper_start", C.c_void_p), ("width_ramp", C.c_void_p), ("has_cut", C.c_int),
-                ("description_id", C.c_uint32), ("name_seed", C.c_uint32), ("clump_count", C.c_uint32),
-                ("clump", (C.c_void_p * 7) * 3), ("binding", C.c_void_p * 3), ("clumps_after_cut", C.c_uint32),
-                ("has_noise", C.c_int), ("noise", C.c_void_p * 6), ("surface_tangents3", C.c_void_p),
+    _fields_ = [("length", C.c_void_p), ("width", C.c_void_p), ("taper", C.c_void_p),
+                ("taper_start", C.c_void_p), ("width_ramp", C.c_void_p),
+                ("description_id", C.c_uint32), ("name_seed", C.c_uint32), ("effect_count", C.c_uint32),
+                ("effect_type", C.c_uint32 * _MAX_EFFECTS), ("effect_module", C.c_uint32 * _MAX_EFFECTS),
+                ("clump", (C.c_void_p * 7) * 3), ("binding", C.c_void_p * 3), ("noise", (C.c_void_p * 6) * 2),
+                ("cut_amount", C.c_void_p * 2), ("root_relative", C.c_int), ("surface_tangents3", C.c_void_p),
                 ("noise_domain3", C.c_void_p), ("gradients", C.c_void_p)]
 
 
@@ -444,15 +449,16 @@ def _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes)
 def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitive_ids=None, random_prefixes=None,
                     columns=None, radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None,
                     guide_random_prefixes=None, clumps=None, surface_tangents=None, noise_domain_positions=None,
-                    workers=0):
+                    root_relative=False, workers=0):
     """orc_classic_runtime: the Classic runtime chain of `plan` (a
-    nanoxgen_amd.classic.ClassicPlan: its compiled, bound programs) over
-    packed float4 points [N * C, 4]; each clump module of the plan takes a
-    binding in `clumps` (dicts of strand_guides, guides (dict(axes, normals,
-    tangents[, reference])), guide_uvs and guide_random_prefixes), or, for
-    one module, those four as keywords.  Returns dict(points, point_counts,
-    roots, root_uvs, keep, strand_count, error_bits); strand_count is None
-    where the reference throws (error_bits says why)."""
+    nanoxgen_amd.classic.ClassicPlan: its compiled, bound programs and its
+    effect list) over packed float4 points [N * C, 4]; each clump module of
+    the plan takes a binding in `clumps` (dicts of strand_guides, guides
+    (dict(axes, normals, tangents[, reference])), guide_uvs,
+    guide_random_prefixes and optional local_axes), or, for one module,
+    the first four as keywords.  Returns dict(points, point_counts, roots,
+    root_uvs, keep, strand_count, error_bits); strand_count is None where the
+    reference throws (error_bits says why)."""
     pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
     n = len(pts) // cvs
     rr = np.ascontiguousarray(roots).view(abi.ROOT_DTYPE).reshape(-1)
@@ -461,23 +467,27 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
     ids, pre = _u32(primitive_ids, n), _u32(random_prefixes, n)
     cols = None if columns is None else np.ascontiguousarray(columns, np.float32).reshape(n, -1)
     ncol = 0 if cols is None else cols.shape[1]
-    op = _ClassicPlan(has_cut=1 if plan.has_cut else 0, description_id=plan.description_id,
-                      name_seed=plan.name_seed, clump_count=plan.clump_count,
-                      clumps_after_cut=plan.clumps_after_cut)
+    op = _ClassicPlan(description_id=plan.description_id, name_seed=plan.name_seed,
+                      effect_count=len(plan.effects), root_relative=1 if root_relative else 0)
+    for e, (kind, module) in enumerate(plan.effects):
+        op.effect_type[e] = _EFFECT_TYPES[kind]
+        op.effect_module[e] = module
     for key, prog in plan.programs.items():
-        if key.startswith("noise_"):
-            op.noise[abi.CLASSIC_NOISE_EXPRS.index(key)] = C.addressof(prog.struct)
-        elif key.startswith("clump"):
-            name, _, module = key.partition("@")
-            op.clump[int(module or 0)][abi.CLASSIC_CLUMP_EXPRS.index(name)] = C.addressof(prog.struct)
+        name, _, module = key.partition("@")
+        module = int(module or 0)
+        if name in abi.CLASSIC_NOISE_EXPRS:
+            op.noise[module][abi.CLASSIC_NOISE_EXPRS.index(name)] = C.addressof(prog.struct)
+        elif name in abi.CLASSIC_CLUMP_EXPRS:
+            op.clump[module][abi.CLASSIC_CLUMP_EXPRS.index(name)] = C.addressof(prog.struct)
+        elif name == "cut_amount":
+            op.cut_amount[module] = C.addressof(prog.struct)
         else:
-            setattr(op, key, C.addressof(prog.struct))
+            setattr(op, name, C.addressof(prog.struct))
     keepalive = []
     if plan.has_noise:
         noise_arrays = [_f3(surface_tangents, n), None if noise_domain_positions is None
                         else _f3(noise_domain_positions, n), seexpr_gradients()]
         keepalive.append(noise_arrays)
-        op.has_noise = 1
         op.surface_tangents3, op.noise_domain3, op.gradients = [abi.ptr(x) for x in noise_arrays]
     bindings = _clump_list(clumps, strand_guides, guides, guide_uvs, guide_random_prefixes)
     for m in range(plan.clump_count):
@@ -486,7 +496,12 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
         ax = _f3(gs["axes"], -1)
         g = len(ax) // cvs
         render, lengths = clump_prepare(ax, cvs)
-        arrays = dict(axes3=ax, render3=render, lengths=lengths, normals3=_f3(gs["normals"], g),
+        local = local_render = None
+        if b.get("local_axes") is not None:  # their lengths, whatever the mode (:871-887)
+            local = _f3(b["local_axes"], g * cvs)
+            local_render, lengths = clump_prepare(local, cvs)
+        arrays = dict(axes3=ax, render3=render, lengths=lengths, local3=local, local_render3=local_render,
+                      normals3=_f3(gs["normals"], g),
                       tangents3=_f3(gs["tangents"], g),
                       reference3=None if gs.get("reference") is None else _f3(gs["reference"], g),
                       guide_uvs=np.ascontiguousarray(b["guide_uvs"], np.float32).reshape(g, 2),
@@ -515,7 +530,7 @@ def classic_runtime(points, cvs, plan, *, roots, counts=None, uvs=None, primitiv
 def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None, primitive_ids=None,
                         random_prefixes=None, columns=None, radius_scale=1.0, clump_name="clump", strand_guides=None,
                         guides=None, guide_uvs=None, guide_random_prefixes=None, clumps=None, clump_names=None,
-                        surface_tangents=None, noise_domain_positions=None):
+                        surface_tangents=None, noise_domain_positions=None, root_relative=False):
     """The reference's parse -> compile_xgen_classic_float_runtime_plan ->
     apply_xgen_classic_float_runtime_plan_cpu over the first description of
     `xgen_text`, with one geometry binding per ClumpingFX module (`clumps`
@@ -549,6 +564,8 @@ def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None,
     gp = None if not m else np.concatenate([np.asarray(b["guide_random_prefixes"]).reshape(-1).astype(np.uint32)
                                             for b in bindings])
     sg = None if not m else np.concatenate([_u32(b["strand_guides"], n) for b in bindings])
+    local_flags = np.array([b.get("local_axes") is not None for b in bindings] or [0], np.uint32)
+    gl = cat(lambda b: _f3(b["local_axes"] if b.get("local_axes") is not None else b["guides"]["axes"], -1), 3)
     name_ptrs = (C.c_char_p * max(m, 1))(*[x.encode() for x in names])
     kept = ref().ref_classic_runtime(xgen_text.encode(), pts.ctypes.data, cc.ctypes.data, rr.ctypes.data,
                                      abi.ptr(uu), n, cvs, abi.ptr(ids), abi.ptr(pre), abi.ptr(cols),
@@ -557,6 +574,7 @@ def ref_classic_runtime(xgen_text, points, cvs, *, roots, counts=None, uvs=None,
                                      abi.ptr(gu), abi.ptr(gp), abi.ptr(sg),
                                      abi.ptr(None if surface_tangents is None else _f3(surface_tangents, n)),
                                      abi.ptr(None if noise_domain_positions is None else _f3(noise_domain_positions, n)),
+                                     local_flags.ctypes.data, abi.ptr(gl), 1 if root_relative else 0,
                                      err, 1024, C.byref(seconds))
     if kept == 0xFFFFFFFF:
         return dict(strand_count=None, error=err.value.decode())

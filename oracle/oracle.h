@@ -151,20 +151,31 @@ typedef struct orc_clump_binding {
     const uint32_t *strand_guides;
     uint32_t guide_count;
     const nxg_vec3 *gradients;
+    /* optional translation-free guide CVs and their render axes
+     * (guide_local_axes / guide_local_render_axes); lengths are theirs then */
+    const float *local3, *local_render3;
 } orc_clump_binding;
+#define ORC_CLASSIC_MAX_CLUMPS 3u
+#define ORC_CLASSIC_MAX_NOISES 2u
+#define ORC_CLASSIC_MAX_CUTS 2u
+#define ORC_CLASSIC_MAX_EFFECTS 8u
+#define ORC_EFFECT_CLUMP 0u /* ClassicFloatEffectType order */
+#define ORC_EFFECT_NOISE 1u
+#define ORC_EFFECT_CUT 2u
 typedef struct orc_classic_plan {
-    const nxg_expr_program *length, *width, *cut_amount, *taper, *taper_start, *width_ramp; /* NULL: absent */
-    int has_cut;              /* one CutFX whose amount is cut_amount */
+    const nxg_expr_program *length, *width, *taper, *taper_start, *width_ramp; /* NULL: absent */
     uint32_t description_id;  /* face seed: runtime hash of (id, name seed, face) */
     uint32_t name_seed;
-    uint32_t clump_count;     /* ClumpingFX modules (at most 3), in order, before the CutFX */
-    /* per module: mask, clump, clumpScale, noise, noiseScale, noiseFrequency, noiseCorrelation */
-    const nxg_expr_program *clump[3][7];
-    const orc_clump_binding *binding[3];
-    uint32_t clumps_after_cut; /* the last modules, which follow the CutFX */
-    int has_noise;             /* one NoiseFX after the clumps, before the CutFX */
-    /* mask, magnitude, magnitudeScale, frequency, correlation, preserveLength */
-    const nxg_expr_program *noise[6];
+    /* plan.effects: type (ORC_EFFECT_*) and module index of each, in order */
+    uint32_t effect_count;
+    uint32_t effect_type[ORC_CLASSIC_MAX_EFFECTS], effect_module[ORC_CLASSIC_MAX_EFFECTS];
+    /* per ClumpingFX module: mask, clump, clumpScale, noise, noiseScale, noiseFrequency, noiseCorrelation */
+    const nxg_expr_program *clump[ORC_CLASSIC_MAX_CLUMPS][7];
+    const orc_clump_binding *binding[ORC_CLASSIC_MAX_CLUMPS];
+    /* per NoiseFX module: mask, magnitude, magnitudeScale, frequency, correlation, preserveLength */
+    const nxg_expr_program *noise[ORC_CLASSIC_MAX_NOISES][6];
+    const nxg_expr_program *cut_amount[ORC_CLASSIC_MAX_CUTS]; /* per CutFX module */
+    int root_relative;                              /* clump goals about each strand's root */
     const float *surface_tangents3, *noise_domain3; /* per strand; noise_domain3 optional */
     const nxg_vec3 *gradients;                      /* the SeExpr gradient table (with a NoiseFX) */
 } orc_classic_plan;

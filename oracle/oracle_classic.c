@@ -228,15 +228,18 @@ void orc_classic_endpoints(const float *points, uint32_t strand_count, uint32_t 
 /* ---- Classic primitive expressions -------------------------------------
  * apply_xgen_classic_float_runtime_plan_cpu (src/xgen_classic_runtime.cpp:
  * 1608-1918) for a plan of SplinePrimitive expressions (length, width,
- * taper, taperStart, widthRamp) and at most one CutFX (rebuildType 1) whose
- * amount is an expression.  Per strand, in the reference's order:
+ * taper, taperStart, widthRamp) and an effect list of ClumpingFX, NoiseFX
+ * and CutFX (rebuildType 1) modules.  Per strand, in the reference's order:
  *   fixed point counts                                          :1619-1622
  *   context: id, u, v, face seed, random prefix, input row      :1628-1647
  *   cLength; length expression, scale_curve about the root      :1648-1657
  *   raw width (expression, else 2 radius[0] / radius_scale)     :1658-1665
  *   cull below 1e-4 (the strand is skipped)                     :1666-1676
- *   CutFX: amount, cut_from_tip_and_rebuild, keep = cut >= 1e-4,
- *          cLength of the rebuilt curve                         :1806-1821
+ *   the effects in list order                                   :1679-1826
+ *     ClumpingFX (root-relative goals, local guide axes)        :1680-1793
+ *     NoiseFX (apply_noise), cLength again                      :1794-1805
+ *     CutFX: amount, cut_from_tip_and_rebuild, culled when the
+ *            cut < 1e-4, cLength of the rebuilt curve           :1806-1821
  *   taper, taperStart; per-CV width profile -> radius           :1827-1858
  *   stable compaction                                           :1893-1918
  * Expression inputs bind as input_value does (:504-553): id, cLength,
@@ -298,21 +301,21 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
     if (c_length < 1.0e-4f || raw_width < 1.0e-4f) return 0u;
     const float c_width = raw_width;
     /* the context's t after the effects: a module with a guide sets it to 0
-     * for its mask, and one that clumps leaves it at its last CV's (1); the
-     * cut amount, taper and taperStart then read it */
+     * for its mask, and one that clumps (or a NoiseFX that runs) leaves it
+     * at its last CV's (1); the cut amounts, taper and taperStart read it */
     float t_after = 0.0f;
     uint8_t k = 1u;
-    const uint32_t before = plan->clump_count - plan->clumps_after_cut;
-    for (uint32_t m = 0; m <= plan->clump_count; ++m) { /* :1680-1823, effects in object order */
-        if (m == before && plan->has_noise) { /* the NoiseFX (apply_noise, :897-1039) */
-            const nxg_expr_program *const *e = plan->noise;
+    for (uint32_t e = 0; e < plan->effect_count; ++e) { /* plan.effects in order, :1679-1826 */
+        const uint32_t type = plan->effect_type[e], module = plan->effect_module[e];
+        if (type == ORC_EFFECT_NOISE) { /* apply_noise, :897-1039 */
+            const nxg_expr_program *const *ex = plan->noise[module];
             float raw_mask, f, c, pl, magnitudes[256];
-            if (classic_eval(e[0], &ctx, vars, c_length, c_width, t_after, &raw_mask)) return ORC_CLASSIC_EXPRESSION;
+            if (classic_eval(ex[0], &ctx, vars, c_length, c_width, t_after, &raw_mask)) return ORC_CLASSIC_EXPRESSION;
             const float mask = raw_mask < 0.0f ? 0.0f : (1.0f < raw_mask ? 1.0f : raw_mask);
             if (mask > 1.0e-6f) {
-                if (classic_eval(e[3], &ctx, vars, c_length, c_width, t_after, &f) ||
-                    classic_eval(e[4], &ctx, vars, c_length, c_width, t_after, &c) ||
-                    classic_eval(e[5], &ctx, vars, c_length, c_width, t_after, &pl))
+                if (classic_eval(ex[3], &ctx, vars, c_length, c_width, t_after, &f) ||
+                    classic_eval(ex[4], &ctx, vars, c_length, c_width, t_after, &c) ||
+                    classic_eval(ex[5], &ctx, vars, c_length, c_width, t_after, &pl))
                     return ORC_CLASSIC_EXPRESSION;
                 const float frequency = max_std(f, 0.0f);
                 const float cc = c * 0.01f, pp = pl * 0.01f;
@@ -325,8 +328,8 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
                 for (uint32_t cv = 0u; cv < cvs; ++cv) {
                     float mg, ms;
                     t_after = (float)cv / (float)(cvs - 1u);
-                    if (classic_eval(e[1], &ctx, vars, c_length, c_width, t_after, &mg) ||
-                        classic_eval(e[2], &ctx, vars, c_length, c_width, t_after, &ms))
+                    if (classic_eval(ex[1], &ctx, vars, c_length, c_width, t_after, &mg) ||
+                        classic_eval(ex[2], &ctx, vars, c_length, c_width, t_after, &ms))
                         return ORC_CLASSIC_EXPRESSION;
                     magnitudes[cv] = max_std(mg, 0.0f) * max_std(ms, 0.0f) * mask;
                     if (!isfinite(magnitudes[cv])) return ORC_CLASSIC_NONFINITE_NOISE;
@@ -337,25 +340,26 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
                              plan->surface_tangents3 + 3u * s, magnitudes, plan->gradients);
             }
             c_length = orc_curve_spline_length_v(p, cvs); /* whatever the mask (:1805) */
+            continue;
         }
-        if (m == before && plan->has_cut) { /* the CutFX (:1806-1821) */
+        if (type == ORC_EFFECT_CUT) { /* :1806-1821 */
             float amount;
-            if (classic_eval(plan->cut_amount, &ctx, vars, c_length, c_width, t_after, &amount))
+            if (classic_eval(plan->cut_amount[module], &ctx, vars, c_length, c_width, t_after, &amount))
                 return ORC_CLASSIC_EXPRESSION;
             if (!isfinite(amount)) return ORC_CLASSIC_NONFINITE_AMOUNT;
             const float cut = orc_cut_from_tip_v(p, cvs, max_std(amount, 0.0f), source);
-            if (cut < 1.0e-4f) k = 0u;
+            if (cut < 1.0e-4f) k = 0u; /* culled; the later effects still run */
             c_length = orc_curve_spline_length_v(p, cvs);
+            continue;
         }
-        if (m == plan->clump_count) break;
-        const orc_clump_binding *b = plan->binding[m];
-        const nxg_expr_program *const *e = plan->clump[m];
+        const orc_clump_binding *b = plan->binding[module]; /* ClumpingFX, :1680-1793 */
+        const nxg_expr_program *const *ex = plan->clump[module];
         const uint32_t g = b->strand_guides[s];
         if (g == NXG_INVALID_INDEX) continue;
         if (g >= b->guide_count) return ORC_CLASSIC_INVALID_GUIDE;
         t_after = 0.0f;
         float raw_mask;
-        if (classic_eval(e[0], &ctx, vars, c_length, c_width, 0.0f, &raw_mask)) return ORC_CLASSIC_EXPRESSION;
+        if (classic_eval(ex[0], &ctx, vars, c_length, c_width, 0.0f, &raw_mask)) return ORC_CLASSIC_EXPRESSION;
         const float mask = raw_mask < 0.0f ? 0.0f : (1.0f < raw_mask ? 1.0f : raw_mask);
         if (!(mask > 0.0f)) continue;
         /* build_clump_noise_axis in the guide's context (:1041-1186) */
@@ -366,37 +370,46 @@ static uint32_t classic_strand(const float *in, uint32_t s, uint32_t cvs, const 
         gctx.random_prefix = b->guide_prefixes[g];
         gctx.has_random_prefix = 1;
         float raw_noise, magnitudes[256], noise3[3 * 256];
-        if (classic_eval(e[3], &gctx, vars, c_length, c_width, 0.0f, &raw_noise)) return ORC_CLASSIC_EXPRESSION;
+        if (classic_eval(ex[3], &gctx, vars, c_length, c_width, 0.0f, &raw_noise)) return ORC_CLASSIC_EXPRESSION;
         const float noise = max_std(raw_noise, 0.0f);
         const int noisy = noise > 1.0e-5f && mask > 1.0e-4f;
         float frequency = 0.0f, correlation = 0.0f;
         if (noisy) {
-            if (classic_eval(e[5], &gctx, vars, c_length, c_width, 0.0f, &frequency) ||
-                classic_eval(e[6], &gctx, vars, c_length, c_width, 0.0f, &correlation))
+            if (classic_eval(ex[5], &gctx, vars, c_length, c_width, 0.0f, &frequency) ||
+                classic_eval(ex[6], &gctx, vars, c_length, c_width, 0.0f, &correlation))
                 return ORC_CLASSIC_EXPRESSION;
             for (uint32_t cv = 1u; cv < cvs; ++cv) {
                 float raw_scale;
                 const float t = (float)cv / (float)(cvs - 1u);
-                if (classic_eval(e[4], &gctx, vars, c_length, c_width, t, &raw_scale)) return ORC_CLASSIC_EXPRESSION;
+                if (classic_eval(ex[4], &gctx, vars, c_length, c_width, t, &raw_scale)) return ORC_CLASSIC_EXPRESSION;
                 magnitudes[cv] = mask * noise * max_std(raw_scale, 0.0f);
                 if (!isfinite(magnitudes[cv])) return ORC_CLASSIC_NONFINITE_CLUMP;
             }
         }
-        const float *axes = b->axes3 + (uint64_t)g * cvs * 3u;
-        const float *render = b->render3 + (uint64_t)g * cvs * 3u;
+        /* root-relative evaluation with local guide axes (:1697-1721): the
+         * source and render axes are the local copies, the goal their render
+         * axis moved to the world guide root, about the strand's root */
+        const int local = plan->root_relative && b->local3 != NULL;
+        const float *world = b->axes3 + (uint64_t)g * cvs * 3u;
+        const float *axes = local ? b->local3 + (uint64_t)g * cvs * 3u : world;
+        const float *render = (local ? b->local_render3 : b->render3) + (uint64_t)g * cvs * 3u;
         orc_clump_noise_axis(render, axes, cvs, frequency, correlation,
-                             b->reference3 ? b->reference3 + 3u * g : axes, b->normals3 + 3u * g,
+                             b->reference3 ? b->reference3 + 3u * g : world, b->normals3 + 3u * g,
                              b->tangents3 + 3u * g, noisy ? magnitudes : NULL, b->gradients, noise3);
         orc_cut_to_length(&p[0].x, cvs, b->lengths[g]);
         c_length = orc_curve_spline_length_v(p, cvs);
         float raw_amount;
-        if (classic_eval(e[1], &ctx, vars, c_length, c_width, 0.0f, &raw_amount)) return ORC_CLASSIC_EXPRESSION;
+        if (classic_eval(ex[1], &ctx, vars, c_length, c_width, 0.0f, &raw_amount)) return ORC_CLASSIC_EXPRESSION;
         const float amount = raw_amount < 0.0f ? 0.0f : (1.0f < raw_amount ? 1.0f : raw_amount);
+        const cv3 origin = plan->root_relative ? c3(root->position.x, root->position.y, root->position.z)
+                                               : c3(0.0f, 0.0f, 0.0f);
+        const cv3 guide_root = c3(world[0], world[1], world[2]);
         for (uint32_t cv = 1u; cv < cvs; ++cv) {
             float scale;
             t_after = (float)cv / (float)(cvs - 1u);
-            if (classic_eval(e[2], &ctx, vars, c_length, c_width, t_after, &scale)) return ORC_CLASSIC_EXPRESSION;
-            const cv3 goal = sub3(c3(render[3 * cv], render[3 * cv + 1], render[3 * cv + 2]), c3(0.0f, 0.0f, 0.0f));
+            if (classic_eval(ex[2], &ctx, vars, c_length, c_width, t_after, &scale)) return ORC_CLASSIC_EXPRESSION;
+            const cv3 r = c3(render[3 * cv], render[3 * cv + 1], render[3 * cv + 2]);
+            const cv3 goal = local ? add3(r, sub3(guide_root, origin)) : sub3(r, origin);
             p[cv] = add3(p[cv], mul3(sub3(goal, p[cv]), mask * amount * (1.0f - 2.0f * scale)));
         }
         for (uint32_t cv = 1u; cv < cvs; ++cv)

@@ -515,7 +515,8 @@ uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *cou
                              const uint32_t *guide_counts, const float *guide_axes3, const float *guide_normals3,
                              const float *guide_tangents3, const float *guide_reference3, const float *guide_uvs2,
                              const uint32_t *guide_prefixes, const uint32_t *strand_guides,
-                             const float *surface_tangents3, const float *noise_domain3, char *err,
+                             const float *surface_tangents3, const float *noise_domain3,
+                             const uint32_t *local_flags, const float *guide_local3, int root_relative, char *err,
                              uint64_t errcap, double *seconds) {
     try {
         std::istringstream text(xgen_text);
@@ -573,6 +574,10 @@ uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *cou
             d.guide_random_prefixes.assign(guide_prefixes + first, guide_prefixes + first + guide_count);
             d.strand_guide_indices.assign(strand_guides + std::size_t{m} * strand_count,
                                           strand_guides + std::size_t{m + 1u} * strand_count);
+            if (local_flags && local_flags[m] && guide_local3) {
+                d.guide_local_axes.resize(axis_count);
+                std::memcpy(d.guide_local_axes.data(), guide_local3 + 3u * first * cvs, axis_count * sizeof(Vec3));
+            }
             clumps.push_back(std::move(d));
             first += guide_count;
         }
@@ -587,7 +592,7 @@ uint32_t ref_classic_runtime(const char *xgen_text, float *points, uint32_t *cou
             std::memcpy(domain.data(), noise_domain3, std::size_t{strand_count} * sizeof(Vec3));
         }
         apply_xgen_classic_float_runtime_plan_cpu(curves, plan, radius_scale, tangents, pre, ids, clumps, rows,
-                                                  domain);
+                                                  domain, root_relative != 0);
         if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
         std::memcpy(points, curves.points.data(), curves.points.size() * sizeof(PackedCurvePoint));
         std::memcpy(counts, curves.point_counts.data(), curves.point_counts.size() * sizeof(uint32_t));

@@ -2,10 +2,16 @@
 fixture writer (make_golden.py classic_runtime) and the tests.
 
 Each case is a Classic description: SplinePrimitive attributes (length,
-width, taper, taperStart, widthRamp), ClumpingFX modules (`clump`: one,
-`clumps`: several, applied in order) and an optional CutFX amount, written
-in the authoring form the reference's plan compiler reads (map('path') for
-PTEX inputs, noise($Prefg*f) for reference-pose noise).  `xgen_text` renders
+width, taper, taperStart, widthRamp) and FX modules, written in the
+authoring form the reference's plan compiler reads (map('path') for PTEX
+inputs, noise($Prefg*f) for reference-pose noise).  The modules are either
+an `effects` list in object order (("clump", its seven attributes),
+("noise", its six), ("cut", the amount)) or the short form: ClumpingFX
+modules (`clump`: one, `clumps`: several), an optional `noise` module and
+an optional CutFX amount (exprs["cut_amount"]) after all clumps but the last
+`clumps_after_cut`.  `root_relative` runs the runtime about each strand's
+root (the strands are given relative to it) and `local` lists the clump
+modules whose guides carry translation-free local axes.  `xgen_text` renders
 the description for the reference's parser; `plan_sources` rewrites the
 same expressions into the bound form (map -> $__nxg_map_k in compile order,
 noise($Prefg*f) -> $__nxg_pref_noise_j per distinct frequency), which is what
@@ -136,14 +142,85 @@ CASES = [
     dict(name="noiseMagnitude", description_id=43, strands="walk", exprs=dict(),
          noise=dict(mask="1", magnitude="1e30", magnitudeScale="1e30", frequency="1", correlation="0",
                     preserveLength="0")),
+    # effect lists in object order: a CutFX before a NoiseFX (the noise runs
+    # on the rebuilt strand; a culled strand still runs it), the NoiseFX's
+    # mask and frequency read the t the cut leaves (0)
+    dict(name="ns:cutNoise", description_id=51, strands="walk", ids=True, prefixes=True,
+         exprs=dict(taper="rampUI(0,0.1,1:1,0.6,1)", width_ramp="1 - 0.2 * $cLength"),
+         effects=[("cut", "$cLength * (0.15 + 0.2 * rand(4))"),
+                  ("noise", dict(mask="0.5 + 0.5 * map('n.ptx')", magnitude="0.08 + 0.04 * rand()",
+                                 magnitudeScale="rampUI(0,0,1:1,1,1)", frequency="2 + $cLength",
+                                 correlation="10", preserveLength="30"))]),
+    # two NoiseFX modules around a clump, then a cut whose amount reads the t
+    # the second NoiseFX leaves; the second one masks a third of the strands out
+    dict(name="twoNoises", description_id=52, strands="ico5", ids=True, guides=9, noise_domain=True,
+         exprs=dict(width_ramp="1 - 0.3 * $cLength"),
+         effects=[("noise", dict(mask="1", magnitude="0.03", magnitudeScale="1", frequency="1.5", correlation="0",
+                                 preserveLength="0")),
+                  ("clump", dict(mask="0.7", clump="0.4", clumpScale="0", noise="0", noiseScale="1",
+                                 noiseFrequency="1", noiseCorrelation="0")),
+                  ("noise", dict(mask="$id < 100 ? 0 : 1", magnitude="0.02 * $cLength",
+                                 magnitudeScale="rampUI(0,0.5,1:1,1,1)", frequency="3", correlation="40",
+                                 preserveLength="60")),
+                  ("cut", "$cLength * 0.1 + rampUI(0,0,1:1,0.02,1)")]),
+    # two CutFX modules around a clump: the first rebuilds the strand for the
+    # clump (a strand it culls still runs the rest), the second reads cLength
+    # and the t the clump leaves
+    dict(name="ns:twoCuts", description_id=53, strands="walk", ids=True, prefixes=True, guides=10,
+         exprs=dict(taper="rampUI(0,0.2,1:1,0.5,1)"),
+         effects=[("cut", "$cLength * 0.2 * map('c.ptx')"),
+                  ("clump", dict(mask="1", clump="0.5 + 0.3 * rand()", clumpScale="rampUI(0,0,1:1,0.2,1)",
+                                 noise="0", noiseScale="1", noiseFrequency="1", noiseCorrelation="0")),
+                  ("cut", "$cLength * 0.1 + rampUI(0,0,1:1,0.05,1)")]),
+    # root-relative evaluation: strands given about their roots, a module
+    # with local (translation-free) guide axes and one without, then a cut
+    dict(name="rootRelative", description_id=54, strands="walk", ids=True, prefixes=True, guides=[8, 6],
+         root_relative=True, local=[0], exprs=dict(taper="0.4"),
+         effects=[("clump", dict(mask="1", clump="0.6", clumpScale="rampUI(0,0,1:1,0.3,1)", noise="0",
+                                 noiseScale="1", noiseFrequency="1", noiseCorrelation="0")),
+                  ("clump", dict(mask="$id < 120 ? 0 : 0.8", clump="0.3 + 0.2 * rand()", clumpScale="0",
+                                 noise="0", noiseScale="1", noiseFrequency="1", noiseCorrelation="0")),
+                  ("cut", "$cLength * 0.15")]),
+    # root-relative with clump noise: the noise frame of the local render
+    # axis, its domain at the guide's world reference position; local axes
+    # given but not root-relative set only the guides' lengths
+    dict(name="rootRelativeNoise", description_id=55, strands="ico5", ids=True, guides=[7, 5], root_relative=True,
+         local=[0, 1], exprs=dict(width_ramp="1 - 0.2 * $cLength"),
+         effects=[("clump", dict(mask="0.9", clump="0.5", clumpScale="0.05", noise="0.2", noiseScale="1",
+                                 noiseFrequency="2", noiseCorrelation="20")),
+                  ("clump", dict(mask="1", clump="0.3", clumpScale="0", noise="0.1",
+                                 noiseScale="rampUI(0,0,1:1,1,1)", noiseFrequency="3", noiseCorrelation="0"))]),
+    dict(name="localLengths", description_id=56, strands="walk", guides=6, local=[0],
+         exprs=dict(cut_amount="$cLength * 0.2"),
+         clump=dict(mask="1", clump="0.5", clumpScale="0", noise="0", noiseScale="1", noiseFrequency="1",
+                    noiseCorrelation="0")),
 ]
+
+
+def case_effects(case):
+    """[(kind, module)] in object order: kind "clump" (attribute dict),
+    "noise" (attribute dict) or "cut" (the amount source)."""
+    if "effects" in case:
+        return list(case["effects"])
+    clumps = [case["clump"]] if "clump" in case else case.get("clumps", [])
+    before = len(clumps) - case.get("clumps_after_cut", 0)
+    effects = [("clump", c) for c in clumps[:before]]
+    if "noise" in case:
+        effects.append(("noise", case["noise"]))
+    if "cut_amount" in case["exprs"]:
+        effects.append(("cut", case["exprs"]["cut_amount"]))
+    return effects + [("clump", c) for c in clumps[before:]]
+
+
+def has_noise(case):
+    """Whether the case has a NoiseFX or a clump with its noise on (positions
+    then go through the device's acosf, cosf and sinf)."""
+    return any(k == "noise" or (k == "clump" and m["noise"] != "0") for k, m in case_effects(case))
 
 
 def case_clumps(case):
     """The case's ClumpingFX modules (attribute dicts), in object order."""
-    if "clumps" in case:
-        return case["clumps"]
-    return [case["clump"]] if "clump" in case else []
+    return [m for k, m in case_effects(case) if k == "clump"]
 
 
 def clump_names(case):
@@ -151,15 +228,29 @@ def clump_names(case):
     return ["clump" if m == 0 else f"clump{m}" for m in range(len(case_clumps(case)))]
 
 
+def indexed_effects(case):
+    """[(kind, index among the modules of its kind, module)] in object order."""
+    seen = dict(clump=0, noise=0, cut=0)
+    out = []
+    for kind, module in case_effects(case):
+        out.append((kind, seen[kind], module))
+        seen[kind] += 1
+    return out
+
+
 def compile_order(case):
-    """(key, attribute) in the reference's compile order; clump keys are
-    "clump<m>.<attribute>"; the CutFX sits before the last
-    case["clumps_after_cut"] modules."""
-    count = len(case_clumps(case))
-    before = count - case.get("clumps_after_cut", 0)
-    clump = lambda ms: tuple((f"clump{m}.{a}", a) for m in ms for a in CLUMP)
-    noise = tuple((f"noise.{a}", a) for a in NOISE) if "noise" in case else ()
-    return PRIMITIVE + clump(range(before)) + noise + (("cut_amount", "amount"),) + clump(range(before, count))
+    """(key, attribute) in the reference's compile order (the primitive's
+    attributes, then the FX modules in object order); module keys are
+    "clump<m>.<attribute>", "noise<n>.<attribute>" and "cut<c>"."""
+    order = PRIMITIVE
+    for kind, index, _ in indexed_effects(case):
+        if kind == "clump":
+            order += tuple((f"clump{index}.{a}", a) for a in CLUMP)
+        elif kind == "noise":
+            order += tuple((f"noise{index}.{a}", a) for a in NOISE)
+        else:
+            order += ((f"cut{index}", "amount"),)
+    return order
 
 
 def xgen_text(case):
@@ -172,18 +263,17 @@ def xgen_text(case):
         if key in e:
             s += f"\t{attribute}\t{e[key]}\n"
     s += "\tendAttrs\n"
-    modules = list(zip(case_clumps(case), clump_names(case)))
-    before = len(modules) - case.get("clumps_after_cut", 0)
-    for i, module in enumerate(modules + [None]):
-        if i == before and "noise" in case:
-            s += "NoiseFXModule\n\tactive\ttrue\n\tname\tnoise\n\tmode\t0\n"
-            s += "".join(f"\t{a}\t{case['noise'][a]}\n" for a in NOISE) + "\tendAttrs\n"
-        if i == before and "cut_amount" in e:
-            s += f"CutFXModule\n\tactive\ttrue\n\tname\tcut\n\tamount\t{e['cut_amount']}\n\trebuildType\t1\n\tendAttrs\n"
-        if module is None:
-            break
-        s += f"ClumpingFXModule\n\tactive\ttrue\n\tname\t{module[1]}\n"
-        s += "".join(f"\t{a}\t{module[0][a]}\n" for a in CLUMP) + "\tendAttrs\n"
+    names = clump_names(case)
+    for kind, index, module in indexed_effects(case):
+        if kind == "noise":
+            s += f"NoiseFXModule\n\tactive\ttrue\n\tname\t{'noise' if index == 0 else f'noise{index}'}\n\tmode\t0\n"
+            s += "".join(f"\t{a}\t{module[a]}\n" for a in NOISE) + "\tendAttrs\n"
+        elif kind == "cut":
+            s += (f"CutFXModule\n\tactive\ttrue\n\tname\t{'cut' if index == 0 else f'cut{index}'}\n"
+                  f"\tamount\t{module}\n\trebuildType\t1\n\tendAttrs\n")
+        else:
+            s += f"ClumpingFXModule\n\tactive\ttrue\n\tname\t{names[index]}\n"
+            s += "".join(f"\t{a}\t{module[a]}\n" for a in CLUMP) + "\tendAttrs\n"
     return s + "\tActive\tSplinePrimitive\n"
 
 
@@ -192,17 +282,21 @@ _PREF = re.compile(r"\bnoise\s*\(\s*\$Prefg\s*\*\s*([-+]?[0-9.]+)\s*\)")
 
 
 def plan_sources(case):
-    """(sources in the bound form, ptex count, pref-noise frequencies); clump
-    expressions under the "clump" key by attribute name (a list of such
-    dicts for several modules)."""
+    """(ClassicPlan keyword arguments with the sources in the bound form, ptex
+    count, pref-noise frequencies): an `effects` list for a case given as one,
+    else the short form (clump expressions under "clump" by attribute name,
+    a list of such dicts for several modules; "noise"; "cut_amount")."""
     out, ptex, freqs = {}, 0, []
-    modules = case_clumps(case)
+    modules = {(kind, index): module for kind, index, module in indexed_effects(case)}
     for key, _ in compile_order(case):
-        if key.startswith("noise."):
-            src = case["noise"][key[6:]]
+        if key.startswith("noise"):
+            n, attribute = key[5:].split(".")
+            src = modules[("noise", int(n))][attribute]
         elif key.startswith("clump"):
             m, attribute = key[5:].split(".")
-            src = modules[int(m)][attribute]
+            src = modules[("clump", int(m))][attribute]
+        elif key.startswith("cut"):
+            src = modules[("cut", int(key[3:]))]
         elif key not in case["exprs"]:
             continue
         else:
@@ -223,13 +317,27 @@ def plan_sources(case):
             return f"$__nxg_pref_noise_{freqs.index(f)}"
 
         out[key] = _PREF.sub(pref_sub, out[key])
-    clumps = [{a: out.pop(f"clump{m}.{a}") for a in CLUMP} for m in range(len(modules))]
+    bound = []
+    for kind, index, _ in indexed_effects(case):
+        if kind == "clump":
+            bound.append((kind, {a: out.pop(f"clump{index}.{a}") for a in CLUMP}))
+        elif kind == "noise":
+            bound.append((kind, {a: out.pop(f"noise{index}.{a}") for a in NOISE}))
+        else:
+            bound.append((kind, out.pop(f"cut{index}")))
+    if "effects" in case:
+        out["effects"] = bound
+        return out, ptex, freqs
+    clumps = [m for k, m in bound if k == "clump"]
     if clumps:
         out["clump"] = clumps if "clumps" in case else clumps[0]
     if case.get("clumps_after_cut"):
         out["clumps_after_cut"] = case["clumps_after_cut"]
-    if "noise" in case:
-        out["noise"] = {a: out.pop(f"noise.{a}") for a in NOISE}
+    for kind, m in bound:
+        if kind == "noise":
+            out["noise"] = m
+        elif kind == "cut":
+            out["cut_amount"] = m
     return out, ptex, freqs
 
 
@@ -262,6 +370,9 @@ def case_inputs(case, strands_of):
     else:
         pts, cvs, roots, uvs = strands_of(case["strands"])
     n = len(pts) // cvs
+    if case.get("root_relative"):  # the strands about their roots
+        pts = pts.copy()
+        pts.reshape(n, cvs, 4)[:, :, :3] -= np.asarray(roots["position"], np.float32)[:, None, :]
     rng = np.random.default_rng(zlib.crc32(case["name"].encode()))
     _, ptex, freqs = plan_sources(case)
     k = ptex + len(freqs)
@@ -270,6 +381,8 @@ def case_inputs(case, strands_of):
     prefixes = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32) if case.get("prefixes") else None
     out = dict(points=pts, cvs=cvs, roots=roots, uvs=uvs, ids=ids, prefixes=prefixes, columns=columns,
                radius_scale=case.get("radius_scale", 1.0))
+    if case.get("root_relative"):
+        out["root_relative"] = np.uint32(1)
     counts = case.get("guides")
     for m, count in enumerate([] if counts is None else counts if isinstance(counts, list) else [counts]):
         g = clump_guides(count, cvs, rng)
@@ -282,7 +395,11 @@ def case_inputs(case, strands_of):
             guide_axes=g["axes"], guide_normals=g["normals"], guide_tangents=g["tangents"],
             guide_reference=g["reference"], guide_uvs=g["uvs"], guide_prefixes=g["prefixes"],
             strand_guides=strand_guides).items()})
-    if "noise" in case:  # surface tangents (every 9th zero: the fallback frame), domain positions
+        if m in case.get("local", ()):  # translation-free copies: each guide about its first CV
+            axes = g["axes"].reshape(count, cvs, 3)
+            out[f"guide_local{suffix}"] = (axes - axes[:, :1]).reshape(-1, 3).astype(np.float32)
+    if any(k == "noise" for k, _ in case_effects(case)):
+        # surface tangents (every 9th zero: the fallback frame), domain positions
         tangents = rng.normal(0, 1, (n, 3)).astype(np.float32)
         tangents[::9] = 0.0
         out["surface_tangents"] = tangents
@@ -296,13 +413,42 @@ def guide_args(inp):
     module (empty without guides)."""
     clumps = []
     while ("guide_axes" + ("" if not clumps else f"_{len(clumps)}")) in inp:
-        x = lambda k: inp[k + ("" if not clumps else f"_{len(clumps)}")]
+        suffix = "" if not clumps else f"_{len(clumps)}"
+        x = lambda k: inp[k + suffix]
         clumps.append(dict(strand_guides=x("strand_guides"), guide_uvs=x("guide_uvs"),
-                           guide_random_prefixes=x("guide_prefixes"),
+                           guide_random_prefixes=x("guide_prefixes"), local_axes=inp.get("guide_local" + suffix),
                            guides=dict(axes=x("guide_axes"), normals=x("guide_normals"),
                                        tangents=x("guide_tangents"), reference=x("guide_reference"))))
     out = dict(clumps=clumps) if clumps else {}
     if "surface_tangents" in inp:
         out["surface_tangents"] = inp["surface_tangents"]
         out["noise_domain_positions"] = inp.get("noise_domain")
+    if inp.get("root_relative") is not None and int(inp["root_relative"]):
+        out["root_relative"] = True
     return out
+
+
+EFFECT_MODULES = dict(
+    clump=[dict(mask="0.3 + 0.7 * map('m')", clump="0.4 + 0.4 * rand()", clumpScale="rampUI(0,0,1:1,0.2,1)",
+                noise="0.1 * rand(5)", noiseScale="1", noiseFrequency="2", noiseCorrelation="10"),
+           dict(mask="$id < 50 ? 0 : 1", clump="clamp($cLength * 0.3, 0, 1)", clumpScale="0", noise="0",
+                noiseScale="1", noiseFrequency="1", noiseCorrelation="0")],
+    noise=[dict(mask="0.5 + 0.5 * map('n')", magnitude="0.05 + 0.05 * rand()", magnitudeScale="rampUI(0,0,1:1,1,1)",
+                frequency="2", correlation="20", preserveLength="40"),
+           dict(mask="$id < 80 ? 0 : 1", magnitude="0.02 * $cLength", magnitudeScale="1", frequency="1 + $cLength",
+                correlation="0", preserveLength="0")],
+    cut=["$cLength * (0.1 + 0.3 * map('c'))", "$cLength * 0.1 + rampUI(0,0,1:1,0.05,1)"])
+
+
+def random_effect_cases(rng, count):
+    """Seeded effect lists of one to six modules in random object order (at
+    most two of each kind), with random primitive expressions."""
+    cases = []
+    for k in range(count):
+        pool = [(kind, m) for kind in ("clump", "noise", "cut") for m in range(2)]
+        picks = [pool[j] for j in rng.permutation(len(pool))[:rng.integers(1, 7)]]
+        effects = [(kind, EFFECT_MODULES[kind][m]) for kind, m in picks]
+        exprs = dict(taper="rampUI(0,0.1,1:1,0.6,1)") if rng.uniform() < 0.5 else dict(width_ramp="1 - 0.2 * $cLength")
+        cases.append(dict(name=f"ns:order{k}", description_id=60 + k, strands="walk", ids=True, prefixes=True,
+                          exprs=exprs, effects=effects))
+    return cases

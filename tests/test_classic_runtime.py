@@ -17,7 +17,7 @@ from nanoxgen_amd import abi, classic
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
-from classic_runtime_cases import CASES, guide_args, plan_sources  # noqa: E402
+from classic_runtime_cases import CASES, guide_args, plan_sources, random_effect_cases  # noqa: E402
 
 FIXTURE = os.path.join(HERE, "golden", "reference_classic_runtime.npz")
 
@@ -136,6 +136,104 @@ def test_oracle_matches_reference_core(orc):
         got = orc.classic_runtime(pts, cvs, plan, roots=roots, primitive_ids=ids, columns=cols)
         assert got["strand_count"] == want["strand_count"], exprs
         assert np.array_equal(bits(got["points"]), bits(want["points"])), exprs
+
+
+def test_effect_lists_match_reference_core(orc):
+    """Random effect orders (ClumpingFX, NoiseFX and CutFX modules, up to two
+    of each, any order) through the reference's parser, plan compiler and
+    runtime, on seeded walks with root-relative evaluation and local guide
+    axes in half of them: the oracle bit-exact."""
+    if orc.ref() is None:
+        pytest.skip("reference core not built here")
+    from classic_runtime_cases import case_clumps, clump_guides, clump_names, xgen_text
+    rng = np.random.default_rng(77)
+    n, cvs = 160, 8
+    steps = rng.normal(0, 0.25, size=(n, cvs, 3)).astype(np.float32)
+    steps[:, 0] = rng.uniform(-3, 3, size=(n, 3))
+    walk = np.cumsum(steps, 1)
+    pts = np.concatenate([walk, rng.uniform(0.001, 0.02, (n, cvs, 1))], 2).astype(np.float32).reshape(-1, 4)
+    roots = np.zeros(n, abi.ROOT_DTYPE)
+    roots["uv"] = rng.uniform(0, 1, (n, 2))
+    roots["position"] = walk[:, 0]
+    tangents = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    ids = rng.permutation(n).astype(np.uint32)
+    prefixes = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    for k, case in enumerate(random_effect_cases(rng, 12)):
+        relative = k % 2 == 1
+        sources, ptex, freqs = plan_sources(case)
+        cols = rng.uniform(0.05, 1.0, (n, ptex + len(freqs))).astype(np.float32) if ptex + len(freqs) else None
+        clumps = []
+        for m in range(len(case_clumps(case))):
+            g = clump_guides(9, cvs, rng)
+            sg = rng.integers(0, 9, n).astype(np.uint32)
+            sg[m::5] = 0xFFFFFFFF
+            axes = g["axes"].reshape(9, cvs, 3)
+            clumps.append(dict(strand_guides=sg, guide_uvs=g["uvs"], guide_random_prefixes=g["prefixes"],
+                               local_axes=(axes - axes[:, :1]).reshape(-1, 3) if relative else None,
+                               guides=dict(axes=g["axes"], normals=g["normals"], tangents=g["tangents"],
+                                           reference=g["reference"])))
+        p = pts.copy()
+        if relative:
+            p.reshape(n, cvs, 4)[:, :, :3] -= walk[:, :1].astype(np.float32)
+        kw = dict(roots=roots, primitive_ids=ids, random_prefixes=prefixes, columns=cols, clumps=clumps,
+                  surface_tangents=tangents, root_relative=relative)
+        want = orc.ref_classic_runtime(xgen_text(case), p, cvs, clump_names=clump_names(case), **kw)
+        plan = classic.ClassicPlan(**sources, description_id=case["description_id"], description_name=case["name"],
+                                   ptex_count=ptex)
+        got = orc.classic_runtime(p, cvs, plan, **kw)
+        label = [e for e, _ in case["effects"]]
+        assert want["strand_count"] is not None, (label, want.get("error"))
+        assert got["strand_count"] == want["strand_count"], label
+        assert np.array_equal(bits(got["points"]), bits(want["points"])), label
+        assert np.array_equal(bits(got["roots"]), bits(want["roots"])), label
+
+
+def test_effect_list_plan_report():
+    """The packer expands the short form into the same effect list, and
+    reports the trailing cut and whether the CVs are rewritten."""
+    one = dict(mask="1", clump="0.5", clumpScale="0", noise="0", noiseScale="1", noiseFrequency="1",
+               noiseCorrelation="0")
+    noise = dict(mask="1", magnitude="0.1", magnitudeScale="1", frequency="1", correlation="0", preserveLength="0")
+    short = classic.ClassicPlan(clump=[one, one], clumps_after_cut=1, cut_amount="0.1")
+    listed = classic.ClassicPlan(effects=[("clump", one), ("cut", "0.1"), ("clump", one)])
+    for plan in (short, listed):
+        info = plan.info
+        assert [(e.type, e.module) for e in info.effects[:info.effect_count]] == [(0, 0), (2, 0), (0, 1)]
+        assert info.trailing_cut == 0 and info.rewrites == 1 and info.cut_count == 1
+    only_cut = classic.ClassicPlan(cut_amount="0.1").info
+    assert only_cut.trailing_cut == 1 and only_cut.rewrites == 0 and only_cut.effect_count == 1
+    two = classic.ClassicPlan(effects=[("noise", noise), ("cut", "0.1"), ("noise", noise), ("cut", "0.2")]).info
+    assert (two.noise_count, two.cut_count, two.trailing_cut, two.rewrites) == (2, 2, 1, 1)
+    # module 1's programs sit in their own slots
+    for key in ("noise_mask@1", "noise_preserve_length@1", "cut_amount@1"):
+        assert (two.present >> abi.CLASSIC_EXPRS.index(key)) & 1
+
+
+def test_effect_list_pack_rejects():
+    L = nx.lib()
+    info = abi.ClassicPlanInfo()
+    amount = nx.expr.compile_expression("0.1", "amount", "CutFXModule")
+
+    def pack(**kw):
+        effects = kw.pop("effects", [])
+        plan = abi.ClassicPlan(effect_count=len(effects), **kw)
+        for e, (t, m) in enumerate(effects):
+            plan.effects[e] = abi.ClassicEffect(t, m)
+        plan.expressions[abi.CLASSIC_EXPRS.index("cut_amount")] = C.addressof(amount.struct)
+        plan.expressions[abi.CLASSIC_EXPRS.index("cut_amount@1")] = C.addressof(amount.struct)
+        return L.nxg_classic_plan_pack(C.byref(plan), None, 0, C.byref(info))
+
+    cut = abi.CLASSIC_EFFECT_CUT
+    assert pack(cut_count=2, effects=[(cut, 1), (cut, 0)]) == 0
+    assert pack(cut_count=2, effects=[(cut, 0), (cut, 0)]) == abi.INVALID_PROGRAM  # a module twice
+    assert pack(cut_count=2, effects=[(cut, 0)]) == abi.INVALID_PROGRAM            # a module missing
+    assert pack(cut_count=1, effects=[(cut, 1)]) == abi.INVALID_PROGRAM            # past the modules
+    assert pack(cut_count=1, effects=[(3, 0)]) == abi.INVALID_PROGRAM              # an unknown type
+    assert pack(cut_count=3, effects=[(cut, 0)]) == abi.INVALID_PROGRAM            # too many modules
+    assert pack(cut_count=1, has_cut=1, effects=[(cut, 0)]) == abi.INVALID_PROGRAM  # the forms mixed
+    assert pack(noise_count=1, has_cut=1) == abi.INVALID_PROGRAM                     # short form with counts
+    # a NoiseFX needs all six programs
+    assert pack(noise_count=1, effects=[(abi.CLASSIC_EFFECT_NOISE, 0)]) == abi.INVALID_PROGRAM
 
 
 # ---- host contracts --------------------------------------------------------

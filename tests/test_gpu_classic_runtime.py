@@ -15,7 +15,8 @@ from nanoxgen_amd import abi, classic, workloads
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
-from classic_runtime_cases import CASES, case_clumps, clump_guides, guide_args, plan_sources  # noqa: E402
+from classic_runtime_cases import (CASES, case_clumps, clump_guides, clump_names, guide_args, has_noise,  # noqa: E402
+                                   plan_sources, random_effect_cases)
 
 pytestmark = pytest.mark.gpu
 
@@ -67,12 +68,13 @@ def make_plan(case):
 
 def run_gpu(torch, plan, pts, cvs, *, roots, uvs=None, counts=None, ids=None, prefixes=None, columns=None,
             radius_scale=1.0, strand_guides=None, guides=None, guide_uvs=None, guide_random_prefixes=None,
-            clumps=None, surface_tangents=None, noise_domain_positions=None):
+            clumps=None, surface_tangents=None, noise_domain_positions=None, root_relative=False):
     if clumps is None and guides is not None:
         clumps = [dict(strand_guides=strand_guides, guides=guides, guide_uvs=guide_uvs,
                        guide_random_prefixes=guide_random_prefixes)]
     bindings = [dict(strand_guides=dev(torch, b["strand_guides"], np.int32), guide_uvs=dev(torch, b["guide_uvs"]),
                      guide_random_prefixes=dev(torch, b["guide_random_prefixes"], np.int32),
+                     local_axes=dev(torch, b.get("local_axes")),
                      guides={k: dev(torch, v) for k, v in b["guides"].items() if v is not None})
                 for b in clumps or []]
     out = classic.classic_runtime(dev(torch, np.asarray(pts, np.float32).reshape(-1, 4)), cvs, plan,
@@ -81,7 +83,8 @@ def run_gpu(torch, plan, pts, cvs, *, roots, uvs=None, counts=None, ids=None, pr
                                   primitive_ids=dev(torch, ids, np.int32), random_prefixes=dev(torch, prefixes, np.int32),
                                   columns=dev(torch, columns), radius_scale=radius_scale, clumps=bindings,
                                   surface_tangents=dev(torch, surface_tangents),
-                                  noise_domain_positions=dev(torch, noise_domain_positions))
+                                  noise_domain_positions=dev(torch, noise_domain_positions),
+                                  root_relative=root_relative)
     return out.to_host()
 
 
@@ -118,8 +121,7 @@ def test_reference_fixture(torch, fixture, i):
         return
     want = dict(strand_count=len(fixture[f"case{i}_out_point_counts"]), points=fixture[f"case{i}_out_points"],
                 point_counts=fixture[f"case{i}_out_point_counts"], roots=fixture[f"case{i}_out_roots"])
-    same(got, want, CASES[i]["name"],
-         noisy_clump="noise" in CASES[i] or any(c["noise"] != "0" for c in case_clumps(CASES[i])))
+    same(got, want, CASES[i]["name"], noisy_clump=has_noise(CASES[i]))
 
 
 PLANS = [
@@ -319,3 +321,51 @@ def test_variable_counts_and_reuse(torch, orc):
     got = outputs.to_host()
     want = orc.classic_runtime(pts, cvs, plan, roots=roots, counts=counts)
     same(got, want, "after an error")
+
+
+@pytest.mark.parametrize("cvs", [3, 8, 24])
+def test_effect_list_walks_oracle(torch, orc, cvs):
+    """Random effect orders (ClumpingFX, NoiseFX and CutFX modules, up to two
+    of each, in any order) on seeded walks, root-relative with local guide
+    axes in half of them; GPU == oracle (bit-exact without noise, within
+    1e-5 of the diagonal with it)."""
+    rng = np.random.default_rng(300 + cvs)
+    n = 64 * 2 + 23
+    steps = rng.normal(0, 0.25, size=(n, cvs, 3)).astype(np.float32)
+    steps[:, 0] = rng.uniform(-3, 3, size=(n, 3))
+    walk = np.cumsum(steps, 1)
+    pts = np.concatenate([walk, rng.uniform(0.001, 0.02, (n, cvs, 1))], 2).astype(np.float32).reshape(-1, 4)
+    roots = np.zeros(n, abi.ROOT_DTYPE)
+    roots["uv"] = rng.uniform(0, 1, (n, 2))
+    roots["position"] = walk[:, 0]
+    tangents = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    ids = rng.permutation(n).astype(np.uint32)
+    prefixes = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    for k, case in enumerate(random_effect_cases(rng, 8)):
+        relative = k % 2 == 1
+        sources, ptex, freqs = plan_sources(case)
+        cols = rng.uniform(0.05, 1.0, (n, ptex + len(freqs))).astype(np.float32) if ptex + len(freqs) else None
+        clumps = []
+        for m in range(len(case_clumps(case))):
+            g = clump_guides(9, cvs, rng)
+            sg = rng.integers(0, 9, n).astype(np.uint32)
+            sg[m::5] = 0xFFFFFFFF
+            axes = g["axes"].reshape(9, cvs, 3)
+            clumps.append(dict(strand_guides=sg, guide_uvs=g["uvs"], guide_random_prefixes=g["prefixes"],
+                               local_axes=(axes - axes[:, :1]).reshape(-1, 3) if relative else None,
+                               guides=dict(axes=g["axes"], normals=g["normals"], tangents=g["tangents"],
+                                           reference=g["reference"])))
+        p = pts.copy()
+        if relative:
+            p.reshape(n, cvs, 4)[:, :, :3] -= walk[:, :1].astype(np.float32)
+        plan = classic.ClassicPlan(**sources, description_id=case["description_id"], description_name=case["name"],
+                                   ptex_count=ptex)
+        kw = dict(primitive_ids=ids, random_prefixes=prefixes, columns=cols, clumps=clumps,
+                  surface_tangents=tangents, root_relative=relative)
+        label = f"{cvs} CVs, {[e for e, _ in case['effects']]}, root_relative={relative}"
+        print("case", label, flush=True)
+        want = orc.classic_runtime(p, cvs, plan, roots=roots, **kw)
+        assert want["strand_count"] is not None, (label, want["error_bits"])
+        got = run_gpu(torch, plan, p, cvs, roots=roots, ids=ids, prefixes=prefixes, columns=cols, clumps=clumps,
+                      surface_tangents=tangents, root_relative=relative)
+        same(got, want, label, noisy_clump=has_noise(case))
