@@ -251,6 +251,7 @@ template <class Curve>
 __device__ __forceinline__ float spline_length(const Sample *samples, const Curve &p, uint32_t n) {
     F3 previous = xyz(p[0]);
     float result = 0.0f;
+#pragma unroll 4
     for (uint32_t k = 0u; k < 2u * n + 3u; ++k) {
         const F3 current = blend(p, n, load_sample(samples, k));
         result += sqrtf(length_squared(current - previous));
@@ -270,6 +271,7 @@ __device__ __forceinline__ float cut_parameter(const Sample *walk, uint32_t walk
     if (!(amount >= 1.0e-10f)) return 1.0f;
     F3 previous = xyz(p[n - 1u]);
     float accumulated = 0.0f;
+#pragma unroll 4
     for (uint32_t j = 0u; j < walk_count; ++j) {
         const Sample s = load_sample(walk, j);
         const F3 current = blend(p, n, s);
@@ -721,6 +723,7 @@ __device__ __forceinline__ float cut_to_length(const Sample *samples_table, cons
     float previous_t = 0.0f, accumulated = 0.0f;
     F3 previous = xyz(p[0]);
     const uint32_t samples = 2u * n + 3u;
+#pragma unroll 4
     for (uint32_t k = 0u; k <= samples; ++k) {
         float t;
         F3 current;
@@ -997,9 +1000,12 @@ __device__ __forceinline__ uint32_t uniform_slot(uint32_t k, uint32_t cv, bool a
 __device__ __forceinline__ bool classic_value(const CutArgs &a, uint32_t k, const ClassicContext &c, float c_length,
                                               float c_width, uint32_t cv, float &out, bool at_one = false) {
     if (k < 32u && ((a.uniform >> k) & 1u)) {
-        const float2 v = a.uniform_values[uniform_slot(k, cv, at_one)];
-        out = v.x;
-        return v.y != 0.0f && isfinite(c.expr.u) && isfinite(c.expr.v) && isfinite(c.expr.face_seed);
+        // the slot is wave-uniform and the table constant during the call: a scalar load
+        ImageOf(float) table = (ImageOf(float))a.uniform_values;
+        const uint32_t slot = uniform_slot(k, cv, at_one);
+        out = table[2u * slot];
+        return table[2u * slot + 1u] != 0.0f && isfinite(c.expr.u) && isfinite(c.expr.v) &&
+               isfinite(c.expr.face_seed);
     }
     const float t = cv_table(k) >= 0 ? cv_parameter(cv, a.cvs) : (at_one ? 1.0f : 0.0f);
     return classic_eval(a, k, c, c_length, c_width, t, out);
@@ -1144,15 +1150,15 @@ __device__ __forceinline__ float taper_scale(float t, float taper, float taper_s
 }
 
 // ClumpingFX module m on one strand (src/xgen_classic_runtime.cpp:1680-1793)
-// over its staged CVs.  A strand the module clumps goes to a.clumped and
-// back into its LDS row (the next module, the cut and the write read it from
-// there); c_length follows the reference's context.  at_one tracks the
+// over its staged CVs, rewritten in the lane's LDS row (the later effects
+// read it there; the search copies every row to a.clumped at the end);
+// c_length follows the reference's context.  at_one tracks the
 // context's t: a module with a valid guide sets it to 0 for its mask and,
 // when it clumps, leaves the t of its last CV, 1.  False (with *err) where
 // the reference throws.
 __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, const ClassicContext &c,
                                               const StagedCurve<true> &p, uint32_t s, float &c_length,
-                                              float c_width, bool &at_one, bool &clumped, uint32_t *err) {
+                                              float c_width, bool &at_one, uint32_t *err) {
     const uint32_t n = a.cvs;
     const ClumpModule &cm = a.clumps[m];
     const uint32_t g = cm.strand_guides[s];
@@ -1212,7 +1218,6 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
         return false;
     }
     const float amount = clamp_std(raw_amount, 0.0f, 1.0f);
-    float4 *out = a.clumped + uint64_t{s} * n;
     const float4 *render = a.guide_render + slot * n;
     const float4 *unit = a.guide_noise + slot * n;
     // the goal's offset (:1627-1628, 1773-1776): render - origin, or for a
@@ -1231,8 +1236,12 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
         }
     }
     const float step = cut / static_cast<float>(n - 1u);
-    out[0] = p[0];
-    for (uint32_t cv = 1u; cv < n; ++cv) {
+    // the clumped CVs go back into the lane's row from the tip down: output
+    // cv reads input CVs up to cv + 2 (its spline position cut * cv is at
+    // most cv), so slot cv + 2 is free once output cv is computed; the two
+    // outputs above wait in registers.  CV 0 stays.
+    float4 pending1 = p[0], pending2 = p[0];  // outputs cv + 1 and cv + 2
+    for (uint32_t cv = n - 1u; cv >= 1u; --cv) {
         F3 q = cut < 0.0f ? xyz(p[0]) : curve_eval(p, n, step * static_cast<float>(cv));
         float scale;
         if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP_SCALE), c, c_length, c_width, cv,
@@ -1251,12 +1260,15 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
             d = xyz(unit[cv]) * (mask * noise * max_std(raw_scale, 0.0f));
         }
         q = q + d;
-        out[cv] = make_float4(q.x, q.y, q.z, p[cv].w);
+        const float4 o = make_float4(q.x, q.y, q.z, p[cv].w);
+        if (cv + 2u < n) p.tile[staged_index<true>(cv + 2u, p.lane)] = pending2;
+        pending2 = pending1;
+        pending1 = o;
     }
-    for (uint32_t cv = 0u; cv < n; ++cv) p.tile[staged_index<true>(cv, p.lane)] = out[cv];
+    if (n > 2u) p.tile[staged_index<true>(2u, p.lane)] = pending2;
+    p.tile[staged_index<true>(1u, p.lane)] = pending1;
     c_length = spline_length(a.length_samples, p, n);
     at_one = true;
-    clumped = true;
     return true;
 }
 
@@ -1271,12 +1283,12 @@ __device__ __forceinline__ F3 fallback_surface_u(F3 normal) {
 // at the context's t; the frame transported from the surface normal along
 // the segments (acos, cos and sin as the reference evaluates them); per CV
 // the magnitude at its t and the classic noise of the domain moved along
-// each axis; the preserve-length rescale.  The result goes to a.clumped and
-// back into the lane's LDS row; c_length follows.  False (with *err) where
-// the reference throws.
+// each axis; the preserve-length rescale.  The result is rewritten in the
+// lane's LDS row; c_length follows.  False (with *err) where the reference
+// throws.
 __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module, const ClassicContext &c,
                                               const StagedCurve<true> &p, uint32_t s, float &c_length, float c_width,
-                                              bool &at_one, bool &written, uint32_t *err) {
+                                              bool &at_one, uint32_t *err) {
     const uint32_t n = a.cvs;
 #define NOISE_PROGRAM(k) NXG_CLASSIC_EXPR_NOISE_OF(module, NXG_CLASSIC_EXPR_NOISE_##k)
     const float t0 = at_one ? 1.0f : 0.0f;
@@ -1324,8 +1336,8 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module,
     F3 prior = length_squared(surface_normal) >= 1.0e-12f ? normalize3(surface_normal) : F3{0.0f, 1.0f, 0.0f};
     F3 transported = surface_tangent;
     float travelled = 0.0f;
-    float4 *out = a.clumped + uint64_t{s} * n;
-    out[0] = p[0];
+    // each output goes into its own slot at once: the loop holds input CVs
+    // cv and cv + 1 in registers and reads cv + 2 from the row
     for (uint32_t cv = 0u; cv < n; ++cv) {
         if (cv > 0u) travelled += sqrtf(length_squared(current_base - previous_base));
         F3 next = prior;
@@ -1366,7 +1378,7 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module,
                            (classic_noise(domain.x, domain.y + distance, domain.z) - 0.5f) * magnitude,
                            (classic_noise(domain.x, domain.y, domain.z + distance) - 0.5f) * magnitude};
             const F3 q = current_base + normal * local.x + binormal * local.y + tangent * local.z;
-            out[cv] = make_float4(q.x, q.y, q.z, p[cv].w);
+            p.tile[staged_index<true>(cv, p.lane)] = make_float4(q.x, q.y, q.z, p[cv].w);
         }
         prior = next;
         previous_base = current_base;
@@ -1377,42 +1389,46 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module,
     }
     if (preserve > 0.001f) {
         float noisy = 0.0f;
-        for (uint32_t cv = 1u; cv < n; ++cv) noisy += sqrtf(length_squared(xyz(out[cv]) - xyz(out[cv - 1u])));
+        for (uint32_t cv = 1u; cv < n; ++cv) noisy += sqrtf(length_squared(xyz(p[cv]) - xyz(p[cv - 1u])));
         if (noisy > 0.0f) {
             const float target = original * preserve + noisy * (1.0f - preserve);
             if (fabsf(noisy - target) >= 0.0001f) {  // scale_curve (:576-585)
                 const float scale = target / noisy;
-                const F3 r0 = xyz(out[0]);
+                const F3 r0 = xyz(p[0]);
                 for (uint32_t cv = 0u; cv < n; ++cv) {
-                    const F3 q = r0 + (xyz(out[cv]) - r0) * scale;
-                    out[cv] = make_float4(q.x, q.y, q.z, out[cv].w);
+                    const float4 v = p[cv];
+                    const F3 q = r0 + (xyz(v) - r0) * scale;
+                    p.tile[staged_index<true>(cv, p.lane)] = make_float4(q.x, q.y, q.z, v.w);
                 }
             }
         }
     }
-    for (uint32_t cv = 0u; cv < n; ++cv) p.tile[staged_index<true>(cv, p.lane)] = out[cv];
     c_length = spline_length(a.length_samples, p, n);
     at_one = true;
-    written = true;
     return true;
 #undef NOISE_PROGRAM
 }
 
 // The rebuild half of cut_from_tip_and_rebuild (src/xgen_classic_runtime.cpp:
 // 729-742) for a CutFX that later modules follow: n CVs uniformly on
-// [0, cut] (all at the root when cut is 0), radii kept, into a.clumped and
-// back into the lane's LDS row.
-__device__ __forceinline__ void rebuild_staged(const CutArgs &a, const StagedCurve<true> &p, uint32_t s, float cut) {
+// [0, cut] (all at the root when cut is 0), radii kept, in the lane's LDS
+// row.
+__device__ __forceinline__ void rebuild_staged(const CutArgs &a, const StagedCurve<true> &p, float cut) {
     const uint32_t n = a.cvs;
-    float4 *out = a.clumped + uint64_t{s} * n;
     const float step = cut / static_cast<float>(n - 1u);
     const F3 root = xyz(p[0]);
-    out[0] = p[0];
-    for (uint32_t cv = 1u; cv < n; ++cv) {
+    // from the tip down with two outputs waiting, as the clump rebuilds (the
+    // cut parameter is at most 1)
+    float4 pending1 = p[0], pending2 = p[0];
+    for (uint32_t cv = n - 1u; cv >= 1u; --cv) {
         const F3 q = cut > 0.0f ? curve_eval(p, n, step * static_cast<float>(cv)) : root;
-        out[cv] = make_float4(q.x, q.y, q.z, p[cv].w);
+        const float4 o = make_float4(q.x, q.y, q.z, p[cv].w);
+        if (cv + 2u < n) p.tile[staged_index<true>(cv + 2u, p.lane)] = pending2;
+        pending2 = pending1;
+        pending1 = o;
     }
-    for (uint32_t cv = 0u; cv < n; ++cv) p.tile[staged_index<true>(cv, p.lane)] = out[cv];
+    if (n > 2u) p.tile[staged_index<true>(2u, p.lane)] = pending2;
+    p.tile[staged_index<true>(1u, p.lane)] = pending1;
 }
 
 // One strand of the runtime chain (src/xgen_classic_runtime.cpp:1619-1858):
@@ -1459,7 +1475,6 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
     const float c_width = width;
     bool clumped = false;  // the context's t is 1: the last effect that set it ran its CV loop
-    bool written = false;  // a.clumped holds the strand's current CVs
     constexpr bool kStaged = std::is_same<Curve, StagedCurve<true>>::value;
     // the effects in list order (:1679-1826); effects that change the CVs
     // run on staged strands (at most 64 CVs), a trailing CutFX also on
@@ -1484,8 +1499,7 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
             if (cut < 1.0e-4f) keep = false;  // culled, but the later effects still run
             if (e + 1u < a.effect_count) {
                 if constexpr (kStaged) {  // later effects read the cut strand: rebuild it now
-                    rebuild_staged(a, source, s, cut);
-                    written = true;
+                    rebuild_staged(a, source, cut);
                     cut = 1.0f;  // nothing left for the write to resample
                     c_length = spline_length(a.length_samples, source, n);
                 }
@@ -1494,19 +1508,13 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
             }
         } else if (type == NXG_CLASSIC_EFFECT_NOISE) {
             if constexpr (kStaged && NOISE) {
-                if (!runtime_noise(a, module, c, source, s, c_length, c_width, clumped, written, err)) return -1.0f;
+                if (!runtime_noise(a, module, c, source, s, c_length, c_width, clumped, err)) return -1.0f;
             }
         } else {
             if constexpr (kStaged) {
-                if (!runtime_clump(a, module, c, source, s, c_length, c_width, clumped, written, err))
+                if (!runtime_clump(a, module, c, source, s, c_length, c_width, clumped, err))
                     return -1.0f;
             }
-        }
-    }
-    if constexpr (kStaged) {
-        if (a.rewrites && !written) {  // the write reads every strand from a.clumped
-            float4 *out = a.clumped + uint64_t{s} * n;
-            for (uint32_t cv = 0u; cv < n; ++cv) out[cv] = source[cv];
         }
     }
     float taper = 0.0f, taper_start = 0.0f;
@@ -1568,6 +1576,17 @@ __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
         else
             cut = runtime_strand<false>(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
         a.cut[s] = cut;
+    }
+    if constexpr (STAGED) {
+        if (a.rewrites) {  // every row as the effects left it, for the write: one contiguous run per tile
+            __syncthreads();
+            float4 *dst = a.clumped + uint64_t{first} * n;
+            const uint32_t total = count * n;
+            for (uint32_t e = lane; e < total; e += kWave) {
+                const uint32_t r = __umulhi(e, a.div_magic);
+                dst[e] = tile[staged_index<true>(e - r * n, r)];
+            }
+        }
     }
     const uint64_t kept = __ballot(cut >= 0.0f);
     uint32_t bits = 0u;  // the wave's NXG_CLASSIC_* bits (all below 1 << 16)
