@@ -400,6 +400,116 @@ def bench_batch(args):
     return 0
 
 
+def train_surrogate(asset, w, es, maps, pairs, epochs):
+    """A network trained on `pairs` exact-runtime attribute rows of roots drawn
+    with a seed disjoint from the workload's (seed + 1000).  Returns (net,
+    seconds, first and last loss)."""
+    import torch
+    from nanoxgen_amd import expr, surrogate
+    t0 = time.perf_counter()
+    train_p = w.params.copy(seed=w.params.seed + 1000, strand_count=pairs)
+    feats = surrogate.root_features(asset, train_p, maps)
+    targets = torch.empty((pairs, 4), device="cuda")
+    expr.evaluate_root_attributes(asset, train_p, es, maps, targets)
+    net = surrogate.Surrogate(seed=1)
+    net.fit_norm(feats, targets)
+    hist = net.train(feats, targets, epochs=epochs, batch=4096, lr=3e-3, log_every=256)
+    torch.cuda.synchronize()
+    return net, time.perf_counter() - t0, (hist[0][1], hist[-1][1])
+
+
+def bench_batch_surrogate(args):
+    """The batched uneven form with the learned surrogate on its descriptions
+    of more than 20 ops (six of the nine).  The programs' noise3 argument
+    scales are multiplied by 0.1 (the surrogate workload's scale; at 1.0 the
+    networks serve no tile at 1e-3).  Each such description gets its own
+    network, trained outside the timed region on 2^20 seed-disjoint exact
+    pairs; in the batch its attributes are the network's, checked on one
+    root per 64-root tile at the threshold, a tile that misses re-evaluated
+    exactly.  The surrogate batch's graph is timed against the exact batch's
+    graph of the same descriptions (same tuning, from the autotuner run on
+    the exact batch); per description: tiles served, errors of the served
+    roots against the exact attributes, and point counts."""
+    import torch
+    import nanoxgen_amd as nx
+    from nanoxgen_amd import batch, expr, workloads
+
+    ws = workloads.config5(frequency_scale=workloads.SURROGATE_FREQUENCY_SCALE)
+    blobs = [nx.build_asset(w.build) for w in ws]
+    assets = [nx.DeviceAsset(b) for b in blobs]
+    stream = torch.cuda.current_stream()
+    exact_b, sur_b = batch.Batch(), batch.Batch()
+    outs_e, outs_s, trained = [], [], []
+    for w, a in zip(ws, assets):
+        es, maps = expr.ExprSet(w.info["programs"]), expr.TexelMaps(w.info["maps"])
+        net = None
+        if w.info["ops"] > 20:
+            net, secs, loss = train_surrogate(a, w, es, maps, 1 << 20, args.train_epochs)
+            trained.append((w.name, round(secs, 2), loss))
+            log(f"{w.name}: trained in {secs:.1f} s, loss {loss[0]:.4f} -> {loss[1]:.2e}")
+        n, c = w.params.strand_count, w.params.cvs_per_strand
+        outs_e.append(nx.PackedOutputs(n, c, roots=False, root_uvs=False))
+        outs_s.append(nx.PackedOutputs(n, c, roots=False, root_uvs=False))
+        exact_b.add(a, w.params, outs_e[-1], octaves=w.octaves, exprs=es, maps=maps)
+        sur_b.add(a, w.params, outs_s[-1], octaves=w.octaves, exprs=es, maps=maps, surrogate=net,
+                  surrogate_threshold=args.threshold, surrogate_stride=args.sample_stride)
+    best, _ = exact_b.autotune(repeats=3)
+    torch.cuda.synchronize()
+    log(f"autotuner picked {best}")
+    g_exact, g_sur = exact_b.graph(best), sur_b.graph(best)
+
+    def timed(g):
+        for _ in range(args.warmup):
+            g.launch(stream)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            g.launch(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / args.steps
+
+    exact_ms = timed(g_exact)
+    sur_ms = timed(g_sur)
+    torch.cuda.synchronize()
+    items = []
+    for k, w in enumerate(ws):
+        e_s, e_e = sur_b.entries[k], exact_b.entries[k]
+        rec = {"description": w.name, "ops": w.info["ops"], "surrogate": e_s["surrogate"] is not None,
+               "point_counts_exact": bool(torch.equal(outs_s[k].point_counts, outs_e[k].point_counts))}
+        if e_s["surrogate"] is not None:
+            redone, checked = sur_b.stats(k)
+            tiles = e_s["surrogate_flags"].numel()
+            exact = e_e["attributes"]
+            rel = ((e_s["attributes"] - exact).abs() / exact.abs().clamp_min(1e-6)).max(1).values
+            served = torch.repeat_interleave(e_s["surrogate_flags"] == 0, 64)[:w.params.strand_count]
+            rs = rel[served]
+            rec.update(tiles=tiles, tiles_served=tiles - redone, samples_checked=checked,
+                       served_roots_within_threshold=float((rs <= args.threshold).float().mean()) if rs.numel()
+                       else None,
+                       served_roots_max_rel_error=float(rs.max()) if rs.numel() else None)
+        items.append(rec)
+        log(f"  {rec}")
+    g_exact.close()
+    g_sur.close()
+    points = sur_b.points
+    line = {"metric": METRIC, "value": points / (sur_ms * 1e-3), "unit": "points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sur_ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (surrogate: bf16 mfma)", "data": "synthetic",
+            "config": {"workload": f"batch-surrogate-{len(ws)}-descriptions-{sur_b.strands}-strands-{points}-points"
+                                   f"-f{workloads.SURROGATE_FREQUENCY_SCALE:g}",
+                       "threshold": args.threshold, "sample_stride": args.sample_stride,
+                       "tuning": dict(zip(["waves_per_block", "chunk_cvs", "tiles_per_claim", "blocks_per_cu",
+                                           "order"], best.as_tuple())),
+                       "parallelism": "single-gpu"},
+            "exact_batch_ms": round(exact_ms, 4), "exact_batch_points_per_s": points / (exact_ms * 1e-3),
+            "speedup_vs_exact": round(exact_ms / sur_ms, 4), "items": items,
+            "training": [{"description": n, "seconds": s, "loss_first_last": l} for n, s, l in trained],
+            "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 CUT_PMC_SUMMARY = os.path.join(ROOT, "profiles", "cut_search_kernel_plane2M_pmc.json")
 RUNTIME_PMC_SUMMARY = os.path.join(ROOT, "profiles", "runtime_chain_plane2M_classic_expr_pmc.json")
 PLAN_PMC_SUMMARY = os.path.join(ROOT, "profiles", "runtime_chain_plane2M_classic_plan_pmc.json")
@@ -1025,7 +1135,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="plane",
-                    choices=["plane", "plane_cdf", "expr", "surrogate", "surrogate_sweep", "batch", "classic",
+                    choices=["plane", "plane_cdf", "expr", "surrogate", "surrogate_sweep", "batch",
+                             "batch_surrogate", "classic",
                              "classic_expr", "classic_plan", "classic_noise", "clump", "transport"],
                     help="plane: 2,000,000 strands x 16 CVs, two octaves (default); "
                          "plane_cdf: the same with the opt-in inverse-CDF root sampler; "
@@ -1034,6 +1145,8 @@ def main():
                          "the trained surrogate + sampled exact check against the exact runtime; "
                          "surrogate_sweep: that comparison over thresholds and program noise frequencies; "
                          "batch: nine uneven descriptions (2,456,139 strands, 4..64 CVs) in one captured graph; "
+                         "batch_surrogate: the same at program noise frequency scale 0.1, the surrogate on the "
+                         "descriptions of more than 20 ops, against the exact batch; "
                          "classic: Classic CutFX (0.25 x cLength) over the plane's 2,000,000 generated strands; "
                          "classic_expr: the same strands through a Classic plan of length, width, taper, "
                          "taperStart, widthRamp and CutFX-amount expressions; "
@@ -1067,6 +1180,8 @@ def main():
         return bench_surrogate_sweep(args)
     if args.workload == "batch" and args.gpus == 1:
         return bench_batch(args)
+    if args.workload == "batch_surrogate" and args.gpus == 1:
+        return bench_batch_surrogate(args)
     if args.workload == "classic" and args.gpus == 1:
         return bench_classic(args)
     if args.workload == "classic_expr" and args.gpus == 1:

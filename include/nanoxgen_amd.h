@@ -757,6 +757,18 @@ typedef struct nxg_batch_item {
     const nxg_stencil_output *stencil;      /* optional */
     void *workspace;                        /* nxg_packed_generation_workspace_size bytes */
     uint64_t workspace_bytes;
+    /* optional learned surrogate for an item with exprs (extension, no reference counterpart): its
+     * attributes are predicted by this MLP state (nxg_surrogate_attributes, with surrogate_map as the
+     * features' texel map, roots in the item's strand order), then checked and, where a sampled root
+     * misses surrogate_threshold, re-evaluated by exprs (nxg_verify_attributes with surrogate_stride,
+     * surrogate_flags and surrogate_stats) */
+    const void *surrogate;
+    const nxg_texel_map *surrogate_map;
+    float surrogate_threshold;
+    uint32_t surrogate_stride;
+    uint32_t *surrogate_flags;              /* device scratch, one word per 64-strand tile */
+    uint64_t surrogate_flag_capacity;
+    uint32_t *surrogate_stats;              /* optional device [2]: tiles re-evaluated, samples checked */
 } nxg_batch_item;
 
 #define NXG_BATCH_ORDER_GIVEN 0u     /* descriptions in item order */
@@ -776,10 +788,12 @@ void nxg_batch_default_tuning(nxg_batch_tuning *tuning);
 /* Validates every item as nxg_generate_packed would, plus the tuning. */
 uint32_t nxg_batch_validate(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning);
 
-/* On `stream`: the attribute programs of items that carry exprs
- * (nxg_evaluate_root_attributes, one launch each), one guide preparation
- * over all items (which also zeroes *counter, a caller-owned device word),
- * then the persistent strand kernel.  Allocates nothing, never synchronises, records no memset: the
+/* On `stream`: the strand order of every item (five launches over all
+ * items), the attributes of items that carry exprs (nxg_evaluate_root_
+ * attributes, one launch each; or, with a surrogate, the surrogate's
+ * prediction in the item's strand order and the exact check and fallback),
+ * one guide preparation over all items (which also zeroes *counter, a
+ * caller-owned device word), then the persistent strand kernel.  Allocates nothing, never synchronises, records no memset: the
  * call may be captured into a graph.  Calls sharing one counter must not
  * overlap. */
 uint32_t nxg_generate_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning,

@@ -447,7 +447,10 @@ class BatchItem(C.Structure):
     _fields_ = [("asset", C.c_void_p), ("deformed", C.c_void_p), ("params", C.c_void_p), ("octaves", C.c_void_p),
                 ("attributes", C.c_void_p), ("attribute_capacity", C.c_uint64), ("exprs", C.c_void_p),
                 ("maps", C.c_void_p), ("map_count", C.c_uint32), ("reserved", C.c_uint32), ("output", C.c_void_p),
-                ("stencil", C.c_void_p), ("workspace", C.c_void_p), ("workspace_bytes", C.c_uint64)]
+                ("stencil", C.c_void_p), ("workspace", C.c_void_p), ("workspace_bytes", C.c_uint64),
+                ("surrogate", C.c_void_p), ("surrogate_map", C.c_void_p), ("surrogate_threshold", C.c_float),
+                ("surrogate_stride", C.c_uint32), ("surrogate_flags", C.c_void_p),
+                ("surrogate_flag_capacity", C.c_uint64), ("surrogate_stats", C.c_void_p)]
 
 
 class BatchTuning(C.Structure):

@@ -39,18 +39,30 @@ class Batch:
         self.counter = torch.zeros(64, dtype=torch.int32, device=device)  # one word used; 256-B line of its own
 
     def add(self, asset, params, outputs, *, octaves=None, deformed=None, attributes=None, exprs=None, maps=None,
-            workspace=None):
+            workspace=None, surrogate=None, surrogate_threshold=1e-3, surrogate_stride=64):
         """attributes: float32 [N, 4] device tensor of per-root multipliers;
         with exprs (an expr.ExprSet) and maps (an expr.TexelMaps) the batch
-        evaluates them into `attributes` (allocated here if None) first."""
+        evaluates them into `attributes` (allocated here if None) first.
+        surrogate (a surrogate.Surrogate, with exprs): the attributes are
+        predicted by the network from the first texel map's features, then
+        checked by the exact programs on every surrogate_stride-th root, a
+        64-root tile whose check misses surrogate_threshold re-evaluated
+        exactly; stats() gives (tiles re-evaluated, samples checked)."""
         torch = self.torch
         if workspace is None:
             workspace = torch.empty(max(1, asset.workspace_bytes(params)), dtype=torch.uint8, device=self.device)
         if exprs is not None and attributes is None:
             attributes = torch.empty((params.strand_count, 4), dtype=torch.float32, device=self.device)
+        if surrogate is not None and exprs is None:
+            raise ValueError("a surrogate needs the item's exact programs (exprs) to check it")
         e = dict(asset=asset, params=params, outputs=outputs, octaves=octaves, deformed=deformed,
                  attributes=attributes, exprs=exprs, maps=maps, workspace=workspace, out=outputs.descriptor(),
-                 stencil=outputs.stencil_descriptor())
+                 stencil=outputs.stencil_descriptor(), surrogate=surrogate)
+        if surrogate is not None:
+            tiles = (params.strand_count + 63) // 64
+            e.update(surrogate_flags=torch.zeros(max(1, tiles), dtype=torch.int32, device=self.device),
+                     surrogate_stats=torch.zeros(2, dtype=torch.int32, device=self.device),
+                     surrogate_threshold=surrogate_threshold, surrogate_stride=surrogate_stride)
         self.entries.append(e)
         return len(self.entries) - 1
 
@@ -74,6 +86,14 @@ class Batch:
             it.stencil = C.addressof(e["stencil"]) if e["stencil"] is not None else None
             it.workspace = e["workspace"].data_ptr()
             it.workspace_bytes = e["workspace"].numel()
+            if e["surrogate"] is not None:
+                it.surrogate = e["surrogate"].ptr()
+                it.surrogate_map = C.addressof(e["maps"].structs[0])
+                it.surrogate_threshold = e["surrogate_threshold"]
+                it.surrogate_stride = e["surrogate_stride"]
+                it.surrogate_flags = e["surrogate_flags"].data_ptr()
+                it.surrogate_flag_capacity = e["surrogate_flags"].numel()
+                it.surrogate_stats = e["surrogate_stats"].data_ptr()
         self._items = arr  # keep alive while the library reads it
         return arr
 
@@ -98,6 +118,12 @@ class Batch:
                                        _stream_handle(stream), arr, len(cands), repeats, C.byref(best), times),
               "nxg_batch_autotune")
         return best, [(cands[i], None if times[i] < 0 else float(times[i])) for i in range(len(cands))]
+
+    def stats(self, k):
+        """(tiles re-evaluated, samples checked) of item k's surrogate check
+        in the last run (synchronises)."""
+        st = self.entries[k]["surrogate_stats"].cpu().tolist()
+        return int(st[0]), int(st[1])
 
     @property
     def points(self):

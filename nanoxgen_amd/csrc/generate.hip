@@ -1658,6 +1658,7 @@ double strand_cost(const Args &a) {
 struct BatchLaunch {
     AttrLaunch attrs[NXG_BATCH_MAX_ITEMS];
     bool has_attrs[NXG_BATCH_MAX_ITEMS];
+    const nxg_batch_item *items;  // the caller's items (surrogate launches read them)
     BatchOrderArgs order;
     uint32_t order_tiles, order_scan_blocks;
     BatchPrepArgs prep;
@@ -1682,6 +1683,14 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
         const nxg_batch_item &it = items[k];
         if (!it.params || !it.output) return NXG_ERROR_NULL_ARGUMENT;
         out.has_attrs[k] = it.exprs != nullptr;
+        if (it.surrogate) {  // the checks nxg_surrogate_attributes and nxg_verify_attributes make
+            const nxg_texel_map *m = it.surrogate_map;
+            if (!it.exprs || !m || !m->texels || m->face_count == 0u || m->resolution == 0u || !it.surrogate_flags)
+                return NXG_ERROR_NULL_ARGUMENT;
+            if (it.surrogate_flag_capacity < (uint64_t{it.params->strand_count} + 63u) / 64u)
+                return NXG_ERROR_BUFFER_TOO_SMALL;
+            if (it.surrogate_stride == 0u || !(it.surrogate_threshold >= 0.0f)) return NXG_ERROR_INVALID_MLP_REQUEST;
+        }
         if (it.exprs) {
             if (uint32_t s = make_attr_launch(it.asset, it.params, it.exprs, it.maps, it.map_count, it.attributes,
                                               it.attribute_capacity, nullptr, kAttrAll, 1u, 0.0f, nullptr, nullptr,
@@ -1735,13 +1744,11 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
     out.strands.total_tiles = static_cast<uint32_t>(tiles);
     out.strands.tiles_per_claim = out.tuning.tiles_per_claim;
     out.strands.counter = counter;
+    out.items = items;
     return NXG_OK;
 }
 
 uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
-    for (uint32_t k = 0; k < b.strands.count; ++k)
-        if (b.has_attrs[k])
-            if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
     if (b.order_tiles) {
         hipLaunchKernelGGL(batch_order_count_kernel, dim3(b.order_tiles), dim3(1024), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scan_blocks_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
@@ -1749,6 +1756,25 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
         hipLaunchKernelGGL(batch_order_scan_add_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scatter_kernel, dim3(b.order_tiles), dim3(1024), 0, stream, b.order);
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
+    }
+    // attributes after the order: a surrogate reads its roots in the item's strand order
+    for (uint32_t k = 0; k < b.strands.count; ++k) {
+        if (!b.has_attrs[k]) continue;
+        const nxg_batch_item &it = b.items[k];
+        if (!it.surrogate) {
+            if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
+            continue;
+        }
+        if (uint32_t s = nxg_surrogate_attributes(it.asset, it.params, it.surrogate, it.surrogate_map, it.attributes,
+                                                  it.attribute_capacity, it.workspace, it.workspace_bytes, stream);
+            s != NXG_OK)
+            return s;
+        if (uint32_t s = nxg_verify_attributes(it.asset, it.params, it.exprs, it.maps, it.map_count, it.attributes,
+                                               it.attribute_capacity, it.surrogate_threshold, it.surrogate_stride,
+                                               it.surrogate_flags, it.surrogate_flag_capacity, it.surrogate_stats,
+                                               stream);
+            s != NXG_OK)
+            return s;
     }
     if (b.prep_blocks) {
         hipLaunchKernelGGL(batch_prep_kernel, dim3(b.prep_blocks), dim3(256), 0, stream, b.prep);

@@ -263,12 +263,14 @@ CONFIG5_DESCRIPTIONS = [
 ]
 
 
-def config5(seed=50, strand_divisor=1, mesh_divisor=1, descriptions=None):
+def config5(seed=50, strand_divisor=1, mesh_divisor=1, descriptions=None, frequency_scale=1.0):
     """Batched uneven form: nine descriptions, each with its own seeded mesh,
     guides, stencil K, octaves and per-root length/width/noise programs
     (plus a 4x4 per-face texel map), generated in one batched launch.
     strand_divisor / mesh_divisor shrink strand counts and mesh resolution
-    (plane quads and guides per side, sphere subdivision) for tests."""
+    (plane quads and guides per side, sphere subdivision) for tests;
+    frequency_scale multiplies the programs' noise3 argument scales (1.0
+    leaves them unchanged bit for bit)."""
     from . import expr
     out = []
     table = CONFIG5_DESCRIPTIONS if descriptions is None else [CONFIG5_DESCRIPTIONS[i] for i in descriptions]
@@ -290,9 +292,11 @@ def config5(seed=50, strand_divisor=1, mesh_divisor=1, descriptions=None):
         octaves = NoiseOctaves.of([(0.08 * base_len, 4.0), (0.04 * base_len, 8.0)]) if octs == 2 else None
         noise_calls = 4 if ops >= 24 else 2
         programs = [expr.random_attribute_program(dseed * 1000 + j, *expr.ATTRIBUTE_RANGES[n], n_ops=ops,
-                                                  noise_calls=noise_calls)
+                                                  noise_calls=noise_calls, frequency_scale=frequency_scale)
                     for j, n in enumerate(["length", "width", "noise"])]
         maps = [expr.texel_map(len(build.triangles), 4, seed=dseed + 1)]
         name = f"d{i}-{mesh[0]}-{len(build.triangles)}tri-{params.strand_count}x{cvs}cv-K{k}-{octs}oct-{ops}op"
+        if frequency_scale != 1.0:
+            name += f"-f{frequency_scale:g}"
         out.append(Workload(name, build, params, octaves, info=dict(programs=programs, maps=maps, ops=ops)))
     return out

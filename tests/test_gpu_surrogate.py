@@ -262,3 +262,71 @@ def test_fused_kernel_equals_forward_on_features(torch, trained):
     plain = net.forward(feats)
     torch.cuda.synchronize()
     assert torch.equal(fused, plain)
+
+
+def test_surrogate_item_in_a_batch(torch, trained):
+    """A batch holding a surrogate item (the network, then the exact check
+    of one root per 64-root tile), an exact item with the same programs and
+    a plain item: the surrogate item's attributes, tile flags and counts
+    equal nxg_surrogate_attributes + nxg_verify_attributes alone bit for
+    bit, and every item's outputs equal its single-description launch from
+    those attributes; a second run (and a graph replay) repeats them."""
+    from nanoxgen_amd import batch
+    t = trained
+    w, asset, es, maps, net = t["w"], t["asset"], t["es"], t["maps"], t["net"]
+    p_sur = w.params.copy(strand_count=20_001)
+    p_exact = w.params.copy(strand_count=5_000, seed=w.params.seed + 11)
+    p_plain = w.params.copy(strand_count=777, seed=3)
+    b = batch.Batch()
+    outs = [nx.PackedOutputs(p.strand_count, p.cvs_per_strand, stencil=True) for p in (p_sur, p_exact, p_plain)]
+    b.add(asset, p_sur, outs[0], octaves=w.octaves, exprs=es, maps=maps, surrogate=net,
+          surrogate_threshold=1e-3, surrogate_stride=64)
+    b.add(asset, p_exact, outs[1], octaves=w.octaves, exprs=es, maps=maps)
+    b.add(asset, p_plain, outs[2], octaves=w.octaves)
+    b.generate()
+    torch.cuda.synchronize()
+    n = p_sur.strand_count
+    attrs = torch.empty((n, 4), device="cuda")
+    surrogate.surrogate_attributes(asset, p_sur, net, maps, attrs)
+    flags = torch.zeros((n + 63) // 64, dtype=torch.int32, device="cuda")
+    stats = torch.zeros(2, dtype=torch.int32, device="cuda")
+    surrogate.verify_attributes(asset, p_sur, es, maps, attrs, 1e-3, 64, flags, stats)
+    torch.cuda.synchronize()
+    e0 = b.entries[0]
+    assert torch.equal(e0["attributes"], attrs)
+    assert torch.equal(e0["surrogate_flags"], flags)
+    assert b.stats(0) == tuple(stats.tolist())
+    redo = int(flags.ne(0).sum())
+    print(f"batch surrogate item: {len(flags) - redo} of {len(flags)} tiles served")
+    exact = torch.empty((p_exact.strand_count, 4), device="cuda")
+    expr.evaluate_root_attributes(asset, p_exact, es, maps, exact)
+    torch.cuda.synchronize()
+    assert torch.equal(b.entries[1]["attributes"], exact)
+    fields = ("points", "roots", "root_uvs", "point_counts", "stencil_indices", "stencil_weights")
+    want = []
+    for p, out, a in zip((p_sur, p_exact, p_plain), outs, (attrs, exact, None)):
+        single = nx.PackedOutputs(p.strand_count, p.cvs_per_strand, stencil=True)
+        nx.generate_packed(asset, p, single, octaves=w.octaves, attributes=a)
+        torch.cuda.synchronize()
+        for f in fields:
+            assert torch.equal(getattr(out, f), getattr(single, f)), f"{p.strand_count} strands: {f}"
+        want.append(out.points.clone())
+    # the flags and counters are cleared by the check itself on each run
+    for run in ("eager", "graph"):
+        for out in outs:
+            out.points.zero_()
+        e0["attributes"].fill_(float("nan"))
+        torch.cuda.synchronize()
+        if run == "eager":
+            b.generate()
+        else:
+            g = b.graph()
+            g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(e0["attributes"], attrs), run
+        assert torch.equal(e0["surrogate_flags"], flags), run
+        assert b.stats(0) == tuple(stats.tolist()), run
+        for out, pts in zip(outs, want):
+            assert torch.equal(out.points, pts), run
+        if run == "graph":
+            g.close()
