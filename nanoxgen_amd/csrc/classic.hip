@@ -1141,6 +1141,62 @@ __device__ __forceinline__ bool scale_in_place(const StagedCurve<ROTATED> &p, ui
 }
 __device__ __forceinline__ bool scale_in_place(const MemoryCurve &, uint32_t, float) { return false; }
 
+// A lane's strand in a runtime-chain tile, W floats per CV, CV-major with
+// the lanes rotated as in staged_index.  W = 3 stages positions only (a
+// 16-CV tile of 64 strands is 12 KB, so 13 tiles share a CU's LDS instead
+// of 10; no effect changes a radius, so radii come from the input points);
+// W = 4 stages whole CVs, one 16-byte LDS access each, for kernels whose
+// register use already keeps fewer tiles resident than LDS would.
+template <uint32_t W>
+struct StagedRows {
+    float *tile;
+    uint32_t lane;
+    __device__ __forceinline__ uint32_t at(uint32_t c) const { return W * staged_index<true>(c, lane); }
+    __device__ __forceinline__ float4 operator[](uint32_t c) const {
+        if constexpr (W == 4u) return reinterpret_cast<const float4 *>(tile)[staged_index<true>(c, lane)];
+        const uint32_t i = at(c);
+        return make_float4(tile[i], tile[i + 1u], tile[i + 2u], 0.0f);
+    }
+    __device__ __forceinline__ void store(uint32_t c, F3 q) const {
+        const uint32_t i = at(c);
+        tile[i] = q.x;
+        tile[i + 1u] = q.y;
+        tile[i + 2u] = q.z;
+    }
+};
+template <class T>
+struct IsStagedRows : std::false_type {};
+template <uint32_t W>
+struct IsStagedRows<StagedRows<W>> : std::true_type {};
+
+// The tile's `count` strands, read coalesced, into LDS rows of W floats.
+template <uint32_t W>
+__device__ __forceinline__ void stage_rows(const CutArgs &a, float *tile, uint32_t first, uint32_t count) {
+    const float4 *src = a.points + uint64_t{first} * a.cvs;
+    const uint32_t total = count * a.cvs;
+    for (uint32_t i = threadIdx.x; i < total; i += kWave) {
+        const uint32_t l = __umulhi(i, a.div_magic);
+        const uint32_t c = i - l * a.cvs;
+        const float4 v = src[i];
+        if constexpr (W == 4u) {
+            reinterpret_cast<float4 *>(tile)[staged_index<true>(c, l)] = v;
+        } else {
+            const uint32_t j = W * staged_index<true>(c, l);
+            tile[j] = v.x;
+            tile[j + 1u] = v.y;
+            tile[j + 2u] = v.z;
+        }
+    }
+    __syncthreads();
+}
+
+template <uint32_t W>
+__device__ __forceinline__ bool scale_in_place(const StagedRows<W> &p, uint32_t n, float scale) {
+    const F3 root = xyz(p[0]);
+    for (uint32_t c = 0u; c < n; ++c) p.store(c, root + (xyz(p[c]) - root) * scale);
+    return true;
+}
+
 // The taper part of the width profile at t (src/xgen_classic_runtime.cpp:
 // 1842-1848).
 __device__ __forceinline__ float taper_scale(float t, float taper, float taper_start) {
@@ -1156,8 +1212,9 @@ __device__ __forceinline__ float taper_scale(float t, float taper, float taper_s
 // context's t: a module with a valid guide sets it to 0 for its mask and,
 // when it clumps, leaves the t of its last CV, 1.  False (with *err) where
 // the reference throws.
+template <class Staged>
 __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, const ClassicContext &c,
-                                              const StagedCurve<true> &p, uint32_t s, float &c_length,
+                                              const Staged &p, uint32_t s, float &c_length,
                                               float c_width, bool &at_one, uint32_t *err) {
     const uint32_t n = a.cvs;
     const ClumpModule &cm = a.clumps[m];
@@ -1210,7 +1267,7 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
     const float cut = cut_to_length(a.length_samples, p, n, info.x);
     const float noise_length = c_length;  // the noise expressions' cLength
     if ((a.clump_length >> m) & 1u)
-        c_length = spline_length(a.length_samples, RebuiltCurve<StagedCurve<true>>{p, n, cut}, n);
+        c_length = spline_length(a.length_samples, RebuiltCurve<Staged>{p, n, cut}, n);
     float raw_amount;
     if (!classic_value(a, NXG_CLASSIC_EXPR_CLUMP_OF(m, NXG_CLASSIC_EXPR_CLUMP), c, c_length, c_width, 0u,
                        raw_amount)) {
@@ -1240,7 +1297,7 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
     // cv reads input CVs up to cv + 2 (its spline position cut * cv is at
     // most cv), so slot cv + 2 is free once output cv is computed; the two
     // outputs above wait in registers.  CV 0 stays.
-    float4 pending1 = p[0], pending2 = p[0];  // outputs cv + 1 and cv + 2
+    F3 pending1 = xyz(p[0]), pending2 = pending1;  // outputs cv + 1 and cv + 2
     for (uint32_t cv = n - 1u; cv >= 1u; --cv) {
         F3 q = cut < 0.0f ? xyz(p[0]) : curve_eval(p, n, step * static_cast<float>(cv));
         float scale;
@@ -1260,13 +1317,12 @@ __device__ __forceinline__ bool runtime_clump(const CutArgs &a, uint32_t m, cons
             d = xyz(unit[cv]) * (mask * noise * max_std(raw_scale, 0.0f));
         }
         q = q + d;
-        const float4 o = make_float4(q.x, q.y, q.z, p[cv].w);
-        if (cv + 2u < n) p.tile[staged_index<true>(cv + 2u, p.lane)] = pending2;
+        if (cv + 2u < n) p.store(cv + 2u, pending2);
         pending2 = pending1;
-        pending1 = o;
+        pending1 = q;
     }
-    if (n > 2u) p.tile[staged_index<true>(2u, p.lane)] = pending2;
-    p.tile[staged_index<true>(1u, p.lane)] = pending1;
+    if (n > 2u) p.store(2u, pending2);
+    p.store(1u, pending1);
     c_length = spline_length(a.length_samples, p, n);
     at_one = true;
     return true;
@@ -1286,8 +1342,9 @@ __device__ __forceinline__ F3 fallback_surface_u(F3 normal) {
 // each axis; the preserve-length rescale.  The result is rewritten in the
 // lane's LDS row; c_length follows.  False (with *err) where the reference
 // throws.
+template <class Staged>
 __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module, const ClassicContext &c,
-                                              const StagedCurve<true> &p, uint32_t s, float &c_length, float c_width,
+                                              const Staged &p, uint32_t s, float &c_length, float c_width,
                                               bool &at_one, uint32_t *err) {
     const uint32_t n = a.cvs;
 #define NOISE_PROGRAM(k) NXG_CLASSIC_EXPR_NOISE_OF(module, NXG_CLASSIC_EXPR_NOISE_##k)
@@ -1377,8 +1434,7 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module,
             const F3 local{(classic_noise(domain.x + distance, domain.y, domain.z) - 0.5f) * magnitude,
                            (classic_noise(domain.x, domain.y + distance, domain.z) - 0.5f) * magnitude,
                            (classic_noise(domain.x, domain.y, domain.z + distance) - 0.5f) * magnitude};
-            const F3 q = current_base + normal * local.x + binormal * local.y + tangent * local.z;
-            p.tile[staged_index<true>(cv, p.lane)] = make_float4(q.x, q.y, q.z, p[cv].w);
+            p.store(cv, current_base + normal * local.x + binormal * local.y + tangent * local.z);
         }
         prior = next;
         previous_base = current_base;
@@ -1395,11 +1451,7 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module,
             if (fabsf(noisy - target) >= 0.0001f) {  // scale_curve (:576-585)
                 const float scale = target / noisy;
                 const F3 r0 = xyz(p[0]);
-                for (uint32_t cv = 0u; cv < n; ++cv) {
-                    const float4 v = p[cv];
-                    const F3 q = r0 + (xyz(v) - r0) * scale;
-                    p.tile[staged_index<true>(cv, p.lane)] = make_float4(q.x, q.y, q.z, v.w);
-                }
+                for (uint32_t cv = 0u; cv < n; ++cv) p.store(cv, r0 + (xyz(p[cv]) - r0) * scale);
             }
         }
     }
@@ -1413,22 +1465,22 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module,
 // 729-742) for a CutFX that later modules follow: n CVs uniformly on
 // [0, cut] (all at the root when cut is 0), radii kept, in the lane's LDS
 // row.
-__device__ __forceinline__ void rebuild_staged(const CutArgs &a, const StagedCurve<true> &p, float cut) {
+template <class Staged>
+__device__ __forceinline__ void rebuild_staged(const CutArgs &a, const Staged &p, float cut) {
     const uint32_t n = a.cvs;
     const float step = cut / static_cast<float>(n - 1u);
     const F3 root = xyz(p[0]);
     // from the tip down with two outputs waiting, as the clump rebuilds (the
     // cut parameter is at most 1)
-    float4 pending1 = p[0], pending2 = p[0];
+    F3 pending1 = root, pending2 = root;
     for (uint32_t cv = n - 1u; cv >= 1u; --cv) {
         const F3 q = cut > 0.0f ? curve_eval(p, n, step * static_cast<float>(cv)) : root;
-        const float4 o = make_float4(q.x, q.y, q.z, p[cv].w);
-        if (cv + 2u < n) p.tile[staged_index<true>(cv + 2u, p.lane)] = pending2;
+        if (cv + 2u < n) p.store(cv + 2u, pending2);
         pending2 = pending1;
-        pending1 = o;
+        pending1 = q;
     }
-    if (n > 2u) p.tile[staged_index<true>(2u, p.lane)] = pending2;
-    p.tile[staged_index<true>(1u, p.lane)] = pending1;
+    if (n > 2u) p.store(2u, pending2);
+    p.store(1u, pending1);
 }
 
 // One strand of the runtime chain (src/xgen_classic_runtime.cpp:1619-1858):
@@ -1466,7 +1518,7 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
             return -1.0f;
         }
     } else {
-        width = a.radius_scale > 0.0f ? 2.0f * source[0].w / a.radius_scale : 0.0f;
+        width = a.radius_scale > 0.0f ? 2.0f * a.points[uint64_t{s} * n].w / a.radius_scale : 0.0f;
     }
     if (!isfinite(width)) {
         *err = NXG_CLASSIC_NONFINITE_WIDTH;
@@ -1475,7 +1527,7 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
     const float c_width = width;
     bool clumped = false;  // the context's t is 1: the last effect that set it ran its CV loop
-    constexpr bool kStaged = std::is_same<Curve, StagedCurve<true>>::value;
+    constexpr bool kStaged = IsStagedRows<Curve>::value;
     // the effects in list order (:1679-1826); effects that change the CVs
     // run on staged strands (at most 64 CVs), a trailing CutFX also on
     // strands read from memory (the write resamples them)
@@ -1566,13 +1618,17 @@ __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
     const uint32_t t = blockIdx.x, lane = threadIdx.x;
     const uint32_t first = t * kWave;
     const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
-    if constexpr (STAGED) stage_tile<true>(a, tile, first, count);
+    // the NoiseFX variant's registers keep fewer tiles resident than whole
+    // CVs in LDS would, so it stages them with their radii
+    constexpr uint32_t W = NOISE ? 4u : 3u;
+    float *rows = reinterpret_cast<float *>(tile);
+    if constexpr (STAGED) stage_rows<W>(a, rows, first, count);
     const uint32_t s = first + lane;
     float cut = -1.0f;
     uint32_t err = 0u;
     if (lane < count) {
         if constexpr (STAGED)
-            cut = runtime_strand<NOISE>(a, StagedCurve<true>{tile, lane}, s, &err);
+            cut = runtime_strand<NOISE>(a, StagedRows<W>{rows, lane}, s, &err);
         else
             cut = runtime_strand<false>(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
         a.cut[s] = cut;
@@ -1581,10 +1637,16 @@ __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
         if (a.rewrites) {  // every row as the effects left it, for the write: one contiguous run per tile
             __syncthreads();
             float4 *dst = a.clumped + uint64_t{first} * n;
+            const float4 *src = a.points + uint64_t{first} * n;  // the radii
             const uint32_t total = count * n;
             for (uint32_t e = lane; e < total; e += kWave) {
                 const uint32_t r = __umulhi(e, a.div_magic);
-                dst[e] = tile[staged_index<true>(e - r * n, r)];
+                if constexpr (W == 4u) {
+                    dst[e] = tile[staged_index<true>(e - r * n, r)];
+                } else {
+                    const uint32_t i = W * staged_index<true>(e - r * n, r);
+                    dst[e] = make_float4(rows[i], rows[i + 1u], rows[i + 2u], src[e].w);
+                }
             }
         }
     }
@@ -1900,11 +1962,13 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
                                stream, a, m);
         if (staged)
         {
+            // W floats per staged CV (runtime_search_kernel): 4 with a NoiseFX, else 3
+            const size_t rows_lds = size_t{kWave} * cvs * (plan.noise_count ? 4u : 3u) * sizeof(float);
             if (plan.noise_count)
-                hipLaunchKernelGGL((runtime_search_kernel<true, true>), dim3(a.tile_count), dim3(kWave), tile_lds,
+                hipLaunchKernelGGL((runtime_search_kernel<true, true>), dim3(a.tile_count), dim3(kWave), rows_lds,
                                    stream, a);
             else
-                hipLaunchKernelGGL((runtime_search_kernel<true, false>), dim3(a.tile_count), dim3(kWave), tile_lds,
+                hipLaunchKernelGGL((runtime_search_kernel<true, false>), dim3(a.tile_count), dim3(kWave), rows_lds,
                                    stream, a);
         }
         else
