@@ -1141,20 +1141,29 @@ __device__ __forceinline__ bool scale_in_place(const StagedCurve<ROTATED> &p, ui
 }
 __device__ __forceinline__ bool scale_in_place(const MemoryCurve &, uint32_t, float) { return false; }
 
-// A lane's strand in a runtime-chain tile, W floats per CV, CV-major with
-// the lanes rotated as in staged_index.  W = 3 stages positions only (a
-// 16-CV tile of 64 strands is 12 KB, so 13 tiles share a CU's LDS instead
-// of 10; no effect changes a radius, so radii come from the input points);
-// W = 4 stages whole CVs, one 16-byte LDS access each, for kernels whose
-// register use already keeps fewer tiles resident than LDS would.
+// A lane's strand in a runtime-chain tile, W floats per CV, CV-major: CV c
+// of lane L at float c * RowStride<W> + W * L.  W = 3 stages positions only
+// (a 16-CV tile of 64 strands is 12 KB, so 13 tiles share a CU's LDS
+// instead of 10; no effect changes a radius, so radii come from the input
+// points); W = 4 stages whole CVs, one 16-byte LDS access each, for kernels
+// whose register use already keeps fewer tiles resident than LDS would.
+// The row padding puts the 16 CVs one lane stages in 16 different banks,
+// and a wave-uniform CV index costs one address add per access (a rotated
+// layout would cost three).
+template <uint32_t W>
+constexpr uint32_t RowStride = W == 4u ? 4u * kWave + 4u : W * kWave + 1u;
+template <uint32_t W>
+__device__ __forceinline__ uint32_t row_index(uint32_t c, uint32_t lane) {
+    return c * RowStride<W> + W * lane;
+}
 template <uint32_t W>
 struct StagedRows {
     float *tile;
     uint32_t lane;
-    __device__ __forceinline__ uint32_t at(uint32_t c) const { return W * staged_index<true>(c, lane); }
+    __device__ __forceinline__ uint32_t at(uint32_t c) const { return row_index<W>(c, lane); }
     __device__ __forceinline__ float4 operator[](uint32_t c) const {
-        if constexpr (W == 4u) return reinterpret_cast<const float4 *>(tile)[staged_index<true>(c, lane)];
         const uint32_t i = at(c);
+        if constexpr (W == 4u) return *reinterpret_cast<const float4 *>(tile + i);
         return make_float4(tile[i], tile[i + 1u], tile[i + 2u], 0.0f);
     }
     __device__ __forceinline__ void store(uint32_t c, F3 q) const {
@@ -1178,10 +1187,10 @@ __device__ __forceinline__ void stage_rows(const CutArgs &a, float *tile, uint32
         const uint32_t l = __umulhi(i, a.div_magic);
         const uint32_t c = i - l * a.cvs;
         const float4 v = src[i];
+        const uint32_t j = row_index<W>(c, l);
         if constexpr (W == 4u) {
-            reinterpret_cast<float4 *>(tile)[staged_index<true>(c, l)] = v;
+            *reinterpret_cast<float4 *>(tile + j) = v;
         } else {
-            const uint32_t j = W * staged_index<true>(c, l);
             tile[j] = v.x;
             tile[j + 1u] = v.y;
             tile[j + 2u] = v.z;
@@ -1641,10 +1650,10 @@ __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
             const uint32_t total = count * n;
             for (uint32_t e = lane; e < total; e += kWave) {
                 const uint32_t r = __umulhi(e, a.div_magic);
+                const uint32_t i = row_index<W>(e - r * n, r);
                 if constexpr (W == 4u) {
-                    dst[e] = tile[staged_index<true>(e - r * n, r)];
+                    dst[e] = *reinterpret_cast<const float4 *>(rows + i);
                 } else {
-                    const uint32_t i = W * staged_index<true>(e - r * n, r);
                     dst[e] = make_float4(rows[i], rows[i + 1u], rows[i + 2u], src[e].w);
                 }
             }
@@ -1962,8 +1971,8 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
                                stream, a, m);
         if (staged)
         {
-            // W floats per staged CV (runtime_search_kernel): 4 with a NoiseFX, else 3
-            const size_t rows_lds = size_t{kWave} * cvs * (plan.noise_count ? 4u : 3u) * sizeof(float);
+            // cvs padded rows of W floats per lane (runtime_search_kernel): W = 4 with a NoiseFX, else 3
+            const size_t rows_lds = size_t{cvs} * (plan.noise_count ? RowStride<4> : RowStride<3>) * sizeof(float);
             if (plan.noise_count)
                 hipLaunchKernelGGL((runtime_search_kernel<true, true>), dim3(a.tile_count), dim3(kWave), rows_lds,
                                    stream, a);
