@@ -163,15 +163,28 @@ struct MemoryCurve {
     __device__ __forceinline__ float4 operator[](uint32_t c) const { return strand[c]; }
 };
 
+// Loads in flight per lane while a tile is staged: a loop that stores each
+// load before issuing the next waits one memory round trip per 64 CVs.
+constexpr uint32_t kStageBatch = 8u;
+
 // The tile's `count` strands, read coalesced, into LDS.
 template <bool ROTATED>
 __device__ __forceinline__ void stage_tile(const CutArgs &a, float4 *tile, uint32_t first, uint32_t count) {
     const float4 *src = a.points + uint64_t{first} * a.cvs;
     const uint32_t total = count * a.cvs;
-    for (uint32_t i = threadIdx.x; i < total; i += kWave) {
-        const uint32_t l = __umulhi(i, a.div_magic);
-        const uint32_t c = i - l * a.cvs;
-        tile[staged_index<ROTATED>(c, l)] = src[i];
+    for (uint32_t base = threadIdx.x; base < total; base += kStageBatch * kWave) {
+        // no branches, so every load issues before the first store: a lane
+        // past the end loads and stores the last CV again (the same value
+        // to the same place)
+        float4 v[kStageBatch];
+#pragma unroll
+        for (uint32_t j = 0u; j < kStageBatch; ++j) v[j] = src[min(base + j * kWave, total - 1u)];
+#pragma unroll
+        for (uint32_t j = 0u; j < kStageBatch; ++j) {
+            const uint32_t i = min(base + j * kWave, total - 1u);
+            const uint32_t l = __umulhi(i, a.div_magic);
+            tile[staged_index<ROTATED>(i - l * a.cvs, l)] = v[j];
+        }
     }
     __syncthreads();
 }
@@ -1183,17 +1196,22 @@ template <uint32_t W>
 __device__ __forceinline__ void stage_rows(const CutArgs &a, float *tile, uint32_t first, uint32_t count) {
     const float4 *src = a.points + uint64_t{first} * a.cvs;
     const uint32_t total = count * a.cvs;
-    for (uint32_t i = threadIdx.x; i < total; i += kWave) {
-        const uint32_t l = __umulhi(i, a.div_magic);
-        const uint32_t c = i - l * a.cvs;
-        const float4 v = src[i];
-        const uint32_t j = row_index<W>(c, l);
-        if constexpr (W == 4u) {
-            *reinterpret_cast<float4 *>(tile + j) = v;
-        } else {
-            tile[j] = v.x;
-            tile[j + 1u] = v.y;
-            tile[j + 2u] = v.z;
+    for (uint32_t base = threadIdx.x; base < total; base += kStageBatch * kWave) {
+        float4 v[kStageBatch];  // no branches, as in stage_tile
+#pragma unroll
+        for (uint32_t k = 0u; k < kStageBatch; ++k) v[k] = src[min(base + k * kWave, total - 1u)];
+#pragma unroll
+        for (uint32_t k = 0u; k < kStageBatch; ++k) {
+            const uint32_t i = min(base + k * kWave, total - 1u);
+            const uint32_t l = __umulhi(i, a.div_magic);
+            const uint32_t j = row_index<W>(i - l * a.cvs, l);
+            if constexpr (W == 4u) {
+                *reinterpret_cast<float4 *>(tile + j) = v[k];
+            } else {
+                tile[j] = v[k].x;
+                tile[j + 1u] = v[k].y;
+                tile[j + 2u] = v[k].z;
+            }
         }
     }
     __syncthreads();
@@ -1648,13 +1666,22 @@ __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
             float4 *dst = a.clumped + uint64_t{first} * n;
             const float4 *src = a.points + uint64_t{first} * n;  // the radii
             const uint32_t total = count * n;
-            for (uint32_t e = lane; e < total; e += kWave) {
-                const uint32_t r = __umulhi(e, a.div_magic);
-                const uint32_t i = row_index<W>(e - r * n, r);
-                if constexpr (W == 4u) {
-                    dst[e] = *reinterpret_cast<const float4 *>(rows + i);
-                } else {
-                    dst[e] = make_float4(rows[i], rows[i + 1u], rows[i + 2u], src[e].w);
+            for (uint32_t base = lane; base < total; base += kStageBatch * kWave) {
+                float radius[kStageBatch];
+                if constexpr (W == 3u) {
+#pragma unroll
+                    for (uint32_t k = 0u; k < kStageBatch; ++k) radius[k] = src[min(base + k * kWave, total - 1u)].w;
+                }
+#pragma unroll
+                for (uint32_t k = 0u; k < kStageBatch; ++k) {
+                    const uint32_t e = min(base + k * kWave, total - 1u);  // as in stage_tile
+                    const uint32_t r = __umulhi(e, a.div_magic);
+                    const uint32_t i = row_index<W>(e - r * n, r);
+                    if constexpr (W == 4u) {
+                        dst[e] = *reinterpret_cast<const float4 *>(rows + i);
+                    } else {
+                        dst[e] = make_float4(rows[i], rows[i + 1u], rows[i + 2u], radius[k]);
+                    }
                 }
             }
         }
