@@ -906,6 +906,7 @@ __global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 && CHUNK < 16 ? 3 : 2
             const uint32_t item_first = lo ? scan[lo - 1u] : 0u;
             const Args &a = b.items[item];
             if (item != loaded) {
+#pragma unroll  // four loads in flight, not one round trip each
                 for (uint32_t i = lane; i < NXG_NOISE_GRADIENT_COUNT; i += kWave) {
                     const nxg_vec3 g = a.gradients[i];
                     grad[i] = make_float4(g.x, g.y, g.z, 0.0f);
@@ -1970,7 +1971,15 @@ __global__ void __launch_bounds__(256) cdf_scan_kernel(CdfArgs c) {
 // mode's root on it, as a RootSample.
 __global__ void __launch_bounds__(256) cdf_roots_kernel(Args a, CdfArgs c, nxg_root_sample *roots) {
     __shared__ uint64_t coarse[kCdfCoarseMax];
-    for (uint32_t k = threadIdx.x; k < c.coarse_count; k += 256u) coarse[k] = c.coarse[k];
+    // eight loads in flight per thread: no branches (a thread past the end
+    // copies the last entry again), so all issue before the first store
+    for (uint32_t base = threadIdx.x; base < c.coarse_count; base += 8u * 256u) {
+        uint64_t v[8];
+#pragma unroll
+        for (uint32_t j = 0u; j < 8u; ++j) v[j] = c.coarse[min(base + j * 256u, c.coarse_count - 1u)];
+#pragma unroll
+        for (uint32_t j = 0u; j < 8u; ++j) coarse[min(base + j * 256u, c.coarse_count - 1u)] = v[j];
+    }
     __syncthreads();
     const uint64_t T = c.triangle_count;
     for (uint64_t slot = uint64_t{blockIdx.x} * 256u + threadIdx.x; slot < a.strand_count;
