@@ -521,21 +521,34 @@ __global__ void __launch_bounds__(256) endpoints_kernel(EndpointArgs a) {
     const uint32_t n = a.cvs, m = n + 2u;
     const float4 *src = a.points + uint64_t{first} * n;
     float4 *dst = a.out + uint64_t{first} * m;
-    for (uint32_t e = threadIdx.x; e < count * m; e += blockDim.x) {
-        const uint32_t r = __umulhi(e, a.magic);
-        const uint32_t k = e - r * m;
-        const float4 *p = src + r * n;
-        float4 v;
-        if (k == 0u) {
-            const float4 p0 = p[0], p1 = p[1];
-            v = make_float4(2.0f * p0.x - p1.x, 2.0f * p0.y - p1.y, 2.0f * p0.z - p1.z, p0.w);
-        } else if (k == m - 1u) {
-            const float4 p0 = p[n - 1u], p1 = p[n - 2u];
-            v = make_float4(2.0f * p0.x - p1.x, 2.0f * p0.y - p1.y, 2.0f * p0.z - p1.z, p0.w);
-        } else {
-            v = p[k - 1u];
+    // output k of a strand: its CV k - 1, or at the ends 2 p0 - p1 with
+    // (p0, p1) = (CV 0, CV 1) or (CV n-1, CV n-2).  Five outputs per thread
+    // in flight, branch-free (every lane loads a p1, its own CV again inside
+    // the strand; a lane past the end redoes the last output), so all loads
+    // issue before the first store.
+    constexpr uint32_t kBatch = 5u;
+    const uint32_t total = count * m;
+    for (uint32_t base = threadIdx.x; base < total; base += kBatch * 256u) {
+        float4 p0[kBatch], p1[kBatch];
+#pragma unroll
+        for (uint32_t j = 0u; j < kBatch; ++j) {
+            const uint32_t e = min(base + j * 256u, total - 1u);
+            const uint32_t r = __umulhi(e, a.magic);
+            const uint32_t k = e - r * m;
+            const uint32_t i0 = k == 0u ? 0u : k == m - 1u ? n - 1u : k - 1u;
+            const uint32_t i1 = k == 0u ? 1u : k == m - 1u ? n - 2u : i0;
+            p0[j] = src[r * n + i0];
+            p1[j] = src[r * n + i1];
         }
-        dst[e] = v;
+#pragma unroll
+        for (uint32_t j = 0u; j < kBatch; ++j) {
+            const uint32_t e = min(base + j * 256u, total - 1u);
+            const uint32_t k = e - __umulhi(e, a.magic) * m;
+            const float4 a0 = p0[j], a1 = p1[j];
+            dst[e] = k == 0u || k == m - 1u
+                         ? make_float4(2.0f * a0.x - a1.x, 2.0f * a0.y - a1.y, 2.0f * a0.z - a1.z, a0.w)
+                         : a0;
+        }
     }
     if (a.counts && threadIdx.x < count) a.counts[first + threadIdx.x] = m;
 }
