@@ -2318,7 +2318,7 @@ uint32_t nxg_batch_graph_destroy(void *graph) {
 uint32_t nxg_batch_tuning_candidates(nxg_batch_tuning *candidates, uint32_t capacity) {
     uint32_t n = 0u;
     for (const auto &wc : nxg::kBatchWaveChunk)
-        for (uint32_t claim : {1u, 4u})
+        for (uint32_t claim : {1u, 2u, 4u, 8u})
             for (uint32_t order : {NXG_BATCH_ORDER_GIVEN, NXG_BATCH_ORDER_HEAVY_FIRST}) {
                 if (candidates && n < capacity) candidates[n] = nxg_batch_tuning{wc[0], wc[1], claim, 0u, order};
                 ++n;

@@ -298,7 +298,7 @@ def test_batch_validation():
 def test_batch_default_tuning_is_a_candidate():
     d = batch.default_tuning()
     assert d.as_tuple() in [c.as_tuple() for c in batch.tuning_candidates()]
-    assert len(batch.tuning_candidates()) == nx.lib().nxg_batch_tuning_candidates(None, 0) == 32
+    assert len(batch.tuning_candidates()) == nx.lib().nxg_batch_tuning_candidates(None, 0) == 64
 
 
 # ---- motion samples (device_contract.h:256-288; tests/test_main.cpp:504-527)
