@@ -952,30 +952,58 @@ __device__ __forceinline__ uint32_t order_triangle(const OrderArgs &o, uint32_t 
     return sample_triangle(o.alias, o.seed, o.triangle_count, s);
 }
 
+// Every counting-sort tile kernel runs kOrderThreads threads, each taking
+// kOrderTile / kOrderThreads strands; a thread's triangle draws (alias-table
+// gathers) are all issued before its first histogram update.  A thread past
+// the last strand draws the last strand again and adds nothing.
+constexpr uint32_t kOrderThreads = 1024u;
+constexpr uint32_t kOrderPerThread = kOrderTile / kOrderThreads;
+static_assert(kOrderTile % kOrderThreads == 0u && kOrderBins % kOrderThreads == 0u, "order tile shape");
+
+__device__ __forceinline__ void order_bins(const OrderArgs &o, uint32_t first, uint32_t (&bin)[kOrderPerThread]) {
+#pragma unroll
+    for (uint32_t j = 0; j < kOrderPerThread; ++j) {
+        const uint32_t s = min(first + threadIdx.x + j * kOrderThreads, o.strand_count - 1u);
+        bin[j] = order_triangle(o, s) >> o.shift;
+    }
+}
+
 __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t tile) {
     __shared__ uint32_t hist[kOrderBins];
-    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) hist[b] = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) hist[threadIdx.x + j * kOrderThreads] = 0u;
     __syncthreads();
     const uint32_t first = tile * kOrderTile;
-    for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
-        const uint32_t s = first + i;
-        if (s >= o.strand_count) break;
-        atomicAdd(&hist[order_triangle(o, s) >> o.shift], 1u);
-    }
+    uint32_t bin[kOrderPerThread];
+    order_bins(o, first, bin);
+#pragma unroll
+    for (uint32_t j = 0; j < kOrderPerThread; ++j)
+        atomicAdd(&hist[bin[j]], first + threadIdx.x + j * kOrderThreads < o.strand_count ? 1u : 0u);
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) o.counts[b * o.tiles + tile] = hist[b];
+#pragma unroll
+    for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) {
+        const uint32_t b = threadIdx.x + j * kOrderThreads;
+        o.counts[b * o.tiles + tile] = hist[b];
+    }
 }
 
 __device__ __forceinline__ void order_scatter_tile(const OrderArgs &o, uint32_t tile) {
     __shared__ uint32_t next[kOrderBins];
-    for (uint32_t b = threadIdx.x; b < kOrderBins; b += blockDim.x) next[b] = o.counts[b * o.tiles + tile];
-    __syncthreads();
     const uint32_t first = tile * kOrderTile;
-    for (uint32_t i = threadIdx.x; i < kOrderTile; i += blockDim.x) {
-        const uint32_t s = first + i;
-        if (s >= o.strand_count) break;
-        const uint32_t bin = order_triangle(o, s) >> o.shift;
-        o.order[atomicAdd(&next[bin], 1u)] = s;
+    uint32_t bin[kOrderPerThread];
+    order_bins(o, first, bin);  // in flight with the starts' loads below
+#pragma unroll
+    for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) {
+        const uint32_t b = threadIdx.x + j * kOrderThreads;
+        next[b] = o.counts[b * o.tiles + tile];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kOrderPerThread; ++j) {
+        const uint32_t s = first + threadIdx.x + j * kOrderThreads;
+        const bool live = s < o.strand_count;
+        const uint32_t at = atomicAdd(&next[bin[j]], live ? 1u : 0u);
+        if (live) o.order[at] = s;
     }
 }
 
@@ -1050,8 +1078,10 @@ __device__ __forceinline__ uint32_t order_scan_blocks(const OrderArgs &o) {
     return (order_count_values(o) + kScanBlock - 1u) / kScanBlock;
 }
 
-__global__ void __launch_bounds__(1024) order_count_kernel(OrderArgs o) { order_count_tile(o, blockIdx.x); }
-__global__ void __launch_bounds__(1024) order_scatter_kernel(OrderArgs o) { order_scatter_tile(o, blockIdx.x); }
+__global__ void __launch_bounds__(kOrderThreads) order_count_kernel(OrderArgs o) { order_count_tile(o, blockIdx.x); }
+__global__ void __launch_bounds__(kOrderThreads) order_scatter_kernel(OrderArgs o) {
+    order_scatter_tile(o, blockIdx.x);
+}
 __global__ void __launch_bounds__(256) order_scan_blocks_kernel(OrderArgs o) {
     scan_block(o.counts, order_count_values(o), o.partials, blockIdx.x);
 }
@@ -1079,12 +1109,12 @@ __device__ __forceinline__ uint32_t batch_item_of(const uint32_t *end, uint32_t 
     return item;
 }
 
-__global__ void __launch_bounds__(1024) batch_order_count_kernel(BatchOrderArgs b) {
+__global__ void __launch_bounds__(kOrderThreads) batch_order_count_kernel(BatchOrderArgs b) {
     uint32_t first;
     const uint32_t k = batch_item_of(b.tile_end, b.count, blockIdx.x, first);
     order_count_tile(b.items[k], blockIdx.x - first);
 }
-__global__ void __launch_bounds__(1024) batch_order_scatter_kernel(BatchOrderArgs b) {
+__global__ void __launch_bounds__(kOrderThreads) batch_order_scatter_kernel(BatchOrderArgs b) {
     uint32_t first;
     const uint32_t k = batch_item_of(b.tile_end, b.count, blockIdx.x, first);
     order_scatter_tile(b.items[k], blockIdx.x - first);
@@ -1448,11 +1478,11 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
 // The counting sort of the strands by root triangle (five small kernels).
 uint32_t launch_order(const OrderArgs &o, hipStream_t stream) {
     const uint32_t scan_blocks = (kOrderBins * o.tiles + kScanBlock - 1u) / kScanBlock;
-    hipLaunchKernelGGL(order_count_kernel, dim3(o.tiles), dim3(1024), 0, stream, o);
+    hipLaunchKernelGGL(order_count_kernel, dim3(o.tiles), dim3(kOrderThreads), 0, stream, o);
     hipLaunchKernelGGL(order_scan_blocks_kernel, dim3(scan_blocks), dim3(256), 0, stream, o);
     hipLaunchKernelGGL(order_scan_partials_kernel, dim3(1), dim3(256), 0, stream, o);
     hipLaunchKernelGGL(order_scan_add_kernel, dim3(scan_blocks), dim3(256), 0, stream, o);
-    hipLaunchKernelGGL(order_scatter_kernel, dim3(o.tiles), dim3(1024), 0, stream, o);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(o.tiles), dim3(kOrderThreads), 0, stream, o);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
 
@@ -1751,11 +1781,11 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
 
 uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
     if (b.order_tiles) {
-        hipLaunchKernelGGL(batch_order_count_kernel, dim3(b.order_tiles), dim3(1024), 0, stream, b.order);
+        hipLaunchKernelGGL(batch_order_count_kernel, dim3(b.order_tiles), dim3(kOrderThreads), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scan_blocks_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scan_partials_kernel, dim3(b.order.count), dim3(256), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scan_add_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
-        hipLaunchKernelGGL(batch_order_scatter_kernel, dim3(b.order_tiles), dim3(1024), 0, stream, b.order);
+        hipLaunchKernelGGL(batch_order_scatter_kernel, dim3(b.order_tiles), dim3(kOrderThreads), 0, stream, b.order);
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     }
     // attributes after the order: a surrogate reads its roots in the item's strand order
