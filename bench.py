@@ -1131,8 +1131,10 @@ def classic_cpu_baseline(host, Cv, scale, offset, target_s, threads, use_ref=Fal
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # 100 timed steps of the plane take about 75 ms: enough that the first
+    # step's launch latency (the queue starts empty) is amortised
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="plane",
                     choices=["plane", "plane_cdf", "expr", "surrogate", "surrogate_sweep", "batch",

@@ -1167,45 +1167,31 @@ __device__ __forceinline__ bool scale_in_place(const StagedCurve<ROTATED> &p, ui
 }
 __device__ __forceinline__ bool scale_in_place(const MemoryCurve &, uint32_t, float) { return false; }
 
-// A lane's strand in a runtime-chain tile, W floats per CV, CV-major: CV c
-// of lane L at float c * RowStride<W> + W * L.  W = 3 stages positions only
-// (a 16-CV tile of 64 strands is 12 KB, so 13 tiles share a CU's LDS
-// instead of 10; no effect changes a radius, so radii come from the input
-// points); W = 4 stages whole CVs, one 16-byte LDS access each, for kernels
-// whose register use already keeps fewer tiles resident than LDS would.
-// The row padding puts the 16 CVs one lane stages in 16 different banks,
-// and a wave-uniform CV index costs one address add per access (a rotated
-// layout would cost three).
-template <uint32_t W>
-constexpr uint32_t RowStride = W == 4u ? 4u * kWave + 4u : W * kWave + 1u;
-template <uint32_t W>
-__device__ __forceinline__ uint32_t row_index(uint32_t c, uint32_t lane) {
-    return c * RowStride<W> + W * lane;
-}
-template <uint32_t W>
+// A lane's strand in a runtime-chain tile: positions only, CV-major, CV c
+// of lane L at float c * kRowStride + 3 L.  A 16-CV tile of 64 strands is
+// 12 KB, so 13 tiles share a CU's LDS (16-byte rows would allow 10); no
+// effect changes a radius, so radii come from the input points.  The row
+// padding puts the 16 CVs one lane stages in 16 different banks, and a
+// wave-uniform CV index costs one address add per access (a rotated layout
+// would cost three).
+constexpr uint32_t kRowStride = 3u * kWave + 1u;
+__device__ __forceinline__ uint32_t row_index(uint32_t c, uint32_t lane) { return c * kRowStride + 3u * lane; }
 struct StagedRows {
     float *tile;
     uint32_t lane;
-    __device__ __forceinline__ uint32_t at(uint32_t c) const { return row_index<W>(c, lane); }
     __device__ __forceinline__ float4 operator[](uint32_t c) const {
-        const uint32_t i = at(c);
-        if constexpr (W == 4u) return *reinterpret_cast<const float4 *>(tile + i);
+        const uint32_t i = row_index(c, lane);
         return make_float4(tile[i], tile[i + 1u], tile[i + 2u], 0.0f);
     }
     __device__ __forceinline__ void store(uint32_t c, F3 q) const {
-        const uint32_t i = at(c);
+        const uint32_t i = row_index(c, lane);
         tile[i] = q.x;
         tile[i + 1u] = q.y;
         tile[i + 2u] = q.z;
     }
 };
-template <class T>
-struct IsStagedRows : std::false_type {};
-template <uint32_t W>
-struct IsStagedRows<StagedRows<W>> : std::true_type {};
 
-// The tile's `count` strands, read coalesced, into LDS rows of W floats.
-template <uint32_t W>
+// The tile's `count` strands' positions, read coalesced, into LDS rows.
 __device__ __forceinline__ void stage_rows(const CutArgs &a, float *tile, uint32_t first, uint32_t count) {
     const float4 *src = a.points + uint64_t{first} * a.cvs;
     const uint32_t total = count * a.cvs;
@@ -1217,21 +1203,16 @@ __device__ __forceinline__ void stage_rows(const CutArgs &a, float *tile, uint32
         for (uint32_t k = 0u; k < kStageBatch; ++k) {
             const uint32_t i = min(base + k * kWave, total - 1u);
             const uint32_t l = __umulhi(i, a.div_magic);
-            const uint32_t j = row_index<W>(i - l * a.cvs, l);
-            if constexpr (W == 4u) {
-                *reinterpret_cast<float4 *>(tile + j) = v[k];
-            } else {
-                tile[j] = v[k].x;
-                tile[j + 1u] = v[k].y;
-                tile[j + 2u] = v[k].z;
-            }
+            const uint32_t j = row_index(i - l * a.cvs, l);
+            tile[j] = v[k].x;
+            tile[j + 1u] = v[k].y;
+            tile[j + 2u] = v[k].z;
         }
     }
     __syncthreads();
 }
 
-template <uint32_t W>
-__device__ __forceinline__ bool scale_in_place(const StagedRows<W> &p, uint32_t n, float scale) {
+__device__ __forceinline__ bool scale_in_place(const StagedRows &p, uint32_t n, float scale) {
     const F3 root = xyz(p[0]);
     for (uint32_t c = 0u; c < n; ++c) p.store(c, root + (xyz(p[c]) - root) * scale);
     return true;
@@ -1567,7 +1548,7 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
     const float c_width = width;
     bool clumped = false;  // the context's t is 1: the last effect that set it ran its CV loop
-    constexpr bool kStaged = IsStagedRows<Curve>::value;
+    constexpr bool kStaged = std::is_same<Curve, StagedRows>::value;
     // the effects in list order (:1679-1826); effects that change the CVs
     // run on staged strands (at most 64 CVs), a trailing CutFX also on
     // strands read from memory (the write resamples them)
@@ -1651,24 +1632,25 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
     return keep ? cut : -1.0f;
 }
 
+// With a NoiseFX the kernel is held to 168 VGPRs (3 wavefronts per SIMD,
+// which the 12 KB tiles allow) at the cost of a small register spill:
+// 2.82 ms against 3.27 ms at 195 VGPRs and 2 wavefronts on the classic_noise
+// workload.
 template <bool STAGED, bool NOISE>
-__global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
+__global__ void __launch_bounds__(kWave, NOISE ? 3 : 1) runtime_search_kernel(CutArgs a) {
     extern __shared__ float4 tile[];
     const uint32_t n = a.cvs;
     const uint32_t t = blockIdx.x, lane = threadIdx.x;
     const uint32_t first = t * kWave;
     const uint32_t count = a.strands - first < kWave ? a.strands - first : kWave;
-    // the NoiseFX variant's registers keep fewer tiles resident than whole
-    // CVs in LDS would, so it stages them with their radii
-    constexpr uint32_t W = NOISE ? 4u : 3u;
     float *rows = reinterpret_cast<float *>(tile);
-    if constexpr (STAGED) stage_rows<W>(a, rows, first, count);
+    if constexpr (STAGED) stage_rows(a, rows, first, count);
     const uint32_t s = first + lane;
     float cut = -1.0f;
     uint32_t err = 0u;
     if (lane < count) {
         if constexpr (STAGED)
-            cut = runtime_strand<NOISE>(a, StagedRows<W>{rows, lane}, s, &err);
+            cut = runtime_strand<NOISE>(a, StagedRows{rows, lane}, s, &err);
         else
             cut = runtime_strand<false>(a, MemoryCurve{a.points + uint64_t{s} * n}, s, &err);
         a.cut[s] = cut;
@@ -1681,20 +1663,14 @@ __global__ void __launch_bounds__(kWave) runtime_search_kernel(CutArgs a) {
             const uint32_t total = count * n;
             for (uint32_t base = lane; base < total; base += kStageBatch * kWave) {
                 float radius[kStageBatch];
-                if constexpr (W == 3u) {
 #pragma unroll
-                    for (uint32_t k = 0u; k < kStageBatch; ++k) radius[k] = src[min(base + k * kWave, total - 1u)].w;
-                }
+                for (uint32_t k = 0u; k < kStageBatch; ++k) radius[k] = src[min(base + k * kWave, total - 1u)].w;
 #pragma unroll
                 for (uint32_t k = 0u; k < kStageBatch; ++k) {
                     const uint32_t e = min(base + k * kWave, total - 1u);  // as in stage_tile
                     const uint32_t r = __umulhi(e, a.div_magic);
-                    const uint32_t i = row_index<W>(e - r * n, r);
-                    if constexpr (W == 4u) {
-                        dst[e] = *reinterpret_cast<const float4 *>(rows + i);
-                    } else {
-                        dst[e] = make_float4(rows[i], rows[i + 1u], rows[i + 2u], radius[k]);
-                    }
+                    const uint32_t i = row_index(e - r * n, r);
+                    dst[e] = make_float4(rows[i], rows[i + 1u], rows[i + 2u], radius[k]);
                 }
             }
         }
@@ -2011,8 +1987,7 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
                                stream, a, m);
         if (staged)
         {
-            // cvs padded rows of W floats per lane (runtime_search_kernel): W = 4 with a NoiseFX, else 3
-            const size_t rows_lds = size_t{cvs} * (plan.noise_count ? RowStride<4> : RowStride<3>) * sizeof(float);
+            const size_t rows_lds = size_t{cvs} * kRowStride * sizeof(float);  // runtime_search_kernel's rows
             if (plan.noise_count)
                 hipLaunchKernelGGL((runtime_search_kernel<true, true>), dim3(a.tile_count), dim3(kWave), rows_lds,
                                    stream, a);
