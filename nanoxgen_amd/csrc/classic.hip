@@ -1742,6 +1742,11 @@ __global__ void __launch_bounds__(kWave) runtime_write_kernel(CutArgs a) {
 // The write with each strand held in registers (n <= MAXN), laid out in
 // output order through the tile's LDS and stored as one contiguous run per
 // wave, as cut_write_held_kernel does.
+// CVs per output pass of the held write: a pass compacts every kept
+// strand's next kHeldPass CVs in LDS (rows padded by one float4) and stores
+// them as runs of kHeldPass * 16 bytes.
+constexpr uint32_t kHeldPass = 8u;
+
 template <uint32_t MAXN>
 __global__ void __launch_bounds__(kWave) runtime_write_held_kernel(CutArgs a) {
     extern __shared__ float4 tile[];
@@ -1755,52 +1760,82 @@ __global__ void __launch_bounds__(kWave) runtime_write_held_kernel(CutArgs a) {
     const bool keep = cut >= 0.0f;
     const uint64_t kept = __ballot(keep);
     if (kept == 0u) return;  // the block is this one wavefront
-    stage_tile<false>(a, tile, first, count);
+    // positions-only rows (the tile's 12 KB at 16 CVs let 13 tiles share a
+    // CU, where whole CVs with a whole compaction area would allow 9)
+    float *rows = reinterpret_cast<float *>(tile);
+    stage_rows(a, rows, first, count);
     const uint32_t rank = static_cast<uint32_t>(__popcll(kept & ((uint64_t{1} << lane) - 1u)));
     const uint32_t slot0 = a.groups[t / kScanBlock] + a.tiles[t];
-    float4 held[MAXN];
+    F3 held[MAXN];
     RuntimeState st{};
-    if (keep) {  // positions (the input radii ride along)
+    if (keep) {  // positions
         st = a.state[s];
-        const StagedCurve<false> p{tile, lane};
+        const StagedRows p{rows, lane};
         if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) scale_in_place(p, n, st.length_scale);
         const float step = cut / static_cast<float>(n - 1u);
 #pragma unroll
         for (uint32_t i = 0u; i < MAXN; ++i) {
             if (i >= n) continue;
-            float4 q = p[i];
-            if (a.has_cut && i > 0u) {
-                const F3 r = curve_eval(p, n, step * static_cast<float>(i));
-                q = make_float4(r.x, r.y, r.z, q.w);
-            }
-            held[i] = q;
+            held[i] = a.has_cut && i > 0u ? curve_eval(p, n, step * static_cast<float>(i)) : xyz(p[i]);
         }
     }
+    // the rows are free after this barrier: the kept lanes' ids, then the
+    // compaction area of one pass
     __syncthreads();
-    const uint32_t stride = n + 1u;
-    if (keep) {
-#pragma unroll
-        for (uint32_t i = 0u; i < MAXN; ++i)
-            if (i < n) tile[rank * stride + i] = held[i];
-        if (a.profile) {  // the width profile's radii, in the lane's own LDS row
-            const ClassicContext c = classic_context(a, s);
-            for (uint32_t i = 0u; i < n; ++i) {
-                float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
-                if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
-                    float ramp = 0.0f;
-                    classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
-                    scale *= ramp;
-                }
-                tile[rank * stride + i].w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
-            }
-        }
-    }
-    __syncthreads();
-    const uint32_t total = static_cast<uint32_t>(__popcll(kept)) * n;
+    uint32_t *lane_of = reinterpret_cast<uint32_t *>(rows);
+    float4 *pass = tile + kWave / 4u;
+    constexpr uint32_t kPitch = kHeldPass + 1u;
+    if (keep) lane_of[rank] = lane;
+    const ClassicContext c = keep && a.profile ? classic_context(a, s) : ClassicContext{};
+    const uint32_t kept_count = static_cast<uint32_t>(__popcll(kept));
     float4 *dst = a.out_points + uint64_t{slot0} * n;
-    for (uint32_t e = lane; e < total; e += kWave) {
-        const uint32_t r = __umulhi(e, a.div_magic);
-        dst[e] = tile[r * stride + (e - r * n)];
+    const float4 *src = a.points + uint64_t{first} * n;  // the input radii (no width profile)
+    for (uint32_t c0 = 0u; c0 < n; c0 += kHeldPass) {
+        const uint32_t h = n - c0 < kHeldPass ? n - c0 : kHeldPass;
+        if (c0) __syncthreads();  // the previous pass's reads are done
+        if (keep) {
+#pragma unroll
+            for (uint32_t i = 0u; i < MAXN; ++i)
+                if (i >= c0 && i < c0 + h) pass[rank * kPitch + (i - c0)] = make_float4(held[i].x, held[i].y,
+                                                                                        held[i].z, 0.0f);
+            if (a.profile) {  // the width profile's radii (src/xgen_classic_runtime.cpp:1836-1858)
+                for (uint32_t i = c0; i < c0 + h; ++i) {
+                    float scale = taper_scale(cv_parameter(i, n), st.taper, st.taper_start);
+                    if (has_program(a, NXG_CLASSIC_EXPR_WIDTH_RAMP)) {
+                        float ramp = 0.0f;
+                        classic_value(a, NXG_CLASSIC_EXPR_WIDTH_RAMP, c, st.c_length, st.c_width, i, ramp);
+                        scale *= ramp;
+                    }
+                    pass[rank * kPitch + (i - c0)].w = 0.5f * max_std(st.c_width * scale, 0.0f) * a.radius_scale;
+                }
+            }
+        }
+        __syncthreads();
+        // element e of the pass: strand e / h (f / h for f < 64 * 8 as
+        // floor((f + 0.5) / h) in float, exact as in flush_chunk), CV c0 + e % h;
+        // branch-free with the input radii's loads ahead of the stores
+        const uint32_t total = kept_count * h;
+        const float inv_h = 1.0f / static_cast<float>(h);
+        for (uint32_t base = lane; base < total; base += kStageBatch * kWave) {
+            float radius[kStageBatch];
+            if (!a.profile) {
+#pragma unroll
+                for (uint32_t k = 0u; k < kStageBatch; ++k) {
+                    const uint32_t e = min(base + k * kWave, total - 1u);
+                    const uint32_t r = static_cast<uint32_t>((static_cast<float>(e) + 0.5f) * inv_h);
+                    radius[k] = src[lane_of[r] * n + c0 + (e - r * h)].w;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0u; k < kStageBatch; ++k) {
+                const uint32_t e = min(base + k * kWave, total - 1u);  // past the end: the last element again
+                const uint32_t r = static_cast<uint32_t>((static_cast<float>(e) + 0.5f) * inv_h);
+                const uint32_t j = e - r * h;
+                float4 v = pass[r * kPitch + j];
+                if (!a.profile) v.w = radius[k];
+                dst[r * n + c0 + j] = v;
+            }
+        }
     }
     if (!keep) return;
     const uint32_t slot = slot0 + rank;
@@ -2008,7 +2043,9 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
             w.points = a.clumped;
             w.present &= ~(uint64_t{1} << NXG_CLASSIC_EXPR_LENGTH);
         }
-        const size_t held_lds = size_t{kWave} * (cvs + 1u) * sizeof(float4);
+        // runtime_write_held_kernel: the staged rows, then the kept lanes' ids and one pass's compaction area
+        const size_t held_lds = std::max(size_t{cvs} * kRowStride * sizeof(float),
+                                         kWave * sizeof(uint32_t) + size_t{kWave} * (kHeldPass + 1u) * sizeof(float4));
         if (cvs <= 8u)
             hipLaunchKernelGGL(runtime_write_held_kernel<8>, dim3(a.tile_count), dim3(kWave), held_lds, stream, w);
         else if (cvs <= 16u)
