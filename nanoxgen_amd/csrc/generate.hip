@@ -1779,7 +1779,28 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
     return NXG_OK;
 }
 
-uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
+// Two side streams and their events for a captured batch: the items'
+// exact attribute programs do not read the strand order, so in a graph they
+// run on the side streams beside the order kernels (and beside each other),
+// joining before the persistent kernel.
+struct BatchSides {
+    static constexpr uint32_t kCount = 2u;
+    hipStream_t stream[kCount];
+    hipEvent_t fork, join[kCount];
+};
+
+uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *sides = nullptr) {
+    if (sides) {
+        if (hipEventRecord(sides->fork, stream) != hipSuccess) return NXG_ERROR_HIP;
+        for (uint32_t i = 0; i < BatchSides::kCount; ++i)
+            if (hipStreamWaitEvent(sides->stream[i], sides->fork, 0) != hipSuccess) return NXG_ERROR_HIP;
+        uint32_t next = 0u;
+        for (uint32_t k = 0; k < b.strands.count; ++k) {
+            if (!b.has_attrs[k] || b.items[k].surrogate) continue;
+            if (uint32_t s = launch_attributes(b.attrs[k], sides->stream[next++ % BatchSides::kCount]); s != NXG_OK)
+                return s;
+        }
+    }
     if (b.order_tiles) {
         hipLaunchKernelGGL(batch_order_count_kernel, dim3(b.order_tiles), dim3(kOrderThreads), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scan_blocks_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
@@ -1793,6 +1814,7 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
         if (!b.has_attrs[k]) continue;
         const nxg_batch_item &it = b.items[k];
         if (!it.surrogate) {
+            if (sides) continue;  // already on a side stream
             if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
             continue;
         }
@@ -1814,6 +1836,12 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream) {
         hipLaunchKernelGGL(batch_prep_kernel, dim3(1), dim3(1), 0, stream, b.prep);
     }
     if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
+    if (sides) {
+        for (uint32_t i = 0; i < BatchSides::kCount; ++i)
+            if (hipEventRecord(sides->join[i], sides->stream[i]) != hipSuccess ||
+                hipStreamWaitEvent(stream, sides->join[i], 0) != hipSuccess)
+                return NXG_ERROR_HIP;
+    }
     if (b.strands.total_tiles == 0u) return NXG_OK;
     const nxg_batch_tuning &t = b.tuning;
     const void *kernel = batch_kernel_for(t.waves_per_block, t.chunk_cvs, b.max_octaves);
@@ -2308,17 +2336,28 @@ uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, con
     *graph = nullptr;
     nxg::BatchLaunch b;
     if (uint32_t s = nxg::make_batch(items, count, tuning, counter, b); s != NXG_OK) return s;
+    // the capture stream and the side streams, all at the default priority
     hipStream_t s = nullptr;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NXG_ERROR_HIP;
+    nxg::BatchSides sides{};
+    bool made = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+                hipEventCreateWithFlags(&sides.fork, hipEventDisableTiming) == hipSuccess;
+    for (uint32_t i = 0; made && i < nxg::BatchSides::kCount; ++i)
+        made = hipStreamCreateWithFlags(&sides.stream[i], hipStreamNonBlocking) == hipSuccess &&
+               hipEventCreateWithFlags(&sides.join[i], hipEventDisableTiming) == hipSuccess;
     hipGraph_t g = nullptr;
-    uint32_t status = NXG_OK;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    uint32_t status = made ? NXG_OK : NXG_ERROR_HIP;
+    if (status == NXG_OK && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         status = NXG_ERROR_HIP;
-    } else {
-        status = nxg::launch_batch(b, s);
+    } else if (status == NXG_OK) {
+        status = nxg::launch_batch(b, s, &sides);
         if (hipStreamEndCapture(s, &g) != hipSuccess) status = NXG_ERROR_HIP;
     }
-    (void)hipStreamDestroy(s);
+    for (uint32_t i = 0; i < nxg::BatchSides::kCount; ++i) {
+        if (sides.join[i]) (void)hipEventDestroy(sides.join[i]);
+        if (sides.stream[i]) (void)hipStreamDestroy(sides.stream[i]);
+    }
+    if (sides.fork) (void)hipEventDestroy(sides.fork);
+    if (s) (void)hipStreamDestroy(s);
     hipGraphExec_t exec = nullptr;
     if (status == NXG_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) status = NXG_ERROR_HIP;
     if (status != NXG_OK) {
