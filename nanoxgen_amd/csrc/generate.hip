@@ -1779,12 +1779,13 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
     return NXG_OK;
 }
 
-// Two side streams and their events for a captured batch: the items'
-// exact attribute programs do not read the strand order, so in a graph they
-// run on the side streams beside the order kernels (and beside each other),
-// joining before the persistent kernel.
+// Side streams and their events for a captured batch: the items' exact
+// attribute programs do not read the strand order, so in a graph they run on
+// the side streams beside the order kernels (and beside each other), joining
+// before the persistent kernel.  Three sides and the capture stream use the
+// four hardware queues a process gets by default (a fourth side was slower).
 struct BatchSides {
-    static constexpr uint32_t kCount = 2u;
+    static constexpr uint32_t kCount = 3u;
     hipStream_t stream[kCount];
     hipEvent_t fork, join[kCount];
 };
@@ -1794,11 +1795,23 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
         if (hipEventRecord(sides->fork, stream) != hipSuccess) return NXG_ERROR_HIP;
         for (uint32_t i = 0; i < BatchSides::kCount; ++i)
             if (hipStreamWaitEvent(sides->stream[i], sides->fork, 0) != hipSuccess) return NXG_ERROR_HIP;
-        uint32_t next = 0u;
-        for (uint32_t k = 0; k < b.strands.count; ++k) {
-            if (!b.has_attrs[k] || b.items[k].surrogate) continue;
-            if (uint32_t s = launch_attributes(b.attrs[k], sides->stream[next++ % BatchSides::kCount]); s != NXG_OK)
-                return s;
+        // largest first, each to the side with the fewest roots so far
+        uint64_t load[BatchSides::kCount] = {};
+        bool done[NXG_BATCH_MAX_ITEMS] = {};
+        for (;;) {
+            uint32_t pick = NXG_BATCH_MAX_ITEMS;
+            for (uint32_t k = 0; k < b.strands.count; ++k)
+                if (b.has_attrs[k] && !b.items[k].surrogate && !done[k] &&
+                    (pick == NXG_BATCH_MAX_ITEMS ||
+                     b.items[k].params->strand_count > b.items[pick].params->strand_count))
+                    pick = k;
+            if (pick == NXG_BATCH_MAX_ITEMS) break;
+            done[pick] = true;
+            uint32_t side = 0u;
+            for (uint32_t i = 1; i < BatchSides::kCount; ++i)
+                if (load[i] < load[side]) side = i;
+            load[side] += b.items[pick].params->strand_count;
+            if (uint32_t s = launch_attributes(b.attrs[pick], sides->stream[side]); s != NXG_OK) return s;
         }
     }
     if (b.order_tiles) {
