@@ -190,32 +190,52 @@ __device__ __forceinline__ uint32_t owner(const uint64_t *offs, uint32_t count, 
     return lo;
 }
 
+// Output points in flight per thread: a loop that stores each point before
+// loading the next waits one memory round trip per point.  The batched
+// loops are branch-free: a thread past the tile's end loads and stores its
+// last point again (the same value to the same place).
+constexpr uint32_t kBatch = 8u;
+
 __global__ void __launch_bounds__(kTile) payload_points_kernel(PayloadArgs a) {
     __shared__ uint64_t offs[kTile + 1u];
     if (*a.status) return;
     const uint32_t first = blockIdx.x * kTile;
     const TileRun run = tile_run(a, offs, first);
-    for (uint64_t e = run.first + threadIdx.x; e < run.end; e += kTile) {
+    if (!a.resample) {  // src/curve_payload.cpp:215-246: point e from input point e
+        for (uint64_t base = run.first + threadIdx.x; base < run.end; base += kBatch * kTile) {
+            nxg_vec3 p[kBatch];
+            float w[kBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kBatch; ++j) p[j] = a.points[min(base + j * kTile, run.end - 1u)];
+            if (a.widths) {
+#pragma unroll
+                for (uint32_t j = 0; j < kBatch; ++j) w[j] = a.widths[min(base + j * kTile, run.end - 1u)];
+            } else {
+#pragma unroll
+                for (uint32_t j = 0; j < kBatch; ++j) w[j] = a.constant_width;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kBatch; ++j) {
+                const nxg_vec3 q = transform(a, p[j]);
+                a.out_points[min(base + j * kTile, run.end - 1u)] = {q.x, q.y, q.z, w[j] * a.radius_factor};
+            }
+        }
+    }
+    for (uint64_t e = run.first + threadIdx.x; a.resample && e < run.end; e += kTile) {
         const uint32_t r = owner(offs, run.count, e);
         const uint32_t s = first + r;
         const uint32_t cv = static_cast<uint32_t>(e - offs[r]);
-        nxg_packed_point out;
-        if (!a.resample) {  // src/curve_payload.cpp:215-246
-            const nxg_vec3 p = transform(a, a.points[e]);
-            out = {p.x, p.y, p.z, (a.widths ? a.widths[e] : a.constant_width) * a.radius_factor};
-        } else {  // src/curve_payload.cpp:247-259
-            const uint32_t c0 = in_offset(a, s), count = in_offset(a, s + 1u) - c0;
-            const float t = static_cast<float>(cv) / static_cast<float>(a.resample - 1u);
-            uint32_t i;
-            float f = 0.0f;
-            const nxg_vec3 p = transform(a, sample_point(a.points, c0, count, t, &i, &f));
-            float width = a.constant_width;  // sample_width (src/curve_payload.cpp:84-93)
-            if (a.widths)
-                width = i >= count - 1u ? a.widths[c0 + count - 1u]
-                                        : a.widths[c0 + i] * (1.0f - f) + a.widths[c0 + i + 1u] * f;
-            out = {p.x, p.y, p.z, 0.5f * width * a.radius_scale};
-        }
-        a.out_points[e] = out;
+        // resampled (src/curve_payload.cpp:247-259)
+        const uint32_t c0 = in_offset(a, s), count = in_offset(a, s + 1u) - c0;
+        const float t = static_cast<float>(cv) / static_cast<float>(a.resample - 1u);
+        uint32_t i;
+        float f = 0.0f;
+        const nxg_vec3 p = transform(a, sample_point(a.points, c0, count, t, &i, &f));
+        float width = a.constant_width;  // sample_width (src/curve_payload.cpp:84-93)
+        if (a.widths)
+            width = i >= count - 1u ? a.widths[c0 + count - 1u]
+                                    : a.widths[c0 + i] * (1.0f - f) + a.widths[c0 + i + 1u] * f;
+        a.out_points[e] = {p.x, p.y, p.z, 0.5f * width * a.radius_scale};
     }
     if (threadIdx.x < run.count) {
         const uint32_t s = first + threadIdx.x;
@@ -231,25 +251,54 @@ __global__ void __launch_bounds__(kTile) payload_motion_kernel(PayloadArgs a) {
     const MotionSet m = a.motion[k];
     const uint32_t first = blockIdx.x * kTile;
     const TileRun run = tile_run(a, offs, first);
-    for (uint64_t e = run.first + threadIdx.x; e < run.end; e += kTile) {
-        const uint32_t r = owner(offs, run.count, e);
-        const uint32_t s = first + r;
-        const uint32_t cv = static_cast<uint32_t>(e - offs[r]);
-        const uint32_t out_count = static_cast<uint32_t>(offs[r + 1u] - offs[r]);
-        const uint32_t m0 = motion_offset(m, s), m_count = motion_offset(m, s + 1u) - m0;
-        nxg_vec3 p;
-        if (!a.resample && m_count == out_count) {  // src/curve_payload.cpp:270-272
-            p = m.points[m0 + cv];
-        } else {
-            const float t = static_cast<float>(cv) / static_cast<float>(out_count - 1u);
-            uint32_t i;
-            float f;
-            p = sample_point(m.points, m0, m_count, t, &i, &f);
+    // kBatch output points per thread with their loads in flight together:
+    // first every point's strand, CV and motion offsets (whose loads, when
+    // the set has offsets, would otherwise wait on the point loads before
+    // them), then its two source indices without branches (the copied point
+    // of src/curve_payload.cpp:270-272 loads the same CV twice; sample_point's
+    // clamp at the last CV likewise) and every load, then the stores
+    for (uint64_t base = run.first + threadIdx.x; base < run.end; base += kBatch * kTile) {
+        uint32_t strand[kBatch], cv[kBatch], out_count[kBatch], m0[kBatch], m_count[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint64_t e = min(base + j * kTile, run.end - 1u);  // as in payload_points_kernel
+            const uint32_t r = owner(offs, run.count, e);
+            strand[j] = first + r;
+            cv[j] = static_cast<uint32_t>(e - offs[r]);
+            out_count[j] = static_cast<uint32_t>(offs[r + 1u] - offs[r]);
+            m0[j] = motion_offset(m, strand[j]);
+            m_count[j] = motion_offset(m, strand[j] + 1u) - m0[j];
         }
-        p = transform(a, p);
-        const uint32_t b = s / a.per_batch;
-        const uint64_t b0 = a.batch_points[b], b1 = a.batch_points[b + 1u];
-        a.out_motion[uint64_t{a.motion_count} * b0 + uint64_t{k} * (b1 - b0) + (e - b0)] = p;
+        nxg_vec3 p0[kBatch], p1[kBatch];
+        float frac[kBatch];
+        bool blend[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const bool copy = !a.resample && m_count[j] == out_count[j];
+            // sample_point (src/curve_payload.cpp:73-82) at t = cv / (out_count - 1)
+            const float t = static_cast<float>(cv[j]) / static_cast<float>(out_count[j] - 1u);
+            const float tc = t < 0.0f ? 0.0f : (1.0f < t ? 1.0f : t);
+            const float x = tc * static_cast<float>(m_count[j] - 1u);
+            const uint32_t i = static_cast<uint32_t>(x);
+            const bool last = i >= m_count[j] - 1u;
+            blend[j] = !copy && !last;
+            frac[j] = x - static_cast<float>(i);
+            const uint32_t i0 = copy ? cv[j] : (last ? m_count[j] - 1u : i);
+            const uint32_t i1 = blend[j] ? i + 1u : i0;
+            p0[j] = m.points[m0[j] + i0];
+            p1[j] = m.points[m0[j] + i1];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint64_t e = min(base + j * kTile, run.end - 1u);
+            const uint32_t b = strand[j] / a.per_batch;
+            const uint64_t b0 = a.batch_points[b], b1 = a.batch_points[b + 1u];
+            const float f = frac[j];
+            const nxg_vec3 q = blend[j] ? nxg_vec3{p0[j].x * (1.0f - f) + p1[j].x * f, p0[j].y * (1.0f - f) + p1[j].y * f,
+                                                   p0[j].z * (1.0f - f) + p1[j].z * f}
+                                        : p0[j];
+            a.out_motion[uint64_t{a.motion_count} * b0 + uint64_t{k} * (b1 - b0) + (e - b0)] = transform(a, q);
+        }
     }
 }
 
