@@ -1141,6 +1141,7 @@ typedef struct nxg_classic_plan_info {
     uint32_t trailing_cut;        /* the last effect is a CutFX (the write resamples the cut strands) */
     uint32_t rewrites;            /* some effect other than a trailing CutFX changes the CVs (they need
                                      cvs_per_strand <= NXG_CLUMP_MAX_CVS and the workspace copy) */
+    uint64_t length_readers;      /* bit k: expressions[k] reads $cLength */
 } nxg_classic_plan_info;
 
 /* Validates and packs a plan into a position-independent image (out NULL:

@@ -339,7 +339,7 @@ class ClassicPlanInfo(C.Structure):
                 ("uniform", C.c_uint32), ("clumps_after_cut", C.c_uint32), ("has_noise", C.c_uint32),
                 ("noise_count", C.c_uint32), ("cut_count", C.c_uint32), ("effect_count", C.c_uint32),
                 ("effects", ClassicEffect * CLASSIC_MAX_EFFECTS), ("trailing_cut", C.c_uint32),
-                ("rewrites", C.c_uint32)]
+                ("rewrites", C.c_uint32), ("length_readers", C.c_uint64)]
 
 
 class ClassicClumpBinding(C.Structure):

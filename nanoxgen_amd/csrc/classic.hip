@@ -136,6 +136,8 @@ struct CutArgs {
     uint32_t clump_count;
     const nxg_vec3 *surface_tangents, *noise_domain;
     uint32_t clump_length;       // bit m: module m's amount or clumpScale reads cLength
+    uint32_t defer_length;       // the first effect is a ClumpingFX that re-measures cLength, and no
+                                 // program before its re-measure reads cLength (see runtime_strand)
     ClumpModule clumps[NXG_CLASSIC_MAX_CLUMPS];
     float4 *clumped;             // [strands x cvs] every strand after the last module
     float4 *guide_render;        // [all modules' guides x cvs] render axes
@@ -1517,7 +1519,14 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
         return -1.0f;
     }
     const ClassicContext c = classic_context(a, s);
-    float c_length = spline_length(a.length_samples, source, n);
+    // cLength of the input curve: with defer_length it is measured only if
+    // the first ClumpingFX leaves this strand alone (no guide, mask 0), since
+    // a clump re-measures it and nothing reads it before; the culling test
+    // then uses the chord root-tip, a lower bound of the sampled spline's
+    // polyline, with a margin well above the sum's rounding
+    const float chord = sqrtf(length_squared(xyz(source[n - 1u]) - xyz(source[0])));
+    const bool deferred = a.defer_length && chord > 1.001e-4f;
+    float c_length = deferred ? 0.0f : spline_length(a.length_samples, source, n);
     float length_scale = 1.0f;
     if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) {
         if (!classic_value(a, NXG_CLASSIC_EXPR_LENGTH, c, c_length, 0.0f, 0u, length_scale)) {
@@ -1530,6 +1539,9 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
         }
         c_length *= length_scale;
     }
+    const bool certain = deferred && chord * length_scale > 1.001e-4f;  // cLength >= chord * scale > 1e-4
+    if (deferred && !certain) c_length = spline_length(a.length_samples, source, n) * length_scale;
+    const bool pending = deferred && certain;  // cLength not measured yet
     const bool scaled = has_program(a, NXG_CLASSIC_EXPR_LENGTH) && !scale_in_place(source, n, length_scale);
     const ScaledCurve<Curve> p{source, xyz(source[0]), length_scale, scaled};
     float width;
@@ -1545,7 +1557,7 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
         *err = NXG_CLASSIC_NONFINITE_WIDTH;
         return -1.0f;
     }
-    if (c_length < 1.0e-4f || width < 1.0e-4f) return -1.0f;
+    if ((!pending && c_length < 1.0e-4f) || width < 1.0e-4f) return -1.0f;
     const float c_width = width;
     bool clumped = false;  // the context's t is 1: the last effect that set it ran its CV loop
     constexpr bool kStaged = std::is_same<Curve, StagedRows>::value;
@@ -1588,6 +1600,12 @@ __device__ __forceinline__ float runtime_strand(const CutArgs &a, const Curve &s
                 if (!runtime_clump(a, module, c, source, s, c_length, c_width, clumped, err))
                     return -1.0f;
             }
+        }
+        if (e == 0u && pending && !clumped) {
+            // the clump left the strand as it was: cLength of the input (the
+            // tile may hold it scaled, the memory copy does not), then scaled
+            c_length = spline_length(a.length_samples, MemoryCurve{a.points + uint64_t{s} * n}, n);
+            if (has_program(a, NXG_CLASSIC_EXPR_LENGTH)) c_length *= length_scale;
         }
     }
     float taper = 0.0f, taper_start = 0.0f;
@@ -1987,6 +2005,18 @@ extern "C" uint32_t nxg_classic_runtime_apply(const nxg_classic_curves *curves,
     if (plan.clump_count) {
         a.clump_count = plan.clump_count;
         a.clump_length = plan.clump_length;
+        // cLength's first measure can wait for a leading ClumpingFX when no
+        // program evaluated before its re-measure reads it: the length and
+        // width, and that module's mask, noise, noiseScale, clump and
+        // clumpScale (its frequency and correlation run per guide)
+        if (plan.effect_count && plan.effects[0].type == NXG_CLASSIC_EFFECT_CLUMP) {
+            const uint32_t m0 = plan.effects[0].module;
+            uint64_t early = (one << NXG_CLASSIC_EXPR_LENGTH) | (one << NXG_CLASSIC_EXPR_WIDTH);
+            for (uint32_t r : {NXG_CLASSIC_EXPR_CLUMP_MASK, NXG_CLASSIC_EXPR_CLUMP_NOISE,
+                               NXG_CLASSIC_EXPR_CLUMP_NOISE_SCALE, NXG_CLASSIC_EXPR_CLUMP, NXG_CLASSIC_EXPR_CLUMP_SCALE})
+                early |= one << NXG_CLASSIC_EXPR_CLUMP_OF(m0, r);
+            a.defer_length = (plan.length_readers & plan.present & early) == 0u ? 1u : 0u;
+        }
         uint32_t base = 0u;
         for (uint32_t m = 0; m < plan.clump_count; ++m) {
             const nxg_classic_clump_binding &b = inputs->clumps[m];

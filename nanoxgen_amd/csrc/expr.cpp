@@ -439,6 +439,7 @@ uint32_t nxg_classic_plan_pack(const nxg_classic_plan *plan, void *out, uint64_t
             const uint32_t b = p->input_bindings[i];
             if (b >= NXG_CLASSIC_VAR_COLUMN0 && b - NXG_CLASSIC_VAR_COLUMN0 + 1u > r.columns_required)
                 r.columns_required = b - NXG_CLASSIC_VAR_COLUMN0 + 1u;
+            if (b == NXG_CLASSIC_VAR_CLENGTH) r.length_readers |= uint64_t{1} << k;
             if (b == NXG_CLASSIC_VAR_CLENGTH &&
                 (k == NXG_CLASSIC_EXPR_TAPER || k == NXG_CLASSIC_EXPR_TAPER_START || k == NXG_CLASSIC_EXPR_WIDTH_RAMP))
                 r.final_length = 1u;

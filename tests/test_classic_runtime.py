@@ -207,6 +207,11 @@ def test_effect_list_plan_report():
     # module 1's programs sit in their own slots
     for key in ("noise_mask@1", "noise_preserve_length@1", "cut_amount@1"):
         assert (two.present >> abi.CLASSIC_EXPRS.index(key)) & 1
+    # the programs that read $cLength
+    reads = classic.ClassicPlan(length="1.5", taper="$cLength * 0.1", cut_amount="$cLength * 0.3",
+                                clump=dict(one, clump="clamp($cLength, 0, 1)")).info
+    readers = {k for k in abi.CLASSIC_EXPRS if (reads.length_readers >> abi.CLASSIC_EXPRS.index(k)) & 1}
+    assert readers == {"taper", "cut_amount", "clump"}, readers
 
 
 def test_effect_list_pack_rejects():

@@ -167,6 +167,12 @@ CLUMP_PLANS = [
     dict(width_ramp="$cLength * 0.2",
          clump=dict(mask="$id < 60 ? 0 : 1", clump="clamp($cLength * 0.5, 0, 1)", clumpScale="-0.1",
                     noise="0.5", noiseScale="$cWidth * 20", noiseFrequency="3", noiseCorrelation="0")),
+    # nothing before the clump's re-measure reads cLength, so its first measure
+    # waits: strands without a guide or with mask 0 measure it afterwards
+    # (scaled by the length expression), degenerate ones at once
+    dict(length="0.5 + $__nxg_map_0", cut_amount="$cLength * 0.25 + 0.01",
+         clump=dict(mask="$id < 60 ? 0 : 1", clump="0.6 + 0.2 * rand()", clumpScale="0.1", noise="0.2",
+                    noiseScale="1", noiseFrequency="2", noiseCorrelation="0")),
 ]
 
 
@@ -179,6 +185,8 @@ def test_clump_walks_oracle(torch, orc, cvs):
     n = 64 * 3 + 29
     steps = rng.normal(0, 0.3, size=(n, cvs, 3)).astype(np.float32)
     pts = np.concatenate([np.cumsum(steps, 1), rng.uniform(0.001, 0.02, (n, cvs, 1)).astype(np.float32)], 2)
+    pts[7:9, :, :3] = pts[7:9, :1, :3]  # two degenerate strands: every CV at the root
+    pts[9, :, :3] = pts[9, :1, :3] + np.linspace(0.0, 2e-5, cvs, dtype=np.float32)[:, None]  # shorter than 1e-4
     pts = pts.reshape(-1, 4).astype(np.float32)
     pts[::5 * cvs, 2] = -0.0
     roots = np.zeros(n, abi.ROOT_DTYPE)
