@@ -1787,16 +1787,22 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
 struct BatchSides {
     static constexpr uint32_t kCount = 3u;
     hipStream_t stream[kCount];
-    hipEvent_t fork, join[kCount];
+    hipEvent_t fork, ordered, join[kCount];  // ordered: the strand order is done (surrogate items wait)
 };
 
 uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *sides = nullptr) {
+    uint64_t load[BatchSides::kCount] = {};  // roots given to each side so far
+    auto lightest = [&load]() {
+        uint32_t side = 0u;
+        for (uint32_t i = 1; i < BatchSides::kCount; ++i)
+            if (load[i] < load[side]) side = i;
+        return side;
+    };
     if (sides) {
         if (hipEventRecord(sides->fork, stream) != hipSuccess) return NXG_ERROR_HIP;
         for (uint32_t i = 0; i < BatchSides::kCount; ++i)
             if (hipStreamWaitEvent(sides->stream[i], sides->fork, 0) != hipSuccess) return NXG_ERROR_HIP;
         // largest first, each to the side with the fewest roots so far
-        uint64_t load[BatchSides::kCount] = {};
         bool done[NXG_BATCH_MAX_ITEMS] = {};
         for (;;) {
             uint32_t pick = NXG_BATCH_MAX_ITEMS;
@@ -1807,9 +1813,7 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
                     pick = k;
             if (pick == NXG_BATCH_MAX_ITEMS) break;
             done[pick] = true;
-            uint32_t side = 0u;
-            for (uint32_t i = 1; i < BatchSides::kCount; ++i)
-                if (load[i] < load[side]) side = i;
+            const uint32_t side = lightest();
             load[side] += b.items[pick].params->strand_count;
             if (uint32_t s = launch_attributes(b.attrs[pick], sides->stream[side]); s != NXG_OK) return s;
         }
@@ -1822,7 +1826,9 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
         hipLaunchKernelGGL(batch_order_scatter_kernel, dim3(b.order_tiles), dim3(kOrderThreads), 0, stream, b.order);
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     }
-    // attributes after the order: a surrogate reads its roots in the item's strand order
+    // attributes after the order: a surrogate reads its roots in the item's
+    // strand order (with sides, on a side that waits for the order)
+    bool recorded = false, waited[BatchSides::kCount] = {};
     for (uint32_t k = 0; k < b.strands.count; ++k) {
         if (!b.has_attrs[k]) continue;
         const nxg_batch_item &it = b.items[k];
@@ -1831,14 +1837,24 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
             if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
             continue;
         }
+        hipStream_t on = stream;
+        if (sides) {
+            const uint32_t side = lightest();
+            load[side] += it.params->strand_count;
+            if (!recorded && hipEventRecord(sides->ordered, stream) != hipSuccess) return NXG_ERROR_HIP;
+            recorded = true;
+            if (!waited[side] && hipStreamWaitEvent(sides->stream[side], sides->ordered, 0) != hipSuccess)
+                return NXG_ERROR_HIP;
+            waited[side] = true;
+            on = sides->stream[side];
+        }
         if (uint32_t s = nxg_surrogate_attributes(it.asset, it.params, it.surrogate, it.surrogate_map, it.attributes,
-                                                  it.attribute_capacity, it.workspace, it.workspace_bytes, stream);
+                                                  it.attribute_capacity, it.workspace, it.workspace_bytes, on);
             s != NXG_OK)
             return s;
         if (uint32_t s = nxg_verify_attributes(it.asset, it.params, it.exprs, it.maps, it.map_count, it.attributes,
                                                it.attribute_capacity, it.surrogate_threshold, it.surrogate_stride,
-                                               it.surrogate_flags, it.surrogate_flag_capacity, it.surrogate_stats,
-                                               stream);
+                                               it.surrogate_flags, it.surrogate_flag_capacity, it.surrogate_stats, on);
             s != NXG_OK)
             return s;
     }
@@ -2353,7 +2369,8 @@ uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, con
     hipStream_t s = nullptr;
     nxg::BatchSides sides{};
     bool made = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
-                hipEventCreateWithFlags(&sides.fork, hipEventDisableTiming) == hipSuccess;
+                hipEventCreateWithFlags(&sides.fork, hipEventDisableTiming) == hipSuccess &&
+                hipEventCreateWithFlags(&sides.ordered, hipEventDisableTiming) == hipSuccess;
     for (uint32_t i = 0; made && i < nxg::BatchSides::kCount; ++i)
         made = hipStreamCreateWithFlags(&sides.stream[i], hipStreamNonBlocking) == hipSuccess &&
                hipEventCreateWithFlags(&sides.join[i], hipEventDisableTiming) == hipSuccess;
@@ -2370,6 +2387,7 @@ uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, con
         if (sides.stream[i]) (void)hipStreamDestroy(sides.stream[i]);
     }
     if (sides.fork) (void)hipEventDestroy(sides.fork);
+    if (sides.ordered) (void)hipEventDestroy(sides.ordered);
     if (s) (void)hipStreamDestroy(s);
     hipGraphExec_t exec = nullptr;
     if (status == NXG_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) status = NXG_ERROR_HIP;

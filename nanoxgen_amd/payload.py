@@ -50,13 +50,15 @@ def _strands(d):
 
 
 def curve_payload(base, motion=(), *, max_strands_per_batch=65536, resample_cv_count=0, radius_scale=1.0,
-                  transform=IDENTITY, stream=None):
+                  transform=IDENTITY, stream=None, out=None):
     """Enqueue build_curve_batches on device curves.  base and each motion
     sample: dict of device tensors points [P, 3] float32, counts [N] int32 (or
     "fixed": CVs per curve), optional widths [P], root_uvs [N, 2], roots
     [N, 12]; "constant_width"; motion samples also "time".  Returns a dict of
     device tensors (points [T, 4], point_counts [N], root_uvs or None,
-    motion_points [S * T, 3], batch_points [B + 1] int64, status [1])."""
+    motion_points [S * T, 3], batch_points [B + 1] int64, status [1]).
+    out: a dict an earlier call of the same shapes returned, whose buffers
+    (and workspace) this call reuses instead of allocating."""
     torch = _torch()
     n = _strands(base)
     batches, total = plan(base, motion, max_strands_per_batch=max_strands_per_batch,
@@ -64,19 +66,27 @@ def curve_payload(base, motion=(), *, max_strands_per_batch=65536, resample_cv_c
     dev = base["points"].device
     s = len(motion)
     has_uvs = base.get("root_uvs") is not None or base.get("roots") is not None
-    out = dict(points=torch.empty((max(total, 1), 4), dtype=torch.float32, device=dev),
-               point_counts=torch.empty(max(n, 1), dtype=torch.int32, device=dev),
-               root_uvs=torch.empty((max(n, 1), 2), dtype=torch.float32, device=dev) if has_uvs else None,
-               motion_points=torch.empty((max(total * s, 1), 3), dtype=torch.float32, device=dev),
-               batch_points=torch.empty(batches + 1, dtype=torch.int64, device=dev),
-               status=torch.zeros(1, dtype=torch.int32, device=dev))
+    if out is not None:
+        if (out["points"].shape[0] != max(total, 1) or out["point_counts"].numel() != max(n, 1) or
+                out["motion_points"].shape[0] != max(total * s, 1) or out["batch_points"].numel() != batches + 1 or
+                (out["root_uvs"] is not None) != has_uvs):
+            raise ValueError("curve_payload: out was made for other shapes")
+    else:
+        out = dict(points=torch.empty((max(total, 1), 4), dtype=torch.float32, device=dev),
+                   point_counts=torch.empty(max(n, 1), dtype=torch.int32, device=dev),
+                   root_uvs=torch.empty((max(n, 1), 2), dtype=torch.float32, device=dev) if has_uvs else None,
+                   motion_points=torch.empty((max(total * s, 1), 3), dtype=torch.float32, device=dev),
+                   batch_points=torch.empty(batches + 1, dtype=torch.int64, device=dev),
+                   status=torch.zeros(1, dtype=torch.int32, device=dev))
     t = lambda x: None if x is None else x.data_ptr()
     o = abi.CurvePayloadOutput(points=out["points"].data_ptr(), point_counts=out["point_counts"].data_ptr(),
                                root_uvs=t(out["root_uvs"]), motion_points=t(out["motion_points"]) if s else None,
                                batch_points=out["batch_points"].data_ptr(), status=out["status"].data_ptr(),
                                point_capacity=total, point_count_capacity=n, root_uv_capacity=n if has_uvs else 0,
                                motion_point_capacity=total * s, batch_capacity=batches + 1)
-    ws = torch.empty(int(lib().nxg_curve_payload_workspace_size(n, s)), dtype=torch.uint8, device=dev)
+    ws = out.get("_workspace")
+    if ws is None:
+        ws = torch.empty(int(lib().nxg_curve_payload_workspace_size(n, s)), dtype=torch.uint8, device=dev)
     b = _set(base, n)
     ms = (abi.CurveSet * max(1, s))(*[_set(m, _strands(m)) for m in motion])
     opt = abi.CurvePayloadOptions(max_strands_per_batch, resample_cv_count, radius_scale, (C.c_float * 12)(*transform))

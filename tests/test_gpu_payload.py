@@ -67,6 +67,18 @@ def test_reference_fixtures(torch, fixtures, case):
     assert [b[0] for b in table] == list(want("batch_first"))
     counts = want("point_counts").astype(np.int64)
     assert [b[2] for b in table] == [int(counts[:f].sum()) for f in want("batch_first")]
+    # a second call into the first call's buffers writes the same bytes
+    points, motion_points = out["points"].clone(), out["motion_points"].clone()
+    out["points"].zero_()
+    out["motion_points"].zero_()
+    again = payload.curve_payload(dev(torch, base), [dev(torch, m) for m in motion],
+                                  max_strands_per_batch=opts["max_strands"], resample_cv_count=opts["resample"],
+                                  radius_scale=opts.get("radius_scale", 1.0),
+                                  transform=opts.get("transform", payload.IDENTITY), out=out)
+    assert again is out and int(out["status"].item()) == 0
+    assert torch.equal(out["points"], points)
+    if motion:  # without motion samples the buffer is a one-row placeholder nothing writes
+        assert torch.equal(out["motion_points"], motion_points)
 
 
 def test_roots_give_the_uvs(torch):
