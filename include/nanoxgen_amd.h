@@ -801,10 +801,10 @@ uint32_t nxg_generate_batch(const nxg_batch_item *items, uint32_t count, const n
 
 /* nxg_generate_batch captured once into an instantiated graph (host handle
  * in *graph); replay with nxg_batch_graph_launch on any stream; the items'
- * device buffers must outlive the graph.  In the graph the exact attribute
- * programs, which do not read the strand order, run on three branches beside
- * the order kernels and join before the strand kernel; the outputs are those
- * of nxg_generate_batch. */
+ * device buffers must outlive the graph.  In the graph the items' attribute
+ * work (exact programs; a surrogate's network, in root index order, and its
+ * check) runs on three branches beside the order kernels and joins before
+ * the strand kernel; the outputs are those of nxg_generate_batch. */
 uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning,
                                 uint32_t *counter, void **graph);
 uint32_t nxg_batch_graph_launch(void *graph, void *stream);

@@ -1787,7 +1787,7 @@ uint32_t make_batch(const nxg_batch_item *items, uint32_t count, const nxg_batch
 struct BatchSides {
     static constexpr uint32_t kCount = 3u;
     hipStream_t stream[kCount];
-    hipEvent_t fork, ordered, join[kCount];  // ordered: the strand order is done (surrogate items wait)
+    hipEvent_t fork, join[kCount];
 };
 
 uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *sides = nullptr) {
@@ -1802,12 +1802,14 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
         if (hipEventRecord(sides->fork, stream) != hipSuccess) return NXG_ERROR_HIP;
         for (uint32_t i = 0; i < BatchSides::kCount; ++i)
             if (hipStreamWaitEvent(sides->stream[i], sides->fork, 0) != hipSuccess) return NXG_ERROR_HIP;
-        // largest first, each to the side with the fewest roots so far
+        // largest first, each to the side with the fewest roots so far; a
+        // surrogate item's network reads its roots in index order here (the
+        // same values as in the strand order), so it need not wait for the order
         bool done[NXG_BATCH_MAX_ITEMS] = {};
         for (;;) {
             uint32_t pick = NXG_BATCH_MAX_ITEMS;
             for (uint32_t k = 0; k < b.strands.count; ++k)
-                if (b.has_attrs[k] && !b.items[k].surrogate && !done[k] &&
+                if (b.has_attrs[k] && !done[k] &&
                     (pick == NXG_BATCH_MAX_ITEMS ||
                      b.items[k].params->strand_count > b.items[pick].params->strand_count))
                     pick = k;
@@ -1815,7 +1817,23 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
             done[pick] = true;
             const uint32_t side = lightest();
             load[side] += b.items[pick].params->strand_count;
-            if (uint32_t s = launch_attributes(b.attrs[pick], sides->stream[side]); s != NXG_OK) return s;
+            const nxg_batch_item &it = b.items[pick];
+            if (!it.surrogate) {
+                if (uint32_t s = launch_attributes(b.attrs[pick], sides->stream[side]); s != NXG_OK) return s;
+                continue;
+            }
+            if (uint32_t s = nxg_surrogate_attributes(it.asset, it.params, it.surrogate, it.surrogate_map,
+                                                      it.attributes, it.attribute_capacity, nullptr, 0u,
+                                                      sides->stream[side]);
+                s != NXG_OK)
+                return s;
+            if (uint32_t s = nxg_verify_attributes(it.asset, it.params, it.exprs, it.maps, it.map_count,
+                                                   it.attributes, it.attribute_capacity, it.surrogate_threshold,
+                                                   it.surrogate_stride, it.surrogate_flags,
+                                                   it.surrogate_flag_capacity, it.surrogate_stats,
+                                                   sides->stream[side]);
+                s != NXG_OK)
+                return s;
         }
     }
     if (b.order_tiles) {
@@ -1826,35 +1844,23 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
         hipLaunchKernelGGL(batch_order_scatter_kernel, dim3(b.order_tiles), dim3(kOrderThreads), 0, stream, b.order);
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     }
-    // attributes after the order: a surrogate reads its roots in the item's
-    // strand order (with sides, on a side that waits for the order)
-    bool recorded = false, waited[BatchSides::kCount] = {};
-    for (uint32_t k = 0; k < b.strands.count; ++k) {
+    // without sides, attributes after the order: a surrogate reads its roots
+    // in the item's strand order
+    for (uint32_t k = 0; !sides && k < b.strands.count; ++k) {
         if (!b.has_attrs[k]) continue;
         const nxg_batch_item &it = b.items[k];
         if (!it.surrogate) {
-            if (sides) continue;  // already on a side stream
             if (uint32_t s = launch_attributes(b.attrs[k], stream); s != NXG_OK) return s;
             continue;
         }
-        hipStream_t on = stream;
-        if (sides) {
-            const uint32_t side = lightest();
-            load[side] += it.params->strand_count;
-            if (!recorded && hipEventRecord(sides->ordered, stream) != hipSuccess) return NXG_ERROR_HIP;
-            recorded = true;
-            if (!waited[side] && hipStreamWaitEvent(sides->stream[side], sides->ordered, 0) != hipSuccess)
-                return NXG_ERROR_HIP;
-            waited[side] = true;
-            on = sides->stream[side];
-        }
         if (uint32_t s = nxg_surrogate_attributes(it.asset, it.params, it.surrogate, it.surrogate_map, it.attributes,
-                                                  it.attribute_capacity, it.workspace, it.workspace_bytes, on);
+                                                  it.attribute_capacity, it.workspace, it.workspace_bytes, stream);
             s != NXG_OK)
             return s;
         if (uint32_t s = nxg_verify_attributes(it.asset, it.params, it.exprs, it.maps, it.map_count, it.attributes,
                                                it.attribute_capacity, it.surrogate_threshold, it.surrogate_stride,
-                                               it.surrogate_flags, it.surrogate_flag_capacity, it.surrogate_stats, on);
+                                               it.surrogate_flags, it.surrogate_flag_capacity, it.surrogate_stats,
+                                               stream);
             s != NXG_OK)
             return s;
     }
@@ -2369,8 +2375,7 @@ uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, con
     hipStream_t s = nullptr;
     nxg::BatchSides sides{};
     bool made = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
-                hipEventCreateWithFlags(&sides.fork, hipEventDisableTiming) == hipSuccess &&
-                hipEventCreateWithFlags(&sides.ordered, hipEventDisableTiming) == hipSuccess;
+                hipEventCreateWithFlags(&sides.fork, hipEventDisableTiming) == hipSuccess;
     for (uint32_t i = 0; made && i < nxg::BatchSides::kCount; ++i)
         made = hipStreamCreateWithFlags(&sides.stream[i], hipStreamNonBlocking) == hipSuccess &&
                hipEventCreateWithFlags(&sides.join[i], hipEventDisableTiming) == hipSuccess;
@@ -2387,7 +2392,6 @@ uint32_t nxg_batch_graph_create(const nxg_batch_item *items, uint32_t count, con
         if (sides.stream[i]) (void)hipStreamDestroy(sides.stream[i]);
     }
     if (sides.fork) (void)hipEventDestroy(sides.fork);
-    if (sides.ordered) (void)hipEventDestroy(sides.ordered);
     if (s) (void)hipStreamDestroy(s);
     hipGraphExec_t exec = nullptr;
     if (status == NXG_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) status = NXG_ERROR_HIP;
