@@ -679,8 +679,8 @@ uint32_t nxg_surrogate_attributes(const nxg_device_asset *asset, const nxg_gener
                                   uint64_t attribute_capacity, const void *prepared_workspace,
                                   uint64_t workspace_bytes, void *stream);
 
-/* The exact runtime as judge: evaluates the set's programs exactly for every
- * sample_stride-th root (strands s with s % stride == stride / 2); a 64-root
+/* The surrogate's check against the exact runtime: evaluates the set's
+ * programs exactly for every sample_stride-th root (strands s with s % stride == stride / 2); a 64-root
  * tile any of whose samples differs from `attributes` by more than
  * relative_threshold * max(|exact|, 1e-6) in any attribute is re-evaluated
  * exactly for all its roots.  tile_flags: ceil(N/64) words of device
