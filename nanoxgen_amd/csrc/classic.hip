@@ -1360,7 +1360,7 @@ __device__ __forceinline__ F3 fallback_surface_u(F3 normal) {
 // The NoiseFX step (apply_noise, src/xgen_classic_runtime.cpp:897-1039) on
 // one strand's staged CVs: mask, frequency, correlation and preserveLength
 // at the context's t; the frame transported from the surface normal along
-// the segments (acos, cos and sin as the reference evaluates them); per CV
+// the segments (the rotation's cosine and sine from the clamped dot product); per CV
 // the magnitude at its t and the classic noise of the domain moved along
 // each axis; the preserve-length rescale.  The result is rewritten in the
 // lane's LDS row; c_length follows.  False (with *err) where the reference
@@ -1430,8 +1430,12 @@ __device__ __forceinline__ bool runtime_noise(const CutArgs &a, uint32_t module,
         const float tangent_dot = clamp_std(dot3(prior, next), -1.0f, 1.0f);
         if (als >= 1.0e-12f) {
             const F3 ra = axis * (1.0f / sqrtf(als));
-            const float angle = acosf(tangent_dot);
-            const float cosine = cosf(angle), sine = sinf(angle);
+            // cos(acos c) = c and sin(acos c) = sqrt((1 - c)(1 + c)) for c in
+            // [-1, 1]: the reference's acos, cos and sin of the same angle
+            // (src/xgen_classic_runtime.cpp:968-970) to a few ulps, without
+            // three library calls per CV
+            const float cosine = tangent_dot;
+            const float sine = sqrtf((1.0f - tangent_dot) * (1.0f + tangent_dot));
             const F3 rotated = transported * cosine + cross3(ra, transported) * sine +
                                ra * (dot3(ra, transported) * (1.0f - cosine));
             transported = length_squared(rotated) >= 1.0e-12f ? normalize3(rotated) : F3{0.0f, 0.0f, 0.0f};
