@@ -112,7 +112,8 @@ WorkspaceLayout workspace_layout(uint32_t guide_count, uint32_t cvs_per_strand, 
     l.counts_offset = align256(l.order_offset + uint64_t{strand_count} * 4u);
     const uint64_t counts = uint64_t{kOrderBins} * order_tiles(strand_count);
     l.partials_offset = align256(l.counts_offset + counts * 4u);
-    l.total = align256(l.partials_offset + (counts / kScanBlock + 1u) * 4u);
+    l.keys_offset = align256(l.partials_offset + (counts / kScanBlock + 1u) * 4u);
+    l.total = align256(l.keys_offset + uint64_t{strand_count} * 4u);
     return l;
 }
 

@@ -941,6 +941,7 @@ struct OrderArgs {
     uint32_t shift, tiles;
     uint32_t *counts, *partials, *order;
     uint32_t cdf_key;  // sort by the inverse-CDF target / 2^32 instead (monotone in the triangle drawn)
+    uint32_t *keys;    // per strand: bin << kOrderRankBits | rank within its tile's bin
 };
 
 __device__ __forceinline__ uint32_t order_triangle(const OrderArgs &o, uint32_t s) {
@@ -968,6 +969,12 @@ __device__ __forceinline__ void order_bins(const OrderArgs &o, uint32_t first, u
     }
 }
 
+// A strand's key packs its bin and its rank among its tile's strands of
+// that bin (the histogram count its update returned), so the scatter needs
+// neither the triangle draw again nor atomics.
+constexpr uint32_t kOrderRankBits = 13u;
+static_assert(kOrderTile <= (1u << kOrderRankBits) && kOrderBins <= (1u << (32u - kOrderRankBits)), "order keys");
+
 __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t tile) {
     __shared__ uint32_t hist[kOrderBins];
 #pragma unroll
@@ -977,8 +984,12 @@ __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t ti
     uint32_t bin[kOrderPerThread];
     order_bins(o, first, bin);
 #pragma unroll
-    for (uint32_t j = 0; j < kOrderPerThread; ++j)
-        atomicAdd(&hist[bin[j]], first + threadIdx.x + j * kOrderThreads < o.strand_count ? 1u : 0u);
+    for (uint32_t j = 0; j < kOrderPerThread; ++j) {
+        const uint32_t s = first + threadIdx.x + j * kOrderThreads;
+        const bool live = s < o.strand_count;
+        const uint32_t rank = atomicAdd(&hist[bin[j]], live ? 1u : 0u);
+        if (live) o.keys[s] = bin[j] << kOrderRankBits | rank;
+    }
     __syncthreads();
 #pragma unroll
     for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) {
@@ -988,22 +999,23 @@ __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t ti
 }
 
 __device__ __forceinline__ void order_scatter_tile(const OrderArgs &o, uint32_t tile) {
-    __shared__ uint32_t next[kOrderBins];
+    __shared__ uint32_t start[kOrderBins];
     const uint32_t first = tile * kOrderTile;
-    uint32_t bin[kOrderPerThread];
-    order_bins(o, first, bin);  // in flight with the starts' loads below
+    uint32_t key[kOrderPerThread];  // in flight with the starts' loads below
+#pragma unroll
+    for (uint32_t j = 0; j < kOrderPerThread; ++j)
+        key[j] = o.keys[min(first + threadIdx.x + j * kOrderThreads, o.strand_count - 1u)];
 #pragma unroll
     for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) {
         const uint32_t b = threadIdx.x + j * kOrderThreads;
-        next[b] = o.counts[b * o.tiles + tile];
+        start[b] = o.counts[b * o.tiles + tile];
     }
     __syncthreads();
 #pragma unroll
     for (uint32_t j = 0; j < kOrderPerThread; ++j) {
         const uint32_t s = first + threadIdx.x + j * kOrderThreads;
-        const bool live = s < o.strand_count;
-        const uint32_t at = atomicAdd(&next[bin[j]], live ? 1u : 0u);
-        if (live) o.order[at] = s;
+        if (s < o.strand_count)
+            o.order[start[key[j] >> kOrderRankBits] + (key[j] & ((1u << kOrderRankBits) - 1u))] = s;
     }
 }
 
@@ -1471,6 +1483,7 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
                            reinterpret_cast<uint32_t *>(ws + L.counts_offset),
                            reinterpret_cast<uint32_t *>(ws + L.partials_offset),
                            reinterpret_cast<uint32_t *>(ws + L.order_offset)};
+    plan.order.keys = reinterpret_cast<uint32_t *>(ws + L.keys_offset);
     a.order = plan.ordered ? plan.order.order : nullptr;
     return NXG_OK;
 }

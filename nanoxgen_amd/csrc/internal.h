@@ -30,12 +30,16 @@ uint32_t validate_packed_output(const nxg_generation_params &p, const nxg_packed
 //   bin counts        kOrderBins x order tiles u32: per-tile counting-sort
 //                     histograms, bin-major, scanned in place into starts
 //   scan partials     one u32 per kScanBlock counts
+//   strand keys       strand_count u32: each strand's bin and its rank among
+//                     its tile's strands of that bin (written by the count,
+//                     read by the scatter)
 struct WorkspaceLayout {
     uint64_t guide_state_offset;
     uint64_t guide_offsets_offset;
     uint64_t order_offset;
     uint64_t counts_offset;
     uint64_t partials_offset;
+    uint64_t keys_offset;
     uint64_t total;
 };
 constexpr uint32_t kScanBlock = 1024u;   // values per scan block
