@@ -788,7 +788,7 @@ void nxg_batch_default_tuning(nxg_batch_tuning *tuning);
 /* Validates every item as nxg_generate_packed would, plus the tuning. */
 uint32_t nxg_batch_validate(const nxg_batch_item *items, uint32_t count, const nxg_batch_tuning *tuning);
 
-/* On `stream`: the strand order of every item (five launches over all
+/* On `stream`: the strand order of every item (four launches over all
  * items), the attributes of items that carry exprs (nxg_evaluate_root_
  * attributes, one launch each; or, with a surrogate, the surrogate's
  * prediction in the item's strand order and the exact check and fallback),

@@ -1005,10 +1005,13 @@ __device__ __forceinline__ void order_scatter_tile(const OrderArgs &o, uint32_t 
 #pragma unroll
     for (uint32_t j = 0; j < kOrderPerThread; ++j)
         key[j] = o.keys[min(first + threadIdx.x + j * kOrderThreads, o.strand_count - 1u)];
+    // the scan left each count's exclusive prefix within its kScanBlock
+    // block, and each block's exclusive prefix in partials
 #pragma unroll
     for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) {
         const uint32_t b = threadIdx.x + j * kOrderThreads;
-        start[b] = o.counts[b * o.tiles + tile];
+        const uint32_t e = b * o.tiles + tile;
+        start[b] = o.counts[e] + o.partials[e / kScanBlock];
     }
     __syncthreads();
 #pragma unroll
@@ -1077,14 +1080,6 @@ __device__ __forceinline__ void scan_partials(uint32_t *partials, uint32_t count
     }
 }
 
-__device__ __forceinline__ void scan_add(uint32_t *data, uint32_t n, const uint32_t *partials, uint32_t block) {
-    const uint32_t base = block * kScanBlock + threadIdx.x * 4u;
-    const uint32_t add = partials[block];
-#pragma unroll
-    for (uint32_t j = 0; j < 4u; ++j)
-        if (base + j < n) data[base + j] += add;
-}
-
 __device__ __forceinline__ uint32_t order_count_values(const OrderArgs &o) { return kOrderBins * o.tiles; }
 __device__ __forceinline__ uint32_t order_scan_blocks(const OrderArgs &o) {
     return (order_count_values(o) + kScanBlock - 1u) / kScanBlock;
@@ -1100,11 +1095,8 @@ __global__ void __launch_bounds__(256) order_scan_blocks_kernel(OrderArgs o) {
 __global__ void __launch_bounds__(256) order_scan_partials_kernel(OrderArgs o) {
     scan_partials(o.partials, order_scan_blocks(o));
 }
-__global__ void __launch_bounds__(256) order_scan_add_kernel(OrderArgs o) {
-    scan_add(o.counts, order_count_values(o), o.partials, blockIdx.x);
-}
 
-// The same five steps over every item of a batch: block ranges per item
+// The same four steps over every item of a batch: block ranges per item
 // come from inclusive scans of tiles and of scan blocks.
 struct BatchOrderArgs {
     OrderArgs items[NXG_BATCH_MAX_ITEMS];
@@ -1138,11 +1130,6 @@ __global__ void __launch_bounds__(256) batch_order_scan_blocks_kernel(BatchOrder
 }
 __global__ void __launch_bounds__(256) batch_order_scan_partials_kernel(BatchOrderArgs b) {
     scan_partials(b.items[blockIdx.x].partials, order_scan_blocks(b.items[blockIdx.x]));
-}
-__global__ void __launch_bounds__(256) batch_order_scan_add_kernel(BatchOrderArgs b) {
-    uint32_t first;
-    const uint32_t k = batch_item_of(b.scan_end, b.count, blockIdx.x, first);
-    scan_add(b.items[k].counts, order_count_values(b.items[k]), b.items[k].partials, blockIdx.x - first);
 }
 
 // ---- float expression runtime ------------------------------------------
@@ -1488,13 +1475,12 @@ uint32_t make_plan(const nxg_device_asset *asset, const nxg_deformed_geometry *d
     return NXG_OK;
 }
 
-// The counting sort of the strands by root triangle (five small kernels).
+// The counting sort of the strands by root triangle (four small kernels).
 uint32_t launch_order(const OrderArgs &o, hipStream_t stream) {
     const uint32_t scan_blocks = (kOrderBins * o.tiles + kScanBlock - 1u) / kScanBlock;
     hipLaunchKernelGGL(order_count_kernel, dim3(o.tiles), dim3(kOrderThreads), 0, stream, o);
     hipLaunchKernelGGL(order_scan_blocks_kernel, dim3(scan_blocks), dim3(256), 0, stream, o);
     hipLaunchKernelGGL(order_scan_partials_kernel, dim3(1), dim3(256), 0, stream, o);
-    hipLaunchKernelGGL(order_scan_add_kernel, dim3(scan_blocks), dim3(256), 0, stream, o);
     hipLaunchKernelGGL(order_scatter_kernel, dim3(o.tiles), dim3(kOrderThreads), 0, stream, o);
     return hipGetLastError() == hipSuccess ? NXG_OK : NXG_ERROR_HIP;
 }
@@ -1853,7 +1839,6 @@ uint32_t launch_batch(BatchLaunch &b, hipStream_t stream, const BatchSides *side
         hipLaunchKernelGGL(batch_order_count_kernel, dim3(b.order_tiles), dim3(kOrderThreads), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scan_blocks_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scan_partials_kernel, dim3(b.order.count), dim3(256), 0, stream, b.order);
-        hipLaunchKernelGGL(batch_order_scan_add_kernel, dim3(b.order_scan_blocks), dim3(256), 0, stream, b.order);
         hipLaunchKernelGGL(batch_order_scatter_kernel, dim3(b.order_tiles), dim3(kOrderThreads), 0, stream, b.order);
         if (hipGetLastError() != hipSuccess) return NXG_ERROR_HIP;
     }
