@@ -929,10 +929,10 @@ __global__ void __launch_bounds__(WAVES * 64, (OCTMAX <= 2 && CHUNK < 16 ? 3 : 2
 // so their guide-offset reads hit the same cache lines (and their root and
 // noise-lattice gathers cluster too).  A counting sort into kOrderBins bins
 // of 2^shift consecutive triangles, with no device-wide atomics: each tile
-// of kOrderTile strands histograms its bins in LDS, the per-tile counts are
-// scanned bin-major, and each tile scatters its strands from its LDS copy of
-// the starts.  The order within a bin depends on LDS atomic timing, which
-// changes no output.
+// of kOrderTile strands histograms its bins in LDS (stored tile-major), the
+// counts are scanned in bin-major order, and each tile places its strands
+// from its LDS copy of the starts.  The order within a bin depends on LDS
+// atomic timing, which changes no output.
 
 struct OrderArgs {
     const nxg_alias_entry *alias;
@@ -994,7 +994,7 @@ __device__ __forceinline__ void order_count_tile(const OrderArgs &o, uint32_t ti
 #pragma unroll
     for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) {
         const uint32_t b = threadIdx.x + j * kOrderThreads;
-        o.counts[b * o.tiles + tile] = hist[b];
+        o.counts[tile * kOrderBins + b] = hist[b];
     }
 }
 
@@ -1006,12 +1006,11 @@ __device__ __forceinline__ void order_scatter_tile(const OrderArgs &o, uint32_t 
     for (uint32_t j = 0; j < kOrderPerThread; ++j)
         key[j] = o.keys[min(first + threadIdx.x + j * kOrderThreads, o.strand_count - 1u)];
     // the scan left each count's exclusive prefix within its kScanBlock
-    // block, and each block's exclusive prefix in partials
+    // block of the bin-major order, and each block's exclusive prefix in partials
 #pragma unroll
     for (uint32_t j = 0; j < kOrderBins / kOrderThreads; ++j) {
         const uint32_t b = threadIdx.x + j * kOrderThreads;
-        const uint32_t e = b * o.tiles + tile;
-        start[b] = o.counts[e] + o.partials[e / kScanBlock];
+        start[b] = o.counts[tile * kOrderBins + b] + o.partials[(b * o.tiles + tile) / kScanBlock];
     }
     __syncthreads();
 #pragma unroll
@@ -1045,27 +1044,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *w
     return before + inc - v;
 }
 
-// data[0..n): exclusive scan of block `block` of kScanBlock values (4 per
-// thread); partials[block] = the block's total.
-__device__ __forceinline__ void scan_block(uint32_t *data, uint32_t n, uint32_t *partials, uint32_t block) {
-    __shared__ uint32_t wave_totals[256 / kWave];
-    const uint32_t base = block * kScanBlock + threadIdx.x * 4u;
-    uint32_t v[4], sum = 0u;
-#pragma unroll
-    for (uint32_t j = 0; j < 4u; ++j) {
-        v[j] = base + j < n ? data[base + j] : 0u;
-        sum += v[j];
-    }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan(sum, wave_totals, total);
-#pragma unroll
-    for (uint32_t j = 0; j < 4u; ++j) {
-        if (base + j < n) data[base + j] = run;
-        run += v[j];
-    }
-    if (threadIdx.x == 0u) partials[block] = total;
-}
-
 // Exclusive scan of the block totals in place, 256 at a time (one block).
 __device__ __forceinline__ void scan_partials(uint32_t *partials, uint32_t count) {
     __shared__ uint32_t wave_totals[256 / kWave];
@@ -1089,9 +1067,37 @@ __global__ void __launch_bounds__(kOrderThreads) order_count_kernel(OrderArgs o)
 __global__ void __launch_bounds__(kOrderThreads) order_scatter_kernel(OrderArgs o) {
     order_scatter_tile(o, blockIdx.x);
 }
-__global__ void __launch_bounds__(256) order_scan_blocks_kernel(OrderArgs o) {
-    scan_block(o.counts, order_count_values(o), o.partials, blockIdx.x);
+// The counts are stored tile-major (bin b of tile t at t * kOrderBins + b, so
+// each tile writes its histogram and reads its starts as one run) and
+// scanned in bin-major order (element e = b * tiles + t): block `block` of
+// kScanBlock elements, exclusive within the block, and the block's total.
+__device__ __forceinline__ void scan_order_block(const OrderArgs &o, uint32_t block) {
+    __shared__ uint32_t wave_totals[256 / kWave];
+    const uint32_t n = order_count_values(o);
+    const uint32_t base = block * kScanBlock + threadIdx.x * 4u;
+    uint32_t b = base / o.tiles, t = base - b * o.tiles;  // element base, then along the bin-major order
+    uint32_t at[4], v[4], sum = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        at[j] = t * kOrderBins + b;
+        v[j] = base + j < n ? o.counts[at[j]] : 0u;
+        sum += v[j];
+        if (++t == o.tiles) {
+            t = 0u;
+            ++b;
+        }
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, wave_totals, total);
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        if (base + j < n) o.counts[at[j]] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0u) o.partials[block] = total;
 }
+
+__global__ void __launch_bounds__(256) order_scan_blocks_kernel(OrderArgs o) { scan_order_block(o, blockIdx.x); }
 __global__ void __launch_bounds__(256) order_scan_partials_kernel(OrderArgs o) {
     scan_partials(o.partials, order_scan_blocks(o));
 }
@@ -1126,7 +1132,7 @@ __global__ void __launch_bounds__(kOrderThreads) batch_order_scatter_kernel(Batc
 __global__ void __launch_bounds__(256) batch_order_scan_blocks_kernel(BatchOrderArgs b) {
     uint32_t first;
     const uint32_t k = batch_item_of(b.scan_end, b.count, blockIdx.x, first);
-    scan_block(b.items[k].counts, order_count_values(b.items[k]), b.items[k].partials, blockIdx.x - first);
+    scan_order_block(b.items[k], blockIdx.x - first);
 }
 __global__ void __launch_bounds__(256) batch_order_scan_partials_kernel(BatchOrderArgs b) {
     scan_partials(b.items[blockIdx.x].partials, order_scan_blocks(b.items[blockIdx.x]));

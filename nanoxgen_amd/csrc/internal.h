@@ -28,7 +28,8 @@ uint32_t validate_packed_output(const nxg_generation_params &p, const nxg_packed
 //   guide offsets     guide_count x cvs_per_strand float4
 //   strand order      strand_count u32: strands grouped by root triangle
 //   bin counts        kOrderBins x order tiles u32: per-tile counting-sort
-//                     histograms, bin-major, scanned in place into starts
+//                     histograms, tile-major, scanned in place in bin-major
+//                     order into starts
 //   scan partials     one u32 per kScanBlock counts
 //   strand keys       strand_count u32: each strand's bin and its rank among
 //                     its tile's strands of that bin (written by the count,
