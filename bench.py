@@ -1170,6 +1170,8 @@ def main():
     ap.add_argument("--sample-stride", type=int, default=64, help="surrogate: exact check of every n-th root")
     ap.add_argument("--train-epochs", type=int, default=200, help="surrogate: epochs over the 2^20 training pairs")
     ap.add_argument("--no-surrogate", action="store_true", help="plane: leave out the surrogate record")
+    ap.add_argument("--serial-steps", action="store_true",
+                    help="plane: run the steps one after another on one stream (no overlapped preparation)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="instead of the benchmark, run a plain device copy of this many MiB "
@@ -1232,7 +1234,20 @@ def main():
         maps = expr.TexelMaps(w.info["maps"])
         attrs = torch.empty((N, 4), dtype=torch.float32, device="cuda")
 
+    # plane: consecutive steps through GenerationPipeline, each step's
+    # preparation (guide state, offset table, strand order) on a side stream
+    # overlapping the previous step's strand kernel; every step runs every
+    # kernel.  The serial schedule is timed beside it for the record.
+    pipe = None
+    if attrs is None and not args.serial_steps:
+        from nanoxgen_amd.pipeline import GenerationPipeline
+        ws = None
+        pipe = GenerationPipeline(asset, w.params, octaves=w.octaves, stream=stream)
+
     def step(ev=None):
+        if pipe is not None:
+            pipe.submit(out, events=ev)
+            return
         if ev:
             ev[0].record(stream)
         if attrs is not None:
@@ -1241,27 +1256,49 @@ def main():
             nx.prepare_packed(asset, w.params, ws, stream=stream)
         if ev:
             ev[1].record(stream)
+            ev[2].record(stream)
         if attrs is not None:
             nx.generate_packed(asset, w.params, out, octaves=w.octaves, workspace=ws, stream=stream,
                                attributes=attrs)
         else:
             nx.generate_packed_prepared(asset, w.params, out, ws, octaves=w.octaves, stream=stream)
         if ev:
-            ev[2].record(stream)
+            ev[3].record(stream)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(events[i])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    def run_steps(k, events=None):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(k):
+            step(events[i] if events else None)
+        if pipe is not None:
+            pipe.drain()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    run_steps(args.warmup)
+    # timing events on every step, or (pipelined) on every tenth: an event
+    # pair on each stream at every step takes away part of the overlap
+    every = 10 if pipe is not None and args.steps >= 10 else 1
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] if i % every == 0 else None
+              for i in range(args.steps)]
+    ms_per_step = run_steps(args.steps, events)
+    events = [e for e in events if e]
+    serial_ms = None
+    if pipe is not None:
+        # the same steps one after another on one stream (not the measured value)
+        workspace = pipe.workspaces[0]
+
+        def serial_step():
+            nx.prepare_packed(asset, w.params, workspace, stream=stream)
+            nx.generate_packed_prepared(asset, w.params, out, workspace, octaves=w.octaves, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            serial_step()
+        torch.cuda.synchronize()
+        serial_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     prep_ms = [e[0].elapsed_time(e[1]) for e in events]
-    kern_ms = [e[1].elapsed_time(e[2]) for e in events]  # the strand kernel (expr: + guide prep and order)
+    kern_ms = [e[2].elapsed_time(e[3]) for e in events]  # the strand kernel (expr: + guide prep and order)
     kern_ms_sorted = sorted(kern_ms)
     kernel_ms = sum(kern_ms) / len(kern_ms)
     points = N * Cv
@@ -1286,6 +1323,9 @@ def main():
                 ("attributes_ms_mean" if attrs is not None else "prep_and_order_ms_mean"):
                     round(sum(prep_ms) / len(prep_ms), 4),
                 "algorithmic_bytes_per_launch": alg_bytes, "valu": valu}
+    if serial_ms is not None:
+        roofline["timed_launches"] = f"every {every}th step of the timed region"
+        roofline["serial_ms_per_step"] = round(serial_ms, 4)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(blob, w, args.cpu_seconds, baseline_threads())
@@ -1304,7 +1344,9 @@ def main():
                        "stencil_k": 8 if args.workload == "plane" else 4,
                        "octaves": ([[w.octaves.amplitude[i], w.octaves.frequency[i]] for i in range(w.octaves.count)]
                                    if w.octaves else [[w.params.noise_amplitude, w.params.noise_frequency]]),
-                       "attribute_programs": len(w.info.get("programs", [])), "parallelism": "single-gpu"},
+                       "attribute_programs": len(w.info.get("programs", [])), "parallelism": "single-gpu",
+                       "schedule": ("pipelined: each step's preparation on a side stream overlaps the previous "
+                                    "step's strand kernel" if pipe is not None else "serial")},
             "roofline": roofline, "cpu_baseline": cpu, "surrogate": sur}
     print(json.dumps(line), flush=True)
     return 0
