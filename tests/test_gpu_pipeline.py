@@ -89,3 +89,44 @@ def test_pipeline_rejects_a_larger_call(torch, plane):
     with pytest.raises(ValueError, match="workspace bytes"):
         pipe.submit(out, big)
     assert C.sizeof(big) == C.sizeof(w.params)
+
+
+def test_pipeline_deformed_frames_with_after(torch, plane):
+    """Frames of a deformed patch written into one device buffer on the
+    caller's stream between calls: each call's preparation follows the write
+    through `after` and every frame equals a serial call on its own copy."""
+    import numpy as np
+    w, asset = plane
+    blob = nx.build_asset(w.build)
+    h = abi.header_of(blob)
+    rest = blob[h.positions_offset:h.positions_offset + 12 * h.vertex_count].view(np.float32).reshape(-1, 3)
+    frames = [(rest + np.float32(0.02 * (f + 1)) * np.sin(rest[:, [1, 2, 0]] * np.float32(3 + f))).astype(np.float32)
+              for f in range(4)]
+    src = [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in frames]
+
+    def desc(t):
+        return abi.DeformedGeometry(positions=t.data_ptr(), position_count=t.shape[0])
+
+    want = []
+    for t in src:
+        out = nx.PackedOutputs(w.params.strand_count, w.params.cvs_per_strand, roots=False, root_uvs=False,
+                               point_counts=True)
+        nx.generate_packed(asset, w.params, out, octaves=w.octaves, deformed=desc(t))
+        want.append(out.points.clone())
+    torch.cuda.synchronize()
+    assert not torch.equal(want[0], want[1])
+    buf = torch.empty_like(src[0])
+    pipe = GenerationPipeline(asset, w.params, octaves=w.octaves, deformed=desc(buf))
+    outs = []
+    for t in src:
+        buf.copy_(t)  # on the caller's stream, after the previous strand kernel that read buf
+        written = torch.cuda.Event()
+        written.record(pipe.stream)
+        o = nx.PackedOutputs(w.params.strand_count, w.params.cvs_per_strand, roots=False, root_uvs=False,
+                             point_counts=True)
+        pipe.submit(o, after=written)
+        outs.append(o)
+    pipe.drain()
+    torch.cuda.synchronize()
+    for f, (o, p) in enumerate(zip(outs, want)):
+        assert torch.equal(o.points, p), f"frame {f} differs from its serial call"
