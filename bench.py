@@ -1239,10 +1239,14 @@ def main():
     # overlapping the previous step's strand kernel; every step runs every
     # kernel.  The serial schedule is timed beside it for the record.
     pipe = None
-    if attrs is None and not args.serial_steps:
+    if not args.serial_steps:
         from nanoxgen_amd.pipeline import GenerationPipeline
         ws = None
-        pipe = GenerationPipeline(asset, w.params, octaves=w.octaves, stream=stream)
+        attribute_fn = None
+        if attrs is not None:  # expr: each step's attributes beside its preparation, on the side stream
+            def attribute_fn(params, out_attrs, side):
+                expr.evaluate_root_attributes(asset, params, exprs, maps, out_attrs, stream=side)
+        pipe = GenerationPipeline(asset, w.params, octaves=w.octaves, stream=stream, attribute_fn=attribute_fn)
 
     def step(ev=None):
         if pipe is not None:
@@ -1289,8 +1293,11 @@ def main():
         workspace = pipe.workspaces[0]
 
         def serial_step():
+            if attrs is not None:
+                expr.evaluate_root_attributes(asset, w.params, exprs, maps, attrs, stream=stream)
             nx.prepare_packed(asset, w.params, workspace, stream=stream)
-            nx.generate_packed_prepared(asset, w.params, out, workspace, octaves=w.octaves, stream=stream)
+            nx.generate_packed_prepared(asset, w.params, out, workspace, octaves=w.octaves, stream=stream,
+                                        attributes=attrs)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -1316,11 +1323,13 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": ("strand_kernel<true, %d>" % (w.octaves.count if w.octaves else 1)
-                           if attrs is None else "guide_prep_kernel + strand order + strand_kernel<true, 1>"),
+                           if attrs is None or pipe is not None
+                           else "guide_prep_kernel + strand order + strand_kernel<true, 1>"),
                 "kernel_ms_mean": round(kernel_ms, 4),
                 "kernel_ms_median": round(kern_ms_sorted[len(kern_ms) // 2], 4),
                 "kernel_ms_p90": round(kern_ms_sorted[min(len(kern_ms) - 1, int(0.9 * len(kern_ms)))], 4),
-                ("attributes_ms_mean" if attrs is not None else "prep_and_order_ms_mean"):
+                ("prep_and_order_ms_mean" if attrs is None else
+                 "attributes_prep_and_order_ms_mean" if pipe is not None else "attributes_ms_mean"):
                     round(sum(prep_ms) / len(prep_ms), 4),
                 "algorithmic_bytes_per_launch": alg_bytes, "valu": valu}
     if serial_ms is not None:
@@ -1345,8 +1354,10 @@ def main():
                        "octaves": ([[w.octaves.amplitude[i], w.octaves.frequency[i]] for i in range(w.octaves.count)]
                                    if w.octaves else [[w.params.noise_amplitude, w.params.noise_frequency]]),
                        "attribute_programs": len(w.info.get("programs", [])), "parallelism": "single-gpu",
-                       "schedule": ("pipelined: each step's preparation on a side stream overlaps the previous "
-                                    "step's strand kernel" if pipe is not None else "serial")},
+                       "schedule": (("pipelined: each step's %s on a side stream overlaps the previous "
+                                     "step's strand kernel" % ("attributes and preparation" if attrs is not None
+                                                               else "preparation"))
+                                    if pipe is not None else "serial")},
             "roofline": roofline, "cpu_baseline": cpu, "surrogate": sur}
     print(json.dumps(line), flush=True)
     return 0

@@ -30,7 +30,7 @@ class GenerationPipeline:
     `after` event of its submit(): work that rewrites the deformed geometry
     between calls passes its event there."""
 
-    def __init__(self, asset, params, *, octaves=None, deformed=None, depth=2, stream=None):
+    def __init__(self, asset, params, *, octaves=None, deformed=None, depth=2, stream=None, attribute_fn=None):
         torch = _torch()
         if depth < 1:
             raise ValueError("GenerationPipeline: depth must be at least 1")
@@ -40,6 +40,12 @@ class GenerationPipeline:
         self.capacity = max(1, asset.workspace_bytes(params))
         self.workspaces = [torch.empty(self.capacity, dtype=torch.uint8, device=asset.tensor.device)
                            for _ in range(depth)]
+        # per-root attribute multipliers (nxg_generate_packed_attributed):
+        # attribute_fn(params, attributes, stream) fills slot k's float32
+        # [N, 4] buffer on the side stream before the preparation
+        self.attribute_fn = attribute_fn
+        self.attributes = ([torch.empty((params.strand_count, 4), dtype=torch.float32, device=asset.tensor.device)
+                            for _ in range(depth)] if attribute_fn is not None else None)
         self.ready = [torch.cuda.Event() for _ in range(depth)]
         self.done = [torch.cuda.Event() for _ in range(depth)]  # after the last strand kernel reading slot k
         self.calls = 0
@@ -49,7 +55,9 @@ class GenerationPipeline:
     def submit(self, outputs, params=None, *, after=None, events=None):
         """Enqueue one generate_packed call writing `outputs` (PackedOutputs).
         `params` (default the constructor's) may change the seed and the
-        other generation parameters.  `after` (optional torch event): work the
+        other generation parameters.  With an attribute_fn, the call's
+        attributes are evaluated into slot i % depth on the side stream
+        before its preparation.  `after` (optional torch event): work the
         preparation must follow.  `events` (optional): four torch events
         recorded around the preparation (side stream) and the strand kernel
         (caller's stream)."""
@@ -66,6 +74,13 @@ class GenerationPipeline:
             self.side.wait_event(after)
         if events:
             events[0].record(self.side)
+        attributes = None
+        if self.attribute_fn is not None:
+            if params.strand_count > self.attributes[k].shape[0]:
+                raise ValueError(f"GenerationPipeline: {params.strand_count} strands, the attribute slots hold "
+                                 f"{self.attributes[k].shape[0]}")
+            attributes = self.attributes[k][:params.strand_count]
+            self.attribute_fn(params, attributes, self.side)
         prepare_packed(self.asset, params, ws, deformed=self.deformed, stream=self.side)
         if events:
             events[1].record(self.side)
@@ -74,7 +89,7 @@ class GenerationPipeline:
         if events:
             events[2].record(self.stream)
         generate_packed_prepared(self.asset, params, outputs, ws, octaves=self.octaves, deformed=self.deformed,
-                                 stream=self.stream)
+                                 stream=self.stream, attributes=attributes)
         if events:
             events[3].record(self.stream)
         self.done[k].record(self.stream)

@@ -130,3 +130,38 @@ def test_pipeline_deformed_frames_with_after(torch, plane):
     torch.cuda.synchronize()
     for f, (o, p) in enumerate(zip(outs, want)):
         assert torch.equal(o.points, p), f"frame {f} differs from its serial call"
+
+
+def test_pipeline_with_attribute_programs(torch):
+    """Expression attributes evaluated per call into the pipeline's slot on
+    the side stream: every call equals attributes + generate_packed run
+    serially, for several seeds in flight."""
+    from nanoxgen_amd import expr
+    w = workloads.config3(strands=40_000)
+    asset = nx.DeviceAsset(nx.build_asset(w.build))
+    es, maps = expr.ExprSet(w.info["programs"]), expr.TexelMaps(w.info["maps"])
+    seeds = [3, 4, 5, 6]
+    want = []
+    for s in seeds:
+        p = with_seed(w.params, s)
+        attrs = torch.empty((p.strand_count, 4), dtype=torch.float32, device="cuda")
+        expr.evaluate_root_attributes(asset, p, es, maps, attrs)
+        out = nx.PackedOutputs(p.strand_count, p.cvs_per_strand, roots=False, root_uvs=False, point_counts=True)
+        nx.generate_packed(asset, p, out, octaves=w.octaves, attributes=attrs)
+        want.append(out.points.clone())
+    torch.cuda.synchronize()
+
+    def attribute_fn(params, out_attrs, side):
+        expr.evaluate_root_attributes(asset, params, es, maps, out_attrs, stream=side)
+
+    pipe = GenerationPipeline(asset, w.params, octaves=w.octaves, attribute_fn=attribute_fn)
+    outs = []
+    for s in seeds:
+        o = nx.PackedOutputs(w.params.strand_count, w.params.cvs_per_strand, roots=False, root_uvs=False,
+                             point_counts=True)
+        pipe.submit(o, with_seed(w.params, s))
+        outs.append(o)
+    pipe.drain()
+    torch.cuda.synchronize()
+    for s, o, p in zip(seeds, outs, want):
+        assert torch.equal(o.points, p), f"seed {s}: points differ from the serial call"
