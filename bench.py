@@ -203,6 +203,25 @@ def surrogate_record(args, w=None, threshold=None, stride=None, epochs=None, ste
             ms.append(ev[0].elapsed_time(ev[1]))
         return sorted(ms)[reps // 2]
 
+    # both pipelines again with consecutive steps overlapped: each step's
+    # attributes (and, for the surrogate, its preparation, network and
+    # check) on a side stream beside the previous step's strand kernel
+    from nanoxgen_amd.pipeline import GenerationPipeline
+
+    def exact_fn(params, out_attrs, side):
+        expr.evaluate_root_attributes(asset, params, es, maps, out_attrs, stream=side)
+
+    def surrogate_fn(params, out_attrs, side, prepared):
+        surrogate.surrogate_attributes(asset, params, net, maps, out_attrs, stream=side, prepared=prepared)
+        surrogate.verify_attributes(asset, params, es, maps, out_attrs, threshold, stride, flags, stats, stream=side)
+
+    pipe_ms = {}
+    for name, fn, after in (("exact", exact_fn, False), ("surrogate", surrogate_fn, True)):
+        gp = GenerationPipeline(asset, w.params, octaves=w.octaves, stream=stream, attribute_fn=fn,
+                                attributes_after_prepare=after)
+        pipe_ms[name] = timed(lambda: gp.submit(out))
+        del gp
+    stats.zero_()
     exact_ms = timed(pipeline([exact_attrs]))
     exact = attrs.clone()
     sur_ms = timed(pipeline([surrogate_attrs, verify]))
@@ -229,6 +248,9 @@ def surrogate_record(args, w=None, threshold=None, stride=None, epochs=None, ste
            "exact_pipeline_points_per_s": points / (exact_ms * 1e-3), "exact_pipeline_ms": round(exact_ms, 4),
            "surrogate_pipeline_points_per_s": points / (sur_ms * 1e-3), "surrogate_pipeline_ms": round(sur_ms, 4),
            "speedup_vs_exact": round(exact_ms / sur_ms, 4),
+           "pipelined_steps": {"exact_pipeline_ms": round(pipe_ms["exact"], 4),
+                               "surrogate_pipeline_ms": round(pipe_ms["surrogate"], 4),
+                               "speedup_vs_exact": round(pipe_ms["exact"] / pipe_ms["surrogate"], 4)},
            "attributes_ms_exact": round(ex_attr_ms, 4), "surrogate_kernel_ms": round(mlp_ms, 4),
            "check_and_fallback_ms": round(verify_ms, 4),
            "tiles": int(flags.numel()), "tiles_served": int(flags.numel()) - tiles_redone,
@@ -514,7 +536,10 @@ def bench_batch_surrogate(args):
                                            "order"], best.as_tuple())),
                        "parallelism": "single-gpu"},
             "exact_batch_ms": round(exact_ms, 4), "exact_batch_points_per_s": points / (exact_ms * 1e-3),
-            "speedup_vs_exact": round(exact_ms / sur_ms, 4), "items": items,
+            "speedup_vs_exact": round(exact_ms / sur_ms, 4),
+           "pipelined_steps": {"exact_pipeline_ms": round(pipe_ms["exact"], 4),
+                               "surrogate_pipeline_ms": round(pipe_ms["surrogate"], 4),
+                               "speedup_vs_exact": round(pipe_ms["exact"] / pipe_ms["surrogate"], 4)}, "items": items,
             "training": [{"description": n, "seconds": s, "loss_first_last": l} for n, s, l in trained],
             "roofline": None, "cpu_baseline": None}
     print(json.dumps(line), flush=True)

@@ -30,7 +30,8 @@ class GenerationPipeline:
     `after` event of its submit(): work that rewrites the deformed geometry
     between calls passes its event there."""
 
-    def __init__(self, asset, params, *, octaves=None, deformed=None, depth=2, stream=None, attribute_fn=None):
+    def __init__(self, asset, params, *, octaves=None, deformed=None, depth=2, stream=None, attribute_fn=None,
+                 attributes_after_prepare=False):
         torch = _torch()
         if depth < 1:
             raise ValueError("GenerationPipeline: depth must be at least 1")
@@ -42,8 +43,12 @@ class GenerationPipeline:
                            for _ in range(depth)]
         # per-root attribute multipliers (nxg_generate_packed_attributed):
         # attribute_fn(params, attributes, stream) fills slot k's float32
-        # [N, 4] buffer on the side stream before the preparation
+        # [N, 4] buffer on the side stream before the preparation, or with
+        # attributes_after_prepare attribute_fn(params, attributes, stream,
+        # workspace) after it (a producer that reads the prepared strand
+        # order, such as the surrogate)
         self.attribute_fn = attribute_fn
+        self.attributes_after_prepare = attributes_after_prepare
         self.attributes = ([torch.empty((params.strand_count, 4), dtype=torch.float32, device=asset.tensor.device)
                             for _ in range(depth)] if attribute_fn is not None else None)
         self.ready = [torch.cuda.Event() for _ in range(depth)]
@@ -80,8 +85,11 @@ class GenerationPipeline:
                 raise ValueError(f"GenerationPipeline: {params.strand_count} strands, the attribute slots hold "
                                  f"{self.attributes[k].shape[0]}")
             attributes = self.attributes[k][:params.strand_count]
-            self.attribute_fn(params, attributes, self.side)
+            if not self.attributes_after_prepare:
+                self.attribute_fn(params, attributes, self.side)
         prepare_packed(self.asset, params, ws, deformed=self.deformed, stream=self.side)
+        if attributes is not None and self.attributes_after_prepare:
+            self.attribute_fn(params, attributes, self.side, ws)
         if events:
             events[1].record(self.side)
         self.ready[k].record(self.side)
