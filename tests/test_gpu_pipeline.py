@@ -132,10 +132,12 @@ def test_pipeline_deformed_frames_with_after(torch, plane):
         assert torch.equal(o.points, p), f"frame {f} differs from its serial call"
 
 
-def test_pipeline_with_attribute_programs(torch):
+@pytest.mark.parametrize("after_prepare", [False, True])
+def test_pipeline_with_attribute_programs(torch, after_prepare):
     """Expression attributes evaluated per call into the pipeline's slot on
-    the side stream: every call equals attributes + generate_packed run
-    serially, for several seeds in flight."""
+    the side stream, before or after the call's preparation: every call
+    equals attributes + generate_packed run serially, for several seeds in
+    flight."""
     from nanoxgen_amd import expr
     w = workloads.config3(strands=40_000)
     asset = nx.DeviceAsset(nx.build_asset(w.build))
@@ -151,10 +153,14 @@ def test_pipeline_with_attribute_programs(torch):
         want.append(out.points.clone())
     torch.cuda.synchronize()
 
-    def attribute_fn(params, out_attrs, side):
+    seen = []
+
+    def attribute_fn(params, out_attrs, side, *workspace):
+        seen.append(len(workspace))
         expr.evaluate_root_attributes(asset, params, es, maps, out_attrs, stream=side)
 
-    pipe = GenerationPipeline(asset, w.params, octaves=w.octaves, attribute_fn=attribute_fn)
+    pipe = GenerationPipeline(asset, w.params, octaves=w.octaves, attribute_fn=attribute_fn,
+                              attributes_after_prepare=after_prepare)
     outs = []
     for s in seeds:
         o = nx.PackedOutputs(w.params.strand_count, w.params.cvs_per_strand, roots=False, root_uvs=False,
@@ -163,5 +169,6 @@ def test_pipeline_with_attribute_programs(torch):
         outs.append(o)
     pipe.drain()
     torch.cuda.synchronize()
+    assert seen == [1 if after_prepare else 0] * len(seeds)  # the workspace only after the preparation
     for s, o, p in zip(seeds, outs, want):
         assert torch.equal(o.points, p), f"seed {s}: points differ from the serial call"
