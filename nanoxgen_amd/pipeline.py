@@ -12,7 +12,7 @@ the strand kernel on the caller's stream after it.  Every call still runs
 every kernel and writes exactly what generate_packed writes; only the
 scheduling differs.
 """
-from . import DeviceAsset, _torch, generate_packed_prepared, prepare_packed  # noqa: F401
+from . import _torch, generate_packed_prepared, prepare_packed
 
 
 class GenerationPipeline:
